@@ -1,0 +1,288 @@
+/*
+ * umbreon_gpu.h -- C-ABI of the MI355X (gfx950) per-ray backend of libumbreon.
+ *
+ * This is the drop-in boundary. Every type below is plain C (no C++ or torch
+ * types), every buffer is a plain pointer plus a count, and every call returns
+ * an int status (UBG_OK == 0). The C++ facade (umbreon::render, include/umbreon/)
+ * flattens umbreon::Scene into ubg_scene_desc and RenderOptions into
+ * ubg_render_params and converts non-zero statuses into std::runtime_error.
+ *
+ * What each entry point replaces in the reference (CueMol/umbreon):
+ *   ubg_context_create / _destroy  -> rtcNewDevice + SharedDevice reuse
+ *                                     (src/umbreon/render/embree_renderer.cpp:1267-1273,
+ *                                      src/umbreon/umbreon.cpp:21-39), releaseEmbree
+ *                                     (embree_renderer.cpp:281-292)
+ *   ubg_scene_create / _destroy    -> buildEmbreeScene (src/umbreon/render/scene_build.cpp:133-170)
+ *                                     incl. buildTriangleMesh (:16-88), buildSpheres (:91-129),
+ *                                     buildCylinderGeometry (src/umbreon/render/curve_build.cpp:41-267)
+ *                                     and the BuiltScene side tables (scene_build.hpp:63-118)
+ *   ubg_render                     -> EmbreeRenderer::render grid path
+ *                                     (embree_renderer.cpp:1260-1427) + applyFog
+ *                                     (src/umbreon/postprocess/fog.cpp:17-36) + boxDownsample /
+ *                                     applyAssumedGamma (src/umbreon/postprocess/image_ops.cpp:9-46),
+ *                                     i.e. renderFrame's trace/fog/resolve steps
+ *                                     (src/umbreon/render/pipeline.cpp:113-129,182-254)
+ *   ubg_intersect / ubg_occluded   -> batched rtcIntersect1 / rtcOccluded1 (Embree 4 public API,
+ *                                     call sites shading/transparency.hpp:382,
+ *                                     shading/secondary_rays.hpp:95) and the edge-pass seam
+ *                                     EmbreeRenderer::occluded (embree_renderer.hpp:68-70)
+ *   ubg_last_error                 -> the reference's std::runtime_error messages
+ *                                     (embree_renderer.cpp:1272, scene_build.cpp:164-168)
+ */
+#ifndef UMBREON_GPU_H
+#define UMBREON_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define UBG_ABI_VERSION 1
+
+/* Status codes. */
+enum {
+  UBG_OK = 0,
+  UBG_ERR_INVALID_ARGUMENT = 1,
+  UBG_ERR_DEVICE = 2,          /* no usable gfx950 device / HIP runtime error */
+  UBG_ERR_OUT_OF_MEMORY = 3,
+  UBG_ERR_CANCELLED = 4,       /* cancel_flag was raised; frame is partial */
+  UBG_ERR_UNSUPPORTED = 5,     /* option/material the GPU path does not implement */
+  UBG_ERR_INTERNAL = 6
+};
+
+/* Geometry kinds. The numbering is the reference's GeomKind (scene_build.hpp),
+ * which also forms the low two bits of the edge objectId. */
+enum {
+  UBG_KIND_MESH = 0,
+  UBG_KIND_SPHERE = 1,
+  UBG_KIND_ROUND_CURVE = 2,   /* `open` cylinders, chained (ROUND_LINEAR_CURVE) */
+  UBG_KIND_CONE_CURVE = 3     /* capped cylinders (CONE_LINEAR_CURVE) */
+};
+
+/* Flattened umbreon::Material (scene.hpp:130-214). 64 bytes. */
+typedef struct ubg_material {
+  float ambient, diffuse, specular, roughness, brilliance, phong, phong_size;
+  int32_t metallic;
+  float reflection, emission;
+  int32_t model; /* 0 = Pov, 1 = Principled, 2 = Toon */
+  float pbr_metallic, pbr_roughness, pbr_specular, pbr_anisotropy,
+      pbr_anisotropy_rotation;
+} ubg_material;
+
+/* Flattened umbreon::Scene geometry (scene.hpp:225-368). All arrays are
+ * borrowed for the duration of ubg_scene_create only. Optional arrays may be
+ * NULL (meaning: the reference default). */
+typedef struct ubg_scene_desc {
+  /* Mesh::positions/normals/colors (per vertex) and Mesh::index. */
+  int64_t n_vertices;
+  const float* positions;       /* 3 * n_vertices */
+  const float* normals;         /* 3 * n_vertices */
+  const float* colors;          /* 4 * n_vertices, rgb + opacity */
+  int64_t n_triangles;
+  const uint32_t* index;        /* 3 * n_triangles, NULL => corner c uses vertex c */
+  /* Mesh materials: tri_material == NULL => every triangle uses
+   * mesh_materials[0] (Mesh::material); else mesh_materials[tri_material[t]]
+   * (Mesh::materials / Mesh::triMaterialId). */
+  const ubg_material* mesh_materials;
+  int32_t n_mesh_materials;
+  const uint8_t* tri_material;  /* n_triangles or NULL */
+  const uint16_t* tri_group;    /* n_triangles or NULL (group 0) */
+
+  /* Scene::spheres. */
+  int64_t n_spheres;
+  const float* sphere_center_radius; /* 4 * n: x, y, z, radius */
+  const float* sphere_color;         /* 4 * n: rgb + opacity */
+  const int32_t* sphere_material;    /* n: index into prim_materials */
+  const uint16_t* sphere_group;      /* n or NULL */
+  const uint8_t* sphere_from_edge;   /* n or NULL (Sphere::fromEdgeMacro) */
+
+  /* Scene::cylinders. cyl_open selects the curve kind exactly like
+   * Cylinder::open: 1 => chained ROUND curve, 0 => capped CONE curve. */
+  int64_t n_cylinders;
+  const float* cyl_p0;               /* 3 * n */
+  const float* cyl_p1;               /* 3 * n */
+  const float* cyl_radius;           /* n */
+  const float* cyl_color;            /* 4 * n: rgb + opacity at p0 */
+  const float* cyl_opacity1;         /* n or NULL (=> -1, uniform) */
+  const int32_t* cyl_material;       /* n: index into prim_materials */
+  const uint16_t* cyl_group;         /* n or NULL */
+  const uint8_t* cyl_open;           /* n or NULL (=> capped) */
+  const uint8_t* cyl_from_edge;      /* n or NULL */
+
+  const ubg_material* prim_materials; /* shared by spheres and cylinders */
+  int32_t n_prim_materials;
+
+  /* Scene::instanceOffsets, baked like the reference (scene_build.cpp:148-149). */
+  int64_t n_instances;
+  const float* instance_offsets;     /* 3 * n or NULL */
+} ubg_scene_desc;
+
+typedef struct ubg_camera {         /* umbreon::Camera (scene.hpp:380-388) */
+  float position[3];
+  float direction[3];
+  float up[3];
+  float fovy;                        /* degrees */
+  int32_t orthographic;
+  float height;                      /* ortho image-plane height */
+} ubg_camera;
+
+typedef struct ubg_light {          /* umbreon::DistantLight (scene.hpp:390-411) */
+  float direction[3];                /* direction the light travels */
+  float color[3];
+  float intensity;
+  int32_t casts_highlight;
+  float angular_radius;              /* radians */
+} ubg_light;
+
+typedef struct ubg_fog {            /* umbreon::Fog (scene.hpp:413-428), linear GL fog */
+  int32_t enabled;
+  float color[3];
+  float start;
+  float end;
+} ubg_fog;
+
+/* ubg_render_params::flags */
+#define UBG_FLAG_HIRES_ONLY 1u   /* stop after trace+fog: color is the hi-res
+                                    (supersampled) linear buffer, no downsample
+                                    and no assumed gamma */
+#define UBG_FLAG_COUNT_VISITS 2u /* also count BVH node visits and primitive
+                                    tests (instrumented kernels; slower) */
+#define UBG_FLAG_KERNEL_TIMES 4u /* time each wavefront kernel with stream
+                                    events (ubg_frame::kernel_ms) */
+
+/* Everything one frame reads: scene-level constants (camera, lights, ambient,
+ * background, fog, clip planes, assumed gamma) plus the per-ray subset of
+ * RenderOptions (render/render_options.hpp:14-328). */
+typedef struct ubg_render_params {
+  int32_t width, height;             /* FINAL image size (RenderOptions::width/height) */
+  int32_t supersample;               /* hi-res grid = width*ss x height*ss */
+  ubg_camera camera;
+  const ubg_light* lights;
+  int32_t n_lights;
+  float ambient_color[3];            /* Scene::ambientColor */
+  float background[3];               /* Scene::background */
+  ubg_fog fog;
+  float clip_near, clip_far;         /* Scene::clipNear/clipFar (+-inf = off) */
+  float assumed_gamma;               /* Scene::assumedGamma */
+  int32_t ao_samples;
+  float ao_distance;
+  float ao_intensity;
+  int32_t shadows;
+  int32_t shadow_samples;
+  float light_radius_deg;            /* RenderOptions::lightRadius */
+  int32_t gi_integrator;             /* only selects the per-light radius rule */
+  float specular_scale;
+  int32_t transparency;
+  int32_t transparent_background;
+  int32_t max_transparent_layers;
+  uint32_t flags;                    /* UBG_FLAG_* */
+  /* Cooperative cancel (RenderProgress::requestCancel): polled between
+   * wavefront passes; a non-zero value stops the frame (UBG_ERR_CANCELLED). */
+  const volatile int32_t* cancel_flag;
+} ubg_render_params;
+
+/* Ray accounting of one frame. A pure function of the scene and options:
+ * identical for every launch configuration. */
+typedef struct ubg_ray_stats {
+  uint64_t primary_rays;      /* closest-hit queries, incl. transparency continuations */
+  uint64_t shaded_hits;       /* hits shaded (one per layer walked) */
+  uint64_t ao_rays_traced;    /* AO any-hit queries */
+  uint64_t ao_rays_sampled;   /* below-horizon AO samples counted occluded, not traced */
+  uint64_t shadow_rays;       /* shadow any-hit queries */
+  uint64_t capped_rays;       /* primary rays that reached the transparent-layer cap */
+  /* UBG_FLAG_COUNT_VISITS only, per ray class [0] primary, [1] AO, [2] shadow: */
+  uint64_t node_visits[3];    /* inner-node fetches */
+  uint64_t prim_tests[3];     /* primitive tests */
+} ubg_ray_stats;
+
+/* ubg_frame::kernel_ms slots (UBG_FLAG_KERNEL_TIMES) */
+enum { UBG_KERNEL_PRIMARY = 0, UBG_KERNEL_AO = 1, UBG_KERNEL_RESOLVE = 2, UBG_KERNEL_POST = 3 };
+
+/* Output buffers, allocated by the caller. memory == 0: host pointers (the
+ * library stages through its own device buffers); memory == 1: device
+ * pointers on the context's GPU (no host copy). Optional outputs may be NULL. */
+typedef struct ubg_frame {
+  float* color;        /* out_w * out_h * 4 linear HDR RGBA (top-left origin) */
+  float* depth;        /* hi_w * hi_h first-hit ray distance (0 on miss) */
+  float* view_z;       /* hi_w * hi_h first-hit linear view-z (0 on miss) */
+  uint32_t* hit_id;    /* hi_w * hi_h first-hit (kind << 30) | primID, 0xFFFFFFFF on miss */
+  float* ao_open;      /* hi_w * hi_h first-hit AO openness (1 when no AO) */
+  int32_t memory;
+  /* Filled by ubg_render: */
+  int32_t out_w, out_h, hi_w, hi_h;
+  ubg_ray_stats stats;
+  double render_seconds;   /* trace + shade + fog window (device events) */
+  double post_seconds;     /* downsample + gamma */
+  int32_t cancelled;
+  int32_t layers;          /* transparency layers walked (wavefront passes) */
+  double kernel_ms[4];     /* UBG_FLAG_KERNEL_TIMES: summed time per kernel kind */
+  int32_t kernel_launches[4];
+} ubg_frame;
+
+typedef struct ubg_scene_info {
+  int64_t n_triangles, n_spheres, n_round_segments, n_cone_segments;
+  int64_t n_nodes;          /* BVH nodes */
+  int64_t n_leaf_prims;     /* primitive records in leaf order */
+  int32_t bvh_depth;
+  int32_t node_bytes;       /* bytes per BVH node */
+  int32_t prim_bytes;       /* bytes per leaf primitive record */
+  double build_seconds;     /* upload + BVH build wall time */
+  double sah_cost;
+  int64_t device_bytes;     /* device memory held by the scene */
+} ubg_scene_info;
+
+/* One ray of the batched query API: origin, direction, [tnear, tfar]. */
+typedef struct ubg_ray {
+  float org[3];
+  float tnear;
+  float dir[3];
+  float tfar;
+} ubg_ray;
+
+/* Closest hit of one ray (RTCHit analogue). */
+typedef struct ubg_hit {
+  float t;             /* +inf on miss */
+  float u, v;          /* triangle barycentrics / curve axial fraction in u */
+  uint32_t kind;       /* UBG_KIND_*, 0xFFFFFFFF on miss */
+  uint32_t prim_id;    /* per-kind primitive id, 0xFFFFFFFF on miss */
+  float ng[3];         /* unnormalized geometric normal */
+} ubg_hit;
+
+typedef struct ubg_context ubg_context;
+typedef struct ubg_scene ubg_scene;
+
+int ubg_abi_version(void);
+const char* ubg_status_string(int status);
+
+/* Device context: one gfx950 device, one HIP stream, reusable scratch.
+ * Calls on one context are serialized by an internal mutex; use one context
+ * per host thread for concurrent renders. */
+int ubg_context_create(int device_ordinal, ubg_context** out);
+void ubg_context_destroy(ubg_context* ctx);
+/* Last error message of the calling thread (never NULL). */
+const char* ubg_last_error(void);
+
+/* Upload geometry and build the BVH. */
+int ubg_scene_create(ubg_context* ctx, const ubg_scene_desc* desc, ubg_scene** out);
+void ubg_scene_destroy(ubg_scene* scene);
+int ubg_scene_get_info(const ubg_scene* scene, ubg_scene_info* out);
+
+/* Render one frame (hi-res trace + shade + fog, then resolve). */
+int ubg_render(ubg_context* ctx, const ubg_scene* scene,
+               const ubg_render_params* params, ubg_frame* frame);
+
+/* Batched closest-hit and any-hit queries (host or device buffers, per
+ * `memory` as in ubg_frame). occluded[i] = 1 when any primitive lies in
+ * [tnear, tfar] along ray i. */
+int ubg_intersect(ubg_context* ctx, const ubg_scene* scene, int64_t n_rays,
+                  const ubg_ray* rays, ubg_hit* hits, int32_t memory);
+int ubg_occluded(ubg_context* ctx, const ubg_scene* scene, int64_t n_rays,
+                 const ubg_ray* rays, uint8_t* occluded, int32_t memory);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* UMBREON_GPU_H */

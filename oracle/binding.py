@@ -1,0 +1,162 @@
+"""ctypes binding of oracle/liboracle.so -- TEST INFRASTRUCTURE.
+
+Imported only by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline
+leg, always as the checker of the gfx950 library (never as the thing that is
+measured or shipped). It takes the same umbreon_amd.Scene / RenderOptions and
+marshals them into the same C structs the product consumes.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+from typing import Optional
+
+import numpy as np
+
+from umbreon_amd import capi
+from umbreon_amd.render import FrameResult
+from umbreon_amd.scene import RenderOptions, Scene, params
+
+ORACLE_LIB = Path(__file__).resolve().parent / "liboracle.so"
+
+
+class OrcOptions(C.Structure):
+    _fields_ = [("precision", C.c_int32), ("threads", C.c_int32), ("row_begin", C.c_int32),
+                ("row_end", C.c_int32), ("row_step", C.c_int32), ("brute_force", C.c_int32)]
+
+
+_lib = None
+
+
+def load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not ORACLE_LIB.exists():
+        raise RuntimeError(f"{ORACLE_LIB} is missing; run umbreon_amd/build.py oracle")
+    lib = C.CDLL(str(ORACLE_LIB))
+    lib.orc_render.argtypes = [C.POINTER(capi.SceneDesc), C.POINTER(capi.RenderParams),
+                               C.POINTER(capi.Frame), C.POINTER(OrcOptions)]
+    lib.orc_render.restype = C.c_int
+    lib.orc_intersect.argtypes = [C.POINTER(capi.SceneDesc), C.c_int64, C.c_void_p, C.c_void_p,
+                                  C.POINTER(OrcOptions)]
+    lib.orc_intersect.restype = C.c_int
+    lib.orc_occluded.argtypes = [C.POINTER(capi.SceneDesc), C.c_int64, C.c_void_p, C.c_void_p,
+                                 C.POINTER(OrcOptions)]
+    lib.orc_occluded.restype = C.c_int
+    lib.orc_chain_curves.argtypes = [C.POINTER(capi.SceneDesc), C.c_int64, C.c_void_p, C.c_void_p]
+    lib.orc_chain_curves.restype = C.c_int64
+    lib.orc_tea2.argtypes = [C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]
+    lib.orc_tea2.restype = None
+    lib.orc_fog_factor.argtypes = [C.POINTER(capi.Fog), C.c_float]
+    lib.orc_fog_factor.restype = C.c_float
+    lib.orc_box_downsample.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p]
+    lib.orc_box_downsample.restype = None
+    lib.orc_srgb_encode.argtypes = [C.c_float]
+    lib.orc_srgb_encode.restype = C.c_float
+    lib.orc_srgb_decode.argtypes = [C.c_float]
+    lib.orc_srgb_decode.restype = C.c_float
+    _lib = lib
+    return lib
+
+
+def _opts(precision=32, threads=0, rows=None, brute=False) -> OrcOptions:
+    o = OrcOptions()
+    o.precision = precision
+    o.threads = threads
+    if rows is None:
+        o.row_begin, o.row_end, o.row_step = 0, 0, 1
+    else:
+        o.row_begin, o.row_end, o.row_step = rows
+    o.brute_force = 1 if brute else 0
+    return o
+
+
+def render(scene: Scene, opt: RenderOptions, hires_only: bool = False, precision: int = 32,
+           threads: int = 0, rows=None, brute: bool = False) -> FrameResult:
+    lib = load()
+    desc, keep1 = scene.to_desc()
+    p, keep2 = params(scene, opt, capi.FLAG_HIRES_ONLY if hires_only else 0)
+    ss = max(1, int(opt.supersample))
+    hw, hh = opt.width * ss, opt.height * ss
+    ow, oh = (hw, hh) if hires_only else (opt.width, opt.height)
+    color = np.zeros((oh, ow, 4), np.float32)
+    depth = np.zeros((hh, hw), np.float32)
+    view_z = np.zeros((hh, hw), np.float32)
+    hit_id = np.zeros((hh, hw), np.uint32)
+    ao = np.zeros((hh, hw), np.float32)
+    fr = capi.Frame()
+    fr.color, fr.depth, fr.view_z = color.ctypes.data, depth.ctypes.data, view_z.ctypes.data
+    fr.hit_id, fr.ao_open = hit_id.ctypes.data, ao.ctypes.data
+    o = _opts(precision, threads, rows, brute)
+    capi.check(lib.orc_render(C.byref(desc), C.byref(p), C.byref(fr), C.byref(o)))
+    del keep1, keep2
+    return FrameResult(width=ow, height=oh, color=color, depth=depth, view_z=view_z, hit_id=hit_id,
+                       ao_open=ao, stats=fr.stats.as_dict(), render_seconds=fr.render_seconds,
+                       post_seconds=fr.post_seconds)
+
+
+def intersect(scene: Scene, rays: np.ndarray, precision: int = 32, brute: bool = False) -> np.ndarray:
+    lib = load()
+    desc, keep = scene.to_desc()
+    rays = np.ascontiguousarray(rays, dtype=capi.ray_dtype())
+    hits = np.zeros(len(rays), dtype=capi.hit_dtype())
+    o = _opts(precision, 1, None, brute)
+    capi.check(lib.orc_intersect(C.byref(desc), len(rays), rays.ctypes.data, hits.ctypes.data, C.byref(o)))
+    del keep
+    return hits
+
+
+def occluded(scene: Scene, rays: np.ndarray, precision: int = 32, brute: bool = False) -> np.ndarray:
+    lib = load()
+    desc, keep = scene.to_desc()
+    rays = np.ascontiguousarray(rays, dtype=capi.ray_dtype())
+    occ = np.zeros(len(rays), dtype=np.uint8)
+    o = _opts(precision, 1, None, brute)
+    capi.check(lib.orc_occluded(C.byref(desc), len(rays), rays.ctypes.data, occ.ctypes.data, C.byref(o)))
+    del keep
+    return occ
+
+
+def chain_curves(scene: Scene):
+    lib = load()
+    desc, keep = scene.to_desc()
+    cap = max(1, 2 * len(scene.cyl_p0))
+    src = np.zeros(cap, np.int32)
+    flags = np.zeros(cap, np.uint8)
+    n = lib.orc_chain_curves(C.byref(desc), cap, src.ctypes.data, flags.ctypes.data)
+    del keep
+    assert n >= 0
+    return src[:n], flags[:n]
+
+
+def tea2(v0: int, v1: int):
+    lib = load()
+    a, b = C.c_uint32(v0), C.c_uint32(v1)
+    lib.orc_tea2(C.byref(a), C.byref(b))
+    return a.value, b.value
+
+
+def fog_factor(enabled: bool, start: float, end: float, z: float) -> float:
+    lib = load()
+    f = capi.Fog()
+    f.enabled, f.start, f.end = int(enabled), start, end
+    return float(lib.orc_fog_factor(C.byref(f), z))
+
+
+def box_downsample(src: np.ndarray, ss: int) -> np.ndarray:
+    lib = load()
+    src = np.ascontiguousarray(src, np.float32)
+    h, w, ch = src.shape
+    out = np.zeros((h // ss, w // ss, ch), np.float32)
+    lib.orc_box_downsample(src.ctypes.data, w, h, ch, ss, out.ctypes.data)
+    return out
+
+
+def srgb_encode(v: float) -> float:
+    return float(load().orc_srgb_encode(v))
+
+
+def srgb_decode(s: float) -> float:
+    return float(load().orc_srgb_decode(s))

@@ -1,0 +1,1523 @@
+// oracle.cpp -- CPU restatement of libumbreon's per-ray path.
+//
+// TEST INFRASTRUCTURE ONLY: loaded by tests/, __graft_entry__.smoke() and the
+// cpu_baseline leg of bench.py as the checker. The MI355X product never calls
+// it. Built with -ffp-contract=off so every float operation rounds exactly as
+// written, in the order the reference writes it.
+//
+// Each function cites the reference file:line it restates (paths relative to
+// the CueMol/umbreon source tree). The template parameter T selects the
+// precision: float follows the reference's float-op order, double is the FP64
+// oracle of the same integrator.
+//
+// Pinning (which check fixes which function) is listed at the top of
+// tests/test_oracle.py. Functions whose exact arithmetic only a live Embree
+// could fix (curve t/u/Ng, triangle edge watertightness, interpolation
+// rounding, tie order among equal-t hits) are "parity unpinned": their
+// oracle here is checked against closed forms and FP64 brute force only.
+
+#include "oracle.h"
+
+#include <algorithm>
+#include <array>
+#include <atomic>
+#include <chrono>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <limits>
+#include <map>
+#include <thread>
+#include <utility>
+#include <vector>
+
+namespace orc {
+
+// ---------------------------------------------------------------------------
+// Vector math (scene.hpp:20-76). Same operation order as the reference.
+template <class T>
+struct V3 {
+  T x, y, z;
+};
+template <class T>
+inline V3<T> operator+(V3<T> a, V3<T> b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+template <class T>
+inline V3<T> operator-(V3<T> a, V3<T> b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+template <class T>
+inline V3<T> operator*(V3<T> a, T s) { return {a.x * s, a.y * s, a.z * s}; }
+template <class T>
+inline V3<T> operator*(T s, V3<T> a) { return a * s; }
+template <class T>
+inline V3<T> neg(V3<T> a) { return {-a.x, -a.y, -a.z}; }
+template <class T>
+inline T dot(V3<T> a, V3<T> b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+template <class T>
+inline V3<T> cross(V3<T> a, V3<T> b) {
+  return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+template <class T>
+inline T length(V3<T> a) { return std::sqrt(dot(a, a)); }
+// scene.hpp:35-38
+template <class T>
+inline V3<T> normalize(V3<T> a) {
+  T l = length(a);
+  return l > T(0) ? a * (T(1) / l) : a;
+}
+// scene.hpp:44-49
+template <class T>
+inline V3<T> safeNormalize(V3<T> a) {
+  T s = T(1) / std::sqrt(std::fmax(dot(a, a), T(0x1.0p-126)));
+  return a * s;
+}
+// scene.hpp:53-57
+template <class T>
+inline V3<T> safeNormalize(V3<T> a, V3<T> fallback) {
+  T d = dot(a, a);
+  if (d < T(1.0e-18)) return fallback;
+  return a * (T(1) / std::sqrt(d));
+}
+template <class T>
+struct Frame {
+  V3<T> t, b, n;
+};
+// scene.hpp:69-76
+template <class T>
+inline Frame<T> frameFromNormal(V3<T> n) {
+  const V3<T> seed0{T(0), n.z, -n.y};
+  const V3<T> seed1{-n.z, T(0), n.x};
+  const V3<T> t = safeNormalize(std::fabs(n.x) < std::fabs(n.y) ? seed0 : seed1,
+                                V3<T>{T(1), T(0), T(0)});
+  const V3<T> b = cross(n, t);
+  return {t, b, n};
+}
+template <class T>
+inline T radiansT(T deg) { return deg * T(0.01745329252f); }
+
+template <class T>
+inline V3<T> v3(const float* p) { return {T(p[0]), T(p[1]), T(p[2])}; }
+
+// shading.hpp:27-30
+template <class T>
+inline V3<T> faceForward(V3<T> n, V3<T> rayDir) {
+  return (dot(n, rayDir) > T(0)) ? neg(n) : n;
+}
+// shading.hpp:34-41
+template <class T>
+inline T blinnExp(T roughness) {
+  T r = roughness;
+  if (r < T(1e-6f)) r = T(1e-6f);
+  T e = T(1) / r;
+  if (e < T(1)) e = T(1);
+  if (e > T(1.0e6f)) e = T(1.0e6f);
+  return e;
+}
+// secondary_rays.hpp:38-46
+template <class T>
+inline T selfIntersectEps(V3<T> P, V3<T> dir, T t) {
+  const T dirMax = std::fmax(std::fabs(dir.x), std::fmax(std::fabs(dir.y), std::fabs(dir.z)));
+  const T epsScale = std::fmax(std::fmax(std::fabs(P.x), std::fabs(P.y)),
+                               std::fmax(std::fabs(P.z), dirMax * t));
+  return epsScale * T(0x1.0fp-21f);
+}
+// secondary_rays.hpp:51-58 (TEA, 8 rounds)
+inline void tea2(uint32_t& v0, uint32_t& v1) {
+  uint32_t sum = 0;
+  for (int i = 0; i < 8; ++i) {
+    sum += 0x9E3779B9u;
+    v0 += ((v1 << 4) + 0xA341316Cu) ^ (v1 + sum) ^ ((v1 >> 5) + 0xC8013EA4u);
+    v1 += ((v0 << 4) + 0xAD90777Du) ^ (v0 + sum) ^ ((v0 >> 5) + 0x7E95761Eu);
+  }
+}
+// secondary_rays.hpp:61
+template <class T>
+inline T u32ToUnorm(uint32_t u) { return T(u >> 8) * T(0x1.0p-24f); }
+// secondary_rays.hpp:66-73 (Malley)
+template <class T>
+inline V3<T> cosineSampleHemisphere(T u1, T u2, const Frame<T>& f) {
+  const T phi = T(6.2831853072f) * u1;
+  const T r = std::sqrt(u2);
+  const T z = std::sqrt(std::fmax(T(0), T(1) - u2));
+  const T x = std::cos(phi) * r;
+  const T y = std::sin(phi) * r;
+  return f.t * x + f.b * y + f.n * z;
+}
+
+// ---------------------------------------------------------------------------
+// Scene assembly (scene_build.cpp:16-170, curve_build.cpp:41-267).
+
+struct Mat {
+  float ambient, diffuse, specular, roughness, brilliance, phong, phongSize;
+  bool metallic;
+  float reflection, emission;
+  int model;
+};
+inline Mat toMat(const ubg_material& m) {
+  return Mat{m.ambient, m.diffuse,    m.specular,   m.roughness,  m.brilliance, m.phong,
+             m.phong_size, m.metallic != 0, m.reflection, m.emission, m.model};
+}
+
+struct PrimSide {  // BuiltScene side-table entry for one primitive
+  float color[4];
+  int32_t mat;
+  uint16_t group;
+  uint8_t fromEdge;
+  float opacity1;
+};
+
+struct Geometry {
+  // Mesh (baked per instance): positions/normals/colors per vertex, index per
+  // corner (scene_build.cpp:37-68).
+  int64_t baseV = 0, baseT = 0, copies = 1;
+  std::vector<float> pos, nrm, col;
+  std::vector<uint32_t> idx;
+  std::vector<Mat> meshMats;
+  std::vector<uint8_t> triMat;
+  std::vector<uint16_t> triGroup;
+  // Spheres (scene_build.cpp:102-123).
+  std::vector<float> sph;  // x y z r
+  std::vector<PrimSide> sphSide;
+  // Round curves: vertex buffer (x y z r), segment start vertex, flags.
+  std::vector<float> rv;
+  std::vector<uint32_t> rseg;
+  std::vector<uint8_t> rflag;
+  std::vector<int32_t> rsrc;
+  std::vector<PrimSide> rSide;
+  // Cone curves: 8 floats per segment (p0 r p1 r).
+  std::vector<float> cone;
+  std::vector<PrimSide> cSide;
+  std::vector<Mat> primMats;
+};
+
+static inline uint32_t cornerVertex(const ubg_scene_desc* d, int64_t corner) {
+  return d->index ? d->index[corner] : static_cast<uint32_t>(corner);
+}
+
+// curve_build.cpp:88-199 -- endpoint-hash chaining of `open` cylinders.
+static void chainOpenCylinders(const ubg_scene_desc* d, const std::vector<int>& openIdx,
+                               const float off[3], Geometry& g) {
+  const std::size_t no = openIdx.size();
+  const float eps = 1.0e-4f;
+  auto keyOf = [&](const float p[3]) {
+    return std::array<long long, 3>{std::llround(p[0] / eps), std::llround(p[1] / eps),
+                                    std::llround(p[2] / eps)};
+  };
+  std::vector<std::array<float, 3>> a0(no), a1(no);
+  for (std::size_t i = 0; i < no; ++i) {
+    const int c = openIdx[i];
+    for (int k = 0; k < 3; ++k) {
+      a0[i][k] = d->cyl_p0[3 * c + k] + off[k];
+      a1[i][k] = d->cyl_p1[3 * c + k] + off[k];
+    }
+  }
+  std::map<std::array<long long, 3>, std::vector<std::pair<int, int>>> inc;
+  for (int i = 0; i < static_cast<int>(no); ++i) {
+    inc[keyOf(a0[i].data())].push_back({i, 0});
+    inc[keyOf(a1[i].data())].push_back({i, 1});
+  }
+  auto grp = [&](int s) -> uint16_t { return d->cyl_group ? d->cyl_group[openIdx[s]] : 0; };
+  auto op1 = [&](int s) -> float { return d->cyl_opacity1 ? d->cyl_opacity1[openIdx[s]] : -1.0f; };
+  auto compatible = [&](int s, int t) {
+    return grp(s) == grp(t) && (op1(s) < 0.0f) == (op1(t) < 0.0f);
+  };
+  std::vector<int> next(no, -1), prev(no, -1);
+  for (int s = 0; s < static_cast<int>(no); ++s) {
+    const auto& ends = inc[keyOf(a1[s].data())];
+    if (ends.size() != 2) continue;
+    int t = -1;
+    for (const auto& e : ends)
+      if (e.first != s) t = (e.second == 0) ? e.first : -1;
+    if (t >= 0 && t != s && compatible(s, t)) next[s] = t;
+  }
+  for (int s = 0; s < static_cast<int>(no); ++s)
+    if (next[s] >= 0) prev[next[s]] = s;
+  std::vector<char> visited(no, 0);
+  auto emitChain = [&](int start) {
+    std::vector<int> chain;
+    for (int s = start; s >= 0 && !visited[s]; s = next[s]) {
+      visited[s] = 1;
+      chain.push_back(s);
+    }
+    const uint32_t base = static_cast<uint32_t>(g.rv.size() / 4);
+    auto pushV = [&](const std::array<float, 3>& p, float r) {
+      g.rv.push_back(p[0]);
+      g.rv.push_back(p[1]);
+      g.rv.push_back(p[2]);
+      g.rv.push_back(r);
+    };
+    pushV(a0[chain[0]], d->cyl_radius[openIdx[chain[0]]]);
+    for (int cs : chain) pushV(a1[cs], d->cyl_radius[openIdx[cs]]);
+    const int n = static_cast<int>(chain.size());
+    for (int j = 0; j < n; ++j) {
+      g.rseg.push_back(base + static_cast<uint32_t>(j));
+      uint8_t fl = 0;
+      if (j > 0) fl |= 1;
+      if (j < n - 1) fl |= 2;
+      g.rflag.push_back(fl);
+      const int c = openIdx[chain[j]];
+      g.rsrc.push_back(c);
+      PrimSide ps;
+      for (int k = 0; k < 4; ++k) ps.color[k] = d->cyl_color[4 * c + k];
+      ps.mat = d->cyl_material ? d->cyl_material[c] : 0;
+      ps.group = d->cyl_group ? d->cyl_group[c] : 0;
+      ps.fromEdge = d->cyl_from_edge ? d->cyl_from_edge[c] : 0;
+      ps.opacity1 = d->cyl_opacity1 ? d->cyl_opacity1[c] : -1.0f;
+      g.rSide.push_back(ps);
+    }
+  };
+  for (int s = 0; s < static_cast<int>(no); ++s)
+    if (prev[s] < 0 && !visited[s]) emitChain(s);
+  for (int s = 0; s < static_cast<int>(no); ++s)
+    if (!visited[s]) emitChain(s);
+}
+
+static void buildGeometry(const ubg_scene_desc* d, Geometry& g) {
+  std::vector<std::array<float, 3>> offs;
+  if (d->n_instances > 0 && d->instance_offsets) {
+    for (int64_t i = 0; i < d->n_instances; ++i)
+      offs.push_back({d->instance_offsets[3 * i], d->instance_offsets[3 * i + 1],
+                      d->instance_offsets[3 * i + 2]});
+  } else {
+    offs.push_back({0.0f, 0.0f, 0.0f});
+  }
+  g.copies = static_cast<int64_t>(offs.size());
+  // Mesh (scene_build.cpp:16-88): skipped unless >= 3 vertices and >= 1 tri.
+  if (d->n_vertices >= 3 && d->n_triangles > 0) {
+    g.baseV = d->n_vertices;
+    g.baseT = d->n_triangles;
+    for (const auto& off : offs) {
+      for (int64_t v = 0; v < g.baseV; ++v) {
+        for (int k = 0; k < 3; ++k) g.pos.push_back(d->positions[3 * v + k] + off[k]);
+        for (int k = 0; k < 3; ++k) g.nrm.push_back(d->normals ? d->normals[3 * v + k] : 0.0f);
+        for (int k = 0; k < 4; ++k)
+          g.col.push_back(d->colors ? d->colors[4 * v + k] : (k == 3 ? 1.0f : 0.0f));
+      }
+    }
+    for (int64_t c = 0; c < g.copies; ++c)
+      for (int64_t t = 0; t < g.baseT; ++t)
+        for (int k = 0; k < 3; ++k)
+          g.idx.push_back(static_cast<uint32_t>(c * g.baseV + cornerVertex(d, t * 3 + k)));
+    for (int32_t i = 0; i < d->n_mesh_materials; ++i) g.meshMats.push_back(toMat(d->mesh_materials[i]));
+    if (d->tri_material) g.triMat.assign(d->tri_material, d->tri_material + g.baseT);
+    if (d->tri_group) g.triGroup.assign(d->tri_group, d->tri_group + g.baseT);
+  }
+  for (int32_t i = 0; i < d->n_prim_materials; ++i) g.primMats.push_back(toMat(d->prim_materials[i]));
+  // Spheres (scene_build.cpp:91-129): instance-major order.
+  for (const auto& off : offs) {
+    for (int64_t s = 0; s < d->n_spheres; ++s) {
+      for (int k = 0; k < 3; ++k) g.sph.push_back(d->sphere_center_radius[4 * s + k] + off[k]);
+      g.sph.push_back(d->sphere_center_radius[4 * s + 3]);
+      PrimSide ps;
+      for (int k = 0; k < 4; ++k) ps.color[k] = d->sphere_color[4 * s + k];
+      ps.mat = d->sphere_material ? d->sphere_material[s] : 0;
+      ps.group = d->sphere_group ? d->sphere_group[s] : 0;
+      ps.fromEdge = d->sphere_from_edge ? d->sphere_from_edge[s] : 0;
+      ps.opacity1 = -1.0f;
+      g.sphSide.push_back(ps);
+    }
+  }
+  // Cylinders (curve_build.cpp:41-267): open -> chained round, else cone.
+  std::vector<int> openIdx, capIdx;
+  for (int64_t i = 0; i < d->n_cylinders; ++i) {
+    if (d->cyl_open && d->cyl_open[i]) openIdx.push_back(static_cast<int>(i));
+    else capIdx.push_back(static_cast<int>(i));
+  }
+  if (!openIdx.empty())
+    for (const auto& off : offs) chainOpenCylinders(d, openIdx, off.data(), g);
+  for (const auto& off : offs) {
+    for (int c : capIdx) {
+      for (int k = 0; k < 3; ++k) g.cone.push_back(d->cyl_p0[3 * c + k] + off[k]);
+      g.cone.push_back(d->cyl_radius[c]);
+      for (int k = 0; k < 3; ++k) g.cone.push_back(d->cyl_p1[3 * c + k] + off[k]);
+      g.cone.push_back(d->cyl_radius[c]);
+      PrimSide ps;
+      for (int k = 0; k < 4; ++k) ps.color[k] = d->cyl_color[4 * c + k];
+      ps.mat = d->cyl_material ? d->cyl_material[c] : 0;
+      ps.group = d->cyl_group ? d->cyl_group[c] : 0;
+      ps.fromEdge = d->cyl_from_edge ? d->cyl_from_edge[c] : 0;
+      ps.opacity1 = d->cyl_opacity1 ? d->cyl_opacity1[c] : -1.0f;
+      g.cSide.push_back(ps);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Intersectors (Embree 4 semantics, restated). A hit is accepted when
+// tnear <= t <= tfar. Curves and spheres solve in a frame whose origin is moved
+// along the ray to the closest approach of the primitive's center, so far
+// cameras do not cancel the discriminant.
+
+template <class T>
+struct Hit {
+  T t;
+  T u, v;
+  int kind;  // -1 = miss
+  uint32_t id;
+  V3<T> ng;
+};
+
+// Closest-approach origin shift: returns ts with o + ts*d nearest to c.
+template <class T>
+inline T shiftTo(V3<T> o, V3<T> d, V3<T> c, T A) {
+  return dot(c - o, d) / A;
+}
+
+// Sphere (RTC_GEOMETRY_TYPE_SPHERE_POINT): near root if in range, else far.
+template <class T>
+inline bool isectSphere(V3<T> o, V3<T> d, T tnear, T tfar, V3<T> c, T r, Hit<T>& h) {
+  const T A = dot(d, d);
+  const T ts = shiftTo(o, d, c, A);
+  const V3<T> f = (o + d * ts) - c;
+  const T B = dot(f, d);
+  const T C = dot(f, f) - r * r;
+  const T disc = B * B - A * C;
+  if (disc < T(0)) return false;
+  const T q = std::sqrt(disc);
+  const T t0 = ts + (-B - q) / A;
+  const T t1 = ts + (-B + q) / A;
+  T t;
+  if (tnear <= t0 && t0 <= tfar) t = t0;
+  else if (tnear <= t1 && t1 <= tfar) t = t1;
+  else return false;
+  h.t = t;
+  h.u = T(0);
+  h.v = T(0);
+  h.ng = (o + d * t) - c;
+  return true;
+}
+
+// Infinite-cone body roots in a shifted frame. Radius r(hax) = r0 + k*hax
+// along the unit axis w from a. Returns number of roots (0 or 2) in dt[].
+template <class T>
+inline int coneBody(V3<T> oa, V3<T> d, V3<T> w, T r0, T k, T dt[2]) {
+  const T ho = dot(oa, w);
+  const T hd = dot(d, w);
+  const T rr = r0 + k * ho;
+  const T A = dot(d, d) - hd * hd - k * k * hd * hd;
+  const T B = dot(oa, d) - ho * hd - k * hd * rr;
+  const T C = dot(oa, oa) - ho * ho - rr * rr;
+  if (A == T(0)) return 0;
+  const T disc = B * B - A * C;
+  if (disc < T(0)) return 0;
+  const T q = std::sqrt(disc);
+  dt[0] = (-B - q) / A;
+  dt[1] = (-B + q) / A;
+  return 2;
+}
+
+// CONE_LINEAR_CURVE: cone body between the end planes plus flat disk caps.
+// u = axial fraction, Ng = outward surface normal (caps: -w at p0, +w at p1).
+template <class T>
+inline bool isectCone(V3<T> o, V3<T> d, T tnear, T tfar, V3<T> a, T ra, V3<T> b, T rb,
+                      Hit<T>& h) {
+  const V3<T> ax = b - a;
+  const T L = length(ax);
+  if (!(L > T(0))) return false;
+  const V3<T> w = ax * (T(1) / L);
+  const T k = (rb - ra) / L;
+  const T A = dot(d, d);
+  const V3<T> mid = (a + b) * T(0.5f);
+  const T ts = shiftTo(o, d, mid, A);
+  const V3<T> oa = (o + d * ts) - a;
+  const T ho = dot(oa, w);
+  const T hd = dot(d, w);
+  bool found = false;
+  T best = tfar;
+  T bu = T(0);
+  V3<T> bn{T(0), T(0), T(0)};
+  T dt[2];
+  if (coneBody(oa, d, w, ra, k, dt) == 2) {
+    for (int i = 0; i < 2; ++i) {
+      const T t = ts + dt[i];
+      if (!(tnear <= t && t <= best)) continue;
+      const T hh = ho + dt[i] * hd;
+      if (hh < T(0) || hh > L) continue;
+      if (ra + k * hh < T(0)) continue;
+      const V3<T> p = oa + d * dt[i];
+      const V3<T> radial = p - w * hh;
+      // cone normal: radial direction tilted by the slope
+      const T rl = length(radial);
+      V3<T> n = radial;
+      if (k != T(0) && rl > T(0)) n = radial * (T(1) / rl) - w * k;
+      best = t;
+      bu = hh / L;
+      bn = n;
+      found = true;
+      break;
+    }
+  }
+  if (hd != T(0)) {
+    // cap at a: plane hax = 0
+    {
+      const T dtc = -ho / hd;
+      const T t = ts + dtc;
+      if (tnear <= t && t <= best) {
+        const V3<T> p = oa + d * dtc;
+        if (dot(p, p) <= ra * ra) {
+          best = t;
+          bu = T(0);
+          bn = neg(w);
+          found = true;
+        }
+      }
+    }
+    {
+      const T dtc = (L - ho) / hd;
+      const T t = ts + dtc;
+      if (tnear <= t && t <= best) {
+        const V3<T> p = (oa + d * dtc) - w * L;
+        if (dot(p, p) <= rb * rb) {
+          best = t;
+          bu = T(1);
+          bn = w;
+          found = true;
+        }
+      }
+    }
+  }
+  if (!found) return false;
+  h.t = best;
+  h.u = bu;
+  h.v = T(0);
+  h.ng = bn;
+  return true;
+}
+
+// ROUND_LINEAR_CURVE segment with neighbor flags: swept sphere (cone body +
+// end spheres) restricted to the segment's cell between the bisector planes of
+// its chained joints, so a chain has no internal surfaces (the dropped caps of
+// RTC_CURVE_FLAG_NEIGHBOR_LEFT/RIGHT).
+template <class T>
+inline bool isectRound(V3<T> o, V3<T> d, T tnear, T tfar, V3<T> a, T ra, V3<T> b, T rb,
+                       bool hasL, V3<T> pl, bool hasR, V3<T> pr, Hit<T>& h) {
+  const V3<T> ax = b - a;
+  const T L = length(ax);
+  if (!(L > T(0))) return false;
+  const V3<T> w = ax * (T(1) / L);
+  const T k = (rb - ra) / L;
+  V3<T> nL = w, nR = w;
+  if (hasL) nL = safeNormalize(normalize(a - pl) + w, w);
+  if (hasR) nR = safeNormalize(w + normalize(pr - b), w);
+  const T A = dot(d, d);
+  const V3<T> mid = (a + b) * T(0.5f);
+  const T ts = shiftTo(o, d, mid, A);
+  const V3<T> os = o + d * ts;
+  const V3<T> oa = os - a;
+  const T ho = dot(oa, w);
+  const T hd = dot(d, w);
+  bool found = false;
+  T best = tfar;
+  T bu = T(0);
+  V3<T> bn{T(0), T(0), T(0)};
+  auto inCell = [&](V3<T> p /* relative to a */) {
+    if (hasL && dot(p, nL) < T(0)) return false;
+    if (hasR && dot(p - ax, nR) > T(0)) return false;
+    return true;
+  };
+  auto consider = [&](T dtv, V3<T> n, T hh) {
+    const T t = ts + dtv;
+    if (!(tnear <= t && t <= best)) return;
+    const V3<T> p = oa + d * dtv;
+    if (!inCell(p)) return;
+    best = t;
+    T uu = hh / L;
+    if (uu < T(0)) uu = T(0);
+    if (uu > T(1)) uu = T(1);
+    bu = uu;
+    bn = n;
+    found = true;
+  };
+  T dt[2];
+  if (coneBody(oa, d, w, ra, k, dt) == 2) {
+    for (int i = 0; i < 2; ++i) {
+      const T hh = ho + dt[i] * hd;
+      if (hh < T(0) || hh > L) continue;
+      const V3<T> p = oa + d * dt[i];
+      consider(dt[i], p - w * hh, hh);
+    }
+  }
+  // end spheres: the part outside the body slab
+  for (int e = 0; e < 2; ++e) {
+    const V3<T> c = e == 0 ? V3<T>{T(0), T(0), T(0)} : ax;
+    const T r = e == 0 ? ra : rb;
+    const V3<T> f = oa - c;
+    const T B = dot(f, d);
+    const T C = dot(f, f) - r * r;
+    const T disc = B * B - A * C;
+    if (disc < T(0)) continue;
+    const T q = std::sqrt(disc);
+    const T r0 = (-B - q) / A, r1 = (-B + q) / A;
+    for (T dtv : {r0, r1}) {
+      const T hh = ho + dtv * hd;
+      if (e == 0 && hh >= T(0)) continue;
+      if (e == 1 && hh <= L) continue;
+      const V3<T> p = oa + d * dtv;
+      consider(dtv, p - c, hh);
+    }
+  }
+  if (!found) return false;
+  h.t = best;
+  h.u = bu;
+  h.v = T(0);
+  h.ng = bn;
+  return true;
+}
+
+// TRIANGLE: Moller-Trumbore; P = (1-u-v) v0 + u v1 + v v2; Ng = (v1-v0)x(v2-v0).
+template <class T>
+inline bool isectTri(V3<T> o, V3<T> d, T tnear, T tfar, V3<T> v0, V3<T> v1, V3<T> v2,
+                     Hit<T>& h) {
+  const V3<T> e1 = v1 - v0;
+  const V3<T> e2 = v2 - v0;
+  const V3<T> pv = cross(d, e2);
+  const T det = dot(e1, pv);
+  if (det == T(0)) return false;
+  const T inv = T(1) / det;
+  const V3<T> tv = o - v0;
+  const T u = dot(tv, pv) * inv;
+  if (u < T(0) || u > T(1)) return false;
+  const V3<T> qv = cross(tv, e1);
+  const T v = dot(d, qv) * inv;
+  if (v < T(0) || u + v > T(1)) return false;
+  const T t = dot(e2, qv) * inv;
+  if (!(tnear <= t && t <= tfar)) return false;
+  h.t = t;
+  h.u = u;
+  h.v = v;
+  h.ng = cross(e1, e2);
+  return true;
+}
+
+// ---------------------------------------------------------------------------
+// BVH over all primitives (binned SAH, 16 bins). Only speed depends on it.
+
+struct Box {
+  double lo[3], hi[3];
+};
+struct PRef {
+  int kind;
+  uint32_t id;
+};
+struct BNode {
+  Box b;
+  int32_t left = -1, right = -1;
+  int32_t first = 0, count = 0;
+};
+
+class Bvh {
+ public:
+  std::vector<BNode> nodes;
+  std::vector<PRef> refs;
+  void build(std::vector<PRef> prefs, std::vector<Box> boxes) {
+    refs = std::move(prefs);
+    boxes_ = std::move(boxes);
+    order_.resize(refs.size());
+    for (std::size_t i = 0; i < order_.size(); ++i) order_[i] = static_cast<int32_t>(i);
+    cent_.resize(refs.size());
+    for (std::size_t i = 0; i < refs.size(); ++i)
+      for (int k = 0; k < 3; ++k) cent_[i][k] = 0.5 * (boxes_[i].lo[k] + boxes_[i].hi[k]);
+    nodes.clear();
+    if (refs.empty()) return;
+    nodes.reserve(2 * refs.size());
+    nodes.push_back(BNode{});
+    buildRec(0, 0, static_cast<int32_t>(refs.size()), 0);
+    std::vector<PRef> r2(refs.size());
+    for (std::size_t i = 0; i < refs.size(); ++i) r2[i] = refs[order_[i]];
+    refs.swap(r2);
+  }
+
+ private:
+  std::vector<Box> boxes_;
+  std::vector<int32_t> order_;
+  std::vector<std::array<double, 3>> cent_;
+  static double area(const Box& b) {
+    const double dx = b.hi[0] - b.lo[0], dy = b.hi[1] - b.lo[1], dz = b.hi[2] - b.lo[2];
+    if (dx < 0) return 0;
+    return 2 * (dx * dy + dy * dz + dz * dx);
+  }
+  static void grow(Box& a, const Box& b) {
+    for (int k = 0; k < 3; ++k) {
+      a.lo[k] = std::min(a.lo[k], b.lo[k]);
+      a.hi[k] = std::max(a.hi[k], b.hi[k]);
+    }
+  }
+  static Box empty() {
+    Box b;
+    for (int k = 0; k < 3; ++k) {
+      b.lo[k] = 1e300;
+      b.hi[k] = -1e300;
+    }
+    return b;
+  }
+  void buildRec(int32_t ni, int32_t first, int32_t count, int depth) {
+    Box b = empty(), cb = empty();
+    for (int32_t i = first; i < first + count; ++i) {
+      grow(b, boxes_[order_[i]]);
+      for (int k = 0; k < 3; ++k) {
+        cb.lo[k] = std::min(cb.lo[k], cent_[order_[i]][k]);
+        cb.hi[k] = std::max(cb.hi[k], cent_[order_[i]][k]);
+      }
+    }
+    nodes[ni].b = b;
+    if (count <= 2 || depth > 60) {
+      nodes[ni].first = first;
+      nodes[ni].count = count;
+      return;
+    }
+    constexpr int NB = 16;
+    double bestCost = 1e300;
+    int bestAxis = -1, bestSplit = -1;
+    for (int ax = 0; ax < 3; ++ax) {
+      const double ext = cb.hi[ax] - cb.lo[ax];
+      if (!(ext > 0)) continue;
+      Box bb[NB];
+      int cnt[NB] = {0};
+      for (int i = 0; i < NB; ++i) bb[i] = empty();
+      for (int32_t i = first; i < first + count; ++i) {
+        int bi = static_cast<int>(NB * (cent_[order_[i]][ax] - cb.lo[ax]) / ext);
+        bi = std::min(NB - 1, std::max(0, bi));
+        cnt[bi]++;
+        grow(bb[bi], boxes_[order_[i]]);
+      }
+      double la[NB], ra[NB];
+      int lc[NB], rc[NB];
+      Box acc = empty();
+      int c = 0;
+      for (int i = 0; i < NB; ++i) {
+        grow(acc, bb[i]);
+        c += cnt[i];
+        la[i] = area(acc);
+        lc[i] = c;
+      }
+      acc = empty();
+      c = 0;
+      for (int i = NB - 1; i >= 0; --i) {
+        grow(acc, bb[i]);
+        c += cnt[i];
+        ra[i] = area(acc);
+        rc[i] = c;
+      }
+      for (int i = 0; i < NB - 1; ++i) {
+        if (lc[i] == 0 || rc[i + 1] == 0) continue;
+        const double cost = la[i] * lc[i] + ra[i + 1] * rc[i + 1];
+        if (cost < bestCost) {
+          bestCost = cost;
+          bestAxis = ax;
+          bestSplit = i;
+        }
+      }
+    }
+    const double leafCost = area(b) * count;
+    if (bestAxis < 0 || (count <= 4 && bestCost >= leafCost)) {
+      nodes[ni].first = first;
+      nodes[ni].count = count;
+      return;
+    }
+    const double ext = cb.hi[bestAxis] - cb.lo[bestAxis];
+    auto mid = std::partition(order_.begin() + first, order_.begin() + first + count, [&](int32_t p) {
+      int bi = static_cast<int>(NB * (cent_[p][bestAxis] - cb.lo[bestAxis]) / ext);
+      bi = std::min(NB - 1, std::max(0, bi));
+      return bi <= bestSplit;
+    });
+    int32_t nl = static_cast<int32_t>(mid - (order_.begin() + first));
+    if (nl == 0 || nl == count) nl = count / 2;
+    const int32_t l = static_cast<int32_t>(nodes.size());
+    nodes.push_back(BNode{});
+    nodes.push_back(BNode{});
+    nodes[ni].left = l;
+    nodes[ni].right = l + 1;
+    buildRec(l, first, nl, depth + 1);
+    buildRec(l + 1, first + nl, count - nl, depth + 1);
+  }
+};
+
+// ---------------------------------------------------------------------------
+// Scene with acceleration structure.
+
+struct OScene {
+  Geometry g;
+  Bvh bvh;
+  bool brute = false;
+  std::vector<PRef> all;  // geomID order: mesh, spheres, round, cone
+};
+
+static Box primBox(const Geometry& g, const PRef& p) {
+  Box b;
+  for (int k = 0; k < 3; ++k) {
+    b.lo[k] = 1e300;
+    b.hi[k] = -1e300;
+  }
+  auto ext = [&](double x, double y, double z, double r) {
+    const double c[3] = {x, y, z};
+    for (int k = 0; k < 3; ++k) {
+      b.lo[k] = std::min(b.lo[k], c[k] - r);
+      b.hi[k] = std::max(b.hi[k], c[k] + r);
+    }
+  };
+  if (p.kind == UBG_KIND_MESH) {
+    for (int c = 0; c < 3; ++c) {
+      const uint32_t vi = g.idx[3 * p.id + c];
+      ext(g.pos[3 * vi], g.pos[3 * vi + 1], g.pos[3 * vi + 2], 0.0);
+    }
+  } else if (p.kind == UBG_KIND_SPHERE) {
+    const float* s = &g.sph[4 * p.id];
+    ext(s[0], s[1], s[2], s[3]);
+  } else if (p.kind == UBG_KIND_ROUND_CURVE) {
+    const uint32_t v = g.rseg[p.id];
+    const float* a = &g.rv[4 * v];
+    ext(a[0], a[1], a[2], a[3]);
+    ext(a[4], a[5], a[6], a[7]);
+  } else {
+    const float* c = &g.cone[8 * p.id];
+    ext(c[0], c[1], c[2], c[3]);
+    ext(c[4], c[5], c[6], c[7]);
+  }
+  // conservative inflation (covers float rounding in the intersectors)
+  for (int k = 0; k < 3; ++k) {
+    const double m = std::max(std::fabs(b.lo[k]), std::fabs(b.hi[k]));
+    b.lo[k] -= 1e-5 * m + 1e-6;
+    b.hi[k] += 1e-5 * m + 1e-6;
+  }
+  return b;
+}
+
+static void buildScene(const ubg_scene_desc* d, bool brute, OScene& s) {
+  buildGeometry(d, s.g);
+  const Geometry& g = s.g;
+  for (uint32_t i = 0; i < g.idx.size() / 3; ++i) s.all.push_back({UBG_KIND_MESH, i});
+  for (uint32_t i = 0; i < g.sph.size() / 4; ++i) s.all.push_back({UBG_KIND_SPHERE, i});
+  for (uint32_t i = 0; i < g.rseg.size(); ++i) s.all.push_back({UBG_KIND_ROUND_CURVE, i});
+  for (uint32_t i = 0; i < g.cone.size() / 8; ++i) s.all.push_back({UBG_KIND_CONE_CURVE, i});
+  s.brute = brute;
+  if (!brute) {
+    std::vector<Box> boxes;
+    boxes.reserve(s.all.size());
+    for (const auto& p : s.all) boxes.push_back(primBox(g, p));
+    s.bvh.build(s.all, std::move(boxes));
+  }
+}
+
+template <class T>
+static bool isectPrim(const Geometry& g, const PRef& p, V3<T> o, V3<T> d, T tnear, T tfar,
+                      Hit<T>& h) {
+  if (p.kind == UBG_KIND_MESH) {
+    const uint32_t* ix = &g.idx[3 * p.id];
+    return isectTri(o, d, tnear, tfar, v3<T>(&g.pos[3 * ix[0]]), v3<T>(&g.pos[3 * ix[1]]),
+                    v3<T>(&g.pos[3 * ix[2]]), h);
+  }
+  if (p.kind == UBG_KIND_SPHERE) {
+    const float* s = &g.sph[4 * p.id];
+    return isectSphere(o, d, tnear, tfar, v3<T>(s), T(s[3]), h);
+  }
+  if (p.kind == UBG_KIND_ROUND_CURVE) {
+    const uint32_t v = g.rseg[p.id];
+    const uint8_t fl = g.rflag[p.id];
+    const float* a = &g.rv[4 * v];
+    const V3<T> pl = (fl & 1) ? v3<T>(&g.rv[4 * (v - 1)]) : V3<T>{T(0), T(0), T(0)};
+    const V3<T> pr = (fl & 2) ? v3<T>(&g.rv[4 * (v + 2)]) : V3<T>{T(0), T(0), T(0)};
+    return isectRound(o, d, tnear, tfar, v3<T>(a), T(a[3]), v3<T>(a + 4), T(a[7]), (fl & 1) != 0,
+                      pl, (fl & 2) != 0, pr, h);
+  }
+  const float* c = &g.cone[8 * p.id];
+  return isectCone(o, d, tnear, tfar, v3<T>(c), T(c[3]), v3<T>(c + 4), T(c[7]), h);
+}
+
+static inline bool boxHit(const Box& b, const double o[3], const double inv[3], double t0,
+                          double t1) {
+  for (int k = 0; k < 3; ++k) {
+    double a = (b.lo[k] - o[k]) * inv[k];
+    double c = (b.hi[k] - o[k]) * inv[k];
+    if (std::isnan(a)) a = -1e300;
+    if (std::isnan(c)) c = 1e300;
+    if (a > c) std::swap(a, c);
+    t0 = std::max(t0, a);
+    t1 = std::min(t1, c);
+  }
+  return t0 <= t1 * (1.0 + 1e-9) + 1e-12;
+}
+
+// rtcIntersect1 analogue: closest hit in [tnear, tfar]; ties keep the first
+// primitive found.
+template <class T>
+static Hit<T> intersect(const OScene& s, V3<T> o, V3<T> d, T tnear, T tfar) {
+  Hit<T> best;
+  best.kind = -1;
+  best.t = tfar;
+  Hit<T> h;
+  auto test = [&](const PRef& p) {
+    if (isectPrim(s.g, p, o, d, tnear, best.t, h) && (best.kind < 0 || h.t < best.t)) {
+      best = h;
+      best.kind = p.kind;
+      best.id = p.id;
+    }
+  };
+  if (s.brute) {
+    for (const auto& p : s.all) test(p);
+    return best;
+  }
+  if (s.bvh.nodes.empty()) return best;
+  const double od[3] = {double(o.x), double(o.y), double(o.z)};
+  const double inv[3] = {1.0 / double(d.x), 1.0 / double(d.y), 1.0 / double(d.z)};
+  int32_t stack[128];
+  int sp = 0;
+  stack[sp++] = 0;
+  while (sp) {
+    const BNode& n = s.bvh.nodes[stack[--sp]];
+    const double tb = best.kind < 0 ? double(tfar) : double(best.t);
+    if (!boxHit(n.b, od, inv, double(tnear), tb)) continue;
+    if (n.count) {
+      for (int32_t i = n.first; i < n.first + n.count; ++i) test(s.bvh.refs[i]);
+    } else {
+      stack[sp++] = n.right;
+      stack[sp++] = n.left;
+    }
+  }
+  return best;
+}
+
+// rtcOccluded1 analogue.
+template <class T>
+static bool occluded(const OScene& s, V3<T> o, V3<T> d, T tnear, T tfar) {
+  Hit<T> h;
+  if (s.brute) {
+    for (const auto& p : s.all)
+      if (isectPrim(s.g, p, o, d, tnear, tfar, h)) return true;
+    return false;
+  }
+  if (s.bvh.nodes.empty()) return false;
+  const double od[3] = {double(o.x), double(o.y), double(o.z)};
+  const double inv[3] = {1.0 / double(d.x), 1.0 / double(d.y), 1.0 / double(d.z)};
+  int32_t stack[128];
+  int sp = 0;
+  stack[sp++] = 0;
+  while (sp) {
+    const BNode& n = s.bvh.nodes[stack[--sp]];
+    if (!boxHit(n.b, od, inv, double(tnear), double(tfar))) continue;
+    if (n.count) {
+      for (int32_t i = n.first; i < n.first + n.count; ++i)
+        if (isectPrim(s.g, s.bvh.refs[i], o, d, tnear, tfar, h)) return true;
+    } else {
+      stack[sp++] = n.right;
+      stack[sp++] = n.left;
+    }
+  }
+  return false;
+}
+
+// ---------------------------------------------------------------------------
+// Shading (hit_shader.hpp:147-496, shading.hpp:76-192, secondary_rays.hpp,
+// ambient_occlusion.hpp:33-57, ao_shade.hpp:128-142).
+
+template <class T>
+struct Light {
+  V3<T> L, color;
+  bool highlight;
+  T radius;
+};
+
+struct Counters {
+  uint64_t primary = 0, shaded = 0, aoTraced = 0, aoSampled = 0, shadow = 0, capped = 0;
+};
+
+template <class T>
+struct Ctx {
+  const OScene* s;
+  const ubg_render_params* p;
+  std::vector<Light<T>> lights;
+  V3<T> ambLight, bg, camDir;
+  int wHi;
+};
+
+// secondary_rays.hpp:102-109
+template <class T>
+inline V3<T> sampleLightDir(const Light<T>& l, T u1, T u2) {
+  if (l.radius <= T(0)) return l.L;
+  const Frame<T> f = frameFromNormal(l.L);
+  const T phi = T(6.2831853072f) * u1;
+  const T rr = std::tan(l.radius) * std::sqrt(u2);
+  const V3<T> dd = f.n + f.t * (rr * std::cos(phi)) + f.b * (rr * std::sin(phi));
+  return safeNormalize(dd, l.L);
+}
+
+// secondary_rays.hpp:118-134
+template <class T>
+inline T computeShadow(const Ctx<T>& c, V3<T> P, V3<T> Ng, V3<T> N, T eps, const Light<T>& l,
+                       int shadowSamples, uint32_t& s0, uint32_t& s1, Counters& cnt) {
+  V3<T> ng = (dot(Ng, N) < T(0)) ? neg(Ng) : Ng;
+  ng = safeNormalize(ng, N);
+  const V3<T> O = P + ng * eps;
+  const T far = std::numeric_limits<T>::infinity();
+  if (shadowSamples <= 1 || l.radius <= T(0)) {
+    cnt.shadow++;
+    return occluded(*c.s, O, l.L, eps, far) ? T(0) : T(1);
+  }
+  int hits = 0;
+  for (int i = 0; i < shadowSamples; ++i) {
+    tea2(s0, s1);
+    const V3<T> dir = sampleLightDir(l, u32ToUnorm<T>(s0), u32ToUnorm<T>(s1));
+    cnt.shadow++;
+    if (occluded(*c.s, O, dir, eps, far)) ++hits;
+  }
+  return T(1) - T(hits) / T(shadowSamples);
+}
+
+// ambient_occlusion.hpp:33-57
+template <class T>
+inline T computeAO(const Ctx<T>& c, V3<T> P, V3<T> Ng, V3<T> Ns, T eps, int nSamples, T aoRadius,
+                   uint32_t px, uint32_t py, Counters& cnt) {
+  if (nSamples <= 0) return T(1);
+  const Frame<T> f = frameFromNormal(Ns);
+  V3<T> ng = (dot(Ng, Ns) < T(0)) ? neg(Ng) : Ng;
+  ng = safeNormalize(ng, Ns);
+  const V3<T> O = P + ng * eps;
+  const uint32_t base = px + py * static_cast<uint32_t>(c.wHi);
+  int hits = 0;
+  for (int i = 0; i < nSamples; ++i) {
+    uint32_t s0 = base;
+    uint32_t s1 = static_cast<uint32_t>(i);
+    tea2(s0, s1);
+    const V3<T> dir = cosineSampleHemisphere(u32ToUnorm<T>(s0), u32ToUnorm<T>(s1), f);
+    if (dot(dir, Ns) < T(0.01f)) {
+      ++hits;
+      cnt.aoSampled++;
+      continue;
+    }
+    cnt.aoTraced++;
+    if (occluded(*c.s, O, dir, eps, aoRadius)) ++hits;
+  }
+  return T(1) - T(hits) / T(nSamples);
+}
+
+// shading.hpp:76-192 (POV model)
+template <class T>
+inline V3<T> shadeLocal(const Ctx<T>& c, const Mat& mat, V3<T> C, V3<T> N, V3<T> V,
+                        V3<T> ambLight, V3<T> aoFactor, T diffuseAo, V3<T> P, V3<T> Ng, T eps,
+                        bool shadowsOn, int shadowSamples, uint32_t px, uint32_t py,
+                        Counters& cnt) {
+  const T em = T(mat.emission), amb = T(mat.ambient);
+  V3<T> out{em * C.x + aoFactor.x * amb * C.x * ambLight.x,
+            em * C.y + aoFactor.y * amb * C.y * ambLight.y,
+            em * C.z + aoFactor.z * amb * C.z * ambLight.z};
+  const T ex = blinnExp(T(mat.roughness));
+  uint32_t s0 = px, s1 = py;
+  for (const Light<T>& l : c.lights) {
+    const T ndl = dot(N, l.L);
+    if (ndl <= T(0)) continue;
+    const T shadowFactor =
+        shadowsOn ? computeShadow(c, P, Ng, N, eps, l, shadowSamples, s0, s1, cnt) : T(1);
+    const V3<T> Lc{l.color.x * shadowFactor, l.color.y * shadowFactor, l.color.z * shadowFactor};
+    T dd;
+    if (T(mat.brilliance) == T(1)) dd = ndl;
+    else if (T(mat.brilliance) == T(0)) dd = T(1);
+    else dd = std::pow(ndl, T(mat.brilliance));
+    const T dk = T(mat.diffuse) * dd * diffuseAo;
+    out.x += dk * C.x * Lc.x;
+    out.y += dk * C.y * Lc.y;
+    out.z += dk * C.z * Lc.z;
+    if (!l.highlight) continue;
+    V3<T> hl;
+    if (mat.metallic) {
+      T cc = ndl;
+      if (cc > T(1)) cc = T(1);
+      const T x = std::acos(cc) * T(0.63661977f);
+      T f = T(0.014567225f) / ((x - T(1.12f)) * (x - T(1.12f))) - T(0.011612903f);
+      if (f < T(0)) f = T(0);
+      if (f > T(1)) f = T(1);
+      hl = V3<T>{Lc.x * (f + (T(1) - f) * C.x), Lc.y * (f + (T(1) - f) * C.y),
+                 Lc.z * (f + (T(1) - f) * C.z)};
+    } else {
+      hl = Lc;
+    }
+    T specW = T(0);
+    if (mat.specular > 0.0f) {
+      const V3<T> H = normalize(V3<T>{l.L.x + V.x, l.L.y + V.y, l.L.z + V.z});
+      const T nh = dot(N, H);
+      if (nh > T(0)) specW += T(mat.specular) * std::pow(nh, ex);
+    }
+    if (mat.phong > 0.0f) {
+      const V3<T> Rr = (T(2) * ndl) * N - l.L;
+      const T rv = dot(Rr, V);
+      if (rv > T(0) && (T(mat.phongSize) < T(60) || rv > T(0.0008f)))
+        specW += T(mat.phong) * std::pow(rv, T(mat.phongSize));
+    }
+    if (specW > T(0)) {
+      const T sc = specW * T(c.p->specular_scale);
+      out.x += sc * hl.x;
+      out.y += sc * hl.y;
+      out.z += sc * hl.z;
+    }
+  }
+  if (mat.reflection > 0.0f) {
+    const T r = T(mat.reflection);
+    out.x += r * c.bg.x;
+    out.y += r * c.bg.y;
+    out.z += r * c.bg.z;
+  }
+  return out;
+}
+
+template <class T>
+struct HitShade {
+  V3<T> color;
+  T opacity;
+  T aoOpen;
+};
+
+// ao_shade.hpp:128-142 legacy path -> (aoFactor, openness)
+template <class T>
+inline T aoOpenness(const Ctx<T>& c, V3<T> P, V3<T> Ng, V3<T> N, T eps, uint32_t px, uint32_t py,
+                    Counters& cnt) {
+  if (c.p->ao_samples <= 0) return T(1);
+  return computeAO(c, P, Ng, N, eps, c.p->ao_samples, T(c.p->ao_distance), px, py, cnt);
+}
+
+// hit_shader.hpp:147-496
+template <class T>
+inline HitShade<T> shadeHit(const Ctx<T>& c, const Hit<T>& h, V3<T> rd, V3<T> org, uint32_t px,
+                            uint32_t py, Counters& cnt) {
+  const Geometry& g = c.s->g;
+  HitShade<T> hs;
+  const bool shadowsActive = c.p->shadows != 0;
+  const V3<T> V = normalize(neg(rd));
+  if (h.kind == UBG_KIND_MESH) {
+    const uint32_t* ix = &g.idx[3 * h.id];
+    const T w0 = T(1) - h.u - h.v;
+    V3<T> nb{T(0), T(0), T(0)};
+    T cb[4];
+    for (int k = 0; k < 3; ++k) {
+      const T a0 = T(g.nrm[3 * ix[0] + k]), a1 = T(g.nrm[3 * ix[1] + k]), a2 = T(g.nrm[3 * ix[2] + k]);
+      const T val = w0 * a0 + h.u * a1 + h.v * a2;
+      if (k == 0) nb.x = val;
+      else if (k == 1) nb.y = val;
+      else nb.z = val;
+    }
+    for (int k = 0; k < 4; ++k) {
+      const T a0 = T(g.col[4 * ix[0] + k]), a1 = T(g.col[4 * ix[1] + k]), a2 = T(g.col[4 * ix[2] + k]);
+      cb[k] = w0 * a0 + h.u * a1 + h.v * a2;
+    }
+    const V3<T> Ng = h.ng;
+    const V3<T> NgF = faceForward(Ng, rd);
+    V3<T> N = normalize(nb);
+    if (dot(N, NgF) < T(0)) N = neg(N);
+    const V3<T> C{cb[0], cb[1], cb[2]};
+    const int64_t baseTri = static_cast<int64_t>(h.id % static_cast<uint32_t>(g.baseT));
+    const Mat& mat = g.triMat.empty() ? g.meshMats[0] : g.meshMats[g.triMat[baseTri]];
+    const V3<T> P{org.x + rd.x * h.t, org.y + rd.y * h.t, org.z + rd.z * h.t};
+    const T secEps = selfIntersectEps(P, rd, h.t);
+    const T open = aoOpenness(c, P, Ng, N, secEps, px, py, cnt);
+    const T s = T(1) - T(c.p->ao_intensity) * (T(1) - open);
+    hs.aoOpen = open;
+    hs.color = shadeLocal(c, mat, C, N, V, c.ambLight, V3<T>{s, s, s}, T(1), P, Ng, secEps,
+                          shadowsActive, c.p->shadow_samples, px, py, cnt);
+    hs.opacity = cb[3];
+    return hs;
+  }
+  const PrimSide* side = h.kind == UBG_KIND_SPHERE        ? &g.sphSide[h.id]
+                         : h.kind == UBG_KIND_CONE_CURVE ? &g.cSide[h.id]
+                                                         : &g.rSide[h.id];
+  const Mat& pm = g.primMats[side->mat];
+  const V3<T> N = faceForward(normalize(h.ng), rd);
+  const V3<T> C{T(side->color[0]), T(side->color[1]), T(side->color[2])};
+  const V3<T> P{org.x + rd.x * h.t, org.y + rd.y * h.t, org.z + rd.z * h.t};
+  const V3<T> Ng = h.ng;
+  const T secEps = selfIntersectEps(P, rd, h.t);
+  const bool fromEdge = side->fromEdge != 0;
+  V3<T> aoFactor{T(1), T(1), T(1)};
+  hs.aoOpen = T(1);
+  if (!fromEdge) {
+    const T open = aoOpenness(c, P, Ng, N, secEps, px, py, cnt);
+    const T s = T(1) - T(c.p->ao_intensity) * (T(1) - open);
+    aoFactor = V3<T>{s, s, s};
+    hs.aoOpen = open;
+  }
+  const bool primShadows = !fromEdge && shadowsActive;
+  hs.color = shadeLocal(c, pm, C, N, V, c.ambLight, aoFactor, T(1), P, Ng, secEps, primShadows,
+                        c.p->shadow_samples, px, py, cnt);
+  hs.opacity = T(side->color[3]);
+  if (h.kind != UBG_KIND_SPHERE && side->opacity1 >= 0.0f) {
+    T u = h.u;
+    if (u < T(0)) u = T(0);
+    if (u > T(1)) u = T(1);
+    hs.opacity = T(side->color[3]) * (T(1) - u) + T(side->opacity1) * u;
+  }
+  return hs;
+}
+
+template <class T>
+struct PixelOut {
+  T r, g, b, a, depth, viewZ, aoOpen;
+  uint32_t hitId;
+};
+
+// transparency.hpp:311-522 (grid path; clip t-range per :118-129)
+template <class T>
+inline PixelOut<T> integratePixel(const Ctx<T>& c, V3<T> org, V3<T> rd, uint32_t px, uint32_t py,
+                                  Counters& cnt) {
+  const ubg_render_params* p = c.p;
+  const T inf = std::numeric_limits<T>::infinity();
+  T clipNear = T(0), clipFar = inf;
+  {
+    const bool off = !(p->clip_near > -std::numeric_limits<float>::infinity()) &&
+                     !(p->clip_far < std::numeric_limits<float>::infinity());
+    if (!off) {
+      const T cosr = dot(rd, c.camDir);
+      if (cosr > T(1.0e-6f)) {
+        if (p->clip_near > 0.0f) clipNear = T(p->clip_near) / cosr;
+        if (p->clip_far < std::numeric_limits<float>::infinity()) clipFar = T(p->clip_far) / cosr;
+      }
+    }
+  }
+  V3<T> Cf{T(0), T(0), T(0)};
+  T Af = T(0);
+  T nearDepth = T(0);
+  T firstViewZ = T(0);
+  T firstAo = T(1);
+  uint32_t firstId = 0xFFFFFFFFu;
+  V3<T> base = c.bg;
+  T baseCov = p->transparent_background ? T(0) : T(1);
+  const T kOpaque = T(1.0f - 1e-4f);
+  T tnear = clipNear;
+  const int maxIters = p->transparency ? (p->max_transparent_layers + 1) : 1;
+  for (int iter = 0; iter < maxIters; ++iter) {
+    cnt.primary++;
+    const Hit<T> h = intersect(*c.s, org, rd, tnear, clipFar);
+    if (h.kind < 0) break;
+    cnt.shaded++;
+    const HitShade<T> hs = shadeHit(c, h, rd, org, px, py, cnt);
+    if (nearDepth == T(0)) {
+      nearDepth = h.t;
+      firstViewZ = h.t * dot(rd, c.camDir);
+      firstAo = hs.aoOpen;
+      firstId = (static_cast<uint32_t>(h.kind) << 30) | h.id;
+    }
+    if (!p->transparency || hs.opacity >= kOpaque) {
+      base = hs.color;
+      baseCov = T(1);
+      break;
+    }
+    const T w = (T(1) - Af) * hs.opacity;
+    Cf.x += w * hs.color.x;
+    Cf.y += w * hs.color.y;
+    Cf.z += w * hs.color.z;
+    Af += w;
+    if (Af >= kOpaque) break;
+    const V3<T> hitP{org.x + rd.x * h.t, org.y + rd.y * h.t, org.z + rd.z * h.t};
+    const T dirMax = std::fmax(std::fabs(rd.x), std::fmax(std::fabs(rd.y), std::fabs(rd.z)));
+    const T epsScale = std::fmax(std::fmax(std::fabs(hitP.x), std::fabs(hitP.y)),
+                                 std::fmax(std::fabs(hitP.z), dirMax * h.t));
+    tnear = h.t + epsScale * T(0x1.0fp-21f);
+    if (iter == maxIters - 1) cnt.capped++;
+  }
+  const T floorW = (T(1) - Af) * baseCov;
+  T outA = Af + floorW;
+  outA = std::fmin(T(1), std::fmax(T(0), outA));
+  PixelOut<T> o;
+  o.r = Cf.x + floorW * base.x;
+  o.g = Cf.y + floorW * base.y;
+  o.b = Cf.z + floorW * base.z;
+  o.a = outA;
+  o.depth = nearDepth;
+  o.viewZ = firstViewZ;
+  o.aoOpen = firstAo;
+  o.hitId = firstId;
+  return o;
+}
+
+// fog.cpp:7-15
+inline float fogFactor(const ubg_fog& fog, float z) {
+  if (!fog.enabled) return 1.0f;
+  const float denom = fog.end - fog.start;
+  if (denom <= 0.0f) return z < fog.end ? 1.0f : 0.0f;
+  float f = (fog.end - z) / denom;
+  if (f < 0.0f) return 0.0f;
+  if (f > 1.0f) return 1.0f;
+  return f;
+}
+
+// image_ops.cpp:9-35
+void boxDownsample(const float* src, int w, int h, int channels, int ss, float* out) {
+  const int ow = w / ss, oh = h / ss;
+  const float inv = 1.0f / static_cast<float>(ss * ss);
+  for (int y = 0; y < oh; ++y)
+    for (int x = 0; x < ow; ++x)
+      for (int c = 0; c < channels; ++c) {
+        float acc = 0.0f;
+        for (int dy = 0; dy < ss; ++dy)
+          for (int dx = 0; dx < ss; ++dx) {
+            const std::size_t si =
+                (static_cast<std::size_t>(y * ss + dy) * w + (x * ss + dx)) * channels + c;
+            const float v = src[si];
+            acc += std::isfinite(v) ? v : 0.0f;
+          }
+        out[(static_cast<std::size_t>(y) * ow + x) * channels + c] = acc * inv;
+      }
+}
+
+template <class T>
+static int renderT(const ubg_scene_desc* desc, const ubg_render_params* p, ubg_frame* fr,
+                   const orc_options* opt) {
+  OScene s;
+  buildScene(desc, opt->brute_force != 0, s);
+  const int ss = std::max(1, p->supersample);
+  const int W = p->width * ss, H = p->height * ss;
+  Ctx<T> c;
+  c.s = &s;
+  c.p = p;
+  c.wHi = W;
+  // embree_renderer.cpp:375-386 (camera basis)
+  const ubg_camera& cam = p->camera;
+  const T aspect = T(static_cast<float>(W)) / T(static_cast<float>(H));
+  const V3<T> dir = normalize(v3<T>(cam.direction));
+  const V3<T> right = normalize(cross(dir, v3<T>(cam.up)));
+  const V3<T> trueUp = normalize(cross(right, dir));
+  const T halfH = T(cam.height) * T(0.5f);
+  const T halfW = halfH * aspect;
+  const T persHalfH = std::tan(radiansT(T(cam.fovy)) * T(0.5f));
+  const T persHalfW = persHalfH * aspect;
+  c.camDir = dir;
+  // embree_renderer.cpp:436-458 (lights)
+  for (int i = 0; i < p->n_lights; ++i) {
+    const ubg_light& dl = p->lights[i];
+    Light<T> l;
+    l.L = normalize(V3<T>{-T(dl.direction[0]), -T(dl.direction[1]), -T(dl.direction[2])});
+    l.color = V3<T>{T(dl.color[0]) * T(dl.intensity), T(dl.color[1]) * T(dl.intensity),
+                    T(dl.color[2]) * T(dl.intensity)};
+    l.highlight = dl.casts_highlight != 0;
+    l.radius = (p->gi_integrator == 2 && dl.angular_radius > 0.0f) ? T(dl.angular_radius)
+                                                                   : radiansT(T(p->light_radius_deg));
+    c.lights.push_back(l);
+  }
+  c.ambLight = v3<T>(p->ambient_color);
+  c.bg = v3<T>(p->background);
+  const V3<T> pos = v3<T>(cam.position);
+
+  const std::size_t npx = static_cast<std::size_t>(W) * H;
+  std::vector<float> hiColor(npx * 4, 0.0f), hiDepth(npx, 0.0f), hiViewZ(npx, 0.0f),
+      hiAo(npx, 1.0f);
+  std::vector<uint32_t> hiId(npx, 0xFFFFFFFFu);
+  const int rowB = std::max(0, opt->row_begin);
+  const int rowE = opt->row_end > 0 ? std::min(H, opt->row_end) : H;
+  const int rowStep = opt->row_step > 0 ? opt->row_step : 1;
+  std::vector<int> rows;
+  for (int r = rowB; r < rowE; r += rowStep) rows.push_back(r);
+  int nth = opt->threads > 0 ? opt->threads : static_cast<int>(std::thread::hardware_concurrency());
+  if (nth < 1) nth = 1;
+  std::atomic<std::size_t> nextRow{0};
+  std::vector<Counters> cnts(nth);
+  const auto t0 = std::chrono::steady_clock::now();
+  auto worker = [&](int tid) {
+    Counters& cnt = cnts[tid];
+    for (;;) {
+      const std::size_t ri = nextRow.fetch_add(1);
+      if (ri >= rows.size()) break;
+      const int py = rows[ri];
+      // embree_renderer.cpp:1400-1414 (primary rays, top-left origin)
+      const T v = T(1) - T(2) * (T(static_cast<float>(py)) + T(0.5f)) / T(static_cast<float>(H));
+      for (int px = 0; px < W; ++px) {
+        const T u = T(2) * (T(static_cast<float>(px)) + T(0.5f)) / T(static_cast<float>(W)) - T(1);
+        V3<T> org, rd;
+        if (cam.orthographic) {
+          org = pos + right * (u * halfW) + trueUp * (v * halfH);
+          rd = dir;
+        } else {
+          org = pos;
+          rd = normalize(dir + right * (u * persHalfW) + trueUp * (v * persHalfH));
+        }
+        const PixelOut<T> po = integratePixel(c, org, rd, static_cast<uint32_t>(px),
+                                              static_cast<uint32_t>(py), cnt);
+        const std::size_t pix = static_cast<std::size_t>(py) * W + px;
+        hiColor[4 * pix + 0] = static_cast<float>(po.r);
+        hiColor[4 * pix + 1] = static_cast<float>(po.g);
+        hiColor[4 * pix + 2] = static_cast<float>(po.b);
+        hiColor[4 * pix + 3] = static_cast<float>(po.a);
+        hiDepth[pix] = static_cast<float>(po.depth);
+        hiViewZ[pix] = static_cast<float>(po.viewZ);
+        hiAo[pix] = static_cast<float>(po.aoOpen);
+        hiId[pix] = po.hitId;
+      }
+    }
+  };
+  std::vector<std::thread> pool;
+  for (int i = 1; i < nth; ++i) pool.emplace_back(worker, i);
+  worker(0);
+  for (auto& t : pool) t.join();
+  const auto t1 = std::chrono::steady_clock::now();
+  // pipeline.cpp:126-129 -> fog.cpp:17-36 (hi-res, before the downsample)
+  if (p->fog.enabled) {
+    for (std::size_t i = 0; i < npx; ++i) {
+      const float z = hiViewZ[i];
+      if (z <= 0.0f) continue;
+      const float f = fogFactor(p->fog, z);
+      float* col = &hiColor[4 * i];
+      if (p->transparent_background) {
+        col[3] *= f;
+      } else {
+        col[0] = p->fog.color[0] * (1.0f - f) + col[0] * f;
+        col[1] = p->fog.color[1] * (1.0f - f) + col[1] * f;
+        col[2] = p->fog.color[2] * (1.0f - f) + col[2] * f;
+      }
+    }
+  }
+  const auto t2 = std::chrono::steady_clock::now();
+  Counters tot;
+  for (const auto& k : cnts) {
+    tot.primary += k.primary;
+    tot.shaded += k.shaded;
+    tot.aoTraced += k.aoTraced;
+    tot.aoSampled += k.aoSampled;
+    tot.shadow += k.shadow;
+    tot.capped += k.capped;
+  }
+  fr->hi_w = W;
+  fr->hi_h = H;
+  fr->stats.primary_rays = tot.primary;
+  fr->stats.shaded_hits = tot.shaded;
+  fr->stats.ao_rays_traced = tot.aoTraced;
+  fr->stats.ao_rays_sampled = tot.aoSampled;
+  fr->stats.shadow_rays = tot.shadow;
+  fr->stats.capped_rays = tot.capped;
+  fr->render_seconds = std::chrono::duration<double>(t2 - t0).count();
+  (void)t1;
+  fr->cancelled = 0;
+  if (fr->depth) std::memcpy(fr->depth, hiDepth.data(), npx * sizeof(float));
+  if (fr->view_z) std::memcpy(fr->view_z, hiViewZ.data(), npx * sizeof(float));
+  if (fr->hit_id) std::memcpy(fr->hit_id, hiId.data(), npx * sizeof(uint32_t));
+  if (fr->ao_open) std::memcpy(fr->ao_open, hiAo.data(), npx * sizeof(float));
+  if (p->flags & UBG_FLAG_HIRES_ONLY) {
+    fr->out_w = W;
+    fr->out_h = H;
+    if (fr->color) std::memcpy(fr->color, hiColor.data(), npx * 4 * sizeof(float));
+    fr->post_seconds = 0.0;
+    return UBG_OK;
+  }
+  // pipeline.cpp:182-223 (box downsample, ss > 1) and :254 (assumed gamma)
+  const int ow = p->width, oh = p->height;
+  std::vector<float> out;
+  if (ss > 1) {
+    out.assign(static_cast<std::size_t>(ow) * oh * 4, 0.0f);
+    boxDownsample(hiColor.data(), W, H, 4, ss, out.data());
+  } else {
+    out.swap(hiColor);
+  }
+  // image_ops.cpp:37-46
+  const float gm = p->assumed_gamma;
+  if (!(std::fabs(gm - 1.0f) <= 1.0e-4f)) {
+    const std::size_t n = static_cast<std::size_t>(ow) * oh;
+    for (std::size_t i = 0; i < n; ++i)
+      for (int k = 0; k < 3; ++k) {
+        const float vv = out[4 * i + k];
+        out[4 * i + k] = (vv > 0.0f) ? std::pow(vv, gm) : 0.0f;
+      }
+  }
+  const auto t3 = std::chrono::steady_clock::now();
+  fr->post_seconds = std::chrono::duration<double>(t3 - t2).count();
+  fr->out_w = ow;
+  fr->out_h = oh;
+  if (fr->color) std::memcpy(fr->color, out.data(), out.size() * sizeof(float));
+  return UBG_OK;
+}
+
+template <class T>
+static int intersectT(const ubg_scene_desc* d, int64_t n, const ubg_ray* rays, ubg_hit* hits,
+                      const orc_options* opt) {
+  OScene s;
+  buildScene(d, opt->brute_force != 0, s);
+  for (int64_t i = 0; i < n; ++i) {
+    const ubg_ray& r = rays[i];
+    const Hit<T> h = intersect(s, v3<T>(r.org), v3<T>(r.dir), T(r.tnear), T(r.tfar));
+    ubg_hit& o = hits[i];
+    if (h.kind < 0) {
+      o.t = std::numeric_limits<float>::infinity();
+      o.u = o.v = 0.0f;
+      o.kind = 0xFFFFFFFFu;
+      o.prim_id = 0xFFFFFFFFu;
+      o.ng[0] = o.ng[1] = o.ng[2] = 0.0f;
+    } else {
+      o.t = static_cast<float>(h.t);
+      o.u = static_cast<float>(h.u);
+      o.v = static_cast<float>(h.v);
+      o.kind = static_cast<uint32_t>(h.kind);
+      o.prim_id = h.id;
+      o.ng[0] = static_cast<float>(h.ng.x);
+      o.ng[1] = static_cast<float>(h.ng.y);
+      o.ng[2] = static_cast<float>(h.ng.z);
+    }
+  }
+  return UBG_OK;
+}
+
+template <class T>
+static int occludedT(const ubg_scene_desc* d, int64_t n, const ubg_ray* rays, uint8_t* out,
+                     const orc_options* opt) {
+  OScene s;
+  buildScene(d, opt->brute_force != 0, s);
+  for (int64_t i = 0; i < n; ++i) {
+    const ubg_ray& r = rays[i];
+    out[i] = occluded(s, v3<T>(r.org), v3<T>(r.dir), T(r.tnear), T(r.tfar)) ? 1 : 0;
+  }
+  return UBG_OK;
+}
+
+}  // namespace orc
+
+extern "C" {
+
+static const orc_options kDefaultOpt = {32, 0, 0, 0, 1, 0};
+
+int orc_render(const ubg_scene_desc* desc, const ubg_render_params* params, ubg_frame* frame,
+               const orc_options* opt) {
+  if (!desc || !params || !frame) return UBG_ERR_INVALID_ARGUMENT;
+  if (params->width <= 0 || params->height <= 0) return UBG_ERR_INVALID_ARGUMENT;
+  const orc_options* o = opt ? opt : &kDefaultOpt;
+  for (int32_t i = 0; i < desc->n_prim_materials; ++i)
+    if (desc->prim_materials[i].model == 1) return UBG_ERR_UNSUPPORTED;
+  for (int32_t i = 0; i < desc->n_mesh_materials; ++i)
+    if (desc->mesh_materials[i].model == 1) return UBG_ERR_UNSUPPORTED;
+  return o->precision == 64 ? orc::renderT<double>(desc, params, frame, o)
+                            : orc::renderT<float>(desc, params, frame, o);
+}
+
+int orc_intersect(const ubg_scene_desc* desc, int64_t n, const ubg_ray* rays, ubg_hit* hits,
+                  const orc_options* opt) {
+  const orc_options* o = opt ? opt : &kDefaultOpt;
+  return o->precision == 64 ? orc::intersectT<double>(desc, n, rays, hits, o)
+                            : orc::intersectT<float>(desc, n, rays, hits, o);
+}
+
+int orc_occluded(const ubg_scene_desc* desc, int64_t n, const ubg_ray* rays, uint8_t* occ,
+                 const orc_options* opt) {
+  const orc_options* o = opt ? opt : &kDefaultOpt;
+  return o->precision == 64 ? orc::occludedT<double>(desc, n, rays, occ, o)
+                            : orc::occludedT<float>(desc, n, rays, occ, o);
+}
+
+int64_t orc_chain_curves(const ubg_scene_desc* desc, int64_t cap, int32_t* src, uint8_t* flags) {
+  orc::Geometry g;
+  orc::buildGeometry(desc, g);
+  const int64_t n = static_cast<int64_t>(g.rseg.size());
+  if (n > cap) return -1;
+  for (int64_t i = 0; i < n; ++i) {
+    src[i] = g.rsrc[i];
+    flags[i] = g.rflag[i];
+  }
+  return n;
+}
+
+void orc_tea2(uint32_t* v0, uint32_t* v1) { orc::tea2(*v0, *v1); }
+
+float orc_fog_factor(const ubg_fog* fog, float z) { return orc::fogFactor(*fog, z); }
+
+void orc_box_downsample(const float* src, int w, int h, int channels, int ss, float* dst) {
+  orc::boxDownsample(src, w, h, channels, ss, dst);
+}
+
+// image_ops.cpp:48-58
+float orc_srgb_encode(float v) {
+  v = std::min(1.0f, std::max(0.0f, v));
+  return (v <= 0.0031308f) ? 12.92f * v : 1.055f * std::pow(v, 1.0f / 2.4f) - 0.055f;
+}
+float orc_srgb_decode(float s) {
+  s = std::min(1.0f, std::max(0.0f, s));
+  return (s <= 0.04045f) ? s / 12.92f : std::pow((s + 0.055f) / 1.055f, 2.4f);
+}
+
+}  // extern "C"

@@ -1,0 +1,63 @@
+/*
+ * oracle.h -- CPU restatement of libumbreon's per-ray path (TEST INFRASTRUCTURE).
+ *
+ * This library is the correctness oracle and the CPU baseline of the MI355X
+ * backend. Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg
+ * may load it; the product (umbreon_amd) never links or calls it.
+ *
+ * It consumes the same flattened scene / parameter structs as the GPU C-ABI
+ * (include/umbreon_gpu.h) so both sides see byte-identical inputs, and it
+ * implements everything itself: its own curve chaining, its own binned-SAH
+ * BVH (or brute force), its own intersectors and the restated shading chain.
+ * precision 32 follows the reference's float-op order; precision 64 is the
+ * FP64 oracle of the same integrator.
+ */
+#ifndef UMBREON_ORACLE_H
+#define UMBREON_ORACLE_H
+
+#include "../include/umbreon_gpu.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct orc_options {
+  int32_t precision;    /* 32 or 64 */
+  int32_t threads;      /* 0 = hardware concurrency */
+  int32_t row_begin;    /* hi-res rows [row_begin, row_end) with step row_step; */
+  int32_t row_end;      /*   row_end <= 0 means all rows */
+  int32_t row_step;     /*   <= 0 means 1 */
+  int32_t brute_force;  /* 1 = test every primitive (no BVH) */
+} orc_options;
+
+/* Full frame (renderFrame trace/fog/resolve). Outputs are host pointers. */
+int orc_render(const ubg_scene_desc* desc, const ubg_render_params* params,
+               ubg_frame* frame, const orc_options* opt);
+
+/* Batched rtcIntersect1 / rtcOccluded1 analogues. */
+int orc_intersect(const ubg_scene_desc* desc, int64_t n, const ubg_ray* rays,
+                  ubg_hit* hits, const orc_options* opt);
+int orc_occluded(const ubg_scene_desc* desc, int64_t n, const ubg_ray* rays,
+                 uint8_t* occluded, const orc_options* opt);
+
+/* Curve chaining result (curve_build.cpp:88-199): per round segment, the
+ * source cylinder index and the neighbor flags (bit0 LEFT, bit1 RIGHT), in
+ * primID order. Returns the segment count, or -1 if `cap` is too small. */
+int64_t orc_chain_curves(const ubg_scene_desc* desc, int64_t cap,
+                         int32_t* src_cylinder, uint8_t* flags);
+
+/* Reference TEA-8 (secondary_rays.hpp:51-58), exposed for known-answer tests. */
+void orc_tea2(uint32_t* v0, uint32_t* v1);
+
+/* Post-process helpers (fog.cpp:7-36, image_ops.cpp:9-79). */
+float orc_fog_factor(const ubg_fog* fog, float z);
+void orc_box_downsample(const float* src, int w, int h, int channels, int ss,
+                        float* dst);
+float orc_srgb_encode(float v);
+float orc_srgb_decode(float s);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,598 @@
+"""Known-answer cases restated from the reference's own test suite.
+
+Each case builds one or more small synthetic scenes (the reference's tests use
+5x5 ortho views of quads and spheres with closed-form centre pixels) and a
+check over the rendered frames. The same cases run against the CPU oracle
+(tests/test_oracle.py) and the gfx950 path (tests/test_gpu_refcases.py).
+
+Fixture builders follow tests/render_test_util.hpp:14-71 of the reference;
+every case cites the assertion it restates.
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass
+from typing import Callable, Dict, Tuple
+
+import numpy as np
+
+from umbreon_amd.scene import Camera, DistantLight, Fog, Material, RenderOptions, Scene
+
+PIGMENT = (0.5, 0.6, 0.7, 1.0)
+C = (2, 2)  # centre pixel (row, col) of a 5x5 frame
+
+
+def quad_mesh(color, z=0.0):
+    """makeQuad (render_test_util.hpp:14-28): [-2,2]^2 at z, +Z, 2 tris."""
+    p = [(-2, -2, z), (2, -2, z), (2, 2, z), (-2, -2, z), (2, 2, z), (-2, 2, z)]
+    return p, [(0, 0, 1)] * 6, [tuple(color)] * 6
+
+
+def ortho_cam(z=10.0) -> Camera:
+    """makeOrthoCam (render_test_util.hpp:32-40)."""
+    return Camera(position=(0, 0, z), direction=(0, 0, -1), up=(0, 1, 0), orthographic=True, height=4.0)
+
+
+def key_light() -> DistantLight:
+    """makeKeyLight (render_test_util.hpp:45-51): head-on, intensity 0.5."""
+    return DistantLight(direction=(0, 0, -1), color=(1, 1, 1), intensity=0.5)
+
+
+def quad_scene(color=PIGMENT, bg=(0.4, 0.1, 0.2)) -> Scene:
+    sc = Scene()
+    p, n, c = quad_mesh(color)
+    sc.set_mesh(p, n, c, material=Material(ambient=0.2, diffuse=0.8))
+    sc.camera = ortho_cam()
+    sc.lights = [key_light()]
+    sc.background = bg
+    return sc
+
+
+def material_sphere_scene(color, mat: Material, bg) -> Scene:
+    """makeMaterialSphereScene (render_test_util.hpp:57-71)."""
+    sc = Scene()
+    sc.camera = ortho_cam()
+    sc.lights = [key_light()]
+    sc.background = bg
+    sc.add_sphere((0, 0, 0), 1.0, color, mat)
+    return sc
+
+
+def opts(w=5, h=5, **kw) -> RenderOptions:
+    return RenderOptions(width=w, height=h, **kw)
+
+
+@dataclass
+class Case:
+    name: str
+    renders: Dict[str, Tuple[Scene, RenderOptions]]
+    check: Callable[[dict], None]
+    # the GPU must match the oracle per pixel within this max abs error
+    tol: float = 1e-4
+
+
+def approx(a, b, eps):
+    assert abs(float(a) - float(b)) <= eps, f"{a} != {b} (+-{eps})"
+
+
+CASES = []
+
+
+def case(fn):
+    CASES.append(fn())
+    return fn
+
+
+# ---------------------------------------------------------------- shading
+@case
+def mesh_direct():
+    # test_render_shading.cpp:19-36
+    def chk(r):
+        f = r["f"]
+        approx(f.color[C][0], 0.30, 1e-4)
+        approx(f.color[C][1], 0.36, 1e-4)
+        approx(f.color[C][2], 0.42, 1e-4)
+        approx(f.color[C][3], 1.0, 1e-6)
+        approx(f.depth[C], 10.0, 1e-3)
+    return Case("mesh_direct", {"f": (quad_scene(), opts())}, chk)
+
+
+@case
+def background_miss():
+    # test_render_shading.cpp:39-53
+    sc = Scene()
+    sc.camera = ortho_cam()
+    sc.lights = [key_light()]
+    sc.background = (0.4, 0.1, 0.2)
+
+    def chk(r):
+        f = r["f"]
+        approx(f.color[C][0], 0.4, 1e-6)
+        approx(f.color[C][1], 0.1, 1e-6)
+        approx(f.color[C][2], 0.2, 1e-6)
+        approx(f.depth[C], 0.0, 1e-6)
+    return Case("background_miss", {"f": (sc, opts())}, chk)
+
+
+@case
+def outline_sphere():
+    # test_render_shading.cpp:58-76 (default flatOutline material = raw colour)
+    sc = Scene()
+    sc.camera = ortho_cam()
+    sc.lights = [key_light()]
+    sc.background = (0.4, 0.1, 0.2)
+    sc.add_sphere((0, 0, 0), 1.0, (0.3, 0.4, 0.5, 1.0))
+
+    def chk(r):
+        f = r["f"]
+        approx(f.color[C][0], 0.3, 1e-4)
+        approx(f.color[C][1], 0.4, 1e-4)
+        approx(f.color[C][2], 0.5, 1e-4)
+    return Case("outline_sphere", {"f": (sc, opts())}, chk)
+
+
+@case
+def assumed_gamma():
+    # test_render_shading.cpp:80-94
+    sc = quad_scene(bg=(0, 0, 0))
+    sc.assumed_gamma = 2.0
+
+    def chk(r):
+        f = r["f"]
+        approx(f.color[C][0], 0.30 * 0.30, 1e-4)
+        approx(f.color[C][1], 0.36 * 0.36, 1e-4)
+        approx(f.color[C][3], 1.0, 1e-6)
+    return Case("assumed_gamma", {"f": (sc, opts())}, chk)
+
+
+@case
+def supersample_uniform():
+    # test_render_shading.cpp:98-114
+    def chk(r):
+        f = r["f"]
+        assert f.width == 5 and f.height == 5
+        approx(f.color[C][0], 0.30, 1e-4)
+    return Case("supersample_uniform", {"f": (quad_scene(bg=(0, 0, 0)), opts(supersample=2))}, chk)
+
+
+@case
+def matte_sphere():
+    # test_render_shading.cpp:119-132
+    m = Material(ambient=0.0, diffuse=0.8, brilliance=0.0, specular=0.0, phong=0.0)
+
+    def chk(r):
+        f = r["f"]
+        approx(f.color[C][0], 0.24, 1e-4)
+        approx(f.color[C][1], 0.16, 1e-4)
+        approx(f.color[C][2], 0.08, 1e-4)
+    return Case("matte_sphere", {"f": (material_sphere_scene((0.6, 0.4, 0.2, 1.0), m, (0, 0, 0)), opts())}, chk)
+
+
+METAL = Material(ambient=0.35, diffuse=0.3, brilliance=2.0, specular=0.80, roughness=1.0 / 20.0,
+                 metallic=True, phong=0.0, reflection=0.10)
+
+
+@case
+def metallic_sphere():
+    # test_render_shading.cpp:137-149
+    def chk(r):
+        f = r["f"]
+        approx(f.color[C][0], 0.54, 1e-3)
+        approx(f.color[C][1], 0.36, 1e-3)
+        approx(f.color[C][2], 0.18, 1e-3)
+    return Case("metallic_sphere", {"f": (material_sphere_scene((0.6, 0.4, 0.2, 1.0), METAL, (0, 0, 0)), opts())},
+                chk)
+
+
+@case
+def reflection_term():
+    # test_render_shading.cpp:153-164
+    def chk(r):
+        f = r["f"]
+        approx(f.color[C][0], 0.64, 1e-3)
+        approx(f.color[C][1], 0.46, 1e-3)
+    return Case("reflection_term", {"f": (material_sphere_scene((0.6, 0.4, 0.2, 1.0), METAL, (1, 1, 1)), opts())},
+                chk)
+
+
+@case
+def phong_lift():
+    # test_render_shading.cpp:171-186
+    m = Material(ambient=0.3, diffuse=0.5, brilliance=0.0, specular=0.0, phong=10000.0, phong_size=50.0)
+
+    def chk(r):
+        f = r["f"]
+        lr, lg, lb = f.color[C][0] - 0.11, f.color[C][1] - 0.22, f.color[C][2] - 0.33
+        approx(lr, 5000.0, 1.0)
+        approx(lr, lg, 1e-3)
+        approx(lg, lb, 1e-3)
+    return Case("phong_lift", {"f": (material_sphere_scene((0.2, 0.4, 0.6, 1.0), m, (0, 0, 0)), opts())}, chk,
+                tol=2e-3)
+
+
+@case
+def fill_light():
+    # test_render_shading.cpp:192-213
+    m = Material(ambient=0.0, diffuse=0.0, brilliance=1.0, specular=0.5, roughness=0.05, phong=0.0)
+    lit = material_sphere_scene((0.6, 0.4, 0.2, 1.0), m, (0, 0, 0))
+    fill = material_sphere_scene((0.6, 0.4, 0.2, 1.0), m, (0, 0, 0))
+    fill.lights[0].casts_highlight = False
+
+    def chk(r):
+        approx(r["lit"].color[C][0], 0.25, 1e-3)
+        approx(r["fill"].color[C][0], 0.0, 1e-4)
+        approx(r["fill"].color[C][1], 0.0, 1e-4)
+    return Case("fill_light", {"lit": (lit, opts()), "fill": (fill, opts())}, chk)
+
+
+@case
+def metallic_fresnel():
+    # test_render_shading.cpp:220-242
+    sc = Scene()
+    p, n, c = quad_mesh((1, 0, 0, 1))
+    sc.set_mesh(p, n, c, material=Material(ambient=0.0, diffuse=0.0, specular=1.0, roughness=0.5,
+                                            brilliance=1.0, metallic=True))
+    sc.camera = ortho_cam()
+    sc.lights = [DistantLight(direction=(-0.8660254, 0.0, -0.5), color=(1, 1, 1), intensity=1.0)]
+
+    def chk(r):
+        f = r["f"]
+        approx(f.color[C][0], 0.75, 2e-3)
+        approx(f.color[C][1], 0.04445, 2e-3)
+        approx(f.color[C][1], f.color[C][2], 1e-5)
+    return Case("metallic_fresnel", {"f": (sc, opts())}, chk)
+
+
+# ---------------------------------------------------------------- shadows
+def floor_slab_mesh(with_slab=True, mat=None):
+    p = [(-2, -2, 0), (2, -2, 0), (2, 2, 0), (-2, -2, 0), (2, 2, 0), (-2, 2, 0)]
+    n = [(0, 0, 1)] * 6
+    c = [(1, 1, 1, 1)] * 6
+    if with_slab:
+        z = 0.4
+        p += [(0.1, -2, z), (4, -2, z), (4, 2, z), (0.1, -2, z), (4, 2, z), (0.1, 2, z)]
+        n += [(0, 0, -1)] * 6
+        c += [(0, 0, 0, 1)] * 6
+    return p, n, c
+
+
+@case
+def hard_shadow():
+    # test_render_shadows_gi.cpp:27-74
+    sc = Scene()
+    p, n, c = floor_slab_mesh(True)
+    sc.set_mesh(p, n, c, material=Material(ambient=0.2, diffuse=0.8))
+    sc.camera = ortho_cam()
+    sc.lights = [DistantLight(direction=(-0.6, 0.0, -0.8), color=(1, 1, 1), intensity=1.0)]
+
+    def chk(r):
+        approx(r["off"].color[C][0], 0.84, 0.03)
+        approx(r["on"].color[C][0], 0.2, 0.03)
+        assert r["on"].color[C][0] + 0.2 < r["off"].color[C][0]
+    return Case("hard_shadow", {"off": (sc, opts(shadows=False)), "on": (sc, opts(shadows=True))}, chk)
+
+
+@case
+def soft_shadow():
+    # test_render_shadows_gi.cpp:80-117
+    sc = quad_scene(color=(1, 1, 1, 1), bg=(0, 0, 0))
+    sc.lights = [DistantLight(direction=(-0.6, 0.0, -0.8), color=(1, 1, 1), intensity=1.0)]
+    sc.add_sphere((0.9, 0.0, 1.2), 0.45, (0, 0, 0, 1.0))
+
+    def chk(r):
+        lit, hard, soft = (r[k].color[C][0] for k in ("lit", "hard", "soft"))
+        approx(hard, 0.2, 0.03)
+        approx(lit, 0.84, 0.03)
+        assert hard + 0.05 < soft and soft + 0.05 < lit
+        assert np.array_equal(r["soft"].color, r["soft2"].color)
+    soft = opts(shadows=True, light_radius=22.0, shadow_samples=64)
+    return Case("soft_shadow", {"lit": (sc, opts(shadows=False)), "hard": (sc, opts(shadows=True)),
+                                "soft": (sc, soft), "soft2": (sc, soft)}, chk, tol=1e-3)
+
+
+@case
+def acne_guard():
+    # test_render_shadows_gi.cpp:129-158
+    sc = Scene()
+    q = 100.0
+    p = [(-q, -q, 0), (q, -q, 0), (q, q, 0), (-q, -q, 0), (q, q, 0), (-q, q, 0)]
+    sc.set_mesh(p, [(0, 0, 1)] * 6, [(1, 1, 1, 1)] * 6, material=Material(ambient=0.2, diffuse=0.8))
+    sc.camera = Camera(position=(0.0, 1200.0, 1600.0), direction=(0.0, -0.6, -0.8), up=(0, 1, 0),
+                       orthographic=True, height=6.0)
+    sc.lights = [key_light()]
+
+    def chk(r):
+        assert r["f"].color[:, :, 0].min() > 0.5
+    return Case("acne_guard", {"f": (sc, opts(shadows=True))}, chk)
+
+
+# ---------------------------------------------------------------- AO
+@case
+def ao_open():
+    # test_render_ao.cpp:28-41
+    sc = quad_scene(bg=(0, 0, 0))
+
+    def chk(r):
+        approx(r["f"].color[C][0], 0.30, 0.02)
+        approx(r["f"].color[C][1], 0.36, 0.02)
+    return Case("ao_open", {"f": (sc, opts(ao_samples=64, ao_distance=100.0))}, chk)
+
+
+@case
+def ao_slab():
+    # test_render_ao.cpp:47-93
+    def scene(with_slab):
+        sc = Scene()
+        p, n, c = floor_slab_mesh(with_slab)
+        sc.set_mesh(p, n, c, material=Material(ambient=1.0, diffuse=0.0))
+        sc.camera = ortho_cam()
+        return sc
+    o = opts(ao_samples=128, ao_distance=10.0)
+
+    def chk(r):
+        approx(r["open"].color[C][0], 1.0, 0.03)
+        assert r["occ"].color[C][0] + 0.05 < r["open"].color[C][0]
+        assert r["occ"].color[C][0] > 0.05
+    return Case("ao_slab", {"open": (scene(False), o), "occ": (scene(True), o)}, chk)
+
+
+@case
+def ao_outline_gate():
+    # test_render_ao.cpp:103-148
+    def scene(probed_from_edge):
+        sc = Scene()
+        sc.camera = ortho_cam()
+        m = Material(ambient=1.0, diffuse=0.0)
+        sc.add_sphere((0, 0, 0), 1.0, (1, 1, 1, 1), m, from_edge=probed_from_edge)
+        sc.add_sphere((0, 2.2, 0), 2.0, (0, 0, 0, 1), m, from_edge=True)
+        return sc
+    off = opts(ao_samples=0)
+    on = opts(ao_samples=128, ao_distance=10.0)
+
+    def chk(r):
+        assert np.abs(r["rOn"].color - r["rOff"].color).sum() > 0.05
+        assert np.array_equal(r["oOn"].color, r["oOff"].color)
+    return Case("ao_outline_gate", {"rOff": (scene(False), off), "rOn": (scene(False), on),
+                                    "oOff": (scene(True), off), "oOn": (scene(True), on)}, chk)
+
+
+@case
+def ao_determinism():
+    # test_render_ao.cpp:150-168
+    sc = quad_scene(bg=(0, 0, 0))
+    o = opts(ao_samples=32, ao_distance=100.0)
+
+    def chk(r):
+        assert np.array_equal(r["a"].color, r["b"].color)
+    return Case("ao_determinism", {"a": (sc, o), "b": (sc, o)}, chk)
+
+
+# ---------------------------------------------------------------- transparency
+def blend_scene(quads, bg):
+    """sceneOfBlend (test_render_transparency.cpp:53-66) with fragment alpha."""
+    sc = Scene()
+    P, N, Cc = [], [], []
+    for color, z in quads:
+        p, n, c = quad_mesh(color, z)
+        P += p
+        N += n
+        Cc += c
+    sc.set_mesh(P, N, Cc, material=Material.flat_outline())
+    sc.camera = ortho_cam()
+    sc.background = bg
+    return sc
+
+
+@case
+def over_f1():
+    # test_render_transparency.cpp:236-247
+    sc = blend_scene([((1, 0, 0, 1.0), 0.0), ((0, 0, 1, 0.6), 1.0)], (0, 0, 0))
+
+    def chk(r):
+        approx(r["f"].color[C][0], 0.4, 1e-4)
+        approx(r["f"].color[C][2], 0.6, 1e-4)
+        approx(r["f"].color[C][3], 1.0, 1e-6)
+    return Case("over_f1", {"f": (sc, opts())}, chk)
+
+
+@case
+def over_f2_order():
+    # test_render_transparency.cpp:251-273
+    a = blend_scene([((1, 0, 0, 1.0), 0.0), ((0, 0, 1, 0.5), 0.5), ((0, 1, 0, 0.5), 1.0)], (0, 0, 0))
+    b = blend_scene([((1, 0, 0, 1.0), 0.0), ((0, 1, 0, 0.5), 0.5), ((0, 0, 1, 0.5), 1.0)], (0, 0, 0))
+
+    def chk(r):
+        approx(r["a"].color[C][1], 0.5, 1e-4)
+        approx(r["a"].color[C][2], 0.25, 1e-4)
+        approx(r["b"].color[C][2], 0.5, 1e-4)
+        approx(r["b"].color[C][1], 0.25, 1e-4)
+    return Case("over_f2_order", {"a": (a, opts()), "b": (b, opts())}, chk)
+
+
+@case
+def over_f3_nodedup():
+    # test_render_transparency.cpp:277-288
+    sc = blend_scene([((1, 0, 0, 1.0), 0.0), ((0, 0, 1, 0.5), 0.5), ((0, 0, 1, 0.5), 1.0)], (0, 0, 0))
+
+    def chk(r):
+        approx(r["f"].color[C][2], 0.75, 1e-4)
+        approx(r["f"].color[C][0], 0.25, 1e-4)
+    return Case("over_f3_nodedup", {"f": (sc, opts())}, chk)
+
+
+@case
+def over_f4_transparent_bg():
+    # test_render_transparency.cpp:291-301
+    sc = blend_scene([((0, 0, 1, 0.6), 1.0)], (0.2, 0.2, 0.2))
+
+    def chk(r):
+        approx(r["f"].color[C][2], 0.6, 1e-4)
+        approx(r["f"].color[C][0], 0.0, 1e-4)
+        approx(r["f"].color[C][3], 0.6, 1e-4)
+    return Case("over_f4_transparent_bg", {"f": (sc, opts(transparent_background=True))}, chk)
+
+
+# ---------------------------------------------------------------- curves
+def cyl_scene(cyls, bg=(1, 1, 1)):
+    sc = Scene()
+    sc.camera = ortho_cam()
+    sc.background = bg
+    sc.ambient_color = (1, 1, 1)
+    for c in cyls:
+        sc.add_cylinder(**c)
+    return sc
+
+
+@case
+def curve_c1_solid():
+    # test_render_edges.cpp:34-56
+    sc = cyl_scene([dict(p0=(-2, 0, 0), p1=(2, 0, 0), radius=0.5, color=(0, 0, 0, 1), open=True)])
+
+    def chk(r):
+        f = r["f"]
+        assert f.color[C][0] < 0.05 and f.color[C][1] < 0.05 and f.color[C][2] < 0.05
+        approx(f.color[C][3], 1.0, 1e-4)
+    return Case("curve_c1_solid", {"f": (sc, opts())}, chk)
+
+
+@case
+def curve_c2_gradient():
+    # test_render_edges.cpp:65-91
+    sc = cyl_scene([dict(p0=(-2, 0, 0), p1=(2, 0, 0), radius=0.5, color=(0, 0, 0, 1), opacity1=0.0, open=True)])
+
+    def chk(r):
+        f = r["f"]
+        lft, ctr, rgt = f.color[2, 1, 0], f.color[2, 2, 0], f.color[2, 3, 0]
+        assert lft + 0.05 < ctr and ctr + 0.05 < rgt
+        approx(ctr, 0.25, 0.02)
+    return Case("curve_c2_gradient", {"f": (sc, opts(transparency=True))}, chk)
+
+
+@case
+def curve_c3_uniform():
+    # test_render_edges.cpp:98-121
+    sc = cyl_scene([dict(p0=(-2, 0, 0), p1=(2, 0, 0), radius=0.5, color=(0, 0, 0, 0.5), opacity1=-1.0, open=True)])
+
+    def chk(r):
+        f = r["f"]
+        lft, ctr, rgt = f.color[2, 1, 0], f.color[2, 2, 0], f.color[2, 3, 0]
+        approx(ctr, 0.25, 0.02)
+        approx(lft, ctr, 0.02)
+        approx(rgt, ctr, 0.02)
+    return Case("curve_c3_uniform", {"f": (sc, opts(transparency=True))}, chk)
+
+
+@case
+def curve_c4_seam():
+    # test_render_edges.cpp:134-160
+    sc = cyl_scene([dict(p0=(-2, 0, 0), p1=(0, 0, 0), radius=0.5, color=(0, 0, 0, 0.5), open=True),
+                    dict(p0=(0, 0, 0), p1=(2, 0, 0), radius=0.5, color=(0, 0, 0, 0.5), open=True)])
+
+    def chk(r):
+        f = r["f"]
+        lft, ctr, rgt = f.color[2, 1, 0], f.color[2, 2, 0], f.color[2, 3, 0]
+        approx(ctr, 0.25, 0.05)
+        assert ctr + 0.03 >= lft and ctr + 0.03 >= rgt
+    return Case("curve_c4_seam", {"f": (sc, opts(transparency=True))}, chk)
+
+
+@case
+def curve_c5_caps():
+    # test_render_edges.cpp:172-205
+    capped = cyl_scene([dict(p0=(-2, 0, 0), p1=(0, 0, 0), radius=0.45, color=(0, 0, 0, 1), open=False)])
+    rnd = cyl_scene([dict(p0=(-2, 0, 0), p1=(0, 0, 0), radius=0.45, color=(0, 0, 0, 1), open=True)])
+
+    def chk(r):
+        fc, fo = r["capped"], r["round"]
+        body, past = (5, 4), (5, 6)
+        assert fc.color[body][0] < 0.1 and fo.color[body][0] < 0.1
+        assert fc.color[past][0] > 0.8
+        assert fo.color[past][0] + 0.2 < fc.color[past][0]
+    return Case("curve_c5_caps", {"capped": (capped, opts(11, 11)), "round": (rnd, opts(11, 11))}, chk)
+
+
+@case
+def curve_c6_far_skip():
+    # test_render_edges.cpp:206-233
+    sc = Scene()
+    sc.camera = ortho_cam(200.0)
+    sc.background = (0, 0, 0)
+    sc.ambient_color = (1, 1, 1)
+    p, n, c = quad_mesh((1, 1, 1, 0.5))
+    sc.set_mesh(p, n, c, material=Material.flat_outline())
+    sc.add_sphere((0, 0, -5.001), 5.0, (0, 0, 1, 1), Material.flat_outline())
+    sc.add_sphere((0, 0, -1.0), 0.5, (1, 0, 0, 1), Material.flat_outline())
+
+    def chk(r):
+        f = r["f"]
+        assert f.color[C][2] > f.color[C][0] + 0.3
+    return Case("curve_c6_far_skip", {"f": (sc, opts(transparency=True))}, chk)
+
+
+# ---------------------------------------------------------------- clip planes
+@case
+def clip_planes():
+    # test_clip_planes.cpp:20-77 (colour / view-z parts; the clip-cut G-buffer
+    # belongs to the stroke-edge pass, which is outside this path)
+    mat = Material(ambient=0.2, diffuse=0.8)
+
+    def sc(near=-math.inf, far=math.inf):
+        s = material_sphere_scene((1, 0, 0, 1), mat, (0, 0, 1))
+        s.clip_near, s.clip_far = near, far
+        return s
+    sc_near = sc(near=10.0)
+    sc_near.fog = Fog(enabled=False)
+
+    def chk(r):
+        f = r["none"]
+        assert f.color[C][0] > 0.3 and f.color[C][2] < 0.5
+        assert abs(r["far8"].color[C][2] - 1.0) <= 1e-4 and r["far8"].color[C][0] < 1e-4
+        assert r["near12"].color[C][0] < 1e-4
+        assert 10.5 < r["near10"].view_z[C] < 11.5
+    return Case("clip_planes", {"none": (sc(), opts()), "far8": (sc(far=8.0), opts()),
+                                "near12": (sc(near=12.0), opts()), "near10": (sc_near, opts())}, chk)
+
+
+# ---------------------------------------------------------------- indexed mesh
+@case
+def mesh_indexed():
+    # test_mesh_indexed.cpp:61-93 (indexed render == de-indexed, bitwise)
+    verts = [(-2, -2, 0), (2, -2, 0), (2, 2, 0), (-2, 2, 0)]
+    idx = [0, 1, 2, 0, 2, 3]
+
+    def mk(indexed):
+        sc = Scene()
+        if indexed:
+            sc.set_mesh(verts, [(0, 0, 1)] * 4, [(0.5, 0.6, 0.7, 1)] * 4, index=np.array(idx).reshape(-1, 3),
+                        material=Material(ambient=0.2, diffuse=0.8))
+        else:
+            sc.set_mesh([verts[i] for i in idx], [(0, 0, 1)] * 6, [(0.5, 0.6, 0.7, 1)] * 6,
+                        material=Material(ambient=0.2, diffuse=0.8))
+        sc.camera = ortho_cam()
+        sc.background = (0.1, 0.2, 0.3)
+        sc.lights = [DistantLight(direction=(0, 0, -1), intensity=0.5)]
+        return sc
+    o = opts(24, 24, supersample=2)
+
+    def chk(r):
+        assert np.array_equal(r["idx"].color, r["soup"].color)
+    return Case("mesh_indexed", {"idx": (mk(True), o), "soup": (mk(False), o)}, chk)
+
+
+# ---------------------------------------------------------------- fog in the frame
+@case
+def fog_frame():
+    # fog.cpp:17-36 applied by pipeline.cpp:126-129 on the hi-res frame:
+    # quad at view-z 10 with fog start 5, end 15 -> f = 0.5 toward fog colour
+    sc = quad_scene(bg=(0, 0, 0))
+    sc.fog = Fog(enabled=True, color=(1.0, 1.0, 1.0), start=5.0, end=15.0)
+
+    def chk(r):
+        f = r["f"]
+        approx(f.color[C][0], 0.5 * 1.0 + 0.5 * 0.30, 1e-4)
+        approx(f.color[C][2], 0.5 * 1.0 + 0.5 * 0.42, 1e-4)
+        approx(r["bg"].color[C][0], 0.0, 1e-6)
+    sc_bg = Scene()
+    sc_bg.camera = ortho_cam()
+    sc_bg.fog = Fog(enabled=True, color=(1, 1, 1), start=5.0, end=15.0)
+    return Case("fog_frame", {"f": (sc, opts()), "bg": (sc_bg, opts())}, chk)

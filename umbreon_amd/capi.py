@@ -1,0 +1,219 @@
+"""ctypes mirror of the C-ABI in ``include/umbreon_gpu.h``.
+
+The structs below are field-for-field copies of the C declarations; the
+loader fails loudly when the gfx950 library has not been built, so nothing in
+this package can silently fall back to a CPU path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent.parent
+LIB_DIR = Path(__file__).resolve().parent / "lib"
+GPU_LIB = LIB_DIR / "libumbreon_gpu.so"
+
+UBG_OK = 0
+UBG_ERR_INVALID_ARGUMENT = 1
+UBG_ERR_DEVICE = 2
+UBG_ERR_OUT_OF_MEMORY = 3
+UBG_ERR_CANCELLED = 4
+UBG_ERR_UNSUPPORTED = 5
+UBG_ERR_INTERNAL = 6
+
+KIND_MESH = 0
+KIND_SPHERE = 1
+KIND_ROUND_CURVE = 2
+KIND_CONE_CURVE = 3
+
+FLAG_HIRES_ONLY = 1
+FLAG_COUNT_VISITS = 2
+FLAG_KERNEL_TIMES = 4
+KERNELS = ("primary", "ao", "resolve", "post")
+
+MISS_ID = 0xFFFFFFFF
+
+f32p = C.POINTER(C.c_float)
+u32p = C.POINTER(C.c_uint32)
+i32p = C.POINTER(C.c_int32)
+u16p = C.POINTER(C.c_uint16)
+u8p = C.POINTER(C.c_uint8)
+
+
+class Material(C.Structure):
+    _fields_ = [
+        ("ambient", C.c_float), ("diffuse", C.c_float), ("specular", C.c_float),
+        ("roughness", C.c_float), ("brilliance", C.c_float), ("phong", C.c_float),
+        ("phong_size", C.c_float), ("metallic", C.c_int32), ("reflection", C.c_float),
+        ("emission", C.c_float), ("model", C.c_int32), ("pbr_metallic", C.c_float),
+        ("pbr_roughness", C.c_float), ("pbr_specular", C.c_float),
+        ("pbr_anisotropy", C.c_float), ("pbr_anisotropy_rotation", C.c_float),
+    ]
+
+
+class SceneDesc(C.Structure):
+    _fields_ = [
+        ("n_vertices", C.c_int64), ("positions", f32p), ("normals", f32p), ("colors", f32p),
+        ("n_triangles", C.c_int64), ("index", u32p),
+        ("mesh_materials", C.POINTER(Material)), ("n_mesh_materials", C.c_int32),
+        ("tri_material", u8p), ("tri_group", u16p),
+        ("n_spheres", C.c_int64), ("sphere_center_radius", f32p), ("sphere_color", f32p),
+        ("sphere_material", i32p), ("sphere_group", u16p), ("sphere_from_edge", u8p),
+        ("n_cylinders", C.c_int64), ("cyl_p0", f32p), ("cyl_p1", f32p), ("cyl_radius", f32p),
+        ("cyl_color", f32p), ("cyl_opacity1", f32p), ("cyl_material", i32p),
+        ("cyl_group", u16p), ("cyl_open", u8p), ("cyl_from_edge", u8p),
+        ("prim_materials", C.POINTER(Material)), ("n_prim_materials", C.c_int32),
+        ("n_instances", C.c_int64), ("instance_offsets", f32p),
+    ]
+
+
+class Camera(C.Structure):
+    _fields_ = [("position", C.c_float * 3), ("direction", C.c_float * 3), ("up", C.c_float * 3),
+                ("fovy", C.c_float), ("orthographic", C.c_int32), ("height", C.c_float)]
+
+
+class Light(C.Structure):
+    _fields_ = [("direction", C.c_float * 3), ("color", C.c_float * 3), ("intensity", C.c_float),
+                ("casts_highlight", C.c_int32), ("angular_radius", C.c_float)]
+
+
+class Fog(C.Structure):
+    _fields_ = [("enabled", C.c_int32), ("color", C.c_float * 3), ("start", C.c_float),
+                ("end", C.c_float)]
+
+
+class RenderParams(C.Structure):
+    _fields_ = [
+        ("width", C.c_int32), ("height", C.c_int32), ("supersample", C.c_int32),
+        ("camera", Camera), ("lights", C.POINTER(Light)), ("n_lights", C.c_int32),
+        ("ambient_color", C.c_float * 3), ("background", C.c_float * 3), ("fog", Fog),
+        ("clip_near", C.c_float), ("clip_far", C.c_float), ("assumed_gamma", C.c_float),
+        ("ao_samples", C.c_int32), ("ao_distance", C.c_float), ("ao_intensity", C.c_float),
+        ("shadows", C.c_int32), ("shadow_samples", C.c_int32), ("light_radius_deg", C.c_float),
+        ("gi_integrator", C.c_int32), ("specular_scale", C.c_float),
+        ("transparency", C.c_int32), ("transparent_background", C.c_int32),
+        ("max_transparent_layers", C.c_int32), ("flags", C.c_uint32),
+        ("cancel_flag", C.POINTER(C.c_int32)),
+    ]
+
+
+class RayStats(C.Structure):
+    _fields_ = [("primary_rays", C.c_uint64), ("shaded_hits", C.c_uint64),
+                ("ao_rays_traced", C.c_uint64), ("ao_rays_sampled", C.c_uint64),
+                ("shadow_rays", C.c_uint64), ("capped_rays", C.c_uint64),
+                ("node_visits", C.c_uint64 * 3), ("prim_tests", C.c_uint64 * 3)]
+
+    def as_dict(self) -> dict:
+        d = {}
+        for k, _ in self._fields_:
+            v = getattr(self, k)
+            d[k] = [int(x) for x in v] if k in ("node_visits", "prim_tests") else int(v)
+        return d
+
+
+class Frame(C.Structure):
+    _fields_ = [
+        ("color", C.c_void_p), ("depth", C.c_void_p), ("view_z", C.c_void_p),
+        ("hit_id", C.c_void_p), ("ao_open", C.c_void_p), ("memory", C.c_int32),
+        ("out_w", C.c_int32), ("out_h", C.c_int32), ("hi_w", C.c_int32), ("hi_h", C.c_int32),
+        ("stats", RayStats), ("render_seconds", C.c_double), ("post_seconds", C.c_double),
+        ("cancelled", C.c_int32), ("layers", C.c_int32), ("kernel_ms", C.c_double * 4),
+        ("kernel_launches", C.c_int32 * 4),
+    ]
+
+
+class SceneInfo(C.Structure):
+    _fields_ = [
+        ("n_triangles", C.c_int64), ("n_spheres", C.c_int64), ("n_round_segments", C.c_int64),
+        ("n_cone_segments", C.c_int64), ("n_nodes", C.c_int64), ("n_leaf_prims", C.c_int64),
+        ("bvh_depth", C.c_int32), ("node_bytes", C.c_int32), ("prim_bytes", C.c_int32),
+        ("build_seconds", C.c_double), ("sah_cost", C.c_double), ("device_bytes", C.c_int64),
+    ]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class Ray(C.Structure):
+    _fields_ = [("org", C.c_float * 3), ("tnear", C.c_float), ("dir", C.c_float * 3),
+                ("tfar", C.c_float)]
+
+
+class Hit(C.Structure):
+    _fields_ = [("t", C.c_float), ("u", C.c_float), ("v", C.c_float), ("kind", C.c_uint32),
+                ("prim_id", C.c_uint32), ("ng", C.c_float * 3)]
+
+
+# numpy structured dtypes matching Ray / Hit (for zero-copy batches)
+def ray_dtype():
+    import numpy as np
+    return np.dtype([("org", "<f4", 3), ("tnear", "<f4"), ("dir", "<f4", 3), ("tfar", "<f4")])
+
+
+def hit_dtype():
+    import numpy as np
+    return np.dtype([("t", "<f4"), ("u", "<f4"), ("v", "<f4"), ("kind", "<u4"),
+                     ("prim_id", "<u4"), ("ng", "<f4", 3)])
+
+
+# Exported symbols of libumbreon_gpu.so (the C-ABI). Kept in sync with the
+# header; tests check both directions.
+EXPORTS = [
+    "ubg_abi_version", "ubg_status_string", "ubg_context_create", "ubg_context_destroy",
+    "ubg_last_error", "ubg_scene_create", "ubg_scene_destroy", "ubg_scene_get_info",
+    "ubg_render", "ubg_intersect", "ubg_occluded",
+]
+
+_lib = None
+
+
+class UmbreonError(RuntimeError):
+    """A non-zero status from the C-ABI (the reference throws std::runtime_error)."""
+
+    def __init__(self, status: int, message: str):
+        super().__init__(f"{message} (status {status})")
+        self.status = status
+
+
+def load() -> C.CDLL:
+    """Load libumbreon_gpu.so, raising if it was never built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not GPU_LIB.exists():
+        raise RuntimeError(
+            f"{GPU_LIB} is missing: build the gfx950 backend first "
+            "(python -c 'import __graft_entry__; __graft_entry__.build()')")
+    lib = C.CDLL(str(GPU_LIB))
+    lib.ubg_abi_version.restype = C.c_int
+    lib.ubg_status_string.restype = C.c_char_p
+    lib.ubg_status_string.argtypes = [C.c_int]
+    lib.ubg_last_error.restype = C.c_char_p
+    lib.ubg_context_create.argtypes = [C.c_int, C.POINTER(C.c_void_p)]
+    lib.ubg_context_create.restype = C.c_int
+    lib.ubg_context_destroy.argtypes = [C.c_void_p]
+    lib.ubg_context_destroy.restype = None
+    lib.ubg_scene_create.argtypes = [C.c_void_p, C.POINTER(SceneDesc), C.POINTER(C.c_void_p)]
+    lib.ubg_scene_create.restype = C.c_int
+    lib.ubg_scene_destroy.argtypes = [C.c_void_p]
+    lib.ubg_scene_destroy.restype = None
+    lib.ubg_scene_get_info.argtypes = [C.c_void_p, C.POINTER(SceneInfo)]
+    lib.ubg_scene_get_info.restype = C.c_int
+    lib.ubg_render.argtypes = [C.c_void_p, C.c_void_p, C.POINTER(RenderParams), C.POINTER(Frame)]
+    lib.ubg_render.restype = C.c_int
+    lib.ubg_intersect.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_int32]
+    lib.ubg_intersect.restype = C.c_int
+    lib.ubg_occluded.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_int32]
+    lib.ubg_occluded.restype = C.c_int
+    if lib.ubg_abi_version() != 1:
+        raise RuntimeError("libumbreon_gpu.so ABI version mismatch")
+    _lib = lib
+    return lib
+
+
+def check(status: int) -> None:
+    if status != UBG_OK:
+        lib = load()
+        msg = lib.ubg_last_error().decode(errors="replace") or lib.ubg_status_string(status).decode()
+        raise UmbreonError(status, msg)

@@ -1,0 +1,52 @@
+// ub_build.h -- host-side scene flattening and BVH construction.
+#pragma once
+
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/umbreon_gpu.h"
+#include "ub_layout.h"
+
+namespace ubg {
+
+struct F4 {
+  float x, y, z, w;
+};
+struct U4 {
+  uint32_t x, y, z, w;
+};
+
+// Everything that goes to HBM for one scene, still on the host.
+struct HostScene {
+  std::vector<F4> nodes;   // 4 per inner node
+  std::vector<F4> prims;   // 3 per primitive, leaf order
+  std::vector<F4> sphColor;
+  std::vector<uint32_t> sphInfo;
+  std::vector<F4> coneColor;
+  std::vector<uint32_t> coneInfo;
+  std::vector<float> coneOp1;
+  std::vector<F4> rndColor;
+  std::vector<uint32_t> rndInfo;
+  std::vector<float> rndOp1;
+  std::vector<F4> rndPlanes;  // 2 per round segment
+  std::vector<int32_t> rndSource;  // source cylinder of each round segment
+  std::vector<U4> triIdx;
+  std::vector<F4> vtxNrm;
+  std::vector<F4> vtxCol;
+  std::vector<DevMaterial> primMats;
+  std::vector<DevMaterial> meshMats;
+  int64_t nTris = 0, nSpheres = 0, nRound = 0, nCone = 0;
+  int32_t depth = 0;
+  double sahCost = 0.0;
+};
+
+// Flatten `desc` (curve chaining, instance baking, side tables) and build the
+// BVH. Returns UBG_OK or an error status with `err` set.
+int buildHostScene(const ubg_scene_desc* desc, HostScene& out, std::string& err);
+
+// shading.hpp:34-41
+float blinnExponent(float roughness);
+
+}  // namespace ubg

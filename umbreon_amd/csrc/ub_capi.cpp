@@ -1,0 +1,564 @@
+// ub_capi.cpp -- the extern "C" boundary of the gfx950 per-ray backend
+// (declared in include/umbreon_gpu.h).
+//
+// Device context <-> rtcNewDevice/SharedDevice (embree_renderer.cpp:1267-1273,
+// umbreon.cpp:21-39); scene <-> buildEmbreeScene (scene_build.cpp:133-170);
+// ubg_render <-> the EmbreeRenderer::render grid path
+// (embree_renderer.cpp:1260-1427) followed by renderFrame's fog, box
+// downsample and assumed gamma (pipeline.cpp:113-129,182-254). Errors never
+// abort: every failure returns a status and leaves a message for
+// ubg_last_error(), which the C++ facade turns into std::runtime_error.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/umbreon_gpu.h"
+#include "ub_build.h"
+#include "ub_kernels.h"
+#include "ub_layout.h"
+
+namespace {
+
+thread_local std::string g_lastError;
+
+int fail(int status, const std::string& msg) {
+  g_lastError = msg;
+  return status;
+}
+
+int hipFail(hipError_t e, const char* what) {
+  const int st = (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) ? UBG_ERR_OUT_OF_MEMORY
+                                                                            : UBG_ERR_DEVICE;
+  return fail(st, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define UBG_HIP(call, what)                 \
+  do {                                      \
+    hipError_t e__ = (call);                \
+    if (e__ != hipSuccess) return hipFail(e__, what); \
+  } while (0)
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+};
+
+}  // namespace
+
+struct ubg_context {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  std::vector<hipEvent_t> tev;  // per-kernel timing events (grow-only)
+  std::mutex mu;
+  // grow-only frame scratch
+  DevBuf hitPix, hitT, hitO, hitN, hitC, hitAo, activeA, activeB, acc, tnear, hiColor, depth, viewZ,
+      hitId, aoOpen, outColor, counters, query, queryOut;
+  unsigned long long* hostCounters = nullptr;  // pinned
+  std::vector<DevBuf*> all() {
+    return {&hitPix, &hitT,   &hitO,    &hitN,  &hitC,  &hitAo,   &activeA,  &activeB,  &acc,     &tnear,
+            &hiColor, &depth, &viewZ,   &hitId, &aoOpen, &outColor, &counters, &query, &queryOut};
+  }
+};
+
+struct ubg_scene {
+  ubg_context* ctx = nullptr;
+  ubg::SceneView view{};
+  std::vector<void*> allocs;
+  ubg_scene_info info{};
+};
+
+namespace {
+
+int ensure(DevBuf& b, size_t bytes) {
+  if (b.bytes >= bytes && b.p) return UBG_OK;
+  if (b.p) {
+    (void)hipFree(b.p);
+    b.p = nullptr;
+    b.bytes = 0;
+  }
+  const size_t want = std::max<size_t>(bytes, 256);
+  UBG_HIP(hipMalloc(&b.p, want), "hipMalloc(frame scratch)");
+  b.bytes = want;
+  return UBG_OK;
+}
+
+template <class T>
+int upload(ubg_scene* s, const std::vector<T>& v, const void** dst) {
+  *dst = nullptr;
+  if (v.empty()) return UBG_OK;
+  void* p = nullptr;
+  UBG_HIP(hipMalloc(&p, v.size() * sizeof(T)), "hipMalloc(scene)");
+  s->allocs.push_back(p);
+  UBG_HIP(hipMemcpyAsync(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s->ctx->stream),
+          "hipMemcpy(scene upload)");
+  s->info.device_bytes += static_cast<int64_t>(v.size() * sizeof(T));
+  *dst = p;
+  return UBG_OK;
+}
+
+struct V3h {
+  float x, y, z;
+};
+inline V3h scale(V3h a, float s) { return {a.x * s, a.y * s, a.z * s}; }
+inline float dot(V3h a, V3h b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+inline V3h cross(V3h a, V3h b) {
+  return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+inline V3h normalize(V3h a) {
+  const float l = std::sqrt(dot(a, a));
+  return l > 0.0f ? scale(a, 1.0f / l) : a;
+}
+
+int copyOut(void* dst, const void* src, size_t bytes, int32_t memory, hipStream_t st) {
+  if (!dst || bytes == 0) return UBG_OK;
+  UBG_HIP(hipMemcpyAsync(dst, src, bytes, memory ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, st),
+          "hipMemcpy(frame output)");
+  return UBG_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ubg_abi_version(void) { return UBG_ABI_VERSION; }
+
+const char* ubg_status_string(int status) {
+  switch (status) {
+    case UBG_OK: return "ok";
+    case UBG_ERR_INVALID_ARGUMENT: return "invalid argument";
+    case UBG_ERR_DEVICE: return "device error";
+    case UBG_ERR_OUT_OF_MEMORY: return "out of device memory";
+    case UBG_ERR_CANCELLED: return "cancelled";
+    case UBG_ERR_UNSUPPORTED: return "unsupported";
+    case UBG_ERR_INTERNAL: return "internal error";
+    default: return "unknown status";
+  }
+}
+
+const char* ubg_last_error(void) { return g_lastError.c_str(); }
+
+int ubg_context_create(int device_ordinal, ubg_context** out) {
+  if (!out) return fail(UBG_ERR_INVALID_ARGUMENT, "ubg_context_create: out is null");
+  *out = nullptr;
+  int n = 0;
+  hipError_t e = hipGetDeviceCount(&n);
+  if (e != hipSuccess || n <= 0)
+    return fail(UBG_ERR_DEVICE, "ubg_context_create: no HIP device available");
+  if (device_ordinal < 0 || device_ordinal >= n)
+    return fail(UBG_ERR_INVALID_ARGUMENT, "ubg_context_create: device ordinal out of range");
+  hipDeviceProp_t prop;
+  UBG_HIP(hipGetDeviceProperties(&prop, device_ordinal), "hipGetDeviceProperties");
+  if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+    return fail(UBG_ERR_DEVICE, std::string("ubg_context_create: device is ") + prop.gcnArchName +
+                                    ", this library is built for gfx950 only");
+  UBG_HIP(hipSetDevice(device_ordinal), "hipSetDevice");
+  ubg_context* c = new ubg_context();
+  c->device = device_ordinal;
+  hipError_t se = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  if (se != hipSuccess) {
+    delete c;
+    return hipFail(se, "hipStreamCreate");
+  }
+  for (auto& ev : c->ev) {
+    if (hipEventCreate(&ev) != hipSuccess) {
+      ubg_context_destroy(c);
+      return fail(UBG_ERR_DEVICE, "hipEventCreate failed");
+    }
+  }
+  if (hipHostMalloc(reinterpret_cast<void**>(&c->hostCounters), ubg::kNumCounters * sizeof(unsigned long long),
+                    hipHostMallocDefault) != hipSuccess) {
+    ubg_context_destroy(c);
+    return fail(UBG_ERR_OUT_OF_MEMORY, "hipHostMalloc(counters) failed");
+  }
+  *out = c;
+  return UBG_OK;
+}
+
+void ubg_context_destroy(ubg_context* ctx) {
+  if (!ctx) return;
+  (void)hipSetDevice(ctx->device);
+  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  for (DevBuf* b : ctx->all())
+    if (b->p) (void)hipFree(b->p);
+  for (auto& ev : ctx->ev)
+    if (ev) (void)hipEventDestroy(ev);
+  for (hipEvent_t ev : ctx->tev) (void)hipEventDestroy(ev);
+  if (ctx->hostCounters) (void)hipHostFree(ctx->hostCounters);
+  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+}
+
+int ubg_scene_create(ubg_context* ctx, const ubg_scene_desc* desc, ubg_scene** out) {
+  if (!ctx || !desc || !out) return fail(UBG_ERR_INVALID_ARGUMENT, "ubg_scene_create: null argument");
+  *out = nullptr;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  UBG_HIP(hipSetDevice(ctx->device), "hipSetDevice");
+  const auto t0 = std::chrono::steady_clock::now();
+  ubg::HostScene hs;
+  std::string err;
+  const int st = ubg::buildHostScene(desc, hs, err);
+  if (st != UBG_OK) return fail(st, "scene build failed: " + err);
+  ubg_scene* s = new ubg_scene();
+  s->ctx = ctx;
+  int r = UBG_OK;
+  auto up = [&](auto& vec, const void** dst) {
+    if (r == UBG_OK) r = upload(s, vec, dst);
+  };
+  const void* p = nullptr;
+  up(hs.nodes, &s->view.nodes);
+  up(hs.prims, &s->view.prims);
+  up(hs.sphColor, &s->view.sphColor);
+  up(hs.sphInfo, &p);
+  s->view.sphInfo = static_cast<const uint32_t*>(p);
+  up(hs.coneColor, &s->view.coneColor);
+  up(hs.coneInfo, &p);
+  s->view.coneInfo = static_cast<const uint32_t*>(p);
+  up(hs.coneOp1, &p);
+  s->view.coneOp1 = static_cast<const float*>(p);
+  up(hs.rndColor, &s->view.rndColor);
+  up(hs.rndInfo, &p);
+  s->view.rndInfo = static_cast<const uint32_t*>(p);
+  up(hs.rndOp1, &p);
+  s->view.rndOp1 = static_cast<const float*>(p);
+  up(hs.rndPlanes, &s->view.rndPlanes);
+  up(hs.triIdx, &s->view.triIdx);
+  up(hs.vtxNrm, &s->view.vtxNrm);
+  up(hs.vtxCol, &s->view.vtxCol);
+  up(hs.primMats, &p);
+  s->view.primMats = static_cast<const ubg::DevMaterial*>(p);
+  up(hs.meshMats, &p);
+  s->view.meshMats = static_cast<const ubg::DevMaterial*>(p);
+  if (r == UBG_OK) {
+    hipError_t e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) r = hipFail(e, "scene upload");
+  }
+  if (r != UBG_OK) {
+    ubg_scene_destroy(s);
+    return r;
+  }
+  s->view.nNodes = static_cast<int32_t>(hs.nodes.size() / 4);
+  s->view.nPrims = static_cast<int32_t>(hs.prims.size() / 3);
+  const auto t1 = std::chrono::steady_clock::now();
+  s->info.n_triangles = hs.nTris;
+  s->info.n_spheres = hs.nSpheres;
+  s->info.n_round_segments = hs.nRound;
+  s->info.n_cone_segments = hs.nCone;
+  s->info.n_nodes = s->view.nNodes;
+  s->info.n_leaf_prims = s->view.nPrims;
+  s->info.bvh_depth = hs.depth;
+  s->info.node_bytes = 64;
+  s->info.prim_bytes = 48;
+  s->info.build_seconds = std::chrono::duration<double>(t1 - t0).count();
+  s->info.sah_cost = hs.sahCost;
+  *out = s;
+  return UBG_OK;
+}
+
+void ubg_scene_destroy(ubg_scene* scene) {
+  if (!scene) return;
+  if (scene->ctx) {
+    std::lock_guard<std::mutex> lk(scene->ctx->mu);
+    (void)hipSetDevice(scene->ctx->device);
+    (void)hipStreamSynchronize(scene->ctx->stream);
+    for (void* p : scene->allocs) (void)hipFree(p);
+  }
+  delete scene;
+}
+
+int ubg_scene_get_info(const ubg_scene* scene, ubg_scene_info* out) {
+  if (!scene || !out) return fail(UBG_ERR_INVALID_ARGUMENT, "ubg_scene_get_info: null argument");
+  *out = scene->info;
+  return UBG_OK;
+}
+
+int ubg_render(ubg_context* ctx, const ubg_scene* scene, const ubg_render_params* p,
+               ubg_frame* fr) {
+  if (!ctx || !scene || !p || !fr) return fail(UBG_ERR_INVALID_ARGUMENT, "ubg_render: null argument");
+  if (scene->ctx != ctx) return fail(UBG_ERR_INVALID_ARGUMENT, "ubg_render: scene belongs to another context");
+  if (p->width <= 0 || p->height <= 0) return fail(UBG_ERR_INVALID_ARGUMENT, "ubg_render: width/height must be > 0");
+  if (p->n_lights < 0 || (p->n_lights > 0 && !p->lights))
+    return fail(UBG_ERR_INVALID_ARGUMENT, "ubg_render: bad light array");
+  if (p->n_lights > ubg::kMaxLights)
+    return fail(UBG_ERR_UNSUPPORTED, "ubg_render: more than 16 lights");
+  if (p->ao_samples < 0 || p->shadow_samples < 0 || p->max_transparent_layers < 0)
+    return fail(UBG_ERR_INVALID_ARGUMENT, "ubg_render: negative sample or layer count");
+  const int ss = std::max(1, p->supersample);
+  const int64_t W64 = static_cast<int64_t>(p->width) * ss, H64 = static_cast<int64_t>(p->height) * ss;
+  if (W64 * H64 >= (int64_t(1) << 31) || W64 * H64 * 4 >= (int64_t(1) << 40))
+    return fail(UBG_ERR_INVALID_ARGUMENT, "ubg_render: frame too large");
+  const int W = static_cast<int>(W64), H = static_cast<int>(H64);
+  const int npx = W * H;
+  const bool countVisits = (p->flags & UBG_FLAG_COUNT_VISITS) != 0u;
+
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  UBG_HIP(hipSetDevice(ctx->device), "hipSetDevice");
+  hipStream_t st = ctx->stream;
+
+  // ---- frame constants (embree_renderer.cpp:375-386, 436-458) ----
+  ubg::FrameConst fc;
+  std::memset(&fc, 0, sizeof(fc));
+  fc.W = W;
+  fc.H = H;
+  const ubg_camera& cam = p->camera;
+  const float aspect = static_cast<float>(W) / static_cast<float>(H);
+  const V3h dir = normalize(V3h{cam.direction[0], cam.direction[1], cam.direction[2]});
+  const V3h right = normalize(cross(dir, V3h{cam.up[0], cam.up[1], cam.up[2]}));
+  const V3h trueUp = normalize(cross(right, dir));
+  fc.ortho = cam.orthographic ? 1 : 0;
+  std::memcpy(fc.pos, cam.position, sizeof(fc.pos));
+  fc.dir[0] = dir.x, fc.dir[1] = dir.y, fc.dir[2] = dir.z;
+  fc.right[0] = right.x, fc.right[1] = right.y, fc.right[2] = right.z;
+  fc.up[0] = trueUp.x, fc.up[1] = trueUp.y, fc.up[2] = trueUp.z;
+  fc.halfH = cam.height * 0.5f;
+  fc.halfW = fc.halfH * aspect;
+  fc.persHalfH = std::tan(cam.fovy * 0.01745329252f * 0.5f);
+  fc.persHalfW = fc.persHalfH * aspect;
+  const float inf = std::numeric_limits<float>::infinity();
+  fc.clipOn = (p->clip_near <= -inf && p->clip_far >= inf) ? 0 : 1;
+  fc.clipNear = p->clip_near;
+  fc.clipFar = p->clip_far;
+  std::memcpy(fc.bg, p->background, sizeof(fc.bg));
+  std::memcpy(fc.amb, p->ambient_color, sizeof(fc.amb));
+  fc.transparentBg = p->transparent_background ? 1 : 0;
+  fc.transparency = p->transparency ? 1 : 0;
+  fc.maxIters = p->transparency ? (p->max_transparent_layers + 1) : 1;
+  fc.aoSamples = p->ao_samples;
+  fc.aoDist = p->ao_distance;
+  fc.aoIntensity = p->ao_intensity;
+  int lanes = 1;
+  while (lanes < p->ao_samples && lanes < 64) lanes <<= 1;
+  fc.aoLanes = lanes;
+  fc.shadows = p->shadows ? 1 : 0;
+  fc.shadowSamples = p->shadow_samples;
+  fc.specScale = p->specular_scale;
+  fc.fogOn = p->fog.enabled ? 1 : 0;
+  std::memcpy(fc.fogColor, p->fog.color, sizeof(fc.fogColor));
+  fc.fogStart = p->fog.start;
+  fc.fogEnd = p->fog.end;
+  fc.nLights = p->n_lights;
+  for (int i = 0; i < p->n_lights; ++i) {
+    const ubg_light& dl = p->lights[i];
+    ubg::DevLight& l = fc.lights[i];
+    const V3h L = normalize(V3h{-dl.direction[0], -dl.direction[1], -dl.direction[2]});
+    l.L[0] = L.x, l.L[1] = L.y, l.L[2] = L.z;
+    l.color[0] = dl.color[0] * dl.intensity;
+    l.color[1] = dl.color[1] * dl.intensity;
+    l.color[2] = dl.color[2] * dl.intensity;
+    l.highlight = dl.casts_highlight ? 1 : 0;
+    l.radius = (p->gi_integrator == 2 && dl.angular_radius > 0.0f) ? dl.angular_radius
+                                                                   : p->light_radius_deg * 0.01745329252f;
+    l.tanRadius = std::tan(l.radius);
+    l.soft = (p->shadow_samples > 1 && l.radius > 0.0f) ? 1 : 0;
+  }
+
+  // ---- frame scratch ----
+  const size_t n = static_cast<size_t>(npx);
+  int r = UBG_OK;
+  auto need = [&](DevBuf& b, size_t bytes) {
+    if (r == UBG_OK) r = ensure(b, bytes);
+  };
+  need(ctx->hitPix, n * 4);
+  need(ctx->hitT, n * 4);
+  need(ctx->hitO, n * 16);
+  need(ctx->hitN, n * 16);
+  need(ctx->hitC, n * 16);
+  need(ctx->hitAo, n * 4);
+  need(ctx->activeA, n * 4);
+  need(ctx->activeB, n * 4);
+  need(ctx->acc, n * 16);
+  need(ctx->tnear, n * 4);
+  need(ctx->hiColor, n * 16);
+  need(ctx->depth, n * 4);
+  need(ctx->viewZ, n * 4);
+  if (fr->hit_id) need(ctx->hitId, n * 4);
+  if (fr->ao_open) need(ctx->aoOpen, n * 4);
+  const size_t outN = static_cast<size_t>(p->width) * p->height;
+  need(ctx->outColor, outN * 16);
+  need(ctx->counters, ubg::kNumCounters * sizeof(unsigned long long));
+  if (r != UBG_OK) return r;
+
+  ubg::FrameBufs b;
+  b.hitPix = static_cast<uint32_t*>(ctx->hitPix.p);
+  b.hitT = static_cast<float*>(ctx->hitT.p);
+  b.hitO = static_cast<float4*>(ctx->hitO.p);
+  b.hitN = static_cast<float4*>(ctx->hitN.p);
+  b.hitC = static_cast<float4*>(ctx->hitC.p);
+  b.hitAo = static_cast<uint32_t*>(ctx->hitAo.p);
+  b.activeIn = static_cast<uint32_t*>(ctx->activeA.p);
+  b.activeOut = static_cast<uint32_t*>(ctx->activeB.p);
+  b.acc = static_cast<float4*>(ctx->acc.p);
+  b.tnear = static_cast<float*>(ctx->tnear.p);
+  b.hiColor = static_cast<float4*>(ctx->hiColor.p);
+  b.depth = static_cast<float*>(ctx->depth.p);
+  b.viewZ = static_cast<float*>(ctx->viewZ.p);
+  b.hitId = fr->hit_id ? static_cast<uint32_t*>(ctx->hitId.p) : nullptr;
+  b.aoOpen = fr->ao_open ? static_cast<float*>(ctx->aoOpen.p) : nullptr;
+  b.counters = static_cast<unsigned long long*>(ctx->counters.p);
+
+  fr->cancelled = 0;
+  fr->layers = 0;
+  for (int k = 0; k < 4; ++k) {
+    fr->kernel_ms[k] = 0.0;
+    fr->kernel_launches[k] = 0;
+  }
+  // Optional per-kernel stream events (UBG_FLAG_KERNEL_TIMES).
+  const bool timing = (p->flags & UBG_FLAG_KERNEL_TIMES) != 0u;
+  struct Mark {
+    int kind;
+    size_t e0, e1;
+  };
+  std::vector<Mark> marks;
+  size_t nextEv = 0;
+  auto evAt = [&](size_t i) -> hipEvent_t {
+    while (ctx->tev.size() <= i) {
+      hipEvent_t e = nullptr;
+      if (hipEventCreate(&e) != hipSuccess) return nullptr;
+      ctx->tev.push_back(e);
+    }
+    return ctx->tev[i];
+  };
+  auto timed = [&](int kind, auto&& launch) {
+    if (!timing) {
+      launch();
+      return;
+    }
+    const size_t e0 = nextEv++, e1 = nextEv++;
+    hipEvent_t a = evAt(e0), bb = evAt(e1);
+    if (a && bb) (void)hipEventRecord(a, st);
+    launch();
+    if (a && bb) {
+      (void)hipEventRecord(bb, st);
+      marks.push_back({kind, e0, e1});
+    }
+  };
+  ubg::launchZeroCounters(b.counters, ubg::kNumCounters, st);
+  UBG_HIP(hipEventRecord(ctx->ev[0], st), "hipEventRecord");
+  int nActive = npx;
+  bool cancelled = false;
+  for (int iter = 0; iter < fc.maxIters && nActive > 0; ++iter) {
+    if (p->cancel_flag && *p->cancel_flag) {
+      cancelled = true;
+      break;
+    }
+    ++fr->layers;
+    ubg::launchResetQueues(b.counters, st);
+    timed(UBG_KERNEL_PRIMARY, [&] { ubg::launchPrimary(fc, scene->view, b, iter, nActive, countVisits, st); });
+    if (fc.aoSamples > 0)
+      timed(UBG_KERNEL_AO, [&] { ubg::launchAo(fc, scene->view, b, nActive, countVisits, st); });
+    timed(UBG_KERNEL_RESOLVE, [&] { ubg::launchResolve(fc, scene->view, b, iter, nActive, countVisits, st); });
+    UBG_HIP(hipGetLastError(), "kernel launch");
+    if (iter + 1 >= fc.maxIters) break;
+    // One host round trip per transparency layer: the continuation count.
+    UBG_HIP(hipMemcpyAsync(ctx->hostCounters, b.counters, ubg::kNumCounters * sizeof(unsigned long long),
+                           hipMemcpyDeviceToHost, st),
+            "hipMemcpy(counters)");
+    UBG_HIP(hipStreamSynchronize(st), "layer sync");
+    nActive = static_cast<int>(ctx->hostCounters[ubg::kCntActive]);
+    std::swap(b.activeIn, b.activeOut);
+  }
+  UBG_HIP(hipEventRecord(ctx->ev[1], st), "hipEventRecord");
+  if (cancelled) {
+    UBG_HIP(hipStreamSynchronize(st), "cancel sync");
+    fr->cancelled = 1;
+    return fail(UBG_ERR_CANCELLED, "ubg_render: cancelled");
+  }
+  const bool hiresOnly = (p->flags & UBG_FLAG_HIRES_ONLY) != 0u;
+  if (!hiresOnly)
+    timed(UBG_KERNEL_POST, [&] {
+      ubg::launchPost(b.hiColor, W, H, ss, p->assumed_gamma, static_cast<float4*>(ctx->outColor.p), st);
+    });
+  UBG_HIP(hipEventRecord(ctx->ev[2], st), "hipEventRecord");
+  UBG_HIP(hipGetLastError(), "kernel launch");
+  // ---- outputs ----
+  fr->hi_w = W;
+  fr->hi_h = H;
+  fr->out_w = hiresOnly ? W : p->width;
+  fr->out_h = hiresOnly ? H : p->height;
+  r = copyOut(fr->color, hiresOnly ? ctx->hiColor.p : ctx->outColor.p,
+              (hiresOnly ? n : outN) * 16, fr->memory, st);
+  if (r == UBG_OK) r = copyOut(fr->depth, ctx->depth.p, n * 4, fr->memory, st);
+  if (r == UBG_OK) r = copyOut(fr->view_z, ctx->viewZ.p, n * 4, fr->memory, st);
+  if (r == UBG_OK && fr->hit_id) r = copyOut(fr->hit_id, ctx->hitId.p, n * 4, fr->memory, st);
+  if (r == UBG_OK && fr->ao_open) r = copyOut(fr->ao_open, ctx->aoOpen.p, n * 4, fr->memory, st);
+  if (r != UBG_OK) return r;
+  UBG_HIP(hipMemcpyAsync(ctx->hostCounters, b.counters, ubg::kNumCounters * sizeof(unsigned long long),
+                         hipMemcpyDeviceToHost, st),
+          "hipMemcpy(counters)");
+  UBG_HIP(hipStreamSynchronize(st), "frame sync");
+  float msTrace = 0.0f, msPost = 0.0f;
+  UBG_HIP(hipEventElapsedTime(&msTrace, ctx->ev[0], ctx->ev[1]), "hipEventElapsedTime");
+  UBG_HIP(hipEventElapsedTime(&msPost, ctx->ev[1], ctx->ev[2]), "hipEventElapsedTime");
+  fr->render_seconds = msTrace * 1e-3;
+  fr->post_seconds = msPost * 1e-3;
+  const unsigned long long* c = ctx->hostCounters;
+  fr->stats.primary_rays = c[ubg::kCntPrimary];
+  fr->stats.shaded_hits = c[ubg::kCntShaded];
+  fr->stats.ao_rays_traced = c[ubg::kCntAoTraced];
+  fr->stats.ao_rays_sampled = c[ubg::kCntAoSampled];
+  fr->stats.shadow_rays = c[ubg::kCntShadow];
+  fr->stats.capped_rays = c[ubg::kCntCapped];
+  for (int k = 0; k < 3; ++k) {
+    fr->stats.node_visits[k] = countVisits ? c[ubg::kCntNodes + k] : 0;
+    fr->stats.prim_tests[k] = countVisits ? c[ubg::kCntPrimTests + k] : 0;
+  }
+  for (const Mark& m : marks) {
+    float ms = 0.0f;
+    if (hipEventElapsedTime(&ms, ctx->tev[m.e0], ctx->tev[m.e1]) == hipSuccess) {
+      fr->kernel_ms[m.kind] += ms;
+      fr->kernel_launches[m.kind] += 1;
+    }
+  }
+  return UBG_OK;
+}
+
+static int runQuery(ubg_context* ctx, const ubg_scene* scene, int64_t n, const ubg_ray* rays,
+                    void* out, size_t outBytes, int32_t memory, bool any) {
+  if (!ctx || !scene || (n > 0 && (!rays || !out)) || n < 0)
+    return fail(UBG_ERR_INVALID_ARGUMENT, "ubg query: bad argument");
+  if (scene->ctx != ctx) return fail(UBG_ERR_INVALID_ARGUMENT, "ubg query: scene belongs to another context");
+  if (n == 0) return UBG_OK;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  UBG_HIP(hipSetDevice(ctx->device), "hipSetDevice");
+  hipStream_t st = ctx->stream;
+  const void* dRays = rays;
+  void* dOut = out;
+  if (!memory) {
+    int r = ensure(ctx->query, static_cast<size_t>(n) * sizeof(ubg_ray));
+    if (r == UBG_OK) r = ensure(ctx->queryOut, static_cast<size_t>(n) * outBytes);
+    if (r != UBG_OK) return r;
+    UBG_HIP(hipMemcpyAsync(ctx->query.p, rays, static_cast<size_t>(n) * sizeof(ubg_ray),
+                           hipMemcpyHostToDevice, st),
+            "hipMemcpy(rays)");
+    dRays = ctx->query.p;
+    dOut = ctx->queryOut.p;
+  }
+  ubg::launchQuery(scene->view, dRays, n, any ? nullptr : dOut, any ? static_cast<uint8_t*>(dOut) : nullptr,
+                   any, st);
+  UBG_HIP(hipGetLastError(), "kernel launch");
+  if (!memory)
+    UBG_HIP(hipMemcpyAsync(out, dOut, static_cast<size_t>(n) * outBytes, hipMemcpyDeviceToHost, st),
+            "hipMemcpy(query result)");
+  UBG_HIP(hipStreamSynchronize(st), "query sync");
+  return UBG_OK;
+}
+
+int ubg_intersect(ubg_context* ctx, const ubg_scene* scene, int64_t n_rays, const ubg_ray* rays,
+                  ubg_hit* hits, int32_t memory) {
+  return runQuery(ctx, scene, n_rays, rays, hits, sizeof(ubg_hit), memory, false);
+}
+
+int ubg_occluded(ubg_context* ctx, const ubg_scene* scene, int64_t n_rays, const ubg_ray* rays,
+                 uint8_t* occluded, int32_t memory) {
+  return runQuery(ctx, scene, n_rays, rays, occluded, 1, memory, true);
+}
+
+}  // extern "C"

@@ -1,0 +1,597 @@
+// ub_kernels.hip -- wavefront per-ray kernels for gfx950 (MI355X).
+//
+// One frame runs, per transparency layer:
+//   k_primary : camera (or continuation) ray -> closest hit -> hit setup
+//               (interpolation / side tables, offset origin), compacted into
+//               a structure-of-arrays hit queue with one atomic per wave;
+//               misses finalize their pixel (composite + fused fog).
+//   k_ao      : aoLanes lanes per queued hit, each tracing its share of the
+//               hit's AO rays (TEA-seeded, cosine-weighted), reduced with
+//               cross-lane shuffles to one occluded count per hit.
+//   k_resolve : per hit, the shadow rays, POV local shading, "over"
+//               compositing and either the final store (+ fog) or a push to
+//               the compacted continuation queue for the next layer.
+//   k_post    : supersample box resolve + assumed gamma.
+// Reference: integratePixel (shading/transparency.hpp:311-522), shadeHit
+// (shading/hit_shader.hpp:147-496), shadeLocal (shading/shading.hpp:76-192),
+// computeShadow / computeAO (shading/secondary_rays.hpp:118-134,
+// ao/ambient_occlusion.hpp:33-57), applyFog (postprocess/fog.cpp:17-36),
+// boxDownsample / applyAssumedGamma (postprocess/image_ops.cpp:9-46).
+
+#include <hip/hip_runtime.h>
+
+#include "ub_device.h"
+#include "ub_kernels.h"
+
+namespace ubg {
+
+constexpr int kTraceBlock = 128;
+
+// hitN.w flag bits
+constexpr uint32_t kFlagMesh = 1u << 24;
+constexpr uint32_t kFlagAo = 1u << 25;
+constexpr uint32_t kFlagShadow = 1u << 26;
+
+__device__ __forceinline__ void waveCount(unsigned long long* cnt, int slot, bool pred) {
+  const unsigned long long m = __ballot(pred);
+  if (__lane_id() == 0 && m != 0ull) atomicAdd(&cnt[slot], (unsigned long long)__popcll(m));
+}
+
+__device__ __forceinline__ uint32_t waveSum(uint32_t v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+// Visit-counting launches: add the wave's node fetches / primitive tests to
+// the counters of ray class `cls` (0 primary, 1 AO, 2 shadow).
+__device__ __forceinline__ void addVisits(unsigned long long* cnt, int cls, uint32_t nv, uint32_t nt) {
+  const uint32_t sv = waveSum(nv), st = waveSum(nt);
+  if (__lane_id() == 0) {
+    atomicAdd(&cnt[kCntNodes + cls], (unsigned long long)sv);
+    atomicAdd(&cnt[kCntPrimTests + cls], (unsigned long long)st);
+  }
+}
+
+// embree_renderer.cpp:1400-1414 (top-left origin, pixel centers)
+__device__ __forceinline__ void genRay(const FrameConst& fc, uint32_t px, uint32_t py, V3& org,
+                                       V3& rd) {
+  const float v = 1.0f - 2.0f * ((float)py + 0.5f) / (float)fc.H;
+  const float u = 2.0f * ((float)px + 0.5f) / (float)fc.W - 1.0f;
+  const V3 pos = mk(fc.pos[0], fc.pos[1], fc.pos[2]);
+  const V3 dir = mk(fc.dir[0], fc.dir[1], fc.dir[2]);
+  const V3 right = mk(fc.right[0], fc.right[1], fc.right[2]);
+  const V3 up = mk(fc.up[0], fc.up[1], fc.up[2]);
+  if (fc.ortho) {
+    org = pos + right * (u * fc.halfW) + up * (v * fc.halfH);
+    rd = dir;
+  } else {
+    org = pos;
+    rd = normalize(dir + right * (u * fc.persHalfW) + up * (v * fc.persHalfH));
+  }
+}
+
+// transparency.hpp:118-129 (view-z clip planes -> ray t range)
+__device__ __forceinline__ void clipRange(const FrameConst& fc, V3 rd, float& tn, float& tf) {
+  tn = 0.0f;
+  tf = __builtin_inff();
+  if (!fc.clipOn) return;
+  const float cosr = dot(rd, mk(fc.dir[0], fc.dir[1], fc.dir[2]));
+  if (cosr <= 1.0e-6f) return;
+  if (fc.clipNear > 0.0f) tn = fc.clipNear / cosr;
+  if (fc.clipFar < __builtin_inff()) tf = fc.clipFar / cosr;
+}
+
+// fog.cpp:7-15
+__device__ __forceinline__ float fogFactor(const FrameConst& fc, float z) {
+  const float denom = fc.fogEnd - fc.fogStart;
+  if (denom <= 0.0f) return z < fc.fogEnd ? 1.0f : 0.0f;
+  float f = (fc.fogEnd - z) / denom;
+  if (f < 0.0f) return 0.0f;
+  if (f > 1.0f) return 1.0f;
+  return f;
+}
+
+// transparency.hpp:489-521 (floor over background) + fog.cpp:17-36 fused.
+__device__ __forceinline__ void finalizePixel(const FrameConst& fc, const FrameBufs& b, uint32_t pix,
+                                              float4 acc, V3 base, float baseCov) {
+  const float floorW = (1.0f - acc.w) * baseCov;
+  float outA = acc.w + floorW;
+  outA = fminf(1.0f, fmaxf(0.0f, outA));
+  float r = acc.x + floorW * base.x;
+  float g = acc.y + floorW * base.y;
+  float bl = acc.z + floorW * base.z;
+  if (fc.fogOn) {
+    const float z = b.viewZ[pix];
+    if (z > 0.0f) {
+      const float f = fogFactor(fc, z);
+      if (fc.transparentBg) {
+        outA *= f;
+      } else {
+        r = fc.fogColor[0] * (1.0f - f) + r * f;
+        g = fc.fogColor[1] * (1.0f - f) + g * f;
+        bl = fc.fogColor[2] * (1.0f - f) + bl * f;
+      }
+    }
+  }
+  b.hiColor[pix] = make_float4(r, g, bl, outA);
+}
+
+template <bool kCount>
+__global__ __launch_bounds__(kTraceBlock) void k_primary(FrameConst fc, SceneView s, FrameBufs b,
+                                                         int iter, int nActive) {
+  __shared__ int stk[kMaxDepth * kTraceBlock];
+  const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool valid = tid < nActive;
+  uint32_t pix = 0;
+  bool hit = false;
+  Hit h;
+  V3 org = mk(0, 0, 0), rd = mk(0, 0, 1);
+  uint32_t nv = 0, nt = 0;
+  if (valid) {
+    pix = iter == 0 ? static_cast<uint32_t>(tid) : b.activeIn[tid];
+    const uint32_t px = pix % static_cast<uint32_t>(fc.W), py = pix / static_cast<uint32_t>(fc.W);
+    genRay(fc, px, py, org, rd);
+    float tn, tf;
+    clipRange(fc, rd, tn, tf);
+    if (iter > 0) tn = b.tnear[pix];
+    hit = traverse<false, kCount>(s, org, rd, tn, tf, h, stk + threadIdx.x, kTraceBlock, &nv, &nt);
+    if (iter == 0) {
+      const float vz = hit ? h.t * dot(rd, mk(fc.dir[0], fc.dir[1], fc.dir[2])) : 0.0f;
+      b.depth[pix] = hit ? h.t : 0.0f;
+      b.viewZ[pix] = vz;
+      if (b.hitId) b.hitId[pix] = hit ? h.bits : 0xFFFFFFFFu;
+      if (b.aoOpen && !hit) b.aoOpen[pix] = 1.0f;
+    }
+    if (!hit) {
+      const float4 acc = iter == 0 ? make_float4(0.0f, 0.0f, 0.0f, 0.0f) : b.acc[pix];
+      finalizePixel(fc, b, pix, acc, mk(fc.bg[0], fc.bg[1], fc.bg[2]),
+                    fc.transparentBg ? 0.0f : 1.0f);
+    }
+  }
+  waveCount(b.counters, kCntPrimary, valid);
+  if (kCount) addVisits(b.counters, 0, nv, nt);
+  // Compaction: one atomic per wave reserves the wave's slots.
+  const unsigned long long m = __ballot(hit);
+  uint32_t base = 0;
+  if (__lane_id() == 0 && m != 0ull)
+    base = static_cast<uint32_t>(atomicAdd(&b.counters[kCntHits], (unsigned long long)__popcll(m)));
+  base = __shfl(base, 0, 64);
+  if (!hit) return;
+  const uint32_t slot = base + static_cast<uint32_t>(__popcll(m & ((1ull << __lane_id()) - 1ull)));
+
+  // ---- hit setup (hit_shader.hpp:147-316) ----
+  const uint32_t kind = h.bits >> kKindShift;
+  const uint32_t id = h.bits & kIdMask;
+  V3 N, C;
+  float opacity;
+  uint32_t flags;
+  bool fromEdge = false;
+  if (kind == 0u) {
+    const uint4 tri = reinterpret_cast<const uint4*>(s.triIdx)[id];
+    const float4* vn = reinterpret_cast<const float4*>(s.vtxNrm);
+    const float4* vc = reinterpret_cast<const float4*>(s.vtxCol);
+    const float4 n0 = vn[tri.x], n1 = vn[tri.y], n2 = vn[tri.z];
+    const float4 c0 = vc[tri.x], c1 = vc[tri.y], c2 = vc[tri.z];
+    const float w0 = 1.0f - h.u - h.v;
+    const V3 nb = mk(w0 * n0.x + h.u * n1.x + h.v * n2.x, w0 * n0.y + h.u * n1.y + h.v * n2.y,
+                     w0 * n0.z + h.u * n1.z + h.v * n2.z);
+    C = mk(w0 * c0.x + h.u * c1.x + h.v * c2.x, w0 * c0.y + h.u * c1.y + h.v * c2.y,
+           w0 * c0.z + h.u * c1.z + h.v * c2.z);
+    opacity = w0 * c0.w + h.u * c1.w + h.v * c2.w;
+    const V3 ngF = faceForward(h.ng, rd);
+    N = normalize(nb);
+    if (dot(N, ngF) < 0.0f) N = neg(N);
+    flags = (tri.w & kInfoMatMask) | kFlagMesh;
+  } else {
+    float4 col;
+    uint32_t info;
+    float op1 = -1.0f;
+    if (kind == 1u) {
+      col = reinterpret_cast<const float4*>(s.sphColor)[id];
+      info = s.sphInfo[id];
+    } else if (kind == 3u) {
+      col = reinterpret_cast<const float4*>(s.coneColor)[id];
+      info = s.coneInfo[id];
+      op1 = s.coneOp1[id];
+    } else {
+      col = reinterpret_cast<const float4*>(s.rndColor)[id];
+      info = s.rndInfo[id];
+      op1 = s.rndOp1[id];
+    }
+    N = faceForward(normalize(h.ng), rd);
+    C = mk(col.x, col.y, col.z);
+    opacity = col.w;
+    if (op1 >= 0.0f) {
+      float u = h.u;
+      if (u < 0.0f) u = 0.0f;
+      if (u > 1.0f) u = 1.0f;
+      opacity = col.w * (1.0f - u) + op1 * u;
+    }
+    fromEdge = (info & kInfoFromEdge) != 0u;
+    flags = info & kInfoMatMask;
+  }
+  if (!fromEdge && fc.aoSamples > 0) flags |= kFlagAo;
+  if (!fromEdge && fc.shadows) flags |= kFlagShadow;
+  const V3 P = mk(org.x + rd.x * h.t, org.y + rd.y * h.t, org.z + rd.z * h.t);
+  const float eps = selfIntersectEps(P, rd, h.t);
+  // ambient_occlusion.hpp:40-43 / secondary_rays.hpp:121-123: the same offset
+  // origin serves the AO and the shadow rays.
+  V3 ng = dot(h.ng, N) < 0.0f ? neg(h.ng) : h.ng;
+  ng = safeNormalize(ng, N);
+  const V3 O = P + ng * eps;
+  b.hitPix[slot] = pix;
+  b.hitT[slot] = h.t;
+  b.hitO[slot] = make_float4(O.x, O.y, O.z, eps);
+  b.hitN[slot] = make_float4(N.x, N.y, N.z, __uint_as_float(flags));
+  b.hitC[slot] = make_float4(C.x, C.y, C.z, opacity);
+}
+
+template <bool kCount>
+__global__ __launch_bounds__(kTraceBlock) void k_ao(FrameConst fc, SceneView s, FrameBufs b,
+                                                    int nMax) {
+  __shared__ int stk[kMaxDepth * kTraceBlock];
+  const int nHits = min(nMax, static_cast<int>(b.counters[kCntHits]));
+  const int lanes = fc.aoLanes;
+  const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+  const int hi = tid / lanes;
+  const int j = tid % lanes;
+  const bool valid = hi < nHits;
+  float4 O4 = make_float4(0, 0, 0, 0), N4 = make_float4(0, 0, 1, 0);
+  uint32_t pix = 0;
+  bool on = false;
+  if (valid) {
+    O4 = b.hitO[hi];
+    N4 = b.hitN[hi];
+    pix = b.hitPix[hi];
+    on = (__float_as_uint(N4.w) & kFlagAo) != 0u;
+  }
+  const V3 N = ld3(N4);
+  const V3 O = ld3(O4);
+  const Frame3 f = frameFromNormal(N);
+  const uint32_t seed = (pix % static_cast<uint32_t>(fc.W)) +
+                        (pix / static_cast<uint32_t>(fc.W)) * static_cast<uint32_t>(fc.W);
+  const int n = fc.aoSamples;
+  const int rounds = (n + lanes - 1) / lanes;
+  uint32_t occ = 0, nv = 0, nt = 0;
+  for (int r = 0; r < rounds; ++r) {
+    const int si = j + r * lanes;
+    const bool act = valid && on && si < n;
+    bool traced = false, sampled = false;
+    if (act) {
+      uint32_t s0 = seed, s1 = static_cast<uint32_t>(si);
+      tea2(s0, s1);
+      const V3 dir = cosineSample(u32ToUnorm(s0), u32ToUnorm(s1), f);
+      if (dot(dir, N) < 0.01f) {
+        sampled = true;
+        ++occ;
+      } else {
+        traced = true;
+        Hit h;
+        if (traverse<true, kCount>(s, O, dir, O4.w, fc.aoDist, h, stk + threadIdx.x, kTraceBlock,
+                                   &nv, &nt))
+          ++occ;
+      }
+    }
+    waveCount(b.counters, kCntAoTraced, traced);
+    waveCount(b.counters, kCntAoSampled, sampled);
+  }
+  if (kCount) addVisits(b.counters, 1, nv, nt);
+  for (int off = lanes >> 1; off > 0; off >>= 1) occ += __shfl_xor(occ, off, lanes);
+  if (valid && j == 0) b.hitAo[hi] = occ;
+}
+
+template <bool kCount>
+__global__ __launch_bounds__(kTraceBlock) void k_resolve(FrameConst fc, SceneView s, FrameBufs b,
+                                                         int iter, int nMax) {
+  __shared__ int stk[kMaxDepth * kTraceBlock];
+  const int nHits = min(nMax, static_cast<int>(b.counters[kCntHits]));
+  const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool valid = tid < nHits;
+  uint32_t nShadow = 0, nv = 0, nt = 0;
+  bool capped = false, cont = false;
+  uint32_t pix = 0;
+  float4 acc = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  float tnext = 0.0f;
+  if (valid) {
+    pix = b.hitPix[tid];
+    const float t = b.hitT[tid];
+    const float4 O4 = b.hitO[tid];
+    const float4 N4 = b.hitN[tid];
+    const float4 C4 = b.hitC[tid];
+    const uint32_t flags = __float_as_uint(N4.w);
+    const uint32_t px = pix % static_cast<uint32_t>(fc.W), py = pix / static_cast<uint32_t>(fc.W);
+    V3 org, rd;
+    genRay(fc, px, py, org, rd);
+    const V3 V = normalize(neg(rd));
+    const V3 N = ld3(N4);
+    const V3 C = ld3(C4);
+    const V3 O = ld3(O4);
+    const float eps = O4.w;
+    const DevMaterial m = (flags & kFlagMesh) ? s.meshMats[flags & kInfoMatMask]
+                                              : s.primMats[flags & kInfoMatMask];
+    // ao_shade.hpp:100-103 (legacy scalar aoFactor)
+    float aoF = 1.0f, open = 1.0f;
+    if (flags & kFlagAo) {
+      open = 1.0f - (float)b.hitAo[tid] / (float)fc.aoSamples;
+      aoF = 1.0f - fc.aoIntensity * (1.0f - open);
+    }
+    if (iter == 0 && b.aoOpen) b.aoOpen[pix] = open;
+    // shading.hpp:92-190
+    V3 out = mk(m.emission * C.x + aoF * m.ambient * C.x * fc.amb[0],
+                m.emission * C.y + aoF * m.ambient * C.y * fc.amb[1],
+                m.emission * C.z + aoF * m.ambient * C.z * fc.amb[2]);
+    const bool shOn = (flags & kFlagShadow) != 0u;
+    uint32_t s0 = px, s1 = py;
+    for (int li = 0; li < fc.nLights; ++li) {
+      const DevLight& l = fc.lights[li];
+      const V3 L = mk(l.L[0], l.L[1], l.L[2]);
+      const float ndl = dot(N, L);
+      if (ndl <= 0.0f) continue;
+      float sf = 1.0f;
+      if (shOn) {
+        Hit h;
+        if (!l.soft) {
+          ++nShadow;
+          sf = traverse<true, kCount>(s, O, L, eps, __builtin_inff(), h, stk + threadIdx.x,
+                                      kTraceBlock, &nv, &nt)
+                   ? 0.0f
+                   : 1.0f;
+        } else {
+          int hits = 0;
+          for (int i = 0; i < fc.shadowSamples; ++i) {
+            tea2(s0, s1);
+            const V3 dir = sampleLightDir(L, l.tanRadius, u32ToUnorm(s0), u32ToUnorm(s1));
+            ++nShadow;
+            if (traverse<true, kCount>(s, O, dir, eps, __builtin_inff(), h, stk + threadIdx.x,
+                                       kTraceBlock, &nv, &nt))
+              ++hits;
+          }
+          sf = 1.0f - (float)hits / (float)fc.shadowSamples;
+        }
+      }
+      const V3 Lc = mk(l.color[0] * sf, l.color[1] * sf, l.color[2] * sf);
+      float dd;
+      if (m.brilliance == 1.0f) dd = ndl;
+      else if (m.brilliance == 0.0f) dd = 1.0f;
+      else dd = powf(ndl, m.brilliance);
+      const float dk = m.diffuse * dd * 1.0f;
+      out.x += dk * C.x * Lc.x;
+      out.y += dk * C.y * Lc.y;
+      out.z += dk * C.z * Lc.z;
+      if (!l.highlight) continue;
+      V3 hl;
+      if (m.metallic) {
+        float cc = ndl;
+        if (cc > 1.0f) cc = 1.0f;
+        const float x = acosf(cc) * 0.63661977f;
+        float ff = 0.014567225f / ((x - 1.12f) * (x - 1.12f)) - 0.011612903f;
+        if (ff < 0.0f) ff = 0.0f;
+        if (ff > 1.0f) ff = 1.0f;
+        hl = mk(Lc.x * (ff + (1.0f - ff) * C.x), Lc.y * (ff + (1.0f - ff) * C.y),
+                Lc.z * (ff + (1.0f - ff) * C.z));
+      } else {
+        hl = Lc;
+      }
+      float specW = 0.0f;
+      if (m.specular > 0.0f) {
+        const V3 H = normalize(mk(L.x + V.x, L.y + V.y, L.z + V.z));
+        const float nh = dot(N, H);
+        if (nh > 0.0f) specW += m.specular * powf(nh, m.blinnExp);
+      }
+      if (m.phong > 0.0f) {
+        const V3 Rr = N * (2.0f * ndl) - L;
+        const float rv = dot(Rr, V);
+        if (rv > 0.0f && (m.phongSize < 60.0f || rv > 0.0008f))
+          specW += m.phong * powf(rv, m.phongSize);
+      }
+      if (specW > 0.0f) {
+        const float sc = specW * fc.specScale;
+        out.x += sc * hl.x;
+        out.y += sc * hl.y;
+        out.z += sc * hl.z;
+      }
+    }
+    if (m.reflection > 0.0f) {
+      out.x += m.reflection * fc.bg[0];
+      out.y += m.reflection * fc.bg[1];
+      out.z += m.reflection * fc.bg[2];
+    }
+    // transparency.hpp:431-467 (opaque floor / over / step past the surface)
+    if (iter > 0) acc = b.acc[pix];
+    const float opacity = C4.w;
+    const float kOpaque = 1.0f - 1e-4f;
+    if (!fc.transparency || opacity >= kOpaque) {
+      finalizePixel(fc, b, pix, acc, out, 1.0f);
+    } else {
+      const float w = (1.0f - acc.w) * opacity;
+      acc.x += w * out.x;
+      acc.y += w * out.y;
+      acc.z += w * out.z;
+      acc.w += w;
+      if (acc.w >= kOpaque) {
+        finalizePixel(fc, b, pix, acc, mk(fc.bg[0], fc.bg[1], fc.bg[2]),
+                      fc.transparentBg ? 0.0f : 1.0f);
+      } else {
+        const V3 hp = mk(org.x + rd.x * t, org.y + rd.y * t, org.z + rd.z * t);
+        const float dirMax = fmaxf(fabsf(rd.x), fmaxf(fabsf(rd.y), fabsf(rd.z)));
+        const float es = fmaxf(fmaxf(fabsf(hp.x), fabsf(hp.y)), fmaxf(fabsf(hp.z), dirMax * t));
+        tnext = t + es * 0x1.0fp-21f;
+        if (iter == fc.maxIters - 1) {
+          capped = true;
+          finalizePixel(fc, b, pix, acc, mk(fc.bg[0], fc.bg[1], fc.bg[2]),
+                        fc.transparentBg ? 0.0f : 1.0f);
+        } else {
+          cont = true;
+        }
+      }
+    }
+  }
+  waveCount(b.counters, kCntShaded, valid);
+  waveCount(b.counters, kCntCapped, capped);
+  {
+    const uint32_t sh = waveSum(nShadow);
+    if (__lane_id() == 0 && sh) atomicAdd(&b.counters[kCntShadow], (unsigned long long)sh);
+  }
+  if (kCount) addVisits(b.counters, 2, nv, nt);
+  const unsigned long long m = __ballot(cont);
+  uint32_t base = 0;
+  if (__lane_id() == 0 && m != 0ull)
+    base = static_cast<uint32_t>(atomicAdd(&b.counters[kCntActive], (unsigned long long)__popcll(m)));
+  base = __shfl(base, 0, 64);
+  if (!cont) return;
+  const uint32_t slot = base + static_cast<uint32_t>(__popcll(m & ((1ull << __lane_id()) - 1ull)));
+  b.activeOut[slot] = pix;
+  b.acc[pix] = acc;
+  b.tnear[pix] = tnext;
+}
+
+// image_ops.cpp:9-46 -- box resolve (non-finite samples contribute 0) and
+// assumed gamma. ss == 1 keeps values as they are (the reference skips the
+// downsample entirely).
+__global__ void k_post(const float4* __restrict__ hi, int W, int H, int ss, float gamma,
+                       float4* __restrict__ out) {
+  const int ow = W / ss, oh = H / ss;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= ow * oh) return;
+  const int x = i % ow, y = i / ow;
+  float4 v;
+  if (ss == 1) {
+    v = hi[i];
+  } else {
+    const float inv = 1.0f / (float)(ss * ss);
+    float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f, a3 = 0.0f;
+    for (int dy = 0; dy < ss; ++dy)
+      for (int dx = 0; dx < ss; ++dx) {
+        const float4 sv = hi[(size_t)(y * ss + dy) * W + (x * ss + dx)];
+        a0 += isfinite(sv.x) ? sv.x : 0.0f;
+        a1 += isfinite(sv.y) ? sv.y : 0.0f;
+        a2 += isfinite(sv.z) ? sv.z : 0.0f;
+        a3 += isfinite(sv.w) ? sv.w : 0.0f;
+      }
+    v = make_float4(a0 * inv, a1 * inv, a2 * inv, a3 * inv);
+  }
+  if (!(fabsf(gamma - 1.0f) <= 1.0e-4f)) {
+    v.x = v.x > 0.0f ? powf(v.x, gamma) : 0.0f;
+    v.y = v.y > 0.0f ? powf(v.y, gamma) : 0.0f;
+    v.z = v.z > 0.0f ? powf(v.z, gamma) : 0.0f;
+  }
+  out[i] = v;
+}
+
+struct RayIn {
+  float org[3];
+  float tnear;
+  float dir[3];
+  float tfar;
+};
+struct HitOut {
+  float t, u, v;
+  uint32_t kind, prim;
+  float ng[3];
+};
+
+__global__ __launch_bounds__(kTraceBlock) void k_query(SceneView s, const RayIn* rays, int64_t n,
+                                                       HitOut* hits, uint8_t* occ, int any) {
+  __shared__ int stk[kMaxDepth * kTraceBlock];
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const RayIn r = rays[i];
+  const V3 o = mk(r.org[0], r.org[1], r.org[2]);
+  const V3 d = mk(r.dir[0], r.dir[1], r.dir[2]);
+  Hit h;
+  uint32_t nv = 0, nt = 0;
+  if (any) {
+    occ[i] = traverse<true, false>(s, o, d, r.tnear, r.tfar, h, stk + threadIdx.x, kTraceBlock, &nv,
+                                   &nt)
+                 ? 1
+                 : 0;
+    return;
+  }
+  HitOut out;
+  if (traverse<false, false>(s, o, d, r.tnear, r.tfar, h, stk + threadIdx.x, kTraceBlock, &nv,
+                             &nt)) {
+    out.t = h.t;
+    out.u = h.u;
+    out.v = h.v;
+    out.kind = h.bits >> kKindShift;
+    out.prim = h.bits & kIdMask;
+    out.ng[0] = h.ng.x;
+    out.ng[1] = h.ng.y;
+    out.ng[2] = h.ng.z;
+  } else {
+    out.t = __builtin_inff();
+    out.u = out.v = 0.0f;
+    out.kind = 0xFFFFFFFFu;
+    out.prim = 0xFFFFFFFFu;
+    out.ng[0] = out.ng[1] = out.ng[2] = 0.0f;
+  }
+  hits[i] = out;
+}
+
+__global__ void k_zero(unsigned long long* c, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) c[i] = 0ull;
+}
+__global__ void k_reset_queues(unsigned long long* c) {
+  if (threadIdx.x == 0) {
+    c[kCntHits] = 0ull;
+    c[kCntActive] = 0ull;
+  }
+}
+
+static inline unsigned blocksFor(int64_t n, int bs) {
+  return static_cast<unsigned>((n + bs - 1) / bs);
+}
+
+void launchZeroCounters(unsigned long long* c, int n, hipStream_t st) {
+  hipLaunchKernelGGL(k_zero, dim3(blocksFor(n, 64)), dim3(64), 0, st, c, n);
+}
+void launchResetQueues(unsigned long long* c, hipStream_t st) {
+  hipLaunchKernelGGL(k_reset_queues, dim3(1), dim3(64), 0, st, c);
+}
+void launchPrimary(const FrameConst& fc, const SceneView& s, const FrameBufs& b, int iter,
+                   int nActive, bool countVisits, hipStream_t st) {
+  if (nActive <= 0) return;
+  if (countVisits)
+    hipLaunchKernelGGL(k_primary<true>, dim3(blocksFor(nActive, kTraceBlock)), dim3(kTraceBlock), 0,
+                       st, fc, s, b, iter, nActive);
+  else
+    hipLaunchKernelGGL(k_primary<false>, dim3(blocksFor(nActive, kTraceBlock)), dim3(kTraceBlock), 0,
+                       st, fc, s, b, iter, nActive);
+}
+void launchAo(const FrameConst& fc, const SceneView& s, const FrameBufs& b, int nMax,
+              bool countVisits, hipStream_t st) {
+  if (nMax <= 0 || fc.aoSamples <= 0) return;
+  const int64_t n = (int64_t)nMax * fc.aoLanes;
+  if (countVisits)
+    hipLaunchKernelGGL(k_ao<true>, dim3(blocksFor(n, kTraceBlock)), dim3(kTraceBlock), 0, st, fc, s,
+                       b, nMax);
+  else
+    hipLaunchKernelGGL(k_ao<false>, dim3(blocksFor(n, kTraceBlock)), dim3(kTraceBlock), 0, st, fc,
+                       s, b, nMax);
+}
+void launchResolve(const FrameConst& fc, const SceneView& s, const FrameBufs& b, int iter,
+                   int nMax, bool countVisits, hipStream_t st) {
+  if (nMax <= 0) return;
+  if (countVisits)
+    hipLaunchKernelGGL(k_resolve<true>, dim3(blocksFor(nMax, kTraceBlock)), dim3(kTraceBlock), 0,
+                       st, fc, s, b, iter, nMax);
+  else
+    hipLaunchKernelGGL(k_resolve<false>, dim3(blocksFor(nMax, kTraceBlock)), dim3(kTraceBlock), 0,
+                       st, fc, s, b, iter, nMax);
+}
+void launchPost(const float4* hi, int W, int H, int ss, float gamma, float4* out, hipStream_t st) {
+  const int n = (W / ss) * (H / ss);
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_post, dim3(blocksFor(n, 256)), dim3(256), 0, st, hi, W, H, ss, gamma, out);
+}
+void launchQuery(const SceneView& s, const void* rays, int64_t n, void* hits, uint8_t* occ,
+                 bool any, hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_query, dim3(blocksFor(n, kTraceBlock)), dim3(kTraceBlock), 0, st, s,
+                     static_cast<const RayIn*>(rays), n, static_cast<HitOut*>(hits), occ,
+                     any ? 1 : 0);
+}
+
+}  // namespace ubg

@@ -1,0 +1,120 @@
+// ub_layout.h -- HBM layout of a built scene and the per-frame constants,
+// shared by the host builder (ub_build.cpp), the C-ABI (ub_capi.cpp) and the
+// gfx950 kernels (ub_kernels.hip).
+//
+// Scene layout (all arrays 16-byte aligned, structure-of-arrays by role):
+//   nodes  : BVH2, 64 B per inner node = 4 x float4
+//              [0] = (c0.lo.x, c0.hi.x, c0.lo.y, c0.hi.y)
+//              [1] = (c1.lo.x, c1.hi.x, c1.lo.y, c1.hi.y)
+//              [2] = (c0.lo.z, c0.hi.z, c1.lo.z, c1.hi.z)
+//              [3] = (child0, child1, -, -) as int; child >= 0 is an inner
+//                    node, child < 0 is a leaf ~((first << 3) | (count - 1))
+//   prims  : 48 B per primitive = 3 x float4, in leaf order, so a leaf is one
+//            contiguous read. The last word holds (kind << 30) | primID.
+//              sphere : (cx, cy, cz, r) | - | (-, -, -, bits)
+//              cone   : (ax, ay, az, ra) | (bx, by, bz, rb) | (-, -, -, bits)
+//              round  : (ax, ay, az, ra) | (bx, by, bz, rb) | (-, -, -, bits)
+//              tri    : (v0x, v0y, v0z, v1x) | (v1y, v1z, v2x, v2y) | (v2z, -, -, bits)
+//   shading side tables, indexed by the per-kind primID (read once per hit,
+//   never during traversal): color float4, info uint32 (material index in
+//   bits 0..23, fromEdge in bit 31), opacity1 float; round curves add two
+//   float4 joint planes (nL.xyz, hasL) and (nR.xyz, hasR); triangles carry
+//   uint4 (i0, i1, i2, material) into per-vertex normal/color float4 arrays.
+#pragma once
+
+#include <stdint.h>
+
+namespace ubg {
+
+constexpr int kMaxDepth = 32;   // builder guarantee; = traversal stack entries
+constexpr int kMaxLeaf = 8;     // primitives per leaf (3 count bits)
+constexpr int kMaxLights = 16;  // lights carried in the kernel argument block
+
+constexpr uint32_t kKindShift = 30;
+constexpr uint32_t kIdMask = (1u << 30) - 1u;
+constexpr uint32_t kInfoFromEdge = 1u << 31;
+constexpr uint32_t kInfoMatMask = (1u << 24) - 1u;
+
+// Shading-ready material (precomputed Blinn exponent). 48 bytes.
+struct DevMaterial {
+  float ambient, diffuse, specular, brilliance;
+  float phong, phongSize, reflection, emission;
+  float blinnExp;
+  uint32_t metallic;
+  float roughness, pad;
+};
+
+struct SceneView {
+  const void* nodes;      // float4 x 4 per inner node
+  const void* prims;      // float4 x 3 per primitive
+  int32_t nNodes;
+  int32_t nPrims;
+  const void* sphColor;   // float4
+  const uint32_t* sphInfo;
+  const void* coneColor;  // float4
+  const uint32_t* coneInfo;
+  const float* coneOp1;
+  const void* rndColor;   // float4
+  const uint32_t* rndInfo;
+  const float* rndOp1;
+  const void* rndPlanes;  // float4 x 2
+  const void* triIdx;     // uint4
+  const void* vtxNrm;     // float4
+  const void* vtxCol;     // float4
+  const DevMaterial* primMats;
+  const DevMaterial* meshMats;
+};
+
+struct DevLight {
+  float L[3];
+  float color[3];
+  float radius;    // radians
+  float tanRadius;
+  int32_t highlight;
+  int32_t soft;    // shadowSamples > 1 && radius > 0
+  int32_t pad[2];
+};
+
+// Per-frame constants, passed by value in the kernel argument block.
+struct FrameConst {
+  int32_t W, H;            // hi-res grid
+  int32_t ortho;
+  float pos[3], dir[3], right[3], up[3];
+  float halfW, halfH, persHalfW, persHalfH;
+  float clipNear, clipFar; // view-z planes; clipOn == 0 => [0, inf)
+  int32_t clipOn;
+  float bg[3];
+  float amb[3];
+  int32_t transparentBg;
+  int32_t transparency;
+  int32_t maxIters;
+  int32_t aoSamples;
+  float aoDist;
+  float aoIntensity;
+  int32_t aoLanes;         // lanes per hit in the AO kernel (power of two <= 64)
+  int32_t shadows;
+  int32_t shadowSamples;
+  float specScale;
+  int32_t fogOn;
+  float fogColor[3];
+  float fogStart, fogEnd;
+  int32_t nLights;
+  DevLight lights[kMaxLights];
+};
+
+// Frame-time counters (u64), one slot each.
+enum Counter {
+  kCntPrimary = 0,
+  kCntShaded = 1,
+  kCntAoTraced = 2,
+  kCntAoSampled = 3,
+  kCntShadow = 4,
+  kCntCapped = 5,
+  kCntHits = 6,      // hit-queue fill of the current pass
+  kCntActive = 7,    // continuation-queue fill of the current pass
+  kCntNodes = 8,      // node visits per ray class: 8 primary, 9 AO, 10 shadow
+  kCntPrimTests = 11, // primitive tests per ray class: 11, 12, 13
+  kNumCounters = 16
+};
+
+}  // namespace ubg

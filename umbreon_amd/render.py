@@ -1,0 +1,162 @@
+"""Host-side driver over the C-ABI: device context, built scene, frame.
+
+Mirrors the reference's public flow -- ``umbreon::render(scene, opt) ->
+FrameResult`` (src/umbreon/umbreon.hpp:29) -- with the device context and the
+built scene exposed so a caller can reuse them across frames (the
+``SharedDevice`` reuse of src/umbreon/umbreon.cpp:21-39).
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+from typing import Optional
+
+import numpy as np
+
+from . import capi
+from .scene import RenderOptions, Scene, params
+
+
+@dataclass
+class FrameResult:
+    """umbreon::FrameResult (render/frame_result.hpp:43-124), numpy-backed."""
+    width: int
+    height: int
+    color: np.ndarray                  # (H, W, 4) linear HDR RGBA
+    depth: Optional[np.ndarray]        # (hiH, hiW) first-hit distance (0 on miss)
+    view_z: Optional[np.ndarray]       # (hiH, hiW) linear view-z
+    hit_id: Optional[np.ndarray] = None
+    ao_open: Optional[np.ndarray] = None
+    stats: dict = field(default_factory=dict)
+    render_seconds: float = 0.0
+    post_seconds: float = 0.0
+    cancelled: bool = False
+    layers: int = 0
+    kernel_ms: dict = field(default_factory=dict)
+    kernel_launches: dict = field(default_factory=dict)
+
+    @property
+    def total_rays(self) -> int:
+        s = self.stats
+        return int(s["primary_rays"] + s["ao_rays_traced"] + s["ao_rays_sampled"] + s["shadow_rays"])
+
+
+class Device:
+    """One gfx950 device context (stream + reusable frame scratch)."""
+
+    def __init__(self, ordinal: int = 0):
+        self._lib = capi.load()
+        h = C.c_void_p()
+        capi.check(self._lib.ubg_context_create(int(ordinal), C.byref(h)))
+        self.handle = h
+
+    def close(self) -> None:
+        if self.handle:
+            self._lib.ubg_context_destroy(self.handle)
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class BuiltScene:
+    """Scene geometry resident in HBM with its BVH (buildEmbreeScene analogue)."""
+
+    def __init__(self, device: Device, scene: Scene):
+        self.device = device
+        self._lib = device._lib
+        desc, keep = scene.to_desc()
+        h = C.c_void_p()
+        capi.check(self._lib.ubg_scene_create(device.handle, C.byref(desc), C.byref(h)))
+        del keep
+        self.handle = h
+        info = capi.SceneInfo()
+        capi.check(self._lib.ubg_scene_get_info(self.handle, C.byref(info)))
+        self.info = info.as_dict()
+
+    def close(self) -> None:
+        if self.handle:
+            self._lib.ubg_scene_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # rtcIntersect1 / rtcOccluded1, batched
+    def intersect(self, rays: np.ndarray) -> np.ndarray:
+        rays = np.ascontiguousarray(rays, dtype=capi.ray_dtype())
+        hits = np.zeros(len(rays), dtype=capi.hit_dtype())
+        capi.check(self._lib.ubg_intersect(self.device.handle, self.handle, len(rays),
+                                           rays.ctypes.data, hits.ctypes.data, 0))
+        return hits
+
+    def occluded(self, rays: np.ndarray) -> np.ndarray:
+        rays = np.ascontiguousarray(rays, dtype=capi.ray_dtype())
+        occ = np.zeros(len(rays), dtype=np.uint8)
+        capi.check(self._lib.ubg_occluded(self.device.handle, self.handle, len(rays),
+                                          rays.ctypes.data, occ.ctypes.data, 0))
+        return occ
+
+    def render(self, scene: Scene, opt: RenderOptions, hires_only: bool = False,
+               want_ids: bool = False, want_ao: bool = False, count_visits: bool = False,
+               kernel_times: bool = False, outputs: bool = True,
+               cancel_flag: Optional[C.c_int32] = None) -> FrameResult:
+        """One frame. outputs=False leaves the frame in HBM (no host copy)."""
+        flags = ((capi.FLAG_HIRES_ONLY if hires_only else 0) | (capi.FLAG_COUNT_VISITS if count_visits else 0)
+                 | (capi.FLAG_KERNEL_TIMES if kernel_times else 0))
+        p, keep = params(scene, opt, flags, cancel_flag)
+        ss = max(1, int(opt.supersample))
+        hw, hh = opt.width * ss, opt.height * ss
+        ow, oh = (hw, hh) if hires_only else (opt.width, opt.height)
+        color = np.zeros((oh, ow, 4), np.float32) if outputs else None
+        depth = np.zeros((hh, hw), np.float32) if outputs else None
+        view_z = np.zeros((hh, hw), np.float32) if outputs else None
+        hit_id = np.zeros((hh, hw), np.uint32) if want_ids else None
+        ao = np.zeros((hh, hw), np.float32) if want_ao else None
+        fr = capi.Frame()
+        fr.color = color.ctypes.data if color is not None else None
+        fr.depth = depth.ctypes.data if depth is not None else None
+        fr.view_z = view_z.ctypes.data if view_z is not None else None
+        fr.hit_id = hit_id.ctypes.data if hit_id is not None else None
+        fr.ao_open = ao.ctypes.data if ao is not None else None
+        fr.memory = 0
+        capi.check(self._lib.ubg_render(self.device.handle, self.handle, C.byref(p), C.byref(fr)))
+        del keep
+        return FrameResult(width=ow, height=oh, color=color, depth=depth, view_z=view_z,
+                           hit_id=hit_id, ao_open=ao, stats=fr.stats.as_dict(),
+                           render_seconds=fr.render_seconds, post_seconds=fr.post_seconds,
+                           cancelled=bool(fr.cancelled), layers=int(fr.layers),
+                           kernel_ms={k: float(fr.kernel_ms[i]) for i, k in enumerate(capi.KERNELS)},
+                           kernel_launches={k: int(fr.kernel_launches[i]) for i, k in enumerate(capi.KERNELS)})
+
+
+_default_device: Optional[Device] = None
+
+
+def default_device() -> Device:
+    global _default_device
+    if _default_device is None:
+        _default_device = Device(0)
+    return _default_device
+
+
+def render(scene: Scene, opt: RenderOptions, **kw) -> FrameResult:
+    """umbreon::render(scene, opt): build, trace, resolve; one call."""
+    dev = default_device()
+    bs = BuiltScene(dev, scene)
+    try:
+        return bs.render(scene, opt, **kw)
+    finally:
+        bs.close()

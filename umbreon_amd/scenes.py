@@ -1,0 +1,224 @@
+"""Seeded synthetic molecule scenes for the benchmark configurations.
+
+The reference's real CueMol exports are Git-LFS pointers, so every workload
+is generated here from a seed:
+
+* :func:`protein_atoms` -- a C-alpha random walk (3.8 A steps, bond angle
+  uniform in [90, 120] degrees, uniform dihedral) with 3-5 side atoms per
+  residue chained at 1.5 A, element drawn from a 5-entry palette.
+* :func:`spheres_scene` -- "2,000 spheres of a synthetic protein", radii from
+  {1.2, 1.55, 1.7, 1.8} A, POV Phong (diffuse 0.7, specular 0.3, roughness
+  0.01), one distant light plus ambient, hard shadows, 800x600 at 1 spp.
+* :func:`ball_and_stick_scene` -- 20,000 atom spheres (r 0.4 A) plus every
+  pair closer than 1.9 A drawn as two capped half-bond cylinders (r 0.2 A,
+  one per atom colour), AO 16 (max distance 8 A), shadows, linear depth fog
+  over 60 A, 1920x1080 with 2x2 supersampling.
+"""
+from __future__ import annotations
+
+import functools
+import math
+from typing import Tuple
+
+import numpy as np
+
+from .scene import Camera, DistantLight, Fog, Material, RenderOptions, Scene
+
+# C, N, O, S, H-like colours (linear RGB)
+PALETTE = np.array([
+    [0.55, 0.55, 0.55, 1.0],
+    [0.20, 0.30, 1.00, 1.0],
+    [1.00, 0.15, 0.15, 1.0],
+    [1.00, 0.85, 0.20, 1.0],
+    [0.92, 0.92, 0.92, 1.0],
+], np.float32)
+RADII = np.array([1.2, 1.55, 1.7, 1.8], np.float32)
+
+LIT = Material(ambient=0.2, diffuse=0.7, specular=0.3, roughness=0.01)
+
+
+def _perp(v: np.ndarray, rng: np.random.Generator) -> np.ndarray:
+    a = np.array([1.0, 0.0, 0.0]) if abs(v[0]) < 0.9 else np.array([0.0, 1.0, 0.0])
+    p = np.cross(v, a)
+    p /= np.linalg.norm(p)
+    ang = rng.uniform(0.0, 2.0 * math.pi)
+    q = np.cross(v, p)
+    return math.cos(ang) * p + math.sin(ang) * q
+
+
+def protein_atoms(n_atoms: int, seed: int = 0) -> Tuple[np.ndarray, np.ndarray]:
+    p, e = _protein_atoms(int(n_atoms), int(seed))
+    return p.copy(), e.copy()
+
+
+@functools.lru_cache(maxsize=8)
+def _protein_atoms(n_atoms: int, seed: int = 0) -> Tuple[np.ndarray, np.ndarray]:
+    """Positions (n, 3) float32 and element indices (n,) of a synthetic chain.
+
+    The C-alpha walk is kept globular like a folded protein: once it leaves a
+    sphere holding ~25 A^3 per atom, each step takes the most inward of four
+    candidate directions that satisfy the bond-angle constraint. Side atoms
+    avoid steric clashes: each takes the first of up to 12 random placements
+    that keeps 2.2 A from every atom it is not bonded to (a uniform grid hash
+    finds the neighbours), so nearly all sub-1.9 A pairs are real bonds.
+    """
+    rng = np.random.default_rng(seed)
+    pos = []
+    elem = []
+    ca = np.zeros(3)
+    d = np.array([1.0, 0.0, 0.0])
+    r_glob = (3.0 * 25.0 * n_atoms / (4.0 * math.pi)) ** (1.0 / 3.0)
+    cell = 2.2
+    grid: dict = {}
+
+    def key(p):
+        return (int(math.floor(p[0] / cell)), int(math.floor(p[1] / cell)), int(math.floor(p[2] / cell)))
+
+    def clear(p, skip):
+        kx, ky, kz = key(p)
+        for dx in (-1, 0, 1):
+            for dy in (-1, 0, 1):
+                for dz in (-1, 0, 1):
+                    for j in grid.get((kx + dx, ky + dy, kz + dz), ()):
+                        if j != skip and float(np.sum((pos[j] - p) ** 2)) < cell * cell:
+                            return False
+        return True
+
+    def place(p, e):
+        grid.setdefault(key(p), []).append(len(pos))
+        pos.append(p)
+        elem.append(e)
+
+    while len(pos) < n_atoms:
+        ca_idx = len(pos)
+        place(ca.copy(), 0)
+        k = int(rng.integers(3, 6))
+        prev, prev_idx, prevdir = ca, ca_idx, _perp(d, rng)
+        for _ in range(k):
+            # chained side atoms, ~tetrahedral turn at each step
+            for _try in range(12):
+                nd = prevdir * math.cos(math.radians(70.5)) + _perp(prevdir, rng) * math.sin(math.radians(70.5))
+                nd /= np.linalg.norm(nd)
+                p = prev + 1.5 * nd
+                if clear(p, prev_idx):
+                    break
+            place(p, int(rng.integers(0, 5)))
+            prev, prev_idx, prevdir = p, len(pos) - 1, nd
+            if len(pos) >= n_atoms:
+                break
+        # next C-alpha: bond angle theta in [90, 120] deg between successive steps
+        theta = math.radians(rng.uniform(90.0, 120.0))
+        turn = math.pi - theta
+        cands = [d * math.cos(turn) + _perp(d, rng) * math.sin(turn) for _ in range(8)]
+        free = [c for c in cands if clear(ca + 3.8 * c, -1)] or cands
+        if np.linalg.norm(ca) > r_glob:
+            nd = min(free[:4], key=lambda c: float(np.linalg.norm(ca + 3.8 * c)))
+        else:
+            nd = free[0]
+        d = nd / np.linalg.norm(nd)
+        ca = ca + 3.8 * d
+    p = np.asarray(pos[:n_atoms], np.float32)
+    p -= p.mean(axis=0, keepdims=True)
+    return p, np.asarray(elem[:n_atoms], np.int32)
+
+
+def bonds_within(pos: np.ndarray, cutoff: float) -> np.ndarray:
+    """All pairs (i < j) with |p_i - p_j| < cutoff, via a uniform grid."""
+    cell = np.floor(pos / cutoff).astype(np.int64)
+    cell -= cell.min(axis=0)
+    dims = cell.max(axis=0) + 1
+    key = (cell[:, 0] * dims[1] + cell[:, 1]) * dims[2] + cell[:, 2]
+    order = np.argsort(key, kind="stable")
+    skey = key[order]
+    pairs = []
+    for dx in (-1, 0, 1):
+        for dy in (-1, 0, 1):
+            for dz in (-1, 0, 1):
+                nk = ((cell[:, 0] + dx) * dims[1] + (cell[:, 1] + dy)) * dims[2] + (cell[:, 2] + dz)
+                lo = np.searchsorted(skey, nk, "left")
+                hi = np.searchsorted(skey, nk, "right")
+                cnt = hi - lo
+                if cnt.max(initial=0) == 0:
+                    continue
+                i = np.repeat(np.arange(len(pos)), cnt)
+                j = order[np.concatenate([np.arange(a, b) for a, b in zip(lo, hi) if b > a])] if cnt.sum() else np.zeros(0, np.int64)
+                m = i < j
+                i, j = i[m], j[m]
+                dd = np.linalg.norm(pos[i] - pos[j], axis=1)
+                sel = dd < cutoff
+                pairs.append(np.stack([i[sel], j[sel]], axis=1))
+    if not pairs:
+        return np.zeros((0, 2), np.int64)
+    pr = np.concatenate(pairs)
+    pr = np.unique(pr, axis=0)
+    return pr
+
+
+def frame_camera(pos: np.ndarray, pad: np.ndarray, fovy: float = 40.0, aspect: float = 4 / 3) -> Tuple[Camera, float]:
+    """Perspective camera on +z looking down -z that frames the bounding sphere."""
+    lo = (pos - pad[:, None]).min(axis=0)
+    hi = (pos + pad[:, None]).max(axis=0)
+    c = 0.5 * (lo + hi)
+    r = float(np.max(np.linalg.norm(pos - c, axis=1) + pad))
+    half = math.radians(fovy) * 0.5
+    half_w = math.atan(math.tan(half) * aspect)
+    dist = r / math.sin(min(half, half_w)) * 1.02
+    cam = Camera(position=(float(c[0]), float(c[1]), float(c[2] + dist)), direction=(0.0, 0.0, -1.0),
+                 up=(0.0, 1.0, 0.0), fovy=fovy)
+    return cam, dist
+
+
+def key_light() -> DistantLight:
+    return DistantLight(direction=(-0.35, -0.45, -0.82), color=(1.0, 1.0, 1.0), intensity=0.9)
+
+
+def spheres_scene(n_atoms: int = 2000, width: int = 800, height: int = 600, seed: int = 0
+                  ) -> Tuple[Scene, RenderOptions]:
+    """Plain method, small: 2,000 CPK spheres, hard shadows, no AO."""
+    rng = np.random.default_rng(seed + 1)
+    pos, elem = protein_atoms(n_atoms, seed)
+    radii = RADII[rng.integers(0, len(RADII), n_atoms)]
+    sc = Scene()
+    sc.add_spheres(pos, radii, PALETTE[elem], LIT)
+    sc.camera, _ = frame_camera(pos, radii, 40.0, width / height)
+    sc.lights = [key_light()]
+    sc.ambient_color = (1.0, 1.0, 1.0)
+    sc.background = (0.0, 0.0, 0.0)
+    opt = RenderOptions(width=width, height=height, supersample=1, shadows=True, ao_samples=0)
+    return sc, opt
+
+
+def ball_and_stick_scene(n_atoms: int = 20000, width: int = 1920, height: int = 1080,
+                         supersample: int = 2, ao_samples: int = 16, seed: int = 0
+                         ) -> Tuple[Scene, RenderOptions]:
+    """Ball-and-stick: atoms r 0.4, half-bond capped cylinders r 0.2, AO, shadows, fog."""
+    pos, elem = protein_atoms(n_atoms, seed)
+    bonds = bonds_within(pos, 1.9)
+    sc = Scene()
+    sc.add_spheres(pos, np.full(n_atoms, 0.4, np.float32), PALETTE[elem], LIT)
+    if len(bonds):
+        i, j = bonds[:, 0], bonds[:, 1]
+        mid = 0.5 * (pos[i] + pos[j])
+        p0 = np.concatenate([pos[i], mid])
+        p1 = np.concatenate([mid, pos[j]])
+        col = np.concatenate([PALETTE[elem[i]], PALETTE[elem[j]]])
+        # interleave so each bond's two halves are adjacent
+        order = np.stack([np.arange(len(bonds)), np.arange(len(bonds)) + len(bonds)], axis=1).reshape(-1)
+        sc.add_cylinders(p0[order], p1[order], np.full(2 * len(bonds), 0.2, np.float32), col[order], LIT,
+                         open=False)
+    sc.camera, dist = frame_camera(pos, np.full(n_atoms, 0.4, np.float32), 40.0, width / height)
+    sc.lights = [key_light()]
+    sc.ambient_color = (1.0, 1.0, 1.0)
+    sc.background = (0.0, 0.0, 0.0)
+    sc.fog = Fog(enabled=True, color=(0.0, 0.0, 0.0), start=float(dist), end=float(dist + 60.0))
+    opt = RenderOptions(width=width, height=height, supersample=supersample, shadows=True,
+                        ao_samples=ao_samples, ao_distance=8.0)
+    return sc, opt
+
+
+def scaled(scene_fn, scale: float, **kw):
+    """Same scene at a reduced image size (for oracle-speed parity runs)."""
+    sc, opt = scene_fn(**kw)
+    opt.width = max(8, int(opt.width * scale))
+    opt.height = max(8, int(opt.height * scale))
+    return sc, opt
