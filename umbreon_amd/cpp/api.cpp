@@ -1,0 +1,496 @@
+// umbreon::render / renderAsync on the MI355X backend.
+//
+// Host orchestration of the reference's public flow (umbreon.cpp:141-341 and
+// pipeline.cpp:20-256): option checks, the group-alpha multipass blend in the
+// sRGB domain, and per pass the scene flattening into the C-ABI's
+// structure-of-arrays description (umbreon_gpu.h), one ubg_scene_create
+// (upload + BVH) and one ubg_render (hi-res trace + shade + fog, downsample,
+// gamma) on a process-wide device context. Failures surface as
+// std::runtime_error with the C-ABI's message.
+
+#include <algorithm>
+#include <array>
+#include <atomic>
+#include <condition_variable>
+#include <cstring>
+#include <exception>
+#include <map>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "postprocess/image_ops.hpp"
+#include "umbreon.hpp"
+#include "umbreon_gpu.h"
+
+namespace umbreon {
+
+namespace detail {
+// Grants the backend the address of RenderProgress's cancel flag.
+class ProgressAccess {
+ public:
+  static const volatile int32_t* cancelFlag(RenderProgress& p) {
+    return reinterpret_cast<const volatile int32_t*>(&p.cancel_);
+  }
+};
+}  // namespace detail
+
+namespace {
+
+static_assert(sizeof(std::atomic<int32_t>) == sizeof(int32_t), "lock-free int32 atomic expected");
+
+[[noreturn]] void throwStatus(int st, const char* what) {
+  throw std::runtime_error(std::string("umbreon (MI355X): ") + what + ": " + ubg_last_error() + " [" +
+                           ubg_status_string(st) + "]");
+}
+
+// One gfx950 context for the process (the SharedDevice of the reference);
+// the C-ABI serializes concurrent calls on it.
+ubg_context* device() {
+  static std::once_flag once;
+  static ubg_context* ctx = nullptr;
+  static int status = UBG_OK;
+  static std::string msg;
+  std::call_once(once, [] {
+    status = ubg_context_create(0, &ctx);
+    if (status != UBG_OK) msg = ubg_last_error();
+  });
+  if (status != UBG_OK)
+    throw std::runtime_error("umbreon (MI355X): device init failed: " + msg + " [" + ubg_status_string(status) +
+                             "]");
+  return ctx;
+}
+
+ubg_material toC(const Material& m) {
+  ubg_material c{};
+  c.ambient = m.ambient;
+  c.diffuse = m.diffuse;
+  c.specular = m.specular;
+  c.roughness = m.roughness;
+  c.brilliance = m.brilliance;
+  c.phong = m.phong;
+  c.phong_size = m.phongSize;
+  c.metallic = m.metallic ? 1 : 0;
+  c.reflection = m.reflection;
+  c.emission = m.emission;
+  c.model = static_cast<int32_t>(m.model);
+  c.pbr_metallic = m.pbr.metallic;
+  c.pbr_roughness = m.pbr.roughness;
+  c.pbr_specular = m.pbr.specular;
+  c.pbr_anisotropy = m.pbr.anisotropy;
+  c.pbr_anisotropy_rotation = m.pbr.anisotropyRotation;
+  return c;
+}
+
+// Scene -> structure-of-arrays C description (owns the arrays).
+struct FlatScene {
+  std::vector<float> pos, nrm, col;
+  std::vector<uint32_t> idx;
+  std::vector<ubg_material> meshMats;
+  std::vector<uint8_t> triMat;
+  std::vector<uint16_t> triGroup;
+  std::vector<float> sph, sphCol;
+  std::vector<int32_t> sphMat;
+  std::vector<uint16_t> sphGroup;
+  std::vector<uint8_t> sphEdge;
+  std::vector<float> c0, c1, crad, ccol, cop1;
+  std::vector<int32_t> cmat;
+  std::vector<uint16_t> cgroup;
+  std::vector<uint8_t> copen, cedge;
+  std::vector<ubg_material> primMats;
+  std::vector<float> inst;
+  std::map<std::array<uint32_t, 16>, int32_t> matIndex;
+  ubg_scene_desc desc{};
+
+  int32_t primMaterial(const Material& m) {
+    const ubg_material c = toC(m);
+    std::array<uint32_t, 16> key;
+    static_assert(sizeof(ubg_material) == sizeof(key), "material key size");
+    std::memcpy(key.data(), &c, sizeof(c));
+    auto it = matIndex.find(key);
+    if (it != matIndex.end()) return it->second;
+    const int32_t i = static_cast<int32_t>(primMats.size());
+    primMats.push_back(c);
+    matIndex.emplace(key, i);
+    return i;
+  }
+
+  explicit FlatScene(const Scene& s) {
+    const Mesh& m = s.mesh;
+    const std::size_t nv = m.positions.size();
+    pos.reserve(3 * nv);
+    for (std::size_t v = 0; v < nv; ++v) {
+      const Vec3& p = m.positions[v];
+      pos.insert(pos.end(), {p.x, p.y, p.z});
+      const Vec3 n = v < m.normals.size() ? m.normals[v] : Vec3{};
+      nrm.insert(nrm.end(), {n.x, n.y, n.z});
+      const Vec4 c = v < m.colors.size() ? m.colors[v] : Vec4{};
+      col.insert(col.end(), {c.x, c.y, c.z, c.w});
+    }
+    idx = m.index;
+    if (m.triMaterialId.empty()) {
+      meshMats.push_back(toC(m.material));
+    } else {
+      for (const Material& mm : m.materials) meshMats.push_back(toC(mm));
+      triMat = m.triMaterialId;
+    }
+    triGroup = m.triGroupId;
+    for (const Sphere& sp : s.spheres) {
+      sph.insert(sph.end(), {sp.center.x, sp.center.y, sp.center.z, sp.radius});
+      sphCol.insert(sphCol.end(), {sp.color.x, sp.color.y, sp.color.z, sp.color.w});
+      sphMat.push_back(primMaterial(sp.material));
+      sphGroup.push_back(sp.group);
+      sphEdge.push_back(sp.fromEdgeMacro ? 1 : 0);
+    }
+    for (const Cylinder& cy : s.cylinders) {
+      c0.insert(c0.end(), {cy.p0.x, cy.p0.y, cy.p0.z});
+      c1.insert(c1.end(), {cy.p1.x, cy.p1.y, cy.p1.z});
+      crad.push_back(cy.radius);
+      ccol.insert(ccol.end(), {cy.color.x, cy.color.y, cy.color.z, cy.color.w});
+      cop1.push_back(cy.opacity1);
+      cmat.push_back(primMaterial(cy.material));
+      cgroup.push_back(cy.group);
+      copen.push_back(cy.open ? 1 : 0);
+      cedge.push_back(cy.fromEdgeMacro ? 1 : 0);
+    }
+    for (const Vec3& o : s.instanceOffsets) inst.insert(inst.end(), {o.x, o.y, o.z});
+
+    desc.n_vertices = static_cast<int64_t>(nv);
+    desc.positions = pos.data();
+    desc.normals = nrm.data();
+    desc.colors = col.data();
+    desc.n_triangles = static_cast<int64_t>(m.triangleCount());
+    desc.index = idx.empty() ? nullptr : idx.data();
+    desc.mesh_materials = meshMats.data();
+    desc.n_mesh_materials = static_cast<int32_t>(meshMats.size());
+    desc.tri_material = triMat.empty() ? nullptr : triMat.data();
+    desc.tri_group = triGroup.empty() ? nullptr : triGroup.data();
+    desc.n_spheres = static_cast<int64_t>(s.spheres.size());
+    desc.sphere_center_radius = sph.data();
+    desc.sphere_color = sphCol.data();
+    desc.sphere_material = sphMat.data();
+    desc.sphere_group = sphGroup.data();
+    desc.sphere_from_edge = sphEdge.data();
+    desc.n_cylinders = static_cast<int64_t>(s.cylinders.size());
+    desc.cyl_p0 = c0.data();
+    desc.cyl_p1 = c1.data();
+    desc.cyl_radius = crad.data();
+    desc.cyl_color = ccol.data();
+    desc.cyl_opacity1 = cop1.data();
+    desc.cyl_material = cmat.data();
+    desc.cyl_group = cgroup.data();
+    desc.cyl_open = copen.data();
+    desc.cyl_from_edge = cedge.data();
+    desc.prim_materials = primMats.data();
+    desc.n_prim_materials = static_cast<int32_t>(primMats.size());
+    desc.n_instances = static_cast<int64_t>(s.instanceOffsets.size());
+    desc.instance_offsets = inst.empty() ? nullptr : inst.data();
+  }
+};
+
+// Options the per-ray GPU path does not implement: reject rather than return
+// a different image.
+void checkSupported(const Scene& s, const RenderOptions& o) {
+  if (o.strokeEdges.enable && o.objectSpaceEdges.enable)
+    throw std::runtime_error(
+        "umbreon::render: strokeEdges and objectSpaceEdges are mutually exclusive; enable at most one");
+  auto no = [](const char* what) {
+    throw std::runtime_error(std::string("umbreon (MI355X): ") + what + " is not supported by the GPU backend");
+  };
+  if (o.gi) no("global illumination (gi)");
+  if (o.aaMode != 0) no("adaptive antialiasing (aaMode)");
+  if (o.aoEnhanced()) no("enhanced AO (falloff/multi-scale/bent/multibounce/low-discrepancy/diffuse factor)");
+  if (o.aoWriteAov) no("AO AOVs (aoWriteAov)");
+  if (o.aoResDiv > 1 || o.aoResDiv < 0) no("coarse-grid AO (aoResDiv)");
+  if (o.denoiser != 0) no("the final-colour denoiser");
+  if (o.envLights > 0) no("environment dome lights (envLights)");
+  if (o.strokeEdges.enable) no("stroke edges");
+  if (o.objectSpaceEdges.enable) no("object-space edges");
+  if (s.lights.size() > 16) no("more than 16 lights");
+  auto principled = [](const Material& m) { return m.model == ShadingModel::Principled; };
+  bool p = principled(s.mesh.material);
+  for (const Material& m : s.mesh.materials) p = p || principled(m);
+  for (const Sphere& sp : s.spheres) p = p || principled(sp.material);
+  for (const Cylinder& cy : s.cylinders) p = p || principled(cy.material);
+  if (p) no("ShadingModel::Principled");
+}
+
+// renderFrame (pipeline.cpp:20-256) for one pass on the GPU.
+FrameResult renderPass(const Scene& scene, const RenderOptions& opt, RenderProgress* progress) {
+  ubg_context* ctx = device();
+  const int ss = std::max(1, opt.supersample);
+  if (progress) progress->beginPhase(RenderPhase::Setup);
+  logMessage(LogLevel::Info, "render %dx%d ss%d (grid %dx%d), %zu tris / %zu spheres / %zu cylinders%s", opt.width,
+             opt.height, ss, opt.width * ss, opt.height * ss, scene.mesh.triangleCount(), scene.spheres.size(),
+             scene.cylinders.size(), opt.aoSamples > 0 ? ", AO" : "");
+  FrameResult f;
+  f.effectiveTriangles = scene.effectiveTriangles();
+  if (progress && progress->cancelRequested()) {
+    f.cancelled = true;
+    return f;
+  }
+  FlatScene flat(scene);
+  ubg_scene* gs = nullptr;
+  int st = ubg_scene_create(ctx, &flat.desc, &gs);
+  if (st != UBG_OK) throwStatus(st, "scene build");
+  ubg_scene_info info{};
+  ubg_scene_get_info(gs, &info);
+
+  ubg_render_params p{};
+  p.width = opt.width;
+  p.height = opt.height;
+  p.supersample = ss;
+  const Camera& c = scene.camera;
+  const float cp[3] = {c.position.x, c.position.y, c.position.z};
+  const float cd[3] = {c.direction.x, c.direction.y, c.direction.z};
+  const float cu[3] = {c.up.x, c.up.y, c.up.z};
+  std::memcpy(p.camera.position, cp, sizeof(cp));
+  std::memcpy(p.camera.direction, cd, sizeof(cd));
+  std::memcpy(p.camera.up, cu, sizeof(cu));
+  p.camera.fovy = c.fovy;
+  p.camera.orthographic = c.orthographic ? 1 : 0;
+  p.camera.height = c.height;
+  std::vector<ubg_light> lights;
+  for (const DistantLight& l : scene.lights) {
+    ubg_light L{};
+    L.direction[0] = l.direction.x, L.direction[1] = l.direction.y, L.direction[2] = l.direction.z;
+    L.color[0] = l.color.x, L.color[1] = l.color.y, L.color[2] = l.color.z;
+    L.intensity = l.intensity;
+    L.casts_highlight = l.castsHighlight ? 1 : 0;
+    L.angular_radius = l.angularRadius;
+    lights.push_back(L);
+  }
+  p.lights = lights.data();
+  p.n_lights = static_cast<int32_t>(lights.size());
+  p.ambient_color[0] = scene.ambientColor.x, p.ambient_color[1] = scene.ambientColor.y,
+  p.ambient_color[2] = scene.ambientColor.z;
+  p.background[0] = scene.background.x, p.background[1] = scene.background.y, p.background[2] = scene.background.z;
+  p.fog.enabled = scene.fog.enabled ? 1 : 0;
+  p.fog.color[0] = scene.fog.color.x, p.fog.color[1] = scene.fog.color.y, p.fog.color[2] = scene.fog.color.z;
+  p.fog.start = scene.fog.start;
+  p.fog.end = scene.fog.end;
+  p.clip_near = scene.clipNear;
+  p.clip_far = scene.clipFar;
+  p.assumed_gamma = scene.assumedGamma;
+  p.ao_samples = opt.aoSamples;
+  p.ao_distance = opt.aoDistance;
+  p.ao_intensity = opt.aoIntensity;
+  p.shadows = opt.shadows ? 1 : 0;
+  p.shadow_samples = opt.shadowSamples;
+  p.light_radius_deg = opt.lightRadius;
+  p.gi_integrator = opt.giIntegrator;
+  p.specular_scale = opt.specularScale;
+  p.transparency = opt.transparency ? 1 : 0;
+  p.transparent_background = opt.transparentBackground ? 1 : 0;
+  p.max_transparent_layers = opt.maxTransparentLayers;
+  p.cancel_flag = progress ? detail::ProgressAccess::cancelFlag(*progress) : nullptr;
+
+  const std::size_t hiN = static_cast<std::size_t>(opt.width) * ss * opt.height * ss;
+  f.width = opt.width;
+  f.height = opt.height;
+  f.color.assign(static_cast<std::size_t>(opt.width) * opt.height * 4, 0.0f);
+  f.depth.assign(hiN, 0.0f);
+  if (scene.fog.enabled) f.viewZ.assign(hiN, 0.0f);
+  ubg_frame fr{};
+  fr.color = f.color.data();
+  fr.depth = f.depth.data();
+  fr.view_z = f.viewZ.empty() ? nullptr : f.viewZ.data();
+  fr.memory = 0;
+  if (progress) progress->beginPhase(RenderPhase::Primary);
+  st = ubg_render(ctx, gs, &p, &fr);
+  ubg_scene_destroy(gs);
+  if (st == UBG_ERR_CANCELLED) {
+    f.cancelled = true;
+    return f;
+  }
+  if (st != UBG_OK) throwStatus(st, "render");
+  if (fr.stats.capped_rays > 0)
+    logMessage(LogLevel::Warning,
+               "%llu ray(s) reached the transparent-layer cap (%d); transmission past that depth was dropped -- "
+               "raise maxTransparentLayers if it is visible",
+               static_cast<unsigned long long>(fr.stats.capped_rays), opt.maxTransparentLayers);
+  if (progress) progress->beginPhase(RenderPhase::Postprocess);
+  f.renderSeconds = fr.render_seconds;
+  f.pt1Timing.bvhBuild = info.build_seconds;
+  f.pt1Timing.primary = fr.render_seconds;
+  return f;
+}
+
+// Scene minus the primitives of hidden groups (umbreon.cpp:53-94).
+void hideGroups(const Scene& s, const std::vector<uint8_t>& hide, Scene& out) {
+  auto hidden = [&](uint16_t g) { return g < hide.size() && hide[g] != 0; };
+  out.mesh = Mesh{};
+  out.mesh.positions = s.mesh.positions;
+  out.mesh.normals = s.mesh.normals;
+  out.mesh.colors = s.mesh.colors;
+  out.mesh.material = s.mesh.material;
+  out.mesh.materials = s.mesh.materials;
+  std::vector<uint32_t> idx;
+  for (std::size_t t = 0; t < s.mesh.triangleCount(); ++t) {
+    if (hidden(s.mesh.groupForTri(t))) continue;
+    for (int k = 0; k < 3; ++k) idx.push_back(s.mesh.cornerVertex(3 * t + k));
+    if (!s.mesh.triMaterialId.empty()) out.mesh.triMaterialId.push_back(s.mesh.triMaterialId[t]);
+    if (!s.mesh.triGroupId.empty()) out.mesh.triGroupId.push_back(s.mesh.triGroupId[t]);
+  }
+  if (idx.empty()) out.mesh = Mesh{};  // an empty index would mean "soup": drop the mesh instead
+  else out.mesh.index = std::move(idx);
+  out.spheres.clear();
+  for (const Sphere& sp : s.spheres)
+    if (!hidden(sp.group)) out.spheres.push_back(sp);
+  out.cylinders.clear();
+  for (const Cylinder& cy : s.cylinders)
+    if (!hidden(cy.group)) out.cylinders.push_back(cy);
+}
+
+FrameResult renderImpl(const Scene& scene, const RenderOptions& opt, RenderProgress* progress) {
+  checkSupported(scene, opt);
+  if (scene.groupBlend.empty()) {
+    FrameResult f = renderPass(scene, opt, progress);
+    if (progress && !f.cancelled) progress->markDone();
+    return f;
+  }
+  // Group-alpha multipass (umbreon.cpp:149-254): out = (1 - sum a) * bg pass
+  // + sum a_i * pass_i, blended on sRGB-encoded RGB (linear alpha).
+  float sumA = 0.0f;
+  uint16_t maxGroup = 0;
+  for (const GroupBlend& gb : scene.groupBlend) {
+    sumA += gb.alpha;
+    maxGroup = std::max(maxGroup, gb.group);
+  }
+  const float bgW = 1.0f - sumA;
+  std::vector<uint8_t> hideAll(static_cast<std::size_t>(maxGroup) + 1, 0);
+  for (const GroupBlend& gb : scene.groupBlend) hideAll[gb.group] = 1;
+  Scene work = scene;
+  work.groupBlend.clear();
+  FrameResult carrier;
+  std::vector<float> acc;
+  double seconds = 0.0, bvh = 0.0;
+  const std::uint64_t passCount = 1 + scene.groupBlend.size();
+  std::uint64_t passIndex = 0;
+  bool cancelled = false;
+  auto addPass = [&](const std::vector<uint8_t>& hide, float w) {
+    if (cancelled) return;
+    if (progress) progress->beginPass(passIndex, passCount);
+    ++passIndex;
+    logMessage(LogLevel::Info, "group-alpha pass %llu/%llu (weight %.3f)", static_cast<unsigned long long>(passIndex),
+               static_cast<unsigned long long>(passCount), w);
+    hideGroups(scene, hide, work);
+    FrameResult f = renderPass(work, opt, progress);
+    if (f.cancelled) cancelled = true;
+    seconds += f.renderSeconds;
+    bvh += f.pt1Timing.bvhBuild;
+    if (acc.empty()) acc.assign(f.color.size(), 0.0f);
+    const std::size_t n = std::min(acc.size(), f.color.size()) / 4;
+    for (std::size_t q = 0; q < n; ++q) {
+      for (int ch = 0; ch < 3; ++ch) acc[4 * q + ch] += w * srgbEncodeF(f.color[4 * q + ch]);
+      acc[4 * q + 3] += w * f.color[4 * q + 3];
+    }
+    carrier = std::move(f);
+  };
+  addPass(hideAll, bgW);
+  for (const GroupBlend& gb : scene.groupBlend) {
+    std::vector<uint8_t> hide = hideAll;
+    hide[gb.group] = 0;
+    addPass(hide, gb.alpha);
+  }
+  for (std::size_t q = 0; q < acc.size() / 4; ++q) {
+    for (int ch = 0; ch < 3; ++ch) acc[4 * q + ch] = srgbDecodeF(std::fmax(0.0f, acc[4 * q + ch]));
+    acc[4 * q + 3] = std::fmin(1.0f, std::fmax(0.0f, acc[4 * q + 3]));
+  }
+  carrier.color = std::move(acc);
+  carrier.renderSeconds = seconds;
+  carrier.pt1Timing.bvhBuild = bvh;
+  carrier.pt1Timing.primary = seconds;
+  carrier.cancelled = cancelled;
+  if (progress && !cancelled) progress->markDone();
+  return carrier;
+}
+
+}  // namespace
+
+FrameResult render(const Scene& scene, const RenderOptions& opt) { return renderImpl(scene, opt, nullptr); }
+
+FrameResult render(const Scene& scene, const RenderOptions& opt, RenderProgress& progress) {
+  return renderImpl(scene, opt, &progress);
+}
+
+// ---------------------------------------------------------------- async
+struct RenderTask::Impl {
+  Scene scene;
+  RenderOptions opt;
+  RenderProgress progress;
+  FrameResult result;
+  std::exception_ptr error;
+  std::atomic<bool> finished{false};
+  mutable std::mutex mu;
+  mutable std::condition_variable cv;
+  std::thread worker;
+  bool taken = false;
+};
+
+RenderTask::RenderTask(std::unique_ptr<Impl> impl) noexcept : p_(std::move(impl)) {}
+RenderTask::RenderTask(RenderTask&&) noexcept = default;
+RenderTask& RenderTask::operator=(RenderTask&& o) noexcept {
+  if (this != &o) {
+    if (p_) {
+      p_->progress.requestCancel();
+      if (p_->worker.joinable()) p_->worker.join();
+    }
+    p_ = std::move(o.p_);
+  }
+  return *this;
+}
+RenderTask::~RenderTask() {
+  if (!p_) return;
+  p_->progress.requestCancel();
+  if (p_->worker.joinable()) p_->worker.join();
+}
+
+float RenderTask::progress() const noexcept { return p_ ? p_->progress.fraction() : 0.0f; }
+RenderPhase RenderTask::phase() const noexcept { return p_ ? p_->progress.phase() : RenderPhase::Idle; }
+bool RenderTask::done() const noexcept { return p_ && p_->finished.load(); }
+void RenderTask::cancel() noexcept {
+  if (p_) p_->progress.requestCancel();
+}
+bool RenderTask::wait_for(std::chrono::milliseconds timeout) const {
+  if (!p_) return true;
+  std::unique_lock<std::mutex> lk(p_->mu);
+  return p_->cv.wait_for(lk, timeout, [&] { return p_->finished.load(); });
+}
+void RenderTask::wait() const {
+  if (!p_) return;
+  std::unique_lock<std::mutex> lk(p_->mu);
+  p_->cv.wait(lk, [&] { return p_->finished.load(); });
+}
+FrameResult RenderTask::get() {
+  if (!p_) throw std::runtime_error("umbreon::RenderTask::get on an empty task");
+  if (p_->worker.joinable()) p_->worker.join();
+  if (p_->taken) throw std::runtime_error("umbreon::RenderTask::get called twice");
+  p_->taken = true;
+  if (p_->error) std::rethrow_exception(p_->error);
+  return std::move(p_->result);
+}
+
+RenderTask renderAsync(Scene scene, RenderOptions opt) {
+  auto impl = std::make_unique<RenderTask::Impl>();
+  impl->scene = std::move(scene);
+  impl->opt = std::move(opt);
+  RenderTask::Impl* raw = impl.get();
+  raw->worker = std::thread([raw] {
+    try {
+      raw->result = render(raw->scene, raw->opt, raw->progress);
+    } catch (...) {
+      raw->error = std::current_exception();
+    }
+    {
+      std::lock_guard<std::mutex> lk(raw->mu);
+      raw->finished.store(true);
+    }
+    raw->cv.notify_all();
+  });
+  return RenderTask(std::move(impl));
+}
+
+}  // namespace umbreon
