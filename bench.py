@@ -71,12 +71,15 @@ def main() -> int:
     ap.add_argument("--workload", choices=["ballstick", "spheres"], default="ballstick")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--layout", choices=["bvh8", "bvh2"], default="bvh8",
+                    help="acceleration structure: compressed 8-wide (default) or full-precision binary")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one MI355X: this build runs on a single GPU; "
               "multi-GPU runs (--gpus > 1) are not supported", file=sys.stderr)
         return 2
 
+    from umbreon_amd import capi
     from umbreon_amd.render import BuiltScene, Device
     from umbreon_amd.scenes import ball_and_stick_scene, spheres_scene
 
@@ -92,9 +95,10 @@ def main() -> int:
     log(f"scene generated in {time.perf_counter() - t0:.1f}s: {workload}")
 
     dev = Device(0)
-    bs = BuiltScene(dev, scene)
+    bs = BuiltScene(dev, scene, capi.BVH8 if args.layout == "bvh8" else capi.BVH2)
     info = bs.info
-    log(f"BVH: {info['n_nodes']} nodes, depth {info['bvh_depth']}, build {info['build_seconds'] * 1e3:.1f} ms")
+    log(f"BVH ({args.layout}): {info['n_nodes']} nodes of {info['node_bytes']} B, depth {info['bvh_depth']}, "
+        f"build {info['build_seconds'] * 1e3:.1f} ms")
 
     # visit counts on the same BVH and rays (instrumented kernels, untimed)
     vc = bs.render(scene, opt, outputs=False, count_visits=True)
@@ -164,8 +168,10 @@ def main() -> int:
             "rays_per_frame": rays_per_frame,
             "ray_split": {"primary": st["primary_rays"], "ao_traced": st["ao_rays_traced"],
                           "ao_sampled": st["ao_rays_sampled"], "shadow": st["shadow_rays"]},
+            "bvh_layout": args.layout,
             "bvh_build_ms": round(info["build_seconds"] * 1e3, 2),
             "bvh_nodes": info["n_nodes"],
+            "bvh_depth": info["bvh_depth"],
             "parallelism": "single GPU",
         },
         "roofline": roofline,

@@ -221,12 +221,20 @@ typedef struct ubg_frame {
   int32_t kernel_launches[4];
 } ubg_frame;
 
+/* Acceleration-structure layouts (ubg_scene_create_ex). */
+enum {
+  UBG_BVH2 = 0,  /* binary BVH, full fp32 child boxes, 64 B nodes */
+  UBG_BVH8 = 1   /* 8-wide BVH, 8-bit quantized child boxes, 80 B nodes (default) */
+};
+
 typedef struct ubg_scene_info {
   int64_t n_triangles, n_spheres, n_round_segments, n_cone_segments;
   int64_t n_nodes;          /* BVH nodes */
   int64_t n_leaf_prims;     /* primitive records in leaf order */
-  int32_t bvh_depth;
+  int32_t bvh_depth;        /* tree levels */
   int32_t node_bytes;       /* bytes per BVH node */
+  int32_t bvh_layout;       /* UBG_BVH2 / UBG_BVH8 */
+  int32_t stack_depth;      /* traversal stack entries per lane */
   int32_t prim_bytes;       /* bytes per leaf primitive record */
   double build_seconds;     /* upload + BVH build wall time */
   double sah_cost;
@@ -264,8 +272,11 @@ void ubg_context_destroy(ubg_context* ctx);
 /* Last error message of the calling thread (never NULL). */
 const char* ubg_last_error(void);
 
-/* Upload geometry and build the BVH. */
+/* Upload geometry and build the BVH (UBG_BVH8). */
 int ubg_scene_create(ubg_context* ctx, const ubg_scene_desc* desc, ubg_scene** out);
+/* Same with an explicit acceleration-structure layout (UBG_BVH2 / UBG_BVH8);
+ * both give identical images, the layout only changes traversal speed. */
+int ubg_scene_create_ex(ubg_context* ctx, const ubg_scene_desc* desc, int32_t bvh_layout, ubg_scene** out);
 void ubg_scene_destroy(ubg_scene* scene);
 int ubg_scene_get_info(const ubg_scene* scene, ubg_scene_info* out);
 

@@ -133,3 +133,22 @@ def test_config2_determinism_and_accounting(device):
     assert s["ao_rays_traced"] + s["ao_rays_sampled"] == 16 * hits
     assert 0 < s["shadow_rays"] <= hits
     assert np.isfinite(a.color).all()
+
+
+def test_bvh_layouts_render_identically(device):
+    """Full-precision BVH2 and quantized BVH8 give the same first hits and
+    colours (the layout only changes which nodes are visited)."""
+    from umbreon_amd.render import BuiltScene
+    sc, opt = ball_and_stick_scene(width=480, height=270)
+    out = {}
+    for layout in (capi.BVH2, capi.BVH8):
+        bs = BuiltScene(device, sc, layout)
+        try:
+            out[layout] = bs.render(sc, opt, hires_only=True, want_ids=True, want_ao=True)
+        finally:
+            bs.close()
+    a, b = out[capi.BVH2], out[capi.BVH8]
+    assert (a.hit_id == b.hit_id).mean() >= 0.99999
+    d = np.abs(a.color - b.color).max(axis=2)
+    assert (d > 1e-6).mean() <= 1e-4
+    assert a.stats["shaded_hits"] == b.stats["shaded_hits"]

@@ -22,6 +22,9 @@ UBG_ERR_CANCELLED = 4
 UBG_ERR_UNSUPPORTED = 5
 UBG_ERR_INTERNAL = 6
 
+BVH2 = 0
+BVH8 = 1
+
 KIND_MESH = 0
 KIND_SPHERE = 1
 KIND_ROUND_CURVE = 2
@@ -127,7 +130,8 @@ class SceneInfo(C.Structure):
     _fields_ = [
         ("n_triangles", C.c_int64), ("n_spheres", C.c_int64), ("n_round_segments", C.c_int64),
         ("n_cone_segments", C.c_int64), ("n_nodes", C.c_int64), ("n_leaf_prims", C.c_int64),
-        ("bvh_depth", C.c_int32), ("node_bytes", C.c_int32), ("prim_bytes", C.c_int32),
+        ("bvh_depth", C.c_int32), ("node_bytes", C.c_int32), ("bvh_layout", C.c_int32),
+        ("stack_depth", C.c_int32), ("prim_bytes", C.c_int32),
         ("build_seconds", C.c_double), ("sah_cost", C.c_double), ("device_bytes", C.c_int64),
     ]
 
@@ -161,7 +165,7 @@ def hit_dtype():
 # header; tests check both directions.
 EXPORTS = [
     "ubg_abi_version", "ubg_status_string", "ubg_context_create", "ubg_context_destroy",
-    "ubg_last_error", "ubg_scene_create", "ubg_scene_destroy", "ubg_scene_get_info",
+    "ubg_last_error", "ubg_scene_create", "ubg_scene_create_ex", "ubg_scene_destroy", "ubg_scene_get_info",
     "ubg_render", "ubg_intersect", "ubg_occluded",
 ]
 
@@ -196,6 +200,11 @@ def load() -> C.CDLL:
     lib.ubg_context_destroy.restype = None
     lib.ubg_scene_create.argtypes = [C.c_void_p, C.POINTER(SceneDesc), C.POINTER(C.c_void_p)]
     lib.ubg_scene_create.restype = C.c_int
+    lib.ubg_scene_create_ex.argtypes = [C.c_void_p, C.POINTER(SceneDesc), C.c_int32, C.POINTER(C.c_void_p)]
+    lib.ubg_scene_create_ex.restype = C.c_int
+    lib.ubgx_host_query.argtypes = [C.POINTER(SceneDesc), C.c_int32, C.c_int64, C.c_void_p, C.c_void_p,
+                                    C.c_void_p, C.c_void_p]
+    lib.ubgx_host_query.restype = C.c_int
     lib.ubg_scene_destroy.argtypes = [C.c_void_p]
     lib.ubg_scene_destroy.restype = None
     lib.ubg_scene_get_info.argtypes = [C.c_void_p, C.POINTER(SceneInfo)]

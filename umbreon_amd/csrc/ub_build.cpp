@@ -199,10 +199,10 @@ inline void growBox(Box& a, const Box& b) {
   }
 }
 
-// Internal levels a balanced split needs to reach leaves of kMaxLeaf.
-inline int levelsNeeded(int64_t c) {
+// Internal levels a balanced split needs to reach leaves of `maxLeaf`.
+inline int levelsNeeded(int64_t c, int maxLeaf) {
   int l = 0;
-  while (c > kMaxLeaf) {
+  while (c > maxLeaf) {
     c = (c + 1) / 2;
     ++l;
   }
@@ -211,7 +211,7 @@ inline int levelsNeeded(int64_t c) {
 
 class Builder {
  public:
-  Builder(const std::vector<Box>& boxes) : boxes_(boxes) {
+  Builder(const std::vector<Box>& boxes, int maxLeaf) : boxes_(boxes), maxLeaf_(maxLeaf) {
     const size_t n = boxes.size();
     idx_.resize(n);
     cent_.resize(3 * n);
@@ -233,6 +233,7 @@ class Builder {
 
  private:
   const std::vector<Box>& boxes_;
+  const int maxLeaf_;
   std::vector<uint32_t> idx_;
   std::vector<float> cent_;
   std::vector<BNode> nodes_;
@@ -311,7 +312,7 @@ class Builder {
     const float A = areaOf(b);
     const float leafCost = A * count;
     const float splitCost = A * 1.0f + bestCost;
-    if (depth > 0 && count <= kMaxLeaf && (count <= 2 || leafCost <= splitCost)) {
+    if (depth > 0 && count <= maxLeaf_ && (count <= 2 || leafCost <= splitCost)) {
       makeLeaf(ni, first, count);
       return;
     }
@@ -327,7 +328,7 @@ class Builder {
       nl = static_cast<int32_t>(mid - (idx_.begin() + first));
       // Depth guard: a lopsided SAH split may not leave room for the rest.
       const int32_t worst = std::max(nl, count - nl);
-      if (nl <= 0 || nl >= count || depth + 1 + levelsNeeded(worst) > kMaxDepth - 1) nl = -1;
+      if (nl <= 0 || nl >= count || depth + 1 + levelsNeeded(worst, maxLeaf_) > kMaxDepth - 1) nl = -1;
     }
     if (nl < 0) {
       // Balanced object-median split on the widest centroid axis.
@@ -391,10 +392,223 @@ struct DBox {
   }
 };
 
+// BVH2 flattening: inner nodes in depth-first preorder, primitives in leaf order.
+int flattenBvh2(const std::vector<BNode>& bn, int32_t root, const std::vector<uint32_t>& order,
+                const std::vector<F4>& recs, HostScene& out, std::string& err) {
+  const size_t nPrims = order.size();
+  out.prims.resize(3 * nPrims);
+  for (size_t i = 0; i < nPrims; ++i)
+    for (int k = 0; k < 3; ++k) out.prims[3 * i + k] = recs[3 * order[i] + k];
+  std::vector<int32_t> gpuIndex(bn.size(), -1);
+  std::vector<int32_t> stackv{root};
+  std::vector<int32_t> innerOrder;
+  while (!stackv.empty()) {
+    const int32_t i = stackv.back();
+    stackv.pop_back();
+    if (bn[i].count > 0) continue;
+    gpuIndex[i] = static_cast<int32_t>(innerOrder.size());
+    innerOrder.push_back(i);
+    stackv.push_back(bn[i].right);
+    stackv.push_back(bn[i].left);
+  }
+  auto childRef = [&](int32_t c) -> int32_t {
+    if (bn[c].count > 0) {
+      const uint32_t code = (static_cast<uint32_t>(bn[c].first) << 3) | static_cast<uint32_t>(bn[c].count - 1);
+      return static_cast<int32_t>(~code);
+    }
+    return gpuIndex[c];
+  };
+  int32_t maxDepth = 0;
+  if (bn[root].count > 0) {
+    // single leaf: inner root with the leaf in both slots
+    out.nodes.resize(4);
+    const Box& b = bn[root].box;
+    out.nodes[0] = F4{b.lo[0], b.hi[0], b.lo[1], b.hi[1]};
+    out.nodes[1] = F4{b.lo[0], b.hi[0], b.lo[1], b.hi[1]};
+    out.nodes[2] = F4{b.lo[2], b.hi[2], b.lo[2], b.hi[2]};
+    const int32_t ref = childRef(root);
+    const int32_t refs[4] = {ref, ref, 0, 0};
+    std::memcpy(&out.nodes[3], refs, 16);
+    maxDepth = 1;
+  } else {
+    out.nodes.resize(4 * innerOrder.size());
+    for (size_t g = 0; g < innerOrder.size(); ++g) {
+      const BNode& n = bn[innerOrder[g]];
+      const Box& b0 = bn[n.left].box;
+      const Box& b1 = bn[n.right].box;
+      out.nodes[4 * g + 0] = F4{b0.lo[0], b0.hi[0], b0.lo[1], b0.hi[1]};
+      out.nodes[4 * g + 1] = F4{b1.lo[0], b1.hi[0], b1.lo[1], b1.hi[1]};
+      out.nodes[4 * g + 2] = F4{b0.lo[2], b0.hi[2], b1.lo[2], b1.hi[2]};
+      const int32_t refs[4] = {childRef(n.left), childRef(n.right), 0, 0};
+      std::memcpy(&out.nodes[4 * g + 3], refs, 16);
+      maxDepth = std::max(maxDepth, n.depth + 1);
+    }
+  }
+  if (maxDepth > kMaxDepth) {
+    err = "internal error: BVH deeper than the traversal stack";
+    return UBG_ERR_INTERNAL;
+  }
+  out.depth = maxDepth;
+  out.stackDepth = maxDepth;
+  out.nNodes = static_cast<int64_t>(out.nodes.size() / 4);
+  out.nodeBytes = 64;
+  return UBG_OK;
+}
+
+// Compressed 8-wide BVH from the binary one (layout in ub_device.h). Each
+// wide node absorbs binary inner nodes, largest surface area first, until it
+// holds 8 children; children go to the slot of their octant around the node
+// centre (greedy best-match); inner children are numbered contiguously in
+// breadth-first order and each node's leaf primitives are stored contiguously.
+// Child boxes are quantized to 8 bits with power-of-two scales, rounded
+// outward, so the quantized box always contains the exact one.
+struct Node8 {
+  float p[3];
+  uint8_t e[3];
+  uint8_t imask;
+  uint32_t childBase, primBase;
+  uint8_t meta[8];
+  uint8_t qlo[3][8];
+  uint8_t qhi[3][8];
+};
+static_assert(sizeof(Node8) == 80, "BVH8 node is 80 bytes");
+
+int collapseBvh8(const std::vector<BNode>& bn, int32_t root, const std::vector<uint32_t>& order,
+                 const std::vector<F4>& recs, HostScene& out, std::string& err) {
+  struct Item {
+    int32_t node;
+    uint32_t idx;
+    int depth;
+  };
+  std::vector<Node8> nodes(1);
+  std::vector<F4> prims;
+  prims.reserve(3 * order.size());
+  std::vector<Item> queue{{root, 0u, 1}};
+  int maxDepth = 0;
+  for (size_t qi = 0; qi < queue.size(); ++qi) {
+    const Item it = queue[qi];
+    maxDepth = std::max(maxDepth, it.depth);
+    std::vector<int32_t> ch;
+    if (bn[it.node].count > 0) {
+      ch.push_back(it.node);
+    } else {
+      ch.push_back(bn[it.node].left);
+      ch.push_back(bn[it.node].right);
+    }
+    while (ch.size() < 8) {
+      int best = -1;
+      float bestA = -1.0f;
+      for (size_t j = 0; j < ch.size(); ++j)
+        if (bn[ch[j]].count == 0 && areaOf(bn[ch[j]].box) > bestA) {
+          bestA = areaOf(bn[ch[j]].box);
+          best = static_cast<int>(j);
+        }
+      if (best < 0) break;
+      const int32_t c = ch[best];
+      ch[best] = bn[c].left;
+      ch.push_back(bn[c].right);
+    }
+    Box nb = emptyBox();
+    for (int32_t c : ch) growBox(nb, bn[c].box);
+    // octant slots
+    int slotChild[8];
+    for (int& v : slotChild) v = -1;
+    {
+      double cc[3];
+      for (int k = 0; k < 3; ++k) cc[k] = 0.5 * (static_cast<double>(nb.lo[k]) + nb.hi[k]);
+      std::vector<std::array<double, 3>> rel(ch.size());
+      for (size_t j = 0; j < ch.size(); ++j)
+        for (int k = 0; k < 3; ++k)
+          rel[j][k] = 0.5 * (static_cast<double>(bn[ch[j]].box.lo[k]) + bn[ch[j]].box.hi[k]) - cc[k];
+      std::vector<char> used(ch.size(), 0);
+      for (size_t n = 0; n < ch.size(); ++n) {
+        double bestS = -1e300;
+        int bj = -1, bs = -1;
+        for (size_t j = 0; j < ch.size(); ++j) {
+          if (used[j]) continue;
+          for (int sl = 0; sl < 8; ++sl) {
+            if (slotChild[sl] >= 0) continue;
+            const double sc = rel[j][0] * ((sl & 1) ? 1.0 : -1.0) + rel[j][1] * ((sl & 2) ? 1.0 : -1.0) +
+                              rel[j][2] * ((sl & 4) ? 1.0 : -1.0);
+            if (sc > bestS) {
+              bestS = sc;
+              bj = static_cast<int>(j);
+              bs = sl;
+            }
+          }
+        }
+        used[bj] = 1;
+        slotChild[bs] = bj;
+      }
+    }
+    Node8 n{};
+    double scale[3];
+    for (int k = 0; k < 3; ++k) {
+      n.p[k] = nb.lo[k];
+      const double ext = static_cast<double>(nb.hi[k]) - static_cast<double>(nb.lo[k]);
+      int e = -126;
+      if (ext > 0.0) {
+        std::frexp(ext / 255.0, &e);  // 2^e >= ext / 255
+        e = std::max(-126, std::min(127, e));
+      }
+      n.e[k] = static_cast<uint8_t>(e + 127);
+      scale[k] = std::ldexp(1.0, e);
+    }
+    n.childBase = static_cast<uint32_t>(nodes.size());
+    n.primBase = static_cast<uint32_t>(prims.size() / 3);
+    uint32_t nInner = 0;
+    for (int sl = 0; sl < 8; ++sl) {
+      const int j = slotChild[sl];
+      if (j < 0) continue;
+      const BNode& c = bn[ch[j]];
+      for (int k = 0; k < 3; ++k) {
+        const double lo = std::floor((static_cast<double>(c.box.lo[k]) - n.p[k]) / scale[k]);
+        const double hi = std::ceil((static_cast<double>(c.box.hi[k]) - n.p[k]) / scale[k]);
+        n.qlo[k][sl] = static_cast<uint8_t>(std::max(0.0, std::min(255.0, lo)));
+        n.qhi[k][sl] = static_cast<uint8_t>(std::max(0.0, std::min(255.0, hi)));
+      }
+      if (c.count > 0) {
+        if (c.count > kMaxLeaf8) {
+          err = "internal error: BVH8 leaf larger than 3 primitives";
+          return UBG_ERR_INTERNAL;
+        }
+        const uint32_t off = static_cast<uint32_t>(prims.size() / 3) - n.primBase;
+        const uint32_t unary = (1u << c.count) - 1u;
+        n.meta[sl] = static_cast<uint8_t>((unary << 5) | off);
+        for (int32_t q = c.first; q < c.first + c.count; ++q)
+          for (int k = 0; k < 3; ++k) prims.push_back(recs[3 * order[q] + k]);
+      } else {
+        n.imask |= static_cast<uint8_t>(1u << sl);
+        n.meta[sl] = static_cast<uint8_t>((1u << 5) | (24u + static_cast<uint32_t>(sl)));
+        queue.push_back({ch[j], n.childBase + nInner, it.depth + 1});
+        ++nInner;
+      }
+    }
+    nodes.resize(nodes.size() + nInner);
+    nodes[it.idx] = n;
+  }
+  if (maxDepth > 64) {
+    err = "internal error: BVH8 deeper than the traversal stack";
+    return UBG_ERR_INTERNAL;
+  }
+  out.prims = std::move(prims);
+  out.nodes.resize(5 * nodes.size());
+  std::memcpy(out.nodes.data(), nodes.data(), nodes.size() * sizeof(Node8));
+  out.depth = maxDepth;
+  out.stackDepth = maxDepth;
+  out.nNodes = static_cast<int64_t>(nodes.size());
+  out.nodeBytes = 80;
+  return UBG_OK;
+}
+
 }  // namespace
 
-int buildHostScene(const ubg_scene_desc* d, HostScene& out, std::string& err) {
+int buildHostScene(const ubg_scene_desc* d, int layout, HostScene& out, std::string& err) {
   out = HostScene();
+  if (layout != kLayoutBvh2 && layout != kLayoutBvh8) {
+    err = "unknown BVH layout";
+    return UBG_ERR_INVALID_ARGUMENT;
+  }
   if (!d) {
     err = "null scene description";
     return UBG_ERR_INVALID_ARGUMENT;
@@ -640,71 +854,18 @@ int buildHostScene(const ubg_scene_desc* d, HostScene& out, std::string& err) {
     return UBG_ERR_INVALID_ARGUMENT;
   }
   // ---- BVH ----
-  Builder bld(boxes);
+  Builder bld(boxes, layout == kLayoutBvh8 ? kMaxLeaf8 : kMaxLeaf);
   const int32_t root = bld.run();
-  std::vector<BNode>& bn = bld.nodes();
+  const std::vector<BNode>& bn = bld.nodes();
   const std::vector<uint32_t>& order = bld.order();
-  out.prims.resize(3 * nPrims);
-  for (size_t i = 0; i < nPrims; ++i)
-    for (int k = 0; k < 3; ++k) out.prims[3 * i + k] = recs[3 * order[i] + k];
-  // Flatten inner nodes in depth-first preorder.
-  std::vector<int32_t> gpuIndex(bn.size(), -1);
-  std::vector<int32_t> stackv;
-  std::vector<int32_t> innerOrder;
-  stackv.push_back(root);
-  while (!stackv.empty()) {
-    const int32_t i = stackv.back();
-    stackv.pop_back();
-    if (bn[i].count > 0) continue;
-    gpuIndex[i] = static_cast<int32_t>(innerOrder.size());
-    innerOrder.push_back(i);
-    stackv.push_back(bn[i].right);
-    stackv.push_back(bn[i].left);
-  }
-  auto childRef = [&](int32_t c) -> int32_t {
-    if (bn[c].count > 0) {
-      const uint32_t code = (static_cast<uint32_t>(bn[c].first) << 3) | static_cast<uint32_t>(bn[c].count - 1);
-      return static_cast<int32_t>(~code);
-    }
-    return gpuIndex[c];
-  };
+  out.layout = layout;
+  int st = layout == kLayoutBvh8 ? collapseBvh8(bn, root, order, recs, out, err)
+                                 : flattenBvh2(bn, root, order, recs, out, err);
+  if (st != UBG_OK) return st;
+  // SAH cost of the binary tree (traversal 1, intersection 1 per primitive)
   const double rootArea = std::max(1e-30, static_cast<double>(areaOf(bn[root].box)));
   double sah = 0.0;
-  int32_t maxDepth = 0;
-  if (bn[root].count > 0) {
-    // single primitive: inner root with the leaf in both slots
-    out.nodes.resize(4);
-    const Box& b = bn[root].box;
-    out.nodes[0] = F4{b.lo[0], b.hi[0], b.lo[1], b.hi[1]};
-    out.nodes[1] = F4{b.lo[0], b.hi[0], b.lo[1], b.hi[1]};
-    out.nodes[2] = F4{b.lo[2], b.hi[2], b.lo[2], b.hi[2]};
-    const int32_t ref = childRef(root);
-    int32_t refs[4] = {ref, ref, 0, 0};
-    std::memcpy(&out.nodes[3], refs, 16);
-    maxDepth = 1;
-    sah = 1.0 + bn[root].count;
-  } else {
-    out.nodes.resize(4 * innerOrder.size());
-    for (size_t g = 0; g < innerOrder.size(); ++g) {
-      const BNode& n = bn[innerOrder[g]];
-      const Box& b0 = bn[n.left].box;
-      const Box& b1 = bn[n.right].box;
-      out.nodes[4 * g + 0] = F4{b0.lo[0], b0.hi[0], b0.lo[1], b0.hi[1]};
-      out.nodes[4 * g + 1] = F4{b1.lo[0], b1.hi[0], b1.lo[1], b1.hi[1]};
-      out.nodes[4 * g + 2] = F4{b0.lo[2], b0.hi[2], b1.lo[2], b1.hi[2]};
-      int32_t refs[4] = {childRef(n.left), childRef(n.right), 0, 0};
-      std::memcpy(&out.nodes[4 * g + 3], refs, 16);
-      maxDepth = std::max(maxDepth, n.depth + 1);
-      sah += areaOf(n.box) / rootArea;
-      for (int32_t c : {n.left, n.right})
-        if (bn[c].count > 0) sah += areaOf(bn[c].box) / rootArea * bn[c].count;
-    }
-  }
-  if (maxDepth > kMaxDepth) {
-    err = "internal error: BVH deeper than the traversal stack";
-    return UBG_ERR_INTERNAL;
-  }
-  out.depth = maxDepth;
+  for (const BNode& n : bn) sah += areaOf(n.box) / rootArea * (n.count > 0 ? n.count : 1);
   out.sahCost = sah;
   return UBG_OK;
 }

@@ -20,7 +20,7 @@ struct U4 {
 
 // Everything that goes to HBM for one scene, still on the host.
 struct HostScene {
-  std::vector<F4> nodes;   // 4 per inner node
+  std::vector<F4> nodes;   // BVH2: 4 per inner node; BVH8: 5 per node
   std::vector<F4> prims;   // 3 per primitive, leaf order
   std::vector<F4> sphColor;
   std::vector<uint32_t> sphInfo;
@@ -38,13 +38,18 @@ struct HostScene {
   std::vector<DevMaterial> primMats;
   std::vector<DevMaterial> meshMats;
   int64_t nTris = 0, nSpheres = 0, nRound = 0, nCone = 0;
-  int32_t depth = 0;
+  int32_t layout = kLayoutBvh8;
+  int32_t depth = 0;       // tree levels
+  int32_t stackDepth = 0;  // traversal stack entries needed
+  int64_t nNodes = 0;
+  int32_t nodeBytes = 0;
   double sahCost = 0.0;
 };
 
 // Flatten `desc` (curve chaining, instance baking, side tables) and build the
-// BVH. Returns UBG_OK or an error status with `err` set.
-int buildHostScene(const ubg_scene_desc* desc, HostScene& out, std::string& err);
+// BVH in `layout` (kLayoutBvh2 / kLayoutBvh8). Returns UBG_OK or an error
+// status with `err` set.
+int buildHostScene(const ubg_scene_desc* desc, int layout, HostScene& out, std::string& err);
 
 // shading.hpp:34-41
 float blinnExponent(float roughness);

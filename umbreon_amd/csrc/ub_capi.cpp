@@ -197,15 +197,23 @@ void ubg_context_destroy(ubg_context* ctx) {
   delete ctx;
 }
 
-int ubg_scene_create(ubg_context* ctx, const ubg_scene_desc* desc, ubg_scene** out) {
+static void freeSceneLocked(ubg_scene* s) {
+  (void)hipStreamSynchronize(s->ctx->stream);
+  for (void* p : s->allocs) (void)hipFree(p);
+  delete s;
+}
+
+int ubg_scene_create_ex(ubg_context* ctx, const ubg_scene_desc* desc, int32_t bvh_layout, ubg_scene** out) {
   if (!ctx || !desc || !out) return fail(UBG_ERR_INVALID_ARGUMENT, "ubg_scene_create: null argument");
   *out = nullptr;
+  if (bvh_layout != UBG_BVH2 && bvh_layout != UBG_BVH8)
+    return fail(UBG_ERR_INVALID_ARGUMENT, "ubg_scene_create: unknown BVH layout");
   std::lock_guard<std::mutex> lk(ctx->mu);
   UBG_HIP(hipSetDevice(ctx->device), "hipSetDevice");
   const auto t0 = std::chrono::steady_clock::now();
   ubg::HostScene hs;
   std::string err;
-  const int st = ubg::buildHostScene(desc, hs, err);
+  const int st = ubg::buildHostScene(desc, bvh_layout == UBG_BVH8 ? ubg::kLayoutBvh8 : ubg::kLayoutBvh2, hs, err);
   if (st != UBG_OK) return fail(st, "scene build failed: " + err);
   ubg_scene* s = new ubg_scene();
   s->ctx = ctx;
@@ -242,11 +250,13 @@ int ubg_scene_create(ubg_context* ctx, const ubg_scene_desc* desc, ubg_scene** o
     if (e != hipSuccess) r = hipFail(e, "scene upload");
   }
   if (r != UBG_OK) {
-    ubg_scene_destroy(s);
+    freeSceneLocked(s);
     return r;
   }
-  s->view.nNodes = static_cast<int32_t>(hs.nodes.size() / 4);
+  s->view.nNodes = static_cast<int32_t>(hs.nNodes);
   s->view.nPrims = static_cast<int32_t>(hs.prims.size() / 3);
+  s->view.layout = hs.layout;
+  s->view.stackDepth = std::max(1, hs.stackDepth);
   const auto t1 = std::chrono::steady_clock::now();
   s->info.n_triangles = hs.nTris;
   s->info.n_spheres = hs.nSpheres;
@@ -255,7 +265,9 @@ int ubg_scene_create(ubg_context* ctx, const ubg_scene_desc* desc, ubg_scene** o
   s->info.n_nodes = s->view.nNodes;
   s->info.n_leaf_prims = s->view.nPrims;
   s->info.bvh_depth = hs.depth;
-  s->info.node_bytes = 64;
+  s->info.node_bytes = hs.nodeBytes;
+  s->info.bvh_layout = bvh_layout;
+  s->info.stack_depth = s->view.stackDepth;
   s->info.prim_bytes = 48;
   s->info.build_seconds = std::chrono::duration<double>(t1 - t0).count();
   s->info.sah_cost = hs.sahCost;
@@ -263,13 +275,17 @@ int ubg_scene_create(ubg_context* ctx, const ubg_scene_desc* desc, ubg_scene** o
   return UBG_OK;
 }
 
+int ubg_scene_create(ubg_context* ctx, const ubg_scene_desc* desc, ubg_scene** out) {
+  return ubg_scene_create_ex(ctx, desc, UBG_BVH8, out);
+}
+
 void ubg_scene_destroy(ubg_scene* scene) {
   if (!scene) return;
   if (scene->ctx) {
     std::lock_guard<std::mutex> lk(scene->ctx->mu);
     (void)hipSetDevice(scene->ctx->device);
-    (void)hipStreamSynchronize(scene->ctx->stream);
-    for (void* p : scene->allocs) (void)hipFree(p);
+    freeSceneLocked(scene);
+    return;
   }
   delete scene;
 }

@@ -1,13 +1,15 @@
-// ub_device.h -- gfx950 device math, primitive intersectors and BVH
-// traversal. Included by ub_kernels.hip only.
+// ub_device.h -- math, primitive intersectors and BVH traversal of the
+// per-ray path. Every function is __host__ __device__: the gfx950 kernels run
+// them, and the library's host test hook (ubgx_host_query) runs the very same
+// code on the CPU so the traversal can be checked without a GPU.
 //
 // The whole library is compiled with -ffp-contract=off and correctly rounded
 // fp32 divide/sqrt, so every expression below rounds exactly as written. The
 // intersectors and the shading chain use the same operation order as the
 // reference's float code (scene.hpp vector helpers, shading/*.hpp), which is
 // what makes first hits bit-identical to the CPU restatement. Only the
-// traversal box test uses explicit FMAs: it merely decides which nodes are
-// visited and every box is inflated by the builder, so it can never cull a
+// traversal box tests use explicit FMAs: they merely decide which nodes are
+// visited and every box is inflated by the builder, so they can never cull a
 // primitive the exact test would hit.
 #pragma once
 
@@ -16,58 +18,69 @@
 
 #include "ub_layout.h"
 
+#define UBG_HD __host__ __device__ __forceinline__
+
 namespace ubg {
+
+UBG_HD uint32_t f2u(float f) {
+  uint32_t u;
+  __builtin_memcpy(&u, &f, 4);
+  return u;
+}
+UBG_HD float u2f(uint32_t u) {
+  float f;
+  __builtin_memcpy(&f, &u, 4);
+  return f;
+}
 
 struct V3 {
   float x, y, z;
 };
-__device__ __forceinline__ V3 mk(float x, float y, float z) { return V3{x, y, z}; }
-__device__ __forceinline__ V3 operator+(V3 a, V3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
-__device__ __forceinline__ V3 operator-(V3 a, V3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
-__device__ __forceinline__ V3 operator*(V3 a, float s) { return {a.x * s, a.y * s, a.z * s}; }
-__device__ __forceinline__ V3 neg(V3 a) { return {-a.x, -a.y, -a.z}; }
-__device__ __forceinline__ float dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
-__device__ __forceinline__ V3 cross(V3 a, V3 b) {
-  return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
-}
-__device__ __forceinline__ float len(V3 a) { return sqrtf(dot(a, a)); }
+UBG_HD V3 mk(float x, float y, float z) { return V3{x, y, z}; }
+UBG_HD V3 operator+(V3 a, V3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+UBG_HD V3 operator-(V3 a, V3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+UBG_HD V3 operator*(V3 a, float s) { return {a.x * s, a.y * s, a.z * s}; }
+UBG_HD V3 neg(V3 a) { return {-a.x, -a.y, -a.z}; }
+UBG_HD float dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+UBG_HD V3 cross(V3 a, V3 b) { return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
+UBG_HD float len(V3 a) { return sqrtf(dot(a, a)); }
 // scene.hpp:35-38
-__device__ __forceinline__ V3 normalize(V3 a) {
+UBG_HD V3 normalize(V3 a) {
   const float l = len(a);
   return l > 0.0f ? a * (1.0f / l) : a;
 }
 // scene.hpp:44-49
-__device__ __forceinline__ V3 safeNormalize(V3 a) {
+UBG_HD V3 safeNormalize(V3 a) {
   const float s = 1.0f / sqrtf(fmaxf(dot(a, a), 0x1.0p-126f));
   return a * s;
 }
 // scene.hpp:53-57
-__device__ __forceinline__ V3 safeNormalize(V3 a, V3 fb) {
+UBG_HD V3 safeNormalize(V3 a, V3 fb) {
   const float d = dot(a, a);
   if (d < 1.0e-18f) return fb;
   return a * (1.0f / sqrtf(d));
 }
 // shading.hpp:27-30
-__device__ __forceinline__ V3 faceForward(V3 n, V3 rd) { return dot(n, rd) > 0.0f ? neg(n) : n; }
+UBG_HD V3 faceForward(V3 n, V3 rd) { return dot(n, rd) > 0.0f ? neg(n) : n; }
 
 struct Frame3 {
   V3 t, b, n;
 };
 // scene.hpp:69-76
-__device__ __forceinline__ Frame3 frameFromNormal(V3 n) {
+UBG_HD Frame3 frameFromNormal(V3 n) {
   const V3 s0 = mk(0.0f, n.z, -n.y);
   const V3 s1 = mk(-n.z, 0.0f, n.x);
   const V3 t = safeNormalize(fabsf(n.x) < fabsf(n.y) ? s0 : s1, mk(1.0f, 0.0f, 0.0f));
   return Frame3{t, cross(n, t), n};
 }
 // secondary_rays.hpp:38-46
-__device__ __forceinline__ float selfIntersectEps(V3 P, V3 d, float t) {
+UBG_HD float selfIntersectEps(V3 P, V3 d, float t) {
   const float dirMax = fmaxf(fabsf(d.x), fmaxf(fabsf(d.y), fabsf(d.z)));
   const float s = fmaxf(fmaxf(fabsf(P.x), fabsf(P.y)), fmaxf(fabsf(P.z), dirMax * t));
   return s * 0x1.0fp-21f;
 }
 // secondary_rays.hpp:51-58 (TEA-8)
-__device__ __forceinline__ void tea2(uint32_t& v0, uint32_t& v1) {
+UBG_HD void tea2(uint32_t& v0, uint32_t& v1) {
   uint32_t sum = 0;
 #pragma unroll
   for (int i = 0; i < 8; ++i) {
@@ -76,9 +89,9 @@ __device__ __forceinline__ void tea2(uint32_t& v0, uint32_t& v1) {
     v1 += ((v0 << 4) + 0xAD90777Du) ^ (v0 + sum) ^ ((v0 >> 5) + 0x7E95761Eu);
   }
 }
-__device__ __forceinline__ float u32ToUnorm(uint32_t u) { return (float)(u >> 8) * 0x1.0p-24f; }
+UBG_HD float u32ToUnorm(uint32_t u) { return (float)(u >> 8) * 0x1.0p-24f; }
 // secondary_rays.hpp:66-73 (Malley cosine hemisphere)
-__device__ __forceinline__ V3 cosineSample(float u1, float u2, const Frame3& f) {
+UBG_HD V3 cosineSample(float u1, float u2, const Frame3& f) {
   const float phi = 6.2831853072f * u1;
   const float r = sqrtf(u2);
   const float z = sqrtf(fmaxf(0.0f, 1.0f - u2));
@@ -87,7 +100,7 @@ __device__ __forceinline__ V3 cosineSample(float u1, float u2, const Frame3& f) 
   return f.t * x + f.b * y + f.n * z;
 }
 // secondary_rays.hpp:102-109
-__device__ __forceinline__ V3 sampleLightDir(V3 L, float tanR, float u1, float u2) {
+UBG_HD V3 sampleLightDir(V3 L, float tanR, float u1, float u2) {
   const Frame3 f = frameFromNormal(L);
   const float phi = 6.2831853072f * u1;
   const float rr = tanR * sqrtf(u2);
@@ -95,7 +108,7 @@ __device__ __forceinline__ V3 sampleLightDir(V3 L, float tanR, float u1, float u
   return safeNormalize(d, L);
 }
 
-__device__ __forceinline__ V3 ld3(const float4& a) { return mk(a.x, a.y, a.z); }
+UBG_HD V3 ld3(const float4& a) { return mk(a.x, a.y, a.z); }
 
 struct Hit {
   float t, u, v;
@@ -108,8 +121,7 @@ struct Hit {
 // solve in a frame moved along the ray to the closest approach of the
 // primitive's center so far cameras do not cancel the discriminant.
 
-__device__ __forceinline__ bool isectSphere(V3 o, V3 d, float tnear, float tfar, V3 c, float r,
-                                            Hit& h) {
+UBG_HD bool isectSphere(V3 o, V3 d, float tnear, float tfar, V3 c, float r, Hit& h) {
   const float A = dot(d, d);
   const float ts = dot(c - o, d) / A;
   const V3 f = (o + d * ts) - c;
@@ -131,25 +143,26 @@ __device__ __forceinline__ bool isectSphere(V3 o, V3 d, float tnear, float tfar,
   return true;
 }
 
-__device__ __forceinline__ int coneBody(V3 oa, V3 d, V3 w, float r0, float k, float dt[2]) {
+// Roots (in the shifted frame) of the infinite cone r(h) = r0 + k*h around
+// the unit axis w through the origin of `oa`. Returns false when none.
+UBG_HD bool coneBody(V3 oa, V3 d, V3 w, float r0, float k, float& dt0, float& dt1) {
   const float ho = dot(oa, w);
   const float hd = dot(d, w);
   const float rr = r0 + k * ho;
   const float A = dot(d, d) - hd * hd - k * k * hd * hd;
   const float B = dot(oa, d) - ho * hd - k * hd * rr;
   const float C = dot(oa, oa) - ho * ho - rr * rr;
-  if (A == 0.0f) return 0;
+  if (A == 0.0f) return false;
   const float disc = B * B - A * C;
-  if (disc < 0.0f) return 0;
+  if (disc < 0.0f) return false;
   const float q = sqrtf(disc);
-  dt[0] = (-B - q) / A;
-  dt[1] = (-B + q) / A;
-  return 2;
+  dt0 = (-B - q) / A;
+  dt1 = (-B + q) / A;
+  return true;
 }
 
 // CONE_LINEAR_CURVE: body between the end planes plus flat disk caps.
-__device__ __forceinline__ bool isectCone(V3 o, V3 d, float tnear, float tfar, V3 a, float ra, V3 b,
-                                          float rb, Hit& h) {
+UBG_HD bool isectCone(V3 o, V3 d, float tnear, float tfar, V3 a, float ra, V3 b, float rb, Hit& h) {
   const V3 ax = b - a;
   const float L = len(ax);
   if (!(L > 0.0f)) return false;
@@ -164,16 +177,17 @@ __device__ __forceinline__ bool isectCone(V3 o, V3 d, float tnear, float tfar, V
   bool found = false;
   float best = tfar, bu = 0.0f;
   V3 bn = mk(0.0f, 0.0f, 0.0f);
-  float dt[2];
-  if (coneBody(oa, d, w, ra, k, dt) == 2) {
+  float r0, r1;
+  if (coneBody(oa, d, w, ra, k, r0, r1)) {
 #pragma unroll
     for (int i = 0; i < 2; ++i) {
-      const float t = ts + dt[i];
+      const float dt = i == 0 ? r0 : r1;
+      const float t = ts + dt;
       if (!(tnear <= t && t <= best)) continue;
-      const float hh = ho + dt[i] * hd;
+      const float hh = ho + dt * hd;
       if (hh < 0.0f || hh > L) continue;
       if (ra + k * hh < 0.0f) continue;
-      const V3 p = oa + d * dt[i];
+      const V3 p = oa + d * dt;
       const V3 radial = p - w * hh;
       const float rl = len(radial);
       V3 n = radial;
@@ -223,8 +237,8 @@ __device__ __forceinline__ bool isectCone(V3 o, V3 d, float tnear, float tfar, V
 // ROUND_LINEAR_CURVE segment restricted to its cell between the joint planes
 // (nL at a when hasL, nR at b when hasR): a chained polyline has no internal
 // caps, which is what RTC_CURVE_FLAG_NEIGHBOR_LEFT/RIGHT request.
-__device__ __noinline__ bool isectRound(V3 o, V3 d, float tnear, float tfar, V3 a, float ra, V3 b,
-                                        float rb, float4 pl, float4 pr, Hit& h) {
+UBG_HD bool isectRound(V3 o, V3 d, float tnear, float tfar, V3 a, float ra, V3 b, float rb, float4 pl,
+                       float4 pr, Hit& h) {
   const V3 ax = b - a;
   const float L = len(ax);
   if (!(L > 0.0f)) return false;
@@ -241,29 +255,18 @@ __device__ __noinline__ bool isectRound(V3 o, V3 d, float tnear, float tfar, V3 
   bool found = false;
   float best = tfar, bu = 0.0f;
   V3 bn = mk(0.0f, 0.0f, 0.0f);
-  auto consider = [&](float dtv, V3 n, float hh) {
-    const float t = ts + dtv;
-    if (!(tnear <= t && t <= best)) return;
-    const V3 p = oa + d * dtv;
-    if (hasL && dot(p, nL) < 0.0f) return;
-    if (hasR && dot(p - ax, nR) > 0.0f) return;
-    best = t;
-    float uu = hh / L;
-    if (uu < 0.0f) uu = 0.0f;
-    if (uu > 1.0f) uu = 1.0f;
-    bu = uu;
-    bn = n;
-    found = true;
-  };
-  float dt[2];
-  if (coneBody(oa, d, w, ra, k, dt) == 2) {
-    for (int i = 0; i < 2; ++i) {
-      const float hh = ho + dt[i] * hd;
-      if (hh < 0.0f || hh > L) continue;
-      const V3 p = oa + d * dt[i];
-      consider(dt[i], p - w * hh, hh);
-    }
+  // candidate roots: 2 on the body, 2 on each end sphere
+  float cdt[6];
+  int ckind[6];  // 0 body, 1 sphere at a, 2 sphere at b, -1 none
+  {
+    float r0 = 0.0f, r1 = 0.0f;
+    const bool ok = coneBody(oa, d, w, ra, k, r0, r1);
+    cdt[0] = r0;
+    cdt[1] = r1;
+    ckind[0] = ok ? 0 : -1;
+    ckind[1] = ok ? 0 : -1;
   }
+#pragma unroll
   for (int e = 0; e < 2; ++e) {
     const V3 c = e == 0 ? mk(0.0f, 0.0f, 0.0f) : ax;
     const float r = e == 0 ? ra : rb;
@@ -271,17 +274,34 @@ __device__ __noinline__ bool isectRound(V3 o, V3 d, float tnear, float tfar, V3 
     const float B = dot(f, d);
     const float C = dot(f, f) - r * r;
     const float disc = B * B - A * C;
-    if (disc < 0.0f) continue;
-    const float q = sqrtf(disc);
-    const float roots[2] = {(-B - q) / A, (-B + q) / A};
-    for (int i = 0; i < 2; ++i) {
-      const float dtv = roots[i];
-      const float hh = ho + dtv * hd;
-      if (e == 0 && hh >= 0.0f) continue;
-      if (e == 1 && hh <= L) continue;
-      const V3 p = oa + d * dtv;
-      consider(dtv, p - c, hh);
-    }
+    const bool ok = disc >= 0.0f;
+    const float q = ok ? sqrtf(disc) : 0.0f;
+    cdt[2 + 2 * e] = (-B - q) / A;
+    cdt[3 + 2 * e] = (-B + q) / A;
+    ckind[2 + 2 * e] = ok ? 1 + e : -1;
+    ckind[3 + 2 * e] = ok ? 1 + e : -1;
+  }
+#pragma unroll
+  for (int i = 0; i < 6; ++i) {
+    if (ckind[i] < 0) continue;
+    const float dtv = cdt[i];
+    const float hh = ho + dtv * hd;
+    if (ckind[i] == 0 && (hh < 0.0f || hh > L)) continue;
+    if (ckind[i] == 1 && hh >= 0.0f) continue;
+    if (ckind[i] == 2 && hh <= L) continue;
+    const float t = ts + dtv;
+    if (!(tnear <= t && t <= best)) continue;
+    const V3 p = oa + d * dtv;
+    if (hasL && dot(p, nL) < 0.0f) continue;
+    if (hasR && dot(p - ax, nR) > 0.0f) continue;
+    const V3 n = ckind[i] == 0 ? p - w * hh : (ckind[i] == 1 ? p : p - ax);
+    best = t;
+    float uu = hh / L;
+    if (uu < 0.0f) uu = 0.0f;
+    if (uu > 1.0f) uu = 1.0f;
+    bu = uu;
+    bn = n;
+    found = true;
   }
   if (!found) return false;
   h.t = best;
@@ -292,8 +312,7 @@ __device__ __noinline__ bool isectRound(V3 o, V3 d, float tnear, float tfar, V3 
 }
 
 // TRIANGLE (Moller-Trumbore): P = (1-u-v) v0 + u v1 + v v2, Ng = (v1-v0)x(v2-v0).
-__device__ __forceinline__ bool isectTri(V3 o, V3 d, float tnear, float tfar, V3 v0, V3 v1, V3 v2,
-                                         Hit& h) {
+UBG_HD bool isectTri(V3 o, V3 d, float tnear, float tfar, V3 v0, V3 v1, V3 v2, Hit& h) {
   const V3 e1 = v1 - v0;
   const V3 e2 = v2 - v0;
   const V3 pv = cross(d, e2);
@@ -315,13 +334,12 @@ __device__ __forceinline__ bool isectTri(V3 o, V3 d, float tnear, float tfar, V3
   return true;
 }
 
-__device__ __forceinline__ bool isectPrim(const SceneView& s, int i, V3 o, V3 d, float tnear,
-                                          float tfar, Hit& h) {
-  const float4* p = reinterpret_cast<const float4*>(s.prims) + 3 * i;
+UBG_HD bool isectPrim(const SceneView& s, uint32_t i, V3 o, V3 d, float tnear, float tfar, Hit& h) {
+  const float4* p = reinterpret_cast<const float4*>(s.prims) + 3 * static_cast<size_t>(i);
   const float4 p0 = p[0];
   const float4 p1 = p[1];
   const float4 p2 = p[2];
-  const uint32_t bits = __float_as_uint(p2.w);
+  const uint32_t bits = f2u(p2.w);
   const uint32_t kind = bits >> kKindShift;
   bool r;
   if (kind == 1u) {
@@ -338,20 +356,25 @@ __device__ __forceinline__ bool isectPrim(const SceneView& s, int i, V3 o, V3 d,
   return r;
 }
 
+// Reciprocal direction with zero components replaced by a signed tiny value
+// (keeps every slab product finite).
+UBG_HD V3 safeInv(V3 d) {
+  return mk(1.0f / (d.x != 0.0f ? d.x : copysignf(1e-30f, d.x)),
+            1.0f / (d.y != 0.0f ? d.y : copysignf(1e-30f, d.y)),
+            1.0f / (d.z != 0.0f ? d.z : copysignf(1e-30f, d.z)));
+}
+
 // ---------------------------------------------------------------------------
-// BVH2 traversal with a per-lane stack in LDS (stack slot i of lane `lane`
-// lives at stk[i * stride + lane], so the 64 lanes of a wave hit 64 distinct
-// banks). kAny: return at the first hit in [tnear, tfar]. Otherwise the
-// closest hit (strictly nearer hits replace the current best).
+// BVH2 traversal (full-precision nodes). Stack slot i of this lane lives at
+// stk[i * stride] (LDS on the GPU: stride = block size, so the 64 lanes of a
+// wave use 64 distinct banks). kAny returns at the first hit in
+// [tnear, tfar]; otherwise the closest hit (strictly nearer hits replace it).
 template <bool kAny, bool kCount>
-__device__ __forceinline__ bool traverse(const SceneView& s, V3 o, V3 d, float tnear, float tfar,
-                                         Hit& best, int* stk, int stride, uint32_t* nodeVisits,
-                                         uint32_t* primTests) {
+UBG_HD bool traverse2(const SceneView& s, V3 o, V3 d, float tnear, float tfar, Hit& best, uint32_t* stk,
+                      int stride, uint32_t* nodeVisits, uint32_t* primTests) {
   if (s.nNodes == 0) return false;
-  const float ix = 1.0f / (d.x != 0.0f ? d.x : copysignf(1e-30f, d.x));
-  const float iy = 1.0f / (d.y != 0.0f ? d.y : copysignf(1e-30f, d.y));
-  const float iz = 1.0f / (d.z != 0.0f ? d.z : copysignf(1e-30f, d.z));
-  const float oix = o.x * ix, oiy = o.y * iy, oiz = o.z * iz;
+  const V3 iv = safeInv(d);
+  const float oix = o.x * iv.x, oiy = o.y * iv.y, oiz = o.z * iv.z;
   const float4* nodes = reinterpret_cast<const float4*>(s.nodes);
   bool found = false;
   float tmax = tfar;
@@ -363,32 +386,34 @@ __device__ __forceinline__ bool traverse(const SceneView& s, V3 o, V3 d, float t
     const float4 a = n[0];
     const float4 b = n[1];
     const float4 c = n[2];
-    const int4 ch = *reinterpret_cast<const int4*>(n + 3);
+    const float4 cf = n[3];
+    const int c0 = static_cast<int>(f2u(cf.x)), c1 = static_cast<int>(f2u(cf.y));
     if (kCount) ++*nodeVisits;
     const float tmx = tmax * 1.00000012f;
-    const float l0x = __builtin_fmaf(a.x, ix, -oix), h0x = __builtin_fmaf(a.y, ix, -oix);
-    const float l0y = __builtin_fmaf(a.z, iy, -oiy), h0y = __builtin_fmaf(a.w, iy, -oiy);
-    const float l0z = __builtin_fmaf(c.x, iz, -oiz), h0z = __builtin_fmaf(c.y, iz, -oiz);
-    const float l1x = __builtin_fmaf(b.x, ix, -oix), h1x = __builtin_fmaf(b.y, ix, -oix);
-    const float l1y = __builtin_fmaf(b.z, iy, -oiy), h1y = __builtin_fmaf(b.w, iy, -oiy);
-    const float l1z = __builtin_fmaf(c.z, iz, -oiz), h1z = __builtin_fmaf(c.w, iz, -oiz);
+    const float l0x = __builtin_fmaf(a.x, iv.x, -oix), h0x = __builtin_fmaf(a.y, iv.x, -oix);
+    const float l0y = __builtin_fmaf(a.z, iv.y, -oiy), h0y = __builtin_fmaf(a.w, iv.y, -oiy);
+    const float l0z = __builtin_fmaf(c.x, iv.z, -oiz), h0z = __builtin_fmaf(c.y, iv.z, -oiz);
+    const float l1x = __builtin_fmaf(b.x, iv.x, -oix), h1x = __builtin_fmaf(b.y, iv.x, -oix);
+    const float l1y = __builtin_fmaf(b.z, iv.y, -oiy), h1y = __builtin_fmaf(b.w, iv.y, -oiy);
+    const float l1z = __builtin_fmaf(c.z, iv.z, -oiz), h1z = __builtin_fmaf(c.w, iv.z, -oiz);
     const float e0 = fmaxf(fmaxf(fminf(l0x, h0x), fminf(l0y, h0y)), fmaxf(fminf(l0z, h0z), tnear));
     const float x0 = fminf(fminf(fmaxf(l0x, h0x), fmaxf(l0y, h0y)), fminf(fmaxf(l0z, h0z), tmx));
     const float e1 = fmaxf(fmaxf(fminf(l1x, h1x), fminf(l1y, h1y)), fmaxf(fminf(l1z, h1z), tnear));
     const float x1 = fminf(fminf(fmaxf(l1x, h1x), fmaxf(l1y, h1y)), fminf(fmaxf(l1z, h1z), tmx));
     bool hit0 = e0 <= x0;
     bool hit1 = e1 <= x1;
+    const bool near1 = e1 < e0;
     // Leaves are intersected on the spot, nearer one first.
-    const bool swapOrder = e1 < e0;
+#pragma unroll
     for (int k = 0; k < 2; ++k) {
-      const bool first = (k == 0) != swapOrder;
-      const int cidx = first ? ch.x : ch.y;
-      const bool hc = first ? hit0 : hit1;
+      const bool isChild0 = (k == 0) != near1;
+      const int cidx = isChild0 ? c0 : c1;
+      const bool hc = isChild0 ? hit0 : hit1;
       if (!hc || cidx >= 0) continue;
       const uint32_t code = ~static_cast<uint32_t>(cidx);
-      const int beg = static_cast<int>(code >> 3);
-      const int cnt = static_cast<int>(code & 7u) + 1;
-      for (int i = beg; i < beg + cnt; ++i) {
+      const uint32_t beg = code >> 3;
+      const uint32_t cnt = (code & 7u) + 1u;
+      for (uint32_t i = beg; i < beg + cnt; ++i) {
         if (kCount) ++*primTests;
         if (isectPrim(s, i, o, d, tnear, tmax, h)) {
           if (kAny) return true;
@@ -399,23 +424,131 @@ __device__ __forceinline__ bool traverse(const SceneView& s, V3 o, V3 d, float t
           }
         }
       }
-      if (first) hit0 = false;
+      if (isChild0) hit0 = false;
       else hit1 = false;
     }
     if (hit0 && hit1) {
-      const bool near1 = e1 < e0;
-      stk[sp * stride] = near1 ? ch.x : ch.y;
+      stk[sp * stride] = static_cast<uint32_t>(near1 ? c0 : c1);
       ++sp;
-      node = near1 ? ch.y : ch.x;
+      node = near1 ? c1 : c0;
     } else if (hit0) {
-      node = ch.x;
+      node = c0;
     } else if (hit1) {
-      node = ch.y;
+      node = c1;
     } else {
       if (sp == 0) break;
       --sp;
-      node = stk[sp * stride];
+      node = static_cast<int>(stk[sp * stride]);
     }
+  }
+  return found;
+}
+
+// ---------------------------------------------------------------------------
+// Compressed 8-wide BVH (80 B nodes, 8-bit child boxes).
+//
+// Node = 5 x uint4:
+//   [0] px, py, pz (box origin, f32), w = ex | ey << 8 | ez << 16 | imask << 24
+//       with per-axis scale 2^(e - 127) and imask bit i = slot i is an inner node
+//   [1] childBase (first inner child), primBase (first leaf primitive),
+//       meta[0..3], meta[4..7]: 0 = empty slot; leaf = (unary prim count) << 5
+//       | offset from primBase; inner = 1 << 5 | (24 + slot)
+//   [2..4] qlo.x[8], qlo.y[8], qlo.z[8], qhi.x[8], qhi.y[8], qhi.z[8] (u8)
+// The builder places each child in the slot whose octant (bit k = child centre
+// on the + side of the node centre along axis k) matches its position, so
+// visiting inner children in increasing (slot ^ rayOctant) order is roughly
+// front to back without sorting. A traversal stack entry is a node group
+// (childBase, pending inner-hit bits 24..31 | imask 0..7): one entry per
+// level, i.e. the stack never holds more entries than the tree is deep.
+
+UBG_HD uint32_t byteOf(uint32_t lo, uint32_t hi, int i) { return ((i < 4 ? lo : hi) >> (8 * (i & 3))) & 0xFFu; }
+
+// Box tests of one node's 8 children: returns pending inner-child bits
+// (24 + (slot ^ oct)) | imask, and the leaf-primitive bits via `leafBits`.
+template <bool kCount>
+UBG_HD uint32_t intersectNode8(const uint4* nodes, uint32_t idx, V3 iv, V3 o, float tnear, float tmax, uint32_t oct,
+                               uint32_t& childBase, uint32_t& primBase, uint32_t& leafBits) {
+  const uint4* n = nodes + 5 * static_cast<size_t>(idx);
+  const uint4 w0 = n[0];
+  const uint4 w1 = n[1];
+  const uint4 w2 = n[2];
+  const uint4 w3 = n[3];
+  const uint4 w4 = n[4];
+  const float sx = u2f((w0.w & 0xFFu) << 23) * iv.x;
+  const float sy = u2f(((w0.w >> 8) & 0xFFu) << 23) * iv.y;
+  const float sz = u2f(((w0.w >> 16) & 0xFFu) << 23) * iv.z;
+  const float ox = (u2f(w0.x) - o.x) * iv.x;
+  const float oy = (u2f(w0.y) - o.y) * iv.y;
+  const float oz = (u2f(w0.z) - o.z) * iv.z;
+  const uint32_t imask = w0.w >> 24;
+  const float tmx = tmax * 1.00000012f;
+  uint32_t inner = 0, leaf = 0;
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    const uint32_t meta = byteOf(w1.z, w1.w, i);
+    const float lx = __builtin_fmaf(static_cast<float>(byteOf(w2.x, w2.y, i)), sx, ox);
+    const float hx = __builtin_fmaf(static_cast<float>(byteOf(w3.z, w3.w, i)), sx, ox);
+    const float ly = __builtin_fmaf(static_cast<float>(byteOf(w2.z, w2.w, i)), sy, oy);
+    const float hy = __builtin_fmaf(static_cast<float>(byteOf(w4.x, w4.y, i)), sy, oy);
+    const float lz = __builtin_fmaf(static_cast<float>(byteOf(w3.x, w3.y, i)), sz, oz);
+    const float hz = __builtin_fmaf(static_cast<float>(byteOf(w4.z, w4.w, i)), sz, oz);
+    const float en = fmaxf(fmaxf(fminf(lx, hx), fminf(ly, hy)), fmaxf(fminf(lz, hz), tnear));
+    const float ex = fminf(fminf(fmaxf(lx, hx), fmaxf(ly, hy)), fminf(fmaxf(lz, hz), tmx));
+    if (meta != 0u && en <= ex) {
+      if ((imask >> i) & 1u) inner |= 1u << (24 + (static_cast<uint32_t>(i) ^ oct));
+      else leaf |= (meta >> 5) << (meta & 31u);
+    }
+  }
+  childBase = w1.x;
+  primBase = w1.y;
+  leafBits = leaf;
+  return inner | imask;
+}
+
+template <bool kAny, bool kCount>
+UBG_HD bool traverse8(const SceneView& s, V3 o, V3 d, float tnear, float tfar, Hit& best, uint32_t* stkBase,
+                      uint32_t* stkHits, int stride, uint32_t* nodeVisits, uint32_t* primTests) {
+  if (s.nNodes == 0) return false;
+  const V3 iv = safeInv(d);
+  const uint32_t oct = (d.x < 0.0f ? 1u : 0u) | (d.y < 0.0f ? 2u : 0u) | (d.z < 0.0f ? 4u : 0u);
+  const uint4* nodes = reinterpret_cast<const uint4*>(s.nodes);
+  uint32_t gBase = 0, gHits = (1u << (24 + oct)) | 1u;  // the root as slot 0 of a virtual parent
+  uint32_t tBase = 0, tHits = 0;
+  int sp = 0;
+  bool found = false;
+  float tmax = tfar;
+  Hit h;
+  for (;;) {
+    while (tHits != 0u) {
+      const uint32_t b = static_cast<uint32_t>(__builtin_ctz(tHits));
+      tHits &= tHits - 1u;
+      if (kCount) ++*primTests;
+      if (isectPrim(s, tBase + b, o, d, tnear, tmax, h)) {
+        if (kAny) return true;
+        if (!found || h.t < best.t) {
+          best = h;
+          tmax = h.t;
+          found = true;
+        }
+      }
+    }
+    if ((gHits & 0xFF000000u) == 0u) {
+      if (sp == 0) break;
+      --sp;
+      gBase = stkBase[sp * stride];
+      gHits = stkHits[sp * stride];
+    }
+    const uint32_t key = static_cast<uint32_t>(__builtin_ctz(gHits >> 24));
+    gHits &= ~(1u << (24 + key));
+    const uint32_t slot = key ^ oct;
+    const uint32_t idx = gBase + static_cast<uint32_t>(__builtin_popcount(gHits & 0xFFu & ((1u << slot) - 1u)));
+    if (gHits & 0xFF000000u) {
+      stkBase[sp * stride] = gBase;
+      stkHits[sp * stride] = gHits;
+      ++sp;
+    }
+    if (kCount) ++*nodeVisits;
+    gHits = intersectNode8<kCount>(nodes, idx, iv, o, tnear, tmax, oct, gBase, tBase, tHits);
   }
   return found;
 }

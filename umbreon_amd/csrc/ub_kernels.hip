@@ -27,6 +27,25 @@ namespace ubg {
 
 constexpr int kTraceBlock = 128;
 
+// Dynamic LDS: the per-lane traversal stacks, sized at launch from the scene's
+// tree depth (BVH2: one u32 node index per entry; BVH8: a u32 base array and
+// a u32 hit-mask array).
+extern __shared__ uint32_t ubgLds[];
+
+__host__ size_t stackLdsBytes(const SceneView& s, int block) {
+  const int words = s.layout == kLayoutBvh8 ? 2 : 1;
+  return static_cast<size_t>(words) * static_cast<size_t>(s.stackDepth > 0 ? s.stackDepth : 1) * block * 4u;
+}
+
+template <int kLayout, bool kAny, bool kCount>
+__device__ __forceinline__ bool trace(const SceneView& s, V3 o, V3 d, float tn, float tf, Hit& h, uint32_t* nv,
+                                      uint32_t* nt) {
+  uint32_t* base = ubgLds + threadIdx.x;
+  if (kLayout == kLayoutBvh8)
+    return traverse8<kAny, kCount>(s, o, d, tn, tf, h, base, base + s.stackDepth * blockDim.x, blockDim.x, nv, nt);
+  return traverse2<kAny, kCount>(s, o, d, tn, tf, h, base, blockDim.x, nv, nt);
+}
+
 // hitN.w flag bits
 constexpr uint32_t kFlagMesh = 1u << 24;
 constexpr uint32_t kFlagAo = 1u << 25;
@@ -117,10 +136,9 @@ __device__ __forceinline__ void finalizePixel(const FrameConst& fc, const FrameB
   b.hiColor[pix] = make_float4(r, g, bl, outA);
 }
 
-template <bool kCount>
+template <bool kCount, int kLayout>
 __global__ __launch_bounds__(kTraceBlock) void k_primary(FrameConst fc, SceneView s, FrameBufs b,
                                                          int iter, int nActive) {
-  __shared__ int stk[kMaxDepth * kTraceBlock];
   const int tid = blockIdx.x * blockDim.x + threadIdx.x;
   const bool valid = tid < nActive;
   uint32_t pix = 0;
@@ -135,7 +153,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_primary(FrameConst fc, SceneVie
     float tn, tf;
     clipRange(fc, rd, tn, tf);
     if (iter > 0) tn = b.tnear[pix];
-    hit = traverse<false, kCount>(s, org, rd, tn, tf, h, stk + threadIdx.x, kTraceBlock, &nv, &nt);
+    hit = trace<kLayout, false, kCount>(s, org, rd, tn, tf, h, &nv, &nt);
     if (iter == 0) {
       const float vz = hit ? h.t * dot(rd, mk(fc.dir[0], fc.dir[1], fc.dir[2])) : 0.0f;
       b.depth[pix] = hit ? h.t : 0.0f;
@@ -227,10 +245,9 @@ __global__ __launch_bounds__(kTraceBlock) void k_primary(FrameConst fc, SceneVie
   b.hitC[slot] = make_float4(C.x, C.y, C.z, opacity);
 }
 
-template <bool kCount>
+template <bool kCount, int kLayout>
 __global__ __launch_bounds__(kTraceBlock) void k_ao(FrameConst fc, SceneView s, FrameBufs b,
                                                     int nMax) {
-  __shared__ int stk[kMaxDepth * kTraceBlock];
   const int nHits = min(nMax, static_cast<int>(b.counters[kCntHits]));
   const int lanes = fc.aoLanes;
   const int tid = blockIdx.x * blockDim.x + threadIdx.x;
@@ -268,8 +285,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_ao(FrameConst fc, SceneView s, 
       } else {
         traced = true;
         Hit h;
-        if (traverse<true, kCount>(s, O, dir, O4.w, fc.aoDist, h, stk + threadIdx.x, kTraceBlock,
-                                   &nv, &nt))
+        if (trace<kLayout, true, kCount>(s, O, dir, O4.w, fc.aoDist, h, &nv, &nt))
           ++occ;
       }
     }
@@ -281,10 +297,9 @@ __global__ __launch_bounds__(kTraceBlock) void k_ao(FrameConst fc, SceneView s, 
   if (valid && j == 0) b.hitAo[hi] = occ;
 }
 
-template <bool kCount>
+template <bool kCount, int kLayout>
 __global__ __launch_bounds__(kTraceBlock) void k_resolve(FrameConst fc, SceneView s, FrameBufs b,
                                                          int iter, int nMax) {
-  __shared__ int stk[kMaxDepth * kTraceBlock];
   const int nHits = min(nMax, static_cast<int>(b.counters[kCntHits]));
   const int tid = blockIdx.x * blockDim.x + threadIdx.x;
   const bool valid = tid < nHits;
@@ -333,8 +348,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_resolve(FrameConst fc, SceneVie
         Hit h;
         if (!l.soft) {
           ++nShadow;
-          sf = traverse<true, kCount>(s, O, L, eps, __builtin_inff(), h, stk + threadIdx.x,
-                                      kTraceBlock, &nv, &nt)
+          sf = trace<kLayout, true, kCount>(s, O, L, eps, __builtin_inff(), h, &nv, &nt)
                    ? 0.0f
                    : 1.0f;
         } else {
@@ -343,8 +357,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_resolve(FrameConst fc, SceneVie
             tea2(s0, s1);
             const V3 dir = sampleLightDir(L, l.tanRadius, u32ToUnorm(s0), u32ToUnorm(s1));
             ++nShadow;
-            if (traverse<true, kCount>(s, O, dir, eps, __builtin_inff(), h, stk + threadIdx.x,
-                                       kTraceBlock, &nv, &nt))
+            if (trace<kLayout, true, kCount>(s, O, dir, eps, __builtin_inff(), h, &nv, &nt))
               ++hits;
           }
           sf = 1.0f - (float)hits / (float)fc.shadowSamples;
@@ -491,9 +504,9 @@ struct HitOut {
   float ng[3];
 };
 
+template <int kLayout>
 __global__ __launch_bounds__(kTraceBlock) void k_query(SceneView s, const RayIn* rays, int64_t n,
                                                        HitOut* hits, uint8_t* occ, int any) {
-  __shared__ int stk[kMaxDepth * kTraceBlock];
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const RayIn r = rays[i];
@@ -502,14 +515,14 @@ __global__ __launch_bounds__(kTraceBlock) void k_query(SceneView s, const RayIn*
   Hit h;
   uint32_t nv = 0, nt = 0;
   if (any) {
-    occ[i] = traverse<true, false>(s, o, d, r.tnear, r.tfar, h, stk + threadIdx.x, kTraceBlock, &nv,
+    occ[i] = trace<kLayout, true, false>(s, o, d, r.tnear, r.tfar, h, &nv,
                                    &nt)
                  ? 1
                  : 0;
     return;
   }
   HitOut out;
-  if (traverse<false, false>(s, o, d, r.tnear, r.tfar, h, stk + threadIdx.x, kTraceBlock, &nv,
+  if (trace<kLayout, false, false>(s, o, d, r.tnear, r.tfar, h, &nv,
                              &nt)) {
     out.t = h.t;
     out.u = h.u;
@@ -550,36 +563,37 @@ void launchZeroCounters(unsigned long long* c, int n, hipStream_t st) {
 void launchResetQueues(unsigned long long* c, hipStream_t st) {
   hipLaunchKernelGGL(k_reset_queues, dim3(1), dim3(64), 0, st, c);
 }
+#define UBG_DISPATCH(KERNEL, GRID, COUNT, ...)                                                  \
+  do {                                                                                          \
+    const size_t lds = stackLdsBytes(s, kTraceBlock);                                           \
+    if (s.layout == kLayoutBvh8) {                                                              \
+      if (COUNT)                                                                                \
+        hipLaunchKernelGGL((KERNEL<true, kLayoutBvh8>), dim3(GRID), dim3(kTraceBlock), lds, st, __VA_ARGS__); \
+      else                                                                                      \
+        hipLaunchKernelGGL((KERNEL<false, kLayoutBvh8>), dim3(GRID), dim3(kTraceBlock), lds, st, __VA_ARGS__); \
+    } else {                                                                                    \
+      if (COUNT)                                                                                \
+        hipLaunchKernelGGL((KERNEL<true, kLayoutBvh2>), dim3(GRID), dim3(kTraceBlock), lds, st, __VA_ARGS__); \
+      else                                                                                      \
+        hipLaunchKernelGGL((KERNEL<false, kLayoutBvh2>), dim3(GRID), dim3(kTraceBlock), lds, st, __VA_ARGS__); \
+    }                                                                                           \
+  } while (0)
+
 void launchPrimary(const FrameConst& fc, const SceneView& s, const FrameBufs& b, int iter,
                    int nActive, bool countVisits, hipStream_t st) {
   if (nActive <= 0) return;
-  if (countVisits)
-    hipLaunchKernelGGL(k_primary<true>, dim3(blocksFor(nActive, kTraceBlock)), dim3(kTraceBlock), 0,
-                       st, fc, s, b, iter, nActive);
-  else
-    hipLaunchKernelGGL(k_primary<false>, dim3(blocksFor(nActive, kTraceBlock)), dim3(kTraceBlock), 0,
-                       st, fc, s, b, iter, nActive);
+  UBG_DISPATCH(k_primary, blocksFor(nActive, kTraceBlock), countVisits, fc, s, b, iter, nActive);
 }
 void launchAo(const FrameConst& fc, const SceneView& s, const FrameBufs& b, int nMax,
               bool countVisits, hipStream_t st) {
   if (nMax <= 0 || fc.aoSamples <= 0) return;
   const int64_t n = (int64_t)nMax * fc.aoLanes;
-  if (countVisits)
-    hipLaunchKernelGGL(k_ao<true>, dim3(blocksFor(n, kTraceBlock)), dim3(kTraceBlock), 0, st, fc, s,
-                       b, nMax);
-  else
-    hipLaunchKernelGGL(k_ao<false>, dim3(blocksFor(n, kTraceBlock)), dim3(kTraceBlock), 0, st, fc,
-                       s, b, nMax);
+  UBG_DISPATCH(k_ao, blocksFor(n, kTraceBlock), countVisits, fc, s, b, nMax);
 }
 void launchResolve(const FrameConst& fc, const SceneView& s, const FrameBufs& b, int iter,
                    int nMax, bool countVisits, hipStream_t st) {
   if (nMax <= 0) return;
-  if (countVisits)
-    hipLaunchKernelGGL(k_resolve<true>, dim3(blocksFor(nMax, kTraceBlock)), dim3(kTraceBlock), 0,
-                       st, fc, s, b, iter, nMax);
-  else
-    hipLaunchKernelGGL(k_resolve<false>, dim3(blocksFor(nMax, kTraceBlock)), dim3(kTraceBlock), 0,
-                       st, fc, s, b, iter, nMax);
+  UBG_DISPATCH(k_resolve, blocksFor(nMax, kTraceBlock), countVisits, fc, s, b, iter, nMax);
 }
 void launchPost(const float4* hi, int W, int H, int ss, float gamma, float4* out, hipStream_t st) {
   const int n = (W / ss) * (H / ss);
@@ -589,9 +603,90 @@ void launchPost(const float4* hi, int W, int H, int ss, float gamma, float4* out
 void launchQuery(const SceneView& s, const void* rays, int64_t n, void* hits, uint8_t* occ,
                  bool any, hipStream_t st) {
   if (n <= 0) return;
-  hipLaunchKernelGGL(k_query, dim3(blocksFor(n, kTraceBlock)), dim3(kTraceBlock), 0, st, s,
-                     static_cast<const RayIn*>(rays), n, static_cast<HitOut*>(hits), occ,
-                     any ? 1 : 0);
+  const size_t lds = stackLdsBytes(s, kTraceBlock);
+  if (s.layout == kLayoutBvh8)
+    hipLaunchKernelGGL(k_query<kLayoutBvh8>, dim3(blocksFor(n, kTraceBlock)), dim3(kTraceBlock), lds, st, s,
+                       static_cast<const RayIn*>(rays), n, static_cast<HitOut*>(hits), occ, any ? 1 : 0);
+  else
+    hipLaunchKernelGGL(k_query<kLayoutBvh2>, dim3(blocksFor(n, kTraceBlock)), dim3(kTraceBlock), lds, st, s,
+                       static_cast<const RayIn*>(rays), n, static_cast<HitOut*>(hits), occ, any ? 1 : 0);
 }
 
 }  // namespace ubg
+
+// ---------------------------------------------------------------------------
+// Host test hook (ub_testing.h): the traversal above, run on the CPU.
+#include <string>
+#include <vector>
+
+#include "ub_build.h"
+#include "ub_testing.h"
+
+extern "C" int ubgx_host_query(const ubg_scene_desc* desc, int32_t bvh_layout, int64_t n_rays, const ubg_ray* rays,
+                               ubg_hit* hits, uint8_t* occ, uint64_t* stats) {
+  using namespace ubg;
+  if (!desc || (n_rays > 0 && !rays)) return UBG_ERR_INVALID_ARGUMENT;
+  HostScene hs;
+  std::string err;
+  const int layout = bvh_layout == UBG_BVH2 ? kLayoutBvh2 : kLayoutBvh8;
+  const int st = buildHostScene(desc, layout, hs, err);
+  if (st != UBG_OK) return st;
+  SceneView v{};
+  v.nodes = hs.nodes.data();
+  v.prims = hs.prims.data();
+  v.nNodes = static_cast<int32_t>(hs.nNodes);
+  v.nPrims = static_cast<int32_t>(hs.prims.size() / 3);
+  v.layout = hs.layout;
+  v.stackDepth = hs.stackDepth;
+  v.rndPlanes = hs.rndPlanes.data();
+  std::vector<uint32_t> s0(static_cast<size_t>(hs.stackDepth) + 1), s1(static_cast<size_t>(hs.stackDepth) + 1);
+  uint64_t visits = 0, tests = 0;
+  for (int64_t i = 0; i < n_rays; ++i) {
+    const ubg_ray& r = rays[i];
+    const V3 o = mk(r.org[0], r.org[1], r.org[2]);
+    const V3 d = mk(r.dir[0], r.dir[1], r.dir[2]);
+    Hit h{};
+    uint32_t nv = 0, nt = 0;
+    if (hits) {
+      const bool f = layout == kLayoutBvh8
+                         ? traverse8<false, true>(v, o, d, r.tnear, r.tfar, h, s0.data(), s1.data(), 1, &nv, &nt)
+                         : traverse2<false, true>(v, o, d, r.tnear, r.tfar, h, s0.data(), 1, &nv, &nt);
+      visits += nv;
+      tests += nt;
+      ubg_hit& out = hits[i];
+      if (f) {
+        out.t = h.t;
+        out.u = h.u;
+        out.v = h.v;
+        out.kind = h.bits >> kKindShift;
+        out.prim_id = h.bits & kIdMask;
+        out.ng[0] = h.ng.x;
+        out.ng[1] = h.ng.y;
+        out.ng[2] = h.ng.z;
+      } else {
+        out.t = __builtin_inff();
+        out.u = out.v = 0.0f;
+        out.kind = 0xFFFFFFFFu;
+        out.prim_id = 0xFFFFFFFFu;
+        out.ng[0] = out.ng[1] = out.ng[2] = 0.0f;
+      }
+    }
+    if (occ) {
+      uint32_t a = 0, b = 0;
+      occ[i] = (layout == kLayoutBvh8
+                    ? traverse8<true, false>(v, o, d, r.tnear, r.tfar, h, s0.data(), s1.data(), 1, &a, &b)
+                    : traverse2<true, false>(v, o, d, r.tnear, r.tfar, h, s0.data(), 1, &a, &b))
+                   ? 1
+                   : 0;
+    }
+  }
+  if (stats) {
+    stats[0] = visits;
+    stats[1] = tests;
+    stats[2] = static_cast<uint64_t>(hs.depth);
+    stats[3] = static_cast<uint64_t>(hs.nNodes);
+    stats[4] = static_cast<uint64_t>(hs.stackDepth);
+    stats[5] = static_cast<uint64_t>(hs.nodeBytes);
+  }
+  return UBG_OK;
+}

@@ -26,9 +26,14 @@
 
 namespace ubg {
 
-constexpr int kMaxDepth = 32;   // builder guarantee; = traversal stack entries
-constexpr int kMaxLeaf = 8;     // primitives per leaf (3 count bits)
+constexpr int kMaxDepth = 32;   // BVH2 builder guarantee; = traversal stack entries
+constexpr int kMaxLeaf = 8;     // BVH2 primitives per leaf (3 count bits)
+constexpr int kMaxLeaf8 = 3;    // BVH8 primitives per leaf slot (unary count in meta)
 constexpr int kMaxLights = 16;  // lights carried in the kernel argument block
+
+// Acceleration-structure layouts (ubg_scene_create_ex).
+constexpr int kLayoutBvh2 = 0;  // full-precision BVH2, 64 B nodes
+constexpr int kLayoutBvh8 = 1;  // compressed 8-wide BVH, 80 B nodes, 8-bit child boxes
 
 constexpr uint32_t kKindShift = 30;
 constexpr uint32_t kIdMask = (1u << 30) - 1u;
@@ -45,10 +50,12 @@ struct DevMaterial {
 };
 
 struct SceneView {
-  const void* nodes;      // float4 x 4 per inner node
-  const void* prims;      // float4 x 3 per primitive
+  const void* nodes;      // BVH2: float4 x 4 per inner node; BVH8: uint4 x 5 per node
+  const void* prims;      // float4 x 3 per primitive, in the layout's leaf order
   int32_t nNodes;
   int32_t nPrims;
+  int32_t layout;         // kLayoutBvh2 / kLayoutBvh8
+  int32_t stackDepth;     // traversal stack entries the tree needs
   const void* sphColor;   // float4
   const uint32_t* sphInfo;
   const void* coneColor;  // float4

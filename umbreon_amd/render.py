@@ -71,12 +71,12 @@ class Device:
 class BuiltScene:
     """Scene geometry resident in HBM with its BVH (buildEmbreeScene analogue)."""
 
-    def __init__(self, device: Device, scene: Scene):
+    def __init__(self, device: Device, scene: Scene, layout: int = capi.BVH8):
         self.device = device
         self._lib = device._lib
         desc, keep = scene.to_desc()
         h = C.c_void_p()
-        capi.check(self._lib.ubg_scene_create(device.handle, C.byref(desc), C.byref(h)))
+        capi.check(self._lib.ubg_scene_create_ex(device.handle, C.byref(desc), int(layout), C.byref(h)))
         del keep
         self.handle = h
         info = capi.SceneInfo()
@@ -140,6 +140,22 @@ class BuiltScene:
                            cancelled=bool(fr.cancelled), layers=int(fr.layers),
                            kernel_ms={k: float(fr.kernel_ms[i]) for i, k in enumerate(capi.KERNELS)},
                            kernel_launches={k: int(fr.kernel_launches[i]) for i, k in enumerate(capi.KERNELS)})
+
+
+def host_query(scene: Scene, rays: np.ndarray, layout: int = capi.BVH8):
+    """The library's traversal run on the CPU (test hook, no GPU needed):
+    returns (closest hits, occluded flags, stats dict)."""
+    lib = capi.load()
+    desc, keep = scene.to_desc()
+    rays = np.ascontiguousarray(rays, dtype=capi.ray_dtype())
+    hits = np.zeros(len(rays), dtype=capi.hit_dtype())
+    occ = np.zeros(len(rays), dtype=np.uint8)
+    st = np.zeros(6, np.uint64)
+    capi.check(lib.ubgx_host_query(C.byref(desc), int(layout), len(rays), rays.ctypes.data, hits.ctypes.data,
+                                   occ.ctypes.data, st.ctypes.data))
+    del keep
+    names = ("node_visits", "prim_tests", "depth", "nodes", "stack_depth", "node_bytes")
+    return hits, occ, {k: int(v) for k, v in zip(names, st)}
 
 
 _default_device: Optional[Device] = None
