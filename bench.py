@@ -63,6 +63,24 @@ def cpu_baseline(scene, opt, target_s: float):
     }
 
 
+def calib_copy(mib: int) -> None:
+    """One hipMemcpy device-to-device of `mib` MiB (the runtime's copy kernel:
+    reads and writes exactly that many bytes), for counter calibration."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    n = mib << 20
+    a, b = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(a), C.c_size_t(n)) == 0
+    assert hip.hipMalloc(C.byref(b), C.c_size_t(n)) == 0
+    assert hip.hipMemset(a, 1, C.c_size_t(n)) == 0
+    assert hip.hipDeviceSynchronize() == 0
+    assert hip.hipMemcpy(b, a, C.c_size_t(n), 3) == 0  # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+    log(f"calibration copy: {n} bytes device-to-device")
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -73,6 +91,9 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--layout", choices=["bvh8", "bvh2"], default="bvh8",
                     help="acceleration structure: compressed 8-wide (default) or full-precision binary")
+    ap.add_argument("--calib-copy-mib", type=int, default=0,
+                    help="also run one device-to-device copy of this many MiB (a copy kernel of known "
+                         "traffic, for scaling FETCH_SIZE / WRITE_SIZE counter readings)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one MI355X: this build runs on a single GPU; "
@@ -144,6 +165,9 @@ def main() -> int:
                     "prim_tests": round(tests / max(1, st["ao_rays_traced"] if dom == "ao" else rays_in), 2),
                     "node_bytes": node_b, "prim_bytes": prim_b, "ray_bytes": 48},
     }
+
+    if args.calib_copy_mib > 0:
+        calib_copy(args.calib_copy_mib)
 
     cpu = None
     if not args.no_cpu_baseline:

@@ -64,6 +64,7 @@ def test_host_traversal_matches_oracle(layout, short):
     assert np.array_equal(np.isfinite(h["t"]), hit)
     # identical fp32 formulas: t bit-identical (ties may pick another primitive at the same t)
     assert np.array_equal(h["t"][hit], ref["t"][hit])
+    assert occ.max(initial=0) <= 1  # 2 = one-shot and resumable BVH8 any-hit traversals disagree
     assert np.array_equal(occ.astype(bool), hit)
     assert st["stack_depth"] >= 1 and st["node_visits"] > 0
 

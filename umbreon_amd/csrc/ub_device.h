@@ -482,27 +482,43 @@ UBG_HD uint32_t intersectNode8(const uint4* nodes, uint32_t idx, V3 iv, V3 o, fl
   const float oz = (u2f(w0.z) - o.z) * iv.z;
   const uint32_t imask = w0.w >> 24;
   const float tmx = tmax * 1.00000012f;
-  uint32_t inner = 0, leaf = 0;
+  // Per axis, the plane the ray enters through is the low one for a positive
+  // direction and the high one for a negative direction: pick the byte arrays
+  // once per node instead of min/max per child.
+  const bool px = iv.x >= 0.0f, py = iv.y >= 0.0f, pz = iv.z >= 0.0f;
+  const uint32_t nx0 = px ? w2.x : w3.z, nx1 = px ? w2.y : w3.w, fx0 = px ? w3.z : w2.x, fx1 = px ? w3.w : w2.y;
+  const uint32_t ny0 = py ? w2.z : w4.x, ny1 = py ? w2.w : w4.y, fy0 = py ? w4.x : w2.z, fy1 = py ? w4.y : w2.w;
+  const uint32_t nz0 = pz ? w3.x : w4.z, nz1 = pz ? w3.y : w4.w, fz0 = pz ? w4.z : w3.x, fz1 = pz ? w4.w : w3.y;
+  // Branch-free: collect the slots whose box is hit, then split them into
+  // inner children (pending bits, reordered by the ray octant) and leaf
+  // primitives.
+  uint32_t hitSlots = 0, leaf = 0;
 #pragma unroll
   for (int i = 0; i < 8; ++i) {
     const uint32_t meta = byteOf(w1.z, w1.w, i);
-    const float lx = __builtin_fmaf(static_cast<float>(byteOf(w2.x, w2.y, i)), sx, ox);
-    const float hx = __builtin_fmaf(static_cast<float>(byteOf(w3.z, w3.w, i)), sx, ox);
-    const float ly = __builtin_fmaf(static_cast<float>(byteOf(w2.z, w2.w, i)), sy, oy);
-    const float hy = __builtin_fmaf(static_cast<float>(byteOf(w4.x, w4.y, i)), sy, oy);
-    const float lz = __builtin_fmaf(static_cast<float>(byteOf(w3.x, w3.y, i)), sz, oz);
-    const float hz = __builtin_fmaf(static_cast<float>(byteOf(w4.z, w4.w, i)), sz, oz);
-    const float en = fmaxf(fmaxf(fminf(lx, hx), fminf(ly, hy)), fmaxf(fminf(lz, hz), tnear));
-    const float ex = fminf(fminf(fmaxf(lx, hx), fmaxf(ly, hy)), fminf(fmaxf(lz, hz), tmx));
-    if (meta != 0u && en <= ex) {
-      if ((imask >> i) & 1u) inner |= 1u << (24 + (static_cast<uint32_t>(i) ^ oct));
-      else leaf |= (meta >> 5) << (meta & 31u);
-    }
+    const float nxv = __builtin_fmaf(static_cast<float>(byteOf(nx0, nx1, i)), sx, ox);
+    const float fxv = __builtin_fmaf(static_cast<float>(byteOf(fx0, fx1, i)), sx, ox);
+    const float nyv = __builtin_fmaf(static_cast<float>(byteOf(ny0, ny1, i)), sy, oy);
+    const float fyv = __builtin_fmaf(static_cast<float>(byteOf(fy0, fy1, i)), sy, oy);
+    const float nzv = __builtin_fmaf(static_cast<float>(byteOf(nz0, nz1, i)), sz, oz);
+    const float fzv = __builtin_fmaf(static_cast<float>(byteOf(fz0, fz1, i)), sz, oz);
+    const float en = fmaxf(fmaxf(nxv, nyv), fmaxf(nzv, tnear));
+    const float ex = fminf(fminf(fxv, fyv), fminf(fzv, tmx));
+    const bool ok = meta != 0u && en <= ex;
+    hitSlots |= ok ? (1u << i) : 0u;
+    // leaf meta = (unary count) << 5 | offset; for an inner slot this sets a
+    // bit >= 24, masked off below
+    leaf |= ok ? (meta >> 5) << (meta & 31u) : 0u;
   }
+  uint32_t inner = hitSlots & imask;
+  // slot i -> key i ^ oct, as three conditional swaps of bit groups
+  if (oct & 1u) inner = ((inner & 0x55u) << 1) | ((inner >> 1) & 0x55u);
+  if (oct & 2u) inner = ((inner & 0x33u) << 2) | ((inner >> 2) & 0x33u);
+  if (oct & 4u) inner = ((inner & 0x0Fu) << 4) | ((inner >> 4) & 0x0Fu);
   childBase = w1.x;
   primBase = w1.y;
-  leafBits = leaf;
-  return inner | imask;
+  leafBits = leaf & 0x00FFFFFFu;
+  return (inner << 24) | imask;
 }
 
 template <bool kAny, bool kCount>
@@ -551,6 +567,80 @@ UBG_HD bool traverse8(const SceneView& s, V3 o, V3 d, float tnear, float tfar, H
     gHits = intersectNode8<kCount>(nodes, idx, iv, o, tnear, tmax, oct, gBase, tBase, tHits);
   }
   return found;
+}
+
+// Any-hit BVH8 traversal as a resumable state machine: one call does one unit
+// of work (one primitive test or one node test), so a lane that finishes its
+// ray can take a new one while the rest of its wave is still traversing.
+struct Trav8 {
+  V3 o, d, iv;
+  float tnear, tmax;
+  uint32_t oct, gBase, gHits, tBase, tHits;
+  int sp;
+};
+
+UBG_HD void trav8Init(Trav8& t, V3 o, V3 d, float tnear, float tfar) {
+  t.o = o;
+  t.d = d;
+  t.iv = safeInv(d);
+  t.tnear = tnear;
+  t.tmax = tfar;
+  t.oct = (d.x < 0.0f ? 1u : 0u) | (d.y < 0.0f ? 2u : 0u) | (d.z < 0.0f ? 4u : 0u);
+  t.gBase = 0;
+  t.gHits = (1u << (24 + t.oct)) | 1u;
+  t.tBase = 0;
+  t.tHits = 0;
+  t.sp = 0;
+}
+
+// Node work is pending: inner children of the current group or a stacked group.
+UBG_HD bool trav8HasNodes(const Trav8& t) { return (t.gHits & 0xFF000000u) != 0u || t.sp != 0; }
+
+// One node test (requires no pending primitives and trav8HasNodes). Returns
+// -1 when the traversal is exhausted, else 0.
+template <bool kCount>
+UBG_HD int trav8Node(const SceneView& s, Trav8& t, uint32_t* stkBase, uint32_t* stkHits, int stride,
+                     uint32_t* nodeVisits) {
+  if ((t.gHits & 0xFF000000u) == 0u) {
+    --t.sp;
+    t.gBase = stkBase[t.sp * stride];
+    t.gHits = stkHits[t.sp * stride];
+  }
+  const uint32_t key = static_cast<uint32_t>(__builtin_ctz(t.gHits >> 24));
+  t.gHits &= ~(1u << (24 + key));
+  const uint32_t slot = key ^ t.oct;
+  const uint32_t idx = t.gBase + static_cast<uint32_t>(__builtin_popcount(t.gHits & 0xFFu & ((1u << slot) - 1u)));
+  if (t.gHits & 0xFF000000u) {
+    stkBase[t.sp * stride] = t.gBase;
+    stkHits[t.sp * stride] = t.gHits;
+    ++t.sp;
+  }
+  if (kCount) ++*nodeVisits;
+  t.gHits = intersectNode8<kCount>(reinterpret_cast<const uint4*>(s.nodes), idx, t.iv, t.o, t.tnear, t.tmax, t.oct,
+                                   t.gBase, t.tBase, t.tHits);
+  return (t.tHits == 0u && !trav8HasNodes(t)) ? -1 : 0;
+}
+
+// One primitive test (requires pending primitives). Returns 1 when it
+// occludes the ray, -1 when nothing is left to visit, else 0.
+template <bool kCount>
+UBG_HD int trav8Prim(const SceneView& s, Trav8& t, uint32_t* primTests) {
+  const uint32_t b = static_cast<uint32_t>(__builtin_ctz(t.tHits));
+  t.tHits &= t.tHits - 1u;
+  if (kCount) ++*primTests;
+  Hit h;
+  if (isectPrim(s, t.tBase + b, t.o, t.d, t.tnear, t.tmax, h)) return 1;
+  return (t.tHits == 0u && !trav8HasNodes(t)) ? -1 : 0;
+}
+
+// One unit of any-hit work: a primitive test if one is pending, else a node
+// test. Returns 1 when a primitive occludes the ray, -1 when the traversal is
+// exhausted without a hit, 0 to continue.
+template <bool kCount>
+UBG_HD int trav8StepAny(const SceneView& s, Trav8& t, uint32_t* stkBase, uint32_t* stkHits, int stride,
+                        uint32_t* nodeVisits, uint32_t* primTests) {
+  if (t.tHits != 0u) return trav8Prim<kCount>(s, t, primTests);
+  return trav8Node<kCount>(s, t, stkBase, stkHits, stride, nodeVisits);
 }
 
 }  // namespace ubg

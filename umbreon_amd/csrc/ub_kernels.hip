@@ -20,6 +20,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "ub_device.h"
 #include "ub_kernels.h"
 
@@ -243,6 +245,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_primary(FrameConst fc, SceneVie
   b.hitO[slot] = make_float4(O.x, O.y, O.z, eps);
   b.hitN[slot] = make_float4(N.x, N.y, N.z, __uint_as_float(flags));
   b.hitC[slot] = make_float4(C.x, C.y, C.z, opacity);
+  b.hitAo[slot] = 0u;
 }
 
 template <bool kCount, int kLayout>
@@ -295,6 +298,105 @@ __global__ __launch_bounds__(kTraceBlock) void k_ao(FrameConst fc, SceneView s, 
   if (kCount) addVisits(b.counters, 1, nv, nt);
   for (int off = lanes >> 1; off > 0; off >>= 1) occ += __shfl_xor(occ, off, lanes);
   if (valid && j == 0) b.hitAo[hi] = occ;
+}
+
+// Persistent AO over the compressed BVH8. A wave takes AO rays (hit, sample)
+// in chunks of kAoChunk from a global counter (one atomic per chunk) and
+// hands them to its lanes from a wave-uniform cursor; each lane advances its
+// own traversal one node or one primitive per iteration and takes the next
+// ray once its own has finished, so lanes do not idle behind the longest ray
+// of their wave. Refills wait until at least kAoRefill lanes are idle (or the
+// wave would otherwise stall), and each iteration runs one kind of step for
+// the whole wave (node tests or primitive tests, by lane-count vote).
+// Occluded rays add 1 to their hit's count: integer atomics, so the result
+// does not depend on the order.
+constexpr int kAoChunk = 256;
+constexpr int kAoRefill = 16;
+
+template <bool kCount>
+__global__ __launch_bounds__(kTraceBlock) void k_ao_persistent(FrameConst fc, SceneView s, FrameBufs b, int nMax) {
+  const int nHits = min(nMax, static_cast<int>(b.counters[kCntHits]));
+  const uint32_t n = static_cast<uint32_t>(fc.aoSamples);
+  const uint64_t total = static_cast<uint64_t>(nHits) * n;
+  uint32_t* sb = ubgLds + threadIdx.x;
+  uint32_t* sh = sb + s.stackDepth * blockDim.x;
+  const int stride = blockDim.x;
+  const uint64_t laneBit = 1ull << __lane_id();
+  // wave-uniform cursor into the current chunk
+  uint64_t next = 0, end = 0;
+  bool drained = false;  // the global counter has passed `total`
+  bool active = false;
+  uint32_t hitIdx = 0;
+  uint32_t nTraced = 0, nSampled = 0, nv = 0, nt = 0;
+  Trav8 t;
+  trav8Init(t, mk(0, 0, 0), mk(0, 0, 1), 0.0f, 0.0f);
+  for (;;) {
+    const unsigned long long act = __ballot(active);
+    const unsigned long long idle = ~act;
+    const int nIdle = __popcll(idle);
+    if (!drained && (nIdle >= kAoRefill || act == 0ull)) {
+      if (next == end) {
+        uint64_t base = 0;
+        if (__lane_id() == 0) base = atomicAdd(&b.counters[kCntAoNext], (unsigned long long)kAoChunk);
+        base = __shfl(base, 0, 64);
+        if (base >= total) {
+          drained = true;
+        } else {
+          next = base;
+          end = min(base + static_cast<uint64_t>(kAoChunk), total);
+        }
+      }
+      if (next < end) {
+        const uint64_t rank = static_cast<uint64_t>(__popcll(idle & (laneBit - 1ull)));
+        const uint64_t idx = next + rank;
+        const uint64_t taken = min(static_cast<uint64_t>(nIdle), end - next);
+        next += taken;
+        if (!active && rank < taken) {
+          hitIdx = static_cast<uint32_t>(idx / n);
+          const uint32_t si = static_cast<uint32_t>(idx - static_cast<uint64_t>(hitIdx) * n);
+          const float4 N4 = b.hitN[hitIdx];
+          if (__float_as_uint(N4.w) & kFlagAo) {
+            const float4 O4 = b.hitO[hitIdx];
+            const V3 N = ld3(N4);
+            const Frame3 f = frameFromNormal(N);
+            uint32_t s0 = b.hitPix[hitIdx], s1 = si;  // seed (px + py*W, i)
+            tea2(s0, s1);
+            const V3 dir = cosineSample(u32ToUnorm(s0), u32ToUnorm(s1), f);
+            if (dot(dir, N) < 0.01f) {
+              ++nSampled;
+              atomicAdd(&b.hitAo[hitIdx], 1u);
+            } else {
+              ++nTraced;
+              trav8Init(t, ld3(O4), dir, O4.w, fc.aoDist);
+              active = true;
+            }
+          }
+        }
+      }
+    }
+    if (act == 0ull && drained) break;
+    // The wave runs either a primitive step or a node step, never both: it
+    // takes primitives once their lanes number at least half the node lanes.
+    const bool wantPrim = active && t.tHits != 0u;
+    const int nPrim = __popcll(__ballot(wantPrim));
+    const int nNode = __popcll(__ballot(active && !wantPrim));
+    int r = 0;
+    if (2 * nPrim >= nNode) {
+      if (wantPrim) r = trav8Prim<kCount>(s, t, &nt);
+    } else {
+      if (active && !wantPrim) r = trav8Node<kCount>(s, t, sb, sh, stride, &nv);
+    }
+    if (r != 0) {
+      active = false;
+      if (r > 0) atomicAdd(&b.hitAo[hitIdx], 1u);
+    }
+  }
+  const uint32_t tr = waveSum(nTraced), sa = waveSum(nSampled);
+  if (__lane_id() == 0) {
+    if (tr) atomicAdd(&b.counters[kCntAoTraced], (unsigned long long)tr);
+    if (sa) atomicAdd(&b.counters[kCntAoSampled], (unsigned long long)sa);
+  }
+  if (kCount) addVisits(b.counters, 1, nv, nt);
 }
 
 template <bool kCount, int kLayout>
@@ -550,6 +652,7 @@ __global__ void k_reset_queues(unsigned long long* c) {
   if (threadIdx.x == 0) {
     c[kCntHits] = 0ull;
     c[kCntActive] = 0ull;
+    c[kCntAoNext] = 0ull;
   }
 }
 
@@ -587,6 +690,17 @@ void launchPrimary(const FrameConst& fc, const SceneView& s, const FrameBufs& b,
 void launchAo(const FrameConst& fc, const SceneView& s, const FrameBufs& b, int nMax,
               bool countVisits, hipStream_t st) {
   if (nMax <= 0 || fc.aoSamples <= 0) return;
+  if (s.layout == kLayoutBvh8) {
+    // persistent grid: enough waves to fill every CU, no more than the work
+    const int64_t rays = (int64_t)nMax * fc.aoSamples;
+    const int64_t blocks = std::min<int64_t>(blocksFor(rays, kTraceBlock), 256 * 12);
+    const size_t lds = stackLdsBytes(s, kTraceBlock);
+    if (countVisits)
+      hipLaunchKernelGGL(k_ao_persistent<true>, dim3(blocks), dim3(kTraceBlock), lds, st, fc, s, b, nMax);
+    else
+      hipLaunchKernelGGL(k_ao_persistent<false>, dim3(blocks), dim3(kTraceBlock), lds, st, fc, s, b, nMax);
+    return;
+  }
   const int64_t n = (int64_t)nMax * fc.aoLanes;
   UBG_DISPATCH(k_ao, blocksFor(n, kTraceBlock), countVisits, fc, s, b, nMax);
 }
@@ -673,11 +787,23 @@ extern "C" int ubgx_host_query(const ubg_scene_desc* desc, int32_t bvh_layout, i
     }
     if (occ) {
       uint32_t a = 0, b = 0;
-      occ[i] = (layout == kLayoutBvh8
-                    ? traverse8<true, false>(v, o, d, r.tnear, r.tfar, h, s0.data(), s1.data(), 1, &a, &b)
-                    : traverse2<true, false>(v, o, d, r.tnear, r.tfar, h, s0.data(), 1, &a, &b))
-                   ? 1
-                   : 0;
+      if (layout == kLayoutBvh8) {
+        // the one-shot any-hit traversal and the resumable step machine must
+        // agree; a disagreement is reported as 2
+        const bool o1 = traverse8<true, false>(v, o, d, r.tnear, r.tfar, h, s0.data(), s1.data(), 1, &a, &b);
+        bool o2 = false;
+        if (v.nNodes > 0) {
+          Trav8 t;
+          trav8Init(t, o, d, r.tnear, r.tfar);
+          int rs = 0;
+          while ((rs = trav8StepAny<false>(v, t, s0.data(), s1.data(), 1, &a, &b)) == 0) {
+          }
+          o2 = rs > 0;
+        }
+        occ[i] = o1 == o2 ? (o1 ? 1 : 0) : 2;
+      } else {
+        occ[i] = traverse2<true, false>(v, o, d, r.tnear, r.tfar, h, s0.data(), 1, &a, &b) ? 1 : 0;
+      }
     }
   }
   if (stats) {

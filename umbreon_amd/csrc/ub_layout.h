@@ -121,6 +121,7 @@ enum Counter {
   kCntActive = 7,    // continuation-queue fill of the current pass
   kCntNodes = 8,      // node visits per ray class: 8 primary, 9 AO, 10 shadow
   kCntPrimTests = 11, // primitive tests per ray class: 11, 12, 13
+  kCntAoNext = 14,    // persistent AO kernel: next AO ray to hand out
   kNumCounters = 16
 };
 
