@@ -473,8 +473,128 @@ struct Node8 {
 };
 static_assert(sizeof(Node8) == 80, "BVH8 node is 80 bytes");
 
+// SAH-optimal conversion of the binary tree into 8-wide nodes: dynamic
+// programming over how many slots of its parent each binary subtree takes
+// (the wide-BVH collapse of Ylitie, Karras and Laine, "Efficient Incoherent
+// Ray Traversal on GPUs Through Compressed Wide BVHs", HPG 2017). cost[n][i]
+// is the least SAH cost of subtree n spread over at most i slots; a subtree
+// of at most kMaxLeaf8 primitives may become one leaf slot. Node and
+// primitive costs are the relative VALU costs of one 8-slot box test and one
+// primitive test of the gfx950 kernels.
+class WideCollapse {
+ public:
+  static constexpr float kNodeCost = 1.0f;
+  static constexpr float kPrimCost = 1.0f;
+
+  WideCollapse(const std::vector<BNode>& bn, int32_t root) : bn_(bn) {
+    const size_t n = bn.size();
+    cost_.resize(n);
+    distK_.resize(n);
+    useDist_.resize(n);
+    leaf_.assign(n, 0);
+    count_.assign(n, 0);
+    first_.assign(n, 0);
+    // post-order over the binary tree
+    std::vector<std::pair<int32_t, bool>> st{{root, false}};
+    while (!st.empty()) {
+      auto [ni, done] = st.back();
+      st.pop_back();
+      const BNode& b = bn[ni];
+      if (b.count > 0 || b.left < 0) {
+        visitLeaf(ni);
+        continue;
+      }
+      if (!done) {
+        st.push_back({ni, true});
+        st.push_back({b.right, false});
+        st.push_back({b.left, false});
+        continue;
+      }
+      visitInner(ni);
+    }
+  }
+
+  bool isLeafSlot(int32_t ni) const { return leaf_[ni] != 0; }
+  int32_t first(int32_t ni) const { return first_[ni]; }
+  int32_t count(int32_t ni) const { return count_[ni]; }
+
+  // Slots of the 8-wide node made from binary node ni.
+  void nodeSlots(int32_t ni, std::vector<int32_t>& out) const {
+    out.clear();
+    const BNode& b = bn_[ni];
+    if (b.count > 0 || b.left < 0) {
+      out.push_back(ni);
+      return;
+    }
+    const int k = distK_[ni][8];
+    expand(b.left, k, out);
+    expand(b.right, 8 - k, out);
+  }
+
+ private:
+  const std::vector<BNode>& bn_;
+  std::vector<std::array<float, 9>> cost_;
+  std::vector<std::array<uint8_t, 9>> distK_;
+  std::vector<std::array<uint8_t, 9>> useDist_;
+  std::vector<uint8_t> leaf_;
+  std::vector<int32_t> count_, first_;
+
+  void visitLeaf(int32_t ni) {
+    const BNode& b = bn_[ni];
+    count_[ni] = b.count;
+    first_[ni] = b.first;
+    leaf_[ni] = 1;
+    const float c = areaOf(b.box) * static_cast<float>(b.count) * kPrimCost;
+    for (int i = 1; i <= 8; ++i) cost_[ni][i] = c;
+  }
+
+  void visitInner(int32_t ni) {
+    const BNode& b = bn_[ni];
+    const int32_t l = b.left, r = b.right;
+    count_[ni] = count_[l] + count_[r];
+    first_[ni] = std::min(first_[l], first_[r]);
+    const float A = areaOf(b.box);
+    float dist[9];
+    for (int j = 2; j <= 8; ++j) {
+      float best = std::numeric_limits<float>::infinity();
+      int bk = 1;
+      for (int k = 1; k < j; ++k) {
+        const float c = cost_[l][k] + cost_[r][j - k];
+        if (c < best) {
+          best = c;
+          bk = k;
+        }
+      }
+      dist[j] = best;
+      distK_[ni][j] = static_cast<uint8_t>(bk);
+    }
+    const float asNode = A * kNodeCost + dist[8];
+    const float asLeaf = count_[ni] <= kMaxLeaf8 ? A * static_cast<float>(count_[ni]) * kPrimCost
+                                                 : std::numeric_limits<float>::infinity();
+    leaf_[ni] = asLeaf <= asNode ? 1 : 0;
+    cost_[ni][1] = std::min(asLeaf, asNode);
+    for (int i = 2; i <= 8; ++i) {
+      useDist_[ni][i] = dist[i] < cost_[ni][i - 1] ? 1 : 0;
+      cost_[ni][i] = useDist_[ni][i] ? dist[i] : cost_[ni][i - 1];
+    }
+  }
+
+  void expand(int32_t ni, int i, std::vector<int32_t>& out) const {
+    const BNode& b = bn_[ni];
+    while (i > 1 && !(b.count > 0 || b.left < 0) && !useDist_[ni][i]) --i;
+    if (i <= 1 || b.count > 0 || b.left < 0) {
+      out.push_back(ni);
+      return;
+    }
+    const int k = distK_[ni][i];
+    expand(b.left, k, out);
+    expand(b.right, i - k, out);
+  }
+};
+
 int collapseBvh8(const std::vector<BNode>& bn, int32_t root, const std::vector<uint32_t>& order,
                  const std::vector<F4>& recs, HostScene& out, std::string& err) {
+  const WideCollapse wc(bn, root);
   struct Item {
     int32_t node;
     uint32_t idx;
@@ -489,25 +609,7 @@ int collapseBvh8(const std::vector<BNode>& bn, int32_t root, const std::vector<u
     const Item it = queue[qi];
     maxDepth = std::max(maxDepth, it.depth);
     std::vector<int32_t> ch;
-    if (bn[it.node].count > 0) {
-      ch.push_back(it.node);
-    } else {
-      ch.push_back(bn[it.node].left);
-      ch.push_back(bn[it.node].right);
-    }
-    while (ch.size() < 8) {
-      int best = -1;
-      float bestA = -1.0f;
-      for (size_t j = 0; j < ch.size(); ++j)
-        if (bn[ch[j]].count == 0 && areaOf(bn[ch[j]].box) > bestA) {
-          bestA = areaOf(bn[ch[j]].box);
-          best = static_cast<int>(j);
-        }
-      if (best < 0) break;
-      const int32_t c = ch[best];
-      ch[best] = bn[c].left;
-      ch.push_back(bn[c].right);
-    }
+    wc.nodeSlots(it.node, ch);
     Box nb = emptyBox();
     for (int32_t c : ch) growBox(nb, bn[c].box);
     // octant slots
@@ -560,22 +662,24 @@ int collapseBvh8(const std::vector<BNode>& bn, int32_t root, const std::vector<u
     for (int sl = 0; sl < 8; ++sl) {
       const int j = slotChild[sl];
       if (j < 0) continue;
-      const BNode& c = bn[ch[j]];
+      const int32_t cn = ch[j];
+      const BNode& c = bn[cn];
       for (int k = 0; k < 3; ++k) {
         const double lo = std::floor((static_cast<double>(c.box.lo[k]) - n.p[k]) / scale[k]);
         const double hi = std::ceil((static_cast<double>(c.box.hi[k]) - n.p[k]) / scale[k]);
         n.qlo[k][sl] = static_cast<uint8_t>(std::max(0.0, std::min(255.0, lo)));
         n.qhi[k][sl] = static_cast<uint8_t>(std::max(0.0, std::min(255.0, hi)));
       }
-      if (c.count > 0) {
-        if (c.count > kMaxLeaf8) {
-          err = "internal error: BVH8 leaf larger than 3 primitives";
+      if (wc.isLeafSlot(cn)) {
+        const int32_t cc = wc.count(cn), cf = wc.first(cn);
+        if (cc < 1 || cc > kMaxLeaf8) {
+          err = "internal error: BVH8 leaf slot must hold 1 to 3 primitives";
           return UBG_ERR_INTERNAL;
         }
         const uint32_t off = static_cast<uint32_t>(prims.size() / 3) - n.primBase;
-        const uint32_t unary = (1u << c.count) - 1u;
+        const uint32_t unary = (1u << cc) - 1u;
         n.meta[sl] = static_cast<uint8_t>((unary << 5) | off);
-        for (int32_t q = c.first; q < c.first + c.count; ++q)
+        for (int32_t q = cf; q < cf + cc; ++q)
           for (int k = 0; k < 3; ++k) prims.push_back(recs[3 * order[q] + k]);
       } else {
         n.imask |= static_cast<uint8_t>(1u << sl);
@@ -854,7 +958,8 @@ int buildHostScene(const ubg_scene_desc* d, int layout, HostScene& out, std::str
     return UBG_ERR_INVALID_ARGUMENT;
   }
   // ---- BVH ----
-  Builder bld(boxes, layout == kLayoutBvh8 ? kMaxLeaf8 : kMaxLeaf);
+  // BVH8: single-primitive binary leaves; the collapse groups them into leaf slots
+  Builder bld(boxes, layout == kLayoutBvh8 ? 1 : kMaxLeaf);
   const int32_t root = bld.run();
   const std::vector<BNode>& bn = bld.nodes();
   const std::vector<uint32_t>& order = bld.order();
