@@ -174,7 +174,7 @@ int ubg_context_create(int device_ordinal, ubg_context** out) {
       return fail(UBG_ERR_DEVICE, "hipEventCreate failed");
     }
   }
-  if (hipHostMalloc(reinterpret_cast<void**>(&c->hostCounters), ubg::kNumCounters * sizeof(unsigned long long),
+  if (hipHostMalloc(reinterpret_cast<void**>(&c->hostCounters), ubg::kCounterWords * sizeof(unsigned long long),
                     hipHostMallocDefault) != hipSuccess) {
     ubg_context_destroy(c);
     return fail(UBG_ERR_OUT_OF_MEMORY, "hipHostMalloc(counters) failed");
@@ -399,7 +399,7 @@ int ubg_render(ubg_context* ctx, const ubg_scene* scene, const ubg_render_params
   if (fr->ao_open) need(ctx->aoOpen, n * 4);
   const size_t outN = static_cast<size_t>(p->width) * p->height;
   need(ctx->outColor, outN * 16);
-  need(ctx->counters, ubg::kNumCounters * sizeof(unsigned long long));
+  need(ctx->counters, ubg::kCounterWords * sizeof(unsigned long long));
   if (r != UBG_OK) return r;
 
   ubg::FrameBufs b;
@@ -456,7 +456,7 @@ int ubg_render(ubg_context* ctx, const ubg_scene* scene, const ubg_render_params
       marks.push_back({kind, e0, e1});
     }
   };
-  ubg::launchZeroCounters(b.counters, ubg::kNumCounters, st);
+  ubg::launchZeroCounters(b.counters, ubg::kCounterWords, st);
   UBG_HIP(hipEventRecord(ctx->ev[0], st), "hipEventRecord");
   int nActive = npx;
   bool cancelled = false;
@@ -506,16 +506,19 @@ int ubg_render(ubg_context* ctx, const ubg_scene* scene, const ubg_render_params
   if (r == UBG_OK && fr->hit_id) r = copyOut(fr->hit_id, ctx->hitId.p, n * 4, fr->memory, st);
   if (r == UBG_OK && fr->ao_open) r = copyOut(fr->ao_open, ctx->aoOpen.p, n * 4, fr->memory, st);
   if (r != UBG_OK) return r;
-  UBG_HIP(hipMemcpyAsync(ctx->hostCounters, b.counters, ubg::kNumCounters * sizeof(unsigned long long),
+  UBG_HIP(hipMemcpyAsync(ctx->hostCounters, b.counters, ubg::kCounterWords * sizeof(unsigned long long),
                          hipMemcpyDeviceToHost, st),
           "hipMemcpy(counters)");
   UBG_HIP(hipStreamSynchronize(st), "frame sync");
+  // statistics: sum of the per-block-group partial counts
+  unsigned long long c[ubg::kNumCounters] = {0};
+  for (int g = 1; g <= ubg::kStatSpread; ++g)
+    for (int k = 0; k < ubg::kNumCounters; ++k) c[k] += ctx->hostCounters[g * ubg::kNumCounters + k];
   float msTrace = 0.0f, msPost = 0.0f;
   UBG_HIP(hipEventElapsedTime(&msTrace, ctx->ev[0], ctx->ev[1]), "hipEventElapsedTime");
   UBG_HIP(hipEventElapsedTime(&msPost, ctx->ev[1], ctx->ev[2]), "hipEventElapsedTime");
   fr->render_seconds = msTrace * 1e-3;
   fr->post_seconds = msPost * 1e-3;
-  const unsigned long long* c = ctx->hostCounters;
   fr->stats.primary_rays = c[ubg::kCntPrimary];
   fr->stats.shaded_hits = c[ubg::kCntShaded];
   fr->stats.ao_rays_traced = c[ubg::kCntAoTraced];

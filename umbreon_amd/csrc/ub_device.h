@@ -633,6 +633,25 @@ UBG_HD int trav8Prim(const SceneView& s, Trav8& t, uint32_t* primTests) {
   return (t.tHits == 0u && !trav8HasNodes(t)) ? -1 : 0;
 }
 
+// One primitive test of a closest-hit traversal: a strictly nearer hit
+// replaces `best` and shortens the ray. Returns -1 when nothing is left to
+// visit, else 0.
+template <bool kCount>
+UBG_HD int trav8PrimClosest(const SceneView& s, Trav8& t, Hit& best, bool& found, uint32_t* primTests) {
+  const uint32_t b = static_cast<uint32_t>(__builtin_ctz(t.tHits));
+  t.tHits &= t.tHits - 1u;
+  if (kCount) ++*primTests;
+  Hit h;
+  if (isectPrim(s, t.tBase + b, t.o, t.d, t.tnear, t.tmax, h)) {
+    if (!found || h.t < best.t) {
+      best = h;
+      t.tmax = h.t;
+      found = true;
+    }
+  }
+  return (t.tHits == 0u && !trav8HasNodes(t)) ? -1 : 0;
+}
+
 // One unit of any-hit work: a primitive test if one is pending, else a node
 // test. Returns 1 when a primitive occludes the ray, -1 when the traversal is
 // exhausted without a hit, 0 to continue.

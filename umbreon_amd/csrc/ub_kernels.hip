@@ -21,6 +21,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 
 #include "ub_device.h"
 #include "ub_kernels.h"
@@ -28,6 +29,8 @@
 namespace ubg {
 
 constexpr int kTraceBlock = 128;
+// Waves per SIMD the traversal kernels are register-limited to (<= 72 VGPRs).
+constexpr int kTraceWavesPerSimd = 7;
 
 // Dynamic LDS: the per-lane traversal stacks, sized at launch from the scene's
 // tree depth (BVH2: one u32 node index per entry; BVH8: a u32 base array and
@@ -37,6 +40,36 @@ extern __shared__ uint32_t ubgLds[];
 __host__ size_t stackLdsBytes(const SceneView& s, int block) {
   const int words = s.layout == kLayoutBvh8 ? 2 : 1;
   return static_cast<size_t>(words) * static_cast<size_t>(s.stackDepth > 0 ? s.stackDepth : 1) * block * 4u;
+}
+
+// Closest hit over the BVH8 for every lane of a wave at once: each
+// iteration the wave runs either primitive tests or node tests (whichever
+// more lanes are waiting for, primitives preferred at half the node lanes),
+// so the two code paths are never both executed by one iteration. Each lane
+// visits nodes and primitives in the same order as traverse8, so the result
+// (ties included) is the same.
+template <bool kCount>
+__device__ __forceinline__ bool traceClosestWave8(const SceneView& s, bool live, V3 o, V3 d, float tn, float tf,
+                                                  Hit& best, uint32_t* nv, uint32_t* nt) {
+  uint32_t* sb = ubgLds + threadIdx.x;
+  uint32_t* sh = sb + s.stackDepth * blockDim.x;
+  Trav8 t;
+  trav8Init(t, o, d, tn, tf);
+  bool found = false;
+  live = live && s.nNodes > 0;
+  while (__ballot(live) != 0ull) {
+    const bool wantPrim = live && t.tHits != 0u;
+    const int nPrim = __popcll(__ballot(wantPrim));
+    const int nNode = __popcll(__ballot(live && !wantPrim));
+    int r = 0;
+    if (2 * nPrim >= nNode) {
+      if (wantPrim) r = trav8PrimClosest<kCount>(s, t, best, found, nt);
+    } else {
+      if (live && !wantPrim) r = trav8Node<kCount>(s, t, sb, sh, blockDim.x, nv);
+    }
+    if (r != 0) live = false;
+  }
+  return found;
 }
 
 template <int kLayout, bool kAny, bool kCount>
@@ -53,9 +86,15 @@ constexpr uint32_t kFlagMesh = 1u << 24;
 constexpr uint32_t kFlagAo = 1u << 25;
 constexpr uint32_t kFlagShadow = 1u << 26;
 
+// This block's statistics group of the counter buffer.
+__device__ __forceinline__ unsigned long long* statGroup(unsigned long long* cnt) {
+  return cnt + kNumCounters * (1 + static_cast<int>(blockIdx.x % kStatSpread));
+}
+
+// Adds the number of lanes with `pred` to statistic `slot`.
 __device__ __forceinline__ void waveCount(unsigned long long* cnt, int slot, bool pred) {
   const unsigned long long m = __ballot(pred);
-  if (__lane_id() == 0 && m != 0ull) atomicAdd(&cnt[slot], (unsigned long long)__popcll(m));
+  if (__lane_id() == 0 && m != 0ull) atomicAdd(&statGroup(cnt)[slot], (unsigned long long)__popcll(m));
 }
 
 __device__ __forceinline__ uint32_t waveSum(uint32_t v) {
@@ -69,8 +108,8 @@ __device__ __forceinline__ uint32_t waveSum(uint32_t v) {
 __device__ __forceinline__ void addVisits(unsigned long long* cnt, int cls, uint32_t nv, uint32_t nt) {
   const uint32_t sv = waveSum(nv), st = waveSum(nt);
   if (__lane_id() == 0) {
-    atomicAdd(&cnt[kCntNodes + cls], (unsigned long long)sv);
-    atomicAdd(&cnt[kCntPrimTests + cls], (unsigned long long)st);
+    atomicAdd(&statGroup(cnt)[kCntNodes + cls], (unsigned long long)sv);
+    atomicAdd(&statGroup(cnt)[kCntPrimTests + cls], (unsigned long long)st);
   }
 }
 
@@ -139,23 +178,39 @@ __device__ __forceinline__ void finalizePixel(const FrameConst& fc, const FrameB
 }
 
 template <bool kCount, int kLayout>
-__global__ __launch_bounds__(kTraceBlock) void k_primary(FrameConst fc, SceneView s, FrameBufs b,
+__global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(kTraceWavesPerSimd))) void k_primary(FrameConst fc, SceneView s, FrameBufs b,
                                                          int iter, int nActive) {
   const int tid = blockIdx.x * blockDim.x + threadIdx.x;
-  const bool valid = tid < nActive;
+  bool valid;
   uint32_t pix = 0;
+  if (iter == 0) {
+    // first layer: each wave takes an 8x8 pixel tile (coherent rays)
+    const uint32_t tilesX = (static_cast<uint32_t>(fc.W) + 7u) >> 3;
+    const uint32_t wave = static_cast<uint32_t>(tid) >> 6, lane = static_cast<uint32_t>(tid) & 63u;
+    const uint32_t px = (wave % tilesX) * 8u + (lane & 7u), py = (wave / tilesX) * 8u + (lane >> 3);
+    valid = px < static_cast<uint32_t>(fc.W) && py < static_cast<uint32_t>(fc.H);
+    pix = py * static_cast<uint32_t>(fc.W) + px;
+  } else {
+    valid = tid < nActive;
+    if (valid) pix = b.activeIn[tid];
+  }
   bool hit = false;
   Hit h;
   V3 org = mk(0, 0, 0), rd = mk(0, 0, 1);
+  float tn = 0.0f, tf = 0.0f;
   uint32_t nv = 0, nt = 0;
   if (valid) {
-    pix = iter == 0 ? static_cast<uint32_t>(tid) : b.activeIn[tid];
     const uint32_t px = pix % static_cast<uint32_t>(fc.W), py = pix / static_cast<uint32_t>(fc.W);
     genRay(fc, px, py, org, rd);
-    float tn, tf;
     clipRange(fc, rd, tn, tf);
     if (iter > 0) tn = b.tnear[pix];
+  }
+  if (kLayout == kLayoutBvh8) {
+    hit = traceClosestWave8<kCount>(s, valid, org, rd, tn, tf, h, &nv, &nt);
+  } else if (valid) {
     hit = trace<kLayout, false, kCount>(s, org, rd, tn, tf, h, &nv, &nt);
+  }
+  if (valid) {
     if (iter == 0) {
       const float vz = hit ? h.t * dot(rd, mk(fc.dir[0], fc.dir[1], fc.dir[2])) : 0.0f;
       b.depth[pix] = hit ? h.t : 0.0f;
@@ -314,7 +369,7 @@ constexpr int kAoChunk = 256;
 constexpr int kAoRefill = 16;
 
 template <bool kCount>
-__global__ __launch_bounds__(kTraceBlock) void k_ao_persistent(FrameConst fc, SceneView s, FrameBufs b, int nMax) {
+__global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(kTraceWavesPerSimd))) void k_ao_persistent(FrameConst fc, SceneView s, FrameBufs b, int nMax) {
   const int nHits = min(nMax, static_cast<int>(b.counters[kCntHits]));
   const uint32_t n = static_cast<uint32_t>(fc.aoSamples);
   const uint64_t total = static_cast<uint64_t>(nHits) * n;
@@ -393,8 +448,8 @@ __global__ __launch_bounds__(kTraceBlock) void k_ao_persistent(FrameConst fc, Sc
   }
   const uint32_t tr = waveSum(nTraced), sa = waveSum(nSampled);
   if (__lane_id() == 0) {
-    if (tr) atomicAdd(&b.counters[kCntAoTraced], (unsigned long long)tr);
-    if (sa) atomicAdd(&b.counters[kCntAoSampled], (unsigned long long)sa);
+    if (tr) atomicAdd(&statGroup(b.counters)[kCntAoTraced], (unsigned long long)tr);
+    if (sa) atomicAdd(&statGroup(b.counters)[kCntAoSampled], (unsigned long long)sa);
   }
   if (kCount) addVisits(b.counters, 1, nv, nt);
 }
@@ -546,7 +601,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_resolve(FrameConst fc, SceneVie
   waveCount(b.counters, kCntCapped, capped);
   {
     const uint32_t sh = waveSum(nShadow);
-    if (__lane_id() == 0 && sh) atomicAdd(&b.counters[kCntShadow], (unsigned long long)sh);
+    if (__lane_id() == 0 && sh) atomicAdd(&statGroup(b.counters)[kCntShadow], (unsigned long long)sh);
   }
   if (kCount) addVisits(b.counters, 2, nv, nt);
   const unsigned long long m = __ballot(cont);
@@ -682,10 +737,25 @@ void launchResetQueues(unsigned long long* c, hipStream_t st) {
     }                                                                                           \
   } while (0)
 
+// Compute units of the current device (cached per device).
+static int computeUnits() {
+  static std::atomic<int> cu[64];
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
+  int n = cu[dev].load(std::memory_order_relaxed);
+  if (n == 0) {
+    if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n <= 0) n = 256;
+    cu[dev].store(n, std::memory_order_relaxed);
+  }
+  return n;
+}
+
 void launchPrimary(const FrameConst& fc, const SceneView& s, const FrameBufs& b, int iter,
                    int nActive, bool countVisits, hipStream_t st) {
   if (nActive <= 0) return;
-  UBG_DISPATCH(k_primary, blocksFor(nActive, kTraceBlock), countVisits, fc, s, b, iter, nActive);
+  // the first layer runs one 64-lane wave per 8x8 tile of the hi-res grid
+  const int64_t n = iter == 0 ? static_cast<int64_t>((fc.W + 7) / 8) * ((fc.H + 7) / 8) * 64 : nActive;
+  UBG_DISPATCH(k_primary, blocksFor(n, kTraceBlock), countVisits, fc, s, b, iter, nActive);
 }
 void launchAo(const FrameConst& fc, const SceneView& s, const FrameBufs& b, int nMax,
               bool countVisits, hipStream_t st) {
@@ -693,7 +763,8 @@ void launchAo(const FrameConst& fc, const SceneView& s, const FrameBufs& b, int 
   if (s.layout == kLayoutBvh8) {
     // persistent grid: enough waves to fill every CU, no more than the work
     const int64_t rays = (int64_t)nMax * fc.aoSamples;
-    const int64_t blocks = std::min<int64_t>(blocksFor(rays, kTraceBlock), 256 * 12);
+    const int64_t resident = (int64_t)computeUnits() * 4 * kTraceWavesPerSimd * 64 / kTraceBlock;
+    const int64_t blocks = std::min<int64_t>(blocksFor(rays, kTraceBlock), resident);
     const size_t lds = stackLdsBytes(s, kTraceBlock);
     if (countVisits)
       hipLaunchKernelGGL(k_ao_persistent<true>, dim3(blocks), dim3(kTraceBlock), lds, st, fc, s, b, nMax);
@@ -767,6 +838,21 @@ extern "C" int ubgx_host_query(const ubg_scene_desc* desc, int32_t bvh_layout, i
                          : traverse2<false, true>(v, o, d, r.tnear, r.tfar, h, s0.data(), 1, &nv, &nt);
       visits += nv;
       tests += nt;
+      if (layout == kLayoutBvh8 && v.nNodes > 0) {
+        // the resumable closest-hit steps (as the kernels run them) must find
+        // the same hit as the one-shot traversal
+        Trav8 t;
+        trav8Init(t, o, d, r.tnear, r.tfar);
+        Hit h2{};
+        bool f2 = false;
+        uint32_t a = 0, b = 0;
+        for (;;) {
+          const int rs = t.tHits != 0u ? trav8PrimClosest<false>(v, t, h2, f2, &b)
+                                       : trav8Node<false>(v, t, s0.data(), s1.data(), 1, &a);
+          if (rs != 0) break;
+        }
+        if (f2 != f || (f && (f2u(h2.t) != f2u(h.t) || h2.bits != h.bits))) return UBG_ERR_INTERNAL;
+      }
       ubg_hit& out = hits[i];
       if (f) {
         out.t = h.t;

@@ -109,7 +109,14 @@ struct FrameConst {
   DevLight lights[kMaxLights];
 };
 
-// Frame-time counters (u64), one slot each.
+// Frame-time counters (u64). The counter buffer holds 1 + kStatSpread groups
+// of kNumCounters slots, each group on its own 128-byte line: group 0 holds
+// the queue counters the kernels read back (hit and continuation queue fills,
+// the AO work cursor); the statistics (ray counts, visit counts) are added by
+// each wave into group 1 + (block % kStatSpread) and summed on the host, so no
+// single address takes an atomic from every wave of a launch.
+constexpr int kStatSpread = 64;
+
 enum Counter {
   kCntPrimary = 0,
   kCntShaded = 1,
@@ -124,5 +131,7 @@ enum Counter {
   kCntAoNext = 14,    // persistent AO kernel: next AO ray to hand out
   kNumCounters = 16
 };
+
+constexpr int kCounterWords = kNumCounters * (1 + kStatSpread);
 
 }  // namespace ubg

@@ -14,7 +14,9 @@ extern "C" {
 #endif
 
 /* hits and/or occ may be NULL. For BVH8, occ[i] is 2 when the one-shot and
- * the resumable (per-step) any-hit traversals disagree. stats (may be NULL) receives
+ * the resumable (per-step) any-hit traversals disagree; the call returns
+ * UBG_ERR_INTERNAL when the resumable closest-hit steps find another hit than
+ * the one-shot closest-hit traversal. stats (may be NULL) receives
  * [0] node visits, [1] primitive tests (closest-hit queries),
  * [2] tree levels, [3] node count, [4] stack entries, [5] node bytes. */
 int ubgx_host_query(const ubg_scene_desc* desc, int32_t bvh_layout, int64_t n_rays, const ubg_ray* rays,
