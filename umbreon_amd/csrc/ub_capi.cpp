@@ -382,14 +382,17 @@ int ubg_render(ubg_context* ctx, const ubg_scene* scene, const ubg_render_params
   auto need = [&](DevBuf& b, size_t bytes) {
     if (r == UBG_OK) r = ensure(b, bytes);
   };
-  need(ctx->hitPix, n * 4);
-  need(ctx->hitT, n * 4);
-  need(ctx->hitO, n * 16);
-  need(ctx->hitN, n * 16);
-  need(ctx->hitC, n * 16);
-  need(ctx->hitAo, n * 4);
-  need(ctx->activeA, n * 4);
-  need(ctx->activeB, n * 4);
+  // queues: kQueueParts partitions of partCap slots
+  const int partCap = ubg::queuePartCap(W, H);
+  const size_t qn = static_cast<size_t>(ubg::kQueueParts) * static_cast<size_t>(partCap);
+  need(ctx->hitPix, qn * 4);
+  need(ctx->hitT, qn * 4);
+  need(ctx->hitO, qn * 16);
+  need(ctx->hitN, qn * 16);
+  need(ctx->hitC, qn * 16);
+  need(ctx->hitAo, qn * 4);
+  need(ctx->activeA, qn * 4);
+  need(ctx->activeB, qn * 4);
   need(ctx->acc, n * 16);
   need(ctx->tnear, n * 4);
   need(ctx->hiColor, n * 16);
@@ -419,6 +422,7 @@ int ubg_render(ubg_context* ctx, const ubg_scene* scene, const ubg_render_params
   b.hitId = fr->hit_id ? static_cast<uint32_t*>(ctx->hitId.p) : nullptr;
   b.aoOpen = fr->ao_open ? static_cast<float*>(ctx->aoOpen.p) : nullptr;
   b.counters = static_cast<unsigned long long*>(ctx->counters.p);
+  b.partCap = partCap;
 
   fr->cancelled = 0;
   fr->layers = 0;
@@ -458,7 +462,8 @@ int ubg_render(ubg_context* ctx, const ubg_scene* scene, const ubg_render_params
   };
   ubg::launchZeroCounters(b.counters, ubg::kCounterWords, st);
   UBG_HIP(hipEventRecord(ctx->ev[0], st), "hipEventRecord");
-  int nActive = npx;
+  int64_t nActive = npx;
+  ubg::PartFill fill{};  // continuation-queue fills the next layer consumes
   bool cancelled = false;
   for (int iter = 0; iter < fc.maxIters && nActive > 0; ++iter) {
     if (p->cancel_flag && *p->cancel_flag) {
@@ -467,18 +472,21 @@ int ubg_render(ubg_context* ctx, const ubg_scene* scene, const ubg_render_params
     }
     ++fr->layers;
     ubg::launchResetQueues(b.counters, st);
-    timed(UBG_KERNEL_PRIMARY, [&] { ubg::launchPrimary(fc, scene->view, b, iter, nActive, countVisits, st); });
-    if (fc.aoSamples > 0)
-      timed(UBG_KERNEL_AO, [&] { ubg::launchAo(fc, scene->view, b, nActive, countVisits, st); });
-    timed(UBG_KERNEL_RESOLVE, [&] { ubg::launchResolve(fc, scene->view, b, iter, nActive, countVisits, st); });
+    timed(UBG_KERNEL_PRIMARY, [&] { ubg::launchPrimary(fc, scene->view, b, iter, fill, countVisits, st); });
+    if (fc.aoSamples > 0) timed(UBG_KERNEL_AO, [&] { ubg::launchAo(fc, scene->view, b, countVisits, st); });
+    timed(UBG_KERNEL_RESOLVE, [&] { ubg::launchResolve(fc, scene->view, b, iter, countVisits, st); });
     UBG_HIP(hipGetLastError(), "kernel launch");
     if (iter + 1 >= fc.maxIters) break;
-    // One host round trip per transparency layer: the continuation count.
-    UBG_HIP(hipMemcpyAsync(ctx->hostCounters, b.counters, ubg::kNumCounters * sizeof(unsigned long long),
+    // One host round trip per transparency layer: the continuation fills.
+    UBG_HIP(hipMemcpyAsync(ctx->hostCounters, b.counters, ubg::kCounterWords * sizeof(unsigned long long),
                            hipMemcpyDeviceToHost, st),
             "hipMemcpy(counters)");
     UBG_HIP(hipStreamSynchronize(st), "layer sync");
-    nActive = static_cast<int>(ctx->hostCounters[ubg::kCntActive]);
+    nActive = 0;
+    for (int q = 0; q < ubg::kQueueParts; ++q) {
+      fill.n[q] = static_cast<uint32_t>(ctx->hostCounters[ubg::kActiveFill + ubg::kLineWords * q]);
+      nActive += fill.n[q];
+    }
     std::swap(b.activeIn, b.activeOut);
   }
   UBG_HIP(hipEventRecord(ctx->ev[1], st), "hipEventRecord");
@@ -512,7 +520,7 @@ int ubg_render(ubg_context* ctx, const ubg_scene* scene, const ubg_render_params
   UBG_HIP(hipStreamSynchronize(st), "frame sync");
   // statistics: sum of the per-block-group partial counts
   unsigned long long c[ubg::kNumCounters] = {0};
-  for (int g = 1; g <= ubg::kStatSpread; ++g)
+  for (int g = 0; g < ubg::kStatSpread; ++g)
     for (int k = 0; k < ubg::kNumCounters; ++k) c[k] += ctx->hostCounters[g * ubg::kNumCounters + k];
   float msTrace = 0.0f, msPost = 0.0f;
   UBG_HIP(hipEventElapsedTime(&msTrace, ctx->ev[0], ctx->ev[1]), "hipEventElapsedTime");

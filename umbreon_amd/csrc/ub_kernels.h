@@ -9,7 +9,11 @@
 namespace ubg {
 
 // Device buffers one frame's passes read and write (all owned by the context
-// except the optional user outputs). Capacity = hi-res pixel count.
+// except the optional user outputs). The hit queue and the continuation
+// queue are split into kQueueParts partitions of partCap slots each
+// (partition p = slots [p * partCap, (p + 1) * partCap)); partCap is a
+// multiple of the trace block size and kQueueParts * partCap covers every
+// thread of the first primary launch.
 struct FrameBufs {
   // hit queue (structure of arrays, filled by the primary pass)
   uint32_t* hitPix;
@@ -30,18 +34,27 @@ struct FrameBufs {
   uint32_t* hitId;    // may be null
   float* aoOpen;      // may be null
   unsigned long long* counters;
+  int32_t partCap;
+};
+
+// Queue slots per partition for a W x H hi-res grid.
+int queuePartCap(int W, int H);
+
+// Per-partition fills of the continuation queue a layer consumes.
+struct PartFill {
+  uint32_t n[kQueueParts];
 };
 
 void launchZeroCounters(unsigned long long* c, int n, hipStream_t st);
 void launchResetQueues(unsigned long long* c, hipStream_t st);
+// iter == 0 traces every hi-res pixel; later layers trace the continuation
+// queue whose partition fills are `fill` (read back by the host).
 void launchPrimary(const FrameConst& fc, const SceneView& s, const FrameBufs& b, int iter,
-                   int nActive, bool countVisits, hipStream_t st);
-// nMax bounds the hit-queue fill; the kernels read the exact fill from
-// counters[kCntHits] on the device, so no host round trip is needed.
-void launchAo(const FrameConst& fc, const SceneView& s, const FrameBufs& b, int nMax,
-              bool countVisits, hipStream_t st);
-void launchResolve(const FrameConst& fc, const SceneView& s, const FrameBufs& b, int iter,
-                   int nMax, bool countVisits, hipStream_t st);
+                   const PartFill& fill, bool countVisits, hipStream_t st);
+// AO and resolve read the hit-queue fills on the device (no host round trip).
+void launchAo(const FrameConst& fc, const SceneView& s, const FrameBufs& b, bool countVisits, hipStream_t st);
+void launchResolve(const FrameConst& fc, const SceneView& s, const FrameBufs& b, int iter, bool countVisits,
+                   hipStream_t st);
 void launchPost(const float4* hi, int W, int H, int ss, float gamma, float4* out, hipStream_t st);
 void launchQuery(const SceneView& s, const void* rays, int64_t n, void* hits, uint8_t* occ,
                  bool any, hipStream_t st);

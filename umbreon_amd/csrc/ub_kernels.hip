@@ -88,7 +88,25 @@ constexpr uint32_t kFlagShadow = 1u << 26;
 
 // This block's statistics group of the counter buffer.
 __device__ __forceinline__ unsigned long long* statGroup(unsigned long long* cnt) {
-  return cnt + kNumCounters * (1 + static_cast<int>(blockIdx.x % kStatSpread));
+  return cnt + kNumCounters * static_cast<int>(blockIdx.x % kStatSpread);
+}
+
+// Fill of partition p of the queue whose fill counters start at `fillBase`.
+__device__ __forceinline__ uint32_t queueFill(const unsigned long long* cnt, int fillBase, int p) {
+  return static_cast<uint32_t>(cnt[fillBase + kLineWords * p]);
+}
+
+// Appends the lanes with `pred` to partition (block % kQueueParts) of a queue
+// (one atomic per wave); returns the lane's slot, meaningful where pred.
+__device__ __forceinline__ uint32_t queueAppend(unsigned long long* cnt, int fillBase, int partCap, bool pred) {
+  const unsigned long long m = __ballot(pred);
+  const uint32_t p = blockIdx.x % kQueueParts;
+  uint32_t base = 0;
+  if (__lane_id() == 0 && m != 0ull)
+    base = static_cast<uint32_t>(atomicAdd(&cnt[fillBase + kLineWords * p], (unsigned long long)__popcll(m)));
+  base = __shfl(base, 0, 64);
+  return p * static_cast<uint32_t>(partCap) + base +
+         static_cast<uint32_t>(__popcll(m & ((1ull << __lane_id()) - 1ull)));
 }
 
 // Adds the number of lanes with `pred` to statistic `slot`.
@@ -179,7 +197,7 @@ __device__ __forceinline__ void finalizePixel(const FrameConst& fc, const FrameB
 
 template <bool kCount, int kLayout>
 __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(kTraceWavesPerSimd))) void k_primary(FrameConst fc, SceneView s, FrameBufs b,
-                                                         int iter, int nActive) {
+                                                         int iter, PartFill fill, int partBlocks) {
   const int tid = blockIdx.x * blockDim.x + threadIdx.x;
   bool valid;
   uint32_t pix = 0;
@@ -191,8 +209,11 @@ __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(kTr
     valid = px < static_cast<uint32_t>(fc.W) && py < static_cast<uint32_t>(fc.H);
     pix = py * static_cast<uint32_t>(fc.W) + px;
   } else {
-    valid = tid < nActive;
-    if (valid) pix = b.activeIn[tid];
+    // later layers: partBlocks blocks per continuation-queue partition
+    const uint32_t p = blockIdx.x / static_cast<uint32_t>(partBlocks);
+    const uint32_t local = (blockIdx.x % static_cast<uint32_t>(partBlocks)) * blockDim.x + threadIdx.x;
+    valid = local < fill.n[p];
+    if (valid) pix = b.activeIn[p * static_cast<uint32_t>(b.partCap) + local];
   }
   bool hit = false;
   Hit h;
@@ -226,14 +247,9 @@ __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(kTr
   }
   waveCount(b.counters, kCntPrimary, valid);
   if (kCount) addVisits(b.counters, 0, nv, nt);
-  // Compaction: one atomic per wave reserves the wave's slots.
-  const unsigned long long m = __ballot(hit);
-  uint32_t base = 0;
-  if (__lane_id() == 0 && m != 0ull)
-    base = static_cast<uint32_t>(atomicAdd(&b.counters[kCntHits], (unsigned long long)__popcll(m)));
-  base = __shfl(base, 0, 64);
+  // Compaction into the hit queue: one atomic per wave reserves its slots.
+  const uint32_t slot = queueAppend(b.counters, kHitFill, b.partCap, hit);
   if (!hit) return;
-  const uint32_t slot = base + static_cast<uint32_t>(__popcll(m & ((1ull << __lane_id()) - 1ull)));
 
   // ---- hit setup (hit_shader.hpp:147-316) ----
   const uint32_t kind = h.bits >> kKindShift;
@@ -305,13 +321,14 @@ __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(kTr
 
 template <bool kCount, int kLayout>
 __global__ __launch_bounds__(kTraceBlock) void k_ao(FrameConst fc, SceneView s, FrameBufs b,
-                                                    int nMax) {
-  const int nHits = min(nMax, static_cast<int>(b.counters[kCntHits]));
+                                                    int unused) {
+  (void)unused;
   const int lanes = fc.aoLanes;
   const int tid = blockIdx.x * blockDim.x + threadIdx.x;
-  const int hi = tid / lanes;
+  const int hi = tid / lanes;  // hit-queue slot
   const int j = tid % lanes;
-  const bool valid = hi < nHits;
+  const int part = hi / b.partCap;
+  const bool valid = part < kQueueParts && static_cast<uint32_t>(hi % b.partCap) < queueFill(b.counters, kHitFill, part);
   float4 O4 = make_float4(0, 0, 0, 0), N4 = make_float4(0, 0, 1, 0);
   uint32_t pix = 0;
   bool on = false;
@@ -355,31 +372,35 @@ __global__ __launch_bounds__(kTraceBlock) void k_ao(FrameConst fc, SceneView s, 
   if (valid && j == 0) b.hitAo[hi] = occ;
 }
 
-// Persistent AO over the compressed BVH8. A wave takes AO rays (hit, sample)
-// in chunks of kAoChunk from a global counter (one atomic per chunk) and
-// hands them to its lanes from a wave-uniform cursor; each lane advances its
-// own traversal one node or one primitive per iteration and takes the next
-// ray once its own has finished, so lanes do not idle behind the longest ray
-// of their wave. Refills wait until at least kAoRefill lanes are idle (or the
-// wave would otherwise stall), and each iteration runs one kind of step for
-// the whole wave (node tests or primitive tests, by lane-count vote).
-// Occluded rays add 1 to their hit's count: integer atomics, so the result
-// does not depend on the order.
+// Persistent AO over the compressed BVH8. The AO rays of hit-queue
+// partition p are (hit, sample) pairs 0 .. fill(p) * aoSamples - 1; a wave
+// takes them in chunks of kAoChunk from partition p's cursor (one atomic per
+// chunk), starting at partition (wave % kQueueParts) and moving on to the
+// next partition when one is used up, and hands them to its lanes from a
+// wave-uniform cursor. Each lane advances its own traversal one node or one
+// primitive per iteration and takes the next ray once its own has finished,
+// so lanes do not idle behind the longest ray of their wave. Refills wait
+// until at least kAoRefill lanes are idle (or the wave would otherwise
+// stall), and each iteration runs one kind of step for the whole wave (node
+// tests or primitive tests, by lane-count vote). Occluded rays add 1 to their
+// hit's count: integer atomics, so the result does not depend on the order.
 constexpr int kAoChunk = 256;
 constexpr int kAoRefill = 16;
 
 template <bool kCount>
-__global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(kTraceWavesPerSimd))) void k_ao_persistent(FrameConst fc, SceneView s, FrameBufs b, int nMax) {
-  const int nHits = min(nMax, static_cast<int>(b.counters[kCntHits]));
+__global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(kTraceWavesPerSimd))) void k_ao_persistent(
+    FrameConst fc, SceneView s, FrameBufs b) {
   const uint32_t n = static_cast<uint32_t>(fc.aoSamples);
-  const uint64_t total = static_cast<uint64_t>(nHits) * n;
   uint32_t* sb = ubgLds + threadIdx.x;
   uint32_t* sh = sb + s.stackDepth * blockDim.x;
   const int stride = blockDim.x;
   const uint64_t laneBit = 1ull << __lane_id();
-  // wave-uniform cursor into the current chunk
+  // wave-uniform: current partition and cursor into the current chunk
+  uint32_t part = ((blockIdx.x * blockDim.x + threadIdx.x) >> 6) % kQueueParts;
+  uint32_t chunkPart = part;
+  int tried = 0;
   uint64_t next = 0, end = 0;
-  bool drained = false;  // the global counter has passed `total`
+  bool drained = false;  // every partition is used up
   bool active = false;
   uint32_t hitIdx = 0;
   uint32_t nTraced = 0, nSampled = 0, nv = 0, nt = 0;
@@ -390,15 +411,26 @@ __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(kTr
     const unsigned long long idle = ~act;
     const int nIdle = __popcll(idle);
     if (!drained && (nIdle >= kAoRefill || act == 0ull)) {
-      if (next == end) {
-        uint64_t base = 0;
-        if (__lane_id() == 0) base = atomicAdd(&b.counters[kCntAoNext], (unsigned long long)kAoChunk);
+      while (next == end) {
+        const uint64_t len = static_cast<uint64_t>(queueFill(b.counters, kHitFill, part)) * n;
+        unsigned long long* cur = &b.counters[kAoCursor + kLineWords * part];
+        uint64_t base = len;
+        if (__lane_id() == 0) {
+          // a plain read first: a used-up partition costs no atomic
+          const uint64_t seen = __hip_atomic_load(cur, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          if (seen < len) base = atomicAdd(cur, (unsigned long long)kAoChunk);
+        }
         base = __shfl(base, 0, 64);
-        if (base >= total) {
-          drained = true;
-        } else {
+        if (base < len) {
           next = base;
-          end = min(base + static_cast<uint64_t>(kAoChunk), total);
+          end = min(base + static_cast<uint64_t>(kAoChunk), len);
+          chunkPart = part;
+        } else {
+          if (++tried == kQueueParts) {
+            drained = true;
+            break;
+          }
+          part = (part + 1u) % kQueueParts;
         }
       }
       if (next < end) {
@@ -407,8 +439,9 @@ __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(kTr
         const uint64_t taken = min(static_cast<uint64_t>(nIdle), end - next);
         next += taken;
         if (!active && rank < taken) {
-          hitIdx = static_cast<uint32_t>(idx / n);
-          const uint32_t si = static_cast<uint32_t>(idx - static_cast<uint64_t>(hitIdx) * n);
+          const uint32_t local = static_cast<uint32_t>(idx / n);
+          const uint32_t si = static_cast<uint32_t>(idx - static_cast<uint64_t>(local) * n);
+          hitIdx = chunkPart * static_cast<uint32_t>(b.partCap) + local;
           const float4 N4 = b.hitN[hitIdx];
           if (__float_as_uint(N4.w) & kFlagAo) {
             const float4 O4 = b.hitO[hitIdx];
@@ -456,10 +489,11 @@ __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(kTr
 
 template <bool kCount, int kLayout>
 __global__ __launch_bounds__(kTraceBlock) void k_resolve(FrameConst fc, SceneView s, FrameBufs b,
-                                                         int iter, int nMax) {
-  const int nHits = min(nMax, static_cast<int>(b.counters[kCntHits]));
-  const int tid = blockIdx.x * blockDim.x + threadIdx.x;
-  const bool valid = tid < nHits;
+                                                         int iter, int unused) {
+  (void)unused;
+  const int tid = blockIdx.x * blockDim.x + threadIdx.x;  // hit-queue slot
+  const int part = tid / b.partCap;
+  const bool valid = part < kQueueParts && static_cast<uint32_t>(tid % b.partCap) < queueFill(b.counters, kHitFill, part);
   uint32_t nShadow = 0, nv = 0, nt = 0;
   bool capped = false, cont = false;
   uint32_t pix = 0;
@@ -604,13 +638,8 @@ __global__ __launch_bounds__(kTraceBlock) void k_resolve(FrameConst fc, SceneVie
     if (__lane_id() == 0 && sh) atomicAdd(&statGroup(b.counters)[kCntShadow], (unsigned long long)sh);
   }
   if (kCount) addVisits(b.counters, 2, nv, nt);
-  const unsigned long long m = __ballot(cont);
-  uint32_t base = 0;
-  if (__lane_id() == 0 && m != 0ull)
-    base = static_cast<uint32_t>(atomicAdd(&b.counters[kCntActive], (unsigned long long)__popcll(m)));
-  base = __shfl(base, 0, 64);
+  const uint32_t slot = queueAppend(b.counters, kActiveFill, b.partCap, cont);
   if (!cont) return;
-  const uint32_t slot = base + static_cast<uint32_t>(__popcll(m & ((1ull << __lane_id()) - 1ull)));
   b.activeOut[slot] = pix;
   b.acc[pix] = acc;
   b.tnear[pix] = tnext;
@@ -703,12 +732,9 @@ __global__ void k_zero(unsigned long long* c, int n) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) c[i] = 0ull;
 }
+// Zeroes the hit-queue fills, continuation-queue fills and AO cursors.
 __global__ void k_reset_queues(unsigned long long* c) {
-  if (threadIdx.x == 0) {
-    c[kCntHits] = 0ull;
-    c[kCntActive] = 0ull;
-    c[kCntAoNext] = 0ull;
-  }
+  if (threadIdx.x < 3 * kQueueParts) c[kHitFill + kLineWords * threadIdx.x] = 0ull;
 }
 
 static inline unsigned blocksFor(int64_t n, int bs) {
@@ -750,35 +776,47 @@ static int computeUnits() {
   return n;
 }
 
-void launchPrimary(const FrameConst& fc, const SceneView& s, const FrameBufs& b, int iter,
-                   int nActive, bool countVisits, hipStream_t st) {
-  if (nActive <= 0) return;
-  // the first layer runs one 64-lane wave per 8x8 tile of the hi-res grid
-  const int64_t n = iter == 0 ? static_cast<int64_t>((fc.W + 7) / 8) * ((fc.H + 7) / 8) * 64 : nActive;
-  UBG_DISPATCH(k_primary, blocksFor(n, kTraceBlock), countVisits, fc, s, b, iter, nActive);
+int queuePartCap(int W, int H) {
+  const int64_t threads = static_cast<int64_t>((W + 7) / 8) * ((H + 7) / 8) * 64;
+  const int64_t blocks = (threads + kTraceBlock - 1) / kTraceBlock;
+  return static_cast<int>((blocks + kQueueParts - 1) / kQueueParts * kTraceBlock);
 }
-void launchAo(const FrameConst& fc, const SceneView& s, const FrameBufs& b, int nMax,
-              bool countVisits, hipStream_t st) {
-  if (nMax <= 0 || fc.aoSamples <= 0) return;
+
+void launchPrimary(const FrameConst& fc, const SceneView& s, const FrameBufs& b, int iter,
+                   const PartFill& fill, bool countVisits, hipStream_t st) {
+  if (iter == 0) {
+    // one 64-lane wave per 8x8 tile of the hi-res grid
+    const int64_t n = static_cast<int64_t>((fc.W + 7) / 8) * ((fc.H + 7) / 8) * 64;
+    UBG_DISPATCH(k_primary, blocksFor(n, kTraceBlock), countVisits, fc, s, b, iter, fill, 1);
+    return;
+  }
+  uint32_t mx = 0;
+  for (int p = 0; p < kQueueParts; ++p) mx = std::max(mx, fill.n[p]);
+  if (mx == 0) return;
+  const int partBlocks = static_cast<int>(blocksFor(mx, kTraceBlock));
+  UBG_DISPATCH(k_primary, partBlocks * kQueueParts, countVisits, fc, s, b, iter, fill, partBlocks);
+}
+void launchAo(const FrameConst& fc, const SceneView& s, const FrameBufs& b, bool countVisits, hipStream_t st) {
+  if (fc.aoSamples <= 0) return;
+  const int64_t slots = static_cast<int64_t>(kQueueParts) * b.partCap;
   if (s.layout == kLayoutBvh8) {
-    // persistent grid: enough waves to fill every CU, no more than the work
-    const int64_t rays = (int64_t)nMax * fc.aoSamples;
+    // persistent grid: every wave slot of the chip, no more than the work
+    const int64_t rays = slots * fc.aoSamples;
     const int64_t resident = (int64_t)computeUnits() * 4 * kTraceWavesPerSimd * 64 / kTraceBlock;
     const int64_t blocks = std::min<int64_t>(blocksFor(rays, kTraceBlock), resident);
     const size_t lds = stackLdsBytes(s, kTraceBlock);
     if (countVisits)
-      hipLaunchKernelGGL(k_ao_persistent<true>, dim3(blocks), dim3(kTraceBlock), lds, st, fc, s, b, nMax);
+      hipLaunchKernelGGL(k_ao_persistent<true>, dim3(blocks), dim3(kTraceBlock), lds, st, fc, s, b);
     else
-      hipLaunchKernelGGL(k_ao_persistent<false>, dim3(blocks), dim3(kTraceBlock), lds, st, fc, s, b, nMax);
+      hipLaunchKernelGGL(k_ao_persistent<false>, dim3(blocks), dim3(kTraceBlock), lds, st, fc, s, b);
     return;
   }
-  const int64_t n = (int64_t)nMax * fc.aoLanes;
-  UBG_DISPATCH(k_ao, blocksFor(n, kTraceBlock), countVisits, fc, s, b, nMax);
+  UBG_DISPATCH(k_ao, blocksFor(slots * fc.aoLanes, kTraceBlock), countVisits, fc, s, b, 0);
 }
-void launchResolve(const FrameConst& fc, const SceneView& s, const FrameBufs& b, int iter,
-                   int nMax, bool countVisits, hipStream_t st) {
-  if (nMax <= 0) return;
-  UBG_DISPATCH(k_resolve, blocksFor(nMax, kTraceBlock), countVisits, fc, s, b, iter, nMax);
+void launchResolve(const FrameConst& fc, const SceneView& s, const FrameBufs& b, int iter, bool countVisits,
+                   hipStream_t st) {
+  const int64_t slots = static_cast<int64_t>(kQueueParts) * b.partCap;
+  UBG_DISPATCH(k_resolve, blocksFor(slots, kTraceBlock), countVisits, fc, s, b, iter, 0);
 }
 void launchPost(const float4* hi, int W, int H, int ss, float gamma, float4* out, hipStream_t st) {
   const int n = (W / ss) * (H / ss);

@@ -109,13 +109,20 @@ struct FrameConst {
   DevLight lights[kMaxLights];
 };
 
-// Frame-time counters (u64). The counter buffer holds 1 + kStatSpread groups
-// of kNumCounters slots, each group on its own 128-byte line: group 0 holds
-// the queue counters the kernels read back (hit and continuation queue fills,
-// the AO work cursor); the statistics (ray counts, visit counts) are added by
-// each wave into group 1 + (block % kStatSpread) and summed on the host, so no
-// single address takes an atomic from every wave of a launch.
+// Frame-time counters (u64), in one buffer of kCounterWords words:
+// * statistics (ray and visit counts): kStatSpread groups of kNumCounters
+//   slots, each group on its own 128-byte line; a wave adds into group
+//   (block % kStatSpread) and the host sums the groups, so no single address
+//   takes an atomic from every wave of a launch;
+// * queue fills: the hit queue and the continuation queue are each split
+//   into kQueueParts partitions (a producing block appends to partition
+//   block % kQueueParts), each fill counter on its own line;
+// * AO work cursors, one per hit-queue partition, each on its own line.
+// Same-address device atomics serialize at the memory side (they come from
+// all XCDs), so every counter that many waves touch is spread this way.
 constexpr int kStatSpread = 64;
+constexpr int kQueueParts = 16;
+constexpr int kLineWords = 16;  // u64 words per 128-byte line
 
 enum Counter {
   kCntPrimary = 0,
@@ -124,14 +131,15 @@ enum Counter {
   kCntAoSampled = 3,
   kCntShadow = 4,
   kCntCapped = 5,
-  kCntHits = 6,      // hit-queue fill of the current pass
-  kCntActive = 7,    // continuation-queue fill of the current pass
   kCntNodes = 8,      // node visits per ray class: 8 primary, 9 AO, 10 shadow
   kCntPrimTests = 11, // primitive tests per ray class: 11, 12, 13
-  kCntAoNext = 14,    // persistent AO kernel: next AO ray to hand out
   kNumCounters = 16
 };
 
-constexpr int kCounterWords = kNumCounters * (1 + kStatSpread);
+// word offsets; partition p's counter is at base + kLineWords * p
+constexpr int kHitFill = kNumCounters * kStatSpread;
+constexpr int kActiveFill = kHitFill + kLineWords * kQueueParts;
+constexpr int kAoCursor = kActiveFill + kLineWords * kQueueParts;
+constexpr int kCounterWords = kAoCursor + kLineWords * kQueueParts;
 
 }  // namespace ubg
