@@ -19,7 +19,7 @@ import pytest
 
 from oracle import binding as ob
 from umbreon_amd import capi
-from umbreon_amd.scenes import ball_and_stick_scene, spheres_scene
+from umbreon_amd.scenes import ball_and_stick_scene, molecular_surface_scene, spheres_scene
 
 pytestmark = pytest.mark.gpu
 
@@ -152,3 +152,15 @@ def test_bvh_layouts_render_identically(device):
     d = np.abs(a.color - b.color).max(axis=2)
     assert (d > 1e-6).mean() <= 1e-4
     assert a.stats["shaded_hits"] == b.stats["shaded_hits"]
+
+
+def test_config3_surface_reduced(device):
+    """Alpha-0.6 Gaussian surface over an opaque CPK layer: several
+    transparency layers per pixel through the continuation queue."""
+    sc, opt = molecular_surface_scene(n_atoms=3000, width=480, height=270)
+    print(f"config3 reduced {opt.width}x{opt.height}, {len(sc.index)} triangles, "
+          f"{len(sc.sphere_center)} spheres", flush=True)
+    g = _gpu(device, sc, opt, hires_only=True, want_ids=True, want_ao=True)
+    c = ob.render(sc, opt, hires_only=True)
+    _compare("config3-reduced", g, c, ao=True)
+    assert g.layers >= 3  # primary_rays (continuations included) already match the oracle

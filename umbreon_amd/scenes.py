@@ -222,3 +222,169 @@ def scaled(scene_fn, scale: float, **kw):
     opt.width = max(8, int(opt.width * scale))
     opt.height = max(8, int(opt.height * scale))
     return sc, opt
+
+
+def gaussian_density(pos: np.ndarray, sigma: float = 1.0, spacing: float = 0.5, pad: float = 4.0
+                     ) -> Tuple[np.ndarray, np.ndarray]:
+    """Sum of unit Gaussians exp(-|x - a|^2 / (2 sigma^2)) on a regular grid.
+
+    Returns (density float32 (nx, ny, nz), origin (3,)). The Gaussian factors
+    per axis, so each x slab is one matrix product over the atoms within
+    `pad` sigma of it: rho[X] = (gx(X) * Gy)^T Gz, exact up to that cutoff
+    along x (no cutoff along y and z).
+    """
+    p = np.asarray(pos, np.float64)
+    lo = p.min(axis=0) - pad * sigma
+    hi = p.max(axis=0) + pad * sigma
+    dims = np.ceil((hi - lo) / spacing).astype(np.int64) + 1
+    axes = [lo[k] + spacing * np.arange(dims[k]) for k in range(3)]
+    inv = 1.0 / (2.0 * sigma * sigma)
+    gy = np.exp(-(axes[1][None, :] - p[:, 1:2]) ** 2 * inv)  # (atoms, ny)
+    gz = np.exp(-(axes[2][None, :] - p[:, 2:3]) ** 2 * inv)  # (atoms, nz)
+    order = np.argsort(p[:, 0], kind="stable")
+    xs = p[order, 0]
+    rho = np.empty((int(dims[0]), int(dims[1]), int(dims[2])), np.float32)
+    cut = pad * sigma
+    for X in range(int(dims[0])):
+        x = axes[0][X]
+        a, b = np.searchsorted(xs, x - cut), np.searchsorted(xs, x + cut, "right")
+        if a == b:
+            rho[X] = 0.0
+            continue
+        sel = order[a:b]
+        gx = np.exp(-(x - p[sel, 0]) ** 2 * inv)
+        rho[X] = ((gy[sel] * gx[:, None]).T @ gz[sel]).astype(np.float32)
+    return rho, lo
+
+
+def surface_nets(rho: np.ndarray, iso: float, origin: np.ndarray, spacing: float
+                 ) -> Tuple[np.ndarray, np.ndarray, np.ndarray]:
+    """Isosurface {rho = iso} as an indexed triangle mesh (naive surface nets,
+    Gibson 1998): one vertex per grid cell the surface crosses, at the mean of
+    the cell's edge crossings; one quad (two triangles) per grid edge with a
+    sign change, joining the four cells around it, wound so the triangle
+    normal points to decreasing density (outwards). Vertex normals are the
+    normalised negative density gradient at the vertex (trilinear blend of
+    central differences). Returns (positions (v, 3), normals (v, 3),
+    index (t, 3) uint32)."""
+    inside = rho > iso
+    nx, ny, nz = rho.shape
+    cx, cy, cz = nx - 1, ny - 1, nz - 1
+    # cell corner signs -> active cells
+    corners = [(i, j, k) for i in (0, 1) for j in (0, 1) for k in (0, 1)]
+    cnt_in = np.zeros((cx, cy, cz), np.int8)
+    for (i, j, k) in corners:
+        cnt_in += inside[i:i + cx, j:j + cy, k:k + cz]
+    active = (cnt_in > 0) & (cnt_in < 8)
+    cell_id = np.full((cx, cy, cz), -1, np.int64)
+    ai = np.nonzero(active)
+    nv = len(ai[0])
+    cell_id[ai] = np.arange(nv)
+    # vertex = mean of the crossings of the cell's 12 edges
+    acc = np.zeros((nv, 3), np.float64)
+    num = np.zeros(nv, np.float64)
+    base = np.stack(ai, axis=1).astype(np.float64)
+    for axis in range(3):
+        for a in (0, 1):
+            for b in (0, 1):
+                off0 = [0, 0, 0]
+                o = [k for k in range(3) if k != axis]
+                off0[o[0]] = a
+                off0[o[1]] = b
+                off1 = list(off0)
+                off1[axis] = 1
+                v0 = rho[ai[0] + off0[0], ai[1] + off0[1], ai[2] + off0[2]].astype(np.float64)
+                v1 = rho[ai[0] + off1[0], ai[1] + off1[1], ai[2] + off1[2]].astype(np.float64)
+                cross = (v0 > iso) != (v1 > iso)
+                t = np.where(cross, (iso - v0) / np.where(cross, v1 - v0, 1.0), 0.0)
+                pt = base + np.asarray(off0, np.float64)
+                pt[:, axis] += t
+                acc += pt * cross[:, None]
+                num += cross
+    vgrid = acc / num[:, None]
+    positions = (origin[None, :] + spacing * vgrid).astype(np.float32)
+    # normals: -grad(rho), trilinear from central differences
+    g = np.stack(np.gradient(rho.astype(np.float64)), axis=-1)
+    i0 = np.clip(np.floor(vgrid).astype(np.int64), 0, np.array([nx - 2, ny - 2, nz - 2]))
+    f = vgrid - i0
+    grad = np.zeros((nv, 3))
+    for (i, j, k) in corners:
+        w = (f[:, 0] if i else 1 - f[:, 0]) * (f[:, 1] if j else 1 - f[:, 1]) * (f[:, 2] if k else 1 - f[:, 2])
+        grad += w[:, None] * g[i0[:, 0] + i, i0[:, 1] + j, i0[:, 2] + k]
+    nrm = -grad
+    nrm /= np.maximum(np.linalg.norm(nrm, axis=1, keepdims=True), 1e-30)
+    # quads around sign-changing grid edges
+    tris = []
+    for axis in range(3):
+        o1, o2 = (axis + 1) % 3, (axis + 2) % 3  # o1 x o2 = +axis
+        sl0 = [slice(None)] * 3
+        sl1 = [slice(None)] * 3
+        sl0[axis] = slice(0, rho.shape[axis] - 1)
+        sl1[axis] = slice(1, rho.shape[axis])
+        a = inside[tuple(sl0)]
+        b = inside[tuple(sl1)]
+        ch = a != b
+        # the four cells around the edge need o1, o2 indices >= 1 and < dim - 1
+        for k in (o1, o2):
+            s = [slice(None)] * 3
+            s[k] = 0
+            ch[tuple(s)] = False
+            s[k] = ch.shape[k] - 1
+            ch[tuple(s)] = False
+        e = np.nonzero(ch)
+        ei = np.stack(e, axis=1)
+        outward = a[e]  # inside at the lower end: density falls along +axis
+
+        def cell(d1, d2):
+            c = ei.copy()
+            c[:, o1] -= d1
+            c[:, o2] -= d2
+            return cell_id[c[:, 0], c[:, 1], c[:, 2]]
+        q0, q1, q2, q3 = cell(1, 1), cell(0, 1), cell(0, 0), cell(1, 0)
+        # q0 -> q1 -> q2 -> q3 turns counter-clockwise about +axis
+        t1 = np.stack([q0, q1, q2], axis=1)
+        t2 = np.stack([q0, q2, q3], axis=1)
+        flip = ~outward
+        t1[flip] = t1[flip][:, ::-1]
+        t2[flip] = t2[flip][:, ::-1]
+        tris.append(np.concatenate([t1, t2]))
+    index = np.concatenate(tris).astype(np.uint32)
+    return positions, nrm.astype(np.float32), index
+
+
+@functools.lru_cache(maxsize=2)
+def _surface_mesh(n_atoms: int, seed: int, sigma: float, iso: float, spacing: float):
+    pos, elem = protein_atoms(n_atoms, seed)
+    rho, origin = gaussian_density(pos, sigma, spacing)
+    v, nrm, idx = surface_nets(rho, iso, origin, spacing)
+    # per-vertex colour of the nearest atom
+    from scipy.spatial import cKDTree
+    _, near = cKDTree(pos).query(v, k=1)
+    return pos, elem, v, nrm, idx, elem[near]
+
+
+def molecular_surface_scene(n_atoms: int = 40000, width: int = 1920, height: int = 1080,
+                            supersample: int = 1, ao_samples: int = 16, alpha: float = 0.6,
+                            sigma: float = 1.0, iso: float = 0.5, spacing: float = 0.5, seed: int = 0
+                            ) -> Tuple[Scene, RenderOptions]:
+    """Molecular surface: the Gaussian-density isosurface (sigma 1 A, iso 0.5,
+    0.5 A grid) of a 40,000-atom synthetic protein as an indexed triangle mesh
+    with per-vertex normals, coloured by the nearest atom at alpha 0.6, over an
+    opaque inner CPK layer of the same 40,000 atoms; shadows, AO 16 (8 A),
+    1920x1080 at 1 spp."""
+    pos, elem, v, nrm, idx, velem = _surface_mesh(int(n_atoms), int(seed), float(sigma), float(iso),
+                                                  float(spacing))
+    rng = np.random.default_rng(seed + 1)
+    radii = RADII[rng.integers(0, len(RADII), len(pos))]
+    col = PALETTE[velem].copy()
+    col[:, 3] = alpha
+    sc = Scene()
+    sc.set_mesh(v, nrm, col, idx, material=LIT)
+    sc.add_spheres(pos, radii, PALETTE[elem], LIT)
+    sc.camera, _ = frame_camera(v, np.zeros(len(v), np.float32), 40.0, width / height)
+    sc.lights = [key_light()]
+    sc.ambient_color = (1.0, 1.0, 1.0)
+    sc.background = (0.0, 0.0, 0.0)
+    opt = RenderOptions(width=width, height=height, supersample=supersample, shadows=True,
+                        ao_samples=ao_samples, ao_distance=8.0)
+    return sc, opt
