@@ -20,6 +20,8 @@ import sys
 import time
 from pathlib import Path
 
+import numpy as np
+
 ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
@@ -32,35 +34,61 @@ def log(*a):
 
 
 def cpu_baseline(scene, opt, target_s: float):
-    """Oracle (CPU restatement, FP32) on the host cores over a bounded row sample."""
+    """Oracle (CPU restatement, FP32) on the host cores: the full frame, median
+    of 5 runs after 1 warm-up, when that fits the time target; otherwise a
+    row sample sized to about target_s seconds."""
     from oracle import binding as ob
     threads = max(1, min(16, os.cpu_count() or 1))
     H = opt.height * opt.supersample
     W = opt.width * opt.supersample
-    # calibrate on every 64th row, then size the sample to ~target_s seconds
     step = 64
-    t0 = time.perf_counter()
     f = ob.render(scene, opt, hires_only=True, threads=threads, rows=(0, H, step))
     cal = max(1e-3, f.render_seconds)
-    rays_cal = f.total_rays
-    log(f"cpu calibration: {rays_cal} rays in {cal:.2f}s ({rays_cal / cal / 1e6:.2f} Mrays/s)")
-    want = max(1, int(step * cal / target_s))
-    step2 = max(1, min(step, want))
-    f = ob.render(scene, opt, hires_only=True, threads=threads, rows=(0, H, step2))
-    rays = f.total_rays
-    sec = f.render_seconds
-    log(f"cpu baseline: {rays} rays in {sec:.2f}s with {threads} threads "
-        f"(wall incl. BVH build {time.perf_counter() - t0:.1f}s)")
-    nrows = len(range(0, H, step2))
+    log(f"cpu calibration: {f.total_rays} rays in {cal:.2f}s ({f.total_rays / cal / 1e6:.2f} Mrays/s)")
+    est_full = cal * step
+    if 6 * est_full <= 2 * target_s:
+        times = []
+        for i in range(6):
+            f = ob.render(scene, opt, hires_only=True, threads=threads)
+            if i > 0:
+                times.append(f.render_seconds)
+        sec = float(np.median(times))
+        rays = f.total_rays
+        sample = (f"full frame ({H} x {W} hi-res px, {rays} rays), median of 5 runs after 1 warm-up "
+                  f"({', '.join(f'{t:.3f}' for t in times)} s)")
+    else:
+        step2 = max(1, min(step, int(step * cal / target_s)))
+        f = ob.render(scene, opt, hires_only=True, threads=threads, rows=(0, H, step2))
+        rays, sec = f.total_rays, f.render_seconds
+        sample = f"hi-res rows y = 0 mod {step2} ({len(range(0, H, step2))} of {H} rows x {W} px), {rays} rays"
+    log(f"cpu baseline: {rays} rays in {sec:.3f}s with {threads} threads")
     return {
         "value": rays / sec / 1e6,
         "unit": "Mrays/s",
         "cores": threads,
         "kind": "port",
-        "sample": f"hi-res rows y = 0 mod {step2} ({nrows} of {H} rows x {W} px), {rays} rays; "
-                  f"CPU restatement (not Embree), FP32, own binned-SAH BVH, std::thread over rows; "
-                  f"time window = trace + shade + fog (BVH build excluded)",
+        "sample": sample + "; CPU restatement (not Embree), FP32, own binned-SAH BVH, std::thread over rows; "
+                           "time window = trace + shade + fog (BVH build excluded)",
     }
+
+
+TRAFFIC_TABLE = ROOT / "profiles" / "hbm_traffic_ballstick_surface.md"
+
+
+def measured_traffic(workload: str, layout: str, kernel: str):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter
+    summary (FETCH_SIZE and WRITE_SIZE scaled by a 256 MiB copy kernel's
+    readings); None when the table has no row for this configuration."""
+    if not TRAFFIC_TABLE.exists():
+        return None
+    for line in TRAFFIC_TABLE.read_text().splitlines():
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cells) >= 6 and cells[0] == workload and cells[1] == layout and cells[2] == kernel:
+            try:
+                return float(cells[5]) * 1e6
+            except ValueError:
+                return None
+    return None
 
 
 def calib_copy(mib: int) -> None:
@@ -161,10 +189,15 @@ def main() -> int:
     bytes_per_frame = 48 * rays_in + node_b * nodes + prim_b * tests
     dur_s = ks[dom] / args.steps / 1e3
     achieved = bytes_per_frame / dur_s / 1e9
+    kname = {"primary": "k_primary", "ao": "k_ao_persistent" if args.layout == "bvh8" else "k_ao",
+             "resolve": "k_resolve"}[dom]
+    per_launch = measured_traffic(args.workload, args.layout, kname)
+    launches = kn[dom] // args.steps
     roofline = {
         "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-        "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-        "kernel": {"primary": "k_primary", "ao": "k_ao", "resolve": "k_resolve"}[dom],
+        "frac": round(achieved / HBM_PEAK_GBS, 4),
+        "traffic": None if per_launch is None else int(per_launch * launches),
+        "kernel": kname,
         "kernel_ms_per_frame": round(dur_s * 1e3, 4),
         "launches_per_frame": kn[dom] // args.steps,
         "algorithmic_bytes_per_frame": int(bytes_per_frame),
