@@ -67,3 +67,13 @@ def test_fp32_oracle_meets_gates_against_fixture(name):
     sc, opt = make_golden.build(name)
     f = ob.render(sc, opt, hires_only=True)
     gates(f"fp32 oracle {name}", f.color, f.hit_id, f.ao_open, f.stats, z, meta)
+
+
+def test_psnr_helper_closed_forms():
+    from imagecmp import linear_to_srgb, psnr_srgb
+    assert abs(float(linear_to_srgb(np.array(0.5))) - 0.7353569830524495) < 1e-12  # 188/255 in the reference's test
+    a = np.zeros((2, 2, 4))
+    b = a.copy()
+    assert psnr_srgb(a, b) == 99.0
+    b[0, 0, 0] = 1.0  # one channel of 12 off by 1.0 in sRGB
+    assert abs(psnr_srgb(a, b) - 10.0 * np.log10(12.0)) < 1e-12

@@ -18,6 +18,7 @@ import numpy as np
 import pytest
 
 from oracle import binding as ob
+from imagecmp import psnr_srgb
 from umbreon_amd import capi
 from umbreon_amd.scenes import ball_and_stick_scene, molecular_surface_scene, spheres_scene
 
@@ -164,3 +165,8 @@ def test_config3_surface_reduced(device):
     c = ob.render(sc, opt, hires_only=True)
     _compare("config3-reduced", g, c, ao=True)
     assert g.layers >= 3  # primary_rays (continuations included) already match the oracle
+    # the north-star output bar for this scene: PSNR >= 60 dB (sRGB) against the FP64 oracle
+    c64 = ob.render(sc, opt, hires_only=True, precision=64)
+    p = psnr_srgb(g.color, c64.color)
+    print(f"config3-reduced PSNR vs FP64 oracle: {p:.1f} dB", flush=True)
+    assert p >= 60.0
