@@ -117,8 +117,9 @@ def main() -> int:
     ap.add_argument("--workload", choices=["ballstick", "spheres", "surface"], default="ballstick")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
-    ap.add_argument("--layout", choices=["bvh8", "bvh2"], default="bvh8",
-                    help="acceleration structure: compressed 8-wide (default) or full-precision binary")
+    ap.add_argument("--layout", choices=["bvh8", "bvh8gpu", "bvh2"], default="bvh8",
+                    help="acceleration structure: compressed 8-wide built on the host (default) or on the GPU, "
+                         "or full-precision binary")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="also run one device-to-device copy of this many MiB (a copy kernel of known "
                          "traffic, for scaling FETCH_SIZE / WRITE_SIZE counter readings)")
@@ -149,10 +150,10 @@ def main() -> int:
     log(f"scene generated in {time.perf_counter() - t0:.1f}s: {workload}")
 
     dev = Device(0)
-    bs = BuiltScene(dev, scene, capi.BVH8 if args.layout == "bvh8" else capi.BVH2)
+    bs = BuiltScene(dev, scene, {"bvh8": capi.BVH8, "bvh8gpu": capi.BVH8_GPU_BUILD, "bvh2": capi.BVH2}[args.layout])
     info = bs.info
     log(f"BVH ({args.layout}): {info['n_nodes']} nodes of {info['node_bytes']} B, depth {info['bvh_depth']}, "
-        f"build {info['build_seconds'] * 1e3:.1f} ms")
+        f"build {info['build_seconds'] * 1e3:.1f} ms, SAH {info['sah_cost']:.1f}")
 
     # visit counts on the same BVH and rays (instrumented kernels, untimed)
     vc = bs.render(scene, opt, outputs=False, count_visits=True)
@@ -189,7 +190,7 @@ def main() -> int:
     bytes_per_frame = 48 * rays_in + node_b * nodes + prim_b * tests
     dur_s = ks[dom] / args.steps / 1e3
     achieved = bytes_per_frame / dur_s / 1e9
-    kname = {"primary": "k_primary", "ao": "k_ao_persistent" if args.layout == "bvh8" else "k_ao",
+    kname = {"primary": "k_primary", "ao": "k_ao" if args.layout == "bvh2" else "k_ao_persistent",
              "resolve": "k_resolve"}[dom]
     per_launch = measured_traffic(args.workload, args.layout, kname)
     launches = kn[dom] // args.steps

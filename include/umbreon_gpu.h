@@ -223,8 +223,10 @@ typedef struct ubg_frame {
 
 /* Acceleration-structure layouts (ubg_scene_create_ex). */
 enum {
-  UBG_BVH2 = 0,  /* binary BVH, full fp32 child boxes, 64 B nodes */
-  UBG_BVH8 = 1   /* 8-wide BVH, 8-bit quantized child boxes, 80 B nodes (default) */
+  UBG_BVH2 = 0,          /* binary BVH, full fp32 child boxes, 64 B nodes */
+  UBG_BVH8 = 1,          /* 8-wide BVH, 8-bit quantized child boxes, 80 B nodes (default) */
+  UBG_BVH8_GPU_BUILD = 2 /* the UBG_BVH8 layout, built on the GPU (Morton order + PLOC
+                            clustering + the same SAH-optimal 8-wide collapse) */
 };
 
 typedef struct ubg_scene_info {
@@ -233,7 +235,7 @@ typedef struct ubg_scene_info {
   int64_t n_leaf_prims;     /* primitive records in leaf order */
   int32_t bvh_depth;        /* tree levels */
   int32_t node_bytes;       /* bytes per BVH node */
-  int32_t bvh_layout;       /* UBG_BVH2 / UBG_BVH8 */
+  int32_t bvh_layout;       /* UBG_BVH2 / UBG_BVH8 / UBG_BVH8_GPU_BUILD */
   int32_t stack_depth;      /* traversal stack entries per lane */
   int32_t prim_bytes;       /* bytes per leaf primitive record */
   double build_seconds;     /* upload + BVH build wall time */
@@ -274,8 +276,10 @@ const char* ubg_last_error(void);
 
 /* Upload geometry and build the BVH (UBG_BVH8). */
 int ubg_scene_create(ubg_context* ctx, const ubg_scene_desc* desc, ubg_scene** out);
-/* Same with an explicit acceleration-structure layout (UBG_BVH2 / UBG_BVH8);
- * both give identical images, the layout only changes traversal speed. */
+/* Same with an explicit acceleration-structure layout (UBG_BVH2 / UBG_BVH8 /
+ * UBG_BVH8_GPU_BUILD); all give the same images (up to which of two
+ * primitives hit at exactly the same distance wins), the layout and the
+ * builder only change build and traversal speed. */
 int ubg_scene_create_ex(ubg_context* ctx, const ubg_scene_desc* desc, int32_t bvh_layout, ubg_scene** out);
 void ubg_scene_destroy(ubg_scene* scene);
 int ubg_scene_get_info(const ubg_scene* scene, ubg_scene_info* out);

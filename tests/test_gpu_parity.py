@@ -170,3 +170,78 @@ def test_config3_surface_reduced(device):
     p = psnr_srgb(g.color, c64.color)
     print(f"config3-reduced PSNR vs FP64 oracle: {p:.1f} dB", flush=True)
     assert p >= 60.0
+
+
+def test_gpu_built_bvh8_matches_host_built():
+    """The GPU builder (Morton + PLOC + SAH-optimal collapse) gives the same
+    closest hits as the host builder on random rays (equal t; the primitive
+    may differ only at exact t ties) and the same images."""
+    from umbreon_amd.render import BuiltScene, Device
+    from test_oracle import _random_prim_scene, _rays
+    dev = Device(0)
+    try:
+        rng = np.random.default_rng(5)
+        sc = _random_prim_scene(rng, 3000)
+        n = 50000
+        org = rng.uniform(-1, 1, size=(n, 3)) + np.array([0, 0, 12.0])
+        d = rng.uniform(-3, 3, size=(n, 3)) - org
+        d /= np.linalg.norm(d, axis=1, keepdims=True)
+        r = _rays(org, d)
+        out = {}
+        for lay in (capi.BVH8, capi.BVH8_GPU_BUILD):
+            bs = BuiltScene(dev, sc, lay)
+            try:
+                out[lay] = (bs.intersect(r), bs.occluded(r), bs.info)
+            finally:
+                bs.close()
+        (h1, o1, i1), (h2, o2, i2) = out[capi.BVH8], out[capi.BVH8_GPU_BUILD]
+        print(f"host SAH {i1['sah_cost']:.1f} nodes {i1['n_nodes']} depth {i1['bvh_depth']}; "
+              f"gpu SAH {i2['sah_cost']:.1f} nodes {i2['n_nodes']} depth {i2['bvh_depth']}", flush=True)
+        assert np.array_equal(h1["t"], h2["t"])
+        assert (h1["prim_id"] == h2["prim_id"]).mean() >= 0.9999
+        assert np.array_equal(o1, o2)
+        assert i2["n_leaf_prims"] == i1["n_leaf_prims"]
+        # images: the ball-and-stick scene at reduced size
+        sb, opt = ball_and_stick_scene(n_atoms=6000, width=320, height=180)
+        img = {}
+        for lay in (capi.BVH8, capi.BVH8_GPU_BUILD):
+            bs = BuiltScene(dev, sb, lay)
+            try:
+                img[lay] = bs.render(sb, opt, hires_only=True, want_ids=True, want_ao=True)
+            finally:
+                bs.close()
+        a, b = img[capi.BVH8], img[capi.BVH8_GPU_BUILD]
+        assert (a.hit_id == b.hit_id).mean() >= 0.99999
+        assert (np.abs(a.color - b.color).max(axis=2) > 1e-6).mean() <= 1e-4
+        assert a.stats == b.stats or a.stats["shaded_hits"] == b.stats["shaded_hits"]
+    finally:
+        dev.close()
+
+
+def test_gpu_build_single_primitive_and_large_counts():
+    from umbreon_amd.render import BuiltScene, Device
+    from umbreon_amd.scene import Scene
+    from test_oracle import _rays
+    dev = Device(0)
+    try:
+        sc = Scene()
+        sc.add_spheres([[0.0, 0.0, 0.0]], [1.0], [[1, 1, 1, 1]])
+        bs = BuiltScene(dev, sc, capi.BVH8_GPU_BUILD)
+        try:
+            r = _rays(np.array([[0.0, 0.0, 5.0]]), np.array([[0.0, 0.0, -1.0]]))
+            h = bs.intersect(r)
+            assert h["t"][0] == np.float32(4.0) and bs.info["n_nodes"] == 1
+        finally:
+            bs.close()
+        # many coincident spheres (equal Morton codes, equal merge costs)
+        sc = Scene()
+        sc.add_spheres(np.zeros((5000, 3)), np.full(5000, 0.5), np.ones((5000, 4)))
+        bs = BuiltScene(dev, sc, capi.BVH8_GPU_BUILD)
+        try:
+            r = _rays(np.array([[0.0, 0.0, 5.0]]), np.array([[0.0, 0.0, -1.0]]))
+            assert bs.intersect(r)["t"][0] == np.float32(4.5)
+            assert bs.info["n_leaf_prims"] == 5000
+        finally:
+            bs.close()
+    finally:
+        dev.close()

@@ -24,6 +24,7 @@ UBG_ERR_INTERNAL = 6
 
 BVH2 = 0
 BVH8 = 1
+BVH8_GPU_BUILD = 2  # the BVH8 layout, built on the GPU
 
 KIND_MESH = 0
 KIND_SPHERE = 1

@@ -462,16 +462,6 @@ int flattenBvh2(const std::vector<BNode>& bn, int32_t root, const std::vector<ui
 // breadth-first order and each node's leaf primitives are stored contiguously.
 // Child boxes are quantized to 8 bits with power-of-two scales, rounded
 // outward, so the quantized box always contains the exact one.
-struct Node8 {
-  float p[3];
-  uint8_t e[3];
-  uint8_t imask;
-  uint32_t childBase, primBase;
-  uint8_t meta[8];
-  uint8_t qlo[3][8];
-  uint8_t qhi[3][8];
-};
-static_assert(sizeof(Node8) == 80, "BVH8 node is 80 bytes");
 
 // SAH-optimal conversion of the binary tree into 8-wide nodes: dynamic
 // programming over how many slots of its parent each binary subtree takes
@@ -707,7 +697,7 @@ int collapseBvh8(const std::vector<BNode>& bn, int32_t root, const std::vector<u
 
 }  // namespace
 
-int buildHostScene(const ubg_scene_desc* d, int layout, HostScene& out, std::string& err) {
+int buildHostScene(const ubg_scene_desc* d, int layout, HostScene& out, std::string& err, bool recordsOnly) {
   out = HostScene();
   if (layout != kLayoutBvh2 && layout != kLayoutBvh8) {
     err = "unknown BVH layout";
@@ -956,6 +946,16 @@ int buildHostScene(const ubg_scene_desc* d, int layout, HostScene& out, std::str
   if (nPrims >= (1u << 28)) {
     err = "scene has too many primitives";
     return UBG_ERR_INVALID_ARGUMENT;
+  }
+  if (recordsOnly) {
+    out.layout = layout;
+    out.prims = std::move(recs);
+    out.boxes.resize(2 * nPrims);
+    for (size_t i = 0; i < nPrims; ++i) {
+      out.boxes[2 * i] = F4{boxes[i].lo[0], boxes[i].lo[1], boxes[i].lo[2], 0.0f};
+      out.boxes[2 * i + 1] = F4{boxes[i].hi[0], boxes[i].hi[1], boxes[i].hi[2], 0.0f};
+    }
+    return UBG_OK;
   }
   // ---- BVH ----
   // BVH8: single-primitive binary leaves; the collapse groups them into leaf slots

@@ -21,7 +21,8 @@ struct U4 {
 // Everything that goes to HBM for one scene, still on the host.
 struct HostScene {
   std::vector<F4> nodes;   // BVH2: 4 per inner node; BVH8: 5 per node
-  std::vector<F4> prims;   // 3 per primitive, leaf order
+  std::vector<F4> prims;   // 3 per primitive, leaf order (record order when recordsOnly)
+  std::vector<F4> boxes;   // recordsOnly: 2 per primitive (lo.xyz, hi.xyz), conservative FP32
   std::vector<F4> sphColor;
   std::vector<uint32_t> sphInfo;
   std::vector<F4> coneColor;
@@ -47,9 +48,12 @@ struct HostScene {
 };
 
 // Flatten `desc` (curve chaining, instance baking, side tables) and build the
-// BVH in `layout` (kLayoutBvh2 / kLayoutBvh8). Returns UBG_OK or an error
-// status with `err` set.
-int buildHostScene(const ubg_scene_desc* desc, int layout, HostScene& out, std::string& err);
+// BVH in `layout` (kLayoutBvh2 / kLayoutBvh8). With recordsOnly the BVH is
+// left to the caller: `prims` keeps the records in geometry order and
+// `boxes` holds their bounds. Returns UBG_OK or an error status with `err`
+// set.
+int buildHostScene(const ubg_scene_desc* desc, int layout, HostScene& out, std::string& err,
+                   bool recordsOnly = false);
 
 // shading.hpp:34-41
 float blinnExponent(float roughness);

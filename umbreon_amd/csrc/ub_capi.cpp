@@ -22,6 +22,7 @@
 
 #include "../../include/umbreon_gpu.h"
 #include "ub_build.h"
+#include "ub_devbuild.h"
 #include "ub_kernels.h"
 #include "ub_layout.h"
 
@@ -206,14 +207,16 @@ static void freeSceneLocked(ubg_scene* s) {
 int ubg_scene_create_ex(ubg_context* ctx, const ubg_scene_desc* desc, int32_t bvh_layout, ubg_scene** out) {
   if (!ctx || !desc || !out) return fail(UBG_ERR_INVALID_ARGUMENT, "ubg_scene_create: null argument");
   *out = nullptr;
-  if (bvh_layout != UBG_BVH2 && bvh_layout != UBG_BVH8)
+  if (bvh_layout != UBG_BVH2 && bvh_layout != UBG_BVH8 && bvh_layout != UBG_BVH8_GPU_BUILD)
     return fail(UBG_ERR_INVALID_ARGUMENT, "ubg_scene_create: unknown BVH layout");
+  const bool gpuBuild = bvh_layout == UBG_BVH8_GPU_BUILD;
   std::lock_guard<std::mutex> lk(ctx->mu);
   UBG_HIP(hipSetDevice(ctx->device), "hipSetDevice");
   const auto t0 = std::chrono::steady_clock::now();
   ubg::HostScene hs;
   std::string err;
-  const int st = ubg::buildHostScene(desc, bvh_layout == UBG_BVH8 ? ubg::kLayoutBvh8 : ubg::kLayoutBvh2, hs, err);
+  const int st = ubg::buildHostScene(desc, bvh_layout == UBG_BVH2 ? ubg::kLayoutBvh2 : ubg::kLayoutBvh8, hs, err,
+                                     gpuBuild);
   if (st != UBG_OK) return fail(st, "scene build failed: " + err);
   ubg_scene* s = new ubg_scene();
   s->ctx = ctx;
@@ -222,8 +225,45 @@ int ubg_scene_create_ex(ubg_context* ctx, const ubg_scene_desc* desc, int32_t bv
     if (r == UBG_OK) r = upload(s, vec, dst);
   };
   const void* p = nullptr;
-  up(hs.nodes, &s->view.nodes);
-  up(hs.prims, &s->view.prims);
+  if (gpuBuild && !hs.prims.empty()) {
+    // records and boxes go up in geometry order; the GPU builds the BVH8 and
+    // writes the records again in leaf order
+    const void* recs = nullptr;
+    const void* boxes = nullptr;
+    up(hs.prims, &recs);
+    up(hs.boxes, &boxes);
+    ubg::DeviceBvh8 db;
+    if (r == UBG_OK) {
+      r = ubg::deviceBuildBvh8(static_cast<const float4*>(recs), static_cast<const float4*>(boxes),
+                               static_cast<int64_t>(hs.prims.size() / 3), ctx->stream, db, err);
+      if (r != UBG_OK) (void)fail(r, "scene build failed: " + err);
+    }
+    if (r == UBG_OK) {
+      s->allocs.push_back(db.nodes);
+      s->allocs.push_back(db.prims);
+      s->view.nodes = db.nodes;
+      s->view.prims = db.prims;
+      hs.nNodes = db.nNodes;
+      hs.depth = db.depth;
+      hs.stackDepth = db.depth;
+      hs.nodeBytes = 80;
+      hs.sahCost = db.sahCost;
+    }
+    // the geometry-order copies are not needed any more
+    (void)hipStreamSynchronize(ctx->stream);
+    for (const void* tmp : {recs, boxes}) {
+      auto it = std::find(s->allocs.begin(), s->allocs.end(), tmp);
+      if (tmp && it != s->allocs.end()) {
+        (void)hipFree(const_cast<void*>(tmp));
+        s->allocs.erase(it);
+      }
+    }
+    s->info.device_bytes -= static_cast<int64_t>((hs.prims.size() + hs.boxes.size()) * sizeof(ubg::F4));
+    if (r == UBG_OK) s->info.device_bytes += db.nNodes * 80 + static_cast<int64_t>(hs.prims.size() / 3) * 48;
+  } else {
+    up(hs.nodes, &s->view.nodes);
+    up(hs.prims, &s->view.prims);
+  }
   up(hs.sphColor, &s->view.sphColor);
   up(hs.sphInfo, &p);
   s->view.sphInfo = static_cast<const uint32_t*>(p);
@@ -254,7 +294,7 @@ int ubg_scene_create_ex(ubg_context* ctx, const ubg_scene_desc* desc, int32_t bv
     return r;
   }
   s->view.nNodes = static_cast<int32_t>(hs.nNodes);
-  s->view.nPrims = static_cast<int32_t>(hs.prims.size() / 3);
+  s->view.nPrims = static_cast<int32_t>(hs.prims.size() / 3);  // records in either order
   s->view.layout = hs.layout;
   s->view.stackDepth = std::max(1, hs.stackDepth);
   const auto t1 = std::chrono::steady_clock::now();

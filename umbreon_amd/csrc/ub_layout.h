@@ -49,6 +49,22 @@ struct DevMaterial {
   float roughness, pad;
 };
 
+// One BVH8 node as the builders write it (80 bytes; the traversal reads it
+// as 5 x uint4): box origin, per-axis scale exponents (scale = 2^(e - 127)),
+// inner-child mask, first inner child, first leaf primitive, per-slot meta
+// (0 empty; leaf: unary count << 5 | offset; inner: 1 << 5 | (24 + slot)),
+// quantized child planes lo/hi per axis and slot.
+struct Node8 {
+  float p[3];
+  uint8_t e[3];
+  uint8_t imask;
+  uint32_t childBase, primBase;
+  uint8_t meta[8];
+  uint8_t qlo[3][8];
+  uint8_t qhi[3][8];
+};
+static_assert(sizeof(Node8) == 80, "BVH8 node is 80 bytes");
+
 struct SceneView {
   const void* nodes;      // BVH2: float4 x 4 per inner node; BVH8: uint4 x 5 per node
   const void* prims;      // float4 x 3 per primitive, in the layout's leaf order
