@@ -489,13 +489,13 @@ UBG_HD uint32_t intersectNode8(const uint4* nodes, uint32_t idx, V3 iv, V3 o, fl
   const uint32_t nx0 = px ? w2.x : w3.z, nx1 = px ? w2.y : w3.w, fx0 = px ? w3.z : w2.x, fx1 = px ? w3.w : w2.y;
   const uint32_t ny0 = py ? w2.z : w4.x, ny1 = py ? w2.w : w4.y, fy0 = py ? w4.x : w2.z, fy1 = py ? w4.y : w2.w;
   const uint32_t nz0 = pz ? w3.x : w4.z, nz1 = pz ? w3.y : w4.w, fz0 = pz ? w4.z : w3.x, fz1 = pz ? w4.w : w3.y;
-  // Branch-free: collect the slots whose box is hit, then split them into
-  // inner children (pending bits, reordered by the ray octant) and leaf
-  // primitives.
-  uint32_t hitSlots = 0, leaf = 0;
+  // Branch-free box tests collect the slots whose box is hit; they are then
+  // split into inner children (pending bits, reordered by the ray octant) and
+  // leaf primitives. An empty slot needs no test of its own: its meta is 0,
+  // so it adds no primitive bits, and imask has no bit for it.
+  uint32_t hitSlots = 0;
 #pragma unroll
   for (int i = 0; i < 8; ++i) {
-    const uint32_t meta = byteOf(w1.z, w1.w, i);
     const float nxv = __builtin_fmaf(static_cast<float>(byteOf(nx0, nx1, i)), sx, ox);
     const float fxv = __builtin_fmaf(static_cast<float>(byteOf(fx0, fx1, i)), sx, ox);
     const float nyv = __builtin_fmaf(static_cast<float>(byteOf(ny0, ny1, i)), sy, oy);
@@ -504,11 +504,13 @@ UBG_HD uint32_t intersectNode8(const uint4* nodes, uint32_t idx, V3 iv, V3 o, fl
     const float fzv = __builtin_fmaf(static_cast<float>(byteOf(fz0, fz1, i)), sz, oz);
     const float en = fmaxf(fmaxf(nxv, nyv), fmaxf(nzv, tnear));
     const float ex = fminf(fminf(fxv, fyv), fminf(fzv, tmx));
-    const bool ok = meta != 0u && en <= ex;
-    hitSlots |= ok ? (1u << i) : 0u;
-    // leaf meta = (unary count) << 5 | offset; for an inner slot this sets a
-    // bit >= 24, masked off below
-    leaf |= ok ? (meta >> 5) << (meta & 31u) : 0u;
+    hitSlots |= en <= ex ? (1u << i) : 0u;
+  }
+  // leaf slots hit: meta = (unary count) << 5 | offset of their primitives
+  uint32_t leaf = 0;
+  for (uint32_t ls = hitSlots & ~imask; ls != 0u; ls &= ls - 1u) {
+    const uint32_t meta = byteOf(w1.z, w1.w, __builtin_ctz(ls));
+    leaf |= (meta >> 5) << (meta & 31u);
   }
   uint32_t inner = hitSlots & imask;
   // slot i -> key i ^ oct, as three conditional swaps of bit groups
@@ -517,7 +519,7 @@ UBG_HD uint32_t intersectNode8(const uint4* nodes, uint32_t idx, V3 iv, V3 o, fl
   if (oct & 4u) inner = ((inner & 0x0Fu) << 4) | ((inner >> 4) & 0x0Fu);
   childBase = w1.x;
   primBase = w1.y;
-  leafBits = leaf & 0x00FFFFFFu;
+  leafBits = leaf;
   return (inner << 24) | imask;
 }
 
