@@ -591,6 +591,7 @@ int collapseBvh8(const std::vector<BNode>& bn, int32_t root, const std::vector<u
     int depth;
   };
   std::vector<Node8> nodes(1);
+  std::vector<uint32_t> kinds(2);  // leaf primitive kind bit planes per node
   std::vector<F4> prims;
   prims.reserve(3 * order.size());
   std::vector<Item> queue{{root, 0u, 1}};
@@ -669,8 +670,15 @@ int collapseBvh8(const std::vector<BNode>& bn, int32_t root, const std::vector<u
         const uint32_t off = static_cast<uint32_t>(prims.size() / 3) - n.primBase;
         const uint32_t unary = (1u << cc) - 1u;
         n.meta[sl] = static_cast<uint8_t>((unary << 5) | off);
-        for (int32_t q = cf; q < cf + cc; ++q)
+        for (int32_t q = cf; q < cf + cc; ++q) {
+          const uint32_t bit = off + static_cast<uint32_t>(q - cf);
+          uint32_t w;
+          std::memcpy(&w, &recs[3 * order[q] + 2].w, 4);
+          const uint32_t kind = w >> kKindShift;
+          kinds[2 * it.idx] |= (kind & 1u) << bit;
+          kinds[2 * it.idx + 1] |= (kind >> 1) << bit;
           for (int k = 0; k < 3; ++k) prims.push_back(recs[3 * order[q] + k]);
+        }
       } else {
         n.imask |= static_cast<uint8_t>(1u << sl);
         n.meta[sl] = static_cast<uint8_t>((1u << 5) | (24u + static_cast<uint32_t>(sl)));
@@ -679,6 +687,7 @@ int collapseBvh8(const std::vector<BNode>& bn, int32_t root, const std::vector<u
       }
     }
     nodes.resize(nodes.size() + nInner);
+    kinds.resize(2 * nodes.size(), 0u);
     nodes[it.idx] = n;
   }
   if (maxDepth > 64) {
@@ -686,6 +695,7 @@ int collapseBvh8(const std::vector<BNode>& bn, int32_t root, const std::vector<u
     return UBG_ERR_INTERNAL;
   }
   out.prims = std::move(prims);
+  out.nodeKinds = std::move(kinds);
   out.nodes.resize(5 * nodes.size());
   std::memcpy(out.nodes.data(), nodes.data(), nodes.size() * sizeof(Node8));
   out.depth = maxDepth;

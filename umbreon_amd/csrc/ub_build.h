@@ -21,6 +21,7 @@ struct U4 {
 // Everything that goes to HBM for one scene, still on the host.
 struct HostScene {
   std::vector<F4> nodes;   // BVH2: 4 per inner node; BVH8: 5 per node
+  std::vector<uint32_t> nodeKinds;  // BVH8: 2 per node (leaf primitive kind bit planes)
   std::vector<F4> prims;   // 3 per primitive, leaf order (record order when recordsOnly)
   std::vector<F4> boxes;   // recordsOnly: 2 per primitive (lo.xyz, hi.xyz), conservative FP32
   std::vector<F4> sphColor;

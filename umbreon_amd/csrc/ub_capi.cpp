@@ -241,8 +241,10 @@ int ubg_scene_create_ex(ubg_context* ctx, const ubg_scene_desc* desc, int32_t bv
     if (r == UBG_OK) {
       s->allocs.push_back(db.nodes);
       s->allocs.push_back(db.prims);
+      s->allocs.push_back(db.kinds);
       s->view.nodes = db.nodes;
       s->view.prims = db.prims;
+      s->view.nodeKinds = db.kinds;
       hs.nNodes = db.nNodes;
       hs.depth = db.depth;
       hs.stackDepth = db.depth;
@@ -259,10 +261,11 @@ int ubg_scene_create_ex(ubg_context* ctx, const ubg_scene_desc* desc, int32_t bv
       }
     }
     s->info.device_bytes -= static_cast<int64_t>((hs.prims.size() + hs.boxes.size()) * sizeof(ubg::F4));
-    if (r == UBG_OK) s->info.device_bytes += db.nNodes * 80 + static_cast<int64_t>(hs.prims.size() / 3) * 48;
+    if (r == UBG_OK) s->info.device_bytes += db.nNodes * 88 + static_cast<int64_t>(hs.prims.size() / 3) * 48;
   } else {
     up(hs.nodes, &s->view.nodes);
     up(hs.prims, &s->view.prims);
+    up(hs.nodeKinds, &s->view.nodeKinds);
   }
   up(hs.sphColor, &s->view.sphColor);
   up(hs.sphInfo, &p);

@@ -16,6 +16,7 @@ namespace ubg {
 struct DeviceBvh8 {
   void* nodes = nullptr;  // 80 B per node
   void* prims = nullptr;  // 48 B per primitive, leaf order
+  void* kinds = nullptr;  // 8 B per node: leaf primitive kind bit planes
   int64_t nNodes = 0;
   int32_t depth = 0;      // levels of 8-wide nodes
   double sahCost = 0.0;   // SAH of the binary tree (traversal 1, primitive 1), as the host builder reports

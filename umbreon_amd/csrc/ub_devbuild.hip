@@ -313,7 +313,7 @@ __device__ __forceinline__ uint32_t waveReserve(unsigned int* c, uint32_t v) {
 // One level of 8-wide nodes [begin, end).
 __global__ __launch_bounds__(64) void k_emit(int32_t begin, int32_t end, int32_t* srcOf, const Tree t, int32_t n,
                                              const int32_t* perm, const float4* recs, Node8* nodes, float4* prims,
-                                             unsigned int* counters) {
+                                             uint2* kinds, unsigned int* counters) {
   const int32_t k = begin + static_cast<int32_t>(blockIdx.x * 64 + threadIdx.x);
   const bool valid = k < end;
   int32_t ch[8];
@@ -415,6 +415,7 @@ __global__ __launch_bounds__(64) void k_emit(int32_t begin, int32_t end, int32_t
   nd.childBase = childBase;
   nd.primBase = primBase;
   uint32_t inner = 0, off = 0;
+  uint2 kw = make_uint2(0u, 0u);
   for (int sl = 0; sl < 8; ++sl) {
     const int j = slotChild[sl];
     nd.meta[sl] = 0;
@@ -445,9 +446,13 @@ __global__ __launch_bounds__(64) void k_emit(int32_t begin, int32_t end, int32_t
         if (x < n) {
           const int32_t pr = perm[x];
           float4* dst = prims + 3 * static_cast<size_t>(primBase + off + q);
+          const float4 r2 = recs[3 * static_cast<size_t>(pr) + 2];
           dst[0] = recs[3 * static_cast<size_t>(pr)];
           dst[1] = recs[3 * static_cast<size_t>(pr) + 1];
-          dst[2] = recs[3 * static_cast<size_t>(pr) + 2];
+          dst[2] = r2;
+          const uint32_t kind = __float_as_uint(r2.w) >> kKindShift;
+          kw.x |= (kind & 1u) << (off + q);
+          kw.y |= (kind >> 1) << (off + q);
           ++q;
         } else {
           st[sp++] = t.right[x];
@@ -463,6 +468,7 @@ __global__ __launch_bounds__(64) void k_emit(int32_t begin, int32_t end, int32_t
     }
   }
   nodes[k] = nd;
+  kinds[k] = kw;
 }
 
 struct DevMem {
@@ -591,18 +597,19 @@ int deviceBuildBvh8(const float4* recs, const float4* boxes, int64_t n, hipStrea
   unsigned int* counters = nullptr;
   void* rawNodes = nullptr;
   void* rawPrims = nullptr;
-  if (!ok(hipMalloc(&rawNodes, std::max<int64_t>(1, n) * sizeof(Node8)), "alloc nodes")) return UBG_ERR_OUT_OF_MEMORY;
-  if (!ok(hipMalloc(&rawPrims, 3 * n * sizeof(float4)), "alloc prims")) {
-    (void)hipFree(rawNodes);
-    return UBG_ERR_OUT_OF_MEMORY;
-  }
-  nodes = static_cast<Node8*>(rawNodes);
-  prims = static_cast<float4*>(rawPrims);
+  void* rawKinds = nullptr;
   auto failOut = [&](int status) {
-    (void)hipFree(rawNodes);
-    (void)hipFree(rawPrims);
+    if (rawNodes) (void)hipFree(rawNodes);
+    if (rawPrims) (void)hipFree(rawPrims);
+    if (rawKinds) (void)hipFree(rawKinds);
     return status;
   };
+  if (!ok(hipMalloc(&rawNodes, std::max<int64_t>(1, n) * sizeof(Node8)), "alloc nodes") ||
+      !ok(hipMalloc(&rawPrims, 3 * n * sizeof(float4)), "alloc prims") ||
+      !ok(hipMalloc(&rawKinds, std::max<int64_t>(1, n) * sizeof(uint2)), "alloc kinds"))
+    return failOut(UBG_ERR_OUT_OF_MEMORY);
+  nodes = static_cast<Node8*>(rawNodes);
+  prims = static_cast<float4*>(rawPrims);
   if (!ok(mem.alloc(&srcOf, std::max<int64_t>(1, n)), "alloc") || !ok(mem.alloc(&counters, 2), "alloc"))
     return failOut(UBG_ERR_OUT_OF_MEMORY);
   const unsigned int init[2] = {1u, 0u};
@@ -613,7 +620,7 @@ int deviceBuildBvh8(const float4* recs, const float4* boxes, int64_t n, hipStrea
   unsigned int hostCnt[2] = {1u, 0u};
   while (begin < end && e == hipSuccess) {
     hipLaunchKernelGGL(k_emit, dim3(gridFor(end - begin, 64)), dim3(64), 0, st, begin, end, srcOf, t,
-                       static_cast<int32_t>(n), perm, recs, nodes, prims, counters);
+                       static_cast<int32_t>(n), perm, recs, nodes, prims, static_cast<uint2*>(rawKinds), counters);
     ok(hipGetLastError(), "launch");
     ok(hipMemcpyAsync(hostCnt, counters, sizeof(hostCnt), hipMemcpyDeviceToHost, st), "copy");
     ok(hipStreamSynchronize(st), "level");
@@ -633,6 +640,7 @@ int deviceBuildBvh8(const float4* recs, const float4* boxes, int64_t n, hipStrea
   const auto t1 = std::chrono::steady_clock::now();
   out.nodes = rawNodes;
   out.prims = rawPrims;
+  out.kinds = rawKinds;
   out.nNodes = end;
   out.depth = depth;
   // the host builder's figure: sum over binary nodes of area / root area

@@ -578,6 +578,7 @@ struct Trav8 {
   V3 o, d, iv;
   float tnear, tmax;
   uint32_t oct, gBase, gHits, tBase, tHits;
+  uint32_t kLo, kHi;  // kind bit planes of the pending primitives (trav8Node<.., true>)
   int sp;
 };
 
@@ -592,6 +593,8 @@ UBG_HD void trav8Init(Trav8& t, V3 o, V3 d, float tnear, float tfar) {
   t.gHits = (1u << (24 + t.oct)) | 1u;
   t.tBase = 0;
   t.tHits = 0;
+  t.kLo = 0;
+  t.kHi = 0;
   t.sp = 0;
 }
 
@@ -599,8 +602,9 @@ UBG_HD void trav8Init(Trav8& t, V3 o, V3 d, float tnear, float tfar) {
 UBG_HD bool trav8HasNodes(const Trav8& t) { return (t.gHits & 0xFF000000u) != 0u || t.sp != 0; }
 
 // One node test (requires no pending primitives and trav8HasNodes). Returns
-// -1 when the traversal is exhausted, else 0.
-template <bool kCount>
+// -1 when the traversal is exhausted, else 0. kKinds also fetches the kinds
+// of the node's leaf primitives (for trav8PrimKind).
+template <bool kCount, bool kKinds = false>
 UBG_HD int trav8Node(const SceneView& s, Trav8& t, uint32_t* stkBase, uint32_t* stkHits, int stride,
                      uint32_t* nodeVisits) {
   if ((t.gHits & 0xFF000000u) == 0u) {
@@ -620,6 +624,48 @@ UBG_HD int trav8Node(const SceneView& s, Trav8& t, uint32_t* stkBase, uint32_t* 
   if (kCount) ++*nodeVisits;
   t.gHits = intersectNode8<kCount>(reinterpret_cast<const uint4*>(s.nodes), idx, t.iv, t.o, t.tnear, t.tmax, t.oct,
                                    t.gBase, t.tBase, t.tHits);
+  if (kKinds && t.tHits != 0u) {
+    const uint32_t* kw = reinterpret_cast<const uint32_t*>(s.nodeKinds) + 2 * static_cast<size_t>(idx);
+    t.kLo = kw[0];
+    t.kHi = kw[1];
+  }
+  return (t.tHits == 0u && !trav8HasNodes(t)) ? -1 : 0;
+}
+
+// Pending primitives of kind `kind` (GeomKind numbering: mesh 0, sphere 1,
+// round curve 2, cone curve 3).
+UBG_HD uint32_t trav8KindMask(const Trav8& t, uint32_t kind) {
+  const uint32_t lo = (kind & 1u) ? t.kLo : ~t.kLo;
+  const uint32_t hi = (kind & 2u) ? t.kHi : ~t.kHi;
+  return t.tHits & lo & hi;
+}
+
+// One any-hit test of a pending primitive of kind kKind (requires one; any
+// order of primitive tests gives the same any-hit answer). Returns 1 when it
+// occludes the ray, -1 when nothing is left to visit, else 0.
+template <int kKind, bool kCount>
+UBG_HD int trav8PrimKind(const SceneView& s, Trav8& t, uint32_t* primTests) {
+  const uint32_t b = static_cast<uint32_t>(__builtin_ctz(trav8KindMask(t, kKind)));
+  t.tHits &= ~(1u << b);
+  if (kCount) ++*primTests;
+  const float4* p = reinterpret_cast<const float4*>(s.prims) + 3 * static_cast<size_t>(t.tBase + b);
+  const float4 p0 = p[0];
+  Hit h;
+  bool r;
+  if (kKind == 1) {
+    r = isectSphere(t.o, t.d, t.tnear, t.tmax, ld3(p0), p0.w, h);
+  } else if (kKind == 3) {
+    const float4 p1 = p[1];
+    r = isectCone(t.o, t.d, t.tnear, t.tmax, ld3(p0), p0.w, ld3(p1), p1.w, h);
+  } else if (kKind == 0) {
+    const float4 p1 = p[1], p2 = p[2];
+    r = isectTri(t.o, t.d, t.tnear, t.tmax, ld3(p0), mk(p0.w, p1.x, p1.y), mk(p1.z, p1.w, p2.x), h);
+  } else {
+    const float4 p1 = p[1], p2 = p[2];
+    const float4* pl = reinterpret_cast<const float4*>(s.rndPlanes) + 2 * (f2u(p2.w) & kIdMask);
+    r = isectRound(t.o, t.d, t.tnear, t.tmax, ld3(p0), p0.w, ld3(p1), p1.w, pl[0], pl[1], h);
+  }
+  if (r) return 1;
   return (t.tHits == 0u && !trav8HasNodes(t)) ? -1 : 0;
 }
 

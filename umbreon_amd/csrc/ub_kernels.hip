@@ -29,8 +29,10 @@
 namespace ubg {
 
 constexpr int kTraceBlock = 128;
-// Waves per SIMD the traversal kernels are register-limited to (<= 72 VGPRs).
+// Waves per SIMD the traversal kernels are register-limited to: k_primary to
+// 7 (<= 72 VGPRs), the persistent AO kernel to 6 (<= 84 VGPRs, no spills).
 constexpr int kTraceWavesPerSimd = 7;
+constexpr int kAoWavesPerSimd = 6;
 
 // Dynamic LDS: the per-lane traversal stacks, sized at launch from the scene's
 // tree depth (BVH2: one u32 node index per entry; BVH8: a u32 base array and
@@ -388,7 +390,7 @@ constexpr int kAoChunk = 256;
 constexpr int kAoRefill = 16;
 
 template <bool kCount>
-__global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(kTraceWavesPerSimd))) void k_ao_persistent(
+__global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(kAoWavesPerSimd))) void k_ao_persistent(
     FrameConst fc, SceneView s, FrameBufs b) {
   const uint32_t n = static_cast<uint32_t>(fc.aoSamples);
   uint32_t* sb = ubgLds + threadIdx.x;
@@ -463,16 +465,32 @@ __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(kTr
       }
     }
     if (act == 0ull && drained) break;
-    // The wave runs either a primitive step or a node step, never both: it
-    // takes primitives once their lanes number at least half the node lanes.
+    // Each iteration runs one kind of step for the whole wave: node tests, or
+    // the tests of one primitive kind (the kind most lanes have pending; any
+    // order of primitive tests gives the same any-hit answer). Primitives go
+    // first once their lanes number at least half the node lanes.
     const bool wantPrim = active && t.tHits != 0u;
     const int nPrim = __popcll(__ballot(wantPrim));
     const int nNode = __popcll(__ballot(active && !wantPrim));
     int r = 0;
     if (2 * nPrim >= nNode) {
-      if (wantPrim) r = trav8Prim<kCount>(s, t, &nt);
+      const unsigned long long ms = __ballot(wantPrim && trav8KindMask(t, 1u) != 0u);
+      const unsigned long long mc = __ballot(wantPrim && trav8KindMask(t, 3u) != 0u);
+      const unsigned long long mt = __ballot(wantPrim && trav8KindMask(t, 0u) != 0u);
+      const unsigned long long mr = __ballot(wantPrim && trav8KindMask(t, 2u) != 0u);
+      const int cs = __popcll(ms), cc = __popcll(mc), ct = __popcll(mt), cr = __popcll(mr);
+      const uint64_t me = 1ull << __lane_id();
+      if (cs >= cc && cs >= ct && cs >= cr) {
+        if (ms & me) r = trav8PrimKind<1, kCount>(s, t, &nt);
+      } else if (cc >= ct && cc >= cr) {
+        if (mc & me) r = trav8PrimKind<3, kCount>(s, t, &nt);
+      } else if (ct >= cr) {
+        if (mt & me) r = trav8PrimKind<0, kCount>(s, t, &nt);
+      } else {
+        if (mr & me) r = trav8PrimKind<2, kCount>(s, t, &nt);
+      }
     } else {
-      if (active && !wantPrim) r = trav8Node<kCount>(s, t, sb, sh, stride, &nv);
+      if (active && !wantPrim) r = trav8Node<kCount, true>(s, t, sb, sh, stride, &nv);
     }
     if (r != 0) {
       active = false;
@@ -802,7 +820,7 @@ void launchAo(const FrameConst& fc, const SceneView& s, const FrameBufs& b, bool
   if (s.layout == kLayoutBvh8) {
     // persistent grid: every wave slot of the chip, no more than the work
     const int64_t rays = slots * fc.aoSamples;
-    const int64_t resident = (int64_t)computeUnits() * 4 * kTraceWavesPerSimd * 64 / kTraceBlock;
+    const int64_t resident = (int64_t)computeUnits() * 4 * kAoWavesPerSimd * 64 / kTraceBlock;
     const int64_t blocks = std::min<int64_t>(blocksFor(rays, kTraceBlock), resident);
     const size_t lds = stackLdsBytes(s, kTraceBlock);
     if (countVisits)
@@ -862,6 +880,7 @@ extern "C" int ubgx_host_query(const ubg_scene_desc* desc, int32_t bvh_layout, i
   v.layout = hs.layout;
   v.stackDepth = hs.stackDepth;
   v.rndPlanes = hs.rndPlanes.data();
+  v.nodeKinds = hs.nodeKinds.empty() ? nullptr : hs.nodeKinds.data();
   std::vector<uint32_t> s0(static_cast<size_t>(hs.stackDepth) + 1), s1(static_cast<size_t>(hs.stackDepth) + 1);
   uint64_t visits = 0, tests = 0;
   for (int64_t i = 0; i < n_rays; ++i) {
@@ -915,7 +934,7 @@ extern "C" int ubgx_host_query(const ubg_scene_desc* desc, int32_t bvh_layout, i
         // the one-shot any-hit traversal and the resumable step machine must
         // agree; a disagreement is reported as 2
         const bool o1 = traverse8<true, false>(v, o, d, r.tnear, r.tfar, h, s0.data(), s1.data(), 1, &a, &b);
-        bool o2 = false;
+        bool o2 = false, o3 = false;
         if (v.nNodes > 0) {
           Trav8 t;
           trav8Init(t, o, d, r.tnear, r.tfar);
@@ -923,8 +942,23 @@ extern "C" int ubgx_host_query(const ubg_scene_desc* desc, int32_t bvh_layout, i
           while ((rs = trav8StepAny<false>(v, t, s0.data(), s1.data(), 1, &a, &b)) == 0) {
           }
           o2 = rs > 0;
+          // the AO kernel's order: primitives by kind (cones first here, to
+          // differ from the node order), nodes with their kind planes
+          trav8Init(t, o, d, r.tnear, r.tfar);
+          for (;;) {
+            if (t.tHits != 0u) {
+              rs = trav8KindMask(t, 3u)   ? trav8PrimKind<3, false>(v, t, &b)
+                   : trav8KindMask(t, 1u) ? trav8PrimKind<1, false>(v, t, &b)
+                   : trav8KindMask(t, 2u) ? trav8PrimKind<2, false>(v, t, &b)
+                                          : trav8PrimKind<0, false>(v, t, &b);
+            } else {
+              rs = trav8Node<false, true>(v, t, s0.data(), s1.data(), 1, &a);
+            }
+            if (rs != 0) break;
+          }
+          o3 = rs > 0;
         }
-        occ[i] = o1 == o2 ? (o1 ? 1 : 0) : 2;
+        occ[i] = (o1 == o2 && o1 == o3) ? (o1 ? 1 : 0) : 2;
       } else {
         occ[i] = traverse2<true, false>(v, o, d, r.tnear, r.tfar, h, s0.data(), 1, &a, &b) ? 1 : 0;
       }

@@ -67,6 +67,9 @@ static_assert(sizeof(Node8) == 80, "BVH8 node is 80 bytes");
 
 struct SceneView {
   const void* nodes;      // BVH2: float4 x 4 per inner node; BVH8: uint4 x 5 per node
+  const void* nodeKinds;  // BVH8: uint2 per node, the kinds of its leaf primitives as two
+                          // bit planes (bit i of .x / .y = bit 0 / 1 of the kind of primitive
+                          // primBase + i); null for BVH2
   const void* prims;      // float4 x 3 per primitive, in the layout's leaf order
   int32_t nNodes;
   int32_t nPrims;
