@@ -39,7 +39,7 @@
 extern "C" {
 #endif
 
-#define UBG_ABI_VERSION 1
+#define UBG_ABI_VERSION 2
 
 /* Status codes. */
 enum {
@@ -181,6 +181,22 @@ typedef struct ubg_render_params {
   /* Cooperative cancel (RenderProgress::requestCancel): polled between
    * wavefront passes; a non-zero value stops the frame (UBG_ERR_CANCELLED). */
   const volatile int32_t* cancel_flag;
+  /* Enhanced AO (render_options.hpp:52-61, ambient_occlusion.hpp:98-267,
+   * ao_shade.hpp:44-141). Any of falloff > 0, multi_scale, bent_normal,
+   * multibounce, low_discrepancy or diffuse_factor > 0 switches AO to the
+   * nearest-hit estimator; all zero keeps the legacy binary estimator. The sky
+   * and ground colours and the gradient axis are read only with bent_normal;
+   * ao_up is used when ao_use_camera_up == 0 (else the camera's true up). */
+  float ao_falloff_power;
+  int32_t ao_multi_scale;
+  int32_t ao_bent_normal;
+  float ao_sky_color[3];
+  float ao_ground_color[3];
+  int32_t ao_use_camera_up;
+  float ao_up[3];
+  int32_t ao_multibounce;
+  int32_t ao_low_discrepancy;
+  float ao_diffuse_factor;
 } ubg_render_params;
 
 /* Ray accounting of one frame. A pure function of the scene and options:

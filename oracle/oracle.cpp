@@ -924,6 +924,7 @@ struct Ctx {
   const ubg_render_params* p;
   std::vector<Light<T>> lights;
   V3<T> ambLight, bg, camDir;
+  V3<T> aoUp;  // resolved bent-normal gradient axis (embree_renderer.cpp:1320-1323)
   int wHi;
 };
 
@@ -985,6 +986,146 @@ inline T computeAO(const Ctx<T>& c, V3<T> P, V3<T> Ng, V3<T> Ns, T eps, int nSam
     if (occluded(*c.s, O, dir, eps, aoRadius)) ++hits;
   }
   return T(1) - T(hits) / T(nSamples);
+}
+
+// ambient_occlusion.hpp:96-104 -- base-2 radical inverse (bit reversal)
+inline uint32_t reverseBits32(uint32_t b) {
+  b = (b << 16) | (b >> 16);
+  b = ((b & 0x55555555u) << 1) | ((b & 0xAAAAAAAAu) >> 1);
+  b = ((b & 0x33333333u) << 2) | ((b & 0xCCCCCCCCu) >> 2);
+  b = ((b & 0x0F0F0F0Fu) << 4) | ((b & 0xF0F0F0F0u) >> 4);
+  b = ((b & 0x00FF00FFu) << 8) | ((b & 0xFF00FF00u) >> 8);
+  return b;
+}
+
+// ambient_occlusion.hpp:111-125 -- the i-th AO sample: the tea2 stream, or
+// the Hammersley point (i/N, radicalInverse2(i)) shifted by (cpx, cpy) mod 1
+template <class T>
+inline void aoSample2d(bool lowDiscrepancy, uint32_t base, int i, int n, T cpx, T cpy, T& u1, T& u2) {
+  if (lowDiscrepancy) {
+    u1 = T(static_cast<float>(i)) / T(static_cast<float>(n)) + cpx;
+    u2 = T(static_cast<float>(reverseBits32(static_cast<uint32_t>(i)))) * T(0x1.0p-32f) + cpy;
+    if (u1 >= T(1)) u1 -= T(1);
+    if (u2 >= T(1)) u2 -= T(1);
+  } else {
+    uint32_t s0 = base, s1 = static_cast<uint32_t>(i);
+    tea2(s0, s1);
+    u1 = u32ToUnorm<T>(s0);
+    u2 = u32ToUnorm<T>(s1);
+  }
+}
+
+// ambient_occlusion.hpp:133-138 -- GTAO multibounce lift of one channel
+template <class T>
+inline T aoMultibounce(T ao, T albedo) {
+  const T a = T(2.0404f) * albedo - T(0.3324f);
+  const T b = T(-4.7951f) * albedo + T(0.6417f);
+  const T cc = T(2.7552f) * albedo + T(0.6903f);
+  return std::fmax(ao, ((ao * a + b) * ao + cc) * ao);
+}
+
+template <class T>
+struct AOQuality {
+  T contact = T(1), shape = T(1), openness = T(1);
+  V3<T> bent{T(0), T(0), T(0)};
+  T avgHitDist = T(0);
+};
+
+template <class T>
+inline bool aoEnhanced(const ubg_render_params* p) {
+  return p->ao_falloff_power > 0.0f || p->ao_multi_scale || p->ao_bent_normal || p->ao_multibounce ||
+         p->ao_low_discrepancy || p->ao_diffuse_factor > 0.0f;
+}
+
+// ambient_occlusion.hpp:171-264 -- nearest-hit multi-scale / falloff / bent
+// normal estimator from one cosine ray set
+template <class T>
+inline AOQuality<T> computeAOQuality(const Ctx<T>& c, V3<T> P, V3<T> Ng, V3<T> Ns, T eps, int nSamples,
+                                     uint32_t px, uint32_t py, Counters& cnt) {
+  const ubg_render_params* p = c.p;
+  AOQuality<T> r;
+  if (nSamples <= 0) {
+    r.bent = Ns;
+    return r;
+  }
+  const Frame<T> f = frameFromNormal(Ns);
+  V3<T> ng = (dot(Ng, Ns) < T(0)) ? neg(Ng) : Ng;
+  ng = safeNormalize(ng, Ns);
+  const V3<T> O = P + ng * eps;
+  const uint32_t base = px + py * static_cast<uint32_t>(c.wHi);
+  const T R = T(p->ao_distance);
+  T radius[3], weight[3];
+  int K;
+  if (p->ao_multi_scale) {
+    K = 3;
+    radius[0] = T(0.08f) * R;
+    radius[1] = T(0.30f) * R;
+    radius[2] = T(1.00f) * R;
+    weight[0] = T(0.55f);
+    weight[1] = T(0.35f);
+    weight[2] = T(0.10f);
+  } else {
+    K = 1;
+    radius[0] = radius[1] = radius[2] = R;
+    weight[0] = T(1);
+    weight[1] = weight[2] = T(0);
+  }
+  uint32_t c0 = px + 1u, c1 = py + 1u;
+  tea2(c0, c1);
+  const T cpx = u32ToUnorm<T>(c0), cpy = u32ToUnorm<T>(c1);
+  T obsScale[3] = {T(0), T(0), T(0)};
+  T weightedObs = T(0);
+  V3<T> bentAccum{T(0), T(0), T(0)};
+  T hitDistSum = T(0);
+  int nHit = 0;
+  const T power = T(p->ao_falloff_power);
+  for (int i = 0; i < nSamples; ++i) {
+    T u1, u2;
+    aoSample2d(p->ao_low_discrepancy != 0, base, i, nSamples, cpx, cpy, u1, u2);
+    const V3<T> dir = cosineSampleHemisphere(u1, u2, f);
+    if (dot(dir, Ns) < T(0.01f)) {
+      cnt.aoSampled++;
+      obsScale[0] += T(1);
+      obsScale[1] += T(1);
+      obsScale[2] += T(1);
+      weightedObs += T(1);
+      continue;
+    }
+    cnt.aoTraced++;
+    const Hit<T> h = intersect(*c.s, O, dir, eps, R);
+    const T t = h.kind >= 0 ? h.t : std::numeric_limits<T>::infinity();
+    T perSample = T(0);
+    for (int k = 0; k < K; ++k) {
+      if (t < radius[k]) {
+        T fk = T(1);
+        if (power > T(0)) {
+          T a = T(1) - t / radius[k];
+          if (a < T(0)) a = T(0);
+          fk = std::pow(a, power);
+        }
+        obsScale[k] += fk;
+        perSample += weight[k] * fk;
+      }
+    }
+    weightedObs += perSample;
+    bentAccum = bentAccum + dir * (T(1) - perSample);
+    if (t < R) {
+      hitDistSum += t;
+      ++nHit;
+    }
+  }
+  const T invN = T(1) / T(static_cast<float>(nSamples));
+  r.openness = T(1) - weightedObs * invN;
+  r.contact = T(1) - obsScale[0] * invN;
+  if (p->ao_multi_scale) {
+    const T wMid = weight[1] + weight[2];
+    r.shape = T(1) - (weight[1] * obsScale[1] + weight[2] * obsScale[2]) / (wMid * T(static_cast<float>(nSamples)));
+  } else {
+    r.shape = r.openness;
+  }
+  r.bent = safeNormalize(bentAccum, Ns);
+  r.avgHitDist = nHit > 0 ? hitDistSum / T(static_cast<float>(nHit)) : R;
+  return r;
 }
 
 // shading.hpp:76-192 (POV model)
@@ -1062,12 +1203,54 @@ struct HitShade {
   T aoOpen;
 };
 
-// ao_shade.hpp:128-142 legacy path -> (aoFactor, openness)
 template <class T>
-inline T aoOpenness(const Ctx<T>& c, V3<T> P, V3<T> Ng, V3<T> N, T eps, uint32_t px, uint32_t py,
-                    Counters& cnt) {
-  if (c.p->ao_samples <= 0) return T(1);
-  return computeAO(c, P, Ng, N, eps, c.p->ao_samples, T(c.p->ao_distance), px, py, cnt);
+struct AoShadeT {
+  V3<T> aoFactor{T(1), T(1), T(1)};
+  V3<T> ambLight;
+  T diffuseAo = T(1);
+  T openness = T(1);
+};
+
+// ao_shade.hpp:44-141 -- gather (legacy binary or enhanced), then the shading
+// factors: bent-normal sky/ground ambient, per-channel (multibounce) aoFactor,
+// direct-diffuse darkening
+template <class T>
+inline AoShadeT<T> aoShade(const Ctx<T>& c, V3<T> P, V3<T> Ng, V3<T> N, V3<T> C, T eps, uint32_t px,
+                           uint32_t py, Counters& cnt) {
+  const ubg_render_params* p = c.p;
+  AoShadeT<T> r;
+  r.ambLight = c.ambLight;
+  if (p->ao_samples <= 0) return r;
+  const bool enh = aoEnhanced<T>(p);
+  T openness;
+  V3<T> bent = N;
+  if (enh) {
+    const AOQuality<T> q = computeAOQuality(c, P, Ng, N, eps, p->ao_samples, px, py, cnt);
+    openness = q.openness;
+    bent = q.bent;
+  } else {
+    openness = computeAO(c, P, Ng, N, eps, p->ao_samples, T(p->ao_distance), px, py, cnt);
+  }
+  r.openness = openness;
+  if (enh && p->ao_bent_normal) {
+    const T w = T(0.5f) * (dot(bent, c.aoUp) + T(1));
+    const T gx = T(p->ao_ground_color[0]), sx = T(p->ao_sky_color[0]);
+    const T gy = T(p->ao_ground_color[1]), sy = T(p->ao_sky_color[1]);
+    const T gz = T(p->ao_ground_color[2]), sz = T(p->ao_sky_color[2]);
+    r.ambLight = V3<T>{c.ambLight.x * (gx + (sx - gx) * w), c.ambLight.y * (gy + (sy - gy) * w),
+                       c.ambLight.z * (gz + (sz - gz) * w)};
+  }
+  const T I = T(p->ao_intensity);
+  if (p->ao_multibounce) {
+    const T ox = aoMultibounce(openness, C.x), oy = aoMultibounce(openness, C.y),
+            oz = aoMultibounce(openness, C.z);
+    r.aoFactor = V3<T>{T(1) - I * (T(1) - ox), T(1) - I * (T(1) - oy), T(1) - I * (T(1) - oz)};
+  } else {
+    const T s = T(1) - I * (T(1) - openness);
+    r.aoFactor = V3<T>{s, s, s};
+  }
+  if (p->ao_diffuse_factor > 0.0f) r.diffuseAo = T(1) - T(p->ao_diffuse_factor) * (T(1) - openness);
+  return r;
 }
 
 // hit_shader.hpp:147-496
@@ -1103,10 +1286,9 @@ inline HitShade<T> shadeHit(const Ctx<T>& c, const Hit<T>& h, V3<T> rd, V3<T> or
     const Mat& mat = g.triMat.empty() ? g.meshMats[0] : g.meshMats[g.triMat[baseTri]];
     const V3<T> P{org.x + rd.x * h.t, org.y + rd.y * h.t, org.z + rd.z * h.t};
     const T secEps = selfIntersectEps(P, rd, h.t);
-    const T open = aoOpenness(c, P, Ng, N, secEps, px, py, cnt);
-    const T s = T(1) - T(c.p->ao_intensity) * (T(1) - open);
-    hs.aoOpen = open;
-    hs.color = shadeLocal(c, mat, C, N, V, c.ambLight, V3<T>{s, s, s}, T(1), P, Ng, secEps,
+    const AoShadeT<T> ao = aoShade(c, P, Ng, N, C, secEps, px, py, cnt);
+    hs.aoOpen = ao.openness;
+    hs.color = shadeLocal(c, mat, C, N, V, ao.ambLight, ao.aoFactor, ao.diffuseAo, P, Ng, secEps,
                           shadowsActive, c.p->shadow_samples, px, py, cnt);
     hs.opacity = cb[3];
     return hs;
@@ -1121,16 +1303,12 @@ inline HitShade<T> shadeHit(const Ctx<T>& c, const Hit<T>& h, V3<T> rd, V3<T> or
   const V3<T> Ng = h.ng;
   const T secEps = selfIntersectEps(P, rd, h.t);
   const bool fromEdge = side->fromEdge != 0;
-  V3<T> aoFactor{T(1), T(1), T(1)};
-  hs.aoOpen = T(1);
-  if (!fromEdge) {
-    const T open = aoOpenness(c, P, Ng, N, secEps, px, py, cnt);
-    const T s = T(1) - T(c.p->ao_intensity) * (T(1) - open);
-    aoFactor = V3<T>{s, s, s};
-    hs.aoOpen = open;
-  }
+  AoShadeT<T> ao;
+  ao.ambLight = c.ambLight;
+  if (!fromEdge) ao = aoShade(c, P, Ng, N, C, secEps, px, py, cnt);
+  hs.aoOpen = ao.openness;
   const bool primShadows = !fromEdge && shadowsActive;
-  hs.color = shadeLocal(c, pm, C, N, V, c.ambLight, aoFactor, T(1), P, Ng, secEps, primShadows,
+  hs.color = shadeLocal(c, pm, C, N, V, ao.ambLight, ao.aoFactor, ao.diffuseAo, P, Ng, secEps, primShadows,
                         c.p->shadow_samples, px, py, cnt);
   hs.opacity = T(side->color[3]);
   if (h.kind != UBG_KIND_SPHERE && side->opacity1 >= 0.0f) {
@@ -1287,6 +1465,7 @@ static int renderT(const ubg_scene_desc* desc, const ubg_render_params* p, ubg_f
     c.lights.push_back(l);
   }
   c.ambLight = v3<T>(p->ambient_color);
+  c.aoUp = p->ao_use_camera_up ? trueUp : safeNormalize(v3<T>(p->ao_up), trueUp);
   c.bg = v3<T>(p->background);
   const V3<T> pos = v3<T>(cam.position);
 

@@ -367,6 +367,106 @@ def ao_determinism():
     return Case("ao_determinism", {"a": (sc, o), "b": (sc, o)}, chk)
 
 
+# ---------------------------------------------------------------- enhanced AO
+def _ao_floor(slab_z=None, gray=1.0, mat=None):
+    p = [(-2, -2, 0), (2, -2, 0), (2, 2, 0), (-2, -2, 0), (2, 2, 0), (-2, 2, 0)]
+    n = [(0, 0, 1)] * 6
+    c = [(gray, gray, gray, 1)] * 6
+    if slab_z is not None:
+        z = slab_z
+        p += [(0.1, -2, z), (4, -2, z), (4, 2, z), (0.1, -2, z), (4, 2, z), (0.1, 2, z)]
+        n += [(0, 0, -1)] * 6
+        c += [(0, 0, 0, 1)] * 6
+    sc = Scene()
+    sc.set_mesh(p, n, c, material=mat or Material(ambient=1.0, diffuse=0.0))
+    sc.camera = ortho_cam()
+    sc.ambient_color = (1, 1, 1)
+    sc.background = (0, 0, 0)
+    return sc
+
+
+@case
+def ao_falloff_multiscale():
+    # test_render_ao.cpp:170-231 (distance falloff and multi-scale vs binary)
+    def o(falloff, ms):
+        return opts(ao_samples=256, ao_distance=10.0, ao_falloff_power=falloff, ao_multi_scale=ms)
+    r = {"binNear": (_ao_floor(0.4), o(0.0, False)), "binFar": (_ao_floor(3.0), o(0.0, False)),
+         "foNear": (_ao_floor(0.4), o(2.0, False)), "foFar": (_ao_floor(3.0), o(2.0, False)),
+         "msFar": (_ao_floor(4.0), o(0.0, True)), "msNear": (_ao_floor(0.4), o(0.0, True)),
+         "binFar4": (_ao_floor(4.0), o(0.0, False))}
+
+    def chk(f):
+        v = {k: float(f[k].color[C][0]) for k in f}
+        assert v["foNear"] >= v["binNear"] - 1e-4 and v["foFar"] >= v["binFar"] - 1e-4, v
+        assert (v["foFar"] - v["binFar"]) > (v["foNear"] - v["binNear"]) + 0.05, v
+        assert v["msFar"] > v["binFar4"] + 0.05 and v["msNear"] < v["msFar"], v
+    return Case("ao_falloff_multiscale", r, chk)
+
+
+@case
+def ao_bent_normal():
+    # test_render_ao.cpp:233-278 (sky/ground gradient along the bent normal)
+    def o(up):
+        return opts(ao_samples=64, ao_distance=10.0, ao_bent_normal=True, ao_sky_color=(1, 1, 1),
+                    ao_ground_color=(0, 0, 0), ao_use_camera_up=False, ao_up=up)
+    r = {"plus": (_ao_floor(), o((0, 0, 1))), "minus": (_ao_floor(), o((0, 0, -1))),
+         "perp": (_ao_floor(), o((1, 0, 0)))}
+
+    def chk(f):
+        up, um, pp = (float(f[k].color[C][0]) for k in ("plus", "minus", "perp"))
+        assert up > 0.9 and um < 0.1, (up, um)
+        approx(pp, 0.5, 0.1)
+        assert um < pp < up
+    return Case("ao_bent_normal", r, chk)
+
+
+@case
+def ao_multibounce():
+    # test_render_ao.cpp:280-336 (albedo-aware lift)
+    def o(mb):
+        return opts(ao_samples=256, ao_distance=10.0, ao_multibounce=mb)
+    r = {"openW": (_ao_floor(None, 1.0), o(True)), "occW": (_ao_floor(0.4, 1.0), o(True)),
+         "openG": (_ao_floor(None, 0.5), o(True)), "occG": (_ao_floor(0.4, 0.5), o(True)),
+         "occWplain": (_ao_floor(0.4, 1.0), o(False))}
+
+    def chk(f):
+        v = {k: float(f[k].color[C][0]) for k in f}
+        assert v["occW"] / v["openW"] > v["occG"] / v["openG"] + 0.02, v
+        assert v["occW"] > v["occWplain"] + 0.01, v
+    return Case("ao_multibounce", r, chk)
+
+
+@case
+def ao_diffuse_factor():
+    # test_render_ao.cpp:338-380 (aoDiffuseFactor darkens occluded direct diffuse)
+    def scene():
+        sc = _ao_floor(0.4, 1.0, Material(ambient=0.2, diffuse=0.8))
+        sc.lights = [key_light()]
+        return sc
+    r = {"on": (scene(), opts(ao_samples=256, ao_distance=10.0, ao_diffuse_factor=0.7)),
+         "off": (scene(), opts(ao_samples=256, ao_distance=10.0, ao_diffuse_factor=0.0))}
+
+    def chk(f):
+        assert f["on"].color[C][0] + 0.01 < f["off"].color[C][0]
+    return Case("ao_diffuse_factor", r, chk)
+
+
+@case
+def ao_low_discrepancy():
+    # test_render_ao.cpp:382-436 (Hammersley + per-pixel rotation beats white
+    # noise against a converged reference; multi-scale keeps all on the
+    # enhanced estimator)
+    def o(n, ld):
+        return opts(ao_samples=n, ao_distance=10.0, ao_multi_scale=True, ao_low_discrepancy=ld)
+    sc = _ao_floor(0.6)
+    r = {"ref": (sc, o(2048, False)), "noise": (sc, o(32, False)), "ld": (sc, o(32, True))}
+
+    def chk(f):
+        ref = f["ref"].color[..., 0]
+        assert np.abs(f["ld"].color[..., 0] - ref).sum() < np.abs(f["noise"].color[..., 0] - ref).sum()
+    return Case("ao_low_discrepancy", r, chk)
+
+
 # ---------------------------------------------------------------- transparency
 def blend_scene(quads, bg):
     """sceneOfBlend (test_render_transparency.cpp:53-66) with fragment alpha."""

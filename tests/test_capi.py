@@ -72,7 +72,7 @@ def test_struct_layouts_match_header():
 
 def test_status_strings_and_null_arguments():
     lib = capi.load()
-    assert lib.ubg_abi_version() == 1
+    assert lib.ubg_abi_version() == capi.ABI_VERSION
     assert lib.ubg_status_string(capi.UBG_ERR_UNSUPPORTED) == b"unsupported"
     assert lib.ubg_context_create(0, None) == capi.UBG_ERR_INVALID_ARGUMENT
     assert b"null" in lib.ubg_last_error()

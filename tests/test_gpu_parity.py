@@ -245,3 +245,22 @@ def test_gpu_build_single_primitive_and_large_counts():
             bs.close()
     finally:
         dev.close()
+
+
+def test_enhanced_ao_ballstick_reduced(device):
+    """Every enhanced-AO option at once (nearest-hit falloff, three scales,
+    bent-normal sky/ground ambient, multibounce, low-discrepancy samples,
+    direct-diffuse darkening) against the oracle."""
+    sc, opt = ball_and_stick_scene(n_atoms=4000, width=240, height=135)
+    opt.ao_falloff_power = 2.0
+    opt.ao_multi_scale = True
+    opt.ao_bent_normal = True
+    opt.ao_sky_color = (1.0, 0.9, 0.8)
+    opt.ao_ground_color = (0.3, 0.3, 0.4)
+    opt.ao_multibounce = True
+    opt.ao_low_discrepancy = True
+    opt.ao_diffuse_factor = 0.5
+    g = _gpu(device, sc, opt, hires_only=True, want_ids=True, want_ao=True)
+    c = ob.render(sc, opt, hires_only=True)
+    c64 = ob.render(sc, opt, hires_only=True, precision=64)
+    _compare("enhanced-AO", g, c, c64, ao=True)

@@ -22,6 +22,9 @@ UBG_ERR_CANCELLED = 4
 UBG_ERR_UNSUPPORTED = 5
 UBG_ERR_INTERNAL = 6
 
+# UBG_ABI_VERSION of include/umbreon_gpu.h this mirror is written against
+ABI_VERSION = 2
+
 BVH2 = 0
 BVH8 = 1
 BVH8_GPU_BUILD = 2  # the BVH8 layout, built on the GPU
@@ -99,6 +102,10 @@ class RenderParams(C.Structure):
         ("transparency", C.c_int32), ("transparent_background", C.c_int32),
         ("max_transparent_layers", C.c_int32), ("flags", C.c_uint32),
         ("cancel_flag", C.POINTER(C.c_int32)),
+        ("ao_falloff_power", C.c_float), ("ao_multi_scale", C.c_int32), ("ao_bent_normal", C.c_int32),
+        ("ao_sky_color", C.c_float * 3), ("ao_ground_color", C.c_float * 3), ("ao_use_camera_up", C.c_int32),
+        ("ao_up", C.c_float * 3), ("ao_multibounce", C.c_int32), ("ao_low_discrepancy", C.c_int32),
+        ("ao_diffuse_factor", C.c_float),
     ]
 
 
@@ -216,7 +223,7 @@ def load() -> C.CDLL:
     lib.ubg_intersect.restype = C.c_int
     lib.ubg_occluded.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_int32]
     lib.ubg_occluded.restype = C.c_int
-    if lib.ubg_abi_version() != 1:
+    if lib.ubg_abi_version() != ABI_VERSION:
         raise RuntimeError("libumbreon_gpu.so ABI version mismatch")
     _lib = lib
     return lib

@@ -201,7 +201,6 @@ void checkSupported(const Scene& s, const RenderOptions& o) {
   };
   if (o.gi) no("global illumination (gi)");
   if (o.aaMode != 0) no("adaptive antialiasing (aaMode)");
-  if (o.aoEnhanced()) no("enhanced AO (falloff/multi-scale/bent/multibounce/low-discrepancy/diffuse factor)");
   if (o.aoWriteAov) no("AO AOVs (aoWriteAov)");
   if (o.aoResDiv > 1 || o.aoResDiv < 0) no("coarse-grid AO (aoResDiv)");
   if (o.denoiser != 0) no("the final-colour denoiser");
@@ -286,6 +285,18 @@ FrameResult renderPass(const Scene& scene, const RenderOptions& opt, RenderProgr
   p.transparent_background = opt.transparentBackground ? 1 : 0;
   p.max_transparent_layers = opt.maxTransparentLayers;
   p.cancel_flag = progress ? detail::ProgressAccess::cancelFlag(*progress) : nullptr;
+  p.ao_falloff_power = opt.aoFalloffPower;
+  p.ao_multi_scale = opt.aoMultiScale ? 1 : 0;
+  p.ao_bent_normal = opt.aoBentNormal ? 1 : 0;
+  p.ao_use_camera_up = opt.aoUseCameraUp ? 1 : 0;
+  for (int k = 0; k < 3; ++k) {
+    p.ao_sky_color[k] = opt.aoSkyColor[k];
+    p.ao_ground_color[k] = opt.aoGroundColor[k];
+    p.ao_up[k] = opt.aoUp[k];
+  }
+  p.ao_multibounce = opt.aoMultibounce ? 1 : 0;
+  p.ao_low_discrepancy = opt.aoLowDiscrepancy ? 1 : 0;
+  p.ao_diffuse_factor = opt.aoDiffuseFactor;
 
   const std::size_t hiN = static_cast<std::size_t>(opt.width) * ss * opt.height * ss;
   f.width = opt.width;

@@ -61,12 +61,12 @@ struct ubg_context {
   std::vector<hipEvent_t> tev;  // per-kernel timing events (grow-only)
   std::mutex mu;
   // grow-only frame scratch
-  DevBuf hitPix, hitT, hitO, hitN, hitC, hitAo, activeA, activeB, acc, tnear, hiColor, depth, viewZ,
+  DevBuf hitPix, hitT, hitO, hitN, hitC, hitAo, aoT, activeA, activeB, acc, tnear, hiColor, depth, viewZ,
       hitId, aoOpen, outColor, counters, query, queryOut;
   unsigned long long* hostCounters = nullptr;  // pinned
   std::vector<DevBuf*> all() {
     return {&hitPix, &hitT,   &hitO,    &hitN,  &hitC,  &hitAo,   &activeA,  &activeB,  &acc,     &tnear,
-            &hiColor, &depth, &viewZ,   &hitId, &aoOpen, &outColor, &counters, &query, &queryOut};
+            &hiColor, &depth, &viewZ,   &hitId, &aoOpen, &outColor, &counters, &query, &queryOut, &aoT};
   }
 };
 
@@ -403,6 +403,28 @@ int ubg_render(ubg_context* ctx, const ubg_scene* scene, const ubg_render_params
   std::memcpy(fc.fogColor, p->fog.color, sizeof(fc.fogColor));
   fc.fogStart = p->fog.start;
   fc.fogEnd = p->fog.end;
+  // enhanced AO (render_options.hpp:79-82; embree_renderer.cpp:1320-1323 for the axis)
+  fc.aoEnh = (p->ao_falloff_power > 0.0f || p->ao_multi_scale || p->ao_bent_normal || p->ao_multibounce ||
+              p->ao_low_discrepancy || p->ao_diffuse_factor > 0.0f)
+                 ? 1
+                 : 0;
+  fc.aoFalloff = p->ao_falloff_power;
+  fc.aoMultiScale = p->ao_multi_scale ? 1 : 0;
+  fc.aoBent = p->ao_bent_normal ? 1 : 0;
+  fc.aoMultibounce = p->ao_multibounce ? 1 : 0;
+  fc.aoLowDisc = p->ao_low_discrepancy ? 1 : 0;
+  fc.aoDiffuse = p->ao_diffuse_factor;
+  std::memcpy(fc.aoSky, p->ao_sky_color, sizeof(fc.aoSky));
+  std::memcpy(fc.aoGround, p->ao_ground_color, sizeof(fc.aoGround));
+  {
+    V3h up = trueUp;
+    if (!p->ao_use_camera_up) {
+      const V3h a{p->ao_up[0], p->ao_up[1], p->ao_up[2]};
+      const float d = dot(a, a);
+      if (!(d < 1.0e-18f)) up = scale(a, 1.0f / std::sqrt(d));
+    }
+    fc.aoUpAxis[0] = up.x, fc.aoUpAxis[1] = up.y, fc.aoUpAxis[2] = up.z;
+  }
   fc.nLights = p->n_lights;
   for (int i = 0; i < p->n_lights; ++i) {
     const ubg_light& dl = p->lights[i];
@@ -434,6 +456,7 @@ int ubg_render(ubg_context* ctx, const ubg_scene* scene, const ubg_render_params
   need(ctx->hitN, qn * 16);
   need(ctx->hitC, qn * 16);
   need(ctx->hitAo, qn * 4);
+  if (fc.aoEnh && p->ao_samples > 0) need(ctx->aoT, qn * static_cast<size_t>(p->ao_samples) * 4);
   need(ctx->activeA, qn * 4);
   need(ctx->activeB, qn * 4);
   need(ctx->acc, n * 16);
@@ -455,6 +478,7 @@ int ubg_render(ubg_context* ctx, const ubg_scene* scene, const ubg_render_params
   b.hitN = static_cast<float4*>(ctx->hitN.p);
   b.hitC = static_cast<float4*>(ctx->hitC.p);
   b.hitAo = static_cast<uint32_t*>(ctx->hitAo.p);
+  b.aoT = (fc.aoEnh && p->ao_samples > 0) ? static_cast<float*>(ctx->aoT.p) : nullptr;
   b.activeIn = static_cast<uint32_t*>(ctx->activeA.p);
   b.activeOut = static_cast<uint32_t*>(ctx->activeB.p);
   b.acc = static_cast<float4*>(ctx->acc.p);

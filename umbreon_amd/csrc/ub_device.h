@@ -110,6 +110,38 @@ UBG_HD V3 sampleLightDir(V3 L, float tanR, float u1, float u2) {
 
 UBG_HD V3 ld3(const float4& a) { return mk(a.x, a.y, a.z); }
 
+// ambient_occlusion.hpp:96-104 (base-2 radical inverse = bit reversal)
+UBG_HD uint32_t reverseBits32(uint32_t b) {
+  b = (b << 16) | (b >> 16);
+  b = ((b & 0x55555555u) << 1) | ((b & 0xAAAAAAAAu) >> 1);
+  b = ((b & 0x33333333u) << 2) | ((b & 0xCCCCCCCCu) >> 2);
+  b = ((b & 0x0F0F0F0Fu) << 4) | ((b & 0xF0F0F0F0u) >> 4);
+  b = ((b & 0x00FF00FFu) << 8) | ((b & 0xFF00FF00u) >> 8);
+  return b;
+}
+// ambient_occlusion.hpp:111-125: the legacy tea2 stream or the Hammersley
+// point shifted per pixel by (cpx, cpy) mod 1
+UBG_HD void aoSample2d(bool lowDisc, uint32_t base, int i, int n, float cpx, float cpy, float& u1, float& u2) {
+  if (lowDisc) {
+    u1 = static_cast<float>(i) / static_cast<float>(n) + cpx;
+    u2 = static_cast<float>(reverseBits32(static_cast<uint32_t>(i))) * 0x1.0p-32f + cpy;
+    if (u1 >= 1.0f) u1 -= 1.0f;
+    if (u2 >= 1.0f) u2 -= 1.0f;
+  } else {
+    uint32_t s0 = base, s1 = static_cast<uint32_t>(i);
+    tea2(s0, s1);
+    u1 = u32ToUnorm(s0);
+    u2 = u32ToUnorm(s1);
+  }
+}
+// ambient_occlusion.hpp:133-138 (GTAO multibounce)
+UBG_HD float aoMultibounce(float ao, float albedo) {
+  const float a = 2.0404f * albedo - 0.3324f;
+  const float b = -4.7951f * albedo + 0.6417f;
+  const float c = 2.7552f * albedo + 0.6903f;
+  return fmaxf(ao, ((ao * a + b) * ao + c) * ao);
+}
+
 struct Hit {
   float t, u, v;
   uint32_t bits;

@@ -22,6 +22,7 @@ struct FrameBufs {
   float4* hitN;       // shading normal xyz, packed flags
   float4* hitC;       // pigment rgb, opacity
   uint32_t* hitAo;    // AO occluded count
+  float* aoT;         // enhanced AO: nearest-hit distance per (slot, sample); -1 = below the horizon
   // continuation (transparency) queue and per-pixel walk state
   uint32_t* activeIn;
   uint32_t* activeOut;

@@ -505,6 +505,133 @@ __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(kAo
   if (kCount) addVisits(b.counters, 1, nv, nt);
 }
 
+// Enhanced AO rays (ambient_occlusion.hpp:171-246): one lane per (hit-queue
+// slot, sample) records the nearest-hit distance within the AO radius (+inf
+// on a miss, -1 for a below-horizon sample that counts as full contact);
+// k_resolve reduces a hit's samples in order, so the float sums are the same
+// on every run.
+template <bool kCount, int kLayout>
+__global__ __launch_bounds__(kTraceBlock) void k_ao_quality(FrameConst fc, SceneView s, FrameBufs b) {
+  const int n = fc.aoSamples;
+  const int64_t gid = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int64_t slot = gid / n;
+  const int si = static_cast<int>(gid - slot * n);
+  const int part = static_cast<int>(slot / b.partCap);
+  bool traced = false, sampled = false;
+  uint32_t nv = 0, nt = 0;
+  if (part < kQueueParts && static_cast<uint32_t>(slot % b.partCap) < queueFill(b.counters, kHitFill, part)) {
+    const float4 N4 = b.hitN[slot];
+    if (__float_as_uint(N4.w) & kFlagAo) {
+      const float4 O4 = b.hitO[slot];
+      const uint32_t pix = b.hitPix[slot];
+      const uint32_t px = pix % static_cast<uint32_t>(fc.W), py = pix / static_cast<uint32_t>(fc.W);
+      const V3 N = ld3(N4);
+      const Frame3 f = frameFromNormal(N);
+      uint32_t c0 = px + 1u, c1 = py + 1u;
+      tea2(c0, c1);
+      float u1, u2;
+      aoSample2d(fc.aoLowDisc != 0, pix, si, n, u32ToUnorm(c0), u32ToUnorm(c1), u1, u2);
+      const V3 dir = cosineSample(u1, u2, f);
+      float t;
+      if (dot(dir, N) < 0.01f) {
+        sampled = true;
+        t = -1.0f;
+      } else {
+        traced = true;
+        Hit h;
+        t = trace<kLayout, false, kCount>(s, ld3(O4), dir, O4.w, fc.aoDist, h, &nv, &nt) ? h.t
+                                                                                          : __builtin_inff();
+      }
+      b.aoT[slot * n + si] = t;
+    }
+  }
+  waveCount(b.counters, kCntAoTraced, traced);
+  waveCount(b.counters, kCntAoSampled, sampled);
+  if (kCount) addVisits(b.counters, 1, nv, nt);
+}
+
+// computeAOQuality's reduction (ambient_occlusion.hpp:211-264) over the
+// recorded distances of one hit, in sample order, and aoApplyFactors
+// (ao_shade.hpp:83-120): per-channel AO factor, ambient light (bent-normal
+// sky/ground gradient) and the direct-diffuse factor.
+__device__ void aoQualityFactors(const FrameConst& fc, const FrameBufs& b, int64_t slot, uint32_t px, uint32_t py,
+                                 V3 N, V3 C, float& openness, V3& aoF, V3& ambL, float& diffAo) {
+  const int n = fc.aoSamples;
+  const float R = fc.aoDist;
+  float radius[3], weight[3];
+  int K;
+  if (fc.aoMultiScale) {
+    K = 3;
+    radius[0] = 0.08f * R;
+    radius[1] = 0.30f * R;
+    radius[2] = 1.00f * R;
+    weight[0] = 0.55f;
+    weight[1] = 0.35f;
+    weight[2] = 0.10f;
+  } else {
+    K = 1;
+    radius[0] = radius[1] = radius[2] = R;
+    weight[0] = 1.0f;
+    weight[1] = weight[2] = 0.0f;
+  }
+  const Frame3 f = frameFromNormal(N);
+  uint32_t c0 = px + 1u, c1 = py + 1u;
+  tea2(c0, c1);
+  const float cpx = u32ToUnorm(c0), cpy = u32ToUnorm(c1);
+  const uint32_t base = px + py * static_cast<uint32_t>(fc.W);
+  // (the per-scale sums of the reference only feed the contact / shape AOVs,
+  // which this path does not write; the colour needs the weighted sum)
+  float weightedObs = 0.0f;
+  V3 bentAccum = mk(0.0f, 0.0f, 0.0f);
+  const float power = fc.aoFalloff;
+  for (int i = 0; i < n; ++i) {
+    const float t = b.aoT[slot * n + i];
+    if (t == -1.0f) {
+      weightedObs += 1.0f;
+      continue;
+    }
+    float perSample = 0.0f;
+    for (int k = 0; k < K; ++k) {
+      if (t < radius[k]) {
+        float fk = 1.0f;
+        if (power > 0.0f) {
+          float a = 1.0f - t / radius[k];
+          if (a < 0.0f) a = 0.0f;
+          fk = powf(a, power);
+        }
+        perSample += weight[k] * fk;
+      }
+    }
+    weightedObs += perSample;
+    if (fc.aoBent) {
+      float u1, u2;
+      aoSample2d(fc.aoLowDisc != 0, base, i, n, cpx, cpy, u1, u2);
+      const V3 dir = cosineSample(u1, u2, f);
+      bentAccum = bentAccum + dir * (1.0f - perSample);
+    }
+  }
+  const float invN = 1.0f / static_cast<float>(n);
+  openness = 1.0f - weightedObs * invN;
+  ambL = mk(fc.amb[0], fc.amb[1], fc.amb[2]);
+  if (fc.aoBent) {
+    const V3 bent = safeNormalize(bentAccum, N);
+    const float w = 0.5f * (dot(bent, mk(fc.aoUpAxis[0], fc.aoUpAxis[1], fc.aoUpAxis[2])) + 1.0f);
+    ambL = mk(fc.amb[0] * (fc.aoGround[0] + (fc.aoSky[0] - fc.aoGround[0]) * w),
+              fc.amb[1] * (fc.aoGround[1] + (fc.aoSky[1] - fc.aoGround[1]) * w),
+              fc.amb[2] * (fc.aoGround[2] + (fc.aoSky[2] - fc.aoGround[2]) * w));
+  }
+  if (fc.aoMultibounce) {
+    const float ox = aoMultibounce(openness, C.x), oy = aoMultibounce(openness, C.y),
+                oz = aoMultibounce(openness, C.z);
+    aoF = mk(1.0f - fc.aoIntensity * (1.0f - ox), 1.0f - fc.aoIntensity * (1.0f - oy),
+             1.0f - fc.aoIntensity * (1.0f - oz));
+  } else {
+    const float sv = 1.0f - fc.aoIntensity * (1.0f - openness);
+    aoF = mk(sv, sv, sv);
+  }
+  diffAo = fc.aoDiffuse > 0.0f ? 1.0f - fc.aoDiffuse * (1.0f - openness) : 1.0f;
+}
+
 template <bool kCount, int kLayout>
 __global__ __launch_bounds__(kTraceBlock) void k_resolve(FrameConst fc, SceneView s, FrameBufs b,
                                                          int iter, int unused) {
@@ -534,17 +661,23 @@ __global__ __launch_bounds__(kTraceBlock) void k_resolve(FrameConst fc, SceneVie
     const float eps = O4.w;
     const DevMaterial m = (flags & kFlagMesh) ? s.meshMats[flags & kInfoMatMask]
                                               : s.primMats[flags & kInfoMatMask];
-    // ao_shade.hpp:100-103 (legacy scalar aoFactor)
-    float aoF = 1.0f, open = 1.0f;
+    // ao_shade.hpp:83-120 (legacy: scalar aoFactor, scene ambient)
+    V3 aoF = mk(1.0f, 1.0f, 1.0f), ambL = mk(fc.amb[0], fc.amb[1], fc.amb[2]);
+    float open = 1.0f, diffAo = 1.0f;
     if (flags & kFlagAo) {
-      open = 1.0f - (float)b.hitAo[tid] / (float)fc.aoSamples;
-      aoF = 1.0f - fc.aoIntensity * (1.0f - open);
+      if (fc.aoEnh) {
+        aoQualityFactors(fc, b, tid, px, py, N, C, open, aoF, ambL, diffAo);
+      } else {
+        open = 1.0f - (float)b.hitAo[tid] / (float)fc.aoSamples;
+        const float sv = 1.0f - fc.aoIntensity * (1.0f - open);
+        aoF = mk(sv, sv, sv);
+      }
     }
     if (iter == 0 && b.aoOpen) b.aoOpen[pix] = open;
     // shading.hpp:92-190
-    V3 out = mk(m.emission * C.x + aoF * m.ambient * C.x * fc.amb[0],
-                m.emission * C.y + aoF * m.ambient * C.y * fc.amb[1],
-                m.emission * C.z + aoF * m.ambient * C.z * fc.amb[2]);
+    V3 out = mk(m.emission * C.x + aoF.x * m.ambient * C.x * ambL.x,
+                m.emission * C.y + aoF.y * m.ambient * C.y * ambL.y,
+                m.emission * C.z + aoF.z * m.ambient * C.z * ambL.z);
     const bool shOn = (flags & kFlagShadow) != 0u;
     uint32_t s0 = px, s1 = py;
     for (int li = 0; li < fc.nLights; ++li) {
@@ -577,7 +710,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_resolve(FrameConst fc, SceneVie
       if (m.brilliance == 1.0f) dd = ndl;
       else if (m.brilliance == 0.0f) dd = 1.0f;
       else dd = powf(ndl, m.brilliance);
-      const float dk = m.diffuse * dd * 1.0f;
+      const float dk = m.diffuse * dd * diffAo;
       out.x += dk * C.x * Lc.x;
       out.y += dk * C.y * Lc.y;
       out.z += dk * C.z * Lc.z;
@@ -817,6 +950,10 @@ void launchPrimary(const FrameConst& fc, const SceneView& s, const FrameBufs& b,
 void launchAo(const FrameConst& fc, const SceneView& s, const FrameBufs& b, bool countVisits, hipStream_t st) {
   if (fc.aoSamples <= 0) return;
   const int64_t slots = static_cast<int64_t>(kQueueParts) * b.partCap;
+  if (fc.aoEnh) {
+    UBG_DISPATCH(k_ao_quality, blocksFor(slots * fc.aoSamples, kTraceBlock), countVisits, fc, s, b);
+    return;
+  }
   if (s.layout == kLayoutBvh8) {
     // persistent grid: every wave slot of the chip, no more than the work
     const int64_t rays = slots * fc.aoSamples;

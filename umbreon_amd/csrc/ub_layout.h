@@ -125,6 +125,12 @@ struct FrameConst {
   float fogColor[3];
   float fogStart, fogEnd;
   int32_t nLights;
+  // enhanced AO (nearest-hit estimator); aoEnh == 0 = legacy binary AO
+  int32_t aoEnh;
+  float aoFalloff;
+  int32_t aoMultiScale, aoBent, aoMultibounce, aoLowDisc;
+  float aoDiffuse;
+  float aoSky[3], aoGround[3], aoUpAxis[3];
   DevLight lights[kMaxLights];
 };
 

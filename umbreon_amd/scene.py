@@ -98,6 +98,17 @@ class RenderOptions:
     transparency: bool = True
     transparent_background: bool = False
     max_transparent_layers: int = 256
+    # enhanced AO (render_options.hpp:52-61); all defaults = legacy binary AO
+    ao_falloff_power: float = 0.0
+    ao_multi_scale: bool = False
+    ao_bent_normal: bool = False
+    ao_sky_color: Tuple[float, float, float] = (1.0, 1.0, 1.0)
+    ao_ground_color: Tuple[float, float, float] = (1.0, 1.0, 1.0)
+    ao_use_camera_up: bool = True
+    ao_up: Tuple[float, float, float] = (0.0, 1.0, 0.0)
+    ao_multibounce: bool = False
+    ao_low_discrepancy: bool = False
+    ao_diffuse_factor: float = 0.0
     # options the gfx950 path does not implement; must stay at their defaults
     gi: bool = False
     aa_mode: int = 0
@@ -336,6 +347,16 @@ def params(scene: Scene, opt: RenderOptions, flags: int = 0,
     p.transparency = int(bool(opt.transparency))
     p.transparent_background = int(bool(opt.transparent_background))
     p.max_transparent_layers = int(opt.max_transparent_layers)
+    p.ao_falloff_power = opt.ao_falloff_power
+    p.ao_multi_scale = int(bool(opt.ao_multi_scale))
+    p.ao_bent_normal = int(bool(opt.ao_bent_normal))
+    p.ao_sky_color[:] = opt.ao_sky_color
+    p.ao_ground_color[:] = opt.ao_ground_color
+    p.ao_use_camera_up = int(bool(opt.ao_use_camera_up))
+    p.ao_up[:] = opt.ao_up
+    p.ao_multibounce = int(bool(opt.ao_multibounce))
+    p.ao_low_discrepancy = int(bool(opt.ao_low_discrepancy))
+    p.ao_diffuse_factor = opt.ao_diffuse_factor
     p.flags = flags
     if cancel_flag is not None:
         keep.append(cancel_flag)
