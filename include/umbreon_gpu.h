@@ -312,6 +312,31 @@ int ubg_intersect(ubg_context* ctx, const ubg_scene* scene, int64_t n_rays,
 int ubg_occluded(ubg_context* ctx, const ubg_scene* scene, int64_t n_rays,
                  const ubg_ray* rays, uint8_t* occluded, int32_t memory);
 
+/* One segment of the edge pass's visibility query, from p to q. Its
+ * exclusion list (the edge's own incident faces, as base mesh triangle ids)
+ * is exclude_faces[exclude_first .. exclude_first + exclude_count). */
+typedef struct ubg_segment {
+  float p[3];
+  float q[3];
+  int32_t exclude_first;
+  int32_t exclude_count;
+} ubg_segment;
+
+/* Batched EmbreeRenderer::occluded(p, q, excludeFaces, nExclude, eps,
+ * grazeCosEps, coplanarEps) (render/embree_renderer.cpp:294-357, with the
+ * argument filter of :48-131): occluded[i] = 1 when a primitive hit lies
+ * inside the trimmed segment (max(eps*len, eps), len*(1-eps)) and the filter
+ * does not reject it. The filter rejects hits on the segment's excluded mesh
+ * faces (instance copies map to their base triangle), hits on faces grazed
+ * nearly edge-on (|cos(dir, Ng)| <= graze_cos_eps, when > 0) and hits whose
+ * face plane passes within coplanar_eps of p (when > 0); after a rejected
+ * hit the primitive's next root along the segment is tried. exclude_faces
+ * may be NULL when every exclude_count is 0; all buffers per `memory`. */
+int ubg_occluded_segments(ubg_context* ctx, const ubg_scene* scene, int64_t n_segments,
+                          const ubg_segment* segments, const int32_t* exclude_faces, int64_t n_exclude_faces,
+                          float eps, float graze_cos_eps, float coplanar_eps, uint8_t* occluded,
+                          int32_t memory);
+
 #ifdef __cplusplus
 }
 #endif

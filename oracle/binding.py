@@ -45,6 +45,9 @@ def load():
     lib.orc_occluded.argtypes = [C.POINTER(capi.SceneDesc), C.c_int64, C.c_void_p, C.c_void_p,
                                  C.POINTER(OrcOptions)]
     lib.orc_occluded.restype = C.c_int
+    lib.orc_occluded_segments.argtypes = [C.POINTER(capi.SceneDesc), C.c_int64, C.c_void_p, C.c_void_p, C.c_int64,
+                                          C.c_float, C.c_float, C.c_float, C.c_void_p]
+    lib.orc_occluded_segments.restype = C.c_int
     lib.orc_chain_curves.argtypes = [C.POINTER(capi.SceneDesc), C.c_int64, C.c_void_p, C.c_void_p]
     lib.orc_chain_curves.restype = C.c_int64
     lib.orc_tea2.argtypes = [C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]
@@ -115,6 +118,20 @@ def occluded(scene: Scene, rays: np.ndarray, precision: int = 32, brute: bool = 
     occ = np.zeros(len(rays), dtype=np.uint8)
     o = _opts(precision, 1, None, brute)
     capi.check(lib.orc_occluded(C.byref(desc), len(rays), rays.ctypes.data, occ.ctypes.data, C.byref(o)))
+    del keep
+    return occ
+
+
+def occluded_segments(scene: Scene, segments: np.ndarray, exclude_faces=None, eps: float = 1.0e-4,
+                      graze_cos_eps: float = 0.0, coplanar_eps: float = 0.0) -> np.ndarray:
+    lib = load()
+    desc, keep = scene.to_desc()
+    segments = np.ascontiguousarray(segments, dtype=capi.segment_dtype())
+    ex = np.ascontiguousarray(np.zeros(0, np.int32) if exclude_faces is None else exclude_faces, dtype=np.int32)
+    occ = np.zeros(len(segments), dtype=np.uint8)
+    capi.check(lib.orc_occluded_segments(C.byref(desc), len(segments), segments.ctypes.data,
+                                         ex.ctypes.data if len(ex) else None, len(ex), eps, graze_cos_eps,
+                                         coplanar_eps, occ.ctypes.data))
     del keep
     return occ
 

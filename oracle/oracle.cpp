@@ -1636,9 +1636,81 @@ static int occludedT(const ubg_scene_desc* d, int64_t n, const ubg_ray* rays, ui
   return UBG_OK;
 }
 
+// embree_renderer.cpp:294-357 with the argument filter of :71-131, FP32,
+// every primitive tested (no BVH); a rejected root moves the search just
+// past it along the same primitive.
+static int occludedSegments(const ubg_scene_desc* d, int64_t n, const ubg_segment* segs, const int32_t* excl,
+                            int64_t nExcl, float eps, float graze, float coplanar, uint8_t* out) {
+  OScene s;
+  buildScene(d, true, s);
+  const Geometry& g = s.g;
+  const bool hasMesh = !g.idx.empty();
+  for (int64_t i = 0; i < n; ++i) {
+    const ubg_segment& sg = segs[i];
+    out[i] = 0;
+    if (sg.exclude_first < 0 || sg.exclude_count < 0 ||
+        static_cast<int64_t>(sg.exclude_first) + sg.exclude_count > nExcl)
+      return UBG_ERR_INVALID_ARGUMENT;
+    const V3<float> p = v3<float>(sg.p), q = v3<float>(sg.q);
+    const V3<float> dd{q.x - p.x, q.y - p.y, q.z - p.z};
+    const float len = length(dd);
+    if (!(len > 0.0f)) continue;
+    const V3<float> dir{dd.x / len, dd.y / len, dd.z / len};
+    const float tnear = std::max(eps * len, eps);
+    const float tfar = len * (1.0f - eps);
+    if (tnear >= tfar) continue;
+    const int nEx = hasMesh ? sg.exclude_count : 0;
+    const int32_t* ex = nEx > 0 ? excl + sg.exclude_first : nullptr;
+    const bool plain = nEx == 0 && graze <= 0.0f && coplanar <= 0.0f;
+    auto accept = [&](const Hit<float>& h, int kind, uint32_t id) {
+      if (nEx > 0 && kind == UBG_KIND_MESH) {
+        const int64_t base = g.baseT > 0 ? static_cast<int64_t>(id) % g.baseT : static_cast<int64_t>(id);
+        for (int k = 0; k < nEx; ++k)
+          if (static_cast<int64_t>(ex[k]) == base) return false;
+      }
+      if (graze > 0.0f || coplanar > 0.0f) {
+        const float nx = h.ng.x, ny = h.ng.y, nz = h.ng.z;
+        const float nlen = std::sqrt(nx * nx + ny * ny + nz * nz);
+        const float dlen = std::sqrt(dir.x * dir.x + dir.y * dir.y + dir.z * dir.z);
+        if (nlen > 0.0f && dlen > 0.0f) {
+          const float dDotN = nx * dir.x + ny * dir.y + nz * dir.z;
+          const float c = std::fabs(dDotN) / (nlen * dlen);
+          if (graze > 0.0f && c <= graze) return false;
+          if (coplanar > 0.0f && h.t * std::fabs(dDotN) / nlen <= coplanar) return false;
+        }
+      }
+      return true;
+    };
+    Hit<float> h;
+    for (const PRef& pr : s.all) {
+      float lo = tnear;
+      bool hit = false;
+      for (int k = 0; k < 4; ++k) {
+        if (!isectPrim(g, pr, p, dir, lo, tfar, h)) break;
+        if (plain || accept(h, pr.kind, pr.id)) {
+          hit = true;
+          break;
+        }
+        lo = std::nextafter(h.t, std::numeric_limits<float>::infinity());
+      }
+      if (hit) {
+        out[i] = 1;
+        break;
+      }
+    }
+  }
+  return UBG_OK;
+}
+
 }  // namespace orc
 
 extern "C" {
+
+int orc_occluded_segments(const ubg_scene_desc* desc, int64_t n, const ubg_segment* segs, const int32_t* excl,
+                          int64_t n_excl, float eps, float graze_cos_eps, float coplanar_eps, uint8_t* occ) {
+  if (!desc || n < 0 || (n > 0 && (!segs || !occ)) || (n_excl > 0 && !excl)) return UBG_ERR_INVALID_ARGUMENT;
+  return orc::occludedSegments(desc, n, segs, excl, n_excl, eps, graze_cos_eps, coplanar_eps, occ);
+}
 
 static const orc_options kDefaultOpt = {32, 0, 0, 0, 1, 0};
 

@@ -40,6 +40,12 @@ int orc_intersect(const ubg_scene_desc* desc, int64_t n, const ubg_ray* rays,
 int orc_occluded(const ubg_scene_desc* desc, int64_t n, const ubg_ray* rays,
                  uint8_t* occluded, const orc_options* opt);
 
+/* Batched EmbreeRenderer::occluded (embree_renderer.cpp:294-357) with its
+ * face-exclusion / grazing / coincident-plane filter, FP32, brute force. */
+int orc_occluded_segments(const ubg_scene_desc* desc, int64_t n, const ubg_segment* segs,
+                          const int32_t* exclude_faces, int64_t n_exclude_faces, float eps,
+                          float graze_cos_eps, float coplanar_eps, uint8_t* occluded);
+
 /* Curve chaining result (curve_build.cpp:88-199): per round segment, the
  * source cylinder index and the neighbor flags (bit0 LEFT, bit1 RIGHT), in
  * primID order. Returns the segment count, or -1 if `cap` is too small. */

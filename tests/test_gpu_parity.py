@@ -264,3 +264,32 @@ def test_enhanced_ao_ballstick_reduced(device):
     c = ob.render(sc, opt, hires_only=True)
     c64 = ob.render(sc, opt, hires_only=True, precision=64)
     _compare("enhanced-AO", g, c, c64, ao=True)
+
+
+@pytest.mark.parametrize("layout", [capi.BVH2, capi.BVH8], ids=["bvh2", "bvh8"])
+def test_occluded_segments_match_oracle(device, layout):
+    """ubg_occluded_segments == the oracle's restatement of
+    EmbreeRenderer::occluded on random segments with random face exclusions,
+    grazing and coincident-plane filters."""
+    from umbreon_amd.render import BuiltScene
+    from test_oracle import _random_prim_scene
+    rng = np.random.default_rng(23)
+    sc = _random_prim_scene(rng, 300)
+    ntri = len(sc.index) if sc.index is not None else len(sc.positions) // 3
+    n = 20000
+    s = np.zeros(n, capi.segment_dtype())
+    s["p"] = rng.uniform(-4, 4, size=(n, 3))
+    s["q"] = rng.uniform(-4, 4, size=(n, 3))
+    cnt = rng.integers(0, 4, size=n)
+    s["exclude_count"] = cnt
+    s["exclude_first"] = np.concatenate([[0], np.cumsum(cnt)[:-1]])
+    excl = rng.integers(0, max(1, ntri), size=int(cnt.sum())).astype(np.int32)
+    bs = BuiltScene(device, sc, layout)
+    try:
+        for graze, cop in ((0.0, 0.0), (0.05, 0.0), (0.0, 0.05), (0.1, 0.02)):
+            g = bs.occluded_segments(s, excl, 1e-4, graze, cop)
+            c = ob.occluded_segments(sc, s, excl, 1e-4, graze, cop)
+            print(f"graze {graze} coplanar {cop}: occluded {g.mean():.3f}, agree {(g == c).mean():.6f}", flush=True)
+            assert (g == c).mean() >= 0.9999
+    finally:
+        bs.close()

@@ -158,6 +158,12 @@ class Hit(C.Structure):
 
 
 # numpy structured dtypes matching Ray / Hit (for zero-copy batches)
+def segment_dtype():
+    """ubg_segment"""
+    import numpy as np
+    return np.dtype([("p", "<f4", 3), ("q", "<f4", 3), ("exclude_first", "<i4"), ("exclude_count", "<i4")])
+
+
 def ray_dtype():
     import numpy as np
     return np.dtype([("org", "<f4", 3), ("tnear", "<f4"), ("dir", "<f4", 3), ("tfar", "<f4")])
@@ -174,7 +180,7 @@ def hit_dtype():
 EXPORTS = [
     "ubg_abi_version", "ubg_status_string", "ubg_context_create", "ubg_context_destroy",
     "ubg_last_error", "ubg_scene_create", "ubg_scene_create_ex", "ubg_scene_destroy", "ubg_scene_get_info",
-    "ubg_render", "ubg_intersect", "ubg_occluded",
+    "ubg_render", "ubg_intersect", "ubg_occluded", "ubg_occluded_segments",
 ]
 
 _lib = None
@@ -223,6 +229,9 @@ def load() -> C.CDLL:
     lib.ubg_intersect.restype = C.c_int
     lib.ubg_occluded.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_int32]
     lib.ubg_occluded.restype = C.c_int
+    lib.ubg_occluded_segments.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_int64,
+                                          C.c_float, C.c_float, C.c_float, C.c_void_p, C.c_int32]
+    lib.ubg_occluded_segments.restype = C.c_int
     if lib.ubg_abi_version() != ABI_VERSION:
         raise RuntimeError("libumbreon_gpu.so ABI version mismatch")
     _lib = lib

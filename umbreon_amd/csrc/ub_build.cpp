@@ -825,6 +825,7 @@ int buildHostScene(const ubg_scene_desc* d, int layout, HostScene& out, std::str
                 F4{p2[2], 0.0f, 0.0f, bitsF(UBG_KIND_MESH, dt)});
       }
     out.nTris = baseT * copies;
+    out.meshBaseTris = baseT;
   }
 
   auto checkMat = [&](int32_t m) {

@@ -297,6 +297,7 @@ int ubg_scene_create_ex(ubg_context* ctx, const ubg_scene_desc* desc, int32_t bv
     return r;
   }
   s->view.nNodes = static_cast<int32_t>(hs.nNodes);
+  s->view.meshBaseTris = hs.meshBaseTris;
   s->view.nPrims = static_cast<int32_t>(hs.prims.size() / 3);  // records in either order
   s->view.layout = hs.layout;
   s->view.stackDepth = std::max(1, hs.stackDepth);
@@ -653,6 +654,50 @@ int ubg_intersect(ubg_context* ctx, const ubg_scene* scene, int64_t n_rays, cons
 int ubg_occluded(ubg_context* ctx, const ubg_scene* scene, int64_t n_rays, const ubg_ray* rays,
                  uint8_t* occluded, int32_t memory) {
   return runQuery(ctx, scene, n_rays, rays, occluded, 1, memory, true);
+}
+
+int ubg_occluded_segments(ubg_context* ctx, const ubg_scene* scene, int64_t n, const ubg_segment* segs,
+                          const int32_t* excl, int64_t nExcl, float eps, float grazeCosEps, float coplanarEps,
+                          uint8_t* occluded, int32_t memory) {
+  if (!ctx || !scene || n < 0 || nExcl < 0 || (n > 0 && (!segs || !occluded)) || (nExcl > 0 && !excl))
+    return fail(UBG_ERR_INVALID_ARGUMENT, "ubg_occluded_segments: bad argument");
+  if (scene->ctx != ctx)
+    return fail(UBG_ERR_INVALID_ARGUMENT, "ubg_occluded_segments: scene belongs to another context");
+  if (!memory)
+    for (int64_t i = 0; i < n; ++i)
+      if (segs[i].exclude_count < 0 || segs[i].exclude_first < 0 ||
+          static_cast<int64_t>(segs[i].exclude_first) + segs[i].exclude_count > nExcl)
+        return fail(UBG_ERR_INVALID_ARGUMENT, "ubg_occluded_segments: exclusion range out of bounds");
+  if (n == 0) return UBG_OK;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  UBG_HIP(hipSetDevice(ctx->device), "hipSetDevice");
+  hipStream_t st = ctx->stream;
+  const void* dSegs = segs;
+  const void* dExcl = excl;
+  void* dOut = occluded;
+  if (!memory) {
+    const size_t segBytes = static_cast<size_t>(n) * sizeof(ubg_segment);
+    const size_t exclBytes = static_cast<size_t>(nExcl) * sizeof(int32_t);
+    int r = ensure(ctx->query, segBytes + exclBytes);
+    if (r == UBG_OK) r = ensure(ctx->queryOut, static_cast<size_t>(n));
+    if (r != UBG_OK) return r;
+    UBG_HIP(hipMemcpyAsync(ctx->query.p, segs, segBytes, hipMemcpyHostToDevice, st), "hipMemcpy(segments)");
+    if (exclBytes)
+      UBG_HIP(hipMemcpyAsync(static_cast<char*>(ctx->query.p) + segBytes, excl, exclBytes, hipMemcpyHostToDevice,
+                             st),
+              "hipMemcpy(exclusions)");
+    dSegs = ctx->query.p;
+    dExcl = exclBytes ? static_cast<const void*>(static_cast<char*>(ctx->query.p) + segBytes) : nullptr;
+    dOut = ctx->queryOut.p;
+  }
+  ubg::launchSegments(scene->view, dSegs, n, static_cast<const int32_t*>(dExcl), nExcl, eps, grazeCosEps,
+                      coplanarEps, static_cast<uint8_t*>(dOut), st);
+  UBG_HIP(hipGetLastError(), "kernel launch");
+  if (!memory)
+    UBG_HIP(hipMemcpyAsync(occluded, dOut, static_cast<size_t>(n), hipMemcpyDeviceToHost, st),
+            "hipMemcpy(segment result)");
+  UBG_HIP(hipStreamSynchronize(st), "segment query sync");
+  return UBG_OK;
 }
 
 }  // extern "C"

@@ -388,6 +388,27 @@ UBG_HD bool isectPrim(const SceneView& s, uint32_t i, V3 o, V3 d, float tnear, f
   return r;
 }
 
+// Hit filters (the Embree argument-filter stage): a rejected hit does not
+// count and the query goes on. AcceptAll is the plain query.
+struct AcceptAll {
+  UBG_HD bool operator()(const Hit&) const { return true; }
+};
+
+// isectPrim with a filter: when the filter rejects the first root in
+// [tnear, tfar], the primitive's next root beyond it is tried (a primitive
+// has at most four roots along a line: two body roots and two caps).
+template <class F>
+UBG_HD bool isectPrimF(const SceneView& s, uint32_t i, V3 o, V3 d, float tnear, float tfar, Hit& h, const F& f) {
+  float lo = tnear;
+#pragma unroll 1
+  for (int k = 0; k < 4; ++k) {
+    if (!isectPrim(s, i, o, d, lo, tfar, h)) return false;
+    if (f(h)) return true;
+    lo = h.t > 0.0f ? u2f(f2u(h.t) + 1u) : (h.t < 0.0f ? u2f(f2u(h.t) - 1u) : 0x1.0p-149f);
+  }
+  return false;
+}
+
 // Reciprocal direction with zero components replaced by a signed tiny value
 // (keeps every slab product finite).
 UBG_HD V3 safeInv(V3 d) {
@@ -401,9 +422,9 @@ UBG_HD V3 safeInv(V3 d) {
 // stk[i * stride] (LDS on the GPU: stride = block size, so the 64 lanes of a
 // wave use 64 distinct banks). kAny returns at the first hit in
 // [tnear, tfar]; otherwise the closest hit (strictly nearer hits replace it).
-template <bool kAny, bool kCount>
+template <bool kAny, bool kCount, class F = AcceptAll>
 UBG_HD bool traverse2(const SceneView& s, V3 o, V3 d, float tnear, float tfar, Hit& best, uint32_t* stk,
-                      int stride, uint32_t* nodeVisits, uint32_t* primTests) {
+                      int stride, uint32_t* nodeVisits, uint32_t* primTests, const F& filt = F()) {
   if (s.nNodes == 0) return false;
   const V3 iv = safeInv(d);
   const float oix = o.x * iv.x, oiy = o.y * iv.y, oiz = o.z * iv.z;
@@ -447,7 +468,7 @@ UBG_HD bool traverse2(const SceneView& s, V3 o, V3 d, float tnear, float tfar, H
       const uint32_t cnt = (code & 7u) + 1u;
       for (uint32_t i = beg; i < beg + cnt; ++i) {
         if (kCount) ++*primTests;
-        if (isectPrim(s, i, o, d, tnear, tmax, h)) {
+        if (isectPrimF(s, i, o, d, tnear, tmax, h, filt)) {
           if (kAny) return true;
           if (!found || h.t < best.t) {
             best = h;
@@ -555,9 +576,10 @@ UBG_HD uint32_t intersectNode8(const uint4* nodes, uint32_t idx, V3 iv, V3 o, fl
   return (inner << 24) | imask;
 }
 
-template <bool kAny, bool kCount>
+template <bool kAny, bool kCount, class F = AcceptAll>
 UBG_HD bool traverse8(const SceneView& s, V3 o, V3 d, float tnear, float tfar, Hit& best, uint32_t* stkBase,
-                      uint32_t* stkHits, int stride, uint32_t* nodeVisits, uint32_t* primTests) {
+                      uint32_t* stkHits, int stride, uint32_t* nodeVisits, uint32_t* primTests,
+                      const F& filt = F()) {
   if (s.nNodes == 0) return false;
   const V3 iv = safeInv(d);
   const uint32_t oct = (d.x < 0.0f ? 1u : 0u) | (d.y < 0.0f ? 2u : 0u) | (d.z < 0.0f ? 4u : 0u);
@@ -573,7 +595,7 @@ UBG_HD bool traverse8(const SceneView& s, V3 o, V3 d, float tnear, float tfar, H
       const uint32_t b = static_cast<uint32_t>(__builtin_ctz(tHits));
       tHits &= tHits - 1u;
       if (kCount) ++*primTests;
-      if (isectPrim(s, tBase + b, o, d, tnear, tmax, h)) {
+      if (isectPrimF(s, tBase + b, o, d, tnear, tmax, h, filt)) {
         if (kAny) return true;
         if (!found || h.t < best.t) {
           best = h;

@@ -59,5 +59,9 @@ void launchResolve(const FrameConst& fc, const SceneView& s, const FrameBufs& b,
 void launchPost(const float4* hi, int W, int H, int ss, float gamma, float4* out, hipStream_t st);
 void launchQuery(const SceneView& s, const void* rays, int64_t n, void* hits, uint8_t* occ,
                  bool any, hipStream_t st);
+// ubg_occluded_segments: `segs` = n ubg_segment records, `excl` = the flat
+// exclusion list (nExcl entries; segment ranges are checked against it).
+void launchSegments(const SceneView& s, const void* segs, int64_t n, const int32_t* excl, int64_t nExcl, float eps,
+                    float grazeCosEps, float coplanarEps, uint8_t* occ, hipStream_t st);
 
 }  // namespace ubg

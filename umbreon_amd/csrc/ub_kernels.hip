@@ -879,6 +879,84 @@ __global__ __launch_bounds__(kTraceBlock) void k_query(SceneView s, const RayIn*
   hits[i] = out;
 }
 
+struct SegIn {
+  float p[3];
+  float q[3];
+  int32_t first, count;
+};
+
+// The edge pass's argument filter (embree_renderer.cpp:71-131).
+struct SegFilter {
+  const int32_t* excl;
+  int32_t nExcl;
+  int64_t baseTris;
+  float graze, coplanar;
+  V3 dir;
+  __device__ bool operator()(const Hit& h) const {
+    if (nExcl > 0 && (h.bits >> kKindShift) == 0u) {
+      int64_t id = static_cast<int64_t>(h.bits & kIdMask);
+      if (baseTris > 0) id %= baseTris;  // instance copies map to their base triangle
+      for (int k = 0; k < nExcl; ++k)
+        if (static_cast<int64_t>(excl[k]) == id) return false;
+    }
+    if (graze > 0.0f || coplanar > 0.0f) {
+      const float nx = h.ng.x, ny = h.ng.y, nz = h.ng.z;
+      const float nlen = sqrtf(nx * nx + ny * ny + nz * nz);
+      const float dlen = sqrtf(dir.x * dir.x + dir.y * dir.y + dir.z * dir.z);
+      if (nlen > 0.0f && dlen > 0.0f) {
+        const float dDotN = nx * dir.x + ny * dir.y + nz * dir.z;
+        const float c = fabsf(dDotN) / (nlen * dlen);
+        if (graze > 0.0f && c <= graze) return false;
+        if (coplanar > 0.0f && h.t * fabsf(dDotN) / nlen <= coplanar) return false;
+      }
+    }
+    return true;
+  }
+};
+
+// embree_renderer.cpp:294-357, one thread per segment.
+template <int kLayout>
+__global__ __launch_bounds__(kTraceBlock) void k_segments(SceneView s, const SegIn* segs, int64_t n,
+                                                          const int32_t* excl, int64_t nExcl, float eps,
+                                                          float graze, float coplanar, uint8_t* occ) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const SegIn sg = segs[i];
+  const V3 p = mk(sg.p[0], sg.p[1], sg.p[2]);
+  const V3 d = mk(sg.q[0] - sg.p[0], sg.q[1] - sg.p[1], sg.q[2] - sg.p[2]);
+  const float L = len(d);
+  uint8_t r = 0;
+  if (L > 0.0f) {
+    const V3 dir = mk(d.x / L, d.y / L, d.z / L);
+    const float tnear = fmaxf(eps * L, eps);
+    const float tfar = L * (1.0f - eps);
+    if (tnear < tfar) {
+      SegFilter f;
+      const bool rangeOk = sg.first >= 0 && sg.count >= 0 && static_cast<int64_t>(sg.first) + sg.count <= nExcl;
+      f.excl = rangeOk && sg.count > 0 ? excl + sg.first : nullptr;
+      f.nExcl = (rangeOk && s.meshBaseTris > 0) ? sg.count : 0;
+      f.baseTris = s.meshBaseTris;
+      f.graze = graze;
+      f.coplanar = coplanar;
+      f.dir = dir;
+      uint32_t* base = ubgLds + threadIdx.x;
+      uint32_t nv = 0, nt = 0;
+      Hit h;
+      bool o;
+      if (f.nExcl == 0 && graze <= 0.0f && coplanar <= 0.0f) {
+        o = trace<kLayout, true, false>(s, p, dir, tnear, tfar, h, &nv, &nt);
+      } else if (kLayout == kLayoutBvh8) {
+        o = traverse8<true, false>(s, p, dir, tnear, tfar, h, base, base + s.stackDepth * blockDim.x, blockDim.x,
+                                   &nv, &nt, f);
+      } else {
+        o = traverse2<true, false>(s, p, dir, tnear, tfar, h, base, blockDim.x, &nv, &nt, f);
+      }
+      r = o ? 1 : 0;
+    }
+  }
+  occ[i] = r;
+}
+
 __global__ void k_zero(unsigned long long* c, int n) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) c[i] = 0ull;
@@ -988,6 +1066,18 @@ void launchQuery(const SceneView& s, const void* rays, int64_t n, void* hits, ui
   else
     hipLaunchKernelGGL(k_query<kLayoutBvh2>, dim3(blocksFor(n, kTraceBlock)), dim3(kTraceBlock), lds, st, s,
                        static_cast<const RayIn*>(rays), n, static_cast<HitOut*>(hits), occ, any ? 1 : 0);
+}
+
+void launchSegments(const SceneView& s, const void* segs, int64_t n, const int32_t* excl, int64_t nExcl, float eps,
+                    float grazeCosEps, float coplanarEps, uint8_t* occ, hipStream_t st) {
+  if (n <= 0) return;
+  const size_t lds = stackLdsBytes(s, kTraceBlock);
+  if (s.layout == kLayoutBvh8)
+    hipLaunchKernelGGL(k_segments<kLayoutBvh8>, dim3(blocksFor(n, kTraceBlock)), dim3(kTraceBlock), lds, st, s,
+                       static_cast<const SegIn*>(segs), n, excl, nExcl, eps, grazeCosEps, coplanarEps, occ);
+  else
+    hipLaunchKernelGGL(k_segments<kLayoutBvh2>, dim3(blocksFor(n, kTraceBlock)), dim3(kTraceBlock), lds, st, s,
+                       static_cast<const SegIn*>(segs), n, excl, nExcl, eps, grazeCosEps, coplanarEps, occ);
 }
 
 }  // namespace ubg

@@ -89,6 +89,7 @@ struct SceneView {
   const void* vtxCol;     // float4
   const DevMaterial* primMats;
   const DevMaterial* meshMats;
+  int64_t meshBaseTris;   // triangles per mesh instance copy (0 = no mesh)
 };
 
 struct DevLight {

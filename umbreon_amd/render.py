@@ -109,6 +109,17 @@ class BuiltScene:
                                           rays.ctypes.data, occ.ctypes.data, 0))
         return occ
 
+    def occluded_segments(self, segments: np.ndarray, exclude_faces=None, eps: float = 1.0e-4,
+                          graze_cos_eps: float = 0.0, coplanar_eps: float = 0.0) -> np.ndarray:
+        """EmbreeRenderer::occluded over a batch (ubg_occluded_segments)."""
+        segments = np.ascontiguousarray(segments, dtype=capi.segment_dtype())
+        ex = np.ascontiguousarray(np.zeros(0, np.int32) if exclude_faces is None else exclude_faces, dtype=np.int32)
+        occ = np.zeros(len(segments), dtype=np.uint8)
+        capi.check(self._lib.ubg_occluded_segments(self.device.handle, self.handle, len(segments),
+                                                   segments.ctypes.data, ex.ctypes.data if len(ex) else None,
+                                                   len(ex), eps, graze_cos_eps, coplanar_eps, occ.ctypes.data, 0))
+        return occ
+
     def render(self, scene: Scene, opt: RenderOptions, hires_only: bool = False,
                want_ids: bool = False, want_ao: bool = False, count_visits: bool = False,
                kernel_times: bool = False, outputs: bool = True,
