@@ -946,7 +946,13 @@ int buildHostScene(const ubg_scene_desc* d, int layout, HostScene& out, std::str
         bx.ext(a.x, a.y, a.z, std::fabs(static_cast<double>(r)));
         bx.ext(b.x, b.y, b.z, std::fabs(static_cast<double>(r)));
         boxes.push_back(bx.toFloat());
-        pushRec(a, b, F4{0, 0, 0, bitsF(UBG_KIND_CONE_CURVE, k)});
+        // axis length, its reciprocal and the radius slope, with the float
+        // operations the intersector would otherwise repeat per test
+        const float axx = b.x - a.x, axy = b.y - a.y, axz = b.z - a.z;
+        const float L = std::sqrt(axx * axx + axy * axy + axz * axz);
+        const float invL = 1.0f / L;
+        const float slope = (b.w - a.w) / L;
+        pushRec(a, b, F4{L, invL, slope, bitsF(UBG_KIND_CONE_CURVE, k)});
         ++k;
       }
     out.nCone = k;

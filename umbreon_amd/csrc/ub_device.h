@@ -193,13 +193,14 @@ UBG_HD bool coneBody(V3 oa, V3 d, V3 w, float r0, float k, float& dt0, float& dt
   return true;
 }
 
-// CONE_LINEAR_CURVE: body between the end planes plus flat disk caps.
-UBG_HD bool isectCone(V3 o, V3 d, float tnear, float tfar, V3 a, float ra, V3 b, float rb, Hit& h) {
+// CONE_LINEAR_CURVE: body between the end planes plus flat disk caps. L =
+// |b - a|, invL = 1 / L and k = (rb - ra) / L come precomputed with the record
+// (the same float operations, done once by the builder).
+UBG_HD bool isectCone(V3 o, V3 d, float tnear, float tfar, V3 a, float ra, V3 b, float rb, float L, float invL,
+                      float k, Hit& h) {
   const V3 ax = b - a;
-  const float L = len(ax);
   if (!(L > 0.0f)) return false;
-  const V3 w = ax * (1.0f / L);
-  const float k = (rb - ra) / L;
+  const V3 w = ax * invL;
   const float A = dot(d, d);
   const V3 mid = (a + b) * 0.5f;
   const float ts = dot(mid - o, d) / A;
@@ -377,7 +378,7 @@ UBG_HD bool isectPrim(const SceneView& s, uint32_t i, V3 o, V3 d, float tnear, f
   if (kind == 1u) {
     r = isectSphere(o, d, tnear, tfar, ld3(p0), p0.w, h);
   } else if (kind == 3u) {
-    r = isectCone(o, d, tnear, tfar, ld3(p0), p0.w, ld3(p1), p1.w, h);
+    r = isectCone(o, d, tnear, tfar, ld3(p0), p0.w, ld3(p1), p1.w, p2.x, p2.y, p2.z, h);
   } else if (kind == 0u) {
     r = isectTri(o, d, tnear, tfar, ld3(p0), mk(p0.w, p1.x, p1.y), mk(p1.z, p1.w, p2.x), h);
   } else {
@@ -709,8 +710,8 @@ UBG_HD int trav8PrimKind(const SceneView& s, Trav8& t, uint32_t* primTests) {
   if (kKind == 1) {
     r = isectSphere(t.o, t.d, t.tnear, t.tmax, ld3(p0), p0.w, h);
   } else if (kKind == 3) {
-    const float4 p1 = p[1];
-    r = isectCone(t.o, t.d, t.tnear, t.tmax, ld3(p0), p0.w, ld3(p1), p1.w, h);
+    const float4 p1 = p[1], p2 = p[2];
+    r = isectCone(t.o, t.d, t.tnear, t.tmax, ld3(p0), p0.w, ld3(p1), p1.w, p2.x, p2.y, p2.z, h);
   } else if (kKind == 0) {
     const float4 p1 = p[1], p2 = p[2];
     r = isectTri(t.o, t.d, t.tnear, t.tmax, ld3(p0), mk(p0.w, p1.x, p1.y), mk(p1.z, p1.w, p2.x), h);

@@ -12,7 +12,7 @@
 //   prims  : 48 B per primitive = 3 x float4, in leaf order, so a leaf is one
 //            contiguous read. The last word holds (kind << 30) | primID.
 //              sphere : (cx, cy, cz, r) | - | (-, -, -, bits)
-//              cone   : (ax, ay, az, ra) | (bx, by, bz, rb) | (-, -, -, bits)
+//              cone   : (ax, ay, az, ra) | (bx, by, bz, rb) | (L, 1/L, (rb-ra)/L, bits)
 //              round  : (ax, ay, az, ra) | (bx, by, bz, rb) | (-, -, -, bits)
 //              tri    : (v0x, v0y, v0z, v1x) | (v1y, v1z, v2x, v2y) | (v2z, -, -, bits)
 //   shading side tables, indexed by the per-kind primID (read once per hit,
