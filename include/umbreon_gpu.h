@@ -39,7 +39,7 @@
 extern "C" {
 #endif
 
-#define UBG_ABI_VERSION 2
+#define UBG_ABI_VERSION 3
 
 /* Status codes. */
 enum {
@@ -197,6 +197,12 @@ typedef struct ubg_render_params {
   int32_t ao_multibounce;
   int32_t ao_low_discrepancy;
   float ao_diffuse_factor;
+  /* Coarse-grid AO (render_options.hpp:64-74, ao/ao_coarse.hpp): > 1 gathers
+   * AO once per cell of the hi-res grid divided by this factor on a first-hit
+   * G-buffer, and shading hits take the guided bilateral lookup of it, or the
+   * exact inline gather where the guides reject; -1 = the supersample factor;
+   * 0 or 1 = inline AO everywhere. */
+  int32_t ao_res_div;
 } ubg_render_params;
 
 /* Ray accounting of one frame. A pure function of the scene and options:

@@ -925,8 +925,67 @@ struct Ctx {
   std::vector<Light<T>> lights;
   V3<T> ambLight, bg, camDir;
   V3<T> aoUp;  // resolved bent-normal gradient axis (embree_renderer.cpp:1320-1323)
-  int wHi;
+  int wHi, hHi;
+  const struct CoarseAo* grid = nullptr;  // coarse-grid AO (aoResDiv > 1), else null
 };
+
+// ao_coarse.hpp:54-65 -- the coarse AO grid: gathered openness (and bent
+// normal for the enhanced estimator) plus the lookup guides
+struct CoarseAo {
+  int w = 0, h = 0;
+  std::vector<double> openness, bent, normal, depth;
+  std::vector<uint8_t> hit;
+};
+
+// ao_coarse.hpp:138-205 -- per-hit bilateral lookup over the 2x2 surrounding
+// cells (bilinear x normal^32 x depth weights); false below the 0.3 weight sum
+template <class T>
+inline bool sampleCoarseAo(const CoarseAo& g, int latW, int latH, uint32_t px, uint32_t py, V3<T> N, T z,
+                           T& openness, V3<T>& bent) {
+  if (g.w <= 0 || g.h <= 0) return false;
+  const T sx = (T(static_cast<float>(px)) + T(0.5f)) * T(static_cast<float>(g.w)) / T(static_cast<float>(latW)) -
+               T(0.5f);
+  const T sy = (T(static_cast<float>(py)) + T(0.5f)) * T(static_cast<float>(g.h)) / T(static_cast<float>(latH)) -
+               T(0.5f);
+  const int bx = static_cast<int>(std::floor(sx));
+  const int by = static_cast<int>(std::floor(sy));
+  const T fx = sx - T(static_cast<float>(bx));
+  const T fy = sy - T(static_cast<float>(by));
+  const bool quality = !g.bent.empty();
+  T wSum = T(0), opSum = T(0);
+  V3<T> bentSum{T(0), T(0), T(0)};
+  for (int dy = 0; dy < 2; ++dy) {
+    const int cy = by + dy;
+    if (cy < 0 || cy >= g.h) continue;
+    for (int dx = 0; dx < 2; ++dx) {
+      const int cx = bx + dx;
+      if (cx < 0 || cx >= g.w) continue;
+      const std::size_t cp = static_cast<std::size_t>(cy) * g.w + cx;
+      if (!g.hit[cp]) continue;
+      const T wb = (dx ? fx : T(1) - fx) * (dy ? fy : T(1) - fy);
+      const V3<T> Nc{T(g.normal[3 * cp]), T(g.normal[3 * cp + 1]), T(g.normal[3 * cp + 2])};
+      const T nd = dot(N, Nc);
+      if (nd <= T(0)) continue;
+      const T wn = std::pow(nd, T(32.0f));
+      const T zc = T(g.depth[cp]);
+      const T wz = std::exp(-std::fabs(z - zc) / (T(0.02f) * z + T(1e-6f)));
+      const T wgt = wb * wn * wz;
+      if (wgt <= T(0)) continue;
+      wSum += wgt;
+      opSum += wgt * T(g.openness[cp]);
+      if (quality) {
+        bentSum.x += wgt * T(g.bent[3 * cp]);
+        bentSum.y += wgt * T(g.bent[3 * cp + 1]);
+        bentSum.z += wgt * T(g.bent[3 * cp + 2]);
+      }
+    }
+  }
+  if (wSum < T(0.3f)) return false;
+  const T inv = T(1) / wSum;
+  openness = opSum * inv;
+  if (quality) bent = safeNormalize(V3<T>{bentSum.x * inv, bentSum.y * inv, bentSum.z * inv}, N);
+  return true;
+}
 
 // secondary_rays.hpp:102-109
 template <class T>
@@ -964,13 +1023,13 @@ inline T computeShadow(const Ctx<T>& c, V3<T> P, V3<T> Ng, V3<T> N, T eps, const
 // ambient_occlusion.hpp:33-57
 template <class T>
 inline T computeAO(const Ctx<T>& c, V3<T> P, V3<T> Ng, V3<T> Ns, T eps, int nSamples, T aoRadius,
-                   uint32_t px, uint32_t py, Counters& cnt) {
+                   uint32_t px, uint32_t py, int wSeed, Counters& cnt) {
   if (nSamples <= 0) return T(1);
   const Frame<T> f = frameFromNormal(Ns);
   V3<T> ng = (dot(Ng, Ns) < T(0)) ? neg(Ng) : Ng;
   ng = safeNormalize(ng, Ns);
   const V3<T> O = P + ng * eps;
-  const uint32_t base = px + py * static_cast<uint32_t>(c.wHi);
+  const uint32_t base = px + py * static_cast<uint32_t>(wSeed);
   int hits = 0;
   for (int i = 0; i < nSamples; ++i) {
     uint32_t s0 = base;
@@ -1041,7 +1100,7 @@ inline bool aoEnhanced(const ubg_render_params* p) {
 // normal estimator from one cosine ray set
 template <class T>
 inline AOQuality<T> computeAOQuality(const Ctx<T>& c, V3<T> P, V3<T> Ng, V3<T> Ns, T eps, int nSamples,
-                                     uint32_t px, uint32_t py, Counters& cnt) {
+                                     uint32_t px, uint32_t py, int wSeed, Counters& cnt) {
   const ubg_render_params* p = c.p;
   AOQuality<T> r;
   if (nSamples <= 0) {
@@ -1052,7 +1111,7 @@ inline AOQuality<T> computeAOQuality(const Ctx<T>& c, V3<T> P, V3<T> Ng, V3<T> N
   V3<T> ng = (dot(Ng, Ns) < T(0)) ? neg(Ng) : Ng;
   ng = safeNormalize(ng, Ns);
   const V3<T> O = P + ng * eps;
-  const uint32_t base = px + py * static_cast<uint32_t>(c.wHi);
+  const uint32_t base = px + py * static_cast<uint32_t>(wSeed);
   const T R = T(p->ao_distance);
   T radius[3], weight[3];
   int K;
@@ -1215,21 +1274,25 @@ struct AoShadeT {
 // factors: bent-normal sky/ground ambient, per-channel (multibounce) aoFactor,
 // direct-diffuse darkening
 template <class T>
-inline AoShadeT<T> aoShade(const Ctx<T>& c, V3<T> P, V3<T> Ng, V3<T> N, V3<T> C, T eps, uint32_t px,
+inline AoShadeT<T> aoShade(const Ctx<T>& c, V3<T> P, V3<T> Ng, V3<T> N, V3<T> C, T eps, T hitT, uint32_t px,
                            uint32_t py, Counters& cnt) {
   const ubg_render_params* p = c.p;
   AoShadeT<T> r;
   r.ambLight = c.ambLight;
   if (p->ao_samples <= 0) return r;
   const bool enh = aoEnhanced<T>(p);
-  T openness;
+  T openness = T(1);
   V3<T> bent = N;
-  if (enh) {
-    const AOQuality<T> q = computeAOQuality(c, P, Ng, N, eps, p->ao_samples, px, py, cnt);
+  // hit_shader.hpp:124-140: the coarse grid's bilateral lookup, else the
+  // exact inline gather with the hi-res seeds
+  const bool coarse = c.grid != nullptr && sampleCoarseAo(*c.grid, c.wHi, c.hHi, px, py, N, hitT, openness, bent);
+  if (coarse) {
+  } else if (enh) {
+    const AOQuality<T> q = computeAOQuality(c, P, Ng, N, eps, p->ao_samples, px, py, c.wHi, cnt);
     openness = q.openness;
     bent = q.bent;
   } else {
-    openness = computeAO(c, P, Ng, N, eps, p->ao_samples, T(p->ao_distance), px, py, cnt);
+    openness = computeAO(c, P, Ng, N, eps, p->ao_samples, T(p->ao_distance), px, py, c.wHi, cnt);
   }
   r.openness = openness;
   if (enh && p->ao_bent_normal) {
@@ -1286,7 +1349,7 @@ inline HitShade<T> shadeHit(const Ctx<T>& c, const Hit<T>& h, V3<T> rd, V3<T> or
     const Mat& mat = g.triMat.empty() ? g.meshMats[0] : g.meshMats[g.triMat[baseTri]];
     const V3<T> P{org.x + rd.x * h.t, org.y + rd.y * h.t, org.z + rd.z * h.t};
     const T secEps = selfIntersectEps(P, rd, h.t);
-    const AoShadeT<T> ao = aoShade(c, P, Ng, N, C, secEps, px, py, cnt);
+    const AoShadeT<T> ao = aoShade(c, P, Ng, N, C, secEps, h.t, px, py, cnt);
     hs.aoOpen = ao.openness;
     hs.color = shadeLocal(c, mat, C, N, V, ao.ambLight, ao.aoFactor, ao.diffuseAo, P, Ng, secEps,
                           shadowsActive, c.p->shadow_samples, px, py, cnt);
@@ -1305,7 +1368,7 @@ inline HitShade<T> shadeHit(const Ctx<T>& c, const Hit<T>& h, V3<T> rd, V3<T> or
   const bool fromEdge = side->fromEdge != 0;
   AoShadeT<T> ao;
   ao.ambLight = c.ambLight;
-  if (!fromEdge) ao = aoShade(c, P, Ng, N, C, secEps, px, py, cnt);
+  if (!fromEdge) ao = aoShade(c, P, Ng, N, C, secEps, h.t, px, py, cnt);
   hs.aoOpen = ao.openness;
   const bool primShadows = !fromEdge && shadowsActive;
   hs.color = shadeLocal(c, pm, C, N, V, ao.ambLight, ao.aoFactor, ao.diffuseAo, P, Ng, secEps, primShadows,
@@ -1441,6 +1504,7 @@ static int renderT(const ubg_scene_desc* desc, const ubg_render_params* p, ubg_f
   c.s = &s;
   c.p = p;
   c.wHi = W;
+  c.hHi = H;
   // embree_renderer.cpp:375-386 (camera basis)
   const ubg_camera& cam = p->camera;
   const T aspect = T(static_cast<float>(W)) / T(static_cast<float>(H));
@@ -1468,6 +1532,96 @@ static int renderT(const ubg_scene_desc* desc, const ubg_render_params* p, ubg_f
   c.aoUp = p->ao_use_camera_up ? trueUp : safeNormalize(v3<T>(p->ao_up), trueUp);
   c.bg = v3<T>(p->background);
   const V3<T> pos = v3<T>(cam.position);
+
+  // embree_renderer.cpp:527-553 + pipeline.cpp:68: coarse-grid AO pre-pass
+  // (aoResDiv -1 = the supersample factor) on a first-hit G-buffer
+  // (pt1_gbuffer.hpp:86-200), gathered with the coarse lattice seeds
+  // (ao_coarse.hpp:73-129)
+  CoarseAo grid;
+  Counters coarseCnt;
+  {
+    const int div = p->ao_res_div < 0 ? ss : p->ao_res_div;
+    const Geometry& g = s.g;
+    bool real = !g.idx.empty();
+    for (const auto& sd : g.sphSide) real = real || sd.fromEdge == 0;
+    for (const auto& sd : g.cSide) real = real || sd.fromEdge == 0;
+    for (const auto& sd : g.rSide) real = real || sd.fromEdge == 0;
+    if (div > 1 && p->ao_samples > 0 && real) {
+      const int cw = (W + div - 1) / div, ch = (H + div - 1) / div;
+      const bool quality = aoEnhanced<T>(p);
+      grid.w = cw;
+      grid.h = ch;
+      const std::size_t nc = static_cast<std::size_t>(cw) * ch;
+      grid.openness.assign(nc, 1.0);
+      if (quality) grid.bent.assign(3 * nc, 0.0);
+      grid.normal.assign(3 * nc, 0.0);
+      grid.depth.assign(nc, 0.0);
+      grid.hit.assign(nc, 0);
+      const V3<T> posC = v3<T>(cam.position);
+      for (int cy = 0; cy < ch; ++cy) {
+        const T v = T(1) - T(2) * (T(static_cast<float>(cy)) + T(0.5f)) / T(static_cast<float>(ch));
+        for (int cx = 0; cx < cw; ++cx) {
+          const T u = T(2) * (T(static_cast<float>(cx)) + T(0.5f)) / T(static_cast<float>(cw)) - T(1);
+          V3<T> org, rd;
+          if (cam.orthographic) {
+            org = posC + right * (u * halfW) + trueUp * (v * halfH);
+            rd = dir;
+          } else {
+            org = posC;
+            rd = normalize(dir + right * (u * persHalfW) + trueUp * (v * persHalfH));
+          }
+          coarseCnt.primary++;
+          const Hit<T> h = intersect(s, org, rd, T(0), std::numeric_limits<T>::infinity());
+          if (h.kind < 0) continue;
+          V3<T> N, Ng;
+          if (h.kind == UBG_KIND_MESH) {
+            const uint32_t* ix = &g.idx[3 * h.id];
+            const T w0 = T(1) - h.u - h.v;
+            V3<T> nb{T(0), T(0), T(0)};
+            for (int k = 0; k < 3; ++k) {
+              const T a0 = T(g.nrm[3 * ix[0] + k]), a1 = T(g.nrm[3 * ix[1] + k]), a2 = T(g.nrm[3 * ix[2] + k]);
+              const T val = w0 * a0 + h.u * a1 + h.v * a2;
+              if (k == 0) nb.x = val;
+              else if (k == 1) nb.y = val;
+              else nb.z = val;
+            }
+            N = safeNormalize(nb);
+            Ng = safeNormalize(h.ng);
+          } else {
+            const PrimSide* sd = h.kind == UBG_KIND_SPHERE        ? &g.sphSide[h.id]
+                                 : h.kind == UBG_KIND_CONE_CURVE ? &g.cSide[h.id]
+                                                                 : &g.rSide[h.id];
+            if (sd->fromEdge) continue;
+            N = safeNormalize(h.ng);
+            Ng = N;
+          }
+          if (dot(N, rd) > T(0)) N = neg(N);
+          if (dot(Ng, rd) > T(0)) Ng = neg(Ng);
+          const V3<T> P{org.x + rd.x * h.t, org.y + rd.y * h.t, org.z + rd.z * h.t};
+          const std::size_t cp = static_cast<std::size_t>(cy) * cw + cx;
+          grid.hit[cp] = 1;
+          grid.normal[3 * cp] = double(N.x);
+          grid.normal[3 * cp + 1] = double(N.y);
+          grid.normal[3 * cp + 2] = double(N.z);
+          grid.depth[cp] = double(h.t);
+          const T secEps = selfIntersectEps(P, N, h.t);
+          if (quality) {
+            const AOQuality<T> q = computeAOQuality(c, P, Ng, N, secEps, p->ao_samples, static_cast<uint32_t>(cx),
+                                                    static_cast<uint32_t>(cy), cw, coarseCnt);
+            grid.openness[cp] = double(q.openness);
+            grid.bent[3 * cp] = double(q.bent.x);
+            grid.bent[3 * cp + 1] = double(q.bent.y);
+            grid.bent[3 * cp + 2] = double(q.bent.z);
+          } else {
+            grid.openness[cp] = double(computeAO(c, P, Ng, N, secEps, p->ao_samples, T(p->ao_distance),
+                                                 static_cast<uint32_t>(cx), static_cast<uint32_t>(cy), cw,
+                                                 coarseCnt));
+          }
+        }
+      }
+      c.grid = &grid;
+    }
+  }
 
   const std::size_t npx = static_cast<std::size_t>(W) * H;
   std::vector<float> hiColor(npx * 4, 0.0f), hiDepth(npx, 0.0f), hiViewZ(npx, 0.0f),
@@ -1537,7 +1691,7 @@ static int renderT(const ubg_scene_desc* desc, const ubg_render_params* p, ubg_f
     }
   }
   const auto t2 = std::chrono::steady_clock::now();
-  Counters tot;
+  Counters tot = coarseCnt;
   for (const auto& k : cnts) {
     tot.primary += k.primary;
     tot.shaded += k.shaded;

@@ -467,6 +467,91 @@ def ao_low_discrepancy():
     return Case("ao_low_discrepancy", r, chk)
 
 
+# ---------------------------------------------------------------- coarse-grid AO
+def _coarse_scene(shell=False):
+    """makeAoScene (test_ao_coarse.cpp:37-72): ambient-only white floor, black
+    slab over x > 0.2, ortho camera on +z; optionally a 40%-opaque shell at z=1."""
+    p = [(-2, -2, 0), (2, -2, 0), (2, 2, 0), (-2, -2, 0), (2, 2, 0), (-2, 2, 0)]
+    n = [(0, 0, 1)] * 6
+    c = [(1, 1, 1, 1)] * 6
+    z = 0.4
+    p += [(0.2, -2, z), (4, -2, z), (4, 2, z), (0.2, -2, z), (4, 2, z), (0.2, 2, z)]
+    n += [(0, 0, -1)] * 6
+    c += [(0, 0, 0, 1)] * 6
+    if shell:
+        p += [(-2, -2, 1), (2, -2, 1), (2, 2, 1), (-2, -2, 1), (2, 2, 1), (-2, 2, 1)]
+        n += [(0, 0, 1)] * 6
+        c += [(0.5, 0.7, 0.9, 0.4)] * 6
+    sc = Scene()
+    sc.set_mesh(p, n, c, material=Material(ambient=1.0, diffuse=0.0))
+    sc.camera = ortho_cam()
+    sc.background = (0.1, 0.1, 0.1)
+    return sc
+
+
+@case
+def coarse_ao_inline_neutrality():
+    # test_ao_coarse.cpp:210-230: aoResDiv 0 and 1 are both the inline path
+    sc = _coarse_scene()
+    o0 = opts(8, 8, supersample=2, ao_samples=16, ao_distance=100.0)
+    o1 = opts(8, 8, supersample=2, ao_samples=16, ao_distance=100.0, ao_res_div=1)
+    e0 = opts(8, 8, supersample=2, ao_samples=16, ao_distance=100.0, ao_multi_scale=True, ao_multibounce=True)
+    e1 = opts(8, 8, supersample=2, ao_samples=16, ao_distance=100.0, ao_multi_scale=True, ao_multibounce=True,
+              ao_res_div=1)
+
+    def chk(r):
+        assert np.array_equal(r["o0"].color, r["o1"].color)
+        assert np.array_equal(r["e0"].color, r["e1"].color)
+    return Case("coarse_ao_inline_neutrality", {"o0": (sc, o0), "o1": (sc, o1), "e0": (sc, e0), "e1": (sc, e1)},
+                chk)
+
+
+@case
+def coarse_ao_close_to_inline():
+    # test_ao_coarse.cpp:232-269: "out" (= ss) coarse AO vs inline
+    sc = _coarse_scene()
+    full = opts(16, 16, supersample=3, ao_samples=256, ao_distance=100.0, ao_low_discrepancy=True)
+    coarse = opts(16, 16, supersample=3, ao_samples=256, ao_distance=100.0, ao_low_discrepancy=True, ao_res_div=-1)
+    f32 = opts(16, 16, supersample=3, ao_samples=32, ao_distance=100.0)
+    c32 = opts(16, 16, supersample=3, ao_samples=32, ao_distance=100.0, ao_res_div=-1)
+
+    def chk(r):
+        assert np.abs(r["full"].color - r["coarse"].color).max() <= 0.05
+        assert np.abs(r["f32"].color - r["c32"].color).mean() < 0.02
+        assert not np.array_equal(r["full"].color, r["coarse"].color)
+    return Case("coarse_ao_close_to_inline", {"full": (sc, full), "coarse": (sc, coarse), "f32": (sc, f32),
+                                              "c32": (sc, c32)}, chk)
+
+
+@case
+def coarse_ao_fallback_exact():
+    # test_ao_coarse.cpp:271-302: ss 1 with an explicit div 2: rejected
+    # lookups fall back to the exact inline gather (same seeds), so those
+    # pixels equal the inline render bitwise while interpolated ones do not
+    sc = _coarse_scene()
+    full = opts(24, 24, ao_samples=16, ao_distance=100.0)
+    coarse = opts(24, 24, ao_samples=16, ao_distance=100.0, ao_res_div=2)
+
+    def chk(r):
+        same = np.all(r["full"].color == r["coarse"].color, axis=2)
+        assert 0 < same.sum() < same.size, same.sum()
+        assert r["coarse"].stats["ao_rays_traced"] + r["coarse"].stats["ao_rays_sampled"] < \
+            r["full"].stats["ao_rays_traced"] + r["full"].stats["ao_rays_sampled"]
+    return Case("coarse_ao_fallback_exact", {"full": (sc, full), "coarse": (sc, coarse)}, chk)
+
+
+@case
+def coarse_ao_transparency():
+    # test_ao_coarse.cpp:346-373: a 40%-opaque shell over the floor
+    sc = _coarse_scene(shell=True)
+    full = opts(12, 12, supersample=2, ao_samples=16, ao_distance=100.0)
+    coarse = opts(12, 12, supersample=2, ao_samples=16, ao_distance=100.0, ao_res_div=-1)
+
+    def chk(r):
+        assert np.abs(r["full"].color - r["coarse"].color).max() <= 0.05
+    return Case("coarse_ao_transparency", {"full": (sc, full), "coarse": (sc, coarse)}, chk)
+
+
 # ---------------------------------------------------------------- transparency
 def blend_scene(quads, bg):
     """sceneOfBlend (test_render_transparency.cpp:53-66) with fragment alpha."""

@@ -293,3 +293,19 @@ def test_occluded_segments_match_oracle(device, layout):
             assert (g == c).mean() >= 0.9999
     finally:
         bs.close()
+
+
+@pytest.mark.parametrize("enhanced", [False, True], ids=["legacy", "enhanced"])
+def test_coarse_ao_ballstick_reduced(device, enhanced):
+    """Coarse-grid AO (divisor = the supersample factor) against the oracle:
+    same G-buffer cells, same coarse gathers, same lookup / fallback split."""
+    sc, opt = ball_and_stick_scene(n_atoms=4000, width=240, height=135)
+    opt.ao_res_div = -1
+    if enhanced:
+        opt.ao_multi_scale = True
+        opt.ao_bent_normal = True
+        opt.ao_sky_color = (1.0, 0.95, 0.9)
+        opt.ao_ground_color = (0.4, 0.4, 0.5)
+    g = _gpu(device, sc, opt, hires_only=True, want_ids=True, want_ao=True)
+    c = ob.render(sc, opt, hires_only=True)
+    _compare("coarse-AO", g, c, ao=True)

@@ -23,7 +23,7 @@ UBG_ERR_UNSUPPORTED = 5
 UBG_ERR_INTERNAL = 6
 
 # UBG_ABI_VERSION of include/umbreon_gpu.h this mirror is written against
-ABI_VERSION = 2
+ABI_VERSION = 3
 
 BVH2 = 0
 BVH8 = 1
@@ -105,7 +105,7 @@ class RenderParams(C.Structure):
         ("ao_falloff_power", C.c_float), ("ao_multi_scale", C.c_int32), ("ao_bent_normal", C.c_int32),
         ("ao_sky_color", C.c_float * 3), ("ao_ground_color", C.c_float * 3), ("ao_use_camera_up", C.c_int32),
         ("ao_up", C.c_float * 3), ("ao_multibounce", C.c_int32), ("ao_low_discrepancy", C.c_int32),
-        ("ao_diffuse_factor", C.c_float),
+        ("ao_diffuse_factor", C.c_float), ("ao_res_div", C.c_int32),
     ]
 
 

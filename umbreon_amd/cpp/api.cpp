@@ -202,7 +202,6 @@ void checkSupported(const Scene& s, const RenderOptions& o) {
   if (o.gi) no("global illumination (gi)");
   if (o.aaMode != 0) no("adaptive antialiasing (aaMode)");
   if (o.aoWriteAov) no("AO AOVs (aoWriteAov)");
-  if (o.aoResDiv > 1 || o.aoResDiv < 0) no("coarse-grid AO (aoResDiv)");
   if (o.denoiser != 0) no("the final-colour denoiser");
   if (o.envLights > 0) no("environment dome lights (envLights)");
   if (o.strokeEdges.enable) no("stroke edges");
@@ -297,6 +296,7 @@ FrameResult renderPass(const Scene& scene, const RenderOptions& opt, RenderProgr
   p.ao_multibounce = opt.aoMultibounce ? 1 : 0;
   p.ao_low_discrepancy = opt.aoLowDiscrepancy ? 1 : 0;
   p.ao_diffuse_factor = opt.aoDiffuseFactor;
+  p.ao_res_div = opt.aoResDiv;
 
   const std::size_t hiN = static_cast<std::size_t>(opt.width) * ss * opt.height * ss;
   f.width = opt.width;

@@ -958,6 +958,10 @@ int buildHostScene(const ubg_scene_desc* d, int layout, HostScene& out, std::str
     out.nCone = k;
   }
 
+  out.aoReceivers = out.nTris > 0;
+  for (uint32_t info : out.sphInfo) out.aoReceivers = out.aoReceivers || !(info & kInfoFromEdge);
+  for (uint32_t info : out.coneInfo) out.aoReceivers = out.aoReceivers || !(info & kInfoFromEdge);
+  for (uint32_t info : out.rndInfo) out.aoReceivers = out.aoReceivers || !(info & kInfoFromEdge);
   const size_t nPrims = boxes.size();
   if (nPrims == 0) return UBG_OK;
   if (nPrims >= (1u << 28)) {

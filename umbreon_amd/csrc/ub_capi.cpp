@@ -62,17 +62,19 @@ struct ubg_context {
   std::mutex mu;
   // grow-only frame scratch
   DevBuf hitPix, hitT, hitO, hitN, hitC, hitAo, aoT, activeA, activeB, acc, tnear, hiColor, depth, viewZ,
-      hitId, aoOpen, outColor, counters, query, queryOut;
+      hitId, aoOpen, outColor, counters, query, queryOut, hitOpen, hitBent, cGuide, cOpen, cBent;
   unsigned long long* hostCounters = nullptr;  // pinned
   std::vector<DevBuf*> all() {
     return {&hitPix, &hitT,   &hitO,    &hitN,  &hitC,  &hitAo,   &activeA,  &activeB,  &acc,     &tnear,
-            &hiColor, &depth, &viewZ,   &hitId, &aoOpen, &outColor, &counters, &query, &queryOut, &aoT};
+            &hiColor, &depth, &viewZ,   &hitId, &aoOpen, &outColor, &counters, &query, &queryOut, &aoT,
+            &hitOpen, &hitBent, &cGuide, &cOpen, &cBent};
   }
 };
 
 struct ubg_scene {
   ubg_context* ctx = nullptr;
   ubg::SceneView view{};
+  bool aoReceivers = false;
   std::vector<void*> allocs;
   ubg_scene_info info{};
 };
@@ -298,6 +300,7 @@ int ubg_scene_create_ex(ubg_context* ctx, const ubg_scene_desc* desc, int32_t bv
   }
   s->view.nNodes = static_cast<int32_t>(hs.nNodes);
   s->view.meshBaseTris = hs.meshBaseTris;
+  s->aoReceivers = hs.aoReceivers;
   s->view.nPrims = static_cast<int32_t>(hs.prims.size() / 3);  // records in either order
   s->view.layout = hs.layout;
   s->view.stackDepth = std::max(1, hs.stackDepth);
@@ -426,6 +429,15 @@ int ubg_render(ubg_context* ctx, const ubg_scene* scene, const ubg_render_params
     }
     fc.aoUpAxis[0] = up.x, fc.aoUpAxis[1] = up.y, fc.aoUpAxis[2] = up.z;
   }
+  // coarse-grid AO (pipeline.cpp:68: -1 = the supersample factor;
+  // embree_renderer.cpp:531-532: only with AO and something AO can darken)
+  {
+    const int div = p->ao_res_div < 0 ? ss : p->ao_res_div;
+    fc.coarse = (div > 1 && p->ao_samples > 0 && scene->aoReceivers) ? 1 : 0;
+    fc.cW = fc.coarse ? (W + div - 1) / div : 0;
+    fc.cH = fc.coarse ? (H + div - 1) / div : 0;
+    fc.seedW = W;
+  }
   fc.nLights = p->n_lights;
   for (int i = 0; i < p->n_lights; ++i) {
     const ubg_light& dl = p->lights[i];
@@ -458,6 +470,16 @@ int ubg_render(ubg_context* ctx, const ubg_scene* scene, const ubg_render_params
   need(ctx->hitC, qn * 16);
   need(ctx->hitAo, qn * 4);
   if (fc.aoEnh && p->ao_samples > 0) need(ctx->aoT, qn * static_cast<size_t>(p->ao_samples) * 4);
+  if (fc.coarse) {
+    const size_t cells = static_cast<size_t>(fc.cW) * fc.cH;
+    need(ctx->hitOpen, qn * 4);
+    need(ctx->cGuide, cells * 16);
+    need(ctx->cOpen, cells * 4);
+    if (fc.aoEnh) {
+      need(ctx->hitBent, qn * 16);
+      need(ctx->cBent, cells * 16);
+    }
+  }
   need(ctx->activeA, qn * 4);
   need(ctx->activeB, qn * 4);
   need(ctx->acc, n * 16);
@@ -480,6 +502,11 @@ int ubg_render(ubg_context* ctx, const ubg_scene* scene, const ubg_render_params
   b.hitC = static_cast<float4*>(ctx->hitC.p);
   b.hitAo = static_cast<uint32_t*>(ctx->hitAo.p);
   b.aoT = (fc.aoEnh && p->ao_samples > 0) ? static_cast<float*>(ctx->aoT.p) : nullptr;
+  b.hitOpen = fc.coarse ? static_cast<float*>(ctx->hitOpen.p) : nullptr;
+  b.hitBent = (fc.coarse && fc.aoEnh) ? static_cast<float4*>(ctx->hitBent.p) : nullptr;
+  b.cGuide = fc.coarse ? static_cast<float4*>(ctx->cGuide.p) : nullptr;
+  b.cOpen = fc.coarse ? static_cast<float*>(ctx->cOpen.p) : nullptr;
+  b.cBent = (fc.coarse && fc.aoEnh) ? static_cast<float4*>(ctx->cBent.p) : nullptr;
   b.activeIn = static_cast<uint32_t*>(ctx->activeA.p);
   b.activeOut = static_cast<uint32_t*>(ctx->activeB.p);
   b.acc = static_cast<float4*>(ctx->acc.p);
@@ -530,6 +557,17 @@ int ubg_render(ubg_context* ctx, const ubg_scene* scene, const ubg_render_params
   };
   ubg::launchZeroCounters(b.counters, ubg::kCounterWords, st);
   UBG_HIP(hipEventRecord(ctx->ev[0], st), "hipEventRecord");
+  if (fc.coarse) {
+    // coarse-grid AO pre-pass (embree_renderer.cpp:527-553): G-buffer cells
+    // through the hit queue, the AO kernels on them with the coarse seed
+    // lattice, then the per-cell reduction into the grid
+    ubg::launchResetQueues(b.counters, st);
+    timed(UBG_KERNEL_PRIMARY, [&] { ubg::launchCoarseGbuf(fc, scene->view, b, countVisits, st); });
+    ubg::FrameConst fcCoarse = fc;
+    fcCoarse.seedW = fc.cW;
+    timed(UBG_KERNEL_AO, [&] { ubg::launchAo(fcCoarse, scene->view, b, countVisits, st); });
+    timed(UBG_KERNEL_AO, [&] { ubg::launchCoarseReduce(fcCoarse, b, st); });
+  }
   int64_t nActive = npx;
   ubg::PartFill fill{};  // continuation-queue fills the next layer consumes
   bool cancelled = false;

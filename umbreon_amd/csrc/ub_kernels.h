@@ -23,6 +23,13 @@ struct FrameBufs {
   float4* hitC;       // pigment rgb, opacity
   uint32_t* hitAo;    // AO occluded count
   float* aoT;         // enhanced AO: nearest-hit distance per (slot, sample); -1 = below the horizon
+  float* hitOpen;     // coarse AO: openness looked up for the hit (kFlagCoarse)
+  float4* hitBent;    // coarse AO + enhanced estimator: looked-up bent normal
+  // coarse AO grid (cW x cH): guide (shading normal, first-hit distance; 0 =
+  // no AO cell), gathered openness, bent normal (enhanced estimator)
+  float4* cGuide;
+  float* cOpen;
+  float4* cBent;
   // continuation (transparency) queue and per-pixel walk state
   uint32_t* activeIn;
   uint32_t* activeOut;
@@ -56,6 +63,12 @@ void launchPrimary(const FrameConst& fc, const SceneView& s, const FrameBufs& b,
 void launchAo(const FrameConst& fc, const SceneView& s, const FrameBufs& b, bool countVisits, hipStream_t st);
 void launchResolve(const FrameConst& fc, const SceneView& s, const FrameBufs& b, int iter, bool countVisits,
                    hipStream_t st);
+// Coarse-grid AO pre-pass: the first-hit G-buffer of the cW x cH grid (AO
+// cells appended to the hit queue), then, after the AO kernels ran on that
+// queue with seedW = cW, the per-cell reduction into cOpen / cBent.
+void launchCoarseGbuf(const FrameConst& fc, const SceneView& s, const FrameBufs& b, bool countVisits,
+                      hipStream_t st);
+void launchCoarseReduce(const FrameConst& fc, const FrameBufs& b, hipStream_t st);
 void launchPost(const float4* hi, int W, int H, int ss, float gamma, float4* out, hipStream_t st);
 void launchQuery(const SceneView& s, const void* rays, int64_t n, void* hits, uint8_t* occ,
                  bool any, hipStream_t st);

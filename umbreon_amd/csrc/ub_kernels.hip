@@ -87,6 +87,7 @@ __device__ __forceinline__ bool trace(const SceneView& s, V3 o, V3 d, float tn, 
 constexpr uint32_t kFlagMesh = 1u << 24;
 constexpr uint32_t kFlagAo = 1u << 25;
 constexpr uint32_t kFlagShadow = 1u << 26;
+constexpr uint32_t kFlagCoarse = 1u << 27;  // AO openness taken from the coarse grid
 
 // This block's statistics group of the counter buffer.
 __device__ __forceinline__ unsigned long long* statGroup(unsigned long long* cnt) {
@@ -133,11 +134,12 @@ __device__ __forceinline__ void addVisits(unsigned long long* cnt, int cls, uint
   }
 }
 
-// embree_renderer.cpp:1400-1414 (top-left origin, pixel centers)
-__device__ __forceinline__ void genRay(const FrameConst& fc, uint32_t px, uint32_t py, V3& org,
-                                       V3& rd) {
-  const float v = 1.0f - 2.0f * ((float)py + 0.5f) / (float)fc.H;
-  const float u = 2.0f * ((float)px + 0.5f) / (float)fc.W - 1.0f;
+// embree_renderer.cpp:1400-1414 (top-left origin, pixel centers) on a w x h
+// lattice of the camera plane
+__device__ __forceinline__ void genRayWH(const FrameConst& fc, uint32_t px, uint32_t py, int w, int h, V3& org,
+                                         V3& rd) {
+  const float v = 1.0f - 2.0f * ((float)py + 0.5f) / (float)h;
+  const float u = 2.0f * ((float)px + 0.5f) / (float)w - 1.0f;
   const V3 pos = mk(fc.pos[0], fc.pos[1], fc.pos[2]);
   const V3 dir = mk(fc.dir[0], fc.dir[1], fc.dir[2]);
   const V3 right = mk(fc.right[0], fc.right[1], fc.right[2]);
@@ -149,6 +151,9 @@ __device__ __forceinline__ void genRay(const FrameConst& fc, uint32_t px, uint32
     org = pos;
     rd = normalize(dir + right * (u * fc.persHalfW) + up * (v * fc.persHalfH));
   }
+}
+__device__ __forceinline__ void genRay(const FrameConst& fc, uint32_t px, uint32_t py, V3& org, V3& rd) {
+  genRayWH(fc, px, py, fc.W, fc.H, org, rd);
 }
 
 // transparency.hpp:118-129 (view-z clip planes -> ray t range)
@@ -195,6 +200,49 @@ __device__ __forceinline__ void finalizePixel(const FrameConst& fc, const FrameB
     }
   }
   b.hiColor[pix] = make_float4(r, g, bl, outA);
+}
+
+// ao_coarse.hpp:138-205: bilateral lookup of the coarse AO grid for a hit at
+// hi-res pixel (px, py) with shading normal N at ray distance z: the 2x2
+// surrounding cells weighted bilinear x dot(N, Nc)^32 x exp(-|z - zc| /
+// (0.02 z + 1e-6)); false when the weight sum is below 0.3 (the caller then
+// gathers inline).
+__device__ bool sampleCoarseAo(const FrameConst& fc, const FrameBufs& b, uint32_t px, uint32_t py, V3 N, float z,
+                               float& openness, V3& bent) {
+  const float sx = ((float)px + 0.5f) * (float)fc.cW / (float)fc.W - 0.5f;
+  const float sy = ((float)py + 0.5f) * (float)fc.cH / (float)fc.H - 0.5f;
+  const int bx = static_cast<int>(floorf(sx));
+  const int by = static_cast<int>(floorf(sy));
+  const float fx = sx - (float)bx;
+  const float fy = sy - (float)by;
+  float wSum = 0.0f, opSum = 0.0f;
+  V3 bentSum = mk(0.0f, 0.0f, 0.0f);
+  for (int dy = 0; dy < 2; ++dy) {
+    const int cy = by + dy;
+    if (cy < 0 || cy >= fc.cH) continue;
+    for (int dx = 0; dx < 2; ++dx) {
+      const int cx = bx + dx;
+      if (cx < 0 || cx >= fc.cW) continue;
+      const int cp = cy * fc.cW + cx;
+      const float4 g = b.cGuide[cp];
+      if (!(g.w > 0.0f)) continue;  // no AO cell (miss / outline)
+      const float wb = (dx ? fx : 1.0f - fx) * (dy ? fy : 1.0f - fy);
+      const float nd = dot(N, ld3(g));
+      if (nd <= 0.0f) continue;
+      const float wn = powf(nd, 32.0f);
+      const float wz = expf(-fabsf(z - g.w) / (0.02f * z + 1e-6f));
+      const float wgt = wb * wn * wz;
+      if (wgt <= 0.0f) continue;
+      wSum += wgt;
+      opSum += wgt * b.cOpen[cp];
+      if (fc.aoEnh) bentSum = bentSum + ld3(b.cBent[cp]) * wgt;
+    }
+  }
+  if (wSum < 0.3f) return false;
+  const float inv = 1.0f / wSum;
+  openness = opSum * inv;
+  if (fc.aoEnh) bent = safeNormalize(mk(bentSum.x * inv, bentSum.y * inv, bentSum.z * inv), N);
+  return true;
 }
 
 template <bool kCount, int kLayout>
@@ -313,6 +361,17 @@ __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(kTr
   V3 ng = dot(h.ng, N) < 0.0f ? neg(h.ng) : h.ng;
   ng = safeNormalize(ng, N);
   const V3 O = P + ng * eps;
+  if (fc.coarse && (flags & kFlagAo)) {
+    // hit_shader.hpp:124-140: a hit the coarse grid covers needs no AO rays
+    float open;
+    V3 bent = N;
+    const uint32_t px = pix % static_cast<uint32_t>(fc.W), py = pix / static_cast<uint32_t>(fc.W);
+    if (sampleCoarseAo(fc, b, px, py, N, h.t, open, bent)) {
+      flags = (flags & ~kFlagAo) | kFlagCoarse;
+      b.hitOpen[slot] = open;
+      if (fc.aoEnh) b.hitBent[slot] = make_float4(bent.x, bent.y, bent.z, 0.0f);
+    }
+  }
   b.hitPix[slot] = pix;
   b.hitT[slot] = h.t;
   b.hitO[slot] = make_float4(O.x, O.y, O.z, eps);
@@ -523,8 +582,8 @@ __global__ __launch_bounds__(kTraceBlock) void k_ao_quality(FrameConst fc, Scene
     const float4 N4 = b.hitN[slot];
     if (__float_as_uint(N4.w) & kFlagAo) {
       const float4 O4 = b.hitO[slot];
-      const uint32_t pix = b.hitPix[slot];
-      const uint32_t px = pix % static_cast<uint32_t>(fc.W), py = pix / static_cast<uint32_t>(fc.W);
+      const uint32_t pix = b.hitPix[slot];  // seed base px + py * seedW
+      const uint32_t px = pix % static_cast<uint32_t>(fc.seedW), py = pix / static_cast<uint32_t>(fc.seedW);
       const V3 N = ld3(N4);
       const Frame3 f = frameFromNormal(N);
       uint32_t c0 = px + 1u, c1 = py + 1u;
@@ -551,11 +610,10 @@ __global__ __launch_bounds__(kTraceBlock) void k_ao_quality(FrameConst fc, Scene
 }
 
 // computeAOQuality's reduction (ambient_occlusion.hpp:211-264) over the
-// recorded distances of one hit, in sample order, and aoApplyFactors
-// (ao_shade.hpp:83-120): per-channel AO factor, ambient light (bent-normal
-// sky/ground gradient) and the direct-diffuse factor.
-__device__ void aoQualityFactors(const FrameConst& fc, const FrameBufs& b, int64_t slot, uint32_t px, uint32_t py,
-                                 V3 N, V3 C, float& openness, V3& aoF, V3& ambL, float& diffAo) {
+// recorded distances of one hit, in sample order: openness and bent normal.
+// (px, py) is the hit's cell on the seed lattice of width fc.seedW.
+__device__ void aoQualityGather(const FrameConst& fc, const FrameBufs& b, int64_t slot, uint32_t px, uint32_t py,
+                                V3 N, float& openness, V3& bentOut) {
   const int n = fc.aoSamples;
   const float R = fc.aoDist;
   float radius[3], weight[3];
@@ -578,7 +636,7 @@ __device__ void aoQualityFactors(const FrameConst& fc, const FrameBufs& b, int64
   uint32_t c0 = px + 1u, c1 = py + 1u;
   tea2(c0, c1);
   const float cpx = u32ToUnorm(c0), cpy = u32ToUnorm(c1);
-  const uint32_t base = px + py * static_cast<uint32_t>(fc.W);
+  const uint32_t base = px + py * static_cast<uint32_t>(fc.seedW);
   // (the per-scale sums of the reference only feed the contact / shape AOVs,
   // which this path does not write; the colour needs the weighted sum)
   float weightedObs = 0.0f;
@@ -612,9 +670,21 @@ __device__ void aoQualityFactors(const FrameConst& fc, const FrameBufs& b, int64
   }
   const float invN = 1.0f / static_cast<float>(n);
   openness = 1.0f - weightedObs * invN;
+  bentOut = fc.aoBent ? safeNormalize(bentAccum, N) : N;
+}
+
+// aoApplyFactors (ao_shade.hpp:83-120): per-channel AO factor (scalar for the
+// legacy estimator, multibounce when asked), ambient light (bent-normal
+// sky/ground gradient) and the direct-diffuse factor.
+__device__ void aoApply(const FrameConst& fc, float openness, V3 bent, V3 C, V3& aoF, V3& ambL, float& diffAo) {
   ambL = mk(fc.amb[0], fc.amb[1], fc.amb[2]);
+  if (!fc.aoEnh) {
+    const float sv = 1.0f - fc.aoIntensity * (1.0f - openness);
+    aoF = mk(sv, sv, sv);
+    diffAo = 1.0f;
+    return;
+  }
   if (fc.aoBent) {
-    const V3 bent = safeNormalize(bentAccum, N);
     const float w = 0.5f * (dot(bent, mk(fc.aoUpAxis[0], fc.aoUpAxis[1], fc.aoUpAxis[2])) + 1.0f);
     ambL = mk(fc.amb[0] * (fc.aoGround[0] + (fc.aoSky[0] - fc.aoGround[0]) * w),
               fc.amb[1] * (fc.aoGround[1] + (fc.aoSky[1] - fc.aoGround[1]) * w),
@@ -664,14 +734,17 @@ __global__ __launch_bounds__(kTraceBlock) void k_resolve(FrameConst fc, SceneVie
     // ao_shade.hpp:83-120 (legacy: scalar aoFactor, scene ambient)
     V3 aoF = mk(1.0f, 1.0f, 1.0f), ambL = mk(fc.amb[0], fc.amb[1], fc.amb[2]);
     float open = 1.0f, diffAo = 1.0f;
-    if (flags & kFlagAo) {
-      if (fc.aoEnh) {
-        aoQualityFactors(fc, b, tid, px, py, N, C, open, aoF, ambL, diffAo);
+    if (flags & (kFlagAo | kFlagCoarse)) {
+      V3 bent = N;
+      if (flags & kFlagCoarse) {
+        open = b.hitOpen[tid];
+        if (fc.aoEnh) bent = ld3(b.hitBent[tid]);
+      } else if (fc.aoEnh) {
+        aoQualityGather(fc, b, tid, px, py, N, open, bent);
       } else {
         open = 1.0f - (float)b.hitAo[tid] / (float)fc.aoSamples;
-        const float sv = 1.0f - fc.aoIntensity * (1.0f - open);
-        aoF = mk(sv, sv, sv);
       }
+      aoApply(fc, open, bent, C, aoF, ambL, diffAo);
     }
     if (iter == 0 && b.aoOpen) b.aoOpen[pix] = open;
     // shading.hpp:92-190
@@ -879,6 +952,88 @@ __global__ __launch_bounds__(kTraceBlock) void k_query(SceneView s, const RayIn*
   hits[i] = out;
 }
 
+// Coarse-grid AO, first half (pt1_gbuffer.hpp:86-200 and the setup of
+// ao_coarse.hpp:73-129): one thread per coarse cell traces the cell's camera
+// ray over [0, inf) and takes the first hit's shading and geometric normals
+// (mesh: interpolated vertex normal; real primitive: the hit normal; outline
+// primitives and misses are no AO cell), both faced to the viewer. It writes
+// the lookup guide (N, t) and appends every AO cell to the hit queue (offset
+// origin, normal, seed base cx + cy * cW) for the AO kernels.
+template <bool kCount, int kLayout>
+__global__ __launch_bounds__(kTraceBlock) void k_coarse_gbuf(FrameConst fc, SceneView s, FrameBufs b) {
+  const int cell = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool valid = cell < fc.cW * fc.cH;
+  bool ao = false;
+  V3 N = mk(0.0f, 0.0f, 0.0f), Ng = N, P = N;
+  float t = 0.0f;
+  uint32_t nv = 0, nt = 0;
+  if (valid) {
+    const uint32_t cx = static_cast<uint32_t>(cell % fc.cW), cy = static_cast<uint32_t>(cell / fc.cW);
+    V3 org, rd;
+    genRayWH(fc, cx, cy, fc.cW, fc.cH, org, rd);
+    Hit h;
+    if (trace<kLayout, false, kCount>(s, org, rd, 0.0f, __builtin_inff(), h, &nv, &nt)) {
+      const uint32_t kind = h.bits >> kKindShift, id = h.bits & kIdMask;
+      ao = true;
+      if (kind == 0u) {
+        const uint4 tri = reinterpret_cast<const uint4*>(s.triIdx)[id];
+        const float4* vn = reinterpret_cast<const float4*>(s.vtxNrm);
+        const float4 n0 = vn[tri.x], n1 = vn[tri.y], n2 = vn[tri.z];
+        const float w0 = 1.0f - h.u - h.v;
+        N = safeNormalize(mk(w0 * n0.x + h.u * n1.x + h.v * n2.x, w0 * n0.y + h.u * n1.y + h.v * n2.y,
+                             w0 * n0.z + h.u * n1.z + h.v * n2.z));
+        Ng = safeNormalize(h.ng);
+      } else {
+        const uint32_t info = kind == 1u ? s.sphInfo[id] : kind == 3u ? s.coneInfo[id] : s.rndInfo[id];
+        if (info & kInfoFromEdge) ao = false;
+        N = safeNormalize(h.ng);
+        Ng = N;
+      }
+      if (ao) {
+        if (dot(N, rd) > 0.0f) N = neg(N);
+        if (dot(Ng, rd) > 0.0f) Ng = neg(Ng);
+        t = h.t;
+        P = mk(org.x + rd.x * t, org.y + rd.y * t, org.z + rd.z * t);
+      }
+    }
+    b.cGuide[cell] = ao ? make_float4(N.x, N.y, N.z, t) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  }
+  waveCount(b.counters, kCntPrimary, valid);
+  if (kCount) addVisits(b.counters, 0, nv, nt);
+  const uint32_t slot = queueAppend(b.counters, kHitFill, b.partCap, ao);
+  if (!ao) return;
+  // ambient_occlusion.hpp:40-43 with the G-buffer's eps rule (ao_coarse.hpp:106-110)
+  const float eps = selfIntersectEps(P, N, t);
+  V3 ng = dot(Ng, N) < 0.0f ? neg(Ng) : Ng;
+  ng = safeNormalize(ng, N);
+  const V3 O = P + ng * eps;
+  b.hitPix[slot] = static_cast<uint32_t>(cell);
+  b.hitO[slot] = make_float4(O.x, O.y, O.z, eps);
+  b.hitN[slot] = make_float4(N.x, N.y, N.z, __uint_as_float(kFlagAo));
+  b.hitAo[slot] = 0u;
+}
+
+// Coarse-grid AO, second half: each queued cell's openness (and bent normal)
+// from its AO rays, stored in the grid. fc.seedW = cW.
+__global__ __launch_bounds__(kTraceBlock) void k_coarse_reduce(FrameConst fc, FrameBufs b) {
+  const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+  const int part = tid / b.partCap;
+  if (part >= kQueueParts || static_cast<uint32_t>(tid % b.partCap) >= queueFill(b.counters, kHitFill, part))
+    return;
+  const uint32_t cell = b.hitPix[tid];
+  const V3 N = ld3(b.hitN[tid]);
+  float open;
+  if (fc.aoEnh) {
+    V3 bent;
+    aoQualityGather(fc, b, tid, cell % static_cast<uint32_t>(fc.cW), cell / static_cast<uint32_t>(fc.cW), N, open,
+                    bent);
+    b.cBent[cell] = make_float4(bent.x, bent.y, bent.z, 0.0f);
+  } else {
+    open = 1.0f - (float)b.hitAo[tid] / (float)fc.aoSamples;
+  }
+  b.cOpen[cell] = open;
+}
+
 struct SegIn {
   float p[3];
   float q[3];
@@ -1066,6 +1221,15 @@ void launchQuery(const SceneView& s, const void* rays, int64_t n, void* hits, ui
   else
     hipLaunchKernelGGL(k_query<kLayoutBvh2>, dim3(blocksFor(n, kTraceBlock)), dim3(kTraceBlock), lds, st, s,
                        static_cast<const RayIn*>(rays), n, static_cast<HitOut*>(hits), occ, any ? 1 : 0);
+}
+
+void launchCoarseGbuf(const FrameConst& fc, const SceneView& s, const FrameBufs& b, bool countVisits,
+                      hipStream_t st) {
+  UBG_DISPATCH(k_coarse_gbuf, blocksFor(static_cast<int64_t>(fc.cW) * fc.cH, kTraceBlock), countVisits, fc, s, b);
+}
+void launchCoarseReduce(const FrameConst& fc, const FrameBufs& b, hipStream_t st) {
+  const int64_t slots = static_cast<int64_t>(kQueueParts) * b.partCap;
+  hipLaunchKernelGGL(k_coarse_reduce, dim3(blocksFor(slots, kTraceBlock)), dim3(kTraceBlock), 0, st, fc, b);
 }
 
 void launchSegments(const SceneView& s, const void* segs, int64_t n, const int32_t* excl, int64_t nExcl, float eps,

@@ -132,6 +132,9 @@ struct FrameConst {
   int32_t aoMultiScale, aoBent, aoMultibounce, aoLowDisc;
   float aoDiffuse;
   float aoSky[3], aoGround[3], aoUpAxis[3];
+  // coarse-grid AO: grid cW x cH (coarse != 0); seedW = width of the AO seed
+  // lattice (base = px + py * seedW): the hi-res W, or cW for the grid gather
+  int32_t coarse, cW, cH, seedW;
   DevLight lights[kMaxLights];
 };
 
