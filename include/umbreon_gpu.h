@@ -17,7 +17,8 @@
  *                                     buildCylinderGeometry (src/umbreon/render/curve_build.cpp:41-267)
  *                                     and the BuiltScene side tables (scene_build.hpp:63-118)
  *   ubg_render                     -> EmbreeRenderer::render grid path
- *                                     (embree_renderer.cpp:1260-1427) + applyFog
+ *                                     (embree_renderer.cpp:1260-1427) and its adaptive-AA
+ *                                     driver (:1428-1674, render/adaptive_aa.hpp) + applyFog
  *                                     (src/umbreon/postprocess/fog.cpp:17-36) + boxDownsample /
  *                                     applyAssumedGamma (src/umbreon/postprocess/image_ops.cpp:9-46),
  *                                     i.e. renderFrame's trace/fog/resolve steps
@@ -39,7 +40,7 @@
 extern "C" {
 #endif
 
-#define UBG_ABI_VERSION 3
+#define UBG_ABI_VERSION 4
 
 /* Status codes. */
 enum {
@@ -203,6 +204,19 @@ typedef struct ubg_render_params {
    * exact inline gather where the guides reject; -1 = the supersample factor;
    * 0 or 1 = inline AO everywhere. */
   int32_t ao_res_div;
+  /* Adaptive antialiasing (render_options.hpp:20-39, render/adaptive_aa.hpp,
+   * embree_renderer.cpp:1428-1674). aa_mode 1 with supersample > 1 or
+   * aa_depth > 1: shade the centre subpixel of every output pixel, flag the
+   * output pixels whose 4-neighbour pairs differ (first-hit kind or group,
+   * colour or alpha contrast > aa_threshold, shading-normal fold, depth
+   * crack; then a 3x3 dilation), shade every subpixel of the flagged pixels
+   * (aa_depth > supersample: ceil(aa_depth / ss)^2 box-averaged samples per
+   * subpixel) and replicate the centre elsewhere, re-evaluated with ss^2
+   * times the AO / soft-shadow samples when those are stochastic.
+   * aa_mode 0 = the full supersample grid. */
+  int32_t aa_mode;
+  float aa_threshold;                /* reference default 0.1 */
+  int32_t aa_depth;
 } ubg_render_params;
 
 /* Ray accounting of one frame. A pure function of the scene and options:
@@ -241,6 +255,11 @@ typedef struct ubg_frame {
   int32_t layers;          /* transparency layers walked (wavefront passes) */
   double kernel_ms[4];     /* UBG_FLAG_KERNEL_TIMES: summed time per kernel kind */
   int32_t kernel_launches[4];
+  /* Adaptive AA refinement mask (FrameResult::aaMask): out_w * out_h floats,
+   * 1 = refined, 0 = replicated centre; may be NULL; left untouched when the
+   * frame is not adaptive. Same memory space as the other outputs. */
+  float* aa_mask;
+  int64_t aa_flagged;      /* adaptive AA: output pixels refined (0 otherwise) */
 } ubg_frame;
 
 /* Acceleration-structure layouts (ubg_scene_create_ex). */

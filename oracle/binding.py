@@ -60,6 +60,9 @@ def load():
     lib.orc_srgb_encode.restype = C.c_float
     lib.orc_srgb_decode.argtypes = [C.c_float]
     lib.orc_srgb_decode.restype = C.c_float
+    lib.orc_aa_mask.argtypes = [C.c_int32, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                C.c_float, C.c_float, C.c_int32, C.c_void_p]
+    lib.orc_aa_mask.restype = None
     _lib = lib
     return lib
 
@@ -92,12 +95,34 @@ def render(scene: Scene, opt: RenderOptions, hires_only: bool = False, precision
     fr = capi.Frame()
     fr.color, fr.depth, fr.view_z = color.ctypes.data, depth.ctypes.data, view_z.ctypes.data
     fr.hit_id, fr.ao_open = hit_id.ctypes.data, ao.ctypes.data
+    adaptive = opt.aa_mode == 1 and (ss > 1 or opt.aa_depth > 1)
+    aa_mask = np.zeros((opt.height, opt.width), np.float32) if adaptive else None
+    fr.aa_mask = aa_mask.ctypes.data if aa_mask is not None else None
     o = _opts(precision, threads, rows, brute)
     capi.check(lib.orc_render(C.byref(desc), C.byref(p), C.byref(fr), C.byref(o)))
     del keep1, keep2
     return FrameResult(width=ow, height=oh, color=color, depth=depth, view_z=view_z, hit_id=hit_id,
                        ao_open=ao, stats=fr.stats.as_dict(), render_seconds=fr.render_seconds,
-                       post_seconds=fr.post_seconds)
+                       post_seconds=fr.post_seconds, aa_mask=aa_mask, aa_flagged=int(fr.aa_flagged))
+
+
+def aa_mask(rgba, view_z, normal, geom, group, pixel_world_size: float, threshold: float = 0.1,
+            perspective: bool = False) -> np.ndarray:
+    """buildAaMask (adaptive_aa.hpp:128-194) over hand-built centre results of
+    shape (Hf, Wf): rgba (..., 4), view_z, normal (..., 3), geom (uint32,
+    0xFFFFFFFF = background), group (int32, -1 = background)."""
+    lib = load()
+    rgba = np.ascontiguousarray(rgba, np.float32)
+    hf, wf = rgba.shape[:2]
+    vz = np.ascontiguousarray(view_z, np.float32)
+    nr = np.ascontiguousarray(normal, np.float32)
+    ge = np.ascontiguousarray(geom, np.uint32)
+    gr = np.ascontiguousarray(group, np.int32)
+    assert vz.shape == (hf, wf) and nr.shape == (hf, wf, 3) and ge.shape == (hf, wf) and gr.shape == (hf, wf)
+    out = np.zeros((hf, wf), np.uint8)
+    lib.orc_aa_mask(wf, hf, rgba.ctypes.data, vz.ctypes.data, nr.ctypes.data, ge.ctypes.data, gr.ctypes.data,
+                    float(pixel_world_size), float(threshold), int(bool(perspective)), out.ctypes.data)
+    return out
 
 
 def intersect(scene: Scene, rays: np.ndarray, precision: int = 32, brute: bool = False) -> np.ndarray:

@@ -25,6 +25,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <functional>
 #include <limits>
 #include <map>
 #include <thread>
@@ -927,6 +928,10 @@ struct Ctx {
   V3<T> aoUp;  // resolved bent-normal gradient axis (embree_renderer.cpp:1320-1323)
   int wHi, hHi;
   const struct CoarseAo* grid = nullptr;  // coarse-grid AO (aoResDiv > 1), else null
+  // hit_shader.hpp:101-107 -- seed lattice width (the hi-res width, or W*k for
+  // the adaptive-AA fine samples) and the replicated-centre sample multipliers
+  int seedW = 0;
+  int aoMul = 1, shMul = 1;
 };
 
 // ao_coarse.hpp:54-65 -- the coarse AO grid: gathered openness (and bent
@@ -1260,6 +1265,8 @@ struct HitShade {
   V3<T> color;
   T opacity;
   T aoOpen;
+  V3<T> normal;   // face-forwarded shading normal (the adaptive-AA mask's)
+  uint32_t group; // section group (BuiltScene *Group tables, Mesh::groupForTri)
 };
 
 template <class T>
@@ -1285,14 +1292,16 @@ inline AoShadeT<T> aoShade(const Ctx<T>& c, V3<T> P, V3<T> Ng, V3<T> N, V3<T> C,
   V3<T> bent = N;
   // hit_shader.hpp:124-140: the coarse grid's bilateral lookup, else the
   // exact inline gather with the hi-res seeds
-  const bool coarse = c.grid != nullptr && sampleCoarseAo(*c.grid, c.wHi, c.hHi, px, py, N, hitT, openness, bent);
+  const int latW = c.seedW;
+  const int latH = static_cast<int>(static_cast<int64_t>(latW) * c.hHi / c.wHi);
+  const bool coarse = c.grid != nullptr && sampleCoarseAo(*c.grid, latW, latH, px, py, N, hitT, openness, bent);
   if (coarse) {
   } else if (enh) {
-    const AOQuality<T> q = computeAOQuality(c, P, Ng, N, eps, p->ao_samples, px, py, c.wHi, cnt);
+    const AOQuality<T> q = computeAOQuality(c, P, Ng, N, eps, p->ao_samples * c.aoMul, px, py, c.seedW, cnt);
     openness = q.openness;
     bent = q.bent;
   } else {
-    openness = computeAO(c, P, Ng, N, eps, p->ao_samples, T(p->ao_distance), px, py, c.wHi, cnt);
+    openness = computeAO(c, P, Ng, N, eps, p->ao_samples * c.aoMul, T(p->ao_distance), px, py, c.seedW, cnt);
   }
   r.openness = openness;
   if (enh && p->ao_bent_normal) {
@@ -1352,8 +1361,12 @@ inline HitShade<T> shadeHit(const Ctx<T>& c, const Hit<T>& h, V3<T> rd, V3<T> or
     const AoShadeT<T> ao = aoShade(c, P, Ng, N, C, secEps, h.t, px, py, cnt);
     hs.aoOpen = ao.openness;
     hs.color = shadeLocal(c, mat, C, N, V, ao.ambLight, ao.aoFactor, ao.diffuseAo, P, Ng, secEps,
-                          shadowsActive, c.p->shadow_samples, px, py, cnt);
+                          shadowsActive, c.p->shadow_samples * c.shMul, px, py, cnt);
     hs.opacity = cb[3];
+    hs.normal = N;
+    // Mesh::groupForTri by base triangle (an instance copy's primID would
+    // index past the per-triangle table)
+    hs.group = g.triGroup.empty() ? 0u : g.triGroup[baseTri];
     return hs;
   }
   const PrimSide* side = h.kind == UBG_KIND_SPHERE        ? &g.sphSide[h.id]
@@ -1372,8 +1385,10 @@ inline HitShade<T> shadeHit(const Ctx<T>& c, const Hit<T>& h, V3<T> rd, V3<T> or
   hs.aoOpen = ao.openness;
   const bool primShadows = !fromEdge && shadowsActive;
   hs.color = shadeLocal(c, pm, C, N, V, ao.ambLight, ao.aoFactor, ao.diffuseAo, P, Ng, secEps, primShadows,
-                        c.p->shadow_samples, px, py, cnt);
+                        c.p->shadow_samples * c.shMul, px, py, cnt);
   hs.opacity = T(side->color[3]);
+  hs.normal = N;
+  hs.group = side->group;
   if (h.kind != UBG_KIND_SPHERE && side->opacity1 >= 0.0f) {
     T u = h.u;
     if (u < T(0)) u = T(0);
@@ -1387,6 +1402,10 @@ template <class T>
 struct PixelOut {
   T r, g, b, a, depth, viewZ, aoOpen;
   uint32_t hitId;
+  // adaptive-AA mask inputs (transparency.hpp:402-413): first-hit shading
+  // normal (0 on a miss) and (kind << 16 | group), all ones on a miss
+  V3<T> normal{T(0), T(0), T(0)};
+  uint32_t gid = 0xFFFFFFFFu;
 };
 
 // transparency.hpp:311-522 (grid path; clip t-range per :118-129)
@@ -1413,6 +1432,8 @@ inline PixelOut<T> integratePixel(const Ctx<T>& c, V3<T> org, V3<T> rd, uint32_t
   T firstViewZ = T(0);
   T firstAo = T(1);
   uint32_t firstId = 0xFFFFFFFFu;
+  V3<T> firstN{T(0), T(0), T(0)};
+  uint32_t firstGid = 0xFFFFFFFFu;
   V3<T> base = c.bg;
   T baseCov = p->transparent_background ? T(0) : T(1);
   const T kOpaque = T(1.0f - 1e-4f);
@@ -1429,6 +1450,8 @@ inline PixelOut<T> integratePixel(const Ctx<T>& c, V3<T> org, V3<T> rd, uint32_t
       firstViewZ = h.t * dot(rd, c.camDir);
       firstAo = hs.aoOpen;
       firstId = (static_cast<uint32_t>(h.kind) << 30) | h.id;
+      firstN = hs.normal;
+      firstGid = (static_cast<uint32_t>(h.kind) << 16) | hs.group;
     }
     if (!p->transparency || hs.opacity >= kOpaque) {
       base = hs.color;
@@ -1460,6 +1483,8 @@ inline PixelOut<T> integratePixel(const Ctx<T>& c, V3<T> org, V3<T> rd, uint32_t
   o.viewZ = firstViewZ;
   o.aoOpen = firstAo;
   o.hitId = firstId;
+  o.normal = firstN;
+  o.gid = firstGid;
   return o;
 }
 
@@ -1493,6 +1518,110 @@ void boxDownsample(const float* src, int w, int h, int channels, int ss, float* 
       }
 }
 
+// ---------------------------------------------------------------------------
+// Adaptive antialiasing mask (render/adaptive_aa.hpp), restated.
+
+// What the mask reads of one output pixel's centre result (PixelResult's
+// r, g, b, a, viewZ, worldNormal, firstGeomID, firstGroup). The geometry id
+// is the hit's kind: the reference attaches one Embree geometry per kind
+// (scene_build.cpp:158-161), so equal kinds <=> equal geomIDs.
+struct AaCenter {
+  float r, g, b, a, viewZ;
+  float n[3];
+  uint32_t geom;   // 0xFFFFFFFF = background
+  int32_t group;   // -1 = background
+};
+
+template <class T>
+static AaCenter aaCenterOf(const PixelOut<T>& po) {
+  AaCenter c;
+  c.r = static_cast<float>(po.r);
+  c.g = static_cast<float>(po.g);
+  c.b = static_cast<float>(po.b);
+  c.a = static_cast<float>(po.a);
+  c.viewZ = static_cast<float>(po.viewZ);
+  c.n[0] = static_cast<float>(po.normal.x);
+  c.n[1] = static_cast<float>(po.normal.y);
+  c.n[2] = static_cast<float>(po.normal.z);
+  c.geom = po.gid == 0xFFFFFFFFu ? 0xFFFFFFFFu : (po.gid >> 16);
+  c.group = po.gid == 0xFFFFFFFFu ? -1 : static_cast<int32_t>(po.gid & 0xFFFFu);
+  return c;
+}
+
+// adaptive_aa.hpp:20-29 defaults (normal delta 0.3, depth k 3, 12 pixel
+// worlds) and :66-115 (aaPairNeedsRefine)
+static bool aaPairNeedsRefine(const AaCenter& a, const AaCenter& b, float za0, float zb1, float pws, float threshold,
+                              bool perspective) {
+  if (a.geom != b.geom || a.group != b.group) return true;
+  if (std::fabs(a.r - b.r) > threshold || std::fabs(a.g - b.g) > threshold || std::fabs(a.b - b.b) > threshold ||
+      std::fabs(a.a - b.a) > threshold)
+    return true;
+  const bool aHit = a.geom != 0xFFFFFFFFu, bHit = b.geom != 0xFFFFFFFFu;
+  if (!aHit || !bHit) return false;
+  const float la = a.n[0] * a.n[0] + a.n[1] * a.n[1] + a.n[2] * a.n[2];
+  const float lb = b.n[0] * b.n[0] + b.n[1] * b.n[1] + b.n[2] * b.n[2];
+  if (la > 0.0f && lb > 0.0f) {
+    const float d = a.n[0] * b.n[0] + a.n[1] * b.n[1] + a.n[2] * b.n[2];
+    if (1.0f - d > 0.3f) return true;
+  }
+  const float step = std::fabs(b.viewZ - a.viewZ);
+  const float sa = std::fabs(a.viewZ - za0);
+  const float sb = std::fabs(zb1 - b.viewZ);
+  const float slope = sa < sb ? sa : sb;
+  float absTerm = 12.0f * pws;
+  if (perspective) absTerm *= std::fabs(a.viewZ) < std::fabs(b.viewZ) ? std::fabs(a.viewZ) : std::fabs(b.viewZ);
+  const float thresh = 3.0f * slope > absTerm ? 3.0f * slope : absTerm;
+  return step > thresh;
+}
+
+// adaptive_aa.hpp:128-194 (pairs along both axes with clamped outer slopes)
+// and :36-57 (3x3 dilation); the edges-mode hi-res test is not part of it
+static std::vector<uint8_t> buildAaMask(int Wf, int Hf, const AaCenter* c, float pws, float threshold,
+                                        bool perspective) {
+  std::vector<uint8_t> raw(static_cast<std::size_t>(Wf) * Hf, 0);
+  auto at = [&](int x, int y) -> const AaCenter& { return c[static_cast<std::size_t>(y) * Wf + x]; };
+  for (int y = 0; y < Hf; ++y)
+    for (int x = 0; x + 1 < Wf; ++x) {
+      const AaCenter& a = at(x, y);
+      const AaCenter& b = at(x + 1, y);
+      const float za0 = x > 0 ? at(x - 1, y).viewZ : 2.0f * a.viewZ - b.viewZ;
+      const float zb1 = x + 2 < Wf ? at(x + 2, y).viewZ : 2.0f * b.viewZ - a.viewZ;
+      if (aaPairNeedsRefine(a, b, za0, zb1, pws, threshold, perspective)) {
+        raw[static_cast<std::size_t>(y) * Wf + x] = 1;
+        raw[static_cast<std::size_t>(y) * Wf + x + 1] = 1;
+      }
+    }
+  for (int y = 0; y + 1 < Hf; ++y)
+    for (int x = 0; x < Wf; ++x) {
+      const AaCenter& a = at(x, y);
+      const AaCenter& b = at(x, y + 1);
+      const float za0 = y > 0 ? at(x, y - 1).viewZ : 2.0f * a.viewZ - b.viewZ;
+      const float zb1 = y + 2 < Hf ? at(x, y + 2).viewZ : 2.0f * b.viewZ - a.viewZ;
+      if (aaPairNeedsRefine(a, b, za0, zb1, pws, threshold, perspective)) {
+        raw[static_cast<std::size_t>(y) * Wf + x] = 1;
+        raw[static_cast<std::size_t>(y + 1) * Wf + x] = 1;
+      }
+    }
+  std::vector<uint8_t> out(raw.size(), 0);
+  for (int y = 0; y < Hf; ++y)
+    for (int x = 0; x < Wf; ++x) {
+      uint8_t v = 0;
+      for (int dy = -1; dy <= 1 && !v; ++dy) {
+        const int yy = y + dy;
+        if (yy < 0 || yy >= Hf) continue;
+        for (int dx = -1; dx <= 1; ++dx) {
+          const int xx = x + dx;
+          if (xx >= 0 && xx < Wf && raw[static_cast<std::size_t>(yy) * Wf + xx]) {
+            v = 1;
+            break;
+          }
+        }
+      }
+      out[static_cast<std::size_t>(y) * Wf + x] = v;
+    }
+  return out;
+}
+
 template <class T>
 static int renderT(const ubg_scene_desc* desc, const ubg_render_params* p, ubg_frame* fr,
                    const orc_options* opt) {
@@ -1505,6 +1634,7 @@ static int renderT(const ubg_scene_desc* desc, const ubg_render_params* p, ubg_f
   c.p = p;
   c.wHi = W;
   c.hHi = H;
+  c.seedW = W;
   // embree_renderer.cpp:375-386 (camera basis)
   const ubg_camera& cam = p->camera;
   const T aspect = T(static_cast<float>(W)) / T(static_cast<float>(H));
@@ -1627,52 +1757,136 @@ static int renderT(const ubg_scene_desc* desc, const ubg_render_params* p, ubg_f
   std::vector<float> hiColor(npx * 4, 0.0f), hiDepth(npx, 0.0f), hiViewZ(npx, 0.0f),
       hiAo(npx, 1.0f);
   std::vector<uint32_t> hiId(npx, 0xFFFFFFFFu);
-  const int rowB = std::max(0, opt->row_begin);
-  const int rowE = opt->row_end > 0 ? std::min(H, opt->row_end) : H;
-  const int rowStep = opt->row_step > 0 ? opt->row_step : 1;
-  std::vector<int> rows;
-  for (int r = rowB; r < rowE; r += rowStep) rows.push_back(r);
   int nth = opt->threads > 0 ? opt->threads : static_cast<int>(std::thread::hardware_concurrency());
   if (nth < 1) nth = 1;
-  std::atomic<std::size_t> nextRow{0};
   std::vector<Counters> cnts(nth);
-  const auto t0 = std::chrono::steady_clock::now();
-  auto worker = [&](int tid) {
-    Counters& cnt = cnts[tid];
-    for (;;) {
-      const std::size_t ri = nextRow.fetch_add(1);
-      if (ri >= rows.size()) break;
-      const int py = rows[ri];
-      // embree_renderer.cpp:1400-1414 (primary rays, top-left origin)
-      const T v = T(1) - T(2) * (T(static_cast<float>(py)) + T(0.5f)) / T(static_cast<float>(H));
-      for (int px = 0; px < W; ++px) {
-        const T u = T(2) * (T(static_cast<float>(px)) + T(0.5f)) / T(static_cast<float>(W)) - T(1);
-        V3<T> org, rd;
-        if (cam.orthographic) {
-          org = pos + right * (u * halfW) + trueUp * (v * halfH);
-          rd = dir;
-        } else {
-          org = pos;
-          rd = normalize(dir + right * (u * persHalfW) + trueUp * (v * persHalfH));
-        }
-        const PixelOut<T> po = integratePixel(c, org, rd, static_cast<uint32_t>(px),
-                                              static_cast<uint32_t>(py), cnt);
-        const std::size_t pix = static_cast<std::size_t>(py) * W + px;
-        hiColor[4 * pix + 0] = static_cast<float>(po.r);
-        hiColor[4 * pix + 1] = static_cast<float>(po.g);
-        hiColor[4 * pix + 2] = static_cast<float>(po.b);
-        hiColor[4 * pix + 3] = static_cast<float>(po.a);
-        hiDepth[pix] = static_cast<float>(po.depth);
-        hiViewZ[pix] = static_cast<float>(po.viewZ);
-        hiAo[pix] = static_cast<float>(po.aoOpen);
-        hiId[pix] = po.hitId;
+  // rows [0, nRows) of some grid over the worker pool, dynamically scheduled
+  auto parallelRows = [&](const std::vector<int>& rowList, const std::function<void(int, Counters&)>& fn) {
+    std::atomic<std::size_t> nextRow{0};
+    auto worker = [&](int tid) {
+      for (;;) {
+        const std::size_t ri = nextRow.fetch_add(1);
+        if (ri >= rowList.size()) break;
+        fn(rowList[ri], cnts[tid]);
       }
-    }
+    };
+    std::vector<std::thread> pool;
+    for (int i = 1; i < nth; ++i) pool.emplace_back(worker, i);
+    worker(0);
+    for (auto& t : pool) t.join();
   };
-  std::vector<std::thread> pool;
-  for (int i = 1; i < nth; ++i) pool.emplace_back(worker, i);
-  worker(0);
-  for (auto& t : pool) t.join();
+  // embree_renderer.cpp:1400-1414 / :1444-1455 -- the primary ray through
+  // (lx, ly) of a latW x latH lattice of the image plane, integrated
+  auto shadeAt = [&](const Ctx<T>& cx, int lx, int ly, int latW, int latH, Counters& cnt) {
+    const T v = T(1) - T(2) * (T(static_cast<float>(ly)) + T(0.5f)) / T(static_cast<float>(latH));
+    const T u = T(2) * (T(static_cast<float>(lx)) + T(0.5f)) / T(static_cast<float>(latW)) - T(1);
+    V3<T> org, rd;
+    if (cam.orthographic) {
+      org = pos + right * (u * halfW) + trueUp * (v * halfH);
+      rd = dir;
+    } else {
+      org = pos;
+      rd = normalize(dir + right * (u * persHalfW) + trueUp * (v * persHalfH));
+    }
+    return integratePixel(cx, org, rd, static_cast<uint32_t>(lx), static_cast<uint32_t>(ly), cnt);
+  };
+  auto store = [&](std::size_t pix, const PixelOut<T>& po) {
+    hiColor[4 * pix + 0] = static_cast<float>(po.r);
+    hiColor[4 * pix + 1] = static_cast<float>(po.g);
+    hiColor[4 * pix + 2] = static_cast<float>(po.b);
+    hiColor[4 * pix + 3] = static_cast<float>(po.a);
+    hiDepth[pix] = static_cast<float>(po.depth);
+    hiViewZ[pix] = static_cast<float>(po.viewZ);
+    hiAo[pix] = static_cast<float>(po.aoOpen);
+    hiId[pix] = po.hitId;
+  };
+  const bool adaptive = p->aa_mode == 1 && (ss > 1 || p->aa_depth > 1);
+  const auto t0 = std::chrono::steady_clock::now();
+  int64_t aaFlagged = 0;
+  if (!adaptive) {
+    const int rowB = std::max(0, opt->row_begin);
+    const int rowE = opt->row_end > 0 ? std::min(H, opt->row_end) : H;
+    const int rowStep = opt->row_step > 0 ? opt->row_step : 1;
+    std::vector<int> rows;
+    for (int r = rowB; r < rowE; r += rowStep) rows.push_back(r);
+    parallelRows(rows, [&](int py, Counters& cnt) {
+      for (int px = 0; px < W; ++px) store(static_cast<std::size_t>(py) * W + px, shadeAt(c, px, py, W, H, cnt));
+    });
+  } else {
+    // embree_renderer.cpp:1428-1674 -- adaptive AA, three deterministic phases
+    const int Wf = p->width, Hf = p->height, cSub = ss / 2;
+    const int kFine = std::max(1, (p->aa_depth + ss - 1) / ss);
+    std::vector<int> outRows(Hf);
+    for (int y = 0; y < Hf; ++y) outRows[y] = y;
+    // phase 1: the centre subpixel of every output pixel, plain sample counts
+    std::vector<PixelOut<T>> centers(static_cast<std::size_t>(Wf) * Hf);
+    parallelRows(outRows, [&](int Y, Counters& cnt) {
+      for (int X = 0; X < Wf; ++X)
+        centers[static_cast<std::size_t>(Y) * Wf + X] = shadeAt(c, X * ss + cSub, Y * ss + cSub, W, H, cnt);
+    });
+    // phase 2: the refinement mask (adaptive_aa.hpp), a function of the centres
+    std::vector<AaCenter> ac(centers.size());
+    for (std::size_t i = 0; i < centers.size(); ++i) ac[i] = aaCenterOf(centers[i]);
+    const float pws = cam.orthographic ? (cam.height / static_cast<float>(Hf))
+                                       : (2.0f * static_cast<float>(persHalfH) / static_cast<float>(Hf));
+    const std::vector<uint8_t> mask = buildAaMask(Wf, Hf, ac.data(), pws, p->aa_threshold, !cam.orthographic);
+    for (uint8_t m : mask) aaFlagged += m ? 1 : 0;
+    if (fr->aa_mask)
+      for (std::size_t i = 0; i < mask.size(); ++i) fr->aa_mask[i] = mask[i] ? 1.0f : 0.0f;
+    // the fine lattice (aaDepth > ss) and the replicated-centre boost
+    Ctx<T> cFine = c;
+    cFine.seedW = W * kFine;
+    const bool coarseOn = c.grid != nullptr;
+    const bool softShadows = p->shadow_samples > 1 && p->light_radius_deg > 0.0f;
+    const bool needBoost = ss > 1 && ((p->ao_samples > 0 && !coarseOn) || softShadows);
+    Ctx<T> cBoost = c;
+    cBoost.aoMul = coarseOn ? 1 : ss * ss;
+    cBoost.shMul = softShadows ? ss * ss : 1;
+    auto shadeCell = [&](int px, int py, Counters& cnt) {
+      if (kFine == 1) return shadeAt(c, px, py, W, H, cnt);
+      PixelOut<T> acc{};
+      for (int bb = 0; bb < kFine; ++bb)
+        for (int a = 0; a < kFine; ++a) {
+          const PixelOut<T> pr = shadeAt(cFine, px * kFine + a, py * kFine + bb, W * kFine, H * kFine, cnt);
+          if (a == 0 && bb == 0) {
+            acc = pr;
+          } else {
+            acc.r += pr.r;
+            acc.g += pr.g;
+            acc.b += pr.b;
+            acc.a += pr.a;
+            acc.depth += pr.depth;
+            acc.viewZ += pr.viewZ;
+          }
+        }
+      const T inv = T(1.0f / static_cast<float>(kFine * kFine));
+      acc.r *= inv;
+      acc.g *= inv;
+      acc.b *= inv;
+      acc.a *= inv;
+      acc.depth *= inv;
+      acc.viewZ *= inv;
+      return acc;
+    };
+    // phase 3: flagged pixels shade every subpixel, the rest replicate
+    parallelRows(outRows, [&](int Y, Counters& cnt) {
+      for (int X = 0; X < Wf; ++X) {
+        const std::size_t XY = static_cast<std::size_t>(Y) * Wf + X;
+        if (!mask[XY]) {
+          const PixelOut<T> rep = needBoost ? shadeAt(cBoost, X * ss + cSub, Y * ss + cSub, W, H, cnt) : centers[XY];
+          for (int j = 0; j < ss; ++j)
+            for (int i = 0; i < ss; ++i) store(static_cast<std::size_t>(Y * ss + j) * W + X * ss + i, rep);
+          continue;
+        }
+        for (int j = 0; j < ss; ++j)
+          for (int i = 0; i < ss; ++i) {
+            const int px = X * ss + i, py = Y * ss + j;
+            const PixelOut<T> pr = (kFine == 1 && i == cSub && j == cSub) ? centers[XY] : shadeCell(px, py, cnt);
+            store(static_cast<std::size_t>(py) * W + px, pr);
+          }
+      }
+    });
+  }
   const auto t1 = std::chrono::steady_clock::now();
   // pipeline.cpp:126-129 -> fog.cpp:17-36 (hi-res, before the downsample)
   if (p->fog.enabled) {
@@ -1702,6 +1916,7 @@ static int renderT(const ubg_scene_desc* desc, const ubg_render_params* p, ubg_f
   }
   fr->hi_w = W;
   fr->hi_h = H;
+  fr->aa_flagged = aaFlagged;
   fr->stats.primary_rays = tot.primary;
   fr->stats.shaded_hits = tot.shaded;
   fr->stats.ao_rays_traced = tot.aoTraced;
@@ -1908,6 +2123,25 @@ int64_t orc_chain_curves(const ubg_scene_desc* desc, int64_t cap, int32_t* src, 
 }
 
 void orc_tea2(uint32_t* v0, uint32_t* v1) { orc::tea2(*v0, *v1); }
+
+void orc_aa_mask(int32_t wf, int32_t hf, const float* rgba, const float* view_z, const float* normal,
+                 const uint32_t* geom, const int32_t* group, float pixel_world_size, float threshold,
+                 int32_t perspective, uint8_t* mask) {
+  std::vector<orc::AaCenter> c(static_cast<std::size_t>(wf) * hf);
+  for (std::size_t i = 0; i < c.size(); ++i) {
+    c[i].r = rgba[4 * i];
+    c[i].g = rgba[4 * i + 1];
+    c[i].b = rgba[4 * i + 2];
+    c[i].a = rgba[4 * i + 3];
+    c[i].viewZ = view_z[i];
+    for (int k = 0; k < 3; ++k) c[i].n[k] = normal[3 * i + k];
+    c[i].geom = geom[i];
+    c[i].group = group[i];
+  }
+  const std::vector<uint8_t> m =
+      orc::buildAaMask(wf, hf, c.data(), pixel_world_size, threshold, perspective != 0);
+  std::memcpy(mask, m.data(), m.size());
+}
 
 float orc_fog_factor(const ubg_fog* fog, float z) { return orc::fogFactor(*fog, z); }
 

@@ -52,6 +52,14 @@ int orc_occluded_segments(const ubg_scene_desc* desc, int64_t n, const ubg_segme
 int64_t orc_chain_curves(const ubg_scene_desc* desc, int64_t cap,
                          int32_t* src_cylinder, uint8_t* flags);
 
+/* Adaptive-AA refinement mask (render/adaptive_aa.hpp:66-194, edges mode
+ * excluded) over hand-built centre results of a wf x hf output grid: rgba
+ * (4 per pixel), view z, shading normal (3 per pixel), first-hit geometry id
+ * (0xFFFFFFFF = background) and group (-1 = background). mask = 0/1 per pixel. */
+void orc_aa_mask(int32_t wf, int32_t hf, const float* rgba, const float* view_z, const float* normal,
+                 const uint32_t* geom, const int32_t* group, float pixel_world_size, float threshold,
+                 int32_t perspective, uint8_t* mask);
+
 /* Reference TEA-8 (secondary_rays.hpp:51-58), exposed for known-answer tests. */
 void orc_tea2(uint32_t* v0, uint32_t* v1);
 

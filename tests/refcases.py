@@ -781,3 +781,111 @@ def fog_frame():
     sc_bg.camera = ortho_cam()
     sc_bg.fog = Fog(enabled=True, color=(1, 1, 1), start=5.0, end=15.0)
     return Case("fog_frame", {"f": (sc, opts()), "bg": (sc_bg, opts())}, chk)
+
+
+# ---------------------------------------------------------------- adaptive antialiasing
+def half_quad_scene(x0=-2.0, x1=0.0) -> Scene:
+    """makeQuadX / makeSceneX (test_adaptive_aa.cpp:87-126): a quad over
+    [x0, x1] x [-2, 2] at z = 0 facing +Z, ortho camera of height 4, head-on
+    key light 0.5, background 0.1."""
+    sc = Scene()
+    p = [(x0, -2, 0), (x1, -2, 0), (x1, 2, 0), (x0, -2, 0), (x1, 2, 0), (x0, 2, 0)]
+    sc.set_mesh(p, [(0, 0, 1)] * 6, [(0.5, 0.6, 0.7, 1.0)] * 6, material=Material(ambient=0.2, diffuse=0.8))
+    sc.camera = ortho_cam()
+    sc.lights = [key_light()]
+    sc.background = (0.1, 0.1, 0.1)
+    return sc
+
+
+@case
+def aa_uniform():
+    # test_adaptive_aa.cpp:361-379: nothing flags on a uniform view; the
+    # replicated blocks equal the grid box average within interpolation rounding
+    sc = half_quad_scene(-2.0, 2.0)
+
+    def chk(r):
+        g, a = r["grid"], r["adap"]
+        assert a.color.shape == (6, 6, 4)
+        assert np.abs(g.color - a.color).max() <= 1e-6
+        assert a.aa_flagged == 0
+    return Case("aa_uniform", {"grid": (sc, opts(6, 6, supersample=2)),
+                               "adap": (sc, opts(6, 6, supersample=2, aa_mode=1))}, chk, tol=1e-6)
+
+
+@case
+def aa_silhouette():
+    # test_adaptive_aa.cpp:381-437: flagged pixels re-shade every subpixel
+    # exactly like the grid (bitwise after the resolve), the rest stays within
+    # interpolation rounding; the mask (Wf x Hf) covers the silhouette column 4
+    # and leaves columns 0 and 8 quiet
+    sc = half_quad_scene()
+
+    def chk(r):
+        g, a = r["grid"], r["adap"]
+        assert np.abs(g.color - a.color).max() <= 1e-6
+        m = a.aa_mask
+        assert m is not None and m.shape == (9, 9)
+        flagged = m > 0
+        assert flagged.sum() > 0 and a.aa_flagged == int(flagged.sum())
+        assert np.array_equal(g.color[flagged], a.color[flagged])
+        assert np.all(m[:, 4] > 0) and np.all(m[:, 0] == 0) and np.all(m[:, 8] == 0)
+    return Case("aa_silhouette", {"grid": (sc, opts(9, 9, supersample=3)),
+                                  "adap": (sc, opts(9, 9, supersample=3, aa_mode=1, aa_debug=True))}, chk, tol=1e-6)
+
+
+@case
+def aa_ao_compensation():
+    # test_adaptive_aa.cpp:432-468: the unflagged floor under a slab takes
+    # ONE centre evaluation with ss^2 x the AO samples; it estimates the same
+    # openness as the grid's box average (within 0.05)
+    sc = Scene()
+    p, n, c = floor_slab_mesh(with_slab=True)
+    p[6:] = [(0.2, -2, 0.4), (4, -2, 0.4), (4, 2, 0.4), (0.2, -2, 0.4), (4, 2, 0.4), (0.2, 2, 0.4)]
+    c = [(0.5, 0.6, 0.7, 1.0)] * 6 + [(0, 0, 0, 1)] * 6
+    sc.set_mesh(p, n, c, material=Material(ambient=1.0, diffuse=0.0))
+    sc.camera = ortho_cam()
+    sc.background = (0.1, 0.1, 0.1)
+    grid = opts(8, 8, supersample=2, ao_samples=16, ao_distance=100.0)
+    adap = opts(8, 8, supersample=2, ao_samples=16, ao_distance=100.0, aa_mode=1)
+
+    def chk(r):
+        approx(r["grid"].color[4, 6, 0], r["adap"].color[4, 6, 0], 0.05)
+    return Case("aa_ao_compensation", {"grid": (sc, grid), "adap": (sc, adap)}, chk)
+
+
+@case
+def aa_depth_equals_ss():
+    # test_adaptive_aa.cpp:559-575: aaDepth == ss selects k == 1, the plain
+    # subpixel path -- bitwise-equal to aaDepth 0
+    sc = half_quad_scene()
+
+    def chk(r):
+        assert np.array_equal(r["d0"].color, r["d3"].color)
+        assert np.array_equal(r["d0"].depth, r["d3"].depth)
+    return Case("aa_depth_equals_ss", {"d0": (sc, opts(9, 9, supersample=3, aa_mode=1)),
+                                       "d3": (sc, opts(9, 9, supersample=3, aa_mode=1, aa_depth=3))}, chk,
+                tol=1e-6)
+
+
+@case
+def aa_depth_at_ss1():
+    # test_adaptive_aa.cpp:577-608: ss 1 + aaDepth 2: unflagged pixels are the
+    # plain one-sample render bitwise; flagged silhouette pixels are 2x2 box
+    # averages that differ from the grid and lie between the two side colours
+    sc = half_quad_scene()
+
+    def chk(r):
+        g, a = r["grid"], r["adap"]
+        m = a.aa_mask
+        assert m is not None and m.shape == (9, 9)
+        flagged = m > 0
+        same = np.all(g.color == a.color, axis=2)
+        assert np.all(same[~flagged])
+        diff = flagged & ~same
+        assert diff.any()
+        lo = np.minimum(g.color[4, 0, :3], g.color[4, 8, :3])
+        hi = np.maximum(g.color[4, 0, :3], g.color[4, 8, :3])
+        v = a.color[diff][:, :3]
+        assert np.all(v >= lo - 1e-6) and np.all(v <= hi + 1e-6)
+    return Case("aa_depth_at_ss1", {"grid": (sc, opts(9, 9)),
+                                    "adap": (sc, opts(9, 9, aa_mode=1, aa_depth=2, aa_debug=True))}, chk, tol=1e-6)

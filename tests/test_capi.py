@@ -97,4 +97,6 @@ def test_unsupported_options_raise():
     with pytest.raises(capi.UmbreonError):
         params(Scene(), RenderOptions(gi=True))
     with pytest.raises(capi.UmbreonError):
-        params(Scene(), RenderOptions(aa_mode=1))
+        params(Scene(), RenderOptions(denoiser=1))
+    p, _ = params(Scene(), RenderOptions(aa_mode=1, aa_threshold=0.25, aa_depth=4))
+    assert (p.aa_mode, p.aa_depth) == (1, 4) and abs(p.aa_threshold - 0.25) < 1e-7

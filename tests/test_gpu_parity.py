@@ -309,3 +309,74 @@ def test_coarse_ao_ballstick_reduced(device, enhanced):
     g = _gpu(device, sc, opt, hires_only=True, want_ids=True, want_ao=True)
     c = ob.render(sc, opt, hires_only=True)
     _compare("coarse-AO", g, c, ao=True)
+
+
+def _compare_aa(tag, g, c, ss):
+    """Adaptive AA: the refinement masks must agree (a colour contrast that
+    sits within an ulp of the threshold may flip one pixel pair, so up to
+    0.1% of output pixels are allowed to differ); the hi-res frames are then
+    compared like a grid render outside the blocks whose decision differs."""
+    mg, mc = g.aa_mask, c.aa_mask
+    assert mg is not None and mc is not None and mg.shape == mc.shape
+    same = mg == mc
+    print(f"{tag}: flagged GPU {int((mg > 0).sum())} / oracle {int((mc > 0).sum())} of {mg.size}, "
+          f"mask agree {float(same.mean()):.6f}", flush=True)
+    assert same.mean() >= 0.999
+    assert g.aa_flagged == int((mg > 0).sum())
+    keep = np.repeat(np.repeat(same, ss, axis=0), ss, axis=1)
+    ids = (g.hit_id == c.hit_id)[keep]
+    d = np.abs(g.color - c.color).max(axis=2)[keep]
+    msg = f"{tag}: ids agree {float(ids.mean()):.6f}, max|d| {d.max():.3g}, pixels >1e-3: {int((d > 1e-3).sum())}"
+    print(msg, flush=True)
+    assert ids.mean() >= 0.9999, msg
+    assert (d > 1e-3).mean() <= 5e-4, msg
+    if same.all():
+        for k in ("primary_rays", "shaded_hits", "shadow_rays", "capped_rays"):
+            assert g.stats[k] == c.stats[k], (k, g.stats[k], c.stats[k])
+        assert (g.stats["ao_rays_traced"] + g.stats["ao_rays_sampled"]
+                == c.stats["ao_rays_traced"] + c.stats["ao_rays_sampled"])
+
+
+@pytest.mark.parametrize("variant", ["plain", "boost_soft_enhanced", "coarse_ao"])
+def test_adaptive_aa_ballstick_reduced(device, variant):
+    """Adaptive AA over the ball-and-stick workload against the oracle: the
+    centre pass, the mask, the refined subpixels and the boosted centres
+    (AO x ss^2, soft shadows x ss^2, enhanced estimator; or the coarse grid,
+    which turns the AO boost off)."""
+    sc, opt = ball_and_stick_scene(n_atoms=4000, width=240, height=135)
+    opt.aa_mode = 1
+    opt.aa_debug = True
+    if variant == "boost_soft_enhanced":
+        opt.shadow_samples = 4
+        opt.light_radius = 2.0
+        opt.ao_multi_scale = True
+        opt.ao_falloff_power = 1.0
+    elif variant == "coarse_ao":
+        opt.ao_res_div = -1
+    print(f"adaptive {variant}", flush=True)
+    g = _gpu(device, sc, opt, hires_only=True, want_ids=True, want_ao=True)
+    c = ob.render(sc, opt, hires_only=True)
+    _compare_aa(f"adaptive-{variant}", g, c, opt.supersample)
+    # the resolved frame: box downsample of the composed hi-res buffer
+    gf = _gpu(device, sc, opt)
+    assert np.array_equal(gf.aa_mask, g.aa_mask)
+    assert np.isfinite(gf.color).all()
+
+
+def test_adaptive_aa_fine_lattice_transparency_chunked(device):
+    """aaDepth beyond the supersample factor (k x k fine samples per refined
+    subpixel, seeds on the W*k lattice) on the transparent surface scene:
+    continuation layers run in list mode, and the refined pass is larger than
+    the hi-res grid, so it runs in several chunks."""
+    sc, opt = molecular_surface_scene(n_atoms=1500, width=160, height=90)
+    opt.supersample = 1
+    opt.aa_mode = 1
+    opt.aa_depth = 3
+    opt.aa_debug = True
+    g = _gpu(device, sc, opt, hires_only=True, want_ids=True, want_ao=True)
+    c = ob.render(sc, opt, hires_only=True)
+    nflag = int((c.aa_mask > 0).sum())
+    print(f"fine lattice: {nflag} flagged of {c.aa_mask.size}, refined samples {9 * nflag} "
+          f"vs chunk {opt.width * opt.height}, layers {g.layers}", flush=True)
+    assert 9 * nflag > opt.width * opt.height  # the refined pass is chunked
+    _compare_aa("adaptive-fine", g, c, 1)

@@ -23,7 +23,7 @@ UBG_ERR_UNSUPPORTED = 5
 UBG_ERR_INTERNAL = 6
 
 # UBG_ABI_VERSION of include/umbreon_gpu.h this mirror is written against
-ABI_VERSION = 3
+ABI_VERSION = 4
 
 BVH2 = 0
 BVH8 = 1
@@ -106,6 +106,7 @@ class RenderParams(C.Structure):
         ("ao_sky_color", C.c_float * 3), ("ao_ground_color", C.c_float * 3), ("ao_use_camera_up", C.c_int32),
         ("ao_up", C.c_float * 3), ("ao_multibounce", C.c_int32), ("ao_low_discrepancy", C.c_int32),
         ("ao_diffuse_factor", C.c_float), ("ao_res_div", C.c_int32),
+        ("aa_mode", C.c_int32), ("aa_threshold", C.c_float), ("aa_depth", C.c_int32),
     ]
 
 
@@ -130,7 +131,7 @@ class Frame(C.Structure):
         ("out_w", C.c_int32), ("out_h", C.c_int32), ("hi_w", C.c_int32), ("hi_h", C.c_int32),
         ("stats", RayStats), ("render_seconds", C.c_double), ("post_seconds", C.c_double),
         ("cancelled", C.c_int32), ("layers", C.c_int32), ("kernel_ms", C.c_double * 4),
-        ("kernel_launches", C.c_int32 * 4),
+        ("kernel_launches", C.c_int32 * 4), ("aa_mask", C.c_void_p), ("aa_flagged", C.c_int64),
     ]
 
 

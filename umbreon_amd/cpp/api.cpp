@@ -200,7 +200,6 @@ void checkSupported(const Scene& s, const RenderOptions& o) {
     throw std::runtime_error(std::string("umbreon (MI355X): ") + what + " is not supported by the GPU backend");
   };
   if (o.gi) no("global illumination (gi)");
-  if (o.aaMode != 0) no("adaptive antialiasing (aaMode)");
   if (o.aoWriteAov) no("AO AOVs (aoWriteAov)");
   if (o.denoiser != 0) no("the final-colour denoiser");
   if (o.envLights > 0) no("environment dome lights (envLights)");
@@ -297,6 +296,9 @@ FrameResult renderPass(const Scene& scene, const RenderOptions& opt, RenderProgr
   p.ao_low_discrepancy = opt.aoLowDiscrepancy ? 1 : 0;
   p.ao_diffuse_factor = opt.aoDiffuseFactor;
   p.ao_res_div = opt.aoResDiv;
+  p.aa_mode = opt.aaMode;
+  p.aa_threshold = opt.aaThreshold;
+  p.aa_depth = opt.aaDepth;
 
   const std::size_t hiN = static_cast<std::size_t>(opt.width) * ss * opt.height * ss;
   f.width = opt.width;
@@ -308,6 +310,11 @@ FrameResult renderPass(const Scene& scene, const RenderOptions& opt, RenderProgr
   fr.color = f.color.data();
   fr.depth = f.depth.data();
   fr.view_z = f.viewZ.empty() ? nullptr : f.viewZ.data();
+  // FrameResult::aaMask: Wf x Hf, only for an adaptive render with aaDebug
+  // (embree_renderer.cpp:1379-1385, 1540)
+  if (opt.aaMode == 1 && (ss > 1 || opt.aaDepth > 1) && opt.aaDebug)
+    f.aaMask.assign(static_cast<std::size_t>(opt.width) * opt.height, 0.0f);
+  fr.aa_mask = f.aaMask.empty() ? nullptr : f.aaMask.data();
   fr.memory = 0;
   if (progress) progress->beginPhase(RenderPhase::Primary);
   st = ubg_render(ctx, gs, &p, &fr);

@@ -826,6 +826,8 @@ int buildHostScene(const ubg_scene_desc* d, int layout, HostScene& out, std::str
       }
     out.nTris = baseT * copies;
     out.meshBaseTris = baseT;
+    out.triGroup.resize(baseT);
+    for (int64_t t = 0; t < baseT; ++t) out.triGroup[t] = d->tri_group ? d->tri_group[t] : 0;
   }
 
   auto checkMat = [&](int32_t m) {
@@ -853,6 +855,7 @@ int buildHostScene(const ubg_scene_desc* d, int layout, HostScene& out, std::str
                                   d->sphere_color[4 * s + 2], d->sphere_color[4 * s + 3]});
         const bool fe = d->sphere_from_edge && d->sphere_from_edge[s];
         out.sphInfo.push_back(static_cast<uint32_t>(m) | (fe ? kInfoFromEdge : 0u));
+        out.sphGroup.push_back(d->sphere_group ? d->sphere_group[s] : 0);
         DBox bx;
         bx.ext(cr.x, cr.y, cr.z, std::fabs(static_cast<double>(cr.w)));
         boxes.push_back(bx.toFloat());
@@ -917,6 +920,7 @@ int buildHostScene(const ubg_scene_desc* d, int layout, HostScene& out, std::str
         out.rndInfo.push_back(info);
         out.rndOp1.push_back(op1);
         out.rndSource.push_back(segSrc[j]);
+        out.rndGroup.push_back(d->cyl_group ? d->cyl_group[segSrc[j]] : 0);
         DBox bx;
         bx.ext(A[0], A[1], A[2], std::fabs(static_cast<double>(A[3])));
         bx.ext(B[0], B[1], B[2], std::fabs(static_cast<double>(B[3])));
@@ -942,6 +946,7 @@ int buildHostScene(const ubg_scene_desc* d, int layout, HostScene& out, std::str
         out.coneColor.push_back(col);
         out.coneInfo.push_back(info);
         out.coneOp1.push_back(op1);
+        out.coneGroup.push_back(d->cyl_group ? d->cyl_group[ci] : 0);
         DBox bx;
         bx.ext(a.x, a.y, a.z, std::fabs(static_cast<double>(r)));
         bx.ext(b.x, b.y, b.z, std::fabs(static_cast<double>(r)));

@@ -35,6 +35,8 @@ struct HostScene {
   std::vector<F4> rndPlanes;  // 2 per round segment
   std::vector<int32_t> rndSource;  // source cylinder of each round segment
   std::vector<U4> triIdx;
+  // section groups: per sphere, cone and round segment, and per base triangle
+  std::vector<uint16_t> sphGroup, coneGroup, rndGroup, triGroup;
   std::vector<F4> vtxNrm;
   std::vector<F4> vtxCol;
   std::vector<DevMaterial> primMats;

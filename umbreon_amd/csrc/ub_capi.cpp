@@ -63,11 +63,15 @@ struct ubg_context {
   // grow-only frame scratch
   DevBuf hitPix, hitT, hitO, hitN, hitC, hitAo, aoT, activeA, activeB, acc, tnear, hiColor, depth, viewZ,
       hitId, aoOpen, outColor, counters, query, queryOut, hitOpen, hitBent, cGuide, cOpen, cBent;
+  // adaptive AA: hit output indices, the centre / replicated / refined sample
+  // sets (32 B per sample), first-hit records, per-output-pixel mask state
+  DevBuf hitOut, aaSetC, aaSetR, aaSetF, aaFirstN, aaPix;
   unsigned long long* hostCounters = nullptr;  // pinned
   std::vector<DevBuf*> all() {
     return {&hitPix, &hitT,   &hitO,    &hitN,  &hitC,  &hitAo,   &activeA,  &activeB,  &acc,     &tnear,
             &hiColor, &depth, &viewZ,   &hitId, &aoOpen, &outColor, &counters, &query, &queryOut, &aoT,
-            &hitOpen, &hitBent, &cGuide, &cOpen, &cBent};
+            &hitOpen, &hitBent, &cGuide, &cOpen, &cBent, &hitOut, &aaSetC, &aaSetR, &aaSetF, &aaFirstN,
+            &aaPix};
   }
 };
 
@@ -290,6 +294,14 @@ int ubg_scene_create_ex(ubg_context* ctx, const ubg_scene_desc* desc, int32_t bv
   s->view.primMats = static_cast<const ubg::DevMaterial*>(p);
   up(hs.meshMats, &p);
   s->view.meshMats = static_cast<const ubg::DevMaterial*>(p);
+  up(hs.sphGroup, &p);
+  s->view.sphGroup = static_cast<const uint16_t*>(p);
+  up(hs.coneGroup, &p);
+  s->view.coneGroup = static_cast<const uint16_t*>(p);
+  up(hs.rndGroup, &p);
+  s->view.rndGroup = static_cast<const uint16_t*>(p);
+  up(hs.triGroup, &p);
+  s->view.triGroup = static_cast<const uint16_t*>(p);
   if (r == UBG_OK) {
     hipError_t e = hipStreamSynchronize(ctx->stream);
     if (e != hipSuccess) r = hipFail(e, "scene upload");
@@ -438,6 +450,9 @@ int ubg_render(ubg_context* ctx, const ubg_scene* scene, const ubg_render_params
     fc.cH = fc.coarse ? (H + div - 1) / div : 0;
     fc.seedW = W;
   }
+  fc.latW = W;
+  fc.latH = H;
+  fc.smMode = ubg::kSmGrid;
   fc.nLights = p->n_lights;
   for (int i = 0; i < p->n_lights; ++i) {
     const ubg_light& dl = p->lights[i];
@@ -452,6 +467,32 @@ int ubg_render(ubg_context* ctx, const ubg_scene* scene, const ubg_render_params
                                                                    : p->light_radius_deg * 0.01745329252f;
     l.tanRadius = std::tan(l.radius);
     l.soft = (p->shadow_samples > 1 && l.radius > 0.0f) ? 1 : 0;
+  }
+
+  // ---- adaptive AA plan (embree_renderer.cpp:1379-1385, 1430-1434, 1549, 1597-1602) ----
+  const bool adaptive = p->aa_mode == 1 && (ss > 1 || p->aa_depth > 1);
+  const int kFine = std::max(1, (p->aa_depth + ss - 1) / ss);
+  const int64_t nOut = static_cast<int64_t>(p->width) * p->height;
+  // stochastic effects a replicated centre is re-evaluated for, with ss^2
+  // times their samples (AO not when the coarse grid supplies it)
+  const bool softShadows = p->shadow_samples > 1 && p->light_radius_deg > 0.0f;
+  const bool needBoost = adaptive && ss > 1 && ((p->ao_samples > 0 && !fc.coarse) || softShadows);
+  const int aoMul = (needBoost && !fc.coarse) ? ss * ss : 1;
+  const int shMul = (needBoost && softShadows) ? ss * ss : 1;
+  int64_t maxRefined = 0;  // refined samples if every output pixel were flagged
+  if (adaptive) {
+    if (p->aa_mode != 1 || p->aa_depth < 0)
+      return fail(UBG_ERR_INVALID_ARGUMENT, "ubg_render: bad adaptive AA settings");
+    const int64_t per = kFine == 1 ? static_cast<int64_t>(ss) * ss - 1 : static_cast<int64_t>(ss) * ss * kFine * kFine;
+    maxRefined = nOut * per;
+    if (maxRefined >= (int64_t(1) << 31) || static_cast<int64_t>(W) * kFine >= (int64_t(1) << 31) ||
+        static_cast<int64_t>(H) * kFine >= (int64_t(1) << 31))
+      return fail(UBG_ERR_INVALID_ARGUMENT, "ubg_render: aa_depth too large for this frame");
+    if (static_cast<int64_t>(p->ao_samples) * aoMul >= (int64_t(1) << 24) ||
+        static_cast<int64_t>(p->shadow_samples) * shMul >= (int64_t(1) << 24))
+      return fail(UBG_ERR_INVALID_ARGUMENT, "ubg_render: too many samples for adaptive AA");
+  } else if (p->aa_mode != 0 && p->aa_mode != 1) {
+    return fail(UBG_ERR_INVALID_ARGUMENT, "ubg_render: aa_mode must be 0 or 1");
   }
 
   // ---- frame scratch ----
@@ -469,7 +510,7 @@ int ubg_render(ubg_context* ctx, const ubg_scene* scene, const ubg_render_params
   need(ctx->hitN, qn * 16);
   need(ctx->hitC, qn * 16);
   need(ctx->hitAo, qn * 4);
-  if (fc.aoEnh && p->ao_samples > 0) need(ctx->aoT, qn * static_cast<size_t>(p->ao_samples) * 4);
+  if (fc.aoEnh && p->ao_samples > 0) need(ctx->aoT, qn * static_cast<size_t>(p->ao_samples) * aoMul * 4);
   if (fc.coarse) {
     const size_t cells = static_cast<size_t>(fc.cW) * fc.cH;
     need(ctx->hitOpen, qn * 4);
@@ -492,6 +533,12 @@ int ubg_render(ubg_context* ctx, const ubg_scene* scene, const ubg_render_params
   const size_t outN = static_cast<size_t>(p->width) * p->height;
   need(ctx->outColor, outN * 16);
   need(ctx->counters, ubg::kCounterWords * sizeof(unsigned long long));
+  if (adaptive) {
+    need(ctx->hitOut, qn * 4);
+    need(ctx->aaSetC, static_cast<size_t>(nOut) * 32);
+    need(ctx->aaFirstN, static_cast<size_t>(nOut) * 16);
+    need(ctx->aaPix, static_cast<size_t>(nOut) * 18);  // slot, flagged, replicated (u32), mask (f32), raw, mask (u8)
+  }
   if (r != UBG_OK) return r;
 
   ubg::FrameBufs b;
@@ -518,6 +565,9 @@ int ubg_render(ubg_context* ctx, const ubg_scene* scene, const ubg_render_params
   b.aoOpen = fr->ao_open ? static_cast<float*>(ctx->aoOpen.p) : nullptr;
   b.counters = static_cast<unsigned long long*>(ctx->counters.p);
   b.partCap = partCap;
+  b.hitOut = nullptr;
+  b.firstN = nullptr;
+  b.smList = nullptr;
 
   fr->cancelled = 0;
   fr->layers = 0;
@@ -568,33 +618,176 @@ int ubg_render(ubg_context* ctx, const ubg_scene* scene, const ubg_render_params
     timed(UBG_KERNEL_AO, [&] { ubg::launchAo(fcCoarse, scene->view, b, countVisits, st); });
     timed(UBG_KERNEL_AO, [&] { ubg::launchCoarseReduce(fcCoarse, b, st); });
   }
-  int64_t nActive = npx;
-  ubg::PartFill fill{};  // continuation-queue fills the next layer consumes
+  // One pass: the transparency layers of `nFirst` samples (the first layer
+  // per the pass's sample map, then the continuation queue).
   bool cancelled = false;
-  for (int iter = 0; iter < fc.maxIters && nActive > 0; ++iter) {
-    if (p->cancel_flag && *p->cancel_flag) {
-      cancelled = true;
-      break;
+  auto runLayers = [&](const ubg::FrameConst& f, ubg::FrameBufs bb, int64_t nFirst) -> int {
+    int64_t nActive = nFirst;
+    ubg::PartFill fill{};  // continuation-queue fills the next layer consumes
+    for (int iter = 0; iter < f.maxIters && nActive > 0; ++iter) {
+      if (p->cancel_flag && *p->cancel_flag) {
+        cancelled = true;
+        return UBG_OK;
+      }
+      ++fr->layers;
+      ubg::launchResetQueues(bb.counters, st);
+      timed(UBG_KERNEL_PRIMARY, [&] { ubg::launchPrimary(f, scene->view, bb, iter, fill, countVisits, st); });
+      if (f.aoSamples > 0) timed(UBG_KERNEL_AO, [&] { ubg::launchAo(f, scene->view, bb, countVisits, st); });
+      timed(UBG_KERNEL_RESOLVE, [&] { ubg::launchResolve(f, scene->view, bb, iter, countVisits, st); });
+      UBG_HIP(hipGetLastError(), "kernel launch");
+      if (iter + 1 >= f.maxIters) break;
+      // One host round trip per transparency layer: the continuation fills.
+      UBG_HIP(hipMemcpyAsync(ctx->hostCounters, bb.counters, ubg::kCounterWords * sizeof(unsigned long long),
+                             hipMemcpyDeviceToHost, st),
+              "hipMemcpy(counters)");
+      UBG_HIP(hipStreamSynchronize(st), "layer sync");
+      nActive = 0;
+      for (int q = 0; q < ubg::kQueueParts; ++q) {
+        fill.n[q] = static_cast<uint32_t>(ctx->hostCounters[ubg::kActiveFill + ubg::kLineWords * q]);
+        nActive += fill.n[q];
+      }
+      std::swap(bb.activeIn, bb.activeOut);
     }
-    ++fr->layers;
-    ubg::launchResetQueues(b.counters, st);
-    timed(UBG_KERNEL_PRIMARY, [&] { ubg::launchPrimary(fc, scene->view, b, iter, fill, countVisits, st); });
-    if (fc.aoSamples > 0) timed(UBG_KERNEL_AO, [&] { ubg::launchAo(fc, scene->view, b, countVisits, st); });
-    timed(UBG_KERNEL_RESOLVE, [&] { ubg::launchResolve(fc, scene->view, b, iter, countVisits, st); });
+    return UBG_OK;
+  };
+  // Adaptive AA (embree_renderer.cpp:1428-1674): phase 1 shades the centre
+  // subpixel of every output pixel, phase 2 builds the refinement mask from
+  // those results, phase 3 shades the refined subpixels (or their k x k fine
+  // samples) and re-evaluates the replicated centres with boosted sample
+  // counts; a compose pass then writes the hi-res buffers and the fog. Every
+  // pass is the per-layer wavefront pipeline over a sample map; passes larger
+  // than the hi-res grid run in chunks of at most W x H samples.
+  auto renderAdaptive = [&]() -> int {
+    const int Wf = p->width, Hf = p->height;
+    const int cSub = ss / 2;
+    const bool wantHitId = fr->hit_id != nullptr, wantAo = fr->ao_open != nullptr;
+    // a sample set of `cap` samples: colour 16 B, depth, view z, hit id, AO 4 B each
+    auto setOf = [&](DevBuf& d, int64_t cap) {
+      ubg::AaSet a;
+      char* base = static_cast<char*>(d.p);
+      a.color = reinterpret_cast<float4*>(base);
+      a.depth = reinterpret_cast<float*>(base + 16 * cap);
+      a.viewZ = reinterpret_cast<float*>(base + 20 * cap);
+      a.hitId = wantHitId ? reinterpret_cast<uint32_t*>(base + 24 * cap) : nullptr;
+      a.aoOpen = wantAo ? reinterpret_cast<float*>(base + 28 * cap) : nullptr;
+      return a;
+    };
+    auto bufsFor = [&](const ubg::AaSet& a, int64_t off) {
+      ubg::FrameBufs bb = b;
+      bb.hiColor = a.color + off;
+      bb.depth = a.depth + off;
+      bb.viewZ = a.viewZ + off;
+      bb.hitId = a.hitId ? a.hitId + off : nullptr;
+      bb.aoOpen = a.aoOpen ? a.aoOpen + off : nullptr;
+      bb.hitOut = static_cast<uint32_t*>(ctx->hitOut.p);
+      bb.firstN = nullptr;
+      return bb;
+    };
+    // a list pass over `total` samples, in chunks the queues can hold
+    auto runList = [&](ubg::FrameConst f, const ubg::AaSet& a, const uint32_t* list, int64_t total) -> int {
+      const int64_t chunk = npx;
+      for (int64_t base = 0; base < total && !cancelled; base += chunk) {
+        f.smBase = static_cast<uint32_t>(base);
+        f.nList = static_cast<uint32_t>(std::min(chunk, total - base));
+        ubg::FrameBufs bb = bufsFor(a, base);
+        bb.smList = list;
+        const int rr = runLayers(f, bb, f.nList);
+        if (rr != UBG_OK) return rr;
+      }
+      return UBG_OK;
+    };
+    const ubg::AaSet cs = setOf(ctx->aaSetC, nOut);
+    char* pixBase = static_cast<char*>(ctx->aaPix.p);
+    uint32_t* slotOf = reinterpret_cast<uint32_t*>(pixBase);
+    uint32_t* flagList = slotOf + nOut;
+    uint32_t* repList = flagList + nOut;
+    float* maskF = reinterpret_cast<float*>(repList + nOut);
+    uint8_t* raw = reinterpret_cast<uint8_t*>(maskF + nOut);
+    uint8_t* mask = raw + nOut;
+
+    // phase 1: the centres, at their true hi-res seeds and plain sample counts
+    ubg::FrameConst fa = fc;
+    fa.fogOn = 0;  // fog is applied when the hi-res buffers are composed
+    fa.smMode = ubg::kSmCenter;
+    fa.smSs = ss;
+    fa.smK = 1;
+    fa.smCSub = cSub;
+    fa.smWf = Wf;
+    fa.smHf = Hf;
+    fa.smPer = 1;
+    fa.smBase = 0;
+    fa.nList = static_cast<uint32_t>(nOut);
+    {
+      ubg::FrameBufs ba = bufsFor(cs, 0);
+      ba.firstN = static_cast<float4*>(ctx->aaFirstN.p);
+      const int rr = runLayers(fa, ba, nOut);
+      if (rr != UBG_OK || cancelled) return rr;
+    }
+    // phase 2: the refinement mask and the pixel lists
+    const float pws = cam.orthographic ? (cam.height / static_cast<float>(Hf))
+                                       : (2.0f * fc.persHalfH / static_cast<float>(Hf));
+    ubg::launchAaMask(fa, cs, static_cast<const float4*>(ctx->aaFirstN.p), pws, p->aa_threshold, raw, mask, slotOf,
+                      flagList, repList, needBoost, fr->aa_mask ? maskF : nullptr, b.counters, st);
     UBG_HIP(hipGetLastError(), "kernel launch");
-    if (iter + 1 >= fc.maxIters) break;
-    // One host round trip per transparency layer: the continuation fills.
+    if (fr->aa_mask) {
+      const int rr = copyOut(fr->aa_mask, maskF, static_cast<size_t>(nOut) * 4, fr->memory, st);
+      if (rr != UBG_OK) return rr;
+    }
     UBG_HIP(hipMemcpyAsync(ctx->hostCounters, b.counters, ubg::kCounterWords * sizeof(unsigned long long),
                            hipMemcpyDeviceToHost, st),
             "hipMemcpy(counters)");
-    UBG_HIP(hipStreamSynchronize(st), "layer sync");
-    nActive = 0;
-    for (int q = 0; q < ubg::kQueueParts; ++q) {
-      fill.n[q] = static_cast<uint32_t>(ctx->hostCounters[ubg::kActiveFill + ubg::kLineWords * q]);
-      nActive += fill.n[q];
+    UBG_HIP(hipStreamSynchronize(st), "mask sync");
+    const int64_t nFlag = static_cast<int64_t>(ctx->hostCounters[ubg::kAaFlagFill]);
+    const int64_t nRep = needBoost ? static_cast<int64_t>(ctx->hostCounters[ubg::kAaRepFill]) : 0;
+    fr->aa_flagged = nFlag;
+    // phase 3a: replicated centres re-evaluated with ss^2 x the stochastic samples
+    ubg::AaSet rs{};
+    if (nRep > 0) {
+      int rr = ensure(ctx->aaSetR, static_cast<size_t>(nRep) * 32);
+      if (rr != UBG_OK) return rr;
+      rs = setOf(ctx->aaSetR, nRep);
+      ubg::FrameConst fb = fa;
+      fb.aoSamples = p->ao_samples * aoMul;
+      int lanesB = 1;
+      while (lanesB < fb.aoSamples && lanesB < 64) lanesB <<= 1;
+      fb.aoLanes = lanesB;
+      fb.shadowSamples = p->shadow_samples * shMul;
+      rr = runList(fb, rs, repList, nRep);
+      if (rr != UBG_OK || cancelled) return rr;
     }
-    std::swap(b.activeIn, b.activeOut);
-  }
+    // phase 3b: every subpixel of the flagged pixels (k x k fine samples each when k > 1)
+    ubg::AaSet fs{};
+    if (nFlag > 0) {
+      const int64_t per = kFine == 1 ? static_cast<int64_t>(ss) * ss - 1
+                                     : static_cast<int64_t>(ss) * ss * kFine * kFine;
+      const int64_t total = nFlag * per;
+      int rr = ensure(ctx->aaSetF, static_cast<size_t>(total) * 32);
+      if (rr != UBG_OK) return rr;
+      fs = setOf(ctx->aaSetF, total);
+      ubg::FrameConst ff = fa;
+      ff.smMode = kFine == 1 ? ubg::kSmCells : ubg::kSmFine;
+      ff.smK = kFine;
+      ff.smPer = static_cast<int32_t>(per);
+      ff.latW = W * kFine;
+      ff.latH = H * kFine;
+      ff.seedW = W * kFine;
+      rr = runList(ff, fs, flagList, total);
+      if (rr != UBG_OK || cancelled) return rr;
+    }
+    // compose the hi-res buffers (+ fog)
+    ubg::FrameConst fcomp = fc;
+    fcomp.smSs = ss;
+    fcomp.smK = kFine;
+    fcomp.smCSub = cSub;
+    fcomp.smWf = Wf;
+    fcomp.smHf = Hf;
+    ubg::launchAaCompose(fcomp, cs, rs, fs, mask, slotOf, nRep > 0, b, st);
+    UBG_HIP(hipGetLastError(), "kernel launch");
+    return UBG_OK;
+  };
+  fr->aa_flagged = 0;
+  r = adaptive ? renderAdaptive() : runLayers(fc, b, npx);
+  if (r != UBG_OK) return r;
   UBG_HIP(hipEventRecord(ctx->ev[1], st), "hipEventRecord");
   if (cancelled) {
     UBG_HIP(hipStreamSynchronize(st), "cancel sync");

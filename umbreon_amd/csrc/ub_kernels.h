@@ -16,7 +16,8 @@ namespace ubg {
 // thread of the first primary launch.
 struct FrameBufs {
   // hit queue (structure of arrays, filled by the primary pass)
-  uint32_t* hitPix;
+  uint32_t* hitPix;   // seed base px + py * seedW of the hit's ray-lattice position
+  uint32_t* hitOut;   // list passes: the hit's output index (null: = hitPix)
   float* hitT;
   float4* hitO;       // offset origin xyz, eps
   float4* hitN;       // shading normal xyz, packed flags
@@ -41,9 +42,35 @@ struct FrameBufs {
   float* viewZ;
   uint32_t* hitId;    // may be null
   float* aoOpen;      // may be null
+  // adaptive AA: first-hit shading normal and (kind << 16 | group), all ones
+  // on a miss (may be null); the pass's output-pixel list (sample maps)
+  float4* firstN;
+  const uint32_t* smList;
   unsigned long long* counters;
   int32_t partCap;
 };
+
+// Output buffers of one adaptive-AA sample set (any may be null but color).
+struct AaSet {
+  float4* color;      // linear RGBA before fog
+  float* depth;
+  float* viewZ;
+  uint32_t* hitId;
+  float* aoOpen;
+};
+
+// Adaptive-AA refinement (adaptive_aa.hpp): the raw pair predicates over the
+// Wf x Hf centre results, then the 3x3 dilation, which also fills the
+// flagged list (and, with `wantRep`, the replicated list) with one slot per
+// output pixel (slotOf) and writes the float mask when `maskOut` is set.
+void launchAaMask(const FrameConst& fc, const AaSet& centers, const float4* firstN, float pixelWorldSize,
+                  float threshold, uint8_t* raw, uint8_t* mask, uint32_t* slotOf, uint32_t* flagList,
+                  uint32_t* repList, bool wantRep, float* maskOut, unsigned long long* counters, hipStream_t st);
+// Composes the W x H hi-res buffers (fog applied) from the centre, replicated
+// and refined sample sets (embree_renderer.cpp:1619-1673 + fog.cpp:17-36).
+void launchAaCompose(const FrameConst& fc, const AaSet& centers, const AaSet& rep, const AaSet& cells,
+                     const uint8_t* mask, const uint32_t* slotOf, bool boosted, const FrameBufs& out,
+                     hipStream_t st);
 
 // Queue slots per partition for a W x H hi-res grid.
 int queuePartCap(int W, int H);
@@ -55,8 +82,10 @@ struct PartFill {
 
 void launchZeroCounters(unsigned long long* c, int n, hipStream_t st);
 void launchResetQueues(unsigned long long* c, hipStream_t st);
-// iter == 0 traces every hi-res pixel; later layers trace the continuation
-// queue whose partition fills are `fill` (read back by the host).
+// iter == 0 traces the pass's samples (every hi-res pixel for kSmGrid, the
+// output pixels for an unlisted kSmCenter pass, else fc.nList list samples);
+// later layers trace the continuation queue whose partition fills are
+// `fill` (read back by the host).
 void launchPrimary(const FrameConst& fc, const SceneView& s, const FrameBufs& b, int iter,
                    const PartFill& fill, bool countVisits, hipStream_t st);
 // AO and resolve read the hit-queue fills on the device (no host round trip).

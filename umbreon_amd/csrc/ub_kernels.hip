@@ -152,8 +152,44 @@ __device__ __forceinline__ void genRayWH(const FrameConst& fc, uint32_t px, uint
     rd = normalize(dir + right * (u * fc.persHalfW) + up * (v * fc.persHalfH));
   }
 }
+// Primary ray through (px, py) of the pass's ray lattice (latW x latH).
 __device__ __forceinline__ void genRay(const FrameConst& fc, uint32_t px, uint32_t py, V3& org, V3& rd) {
-  genRayWH(fc, px, py, fc.W, fc.H, org, rd);
+  genRayWH(fc, px, py, fc.latW, fc.latH, org, rd);
+}
+
+// Output index of the current pass -> its ray-lattice position (the sample
+// maps kSm* of ub_layout.h; embree_renderer.cpp:1512-1518 centres,
+// :1654-1660 flagged subpixels, :1551-1567 the aaDepth fine lattice).
+__device__ __forceinline__ void sampleLattice(const FrameConst& fc, const FrameBufs& b, uint32_t out, uint32_t& lx,
+                                              uint32_t& ly) {
+  if (fc.smMode == kSmGrid) {
+    lx = out % static_cast<uint32_t>(fc.W);
+    ly = out / static_cast<uint32_t>(fc.W);
+    return;
+  }
+  const uint32_t sIdx = fc.smBase + out;
+  const uint32_t ss = static_cast<uint32_t>(fc.smSs), wf = static_cast<uint32_t>(fc.smWf);
+  if (fc.smMode == kSmCenter) {
+    const uint32_t xy = b.smList ? b.smList[sIdx] : sIdx;
+    lx = (xy % wf) * ss + static_cast<uint32_t>(fc.smCSub);
+    ly = (xy / wf) * ss + static_cast<uint32_t>(fc.smCSub);
+    return;
+  }
+  const uint32_t per = static_cast<uint32_t>(fc.smPer);
+  const uint32_t xy = b.smList[sIdx / per];
+  const uint32_t r = sIdx % per;
+  const uint32_t X = xy % wf, Y = xy / wf;
+  if (fc.smMode == kSmCells) {
+    const uint32_t centre = static_cast<uint32_t>(fc.smCSub) * ss + static_cast<uint32_t>(fc.smCSub);
+    const uint32_t cell = r + (r >= centre ? 1u : 0u);
+    lx = X * ss + cell % ss;
+    ly = Y * ss + cell / ss;
+    return;
+  }
+  const uint32_t k = static_cast<uint32_t>(fc.smK), kk = k * k;
+  const uint32_t cell = r / kk, f = r % kk;
+  lx = (X * ss + cell % ss) * k + f % k;
+  ly = (Y * ss + cell / ss) * k + f / k;
 }
 
 // transparency.hpp:118-129 (view-z clip planes -> ray t range)
@@ -203,14 +239,15 @@ __device__ __forceinline__ void finalizePixel(const FrameConst& fc, const FrameB
 }
 
 // ao_coarse.hpp:138-205: bilateral lookup of the coarse AO grid for a hit at
-// hi-res pixel (px, py) with shading normal N at ray distance z: the 2x2
+// ray-lattice position (px, py) (hit_shader.hpp:129-133: the lattice is the
+// seed lattice) with shading normal N at ray distance z: the 2x2
 // surrounding cells weighted bilinear x dot(N, Nc)^32 x exp(-|z - zc| /
 // (0.02 z + 1e-6)); false when the weight sum is below 0.3 (the caller then
 // gathers inline).
 __device__ bool sampleCoarseAo(const FrameConst& fc, const FrameBufs& b, uint32_t px, uint32_t py, V3 N, float z,
                                float& openness, V3& bent) {
-  const float sx = ((float)px + 0.5f) * (float)fc.cW / (float)fc.W - 0.5f;
-  const float sy = ((float)py + 0.5f) * (float)fc.cH / (float)fc.H - 0.5f;
+  const float sx = ((float)px + 0.5f) * (float)fc.cW / (float)fc.latW - 0.5f;
+  const float sy = ((float)py + 0.5f) * (float)fc.cH / (float)fc.latH - 0.5f;
   const int bx = static_cast<int>(floorf(sx));
   const int by = static_cast<int>(floorf(sy));
   const float fx = sx - (float)bx;
@@ -250,14 +287,21 @@ __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(kTr
                                                          int iter, PartFill fill, int partBlocks) {
   const int tid = blockIdx.x * blockDim.x + threadIdx.x;
   bool valid;
-  uint32_t pix = 0;
-  if (iter == 0) {
-    // first layer: each wave takes an 8x8 pixel tile (coherent rays)
-    const uint32_t tilesX = (static_cast<uint32_t>(fc.W) + 7u) >> 3;
+  uint32_t pix = 0;  // output index of the pass
+  if (iter == 0 && (fc.smMode == kSmGrid || (fc.smMode == kSmCenter && b.smList == nullptr))) {
+    // first layer over a grid (the hi-res pixels, or the output pixels'
+    // centres): each wave takes an 8x8 tile (coherent rays)
+    const uint32_t gw = static_cast<uint32_t>(fc.smMode == kSmGrid ? fc.W : fc.smWf);
+    const uint32_t gh = static_cast<uint32_t>(fc.smMode == kSmGrid ? fc.H : fc.smHf);
+    const uint32_t tilesX = (gw + 7u) >> 3;
     const uint32_t wave = static_cast<uint32_t>(tid) >> 6, lane = static_cast<uint32_t>(tid) & 63u;
     const uint32_t px = (wave % tilesX) * 8u + (lane & 7u), py = (wave / tilesX) * 8u + (lane >> 3);
-    valid = px < static_cast<uint32_t>(fc.W) && py < static_cast<uint32_t>(fc.H);
-    pix = py * static_cast<uint32_t>(fc.W) + px;
+    valid = px < gw && py < gh;
+    pix = py * gw + px;
+  } else if (iter == 0) {
+    // first layer of a list pass: one lane per sample
+    valid = static_cast<uint32_t>(tid) < fc.nList;
+    pix = static_cast<uint32_t>(tid);
   } else {
     // later layers: partBlocks blocks per continuation-queue partition
     const uint32_t p = blockIdx.x / static_cast<uint32_t>(partBlocks);
@@ -270,9 +314,10 @@ __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(kTr
   V3 org = mk(0, 0, 0), rd = mk(0, 0, 1);
   float tn = 0.0f, tf = 0.0f;
   uint32_t nv = 0, nt = 0;
+  uint32_t lx = 0, ly = 0;  // ray-lattice position
   if (valid) {
-    const uint32_t px = pix % static_cast<uint32_t>(fc.W), py = pix / static_cast<uint32_t>(fc.W);
-    genRay(fc, px, py, org, rd);
+    sampleLattice(fc, b, pix, lx, ly);
+    genRay(fc, lx, ly, org, rd);
     clipRange(fc, rd, tn, tf);
     if (iter > 0) tn = b.tnear[pix];
   }
@@ -288,6 +333,7 @@ __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(kTr
       b.viewZ[pix] = vz;
       if (b.hitId) b.hitId[pix] = hit ? h.bits : 0xFFFFFFFFu;
       if (b.aoOpen && !hit) b.aoOpen[pix] = 1.0f;
+      if (b.firstN && !hit) b.firstN[pix] = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(0xFFFFFFFFu));
     }
     if (!hit) {
       const float4 acc = iter == 0 ? make_float4(0.0f, 0.0f, 0.0f, 0.0f) : b.acc[pix];
@@ -352,6 +398,16 @@ __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(kTr
     fromEdge = (info & kInfoFromEdge) != 0u;
     flags = info & kInfoMatMask;
   }
+  if (iter == 0 && b.firstN) {
+    // adaptive-AA first-hit record (transparency.hpp:402-413, hit_shader.hpp:
+    // 214-220 / :321): shading normal, geometry kind and section group
+    uint32_t grp;
+    if (kind == 0u) grp = s.triGroup[static_cast<int64_t>(id) % s.meshBaseTris];
+    else if (kind == 1u) grp = s.sphGroup[id];
+    else if (kind == 3u) grp = s.coneGroup[id];
+    else grp = s.rndGroup[id];
+    b.firstN[pix] = make_float4(N.x, N.y, N.z, __uint_as_float((kind << 16) | grp));
+  }
   if (!fromEdge && fc.aoSamples > 0) flags |= kFlagAo;
   if (!fromEdge && fc.shadows) flags |= kFlagShadow;
   const V3 P = mk(org.x + rd.x * h.t, org.y + rd.y * h.t, org.z + rd.z * h.t);
@@ -365,14 +421,14 @@ __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(kTr
     // hit_shader.hpp:124-140: a hit the coarse grid covers needs no AO rays
     float open;
     V3 bent = N;
-    const uint32_t px = pix % static_cast<uint32_t>(fc.W), py = pix / static_cast<uint32_t>(fc.W);
-    if (sampleCoarseAo(fc, b, px, py, N, h.t, open, bent)) {
+    if (sampleCoarseAo(fc, b, lx, ly, N, h.t, open, bent)) {
       flags = (flags & ~kFlagAo) | kFlagCoarse;
       b.hitOpen[slot] = open;
       if (fc.aoEnh) b.hitBent[slot] = make_float4(bent.x, bent.y, bent.z, 0.0f);
     }
   }
-  b.hitPix[slot] = pix;
+  b.hitPix[slot] = lx + ly * static_cast<uint32_t>(fc.seedW);
+  if (b.hitOut) b.hitOut[slot] = pix;
   b.hitT[slot] = h.t;
   b.hitO[slot] = make_float4(O.x, O.y, O.z, eps);
   b.hitN[slot] = make_float4(N.x, N.y, N.z, __uint_as_float(flags));
@@ -402,8 +458,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_ao(FrameConst fc, SceneView s, 
   const V3 N = ld3(N4);
   const V3 O = ld3(O4);
   const Frame3 f = frameFromNormal(N);
-  const uint32_t seed = (pix % static_cast<uint32_t>(fc.W)) +
-                        (pix / static_cast<uint32_t>(fc.W)) * static_cast<uint32_t>(fc.W);
+  const uint32_t seed = pix;  // px + py * seedW
   const int n = fc.aoSamples;
   const int rounds = (n + lanes - 1) / lanes;
   uint32_t occ = 0, nv = 0, nt = 0;
@@ -711,17 +766,18 @@ __global__ __launch_bounds__(kTraceBlock) void k_resolve(FrameConst fc, SceneVie
   const bool valid = part < kQueueParts && static_cast<uint32_t>(tid % b.partCap) < queueFill(b.counters, kHitFill, part);
   uint32_t nShadow = 0, nv = 0, nt = 0;
   bool capped = false, cont = false;
-  uint32_t pix = 0;
+  uint32_t pix = 0;  // output index
   float4 acc = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   float tnext = 0.0f;
   if (valid) {
-    pix = b.hitPix[tid];
+    const uint32_t seedPix = b.hitPix[tid];
+    pix = b.hitOut ? b.hitOut[tid] : seedPix;
     const float t = b.hitT[tid];
     const float4 O4 = b.hitO[tid];
     const float4 N4 = b.hitN[tid];
     const float4 C4 = b.hitC[tid];
     const uint32_t flags = __float_as_uint(N4.w);
-    const uint32_t px = pix % static_cast<uint32_t>(fc.W), py = pix / static_cast<uint32_t>(fc.W);
+    const uint32_t px = seedPix % static_cast<uint32_t>(fc.seedW), py = seedPix / static_cast<uint32_t>(fc.seedW);
     V3 org, rd;
     genRay(fc, px, py, org, rd);
     const V3 V = normalize(neg(rd));
@@ -1112,6 +1168,201 @@ __global__ __launch_bounds__(kTraceBlock) void k_segments(SceneView s, const Seg
   occ[i] = r;
 }
 
+// ---- adaptive antialiasing (render/adaptive_aa.hpp, embree_renderer.cpp:1428-1674) ----
+
+// One output pixel's centre result as the refinement predicates see it.
+struct AaPx {
+  float4 c;    // colour + alpha (before fog)
+  float z;     // first-hit view z (0 on a miss)
+  float4 n;    // first-hit shading normal xyz, (kind << 16 | group) bits; all ones on a miss
+};
+
+__device__ __forceinline__ AaPx aaLoad(const AaSet& cs, const float4* firstN, int i) {
+  AaPx p;
+  p.c = cs.color[i];
+  p.z = cs.viewZ[i];
+  p.n = firstN[i];
+  return p;
+}
+
+// adaptive_aa.hpp:66-115 (aaPairNeedsRefine) with the default thresholds:
+// normal delta 0.3, depth crack 3 x the smaller outer slope or 12 output-pixel
+// world sizes (times the nearer |z| in perspective).
+__device__ bool aaPair(const AaPx& a, const AaPx& b, float za0, float zb1, float pws, float thr, bool persp) {
+  const uint32_t ia = __float_as_uint(a.n.w), ib = __float_as_uint(b.n.w);
+  if (ia != ib) return true;  // first-hit geometry or section differs (background included)
+  if (fabsf(a.c.x - b.c.x) > thr || fabsf(a.c.y - b.c.y) > thr || fabsf(a.c.z - b.c.z) > thr ||
+      fabsf(a.c.w - b.c.w) > thr)
+    return true;
+  if (ia == 0xFFFFFFFFu) return false;  // both background
+  const float la = a.n.x * a.n.x + a.n.y * a.n.y + a.n.z * a.n.z;
+  const float lb = b.n.x * b.n.x + b.n.y * b.n.y + b.n.z * b.n.z;
+  if (la > 0.0f && lb > 0.0f) {
+    const float d = a.n.x * b.n.x + a.n.y * b.n.y + a.n.z * b.n.z;
+    if (1.0f - d > 0.3f) return true;
+  }
+  const float step = fabsf(b.z - a.z);
+  const float sa = fabsf(a.z - za0);
+  const float sb = fabsf(zb1 - b.z);
+  const float slope = sa < sb ? sa : sb;
+  float absTerm = 12.0f * pws;
+  if (persp) {
+    const float zmin = fabsf(a.z) < fabsf(b.z) ? fabsf(a.z) : fabsf(b.z);
+    absTerm *= zmin;
+  }
+  const float thresh = 3.0f * slope > absTerm ? 3.0f * slope : absTerm;
+  return step > thresh;
+}
+
+// adaptive_aa.hpp:128-162: an output pixel is raw-flagged when any of the
+// (up to four) 4-neighbour pairs it belongs to needs refinement; outer
+// slopes come from the next pixel along the axis, or the pair's own linear
+// extrapolation at the border.
+__global__ void k_aa_raw(int Wf, int Hf, AaSet cs, const float4* __restrict__ firstN, float pws, float thr,
+                         int persp, uint8_t* __restrict__ raw) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= Wf * Hf) return;
+  const int x = i % Wf, y = i / Wf;
+  bool f = false;
+  // horizontal pairs (x0, x0 + 1) with x0 = x - 1 and x0 = x
+  for (int x0 = x - 1; x0 <= x && !f; ++x0) {
+    if (x0 < 0 || x0 + 1 >= Wf) continue;
+    const AaPx a = aaLoad(cs, firstN, y * Wf + x0), b = aaLoad(cs, firstN, y * Wf + x0 + 1);
+    const float za0 = x0 > 0 ? cs.viewZ[y * Wf + x0 - 1] : 2.0f * a.z - b.z;
+    const float zb1 = x0 + 2 < Wf ? cs.viewZ[y * Wf + x0 + 2] : 2.0f * b.z - a.z;
+    f = aaPair(a, b, za0, zb1, pws, thr, persp != 0);
+  }
+  for (int y0 = y - 1; y0 <= y && !f; ++y0) {
+    if (y0 < 0 || y0 + 1 >= Hf) continue;
+    const AaPx a = aaLoad(cs, firstN, y0 * Wf + x), b = aaLoad(cs, firstN, (y0 + 1) * Wf + x);
+    const float za0 = y0 > 0 ? cs.viewZ[(y0 - 1) * Wf + x] : 2.0f * a.z - b.z;
+    const float zb1 = y0 + 2 < Hf ? cs.viewZ[(y0 + 2) * Wf + x] : 2.0f * b.z - a.z;
+    f = aaPair(a, b, za0, zb1, pws, thr, persp != 0);
+  }
+  raw[i] = f ? 1 : 0;
+}
+
+// adaptive_aa.hpp:36-57 (3x3 dilation, border-clamped) + the pixel lists:
+// a flagged pixel gets the next slot of the flagged list, a replicated one
+// (when its centre is re-evaluated) the next slot of the replicated list
+// (one atomic per wave each).
+__global__ void k_aa_dilate(int Wf, int Hf, const uint8_t* __restrict__ raw, uint8_t* __restrict__ mask,
+                            uint32_t* __restrict__ slotOf, uint32_t* __restrict__ flagList,
+                            uint32_t* __restrict__ repList, int wantRep, float* __restrict__ maskOut,
+                            unsigned long long* cnt) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool valid = i < Wf * Hf;
+  bool m = false;
+  if (valid) {
+    const int x = i % Wf, y = i / Wf;
+    for (int dy = -1; dy <= 1 && !m; ++dy) {
+      const int yy = y + dy;
+      if (yy < 0 || yy >= Hf) continue;
+      for (int dx = -1; dx <= 1; ++dx) {
+        const int xx = x + dx;
+        if (xx < 0 || xx >= Wf) continue;
+        if (raw[yy * Wf + xx]) {
+          m = true;
+          break;
+        }
+      }
+    }
+    mask[i] = m ? 1 : 0;
+    if (maskOut) maskOut[i] = m ? 1.0f : 0.0f;
+  }
+  const unsigned long long lt = (1ull << __lane_id()) - 1ull;
+  const unsigned long long mf = __ballot(valid && m);
+  const unsigned long long mr = __ballot(valid && !m && wantRep);
+  uint32_t bf = 0, br = 0;
+  if (__lane_id() == 0) {
+    if (mf) bf = static_cast<uint32_t>(atomicAdd(&cnt[kAaFlagFill], (unsigned long long)__popcll(mf)));
+    if (mr) br = static_cast<uint32_t>(atomicAdd(&cnt[kAaRepFill], (unsigned long long)__popcll(mr)));
+  }
+  bf = __shfl(bf, 0, 64);
+  br = __shfl(br, 0, 64);
+  if (!valid) return;
+  if (m) {
+    const uint32_t slot = bf + static_cast<uint32_t>(__popcll(mf & lt));
+    flagList[slot] = static_cast<uint32_t>(i);
+    slotOf[i] = slot;
+  } else if (wantRep) {
+    const uint32_t slot = br + static_cast<uint32_t>(__popcll(mr & lt));
+    repList[slot] = static_cast<uint32_t>(i);
+    slotOf[i] = slot;
+  }
+}
+
+// Phase 3 store (embree_renderer.cpp:1619-1673) + fog (fog.cpp:17-36, the
+// float operations of finalizePixel): every hi-res pixel takes its sample
+// set's result -- the replicated centre (re-evaluated or not), the centre
+// itself, a refined subpixel, or the in-order box average of its k x k fine
+// samples (colour, depth and view z averaged; id and AO of the first sample).
+__global__ void k_aa_compose(FrameConst fc, AaSet cs, AaSet rs, AaSet fs, const uint8_t* __restrict__ mask,
+                             const uint32_t* __restrict__ slotOf, int boosted, FrameBufs out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= fc.W * fc.H) return;
+  const int ss = fc.smSs, k = fc.smK, cSub = fc.smCSub;
+  const int px = i % fc.W, py = i / fc.W;
+  const int X = px / ss, Y = py / ss, ci = px % ss, cj = py % ss;
+  const int xy = Y * fc.smWf + X;
+  AaSet src;
+  int64_t si;
+  int n = 1;
+  if (!mask[xy]) {
+    src = boosted ? rs : cs;
+    si = boosted ? static_cast<int64_t>(slotOf[xy]) : xy;
+  } else if (k == 1) {
+    if (ci == cSub && cj == cSub) {
+      src = cs;
+      si = xy;
+    } else {
+      const int cell = cj * ss + ci, centre = cSub * ss + cSub;
+      src = fs;
+      si = static_cast<int64_t>(slotOf[xy]) * (ss * ss - 1) + (cell > centre ? cell - 1 : cell);
+    }
+  } else {
+    src = fs;
+    n = k * k;
+    si = (static_cast<int64_t>(slotOf[xy]) * ss * ss + cj * ss + ci) * n;
+  }
+  float4 c = src.color[si];
+  float depth = src.depth[si];
+  float vz = src.viewZ[si];
+  if (n > 1) {
+    for (int f = 1; f < n; ++f) {
+      const float4 v = src.color[si + f];
+      c.x += v.x;
+      c.y += v.y;
+      c.z += v.z;
+      c.w += v.w;
+      depth += src.depth[si + f];
+      vz += src.viewZ[si + f];
+    }
+    const float inv = 1.0f / static_cast<float>(n);
+    c.x *= inv;
+    c.y *= inv;
+    c.z *= inv;
+    c.w *= inv;
+    depth *= inv;
+    vz *= inv;
+  }
+  if (fc.fogOn && vz > 0.0f) {
+    const float f = fogFactor(fc, vz);
+    if (fc.transparentBg) {
+      c.w *= f;
+    } else {
+      c.x = fc.fogColor[0] * (1.0f - f) + c.x * f;
+      c.y = fc.fogColor[1] * (1.0f - f) + c.y * f;
+      c.z = fc.fogColor[2] * (1.0f - f) + c.z * f;
+    }
+  }
+  out.hiColor[i] = c;
+  out.depth[i] = depth;
+  out.viewZ[i] = vz;
+  if (out.hitId) out.hitId[i] = src.hitId[si];
+  if (out.aoOpen) out.aoOpen[i] = src.aoOpen[si];
+}
+
 __global__ void k_zero(unsigned long long* c, int n) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) c[i] = 0ull;
@@ -1169,8 +1420,14 @@ int queuePartCap(int W, int H) {
 void launchPrimary(const FrameConst& fc, const SceneView& s, const FrameBufs& b, int iter,
                    const PartFill& fill, bool countVisits, hipStream_t st) {
   if (iter == 0) {
-    // one 64-lane wave per 8x8 tile of the hi-res grid
-    const int64_t n = static_cast<int64_t>((fc.W + 7) / 8) * ((fc.H + 7) / 8) * 64;
+    int64_t n;
+    if (fc.smMode == kSmGrid)  // one 64-lane wave per 8x8 tile of the hi-res grid
+      n = static_cast<int64_t>((fc.W + 7) / 8) * ((fc.H + 7) / 8) * 64;
+    else if (fc.smMode == kSmCenter && b.smList == nullptr)  // ... of the output grid
+      n = static_cast<int64_t>((fc.smWf + 7) / 8) * ((fc.smHf + 7) / 8) * 64;
+    else
+      n = fc.nList;
+    if (n <= 0) return;
     UBG_DISPATCH(k_primary, blocksFor(n, kTraceBlock), countVisits, fc, s, b, iter, fill, 1);
     return;
   }
@@ -1205,6 +1462,23 @@ void launchResolve(const FrameConst& fc, const SceneView& s, const FrameBufs& b,
                    hipStream_t st) {
   const int64_t slots = static_cast<int64_t>(kQueueParts) * b.partCap;
   UBG_DISPATCH(k_resolve, blocksFor(slots, kTraceBlock), countVisits, fc, s, b, iter, 0);
+}
+void launchAaMask(const FrameConst& fc, const AaSet& centers, const float4* firstN, float pixelWorldSize,
+                  float threshold, uint8_t* raw, uint8_t* mask, uint32_t* slotOf, uint32_t* flagList,
+                  uint32_t* repList, bool wantRep, float* maskOut, unsigned long long* counters, hipStream_t st) {
+  const int n = fc.smWf * fc.smHf;
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_aa_raw, dim3(blocksFor(n, 256)), dim3(256), 0, st, fc.smWf, fc.smHf, centers, firstN,
+                     pixelWorldSize, threshold, fc.ortho ? 0 : 1, raw);
+  hipLaunchKernelGGL(k_aa_dilate, dim3(blocksFor(n, 256)), dim3(256), 0, st, fc.smWf, fc.smHf, raw, mask, slotOf,
+                     flagList, repList, wantRep ? 1 : 0, maskOut, counters);
+}
+void launchAaCompose(const FrameConst& fc, const AaSet& centers, const AaSet& rep, const AaSet& cells,
+                     const uint8_t* mask, const uint32_t* slotOf, bool boosted, const FrameBufs& out,
+                     hipStream_t st) {
+  const int64_t n = static_cast<int64_t>(fc.W) * fc.H;
+  hipLaunchKernelGGL(k_aa_compose, dim3(blocksFor(n, 256)), dim3(256), 0, st, fc, centers, rep, cells, mask, slotOf,
+                     boosted ? 1 : 0, out);
 }
 void launchPost(const float4* hi, int W, int H, int ss, float gamma, float4* out, hipStream_t st) {
   const int n = (W / ss) * (H / ss);

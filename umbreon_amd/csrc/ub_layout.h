@@ -90,6 +90,12 @@ struct SceneView {
   const DevMaterial* primMats;
   const DevMaterial* meshMats;
   int64_t meshBaseTris;   // triangles per mesh instance copy (0 = no mesh)
+  // section group per primitive (Sphere/Cylinder::group, Mesh::triGroupId by
+  // base triangle), read only for the adaptive-AA first-hit record
+  const uint16_t* sphGroup;
+  const uint16_t* coneGroup;
+  const uint16_t* rndGroup;
+  const uint16_t* triGroup;
 };
 
 struct DevLight {
@@ -135,8 +141,32 @@ struct FrameConst {
   // coarse-grid AO: grid cW x cH (coarse != 0); seedW = width of the AO seed
   // lattice (base = px + py * seedW): the hi-res W, or cW for the grid gather
   int32_t coarse, cW, cH, seedW;
+  // Ray lattice of the pass: primary rays go through the centres of a latW x
+  // latH grid over the image plane (the hi-res W x H, or W*k x H*k for the
+  // adaptive-AA fine samples), and (px, py) on it seed the AO / shadow
+  // streams with seedW = latW. The sample map (smMode) turns a pass's output
+  // index into its lattice position; see kSm* below.
+  int32_t latW, latH;
+  int32_t smMode;          // kSmGrid / kSmCenter / kSmCells / kSmFine
+  int32_t smSs, smK, smCSub, smWf, smHf, smPer;
+  uint32_t smBase;         // first sample of this chunk of the pass
+  uint32_t nList;          // samples in this chunk (list modes)
   DevLight lights[kMaxLights];
 };
+
+// Sample maps of a pass (output index s = smBase + local):
+// * kSmGrid  : s = py * W + px on the hi-res lattice (the full grid);
+// * kSmCenter: s -> output pixel XY (= s, or smList[s]); lattice position
+//              (X * ss + cSub, Y * ss + cSub), cSub = ss / 2;
+// * kSmCells : the ss^2 - 1 non-centre subpixels of flagged output pixel
+//              smList[s / smPer] (smPer = ss^2 - 1), in row order;
+// * kSmFine  : the k x k fine samples of every subpixel of flagged pixel
+//              smList[s / smPer] (smPer = ss^2 k^2), subpixel-major, row order
+//              inside; lattice W*k x H*k.
+constexpr int kSmGrid = 0;
+constexpr int kSmCenter = 1;
+constexpr int kSmCells = 2;
+constexpr int kSmFine = 3;
 
 // Frame-time counters (u64), in one buffer of kCounterWords words:
 // * statistics (ray and visit counts): kStatSpread groups of kNumCounters
@@ -169,6 +199,9 @@ enum Counter {
 constexpr int kHitFill = kNumCounters * kStatSpread;
 constexpr int kActiveFill = kHitFill + kLineWords * kQueueParts;
 constexpr int kAoCursor = kActiveFill + kLineWords * kQueueParts;
-constexpr int kCounterWords = kAoCursor + kLineWords * kQueueParts;
+// adaptive AA: fills of the flagged and the replicated output-pixel lists
+constexpr int kAaFlagFill = kAoCursor + kLineWords * kQueueParts;
+constexpr int kAaRepFill = kAaFlagFill + kLineWords;
+constexpr int kCounterWords = kAaRepFill + kLineWords;
 
 }  // namespace ubg

@@ -34,6 +34,8 @@ class FrameResult:
     layers: int = 0
     kernel_ms: dict = field(default_factory=dict)
     kernel_launches: dict = field(default_factory=dict)
+    aa_mask: Optional[np.ndarray] = None  # (H, W) 1 = refined (adaptive AA with aa_debug)
+    aa_flagged: int = 0
 
     @property
     def total_rays(self) -> int:
@@ -142,6 +144,9 @@ class BuiltScene:
         fr.view_z = view_z.ctypes.data if view_z is not None else None
         fr.hit_id = hit_id.ctypes.data if hit_id is not None else None
         fr.ao_open = ao.ctypes.data if ao is not None else None
+        adaptive = opt.aa_mode == 1 and (ss > 1 or opt.aa_depth > 1)
+        aa_mask = np.zeros((opt.height, opt.width), np.float32) if opt.aa_debug and adaptive else None
+        fr.aa_mask = aa_mask.ctypes.data if aa_mask is not None else None
         fr.memory = 0
         capi.check(self._lib.ubg_render(self.device.handle, self.handle, C.byref(p), C.byref(fr)))
         del keep
@@ -150,7 +155,8 @@ class BuiltScene:
                            render_seconds=fr.render_seconds, post_seconds=fr.post_seconds,
                            cancelled=bool(fr.cancelled), layers=int(fr.layers),
                            kernel_ms={k: float(fr.kernel_ms[i]) for i, k in enumerate(capi.KERNELS)},
-                           kernel_launches={k: int(fr.kernel_launches[i]) for i, k in enumerate(capi.KERNELS)})
+                           kernel_launches={k: int(fr.kernel_launches[i]) for i, k in enumerate(capi.KERNELS)},
+                           aa_mask=aa_mask, aa_flagged=int(fr.aa_flagged))
 
 
 def host_query(scene: Scene, rays: np.ndarray, layout: int = capi.BVH8):

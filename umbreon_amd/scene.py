@@ -110,9 +110,13 @@ class RenderOptions:
     ao_low_discrepancy: bool = False
     ao_diffuse_factor: float = 0.0
     ao_res_div: int = 0  # coarse-grid AO: > 1 divisor, -1 = supersample factor, 0/1 = inline
+    # adaptive antialiasing (render_options.hpp:20-39): 0 grid, 1 adaptive
+    aa_mode: int = 0
+    aa_threshold: float = 0.1
+    aa_depth: int = 0
+    aa_debug: bool = False  # fill FrameResult.aa_mask
     # options the gfx950 path does not implement; must stay at their defaults
     gi: bool = False
-    aa_mode: int = 0
     denoiser: int = 0
     env_lights: int = 0
 
@@ -310,9 +314,9 @@ def u16p():
 def params(scene: Scene, opt: RenderOptions, flags: int = 0,
            cancel_flag: Optional[C.c_int32] = None) -> Tuple[capi.RenderParams, list]:
     """RenderOptions + Scene frame constants -> ubg_render_params."""
-    if opt.gi or opt.aa_mode != 0 or opt.denoiser != 0 or opt.env_lights > 0:
+    if opt.gi or opt.denoiser != 0 or opt.env_lights > 0:
         raise capi.UmbreonError(capi.UBG_ERR_UNSUPPORTED,
-                                "GI / adaptive AA / denoise / env lights are not part of the gfx950 path")
+                                "GI / denoise / env lights are not part of the gfx950 path")
     keep: list = []
     p = capi.RenderParams()
     p.width, p.height, p.supersample = int(opt.width), int(opt.height), int(opt.supersample)
@@ -358,6 +362,9 @@ def params(scene: Scene, opt: RenderOptions, flags: int = 0,
     p.ao_low_discrepancy = int(bool(opt.ao_low_discrepancy))
     p.ao_diffuse_factor = opt.ao_diffuse_factor
     p.ao_res_div = int(opt.ao_res_div)
+    p.aa_mode = int(opt.aa_mode)
+    p.aa_threshold = float(opt.aa_threshold)
+    p.aa_depth = int(opt.aa_depth)
     p.flags = flags
     if cancel_flag is not None:
         keep.append(cancel_flag)
