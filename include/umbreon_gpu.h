@@ -40,7 +40,7 @@
 extern "C" {
 #endif
 
-#define UBG_ABI_VERSION 4
+#define UBG_ABI_VERSION 5
 
 /* Status codes. */
 enum {
@@ -217,6 +217,32 @@ typedef struct ubg_render_params {
   int32_t aa_mode;
   float aa_threshold;                /* reference default 0.1 */
   int32_t aa_depth;
+  /* Path-traced diffuse GI, the pt1 integrator (render_options.hpp:93-162,
+   * integrator/pt1/pt1_gather.hpp, embree_renderer.cpp:560-1131 full-res
+   * branch). gi != 0 requires gi_integrator == 1, a gather grid that
+   * resolves to the render grid (pt1_gather_div 1, or -1 at supersample 1,
+   * or 0 with pt1_half_res 0) and pt1_denoise == 0; otherwise
+   * UBG_ERR_UNSUPPORTED. Mesh hits and real (not outline) primitives become
+   * receivers: their constant ambient is dropped (toon finishes keep unit
+   * ambient) and pt1_spp cosine paths per hi-res pixel gather E (next-event
+   * estimation at the distant lights, Russian roulette from the second
+   * vertex, the sky/ground environment at escapes); the frame adds
+   * gi_intensity * diffuse * pigment * E before fog. Under gi the reference
+   * turns adaptive AA and coarse AO off, and so does this path. */
+  int32_t gi;
+  int32_t gi_bounces;
+  float gi_max_distance;             /* <= 0: unbounded */
+  float gi_intensity;
+  float gi_env_intensity;
+  int32_t pt1_spp;
+  uint32_t pt1_seed;
+  int32_t pt1_sky_mode;              /* 0 uniform sky; 1 gradient (ground = ao_ground_color) */
+  float pt1_sky_radiance[3];
+  int32_t pt1_ld;                    /* Hammersley first bounce with a per-pixel shift */
+  float pt1_clamp;                   /* per-sample luminance clamp; 0 = off */
+  int32_t pt1_gather_div;            /* -1 = supersample factor, 0 = pt1_half_res rule, k */
+  int32_t pt1_half_res;
+  int32_t pt1_denoise;
 } ubg_render_params;
 
 /* Ray accounting of one frame. A pure function of the scene and options:
@@ -228,6 +254,8 @@ typedef struct ubg_ray_stats {
   uint64_t ao_rays_sampled;   /* below-horizon AO samples counted occluded, not traced */
   uint64_t shadow_rays;       /* shadow any-hit queries */
   uint64_t capped_rays;       /* primary rays that reached the transparent-layer cap */
+  uint64_t gi_rays;           /* GI gather / continuation closest-hit queries */
+  uint64_t gi_nee_rays;       /* GI next-event shadow queries */
   /* UBG_FLAG_COUNT_VISITS only, per ray class [0] primary, [1] AO, [2] shadow: */
   uint64_t node_visits[3];    /* inner-node fetches */
   uint64_t prim_tests[3];     /* primitive tests */
@@ -260,6 +288,11 @@ typedef struct ubg_frame {
    * frame is not adaptive. Same memory space as the other outputs. */
   float* aa_mask;
   int64_t aa_flagged;      /* adaptive AA: output pixels refined (0 otherwise) */
+  /* GI AOVs (FrameResult::indirect / giOcclusion), hi_w * hi_h, may be NULL:
+   * the composited indirect rgb (3 floats per pixel) and the first-bounce hit
+   * fraction of the gather; zero on pixels that are not receivers. */
+  float* gi_indirect;
+  float* gi_occlusion;
 } ubg_frame;
 
 /* Acceleration-structure layouts (ubg_scene_create_ex). */

@@ -63,6 +63,10 @@ def load():
     lib.orc_aa_mask.argtypes = [C.c_int32, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                 C.c_float, C.c_float, C.c_int32, C.c_void_p]
     lib.orc_aa_mask.restype = None
+    lib.orc_pt1_gather_point.argtypes = [C.POINTER(capi.SceneDesc), C.POINTER(capi.RenderParams), C.c_void_p,
+                                         C.c_void_p, C.c_void_p, C.c_int32, C.c_uint32, C.c_float, C.c_void_p,
+                                         C.c_void_p, C.c_void_p]
+    lib.orc_pt1_gather_point.restype = C.c_int
     _lib = lib
     return lib
 
@@ -98,12 +102,36 @@ def render(scene: Scene, opt: RenderOptions, hires_only: bool = False, precision
     adaptive = opt.aa_mode == 1 and (ss > 1 or opt.aa_depth > 1)
     aa_mask = np.zeros((opt.height, opt.width), np.float32) if adaptive else None
     fr.aa_mask = aa_mask.ctypes.data if aa_mask is not None else None
+    indirect = np.zeros((hh, hw, 3), np.float32) if opt.gi else None
+    gi_occ = np.zeros((hh, hw), np.float32) if opt.gi else None
+    fr.gi_indirect = indirect.ctypes.data if indirect is not None else None
+    fr.gi_occlusion = gi_occ.ctypes.data if gi_occ is not None else None
     o = _opts(precision, threads, rows, brute)
     capi.check(lib.orc_render(C.byref(desc), C.byref(p), C.byref(fr), C.byref(o)))
     del keep1, keep2
     return FrameResult(width=ow, height=oh, color=color, depth=depth, view_z=view_z, hit_id=hit_id,
                        ao_open=ao, stats=fr.stats.as_dict(), render_seconds=fr.render_seconds,
-                       post_seconds=fr.post_seconds, aa_mask=aa_mask, aa_flagged=int(fr.aa_flagged))
+                       post_seconds=fr.post_seconds, aa_mask=aa_mask, aa_flagged=int(fr.aa_flagged),
+                       indirect=indirect, gi_occlusion=gi_occ)
+
+
+def pt1_gather_point(scene: Scene, opt: RenderOptions, P, N, Ng=None, spp: int = 1024, seed: int = 12345,
+                     eps_t: float = 10.0):
+    """pt1GatherPoint at one shading point (FP32): returns (E rgb, occlusion, gi rays)."""
+    lib = load()
+    desc, keep1 = scene.to_desc()
+    p, keep2 = params(scene, opt)
+    Pa = np.ascontiguousarray(P, np.float32)
+    Na = np.ascontiguousarray(N, np.float32)
+    Ga = np.ascontiguousarray(N if Ng is None else Ng, np.float32)
+    E = np.zeros(3, np.float32)
+    occ = C.c_float()
+    rays = C.c_uint64()
+    capi.check(lib.orc_pt1_gather_point(C.byref(desc), C.byref(p), Pa.ctypes.data, Na.ctypes.data, Ga.ctypes.data,
+                                        int(spp), int(seed) & 0xFFFFFFFF, float(eps_t), E.ctypes.data,
+                                        C.byref(occ), C.byref(rays)))
+    del keep1, keep2
+    return E, float(occ.value), int(rays.value)
 
 
 def aa_mask(rgba, view_z, normal, geom, group, pixel_world_size: float, threshold: float = 0.1,

@@ -917,7 +917,17 @@ struct Light {
 
 struct Counters {
   uint64_t primary = 0, shaded = 0, aoTraced = 0, aoSampled = 0, shadow = 0, capped = 0;
+  uint64_t giRays = 0, giNee = 0;
 };
+
+// scene.hpp:171-196 -- Material::diffuseWeight (POV: diffuse) and toonLike
+inline float diffuseWeight(const Mat& m) { return m.diffuse; }
+inline bool toonLike(const Mat& m) {
+  if (m.model == 2) return true;
+  if (m.model != 0) return false;
+  if (m.specular > 1.0f || m.phong > 1.0f || m.brilliance == 0.0f) return true;
+  return m.diffuse <= 0.0f && m.specular <= 0.0f && m.phong <= 0.0f && m.reflection <= 0.0f;
+}
 
 template <class T>
 struct Ctx {
@@ -932,6 +942,7 @@ struct Ctx {
   // the adaptive-AA fine samples) and the replicated-centre sample multipliers
   int seedW = 0;
   int aoMul = 1, shMul = 1;
+  bool gi = false;  // pt1 GI: receivers drop their constant ambient
 };
 
 // ao_coarse.hpp:54-65 -- the coarse AO grid: gathered openness (and bent
@@ -1267,6 +1278,8 @@ struct HitShade {
   T aoOpen;
   V3<T> normal;   // face-forwarded shading normal (the adaptive-AA mask's)
   uint32_t group; // section group (BuiltScene *Group tables, Mesh::groupForTri)
+  V3<T> giRefl{T(0), T(0), T(0)};  // diffuse reflectance of a GI receiver
+  bool giElig = false;
 };
 
 template <class T>
@@ -1358,7 +1371,17 @@ inline HitShade<T> shadeHit(const Ctx<T>& c, const Hit<T>& h, V3<T> rd, V3<T> or
     const Mat& mat = g.triMat.empty() ? g.meshMats[0] : g.meshMats[g.triMat[baseTri]];
     const V3<T> P{org.x + rd.x * h.t, org.y + rd.y * h.t, org.z + rd.z * h.t};
     const T secEps = selfIntersectEps(P, rd, h.t);
-    const AoShadeT<T> ao = aoShade(c, P, Ng, N, C, secEps, h.t, px, py, cnt);
+    AoShadeT<T> ao = aoShade(c, P, Ng, N, C, secEps, h.t, px, py, cnt);
+    // hit_shader.hpp:229-243: under GI a mesh hit gathers indirect instead of
+    // its constant ambient (toon finishes: unit ambient, no indirect)
+    if (c.gi && toonLike(mat)) {
+      ao.ambLight = V3<T>{T(1), T(1), T(1)};
+    } else if (c.gi) {
+      const T kd = T(diffuseWeight(mat));
+      hs.giRefl = V3<T>{kd * C.x, kd * C.y, kd * C.z};
+      hs.giElig = true;
+      ao.ambLight = V3<T>{T(0), T(0), T(0)};
+    }
     hs.aoOpen = ao.openness;
     hs.color = shadeLocal(c, mat, C, N, V, ao.ambLight, ao.aoFactor, ao.diffuseAo, P, Ng, secEps,
                           shadowsActive, c.p->shadow_samples * c.shMul, px, py, cnt);
@@ -1381,7 +1404,18 @@ inline HitShade<T> shadeHit(const Ctx<T>& c, const Hit<T>& h, V3<T> rd, V3<T> or
   const bool fromEdge = side->fromEdge != 0;
   AoShadeT<T> ao;
   ao.ambLight = c.ambLight;
-  if (!fromEdge) ao = aoShade(c, P, Ng, N, C, secEps, h.t, px, py, cnt);
+  if (!fromEdge) {
+    ao = aoShade(c, P, Ng, N, C, secEps, h.t, px, py, cnt);
+    // hit_shader.hpp:349-368: a real primitive is a GI receiver like the mesh
+    if (c.gi && toonLike(pm)) {
+      ao.ambLight = V3<T>{T(1), T(1), T(1)};
+    } else if (c.gi) {
+      const T kd = T(diffuseWeight(pm));
+      hs.giRefl = V3<T>{kd * C.x, kd * C.y, kd * C.z};
+      hs.giElig = true;
+      ao.ambLight = V3<T>{T(0), T(0), T(0)};
+    }
+  }
   hs.aoOpen = ao.openness;
   const bool primShadows = !fromEdge && shadowsActive;
   hs.color = shadeLocal(c, pm, C, N, V, ao.ambLight, ao.aoFactor, ao.diffuseAo, P, Ng, secEps, primShadows,
@@ -1406,6 +1440,10 @@ struct PixelOut {
   // normal (0 on a miss) and (kind << 16 | group), all ones on a miss
   V3<T> normal{T(0), T(0), T(0)};
   uint32_t gid = 0xFFFFFFFFu;
+  // GI seed of the first hit (transparency.hpp:405-417)
+  V3<T> worldPos{T(0), T(0), T(0)};
+  V3<T> giRefl{T(0), T(0), T(0)};
+  bool giElig = false;
 };
 
 // transparency.hpp:311-522 (grid path; clip t-range per :118-129)
@@ -1434,6 +1472,8 @@ inline PixelOut<T> integratePixel(const Ctx<T>& c, V3<T> org, V3<T> rd, uint32_t
   uint32_t firstId = 0xFFFFFFFFu;
   V3<T> firstN{T(0), T(0), T(0)};
   uint32_t firstGid = 0xFFFFFFFFu;
+  V3<T> firstPos{T(0), T(0), T(0)}, firstRefl{T(0), T(0), T(0)};
+  bool firstElig = false;
   V3<T> base = c.bg;
   T baseCov = p->transparent_background ? T(0) : T(1);
   const T kOpaque = T(1.0f - 1e-4f);
@@ -1452,6 +1492,9 @@ inline PixelOut<T> integratePixel(const Ctx<T>& c, V3<T> org, V3<T> rd, uint32_t
       firstId = (static_cast<uint32_t>(h.kind) << 30) | h.id;
       firstN = hs.normal;
       firstGid = (static_cast<uint32_t>(h.kind) << 16) | hs.group;
+      firstPos = V3<T>{org.x + rd.x * h.t, org.y + rd.y * h.t, org.z + rd.z * h.t};
+      firstRefl = hs.giRefl;
+      firstElig = hs.giElig;
     }
     if (!p->transparency || hs.opacity >= kOpaque) {
       base = hs.color;
@@ -1485,6 +1528,9 @@ inline PixelOut<T> integratePixel(const Ctx<T>& c, V3<T> org, V3<T> rd, uint32_t
   o.hitId = firstId;
   o.normal = firstN;
   o.gid = firstGid;
+  o.worldPos = firstPos;
+  o.giRefl = firstRefl;
+  o.giElig = firstElig;
   return o;
 }
 
@@ -1516,6 +1562,184 @@ void boxDownsample(const float* src, int w, int h, int channels, int ss, float* 
           }
         out[(static_cast<std::size_t>(y) * ow + x) * channels + c] = acc * inv;
       }
+}
+
+// ---------------------------------------------------------------------------
+// pt1 GI gather (integrator/pt1/pt1_gather.hpp), restated.
+
+// irradiance_cache.hpp:176-184
+inline uint32_t hashU32(uint32_t x) {
+  x ^= x >> 16;
+  x *= 0x7FEB352Du;
+  x ^= x >> 15;
+  x *= 0x846CA68Bu;
+  x ^= x >> 16;
+  return x;
+}
+
+// The pt1 gather's fixed inputs (IrradianceCacheParams as runPt1GiPass fills
+// them, embree_renderer.cpp:581-611).
+template <class T>
+struct GiParams {
+  V3<T> sky, ground, up;
+  float envIntensity = 1.0f;
+  int bounces = 1;
+  T maxDist;
+};
+
+template <class T>
+struct Pt1Vertex {
+  V3<T> radiance, albedo, P, N;
+  T tfar;
+};
+
+// pt1_gather.hpp:117-212 (pt1: no emission, the hard NEE ray per light)
+template <class T>
+static bool pt1EvalVertex(const Ctx<T>& c, const Hit<T>& h, V3<T> O, V3<T> wi, Pt1Vertex<T>& v, Counters& cnt) {
+  const Geometry& g = c.s->g;
+  V3<T> Ny, Cy, NgS;
+  T kd;
+  if (h.kind == UBG_KIND_MESH) {
+    const uint32_t* ix = &g.idx[3 * h.id];
+    const T w0 = T(1) - h.u - h.v;
+    T nb[3], cb[3];
+    for (int k = 0; k < 3; ++k) {
+      nb[k] = w0 * T(g.nrm[3 * ix[0] + k]) + h.u * T(g.nrm[3 * ix[1] + k]) + h.v * T(g.nrm[3 * ix[2] + k]);
+      cb[k] = w0 * T(g.col[4 * ix[0] + k]) + h.u * T(g.col[4 * ix[1] + k]) + h.v * T(g.col[4 * ix[2] + k]);
+    }
+    Ny = normalize(V3<T>{nb[0], nb[1], nb[2]});
+    Cy = V3<T>{cb[0], cb[1], cb[2]};
+    const int64_t baseTri = static_cast<int64_t>(h.id % static_cast<uint32_t>(g.baseT));
+    const Mat& mat = g.triMat.empty() ? g.meshMats[0] : g.meshMats[g.triMat[baseTri]];
+    kd = T(diffuseWeight(mat));
+    NgS = h.ng;
+  } else {
+    const PrimSide* side = h.kind == UBG_KIND_SPHERE        ? &g.sphSide[h.id]
+                           : h.kind == UBG_KIND_CONE_CURVE ? &g.cSide[h.id]
+                                                           : &g.rSide[h.id];
+    if (side->fromEdge) return false;
+    Ny = safeNormalize(h.ng);
+    Cy = V3<T>{T(side->color[0]), T(side->color[1]), T(side->color[2])};
+    kd = T(diffuseWeight(g.primMats[side->mat]));
+    NgS = Ny;
+  }
+  if (dot(Ny, wi) > T(0)) Ny = neg(Ny);
+  const V3<T> Py{O.x + wi.x * h.t, O.y + wi.y * h.t, O.z + wi.z * h.t};
+  const T eps = selfIntersectEps(Py, wi, h.t);
+  V3<T> E{T(0), T(0), T(0)};
+  for (const Light<T>& l : c.lights) {
+    const T ndl = dot(Ny, l.L);
+    if (ndl <= T(0)) continue;
+    // computeShadow, one hard ray (secondary_rays.hpp:118-124)
+    V3<T> ng = (dot(NgS, Ny) < T(0)) ? neg(NgS) : NgS;
+    ng = safeNormalize(ng, Ny);
+    const V3<T> So = Py + ng * eps;
+    cnt.giNee++;
+    const T sh = occluded(*c.s, So, l.L, eps, std::numeric_limits<T>::infinity()) ? T(0) : T(1);
+    E.x += ndl * l.color.x * sh;
+    E.y += ndl * l.color.y * sh;
+    E.z += ndl * l.color.z * sh;
+  }
+  v.radiance = V3<T>{kd * Cy.x * E.x, kd * Cy.y * E.y, kd * Cy.z * E.z};
+  v.albedo = V3<T>{kd * Cy.x, kd * Cy.y, kd * Cy.z};
+  v.P = Py;
+  v.N = Ny;
+  v.tfar = h.t;
+  return true;
+}
+
+// pt1_gather.hpp:228-485 (pt1: no Sobol, reservoir, emissive NEE or halves)
+template <class T>
+static V3<T> pt1GatherPoint(const Ctx<T>& c, const GiParams<T>& gp, V3<T> P, V3<T> N, V3<T> Ng, int spp,
+                            uint32_t seed, T epsT, bool ld, float clampLum, T* outOcc, Counters& cnt) {
+  const Frame<T> f = frameFromNormal(N);
+  const T eps = selfIntersectEps(P, N, epsT);
+  const V3<T> O = P + N * eps;
+  const int maxB = gp.bounces < 1 ? 1 : gp.bounces;
+  T cpx = T(0), cpy = T(0);
+  if (ld) {
+    uint32_t c0 = seed, c1 = 0x9E3779B9u;
+    tea2(c0, c1);
+    cpx = u32ToUnorm<T>(c0);
+    cpy = u32ToUnorm<T>(c1);
+  }
+  V3<T> Esum{T(0), T(0), T(0)};
+  int nOcc = 0;
+  for (int si = 0; si < spp; ++si) {
+    uint32_t s0 = seed, s1 = static_cast<uint32_t>(si);
+    tea2(s0, s1);
+    T u1, u2;
+    if (ld) {
+      u1 = T(static_cast<float>(si)) / T(static_cast<float>(spp)) + cpx;
+      u2 = T(static_cast<float>(reverseBits32(static_cast<uint32_t>(si)))) * T(0x1.0p-32f) + cpy;
+      if (u1 >= T(1)) u1 -= T(1);
+      if (u2 >= T(1)) u2 -= T(1);
+    } else {
+      u1 = u32ToUnorm<T>(s0);
+      u2 = u32ToUnorm<T>(s1);
+    }
+    V3<T> wi = cosineSampleHemisphere(u1, u2, f);
+    const bool horizonOk = dot(wi, Ng) > T(0);
+    if (!horizonOk) ++nOcc;
+    V3<T> org = O;
+    T tnear = eps;
+    V3<T> tp{T(1), T(1), T(1)};
+    V3<T> L{T(0), T(0), T(0)};
+    for (int b = 1; horizonOk && b <= maxB; ++b) {
+      cnt.giRays++;
+      const Hit<T> h = intersect(*c.s, org, wi, tnear, gp.maxDist);
+      if (h.kind < 0) {
+        // irradiance_cache.hpp:188-194
+        const T w = T(0.5f) * (dot(wi, gp.up) + T(1));
+        const T e = T(gp.envIntensity);
+        L.x += tp.x * (e * c.ambLight.x * (gp.ground.x + (gp.sky.x - gp.ground.x) * w));
+        L.y += tp.y * (e * c.ambLight.y * (gp.ground.y + (gp.sky.y - gp.ground.y) * w));
+        L.z += tp.z * (e * c.ambLight.z * (gp.ground.z + (gp.sky.z - gp.ground.z) * w));
+        break;
+      }
+      if (b == 1) ++nOcc;
+      Pt1Vertex<T> v;
+      if (!pt1EvalVertex(c, h, org, wi, v, cnt)) break;
+      L.x += tp.x * v.radiance.x;
+      L.y += tp.y * v.radiance.y;
+      L.z += tp.z * v.radiance.z;
+      if (b == maxB) break;
+      tp.x *= v.albedo.x;
+      tp.y *= v.albedo.y;
+      tp.z *= v.albedo.z;
+      if (b >= 2) {
+        tea2(s0, s1);
+        T pc = std::fmax(tp.x, std::fmax(tp.y, tp.z));
+        pc = std::fmin(T(0.95f), std::fmax(T(0.05f), pc));
+        if (u32ToUnorm<T>(s0) >= pc) break;
+        const T inv = T(1) / pc;
+        tp.x *= inv;
+        tp.y *= inv;
+        tp.z *= inv;
+      }
+      tea2(s0, s1);
+      const Frame<T> fv = frameFromNormal(v.N);
+      wi = cosineSampleHemisphere(u32ToUnorm<T>(s0), u32ToUnorm<T>(s1), fv);
+      const T epsV = selfIntersectEps(v.P, wi, v.tfar);
+      org = V3<T>{v.P.x + v.N.x * epsV, v.P.y + v.N.y * epsV, v.P.z + v.N.z * epsV};
+      tnear = epsV;
+    }
+    if (clampLum > 0.0f) {
+      const T lum = T(0.2126f) * L.x + T(0.7152f) * L.y + T(0.0722f) * L.z;
+      if (lum > T(clampLum)) {
+        const T k = T(clampLum) / lum;
+        L.x *= k;
+        L.y *= k;
+        L.z *= k;
+      }
+    }
+    Esum.x += L.x;
+    Esum.y += L.y;
+    Esum.z += L.z;
+  }
+  const T invN = T(1) / T(static_cast<float>(spp > 0 ? spp : 1));
+  if (outOcc) *outOcc = T(static_cast<float>(nOcc)) * invN;
+  return V3<T>{Esum.x * invN, Esum.y * invN, Esum.z * invN};
 }
 
 // ---------------------------------------------------------------------------
@@ -1622,31 +1846,9 @@ static std::vector<uint8_t> buildAaMask(int Wf, int Hf, const AaCenter* c, float
   return out;
 }
 
+// embree_renderer.cpp:436-458 (lights), 1320-1323 (AO / environment up axis)
 template <class T>
-static int renderT(const ubg_scene_desc* desc, const ubg_render_params* p, ubg_frame* fr,
-                   const orc_options* opt) {
-  OScene s;
-  buildScene(desc, opt->brute_force != 0, s);
-  const int ss = std::max(1, p->supersample);
-  const int W = p->width * ss, H = p->height * ss;
-  Ctx<T> c;
-  c.s = &s;
-  c.p = p;
-  c.wHi = W;
-  c.hHi = H;
-  c.seedW = W;
-  // embree_renderer.cpp:375-386 (camera basis)
-  const ubg_camera& cam = p->camera;
-  const T aspect = T(static_cast<float>(W)) / T(static_cast<float>(H));
-  const V3<T> dir = normalize(v3<T>(cam.direction));
-  const V3<T> right = normalize(cross(dir, v3<T>(cam.up)));
-  const V3<T> trueUp = normalize(cross(right, dir));
-  const T halfH = T(cam.height) * T(0.5f);
-  const T halfW = halfH * aspect;
-  const T persHalfH = std::tan(radiansT(T(cam.fovy)) * T(0.5f));
-  const T persHalfW = persHalfH * aspect;
-  c.camDir = dir;
-  // embree_renderer.cpp:436-458 (lights)
+static void initLights(Ctx<T>& c, const ubg_render_params* p, V3<T> trueUp) {
   for (int i = 0; i < p->n_lights; ++i) {
     const ubg_light& dl = p->lights[i];
     Light<T> l;
@@ -1661,6 +1863,45 @@ static int renderT(const ubg_scene_desc* desc, const ubg_render_params* p, ubg_f
   c.ambLight = v3<T>(p->ambient_color);
   c.aoUp = p->ao_use_camera_up ? trueUp : safeNormalize(v3<T>(p->ao_up), trueUp);
   c.bg = v3<T>(p->background);
+}
+
+// GI options the restatement covers: the pt1 integrator over a full-res
+// gather grid without the E denoise (embree_renderer.cpp:716-718)
+static bool giSupported(const ubg_render_params* p, int ss) {
+  if (!p->gi) return true;
+  if (p->gi_integrator != 1 || p->pt1_denoise) return false;
+  int div = p->pt1_gather_div < 0 ? ss : p->pt1_gather_div;
+  if (div == 0) div = p->pt1_half_res ? 2 : 1;
+  return div <= 1;
+}
+
+template <class T>
+static int renderT(const ubg_scene_desc* desc, const ubg_render_params* p, ubg_frame* fr,
+                   const orc_options* opt) {
+  OScene s;
+  buildScene(desc, opt->brute_force != 0, s);
+  const int ss = std::max(1, p->supersample);
+  const int W = p->width * ss, H = p->height * ss;
+  Ctx<T> c;
+  c.s = &s;
+  c.p = p;
+  c.wHi = W;
+  c.hHi = H;
+  c.seedW = W;
+  const bool gi = p->gi != 0;
+  c.gi = gi;
+  // embree_renderer.cpp:375-386 (camera basis)
+  const ubg_camera& cam = p->camera;
+  const T aspect = T(static_cast<float>(W)) / T(static_cast<float>(H));
+  const V3<T> dir = normalize(v3<T>(cam.direction));
+  const V3<T> right = normalize(cross(dir, v3<T>(cam.up)));
+  const V3<T> trueUp = normalize(cross(right, dir));
+  const T halfH = T(cam.height) * T(0.5f);
+  const T halfW = halfH * aspect;
+  const T persHalfH = std::tan(radiansT(T(cam.fovy)) * T(0.5f));
+  const T persHalfW = persHalfH * aspect;
+  c.camDir = dir;
+  initLights(c, p, trueUp);
   const V3<T> pos = v3<T>(cam.position);
 
   // embree_renderer.cpp:527-553 + pipeline.cpp:68: coarse-grid AO pre-pass
@@ -1670,7 +1911,8 @@ static int renderT(const ubg_scene_desc* desc, const ubg_render_params* p, ubg_f
   CoarseAo grid;
   Counters coarseCnt;
   {
-    const int div = p->ao_res_div < 0 ? ss : p->ao_res_div;
+    // (pipeline.cpp:72-80: coarse AO is off under GI)
+    const int div = gi ? 0 : (p->ao_res_div < 0 ? ss : p->ao_res_div);
     const Geometry& g = s.g;
     bool real = !g.idx.empty();
     for (const auto& sd : g.sphSide) real = real || sd.fromEdge == 0;
@@ -1790,7 +2032,22 @@ static int renderT(const ubg_scene_desc* desc, const ubg_render_params* p, ubg_f
     }
     return integratePixel(cx, org, rd, static_cast<uint32_t>(lx), static_cast<uint32_t>(ly), cnt);
   };
+  // GI first-hit G-buffer (embree_renderer.cpp:184-194)
+  std::vector<float> giPos(gi ? npx * 3 : 0), giNrm(gi ? npx * 3 : 0), giRefl(gi ? npx * 3 : 0);
+  std::vector<uint8_t> giElig(gi ? npx : 0);
   auto store = [&](std::size_t pix, const PixelOut<T>& po) {
+    if (gi) {
+      giPos[3 * pix] = static_cast<float>(po.worldPos.x);
+      giPos[3 * pix + 1] = static_cast<float>(po.worldPos.y);
+      giPos[3 * pix + 2] = static_cast<float>(po.worldPos.z);
+      giNrm[3 * pix] = static_cast<float>(po.normal.x);
+      giNrm[3 * pix + 1] = static_cast<float>(po.normal.y);
+      giNrm[3 * pix + 2] = static_cast<float>(po.normal.z);
+      giRefl[3 * pix] = static_cast<float>(po.giRefl.x);
+      giRefl[3 * pix + 1] = static_cast<float>(po.giRefl.y);
+      giRefl[3 * pix + 2] = static_cast<float>(po.giRefl.z);
+      giElig[pix] = po.giElig ? 1 : 0;
+    }
     hiColor[4 * pix + 0] = static_cast<float>(po.r);
     hiColor[4 * pix + 1] = static_cast<float>(po.g);
     hiColor[4 * pix + 2] = static_cast<float>(po.b);
@@ -1800,7 +2057,8 @@ static int renderT(const ubg_scene_desc* desc, const ubg_render_params* p, ubg_f
     hiAo[pix] = static_cast<float>(po.aoOpen);
     hiId[pix] = po.hitId;
   };
-  const bool adaptive = p->aa_mode == 1 && (ss > 1 || p->aa_depth > 1);
+  // (pipeline.cpp:60-65: adaptive AA falls back to the grid under GI)
+  const bool adaptive = p->aa_mode == 1 && (ss > 1 || p->aa_depth > 1) && !gi;
   const auto t0 = std::chrono::steady_clock::now();
   int64_t aaFlagged = 0;
   if (!adaptive) {
@@ -1887,6 +2145,46 @@ static int renderT(const ubg_scene_desc* desc, const ubg_render_params* p, ubg_f
       }
     });
   }
+  // embree_renderer.cpp:1708-1716 -> runPt1GiPass full-res branch
+  // (:959-1049): gather E at every receiver, then color += giIntensity *
+  // reflectance * E, before the fog
+  std::vector<float> giInd(gi ? npx * 3 : 0, 0.0f), giOcc(gi ? npx : 0, 0.0f);
+  if (gi) {
+    GiParams<T> gp;
+    gp.sky = v3<T>(p->pt1_sky_radiance);
+    gp.ground = p->pt1_sky_mode == 1 ? v3<T>(p->ao_ground_color) : gp.sky;
+    gp.up = c.aoUp;
+    gp.envIntensity = p->gi_env_intensity;
+    gp.bounces = std::max(1, p->gi_bounces);
+    gp.maxDist = p->gi_max_distance > 0.0f ? T(p->gi_max_distance) : std::numeric_limits<T>::infinity();
+    const int spp = std::max(1, p->pt1_spp);
+    const uint32_t seedMix = hashU32(p->pt1_seed);
+    std::vector<int> hiRows(H);
+    for (int y = 0; y < H; ++y) hiRows[y] = y;
+    parallelRows(hiRows, [&](int py, Counters& cnt) {
+      for (int px = 0; px < W; ++px) {
+        const std::size_t pix = static_cast<std::size_t>(py) * W + px;
+        if (!giElig[pix]) continue;
+        const V3<T> P = v3<T>(&giPos[3 * pix]);
+        const V3<T> N = v3<T>(&giNrm[3 * pix]);
+        const uint32_t seed = hashU32(static_cast<uint32_t>(pix) ^ seedMix);
+        T occ = T(0);
+        const V3<T> E = pt1GatherPoint(c, gp, P, N, N, spp, seed, T(hiDepth[pix]), p->pt1_ld != 0, p->pt1_clamp,
+                                       &occ, cnt);
+        const float gI = p->gi_intensity;
+        const float ix = static_cast<float>(T(gI) * T(giRefl[3 * pix]) * E.x);
+        const float iy = static_cast<float>(T(gI) * T(giRefl[3 * pix + 1]) * E.y);
+        const float iz = static_cast<float>(T(gI) * T(giRefl[3 * pix + 2]) * E.z);
+        hiColor[4 * pix] += ix;
+        hiColor[4 * pix + 1] += iy;
+        hiColor[4 * pix + 2] += iz;
+        giInd[3 * pix] = ix;
+        giInd[3 * pix + 1] = iy;
+        giInd[3 * pix + 2] = iz;
+        giOcc[pix] = static_cast<float>(occ);
+      }
+    });
+  }
   const auto t1 = std::chrono::steady_clock::now();
   // pipeline.cpp:126-129 -> fog.cpp:17-36 (hi-res, before the downsample)
   if (p->fog.enabled) {
@@ -1913,6 +2211,8 @@ static int renderT(const ubg_scene_desc* desc, const ubg_render_params* p, ubg_f
     tot.aoSampled += k.aoSampled;
     tot.shadow += k.shadow;
     tot.capped += k.capped;
+    tot.giRays += k.giRays;
+    tot.giNee += k.giNee;
   }
   fr->hi_w = W;
   fr->hi_h = H;
@@ -1923,6 +2223,10 @@ static int renderT(const ubg_scene_desc* desc, const ubg_render_params* p, ubg_f
   fr->stats.ao_rays_sampled = tot.aoSampled;
   fr->stats.shadow_rays = tot.shadow;
   fr->stats.capped_rays = tot.capped;
+  fr->stats.gi_rays = tot.giRays;
+  fr->stats.gi_nee_rays = tot.giNee;
+  if (gi && fr->gi_indirect) std::memcpy(fr->gi_indirect, giInd.data(), npx * 3 * sizeof(float));
+  if (gi && fr->gi_occlusion) std::memcpy(fr->gi_occlusion, giOcc.data(), npx * sizeof(float));
   fr->render_seconds = std::chrono::duration<double>(t2 - t0).count();
   (void)t1;
   fr->cancelled = 0;
@@ -2092,6 +2396,7 @@ int orc_render(const ubg_scene_desc* desc, const ubg_render_params* params, ubg_
     if (desc->prim_materials[i].model == 1) return UBG_ERR_UNSUPPORTED;
   for (int32_t i = 0; i < desc->n_mesh_materials; ++i)
     if (desc->mesh_materials[i].model == 1) return UBG_ERR_UNSUPPORTED;
+  if (!orc::giSupported(params, std::max(1, params->supersample))) return UBG_ERR_UNSUPPORTED;
   return o->precision == 64 ? orc::renderT<double>(desc, params, frame, o)
                             : orc::renderT<float>(desc, params, frame, o);
 }
@@ -2123,6 +2428,38 @@ int64_t orc_chain_curves(const ubg_scene_desc* desc, int64_t cap, int32_t* src, 
 }
 
 void orc_tea2(uint32_t* v0, uint32_t* v1) { orc::tea2(*v0, *v1); }
+
+int orc_pt1_gather_point(const ubg_scene_desc* desc, const ubg_render_params* params, const float* P,
+                         const float* N, const float* Ng, int32_t spp, uint32_t seed, float eps_t, float* E,
+                         float* occlusion, uint64_t* gi_rays) {
+  if (!desc || !params || !P || !N || !Ng || !E) return UBG_ERR_INVALID_ARGUMENT;
+  orc::OScene s;
+  orc::buildScene(desc, false, s);
+  orc::Ctx<float> c;
+  c.s = &s;
+  c.p = params;
+  const ubg_camera& cam = params->camera;
+  const orc::V3<float> dir = orc::normalize(orc::v3<float>(cam.direction));
+  const orc::V3<float> right = orc::normalize(orc::cross(dir, orc::v3<float>(cam.up)));
+  orc::initLights(c, params, orc::normalize(orc::cross(right, dir)));
+  orc::GiParams<float> gp;
+  gp.sky = orc::v3<float>(params->pt1_sky_radiance);
+  gp.ground = params->pt1_sky_mode == 1 ? orc::v3<float>(params->ao_ground_color) : gp.sky;
+  gp.up = c.aoUp;
+  gp.envIntensity = params->gi_env_intensity;
+  gp.bounces = std::max(1, params->gi_bounces);
+  gp.maxDist = params->gi_max_distance > 0.0f ? params->gi_max_distance : std::numeric_limits<float>::infinity();
+  orc::Counters cnt;
+  float occ = 0.0f;
+  const orc::V3<float> e = orc::pt1GatherPoint(c, gp, orc::v3<float>(P), orc::v3<float>(N), orc::v3<float>(Ng),
+                                               spp, seed, eps_t, params->pt1_ld != 0, params->pt1_clamp, &occ, cnt);
+  E[0] = e.x;
+  E[1] = e.y;
+  E[2] = e.z;
+  if (occlusion) *occlusion = occ;
+  if (gi_rays) *gi_rays = cnt.giRays;
+  return UBG_OK;
+}
 
 void orc_aa_mask(int32_t wf, int32_t hf, const float* rgba, const float* view_z, const float* normal,
                  const uint32_t* geom, const int32_t* group, float pixel_world_size, float threshold,

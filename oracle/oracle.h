@@ -60,6 +60,15 @@ void orc_aa_mask(int32_t wf, int32_t hf, const float* rgba, const float* view_z,
                  const uint32_t* geom, const int32_t* group, float pixel_world_size, float threshold,
                  int32_t perspective, uint8_t* mask);
 
+/* pt1GatherPoint (integrator/pt1/pt1_gather.hpp:228-485) at one shading
+ * point, FP32, brute force over the scene: the lights, ambient, environment
+ * (pt1_sky_*, gi_env_intensity, ao up axis), gi_bounces, gi_max_distance,
+ * pt1_ld and pt1_clamp come from `params`. E = E_stored (3 floats), occlusion
+ * = first-bounce hit fraction, gi_rays = closest-hit queries (may be NULL). */
+int orc_pt1_gather_point(const ubg_scene_desc* desc, const ubg_render_params* params, const float* P,
+                         const float* N, const float* Ng, int32_t spp, uint32_t seed, float eps_t, float* E,
+                         float* occlusion, uint64_t* gi_rays);
+
 /* Reference TEA-8 (secondary_rays.hpp:51-58), exposed for known-answer tests. */
 void orc_tea2(uint32_t* v0, uint32_t* v1);
 

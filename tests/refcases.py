@@ -889,3 +889,110 @@ def aa_depth_at_ss1():
         assert np.all(v >= lo - 1e-6) and np.all(v <= hi + 1e-6)
     return Case("aa_depth_at_ss1", {"grid": (sc, opts(9, 9)),
                                     "adap": (sc, opts(9, 9, aa_mode=1, aa_depth=2, aa_debug=True))}, chk, tol=1e-6)
+
+
+# ---------------------------------------------------------------- pt1 GI
+def _gi_opts(w, h, **kw):
+    base = dict(gi=True, gi_integrator=1, pt1_half_res=False, pt1_denoise=False, shadows=True)
+    base.update(kw)
+    return RenderOptions(width=w, height=h, **base)
+
+
+def gi_scene(ambient=(0.5, 0.5, 0.5)) -> Scene:
+    """makeGiScene (test_pt1_render.cpp:27-66): a 0.7-grey floor quad and a
+    real red sphere under one angled light, ambient 0.5."""
+    sc = Scene()
+    p = [(-4, 0, -4), (4, 0, -4), (4, 0, 4), (-4, 0, -4), (4, 0, 4), (-4, 0, 4)]
+    sc.set_mesh(p, [(0, 1, 0)] * 6, [(0.7, 0.7, 0.7, 1.0)] * 6, material=Material(ambient=0.2, diffuse=0.8))
+    d = np.array([0.0, -0.35, -1.0])
+    sc.camera = Camera(position=(0, 3, 8), direction=tuple(d / np.linalg.norm(d)), up=(0, 1, 0), fovy=40.0)
+    ld = np.array([-0.4, -1.0, -0.3])
+    sc.lights = [DistantLight(direction=tuple(ld / np.linalg.norm(ld)), color=(1, 1, 1), intensity=0.8)]
+    sc.add_sphere((0, 1, 0), 1.0, (0.8, 0.3, 0.3, 1.0), Material(ambient=0.2, diffuse=0.8))
+    sc.background = (0.1, 0.1, 0.15)
+    sc.ambient_color = ambient
+    return sc
+
+
+@case
+def gi_direct_parity():
+    # test_pt1_render.cpp:143-208: at giIntensity 0 the composite adds +0, so
+    # the frame is the direct shade with the receivers' constant ambient
+    # dropped -- bitwise the plain render with ambient 0; at full strength
+    # colour - indirect is that direct frame and every receiver gets indirect
+    g0 = _gi_opts(48, 36, pt1_spp=4, gi_intensity=0.0)
+    g1 = _gi_opts(48, 36, pt1_spp=4)
+    plain = RenderOptions(width=48, height=36, shadows=True)
+
+    def chk(r):
+        assert np.array_equal(r["gi0"].color, r["amb0"].color)
+        d = np.abs(r["gi1"].color[..., :3] - r["gi1"].indirect - r["gi0"].color[..., :3]).max()
+        assert d <= 1e-5, d
+        hit = r["gi1"].hit_id != 0xFFFFFFFF
+        csg = hit & ((r["gi1"].hit_id >> 30) == 1)
+        assert csg.sum() > 20 and np.all(r["gi1"].indirect[csg].max(axis=1) > 0.0)
+        assert np.all(r["gi1"].indirect[~hit] == 0.0)
+        assert r["gi1"].stats["gi_rays"] == 4 * int(hit.sum())  # one bounce: one query per path
+    return Case("gi_direct_parity", {"gi0": (gi_scene(), g0), "amb0": (gi_scene((0.0, 0.0, 0.0)), plain),
+                                     "gi1": (gi_scene(), g1)}, chk, tol=0.05)
+
+
+def cornell_scene() -> Scene:
+    """cornell::makeScene (test_pt1_render.cpp:85-135): open-top box, white
+    floor and back wall, red left wall, green right wall, lit straight down."""
+    white, red, green = (0.75, 0.75, 0.75, 1.0), (0.75, 0.05, 0.05, 1.0), (0.05, 0.75, 0.05, 1.0)
+    quads = [(((-1, 0, -1), (1, 0, -1), (1, 0, 1), (-1, 0, 1)), (0, 1, 0), white),
+             (((-1, 0, -1), (-1, 2, -1), (1, 2, -1), (1, 0, -1)), (0, 0, 1), white),
+             (((-1, 0, -1), (-1, 0, 1), (-1, 2, 1), (-1, 2, -1)), (1, 0, 0), red),
+             (((1, 0, -1), (1, 2, -1), (1, 2, 1), (1, 0, 1)), (-1, 0, 0), green)]
+    P_, N_, C_ = [], [], []
+    for (a, b, c_, d), n, col in quads:
+        P_ += [a, b, c_, a, c_, d]
+        N_ += [n] * 6
+        C_ += [col] * 6
+    sc = Scene()
+    sc.set_mesh(P_, N_, C_, material=Material(ambient=0.2, diffuse=0.8))
+    sc.camera = Camera(position=(0, 1, 3.2), direction=(0, 0, -1), up=(0, 1, 0), fovy=45.0)
+    sc.lights = [DistantLight(direction=(0, -1, 0), color=(1, 1, 1), intensity=1.0)]
+    sc.background = (0.0, 0.0, 0.0)
+    sc.ambient_color = (0.25, 0.25, 0.25)
+    return sc
+
+
+@case
+def gi_cornell_bleeding():
+    # test_pt1_render.cpp:210-259: the floor next to the red wall picks up red
+    # indirect (R > 1.15 G), mirrored for green; a second bounce raises the
+    # floor-centre indirect
+    sc = cornell_scene()
+
+    def chk(r):
+        ind, ind2 = r["b1"].indirect, r["b2"].indirect
+        red, green, mid = ind[52, 6], ind[52, 57], ind[52, 32]
+        assert red[0] > 1.15 * red[1] and red[0] > 0.0
+        assert green[1] > 1.15 * green[0] and green[1] > 0.0
+        assert np.all(mid > 0.0)
+        assert np.all(ind2[52, 32] > mid)
+    return Case("gi_cornell_bleeding", {"b1": (sc, _gi_opts(64, 64, pt1_spp=128)),
+                                        "b2": (sc, _gi_opts(64, 64, pt1_spp=128, gi_bounces=2))}, chk, tol=0.05)
+
+
+@case
+def gi_isolated_sphere_no_self_occlusion():
+    # test_pt1_render.cpp:261-310: the gather origin's epsilon scales with the
+    # primary-ray distance, so a lone sphere seen from 200 units never
+    # occludes its own hemisphere
+    sc = Scene()
+    sc.add_sphere((0, 0, 0), 1.0, (0.8, 0.3, 0.3, 1.0), Material(ambient=0.2, diffuse=0.8))
+    sc.camera = Camera(position=(0, 0, 200), direction=(0, 0, -1), up=(0, 1, 0), orthographic=True, height=4.0)
+    ld = np.array([-0.4, -0.3, -1.0])
+    sc.lights = [DistantLight(direction=tuple(ld / np.linalg.norm(ld)), color=(1, 1, 1), intensity=0.8)]
+    sc.background = (1.0, 1.0, 1.0)
+    sc.ambient_color = (0.5, 0.5, 0.5)
+
+    def chk(r):
+        occ = r["f"].gi_occlusion
+        assert occ[16, 16] < 0.05
+        assert occ.max() < 0.2
+    return Case("gi_isolated_sphere_no_self_occlusion", {"f": (sc, _gi_opts(33, 33, pt1_spp=16, shadows=False))},
+                chk, tol=0.05)

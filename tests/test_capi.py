@@ -94,8 +94,10 @@ def test_no_device_is_an_error_not_a_fallback():
 
 def test_unsupported_options_raise():
     from umbreon_amd.scene import RenderOptions, Scene, params
-    with pytest.raises(capi.UmbreonError):
-        params(Scene(), RenderOptions(gi=True))
+    # GI combinations beyond the pt1 full-res gather are refused by the
+    # library itself (UBG_ERR_UNSUPPORTED); the parameters pass through
+    p, _ = params(Scene(), RenderOptions(gi=True, gi_integrator=1, pt1_denoise=False, pt1_spp=4))
+    assert (p.gi, p.gi_integrator, p.pt1_spp, p.pt1_denoise) == (1, 1, 4, 0)
     with pytest.raises(capi.UmbreonError):
         params(Scene(), RenderOptions(denoiser=1))
     p, _ = params(Scene(), RenderOptions(aa_mode=1, aa_threshold=0.25, aa_depth=4))

@@ -58,7 +58,9 @@ def test_fp64_oracle_reproduces_fixture(name):
     assert np.array_equal(f.hit_id, z["hit_id"])
     assert np.abs(f.color - z["color"]).max() <= 1e-9
     assert np.abs(f.ao_open - z["ao_open"]).max() <= 1e-12
-    assert f.stats == meta["stats"]
+    # (counters added after the fixture was written, e.g. the GI rays, are 0 here)
+    assert {k: f.stats[k] for k in meta["stats"]} == meta["stats"]
+    assert all(f.stats[k] == 0 for k in f.stats if k not in meta["stats"])
 
 
 @pytest.mark.parametrize("name", NAMES)

@@ -23,7 +23,7 @@ UBG_ERR_UNSUPPORTED = 5
 UBG_ERR_INTERNAL = 6
 
 # UBG_ABI_VERSION of include/umbreon_gpu.h this mirror is written against
-ABI_VERSION = 4
+ABI_VERSION = 5
 
 BVH2 = 0
 BVH8 = 1
@@ -107,6 +107,11 @@ class RenderParams(C.Structure):
         ("ao_up", C.c_float * 3), ("ao_multibounce", C.c_int32), ("ao_low_discrepancy", C.c_int32),
         ("ao_diffuse_factor", C.c_float), ("ao_res_div", C.c_int32),
         ("aa_mode", C.c_int32), ("aa_threshold", C.c_float), ("aa_depth", C.c_int32),
+        ("gi", C.c_int32), ("gi_bounces", C.c_int32), ("gi_max_distance", C.c_float),
+        ("gi_intensity", C.c_float), ("gi_env_intensity", C.c_float), ("pt1_spp", C.c_int32),
+        ("pt1_seed", C.c_uint32), ("pt1_sky_mode", C.c_int32), ("pt1_sky_radiance", C.c_float * 3),
+        ("pt1_ld", C.c_int32), ("pt1_clamp", C.c_float), ("pt1_gather_div", C.c_int32),
+        ("pt1_half_res", C.c_int32), ("pt1_denoise", C.c_int32),
     ]
 
 
@@ -114,6 +119,7 @@ class RayStats(C.Structure):
     _fields_ = [("primary_rays", C.c_uint64), ("shaded_hits", C.c_uint64),
                 ("ao_rays_traced", C.c_uint64), ("ao_rays_sampled", C.c_uint64),
                 ("shadow_rays", C.c_uint64), ("capped_rays", C.c_uint64),
+                ("gi_rays", C.c_uint64), ("gi_nee_rays", C.c_uint64),
                 ("node_visits", C.c_uint64 * 3), ("prim_tests", C.c_uint64 * 3)]
 
     def as_dict(self) -> dict:
@@ -132,6 +138,7 @@ class Frame(C.Structure):
         ("stats", RayStats), ("render_seconds", C.c_double), ("post_seconds", C.c_double),
         ("cancelled", C.c_int32), ("layers", C.c_int32), ("kernel_ms", C.c_double * 4),
         ("kernel_launches", C.c_int32 * 4), ("aa_mask", C.c_void_p), ("aa_flagged", C.c_int64),
+        ("gi_indirect", C.c_void_p), ("gi_occlusion", C.c_void_p),
     ]
 
 

@@ -199,7 +199,8 @@ void checkSupported(const Scene& s, const RenderOptions& o) {
   auto no = [](const char* what) {
     throw std::runtime_error(std::string("umbreon (MI355X): ") + what + " is not supported by the GPU backend");
   };
-  if (o.gi) no("global illumination (gi)");
+  if (o.gi && o.giIntegrator != 1) no("the pt2 / irradiance-cache GI integrators (giIntegrator != 1)");
+  if (o.gi && o.pt1Denoise) no("the pt1 indirect denoise (pt1Denoise)");
   if (o.aoWriteAov) no("AO AOVs (aoWriteAov)");
   if (o.denoiser != 0) no("the final-colour denoiser");
   if (o.envLights > 0) no("environment dome lights (envLights)");
@@ -299,6 +300,20 @@ FrameResult renderPass(const Scene& scene, const RenderOptions& opt, RenderProgr
   p.aa_mode = opt.aaMode;
   p.aa_threshold = opt.aaThreshold;
   p.aa_depth = opt.aaDepth;
+  p.gi = opt.gi ? 1 : 0;
+  p.gi_bounces = opt.giBounces;
+  p.gi_max_distance = opt.giMaxDistance;
+  p.gi_intensity = opt.giIntensity;
+  p.gi_env_intensity = opt.giEnvIntensity;
+  p.pt1_spp = opt.pt1Spp;
+  p.pt1_seed = opt.pt1Seed;
+  p.pt1_sky_mode = opt.pt1SkyMode;
+  for (int k = 0; k < 3; ++k) p.pt1_sky_radiance[k] = opt.pt1SkyRadiance[k];
+  p.pt1_ld = opt.pt1Ld ? 1 : 0;
+  p.pt1_clamp = opt.pt1Clamp;
+  p.pt1_gather_div = opt.pt1GatherDiv;
+  p.pt1_half_res = opt.pt1HalfRes ? 1 : 0;
+  p.pt1_denoise = opt.pt1Denoise ? 1 : 0;
 
   const std::size_t hiN = static_cast<std::size_t>(opt.width) * ss * opt.height * ss;
   f.width = opt.width;
@@ -315,6 +330,13 @@ FrameResult renderPass(const Scene& scene, const RenderOptions& opt, RenderProgr
   if (opt.aaMode == 1 && (ss > 1 || opt.aaDepth > 1) && opt.aaDebug)
     f.aaMask.assign(static_cast<std::size_t>(opt.width) * opt.height, 0.0f);
   fr.aa_mask = f.aaMask.empty() ? nullptr : f.aaMask.data();
+  // GI AOVs (embree_renderer.cpp:463-519 sizes them on the render grid)
+  if (opt.gi) {
+    f.indirect.assign(hiN * 3, 0.0f);
+    f.giOcclusion.assign(hiN, 0.0f);
+  }
+  fr.gi_indirect = f.indirect.empty() ? nullptr : f.indirect.data();
+  fr.gi_occlusion = f.giOcclusion.empty() ? nullptr : f.giOcclusion.data();
   fr.memory = 0;
   if (progress) progress->beginPhase(RenderPhase::Primary);
   st = ubg_render(ctx, gs, &p, &fr);
@@ -331,6 +353,8 @@ FrameResult renderPass(const Scene& scene, const RenderOptions& opt, RenderProgr
                static_cast<unsigned long long>(fr.stats.capped_rays), opt.maxTransparentLayers);
   if (progress) progress->beginPhase(RenderPhase::Postprocess);
   f.renderSeconds = fr.render_seconds;
+  f.pt1Rays.gatherRays = fr.stats.gi_rays;
+  f.pt1Rays.neeRays = fr.stats.gi_nee_rays;
   f.pt1Timing.bvhBuild = info.build_seconds;
   f.pt1Timing.primary = fr.render_seconds;
   return f;

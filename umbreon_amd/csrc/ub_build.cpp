@@ -48,7 +48,13 @@ DevMaterial toDev(const ubg_material& m) {
   d.blinnExp = blinnExponent(m.roughness);
   d.metallic = m.metallic ? 1u : 0u;
   d.roughness = m.roughness;
-  d.pad = 0.0f;
+  // scene.hpp:190-196 (Toon model, overdriven highlight, flat diffuse, or
+  // an unlit flat colour)
+  bool toon = m.model == 2;
+  if (m.model == 0)
+    toon = m.specular > 1.0f || m.phong > 1.0f || m.brilliance == 0.0f ||
+           (m.diffuse <= 0.0f && m.specular <= 0.0f && m.phong <= 0.0f && m.reflection <= 0.0f);
+  d.toon = toon ? 1u : 0u;
   return d;
 }
 

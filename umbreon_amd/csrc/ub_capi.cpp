@@ -66,12 +66,15 @@ struct ubg_context {
   // adaptive AA: hit output indices, the centre / replicated / refined sample
   // sets (32 B per sample), first-hit records, per-output-pixel mask state
   DevBuf hitOut, aaSetC, aaSetR, aaSetF, aaFirstN, aaPix;
+  // pt1 GI: first-hit position / normal / reflectance, gathered E, receiver
+  // list, staging of the indirect and occlusion AOVs
+  DevBuf giPos, giNrm, giRefl, giE, giList, giInd, giOcc;
   unsigned long long* hostCounters = nullptr;  // pinned
   std::vector<DevBuf*> all() {
     return {&hitPix, &hitT,   &hitO,    &hitN,  &hitC,  &hitAo,   &activeA,  &activeB,  &acc,     &tnear,
             &hiColor, &depth, &viewZ,   &hitId, &aoOpen, &outColor, &counters, &query, &queryOut, &aoT,
             &hitOpen, &hitBent, &cGuide, &cOpen, &cBent, &hitOut, &aaSetC, &aaSetR, &aaSetF, &aaFirstN,
-            &aaPix};
+            &aaPix, &giPos, &giNrm, &giRefl, &giE, &giList, &giInd, &giOcc};
   }
 };
 
@@ -441,11 +444,52 @@ int ubg_render(ubg_context* ctx, const ubg_scene* scene, const ubg_render_params
     }
     fc.aoUpAxis[0] = up.x, fc.aoUpAxis[1] = up.y, fc.aoUpAxis[2] = up.z;
   }
+  // pt1 GI (render_options.hpp:93-162): the gather grid must resolve to the
+  // render grid (embree_renderer.cpp:716-718; pipeline.cpp:55 resolves -1 to
+  // the supersample factor) and the E denoise is off
+  const bool gi = p->gi != 0;
+  if (gi) {
+    if (p->gi_integrator != 1)
+      return fail(UBG_ERR_UNSUPPORTED, "ubg_render: only the pt1 GI integrator (gi_integrator 1) is implemented");
+    int div = p->pt1_gather_div < 0 ? ss : p->pt1_gather_div;
+    if (div == 0) div = p->pt1_half_res ? 2 : 1;
+    if (div > 1)
+      return fail(UBG_ERR_UNSUPPORTED, "ubg_render: pt1 GI needs a full-resolution gather grid (pt1_gather_div)");
+    if (p->pt1_denoise)
+      return fail(UBG_ERR_UNSUPPORTED, "ubg_render: the pt1 E denoise is not implemented");
+    if (p->pt1_spp > (1 << 20) || p->gi_bounces > (1 << 16))
+      return fail(UBG_ERR_INVALID_ARGUMENT, "ubg_render: pt1_spp / gi_bounces out of range");
+  }
+  fc.gi = gi ? 1 : 0;
+  fc.giSpp = std::max(1, p->pt1_spp);
+  fc.giBounces = std::max(1, p->gi_bounces);
+  fc.giLd = p->pt1_ld ? 1 : 0;
+  fc.giMaxDist = p->gi_max_distance > 0.0f ? p->gi_max_distance : std::numeric_limits<float>::infinity();
+  fc.giIntensity = p->gi_intensity;
+  fc.giClamp = p->pt1_clamp;
+  {
+    // irradiance_cache.hpp:176-184 of pt1Seed (gatherPt1Grid's seedMix)
+    uint32_t x = p->pt1_seed;
+    x ^= x >> 16;
+    x *= 0x7FEB352Du;
+    x ^= x >> 15;
+    x *= 0x846CA68Bu;
+    x ^= x >> 16;
+    fc.giSeedMix = x;
+  }
+  for (int k = 0; k < 3; ++k) {
+    // embree_renderer.cpp:589-600: sky = pt1SkyRadiance, ground = the sky or
+    // (gradient mode) aoGroundColor; radiance scaled by giEnvIntensity * ambient
+    fc.giEnvScale[k] = p->gi_env_intensity * p->ambient_color[k];
+    fc.giSky[k] = p->pt1_sky_radiance[k];
+    fc.giGround[k] = p->pt1_sky_mode == 1 ? p->ao_ground_color[k] : p->pt1_sky_radiance[k];
+  }
   // coarse-grid AO (pipeline.cpp:68: -1 = the supersample factor;
   // embree_renderer.cpp:531-532: only with AO and something AO can darken)
   {
     const int div = p->ao_res_div < 0 ? ss : p->ao_res_div;
-    fc.coarse = (div > 1 && p->ao_samples > 0 && scene->aoReceivers) ? 1 : 0;
+    // (pipeline.cpp:72-80: off under GI)
+    fc.coarse = (div > 1 && p->ao_samples > 0 && scene->aoReceivers && !gi) ? 1 : 0;
     fc.cW = fc.coarse ? (W + div - 1) / div : 0;
     fc.cH = fc.coarse ? (H + div - 1) / div : 0;
     fc.seedW = W;
@@ -470,7 +514,8 @@ int ubg_render(ubg_context* ctx, const ubg_scene* scene, const ubg_render_params
   }
 
   // ---- adaptive AA plan (embree_renderer.cpp:1379-1385, 1430-1434, 1549, 1597-1602) ----
-  const bool adaptive = p->aa_mode == 1 && (ss > 1 || p->aa_depth > 1);
+  // (pipeline.cpp:60-65: adaptive AA falls back to the grid under GI)
+  const bool adaptive = p->aa_mode == 1 && (ss > 1 || p->aa_depth > 1) && !gi;
   const int kFine = std::max(1, (p->aa_depth + ss - 1) / ss);
   const int64_t nOut = static_cast<int64_t>(p->width) * p->height;
   // stochastic effects a replicated centre is re-evaluated for, with ss^2
@@ -539,6 +584,15 @@ int ubg_render(ubg_context* ctx, const ubg_scene* scene, const ubg_render_params
     need(ctx->aaFirstN, static_cast<size_t>(nOut) * 16);
     need(ctx->aaPix, static_cast<size_t>(nOut) * 18);  // slot, flagged, replicated (u32), mask (f32), raw, mask (u8)
   }
+  if (gi) {
+    need(ctx->giPos, n * 16);
+    need(ctx->giNrm, n * 16);
+    need(ctx->giRefl, n * 16);
+    need(ctx->giE, n * 16);
+    need(ctx->giList, qn * 4);
+    if (fr->gi_indirect) need(ctx->giInd, n * 12);
+    if (fr->gi_occlusion) need(ctx->giOcc, n * 4);
+  }
   if (r != UBG_OK) return r;
 
   ubg::FrameBufs b;
@@ -568,6 +622,11 @@ int ubg_render(ubg_context* ctx, const ubg_scene* scene, const ubg_render_params
   b.hitOut = nullptr;
   b.firstN = nullptr;
   b.smList = nullptr;
+  b.giPos = gi ? static_cast<float4*>(ctx->giPos.p) : nullptr;
+  b.giNrm = gi ? static_cast<float4*>(ctx->giNrm.p) : nullptr;
+  b.giRefl = gi ? static_cast<float4*>(ctx->giRefl.p) : nullptr;
+  b.giE = gi ? static_cast<float4*>(ctx->giE.p) : nullptr;
+  b.giList = gi ? static_cast<uint32_t*>(ctx->giList.p) : nullptr;
 
   fr->cancelled = 0;
   fr->layers = 0;
@@ -786,7 +845,24 @@ int ubg_render(ubg_context* ctx, const ubg_scene* scene, const ubg_render_params
     return UBG_OK;
   };
   fr->aa_flagged = 0;
-  r = adaptive ? renderAdaptive() : runLayers(fc, b, npx);
+  if (adaptive) {
+    r = renderAdaptive();
+  } else if (gi) {
+    // the layers without the fused fog, then the gather and the composite
+    // (which applies the fog after adding the indirect, pipeline.cpp:126-129)
+    ubg::FrameConst fl = fc;
+    fl.fogOn = 0;
+    r = runLayers(fl, b, npx);
+    if (r == UBG_OK && !cancelled) {
+      ubg::launchGiList(fc, b, st);
+      timed(UBG_KERNEL_AO, [&] { ubg::launchGiGather(fc, scene->view, b, countVisits, st); });
+      ubg::launchGiComposite(fc, b, fr->gi_indirect ? static_cast<float*>(ctx->giInd.p) : nullptr,
+                             fr->gi_occlusion ? static_cast<float*>(ctx->giOcc.p) : nullptr, st);
+      UBG_HIP(hipGetLastError(), "kernel launch");
+    }
+  } else {
+    r = runLayers(fc, b, npx);
+  }
   if (r != UBG_OK) return r;
   UBG_HIP(hipEventRecord(ctx->ev[1], st), "hipEventRecord");
   if (cancelled) {
@@ -812,6 +888,8 @@ int ubg_render(ubg_context* ctx, const ubg_scene* scene, const ubg_render_params
   if (r == UBG_OK) r = copyOut(fr->view_z, ctx->viewZ.p, n * 4, fr->memory, st);
   if (r == UBG_OK && fr->hit_id) r = copyOut(fr->hit_id, ctx->hitId.p, n * 4, fr->memory, st);
   if (r == UBG_OK && fr->ao_open) r = copyOut(fr->ao_open, ctx->aoOpen.p, n * 4, fr->memory, st);
+  if (r == UBG_OK && gi && fr->gi_indirect) r = copyOut(fr->gi_indirect, ctx->giInd.p, n * 12, fr->memory, st);
+  if (r == UBG_OK && gi && fr->gi_occlusion) r = copyOut(fr->gi_occlusion, ctx->giOcc.p, n * 4, fr->memory, st);
   if (r != UBG_OK) return r;
   UBG_HIP(hipMemcpyAsync(ctx->hostCounters, b.counters, ubg::kCounterWords * sizeof(unsigned long long),
                          hipMemcpyDeviceToHost, st),
@@ -832,6 +910,8 @@ int ubg_render(ubg_context* ctx, const ubg_scene* scene, const ubg_render_params
   fr->stats.ao_rays_sampled = c[ubg::kCntAoSampled];
   fr->stats.shadow_rays = c[ubg::kCntShadow];
   fr->stats.capped_rays = c[ubg::kCntCapped];
+  fr->stats.gi_rays = c[ubg::kCntGiRays];
+  fr->stats.gi_nee_rays = c[ubg::kCntGiNee];
   for (int k = 0; k < 3; ++k) {
     fr->stats.node_visits[k] = countVisits ? c[ubg::kCntNodes + k] : 0;
     fr->stats.prim_tests[k] = countVisits ? c[ubg::kCntPrimTests + k] : 0;

@@ -46,6 +46,14 @@ struct FrameBufs {
   // on a miss (may be null); the pass's output-pixel list (sample maps)
   float4* firstN;
   const uint32_t* smList;
+  // pt1 GI (hi-res, first hit): position, shading normal + receiver flag (w),
+  // diffuse reflectance; the gathered E + first-bounce hit fraction (w); the
+  // receiver list (partitioned like the hit queue)
+  float4* giPos;
+  float4* giNrm;
+  float4* giRefl;
+  float4* giE;
+  uint32_t* giList;
   unsigned long long* counters;
   int32_t partCap;
 };
@@ -71,6 +79,14 @@ void launchAaMask(const FrameConst& fc, const AaSet& centers, const float4* firs
 void launchAaCompose(const FrameConst& fc, const AaSet& centers, const AaSet& rep, const AaSet& cells,
                      const uint8_t* mask, const uint32_t* slotOf, bool boosted, const FrameBufs& out,
                      hipStream_t st);
+
+// pt1 GI after the layer passes: the receiver list from the first-hit
+// G-buffer, the per-receiver path-traced gather (pt1_gather.hpp:228-612),
+// then the composite of gi_intensity * reflectance * E with the frame's fog
+// (embree_renderer.cpp:1027-1049 + fog.cpp:17-36) and the optional AOVs.
+void launchGiList(const FrameConst& fc, const FrameBufs& b, hipStream_t st);
+void launchGiGather(const FrameConst& fc, const SceneView& s, const FrameBufs& b, bool countVisits, hipStream_t st);
+void launchGiComposite(const FrameConst& fc, const FrameBufs& b, float* indirect, float* occlusion, hipStream_t st);
 
 // Queue slots per partition for a W x H hi-res grid.
 int queuePartCap(int W, int H);

@@ -88,6 +88,8 @@ constexpr uint32_t kFlagMesh = 1u << 24;
 constexpr uint32_t kFlagAo = 1u << 25;
 constexpr uint32_t kFlagShadow = 1u << 26;
 constexpr uint32_t kFlagCoarse = 1u << 27;  // AO openness taken from the coarse grid
+constexpr uint32_t kFlagGiRecv = 1u << 28;  // GI receiver: constant ambient dropped
+constexpr uint32_t kFlagGiToon = 1u << 29;  // toon finish under GI: unit ambient
 
 // This block's statistics group of the counter buffer.
 __device__ __forceinline__ unsigned long long* statGroup(unsigned long long* cnt) {
@@ -334,6 +336,7 @@ __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(kTr
       if (b.hitId) b.hitId[pix] = hit ? h.bits : 0xFFFFFFFFu;
       if (b.aoOpen && !hit) b.aoOpen[pix] = 1.0f;
       if (b.firstN && !hit) b.firstN[pix] = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(0xFFFFFFFFu));
+      if (fc.gi && !hit) b.giNrm[pix] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     }
     if (!hit) {
       const float4 acc = iter == 0 ? make_float4(0.0f, 0.0f, 0.0f, 0.0f) : b.acc[pix];
@@ -426,6 +429,21 @@ __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(kTr
       b.hitOpen[slot] = open;
       if (fc.aoEnh) b.hitBent[slot] = make_float4(bent.x, bent.y, bent.z, 0.0f);
     }
+  }
+  if (fc.gi && !fromEdge) {
+    // hit_shader.hpp:229-243 / :355-368: a mesh or real-primitive hit gathers
+    // indirect instead of its constant ambient; toon finishes keep unit ambient
+    const DevMaterial& m = (flags & kFlagMesh) ? s.meshMats[flags & kInfoMatMask] : s.primMats[flags & kInfoMatMask];
+    const bool recv = m.toon == 0u;
+    flags |= recv ? kFlagGiRecv : kFlagGiToon;
+    if (iter == 0) {
+      // transparency.hpp:402-413 first-hit G-buffer of the gather
+      b.giPos[pix] = make_float4(P.x, P.y, P.z, 0.0f);
+      b.giNrm[pix] = make_float4(N.x, N.y, N.z, recv ? 1.0f : 0.0f);
+      b.giRefl[pix] = make_float4(m.diffuse * C.x, m.diffuse * C.y, m.diffuse * C.z, 0.0f);
+    }
+  } else if (fc.gi && iter == 0) {
+    b.giNrm[pix] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   }
   b.hitPix[slot] = lx + ly * static_cast<uint32_t>(fc.seedW);
   if (b.hitOut) b.hitOut[slot] = pix;
@@ -802,6 +820,8 @@ __global__ __launch_bounds__(kTraceBlock) void k_resolve(FrameConst fc, SceneVie
       }
       aoApply(fc, open, bent, C, aoF, ambL, diffAo);
     }
+    if (flags & kFlagGiRecv) ambL = mk(0.0f, 0.0f, 0.0f);
+    else if (flags & kFlagGiToon) ambL = mk(1.0f, 1.0f, 1.0f);
     if (iter == 0 && b.aoOpen) b.aoOpen[pix] = open;
     // shading.hpp:92-190
     V3 out = mk(m.emission * C.x + aoF.x * m.ambient * C.x * ambL.x,
@@ -1168,6 +1188,233 @@ __global__ __launch_bounds__(kTraceBlock) void k_segments(SceneView s, const Seg
   occ[i] = r;
 }
 
+// ---- pt1 GI (integrator/pt1/pt1_gather.hpp, embree_renderer.cpp:560-1049) ----
+
+// irradiance_cache.hpp:176-184 (32-bit integer hash)
+__device__ __forceinline__ uint32_t hashU32(uint32_t x) {
+  x ^= x >> 16;
+  x *= 0x7FEB352Du;
+  x ^= x >> 15;
+  x *= 0x846CA68Bu;
+  x ^= x >> 16;
+  return x;
+}
+
+// Receivers of the first-hit G-buffer into the partitioned list (one atomic
+// per wave, partition = block % kQueueParts).
+__global__ void k_gi_list(FrameConst fc, FrameBufs b) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool recv = i < fc.W * fc.H && b.giNrm[i].w != 0.0f;
+  const uint32_t slot = queueAppend(b.counters, kGiFill, b.partCap, recv);
+  if (recv) b.giList[slot] = static_cast<uint32_t>(i);
+}
+
+// pt1GatherPoint (pt1_gather.hpp:228-485) for one receiver per lane: spp
+// cosine paths from the first hit, each vertex adding its shadow-tested
+// direct light (pt1EvalVertex, :117-212) times the path throughput, Russian
+// roulette from the second vertex, the environment at an escape; the
+// samples are summed in order, so E is the oracle's to the last bit of the
+// same float operations. E.w = the first-bounce hit fraction.
+template <bool kCount, int kLayout>
+__global__ __launch_bounds__(kTraceBlock) void k_gi_gather(FrameConst fc, SceneView s, FrameBufs b) {
+  const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+  const int part = tid / b.partCap;
+  const bool valid = part < kQueueParts && static_cast<uint32_t>(tid % b.partCap) < queueFill(b.counters, kGiFill, part);
+  uint32_t nGi = 0, nNee = 0, nv = 0, nt = 0;
+  if (valid) {
+    const uint32_t pix = b.giList[tid];
+    const V3 P = ld3(b.giPos[pix]);
+    const V3 N = ld3(b.giNrm[pix]);
+    const V3 Ng = N;  // full-res gather: the render G-buffer has no geometric normal
+    const float epsT = b.depth[pix];
+    const uint32_t seed = hashU32(pix ^ fc.giSeedMix);
+    const Frame3 f = frameFromNormal(N);
+    const float eps = selfIntersectEps(P, N, epsT);
+    const V3 O = P + N * eps;
+    const int spp = fc.giSpp;
+    float cpx = 0.0f, cpy = 0.0f;
+    if (fc.giLd) {
+      uint32_t c0 = seed, c1 = 0x9E3779B9u;
+      tea2(c0, c1);
+      cpx = u32ToUnorm(c0);
+      cpy = u32ToUnorm(c1);
+    }
+    V3 Esum = mk(0.0f, 0.0f, 0.0f);
+    int nOcc = 0;
+    for (int si = 0; si < spp; ++si) {
+      uint32_t s0 = seed, s1 = static_cast<uint32_t>(si);
+      tea2(s0, s1);
+      float u1, u2;
+      if (fc.giLd) {
+        aoSample2d(true, 0u, si, spp, cpx, cpy, u1, u2);
+      } else {
+        u1 = u32ToUnorm(s0);
+        u2 = u32ToUnorm(s1);
+      }
+      V3 wi = cosineSample(u1, u2, f);
+      const bool horizonOk = dot(wi, Ng) > 0.0f;
+      if (!horizonOk) ++nOcc;
+      V3 org = O;
+      float tnear = eps;
+      V3 tp = mk(1.0f, 1.0f, 1.0f);
+      V3 L = mk(0.0f, 0.0f, 0.0f);
+      for (int bo = 1; horizonOk && bo <= fc.giBounces; ++bo) {
+        Hit h;
+        ++nGi;
+        if (!trace<kLayout, false, kCount>(s, org, wi, tnear, fc.giMaxDist, h, &nv, &nt)) {
+          // irradiance_cache.hpp:188-194
+          const float w = 0.5f * (dot(wi, mk(fc.aoUpAxis[0], fc.aoUpAxis[1], fc.aoUpAxis[2])) + 1.0f);
+          L.x += tp.x * (fc.giEnvScale[0] * (fc.giGround[0] + (fc.giSky[0] - fc.giGround[0]) * w));
+          L.y += tp.y * (fc.giEnvScale[1] * (fc.giGround[1] + (fc.giSky[1] - fc.giGround[1]) * w));
+          L.z += tp.z * (fc.giEnvScale[2] * (fc.giGround[2] + (fc.giSky[2] - fc.giGround[2]) * w));
+          break;
+        }
+        if (bo == 1) ++nOcc;
+        // pt1EvalVertex: the hit as a path vertex
+        const uint32_t kind = h.bits >> kKindShift, id = h.bits & kIdMask;
+        V3 Ny, Cy, NgS;
+        float kd;
+        if (kind == 0u) {
+          const uint4 tri = reinterpret_cast<const uint4*>(s.triIdx)[id];
+          const float4* vn = reinterpret_cast<const float4*>(s.vtxNrm);
+          const float4* vc = reinterpret_cast<const float4*>(s.vtxCol);
+          const float4 n0 = vn[tri.x], n1 = vn[tri.y], n2 = vn[tri.z];
+          const float4 c0 = vc[tri.x], c1 = vc[tri.y], c2 = vc[tri.z];
+          const float w0 = 1.0f - h.u - h.v;
+          Ny = normalize(mk(w0 * n0.x + h.u * n1.x + h.v * n2.x, w0 * n0.y + h.u * n1.y + h.v * n2.y,
+                            w0 * n0.z + h.u * n1.z + h.v * n2.z));
+          Cy = mk(w0 * c0.x + h.u * c1.x + h.v * c2.x, w0 * c0.y + h.u * c1.y + h.v * c2.y,
+                  w0 * c0.z + h.u * c1.z + h.v * c2.z);
+          kd = s.meshMats[tri.w & kInfoMatMask].diffuse;
+          NgS = h.ng;
+        } else {
+          float4 col;
+          uint32_t info;
+          if (kind == 1u) {
+            col = reinterpret_cast<const float4*>(s.sphColor)[id];
+            info = s.sphInfo[id];
+          } else if (kind == 3u) {
+            col = reinterpret_cast<const float4*>(s.coneColor)[id];
+            info = s.coneInfo[id];
+          } else {
+            col = reinterpret_cast<const float4*>(s.rndColor)[id];
+            info = s.rndInfo[id];
+          }
+          if (info & kInfoFromEdge) break;  // outline decoration absorbs the path
+          Ny = safeNormalize(h.ng);
+          Cy = mk(col.x, col.y, col.z);
+          kd = s.primMats[info & kInfoMatMask].diffuse;
+          NgS = Ny;
+        }
+        if (dot(Ny, wi) > 0.0f) Ny = neg(Ny);
+        const V3 Py = mk(org.x + wi.x * h.t, org.y + wi.y * h.t, org.z + wi.z * h.t);
+        const float epsY = selfIntersectEps(Py, wi, h.t);
+        V3 E = mk(0.0f, 0.0f, 0.0f);
+        for (int li = 0; li < fc.nLights; ++li) {
+          const DevLight& l = fc.lights[li];
+          const V3 Ld = mk(l.L[0], l.L[1], l.L[2]);
+          const float ndl = dot(Ny, Ld);
+          if (ndl <= 0.0f) continue;
+          // computeShadow's single hard ray (secondary_rays.hpp:118-124)
+          V3 ng = dot(NgS, Ny) < 0.0f ? neg(NgS) : NgS;
+          ng = safeNormalize(ng, Ny);
+          const V3 So = Py + ng * epsY;
+          Hit hs;
+          ++nNee;
+          const float sh = trace<kLayout, true, kCount>(s, So, Ld, epsY, __builtin_inff(), hs, &nv, &nt) ? 0.0f : 1.0f;
+          E.x += ndl * l.color[0] * sh;
+          E.y += ndl * l.color[1] * sh;
+          E.z += ndl * l.color[2] * sh;
+        }
+        L.x += tp.x * (kd * Cy.x * E.x);
+        L.y += tp.y * (kd * Cy.y * E.y);
+        L.z += tp.z * (kd * Cy.z * E.z);
+        if (bo == fc.giBounces) break;
+        tp.x *= kd * Cy.x;
+        tp.y *= kd * Cy.y;
+        tp.z *= kd * Cy.z;
+        if (bo >= 2) {
+          tea2(s0, s1);
+          float pc = fmaxf(tp.x, fmaxf(tp.y, tp.z));
+          pc = fminf(0.95f, fmaxf(0.05f, pc));
+          if (u32ToUnorm(s0) >= pc) break;
+          const float inv = 1.0f / pc;
+          tp.x *= inv;
+          tp.y *= inv;
+          tp.z *= inv;
+        }
+        tea2(s0, s1);
+        const Frame3 fv = frameFromNormal(Ny);
+        wi = cosineSample(u32ToUnorm(s0), u32ToUnorm(s1), fv);
+        const float epsV = selfIntersectEps(Py, wi, h.t);
+        org = mk(Py.x + Ny.x * epsV, Py.y + Ny.y * epsV, Py.z + Ny.z * epsV);
+        tnear = epsV;
+      }
+      if (fc.giClamp > 0.0f) {
+        const float lum = 0.2126f * L.x + 0.7152f * L.y + 0.0722f * L.z;
+        if (lum > fc.giClamp) {
+          const float k = fc.giClamp / lum;
+          L.x *= k;
+          L.y *= k;
+          L.z *= k;
+        }
+      }
+      Esum.x += L.x;
+      Esum.y += L.y;
+      Esum.z += L.z;
+    }
+    const float invN = 1.0f / static_cast<float>(spp);
+    b.giE[pix] = make_float4(Esum.x * invN, Esum.y * invN, Esum.z * invN, static_cast<float>(nOcc) * invN);
+  }
+  {
+    const uint32_t g = waveSum(nGi), ne = waveSum(nNee);
+    if (__lane_id() == 0) {
+      if (g) atomicAdd(&statGroup(b.counters)[kCntGiRays], (unsigned long long)g);
+      if (ne) atomicAdd(&statGroup(b.counters)[kCntGiNee], (unsigned long long)ne);
+    }
+  }
+  if (kCount) addVisits(b.counters, 1, nv, nt);
+}
+
+// embree_renderer.cpp:1027-1049 (color += giIntensity * reflectance * E on
+// receivers) then fog.cpp:17-36 over every hi-res pixel (the layer passes
+// ran without the fused fog).
+__global__ void k_gi_composite(FrameConst fc, FrameBufs b, float* __restrict__ indirect,
+                               float* __restrict__ occlusion) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= fc.W * fc.H) return;
+  float4 c = b.hiColor[i];
+  V3 ind = mk(0.0f, 0.0f, 0.0f);
+  float occ = 0.0f;
+  if (b.giNrm[i].w != 0.0f) {
+    const float4 E = b.giE[i];
+    const float4 R = b.giRefl[i];
+    ind = mk(fc.giIntensity * R.x * E.x, fc.giIntensity * R.y * E.y, fc.giIntensity * R.z * E.z);
+    occ = E.w;
+    c.x += ind.x;
+    c.y += ind.y;
+    c.z += ind.z;
+  }
+  const float vz = b.viewZ[i];
+  if (fc.fogOn && vz > 0.0f) {
+    const float f = fogFactor(fc, vz);
+    if (fc.transparentBg) {
+      c.w *= f;
+    } else {
+      c.x = fc.fogColor[0] * (1.0f - f) + c.x * f;
+      c.y = fc.fogColor[1] * (1.0f - f) + c.y * f;
+      c.z = fc.fogColor[2] * (1.0f - f) + c.z * f;
+    }
+  }
+  b.hiColor[i] = c;
+  if (indirect) {
+    indirect[3 * static_cast<int64_t>(i) + 0] = ind.x;
+    indirect[3 * static_cast<int64_t>(i) + 1] = ind.y;
+    indirect[3 * static_cast<int64_t>(i) + 2] = ind.z;
+  }
+  if (occlusion) occlusion[i] = occ;
+}
+
 // ---- adaptive antialiasing (render/adaptive_aa.hpp, embree_renderer.cpp:1428-1674) ----
 
 // One output pixel's centre result as the refinement predicates see it.
@@ -1462,6 +1709,19 @@ void launchResolve(const FrameConst& fc, const SceneView& s, const FrameBufs& b,
                    hipStream_t st) {
   const int64_t slots = static_cast<int64_t>(kQueueParts) * b.partCap;
   UBG_DISPATCH(k_resolve, blocksFor(slots, kTraceBlock), countVisits, fc, s, b, iter, 0);
+}
+void launchGiList(const FrameConst& fc, const FrameBufs& b, hipStream_t st) {
+  // the same block -> partition map as a first primary launch over W x H
+  const int64_t n = static_cast<int64_t>(fc.W) * fc.H;
+  hipLaunchKernelGGL(k_gi_list, dim3(blocksFor(n, kTraceBlock)), dim3(kTraceBlock), 0, st, fc, b);
+}
+void launchGiGather(const FrameConst& fc, const SceneView& s, const FrameBufs& b, bool countVisits, hipStream_t st) {
+  const int64_t slots = static_cast<int64_t>(kQueueParts) * b.partCap;
+  UBG_DISPATCH(k_gi_gather, blocksFor(slots, kTraceBlock), countVisits, fc, s, b);
+}
+void launchGiComposite(const FrameConst& fc, const FrameBufs& b, float* indirect, float* occlusion, hipStream_t st) {
+  const int64_t n = static_cast<int64_t>(fc.W) * fc.H;
+  hipLaunchKernelGGL(k_gi_composite, dim3(blocksFor(n, 256)), dim3(256), 0, st, fc, b, indirect, occlusion);
 }
 void launchAaMask(const FrameConst& fc, const AaSet& centers, const float4* firstN, float pixelWorldSize,
                   float threshold, uint8_t* raw, uint8_t* mask, uint32_t* slotOf, uint32_t* flagList,

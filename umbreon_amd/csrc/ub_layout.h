@@ -46,7 +46,8 @@ struct DevMaterial {
   float phong, phongSize, reflection, emission;
   float blinnExp;
   uint32_t metallic;
-  float roughness, pad;
+  float roughness;
+  uint32_t toon;  // Material::toonLike() (scene.hpp:190-196)
 };
 
 // One BVH8 node as the builders write it (80 bytes; the traversal reads it
@@ -151,6 +152,14 @@ struct FrameConst {
   int32_t smSs, smK, smCSub, smWf, smHf, smPer;
   uint32_t smBase;         // first sample of this chunk of the pass
   uint32_t nList;          // samples in this chunk (list modes)
+  // pt1 GI (gi != 0): gather paths per pixel, path length, ray reach, seed
+  // mix hashU32(pt1Seed), stratified first bounce, luminance clamp, and the
+  // environment: radiance = envScale * (ground + (sky - ground) * w) with
+  // w = 0.5 (dot(wi, aoUpAxis) + 1), envScale = giEnvIntensity * ambient
+  int32_t gi, giSpp, giBounces, giLd;
+  float giMaxDist, giIntensity, giClamp;
+  uint32_t giSeedMix;
+  float giEnvScale[3], giSky[3], giGround[3];
   DevLight lights[kMaxLights];
 };
 
@@ -190,6 +199,8 @@ enum Counter {
   kCntAoSampled = 3,
   kCntShadow = 4,
   kCntCapped = 5,
+  kCntGiRays = 6,     // GI gather / continuation closest-hit queries
+  kCntGiNee = 7,      // GI next-event shadow queries
   kCntNodes = 8,      // node visits per ray class: 8 primary, 9 AO, 10 shadow
   kCntPrimTests = 11, // primitive tests per ray class: 11, 12, 13
   kNumCounters = 16
@@ -202,6 +213,9 @@ constexpr int kAoCursor = kActiveFill + kLineWords * kQueueParts;
 // adaptive AA: fills of the flagged and the replicated output-pixel lists
 constexpr int kAaFlagFill = kAoCursor + kLineWords * kQueueParts;
 constexpr int kAaRepFill = kAaFlagFill + kLineWords;
-constexpr int kCounterWords = kAaRepFill + kLineWords;
+// GI receiver list (hi-res pixels whose first hit gathers), partitioned like
+// the hit queue
+constexpr int kGiFill = kAaRepFill + kLineWords;
+constexpr int kCounterWords = kGiFill + kLineWords * kQueueParts;
 
 }  // namespace ubg

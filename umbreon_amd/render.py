@@ -36,6 +36,8 @@ class FrameResult:
     kernel_launches: dict = field(default_factory=dict)
     aa_mask: Optional[np.ndarray] = None  # (H, W) 1 = refined (adaptive AA with aa_debug)
     aa_flagged: int = 0
+    indirect: Optional[np.ndarray] = None      # (hiH, hiW, 3) GI indirect added to colour
+    gi_occlusion: Optional[np.ndarray] = None  # (hiH, hiW) first-bounce hit fraction
 
     @property
     def total_rays(self) -> int:
@@ -147,6 +149,10 @@ class BuiltScene:
         adaptive = opt.aa_mode == 1 and (ss > 1 or opt.aa_depth > 1)
         aa_mask = np.zeros((opt.height, opt.width), np.float32) if opt.aa_debug and adaptive else None
         fr.aa_mask = aa_mask.ctypes.data if aa_mask is not None else None
+        indirect = np.zeros((hh, hw, 3), np.float32) if opt.gi and outputs else None
+        gi_occ = np.zeros((hh, hw), np.float32) if opt.gi and outputs else None
+        fr.gi_indirect = indirect.ctypes.data if indirect is not None else None
+        fr.gi_occlusion = gi_occ.ctypes.data if gi_occ is not None else None
         fr.memory = 0
         capi.check(self._lib.ubg_render(self.device.handle, self.handle, C.byref(p), C.byref(fr)))
         del keep
@@ -156,7 +162,8 @@ class BuiltScene:
                            cancelled=bool(fr.cancelled), layers=int(fr.layers),
                            kernel_ms={k: float(fr.kernel_ms[i]) for i, k in enumerate(capi.KERNELS)},
                            kernel_launches={k: int(fr.kernel_launches[i]) for i, k in enumerate(capi.KERNELS)},
-                           aa_mask=aa_mask, aa_flagged=int(fr.aa_flagged))
+                           aa_mask=aa_mask, aa_flagged=int(fr.aa_flagged), indirect=indirect,
+                           gi_occlusion=gi_occ)
 
 
 def host_query(scene: Scene, rays: np.ndarray, layout: int = capi.BVH8):

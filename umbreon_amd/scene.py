@@ -115,8 +115,24 @@ class RenderOptions:
     aa_threshold: float = 0.1
     aa_depth: int = 0
     aa_debug: bool = False  # fill FrameResult.aa_mask
-    # options the gfx950 path does not implement; must stay at their defaults
+    # path-traced GI (render_options.hpp:93-162); the gfx950 path implements
+    # the pt1 integrator (gi_integrator 1) over a full-res gather grid with
+    # the E denoise off, and refuses other combinations
     gi: bool = False
+    gi_bounces: int = 1
+    gi_max_distance: float = 0.0
+    gi_intensity: float = 1.0
+    gi_env_intensity: float = 1.0
+    pt1_spp: int = 8
+    pt1_seed: int = 0
+    pt1_sky_mode: int = 0
+    pt1_sky_radiance: Tuple[float, float, float] = (1.0, 1.0, 1.0)
+    pt1_ld: bool = True
+    pt1_clamp: float = 0.0
+    pt1_gather_div: int = -1
+    pt1_half_res: bool = True
+    pt1_denoise: bool = True
+    # options the gfx950 path does not implement; must stay at their defaults
     denoiser: int = 0
     env_lights: int = 0
 
@@ -314,9 +330,9 @@ def u16p():
 def params(scene: Scene, opt: RenderOptions, flags: int = 0,
            cancel_flag: Optional[C.c_int32] = None) -> Tuple[capi.RenderParams, list]:
     """RenderOptions + Scene frame constants -> ubg_render_params."""
-    if opt.gi or opt.denoiser != 0 or opt.env_lights > 0:
+    if opt.denoiser != 0 or opt.env_lights > 0:
         raise capi.UmbreonError(capi.UBG_ERR_UNSUPPORTED,
-                                "GI / denoise / env lights are not part of the gfx950 path")
+                                "the final-colour denoiser / env lights are not part of the gfx950 path")
     keep: list = []
     p = capi.RenderParams()
     p.width, p.height, p.supersample = int(opt.width), int(opt.height), int(opt.supersample)
@@ -365,6 +381,20 @@ def params(scene: Scene, opt: RenderOptions, flags: int = 0,
     p.aa_mode = int(opt.aa_mode)
     p.aa_threshold = float(opt.aa_threshold)
     p.aa_depth = int(opt.aa_depth)
+    p.gi = int(bool(opt.gi))
+    p.gi_bounces = int(opt.gi_bounces)
+    p.gi_max_distance = float(opt.gi_max_distance)
+    p.gi_intensity = float(opt.gi_intensity)
+    p.gi_env_intensity = float(opt.gi_env_intensity)
+    p.pt1_spp = int(opt.pt1_spp)
+    p.pt1_seed = int(opt.pt1_seed) & 0xFFFFFFFF
+    p.pt1_sky_mode = int(opt.pt1_sky_mode)
+    p.pt1_sky_radiance[:] = opt.pt1_sky_radiance
+    p.pt1_ld = int(bool(opt.pt1_ld))
+    p.pt1_clamp = float(opt.pt1_clamp)
+    p.pt1_gather_div = int(opt.pt1_gather_div)
+    p.pt1_half_res = int(bool(opt.pt1_half_res))
+    p.pt1_denoise = int(bool(opt.pt1_denoise))
     p.flags = flags
     if cancel_flag is not None:
         keep.append(cancel_flag)
