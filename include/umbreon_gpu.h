@@ -242,7 +242,10 @@ typedef struct ubg_render_params {
   float pt1_clamp;                   /* per-sample luminance clamp; 0 = off */
   int32_t pt1_gather_div;            /* -1 = supersample factor, 0 = pt1_half_res rule, k */
   int32_t pt1_half_res;
-  int32_t pt1_denoise;
+  int32_t pt1_denoise;               /* E denoise: the edge-avoiding a-trous filter (the
+                                        reference's fallback when OIDN is not built) */
+  int32_t denoise_iters;             /* a-trous iterations (reference default 5) */
+  float denoise_sigma_z, denoise_sigma_n, denoise_sigma_l;  /* defaults 1, 128, 4 */
 } ubg_render_params;
 
 /* Ray accounting of one frame. A pure function of the scene and options:

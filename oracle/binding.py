@@ -67,6 +67,9 @@ def load():
                                          C.c_void_p, C.c_void_p, C.c_int32, C.c_uint32, C.c_float, C.c_void_p,
                                          C.c_void_p, C.c_void_p]
     lib.orc_pt1_gather_point.restype = C.c_int
+    lib.orc_pt1_denoise.argtypes = [C.c_int32, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_float,
+                                    C.c_float, C.c_float]
+    lib.orc_pt1_denoise.restype = None
     _lib = lib
     return lib
 
@@ -230,3 +233,17 @@ def srgb_encode(v: float) -> float:
 
 def srgb_decode(s: float) -> float:
     return float(load().orc_srgb_decode(s))
+
+
+def pt1_denoise(E, normal, position, iters: int = 5, sigma_z: float = 1.0, sigma_n: float = 128.0,
+                sigma_l: float = 4.0) -> np.ndarray:
+    """denoisePt1E's a-trous path on (H, W, 3) buffers; returns the filtered E."""
+    lib = load()
+    E = np.array(E, np.float32, order="C", copy=True)
+    h, w = E.shape[:2]
+    nr = np.ascontiguousarray(normal, np.float32)
+    ps = np.ascontiguousarray(position, np.float32)
+    assert nr.shape == E.shape and ps.shape == E.shape
+    lib.orc_pt1_denoise(w, h, E.ctypes.data, nr.ctypes.data, ps.ctypes.data, int(iters), float(sigma_z),
+                        float(sigma_n), float(sigma_l))
+    return E

@@ -1280,6 +1280,8 @@ struct HitShade {
   uint32_t group; // section group (BuiltScene *Group tables, Mesh::groupForTri)
   V3<T> giRefl{T(0), T(0), T(0)};  // diffuse reflectance of a GI receiver
   bool giElig = false;
+  // the normal AOV under GI (hit_shader.hpp:214-220 mesh, :364 receivers)
+  V3<T> giGuideN{T(0), T(0), T(0)};
 };
 
 template <class T>
@@ -1374,6 +1376,7 @@ inline HitShade<T> shadeHit(const Ctx<T>& c, const Hit<T>& h, V3<T> rd, V3<T> or
     AoShadeT<T> ao = aoShade(c, P, Ng, N, C, secEps, h.t, px, py, cnt);
     // hit_shader.hpp:229-243: under GI a mesh hit gathers indirect instead of
     // its constant ambient (toon finishes: unit ambient, no indirect)
+    if (c.gi) hs.giGuideN = N;
     if (c.gi && toonLike(mat)) {
       ao.ambLight = V3<T>{T(1), T(1), T(1)};
     } else if (c.gi) {
@@ -1413,6 +1416,7 @@ inline HitShade<T> shadeHit(const Ctx<T>& c, const Hit<T>& h, V3<T> rd, V3<T> or
       const T kd = T(diffuseWeight(pm));
       hs.giRefl = V3<T>{kd * C.x, kd * C.y, kd * C.z};
       hs.giElig = true;
+      hs.giGuideN = N;
       ao.ambLight = V3<T>{T(0), T(0), T(0)};
     }
   }
@@ -1444,6 +1448,7 @@ struct PixelOut {
   V3<T> worldPos{T(0), T(0), T(0)};
   V3<T> giRefl{T(0), T(0), T(0)};
   bool giElig = false;
+  V3<T> giGuideN{T(0), T(0), T(0)};
 };
 
 // transparency.hpp:311-522 (grid path; clip t-range per :118-129)
@@ -1472,7 +1477,7 @@ inline PixelOut<T> integratePixel(const Ctx<T>& c, V3<T> org, V3<T> rd, uint32_t
   uint32_t firstId = 0xFFFFFFFFu;
   V3<T> firstN{T(0), T(0), T(0)};
   uint32_t firstGid = 0xFFFFFFFFu;
-  V3<T> firstPos{T(0), T(0), T(0)}, firstRefl{T(0), T(0), T(0)};
+  V3<T> firstPos{T(0), T(0), T(0)}, firstRefl{T(0), T(0), T(0)}, firstGuideN{T(0), T(0), T(0)};
   bool firstElig = false;
   V3<T> base = c.bg;
   T baseCov = p->transparent_background ? T(0) : T(1);
@@ -1495,6 +1500,7 @@ inline PixelOut<T> integratePixel(const Ctx<T>& c, V3<T> org, V3<T> rd, uint32_t
       firstPos = V3<T>{org.x + rd.x * h.t, org.y + rd.y * h.t, org.z + rd.z * h.t};
       firstRefl = hs.giRefl;
       firstElig = hs.giElig;
+      firstGuideN = hs.giGuideN;
     }
     if (!p->transparency || hs.opacity >= kOpaque) {
       base = hs.color;
@@ -1531,6 +1537,7 @@ inline PixelOut<T> integratePixel(const Ctx<T>& c, V3<T> org, V3<T> rd, uint32_t
   o.worldPos = firstPos;
   o.giRefl = firstRefl;
   o.giElig = firstElig;
+  o.giGuideN = firstGuideN;
   return o;
 }
 
@@ -1865,11 +1872,109 @@ static void initLights(Ctx<T>& c, const ubg_render_params* p, V3<T> trueUp) {
   c.bg = v3<T>(p->background);
 }
 
+// denoise_atrous.cpp (denoiseAtrous) on a 3-channel buffer without albedo
+// demodulation, as denoisePt1E runs it: edge-avoiding a-trous (B3 spline,
+// 2^i holes) with normal^sigmaN, exp(-|dp| / (sigmaZ grad step)) and
+// exp(-|dl| / (sigmaL sqrt(var))) edge stops; pixels whose guide normal is
+// shorter than 0.5 neither change nor contribute.
+static void denoiseAtrous(int W, int H, std::vector<float>& E, const std::vector<float>& nrm,
+                          const std::vector<float>& pos, int itersIn, float sigmaZIn, float sigmaN, float sigmaLIn) {
+  const std::size_t N = static_cast<std::size_t>(W) * H;
+  const int iters = std::max(0, itersIn);
+  if (W <= 0 || H <= 0 || iters == 0) return;
+  const float kEps = 1.0e-3f;
+  const float sigmaZ = std::max(1.0e-4f, sigmaZIn);
+  const float sigmaL = std::max(1.0e-4f, sigmaLIn);
+  auto lumOf = [](const float* v) { return 0.2126f * v[0] + 0.7152f * v[1] + 0.0722f * v[2]; };
+  auto dist = [&](std::size_t a, std::size_t b) {
+    const float dx = pos[3 * a] - pos[3 * b], dy = pos[3 * a + 1] - pos[3 * b + 1], dz = pos[3 * a + 2] - pos[3 * b + 2];
+    return std::sqrt(dx * dx + dy * dy + dz * dz);
+  };
+  std::vector<uint8_t> valid(N);
+  for (std::size_t p = 0; p < N; ++p) {
+    const float nx = nrm[3 * p], ny = nrm[3 * p + 1], nz = nrm[3 * p + 2];
+    valid[p] = (nx * nx + ny * ny + nz * nz) > 0.25f;
+  }
+  std::vector<float> irr = E;
+  std::vector<float> grad(N, kEps), var(N, 0.0f);
+  for (int y = 0; y < H; ++y)
+    for (int x = 0; x < W; ++x) {
+      const std::size_t p = static_cast<std::size_t>(y) * W + x;
+      if (!valid[p]) continue;
+      float g = kEps;
+      const int nb[4][2] = {{1, 0}, {-1, 0}, {0, 1}, {0, -1}};
+      for (auto& d : nb) {
+        const int qx = x + d[0], qy = y + d[1];
+        if (qx < 0 || qx >= W || qy < 0 || qy >= H) continue;
+        const std::size_t q = static_cast<std::size_t>(qy) * W + qx;
+        if (valid[q]) g = std::max(g, dist(p, q));
+      }
+      grad[p] = g;
+      float s1 = 0.0f, s2 = 0.0f;
+      int n = 0;
+      for (int dy = -1; dy <= 1; ++dy)
+        for (int dx = -1; dx <= 1; ++dx) {
+          const int qx = x + dx, qy = y + dy;
+          if (qx < 0 || qx >= W || qy < 0 || qy >= H) continue;
+          const std::size_t q = static_cast<std::size_t>(qy) * W + qx;
+          if (!valid[q]) continue;
+          const float l = lumOf(&irr[3 * q]);
+          s1 += l;
+          s2 += l * l;
+          ++n;
+        }
+      if (n > 0) {
+        const float mean = s1 / n;
+        var[p] = std::max(0.0f, s2 / n - mean * mean);
+      }
+    }
+  const float h5[5] = {0.0625f, 0.25f, 0.375f, 0.25f, 0.0625f};
+  std::vector<float> next(N * 3);
+  for (int it = 0; it < iters; ++it) {
+    const int step = 1 << it;
+    for (int y = 0; y < H; ++y)
+      for (int x = 0; x < W; ++x) {
+        const std::size_t p = static_cast<std::size_t>(y) * W + x;
+        if (!valid[p]) {
+          for (int k = 0; k < 3; ++k) next[3 * p + k] = irr[3 * p + k];
+          continue;
+        }
+        const float lp = lumOf(&irr[3 * p]);
+        const float lDenom = sigmaL * std::sqrt(var[p]) + kEps;
+        const float zDenom = sigmaZ * grad[p] * static_cast<float>(step) + kEps;
+        float sum[3] = {0.0f, 0.0f, 0.0f}, wsum = 0.0f;
+        for (int dy = -2; dy <= 2; ++dy) {
+          const int qy = y + dy * step;
+          if (qy < 0 || qy >= H) continue;
+          for (int dx = -2; dx <= 2; ++dx) {
+            const int qx = x + dx * step;
+            if (qx < 0 || qx >= W) continue;
+            const std::size_t q = static_cast<std::size_t>(qy) * W + qx;
+            if (!valid[q]) continue;
+            float w = h5[dx + 2] * h5[dy + 2];
+            const float nd = std::max(0.0f, nrm[3 * p] * nrm[3 * q] + nrm[3 * p + 1] * nrm[3 * q + 1] +
+                                                nrm[3 * p + 2] * nrm[3 * q + 2]);
+            w *= std::pow(nd, sigmaN);
+            w *= std::exp(-dist(p, q) / zDenom);
+            w *= std::exp(-std::fabs(lp - lumOf(&irr[3 * q])) / lDenom);
+            for (int k = 0; k < 3; ++k) sum[k] += w * irr[3 * q + k];
+            wsum += w;
+          }
+        }
+        for (int k = 0; k < 3; ++k) next[3 * p + k] = wsum > 0.0f ? sum[k] / wsum : irr[3 * p + k];
+      }
+    irr.swap(next);
+  }
+  for (std::size_t p = 0; p < N; ++p)
+    if (valid[p])
+      for (int k = 0; k < 3; ++k) E[3 * p + k] = std::max(0.0f, irr[3 * p + k]);
+}
+
 // GI options the restatement covers: the pt1 integrator over a full-res
-// gather grid without the E denoise (embree_renderer.cpp:716-718)
+// gather grid (embree_renderer.cpp:716-718)
 static bool giSupported(const ubg_render_params* p, int ss) {
   if (!p->gi) return true;
-  if (p->gi_integrator != 1 || p->pt1_denoise) return false;
+  if (p->gi_integrator != 1) return false;
   int div = p->pt1_gather_div < 0 ? ss : p->pt1_gather_div;
   if (div == 0) div = p->pt1_half_res ? 2 : 1;
   return div <= 1;
@@ -2040,9 +2145,9 @@ static int renderT(const ubg_scene_desc* desc, const ubg_render_params* p, ubg_f
       giPos[3 * pix] = static_cast<float>(po.worldPos.x);
       giPos[3 * pix + 1] = static_cast<float>(po.worldPos.y);
       giPos[3 * pix + 2] = static_cast<float>(po.worldPos.z);
-      giNrm[3 * pix] = static_cast<float>(po.normal.x);
-      giNrm[3 * pix + 1] = static_cast<float>(po.normal.y);
-      giNrm[3 * pix + 2] = static_cast<float>(po.normal.z);
+      giNrm[3 * pix] = static_cast<float>(po.giGuideN.x);
+      giNrm[3 * pix + 1] = static_cast<float>(po.giGuideN.y);
+      giNrm[3 * pix + 2] = static_cast<float>(po.giGuideN.z);
       giRefl[3 * pix] = static_cast<float>(po.giRefl.x);
       giRefl[3 * pix + 1] = static_cast<float>(po.giRefl.y);
       giRefl[3 * pix + 2] = static_cast<float>(po.giRefl.z);
@@ -2161,6 +2266,7 @@ static int renderT(const ubg_scene_desc* desc, const ubg_render_params* p, ubg_f
     const uint32_t seedMix = hashU32(p->pt1_seed);
     std::vector<int> hiRows(H);
     for (int y = 0; y < H; ++y) hiRows[y] = y;
+    std::vector<float> Ebuf(npx * 3, 0.0f);
     parallelRows(hiRows, [&](int py, Counters& cnt) {
       for (int px = 0; px < W; ++px) {
         const std::size_t pix = static_cast<std::size_t>(py) * W + px;
@@ -2171,19 +2277,29 @@ static int renderT(const ubg_scene_desc* desc, const ubg_render_params* p, ubg_f
         T occ = T(0);
         const V3<T> E = pt1GatherPoint(c, gp, P, N, N, spp, seed, T(hiDepth[pix]), p->pt1_ld != 0, p->pt1_clamp,
                                        &occ, cnt);
-        const float gI = p->gi_intensity;
-        const float ix = static_cast<float>(T(gI) * T(giRefl[3 * pix]) * E.x);
-        const float iy = static_cast<float>(T(gI) * T(giRefl[3 * pix + 1]) * E.y);
-        const float iz = static_cast<float>(T(gI) * T(giRefl[3 * pix + 2]) * E.z);
-        hiColor[4 * pix] += ix;
-        hiColor[4 * pix + 1] += iy;
-        hiColor[4 * pix + 2] += iz;
-        giInd[3 * pix] = ix;
-        giInd[3 * pix + 1] = iy;
-        giInd[3 * pix + 2] = iz;
+        Ebuf[3 * pix] = static_cast<float>(E.x);
+        Ebuf[3 * pix + 1] = static_cast<float>(E.y);
+        Ebuf[3 * pix + 2] = static_cast<float>(E.z);
         giOcc[pix] = static_cast<float>(occ);
       }
     });
+    // pt1_denoise.cpp:20-81: NaN scrub, then the a-trous filter on E with the
+    // normal / position guides (no albedo demodulation)
+    if (p->pt1_denoise) {
+      for (float& e : Ebuf)
+        if (!std::isfinite(e)) e = 0.0f;
+      denoiseAtrous(W, H, Ebuf, giNrm, giPos, p->denoise_iters, p->denoise_sigma_z, p->denoise_sigma_n,
+                    p->denoise_sigma_l);
+    }
+    const float gI = p->gi_intensity;
+    for (std::size_t pix = 0; pix < npx; ++pix) {
+      if (!giElig[pix]) continue;
+      for (int k = 0; k < 3; ++k) {
+        const float ind = gI * giRefl[3 * pix + k] * Ebuf[3 * pix + k];
+        hiColor[4 * pix + k] += ind;
+        giInd[3 * pix + k] = ind;
+      }
+    }
   }
   const auto t1 = std::chrono::steady_clock::now();
   // pipeline.cpp:126-129 -> fog.cpp:17-36 (hi-res, before the downsample)
@@ -2428,6 +2544,16 @@ int64_t orc_chain_curves(const ubg_scene_desc* desc, int64_t cap, int32_t* src, 
 }
 
 void orc_tea2(uint32_t* v0, uint32_t* v1) { orc::tea2(*v0, *v1); }
+
+void orc_pt1_denoise(int32_t w, int32_t h, float* E, const float* normal, const float* position, int32_t iters,
+                     float sigma_z, float sigma_n, float sigma_l) {
+  const std::size_t n = static_cast<std::size_t>(w) * h;
+  std::vector<float> e(E, E + 3 * n), nrm(normal, normal + 3 * n), pos(position, position + 3 * n);
+  for (float& v : e)
+    if (!std::isfinite(v)) v = 0.0f;
+  orc::denoiseAtrous(w, h, e, nrm, pos, iters, sigma_z, sigma_n, sigma_l);
+  std::memcpy(E, e.data(), 3 * n * sizeof(float));
+}
 
 int orc_pt1_gather_point(const ubg_scene_desc* desc, const ubg_render_params* params, const float* P,
                          const float* N, const float* Ng, int32_t spp, uint32_t seed, float eps_t, float* E,

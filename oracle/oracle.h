@@ -69,6 +69,12 @@ int orc_pt1_gather_point(const ubg_scene_desc* desc, const ubg_render_params* pa
                          const float* N, const float* Ng, int32_t spp, uint32_t seed, float eps_t, float* E,
                          float* occlusion, uint64_t* gi_rays);
 
+/* denoisePt1E without OIDN (pt1_denoise.cpp:20-81): NaN/Inf scrub, then the
+ * a-trous filter (denoise_atrous.cpp) on E (w * h * 3, in place) with the
+ * normal and position guides (w * h * 3 each). */
+void orc_pt1_denoise(int32_t w, int32_t h, float* E, const float* normal, const float* position, int32_t iters,
+                     float sigma_z, float sigma_n, float sigma_l);
+
 /* Reference TEA-8 (secondary_rays.hpp:51-58), exposed for known-answer tests. */
 void orc_tea2(uint32_t* v0, uint32_t* v1);
 

@@ -996,3 +996,23 @@ def gi_isolated_sphere_no_self_occlusion():
         assert occ.max() < 0.2
     return Case("gi_isolated_sphere_no_self_occlusion", {"f": (sc, _gi_opts(33, 33, pt1_spp=16, shadows=False))},
                 chk, tol=0.05)
+
+
+@case
+def gi_denoise():
+    # pt1_denoise.cpp + denoise_atrous.cpp on the gathered E: the filtered
+    # 2-spp indirect is much closer to a 256-spp gather than the raw one, and
+    # 0 iterations leave E bitwise alone
+    sc = gi_scene()
+
+    def chk(r):
+        ref = r["ref"].indirect
+        recv = ref.max(axis=2) > 0
+        err = lambda f: float(((f.indirect - ref)[recv] ** 2).mean())
+        assert err(r["den"]) < 0.5 * err(r["raw"]), (err(r["den"]), err(r["raw"]))
+        assert np.array_equal(r["it0"].color, r["raw"].color)
+    o = dict(pt1_spp=2, pt1_ld=False)
+    return Case("gi_denoise", {"raw": (sc, _gi_opts(96, 72, **o)),
+                               "den": (sc, _gi_opts(96, 72, pt1_denoise=True, **o)),
+                               "it0": (sc, _gi_opts(96, 72, pt1_denoise=True, denoise_iters=0, **o)),
+                               "ref": (sc, _gi_opts(96, 72, pt1_spp=256))}, chk, tol=0.05)

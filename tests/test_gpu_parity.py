@@ -382,7 +382,7 @@ def test_adaptive_aa_fine_lattice_transparency_chunked(device):
     _compare_aa("adaptive-fine", g, c, 1)
 
 
-@pytest.mark.parametrize("workload", ["ballstick", "surface"])
+@pytest.mark.parametrize("workload", ["ballstick", "surface", "surface_denoised"])
 def test_pt1_gi_reduced(device, workload):
     """pt1 GI against the oracle: same first hits, same seeds and sample
     streams; the gathered indirect must agree within 1e-3 per-pixel mean
@@ -396,7 +396,7 @@ def test_pt1_gi_reduced(device, workload):
         sc, opt = molecular_surface_scene(n_atoms=1500, width=160, height=90)
         opt.gi_bounces = 3
     opt.ao_samples = 0
-    opt.gi, opt.gi_integrator, opt.pt1_denoise, opt.pt1_spp = True, 1, False, 8
+    opt.gi, opt.gi_integrator, opt.pt1_denoise, opt.pt1_spp = True, 1, workload.endswith("denoised"), 8
     sc.ambient_color = (0.6, 0.6, 0.6)
     print(f"pt1 GI {workload} {opt.width}x{opt.height} spp{opt.pt1_spp} bounces {opt.gi_bounces}", flush=True)
     g = _gpu(device, sc, opt, hires_only=True, want_ids=True)

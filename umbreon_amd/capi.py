@@ -111,7 +111,8 @@ class RenderParams(C.Structure):
         ("gi_intensity", C.c_float), ("gi_env_intensity", C.c_float), ("pt1_spp", C.c_int32),
         ("pt1_seed", C.c_uint32), ("pt1_sky_mode", C.c_int32), ("pt1_sky_radiance", C.c_float * 3),
         ("pt1_ld", C.c_int32), ("pt1_clamp", C.c_float), ("pt1_gather_div", C.c_int32),
-        ("pt1_half_res", C.c_int32), ("pt1_denoise", C.c_int32),
+        ("pt1_half_res", C.c_int32), ("pt1_denoise", C.c_int32), ("denoise_iters", C.c_int32),
+        ("denoise_sigma_z", C.c_float), ("denoise_sigma_n", C.c_float), ("denoise_sigma_l", C.c_float),
     ]
 
 

@@ -200,7 +200,6 @@ void checkSupported(const Scene& s, const RenderOptions& o) {
     throw std::runtime_error(std::string("umbreon (MI355X): ") + what + " is not supported by the GPU backend");
   };
   if (o.gi && o.giIntegrator != 1) no("the pt2 / irradiance-cache GI integrators (giIntegrator != 1)");
-  if (o.gi && o.pt1Denoise) no("the pt1 indirect denoise (pt1Denoise)");
   if (o.aoWriteAov) no("AO AOVs (aoWriteAov)");
   if (o.denoiser != 0) no("the final-colour denoiser");
   if (o.envLights > 0) no("environment dome lights (envLights)");
@@ -314,6 +313,10 @@ FrameResult renderPass(const Scene& scene, const RenderOptions& opt, RenderProgr
   p.pt1_gather_div = opt.pt1GatherDiv;
   p.pt1_half_res = opt.pt1HalfRes ? 1 : 0;
   p.pt1_denoise = opt.pt1Denoise ? 1 : 0;
+  p.denoise_iters = opt.denoiseIters;
+  p.denoise_sigma_z = opt.denoiseSigmaZ;
+  p.denoise_sigma_n = opt.denoiseSigmaN;
+  p.denoise_sigma_l = opt.denoiseSigmaL;
 
   const std::size_t hiN = static_cast<std::size_t>(opt.width) * ss * opt.height * ss;
   f.width = opt.width;
@@ -355,6 +358,9 @@ FrameResult renderPass(const Scene& scene, const RenderOptions& opt, RenderProgr
   f.renderSeconds = fr.render_seconds;
   f.pt1Rays.gatherRays = fr.stats.gi_rays;
   f.pt1Rays.neeRays = fr.stats.gi_nee_rays;
+  // pt1_denoise.cpp:61-66: without OIDN the E denoise is the a-trous filter
+  // (DenoiserBackend::AtrousBilateral == 1)
+  if (opt.gi && opt.pt1Denoise) f.pt1DenoiserUsed = 1;
   f.pt1Timing.bvhBuild = info.build_seconds;
   f.pt1Timing.primary = fr.render_seconds;
   return f;

@@ -68,13 +68,14 @@ struct ubg_context {
   DevBuf hitOut, aaSetC, aaSetR, aaSetF, aaFirstN, aaPix;
   // pt1 GI: first-hit position / normal / reflectance, gathered E, receiver
   // list, staging of the indirect and occlusion AOVs
-  DevBuf giPos, giNrm, giRefl, giE, giList, giInd, giOcc;
+  DevBuf giPos, giNrm, giRefl, giE, giList, giInd, giOcc, dnA, dnB, dnGrad, dnVar;
   unsigned long long* hostCounters = nullptr;  // pinned
   std::vector<DevBuf*> all() {
     return {&hitPix, &hitT,   &hitO,    &hitN,  &hitC,  &hitAo,   &activeA,  &activeB,  &acc,     &tnear,
             &hiColor, &depth, &viewZ,   &hitId, &aoOpen, &outColor, &counters, &query, &queryOut, &aoT,
             &hitOpen, &hitBent, &cGuide, &cOpen, &cBent, &hitOut, &aaSetC, &aaSetR, &aaSetF, &aaFirstN,
-            &aaPix, &giPos, &giNrm, &giRefl, &giE, &giList, &giInd, &giOcc};
+            &aaPix, &giPos, &giNrm, &giRefl, &giE, &giList, &giInd, &giOcc, &dnA, &dnB, &dnGrad,
+            &dnVar};
   }
 };
 
@@ -455,8 +456,6 @@ int ubg_render(ubg_context* ctx, const ubg_scene* scene, const ubg_render_params
     if (div == 0) div = p->pt1_half_res ? 2 : 1;
     if (div > 1)
       return fail(UBG_ERR_UNSUPPORTED, "ubg_render: pt1 GI needs a full-resolution gather grid (pt1_gather_div)");
-    if (p->pt1_denoise)
-      return fail(UBG_ERR_UNSUPPORTED, "ubg_render: the pt1 E denoise is not implemented");
     if (p->pt1_spp > (1 << 20) || p->gi_bounces > (1 << 16))
       return fail(UBG_ERR_INVALID_ARGUMENT, "ubg_render: pt1_spp / gi_bounces out of range");
   }
@@ -592,6 +591,12 @@ int ubg_render(ubg_context* ctx, const ubg_scene* scene, const ubg_render_params
     need(ctx->giList, qn * 4);
     if (fr->gi_indirect) need(ctx->giInd, n * 12);
     if (fr->gi_occlusion) need(ctx->giOcc, n * 4);
+    if (p->pt1_denoise) {
+      need(ctx->dnA, n * 16);
+      need(ctx->dnB, n * 16);
+      need(ctx->dnGrad, n * 4);
+      need(ctx->dnVar, n * 4);
+    }
   }
   if (r != UBG_OK) return r;
 
@@ -856,6 +861,10 @@ int ubg_render(ubg_context* ctx, const ubg_scene* scene, const ubg_render_params
     if (r == UBG_OK && !cancelled) {
       ubg::launchGiList(fc, b, st);
       timed(UBG_KERNEL_AO, [&] { ubg::launchGiGather(fc, scene->view, b, countVisits, st); });
+      if (p->pt1_denoise)
+        ubg::launchGiDenoise(fc, b, p->denoise_iters, p->denoise_sigma_z, p->denoise_sigma_n, p->denoise_sigma_l,
+                             static_cast<float4*>(ctx->dnA.p), static_cast<float4*>(ctx->dnB.p),
+                             static_cast<float*>(ctx->dnGrad.p), static_cast<float*>(ctx->dnVar.p), st);
       ubg::launchGiComposite(fc, b, fr->gi_indirect ? static_cast<float*>(ctx->giInd.p) : nullptr,
                              fr->gi_occlusion ? static_cast<float*>(ctx->giOcc.p) : nullptr, st);
       UBG_HIP(hipGetLastError(), "kernel launch");

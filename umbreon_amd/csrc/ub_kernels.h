@@ -86,6 +86,10 @@ void launchAaCompose(const FrameConst& fc, const AaSet& centers, const AaSet& re
 // (embree_renderer.cpp:1027-1049 + fog.cpp:17-36) and the optional AOVs.
 void launchGiList(const FrameConst& fc, const FrameBufs& b, hipStream_t st);
 void launchGiGather(const FrameConst& fc, const SceneView& s, const FrameBufs& b, bool countVisits, hipStream_t st);
+// pt1_denoise.cpp / denoise_atrous.cpp on E (iterations, edge-stop sigmas;
+// irrA/irrB/grad/var = W x H scratch)
+void launchGiDenoise(const FrameConst& fc, const FrameBufs& b, int iters, float sigmaZ, float sigmaN, float sigmaL,
+                     float4* irrA, float4* irrB, float* grad, float* var, hipStream_t st);
 void launchGiComposite(const FrameConst& fc, const FrameBufs& b, float* indirect, float* occlusion, hipStream_t st);
 
 // Queue slots per partition for a W x H hi-res grid.

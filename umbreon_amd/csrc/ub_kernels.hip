@@ -336,7 +336,10 @@ __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(kTr
       if (b.hitId) b.hitId[pix] = hit ? h.bits : 0xFFFFFFFFu;
       if (b.aoOpen && !hit) b.aoOpen[pix] = 1.0f;
       if (b.firstN && !hit) b.firstN[pix] = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(0xFFFFFFFFu));
-      if (fc.gi && !hit) b.giNrm[pix] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      if (fc.gi && !hit) {
+        b.giNrm[pix] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        b.giPos[pix] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      }
     }
     if (!hit) {
       const float4 acc = iter == 0 ? make_float4(0.0f, 0.0f, 0.0f, 0.0f) : b.acc[pix];
@@ -430,20 +433,25 @@ __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(kTr
       if (fc.aoEnh) b.hitBent[slot] = make_float4(bent.x, bent.y, bent.z, 0.0f);
     }
   }
-  if (fc.gi && !fromEdge) {
+  if (fc.gi) {
     // hit_shader.hpp:229-243 / :355-368: a mesh or real-primitive hit gathers
     // indirect instead of its constant ambient; toon finishes keep unit ambient
-    const DevMaterial& m = (flags & kFlagMesh) ? s.meshMats[flags & kInfoMatMask] : s.primMats[flags & kInfoMatMask];
-    const bool recv = m.toon == 0u;
-    flags |= recv ? kFlagGiRecv : kFlagGiToon;
-    if (iter == 0) {
-      // transparency.hpp:402-413 first-hit G-buffer of the gather
-      b.giPos[pix] = make_float4(P.x, P.y, P.z, 0.0f);
-      b.giNrm[pix] = make_float4(N.x, N.y, N.z, recv ? 1.0f : 0.0f);
-      b.giRefl[pix] = make_float4(m.diffuse * C.x, m.diffuse * C.y, m.diffuse * C.z, 0.0f);
+    bool recv = false;
+    float kd = 0.0f;
+    if (!fromEdge) {
+      const DevMaterial& m = (flags & kFlagMesh) ? s.meshMats[flags & kInfoMatMask] : s.primMats[flags & kInfoMatMask];
+      recv = m.toon == 0u;
+      kd = m.diffuse;
+      flags |= recv ? kFlagGiRecv : kFlagGiToon;
     }
-  } else if (fc.gi && iter == 0) {
-    b.giNrm[pix] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (iter == 0) {
+      // transparency.hpp:402-413 first-hit G-buffer: position, the normal AOV
+      // (mesh hits, hit_shader.hpp:214-220; receivers, :364), reflectance
+      const bool nOn = recv || (flags & kFlagMesh) != 0u;
+      b.giPos[pix] = make_float4(P.x, P.y, P.z, 0.0f);
+      b.giNrm[pix] = nOn ? make_float4(N.x, N.y, N.z, recv ? 1.0f : 0.0f) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      b.giRefl[pix] = make_float4(kd * C.x, kd * C.y, kd * C.z, 0.0f);
+    }
   }
   b.hitPix[slot] = lx + ly * static_cast<uint32_t>(fc.seedW);
   if (b.hitOut) b.hitOut[slot] = pix;
@@ -1204,9 +1212,124 @@ __device__ __forceinline__ uint32_t hashU32(uint32_t x) {
 // per wave, partition = block % kQueueParts).
 __global__ void k_gi_list(FrameConst fc, FrameBufs b) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  const bool recv = i < fc.W * fc.H && b.giNrm[i].w != 0.0f;
+  const bool in = i < fc.W * fc.H;
+  const bool recv = in && b.giNrm[i].w != 0.0f;
   const uint32_t slot = queueAppend(b.counters, kGiFill, b.partCap, recv);
   if (recv) b.giList[slot] = static_cast<uint32_t>(i);
+  else if (in) b.giE[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);  // E of a non-receiver is 0
+}
+
+// ---- pt1 E denoise: denoise_atrous.cpp as denoisePt1E runs it (no albedo
+// demodulation; guides = the normal AOV and the first-hit position) ----
+constexpr float kAtrousEps = 1.0e-3f;
+
+__device__ __forceinline__ float lum709(float4 v) { return 0.2126f * v.x + 0.7152f * v.y + 0.0722f * v.z; }
+__device__ __forceinline__ bool atrousValid(const FrameBufs& b, int p) {
+  const float4 n = b.giNrm[p];
+  return (n.x * n.x + n.y * n.y + n.z * n.z) > 0.25f;
+}
+__device__ __forceinline__ float posDist(const float4 a, const float4 c) {
+  const float dx = a.x - c.x, dy = a.y - c.y, dz = a.z - c.z;
+  return sqrtf(dx * dx + dy * dy + dz * dz);
+}
+
+// pt1_denoise.cpp:26-29 NaN/Inf scrub into the working buffer
+__global__ void k_atrous_scrub(int n, FrameBufs b, float4* __restrict__ irr) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float4 e = b.giE[i];
+  irr[i] = make_float4(isfinite(e.x) ? e.x : 0.0f, isfinite(e.y) ? e.y : 0.0f, isfinite(e.z) ? e.z : 0.0f, 0.0f);
+}
+
+// denoise_atrous.cpp:79-128: per valid pixel, the largest world-space step to a
+// valid 4-neighbour (floor 1e-3) and the 3x3 luminance variance
+__global__ void k_atrous_stats(int W, int H, FrameBufs b, const float4* __restrict__ irr, float* __restrict__ grad,
+                               float* __restrict__ var) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= W * H || !atrousValid(b, p)) return;
+  const int x = p % W, y = p / W;
+  const float4 pp = b.giPos[p];
+  float g = kAtrousEps;
+  const int nb[4][2] = {{1, 0}, {-1, 0}, {0, 1}, {0, -1}};
+  for (int k = 0; k < 4; ++k) {
+    const int qx = x + nb[k][0], qy = y + nb[k][1];
+    if (qx < 0 || qx >= W || qy < 0 || qy >= H) continue;
+    const int q = qy * W + qx;
+    if (atrousValid(b, q)) g = fmaxf(g, posDist(pp, b.giPos[q]));
+  }
+  grad[p] = g;
+  float s1 = 0.0f, s2 = 0.0f;
+  int n = 0;
+  for (int dy = -1; dy <= 1; ++dy)
+    for (int dx = -1; dx <= 1; ++dx) {
+      const int qx = x + dx, qy = y + dy;
+      if (qx < 0 || qx >= W || qy < 0 || qy >= H) continue;
+      const int q = qy * W + qx;
+      if (!atrousValid(b, q)) continue;
+      const float l = lum709(irr[q]);
+      s1 += l;
+      s2 += l * l;
+      ++n;
+    }
+  float v = 0.0f;
+  if (n > 0) {
+    const float mean = s1 / static_cast<float>(n);
+    v = fmaxf(0.0f, s2 / static_cast<float>(n) - mean * mean);
+  }
+  var[p] = v;
+}
+
+// denoise_atrous.cpp:134-197: one a-trous iteration with holes of `step`
+__global__ void k_atrous_iter(int W, int H, int step, float sigmaZ, float sigmaN, float sigmaL, FrameBufs b,
+                              const float4* __restrict__ in, const float* __restrict__ grad,
+                              const float* __restrict__ var, float4* __restrict__ out) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= W * H) return;
+  const float4 ip = in[p];
+  if (!atrousValid(b, p)) {
+    out[p] = ip;
+    return;
+  }
+  const int x = p % W, y = p / W;
+  const float h5[5] = {0.0625f, 0.25f, 0.375f, 0.25f, 0.0625f};
+  const float4 np = b.giNrm[p];
+  const float4 pp = b.giPos[p];
+  const float lp = lum709(ip);
+  const float lDenom = sigmaL * sqrtf(var[p]) + kAtrousEps;
+  const float zDenom = sigmaZ * grad[p] * static_cast<float>(step) + kAtrousEps;
+  float s0 = 0.0f, s1 = 0.0f, s2 = 0.0f, wsum = 0.0f;
+  for (int dy = -2; dy <= 2; ++dy) {
+    const int qy = y + dy * step;
+    if (qy < 0 || qy >= H) continue;
+    for (int dx = -2; dx <= 2; ++dx) {
+      const int qx = x + dx * step;
+      if (qx < 0 || qx >= W) continue;
+      const int q = qy * W + qx;
+      if (!atrousValid(b, q)) continue;
+      const float4 nq = b.giNrm[q];
+      const float4 iq = in[q];
+      float w = h5[dx + 2] * h5[dy + 2];
+      const float nd = fmaxf(0.0f, np.x * nq.x + np.y * nq.y + np.z * nq.z);
+      w *= powf(nd, sigmaN);
+      w *= expf(-posDist(pp, b.giPos[q]) / zDenom);
+      w *= expf(-fabsf(lp - lum709(iq)) / lDenom);
+      s0 += w * iq.x;
+      s1 += w * iq.y;
+      s2 += w * iq.z;
+      wsum += w;
+    }
+  }
+  out[p] = wsum > 0.0f ? make_float4(s0 / wsum, s1 / wsum, s2 / wsum, 0.0f) : ip;
+}
+
+// denoise_atrous.cpp:199-208 write-back (valid pixels clamped at 0), into E
+// (the first-bounce hit fraction in E.w stays)
+__global__ void k_atrous_store(int n, FrameBufs b, const float4* __restrict__ irr) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  float4 v = irr[p];
+  if (atrousValid(b, p)) v = make_float4(fmaxf(0.0f, v.x), fmaxf(0.0f, v.y), fmaxf(0.0f, v.z), 0.0f);
+  b.giE[p] = make_float4(v.x, v.y, v.z, b.giE[p].w);
 }
 
 // pt1GatherPoint (pt1_gather.hpp:228-485) for one receiver per lane: spp
@@ -1718,6 +1841,27 @@ void launchGiList(const FrameConst& fc, const FrameBufs& b, hipStream_t st) {
 void launchGiGather(const FrameConst& fc, const SceneView& s, const FrameBufs& b, bool countVisits, hipStream_t st) {
   const int64_t slots = static_cast<int64_t>(kQueueParts) * b.partCap;
   UBG_DISPATCH(k_gi_gather, blocksFor(slots, kTraceBlock), countVisits, fc, s, b);
+}
+void launchGiDenoise(const FrameConst& fc, const FrameBufs& b, int iters, float sigmaZ, float sigmaN, float sigmaL,
+                     float4* irrA, float4* irrB, float* grad, float* var, hipStream_t st) {
+  iters = iters < 0 ? 0 : iters;
+  if (iters == 0) return;  // denoise_atrous.cpp:32-33
+  const int n = fc.W * fc.H;
+  const unsigned g = blocksFor(n, 256);
+  sigmaZ = sigmaZ > 1.0e-4f ? sigmaZ : 1.0e-4f;
+  sigmaL = sigmaL > 1.0e-4f ? sigmaL : 1.0e-4f;
+  hipLaunchKernelGGL(k_atrous_scrub, dim3(g), dim3(256), 0, st, n, b, irrA);
+  hipLaunchKernelGGL(k_atrous_stats, dim3(g), dim3(256), 0, st, fc.W, fc.H, b, irrA, grad, var);
+  float4* in = irrA;
+  float4* out = irrB;
+  for (int it = 0; it < iters; ++it) {
+    hipLaunchKernelGGL(k_atrous_iter, dim3(g), dim3(256), 0, st, fc.W, fc.H, 1 << it, sigmaZ, sigmaN, sigmaL, b,
+                       in, grad, var, out);
+    float4* t = in;
+    in = out;
+    out = t;
+  }
+  hipLaunchKernelGGL(k_atrous_store, dim3(g), dim3(256), 0, st, n, b, in);
 }
 void launchGiComposite(const FrameConst& fc, const FrameBufs& b, float* indirect, float* occlusion, hipStream_t st) {
   const int64_t n = static_cast<int64_t>(fc.W) * fc.H;

@@ -116,8 +116,8 @@ class RenderOptions:
     aa_depth: int = 0
     aa_debug: bool = False  # fill FrameResult.aa_mask
     # path-traced GI (render_options.hpp:93-162); the gfx950 path implements
-    # the pt1 integrator (gi_integrator 1) over a full-res gather grid with
-    # the E denoise off, and refuses other combinations
+    # the pt1 integrator (gi_integrator 1) over a full-res gather grid (E
+    # denoise: the a-trous filter), and refuses other combinations
     gi: bool = False
     gi_bounces: int = 1
     gi_max_distance: float = 0.0
@@ -132,6 +132,10 @@ class RenderOptions:
     pt1_gather_div: int = -1
     pt1_half_res: bool = True
     pt1_denoise: bool = True
+    denoise_iters: int = 5
+    denoise_sigma_z: float = 1.0
+    denoise_sigma_n: float = 128.0
+    denoise_sigma_l: float = 4.0
     # options the gfx950 path does not implement; must stay at their defaults
     denoiser: int = 0
     env_lights: int = 0
@@ -395,6 +399,10 @@ def params(scene: Scene, opt: RenderOptions, flags: int = 0,
     p.pt1_gather_div = int(opt.pt1_gather_div)
     p.pt1_half_res = int(bool(opt.pt1_half_res))
     p.pt1_denoise = int(bool(opt.pt1_denoise))
+    p.denoise_iters = int(opt.denoise_iters)
+    p.denoise_sigma_z = float(opt.denoise_sigma_z)
+    p.denoise_sigma_n = float(opt.denoise_sigma_n)
+    p.denoise_sigma_l = float(opt.denoise_sigma_l)
     p.flags = flags
     if cancel_flag is not None:
         keep.append(cancel_flag)
