@@ -218,11 +218,16 @@ typedef struct ubg_render_params {
   float aa_threshold;                /* reference default 0.1 */
   int32_t aa_depth;
   /* Path-traced diffuse GI, the pt1 integrator (render_options.hpp:93-162,
-   * integrator/pt1/pt1_gather.hpp, embree_renderer.cpp:560-1131 full-res
-   * branch). gi != 0 requires gi_integrator == 1, a gather grid that
-   * resolves to the render grid (pt1_gather_div 1, or -1 at supersample 1,
-   * or 0 with pt1_half_res 0) and pt1_denoise == 0; otherwise
-   * UBG_ERR_UNSUPPORTED. Mesh hits and real (not outline) primitives become
+   * integrator/pt1/{pt1_gather,pt1_gbuffer,pt1_upsample}.hpp, pt1_denoise.cpp,
+   * embree_renderer.cpp:560-1131). gi != 0 requires gi_integrator == 1
+   * (otherwise UBG_ERR_UNSUPPORTED). The gather grid is the render grid
+   * divided by pt1_gather_div (-1 = the supersample factor, 0 = 2 with
+   * pt1_half_res else 1); a divisor > 1 traces its own first-hit G-buffer,
+   * gathers (and denoises) there, joint-bilateral upsamples E to the render
+   * grid and, with pt1_edge_patch, re-gathers the rim pixels the upsample
+   * cannot support at full resolution with pt1_edge_patch_spp_mul x the
+   * samples. The E denoise is the a-trous filter (the reference's path when
+   * OIDN is not built). Mesh hits and real (not outline) primitives become
    * receivers: their constant ambient is dropped (toon finishes keep unit
    * ambient) and pt1_spp cosine paths per hi-res pixel gather E (next-event
    * estimation at the distant lights, Russian roulette from the second
@@ -242,10 +247,14 @@ typedef struct ubg_render_params {
   float pt1_clamp;                   /* per-sample luminance clamp; 0 = off */
   int32_t pt1_gather_div;            /* -1 = supersample factor, 0 = pt1_half_res rule, k */
   int32_t pt1_half_res;
-  int32_t pt1_denoise;               /* E denoise: the edge-avoiding a-trous filter (the
-                                        reference's fallback when OIDN is not built) */
+  int32_t pt1_denoise;
   int32_t denoise_iters;             /* a-trous iterations (reference default 5) */
   float denoise_sigma_z, denoise_sigma_n, denoise_sigma_l;  /* defaults 1, 128, 4 */
+  float pt1_upsample_normal_pow;     /* default 32 */
+  float pt1_upsample_depth_scale;    /* default 0.02 */
+  int32_t pt1_edge_patch;            /* default 1 */
+  int32_t pt1_edge_patch_spp_mul;    /* default 4 */
+  float pt1_edge_patch_thresh;       /* default 0.3 */
 } ubg_render_params;
 
 /* Ray accounting of one frame. A pure function of the scene and options:
@@ -259,6 +268,7 @@ typedef struct ubg_ray_stats {
   uint64_t capped_rays;       /* primary rays that reached the transparent-layer cap */
   uint64_t gi_rays;           /* GI gather / continuation closest-hit queries */
   uint64_t gi_nee_rays;       /* GI next-event shadow queries */
+  uint64_t gi_gbuffer_rays;   /* GI reduced-grid first-hit queries */
   /* UBG_FLAG_COUNT_VISITS only, per ray class [0] primary, [1] AO, [2] shadow: */
   uint64_t node_visits[3];    /* inner-node fetches */
   uint64_t prim_tests[3];     /* primitive tests */

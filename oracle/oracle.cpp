@@ -1970,14 +1970,10 @@ static void denoiseAtrous(int W, int H, std::vector<float>& E, const std::vector
       for (int k = 0; k < 3; ++k) E[3 * p + k] = std::max(0.0f, irr[3 * p + k]);
 }
 
-// GI options the restatement covers: the pt1 integrator over a full-res
-// gather grid (embree_renderer.cpp:716-718)
+// GI integrators the restatement covers: pt1
 static bool giSupported(const ubg_render_params* p, int ss) {
-  if (!p->gi) return true;
-  if (p->gi_integrator != 1) return false;
-  int div = p->pt1_gather_div < 0 ? ss : p->pt1_gather_div;
-  if (div == 0) div = p->pt1_half_res ? 2 : 1;
-  return div <= 1;
+  (void)ss;
+  return !p->gi || p->gi_integrator == 1;
 }
 
 template <class T>
@@ -2254,6 +2250,7 @@ static int renderT(const ubg_scene_desc* desc, const ubg_render_params* p, ubg_f
   // (:959-1049): gather E at every receiver, then color += giIntensity *
   // reflectance * E, before the fog
   std::vector<float> giInd(gi ? npx * 3 : 0, 0.0f), giOcc(gi ? npx : 0, 0.0f);
+  uint64_t gbufferRays = 0;
   if (gi) {
     GiParams<T> gp;
     gp.sky = v3<T>(p->pt1_sky_radiance);
@@ -2264,32 +2261,221 @@ static int renderT(const ubg_scene_desc* desc, const ubg_render_params* p, ubg_f
     gp.maxDist = p->gi_max_distance > 0.0f ? T(p->gi_max_distance) : std::numeric_limits<T>::infinity();
     const int spp = std::max(1, p->pt1_spp);
     const uint32_t seedMix = hashU32(p->pt1_seed);
-    std::vector<int> hiRows(H);
-    for (int y = 0; y < H; ++y) hiRows[y] = y;
-    std::vector<float> Ebuf(npx * 3, 0.0f);
-    parallelRows(hiRows, [&](int py, Counters& cnt) {
-      for (int px = 0; px < W; ++px) {
-        const std::size_t pix = static_cast<std::size_t>(py) * W + px;
-        if (!giElig[pix]) continue;
-        const V3<T> P = v3<T>(&giPos[3 * pix]);
-        const V3<T> N = v3<T>(&giNrm[3 * pix]);
-        const uint32_t seed = hashU32(static_cast<uint32_t>(pix) ^ seedMix);
-        T occ = T(0);
-        const V3<T> E = pt1GatherPoint(c, gp, P, N, N, spp, seed, T(hiDepth[pix]), p->pt1_ld != 0, p->pt1_clamp,
-                                       &occ, cnt);
-        Ebuf[3 * pix] = static_cast<float>(E.x);
-        Ebuf[3 * pix + 1] = static_cast<float>(E.y);
-        Ebuf[3 * pix + 2] = static_cast<float>(E.z);
-        giOcc[pix] = static_cast<float>(occ);
-      }
-    });
+    // gatherPt1Grid (pt1_gather.hpp:525-612) over a w x h grid: every
+    // pixel with elig[pix] gathers at its position / normal (horizon test
+    // against geo, or the normal itself), seeded by its index in the grid
+    auto gatherGrid = [&](int w, int h, const std::vector<float>& pos, const std::vector<float>& nrm,
+                          const std::vector<float>* geo, const std::vector<uint8_t>& elig, const std::vector<float>& dep,
+                          int nspp, std::vector<float>& Eo, std::vector<float>& occo) {
+      std::vector<int> rws(h);
+      for (int y = 0; y < h; ++y) rws[y] = y;
+      parallelRows(rws, [&](int py, Counters& cnt) {
+        for (int px = 0; px < w; ++px) {
+          const std::size_t pix = static_cast<std::size_t>(py) * w + px;
+          if (!elig[pix]) continue;
+          const V3<T> P = v3<T>(&pos[3 * pix]);
+          const V3<T> N = v3<T>(&nrm[3 * pix]);
+          const V3<T> Ng = geo ? v3<T>(&(*geo)[3 * pix]) : N;
+          const uint32_t seed = hashU32(static_cast<uint32_t>(pix) ^ seedMix);
+          T occ = T(0);
+          const V3<T> E = pt1GatherPoint(c, gp, P, N, Ng, nspp, seed, T(dep[pix]), p->pt1_ld != 0, p->pt1_clamp,
+                                         &occ, cnt);
+          Eo[3 * pix] = static_cast<float>(E.x);
+          Eo[3 * pix + 1] = static_cast<float>(E.y);
+          Eo[3 * pix + 2] = static_cast<float>(E.z);
+          occo[pix] = static_cast<float>(occ);
+        }
+      });
+    };
     // pt1_denoise.cpp:20-81: NaN scrub, then the a-trous filter on E with the
     // normal / position guides (no albedo demodulation)
-    if (p->pt1_denoise) {
-      for (float& e : Ebuf)
+    auto denoiseE = [&](int w, int h, std::vector<float>& E, const std::vector<float>& nrm,
+                        const std::vector<float>& pos) {
+      if (!p->pt1_denoise) return;
+      for (float& e : E)
         if (!std::isfinite(e)) e = 0.0f;
-      denoiseAtrous(W, H, Ebuf, giNrm, giPos, p->denoise_iters, p->denoise_sigma_z, p->denoise_sigma_n,
+      denoiseAtrous(w, h, E, nrm, pos, p->denoise_iters, p->denoise_sigma_z, p->denoise_sigma_n,
                     p->denoise_sigma_l);
+    };
+    int div = p->pt1_gather_div < 0 ? ss : p->pt1_gather_div;
+    if (div == 0) div = p->pt1_half_res ? 2 : 1;
+    if (div < 1) div = 1;
+    std::vector<float> Ebuf(npx * 3, 0.0f);
+    if (div <= 1) {
+      gatherGrid(W, H, giPos, giNrm, nullptr, giElig, hiDepth, spp, Ebuf, giOcc);
+      denoiseE(W, H, Ebuf, giNrm, giPos);
+    } else {
+      // embree_renderer.cpp:719-958
+      const int hw = (W + div - 1) / div, hh = (H + div - 1) / div;
+      const std::size_t ln = static_cast<std::size_t>(hw) * hh;
+      std::vector<float> lPos(3 * ln, 0.0f), lNrm(3 * ln, 0.0f), lGeo(3 * ln, 0.0f), lDep(ln, 0.0f);
+      std::vector<uint8_t> lHit(ln, 0);
+      gbufferRays += ln;
+      // tracePt1GBuffer (pt1_gbuffer.hpp:86-200)
+      for (int py = 0; py < hh; ++py) {
+        const T v = T(1) - T(2) * (T(static_cast<float>(py)) + T(0.5f)) / T(static_cast<float>(hh));
+        for (int px = 0; px < hw; ++px) {
+          const T u = T(2) * (T(static_cast<float>(px)) + T(0.5f)) / T(static_cast<float>(hw)) - T(1);
+          V3<T> org, rd;
+          if (cam.orthographic) {
+            org = pos + right * (u * halfW) + trueUp * (v * halfH);
+            rd = dir;
+          } else {
+            org = pos;
+            rd = normalize(dir + right * (u * persHalfW) + trueUp * (v * persHalfH));
+          }
+          const Hit<T> h = intersect(s, org, rd, T(0), std::numeric_limits<T>::infinity());
+          if (h.kind < 0) continue;
+          const Geometry& g = s.g;
+          V3<T> N, Ng;
+          if (h.kind == UBG_KIND_MESH) {
+            const uint32_t* ix = &g.idx[3 * h.id];
+            const T w0 = T(1) - h.u - h.v;
+            T nb[3];
+            for (int k = 0; k < 3; ++k)
+              nb[k] = w0 * T(g.nrm[3 * ix[0] + k]) + h.u * T(g.nrm[3 * ix[1] + k]) + h.v * T(g.nrm[3 * ix[2] + k]);
+            N = safeNormalize(V3<T>{nb[0], nb[1], nb[2]});
+            Ng = safeNormalize(h.ng);
+          } else {
+            const PrimSide* sd = h.kind == UBG_KIND_SPHERE        ? &g.sphSide[h.id]
+                                 : h.kind == UBG_KIND_CONE_CURVE ? &g.cSide[h.id]
+                                                                 : &g.rSide[h.id];
+            if (sd->fromEdge) continue;
+            N = safeNormalize(h.ng);
+            Ng = N;
+          }
+          if (dot(N, rd) > T(0)) N = neg(N);
+          if (dot(Ng, rd) > T(0)) Ng = neg(Ng);
+          const std::size_t lp = static_cast<std::size_t>(py) * hw + px;
+          const V3<T> P{org.x + rd.x * h.t, org.y + rd.y * h.t, org.z + rd.z * h.t};
+          lPos[3 * lp] = static_cast<float>(P.x);
+          lPos[3 * lp + 1] = static_cast<float>(P.y);
+          lPos[3 * lp + 2] = static_cast<float>(P.z);
+          lNrm[3 * lp] = static_cast<float>(N.x);
+          lNrm[3 * lp + 1] = static_cast<float>(N.y);
+          lNrm[3 * lp + 2] = static_cast<float>(N.z);
+          lGeo[3 * lp] = static_cast<float>(Ng.x);
+          lGeo[3 * lp + 1] = static_cast<float>(Ng.y);
+          lGeo[3 * lp + 2] = static_cast<float>(Ng.z);
+          lDep[lp] = static_cast<float>(h.t);
+          lHit[lp] = 1;
+        }
+      }
+      std::vector<float> Eh(3 * ln, 0.0f), occh(ln, 0.0f);
+      gatherGrid(hw, hh, lPos, lNrm, &lGeo, lHit, lDep, spp, Eh, occh);
+      denoiseE(hw, hh, Eh, lNrm, lPos);
+      // upsampleJointBilateral (pt1_upsample.hpp:35-176)
+      const float nPow = p->pt1_upsample_normal_pow, dScale = p->pt1_upsample_depth_scale;
+      const int fbR = div <= 2 ? 0 : div;
+      std::vector<uint8_t> patch(npx, 0);
+      for (int py = 0; py < H; ++py) {
+        const float sy = (static_cast<float>(py) + 0.5f) * static_cast<float>(hh) / static_cast<float>(H) - 0.5f;
+        const int by = static_cast<int>(std::floor(sy));
+        const float fy = sy - static_cast<float>(by);
+        for (int px = 0; px < W; ++px) {
+          const std::size_t pix = static_cast<std::size_t>(py) * W + px;
+          if (!giElig[pix]) continue;
+          const float sx = (static_cast<float>(px) + 0.5f) * static_cast<float>(hw) / static_cast<float>(W) - 0.5f;
+          const int bx = static_cast<int>(std::floor(sx));
+          const float fx = sx - static_cast<float>(bx);
+          const float* nf = &giNrm[3 * pix];
+          const float zf = hiDepth[pix];
+          float wSum = 0.0f, e[3] = {0.0f, 0.0f, 0.0f}, oSum = 0.0f, bestBilin = -1.0f;
+          long best = -1;
+          for (int dy = 0; dy < 2; ++dy)
+            for (int dx = 0; dx < 2; ++dx) {
+              const int hx = std::min(std::max(bx + dx, 0), hw - 1), hy = std::min(std::max(by + dy, 0), hh - 1);
+              const std::size_t hp = static_cast<std::size_t>(hy) * hw + hx;
+              if (!lHit[hp]) continue;
+              const float wb = (dx ? fx : 1.0f - fx) * (dy ? fy : 1.0f - fy);
+              if (wb > bestBilin) {
+                bestBilin = wb;
+                best = static_cast<long>(hp);
+              }
+              const float nd = nf[0] * lNrm[3 * hp] + nf[1] * lNrm[3 * hp + 1] + nf[2] * lNrm[3 * hp + 2];
+              const float wn = std::pow(std::fmax(0.0f, nd), nPow);
+              const float wz = std::exp(-std::fabs(zf - lDep[hp]) / (dScale * zf + 1e-6f));
+              const float wgt = wb * wn * wz;
+              wSum += wgt;
+              for (int k = 0; k < 3; ++k) e[k] += wgt * Eh[3 * hp + k];
+              oSum += wgt * occh[hp];
+            }
+          if (wSum >= 1e-6f) {
+            if (p->pt1_edge_patch && wSum < p->pt1_edge_patch_thresh) patch[pix] = 1;
+            const float inv = 1.0f / wSum;
+            for (int k = 0; k < 3; ++k) Ebuf[3 * pix + k] = e[k] * inv;
+            giOcc[pix] = oSum * inv;
+            continue;
+          }
+          if (p->pt1_edge_patch) patch[pix] = 1;
+          if (best < 0 && fbR > 0) {
+            float bestD2 = 1e30f;
+            for (int wy = by - fbR + 1; wy <= by + fbR; ++wy) {
+              if (wy < 0 || wy >= hh) continue;
+              for (int wx = bx - fbR + 1; wx <= bx + fbR; ++wx) {
+                if (wx < 0 || wx >= hw) continue;
+                const std::size_t hp = static_cast<std::size_t>(wy) * hw + wx;
+                if (!lHit[hp]) continue;
+                const float ddx = static_cast<float>(wx) - sx, ddy = static_cast<float>(wy) - sy;
+                const float d2 = ddx * ddx + ddy * ddy;
+                if (d2 < bestD2) {
+                  bestD2 = d2;
+                  best = static_cast<long>(hp);
+                }
+              }
+            }
+          }
+          if (best >= 0) {
+            for (int k = 0; k < 3; ++k) Ebuf[3 * pix + k] = Eh[3 * best + k];
+            giOcc[pix] = occh[best];
+          }
+        }
+      }
+      // the rim patch (embree_renderer.cpp:800-850) and smoothPatchedPixels
+      // (pt1_upsample.hpp:188-245, radius 2)
+      bool anyPatch = false;
+      for (uint8_t m : patch) anyPatch = anyPatch || m;
+      if (anyPatch) {
+        std::vector<float> Ep(npx * 3, 0.0f), occp(npx, 0.0f);
+        gatherGrid(W, H, giPos, giNrm, nullptr, patch, hiDepth, spp * std::max(1, p->pt1_edge_patch_spp_mul), Ep,
+                   occp);
+        for (std::size_t i = 0; i < npx; ++i) {
+          if (!patch[i]) continue;
+          for (int k = 0; k < 3; ++k) Ebuf[3 * i + k] = Ep[3 * i + k];
+          giOcc[i] = occp[i];
+        }
+        const std::vector<float> Esrc = Ebuf, oSrc = giOcc;
+        for (int py = 0; py < H; ++py)
+          for (int px = 0; px < W; ++px) {
+            const std::size_t pix = static_cast<std::size_t>(py) * W + px;
+            if (!patch[pix]) continue;
+            const float* nf = &giNrm[3 * pix];
+            const float zf = hiDepth[pix];
+            float wSum = 1.0f, e[3] = {Esrc[3 * pix], Esrc[3 * pix + 1], Esrc[3 * pix + 2]}, oSum = oSrc[pix];
+            for (int dy = -2; dy <= 2; ++dy) {
+              const int ny = py + dy;
+              if (ny < 0 || ny >= H) continue;
+              for (int dx = -2; dx <= 2; ++dx) {
+                if (dx == 0 && dy == 0) continue;
+                const int nx = px + dx;
+                if (nx < 0 || nx >= W) continue;
+                const std::size_t q = static_cast<std::size_t>(ny) * W + nx;
+                if (!giElig[q]) continue;
+                const float nd = nf[0] * giNrm[3 * q] + nf[1] * giNrm[3 * q + 1] + nf[2] * giNrm[3 * q + 2];
+                const float wn = std::pow(std::fmax(0.0f, nd), nPow);
+                const float wz = std::exp(-std::fabs(zf - hiDepth[q]) / (dScale * zf + 1e-6f));
+                const float wgt = wn * wz;
+                if (wgt < 1e-4f) continue;
+                wSum += wgt;
+                for (int k = 0; k < 3; ++k) e[k] += wgt * Esrc[3 * q + k];
+                oSum += wgt * oSrc[q];
+              }
+            }
+            const float inv = 1.0f / wSum;
+            for (int k = 0; k < 3; ++k) Ebuf[3 * pix + k] = e[k] * inv;
+            giOcc[pix] = oSum * inv;
+          }
+      }
     }
     const float gI = p->gi_intensity;
     for (std::size_t pix = 0; pix < npx; ++pix) {
@@ -2341,6 +2527,7 @@ static int renderT(const ubg_scene_desc* desc, const ubg_render_params* p, ubg_f
   fr->stats.capped_rays = tot.capped;
   fr->stats.gi_rays = tot.giRays;
   fr->stats.gi_nee_rays = tot.giNee;
+  fr->stats.gi_gbuffer_rays = gbufferRays;
   if (gi && fr->gi_indirect) std::memcpy(fr->gi_indirect, giInd.data(), npx * 3 * sizeof(float));
   if (gi && fr->gi_occlusion) std::memcpy(fr->gi_occlusion, giOcc.data(), npx * sizeof(float));
   fr->render_seconds = std::chrono::duration<double>(t2 - t0).count();

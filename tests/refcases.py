@@ -1016,3 +1016,31 @@ def gi_denoise():
                                "den": (sc, _gi_opts(96, 72, pt1_denoise=True, **o)),
                                "it0": (sc, _gi_opts(96, 72, pt1_denoise=True, denoise_iters=0, **o)),
                                "ref": (sc, _gi_opts(96, 72, pt1_spp=256))}, chk, tol=0.05)
+
+
+@case
+def gi_reduced_grid():
+    # test_pt1_render.cpp:312-328 (gatherDiv 4: deterministic, indirect on the
+    # floor, finite colour) and the reduced grid against the full-res gather:
+    # at ss 2 the default divisor (= the supersample factor) gathers on a
+    # half grid, upsamples, and re-gathers the rims; its indirect stays close
+    # to the full-res gather's on average
+    sc = gi_scene()
+    d4 = _gi_opts(48, 36, pt1_spp=4, pt1_gather_div=4)
+    half = _gi_opts(48, 36, supersample=2, pt1_spp=16, pt1_ld=True)
+    full = _gi_opts(48, 36, supersample=2, pt1_spp=16, pt1_ld=True, pt1_gather_div=1)
+    nopatch = _gi_opts(48, 36, supersample=2, pt1_spp=16, pt1_ld=True, pt1_edge_patch=False)
+
+    def chk(r):
+        a, b = r["d4a"], r["d4b"]
+        assert np.array_equal(a.color, b.color) and np.array_equal(a.indirect, b.indirect)
+        assert a.indirect.max() > 0.0 and np.isfinite(a.color).all()
+        assert a.stats["gi_gbuffer_rays"] == 12 * 9
+        recv = r["full"].indirect.max(axis=2) > 0
+        d = np.abs(r["half"].indirect - r["full"].indirect)[recv].mean()
+        assert d < 0.02, d
+        assert r["half"].stats["gi_gbuffer_rays"] == 48 * 36
+        # the rim patch re-gathers the unsupported pixels at full resolution
+        assert r["half"].stats["gi_rays"] > r["nopatch"].stats["gi_rays"]
+    return Case("gi_reduced_grid", {"d4a": (sc, d4), "d4b": (sc, d4), "half": (sc, half), "full": (sc, full),
+                                    "nopatch": (sc, nopatch)}, chk, tol=0.05)

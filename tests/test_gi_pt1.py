@@ -158,7 +158,7 @@ def test_env_intensity_and_ambient_scale_linearly():
 def test_render_refuses_unimplemented_gi():
     from umbreon_amd import capi
     sc = _wall((0.5, 0.5, 0.5, 1.0))
-    for kw in (dict(gi_integrator=2), dict(supersample=2), dict(pt1_gather_div=2)):
+    for kw in (dict(gi_integrator=2), dict(gi_integrator=0)):
         o = _opt(width=4, height=4, **kw)
         with pytest.raises(capi.UmbreonError):
             ob.render(sc, o, threads=1)

@@ -382,21 +382,28 @@ def test_adaptive_aa_fine_lattice_transparency_chunked(device):
     _compare_aa("adaptive-fine", g, c, 1)
 
 
-@pytest.mark.parametrize("workload", ["ballstick", "surface", "surface_denoised"])
+@pytest.mark.parametrize("workload", ["ballstick", "surface", "surface_denoised", "ballstick_ss2_half_grid",
+                                      "surface_div3"])
 def test_pt1_gi_reduced(device, workload):
     """pt1 GI against the oracle: same first hits, same seeds and sample
     streams; the gathered indirect must agree within 1e-3 per-pixel mean
     (a path may turn differently where ocml and glibc cos/sin differ in the
     last ulp), the direct frame (colour - indirect) within the usual gate."""
-    if workload == "ballstick":
+    if workload.startswith("ballstick"):
         sc, opt = ball_and_stick_scene(n_atoms=3000, width=160, height=90)
-        opt.supersample = 1
+        opt.supersample = 2 if workload.endswith("half_grid") else 1
         opt.gi_bounces = 2
     else:
         sc, opt = molecular_surface_scene(n_atoms=1500, width=160, height=90)
         opt.gi_bounces = 3
     opt.ao_samples = 0
-    opt.gi, opt.gi_integrator, opt.pt1_denoise, opt.pt1_spp = True, 1, workload.endswith("denoised"), 8
+    opt.gi, opt.gi_integrator, opt.pt1_spp = True, 1, 8
+    # the reduced grids run the full pipeline: own G-buffer, gather, denoise,
+    # upsample, rim patch (divisor 2 = the supersample factor; 3 widens the
+    # upsample fallback window)
+    opt.pt1_denoise = workload.endswith("denoised") or workload.endswith("half_grid") or workload.endswith("div3")
+    if workload.endswith("div3"):
+        opt.pt1_gather_div = 3
     sc.ambient_color = (0.6, 0.6, 0.6)
     print(f"pt1 GI {workload} {opt.width}x{opt.height} spp{opt.pt1_spp} bounces {opt.gi_bounces}", flush=True)
     g = _gpu(device, sc, opt, hires_only=True, want_ids=True)
@@ -412,4 +419,5 @@ def test_pt1_gi_reduced(device, workload):
     dd = np.abs(direct_g - direct_c).max(axis=2)
     assert (dd > 1e-3).mean() <= 5e-4
     assert abs(g.stats["gi_rays"] - c.stats["gi_rays"]) <= 1e-3 * c.stats["gi_rays"]
+    assert g.stats["gi_gbuffer_rays"] == c.stats["gi_gbuffer_rays"]
     assert np.abs(g.gi_occlusion - c.gi_occlusion).mean() <= 1e-3

@@ -113,6 +113,8 @@ class RenderParams(C.Structure):
         ("pt1_ld", C.c_int32), ("pt1_clamp", C.c_float), ("pt1_gather_div", C.c_int32),
         ("pt1_half_res", C.c_int32), ("pt1_denoise", C.c_int32), ("denoise_iters", C.c_int32),
         ("denoise_sigma_z", C.c_float), ("denoise_sigma_n", C.c_float), ("denoise_sigma_l", C.c_float),
+        ("pt1_upsample_normal_pow", C.c_float), ("pt1_upsample_depth_scale", C.c_float),
+        ("pt1_edge_patch", C.c_int32), ("pt1_edge_patch_spp_mul", C.c_int32), ("pt1_edge_patch_thresh", C.c_float),
     ]
 
 
@@ -120,7 +122,7 @@ class RayStats(C.Structure):
     _fields_ = [("primary_rays", C.c_uint64), ("shaded_hits", C.c_uint64),
                 ("ao_rays_traced", C.c_uint64), ("ao_rays_sampled", C.c_uint64),
                 ("shadow_rays", C.c_uint64), ("capped_rays", C.c_uint64),
-                ("gi_rays", C.c_uint64), ("gi_nee_rays", C.c_uint64),
+                ("gi_rays", C.c_uint64), ("gi_nee_rays", C.c_uint64), ("gi_gbuffer_rays", C.c_uint64),
                 ("node_visits", C.c_uint64 * 3), ("prim_tests", C.c_uint64 * 3)]
 
     def as_dict(self) -> dict:

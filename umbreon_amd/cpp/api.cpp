@@ -317,6 +317,11 @@ FrameResult renderPass(const Scene& scene, const RenderOptions& opt, RenderProgr
   p.denoise_sigma_z = opt.denoiseSigmaZ;
   p.denoise_sigma_n = opt.denoiseSigmaN;
   p.denoise_sigma_l = opt.denoiseSigmaL;
+  p.pt1_upsample_normal_pow = opt.pt1UpsampleNormalPow;
+  p.pt1_upsample_depth_scale = opt.pt1UpsampleDepthScale;
+  p.pt1_edge_patch = opt.pt1EdgePatch ? 1 : 0;
+  p.pt1_edge_patch_spp_mul = opt.pt1EdgePatchSppMul;
+  p.pt1_edge_patch_thresh = opt.pt1EdgePatchThresh;
 
   const std::size_t hiN = static_cast<std::size_t>(opt.width) * ss * opt.height * ss;
   f.width = opt.width;
@@ -358,6 +363,7 @@ FrameResult renderPass(const Scene& scene, const RenderOptions& opt, RenderProgr
   f.renderSeconds = fr.render_seconds;
   f.pt1Rays.gatherRays = fr.stats.gi_rays;
   f.pt1Rays.neeRays = fr.stats.gi_nee_rays;
+  f.pt1Rays.gbufferRays = fr.stats.gi_gbuffer_rays;
   // pt1_denoise.cpp:61-66: without OIDN the E denoise is the a-trous filter
   // (DenoiserBackend::AtrousBilateral == 1)
   if (opt.gi && opt.pt1Denoise) f.pt1DenoiserUsed = 1;

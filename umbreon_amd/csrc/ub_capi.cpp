@@ -68,14 +68,14 @@ struct ubg_context {
   DevBuf hitOut, aaSetC, aaSetR, aaSetF, aaFirstN, aaPix;
   // pt1 GI: first-hit position / normal / reflectance, gathered E, receiver
   // list, staging of the indirect and occlusion AOVs
-  DevBuf giPos, giNrm, giRefl, giE, giList, giInd, giOcc, dnA, dnB, dnGrad, dnVar;
+  DevBuf giPos, giNrm, giRefl, giE, giList, giInd, giOcc, dnA, dnB, dnGrad, dnVar, giLow, giPatch, giSnap;
   unsigned long long* hostCounters = nullptr;  // pinned
   std::vector<DevBuf*> all() {
     return {&hitPix, &hitT,   &hitO,    &hitN,  &hitC,  &hitAo,   &activeA,  &activeB,  &acc,     &tnear,
             &hiColor, &depth, &viewZ,   &hitId, &aoOpen, &outColor, &counters, &query, &queryOut, &aoT,
             &hitOpen, &hitBent, &cGuide, &cOpen, &cBent, &hitOut, &aaSetC, &aaSetR, &aaSetF, &aaFirstN,
             &aaPix, &giPos, &giNrm, &giRefl, &giE, &giList, &giInd, &giOcc, &dnA, &dnB, &dnGrad,
-            &dnVar};
+            &dnVar, &giLow, &giPatch, &giSnap};
   }
 };
 
@@ -445,20 +445,23 @@ int ubg_render(ubg_context* ctx, const ubg_scene* scene, const ubg_render_params
     }
     fc.aoUpAxis[0] = up.x, fc.aoUpAxis[1] = up.y, fc.aoUpAxis[2] = up.z;
   }
-  // pt1 GI (render_options.hpp:93-162): the gather grid must resolve to the
-  // render grid (embree_renderer.cpp:716-718; pipeline.cpp:55 resolves -1 to
-  // the supersample factor) and the E denoise is off
+  // pt1 GI (render_options.hpp:93-162): gather-grid divisor per
+  // embree_renderer.cpp:716-718 (pipeline.cpp:55 resolves -1 to the
+  // supersample factor)
   const bool gi = p->gi != 0;
+  int giDiv = 1;
   if (gi) {
     if (p->gi_integrator != 1)
       return fail(UBG_ERR_UNSUPPORTED, "ubg_render: only the pt1 GI integrator (gi_integrator 1) is implemented");
-    int div = p->pt1_gather_div < 0 ? ss : p->pt1_gather_div;
-    if (div == 0) div = p->pt1_half_res ? 2 : 1;
-    if (div > 1)
-      return fail(UBG_ERR_UNSUPPORTED, "ubg_render: pt1 GI needs a full-resolution gather grid (pt1_gather_div)");
-    if (p->pt1_spp > (1 << 20) || p->gi_bounces > (1 << 16))
-      return fail(UBG_ERR_INVALID_ARGUMENT, "ubg_render: pt1_spp / gi_bounces out of range");
+    giDiv = p->pt1_gather_div < 0 ? ss : p->pt1_gather_div;
+    if (giDiv == 0) giDiv = p->pt1_half_res ? 2 : 1;
+    if (giDiv < 1) giDiv = 1;
+    if (p->pt1_spp > (1 << 20) || p->gi_bounces > (1 << 16) ||
+        static_cast<int64_t>(std::max(1, p->pt1_spp)) * std::max(1, p->pt1_edge_patch_spp_mul) > (1 << 24))
+      return fail(UBG_ERR_INVALID_ARGUMENT, "ubg_render: pt1_spp / gi_bounces / edge-patch samples out of range");
   }
+  const int giLw = (W + giDiv - 1) / giDiv, giLh = (H + giDiv - 1) / giDiv;
+  const size_t giLn = static_cast<size_t>(giLw) * giLh;
   fc.gi = gi ? 1 : 0;
   fc.giSpp = std::max(1, p->pt1_spp);
   fc.giBounces = std::max(1, p->gi_bounces);
@@ -596,6 +599,13 @@ int ubg_render(ubg_context* ctx, const ubg_scene* scene, const ubg_render_params
       need(ctx->dnB, n * 16);
       need(ctx->dnGrad, n * 4);
       need(ctx->dnVar, n * 4);
+    }
+    if (giDiv > 1) {
+      // reduced grid: position, normal (+ hit flag), geometric normal, albedo,
+      // depth, E; the rim-patch mask of the render grid and the E snapshot
+      need(ctx->giLow, giLn * (5 * 16 + 4));
+      need(ctx->giPatch, n);
+      need(ctx->giSnap, n * 16);
     }
   }
   if (r != UBG_OK) return r;
@@ -859,12 +869,66 @@ int ubg_render(ubg_context* ctx, const ubg_scene* scene, const ubg_render_params
     fl.fogOn = 0;
     r = runLayers(fl, b, npx);
     if (r == UBG_OK && !cancelled) {
-      ubg::launchGiList(fc, b, st);
-      timed(UBG_KERNEL_AO, [&] { ubg::launchGiGather(fc, scene->view, b, countVisits, st); });
-      if (p->pt1_denoise)
-        ubg::launchGiDenoise(fc, b, p->denoise_iters, p->denoise_sigma_z, p->denoise_sigma_n, p->denoise_sigma_l,
-                             static_cast<float4*>(ctx->dnA.p), static_cast<float4*>(ctx->dnB.p),
-                             static_cast<float*>(ctx->dnGrad.p), static_cast<float*>(ctx->dnVar.p), st);
+      // the render grid as a gather field (no geometric normal)
+      ubg::GiField full;
+      full.w = W;
+      full.h = H;
+      full.pos = b.giPos;
+      full.nrm = b.giNrm;
+      full.geoN = nullptr;
+      full.depth = b.depth;
+      full.E = b.giE;
+      full.list = b.giList;
+      full.fillBase = ubg::kGiFill;
+      full.spp = fc.giSpp;
+      auto denoise = [&](const ubg::GiField& f) {
+        if (p->pt1_denoise)
+          ubg::launchGiDenoise(f, p->denoise_iters, p->denoise_sigma_z, p->denoise_sigma_n, p->denoise_sigma_l,
+                               static_cast<float4*>(ctx->dnA.p), static_cast<float4*>(ctx->dnB.p),
+                               static_cast<float*>(ctx->dnGrad.p), static_cast<float*>(ctx->dnVar.p), st);
+      };
+      if (giDiv <= 1) {
+        // embree_renderer.cpp:959-1011: gather (and denoise) on the render grid
+        ubg::launchGiList(full, nullptr, true, b, st);
+        timed(UBG_KERNEL_AO, [&] { ubg::launchGiGather(fc, scene->view, b, full, countVisits, st); });
+        denoise(full);
+      } else {
+        // embree_renderer.cpp:719-958: own G-buffer on the reduced grid, gather
+        // and denoise there, joint-bilateral upsample, full-res rim patch
+        char* lb = static_cast<char*>(ctx->giLow.p);
+        ubg::GiField low;
+        low.w = giLw;
+        low.h = giLh;
+        low.pos = reinterpret_cast<float4*>(lb);
+        low.nrm = reinterpret_cast<float4*>(lb + 16 * giLn);
+        low.geoN = reinterpret_cast<float4*>(lb + 32 * giLn);
+        float4* lowAlb = reinterpret_cast<float4*>(lb + 48 * giLn);
+        low.E = reinterpret_cast<float4*>(lb + 64 * giLn);
+        low.depth = reinterpret_cast<float*>(lb + 80 * giLn);
+        low.list = b.giList;
+        low.fillBase = ubg::kGiFill;
+        low.spp = fc.giSpp;
+        (void)lowAlb;
+        timed(UBG_KERNEL_PRIMARY, [&] { ubg::launchGiGbuffer(fc, scene->view, low, lowAlb, b.counters, st); });
+        ubg::launchGiList(low, nullptr, true, b, st);
+        timed(UBG_KERNEL_AO, [&] { ubg::launchGiGather(fc, scene->view, b, low, countVisits, st); });
+        denoise(low);
+        uint8_t* patch = p->pt1_edge_patch ? static_cast<uint8_t*>(ctx->giPatch.p) : nullptr;
+        ubg::launchGiUpsample(fc, b, low, p->pt1_upsample_normal_pow, p->pt1_upsample_depth_scale,
+                              giDiv <= 2 ? 0 : giDiv, patch, p->pt1_edge_patch_thresh, st);
+        if (patch) {
+          // the rims at full resolution with spp x pt1EdgePatchSppMul samples,
+          // then the guided smoothing of exactly those pixels
+          ubg::GiField pf = full;
+          pf.fillBase = ubg::kGiPatchFill;
+          pf.spp = fc.giSpp * std::max(1, p->pt1_edge_patch_spp_mul);
+          ubg::launchGiList(pf, patch, false, b, st);
+          timed(UBG_KERNEL_AO, [&] { ubg::launchGiGather(fc, scene->view, b, pf, countVisits, st); });
+          UBG_HIP(hipMemcpyAsync(ctx->giSnap.p, b.giE, n * 16, hipMemcpyDeviceToDevice, st), "hipMemcpy(E snapshot)");
+          ubg::launchGiSmoothPatched(fc, b, patch, p->pt1_upsample_normal_pow, p->pt1_upsample_depth_scale,
+                                     static_cast<const float4*>(ctx->giSnap.p), st);
+        }
+      }
       ubg::launchGiComposite(fc, b, fr->gi_indirect ? static_cast<float*>(ctx->giInd.p) : nullptr,
                              fr->gi_occlusion ? static_cast<float*>(ctx->giOcc.p) : nullptr, st);
       UBG_HIP(hipGetLastError(), "kernel launch");
@@ -921,6 +985,7 @@ int ubg_render(ubg_context* ctx, const ubg_scene* scene, const ubg_render_params
   fr->stats.capped_rays = c[ubg::kCntCapped];
   fr->stats.gi_rays = c[ubg::kCntGiRays];
   fr->stats.gi_nee_rays = c[ubg::kCntGiNee];
+  fr->stats.gi_gbuffer_rays = c[ubg::kCntGiGbuf];
   for (int k = 0; k < 3; ++k) {
     fr->stats.node_visits[k] = countVisits ? c[ubg::kCntNodes + k] : 0;
     fr->stats.prim_tests[k] = countVisits ? c[ubg::kCntPrimTests + k] : 0;

@@ -80,16 +80,45 @@ void launchAaCompose(const FrameConst& fc, const AaSet& centers, const AaSet& re
                      const uint8_t* mask, const uint32_t* slotOf, bool boosted, const FrameBufs& out,
                      hipStream_t st);
 
-// pt1 GI after the layer passes: the receiver list from the first-hit
-// G-buffer, the per-receiver path-traced gather (pt1_gather.hpp:228-612),
-// then the composite of gi_intensity * reflectance * E with the frame's fog
-// (embree_renderer.cpp:1027-1049 + fog.cpp:17-36) and the optional AOVs.
-void launchGiList(const FrameConst& fc, const FrameBufs& b, hipStream_t st);
-void launchGiGather(const FrameConst& fc, const SceneView& s, const FrameBufs& b, bool countVisits, hipStream_t st);
-// pt1_denoise.cpp / denoise_atrous.cpp on E (iterations, edge-stop sigmas;
-// irrA/irrB/grad/var = W x H scratch)
-void launchGiDenoise(const FrameConst& fc, const FrameBufs& b, int iters, float sigmaZ, float sigmaN, float sigmaL,
-                     float4* irrA, float4* irrB, float* grad, float* var, hipStream_t st);
+// One pt1 GI gather grid: the render grid (the layer pass's first-hit
+// G-buffer) or a reduced grid with its own G-buffer (pt1_gbuffer.hpp).
+struct GiField {
+  int32_t w, h;
+  const float4* pos;   // first-hit position xyz
+  const float4* nrm;   // shading normal xyz (the denoise / upsample guide), w != 0: gathers
+  const float4* geoN;  // geometric normal for the horizon test (null: the shading normal)
+  const float* depth;  // primary-ray distance (the gather origin's epsilon scale)
+  float4* E;           // gathered E rgb, first-bounce hit fraction in w
+  uint32_t* list;      // gather list, partitioned like the hit queue
+  int32_t fillBase;    // counter word of the list's partition fills
+  int32_t spp;
+};
+
+// pt1 GI after the layer passes (embree_renderer.cpp:560-1049):
+// * GiList: the pixels of `f` to gather (nrm.w != 0, or mask != 0) into the
+//   list; with zeroOthers the rest get E = 0;
+// * GiGbuffer: tracePt1GBuffer (pt1_gbuffer.hpp:86-200) over the reduced
+//   grid `f` (albedo guide into `albedo`);
+// * GiGather: pt1GatherPoint (pt1_gather.hpp:228-485) per listed pixel;
+// * GiDenoise: denoisePt1E's a-trous filter on f.E (denoise_atrous.cpp;
+//   irrA / irrB / grad / var: f.w x f.h scratch);
+// * GiUpsample: upsampleJointBilateral (pt1_upsample.hpp:35-176) of the
+//   reduced E to the render grid, the rim-patch mask into `patch`;
+// * GiSmoothPatched: smoothPatchedPixels (pt1_upsample.hpp:188-245) over the
+//   patched render-grid pixels, reading the snapshot `Esrc`;
+// * GiComposite: colour += gi_intensity * reflectance * E, then the fog
+//   (fog.cpp:17-36), and the optional AOVs.
+void launchGiList(const GiField& f, const uint8_t* mask, bool zeroOthers, const FrameBufs& b, hipStream_t st);
+void launchGiGbuffer(const FrameConst& fc, const SceneView& s, const GiField& f, float4* albedo,
+                     unsigned long long* counters, hipStream_t st);
+void launchGiGather(const FrameConst& fc, const SceneView& s, const FrameBufs& b, const GiField& f, bool countVisits,
+                    hipStream_t st);
+void launchGiDenoise(const GiField& f, int iters, float sigmaZ, float sigmaN, float sigmaL, float4* irrA,
+                     float4* irrB, float* grad, float* var, hipStream_t st);
+void launchGiUpsample(const FrameConst& fc, const FrameBufs& b, const GiField& low, float normalPow,
+                      float depthScale, int fallbackRadius, uint8_t* patch, float patchThresh, hipStream_t st);
+void launchGiSmoothPatched(const FrameConst& fc, const FrameBufs& b, const uint8_t* patch, float normalPow,
+                           float depthScale, const float4* Esrc, hipStream_t st);
 void launchGiComposite(const FrameConst& fc, const FrameBufs& b, float* indirect, float* occlusion, hipStream_t st);
 
 // Queue slots per partition for a W x H hi-res grid.

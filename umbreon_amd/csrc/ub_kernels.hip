@@ -1208,24 +1208,100 @@ __device__ __forceinline__ uint32_t hashU32(uint32_t x) {
   return x;
 }
 
-// Receivers of the first-hit G-buffer into the partitioned list (one atomic
-// per wave, partition = block % kQueueParts).
-__global__ void k_gi_list(FrameConst fc, FrameBufs b) {
+// The pixels of a gather grid to gather (f.nrm.w != 0, or mask != 0) into
+// its partitioned list (one atomic per wave, partition = block % kQueueParts);
+// with zeroOthers every other pixel gets E = 0 (gatherPt1Grid's zero fill).
+__global__ void k_gi_list(GiField f, const uint8_t* __restrict__ mask, int zeroOthers, FrameBufs b) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  const bool in = i < fc.W * fc.H;
-  const bool recv = in && b.giNrm[i].w != 0.0f;
-  const uint32_t slot = queueAppend(b.counters, kGiFill, b.partCap, recv);
-  if (recv) b.giList[slot] = static_cast<uint32_t>(i);
-  else if (in) b.giE[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);  // E of a non-receiver is 0
+  const bool in = i < f.w * f.h;
+  const bool on = in && (mask ? mask[i] != 0 : f.nrm[i].w != 0.0f);
+  const uint32_t slot = queueAppend(b.counters, f.fillBase, b.partCap, on);
+  if (on) f.list[slot] = static_cast<uint32_t>(i);
+  else if (in && zeroOthers) f.E[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+}
+
+// tracePt1GBuffer (pt1_gbuffer.hpp:86-200): one camera ray per pixel centre
+// of the reduced grid (the render camera's plane, no clip range), mesh hits
+// and real primitives gather-eligible (toon finishes included), normals
+// face-forwarded against the ray, albedo = diffuse * pigment.
+template <int kLayout>
+__global__ __launch_bounds__(kTraceBlock) void k_gi_gbuf(FrameConst fc, SceneView s, GiField f, float4* albedo,
+                                                         unsigned long long* counters) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool valid = i < f.w * f.h;
+  Hit h;
+  V3 org = mk(0, 0, 0), rd = mk(0, 0, 1);
+  bool hit = false;
+  if (valid) {
+    genRayWH(fc, static_cast<uint32_t>(i % f.w), static_cast<uint32_t>(i / f.w), f.w, f.h, org, rd);
+    uint32_t nv = 0, nt = 0;
+    hit = trace<kLayout, false, false>(s, org, rd, 0.0f, __builtin_inff(), h, &nv, &nt);
+  }
+  waveCount(counters, kCntGiGbuf, valid);
+  if (!valid) return;
+  float4* pos = const_cast<float4*>(f.pos);
+  float4* nrm = const_cast<float4*>(f.nrm);
+  float4* geo = const_cast<float4*>(f.geoN);
+  float* dep = const_cast<float*>(f.depth);
+  const uint32_t kind = h.bits >> kKindShift, id = h.bits & kIdMask;
+  bool ok = hit;
+  V3 N = mk(0, 0, 0), Ng = mk(0, 0, 0), A = mk(0, 0, 0);
+  if (hit && kind == 0u) {
+    const uint4 tri = reinterpret_cast<const uint4*>(s.triIdx)[id];
+    const float4* vn = reinterpret_cast<const float4*>(s.vtxNrm);
+    const float4* vc = reinterpret_cast<const float4*>(s.vtxCol);
+    const float4 n0 = vn[tri.x], n1 = vn[tri.y], n2 = vn[tri.z];
+    const float4 c0 = vc[tri.x], c1 = vc[tri.y], c2 = vc[tri.z];
+    const float w0 = 1.0f - h.u - h.v;
+    N = safeNormalize(mk(w0 * n0.x + h.u * n1.x + h.v * n2.x, w0 * n0.y + h.u * n1.y + h.v * n2.y,
+                         w0 * n0.z + h.u * n1.z + h.v * n2.z));
+    Ng = safeNormalize(h.ng);
+    const float kd = s.meshMats[tri.w & kInfoMatMask].diffuse;
+    A = mk(kd * (w0 * c0.x + h.u * c1.x + h.v * c2.x), kd * (w0 * c0.y + h.u * c1.y + h.v * c2.y),
+           kd * (w0 * c0.z + h.u * c1.z + h.v * c2.z));
+  } else if (hit) {
+    float4 col;
+    uint32_t info;
+    if (kind == 1u) {
+      col = reinterpret_cast<const float4*>(s.sphColor)[id];
+      info = s.sphInfo[id];
+    } else if (kind == 3u) {
+      col = reinterpret_cast<const float4*>(s.coneColor)[id];
+      info = s.coneInfo[id];
+    } else {
+      col = reinterpret_cast<const float4*>(s.rndColor)[id];
+      info = s.rndInfo[id];
+    }
+    ok = (info & kInfoFromEdge) == 0u;
+    N = safeNormalize(h.ng);
+    Ng = N;
+    const float kd = s.primMats[info & kInfoMatMask].diffuse;
+    A = mk(kd * col.x, kd * col.y, kd * col.z);
+  }
+  if (!ok) {
+    pos[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    nrm[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    geo[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    albedo[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    dep[i] = 0.0f;
+    return;
+  }
+  if (dot(N, rd) > 0.0f) N = neg(N);
+  if (dot(Ng, rd) > 0.0f) Ng = neg(Ng);
+  pos[i] = make_float4(org.x + rd.x * h.t, org.y + rd.y * h.t, org.z + rd.z * h.t, 0.0f);
+  nrm[i] = make_float4(N.x, N.y, N.z, 1.0f);
+  geo[i] = make_float4(Ng.x, Ng.y, Ng.z, 0.0f);
+  albedo[i] = make_float4(A.x, A.y, A.z, 0.0f);
+  dep[i] = h.t;
 }
 
 // ---- pt1 E denoise: denoise_atrous.cpp as denoisePt1E runs it (no albedo
-// demodulation; guides = the normal AOV and the first-hit position) ----
+// demodulation; guides = the normal and the first-hit position of the grid) ----
 constexpr float kAtrousEps = 1.0e-3f;
 
 __device__ __forceinline__ float lum709(float4 v) { return 0.2126f * v.x + 0.7152f * v.y + 0.0722f * v.z; }
-__device__ __forceinline__ bool atrousValid(const FrameBufs& b, int p) {
-  const float4 n = b.giNrm[p];
+__device__ __forceinline__ bool atrousValid(const float4* nrm, int p) {
+  const float4 n = nrm[p];
   return (n.x * n.x + n.y * n.y + n.z * n.z) > 0.25f;
 }
 __device__ __forceinline__ float posDist(const float4 a, const float4 c) {
@@ -1234,28 +1310,28 @@ __device__ __forceinline__ float posDist(const float4 a, const float4 c) {
 }
 
 // pt1_denoise.cpp:26-29 NaN/Inf scrub into the working buffer
-__global__ void k_atrous_scrub(int n, FrameBufs b, float4* __restrict__ irr) {
+__global__ void k_atrous_scrub(int n, const float4* __restrict__ E, float4* __restrict__ irr) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const float4 e = b.giE[i];
+  const float4 e = E[i];
   irr[i] = make_float4(isfinite(e.x) ? e.x : 0.0f, isfinite(e.y) ? e.y : 0.0f, isfinite(e.z) ? e.z : 0.0f, 0.0f);
 }
 
 // denoise_atrous.cpp:79-128: per valid pixel, the largest world-space step to a
 // valid 4-neighbour (floor 1e-3) and the 3x3 luminance variance
-__global__ void k_atrous_stats(int W, int H, FrameBufs b, const float4* __restrict__ irr, float* __restrict__ grad,
-                               float* __restrict__ var) {
+__global__ void k_atrous_stats(int W, int H, const float4* __restrict__ nrm, const float4* __restrict__ pos,
+                               const float4* __restrict__ irr, float* __restrict__ grad, float* __restrict__ var) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= W * H || !atrousValid(b, p)) return;
+  if (p >= W * H || !atrousValid(nrm, p)) return;
   const int x = p % W, y = p / W;
-  const float4 pp = b.giPos[p];
+  const float4 pp = pos[p];
   float g = kAtrousEps;
   const int nb[4][2] = {{1, 0}, {-1, 0}, {0, 1}, {0, -1}};
   for (int k = 0; k < 4; ++k) {
     const int qx = x + nb[k][0], qy = y + nb[k][1];
     if (qx < 0 || qx >= W || qy < 0 || qy >= H) continue;
     const int q = qy * W + qx;
-    if (atrousValid(b, q)) g = fmaxf(g, posDist(pp, b.giPos[q]));
+    if (atrousValid(nrm, q)) g = fmaxf(g, posDist(pp, pos[q]));
   }
   grad[p] = g;
   float s1 = 0.0f, s2 = 0.0f;
@@ -1265,7 +1341,7 @@ __global__ void k_atrous_stats(int W, int H, FrameBufs b, const float4* __restri
       const int qx = x + dx, qy = y + dy;
       if (qx < 0 || qx >= W || qy < 0 || qy >= H) continue;
       const int q = qy * W + qx;
-      if (!atrousValid(b, q)) continue;
+      if (!atrousValid(nrm, q)) continue;
       const float l = lum709(irr[q]);
       s1 += l;
       s2 += l * l;
@@ -1280,20 +1356,21 @@ __global__ void k_atrous_stats(int W, int H, FrameBufs b, const float4* __restri
 }
 
 // denoise_atrous.cpp:134-197: one a-trous iteration with holes of `step`
-__global__ void k_atrous_iter(int W, int H, int step, float sigmaZ, float sigmaN, float sigmaL, FrameBufs b,
+__global__ void k_atrous_iter(int W, int H, int step, float sigmaZ, float sigmaN, float sigmaL,
+                              const float4* __restrict__ nrm, const float4* __restrict__ pos,
                               const float4* __restrict__ in, const float* __restrict__ grad,
                               const float* __restrict__ var, float4* __restrict__ out) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= W * H) return;
   const float4 ip = in[p];
-  if (!atrousValid(b, p)) {
+  if (!atrousValid(nrm, p)) {
     out[p] = ip;
     return;
   }
   const int x = p % W, y = p / W;
   const float h5[5] = {0.0625f, 0.25f, 0.375f, 0.25f, 0.0625f};
-  const float4 np = b.giNrm[p];
-  const float4 pp = b.giPos[p];
+  const float4 np = nrm[p];
+  const float4 pp = pos[p];
   const float lp = lum709(ip);
   const float lDenom = sigmaL * sqrtf(var[p]) + kAtrousEps;
   const float zDenom = sigmaZ * grad[p] * static_cast<float>(step) + kAtrousEps;
@@ -1305,13 +1382,13 @@ __global__ void k_atrous_iter(int W, int H, int step, float sigmaZ, float sigmaN
       const int qx = x + dx * step;
       if (qx < 0 || qx >= W) continue;
       const int q = qy * W + qx;
-      if (!atrousValid(b, q)) continue;
-      const float4 nq = b.giNrm[q];
+      if (!atrousValid(nrm, q)) continue;
+      const float4 nq = nrm[q];
       const float4 iq = in[q];
       float w = h5[dx + 2] * h5[dy + 2];
       const float nd = fmaxf(0.0f, np.x * nq.x + np.y * nq.y + np.z * nq.z);
       w *= powf(nd, sigmaN);
-      w *= expf(-posDist(pp, b.giPos[q]) / zDenom);
+      w *= expf(-posDist(pp, pos[q]) / zDenom);
       w *= expf(-fabsf(lp - lum709(iq)) / lDenom);
       s0 += w * iq.x;
       s1 += w * iq.y;
@@ -1324,37 +1401,165 @@ __global__ void k_atrous_iter(int W, int H, int step, float sigmaZ, float sigmaN
 
 // denoise_atrous.cpp:199-208 write-back (valid pixels clamped at 0), into E
 // (the first-bounce hit fraction in E.w stays)
-__global__ void k_atrous_store(int n, FrameBufs b, const float4* __restrict__ irr) {
+__global__ void k_atrous_store(int n, const float4* __restrict__ nrm, const float4* __restrict__ irr,
+                               float4* __restrict__ E) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= n) return;
   float4 v = irr[p];
-  if (atrousValid(b, p)) v = make_float4(fmaxf(0.0f, v.x), fmaxf(0.0f, v.y), fmaxf(0.0f, v.z), 0.0f);
-  b.giE[p] = make_float4(v.x, v.y, v.z, b.giE[p].w);
+  if (atrousValid(nrm, p)) v = make_float4(fmaxf(0.0f, v.x), fmaxf(0.0f, v.y), fmaxf(0.0f, v.z), 0.0f);
+  E[p] = make_float4(v.x, v.y, v.z, E[p].w);
 }
 
-// pt1GatherPoint (pt1_gather.hpp:228-485) for one receiver per lane: spp
-// cosine paths from the first hit, each vertex adding its shadow-tested
+// upsampleJointBilateral (pt1_upsample.hpp:35-176): per receiver of the
+// render grid, the 2x2 reduced-grid neighbourhood weighted bilinear x
+// max(0, N.Nh)^normalPow x exp(-|z - zh| / (depthScale z + 1e-6)); when all
+// weights die, the nearest valid quad pixel, else (fallbackRadius > 0) the
+// nearest valid pixel of the wider window; `patch` marks the pixels whose
+// weight sum is below patchThresh or died (the rims re-gathered at full res).
+__global__ void k_gi_upsample(FrameConst fc, FrameBufs b, GiField low, float normalPow, float depthScale,
+                              int fallbackRadius, uint8_t* __restrict__ patch, float patchThresh) {
+  const int pix = blockIdx.x * blockDim.x + threadIdx.x;
+  if (pix >= fc.W * fc.H) return;
+  const int W = fc.W, H = fc.H, hw = low.w, hh = low.h;
+  if (patch) patch[pix] = 0;
+  const float4 nf = b.giNrm[pix];
+  if (nf.w == 0.0f) {
+    b.giE[pix] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    return;
+  }
+  const int px = pix % W, py = pix / W;
+  const float sy = (static_cast<float>(py) + 0.5f) * static_cast<float>(hh) / static_cast<float>(H) - 0.5f;
+  const int by = static_cast<int>(floorf(sy));
+  const float fy = sy - static_cast<float>(by);
+  const float sx = (static_cast<float>(px) + 0.5f) * static_cast<float>(hw) / static_cast<float>(W) - 0.5f;
+  const int bx = static_cast<int>(floorf(sx));
+  const float fx = sx - static_cast<float>(bx);
+  const float zf = b.depth[pix];
+  float wSum = 0.0f, e0 = 0.0f, e1 = 0.0f, e2 = 0.0f, oSum = 0.0f, bestBilin = -1.0f;
+  int bestPix = -1;
+  for (int dy = 0; dy < 2; ++dy)
+    for (int dx = 0; dx < 2; ++dx) {
+      int hx = bx + dx, hy = by + dy;
+      if (hx < 0) hx = 0;
+      if (hy < 0) hy = 0;
+      if (hx >= hw) hx = hw - 1;
+      if (hy >= hh) hy = hh - 1;
+      const int hp = hy * hw + hx;
+      const float4 nh = low.nrm[hp];
+      if (nh.w == 0.0f) continue;
+      const float wb = (dx ? fx : 1.0f - fx) * (dy ? fy : 1.0f - fy);
+      if (wb > bestBilin) {
+        bestBilin = wb;
+        bestPix = hp;
+      }
+      const float wn = powf(fmaxf(0.0f, nf.x * nh.x + nf.y * nh.y + nf.z * nh.z), normalPow);
+      const float zh = low.depth[hp];
+      const float wz = expf(-fabsf(zf - zh) / (depthScale * zf + 1e-6f));
+      const float w = wb * wn * wz;
+      const float4 eh = low.E[hp];
+      wSum += w;
+      e0 += w * eh.x;
+      e1 += w * eh.y;
+      e2 += w * eh.z;
+      oSum += w * eh.w;
+    }
+  float4 out = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  if (wSum >= 1e-6f) {
+    if (patch && wSum < patchThresh) patch[pix] = 1;
+    const float inv = 1.0f / wSum;
+    out = make_float4(e0 * inv, e1 * inv, e2 * inv, oSum * inv);
+  } else if (bestPix >= 0) {
+    if (patch) patch[pix] = 1;
+    out = low.E[bestPix];
+  } else {
+    if (patch) patch[pix] = 1;
+    int found = -1;
+    if (fallbackRadius > 0) {
+      float bestD2 = 1e30f;
+      for (int wy = by - fallbackRadius + 1; wy <= by + fallbackRadius; ++wy) {
+        if (wy < 0 || wy >= hh) continue;
+        for (int wx = bx - fallbackRadius + 1; wx <= bx + fallbackRadius; ++wx) {
+          if (wx < 0 || wx >= hw) continue;
+          const int hp = wy * hw + wx;
+          if (low.nrm[hp].w == 0.0f) continue;
+          const float ddx = static_cast<float>(wx) - sx, ddy = static_cast<float>(wy) - sy;
+          const float d2 = ddx * ddx + ddy * ddy;
+          if (d2 < bestD2) {
+            bestD2 = d2;
+            found = hp;
+          }
+        }
+      }
+    }
+    if (found >= 0) out = low.E[found];
+  }
+  b.giE[pix] = out;
+}
+
+// smoothPatchedPixels (pt1_upsample.hpp:188-245), radius 2: a patched pixel's
+// E (and hit fraction) becomes the guide-weighted mean of its receiver
+// neighbourhood in the snapshot, itself at weight 1, taps below 1e-4 skipped.
+__global__ void k_gi_smooth_patched(FrameConst fc, FrameBufs b, const uint8_t* __restrict__ patch, float normalPow,
+                                    float depthScale, const float4* __restrict__ Esrc) {
+  const int pix = blockIdx.x * blockDim.x + threadIdx.x;
+  if (pix >= fc.W * fc.H || !patch[pix]) return;
+  const int W = fc.W, H = fc.H, px = pix % W, py = pix / W;
+  const float4 nf = b.giNrm[pix];
+  const float zf = b.depth[pix];
+  const float4 es = Esrc[pix];
+  float wSum = 1.0f, e0 = es.x, e1 = es.y, e2 = es.z, oSum = es.w;
+  for (int dy = -2; dy <= 2; ++dy) {
+    const int ny = py + dy;
+    if (ny < 0 || ny >= H) continue;
+    for (int dx = -2; dx <= 2; ++dx) {
+      if (dx == 0 && dy == 0) continue;
+      const int nx = px + dx;
+      if (nx < 0 || nx >= W) continue;
+      const int q = ny * W + nx;
+      const float4 nn = b.giNrm[q];
+      if (nn.w == 0.0f) continue;
+      const float wn = powf(fmaxf(0.0f, nf.x * nn.x + nf.y * nn.y + nf.z * nn.z), normalPow);
+      const float zn = b.depth[q];
+      const float wz = expf(-fabsf(zf - zn) / (depthScale * zf + 1e-6f));
+      const float w = wn * wz;
+      if (w < 1e-4f) continue;
+      const float4 eq = Esrc[q];
+      wSum += w;
+      e0 += w * eq.x;
+      e1 += w * eq.y;
+      e2 += w * eq.z;
+      oSum += w * eq.w;
+    }
+  }
+  const float inv = 1.0f / wSum;
+  b.giE[pix] = make_float4(e0 * inv, e1 * inv, e2 * inv, oSum * inv);
+}
+
+// pt1GatherPoint (pt1_gather.hpp:228-485) for one listed pixel per lane: spp
+// cosine paths from its first hit, each vertex adding its shadow-tested
 // direct light (pt1EvalVertex, :117-212) times the path throughput, Russian
 // roulette from the second vertex, the environment at an escape; the
 // samples are summed in order, so E is the oracle's to the last bit of the
 // same float operations. E.w = the first-bounce hit fraction.
 template <bool kCount, int kLayout>
-__global__ __launch_bounds__(kTraceBlock) void k_gi_gather(FrameConst fc, SceneView s, FrameBufs b) {
+__global__ __launch_bounds__(kTraceBlock) void k_gi_gather(FrameConst fc, SceneView s, FrameBufs b, GiField gf) {
   const int tid = blockIdx.x * blockDim.x + threadIdx.x;
   const int part = tid / b.partCap;
-  const bool valid = part < kQueueParts && static_cast<uint32_t>(tid % b.partCap) < queueFill(b.counters, kGiFill, part);
+  const bool valid = part < kQueueParts && static_cast<uint32_t>(tid % b.partCap) < queueFill(b.counters, gf.fillBase, part);
   uint32_t nGi = 0, nNee = 0, nv = 0, nt = 0;
   if (valid) {
-    const uint32_t pix = b.giList[tid];
-    const V3 P = ld3(b.giPos[pix]);
-    const V3 N = ld3(b.giNrm[pix]);
-    const V3 Ng = N;  // full-res gather: the render G-buffer has no geometric normal
-    const float epsT = b.depth[pix];
+    const uint32_t pix = gf.list[tid];
+    const V3 P = ld3(gf.pos[pix]);
+    const V3 N = ld3(gf.nrm[pix]);
+    // the render grid's G-buffer has no geometric normal: the horizon test
+    // then uses the shading normal (gatherPt1Grid, pt1_gather.hpp:570-573)
+    const V3 Ng = gf.geoN ? ld3(gf.geoN[pix]) : N;
+    const float epsT = gf.depth[pix];
     const uint32_t seed = hashU32(pix ^ fc.giSeedMix);
     const Frame3 f = frameFromNormal(N);
     const float eps = selfIntersectEps(P, N, epsT);
     const V3 O = P + N * eps;
-    const int spp = fc.giSpp;
+    const int spp = gf.spp;
     float cpx = 0.0f, cpy = 0.0f;
     if (fc.giLd) {
       uint32_t c0 = seed, c1 = 0x9E3779B9u;
@@ -1487,7 +1692,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_gi_gather(FrameConst fc, SceneV
       Esum.z += L.z;
     }
     const float invN = 1.0f / static_cast<float>(spp);
-    b.giE[pix] = make_float4(Esum.x * invN, Esum.y * invN, Esum.z * invN, static_cast<float>(nOcc) * invN);
+    gf.E[pix] = make_float4(Esum.x * invN, Esum.y * invN, Esum.z * invN, static_cast<float>(nOcc) * invN);
   }
   {
     const uint32_t g = waveSum(nGi), ne = waveSum(nNee);
@@ -1833,35 +2038,61 @@ void launchResolve(const FrameConst& fc, const SceneView& s, const FrameBufs& b,
   const int64_t slots = static_cast<int64_t>(kQueueParts) * b.partCap;
   UBG_DISPATCH(k_resolve, blocksFor(slots, kTraceBlock), countVisits, fc, s, b, iter, 0);
 }
-void launchGiList(const FrameConst& fc, const FrameBufs& b, hipStream_t st) {
-  // the same block -> partition map as a first primary launch over W x H
-  const int64_t n = static_cast<int64_t>(fc.W) * fc.H;
-  hipLaunchKernelGGL(k_gi_list, dim3(blocksFor(n, kTraceBlock)), dim3(kTraceBlock), 0, st, fc, b);
+void launchGiList(const GiField& f, const uint8_t* mask, bool zeroOthers, const FrameBufs& b, hipStream_t st) {
+  // (the same block -> partition map as a first primary launch over W x H:
+  // every grid here has at most W x H pixels, so partitions never overflow)
+  const int64_t n = static_cast<int64_t>(f.w) * f.h;
+  hipLaunchKernelGGL(k_gi_list, dim3(blocksFor(n, kTraceBlock)), dim3(kTraceBlock), 0, st, f, mask,
+                     zeroOthers ? 1 : 0, b);
 }
-void launchGiGather(const FrameConst& fc, const SceneView& s, const FrameBufs& b, bool countVisits, hipStream_t st) {
+void launchGiGbuffer(const FrameConst& fc, const SceneView& s, const GiField& f, float4* albedo,
+                     unsigned long long* counters, hipStream_t st) {
+  const int64_t n = static_cast<int64_t>(f.w) * f.h;
+  const size_t lds = stackLdsBytes(s, kTraceBlock);
+  if (s.layout == kLayoutBvh8)
+    hipLaunchKernelGGL(k_gi_gbuf<kLayoutBvh8>, dim3(blocksFor(n, kTraceBlock)), dim3(kTraceBlock), lds, st, fc, s, f,
+                       albedo, counters);
+  else
+    hipLaunchKernelGGL(k_gi_gbuf<kLayoutBvh2>, dim3(blocksFor(n, kTraceBlock)), dim3(kTraceBlock), lds, st, fc, s, f,
+                       albedo, counters);
+}
+void launchGiGather(const FrameConst& fc, const SceneView& s, const FrameBufs& b, const GiField& f, bool countVisits,
+                    hipStream_t st) {
   const int64_t slots = static_cast<int64_t>(kQueueParts) * b.partCap;
-  UBG_DISPATCH(k_gi_gather, blocksFor(slots, kTraceBlock), countVisits, fc, s, b);
+  UBG_DISPATCH(k_gi_gather, blocksFor(slots, kTraceBlock), countVisits, fc, s, b, f);
 }
-void launchGiDenoise(const FrameConst& fc, const FrameBufs& b, int iters, float sigmaZ, float sigmaN, float sigmaL,
-                     float4* irrA, float4* irrB, float* grad, float* var, hipStream_t st) {
+void launchGiDenoise(const GiField& f, int iters, float sigmaZ, float sigmaN, float sigmaL, float4* irrA,
+                     float4* irrB, float* grad, float* var, hipStream_t st) {
   iters = iters < 0 ? 0 : iters;
   if (iters == 0) return;  // denoise_atrous.cpp:32-33
-  const int n = fc.W * fc.H;
+  const int n = f.w * f.h;
   const unsigned g = blocksFor(n, 256);
   sigmaZ = sigmaZ > 1.0e-4f ? sigmaZ : 1.0e-4f;
   sigmaL = sigmaL > 1.0e-4f ? sigmaL : 1.0e-4f;
-  hipLaunchKernelGGL(k_atrous_scrub, dim3(g), dim3(256), 0, st, n, b, irrA);
-  hipLaunchKernelGGL(k_atrous_stats, dim3(g), dim3(256), 0, st, fc.W, fc.H, b, irrA, grad, var);
+  hipLaunchKernelGGL(k_atrous_scrub, dim3(g), dim3(256), 0, st, n, f.E, irrA);
+  hipLaunchKernelGGL(k_atrous_stats, dim3(g), dim3(256), 0, st, f.w, f.h, f.nrm, f.pos, irrA, grad, var);
   float4* in = irrA;
   float4* out = irrB;
   for (int it = 0; it < iters; ++it) {
-    hipLaunchKernelGGL(k_atrous_iter, dim3(g), dim3(256), 0, st, fc.W, fc.H, 1 << it, sigmaZ, sigmaN, sigmaL, b,
-                       in, grad, var, out);
+    hipLaunchKernelGGL(k_atrous_iter, dim3(g), dim3(256), 0, st, f.w, f.h, 1 << it, sigmaZ, sigmaN, sigmaL, f.nrm,
+                       f.pos, in, grad, var, out);
     float4* t = in;
     in = out;
     out = t;
   }
-  hipLaunchKernelGGL(k_atrous_store, dim3(g), dim3(256), 0, st, n, b, in);
+  hipLaunchKernelGGL(k_atrous_store, dim3(g), dim3(256), 0, st, n, f.nrm, in, f.E);
+}
+void launchGiUpsample(const FrameConst& fc, const FrameBufs& b, const GiField& low, float normalPow,
+                      float depthScale, int fallbackRadius, uint8_t* patch, float patchThresh, hipStream_t st) {
+  const int64_t n = static_cast<int64_t>(fc.W) * fc.H;
+  hipLaunchKernelGGL(k_gi_upsample, dim3(blocksFor(n, 256)), dim3(256), 0, st, fc, b, low, normalPow, depthScale,
+                     fallbackRadius, patch, patchThresh);
+}
+void launchGiSmoothPatched(const FrameConst& fc, const FrameBufs& b, const uint8_t* patch, float normalPow,
+                           float depthScale, const float4* Esrc, hipStream_t st) {
+  const int64_t n = static_cast<int64_t>(fc.W) * fc.H;
+  hipLaunchKernelGGL(k_gi_smooth_patched, dim3(blocksFor(n, 256)), dim3(256), 0, st, fc, b, patch, normalPow,
+                     depthScale, Esrc);
 }
 void launchGiComposite(const FrameConst& fc, const FrameBufs& b, float* indirect, float* occlusion, hipStream_t st) {
   const int64_t n = static_cast<int64_t>(fc.W) * fc.H;

@@ -201,8 +201,9 @@ enum Counter {
   kCntCapped = 5,
   kCntGiRays = 6,     // GI gather / continuation closest-hit queries
   kCntGiNee = 7,      // GI next-event shadow queries
-  kCntNodes = 8,      // node visits per ray class: 8 primary, 9 AO, 10 shadow
+  kCntNodes = 8,      // node visits per ray class: 8 primary, 9 AO, 10 shadow      // node visits per ray class: 8 primary, 9 AO, 10 shadow
   kCntPrimTests = 11, // primitive tests per ray class: 11, 12, 13
+  kCntGiGbuf = 14,    // GI reduced-grid first-hit queries
   kNumCounters = 16
 };
 
@@ -216,6 +217,8 @@ constexpr int kAaRepFill = kAaFlagFill + kLineWords;
 // GI receiver list (hi-res pixels whose first hit gathers), partitioned like
 // the hit queue
 constexpr int kGiFill = kAaRepFill + kLineWords;
-constexpr int kCounterWords = kGiFill + kLineWords * kQueueParts;
+// GI rim-patch list (reduced gather grids), partitioned the same way
+constexpr int kGiPatchFill = kGiFill + kLineWords * kQueueParts;
+constexpr int kCounterWords = kGiPatchFill + kLineWords * kQueueParts;
 
 }  // namespace ubg

@@ -116,8 +116,8 @@ class RenderOptions:
     aa_depth: int = 0
     aa_debug: bool = False  # fill FrameResult.aa_mask
     # path-traced GI (render_options.hpp:93-162); the gfx950 path implements
-    # the pt1 integrator (gi_integrator 1) over a full-res gather grid (E
-    # denoise: the a-trous filter), and refuses other combinations
+    # the pt1 integrator (gi_integrator 1; E denoise = the a-trous filter) and
+    # refuses the pt2 and cache integrators
     gi: bool = False
     gi_bounces: int = 1
     gi_max_distance: float = 0.0
@@ -136,6 +136,11 @@ class RenderOptions:
     denoise_sigma_z: float = 1.0
     denoise_sigma_n: float = 128.0
     denoise_sigma_l: float = 4.0
+    pt1_upsample_normal_pow: float = 32.0
+    pt1_upsample_depth_scale: float = 0.02
+    pt1_edge_patch: bool = True
+    pt1_edge_patch_spp_mul: int = 4
+    pt1_edge_patch_thresh: float = 0.3
     # options the gfx950 path does not implement; must stay at their defaults
     denoiser: int = 0
     env_lights: int = 0
@@ -403,6 +408,11 @@ def params(scene: Scene, opt: RenderOptions, flags: int = 0,
     p.denoise_sigma_z = float(opt.denoise_sigma_z)
     p.denoise_sigma_n = float(opt.denoise_sigma_n)
     p.denoise_sigma_l = float(opt.denoise_sigma_l)
+    p.pt1_upsample_normal_pow = float(opt.pt1_upsample_normal_pow)
+    p.pt1_upsample_depth_scale = float(opt.pt1_upsample_depth_scale)
+    p.pt1_edge_patch = int(bool(opt.pt1_edge_patch))
+    p.pt1_edge_patch_spp_mul = int(opt.pt1_edge_patch_spp_mul)
+    p.pt1_edge_patch_thresh = float(opt.pt1_edge_patch_thresh)
     p.flags = flags
     if cancel_flag is not None:
         keep.append(cancel_flag)
