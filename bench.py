@@ -114,7 +114,7 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", choices=["ballstick", "spheres", "surface"], default="ballstick")
+    ap.add_argument("--workload", choices=["ballstick", "spheres", "surface", "gi"], default="ballstick")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--layout", choices=["bvh8", "bvh8gpu", "bvh2"], default="bvh8",
@@ -144,6 +144,13 @@ def main() -> int:
         workload = (f"molecular surface: Gaussian isosurface (sigma 1 A, iso 0.5, 0.5 A grid) of a "
                     f"{len(scene.sphere_center)}-atom protein, {len(scene.index)} indexed triangles at alpha 0.6 "
                     f"over {len(scene.sphere_center)} CPK spheres, 1920x1080 ss1, AO16 (8 A), hard shadows")
+    elif args.workload == "gi":
+        scene, opt = molecular_surface_scene()
+        opt.ao_samples = 0
+        opt.gi, opt.gi_integrator, opt.pt1_spp, opt.gi_bounces, opt.pt1_denoise = True, 1, 8, 2, True
+        workload = (f"pt1 GI on the molecular surface: {len(scene.index)} indexed triangles at alpha 0.6 over "
+                    f"{len(scene.sphere_center)} CPK spheres, 1920x1080 ss1, full-res gather 8 spp, 2 bounces, "
+                    f"NEE + hard shadows, a-trous E denoise")
     else:
         scene, opt = spheres_scene()
         workload = "2,000 CPK spheres, 800x600 ss1, hard shadows, no AO"
@@ -160,7 +167,8 @@ def main() -> int:
     st = vc.stats
     rays_per_frame = vc.total_rays
     log(f"rays/frame {rays_per_frame}: primary {st['primary_rays']}, AO traced {st['ao_rays_traced']} "
-        f"sampled {st['ao_rays_sampled']}, shadow {st['shadow_rays']}")
+        f"sampled {st['ao_rays_sampled']}, shadow {st['shadow_rays']}, GI gather {st['gi_rays']} "
+        f"NEE {st['gi_nee_rays']} G-buffer {st['gi_gbuffer_rays']}")
 
     for _ in range(args.warmup):
         bs.render(scene, opt, outputs=False)
@@ -182,7 +190,9 @@ def main() -> int:
     # roofline of the dominant kernel
     dom = max(("primary", "ao", "resolve"), key=lambda k: ks[k])
     cls = {"primary": 0, "ao": 1, "resolve": 2}[dom]
-    rays_in = {"primary": st["primary_rays"], "ao": st["ao_rays_traced"] + st["ao_rays_sampled"],
+    # (the GI gather runs in the AO slot and counts its visits there)
+    rays_in = {"primary": st["primary_rays"],
+               "ao": st["ao_rays_traced"] + st["ao_rays_sampled"] + st["gi_rays"] + st["gi_nee_rays"],
                "resolve": st["shadow_rays"]}[dom]
     nodes, tests = st["node_visits"][cls], st["prim_tests"][cls]
     node_b, prim_b = info["node_bytes"], info["prim_bytes"]
@@ -192,6 +202,8 @@ def main() -> int:
     achieved = bytes_per_frame / dur_s / 1e9
     kname = {"primary": "k_primary", "ao": "k_ao" if args.layout == "bvh2" else "k_ao_persistent",
              "resolve": "k_resolve"}[dom]
+    if args.workload == "gi" and dom == "ao":
+        kname = "k_gi_gather"
     per_launch = measured_traffic(args.workload, args.layout, kname)
     launches = kn[dom] // args.steps
     roofline = {
@@ -202,8 +214,8 @@ def main() -> int:
         "kernel_ms_per_frame": round(dur_s * 1e3, 4),
         "launches_per_frame": kn[dom] // args.steps,
         "algorithmic_bytes_per_frame": int(bytes_per_frame),
-        "per_ray": {"node_visits": round(nodes / max(1, st["ao_rays_traced"] if dom == "ao" else rays_in), 2),
-                    "prim_tests": round(tests / max(1, st["ao_rays_traced"] if dom == "ao" else rays_in), 2),
+        "per_ray": {"node_visits": round(nodes / max(1, rays_in - st["ao_rays_sampled"] if dom == "ao" else rays_in), 2),
+                    "prim_tests": round(tests / max(1, rays_in - st["ao_rays_sampled"] if dom == "ao" else rays_in), 2),
                     "node_bytes": node_b, "prim_bytes": prim_b, "ray_bytes": 48},
     }
 
@@ -233,7 +245,9 @@ def main() -> int:
             "rays_per_frame": rays_per_frame,
             "layers": fr.layers,
             "ray_split": {"primary": st["primary_rays"], "ao_traced": st["ao_rays_traced"],
-                          "ao_sampled": st["ao_rays_sampled"], "shadow": st["shadow_rays"]},
+                          "ao_sampled": st["ao_rays_sampled"], "shadow": st["shadow_rays"],
+                          **({"gi_gather": st["gi_rays"], "gi_nee": st["gi_nee_rays"],
+                              "gi_gbuffer": st["gi_gbuffer_rays"]} if args.workload == "gi" else {})},
             "bvh_layout": args.layout,
             "bvh_build_ms": round(info["build_seconds"] * 1e3, 2),
             "bvh_nodes": info["n_nodes"],

@@ -41,8 +41,11 @@ class FrameResult:
 
     @property
     def total_rays(self) -> int:
+        """Every ray of the frame: primary (with continuations), AO (traced and
+        sampled), shadow, and the GI gather, next-event and G-buffer rays."""
         s = self.stats
-        return int(s["primary_rays"] + s["ao_rays_traced"] + s["ao_rays_sampled"] + s["shadow_rays"])
+        return int(s["primary_rays"] + s["ao_rays_traced"] + s["ao_rays_sampled"] + s["shadow_rays"]
+                   + s.get("gi_rays", 0) + s.get("gi_nee_rays", 0) + s.get("gi_gbuffer_rays", 0))
 
 
 class Device:
