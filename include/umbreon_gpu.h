@@ -219,8 +219,8 @@ typedef struct ubg_render_params {
   int32_t aa_depth;
   /* Path-traced diffuse GI, the pt1 integrator (render_options.hpp:93-162,
    * integrator/pt1/{pt1_gather,pt1_gbuffer,pt1_upsample}.hpp, pt1_denoise.cpp,
-   * embree_renderer.cpp:560-1131). gi != 0 requires gi_integrator == 1
-   * (otherwise UBG_ERR_UNSUPPORTED). The gather grid is the render grid
+   * embree_renderer.cpp:560-1131). gi != 0 requires gi_integrator 1 or 2
+   * (the irradiance cache, 0, is UBG_ERR_UNSUPPORTED). The gather grid is the render grid
    * divided by pt1_gather_div (-1 = the supersample factor, 0 = 2 with
    * pt1_half_res else 1); a divisor > 1 traces its own first-hit G-buffer,
    * gathers (and denoises) there, joint-bilateral upsamples E to the render
@@ -255,6 +255,20 @@ typedef struct ubg_render_params {
   int32_t pt1_edge_patch;            /* default 1 */
   int32_t pt1_edge_patch_spp_mul;    /* default 4 */
   float pt1_edge_patch_thresh;       /* default 0.3 */
+  /* pt2 (gi_integrator 2, the reference default; render_options.hpp:184-253,
+   * integrator/pt2/): the pt1 gather with the first bounce drawn from the
+   * Owen-scrambled Sobol sequence (pt2_pattern 0 per pixel, 1 blue noise),
+   * area lights sampled by one cone ray per path vertex, and (pt2_emissive)
+   * emission transported at the vertices. ReSTIR (pt2_rounds > 0), adaptive
+   * spp, emissive-triangle NEE (pt2_emissive_nee with an emissive material
+   * in the scene) and traced reflections (pt2_reflect with a reflective
+   * material) are not implemented: UBG_ERR_UNSUPPORTED. */
+  int32_t pt2_pattern;               /* default 1 */
+  int32_t pt2_emissive;              /* default 1 */
+  int32_t pt2_rounds;                /* default 0 */
+  int32_t pt2_adaptive;              /* default 0 */
+  int32_t pt2_emissive_nee;          /* default 1 */
+  int32_t pt2_reflect;               /* default 1 */
 } ubg_render_params;
 
 /* Ray accounting of one frame. A pure function of the scene and options:

@@ -70,6 +70,11 @@ def load():
     lib.orc_pt1_denoise.argtypes = [C.c_int32, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_float,
                                     C.c_float, C.c_float]
     lib.orc_pt1_denoise.restype = None
+    lib.orc_pt2_sobol_2d.argtypes = [C.c_uint32, C.c_uint32, C.POINTER(C.c_float), C.POINTER(C.c_float)]
+    lib.orc_pt2_sobol_2d.restype = None
+    lib.orc_pt2_pixel_sampler.argtypes = [C.c_int32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
+                                          C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]
+    lib.orc_pt2_pixel_sampler.restype = None
     _lib = lib
     return lib
 
@@ -247,3 +252,20 @@ def pt1_denoise(E, normal, position, iters: int = 5, sigma_z: float = 1.0, sigma
     lib.orc_pt1_denoise(w, h, E.ctypes.data, nr.ctypes.data, ps.ctypes.data, int(iters), float(sigma_z),
                         float(sigma_n), float(sigma_l))
     return E
+
+
+def pt2_sobol_2d(index: int, seed: int):
+    """pt2SobolBurley2D: the 2D Owen-scrambled Sobol point `index` under `seed`."""
+    lib = load()
+    a, b = C.c_float(), C.c_float()
+    lib.orc_pt2_sobol_2d(int(index) & 0xFFFFFFFF, int(seed) & 0xFFFFFFFF, C.byref(a), C.byref(b))
+    return a.value, b.value
+
+
+def pt2_pixel_sampler(pattern: int, px: int, py: int, frame_seed: int, pixel_hash: int, spp: int):
+    """pt2MakePixelSampler: (index base, scramble seed) of a gather pixel."""
+    lib = load()
+    base, seed = C.c_uint32(), C.c_uint32()
+    lib.orc_pt2_pixel_sampler(int(pattern), int(px), int(py), int(frame_seed) & 0xFFFFFFFF,
+                              int(pixel_hash) & 0xFFFFFFFF, int(spp), C.byref(base), C.byref(seed))
+    return base.value, seed.value

@@ -1592,20 +1592,98 @@ struct GiParams {
   float envIntensity = 1.0f;
   int bounces = 1;
   T maxDist;
+  // pt2 (giIntegrator 2): Sobol-Burley first bounce with this pixel
+  // arrangement, the path's stream for area-light NEE, emission at vertices
+  bool pt2 = false;
+  int pattern = 1;
+  bool emissive = true;
 };
+
+// pt2_sampler.hpp -- hash-based Owen-scrambled Sobol (Burley, "Practical
+// Hash-based Owen Scrambling", JCGT 9(4) 2020): the Laine-Karras style hash
+// on reversed bits, its base-4 variant for the Morton pixel shuffle, and the
+// second Sobol dimension's direction numbers (rows of Pascal's triangle mod 2
+// in the reversed-bit convention).
+inline uint32_t owenHashRev(uint32_t n, uint32_t seed) {
+  n ^= n * 0x3d20adeau;
+  n += seed;
+  n *= (seed >> 16) | 1u;
+  n ^= n * 0x05526c56u;
+  n ^= n * 0x53a22864u;
+  return n;
+}
+inline uint32_t owenHashRevBase4(uint32_t n, uint32_t seed) {
+  n ^= n * 0x3d20adeau;
+  n ^= (n >> 1) & (n << 1) & 0x55555555u;
+  n += seed;
+  n *= (seed >> 16) | 1u;
+  n ^= (n >> 1) & (n << 1) & 0x55555555u;
+  n ^= n * 0x05526c56u;
+  n ^= n * 0x53a22864u;
+  return n;
+}
+inline uint32_t interleaveBits16(uint32_t x) {
+  x &= 0x0000ffffu;
+  x = (x ^ (x << 8)) & 0x00ff00ffu;
+  x = (x ^ (x << 4)) & 0x0f0f0f0fu;
+  x = (x ^ (x << 2)) & 0x33333333u;
+  x = (x ^ (x << 1)) & 0x55555555u;
+  return x;
+}
+inline float sobolToUnit(uint32_t n) { return static_cast<float>(n) * (1.0f / 4294967808.0f); }
+inline float sobolBurleyDim(uint32_t revIdx, int dim, uint32_t seed) {
+  uint32_t r;
+  if (dim == 0) {
+    r = reverseBits32(revIdx);
+  } else {
+    r = 0;
+    uint32_t row = 1;  // Pascal's triangle mod 2, one row per bit from the top
+    for (int b = 31; b >= 0; --b) {
+      if ((revIdx >> b) & 1u) r ^= row;
+      row ^= row << 1;
+    }
+  }
+  return sobolToUnit(reverseBits32(owenHashRev(r, seed)));
+}
+inline void sobolBurley2D(uint32_t index, uint32_t seed, float& u1, float& u2) {
+  index = owenHashRev(reverseBits32(index), seed ^ 0xf8ade99au);
+  u1 = sobolBurleyDim(index, 0, seed ^ 0xe0aaaf76u);
+  u2 = sobolBurleyDim(index, 1, seed ^ 0x94964d4eu);
+}
+// pt2MakePixelSampler: blue noise = one global sequence, pixel sections of
+// sppPow2 samples ordered along the base-4 scrambled Morton curve; sobol = an
+// independent sequence per pixel
+inline void pt2PixelSampler(int pattern, uint32_t px, uint32_t py, uint32_t frameSeed, uint32_t pixelHash,
+                            uint32_t sppPow2, uint32_t& indexBase, uint32_t& seed) {
+  if (pattern == 1) {
+    const uint32_t m = (interleaveBits16(px) << 1) | interleaveBits16(py);
+    indexBase = reverseBits32(owenHashRevBase4(reverseBits32(m), frameSeed)) * sppPow2;
+    seed = frameSeed;
+  } else {
+    indexBase = 0;
+    seed = pixelHash;
+  }
+}
+inline uint32_t nextPow2(uint32_t n) {
+  uint32_t q = 1;
+  while (q < n) q <<= 1;
+  return q;
+}
 
 template <class T>
 struct Pt1Vertex {
-  V3<T> radiance, albedo, P, N;
+  V3<T> radiance, emitted, albedo, P, N;
   T tfar;
 };
 
 // pt1_gather.hpp:117-212 (pt1: no emission, the hard NEE ray per light)
 template <class T>
-static bool pt1EvalVertex(const Ctx<T>& c, const Hit<T>& h, V3<T> O, V3<T> wi, Pt1Vertex<T>& v, Counters& cnt) {
+static bool pt1EvalVertex(const Ctx<T>& c, const Hit<T>& h, V3<T> O, V3<T> wi, Pt1Vertex<T>& v, Counters& cnt,
+                          bool addEmission = false, uint32_t* softRng0 = nullptr, uint32_t* softRng1 = nullptr) {
   const Geometry& g = c.s->g;
   V3<T> Ny, Cy, NgS;
   T kd;
+  float em = 0.0f;
   if (h.kind == UBG_KIND_MESH) {
     const uint32_t* ix = &g.idx[3 * h.id];
     const T w0 = T(1) - h.u - h.v;
@@ -1619,6 +1697,7 @@ static bool pt1EvalVertex(const Ctx<T>& c, const Hit<T>& h, V3<T> O, V3<T> wi, P
     const int64_t baseTri = static_cast<int64_t>(h.id % static_cast<uint32_t>(g.baseT));
     const Mat& mat = g.triMat.empty() ? g.meshMats[0] : g.meshMats[g.triMat[baseTri]];
     kd = T(diffuseWeight(mat));
+    if (addEmission) em = mat.emission;
     NgS = h.ng;
   } else {
     const PrimSide* side = h.kind == UBG_KIND_SPHERE        ? &g.sphSide[h.id]
@@ -1628,6 +1707,7 @@ static bool pt1EvalVertex(const Ctx<T>& c, const Hit<T>& h, V3<T> O, V3<T> wi, P
     Ny = safeNormalize(h.ng);
     Cy = V3<T>{T(side->color[0]), T(side->color[1]), T(side->color[2])};
     kd = T(diffuseWeight(g.primMats[side->mat]));
+    if (addEmission) em = g.primMats[side->mat].emission;
     NgS = Ny;
   }
   if (dot(Ny, wi) > T(0)) Ny = neg(Ny);
@@ -1637,17 +1717,24 @@ static bool pt1EvalVertex(const Ctx<T>& c, const Hit<T>& h, V3<T> O, V3<T> wi, P
   for (const Light<T>& l : c.lights) {
     const T ndl = dot(Ny, l.L);
     if (ndl <= T(0)) continue;
-    // computeShadow, one hard ray (secondary_rays.hpp:118-124)
+    // computeShadow's one hard ray (secondary_rays.hpp:118-124), or (pt2, an
+    // area light) one cone-jittered ray drawn from the path's stream
     V3<T> ng = (dot(NgS, Ny) < T(0)) ? neg(NgS) : NgS;
     ng = safeNormalize(ng, Ny);
     const V3<T> So = Py + ng * eps;
+    V3<T> ldir = l.L;
+    if (softRng0 && softRng1 && l.radius > T(0)) {
+      tea2(*softRng0, *softRng1);
+      ldir = sampleLightDir(l, u32ToUnorm<T>(*softRng0), u32ToUnorm<T>(*softRng1));
+    }
     cnt.giNee++;
-    const T sh = occluded(*c.s, So, l.L, eps, std::numeric_limits<T>::infinity()) ? T(0) : T(1);
+    const T sh = occluded(*c.s, So, ldir, eps, std::numeric_limits<T>::infinity()) ? T(0) : T(1);
     E.x += ndl * l.color.x * sh;
     E.y += ndl * l.color.y * sh;
     E.z += ndl * l.color.z * sh;
   }
   v.radiance = V3<T>{kd * Cy.x * E.x, kd * Cy.y * E.y, kd * Cy.z * E.z};
+  v.emitted = em > 0.0f ? V3<T>{T(em) * Cy.x, T(em) * Cy.y, T(em) * Cy.z} : V3<T>{T(0), T(0), T(0)};
   v.albedo = V3<T>{kd * Cy.x, kd * Cy.y, kd * Cy.z};
   v.P = Py;
   v.N = Ny;
@@ -1658,7 +1745,8 @@ static bool pt1EvalVertex(const Ctx<T>& c, const Hit<T>& h, V3<T> O, V3<T> wi, P
 // pt1_gather.hpp:228-485 (pt1: no Sobol, reservoir, emissive NEE or halves)
 template <class T>
 static V3<T> pt1GatherPoint(const Ctx<T>& c, const GiParams<T>& gp, V3<T> P, V3<T> N, V3<T> Ng, int spp,
-                            uint32_t seed, T epsT, bool ld, float clampLum, T* outOcc, Counters& cnt) {
+                            uint32_t seed, T epsT, bool ld, float clampLum, T* outOcc, Counters& cnt,
+                            uint32_t sobolBase = 0, uint32_t sobolSeed = 0) {
   const Frame<T> f = frameFromNormal(N);
   const T eps = selfIntersectEps(P, N, epsT);
   const V3<T> O = P + N * eps;
@@ -1676,7 +1764,12 @@ static V3<T> pt1GatherPoint(const Ctx<T>& c, const GiParams<T>& gp, V3<T> P, V3<
     uint32_t s0 = seed, s1 = static_cast<uint32_t>(si);
     tea2(s0, s1);
     T u1, u2;
-    if (ld) {
+    if (gp.pt2) {
+      float a, b2;
+      sobolBurley2D(sobolBase + static_cast<uint32_t>(si), sobolSeed, a, b2);
+      u1 = T(a);
+      u2 = T(b2);
+    } else if (ld) {
       u1 = T(static_cast<float>(si)) / T(static_cast<float>(spp)) + cpx;
       u2 = T(static_cast<float>(reverseBits32(static_cast<uint32_t>(si)))) * T(0x1.0p-32f) + cpy;
       if (u1 >= T(1)) u1 -= T(1);
@@ -1706,10 +1799,12 @@ static V3<T> pt1GatherPoint(const Ctx<T>& c, const GiParams<T>& gp, V3<T> P, V3<
       }
       if (b == 1) ++nOcc;
       Pt1Vertex<T> v;
-      if (!pt1EvalVertex(c, h, org, wi, v, cnt)) break;
-      L.x += tp.x * v.radiance.x;
-      L.y += tp.y * v.radiance.y;
-      L.z += tp.z * v.radiance.z;
+      if (!pt1EvalVertex(c, h, org, wi, v, cnt, gp.pt2 && gp.emissive, gp.pt2 ? &s0 : nullptr,
+                         gp.pt2 ? &s1 : nullptr))
+        break;
+      L.x += tp.x * (v.radiance.x + v.emitted.x);
+      L.y += tp.y * (v.radiance.y + v.emitted.y);
+      L.z += tp.z * (v.radiance.z + v.emitted.z);
       if (b == maxB) break;
       tp.x *= v.albedo.x;
       tp.y *= v.albedo.y;
@@ -1970,10 +2065,34 @@ static void denoiseAtrous(int W, int H, std::vector<float>& E, const std::vector
       for (int k = 0; k < 3; ++k) E[3 * p + k] = std::max(0.0f, irr[3 * p + k]);
 }
 
-// GI integrators the restatement covers: pt1
-static bool giSupported(const ubg_render_params* p, int ss) {
-  (void)ss;
-  return !p->gi || p->gi_integrator == 1;
+// GI integrators the restatement covers: pt1, and pt2 without ReSTIR,
+// adaptive spp, emissive NEE sources (pt2_emissive.hpp:111-126) or
+// reflective receivers (hit_shader.hpp:251-254, :417-420)
+static bool giSupported(const ubg_scene_desc* d, const ubg_render_params* p) {
+  if (!p->gi) return true;
+  if (p->gi_integrator == 1) return true;
+  if (p->gi_integrator != 2 || p->pt2_rounds > 0 || p->pt2_adaptive) return false;
+  bool emit = false, refl = false;
+  if (d->n_triangles > 0 && d->n_vertices >= 3)
+    for (int32_t m = 0; m < d->n_mesh_materials; ++m) {
+      emit = emit || d->mesh_materials[m].emission > 0.0f;
+      refl = refl || d->mesh_materials[m].reflection > 0.0f;
+    }
+  for (int64_t i = 0; i < d->n_spheres; ++i) {
+    if (d->sphere_from_edge && d->sphere_from_edge[i]) continue;
+    const ubg_material& m = d->prim_materials[d->sphere_material ? d->sphere_material[i] : 0];
+    emit = emit || (m.emission > 0.0f && d->sphere_center_radius[4 * i + 3] > 0.0f);
+    refl = refl || m.reflection > 0.0f;
+  }
+  for (int64_t i = 0; i < d->n_cylinders; ++i) {
+    if (d->cyl_from_edge && d->cyl_from_edge[i]) continue;
+    const ubg_material& m = d->prim_materials[d->cyl_material ? d->cyl_material[i] : 0];
+    emit = emit || (!(d->cyl_open && d->cyl_open[i]) && m.emission > 0.0f && d->cyl_radius[i] > 0.0f);
+    refl = refl || m.reflection > 0.0f;
+  }
+  if (p->pt2_emissive && p->pt2_emissive_nee && emit) return false;
+  if (p->pt2_reflect && refl) return false;
+  return true;
 }
 
 template <class T>
@@ -2259,6 +2378,9 @@ static int renderT(const ubg_scene_desc* desc, const ubg_render_params* p, ubg_f
     gp.envIntensity = p->gi_env_intensity;
     gp.bounces = std::max(1, p->gi_bounces);
     gp.maxDist = p->gi_max_distance > 0.0f ? T(p->gi_max_distance) : std::numeric_limits<T>::infinity();
+    gp.pt2 = p->gi_integrator == 2;
+    gp.pattern = p->pt2_pattern;
+    gp.emissive = p->pt2_emissive != 0;
     const int spp = std::max(1, p->pt1_spp);
     const uint32_t seedMix = hashU32(p->pt1_seed);
     // gatherPt1Grid (pt1_gather.hpp:525-612) over a w x h grid: every
@@ -2277,9 +2399,13 @@ static int renderT(const ubg_scene_desc* desc, const ubg_render_params* p, ubg_f
           const V3<T> N = v3<T>(&nrm[3 * pix]);
           const V3<T> Ng = geo ? v3<T>(&(*geo)[3 * pix]) : N;
           const uint32_t seed = hashU32(static_cast<uint32_t>(pix) ^ seedMix);
+          uint32_t sBase = 0, sSeed = 0;
+          if (gp.pt2)
+            pt2PixelSampler(gp.pattern, static_cast<uint32_t>(px), static_cast<uint32_t>(py), seedMix, seed,
+                            nextPow2(static_cast<uint32_t>(nspp)), sBase, sSeed);
           T occ = T(0);
           const V3<T> E = pt1GatherPoint(c, gp, P, N, Ng, nspp, seed, T(dep[pix]), p->pt1_ld != 0, p->pt1_clamp,
-                                         &occ, cnt);
+                                         &occ, cnt, sBase, sSeed);
           Eo[3 * pix] = static_cast<float>(E.x);
           Eo[3 * pix + 1] = static_cast<float>(E.y);
           Eo[3 * pix + 2] = static_cast<float>(E.z);
@@ -2699,7 +2825,7 @@ int orc_render(const ubg_scene_desc* desc, const ubg_render_params* params, ubg_
     if (desc->prim_materials[i].model == 1) return UBG_ERR_UNSUPPORTED;
   for (int32_t i = 0; i < desc->n_mesh_materials; ++i)
     if (desc->mesh_materials[i].model == 1) return UBG_ERR_UNSUPPORTED;
-  if (!orc::giSupported(params, std::max(1, params->supersample))) return UBG_ERR_UNSUPPORTED;
+  if (!orc::giSupported(desc, params)) return UBG_ERR_UNSUPPORTED;
   return o->precision == 64 ? orc::renderT<double>(desc, params, frame, o)
                             : orc::renderT<float>(desc, params, frame, o);
 }
@@ -2742,6 +2868,13 @@ void orc_pt1_denoise(int32_t w, int32_t h, float* E, const float* normal, const 
   std::memcpy(E, e.data(), 3 * n * sizeof(float));
 }
 
+void orc_pt2_sobol_2d(uint32_t index, uint32_t seed, float* u1, float* u2) { orc::sobolBurley2D(index, seed, *u1, *u2); }
+
+void orc_pt2_pixel_sampler(int32_t pattern, uint32_t px, uint32_t py, uint32_t frame_seed, uint32_t pixel_hash,
+                           uint32_t spp, uint32_t* index_base, uint32_t* seed) {
+  orc::pt2PixelSampler(pattern, px, py, frame_seed, pixel_hash, orc::nextPow2(spp < 1 ? 1 : spp), *index_base, *seed);
+}
+
 int orc_pt1_gather_point(const ubg_scene_desc* desc, const ubg_render_params* params, const float* P,
                          const float* N, const float* Ng, int32_t spp, uint32_t seed, float eps_t, float* E,
                          float* occlusion, uint64_t* gi_rays) {
@@ -2762,10 +2895,21 @@ int orc_pt1_gather_point(const ubg_scene_desc* desc, const ubg_render_params* pa
   gp.envIntensity = params->gi_env_intensity;
   gp.bounces = std::max(1, params->gi_bounces);
   gp.maxDist = params->gi_max_distance > 0.0f ? params->gi_max_distance : std::numeric_limits<float>::infinity();
+  gp.pt2 = params->gi_integrator == 2;
+  gp.pattern = params->pt2_pattern;
+  gp.emissive = params->pt2_emissive != 0;
+  // (a pixel at the origin of the gather grid: the per-pixel Sobol
+  // arrangement is seeded by the point's seed, the blue-noise one by the
+  // frame seed of pt1_seed)
+  uint32_t sBase = 0, sSeed = 0;
+  if (gp.pt2)
+    orc::pt2PixelSampler(gp.pattern, 0u, 0u, orc::hashU32(params->pt1_seed), seed,
+                         orc::nextPow2(static_cast<uint32_t>(std::max(1, spp))), sBase, sSeed);
   orc::Counters cnt;
   float occ = 0.0f;
   const orc::V3<float> e = orc::pt1GatherPoint(c, gp, orc::v3<float>(P), orc::v3<float>(N), orc::v3<float>(Ng),
-                                               spp, seed, eps_t, params->pt1_ld != 0, params->pt1_clamp, &occ, cnt);
+                                               spp, seed, eps_t, params->pt1_ld != 0, params->pt1_clamp, &occ, cnt,
+                                               sBase, sSeed);
   E[0] = e.x;
   E[1] = e.y;
   E[2] = e.z;

@@ -75,6 +75,14 @@ int orc_pt1_gather_point(const ubg_scene_desc* desc, const ubg_render_params* pa
 void orc_pt1_denoise(int32_t w, int32_t h, float* E, const float* normal, const float* position, int32_t iters,
                      float sigma_z, float sigma_n, float sigma_l);
 
+/* pt2 first-bounce sampler (integrator/pt2/pt2_sampler.hpp): the 2D
+ * shuffled Owen-scrambled Sobol point `index` under `seed`, and the per-pixel
+ * arrangement (pattern 0 per-pixel sequence, 1 blue noise; spp rounded up to
+ * a power of two for the section length). */
+void orc_pt2_sobol_2d(uint32_t index, uint32_t seed, float* u1, float* u2);
+void orc_pt2_pixel_sampler(int32_t pattern, uint32_t px, uint32_t py, uint32_t frame_seed, uint32_t pixel_hash,
+                           uint32_t spp, uint32_t* index_base, uint32_t* seed);
+
 /* Reference TEA-8 (secondary_rays.hpp:51-58), exposed for known-answer tests. */
 void orc_tea2(uint32_t* v0, uint32_t* v1);
 

@@ -1044,3 +1044,22 @@ def gi_reduced_grid():
         assert r["half"].stats["gi_rays"] > r["nopatch"].stats["gi_rays"]
     return Case("gi_reduced_grid", {"d4a": (sc, d4), "d4b": (sc, d4), "half": (sc, half), "full": (sc, full),
                                     "nopatch": (sc, nopatch)}, chk, tol=0.05)
+
+
+@case
+def gi_pt2_default():
+    # giIntegrator 2 with every reference default (blue-noise Sobol first
+    # bounce, emission on, a-trous E denoise, gather grid = the supersample
+    # factor): the same estimator as pt1 with another sampler, so its
+    # indirect agrees with a converged pt1 gather on average
+    sc = gi_scene()
+    pt2 = RenderOptions(width=48, height=36, supersample=2, gi=True, pt1_spp=16, shadows=True)
+    pt1 = _gi_opts(48, 36, supersample=2, pt1_spp=64, pt1_gather_div=1)
+
+    def chk(r):
+        recv = r["pt1"].indirect.max(axis=2) > 0
+        d = np.abs(r["pt2"].indirect - r["pt1"].indirect)[recv].mean()
+        assert d < 0.02, d
+        assert np.array_equal(r["pt2"].color, r["pt2b"].color)
+        assert r["pt2"].stats["gi_gbuffer_rays"] == 48 * 36
+    return Case("gi_pt2_default", {"pt2": (sc, pt2), "pt2b": (sc, pt2), "pt1": (sc, pt1)}, chk, tol=0.05)

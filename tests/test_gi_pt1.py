@@ -158,7 +158,7 @@ def test_env_intensity_and_ambient_scale_linearly():
 def test_render_refuses_unimplemented_gi():
     from umbreon_amd import capi
     sc = _wall((0.5, 0.5, 0.5, 1.0))
-    for kw in (dict(gi_integrator=2), dict(gi_integrator=0)):
+    for kw in (dict(gi_integrator=0), dict(gi_integrator=3)):
         o = _opt(width=4, height=4, **kw)
         with pytest.raises(capi.UmbreonError):
             ob.render(sc, o, threads=1)
@@ -202,3 +202,105 @@ def test_denoise_reduces_noise_and_keeps_a_normal_edge():
     assert np.array_equal(E[0, 0], noisy[0, 0])
     # iterations 0 is the identity (denoise_atrous.cpp:32-33)
     assert np.array_equal(ob.pt1_denoise(noisy, nrm, pos, iters=0), noisy)
+
+
+# ---------------------------------------------------------------- pt2 (giIntegrator 2)
+def test_sobol_prefixes_are_0k2_nets():
+    # The shuffled Owen-scrambled Sobol sequence (pt2_sampler.hpp:143-151):
+    # every power-of-two prefix of 2^k points is a (0, k, 2)-net -- each
+    # elementary interval of area 2^-k holds exactly one point -- for any
+    # seed (a wrong direction number, bit order or hash breaks it)
+    for seed in (0, 12345, 0xDEADBEEF):
+        pts = np.array([ob.pt2_sobol_2d(i, seed) for i in range(1 << 10)], np.float64)
+        assert np.all((pts >= 0.0) & (pts < 1.0))
+        for k in range(1, 11):
+            p = pts[: 1 << k]
+            for i in range(k + 1):
+                j = k - i
+                cell = (np.floor(p[:, 0] * (1 << i)).astype(np.int64) << j) | np.floor(p[:, 1] * (1 << j)).astype(np.int64)
+                assert len(np.unique(cell)) == 1 << k, (seed, k, i)
+
+
+def test_blue_noise_sections_do_not_overlap():
+    # pt2MakePixelSampler (pt2_sampler.hpp:164-180): the base-4 nested
+    # scramble of the Morton code is a bijection whose low 2m bits, over the
+    # 4^m pixels of a 2^m x 2^m grid, are a permutation -- so the pixels'
+    # sections (nextPow2(spp) samples each) of the one global sequence never
+    # overlap; the scramble seed is the frame's, not the pixel's
+    for seed in (777, 0xC0FFEE):
+        codes = []
+        for y in range(16):
+            for x in range(16):
+                b, sd = ob.pt2_pixel_sampler(1, x, y, seed, 123, 1)
+                assert sd == seed
+                codes.append(b)
+        assert len(set(codes)) == 256
+        assert sorted(c % 256 for c in codes) == list(range(256))
+    for spp, pow2 in ((8, 8), (6, 8), (1, 1)):
+        b1, _ = ob.pt2_pixel_sampler(1, 3, 5, 777, 123, 1)
+        b, _ = ob.pt2_pixel_sampler(1, 3, 5, 777, 123, spp)
+        assert b == (b1 * pow2) & 0xFFFFFFFF
+    assert ob.pt2_pixel_sampler(0, 5, 9, 777, 4242, 8) == (0, 4242)
+
+
+def _pt2(**kw):
+    base = dict(gi=True, gi_integrator=2, pt1_denoise=False)
+    base.update(kw)
+    return RenderOptions(**base)
+
+
+@pytest.mark.parametrize("pattern", [0, 1])
+def test_pt2_closed_forms(pattern):
+    E, occ, rays = ob.pt1_gather_point(_scene(), _pt2(pt2_pattern=pattern), P, N, spp=1024)
+    assert rel(E[0], 1.0, 0.01) and occ == 0.0 and rays == 1024
+    E, _, _ = ob.pt1_gather_point(_scene(), _pt2(pt2_pattern=pattern, pt1_sky_mode=1,
+                                                 ao_ground_color=(0.0, 0.0, 0.0)), P, N, spp=1024)
+    assert rel(E[0], 5.0 / 6.0, 0.01)
+    E, occ, _ = ob.pt1_gather_point(_wall(), _pt2(pt2_pattern=pattern), P, N, spp=8192)
+    assert rel(E[0], 0.5, 0.03) and rel(occ, 0.5, 0.03)
+
+
+def test_pt2_emission_transport():
+    # an emissive (emission 1, white, diffuse 0) wall over the wi.x < 0 half
+    # under a black sky: the gather sees radiance 1 on half the cosine
+    # measure, E = 1/2; pt2_emissive off (or pt1) transports nothing
+    sc = _wall((1.0, 1.0, 1.0, 1.0))
+    sc.set_mesh(sc.positions, sc.normals, sc.colors, material=Material(ambient=0.0, diffuse=0.0, emission=1.0))
+    black = dict(pt1_sky_radiance=(0.0, 0.0, 0.0), pt2_emissive_nee=False)
+    E, _, _ = ob.pt1_gather_point(sc, _pt2(**black), P, N, spp=8192)
+    assert rel(E[0], 0.5, 0.03)
+    E, _, _ = ob.pt1_gather_point(sc, _pt2(pt2_emissive=False, **black), P, N, spp=1024)
+    assert np.all(E == 0.0)
+    E, _, _ = ob.pt1_gather_point(sc, _opt(pt1_sky_radiance=(0.0, 0.0, 0.0)), P, N, spp=1024)
+    assert np.all(E == 0.0)
+
+
+def test_pt2_area_light_nee_converges_to_the_hard_light():
+    # pt2 draws one cone ray per vertex from the path's stream for an area
+    # light; a small cone on an unoccluded scatterer converges to the hard
+    # light's value, a zero radius reproduces it exactly
+    hard = _sphere_scene(False, True)
+    soft = _sphere_scene(False, True)
+    soft.lights[0].angular_radius = 0.02
+    e_hard, _, _ = ob.pt1_gather_point(hard, _pt2(pt2_pattern=0), P, N, spp=8192)
+    e_soft, _, _ = ob.pt1_gather_point(soft, _pt2(pt2_pattern=0), P, N, spp=8192)
+    assert rel(e_soft[0], e_hard[0], 0.01)
+    assert not np.array_equal(e_soft, e_hard)
+
+
+def test_pt2_refuses_the_unimplemented_extensions():
+    from umbreon_amd import capi
+    sc = _wall((0.5, 0.5, 0.5, 1.0))
+    for kw in (dict(pt2_rounds=1), dict(pt2_adaptive=True)):
+        with pytest.raises(capi.UmbreonError):
+            ob.render(sc, _pt2(width=4, height=4, **kw), threads=1)
+    emissive = _wall((1.0, 1.0, 1.0, 1.0))
+    emissive.set_mesh(emissive.positions, emissive.normals, emissive.colors, material=Material(emission=1.0))
+    with pytest.raises(capi.UmbreonError):
+        ob.render(emissive, _pt2(width=4, height=4), threads=1)
+    ob.render(emissive, _pt2(width=4, height=4, pt2_emissive_nee=False), threads=1)
+    mirror = _wall((1.0, 1.0, 1.0, 1.0))
+    mirror.set_mesh(mirror.positions, mirror.normals, mirror.colors, material=Material(reflection=0.3))
+    with pytest.raises(capi.UmbreonError):
+        ob.render(mirror, _pt2(width=4, height=4), threads=1)
+    ob.render(mirror, _pt2(width=4, height=4, pt2_reflect=False), threads=1)

@@ -383,7 +383,7 @@ def test_adaptive_aa_fine_lattice_transparency_chunked(device):
 
 
 @pytest.mark.parametrize("workload", ["ballstick", "surface", "surface_denoised", "ballstick_ss2_half_grid",
-                                      "surface_div3"])
+                                      "surface_div3", "surface_pt2_defaults", "ballstick_pt2_sobol_area_light"])
 def test_pt1_gi_reduced(device, workload):
     """pt1 GI against the oracle: same first hits, same seeds and sample
     streams; the gathered indirect must agree within 1e-3 per-pixel mean
@@ -404,6 +404,14 @@ def test_pt1_gi_reduced(device, workload):
     opt.pt1_denoise = workload.endswith("denoised") or workload.endswith("half_grid") or workload.endswith("div3")
     if workload.endswith("div3"):
         opt.pt1_gather_div = 3
+    if "pt2" in workload:
+        # the reference's default integrator and options (blue noise, emission,
+        # denoise, gather divisor = ss), or the per-pixel Sobol pattern under an
+        # area light (one cone-jittered next-event ray per path vertex)
+        opt.gi_integrator, opt.pt1_denoise = 2, True
+        if workload.endswith("area_light"):
+            opt.pt2_pattern = 0
+            sc.lights[0].angular_radius = 0.05
     sc.ambient_color = (0.6, 0.6, 0.6)
     print(f"pt1 GI {workload} {opt.width}x{opt.height} spp{opt.pt1_spp} bounces {opt.gi_bounces}", flush=True)
     g = _gpu(device, sc, opt, hires_only=True, want_ids=True)

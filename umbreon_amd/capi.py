@@ -115,6 +115,8 @@ class RenderParams(C.Structure):
         ("denoise_sigma_z", C.c_float), ("denoise_sigma_n", C.c_float), ("denoise_sigma_l", C.c_float),
         ("pt1_upsample_normal_pow", C.c_float), ("pt1_upsample_depth_scale", C.c_float),
         ("pt1_edge_patch", C.c_int32), ("pt1_edge_patch_spp_mul", C.c_int32), ("pt1_edge_patch_thresh", C.c_float),
+        ("pt2_pattern", C.c_int32), ("pt2_emissive", C.c_int32), ("pt2_rounds", C.c_int32),
+        ("pt2_adaptive", C.c_int32), ("pt2_emissive_nee", C.c_int32), ("pt2_reflect", C.c_int32),
     ]
 
 

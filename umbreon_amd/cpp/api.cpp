@@ -199,7 +199,7 @@ void checkSupported(const Scene& s, const RenderOptions& o) {
   auto no = [](const char* what) {
     throw std::runtime_error(std::string("umbreon (MI355X): ") + what + " is not supported by the GPU backend");
   };
-  if (o.gi && o.giIntegrator != 1) no("the pt2 / irradiance-cache GI integrators (giIntegrator != 1)");
+  if (o.gi && o.giIntegrator != 1 && o.giIntegrator != 2) no("the irradiance-cache GI integrator (giIntegrator 0)");
   if (o.aoWriteAov) no("AO AOVs (aoWriteAov)");
   if (o.denoiser != 0) no("the final-colour denoiser");
   if (o.envLights > 0) no("environment dome lights (envLights)");
@@ -322,6 +322,12 @@ FrameResult renderPass(const Scene& scene, const RenderOptions& opt, RenderProgr
   p.pt1_edge_patch = opt.pt1EdgePatch ? 1 : 0;
   p.pt1_edge_patch_spp_mul = opt.pt1EdgePatchSppMul;
   p.pt1_edge_patch_thresh = opt.pt1EdgePatchThresh;
+  p.pt2_pattern = opt.pt2Pattern;
+  p.pt2_emissive = opt.pt2Emissive ? 1 : 0;
+  p.pt2_rounds = opt.pt2Rounds;
+  p.pt2_adaptive = opt.pt2Adaptive ? 1 : 0;
+  p.pt2_emissive_nee = opt.pt2EmissiveNee ? 1 : 0;
+  p.pt2_reflect = opt.pt2Reflect ? 1 : 0;
 
   const std::size_t hiN = static_cast<std::size_t>(opt.width) * ss * opt.height * ss;
   f.width = opt.width;

@@ -973,6 +973,31 @@ int buildHostScene(const ubg_scene_desc* d, int layout, HostScene& out, std::str
   for (uint32_t info : out.sphInfo) out.aoReceivers = out.aoReceivers || !(info & kInfoFromEdge);
   for (uint32_t info : out.coneInfo) out.aoReceivers = out.aoReceivers || !(info & kInfoFromEdge);
   for (uint32_t info : out.rndInfo) out.aoReceivers = out.aoReceivers || !(info & kInfoFromEdge);
+  // what the pt2 integrator would need beyond this implementation: emissive
+  // NEE sources (pt2_emissive.hpp:111-126: an emissive mesh material, or a
+  // real sphere / capped cylinder of positive radius with an emissive
+  // material) and reflective receivers (hit_shader.hpp:251-254, :417-420)
+  {
+    auto primMat = [&](int32_t m) -> const ubg_material& { return d->prim_materials[m]; };
+    if (out.nTris > 0)
+      for (int32_t m = 0; m < d->n_mesh_materials; ++m) {
+        out.pt2Emitters = out.pt2Emitters || d->mesh_materials[m].emission > 0.0f;
+        out.pt2Reflective = out.pt2Reflective || d->mesh_materials[m].reflection > 0.0f;
+      }
+    for (int64_t i = 0; i < d->n_spheres; ++i) {
+      if (d->sphere_from_edge && d->sphere_from_edge[i]) continue;
+      const ubg_material& m = primMat(d->sphere_material ? d->sphere_material[i] : 0);
+      out.pt2Emitters = out.pt2Emitters || (m.emission > 0.0f && d->sphere_center_radius[4 * i + 3] > 0.0f);
+      out.pt2Reflective = out.pt2Reflective || m.reflection > 0.0f;
+    }
+    for (int64_t i = 0; i < d->n_cylinders; ++i) {
+      if (d->cyl_from_edge && d->cyl_from_edge[i]) continue;
+      const ubg_material& m = primMat(d->cyl_material ? d->cyl_material[i] : 0);
+      const bool open = d->cyl_open && d->cyl_open[i];
+      out.pt2Emitters = out.pt2Emitters || (!open && m.emission > 0.0f && d->cyl_radius[i] > 0.0f);
+      out.pt2Reflective = out.pt2Reflective || m.reflection > 0.0f;
+    }
+  }
   const size_t nPrims = boxes.size();
   if (nPrims == 0) return UBG_OK;
   if (nPrims >= (1u << 28)) {

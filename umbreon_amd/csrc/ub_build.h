@@ -44,6 +44,8 @@ struct HostScene {
   int64_t nTris = 0, nSpheres = 0, nRound = 0, nCone = 0;
   int64_t meshBaseTris = 0;  // triangles of one instance copy
   bool aoReceivers = false;  // a mesh or a primitive that is not outline decoration
+  bool pt2Emitters = false;  // an emissive-NEE source for the pt2 integrator
+  bool pt2Reflective = false;  // a receiver with Material::reflection > 0
   int32_t layout = kLayoutBvh8;
   int32_t depth = 0;       // tree levels
   int32_t stackDepth = 0;  // traversal stack entries needed

@@ -83,6 +83,7 @@ struct ubg_scene {
   ubg_context* ctx = nullptr;
   ubg::SceneView view{};
   bool aoReceivers = false;
+  bool pt2Emitters = false, pt2Reflective = false;
   std::vector<void*> allocs;
   ubg_scene_info info{};
 };
@@ -317,6 +318,8 @@ int ubg_scene_create_ex(ubg_context* ctx, const ubg_scene_desc* desc, int32_t bv
   s->view.nNodes = static_cast<int32_t>(hs.nNodes);
   s->view.meshBaseTris = hs.meshBaseTris;
   s->aoReceivers = hs.aoReceivers;
+  s->pt2Emitters = hs.pt2Emitters;
+  s->pt2Reflective = hs.pt2Reflective;
   s->view.nPrims = static_cast<int32_t>(hs.prims.size() / 3);  // records in either order
   s->view.layout = hs.layout;
   s->view.stackDepth = std::max(1, hs.stackDepth);
@@ -451,8 +454,18 @@ int ubg_render(ubg_context* ctx, const ubg_scene* scene, const ubg_render_params
   const bool gi = p->gi != 0;
   int giDiv = 1;
   if (gi) {
-    if (p->gi_integrator != 1)
-      return fail(UBG_ERR_UNSUPPORTED, "ubg_render: only the pt1 GI integrator (gi_integrator 1) is implemented");
+    if (p->gi_integrator != 1 && p->gi_integrator != 2)
+      return fail(UBG_ERR_UNSUPPORTED, "ubg_render: the irradiance-cache GI integrator (gi_integrator 0) is not implemented");
+    if (p->gi_integrator == 2) {
+      if (p->pt2_rounds > 0) return fail(UBG_ERR_UNSUPPORTED, "ubg_render: pt2 ReSTIR (pt2_rounds > 0) is not implemented");
+      if (p->pt2_adaptive) return fail(UBG_ERR_UNSUPPORTED, "ubg_render: pt2 adaptive spp is not implemented");
+      if (p->pt2_emissive && p->pt2_emissive_nee && scene->pt2Emitters)
+        return fail(UBG_ERR_UNSUPPORTED, "ubg_render: pt2 emissive NEE (an emissive material with pt2_emissive_nee) "
+                                         "is not implemented");
+      if (p->pt2_reflect && scene->pt2Reflective)
+        return fail(UBG_ERR_UNSUPPORTED, "ubg_render: pt2 traced reflection (a reflective material with "
+                                         "pt2_reflect) is not implemented");
+    }
     giDiv = p->pt1_gather_div < 0 ? ss : p->pt1_gather_div;
     if (giDiv == 0) giDiv = p->pt1_half_res ? 2 : 1;
     if (giDiv < 1) giDiv = 1;
@@ -463,6 +476,9 @@ int ubg_render(ubg_context* ctx, const ubg_scene* scene, const ubg_render_params
   const int giLw = (W + giDiv - 1) / giDiv, giLh = (H + giDiv - 1) / giDiv;
   const size_t giLn = static_cast<size_t>(giLw) * giLh;
   fc.gi = gi ? 1 : 0;
+  fc.giPt2 = (gi && p->gi_integrator == 2) ? 1 : 0;
+  fc.giPattern = p->pt2_pattern;
+  fc.giEmissive = p->pt2_emissive ? 1 : 0;
   fc.giSpp = std::max(1, p->pt1_spp);
   fc.giBounces = std::max(1, p->gi_bounces);
   fc.giLd = p->pt1_ld ? 1 : 0;

@@ -1208,6 +1208,51 @@ __device__ __forceinline__ uint32_t hashU32(uint32_t x) {
   return x;
 }
 
+// pt2 first-bounce sampler (pt2_sampler.hpp): hash-based Owen-scrambled
+// Sobol (Burley, JCGT 9(4) 2020) -- the Laine-Karras style hash on reversed
+// bits, its base-4 variant for the Morton pixel shuffle of the blue-noise
+// arrangement, and the second Sobol dimension's direction numbers (rows of
+// Pascal's triangle mod 2 in the reversed-bit convention).
+__device__ __forceinline__ uint32_t owenHashRev(uint32_t n, uint32_t seed) {
+  n ^= n * 0x3d20adeau;
+  n += seed;
+  n *= (seed >> 16) | 1u;
+  n ^= n * 0x05526c56u;
+  n ^= n * 0x53a22864u;
+  return n;
+}
+__device__ __forceinline__ uint32_t owenHashRevBase4(uint32_t n, uint32_t seed) {
+  n ^= n * 0x3d20adeau;
+  n ^= (n >> 1) & (n << 1) & 0x55555555u;
+  n += seed;
+  n *= (seed >> 16) | 1u;
+  n ^= (n >> 1) & (n << 1) & 0x55555555u;
+  n ^= n * 0x05526c56u;
+  n ^= n * 0x53a22864u;
+  return n;
+}
+__device__ __forceinline__ uint32_t interleaveBits16(uint32_t x) {
+  x &= 0x0000ffffu;
+  x = (x ^ (x << 8)) & 0x00ff00ffu;
+  x = (x ^ (x << 4)) & 0x0f0f0f0fu;
+  x = (x ^ (x << 2)) & 0x33333333u;
+  x = (x ^ (x << 1)) & 0x55555555u;
+  return x;
+}
+__device__ __forceinline__ float sobolToUnit(uint32_t n) { return static_cast<float>(n) * (1.0f / 4294967808.0f); }
+__device__ __forceinline__ void sobolBurley2D(uint32_t index, uint32_t seed, float& u1, float& u2) {
+  index = owenHashRev(reverseBits32(index), seed ^ 0xf8ade99au);
+  // dimension 0: the van der Corput sequence (the reversed index itself)
+  u1 = sobolToUnit(reverseBits32(owenHashRev(reverseBits32(index), seed ^ 0xe0aaaf76u)));
+  // dimension 1: XOR of the direction numbers of the index's set bits
+  uint32_t r = 0, row = 1;
+  for (int bit = 31; bit >= 0; --bit) {
+    if ((index >> bit) & 1u) r ^= row;
+    row ^= row << 1;
+  }
+  u2 = sobolToUnit(reverseBits32(owenHashRev(r, seed ^ 0x94964d4eu)));
+}
+
 // The pixels of a gather grid to gather (f.nrm.w != 0, or mask != 0) into
 // its partitioned list (one atomic per wave, partition = block % kQueueParts);
 // with zeroOthers every other pixel gets E = 0 (gatherPt1Grid's zero fill).
@@ -1560,6 +1605,18 @@ __global__ __launch_bounds__(kTraceBlock) void k_gi_gather(FrameConst fc, SceneV
     const float eps = selfIntersectEps(P, N, epsT);
     const V3 O = P + N * eps;
     const int spp = gf.spp;
+    // pt2MakePixelSampler: blue noise = one global sequence, pixel sections
+    // of nextPow2(spp) samples along the base-4 scrambled Morton curve;
+    // sobol = an independent sequence per pixel
+    uint32_t sBase = 0, sSeed = seed;
+    if (fc.giPt2 && fc.giPattern == 1) {
+      uint32_t pow2 = 1;
+      while (pow2 < static_cast<uint32_t>(spp)) pow2 <<= 1;
+      const uint32_t px = pix % static_cast<uint32_t>(gf.w), py = pix / static_cast<uint32_t>(gf.w);
+      const uint32_t m = (interleaveBits16(px) << 1) | interleaveBits16(py);
+      sBase = reverseBits32(owenHashRevBase4(reverseBits32(m), fc.giSeedMix)) * pow2;
+      sSeed = fc.giSeedMix;
+    }
     float cpx = 0.0f, cpy = 0.0f;
     if (fc.giLd) {
       uint32_t c0 = seed, c1 = 0x9E3779B9u;
@@ -1573,7 +1630,9 @@ __global__ __launch_bounds__(kTraceBlock) void k_gi_gather(FrameConst fc, SceneV
       uint32_t s0 = seed, s1 = static_cast<uint32_t>(si);
       tea2(s0, s1);
       float u1, u2;
-      if (fc.giLd) {
+      if (fc.giPt2) {
+        sobolBurley2D(sBase + static_cast<uint32_t>(si), sSeed, u1, u2);
+      } else if (fc.giLd) {
         aoSample2d(true, 0u, si, spp, cpx, cpy, u1, u2);
       } else {
         u1 = u32ToUnorm(s0);
@@ -1601,7 +1660,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_gi_gather(FrameConst fc, SceneV
         // pt1EvalVertex: the hit as a path vertex
         const uint32_t kind = h.bits >> kKindShift, id = h.bits & kIdMask;
         V3 Ny, Cy, NgS;
-        float kd;
+        float kd, em = 0.0f;
         if (kind == 0u) {
           const uint4 tri = reinterpret_cast<const uint4*>(s.triIdx)[id];
           const float4* vn = reinterpret_cast<const float4*>(s.vtxNrm);
@@ -1613,7 +1672,9 @@ __global__ __launch_bounds__(kTraceBlock) void k_gi_gather(FrameConst fc, SceneV
                             w0 * n0.z + h.u * n1.z + h.v * n2.z));
           Cy = mk(w0 * c0.x + h.u * c1.x + h.v * c2.x, w0 * c0.y + h.u * c1.y + h.v * c2.y,
                   w0 * c0.z + h.u * c1.z + h.v * c2.z);
-          kd = s.meshMats[tri.w & kInfoMatMask].diffuse;
+          const DevMaterial& mm = s.meshMats[tri.w & kInfoMatMask];
+          kd = mm.diffuse;
+          if (fc.giPt2 && fc.giEmissive) em = mm.emission;
           NgS = h.ng;
         } else {
           float4 col;
@@ -1631,7 +1692,9 @@ __global__ __launch_bounds__(kTraceBlock) void k_gi_gather(FrameConst fc, SceneV
           if (info & kInfoFromEdge) break;  // outline decoration absorbs the path
           Ny = safeNormalize(h.ng);
           Cy = mk(col.x, col.y, col.z);
-          kd = s.primMats[info & kInfoMatMask].diffuse;
+          const DevMaterial& pm = s.primMats[info & kInfoMatMask];
+          kd = pm.diffuse;
+          if (fc.giPt2 && fc.giEmissive) em = pm.emission;
           NgS = Ny;
         }
         if (dot(Ny, wi) > 0.0f) Ny = neg(Ny);
@@ -1643,20 +1706,29 @@ __global__ __launch_bounds__(kTraceBlock) void k_gi_gather(FrameConst fc, SceneV
           const V3 Ld = mk(l.L[0], l.L[1], l.L[2]);
           const float ndl = dot(Ny, Ld);
           if (ndl <= 0.0f) continue;
-          // computeShadow's single hard ray (secondary_rays.hpp:118-124)
+          // computeShadow's single hard ray (secondary_rays.hpp:118-124), or
+          // (pt2, an area light) one cone-jittered ray from the path's stream
           V3 ng = dot(NgS, Ny) < 0.0f ? neg(NgS) : NgS;
           ng = safeNormalize(ng, Ny);
           const V3 So = Py + ng * epsY;
+          V3 dir = Ld;
+          if (fc.giPt2 && l.radius > 0.0f) {
+            tea2(s0, s1);
+            dir = sampleLightDir(Ld, l.tanRadius, u32ToUnorm(s0), u32ToUnorm(s1));
+          }
           Hit hs;
           ++nNee;
-          const float sh = trace<kLayout, true, kCount>(s, So, Ld, epsY, __builtin_inff(), hs, &nv, &nt) ? 0.0f : 1.0f;
+          const float sh = trace<kLayout, true, kCount>(s, So, dir, epsY, __builtin_inff(), hs, &nv, &nt) ? 0.0f : 1.0f;
           E.x += ndl * l.color[0] * sh;
           E.y += ndl * l.color[1] * sh;
           E.z += ndl * l.color[2] * sh;
         }
-        L.x += tp.x * (kd * Cy.x * E.x);
-        L.y += tp.y * (kd * Cy.y * E.y);
-        L.z += tp.z * (kd * Cy.z * E.z);
+        // radiance + emitted (pt1EvalVertex's split; emitted is 0 unless pt2
+        // transports the vertex's emission)
+        const V3 emd = em > 0.0f ? mk(em * Cy.x, em * Cy.y, em * Cy.z) : mk(0.0f, 0.0f, 0.0f);
+        L.x += tp.x * (kd * Cy.x * E.x + emd.x);
+        L.y += tp.y * (kd * Cy.y * E.y + emd.y);
+        L.z += tp.z * (kd * Cy.z * E.z + emd.z);
         if (bo == fc.giBounces) break;
         tp.x *= kd * Cy.x;
         tp.y *= kd * Cy.y;

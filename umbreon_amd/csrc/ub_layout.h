@@ -157,6 +157,7 @@ struct FrameConst {
   // environment: radiance = envScale * (ground + (sky - ground) * w) with
   // w = 0.5 (dot(wi, aoUpAxis) + 1), envScale = giEnvIntensity * ambient
   int32_t gi, giSpp, giBounces, giLd;
+  int32_t giPt2, giPattern, giEmissive;  // pt2: Sobol-Burley first bounce, area-light NEE, emission
   float giMaxDist, giIntensity, giClamp;
   uint32_t giSeedMix;
   float giEnvScale[3], giSky[3], giGround[3];

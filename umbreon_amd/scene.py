@@ -115,9 +115,10 @@ class RenderOptions:
     aa_threshold: float = 0.1
     aa_depth: int = 0
     aa_debug: bool = False  # fill FrameResult.aa_mask
-    # path-traced GI (render_options.hpp:93-162); the gfx950 path implements
-    # the pt1 integrator (gi_integrator 1; E denoise = the a-trous filter) and
-    # refuses the pt2 and cache integrators
+    # path-traced GI (render_options.hpp:93-253); the gfx950 path implements
+    # the pt1 and pt2 integrators (pt2 without ReSTIR, adaptive spp, emissive
+    # NEE or traced reflections; E denoise = the a-trous filter) and refuses
+    # the irradiance cache
     gi: bool = False
     gi_bounces: int = 1
     gi_max_distance: float = 0.0
@@ -141,6 +142,12 @@ class RenderOptions:
     pt1_edge_patch: bool = True
     pt1_edge_patch_spp_mul: int = 4
     pt1_edge_patch_thresh: float = 0.3
+    pt2_pattern: int = 1
+    pt2_emissive: bool = True
+    pt2_rounds: int = 0
+    pt2_adaptive: bool = False
+    pt2_emissive_nee: bool = True
+    pt2_reflect: bool = True
     # options the gfx950 path does not implement; must stay at their defaults
     denoiser: int = 0
     env_lights: int = 0
@@ -413,6 +420,12 @@ def params(scene: Scene, opt: RenderOptions, flags: int = 0,
     p.pt1_edge_patch = int(bool(opt.pt1_edge_patch))
     p.pt1_edge_patch_spp_mul = int(opt.pt1_edge_patch_spp_mul)
     p.pt1_edge_patch_thresh = float(opt.pt1_edge_patch_thresh)
+    p.pt2_pattern = int(opt.pt2_pattern)
+    p.pt2_emissive = int(bool(opt.pt2_emissive))
+    p.pt2_rounds = int(opt.pt2_rounds)
+    p.pt2_adaptive = int(bool(opt.pt2_adaptive))
+    p.pt2_emissive_nee = int(bool(opt.pt2_emissive_nee))
+    p.pt2_reflect = int(bool(opt.pt2_reflect))
     p.flags = flags
     if cancel_flag is not None:
         keep.append(cancel_flag)
