@@ -118,6 +118,7 @@ int main() {
     s.camera = orthoCam();
     umbreon::RenderOptions g = o;
     g.gi = true;
+    g.giIntegrator = 0;  // the irradiance cache: not implemented
     bool threw = false;
     try {
       umbreon::render(s, g);
@@ -125,6 +126,18 @@ int main() {
       threw = true;
     }
     check("unsupported option throws", threw);
+    // the default GI (pt2, denoised) renders
+    umbreon::RenderOptions d = o;
+    d.gi = true;
+    d.pt1Spp = 2;
+    bool ok = true;
+    try {
+      const umbreon::FrameResult f = umbreon::render(s, d);
+      ok = f.pt1DenoiserUsed == 1 && f.indirect.size() == f.depth.size() * 3;
+    } catch (const std::exception&) {
+      ok = false;
+    }
+    check("default GI renders", ok);
   }
   std::printf("%d failure(s)\n", failures);
   return failures == 0 ? 0 : 1;
