@@ -1580,35 +1580,49 @@ __global__ void k_gi_smooth_patched(FrameConst fc, FrameBufs b, const uint8_t* _
   b.giE[pix] = make_float4(e0 * inv, e1 * inv, e2 * inv, oSum * inv);
 }
 
+// Closest hit for the lanes with `live`, called by every lane of the wave:
+// the wave-vote BVH8 traversal, or the per-lane one on the BVH2 layout.
+template <bool kCount, int kLayout>
+__device__ __forceinline__ bool closestHitWave(const SceneView& s, bool live, V3 o, V3 d, float tn, float tf, Hit& h,
+                                               uint32_t* nv, uint32_t* nt) {
+  if (kLayout == kLayoutBvh8) return traceClosestWave8<kCount>(s, live, o, d, tn, tf, h, nv, nt);
+  return live && trace<kLayout, false, kCount>(s, o, d, tn, tf, h, nv, nt);
+}
+
 // pt1GatherPoint (pt1_gather.hpp:228-485) for one listed pixel per lane: spp
 // cosine paths from its first hit, each vertex adding its shadow-tested
 // direct light (pt1EvalVertex, :117-212) times the path throughput, Russian
-// roulette from the second vertex, the environment at an escape; the
-// samples are summed in order, so E is the oracle's to the last bit of the
-// same float operations. E.w = the first-bounce hit fraction.
+// roulette from the second vertex, the environment at an escape. A lane's
+// samples run in order and are summed in order, so E is the oracle's to the
+// last bit of the same float operations; E.w = the first-bounce hit
+// fraction. The lanes advance as a state machine with one path segment per
+// iteration, so every iteration's gather rays go through one wave-wide
+// closest-hit traversal (node and primitive steps voted across the wave).
 template <bool kCount, int kLayout>
 __global__ __launch_bounds__(kTraceBlock) void k_gi_gather(FrameConst fc, SceneView s, FrameBufs b, GiField gf) {
   const int tid = blockIdx.x * blockDim.x + threadIdx.x;
   const int part = tid / b.partCap;
-  const bool valid = part < kQueueParts && static_cast<uint32_t>(tid % b.partCap) < queueFill(b.counters, gf.fillBase, part);
+  bool live = part < kQueueParts && static_cast<uint32_t>(tid % b.partCap) < queueFill(b.counters, gf.fillBase, part);
   uint32_t nGi = 0, nNee = 0, nv = 0, nt = 0;
-  if (valid) {
-    const uint32_t pix = gf.list[tid];
+  // per-pixel constants
+  uint32_t pix = 0, seed = 0, sBase = 0, sSeed = 0;
+  V3 N = mk(0, 0, 1), Ng = N, O = mk(0, 0, 0);
+  float eps = 0.0f, cpx = 0.0f, cpy = 0.0f;
+  const int spp = gf.spp;
+  if (live) {
+    pix = gf.list[tid];
     const V3 P = ld3(gf.pos[pix]);
-    const V3 N = ld3(gf.nrm[pix]);
+    N = ld3(gf.nrm[pix]);
     // the render grid's G-buffer has no geometric normal: the horizon test
     // then uses the shading normal (gatherPt1Grid, pt1_gather.hpp:570-573)
-    const V3 Ng = gf.geoN ? ld3(gf.geoN[pix]) : N;
-    const float epsT = gf.depth[pix];
-    const uint32_t seed = hashU32(pix ^ fc.giSeedMix);
-    const Frame3 f = frameFromNormal(N);
-    const float eps = selfIntersectEps(P, N, epsT);
-    const V3 O = P + N * eps;
-    const int spp = gf.spp;
+    Ng = gf.geoN ? ld3(gf.geoN[pix]) : N;
+    seed = hashU32(pix ^ fc.giSeedMix);
+    eps = selfIntersectEps(P, N, gf.depth[pix]);
+    O = P + N * eps;
     // pt2MakePixelSampler: blue noise = one global sequence, pixel sections
     // of nextPow2(spp) samples along the base-4 scrambled Morton curve;
     // sobol = an independent sequence per pixel
-    uint32_t sBase = 0, sSeed = seed;
+    sSeed = seed;
     if (fc.giPt2 && fc.giPattern == 1) {
       uint32_t pow2 = 1;
       while (pow2 < static_cast<uint32_t>(spp)) pow2 <<= 1;
@@ -1617,17 +1631,48 @@ __global__ __launch_bounds__(kTraceBlock) void k_gi_gather(FrameConst fc, SceneV
       sBase = reverseBits32(owenHashRevBase4(reverseBits32(m), fc.giSeedMix)) * pow2;
       sSeed = fc.giSeedMix;
     }
-    float cpx = 0.0f, cpy = 0.0f;
     if (fc.giLd) {
       uint32_t c0 = seed, c1 = 0x9E3779B9u;
       tea2(c0, c1);
       cpx = u32ToUnorm(c0);
       cpy = u32ToUnorm(c1);
     }
-    V3 Esum = mk(0.0f, 0.0f, 0.0f);
-    int nOcc = 0;
-    for (int si = 0; si < spp; ++si) {
-      uint32_t s0 = seed, s1 = static_cast<uint32_t>(si);
+  }
+  // per-sample path state
+  int si = 0, bo = 0, nOcc = 0;
+  bool inSample = false;
+  uint32_t s0 = 0, s1 = 0;
+  V3 org = O, wi = mk(0, 0, 1), tp = mk(1, 1, 1), L = mk(0, 0, 0), Esum = mk(0, 0, 0);
+  float tnear = eps;
+  auto endSample = [&]() {
+    if (fc.giClamp > 0.0f) {
+      const float lum = 0.2126f * L.x + 0.7152f * L.y + 0.0722f * L.z;
+      if (lum > fc.giClamp) {
+        const float k = fc.giClamp / lum;
+        L.x *= k;
+        L.y *= k;
+        L.z *= k;
+      }
+    }
+    Esum.x += L.x;
+    Esum.y += L.y;
+    Esum.z += L.z;
+    ++si;
+    inSample = false;
+  };
+  for (;;) {
+    // lanes between samples start the next one (a below-horizon draw is a
+    // sample of L = 0 that traces nothing); a lane whose samples are done
+    // stores E
+    while (live && !inSample) {
+      if (si == spp) {
+        const float invN = 1.0f / static_cast<float>(spp);
+        gf.E[pix] = make_float4(Esum.x * invN, Esum.y * invN, Esum.z * invN, static_cast<float>(nOcc) * invN);
+        live = false;
+        break;
+      }
+      s0 = seed;
+      s1 = static_cast<uint32_t>(si);
       tea2(s0, s1);
       float u1, u2;
       if (fc.giPt2) {
@@ -1638,133 +1683,139 @@ __global__ __launch_bounds__(kTraceBlock) void k_gi_gather(FrameConst fc, SceneV
         u1 = u32ToUnorm(s0);
         u2 = u32ToUnorm(s1);
       }
-      V3 wi = cosineSample(u1, u2, f);
-      const bool horizonOk = dot(wi, Ng) > 0.0f;
-      if (!horizonOk) ++nOcc;
-      V3 org = O;
-      float tnear = eps;
-      V3 tp = mk(1.0f, 1.0f, 1.0f);
-      V3 L = mk(0.0f, 0.0f, 0.0f);
-      for (int bo = 1; horizonOk && bo <= fc.giBounces; ++bo) {
-        Hit h;
-        ++nGi;
-        if (!trace<kLayout, false, kCount>(s, org, wi, tnear, fc.giMaxDist, h, &nv, &nt)) {
-          // irradiance_cache.hpp:188-194
-          const float w = 0.5f * (dot(wi, mk(fc.aoUpAxis[0], fc.aoUpAxis[1], fc.aoUpAxis[2])) + 1.0f);
-          L.x += tp.x * (fc.giEnvScale[0] * (fc.giGround[0] + (fc.giSky[0] - fc.giGround[0]) * w));
-          L.y += tp.y * (fc.giEnvScale[1] * (fc.giGround[1] + (fc.giSky[1] - fc.giGround[1]) * w));
-          L.z += tp.z * (fc.giEnvScale[2] * (fc.giGround[2] + (fc.giSky[2] - fc.giGround[2]) * w));
-          break;
-        }
-        if (bo == 1) ++nOcc;
-        // pt1EvalVertex: the hit as a path vertex
-        const uint32_t kind = h.bits >> kKindShift, id = h.bits & kIdMask;
-        V3 Ny, Cy, NgS;
-        float kd, em = 0.0f;
-        if (kind == 0u) {
-          const uint4 tri = reinterpret_cast<const uint4*>(s.triIdx)[id];
-          const float4* vn = reinterpret_cast<const float4*>(s.vtxNrm);
-          const float4* vc = reinterpret_cast<const float4*>(s.vtxCol);
-          const float4 n0 = vn[tri.x], n1 = vn[tri.y], n2 = vn[tri.z];
-          const float4 c0 = vc[tri.x], c1 = vc[tri.y], c2 = vc[tri.z];
-          const float w0 = 1.0f - h.u - h.v;
-          Ny = normalize(mk(w0 * n0.x + h.u * n1.x + h.v * n2.x, w0 * n0.y + h.u * n1.y + h.v * n2.y,
-                            w0 * n0.z + h.u * n1.z + h.v * n2.z));
-          Cy = mk(w0 * c0.x + h.u * c1.x + h.v * c2.x, w0 * c0.y + h.u * c1.y + h.v * c2.y,
-                  w0 * c0.z + h.u * c1.z + h.v * c2.z);
-          const DevMaterial& mm = s.meshMats[tri.w & kInfoMatMask];
-          kd = mm.diffuse;
-          if (fc.giPt2 && fc.giEmissive) em = mm.emission;
-          NgS = h.ng;
-        } else {
-          float4 col;
-          uint32_t info;
-          if (kind == 1u) {
-            col = reinterpret_cast<const float4*>(s.sphColor)[id];
-            info = s.sphInfo[id];
-          } else if (kind == 3u) {
-            col = reinterpret_cast<const float4*>(s.coneColor)[id];
-            info = s.coneInfo[id];
-          } else {
-            col = reinterpret_cast<const float4*>(s.rndColor)[id];
-            info = s.rndInfo[id];
-          }
-          if (info & kInfoFromEdge) break;  // outline decoration absorbs the path
-          Ny = safeNormalize(h.ng);
-          Cy = mk(col.x, col.y, col.z);
-          const DevMaterial& pm = s.primMats[info & kInfoMatMask];
-          kd = pm.diffuse;
-          if (fc.giPt2 && fc.giEmissive) em = pm.emission;
-          NgS = Ny;
-        }
-        if (dot(Ny, wi) > 0.0f) Ny = neg(Ny);
-        const V3 Py = mk(org.x + wi.x * h.t, org.y + wi.y * h.t, org.z + wi.z * h.t);
-        const float epsY = selfIntersectEps(Py, wi, h.t);
-        V3 E = mk(0.0f, 0.0f, 0.0f);
-        for (int li = 0; li < fc.nLights; ++li) {
-          const DevLight& l = fc.lights[li];
-          const V3 Ld = mk(l.L[0], l.L[1], l.L[2]);
-          const float ndl = dot(Ny, Ld);
-          if (ndl <= 0.0f) continue;
-          // computeShadow's single hard ray (secondary_rays.hpp:118-124), or
-          // (pt2, an area light) one cone-jittered ray from the path's stream
-          V3 ng = dot(NgS, Ny) < 0.0f ? neg(NgS) : NgS;
-          ng = safeNormalize(ng, Ny);
-          const V3 So = Py + ng * epsY;
-          V3 dir = Ld;
-          if (fc.giPt2 && l.radius > 0.0f) {
-            tea2(s0, s1);
-            dir = sampleLightDir(Ld, l.tanRadius, u32ToUnorm(s0), u32ToUnorm(s1));
-          }
-          Hit hs;
-          ++nNee;
-          const float sh = trace<kLayout, true, kCount>(s, So, dir, epsY, __builtin_inff(), hs, &nv, &nt) ? 0.0f : 1.0f;
-          E.x += ndl * l.color[0] * sh;
-          E.y += ndl * l.color[1] * sh;
-          E.z += ndl * l.color[2] * sh;
-        }
-        // radiance + emitted (pt1EvalVertex's split; emitted is 0 unless pt2
-        // transports the vertex's emission)
-        const V3 emd = em > 0.0f ? mk(em * Cy.x, em * Cy.y, em * Cy.z) : mk(0.0f, 0.0f, 0.0f);
-        L.x += tp.x * (kd * Cy.x * E.x + emd.x);
-        L.y += tp.y * (kd * Cy.y * E.y + emd.y);
-        L.z += tp.z * (kd * Cy.z * E.z + emd.z);
-        if (bo == fc.giBounces) break;
-        tp.x *= kd * Cy.x;
-        tp.y *= kd * Cy.y;
-        tp.z *= kd * Cy.z;
-        if (bo >= 2) {
-          tea2(s0, s1);
-          float pc = fmaxf(tp.x, fmaxf(tp.y, tp.z));
-          pc = fminf(0.95f, fmaxf(0.05f, pc));
-          if (u32ToUnorm(s0) >= pc) break;
-          const float inv = 1.0f / pc;
-          tp.x *= inv;
-          tp.y *= inv;
-          tp.z *= inv;
-        }
-        tea2(s0, s1);
-        const Frame3 fv = frameFromNormal(Ny);
-        wi = cosineSample(u32ToUnorm(s0), u32ToUnorm(s1), fv);
-        const float epsV = selfIntersectEps(Py, wi, h.t);
-        org = mk(Py.x + Ny.x * epsV, Py.y + Ny.y * epsV, Py.z + Ny.z * epsV);
-        tnear = epsV;
+      wi = cosineSample(u1, u2, frameFromNormal(N));
+      L = mk(0.0f, 0.0f, 0.0f);
+      if (!(dot(wi, Ng) > 0.0f)) {
+        ++nOcc;
+        endSample();
+        continue;
       }
-      if (fc.giClamp > 0.0f) {
-        const float lum = 0.2126f * L.x + 0.7152f * L.y + 0.0722f * L.z;
-        if (lum > fc.giClamp) {
-          const float k = fc.giClamp / lum;
-          L.x *= k;
-          L.y *= k;
-          L.z *= k;
-        }
-      }
-      Esum.x += L.x;
-      Esum.y += L.y;
-      Esum.z += L.z;
+      org = O;
+      tnear = eps;
+      tp = mk(1.0f, 1.0f, 1.0f);
+      bo = 1;
+      inSample = true;
     }
-    const float invN = 1.0f / static_cast<float>(spp);
-    gf.E[pix] = make_float4(Esum.x * invN, Esum.y * invN, Esum.z * invN, static_cast<float>(nOcc) * invN);
+    const bool ray = live && inSample;
+    if (__ballot(ray) == 0ull) break;
+    Hit h;
+    const bool hit = closestHitWave<kCount, kLayout>(s, ray, org, wi, tnear, fc.giMaxDist, h, &nv, &nt);
+    if (!ray) continue;
+    ++nGi;
+    if (!hit) {
+      // irradiance_cache.hpp:188-194
+      const float w = 0.5f * (dot(wi, mk(fc.aoUpAxis[0], fc.aoUpAxis[1], fc.aoUpAxis[2])) + 1.0f);
+      L.x += tp.x * (fc.giEnvScale[0] * (fc.giGround[0] + (fc.giSky[0] - fc.giGround[0]) * w));
+      L.y += tp.y * (fc.giEnvScale[1] * (fc.giGround[1] + (fc.giSky[1] - fc.giGround[1]) * w));
+      L.z += tp.z * (fc.giEnvScale[2] * (fc.giGround[2] + (fc.giSky[2] - fc.giGround[2]) * w));
+      endSample();
+      continue;
+    }
+    if (bo == 1) ++nOcc;
+    // pt1EvalVertex: the hit as a path vertex
+    const uint32_t kind = h.bits >> kKindShift, id = h.bits & kIdMask;
+    V3 Ny, Cy, NgS;
+    float kd, em = 0.0f;
+    bool absorbed = false;
+    if (kind == 0u) {
+      const uint4 tri = reinterpret_cast<const uint4*>(s.triIdx)[id];
+      const float4* vn = reinterpret_cast<const float4*>(s.vtxNrm);
+      const float4* vc = reinterpret_cast<const float4*>(s.vtxCol);
+      const float4 n0 = vn[tri.x], n1 = vn[tri.y], n2 = vn[tri.z];
+      const float4 c0 = vc[tri.x], c1 = vc[tri.y], c2 = vc[tri.z];
+      const float w0 = 1.0f - h.u - h.v;
+      Ny = normalize(mk(w0 * n0.x + h.u * n1.x + h.v * n2.x, w0 * n0.y + h.u * n1.y + h.v * n2.y,
+                        w0 * n0.z + h.u * n1.z + h.v * n2.z));
+      Cy = mk(w0 * c0.x + h.u * c1.x + h.v * c2.x, w0 * c0.y + h.u * c1.y + h.v * c2.y,
+              w0 * c0.z + h.u * c1.z + h.v * c2.z);
+      const DevMaterial& mm = s.meshMats[tri.w & kInfoMatMask];
+      kd = mm.diffuse;
+      if (fc.giPt2 && fc.giEmissive) em = mm.emission;
+      NgS = h.ng;
+    } else {
+      float4 col;
+      uint32_t info;
+      if (kind == 1u) {
+        col = reinterpret_cast<const float4*>(s.sphColor)[id];
+        info = s.sphInfo[id];
+      } else if (kind == 3u) {
+        col = reinterpret_cast<const float4*>(s.coneColor)[id];
+        info = s.coneInfo[id];
+      } else {
+        col = reinterpret_cast<const float4*>(s.rndColor)[id];
+        info = s.rndInfo[id];
+      }
+      absorbed = (info & kInfoFromEdge) != 0u;  // outline decoration absorbs the path
+      Ny = safeNormalize(h.ng);
+      Cy = mk(col.x, col.y, col.z);
+      const DevMaterial& pm = s.primMats[info & kInfoMatMask];
+      kd = pm.diffuse;
+      if (fc.giPt2 && fc.giEmissive) em = pm.emission;
+      NgS = Ny;
+    }
+    if (absorbed) {
+      endSample();
+      continue;
+    }
+    if (dot(Ny, wi) > 0.0f) Ny = neg(Ny);
+    const V3 Py = mk(org.x + wi.x * h.t, org.y + wi.y * h.t, org.z + wi.z * h.t);
+    const float epsY = selfIntersectEps(Py, wi, h.t);
+    V3 E = mk(0.0f, 0.0f, 0.0f);
+    for (int li = 0; li < fc.nLights; ++li) {
+      const DevLight& l = fc.lights[li];
+      const V3 Ld = mk(l.L[0], l.L[1], l.L[2]);
+      const float ndl = dot(Ny, Ld);
+      if (ndl <= 0.0f) continue;
+      // computeShadow's single hard ray (secondary_rays.hpp:118-124), or
+      // (pt2, an area light) one cone-jittered ray from the path's stream
+      V3 ng = dot(NgS, Ny) < 0.0f ? neg(NgS) : NgS;
+      ng = safeNormalize(ng, Ny);
+      const V3 So = Py + ng * epsY;
+      V3 dir = Ld;
+      if (fc.giPt2 && l.radius > 0.0f) {
+        tea2(s0, s1);
+        dir = sampleLightDir(Ld, l.tanRadius, u32ToUnorm(s0), u32ToUnorm(s1));
+      }
+      Hit hs;
+      ++nNee;
+      const float sh = trace<kLayout, true, kCount>(s, So, dir, epsY, __builtin_inff(), hs, &nv, &nt) ? 0.0f : 1.0f;
+      E.x += ndl * l.color[0] * sh;
+      E.y += ndl * l.color[1] * sh;
+      E.z += ndl * l.color[2] * sh;
+    }
+    // radiance + emitted (pt1EvalVertex's split; emitted is 0 unless pt2
+    // transports the vertex's emission)
+    const V3 emd = em > 0.0f ? mk(em * Cy.x, em * Cy.y, em * Cy.z) : mk(0.0f, 0.0f, 0.0f);
+    L.x += tp.x * (kd * Cy.x * E.x + emd.x);
+    L.y += tp.y * (kd * Cy.y * E.y + emd.y);
+    L.z += tp.z * (kd * Cy.z * E.z + emd.z);
+    if (bo == fc.giBounces) {
+      endSample();
+      continue;
+    }
+    tp.x *= kd * Cy.x;
+    tp.y *= kd * Cy.y;
+    tp.z *= kd * Cy.z;
+    if (bo >= 2) {
+      tea2(s0, s1);
+      float pc = fmaxf(tp.x, fmaxf(tp.y, tp.z));
+      pc = fminf(0.95f, fmaxf(0.05f, pc));
+      if (u32ToUnorm(s0) >= pc) {
+        endSample();
+        continue;
+      }
+      const float inv = 1.0f / pc;
+      tp.x *= inv;
+      tp.y *= inv;
+      tp.z *= inv;
+    }
+    tea2(s0, s1);
+    const Frame3 fv = frameFromNormal(Ny);
+    wi = cosineSample(u32ToUnorm(s0), u32ToUnorm(s1), fv);
+    const float epsV = selfIntersectEps(Py, wi, h.t);
+    org = mk(Py.x + Ny.x * epsV, Py.y + Ny.y * epsV, Py.z + Ny.z * epsV);
+    tnear = epsV;
+    ++bo;
   }
   {
     const uint32_t g = waveSum(nGi), ne = waveSum(nNee);
