@@ -151,10 +151,14 @@ struct Mat {
   bool metallic;
   float reflection, emission;
   int model;
+  // Material::Pbr (scene.hpp:150-166)
+  float pbrMetallic, pbrRoughness, pbrSpecular, pbrAnisotropy, pbrRotation;
 };
 inline Mat toMat(const ubg_material& m) {
-  return Mat{m.ambient, m.diffuse,    m.specular,   m.roughness,  m.brilliance, m.phong,
-             m.phong_size, m.metallic != 0, m.reflection, m.emission, m.model};
+  return Mat{m.ambient,        m.diffuse,       m.specular,      m.roughness,       m.brilliance,
+             m.phong,          m.phong_size,    m.metallic != 0, m.reflection,      m.emission,
+             m.model,          m.pbr_metallic,  m.pbr_roughness, m.pbr_specular,    m.pbr_anisotropy,
+             m.pbr_anisotropy_rotation};
 }
 
 struct PrimSide {  // BuiltScene side-table entry for one primitive
@@ -163,6 +167,10 @@ struct PrimSide {  // BuiltScene side-table entry for one primitive
   uint16_t group;
   uint8_t fromEdge;
   float opacity1;
+  // cylinders: unit p0 -> p1 axis (BuiltScene::cylAxis / cylCapAxis,
+  // curve_build.cpp:163-167 baked, :243-245 unbaked), the principled
+  // anisotropy tangent
+  float axis[3] = {0.0f, 0.0f, 0.0f};
 };
 
 struct Geometry {
@@ -191,6 +199,15 @@ struct Geometry {
 
 static inline uint32_t cornerVertex(const ubg_scene_desc* d, int64_t corner) {
   return d->index ? d->index[corner] : static_cast<uint32_t>(corner);
+}
+
+// scene.hpp:44-49 safeNormalize (no fallback) of p1 - p0, into ps.axis
+static void setAxis(PrimSide& ps, const float p0[3], const float p1[3]) {
+  const float x = p1[0] - p0[0], y = p1[1] - p0[1], z = p1[2] - p0[2];
+  const float sc = 1.0f / std::sqrt(std::fmax(x * x + y * y + z * z, 0x1.0p-126f));
+  ps.axis[0] = x * sc;
+  ps.axis[1] = y * sc;
+  ps.axis[2] = z * sc;
 }
 
 // curve_build.cpp:88-199 -- endpoint-hash chaining of `open` cylinders.
@@ -262,6 +279,7 @@ static void chainOpenCylinders(const ubg_scene_desc* d, const std::vector<int>& 
       ps.group = d->cyl_group ? d->cyl_group[c] : 0;
       ps.fromEdge = d->cyl_from_edge ? d->cyl_from_edge[c] : 0;
       ps.opacity1 = d->cyl_opacity1 ? d->cyl_opacity1[c] : -1.0f;
+      setAxis(ps, a0[chain[j]].data(), a1[chain[j]].data());
       g.rSide.push_back(ps);
     }
   };
@@ -336,6 +354,7 @@ static void buildGeometry(const ubg_scene_desc* d, Geometry& g) {
       ps.group = d->cyl_group ? d->cyl_group[c] : 0;
       ps.fromEdge = d->cyl_from_edge ? d->cyl_from_edge[c] : 0;
       ps.opacity1 = d->cyl_opacity1 ? d->cyl_opacity1[c] : -1.0f;
+      setAxis(ps, &d->cyl_p0[3 * c], &d->cyl_p1[3 * c]);
       g.cSide.push_back(ps);
     }
   }
@@ -920,8 +939,9 @@ struct Counters {
   uint64_t giRays = 0, giNee = 0;
 };
 
-// scene.hpp:171-196 -- Material::diffuseWeight (POV: diffuse) and toonLike
-inline float diffuseWeight(const Mat& m) { return m.diffuse; }
+// scene.hpp:171-196 -- Material::diffuseWeight (principled: the metal cut
+// diffuse * (1 - metallic); POV and toon: diffuse) and toonLike
+inline float diffuseWeight(const Mat& m) { return m.model == 1 ? m.diffuse * (1.0f - m.pbrMetallic) : m.diffuse; }
 inline bool toonLike(const Mat& m) {
   if (m.model == 2) return true;
   if (m.model != 0) return false;
@@ -1203,12 +1223,178 @@ inline AOQuality<T> computeAOQuality(const Ctx<T>& c, V3<T> P, V3<T> Ng, V3<T> N
   return r;
 }
 
-// shading.hpp:76-192 (POV model)
+// ---- principled.hpp:33-305 -- the GGX principled subset, direct pass ----
+template <class T>
+inline T ggxD(T a2, T ndh) {  // principled.hpp:63-68
+  if (ndh <= T(0)) return T(0);
+  const T t = ndh * ndh * (a2 - T(1)) + T(1);
+  return a2 / (T(3.14159265358979323846f) * t * t);
+}
+template <class T>
+inline T ggxLambda(T a2, T cosT) {  // principled.hpp:72-76
+  const T c2 = std::fmax(cosT * cosT, T(1.0e-12f));
+  const T t2 = (T(1) - c2) / c2;
+  return T(0.5f) * (std::sqrt(T(1) + a2 * t2) - T(1));
+}
+template <class T>
+inline T ggxDAniso(T ax, T ay, T hx, T hy, T hz) {  // principled.hpp:87-92
+  if (hz <= T(0)) return T(0);
+  const T sx = hx / ax, sy = hy / ay;
+  const T t = sx * sx + sy * sy + hz * hz;
+  return T(1) / (T(3.14159265358979323846f) * ax * ay * t * t);
+}
+template <class T>
+inline T ggxLambdaAniso(T ax, T ay, T vx, T vy, T vz) {  // principled.hpp:95-99
+  const T z2 = std::fmax(vz * vz, T(1.0e-12f));
+  const T t2 = (ax * ax * vx * vx + ay * ay * vy * vy) / z2;
+  return T(0.5f) * (std::sqrt(T(1) + t2) - T(1));
+}
+template <class T>
+inline V3<T> schlickF(V3<T> F0, T u) {  // principled.hpp:52-59
+  T c = T(1) - u;
+  if (c < T(0)) c = T(0);
+  if (c > T(1)) c = T(1);
+  const T c5 = c * c * c * c * c;
+  return V3<T>{F0.x + (T(1) - F0.x) * c5, F0.y + (T(1) - F0.y) * c5, F0.z + (T(1) - F0.z) * c5};
+}
+template <class T>
+inline T principledAspect(T anisotropy) {  // principled.hpp:79-83
+  T a = std::fabs(anisotropy);
+  if (a > T(1)) a = T(1);
+  return std::sqrt(T(1) - T(0.9f) * a);
+}
+
+// principled.hpp:139-305 -- shadePrincipled (the traced-reflection flag is
+// always off here: configurations that would set it are refused)
+template <class T>
+inline V3<T> shadePrincipled(const Ctx<T>& c, const Mat& mat, V3<T> C, V3<T> N, V3<T> V, V3<T> ambLight,
+                             V3<T> aoFactor, T diffuseAo, V3<T> P, V3<T> Ng, T eps, bool shadowsOn,
+                             int shadowSamples, uint32_t px, uint32_t py, const V3<T>* tangent,
+                             Counters& cnt) {
+  const T em = T(mat.emission), amb = T(mat.ambient);
+  V3<T> out{em * C.x + aoFactor.x * amb * C.x * ambLight.x, em * C.y + aoFactor.y * amb * C.y * ambLight.y,
+            em * C.z + aoFactor.z * amb * C.z * ambLight.z};
+  const T kd = T(mat.diffuse) * (T(1) - T(mat.pbrMetallic));  // principledDiffuseWeight
+  const T dd = T(0.08f) * T(mat.pbrSpecular), tm = T(mat.pbrMetallic);  // principledF0
+  const V3<T> F0{dd + (C.x - dd) * tm, dd + (C.y - dd) * tm, dd + (C.z - dd) * tm};
+  const T f0max = std::fmax(F0.x, std::fmax(F0.y, F0.z));
+  const T kAlphaMin = T(1.0e-3f);  // kGgxDirectAlphaMin
+  T alpha = T(mat.pbrRoughness) * T(mat.pbrRoughness);
+  if (alpha < kAlphaMin) alpha = kAlphaMin;
+  const T a2 = alpha * alpha;
+  const T ndv = std::fmax(dot(N, V), T(1.0e-4f));
+  const T lamV = ggxLambda(a2, ndv);
+  const T kPi = T(3.14159265358979323846f);
+  // principledAnisoCtx (principled.hpp:113-128)
+  const bool an = tangent != nullptr && mat.pbrAnisotropy != 0.0f;
+  V3<T> t1{T(0), T(0), T(0)}, t2{T(0), T(0), T(0)};
+  T ax = T(0), ay = T(0);
+  if (an) {
+    t1 = *tangent;
+    t2 = cross(N, t1);
+    const T aspect = principledAspect(T(mat.pbrAnisotropy));
+    ax = alpha / aspect;
+    if (ax > T(1)) ax = T(1);
+    ay = alpha * aspect;
+    if (ax < kAlphaMin) ax = kAlphaMin;
+    if (ay < kAlphaMin) ay = kAlphaMin;
+  }
+  const T lamVA = an ? ggxLambdaAniso(ax, ay, dot(V, t1), dot(V, t2), ndv) : T(0);
+  auto specWeight = [&](V3<T> w, T ndlw, V3<T> H, T ndh) -> T {
+    if (!an) {
+      const T G2 = T(1) / (T(1) + lamV + ggxLambda(a2, ndlw));
+      return kPi * ggxD(a2, ndh) * G2 / (T(4) * ndv);
+    }
+    const T G2 = T(1) / (T(1) + lamVA + ggxLambdaAniso(ax, ay, dot(w, t1), dot(w, t2), ndlw));
+    return kPi * ggxDAniso(ax, ay, dot(H, t1), dot(H, t2), ndh) * G2 / (T(4) * ndv);
+  };
+  uint32_t s0 = px, s1 = py;
+  uint32_t q0 = px ^ 0x50425244u, q1 = py;  // the area path's 'PBRD'-tagged stream
+  for (const Light<T>& l : c.lights) {
+    const T ndl = dot(N, l.L);
+    if (ndl <= T(0)) continue;
+    const bool areaPath = shadowsOn && shadowSamples > 1 && l.radius > T(0);
+    if (!areaPath) {
+      const T sf = shadowsOn ? computeShadow(c, P, Ng, N, eps, l, shadowSamples, s0, s1, cnt) : T(1);
+      const V3<T> Lc{l.color.x * sf, l.color.y * sf, l.color.z * sf};
+      const T dk = kd * ndl * diffuseAo;
+      out.x += dk * C.x * Lc.x;
+      out.y += dk * C.y * Lc.y;
+      out.z += dk * C.z * Lc.z;
+      if (!l.highlight) continue;
+      if (f0max > T(0)) {
+        const V3<T> H = normalize(V3<T>{l.L.x + V.x, l.L.y + V.y, l.L.z + V.z});
+        const T ndh = dot(N, H);
+        if (ndh > T(0)) {
+          const T sW = specWeight(l.L, ndl, H, ndh);
+          const V3<T> F = schlickF(F0, dot(V, H));
+          out.x += sW * F.x * Lc.x;
+          out.y += sW * F.y * Lc.y;
+          out.z += sW * F.z * Lc.z;
+        }
+      }
+    } else {
+      V3<T> ng = (dot(Ng, N) < T(0)) ? neg(Ng) : Ng;
+      ng = safeNormalize(ng, N);
+      const V3<T> O = P + ng * eps;
+      const T far = std::numeric_limits<T>::infinity();
+      T accD = T(0);
+      V3<T> accS{T(0), T(0), T(0)};
+      for (int i = 0; i < shadowSamples; ++i) {
+        tea2(q0, q1);
+        const V3<T> w = sampleLightDir(l, u32ToUnorm<T>(q0), u32ToUnorm<T>(q1));
+        const T ndlS = dot(N, w);
+        if (ndlS <= T(0)) continue;
+        cnt.shadow++;
+        if (occluded(*c.s, O, w, eps, far)) continue;
+        accD += ndlS;
+        if (l.highlight && f0max > T(0)) {
+          const V3<T> H = normalize(V3<T>{w.x + V.x, w.y + V.y, w.z + V.z});
+          const T ndh = dot(N, H);
+          if (ndh > T(0)) {
+            const T sW = specWeight(w, ndlS, H, ndh);
+            const V3<T> F = schlickF(F0, dot(V, H));
+            accS.x += sW * F.x;
+            accS.y += sW * F.y;
+            accS.z += sW * F.z;
+          }
+        }
+      }
+      const T inv = T(1) / T(shadowSamples);
+      const T dk = kd * diffuseAo * inv * accD;
+      out.x += dk * C.x * l.color.x;
+      out.y += dk * C.y * l.color.y;
+      out.z += dk * C.z * l.color.z;
+      out.x += inv * accS.x * l.color.x;
+      out.y += inv * accS.y * l.color.y;
+      out.z += inv * accS.z * l.color.z;
+    }
+  }
+  // the fake environment specular: a converted POV reflection scalar, else
+  // the constant F0 x background
+  if (mat.reflection > 0.0f) {
+    const T r = T(mat.reflection);
+    out.x += r * c.bg.x;
+    out.y += r * c.bg.y;
+    out.z += r * c.bg.z;
+  } else if (f0max > T(0)) {
+    out.x += F0.x * c.bg.x;
+    out.y += F0.y * c.bg.y;
+    out.z += F0.z * c.bg.z;
+  }
+  return out;
+}
+
+// shading.hpp:76-192 (POV model; principled materials go to their own
+// evaluator, shading.hpp:88-91)
 template <class T>
 inline V3<T> shadeLocal(const Ctx<T>& c, const Mat& mat, V3<T> C, V3<T> N, V3<T> V,
                         V3<T> ambLight, V3<T> aoFactor, T diffuseAo, V3<T> P, V3<T> Ng, T eps,
                         bool shadowsOn, int shadowSamples, uint32_t px, uint32_t py,
-                        Counters& cnt) {
+                        Counters& cnt, const V3<T>* tangent = nullptr) {
+  if (mat.model == 1)
+    return shadePrincipled(c, mat, C, N, V, ambLight, aoFactor, diffuseAo, P, Ng, eps, shadowsOn, shadowSamples,
+                           px, py, tangent, cnt);
   const T em = T(mat.emission), amb = T(mat.ambient);
   V3<T> out{em * C.x + aoFactor.x * amb * C.x * ambLight.x,
             em * C.y + aoFactor.y * amb * C.y * ambLight.y,
@@ -1422,8 +1608,43 @@ inline HitShade<T> shadeHit(const Ctx<T>& c, const Hit<T>& h, V3<T> rd, V3<T> or
   }
   hs.aoOpen = ao.openness;
   const bool primShadows = !fromEdge && shadowsActive;
+  // hit_shader.hpp:373-425: the anisotropy tangent of a principled sphere
+  // (meridian of the world +Y pole frame) or cylinder (its axis projected on
+  // the tangent plane); poles, caps and degenerate projections stay isotropic
+  V3<T> anisoT{T(0), T(0), T(0)};
+  bool tanValid = false;
+  if (pm.model == 1 && pm.pbrAnisotropy != 0.0f && !fromEdge) {
+    V3<T> t1{T(0), T(0), T(0)};
+    if (h.kind == UBG_KIND_SPHERE) {
+      if (std::fabs(N.y) < T(0.999f)) {
+        const V3<T> tPhi = safeNormalize(cross(V3<T>{T(0), T(1), T(0)}, N));
+        t1 = cross(N, tPhi);
+        tanValid = true;
+      }
+    } else {
+      const V3<T> A{T(side->axis[0]), T(side->axis[1]), T(side->axis[2])};
+      const V3<T> ngu = safeNormalize(Ng, N);
+      if (std::fabs(dot(ngu, A)) < T(0.99f)) {
+        const T andn = dot(A, N);
+        const V3<T> proj{A.x - N.x * andn, A.y - N.y * andn, A.z - N.z * andn};
+        if (dot(proj, proj) >= T(1.0e-12f)) {
+          t1 = safeNormalize(proj);
+          tanValid = true;
+        }
+      }
+    }
+    if (tanValid) {
+      if (pm.pbrRotation != 0.0f) {
+        const T ang = T(pm.pbrRotation) * T(6.2831853072f);
+        const V3<T> t2 = cross(N, t1);
+        const T ca = std::cos(ang), sa = std::sin(ang);
+        t1 = V3<T>{ca * t1.x + sa * t2.x, ca * t1.y + sa * t2.y, ca * t1.z + sa * t2.z};
+      }
+      anisoT = t1;
+    }
+  }
   hs.color = shadeLocal(c, pm, C, N, V, ao.ambLight, ao.aoFactor, ao.diffuseAo, P, Ng, secEps, primShadows,
-                        c.p->shadow_samples * c.shMul, px, py, cnt);
+                        c.p->shadow_samples * c.shMul, px, py, cnt, tanValid ? &anisoT : nullptr);
   hs.opacity = T(side->color[3]);
   hs.normal = N;
   hs.group = side->group;
@@ -2073,22 +2294,27 @@ static bool giSupported(const ubg_scene_desc* d, const ubg_render_params* p) {
   if (p->gi_integrator == 1) return true;
   if (p->gi_integrator != 2 || p->pt2_rounds > 0 || p->pt2_adaptive) return false;
   bool emit = false, refl = false;
+  // hit_shader.hpp:250-256: a principled material owns a traced reflection
+  // whenever its specular lobe exists, a POV one when reflection > 0
+  auto owns = [](const ubg_material& m) {
+    return m.model == 1 ? (m.pbr_metallic > 1e-4f || m.pbr_specular > 1e-4f) : m.reflection > 0.0f;
+  };
   if (d->n_triangles > 0 && d->n_vertices >= 3)
     for (int32_t m = 0; m < d->n_mesh_materials; ++m) {
       emit = emit || d->mesh_materials[m].emission > 0.0f;
-      refl = refl || d->mesh_materials[m].reflection > 0.0f;
+      refl = refl || owns(d->mesh_materials[m]);
     }
   for (int64_t i = 0; i < d->n_spheres; ++i) {
     if (d->sphere_from_edge && d->sphere_from_edge[i]) continue;
     const ubg_material& m = d->prim_materials[d->sphere_material ? d->sphere_material[i] : 0];
     emit = emit || (m.emission > 0.0f && d->sphere_center_radius[4 * i + 3] > 0.0f);
-    refl = refl || m.reflection > 0.0f;
+    refl = refl || owns(m);
   }
   for (int64_t i = 0; i < d->n_cylinders; ++i) {
     if (d->cyl_from_edge && d->cyl_from_edge[i]) continue;
     const ubg_material& m = d->prim_materials[d->cyl_material ? d->cyl_material[i] : 0];
     emit = emit || (!(d->cyl_open && d->cyl_open[i]) && m.emission > 0.0f && d->cyl_radius[i] > 0.0f);
-    refl = refl || m.reflection > 0.0f;
+    refl = refl || owns(m);
   }
   if (p->pt2_emissive && p->pt2_emissive_nee && emit) return false;
   if (p->pt2_reflect && refl) return false;
@@ -2821,10 +3047,6 @@ int orc_render(const ubg_scene_desc* desc, const ubg_render_params* params, ubg_
   if (!desc || !params || !frame) return UBG_ERR_INVALID_ARGUMENT;
   if (params->width <= 0 || params->height <= 0) return UBG_ERR_INVALID_ARGUMENT;
   const orc_options* o = opt ? opt : &kDefaultOpt;
-  for (int32_t i = 0; i < desc->n_prim_materials; ++i)
-    if (desc->prim_materials[i].model == 1) return UBG_ERR_UNSUPPORTED;
-  for (int32_t i = 0; i < desc->n_mesh_materials; ++i)
-    if (desc->mesh_materials[i].model == 1) return UBG_ERR_UNSUPPORTED;
   if (!orc::giSupported(desc, params)) return UBG_ERR_UNSUPPORTED;
   return o->precision == 64 ? orc::renderT<double>(desc, params, frame, o)
                             : orc::renderT<float>(desc, params, frame, o);

@@ -429,3 +429,55 @@ def test_pt1_gi_reduced(device, workload):
     assert abs(g.stats["gi_rays"] - c.stats["gi_rays"]) <= 1e-3 * c.stats["gi_rays"]
     assert g.stats["gi_gbuffer_rays"] == c.stats["gi_gbuffer_rays"]
     assert np.abs(g.gi_occlusion - c.gi_occlusion).mean() <= 1e-3
+
+
+def _principled_materials(sc, mesh: bool):
+    """Principled metals on every primitive (anisotropic, rotated: the
+    sphere pole frame and the cylinder axis frame), or on the mesh (whose
+    anisotropy is inert); a grey background lights the Fresnel x background
+    term."""
+    from umbreon_amd.scene import Material
+    sph = Material(ambient=0.2, diffuse=0.8, model=1, pbr_metallic=1.0, pbr_roughness=0.35, pbr_specular=0.5,
+                   pbr_anisotropy=0.8, pbr_anisotropy_rotation=0.1)
+    cyl = Material(ambient=0.2, diffuse=0.8, model=1, pbr_metallic=0.3, pbr_roughness=0.5, pbr_specular=0.7,
+                   pbr_anisotropy=-0.6, pbr_anisotropy_rotation=0.3)
+    if mesh:
+        sc.mesh_material = Material(ambient=0.2, diffuse=0.8, model=1, pbr_metallic=0.5, pbr_roughness=0.3,
+                                    pbr_specular=0.5, pbr_anisotropy=0.7)
+    sc.prim_materials = [sph, cyl]
+    sc.sphere_material = np.zeros(len(sc.sphere_center), np.int32)
+    sc.cyl_material = np.ones(len(sc.cyl_p0), np.int32)
+    sc.background = (0.3, 0.35, 0.4)
+
+
+@pytest.mark.parametrize("variant", ["ballstick_hard", "ballstick_area_light", "surface", "ballstick_pt1"])
+def test_principled_reduced(device, variant):
+    """shadePrincipled on the GPU against the oracle: GGX highlight, Schlick
+    Fresnel, the anisotropy frames, the area-light per-sample estimator (its
+    own 'PBRD' stream), the environment term and the diffuse weight the GI
+    gather reads."""
+    if variant.startswith("ballstick"):
+        sc, opt = ball_and_stick_scene(n_atoms=3000, width=320, height=180)
+    else:
+        sc, opt = molecular_surface_scene(n_atoms=1500, width=320, height=180)
+    _principled_materials(sc, mesh=variant == "surface")
+    if variant == "ballstick_area_light":
+        opt.shadow_samples, opt.light_radius = 4, 3.0
+    print(f"principled {variant} {opt.width}x{opt.height} ss{opt.supersample}", flush=True)
+    if variant == "ballstick_pt1":
+        opt.width, opt.height, opt.supersample, opt.ao_samples = 160, 90, 1, 0
+        opt.gi, opt.gi_integrator, opt.pt1_spp, opt.gi_bounces = True, 1, 8, 2
+        g = _gpu(device, sc, opt, hires_only=True, want_ids=True)
+        c = ob.render(sc, opt, hires_only=True)
+        assert np.array_equal(g.hit_id, c.hit_id)
+        di = np.abs(g.indirect - c.indirect)
+        print(f"principled GI: indirect mean|d| {di.mean():.3g}, gi rays GPU {g.stats['gi_rays']} "
+              f"oracle {c.stats['gi_rays']}", flush=True)
+        assert di.max(axis=2).mean() <= 1e-3
+        dd = np.abs((g.color[..., :3] - g.indirect) - (c.color[..., :3] - c.indirect)).max(axis=2)
+        assert (dd > 1e-3).mean() <= 5e-4
+        assert abs(g.stats["gi_rays"] - c.stats["gi_rays"]) <= 1e-3 * c.stats["gi_rays"]
+        return
+    g = _gpu(device, sc, opt, hires_only=True, want_ids=True, want_ao=True)
+    c = ob.render(sc, opt, hires_only=True)
+    _compare(f"principled {variant}", g, c, ao=opt.ao_samples > 0)

@@ -206,12 +206,6 @@ void checkSupported(const Scene& s, const RenderOptions& o) {
   if (o.strokeEdges.enable) no("stroke edges");
   if (o.objectSpaceEdges.enable) no("object-space edges");
   if (s.lights.size() > 16) no("more than 16 lights");
-  auto principled = [](const Material& m) { return m.model == ShadingModel::Principled; };
-  bool p = principled(s.mesh.material);
-  for (const Material& m : s.mesh.materials) p = p || principled(m);
-  for (const Sphere& sp : s.spheres) p = p || principled(sp.material);
-  for (const Cylinder& cy : s.cylinders) p = p || principled(cy.material);
-  if (p) no("ShadingModel::Principled");
 }
 
 // renderFrame (pipeline.cpp:20-256) for one pass on the GPU.

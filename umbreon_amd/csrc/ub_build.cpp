@@ -55,7 +55,23 @@ DevMaterial toDev(const ubg_material& m) {
     toon = m.specular > 1.0f || m.phong > 1.0f || m.brilliance == 0.0f ||
            (m.diffuse <= 0.0f && m.specular <= 0.0f && m.phong <= 0.0f && m.reflection <= 0.0f);
   d.toon = toon ? 1u : 0u;
+  d.principled = m.model == 1 ? 1u : 0u;
+  d.pbrMetallic = m.pbr_metallic;
+  d.pbrRoughness = m.pbr_roughness;
+  d.pbrSpecular = m.pbr_specular;
+  d.pbrAnisotropy = m.pbr_anisotropy;
+  d.pbrRotation = m.pbr_anisotropy_rotation;
+  // hit_shader.hpp:415-418
+  const float ang = m.pbr_anisotropy_rotation * 6.2831853072f;
+  d.rotCos = std::cos(ang);
+  d.rotSin = std::sin(ang);
   return d;
+}
+
+// scene.hpp:44-49 (safeNormalize without a fallback)
+F4 unitAxis(float x, float y, float z) {
+  const float s = 1.0f / std::sqrt(std::fmax(x * x + y * y + z * z, 0x1.0p-126f));
+  return F4{x * s, y * s, z * s, 0.0f};
 }
 
 struct Vf {
@@ -739,19 +755,25 @@ int buildHostScene(const ubg_scene_desc* d, int layout, HostScene& out, std::str
 
   // Materials.
   for (int32_t i = 0; i < d->n_prim_materials; ++i) {
-    if (d->prim_materials[i].model == 1) {
-      err = "Principled materials are not supported by the gfx950 path yet";
-      return UBG_ERR_UNSUPPORTED;
-    }
     out.primMats.push_back(toDev(d->prim_materials[i]));
+    out.principled = out.principled || d->prim_materials[i].model == 1;
   }
   for (int32_t i = 0; i < d->n_mesh_materials; ++i) {
-    if (d->mesh_materials[i].model == 1) {
-      err = "Principled materials are not supported by the gfx950 path yet";
-      return UBG_ERR_UNSUPPORTED;
-    }
     out.meshMats.push_back(toDev(d->mesh_materials[i]));
+    out.principled = out.principled || d->mesh_materials[i].model == 1;
   }
+  // embree_renderer.cpp:421-430 / curve_build.cpp:47-55: an anisotropic
+  // principled sphere or cylinder (mesh anisotropy is inert)
+  auto anisoMat = [&](const int32_t* mi, int64_t i) {
+    if (d->n_prim_materials <= 0) return false;
+    const int32_t m = mi ? mi[i] : 0;
+    if (m < 0 || m >= d->n_prim_materials) return false;
+    return d->prim_materials[m].model == 1 && d->prim_materials[m].pbr_anisotropy != 0.0f;
+  };
+  bool anisoCyl = false;
+  for (int64_t i = 0; i < d->n_cylinders && !anisoCyl; ++i) anisoCyl = anisoMat(d->cyl_material, i);
+  out.anisoPrims = anisoCyl;
+  for (int64_t i = 0; i < d->n_spheres && !out.anisoPrims; ++i) out.anisoPrims = anisoMat(d->sphere_material, i);
 
   std::vector<Box> boxes;
   std::vector<F4> recs;  // 3 per primitive, geomID order before the BVH permutes them
@@ -897,7 +919,22 @@ int buildHostScene(const ubg_scene_desc* d, int layout, HostScene& out, std::str
       std::vector<uint32_t> segStart;
       std::vector<uint8_t> segFlags;
       std::vector<int> segSrc;
-      for (int64_t c = 0; c < copies; ++c) chainOpen(d, openIdx, offs[c].data(), verts, segStart, segFlags, segSrc);
+      for (int64_t c = 0; c < copies; ++c) {
+        const size_t first = segSrc.size();
+        chainOpen(d, openIdx, offs[c].data(), verts, segStart, segFlags, segSrc);
+        // curve_build.cpp:163-167: the baked p0 -> p1 axis of each segment's
+        // source cylinder
+        if (anisoCyl)
+          for (size_t j = first; j < segSrc.size(); ++j) {
+            const int ci = segSrc[j];
+            float a0[3], a1[3];
+            for (int k = 0; k < 3; ++k) {
+              a0[k] = d->cyl_p0[3 * ci + k] + offs[c][k];
+              a1[k] = d->cyl_p1[3 * ci + k] + offs[c][k];
+            }
+            out.rndAxis.push_back(unitAxis(a1[0] - a0[0], a1[1] - a0[1], a1[2] - a0[2]));
+          }
+      }
       for (size_t j = 0; j < segStart.size(); ++j) {
         const uint32_t v = segStart[j];
         const float* A = &verts[4 * v];
@@ -953,6 +990,11 @@ int buildHostScene(const ubg_scene_desc* d, int layout, HostScene& out, std::str
         out.coneInfo.push_back(info);
         out.coneOp1.push_back(op1);
         out.coneGroup.push_back(d->cyl_group ? d->cyl_group[ci] : 0);
+        // curve_build.cpp:243-245: the source cylinder's p0 -> p1 axis
+        if (anisoCyl)
+          out.coneAxis.push_back(unitAxis(d->cyl_p1[3 * ci] - d->cyl_p0[3 * ci],
+                                          d->cyl_p1[3 * ci + 1] - d->cyl_p0[3 * ci + 1],
+                                          d->cyl_p1[3 * ci + 2] - d->cyl_p0[3 * ci + 2]));
         DBox bx;
         bx.ext(a.x, a.y, a.z, std::fabs(static_cast<double>(r)));
         bx.ext(b.x, b.y, b.z, std::fabs(static_cast<double>(r)));
@@ -979,23 +1021,28 @@ int buildHostScene(const ubg_scene_desc* d, int layout, HostScene& out, std::str
   // material) and reflective receivers (hit_shader.hpp:251-254, :417-420)
   {
     auto primMat = [&](int32_t m) -> const ubg_material& { return d->prim_materials[m]; };
+    // hit_shader.hpp:250-256: a principled material owns a traced reflection
+    // whenever its specular lobe exists, a POV one when reflection > 0
+    auto ownsReflection = [](const ubg_material& m) {
+      return m.model == 1 ? (m.pbr_metallic > 1e-4f || m.pbr_specular > 1e-4f) : m.reflection > 0.0f;
+    };
     if (out.nTris > 0)
       for (int32_t m = 0; m < d->n_mesh_materials; ++m) {
         out.pt2Emitters = out.pt2Emitters || d->mesh_materials[m].emission > 0.0f;
-        out.pt2Reflective = out.pt2Reflective || d->mesh_materials[m].reflection > 0.0f;
+        out.pt2Reflective = out.pt2Reflective || ownsReflection(d->mesh_materials[m]);
       }
     for (int64_t i = 0; i < d->n_spheres; ++i) {
       if (d->sphere_from_edge && d->sphere_from_edge[i]) continue;
       const ubg_material& m = primMat(d->sphere_material ? d->sphere_material[i] : 0);
       out.pt2Emitters = out.pt2Emitters || (m.emission > 0.0f && d->sphere_center_radius[4 * i + 3] > 0.0f);
-      out.pt2Reflective = out.pt2Reflective || m.reflection > 0.0f;
+      out.pt2Reflective = out.pt2Reflective || ownsReflection(m);
     }
     for (int64_t i = 0; i < d->n_cylinders; ++i) {
       if (d->cyl_from_edge && d->cyl_from_edge[i]) continue;
       const ubg_material& m = primMat(d->cyl_material ? d->cyl_material[i] : 0);
       const bool open = d->cyl_open && d->cyl_open[i];
       out.pt2Emitters = out.pt2Emitters || (!open && m.emission > 0.0f && d->cyl_radius[i] > 0.0f);
-      out.pt2Reflective = out.pt2Reflective || m.reflection > 0.0f;
+      out.pt2Reflective = out.pt2Reflective || ownsReflection(m);
     }
   }
   const size_t nPrims = boxes.size();

@@ -37,6 +37,9 @@ struct HostScene {
   std::vector<U4> triIdx;
   // section groups: per sphere, cone and round segment, and per base triangle
   std::vector<uint16_t> sphGroup, coneGroup, rndGroup, triGroup;
+  // anisotropy tangent axes (curve_build.cpp:163-167, :243-245), built only
+  // when an anisotropic principled cylinder exists
+  std::vector<F4> coneAxis, rndAxis;
   std::vector<F4> vtxNrm;
   std::vector<F4> vtxCol;
   std::vector<DevMaterial> primMats;
@@ -45,7 +48,9 @@ struct HostScene {
   int64_t meshBaseTris = 0;  // triangles of one instance copy
   bool aoReceivers = false;  // a mesh or a primitive that is not outline decoration
   bool pt2Emitters = false;  // an emissive-NEE source for the pt2 integrator
-  bool pt2Reflective = false;  // a receiver with Material::reflection > 0
+  bool pt2Reflective = false;  // a receiver that owns a traced reflection
+  bool principled = false;     // a material with ShadingModel::Principled
+  bool anisoPrims = false;     // an anisotropic principled sphere or cylinder
   int32_t layout = kLayoutBvh8;
   int32_t depth = 0;       // tree levels
   int32_t stackDepth = 0;  // traversal stack entries needed

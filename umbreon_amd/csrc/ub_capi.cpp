@@ -65,7 +65,7 @@ struct ubg_context {
       hitId, aoOpen, outColor, counters, query, queryOut, hitOpen, hitBent, cGuide, cOpen, cBent;
   // adaptive AA: hit output indices, the centre / replicated / refined sample
   // sets (32 B per sample), first-hit records, per-output-pixel mask state
-  DevBuf hitOut, aaSetC, aaSetR, aaSetF, aaFirstN, aaPix;
+  DevBuf hitOut, aaSetC, aaSetR, aaSetF, aaFirstN, aaPix, hitPrim;
   // pt1 GI: first-hit position / normal / reflectance, gathered E, receiver
   // list, staging of the indirect and occlusion AOVs
   DevBuf giPos, giNrm, giRefl, giE, giList, giInd, giOcc, dnA, dnB, dnGrad, dnVar, giLow, giPatch, giSnap;
@@ -73,7 +73,7 @@ struct ubg_context {
   std::vector<DevBuf*> all() {
     return {&hitPix, &hitT,   &hitO,    &hitN,  &hitC,  &hitAo,   &activeA,  &activeB,  &acc,     &tnear,
             &hiColor, &depth, &viewZ,   &hitId, &aoOpen, &outColor, &counters, &query, &queryOut, &aoT,
-            &hitOpen, &hitBent, &cGuide, &cOpen, &cBent, &hitOut, &aaSetC, &aaSetR, &aaSetF, &aaFirstN,
+            &hitOpen, &hitBent, &cGuide, &cOpen, &cBent, &hitOut, &aaSetC, &aaSetR, &aaSetF, &aaFirstN, &hitPrim,
             &aaPix, &giPos, &giNrm, &giRefl, &giE, &giList, &giInd, &giOcc, &dnA, &dnB, &dnGrad,
             &dnVar, &giLow, &giPatch, &giSnap};
   }
@@ -84,6 +84,7 @@ struct ubg_scene {
   ubg::SceneView view{};
   bool aoReceivers = false;
   bool pt2Emitters = false, pt2Reflective = false;
+  bool principled = false, anisoPrims = false;
   std::vector<void*> allocs;
   ubg_scene_info info{};
 };
@@ -307,6 +308,8 @@ int ubg_scene_create_ex(ubg_context* ctx, const ubg_scene_desc* desc, int32_t bv
   s->view.rndGroup = static_cast<const uint16_t*>(p);
   up(hs.triGroup, &p);
   s->view.triGroup = static_cast<const uint16_t*>(p);
+  up(hs.coneAxis, &s->view.coneAxis);
+  up(hs.rndAxis, &s->view.rndAxis);
   if (r == UBG_OK) {
     hipError_t e = hipStreamSynchronize(ctx->stream);
     if (e != hipSuccess) r = hipFail(e, "scene upload");
@@ -320,6 +323,8 @@ int ubg_scene_create_ex(ubg_context* ctx, const ubg_scene_desc* desc, int32_t bv
   s->aoReceivers = hs.aoReceivers;
   s->pt2Emitters = hs.pt2Emitters;
   s->pt2Reflective = hs.pt2Reflective;
+  s->principled = hs.principled;
+  s->anisoPrims = hs.anisoPrims;
   s->view.nPrims = static_cast<int32_t>(hs.prims.size() / 3);  // records in either order
   s->view.layout = hs.layout;
   s->view.stackDepth = std::max(1, hs.stackDepth);
@@ -596,6 +601,7 @@ int ubg_render(ubg_context* ctx, const ubg_scene* scene, const ubg_render_params
   const size_t outN = static_cast<size_t>(p->width) * p->height;
   need(ctx->outColor, outN * 16);
   need(ctx->counters, ubg::kCounterWords * sizeof(unsigned long long));
+  if (scene->anisoPrims) need(ctx->hitPrim, qn * 4);
   if (adaptive) {
     need(ctx->hitOut, qn * 4);
     need(ctx->aaSetC, static_cast<size_t>(nOut) * 32);
@@ -651,6 +657,8 @@ int ubg_render(ubg_context* ctx, const ubg_scene* scene, const ubg_render_params
   b.counters = static_cast<unsigned long long*>(ctx->counters.p);
   b.partCap = partCap;
   b.hitOut = nullptr;
+  b.hitPrim = scene->anisoPrims ? static_cast<uint32_t*>(ctx->hitPrim.p) : nullptr;
+  b.principled = scene->principled ? 1 : 0;
   b.firstN = nullptr;
   b.smList = nullptr;
   b.giPos = gi ? static_cast<float4*>(ctx->giPos.p) : nullptr;

@@ -18,6 +18,9 @@ struct FrameBufs {
   // hit queue (structure of arrays, filled by the primary pass)
   uint32_t* hitPix;   // seed base px + py * seedW of the hit's ray-lattice position
   uint32_t* hitOut;   // list passes: the hit's output index (null: = hitPix)
+  uint32_t* hitPrim;  // the hit's primitive (kind << kKindShift | id); null unless an
+                      // anisotropic principled primitive needs its tangent
+  int32_t principled; // the scene has a principled material
   float* hitT;
   float4* hitO;       // offset origin xyz, eps
   float4* hitN;       // shading normal xyz, packed flags

@@ -441,7 +441,7 @@ __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(kTr
     if (!fromEdge) {
       const DevMaterial& m = (flags & kFlagMesh) ? s.meshMats[flags & kInfoMatMask] : s.primMats[flags & kInfoMatMask];
       recv = m.toon == 0u;
-      kd = m.diffuse;
+      kd = diffuseWeight(m);
       flags |= recv ? kFlagGiRecv : kFlagGiToon;
     }
     if (iter == 0) {
@@ -460,6 +460,7 @@ __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(kTr
   b.hitN[slot] = make_float4(N.x, N.y, N.z, __uint_as_float(flags));
   b.hitC[slot] = make_float4(C.x, C.y, C.z, opacity);
   b.hitAo[slot] = 0u;
+  if (b.hitPrim) b.hitPrim[slot] = h.bits;
 }
 
 template <bool kCount, int kLayout>
@@ -783,10 +784,214 @@ __device__ void aoApply(const FrameConst& fc, float openness, V3 bent, V3 C, V3&
   diffAo = fc.aoDiffuse > 0.0f ? 1.0f - fc.aoDiffuse * (1.0f - openness) : 1.0f;
 }
 
+// ---- principled direct shading (shading/principled.hpp:139-305) ----
+constexpr float kGgxDirectAlphaMin = 1.0e-3f;
+constexpr float kPiF = 3.14159265358979323846f;
+
+// isotropic GGX NDF, 0 below the horizon
+__device__ __forceinline__ float ggxD(float a2, float ndh) {
+  if (ndh <= 0.0f) return 0.0f;
+  const float t = ndh * ndh * (a2 - 1.0f) + 1.0f;
+  return a2 / (kPiF * t * t);
+}
+// Smith Lambda of the isotropic GGX (cosT = |cos| to the normal)
+__device__ __forceinline__ float ggxLambda(float a2, float cosT) {
+  const float c2 = fmaxf(cosT * cosT, 1.0e-12f);
+  const float t2 = (1.0f - c2) / c2;
+  return 0.5f * (sqrtf(1.0f + a2 * t2) - 1.0f);
+}
+// anisotropic NDF and Lambda, components in the (t1, t2, N) frame
+__device__ __forceinline__ float ggxDAniso(float ax, float ay, float hx, float hy, float hz) {
+  if (hz <= 0.0f) return 0.0f;
+  const float sx = hx / ax, sy = hy / ay;
+  const float t = sx * sx + sy * sy + hz * hz;
+  return 1.0f / (kPiF * ax * ay * t * t);
+}
+__device__ __forceinline__ float ggxLambdaAniso(float ax, float ay, float vx, float vy, float vz) {
+  const float z2 = fmaxf(vz * vz, 1.0e-12f);
+  const float t2 = (ax * ax * vx * vx + ay * ay * vy * vy) / z2;
+  return 0.5f * (sqrtf(1.0f + t2) - 1.0f);
+}
+// Schlick Fresnel, F90 = 1
+__device__ __forceinline__ V3 schlickF(V3 F0, float u) {
+  float c = 1.0f - u;
+  if (c < 0.0f) c = 0.0f;
+  if (c > 1.0f) c = 1.0f;
+  const float c5 = c * c * c * c * c;
+  return mk(F0.x + (1.0f - F0.x) * c5, F0.y + (1.0f - F0.y) * c5, F0.z + (1.0f - F0.z) * c5);
+}
+// Disney aspect of the anisotropy stretch
+__device__ __forceinline__ float principledAspect(float anisotropy) {
+  float a = fabsf(anisotropy);
+  if (a > 1.0f) a = 1.0f;
+  return sqrtf(1.0f - 0.9f * a);
+}
+
+// hit_shader.hpp:382-425: the anisotropy tangent of a principled sphere
+// (meridian of the world +Y pole frame) or cylinder (axis projected on the
+// tangent plane, caps excluded), rotated by the material's rotation. Returns
+// false where the frame degenerates (poles, caps): the hit shades isotropic.
+__device__ __forceinline__ bool principledTangent(const SceneView& s, const DevMaterial& m, uint32_t bits, V3 N,
+                                                  V3& t1) {
+  const uint32_t kind = bits >> kKindShift, id = bits & kIdMask;
+  bool ok = false;
+  if (kind == 1u) {
+    if (fabsf(N.y) < 0.999f) {
+      const V3 tPhi = safeNormalize(cross(mk(0.0f, 1.0f, 0.0f), N));
+      t1 = cross(N, tPhi);
+      ok = true;
+    }
+  } else if (kind == 2u || kind == 3u) {
+    const float4* axes = reinterpret_cast<const float4*>(kind == 3u ? s.coneAxis : s.rndAxis);
+    if (axes) {
+      const V3 A = ld3(axes[id]);
+      // the normalized geometric normal: N up to its sign
+      if (fabsf(dot(N, A)) < 0.99f) {
+        const float andn = dot(A, N);
+        const V3 proj = mk(A.x - N.x * andn, A.y - N.y * andn, A.z - N.z * andn);
+        if (dot(proj, proj) >= 1.0e-12f) {
+          t1 = safeNormalize(proj);
+          ok = true;
+        }
+      }
+    }
+  }
+  if (ok && m.pbrRotation != 0.0f) {
+    const V3 t2 = cross(N, t1);
+    t1 = mk(m.rotCos * t1.x + m.rotSin * t2.x, m.rotCos * t1.y + m.rotSin * t2.y,
+            m.rotCos * t1.z + m.rotSin * t2.z);
+  }
+  return ok;
+}
+
+// shadePrincipled for one hit: the POV model's ambient / emission start and
+// light gates, then per light a diffuse lobe (kd = diffuse * (1 - metallic))
+// and a GGX highlight with Schlick Fresnel, pi-scaled to the POV light units;
+// under an area light with soft shadows, visibility and BRDF per cone sample
+// (the 'PBRD'-tagged per-pixel stream); the Fresnel x background stand-in
+// for the environment. `O`, `eps`: the shadow rays' offset origin.
 template <bool kCount, int kLayout>
-__global__ __launch_bounds__(kTraceBlock) void k_resolve(FrameConst fc, SceneView s, FrameBufs b,
-                                                         int iter, int unused) {
-  (void)unused;
+__device__ V3 shadePrincipled(const FrameConst& fc, const SceneView& s, const DevMaterial& m, V3 C, V3 N, V3 V,
+                              V3 ambL, V3 aoF, float diffAo, V3 O, float eps, bool shOn, uint32_t px, uint32_t py,
+                              bool aniso, V3 t1, uint32_t& nShadow, uint32_t* nv, uint32_t* nt) {
+  V3 out = mk(m.emission * C.x + aoF.x * m.ambient * C.x * ambL.x,
+              m.emission * C.y + aoF.y * m.ambient * C.y * ambL.y,
+              m.emission * C.z + aoF.z * m.ambient * C.z * ambL.z);
+  const float kd = m.diffuse * (1.0f - m.pbrMetallic);
+  const float d = 0.08f * m.pbrSpecular, mt = m.pbrMetallic;
+  const V3 F0 = mk(d + (C.x - d) * mt, d + (C.y - d) * mt, d + (C.z - d) * mt);
+  const float f0max = fmaxf(F0.x, fmaxf(F0.y, F0.z));
+  float alpha = m.pbrRoughness * m.pbrRoughness;
+  if (alpha < kGgxDirectAlphaMin) alpha = kGgxDirectAlphaMin;
+  const float a2 = alpha * alpha;
+  const float ndv = fmaxf(dot(N, V), 1.0e-4f);
+  const float lamV = ggxLambda(a2, ndv);
+  // anisotropic frame (principledAnisoCtx): only with a tangent and a
+  // non-zero anisotropy; otherwise every expression stays isotropic
+  const bool an = aniso && m.pbrAnisotropy != 0.0f;
+  V3 t2 = mk(0.0f, 0.0f, 0.0f);
+  float ax = 0.0f, ay = 0.0f, lamVA = 0.0f;
+  if (an) {
+    t2 = cross(N, t1);
+    const float aspect = principledAspect(m.pbrAnisotropy);
+    ax = alpha / aspect;
+    if (ax > 1.0f) ax = 1.0f;
+    ay = alpha * aspect;
+    if (ax < kGgxDirectAlphaMin) ax = kGgxDirectAlphaMin;
+    if (ay < kGgxDirectAlphaMin) ay = kGgxDirectAlphaMin;
+    lamVA = ggxLambdaAniso(ax, ay, dot(V, t1), dot(V, t2), ndv);
+  }
+  // pi * f * (N.L) = pi * D * G2 * F / (4 ndv) for light direction w
+  auto specW = [&](V3 w, float ndlw, V3 H, float ndh) {
+    if (!an) {
+      const float G2 = 1.0f / (1.0f + lamV + ggxLambda(a2, ndlw));
+      return kPiF * ggxD(a2, ndh) * G2 / (4.0f * ndv);
+    }
+    const float G2 = 1.0f / (1.0f + lamVA + ggxLambdaAniso(ax, ay, dot(w, t1), dot(w, t2), ndlw));
+    return kPiF * ggxDAniso(ax, ay, dot(H, t1), dot(H, t2), ndh) * G2 / (4.0f * ndv);
+  };
+  uint32_t q0 = px ^ 0x50425244u, q1 = py;  // 'PBRD'
+  for (int li = 0; li < fc.nLights; ++li) {
+    const DevLight& l = fc.lights[li];
+    const V3 L = mk(l.L[0], l.L[1], l.L[2]);
+    const float ndl = dot(N, L);
+    if (ndl <= 0.0f) continue;
+    Hit h;
+    if (!(shOn && fc.shadowSamples > 1 && l.radius > 0.0f)) {
+      // one direction (hard shadow, shadows off, or one sample)
+      float sf = 1.0f;
+      if (shOn) {
+        ++nShadow;
+        sf = trace<kLayout, true, kCount>(s, O, L, eps, __builtin_inff(), h, nv, nt) ? 0.0f : 1.0f;
+      }
+      const V3 Lc = mk(l.color[0] * sf, l.color[1] * sf, l.color[2] * sf);
+      const float dk = kd * ndl * diffAo;
+      out.x += dk * C.x * Lc.x;
+      out.y += dk * C.y * Lc.y;
+      out.z += dk * C.z * Lc.z;
+      if (!l.highlight) continue;  // fill lights: diffuse only
+      if (f0max > 0.0f) {
+        const V3 H = normalize(mk(L.x + V.x, L.y + V.y, L.z + V.z));
+        const float ndh = dot(N, H);
+        if (ndh > 0.0f) {
+          const float sW = specW(L, ndl, H, ndh);
+          const V3 F = schlickF(F0, dot(V, H));
+          out.x += sW * F.x * Lc.x;
+          out.y += sW * F.y * Lc.y;
+          out.z += sW * F.z * Lc.z;
+        }
+      }
+    } else {
+      // area light: visibility x BRDF per cone sample; below-horizon samples
+      // add nothing but stay in the denominator
+      float accD = 0.0f;
+      V3 accS = mk(0.0f, 0.0f, 0.0f);
+      for (int i = 0; i < fc.shadowSamples; ++i) {
+        tea2(q0, q1);
+        const V3 w = sampleLightDir(L, l.tanRadius, u32ToUnorm(q0), u32ToUnorm(q1));
+        const float ndlS = dot(N, w);
+        if (ndlS <= 0.0f) continue;
+        ++nShadow;
+        if (trace<kLayout, true, kCount>(s, O, w, eps, __builtin_inff(), h, nv, nt)) continue;
+        accD += ndlS;
+        if (l.highlight && f0max > 0.0f) {
+          const V3 H = normalize(mk(w.x + V.x, w.y + V.y, w.z + V.z));
+          const float ndh = dot(N, H);
+          if (ndh > 0.0f) {
+            const float sW = specW(w, ndlS, H, ndh);
+            const V3 F = schlickF(F0, dot(V, H));
+            accS.x += sW * F.x;
+            accS.y += sW * F.y;
+            accS.z += sW * F.z;
+          }
+        }
+      }
+      const float inv = 1.0f / static_cast<float>(fc.shadowSamples);
+      const float dk = kd * diffAo * inv * accD;
+      out.x += dk * C.x * l.color[0];
+      out.y += dk * C.y * l.color[1];
+      out.z += dk * C.z * l.color[2];
+      out.x += inv * accS.x * l.color[0];
+      out.y += inv * accS.y * l.color[1];
+      out.z += inv * accS.z * l.color[2];
+    }
+  }
+  // the environment stand-in outside traced reflections: a converted POV
+  // reflection scalar, else F0 x background
+  if (m.reflection > 0.0f) {
+    out.x += m.reflection * fc.bg[0];
+    out.y += m.reflection * fc.bg[1];
+    out.z += m.reflection * fc.bg[2];
+  } else if (f0max > 0.0f) {
+    out.x += F0.x * fc.bg[0];
+    out.y += F0.y * fc.bg[1];
+    out.z += F0.z * fc.bg[2];
+  }
+  return out;
+}
+
+template <bool kCount, int kLayout, bool kPbr>
+__device__ __forceinline__ void resolveHits(const FrameConst& fc, const SceneView& s, const FrameBufs& b, int iter) {
   const int tid = blockIdx.x * blockDim.x + threadIdx.x;  // hit-queue slot
   const int part = tid / b.partCap;
   const bool valid = part < kQueueParts && static_cast<uint32_t>(tid % b.partCap) < queueFill(b.counters, kHitFill, part);
@@ -831,11 +1036,24 @@ __global__ __launch_bounds__(kTraceBlock) void k_resolve(FrameConst fc, SceneVie
     if (flags & kFlagGiRecv) ambL = mk(0.0f, 0.0f, 0.0f);
     else if (flags & kFlagGiToon) ambL = mk(1.0f, 1.0f, 1.0f);
     if (iter == 0 && b.aoOpen) b.aoOpen[pix] = open;
-    // shading.hpp:92-190
-    V3 out = mk(m.emission * C.x + aoF.x * m.ambient * C.x * ambL.x,
-                m.emission * C.y + aoF.y * m.ambient * C.y * ambL.y,
-                m.emission * C.z + aoF.z * m.ambient * C.z * ambL.z);
     const bool shOn = (flags & kFlagShadow) != 0u;
+    V3 out;
+    if (kPbr && m.principled) {
+      // shading.hpp:88-91: principled materials take their own evaluator
+      V3 t1 = mk(0.0f, 0.0f, 0.0f);
+      bool aniso = false;
+      if (b.hitPrim && m.pbrAnisotropy != 0.0f) {
+        const uint32_t bits = b.hitPrim[tid], kind = bits >> kKindShift, id = bits & kIdMask;
+        const uint32_t info = kind == 0u ? 0u : kind == 1u ? s.sphInfo[id] : kind == 3u ? s.coneInfo[id] : s.rndInfo[id];
+        if (kind != 0u && !(info & kInfoFromEdge)) aniso = principledTangent(s, m, bits, N, t1);
+      }
+      out = shadePrincipled<kCount, kLayout>(fc, s, m, C, N, V, ambL, aoF, diffAo, O, eps, shOn, px, py, aniso, t1,
+                                             nShadow, &nv, &nt);
+    } else {
+    // shading.hpp:92-190
+    out = mk(m.emission * C.x + aoF.x * m.ambient * C.x * ambL.x,
+             m.emission * C.y + aoF.y * m.ambient * C.y * ambL.y,
+             m.emission * C.z + aoF.z * m.ambient * C.z * ambL.z);
     uint32_t s0 = px, s1 = py;
     for (int li = 0; li < fc.nLights; ++li) {
       const DevLight& l = fc.lights[li];
@@ -909,6 +1127,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_resolve(FrameConst fc, SceneVie
       out.y += m.reflection * fc.bg[1];
       out.z += m.reflection * fc.bg[2];
     }
+    }
     // transparency.hpp:431-467 (opaque floor / over / step past the surface)
     if (iter > 0) acc = b.acc[pix];
     const float opacity = C4.w;
@@ -951,6 +1170,15 @@ __global__ __launch_bounds__(kTraceBlock) void k_resolve(FrameConst fc, SceneVie
   b.activeOut[slot] = pix;
   b.acc[pix] = acc;
   b.tnear[pix] = tnext;
+}
+
+template <bool kCount, int kLayout>
+__global__ __launch_bounds__(kTraceBlock) void k_resolve(FrameConst fc, SceneView s, FrameBufs b, int iter) {
+  resolveHits<kCount, kLayout, false>(fc, s, b, iter);
+}
+template <bool kCount, int kLayout>
+__global__ __launch_bounds__(kTraceBlock) void k_resolve_pbr(FrameConst fc, SceneView s, FrameBufs b, int iter) {
+  resolveHits<kCount, kLayout, true>(fc, s, b, iter);
 }
 
 // image_ops.cpp:9-46 -- box resolve (non-finite samples contribute 0) and
@@ -1268,7 +1496,7 @@ __global__ void k_gi_list(GiField f, const uint8_t* __restrict__ mask, int zeroO
 // tracePt1GBuffer (pt1_gbuffer.hpp:86-200): one camera ray per pixel centre
 // of the reduced grid (the render camera's plane, no clip range), mesh hits
 // and real primitives gather-eligible (toon finishes included), normals
-// face-forwarded against the ray, albedo = diffuse * pigment.
+// face-forwarded against the ray, albedo = diffuse weight * pigment.
 template <int kLayout>
 __global__ __launch_bounds__(kTraceBlock) void k_gi_gbuf(FrameConst fc, SceneView s, GiField f, float4* albedo,
                                                          unsigned long long* counters) {
@@ -1301,7 +1529,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_gi_gbuf(FrameConst fc, SceneVie
     N = safeNormalize(mk(w0 * n0.x + h.u * n1.x + h.v * n2.x, w0 * n0.y + h.u * n1.y + h.v * n2.y,
                          w0 * n0.z + h.u * n1.z + h.v * n2.z));
     Ng = safeNormalize(h.ng);
-    const float kd = s.meshMats[tri.w & kInfoMatMask].diffuse;
+    const float kd = diffuseWeight(s.meshMats[tri.w & kInfoMatMask]);
     A = mk(kd * (w0 * c0.x + h.u * c1.x + h.v * c2.x), kd * (w0 * c0.y + h.u * c1.y + h.v * c2.y),
            kd * (w0 * c0.z + h.u * c1.z + h.v * c2.z));
   } else if (hit) {
@@ -1320,7 +1548,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_gi_gbuf(FrameConst fc, SceneVie
     ok = (info & kInfoFromEdge) == 0u;
     N = safeNormalize(h.ng);
     Ng = N;
-    const float kd = s.primMats[info & kInfoMatMask].diffuse;
+    const float kd = diffuseWeight(s.primMats[info & kInfoMatMask]);
     A = mk(kd * col.x, kd * col.y, kd * col.z);
   }
   if (!ok) {
@@ -1729,7 +1957,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_gi_gather(FrameConst fc, SceneV
       Cy = mk(w0 * c0.x + h.u * c1.x + h.v * c2.x, w0 * c0.y + h.u * c1.y + h.v * c2.y,
               w0 * c0.z + h.u * c1.z + h.v * c2.z);
       const DevMaterial& mm = s.meshMats[tri.w & kInfoMatMask];
-      kd = mm.diffuse;
+      kd = diffuseWeight(mm);
       if (fc.giPt2 && fc.giEmissive) em = mm.emission;
       NgS = h.ng;
     } else {
@@ -1749,7 +1977,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_gi_gather(FrameConst fc, SceneV
       Ny = safeNormalize(h.ng);
       Cy = mk(col.x, col.y, col.z);
       const DevMaterial& pm = s.primMats[info & kInfoMatMask];
-      kd = pm.diffuse;
+      kd = diffuseWeight(pm);
       if (fc.giPt2 && fc.giEmissive) em = pm.emission;
       NgS = Ny;
     }
@@ -2159,7 +2387,12 @@ void launchAo(const FrameConst& fc, const SceneView& s, const FrameBufs& b, bool
 void launchResolve(const FrameConst& fc, const SceneView& s, const FrameBufs& b, int iter, bool countVisits,
                    hipStream_t st) {
   const int64_t slots = static_cast<int64_t>(kQueueParts) * b.partCap;
-  UBG_DISPATCH(k_resolve, blocksFor(slots, kTraceBlock), countVisits, fc, s, b, iter, 0);
+  // the principled evaluator is compiled into a kernel of its own, used only
+  // for scenes that have a principled material
+  if (b.principled)
+    UBG_DISPATCH(k_resolve_pbr, blocksFor(slots, kTraceBlock), countVisits, fc, s, b, iter);
+  else
+    UBG_DISPATCH(k_resolve, blocksFor(slots, kTraceBlock), countVisits, fc, s, b, iter);
 }
 void launchGiList(const GiField& f, const uint8_t* mask, bool zeroOthers, const FrameBufs& b, hipStream_t st) {
   // (the same block -> partition map as a first primary launch over W x H:

@@ -48,7 +48,19 @@ struct DevMaterial {
   uint32_t metallic;
   float roughness;
   uint32_t toon;  // Material::toonLike() (scene.hpp:190-196)
+  // ShadingModel::Principled (scene.hpp:127-171): Material::Pbr, with the
+  // anisotropy rotation's cosine and sine (angle = turns * 2 pi) taken on the
+  // host once per material
+  uint32_t principled;
+  float pbrMetallic, pbrRoughness, pbrSpecular, pbrAnisotropy, pbrRotation;
+  float rotCos, rotSin;
 };
+
+// Material::diffuseWeight (scene.hpp:171-174): the diffuse-lobe weight every
+// diffuse consumer reads (GI reflectance, gather albedo)
+__host__ __device__ inline float diffuseWeight(const DevMaterial& m) {
+  return m.principled ? m.diffuse * (1.0f - m.pbrMetallic) : m.diffuse;
+}
 
 // One BVH8 node as the builders write it (80 bytes; the traversal reads it
 // as 5 x uint4): box origin, per-axis scale exponents (scale = 2^(e - 127)),
@@ -97,6 +109,11 @@ struct SceneView {
   const uint16_t* coneGroup;
   const uint16_t* rndGroup;
   const uint16_t* triGroup;
+  // unit p0 -> p1 axis per cone / round segment (float4), the anisotropy
+  // tangent of principled cylinders; null unless an anisotropic principled
+  // cylinder exists
+  const void* coneAxis;
+  const void* rndAxis;
 };
 
 struct DevLight {

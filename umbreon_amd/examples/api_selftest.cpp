@@ -139,6 +139,24 @@ int main() {
     }
     check("default GI renders", ok);
   }
+  {
+    // a principled material with metallic 0 and specular 0 is the POV
+    // diffuse-only finish bit for bit (test_principled.cpp:165-199); a metal
+    // is not
+    umbreon::Scene s;
+    s.mesh = quad({0.5f, 0.6f, 0.7f, 1.0f}, 0.0f);
+    s.camera = orthoCam();
+    s.lights.push_back(key());
+    s.background = {0.4f, 0.1f, 0.2f};
+    const umbreon::FrameResult pov = umbreon::render(s, o);
+    s.mesh.material.model = umbreon::ShadingModel::Principled;
+    s.mesh.material.pbr.metallic = 0.0f;
+    s.mesh.material.pbr.specular = 0.0f;
+    check("principled matte == POV diffuse", umbreon::render(s, o).color == pov.color);
+    s.mesh.material.pbr.metallic = 1.0f;
+    s.mesh.material.pbr.specular = 0.5f;
+    check("principled metal differs", umbreon::render(s, o).color != pov.color);
+  }
   std::printf("%d failure(s)\n", failures);
   return failures == 0 ? 0 : 1;
 }

@@ -22,7 +22,8 @@ Vec3 = Tuple[float, float, float]
 
 @dataclass
 class Material:
-    """umbreon::Material (scene.hpp:130-214); POV finish fields."""
+    """umbreon::Material (scene.hpp:130-214): the POV finish fields, the
+    shading model and the principled subset Material::Pbr (pbr_*)."""
     ambient: float = 0.2
     diffuse: float = 0.8
     specular: float = 0.0
@@ -34,6 +35,11 @@ class Material:
     reflection: float = 0.0
     emission: float = 0.0
     model: int = 0  # 0 Pov, 1 Principled, 2 Toon
+    pbr_metallic: float = 0.0
+    pbr_roughness: float = 0.5
+    pbr_specular: float = 0.5
+    pbr_anisotropy: float = 0.0
+    pbr_anisotropy_rotation: float = 0.0  # turns
 
     @staticmethod
     def flat_outline() -> "Material":
@@ -47,7 +53,8 @@ class Material:
         m.roughness, m.brilliance, m.phong = self.roughness, self.brilliance, self.phong
         m.phong_size, m.metallic = self.phong_size, int(bool(self.metallic))
         m.reflection, m.emission, m.model = self.reflection, self.emission, int(self.model)
-        m.pbr_metallic, m.pbr_roughness, m.pbr_specular = 0.0, 0.5, 0.5
+        m.pbr_metallic, m.pbr_roughness, m.pbr_specular = self.pbr_metallic, self.pbr_roughness, self.pbr_specular
+        m.pbr_anisotropy, m.pbr_anisotropy_rotation = self.pbr_anisotropy, self.pbr_anisotropy_rotation
         return m
 
 
