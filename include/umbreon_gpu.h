@@ -40,7 +40,7 @@
 extern "C" {
 #endif
 
-#define UBG_ABI_VERSION 5
+#define UBG_ABI_VERSION 6
 
 /* Status codes. */
 enum {
@@ -421,6 +421,53 @@ int ubg_occluded_segments(ubg_context* ctx, const ubg_scene* scene, int64_t n_se
                           const ubg_segment* segments, const int32_t* exclude_faces, int64_t n_exclude_faces,
                           float eps, float graze_cos_eps, float coplanar_eps, uint8_t* occluded,
                           int32_t memory);
+
+/* ObjectSpaceEdgeOptions (render/edge_types.hpp:311-409) without `enable`
+ * (calling the entry point is the enable). */
+typedef struct ubg_edge_options {
+  float width;                        /* edge cylinder radius (default 0.03) */
+  float raise;                        /* outward contour offset (0) */
+  int32_t segments;                   /* ring tessellation (48, at least 3) */
+  float color[3];                     /* linear rgb (0, 0, 0) */
+  int32_t clip;                       /* union-boundary clip of the analytic silhouettes (1) */
+  int32_t mesh_silhouette;            /* smooth n.v == 0 contour + hard-edge straddle (1) */
+  int32_t mesh_crease;                /* dihedral creases (1) */
+  int32_t mesh_border;                /* open boundary edges (1) */
+  int32_t visibility_clip;            /* mesh hidden-line clip: must be 0 (UBG_ERR_UNSUPPORTED) */
+  float mesh_hard_edge_deg;           /* 40 */
+  float crease_angle_deg;             /* 30 */
+  float mesh_crease_smooth_veto_deg;  /* 0 = off */
+  int32_t mesh_crease_convex_only;    /* 0 */
+  float mesh_border_coplanar_veto_deg;/* 0 = off */
+  int32_t mesh_crease_max_degree;     /* 0 = off */
+} ubg_edge_options;
+
+/* The generated edges: n open cylinders of radius max(width, 0) with the
+ * flat outline material, in emission order (analytic sphere rings, cylinder
+ * side lines / cap rims, then the mesh silhouette, crease and border edges).
+ * Arrays are owned by the library until ubg_edges_free. */
+typedef struct ubg_edges {
+  int64_t n;
+  float* p0;          /* 3 * n */
+  float* p1;          /* 3 * n */
+  float* color;       /* 4 * n: rgb + opacity at p0 */
+  float* opacity1;    /* n: opacity at p1, -1 = uniform */
+  uint16_t* group;    /* n: the source primitive's / triangle's section */
+  float radius;
+} ubg_edges;
+
+/* generateObjectSpaceEdges (edges/object_space_edges.cpp:172-264) for the
+ * scene's spheres, cylinders and mesh seen by `camera`: the n.v == 0
+ * silhouettes of the analytic primitives (fromEdgeMacro ones skipped), each
+ * chord split into its runs outside every other solid when `clip` is set
+ * (the point-in-solid tests run on the device), then the mesh's feature
+ * edges. The caller appends them to Scene::cylinders (open, flat outline,
+ * not fromEdgeMacro) before building the scene. The instance offsets are
+ * ignored, as in the reference. ctx may be NULL when no clip runs (clip = 0,
+ * or no sphere or cylinder silhouettes): the rest is host work. */
+int ubg_object_space_edges(ubg_context* ctx, const ubg_scene_desc* desc, const ubg_camera* camera,
+                           const ubg_edge_options* options, ubg_edges* out);
+void ubg_edges_free(ubg_edges* edges);
 
 #ifdef __cplusplus
 }

@@ -23,6 +23,8 @@
 #include <atomic>
 #include <chrono>
 #include <cmath>
+#include <type_traits>
+#include <cstdlib>
 #include <cstdint>
 #include <cstring>
 #include <functional>
@@ -3032,6 +3034,192 @@ static int occludedSegments(const ubg_scene_desc* d, int64_t n, const ubg_segmen
 
 }  // namespace orc
 
+namespace orc {
+// ---------------------------------------------------------------------------
+// Object-space silhouette edges of the analytic primitives
+// (edges/object_space_edges.cpp:172-229, edges/analytic_silhouette.cpp:31-223)
+// with the union-boundary clip tested against every solid (the reference's
+// own brute force). Mesh feature edges are not restated here.
+
+struct EdgeAcc {
+  std::vector<float> p0, p1, color, opacity1;
+  std::vector<uint16_t> group;
+  void add(V3<float> a, V3<float> b, const ubg_edge_options& o, uint16_t g) {
+    p0.insert(p0.end(), {a.x, a.y, a.z});
+    p1.insert(p1.end(), {b.x, b.y, b.z});
+    color.insert(color.end(), {o.color[0], o.color[1], o.color[2], 1.0f});
+    opacity1.push_back(-1.0f);  // both ends opaque
+    group.push_back(g);
+  }
+};
+
+struct SilLoop {
+  std::vector<V3<float>> pts;
+  bool ring = false;
+  uint16_t group = 0;
+  int sph = -1, cyl = -1;
+};
+
+static void silhouetteLoops(const ubg_scene_desc* d, const ubg_camera& c, int N, float raise,
+                            std::vector<SilLoop>& loops) {
+  using F = V3<float>;
+  const F eye{c.position[0], c.position[1], c.position[2]};
+  const F view{c.direction[0], c.direction[1], c.direction[2]};
+  const bool ortho = c.orthographic != 0;
+  const float pi = 3.14159265358979323846f;
+  auto ringPoints = [&](F center, Frame<float> fr, float rad, int n, std::vector<F>& pts) {
+    pts.clear();
+    for (int i = 0; i < n; ++i) {
+      const float th = 2.0f * pi * static_cast<float>(i) / static_cast<float>(n);
+      pts.push_back(center + (fr.t * std::cos(th) + fr.b * std::sin(th)) * rad);
+    }
+  };
+  for (int64_t i = 0; i < d->n_spheres; ++i) {
+    if (d->sphere_from_edge && d->sphere_from_edge[i]) continue;
+    const F O{d->sphere_center_radius[4 * i], d->sphere_center_radius[4 * i + 1], d->sphere_center_radius[4 * i + 2]};
+    const float r = d->sphere_center_radius[4 * i + 3];
+    if (r <= 0.0f) continue;
+    F axis, Q;
+    float rho;
+    if (ortho) {
+      axis = normalize(view);
+      Q = O;
+      rho = r;
+    } else {
+      const F dd = O - eye;
+      const float L = length(dd);
+      if (L <= r * 1.0001f) continue;
+      axis = dd * (1.0f / L);
+      Q = O - axis * (r * r / L);
+      const float disc = L * L - r * r;
+      rho = r * std::sqrt(disc > 0.0f ? disc : 0.0f) / L;
+    }
+    if (rho <= 0.0f) continue;
+    SilLoop l;
+    ringPoints(Q, frameFromNormal(axis), rho, N, l.pts);
+    for (F& P : l.pts) P = P + normalize(P - O) * raise;
+    l.ring = true;
+    l.group = d->sphere_group ? d->sphere_group[i] : 0;
+    l.sph = static_cast<int>(i);
+    loops.push_back(l);
+  }
+  // the two grazing contacts on the cross-section circle at P
+  auto contacts = [&](F P, F u, float r, F& dp, F& dm) {
+    if (ortho) {
+      F eT = cross(u, normalize(view) * -1.0f);
+      const float l = length(eT);
+      if (l < 1.0e-4f) return false;
+      eT = eT * (1.0f / l);
+      dp = eT;
+      dm = eT * -1.0f;
+      return true;
+    }
+    const F cp = eye - P;
+    const F perp = cp - u * dot(u, cp);
+    const float Lp = length(perp);
+    if (Lp <= r * 1.0001f) return false;
+    const F eR = perp * (1.0f / Lp);
+    F eT = cross(u, eR);
+    const float lt = length(eT);
+    if (lt < 1.0e-4f) return false;
+    eT = eT * (1.0f / lt);
+    const float ca = r / Lp, sa = std::sqrt(std::fmax(0.0f, 1.0f - ca * ca));
+    dp = eR * ca + eT * sa;
+    dm = eR * ca - eT * sa;
+    return true;
+  };
+  for (int64_t i = 0; i < d->n_cylinders; ++i) {
+    if (d->cyl_from_edge && d->cyl_from_edge[i]) continue;
+    const F A{d->cyl_p0[3 * i], d->cyl_p0[3 * i + 1], d->cyl_p0[3 * i + 2]};
+    const F B{d->cyl_p1[3 * i], d->cyl_p1[3 * i + 1], d->cyl_p1[3 * i + 2]};
+    const float r = d->cyl_radius[i];
+    if (r <= 0.0f) continue;
+    const float axLen = length(B - A);
+    if (axLen <= 0.0f) continue;
+    const F u = (B - A) * (1.0f / axLen);
+    const uint16_t g = d->cyl_group ? d->cyl_group[i] : 0;
+    F ap, am, bp, bm;
+    if (contacts(A, u, r, ap, am) && contacts(B, u, r, bp, bm)) {
+      const float off = r + raise;
+      SilLoop l1, l2;
+      l1.pts = {A + ap * off, B + bp * off};
+      l2.pts = {A + am * off, B + bm * off};
+      l1.group = l2.group = g;
+      l1.cyl = l2.cyl = static_cast<int>(i);
+      loops.push_back(l1);
+      loops.push_back(l2);
+    } else {
+      // end-on: the rim of the nearer cap
+      const F dir = normalize(view);
+      const F cap = ortho ? (dot(A, dir) < dot(B, dir) ? A : B) : (length(eye - A) < length(eye - B) ? A : B);
+      SilLoop l;
+      ringPoints(cap, frameFromNormal(normalize(u)), r + raise, N, l.pts);
+      l.ring = true;
+      l.group = g;
+      l.cyl = static_cast<int>(i);
+      loops.push_back(l);
+    }
+  }
+}
+
+// strictly inside a solid other than the chord's own source
+static bool insideOtherSolid(const ubg_scene_desc* d, V3<float> X, int skipSph, int skipCyl, float ringRs) {
+  using F = V3<float>;
+  for (int64_t i = 0; i < d->n_spheres; ++i) {
+    if (static_cast<int>(i) == skipSph) continue;
+    const float* s = &d->sphere_center_radius[4 * i];
+    if (length(X - F{s[0], s[1], s[2]}) < s[3]) return true;
+  }
+  for (int64_t j = 0; j < d->n_cylinders; ++j) {
+    if (static_cast<int>(j) == skipCyl) continue;
+    const float r = d->cyl_radius[j];
+    if (ringRs > 0.0f && r < ringRs - 1.0e-4f) continue;  // a thinner bond keeps the ring
+    const F p0{d->cyl_p0[3 * j], d->cyl_p0[3 * j + 1], d->cyl_p0[3 * j + 2]};
+    const F ab = F{d->cyl_p1[3 * j], d->cyl_p1[3 * j + 1], d->cyl_p1[3 * j + 2]} - p0;
+    const float L = length(ab);
+    if (L <= 0.0f) continue;
+    const F u = ab * (1.0f / L);
+    const float t = dot(X - p0, u);
+    if (t < 0.0f || t > L) continue;
+    if (length((X - p0) - u * t) < r) return true;
+  }
+  return false;
+}
+
+static void objectSpaceEdges(const ubg_scene_desc* d, const ubg_camera& cam, const ubg_edge_options& o, EdgeAcc& out) {
+  std::vector<SilLoop> loops;
+  silhouetteLoops(d, cam, o.segments < 3 ? 3 : o.segments, o.raise, loops);
+  for (const SilLoop& l : loops) {
+    const size_t n = l.pts.size();
+    if (n < 2) continue;
+    for (size_t i = 0; i < (l.ring ? n : n - 1); ++i) {
+      const V3<float> a = l.pts[i], b = l.pts[(i + 1) % n];
+      if (!o.clip) {
+        out.add(a, b, o, l.group);
+        continue;
+      }
+      const V3<float> dd = b - a;
+      const float len = length(dd);
+      if (len <= 0.0f) continue;
+      const float step = o.width > 0.0f ? o.width : len;
+      const int K = std::max(2, static_cast<int>(std::ceil(len / step)));
+      auto at = [&](int k) { return a + dd * (static_cast<float>(k) / static_cast<float>(K)); };
+      const float ringRs = l.sph >= 0 ? d->sphere_center_radius[4 * l.sph + 3] : -1.0f;
+      int start = -1;
+      for (int k = 0; k <= K; ++k) {
+        const bool in = insideOtherSolid(d, at(k), l.sph, l.cyl, ringRs);
+        if (!in && start < 0) start = k;
+        if (in && start >= 0) {
+          if (k - 1 > start) out.add(at(start), at(k - 1), o, l.group);
+          start = -1;
+        }
+      }
+      if (start >= 0 && K > start) out.add(at(start), at(K), o, l.group);
+    }
+  }
+}
+}  // namespace orc
+
 extern "C" {
 
 int orc_occluded_segments(const ubg_scene_desc* desc, int64_t n, const ubg_segment* segs, const int32_t* excl,
@@ -3079,6 +3267,40 @@ int64_t orc_chain_curves(const ubg_scene_desc* desc, int64_t cap, int32_t* src, 
 }
 
 void orc_tea2(uint32_t* v0, uint32_t* v1) { orc::tea2(*v0, *v1); }
+
+int orc_object_space_edges(const ubg_scene_desc* desc, const ubg_camera* cam, const ubg_edge_options* opt,
+                           ubg_edges* out) {
+  if (!desc || !cam || !opt || !out) return UBG_ERR_INVALID_ARGUMENT;
+  orc::EdgeAcc acc;
+  orc::objectSpaceEdges(desc, *cam, *opt, acc);
+  const int64_t n = static_cast<int64_t>(acc.group.size());
+  *out = ubg_edges{};
+  out->n = n;
+  out->radius = opt->width > 0.0f ? opt->width : 0.0f;
+  if (n == 0) return UBG_OK;
+  auto dup = [](const auto& v) {
+    using T = typename std::decay_t<decltype(v)>::value_type;
+    T* p = static_cast<T*>(std::malloc(v.size() * sizeof(T)));
+    std::memcpy(p, v.data(), v.size() * sizeof(T));
+    return p;
+  };
+  out->p0 = dup(acc.p0);
+  out->p1 = dup(acc.p1);
+  out->color = dup(acc.color);
+  out->opacity1 = dup(acc.opacity1);
+  out->group = dup(acc.group);
+  return UBG_OK;
+}
+
+void orc_edges_free(ubg_edges* e) {
+  if (!e) return;
+  std::free(e->p0);
+  std::free(e->p1);
+  std::free(e->color);
+  std::free(e->opacity1);
+  std::free(e->group);
+  *e = ubg_edges{};
+}
 
 void orc_pt1_denoise(int32_t w, int32_t h, float* E, const float* normal, const float* position, int32_t iters,
                      float sigma_z, float sigma_n, float sigma_l) {

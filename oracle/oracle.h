@@ -86,6 +86,13 @@ void orc_pt2_pixel_sampler(int32_t pattern, uint32_t px, uint32_t py, uint32_t f
 /* Reference TEA-8 (secondary_rays.hpp:51-58), exposed for known-answer tests. */
 void orc_tea2(uint32_t* v0, uint32_t* v1);
 
+/* Object-space silhouette edges of the spheres and cylinders
+ * (generateObjectSpaceEdges without the mesh feature edges), the union clip
+ * tested against every solid. Free with orc_edges_free. */
+int orc_object_space_edges(const ubg_scene_desc* desc, const ubg_camera* cam, const ubg_edge_options* opt,
+                           ubg_edges* out);
+void orc_edges_free(ubg_edges* e);
+
 /* Post-process helpers (fog.cpp:7-36, image_ops.cpp:9-79). */
 float orc_fog_factor(const ubg_fog* fog, float z);
 void orc_box_downsample(const float* src, int w, int h, int channels, int ss,

@@ -23,7 +23,7 @@ UBG_ERR_UNSUPPORTED = 5
 UBG_ERR_INTERNAL = 6
 
 # UBG_ABI_VERSION of include/umbreon_gpu.h this mirror is written against
-ABI_VERSION = 5
+ABI_VERSION = 6
 
 BVH2 = 0
 BVH8 = 1
@@ -160,6 +160,27 @@ class SceneInfo(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class EdgeOptions(C.Structure):
+    """ubg_edge_options (ObjectSpaceEdgeOptions without `enable`)."""
+    _fields_ = [
+        ("width", C.c_float), ("raise_", C.c_float), ("segments", C.c_int32), ("color", C.c_float * 3),
+        ("clip", C.c_int32), ("mesh_silhouette", C.c_int32), ("mesh_crease", C.c_int32),
+        ("mesh_border", C.c_int32), ("visibility_clip", C.c_int32), ("mesh_hard_edge_deg", C.c_float),
+        ("crease_angle_deg", C.c_float), ("mesh_crease_smooth_veto_deg", C.c_float),
+        ("mesh_crease_convex_only", C.c_int32), ("mesh_border_coplanar_veto_deg", C.c_float),
+        ("mesh_crease_max_degree", C.c_int32),
+    ]
+
+
+class Edges(C.Structure):
+    """ubg_edges (library-owned arrays)."""
+    _fields_ = [
+        ("n", C.c_int64), ("p0", C.POINTER(C.c_float)), ("p1", C.POINTER(C.c_float)),
+        ("color", C.POINTER(C.c_float)), ("opacity1", C.POINTER(C.c_float)), ("group", C.POINTER(C.c_uint16)),
+        ("radius", C.c_float),
+    ]
+
+
 class Ray(C.Structure):
     _fields_ = [("org", C.c_float * 3), ("tnear", C.c_float), ("dir", C.c_float * 3),
                 ("tfar", C.c_float)]
@@ -245,6 +266,11 @@ def load() -> C.CDLL:
     lib.ubg_occluded_segments.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_int64,
                                           C.c_float, C.c_float, C.c_float, C.c_void_p, C.c_int32]
     lib.ubg_occluded_segments.restype = C.c_int
+    lib.ubg_object_space_edges.argtypes = [C.c_void_p, C.POINTER(SceneDesc), C.POINTER(Camera),
+                                           C.POINTER(EdgeOptions), C.POINTER(Edges)]
+    lib.ubg_object_space_edges.restype = C.c_int
+    lib.ubg_edges_free.argtypes = [C.POINTER(Edges)]
+    lib.ubg_edges_free.restype = None
     if lib.ubg_abi_version() != ABI_VERSION:
         raise RuntimeError("libumbreon_gpu.so ABI version mismatch")
     _lib = lib

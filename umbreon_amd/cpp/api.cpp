@@ -204,12 +204,73 @@ void checkSupported(const Scene& s, const RenderOptions& o) {
   if (o.denoiser != 0) no("the final-colour denoiser");
   if (o.envLights > 0) no("environment dome lights (envLights)");
   if (o.strokeEdges.enable) no("stroke edges");
-  if (o.objectSpaceEdges.enable) no("object-space edges");
   if (s.lights.size() > 16) no("more than 16 lights");
 }
 
+ubg_camera toCamera(const Camera& c) {
+  ubg_camera k{};
+  const float cp[3] = {c.position.x, c.position.y, c.position.z};
+  const float cd[3] = {c.direction.x, c.direction.y, c.direction.z};
+  const float cu[3] = {c.up.x, c.up.y, c.up.z};
+  std::memcpy(k.position, cp, sizeof(cp));
+  std::memcpy(k.direction, cd, sizeof(cd));
+  std::memcpy(k.up, cu, sizeof(cu));
+  k.fovy = c.fovy;
+  k.orthographic = c.orthographic ? 1 : 0;
+  k.height = c.height;
+  return k;
+}
+
+// generateObjectSpaceEdges (object_space_edges.cpp:172-264): the scene with
+// the generated edges appended to its cylinders (open, flat outline
+// material, not fromEdgeMacro), the caller's scene untouched.
+Scene withObjectSpaceEdges(const Scene& scene, const ObjectSpaceEdgeOptions& o) {
+  FlatScene flat(scene);
+  const ubg_camera cam = toCamera(scene.camera);
+  ubg_edge_options eo{};
+  eo.width = o.width;
+  eo.raise = o.raise;
+  eo.segments = o.segments;
+  for (int k = 0; k < 3; ++k) eo.color[k] = o.color[k];
+  eo.clip = o.clip ? 1 : 0;
+  eo.mesh_silhouette = o.meshSilhouette ? 1 : 0;
+  eo.mesh_crease = o.meshCrease ? 1 : 0;
+  eo.mesh_border = o.meshBorder ? 1 : 0;
+  eo.visibility_clip = o.visibilityClip ? 1 : 0;
+  eo.mesh_hard_edge_deg = o.meshHardEdgeDeg;
+  eo.crease_angle_deg = o.creaseAngleDeg;
+  eo.mesh_crease_smooth_veto_deg = o.meshCreaseSmoothVetoDeg;
+  eo.mesh_crease_convex_only = o.meshCreaseConvexOnly ? 1 : 0;
+  eo.mesh_border_coplanar_veto_deg = o.meshBorderCoplanarVetoDeg;
+  eo.mesh_crease_max_degree = o.meshCreaseMaxDegree;
+  ubg_edges e{};
+  const int st = ubg_object_space_edges(device(), &flat.desc, &cam, &eo, &e);
+  if (st != UBG_OK) throwStatus(st, "object-space edges");
+  Scene out = scene;
+  out.cylinders.reserve(out.cylinders.size() + static_cast<std::size_t>(e.n));
+  for (int64_t i = 0; i < e.n; ++i) {
+    Cylinder c;
+    c.p0 = Vec3{e.p0[3 * i], e.p0[3 * i + 1], e.p0[3 * i + 2]};
+    c.p1 = Vec3{e.p1[3 * i], e.p1[3 * i + 1], e.p1[3 * i + 2]};
+    c.radius = e.radius;
+    c.color = Vec4{e.color[4 * i], e.color[4 * i + 1], e.color[4 * i + 2], e.color[4 * i + 3]};
+    c.material = Material::flatOutline();
+    c.group = e.group[i];
+    c.opacity1 = e.opacity1[i];
+    c.open = true;
+    c.fromEdgeMacro = false;
+    out.cylinders.push_back(c);
+  }
+  ubg_edges_free(&e);
+  return out;
+}
+
 // renderFrame (pipeline.cpp:20-256) for one pass on the GPU.
-FrameResult renderPass(const Scene& scene, const RenderOptions& opt, RenderProgress* progress) {
+FrameResult renderPass(const Scene& sceneIn, const RenderOptions& opt, RenderProgress* progress) {
+  // pipeline.cpp:30-42: object-space edges on a private copy of the pass's scene
+  Scene edged;
+  if (opt.objectSpaceEdges.enable) edged = withObjectSpaceEdges(sceneIn, opt.objectSpaceEdges);
+  const Scene& scene = opt.objectSpaceEdges.enable ? edged : sceneIn;
   ubg_context* ctx = device();
   const int ss = std::max(1, opt.supersample);
   if (progress) progress->beginPhase(RenderPhase::Setup);

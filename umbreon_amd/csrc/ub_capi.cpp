@@ -14,6 +14,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <limits>
 #include <mutex>
@@ -23,6 +24,7 @@
 #include "../../include/umbreon_gpu.h"
 #include "ub_build.h"
 #include "ub_devbuild.h"
+#include "ub_edges.h"
 #include "ub_kernels.h"
 #include "ub_layout.h"
 
@@ -69,13 +71,15 @@ struct ubg_context {
   // pt1 GI: first-hit position / normal / reflectance, gathered E, receiver
   // list, staging of the indirect and occlusion AOVs
   DevBuf giPos, giNrm, giRefl, giE, giList, giInd, giOcc, dnA, dnB, dnGrad, dnVar, giLow, giPatch, giSnap;
+  // object-space edge clip: points + sources, solids + grid, results
+  DevBuf edgePts, edgeSolids, edgeOut;
   unsigned long long* hostCounters = nullptr;  // pinned
   std::vector<DevBuf*> all() {
     return {&hitPix, &hitT,   &hitO,    &hitN,  &hitC,  &hitAo,   &activeA,  &activeB,  &acc,     &tnear,
             &hiColor, &depth, &viewZ,   &hitId, &aoOpen, &outColor, &counters, &query, &queryOut, &aoT,
             &hitOpen, &hitBent, &cGuide, &cOpen, &cBent, &hitOut, &aaSetC, &aaSetR, &aaSetF, &aaFirstN, &hitPrim,
             &aaPix, &giPos, &giNrm, &giRefl, &giE, &giList, &giInd, &giOcc, &dnA, &dnB, &dnGrad,
-            &dnVar, &giLow, &giPatch, &giSnap};
+            &dnVar, &giLow, &giPatch, &giSnap, &edgePts, &edgeSolids, &edgeOut};
   }
 };
 
@@ -1107,6 +1111,148 @@ int ubg_occluded_segments(ubg_context* ctx, const ubg_scene* scene, int64_t n, c
             "hipMemcpy(segment result)");
   UBG_HIP(hipStreamSynchronize(st), "segment query sync");
   return UBG_OK;
+}
+
+int ubg_object_space_edges(ubg_context* ctx, const ubg_scene_desc* d, const ubg_camera* cam,
+                           const ubg_edge_options* o, ubg_edges* out) {
+  if (!d || !cam || !o || !out) return fail(UBG_ERR_INVALID_ARGUMENT, "ubg_object_space_edges: bad argument");
+  *out = ubg_edges{};
+  if ((d->n_spheres > 0 && !d->sphere_center_radius) ||
+      (d->n_cylinders > 0 && (!d->cyl_p0 || !d->cyl_p1 || !d->cyl_radius)) ||
+      (d->n_vertices > 0 && !d->positions) || (d->n_triangles > 0 && d->n_vertices < 3) || d->n_spheres < 0 ||
+      d->n_cylinders < 0 || d->n_triangles < 0)
+    return fail(UBG_ERR_INVALID_ARGUMENT, "ubg_object_space_edges: bad scene arrays");
+  if (d->index)
+    for (int64_t c = 0; c < 3 * d->n_triangles; ++c)
+      if (d->index[c] >= static_cast<uint64_t>(d->n_vertices))
+        return fail(UBG_ERR_INVALID_ARGUMENT, "ubg_object_space_edges: mesh index out of range");
+  if (!d->index && 3 * d->n_triangles > d->n_vertices)
+    return fail(UBG_ERR_INVALID_ARGUMENT, "ubg_object_space_edges: more corners than vertices");
+  const bool meshEdges = d->n_vertices >= 3 && d->n_triangles > 0 &&
+                         (o->mesh_silhouette || o->mesh_crease || o->mesh_border);
+  if (o->visibility_clip && meshEdges)
+    return fail(UBG_ERR_UNSUPPORTED, "object-space edges: the mesh hidden-line clip (visibilityClip) is not implemented");
+  std::vector<ubg::EdgeChord> chords;
+  ubg::analyticChords(d, *cam, o->segments, o->raise, chords);
+  ubg::EdgeList edges;
+  if (!o->clip) {
+    for (const ubg::EdgeChord& c : chords) edges.push(c.a, c.b, *o, c.group);
+  } else if (!chords.empty()) {
+    // clipAndEmit (object_space_edges.cpp:151-182): the chords' sample points
+    // go to the device for the point-in-solid tests, the runs outside every
+    // other solid come back as edges
+    std::vector<int64_t> first(chords.size() + 1, 0);
+    std::vector<int> K(chords.size());
+    for (size_t i = 0; i < chords.size(); ++i) {
+      K[i] = ubg::chordSamples(chords[i], o->width);
+      first[i + 1] = first[i] + (K[i] > 0 ? K[i] + 1 : 0);
+    }
+    const int64_t nPts = first.back();
+    std::vector<float4> pts(static_cast<size_t>(nPts));
+    std::vector<int2> src(static_cast<size_t>(nPts));
+    for (size_t i = 0; i < chords.size(); ++i) {
+      const ubg::EdgeChord& c = chords[i];
+      const float ringRs = c.srcSphere >= 0 ? d->sphere_center_radius[4 * c.srcSphere + 3] : -1.0f;
+      for (int k = 0; k <= K[i] && K[i] > 0; ++k) {
+        float p[3];
+        ubg::chordPoint(c, K[i], k, p);
+        pts[static_cast<size_t>(first[i] + k)] = make_float4(p[0], p[1], p[2], ringRs);
+        src[static_cast<size_t>(first[i] + k)] = make_int2(c.srcSphere, c.srcCyl);
+      }
+    }
+    ubg::SolidGrid g;
+    ubg::buildSolidGrid(d, g);
+    std::vector<uint8_t> inside(static_cast<size_t>(nPts), 0);
+    if (nPts > 0) {
+      if (!ctx) return fail(UBG_ERR_INVALID_ARGUMENT, "ubg_object_space_edges: the clip needs a device context");
+      std::lock_guard<std::mutex> lk(ctx->mu);
+      UBG_HIP(hipSetDevice(ctx->device), "hipSetDevice");
+      hipStream_t st = ctx->stream;
+      const size_t ptBytes = static_cast<size_t>(nPts) * 16, srcBytes = static_cast<size_t>(nPts) * 8;
+      const size_t sphBytes = g.spheres.size() * 4, cylBytes = g.cyls.size() * 4;
+      const size_t startBytes = g.cellStart.size() * 4, itemBytes = g.cellItems.size() * 4;
+      int r = ensure(ctx->edgePts, ptBytes + srcBytes);
+      if (r == UBG_OK) r = ensure(ctx->edgeSolids, sphBytes + cylBytes + startBytes + itemBytes + 64);
+      if (r == UBG_OK) r = ensure(ctx->edgeOut, static_cast<size_t>(nPts));
+      if (r != UBG_OK) return r;
+      char* pb = static_cast<char*>(ctx->edgePts.p);
+      char* sb = static_cast<char*>(ctx->edgeSolids.p);
+      // 16-byte aligned sub-buffers: spheres, cylinders, cell starts, items
+      const size_t oCyl = (sphBytes + 15) & ~size_t(15), oStart = oCyl + ((cylBytes + 15) & ~size_t(15));
+      const size_t oItem = oStart + ((startBytes + 15) & ~size_t(15));
+      UBG_HIP(hipMemcpyAsync(pb, pts.data(), ptBytes, hipMemcpyHostToDevice, st), "hipMemcpy(edge points)");
+      UBG_HIP(hipMemcpyAsync(pb + ptBytes, src.data(), srcBytes, hipMemcpyHostToDevice, st), "hipMemcpy(edge sources)");
+      if (sphBytes) UBG_HIP(hipMemcpyAsync(sb, g.spheres.data(), sphBytes, hipMemcpyHostToDevice, st), "hipMemcpy(spheres)");
+      if (cylBytes)
+        UBG_HIP(hipMemcpyAsync(sb + oCyl, g.cyls.data(), cylBytes, hipMemcpyHostToDevice, st), "hipMemcpy(cylinders)");
+      UBG_HIP(hipMemcpyAsync(sb + oStart, g.cellStart.data(), startBytes, hipMemcpyHostToDevice, st),
+              "hipMemcpy(grid)");
+      if (itemBytes)
+        UBG_HIP(hipMemcpyAsync(sb + oItem, g.cellItems.data(), itemBytes, hipMemcpyHostToDevice, st),
+                "hipMemcpy(grid items)");
+      ubg::launchSolidsContain(nPts, reinterpret_cast<const float4*>(pb), reinterpret_cast<const int2*>(pb + ptBytes),
+                               reinterpret_cast<const float4*>(sb), reinterpret_cast<const float4*>(sb + oCyl),
+                               reinterpret_cast<const uint32_t*>(sb + oStart),
+                               reinterpret_cast<const uint32_t*>(sb + oItem), g.lo, g.inv, g.dim,
+                               static_cast<uint8_t*>(ctx->edgeOut.p), st);
+      UBG_HIP(hipGetLastError(), "kernel launch");
+      UBG_HIP(hipMemcpyAsync(inside.data(), ctx->edgeOut.p, static_cast<size_t>(nPts), hipMemcpyDeviceToHost, st),
+              "hipMemcpy(edge clip result)");
+      UBG_HIP(hipStreamSynchronize(st), "edge clip sync");
+    }
+    for (size_t i = 0; i < chords.size(); ++i) {
+      const int k1 = K[i];
+      if (k1 <= 0) continue;
+      const uint8_t* in = &inside[static_cast<size_t>(first[i])];
+      auto emitRun = [&](int a, int b) {
+        if (b <= a) return;
+        float pa[3], pbp[3];
+        ubg::chordPoint(chords[i], k1, a, pa);
+        ubg::chordPoint(chords[i], k1, b, pbp);
+        edges.push(pa, pbp, *o, chords[i].group);
+      };
+      int runStart = -1;
+      for (int k = 0; k <= k1; ++k) {
+        if (!in[k] && runStart < 0) runStart = k;
+        if (in[k] && runStart >= 0) {
+          emitRun(runStart, k - 1);
+          runStart = -1;
+        }
+      }
+      if (runStart >= 0) emitRun(runStart, k1);
+    }
+  }
+  if (meshEdges) ubg::meshFeatureEdges(d, *cam, *o, edges);
+  const int64_t n = edges.size();
+  out->n = n;
+  out->radius = o->width > 0.0f ? o->width : 0.0f;
+  if (n > 0) {
+    out->p0 = static_cast<float*>(std::malloc(static_cast<size_t>(n) * 12));
+    out->p1 = static_cast<float*>(std::malloc(static_cast<size_t>(n) * 12));
+    out->color = static_cast<float*>(std::malloc(static_cast<size_t>(n) * 16));
+    out->opacity1 = static_cast<float*>(std::malloc(static_cast<size_t>(n) * 4));
+    out->group = static_cast<uint16_t*>(std::malloc(static_cast<size_t>(n) * 2));
+    if (!out->p0 || !out->p1 || !out->color || !out->opacity1 || !out->group) {
+      ubg_edges_free(out);
+      return fail(UBG_ERR_OUT_OF_MEMORY, "ubg_object_space_edges: host allocation");
+    }
+    std::memcpy(out->p0, edges.p0.data(), static_cast<size_t>(n) * 12);
+    std::memcpy(out->p1, edges.p1.data(), static_cast<size_t>(n) * 12);
+    std::memcpy(out->color, edges.color.data(), static_cast<size_t>(n) * 16);
+    std::memcpy(out->opacity1, edges.opacity1.data(), static_cast<size_t>(n) * 4);
+    std::memcpy(out->group, edges.group.data(), static_cast<size_t>(n) * 2);
+  }
+  return UBG_OK;
+}
+
+void ubg_edges_free(ubg_edges* e) {
+  if (!e) return;
+  std::free(e->p0);
+  std::free(e->p1);
+  std::free(e->color);
+  std::free(e->opacity1);
+  std::free(e->group);
+  *e = ubg_edges{};
 }
 
 }  // extern "C"

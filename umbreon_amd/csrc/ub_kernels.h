@@ -142,6 +142,12 @@ void launchPrimary(const FrameConst& fc, const SceneView& s, const FrameBufs& b,
                    const PartFill& fill, bool countVisits, hipStream_t st);
 // AO and resolve read the hit-queue fills on the device (no host round trip).
 void launchAo(const FrameConst& fc, const SceneView& s, const FrameBufs& b, bool countVisits, hipStream_t st);
+// Point-in-solid tests of the object-space edge clip over a uniform grid
+// (spheres x, y, z, r; cylinders p0.xyz r, p1.xyz 0; points x, y, z, ring
+// radius; sources = the skipped sphere / cylinder, -1 for none).
+void launchSolidsContain(int64_t n, const float4* pts, const int2* src, const float4* sph, const float4* cyl,
+                         const uint32_t* cellStart, const uint32_t* cellItems, const float lo[3], float inv,
+                         const int dim[3], uint8_t* inside, hipStream_t st);
 void launchResolve(const FrameConst& fc, const SceneView& s, const FrameBufs& b, int iter, bool countVisits,
                    hipStream_t st);
 // Coarse-grid AO pre-pass: the first-hit G-buffer of the cW x cH grid (AO

@@ -2289,6 +2289,54 @@ __global__ void k_aa_compose(FrameConst fc, AaSet cs, AaSet rs, AaSet fs, const 
   if (out.aoOpen) out.aoOpen[i] = src.aoOpen[si];
 }
 
+// insideAnySolid (edges/object_space_edges.cpp:128-146) for the clip's sample
+// points: whether the point lies strictly inside a sphere or a cylinder
+// (flat caps) other than its own source, a thinner cylinder never clipping
+// a sphere ring (ringRs > 0). Only the solids listed in the point's grid cell
+// are tested; every solid whose box holds the point is listed there.
+__global__ void k_solids_contain(int64_t n, const float4* __restrict__ pts, const int2* __restrict__ src,
+                                 const float4* __restrict__ sph, const float4* __restrict__ cyl,
+                                 const uint32_t* __restrict__ cellStart, const uint32_t* __restrict__ cellItems,
+                                 float lox, float loy, float loz, float inv, int dx, int dy, int dz,
+                                 uint8_t* __restrict__ inside) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float4 q = pts[i];
+  const int2 sk = src[i];
+  const V3 X = mk(q.x, q.y, q.z);
+  const float ringRs = q.w;
+  auto cellOf = [](float v, float lo, float iv, int dim) {
+    const float f = floorf((v - lo) * iv);
+    return min(max(static_cast<int>(f), 0), dim - 1);
+  };
+  const int cx = cellOf(X.x, lox, inv, dx), cy = cellOf(X.y, loy, inv, dy), cz = cellOf(X.z, loz, inv, dz);
+  const uint32_t c = (static_cast<uint32_t>(cz) * static_cast<uint32_t>(dy) + static_cast<uint32_t>(cy)) *
+                         static_cast<uint32_t>(dx) + static_cast<uint32_t>(cx);
+  bool in = false;
+  for (uint32_t k = cellStart[c], e = cellStart[c + 1]; k < e && !in; ++k) {
+    const uint32_t item = cellItems[k];
+    if (!(item & 0x80000000u)) {
+      if (static_cast<int>(item) == sk.x) continue;
+      const float4 sp = sph[item];
+      in = len(X - mk(sp.x, sp.y, sp.z)) < sp.w;
+    } else {
+      const uint32_t j = item & 0x7FFFFFFFu;
+      if (static_cast<int>(j) == sk.y) continue;
+      const float4 a = cyl[2 * j], b4 = cyl[2 * j + 1];
+      if (ringRs > 0.0f && a.w < ringRs - 1.0e-4f) continue;
+      const V3 p0 = mk(a.x, a.y, a.z);
+      const V3 ab = mk(b4.x, b4.y, b4.z) - p0;
+      const float L = len(ab);
+      if (L <= 0.0f) continue;
+      const V3 u = ab * (1.0f / L);
+      const float t = dot(X - p0, u);
+      if (t < 0.0f || t > L) continue;
+      const V3 perp = (X - p0) - u * t;
+      in = len(perp) < a.w;
+    }
+  }
+  inside[i] = in ? 1 : 0;
+}
 __global__ void k_zero(unsigned long long* c, int n) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) c[i] = 0ull;
@@ -2301,6 +2349,14 @@ __global__ void k_reset_queues(unsigned long long* c) {
 static inline unsigned blocksFor(int64_t n, int bs) {
   return static_cast<unsigned>((n + bs - 1) / bs);
 }
+void launchSolidsContain(int64_t n, const float4* pts, const int2* src, const float4* sph, const float4* cyl,
+                         const uint32_t* cellStart, const uint32_t* cellItems, const float lo[3], float inv,
+                         const int dim[3], uint8_t* inside, hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_solids_contain, dim3(static_cast<unsigned>(blocksFor(n, 256))), dim3(256), 0, st, n, pts,
+                     src, sph, cyl, cellStart, cellItems, lo[0], lo[1], lo[2], inv, dim[0], dim[1], dim[2], inside);
+}
+
 
 void launchZeroCounters(unsigned long long* c, int n, hipStream_t st) {
   hipLaunchKernelGGL(k_zero, dim3(blocksFor(n, 64)), dim3(64), 0, st, c, n);

@@ -157,6 +157,37 @@ int main() {
     s.mesh.material.pbr.specular = 0.5f;
     check("principled metal differs", umbreon::render(s, o).color != pov.color);
   }
+  {
+    // object-space edges through RenderOptions (test_object_space_edges.cpp:
+    // 566-700): they change the image, the caller's scene is untouched, and
+    // enabling both edge methods throws
+    umbreon::Scene s;
+    s.camera = orthoCam();
+    s.background = {1, 1, 1};
+    umbreon::Sphere sp;
+    sp.center = {0, 0, 0};
+    sp.radius = 1.0f;
+    sp.color = {0.3f, 0.4f, 0.5f, 1.0f};
+    s.spheres.push_back(sp);
+    umbreon::RenderOptions r;
+    r.width = 32;
+    r.height = 32;
+    const umbreon::FrameResult plain = umbreon::render(s, r);
+    r.objectSpaceEdges.enable = true;
+    r.objectSpaceEdges.width = 0.2f;
+    r.objectSpaceEdges.raise = 0.15f;
+    const umbreon::FrameResult edged = umbreon::render(s, r);
+    check("object-space edges change the image", edged.color != plain.color);
+    check("object-space edges leave the scene untouched", s.cylinders.empty());
+    r.strokeEdges.enable = true;
+    bool threw = false;
+    try {
+      umbreon::render(s, r);
+    } catch (const std::runtime_error&) {
+      threw = true;
+    }
+    check("both edge methods throw", threw);
+  }
   std::printf("%d failure(s)\n", failures);
   return failures == 0 ? 0 : 1;
 }
