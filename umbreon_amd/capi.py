@@ -214,7 +214,8 @@ def hit_dtype():
 EXPORTS = [
     "ubg_abi_version", "ubg_status_string", "ubg_context_create", "ubg_context_destroy",
     "ubg_last_error", "ubg_scene_create", "ubg_scene_create_ex", "ubg_scene_destroy", "ubg_scene_get_info",
-    "ubg_render", "ubg_intersect", "ubg_occluded", "ubg_occluded_segments",
+    "ubg_render", "ubg_intersect", "ubg_occluded", "ubg_occluded_segments", "ubg_object_space_edges",
+    "ubg_edges_free",
 ]
 
 _lib = None
