@@ -469,6 +469,28 @@ int ubg_object_space_edges(ubg_context* ctx, const ubg_scene_desc* desc, const u
                            const ubg_edge_options* options, ubg_edges* out);
 void ubg_edges_free(ubg_edges* edges);
 
+/* The learned denoiser: a U-Net over the GI irradiance field and its guides
+ * (it replaces denoisePt1E / OIDN, pt1_denoise.cpp:20-81), with bf16 MFMA
+ * convolutions and fp32 accumulation, trained on the device. levels 2..6,
+ * channels[l] in {32, 64, 96, 128}. Features per pixel (NHWC fp32, 10):
+ * log(1 + E) rgb, albedo rgb, normal xyz, depth / max depth; outputs and
+ * targets (3): log(1 + E). h and w must be multiples of 2^(levels - 1).
+ * Parameters are fp32, per convolution W[9][cout][cin] then b[cout]. */
+typedef struct ubg_unet ubg_unet;
+int ubg_unet_create(ubg_context* ctx, int32_t levels, const int32_t* channels, uint64_t seed, ubg_unet** out);
+void ubg_unet_destroy(ubg_unet* net);
+int64_t ubg_unet_param_count(const ubg_unet* net);
+int ubg_unet_get_params(ubg_unet* net, float* params);       /* host buffer */
+int ubg_unet_set_params(ubg_unet* net, const float* params); /* host buffer */
+/* Denoised log(1 + E) of n images (features and out per `memory`). */
+int ubg_unet_infer(ubg_unet* net, int32_t n, int32_t h, int32_t w, const float* features, float* out,
+                   int32_t memory);
+/* One training step on a batch: the mean L1 loss on log(1 + E) into *loss,
+ * the gradients into `grads` (host, may be NULL), then an Adam step (beta
+ * 0.9 / 0.999, eps 1e-8) when lr > 0. Features and targets per `memory`. */
+int ubg_unet_train_step(ubg_unet* net, int32_t n, int32_t h, int32_t w, const float* features,
+                        const float* targets, float lr, float* loss, float* grads, int32_t memory);
+
 #ifdef __cplusplus
 }
 #endif

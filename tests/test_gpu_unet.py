@@ -1,0 +1,82 @@
+"""The learned denoiser's device U-Net against the oracle (oracle/unet.py):
+forward predictions and parameter gradients on the same bf16 operands, and
+that training lowers the loss."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from oracle import unet as U
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def device():
+    from umbreon_amd.render import Device
+    d = Device(0)
+    yield d
+    d.close()
+
+
+def _batch(rng, n, h, w):
+    feat = np.zeros((n, h, w, 10), np.float32)
+    feat[..., 0:3] = rng.uniform(0.0, 2.0, (n, h, w, 3))
+    feat[..., 3:6] = rng.uniform(0.0, 1.0, (n, h, w, 3))
+    nrm = rng.normal(size=(n, h, w, 3))
+    feat[..., 6:9] = nrm / np.linalg.norm(nrm, axis=-1, keepdims=True)
+    feat[..., 9] = rng.uniform(0.0, 1.0, (n, h, w))
+    tgt = (feat[..., 0:3] * 0.5 + 0.2).astype(np.float32)
+    return feat, tgt
+
+
+@pytest.mark.parametrize("levels,ch,shape", [(2, [32, 64], (2, 16, 32)), (4, [32, 64, 96, 128], (1, 32, 48))])
+def test_forward_and_gradients_match_oracle(device, levels, ch, shape):
+    from umbreon_amd.denoise import Unet
+    net = Unet(device, levels, ch, seed=3)
+    try:
+        assert net.n_params == U.param_count(levels, ch)
+        p = net.get_params()
+        rng = np.random.default_rng(7)
+        # non-zero biases so every term is exercised
+        _, convs = U.layout(levels, ch)
+        for c in convs:
+            p[c["b_off"]:c["b_off"] + c["cout"]] = rng.normal(0.0, 0.05, c["cout"])
+        net.set_params(p)
+        feat, tgt = _batch(rng, *shape)
+        g_pred = net.infer(feat)
+        c_pred, _ = U.forward(p.astype(np.float64), feat, levels, ch, bf16=True)
+        d = np.abs(g_pred - c_pred)
+        print(f"unet L{levels} forward: max|d| {d.max():.3g} mean|d| {d.mean():.3g}", flush=True)
+        assert d.mean() <= 2e-3 and d.max() <= 5e-2
+        g_loss, g_grads = net.train_step(feat, tgt, lr=0.0, want_grads=True)
+        c_loss, c_grads = U.loss_and_grads(p.astype(np.float64), feat, tgt, levels, ch, bf16=True)
+        assert abs(g_loss - c_loss) <= 1e-3 * c_loss
+        for k, c in enumerate(convs):
+            a = g_grads[c["w_off"]:c["b_off"] + c["cout"]].astype(np.float64)
+            b = c_grads[c["w_off"]:c["b_off"] + c["cout"]]
+            nb = np.linalg.norm(b)
+            if nb == 0.0:
+                assert np.linalg.norm(a) == 0.0
+                continue
+            cos = float(a @ b / (np.linalg.norm(a) * nb))
+            rel = float(np.linalg.norm(a - b) / nb)
+            print(f"  conv {k} ({c['cin']}->{c['cout']}): cos {cos:.6f} rel {rel:.3g}", flush=True)
+            assert cos >= 0.999 and rel <= 5e-2, (k, cos, rel)
+    finally:
+        net.close()
+
+
+def test_training_lowers_the_loss(device):
+    from umbreon_amd.denoise import Unet
+    net = Unet(device, 3, [32, 64, 96], seed=1)
+    try:
+        rng = np.random.default_rng(2)
+        feat, tgt = _batch(rng, 4, 32, 32)
+        first, _ = net.train_step(feat, tgt, lr=0.0)
+        for _ in range(150):
+            loss, _ = net.train_step(feat, tgt, lr=2e-3)
+        print(f"unet training: loss {first:.4f} -> {loss:.4f}", flush=True)
+        assert loss < 0.5 * first
+    finally:
+        net.close()

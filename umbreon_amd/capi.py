@@ -215,7 +215,8 @@ EXPORTS = [
     "ubg_abi_version", "ubg_status_string", "ubg_context_create", "ubg_context_destroy",
     "ubg_last_error", "ubg_scene_create", "ubg_scene_create_ex", "ubg_scene_destroy", "ubg_scene_get_info",
     "ubg_render", "ubg_intersect", "ubg_occluded", "ubg_occluded_segments", "ubg_object_space_edges",
-    "ubg_edges_free",
+    "ubg_edges_free", "ubg_unet_create", "ubg_unet_destroy", "ubg_unet_param_count", "ubg_unet_get_params",
+    "ubg_unet_set_params", "ubg_unet_infer", "ubg_unet_train_step",
 ]
 
 _lib = None
@@ -272,6 +273,21 @@ def load() -> C.CDLL:
     lib.ubg_object_space_edges.restype = C.c_int
     lib.ubg_edges_free.argtypes = [C.POINTER(Edges)]
     lib.ubg_edges_free.restype = None
+    lib.ubg_unet_create.argtypes = [C.c_void_p, C.c_int32, C.POINTER(C.c_int32), C.c_uint64, C.POINTER(C.c_void_p)]
+    lib.ubg_unet_create.restype = C.c_int
+    lib.ubg_unet_destroy.argtypes = [C.c_void_p]
+    lib.ubg_unet_destroy.restype = None
+    lib.ubg_unet_param_count.argtypes = [C.c_void_p]
+    lib.ubg_unet_param_count.restype = C.c_int64
+    lib.ubg_unet_get_params.argtypes = [C.c_void_p, C.c_void_p]
+    lib.ubg_unet_get_params.restype = C.c_int
+    lib.ubg_unet_set_params.argtypes = [C.c_void_p, C.c_void_p]
+    lib.ubg_unet_set_params.restype = C.c_int
+    lib.ubg_unet_infer.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p, C.c_int32]
+    lib.ubg_unet_infer.restype = C.c_int
+    lib.ubg_unet_train_step.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p,
+                                        C.c_float, C.POINTER(C.c_float), C.c_void_p, C.c_int32]
+    lib.ubg_unet_train_step.restype = C.c_int
     if lib.ubg_abi_version() != ABI_VERSION:
         raise RuntimeError("libumbreon_gpu.so ABI version mismatch")
     _lib = lib

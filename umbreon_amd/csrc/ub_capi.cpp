@@ -26,6 +26,7 @@
 #include "ub_devbuild.h"
 #include "ub_edges.h"
 #include "ub_kernels.h"
+#include "ub_unet.h"
 #include "ub_layout.h"
 
 namespace {
@@ -73,14 +74,22 @@ struct ubg_context {
   DevBuf giPos, giNrm, giRefl, giE, giList, giInd, giOcc, dnA, dnB, dnGrad, dnVar, giLow, giPatch, giSnap;
   // object-space edge clip: points + sources, solids + grid, results
   DevBuf edgePts, edgeSolids, edgeOut;
+  // learned denoiser staging of host inputs / outputs
+  DevBuf unetIn, unetTgt, unetOut;
   unsigned long long* hostCounters = nullptr;  // pinned
   std::vector<DevBuf*> all() {
     return {&hitPix, &hitT,   &hitO,    &hitN,  &hitC,  &hitAo,   &activeA,  &activeB,  &acc,     &tnear,
             &hiColor, &depth, &viewZ,   &hitId, &aoOpen, &outColor, &counters, &query, &queryOut, &aoT,
             &hitOpen, &hitBent, &cGuide, &cOpen, &cBent, &hitOut, &aaSetC, &aaSetR, &aaSetF, &aaFirstN, &hitPrim,
             &aaPix, &giPos, &giNrm, &giRefl, &giE, &giList, &giInd, &giOcc, &dnA, &dnB, &dnGrad,
-            &dnVar, &giLow, &giPatch, &giSnap, &edgePts, &edgeSolids, &edgeOut};
+            &dnVar, &giLow, &giPatch, &giSnap, &edgePts, &edgeSolids, &edgeOut, &unetIn,
+            &unetTgt, &unetOut};
   }
+};
+
+struct ubg_unet {
+  ubg_context* ctx = nullptr;
+  ubg::Unet net;
 };
 
 struct ubg_scene {
@@ -1242,6 +1251,115 @@ int ubg_object_space_edges(ubg_context* ctx, const ubg_scene_desc* d, const ubg_
     std::memcpy(out->opacity1, edges.opacity1.data(), static_cast<size_t>(n) * 4);
     std::memcpy(out->group, edges.group.data(), static_cast<size_t>(n) * 2);
   }
+  return UBG_OK;
+}
+
+int ubg_unet_create(ubg_context* ctx, int32_t levels, const int32_t* channels, uint64_t seed, ubg_unet** out) {
+  if (!ctx || !channels || !out) return fail(UBG_ERR_INVALID_ARGUMENT, "ubg_unet_create: bad argument");
+  *out = nullptr;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  UBG_HIP(hipSetDevice(ctx->device), "hipSetDevice");
+  auto* u = new ubg_unet;
+  u->ctx = ctx;
+  int ch[ubg::kUnetMaxLevels] = {0};
+  for (int l = 0; l < levels && l < ubg::kUnetMaxLevels; ++l) ch[l] = channels[l];
+  std::string err;
+  if (u->net.init(levels, ch, seed, err) != 0) {
+    delete u;
+    return fail(UBG_ERR_INVALID_ARGUMENT, err);
+  }
+  *out = u;
+  return UBG_OK;
+}
+
+void ubg_unet_destroy(ubg_unet* u) {
+  if (!u) return;
+  std::lock_guard<std::mutex> lk(u->ctx->mu);
+  (void)hipSetDevice(u->ctx->device);
+  delete u;
+}
+
+int64_t ubg_unet_param_count(const ubg_unet* u) { return u ? u->net.paramCount() : 0; }
+
+int ubg_unet_get_params(ubg_unet* u, float* params) {
+  if (!u || !params) return fail(UBG_ERR_INVALID_ARGUMENT, "ubg_unet_get_params: bad argument");
+  std::lock_guard<std::mutex> lk(u->ctx->mu);
+  UBG_HIP(hipSetDevice(u->ctx->device), "hipSetDevice");
+  UBG_HIP(u->net.getParams(params, u->ctx->stream), "unet parameters");
+  return UBG_OK;
+}
+
+int ubg_unet_set_params(ubg_unet* u, const float* params) {
+  if (!u || !params) return fail(UBG_ERR_INVALID_ARGUMENT, "ubg_unet_set_params: bad argument");
+  std::lock_guard<std::mutex> lk(u->ctx->mu);
+  UBG_HIP(hipSetDevice(u->ctx->device), "hipSetDevice");
+  UBG_HIP(u->net.setParams(params, u->ctx->stream), "unet parameters");
+  return UBG_OK;
+}
+
+namespace {
+int unetShape(const ubg_unet* u, int32_t n, int32_t h, int32_t w) {
+  const int m = 1 << (u->net.levels() - 1);
+  if (n <= 0 || h <= 0 || w <= 0 || h % m != 0 || w % m != 0)
+    return fail(UBG_ERR_INVALID_ARGUMENT, "unet: n, h, w must be positive and h, w multiples of 2^(levels - 1)");
+  return UBG_OK;
+}
+}  // namespace
+
+int ubg_unet_infer(ubg_unet* u, int32_t n, int32_t h, int32_t w, const float* feat, float* out, int32_t memory) {
+  if (!u || !feat || !out) return fail(UBG_ERR_INVALID_ARGUMENT, "ubg_unet_infer: bad argument");
+  int r = unetShape(u, n, h, w);
+  if (r != UBG_OK) return r;
+  ubg_context* ctx = u->ctx;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  UBG_HIP(hipSetDevice(ctx->device), "hipSetDevice");
+  hipStream_t st = ctx->stream;
+  const size_t P = static_cast<size_t>(n) * h * w;
+  const float* dFeat = feat;
+  float* dOut = out;
+  if (!memory) {
+    r = ensure(ctx->unetIn, P * ubg::kUnetFeatures * 4);
+    if (r == UBG_OK) r = ensure(ctx->unetOut, P * 3 * 4);
+    if (r != UBG_OK) return r;
+    UBG_HIP(hipMemcpyAsync(ctx->unetIn.p, feat, P * ubg::kUnetFeatures * 4, hipMemcpyHostToDevice, st),
+            "hipMemcpy(unet features)");
+    dFeat = static_cast<const float*>(ctx->unetIn.p);
+    dOut = static_cast<float*>(ctx->unetOut.p);
+  }
+  UBG_HIP(u->net.forward(n, h, w, dFeat, dOut, st), "unet forward");
+  if (!memory) UBG_HIP(hipMemcpyAsync(out, dOut, P * 3 * 4, hipMemcpyDeviceToHost, st), "hipMemcpy(unet output)");
+  UBG_HIP(hipStreamSynchronize(st), "unet sync");
+  return UBG_OK;
+}
+
+int ubg_unet_train_step(ubg_unet* u, int32_t n, int32_t h, int32_t w, const float* feat, const float* target,
+                        float lr, float* loss, float* grads, int32_t memory) {
+  if (!u || !feat || !target) return fail(UBG_ERR_INVALID_ARGUMENT, "ubg_unet_train_step: bad argument");
+  int r = unetShape(u, n, h, w);
+  if (r != UBG_OK) return r;
+  ubg_context* ctx = u->ctx;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  UBG_HIP(hipSetDevice(ctx->device), "hipSetDevice");
+  hipStream_t st = ctx->stream;
+  const size_t P = static_cast<size_t>(n) * h * w;
+  const float* dFeat = feat;
+  const float* dTgt = target;
+  if (!memory) {
+    r = ensure(ctx->unetIn, P * ubg::kUnetFeatures * 4);
+    if (r == UBG_OK) r = ensure(ctx->unetTgt, P * 3 * 4);
+    if (r != UBG_OK) return r;
+    UBG_HIP(hipMemcpyAsync(ctx->unetIn.p, feat, P * ubg::kUnetFeatures * 4, hipMemcpyHostToDevice, st),
+            "hipMemcpy(unet features)");
+    UBG_HIP(hipMemcpyAsync(ctx->unetTgt.p, target, P * 3 * 4, hipMemcpyHostToDevice, st), "hipMemcpy(unet targets)");
+    dFeat = static_cast<const float*>(ctx->unetIn.p);
+    dTgt = static_cast<const float*>(ctx->unetTgt.p);
+  }
+  float l = 0.0f;
+  UBG_HIP(u->net.gradients(n, h, w, dFeat, dTgt, &l, st), "unet gradients");
+  if (loss) *loss = l;
+  if (grads) UBG_HIP(u->net.copyGrads(grads, st), "unet gradient copy");
+  if (lr > 0.0f) UBG_HIP(u->net.adam(lr, st), "unet adam");
+  UBG_HIP(hipStreamSynchronize(st), "unet sync");
   return UBG_OK;
 }
 

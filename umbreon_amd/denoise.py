@@ -1,0 +1,74 @@
+"""The learned GI denoiser over the C-ABI (``ubg_unet_*``): a U-Net over the
+irradiance field E and its guides that stands in for denoisePt1E / OIDN
+(src/umbreon/integrator/pt1/pt1_denoise.cpp:20-81), trained on the device.
+
+Features per pixel (NHWC float32, 10 channels): log(1 + E) rgb, albedo rgb,
+normal xyz, depth / max depth; the network predicts log(1 + E).
+"""
+from __future__ import annotations
+
+import ctypes as C
+from typing import Optional, Sequence, Tuple
+
+import numpy as np
+
+from . import capi
+
+N_FEATURES = 10
+
+
+class Unet:
+    """One U-Net on a device context (umbreon_amd.render.Device)."""
+
+    def __init__(self, device, levels: int = 4, channels: Sequence[int] = (32, 64, 96, 128), seed: int = 0):
+        self._lib = capi.load()
+        self.device = device
+        self.levels = int(levels)
+        self.channels = tuple(int(c) for c in channels[:levels])
+        ch = (C.c_int32 * 6)(*(list(self.channels) + [0] * (6 - len(self.channels))))
+        h = C.c_void_p()
+        capi.check(self._lib.ubg_unet_create(device.handle, self.levels, ch, C.c_uint64(seed), C.byref(h)))
+        self.handle = h
+        self.n_params = int(self._lib.ubg_unet_param_count(h))
+
+    def close(self) -> None:
+        if self.handle:
+            self._lib.ubg_unet_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def get_params(self) -> np.ndarray:
+        p = np.empty(self.n_params, np.float32)
+        capi.check(self._lib.ubg_unet_get_params(self.handle, p.ctypes.data))
+        return p
+
+    def set_params(self, p: np.ndarray) -> None:
+        p = np.ascontiguousarray(p, np.float32)
+        if p.size != self.n_params:
+            raise ValueError(f"expected {self.n_params} parameters, got {p.size}")
+        capi.check(self._lib.ubg_unet_set_params(self.handle, p.ctypes.data))
+
+    def infer(self, features: np.ndarray) -> np.ndarray:
+        """features (n, h, w, 10) -> predicted log(1 + E) (n, h, w, 3)."""
+        f = np.ascontiguousarray(features, np.float32)
+        n, h, w, c = f.shape
+        assert c == N_FEATURES
+        out = np.empty((n, h, w, 3), np.float32)
+        capi.check(self._lib.ubg_unet_infer(self.handle, n, h, w, f.ctypes.data, out.ctypes.data, 0))
+        return out
+
+    def train_step(self, features: np.ndarray, targets: np.ndarray, lr: float,
+                   want_grads: bool = False) -> Tuple[float, Optional[np.ndarray]]:
+        f = np.ascontiguousarray(features, np.float32)
+        t = np.ascontiguousarray(targets, np.float32)
+        n, h, w, _ = f.shape
+        loss = C.c_float(0.0)
+        g = np.empty(self.n_params, np.float32) if want_grads else None
+        capi.check(self._lib.ubg_unet_train_step(self.handle, n, h, w, f.ctypes.data, t.ctypes.data, float(lr),
+                                                  C.byref(loss), g.ctypes.data if g is not None else None, 0))
+        return float(loss.value), g
