@@ -40,7 +40,7 @@
 extern "C" {
 #endif
 
-#define UBG_ABI_VERSION 6
+#define UBG_ABI_VERSION 7
 
 /* Status codes. */
 enum {
@@ -269,6 +269,10 @@ typedef struct ubg_render_params {
   int32_t pt2_adaptive;              /* default 0 */
   int32_t pt2_emissive_nee;          /* default 1 */
   int32_t pt2_reflect;               /* default 1 */
+  /* pt1_denoise == 2: the learned denoiser (ubg_unet_*, a network on this
+   * context) replaces the a-trous filter on the gathered irradiance; needs
+   * the full-resolution gather (pt1_gather_div resolving to 1). */
+  struct ubg_unet* pt1_denoise_net;
 } ubg_render_params;
 
 /* Ray accounting of one frame. A pure function of the scene and options:
@@ -320,6 +324,11 @@ typedef struct ubg_frame {
    * fraction of the gather; zero on pixels that are not receivers. */
   float* gi_indirect;
   float* gi_occlusion;
+  /* The learned denoiser's input features of the gathered (not yet
+   * denoised) irradiance, hi_w * hi_h * 10: log(1 + E) rgb, albedo rgb,
+   * normal xyz, first-hit distance / its frame maximum; may be NULL; needs
+   * the full-resolution gather. */
+  float* gi_features;
 } ubg_frame;
 
 /* Acceleration-structure layouts (ubg_scene_create_ex). */

@@ -97,7 +97,7 @@ def _opts(precision=32, threads=0, rows=None, brute=False) -> OrcOptions:
 
 
 def render(scene: Scene, opt: RenderOptions, hires_only: bool = False, precision: int = 32,
-           threads: int = 0, rows=None, brute: bool = False) -> FrameResult:
+           threads: int = 0, rows=None, brute: bool = False, want_gi_features: bool = False) -> FrameResult:
     lib = load()
     desc, keep1 = scene.to_desc()
     p, keep2 = params(scene, opt, capi.FLAG_HIRES_ONLY if hires_only else 0)
@@ -119,13 +119,15 @@ def render(scene: Scene, opt: RenderOptions, hires_only: bool = False, precision
     gi_occ = np.zeros((hh, hw), np.float32) if opt.gi else None
     fr.gi_indirect = indirect.ctypes.data if indirect is not None else None
     fr.gi_occlusion = gi_occ.ctypes.data if gi_occ is not None else None
+    feats = np.zeros((hh, hw, 10), np.float32) if want_gi_features else None
+    fr.gi_features = feats.ctypes.data if feats is not None else None
     o = _opts(precision, threads, rows, brute)
     capi.check(lib.orc_render(C.byref(desc), C.byref(p), C.byref(fr), C.byref(o)))
     del keep1, keep2
     return FrameResult(width=ow, height=oh, color=color, depth=depth, view_z=view_z, hit_id=hit_id,
                        ao_open=ao, stats=fr.stats.as_dict(), render_seconds=fr.render_seconds,
                        post_seconds=fr.post_seconds, aa_mask=aa_mask, aa_flagged=int(fr.aa_flagged),
-                       indirect=indirect, gi_occlusion=gi_occ)
+                       indirect=indirect, gi_occlusion=gi_occ, gi_features=feats)
 
 
 def pt1_gather_point(scene: Scene, opt: RenderOptions, P, N, Ng=None, spp: int = 1024, seed: int = 12345,

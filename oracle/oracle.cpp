@@ -2657,6 +2657,21 @@ static int renderT(const ubg_scene_desc* desc, const ubg_render_params* p, ubg_f
     std::vector<float> Ebuf(npx * 3, 0.0f);
     if (div <= 1) {
       gatherGrid(W, H, giPos, giNrm, nullptr, giElig, hiDepth, spp, Ebuf, giOcc);
+      if (fr->gi_features) {
+        // the learned denoiser's features of the gathered E: log(1 + E), the
+        // receiver's albedo, the normal AOV, distance / its frame maximum
+        float dmax = 0.0f;
+        for (std::size_t q = 0; q < npx; ++q) dmax = std::max(dmax, hiDepth[q]);
+        for (std::size_t q = 0; q < npx; ++q) {
+          float* f = fr->gi_features + 10 * q;
+          for (int k = 0; k < 3; ++k) {
+            f[k] = std::log1p(std::max(Ebuf[3 * q + k], 0.0f));
+            f[3 + k] = giElig[q] ? giRefl[3 * q + k] : 0.0f;
+            f[6 + k] = giNrm[3 * q + k];
+          }
+          f[9] = dmax > 0.0f ? hiDepth[q] / dmax : 0.0f;
+        }
+      }
       denoiseE(W, H, Ebuf, giNrm, giPos);
     } else {
       // embree_renderer.cpp:719-958
@@ -3236,6 +3251,8 @@ int orc_render(const ubg_scene_desc* desc, const ubg_render_params* params, ubg_
   if (params->width <= 0 || params->height <= 0) return UBG_ERR_INVALID_ARGUMENT;
   const orc_options* o = opt ? opt : &kDefaultOpt;
   if (!orc::giSupported(desc, params)) return UBG_ERR_UNSUPPORTED;
+  // the learned denoiser is checked through its network, not restated here
+  if (params->gi && params->pt1_denoise == 2) return UBG_ERR_UNSUPPORTED;
   return o->precision == 64 ? orc::renderT<double>(desc, params, frame, o)
                             : orc::renderT<float>(desc, params, frame, o);
 }

@@ -23,7 +23,7 @@ UBG_ERR_UNSUPPORTED = 5
 UBG_ERR_INTERNAL = 6
 
 # UBG_ABI_VERSION of include/umbreon_gpu.h this mirror is written against
-ABI_VERSION = 6
+ABI_VERSION = 7
 
 BVH2 = 0
 BVH8 = 1
@@ -117,6 +117,7 @@ class RenderParams(C.Structure):
         ("pt1_edge_patch", C.c_int32), ("pt1_edge_patch_spp_mul", C.c_int32), ("pt1_edge_patch_thresh", C.c_float),
         ("pt2_pattern", C.c_int32), ("pt2_emissive", C.c_int32), ("pt2_rounds", C.c_int32),
         ("pt2_adaptive", C.c_int32), ("pt2_emissive_nee", C.c_int32), ("pt2_reflect", C.c_int32),
+        ("pt1_denoise_net", C.c_void_p),
     ]
 
 
@@ -143,7 +144,7 @@ class Frame(C.Structure):
         ("stats", RayStats), ("render_seconds", C.c_double), ("post_seconds", C.c_double),
         ("cancelled", C.c_int32), ("layers", C.c_int32), ("kernel_ms", C.c_double * 4),
         ("kernel_launches", C.c_int32 * 4), ("aa_mask", C.c_void_p), ("aa_flagged", C.c_int64),
-        ("gi_indirect", C.c_void_p), ("gi_occlusion", C.c_void_p),
+        ("gi_indirect", C.c_void_p), ("gi_occlusion", C.c_void_p), ("gi_features", C.c_void_p),
     ]
 
 

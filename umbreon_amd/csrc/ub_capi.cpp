@@ -74,8 +74,9 @@ struct ubg_context {
   DevBuf giPos, giNrm, giRefl, giE, giList, giInd, giOcc, dnA, dnB, dnGrad, dnVar, giLow, giPatch, giSnap;
   // object-space edge clip: points + sources, solids + grid, results
   DevBuf edgePts, edgeSolids, edgeOut;
-  // learned denoiser staging of host inputs / outputs
-  DevBuf unetIn, unetTgt, unetOut;
+  // learned denoiser staging of host inputs / outputs; the render's feature
+  // AOV, the padded network input, its prediction and a max-depth word
+  DevBuf unetIn, unetTgt, unetOut, giFeat, giFeatP, giPred, giMax;
   unsigned long long* hostCounters = nullptr;  // pinned
   std::vector<DevBuf*> all() {
     return {&hitPix, &hitT,   &hitO,    &hitN,  &hitC,  &hitAo,   &activeA,  &activeB,  &acc,     &tnear,
@@ -83,7 +84,7 @@ struct ubg_context {
             &hitOpen, &hitBent, &cGuide, &cOpen, &cBent, &hitOut, &aaSetC, &aaSetR, &aaSetF, &aaFirstN, &hitPrim,
             &aaPix, &giPos, &giNrm, &giRefl, &giE, &giList, &giInd, &giOcc, &dnA, &dnB, &dnGrad,
             &dnVar, &giLow, &giPatch, &giSnap, &edgePts, &edgeSolids, &edgeOut, &unetIn,
-            &unetTgt, &unetOut};
+            &unetTgt, &unetOut, &giFeat, &giFeatP, &giPred, &giMax};
   }
 };
 
@@ -487,6 +488,11 @@ int ubg_render(ubg_context* ctx, const ubg_scene* scene, const ubg_render_params
     giDiv = p->pt1_gather_div < 0 ? ss : p->pt1_gather_div;
     if (giDiv == 0) giDiv = p->pt1_half_res ? 2 : 1;
     if (giDiv < 1) giDiv = 1;
+    if ((p->pt1_denoise == 2 || fr->gi_features) && giDiv != 1)
+      return fail(UBG_ERR_UNSUPPORTED, "ubg_render: the learned denoiser and its features need the full-resolution "
+                                       "gather (pt1_gather_div resolving to 1)");
+    if (p->pt1_denoise == 2 && (!p->pt1_denoise_net || p->pt1_denoise_net->ctx != ctx))
+      return fail(UBG_ERR_INVALID_ARGUMENT, "ubg_render: pt1_denoise 2 needs pt1_denoise_net, a network on this context");
     if (p->pt1_spp > (1 << 20) || p->gi_bounces > (1 << 16) ||
         static_cast<int64_t>(std::max(1, p->pt1_spp)) * std::max(1, p->pt1_edge_patch_spp_mul) > (1 << 24))
       return fail(UBG_ERR_INVALID_ARGUMENT, "ubg_render: pt1_spp / gi_bounces / edge-patch samples out of range");
@@ -628,6 +634,14 @@ int ubg_render(ubg_context* ctx, const ubg_scene* scene, const ubg_render_params
     need(ctx->giE, n * 16);
     need(ctx->giList, qn * 4);
     if (fr->gi_indirect) need(ctx->giInd, n * 12);
+    if (fr->gi_features || p->pt1_denoise == 2) need(ctx->giMax, 256);
+    if (fr->gi_features) need(ctx->giFeat, n * 40);
+    if (p->pt1_denoise == 2) {
+      const int m = 1 << (p->pt1_denoise_net->net.levels() - 1);
+      const size_t np = static_cast<size_t>((W + m - 1) / m * m) * ((H + m - 1) / m * m);
+      need(ctx->giFeatP, np * 40);
+      need(ctx->giPred, np * 12);
+    }
     if (fr->gi_occlusion) need(ctx->giOcc, n * 4);
     if (p->pt1_denoise) {
       need(ctx->dnA, n * 16);
@@ -928,7 +942,25 @@ int ubg_render(ubg_context* ctx, const ubg_scene* scene, const ubg_render_params
         // embree_renderer.cpp:959-1011: gather (and denoise) on the render grid
         ubg::launchGiList(full, nullptr, true, b, st);
         timed(UBG_KERNEL_AO, [&] { ubg::launchGiGather(fc, scene->view, b, full, countVisits, st); });
-        denoise(full);
+        if (fr->gi_features)
+          ubg::launchGiFeatures(W, H, W, H, b, static_cast<unsigned int*>(ctx->giMax.p),
+                                static_cast<float*>(ctx->giFeat.p), st);
+        if (p->pt1_denoise == 2) {
+          // the learned denoiser in place of the a-trous filter: features on
+          // the grid padded to the network's multiple, the prediction back
+          // onto the receivers
+          ubg::Unet& net = p->pt1_denoise_net->net;
+          const int m = 1 << (net.levels() - 1);
+          const int Wp = (W + m - 1) / m * m, Hp = (H + m - 1) / m * m;
+          ubg::launchGiFeatures(W, H, Wp, Hp, b, static_cast<unsigned int*>(ctx->giMax.p),
+                                static_cast<float*>(ctx->giFeatP.p), st);
+          UBG_HIP(net.forward(1, Hp, Wp, static_cast<const float*>(ctx->giFeatP.p), static_cast<float*>(ctx->giPred.p),
+                              st),
+                  "learned denoiser");
+          ubg::launchGiApplyNet(W, H, Wp, static_cast<const float*>(ctx->giPred.p), b, st);
+        } else {
+          denoise(full);
+        }
       } else {
         // embree_renderer.cpp:719-958: own G-buffer on the reduced grid, gather
         // and denoise there, joint-bilateral upsample, full-res rim patch
@@ -1000,6 +1032,7 @@ int ubg_render(ubg_context* ctx, const ubg_scene* scene, const ubg_render_params
   if (r == UBG_OK && fr->ao_open) r = copyOut(fr->ao_open, ctx->aoOpen.p, n * 4, fr->memory, st);
   if (r == UBG_OK && gi && fr->gi_indirect) r = copyOut(fr->gi_indirect, ctx->giInd.p, n * 12, fr->memory, st);
   if (r == UBG_OK && gi && fr->gi_occlusion) r = copyOut(fr->gi_occlusion, ctx->giOcc.p, n * 4, fr->memory, st);
+  if (r == UBG_OK && gi && fr->gi_features) r = copyOut(fr->gi_features, ctx->giFeat.p, n * 40, fr->memory, st);
   if (r != UBG_OK) return r;
   UBG_HIP(hipMemcpyAsync(ctx->hostCounters, b.counters, ubg::kCounterWords * sizeof(unsigned long long),
                          hipMemcpyDeviceToHost, st),

@@ -2289,6 +2289,57 @@ __global__ void k_aa_compose(FrameConst fc, AaSet cs, AaSet rs, AaSet fs, const 
   if (out.aoOpen) out.aoOpen[i] = src.aoOpen[si];
 }
 
+// The learned denoiser's input features on the render grid, from the gather
+// before any denoise: log(1 + E) rgb, the receiver's albedo (its diffuse
+// reflectance), the normal AOV, and the first-hit distance over the frame's
+// largest (k_gi_depth_max). Pixels of the Wp x Hp feature grid beyond W x H
+// (the network's padding) repeat the nearest edge pixel.
+__global__ void k_gi_depth_max(int64_t n, const float* __restrict__ depth, unsigned int* __restrict__ maxBits) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  unsigned int v = 0u;
+  if (i < n) v = __float_as_uint(fmaxf(depth[i], 0.0f));  // non-negative floats order as their bits
+  for (int o = 32; o > 0; o >>= 1) v = max(v, static_cast<unsigned int>(__shfl_xor(static_cast<int>(v), o, 64)));
+  if (__lane_id() == 0 && v) atomicMax(maxBits, v);
+}
+__global__ void k_gi_features(int W, int H, int Wp, int Hp, const float4* __restrict__ E,
+                              const float4* __restrict__ refl, const float4* __restrict__ nrm,
+                              const float* __restrict__ depth, const unsigned int* __restrict__ maxBits,
+                              float* __restrict__ feat) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= static_cast<int64_t>(Wp) * Hp) return;
+  const int x = min(static_cast<int>(i % Wp), W - 1), y = min(static_cast<int>(i / Wp), H - 1);
+  const int64_t q = static_cast<int64_t>(y) * W + x;
+  const float4 e = E[q], a = refl[q], nn = nrm[q];
+  const bool recv = nn.w != 0.0f;
+  const float dmax = __uint_as_float(*maxBits);
+  float* f = feat + i * 10;
+  f[0] = log1pf(fmaxf(e.x, 0.0f));
+  f[1] = log1pf(fmaxf(e.y, 0.0f));
+  f[2] = log1pf(fmaxf(e.z, 0.0f));
+  f[3] = recv ? a.x : 0.0f;
+  f[4] = recv ? a.y : 0.0f;
+  f[5] = recv ? a.z : 0.0f;
+  f[6] = nn.x;
+  f[7] = nn.y;
+  f[8] = nn.z;
+  f[9] = dmax > 0.0f ? depth[q] / dmax : 0.0f;
+}
+// the network's log(1 + E) back onto the receivers' E (the gather's hit
+// fraction in E.w is kept)
+__global__ void k_gi_apply_net(int W, int H, int Wp, const float* __restrict__ pred, const float4* __restrict__ nrm,
+                               float4* __restrict__ E) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= static_cast<int64_t>(W) * H) return;
+  if (nrm[i].w == 0.0f) return;
+  const int x = static_cast<int>(i % W), y = static_cast<int>(i / W);
+  const float* p = pred + (static_cast<int64_t>(y) * Wp + x) * 3;
+  float4 e = E[i];
+  e.x = fmaxf(expm1f(p[0]), 0.0f);
+  e.y = fmaxf(expm1f(p[1]), 0.0f);
+  e.z = fmaxf(expm1f(p[2]), 0.0f);
+  E[i] = e;
+}
+
 // insideAnySolid (edges/object_space_edges.cpp:128-146) for the clip's sample
 // points: whether the point lies strictly inside a sphere or a cylinder
 // (flat caps) other than its own source, a thinner cylinder never clipping
@@ -2337,6 +2388,10 @@ __global__ void k_solids_contain(int64_t n, const float4* __restrict__ pts, cons
   }
   inside[i] = in ? 1 : 0;
 }
+__global__ void k_zero_u32(unsigned int* c, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) c[i] = 0u;
+}
 __global__ void k_zero(unsigned long long* c, int n) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) c[i] = 0ull;
@@ -2348,6 +2403,19 @@ __global__ void k_reset_queues(unsigned long long* c) {
 
 static inline unsigned blocksFor(int64_t n, int bs) {
   return static_cast<unsigned>((n + bs - 1) / bs);
+}
+void launchGiFeatures(int W, int H, int Wp, int Hp, const FrameBufs& b, unsigned int* maxBits, float* feat,
+                      hipStream_t st) {
+  const int64_t n = static_cast<int64_t>(W) * H;
+  hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, st, maxBits, 1);
+  hipLaunchKernelGGL(k_gi_depth_max, dim3(blocksFor(n, 256)), dim3(256), 0, st, n, b.depth, maxBits);
+  const int64_t np = static_cast<int64_t>(Wp) * Hp;
+  hipLaunchKernelGGL(k_gi_features, dim3(blocksFor(np, 256)), dim3(256), 0, st, W, H, Wp, Hp, b.giE, b.giRefl,
+                     b.giNrm, b.depth, maxBits, feat);
+}
+void launchGiApplyNet(int W, int H, int Wp, const float* pred, const FrameBufs& b, hipStream_t st) {
+  const int64_t n = static_cast<int64_t>(W) * H;
+  hipLaunchKernelGGL(k_gi_apply_net, dim3(blocksFor(n, 256)), dim3(256), 0, st, W, H, Wp, pred, b.giNrm, b.giE);
 }
 void launchSolidsContain(int64_t n, const float4* pts, const int2* src, const float4* sph, const float4* cyl,
                          const uint32_t* cellStart, const uint32_t* cellItems, const float lo[3], float inv,

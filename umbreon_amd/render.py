@@ -38,6 +38,7 @@ class FrameResult:
     aa_flagged: int = 0
     indirect: Optional[np.ndarray] = None      # (hiH, hiW, 3) GI indirect added to colour
     gi_occlusion: Optional[np.ndarray] = None  # (hiH, hiW) first-bounce hit fraction
+    gi_features: Optional[np.ndarray] = None   # (hiH, hiW, 10) learned-denoiser features of the gather
 
     @property
     def total_rays(self) -> int:
@@ -130,11 +131,19 @@ class BuiltScene:
     def render(self, scene: Scene, opt: RenderOptions, hires_only: bool = False,
                want_ids: bool = False, want_ao: bool = False, count_visits: bool = False,
                kernel_times: bool = False, outputs: bool = True,
-               cancel_flag: Optional[C.c_int32] = None) -> FrameResult:
-        """One frame. outputs=False leaves the frame in HBM (no host copy)."""
+               cancel_flag: Optional[C.c_int32] = None, want_gi_features: bool = False,
+               denoise_net=None) -> FrameResult:
+        """One frame. outputs=False leaves the frame in HBM (no host copy).
+        want_gi_features returns the learned denoiser's input features of the
+        gathered irradiance (hi-res H x W x 10); denoise_net (an
+        umbreon_amd.denoise.Unet) runs the learned denoiser in place of the
+        a-trous filter (opt.pt1_denoise is then 2)."""
         flags = ((capi.FLAG_HIRES_ONLY if hires_only else 0) | (capi.FLAG_COUNT_VISITS if count_visits else 0)
                  | (capi.FLAG_KERNEL_TIMES if kernel_times else 0))
         p, keep = params(scene, opt, flags, cancel_flag)
+        if denoise_net is not None:
+            p.pt1_denoise = 2
+            p.pt1_denoise_net = denoise_net.handle
         ss = max(1, int(opt.supersample))
         hw, hh = opt.width * ss, opt.height * ss
         ow, oh = (hw, hh) if hires_only else (opt.width, opt.height)
@@ -156,6 +165,8 @@ class BuiltScene:
         gi_occ = np.zeros((hh, hw), np.float32) if opt.gi and outputs else None
         fr.gi_indirect = indirect.ctypes.data if indirect is not None else None
         fr.gi_occlusion = gi_occ.ctypes.data if gi_occ is not None else None
+        feats = np.zeros((hh, hw, 10), np.float32) if want_gi_features else None
+        fr.gi_features = feats.ctypes.data if feats is not None else None
         fr.memory = 0
         capi.check(self._lib.ubg_render(self.device.handle, self.handle, C.byref(p), C.byref(fr)))
         del keep
@@ -166,7 +177,7 @@ class BuiltScene:
                            kernel_ms={k: float(fr.kernel_ms[i]) for i, k in enumerate(capi.KERNELS)},
                            kernel_launches={k: int(fr.kernel_launches[i]) for i, k in enumerate(capi.KERNELS)},
                            aa_mask=aa_mask, aa_flagged=int(fr.aa_flagged), indirect=indirect,
-                           gi_occlusion=gi_occ)
+                           gi_occlusion=gi_occ, gi_features=feats)
 
 
 def host_query(scene: Scene, rays: np.ndarray, layout: int = capi.BVH8):
