@@ -109,6 +109,32 @@ def calib_copy(mib: int) -> None:
     log(f"calibration copy: {n} bytes device-to-device")
 
 
+def time_to_40db(dev) -> dict:
+    """The metric's second half: wall time per frame for the GI frame to reach
+    40 dB PSNR (sRGB, the whole frame) against a 4096-sample render, for the
+    plain path tracer (1-256 samples per pixel) and for the learned denoiser
+    (1-8 samples, umbreon_amd/weights/gi_unet.npz), on a held-out molecular
+    surface at 1024 x 1024 with pt2 GI (2 bounces, hard shadows)."""
+    from umbreon_amd.denoise_train import evaluate, load
+    weights = Path(__file__).resolve().parent / "umbreon_amd" / "weights" / "gi_unet.npz"
+    net = load(dev, str(weights))
+    try:
+        seed = 100001  # a molecular surface the training never saw
+        r = evaluate(dev, net, [seed], size=1024, ref_spp=4096, log=log)["scenes"][0]
+    finally:
+        net.close()
+    plain = {str(n): [round(ms, 3), round(db, 2)] for n, (ms, db) in zip((1, 2, 4, 8, 16, 32, 64, 128, 256), r["plain"])}
+    learned = {str(n): [round(ms, 3), round(db, 2)] for n, (ms, db) in zip((1, 2, 4, 8), r["learned"])}
+    atrous = {str(n): [round(ms, 3), round(db, 2)] for n, (ms, db) in zip((1, 2, 4, 8), r["atrous"])}
+    tp, tl = r["t40_plain"], r["t40_learned"]
+    return {"workload": "held-out molecular surface (seed 100001), 1024x1024 ss1, pt2 GI 2 bounces, "
+                        "PSNR vs 4096 spp",
+            "plain_ms": None if tp is None else round(tp, 3), "learned_ms": None if tl is None else round(tl, 3),
+            "atrous_ms": None if r["t40_atrous"] is None else round(r["t40_atrous"], 3),
+            "speedup": None if (tp is None or tl is None) else round(tp / tl, 2),
+            "per_spp_ms_db": {"plain": plain, "learned": learned, "atrous": atrous}}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -120,6 +146,8 @@ def main() -> int:
     ap.add_argument("--layout", choices=["bvh8", "bvh8gpu", "bvh2"], default="bvh8",
                     help="acceleration structure: compressed 8-wide built on the host (default) or on the GPU, "
                          "or full-precision binary")
+    ap.add_argument("--no-denoise-metric", action="store_true",
+                    help="skip the time-to-40-dB measurement of the learned GI denoiser")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="also run one device-to-device copy of this many MiB (a copy kernel of known "
                          "traffic, for scaling FETCH_SIZE / WRITE_SIZE counter readings)")
@@ -226,6 +254,10 @@ def main() -> int:
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(scene, opt, args.cpu_seconds)
 
+    t40 = None
+    if not args.no_denoise_metric:
+        t40 = time_to_40db(dev)
+
     out = {
         "metric": METRIC,
         "value": round(value, 2),
@@ -256,6 +288,7 @@ def main() -> int:
         },
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "time_to_40db": t40,
     }
     print(json.dumps(out), flush=True)
     bs.close()

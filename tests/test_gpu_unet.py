@@ -80,3 +80,42 @@ def test_training_lowers_the_loss(device):
         assert loss < 0.5 * first
     finally:
         net.close()
+
+
+def test_gi_features_match_oracle(device):
+    """The gi_features AOV (log(1 + E) of the gather, albedo, normal AOV,
+    depth / max) against the oracle's, and the learned denoiser running in
+    the render in place of the a-trous filter."""
+    from umbreon_amd.denoise import Unet
+    from umbreon_amd.denoise_train import gi_options, training_scene
+    from umbreon_amd.render import BuiltScene
+    from oracle import binding as ob
+    sc = training_scene(1, 96, 64)
+    opt = gi_options(96, 64, 4, seed=3)
+    bs = BuiltScene(device, sc)
+    net = Unet(device, 3, [32, 64, 96], seed=5)
+    try:
+        g = bs.render(sc, opt, hires_only=True, want_gi_features=True)
+        c = ob.render(sc, opt, hires_only=True, want_gi_features=True)
+        d = np.abs(g.gi_features - c.gi_features)
+        print(f"gi features: max|d| per channel {np.round(d.max(axis=(0, 1)), 6)}", flush=True)
+        assert np.array_equal(g.gi_features[..., 3:10], c.gi_features[..., 3:10])
+        assert d[..., :3].max(axis=2).mean() <= 1e-3
+        # the learned path: the receivers' E replaced by the network's
+        # prediction, everything else as without a denoiser
+        fl = bs.render(sc, opt, hires_only=True, denoise_net=net, want_gi_features=True)
+        assert np.array_equal(fl.gi_features, g.gi_features)  # the features are taken before the denoise
+        recv = g.gi_features[..., 3:6].max(axis=2) > 0
+        assert np.array_equal(fl.color[~recv], g.color[~recv])
+        assert not np.array_equal(fl.color[recv], g.color[recv])
+        # pad + predict + apply equals the network on the padded features
+        m = 4
+        Hp, Wp = (64 + m - 1) // m * m, (96 + m - 1) // m * m
+        pred = net.infer(np.pad(g.gi_features, ((0, Hp - 64), (0, Wp - 96), (0, 0)), mode="edge")[None])[0, :64, :96]
+        E_net = np.maximum(np.expm1(pred), 0.0)
+        refl = g.gi_features[..., 3:6]
+        ind = fl.indirect
+        assert np.allclose(ind[recv], (opt.gi_intensity * refl * E_net)[recv], rtol=1e-4, atol=1e-6)
+    finally:
+        net.close()
+        bs.close()
