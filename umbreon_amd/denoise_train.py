@@ -28,6 +28,10 @@ from .render import BuiltScene, Device
 from .scene import RenderOptions, Scene
 from .scenes import ball_and_stick_scene, molecular_surface_scene, spheres_scene
 
+def _say(msg: str) -> None:
+    print(msg, flush=True)
+
+
 DEFAULT_LEVELS = 4
 DEFAULT_CHANNELS = (32, 64, 96, 128)
 
@@ -67,9 +71,10 @@ def render_features(dev: Device, bs: BuiltScene, sc: Scene, opt: RenderOptions) 
     return bs.render(sc, opt, hires_only=True, want_gi_features=True).gi_features
 
 
-def make_pairs(dev: Device, seeds: List[int], size: int = 256, target_spp: int = 1024, log=print):
+def make_pairs(dev: Device, seeds: List[int], size: int = 256, target_spp: int = 1024, log=None):
     """(features (n, size, size, 10), targets (n, size, size, 3)): noisy
     features at 1-8 spp, targets = log(1 + E) at target_spp."""
+    log = log or _say
     feats, tgts = [], []
     t0 = time.perf_counter()
     for i, s in enumerate(seeds):
@@ -84,14 +89,15 @@ def make_pairs(dev: Device, seeds: List[int], size: int = 256, target_spp: int =
         feats.append(f)
         tgts.append(t[..., 0:3].copy())
         if (i + 1) % 16 == 0:
-            log(f"pairs {i + 1}/{len(seeds)} ({time.perf_counter() - t0:.1f} s)", flush=True)
+            log(f"pairs {i + 1}/{len(seeds)} ({time.perf_counter() - t0:.1f} s)")
     return np.stack(feats), np.stack(tgts)
 
 
 def train(dev: Device, feats: np.ndarray, tgts: np.ndarray, steps: int = 3000, batch: int = 8, crop: int = 128,
           lr: float = 1e-3, seed: int = 0, levels: int = DEFAULT_LEVELS, channels=DEFAULT_CHANNELS,
-          log=print) -> Unet:
+          log=None) -> Unet:
     """Adam on random crops (with flips), cosine-decayed learning rate."""
+    log = log or _say
     net = Unet(dev, levels, channels, seed=seed)
     rng = np.random.default_rng(seed)
     n, h, w, _ = feats.shape
@@ -111,7 +117,7 @@ def train(dev: Device, feats: np.ndarray, tgts: np.ndarray, steps: int = 3000, b
         loss, _ = net.train_step(fb, tb, cur)
         avg = loss if avg is None else 0.98 * avg + 0.02 * loss
         if (it + 1) % 250 == 0:
-            log(f"step {it + 1}/{steps}: loss {avg:.5f} ({time.perf_counter() - t0:.1f} s)", flush=True)
+            log(f"step {it + 1}/{steps}: loss {avg:.5f} ({time.perf_counter() - t0:.1f} s)")
     return net
 
 
@@ -138,10 +144,11 @@ def time_to_psnr(points: List[Tuple[float, float]], target: float) -> Optional[f
 
 def evaluate(dev: Device, net: Optional[Unet], seeds: List[int], size: int = 512, ref_spp: int = 4096,
              plain_spp=(1, 2, 4, 8, 16, 32, 64, 128, 256), den_spp=(1, 2, 4, 8), repeats: int = 3,
-             log=print) -> dict:
+             log=None) -> dict:
     """Per held-out scene: frame time and PSNR (sRGB, full frame) against a
     ref_spp render for the plain path tracer, the a-trous denoiser and the
     learned one; the time each first reaches 40 dB."""
+    log = log or _say
     out = []
     for s in seeds:
         sc = training_scene(s, size, size)
@@ -166,12 +173,12 @@ def evaluate(dev: Device, net: Optional[Unet], seeds: List[int], size: int = 512
         row = {"seed": s, "plain": plain, "atrous": atrous, "learned": learned,
                "t40_plain": time_to_psnr(plain, 40.0), "t40_atrous": time_to_psnr(atrous, 40.0),
                "t40_learned": time_to_psnr(learned, 40.0) if net else None}
-        log(f"scene {s}: plain {[(n, round(db, 2)) for n, (_, db) in zip(plain_spp, plain)]}", flush=True)
-        log(f"  atrous {[(n, round(db, 2)) for n, (_, db) in zip(den_spp, atrous)]}", flush=True)
+        log(f"scene {s}: plain {[(n, round(db, 2)) for n, (_, db) in zip(plain_spp, plain)]}")
+        log(f"  atrous {[(n, round(db, 2)) for n, (_, db) in zip(den_spp, atrous)]}")
         if net:
-            log(f"  learned {[(n, round(db, 2)) for n, (_, db) in zip(den_spp, learned)]}", flush=True)
+            log(f"  learned {[(n, round(db, 2)) for n, (_, db) in zip(den_spp, learned)]}")
         log(f"  time to 40 dB: plain {row['t40_plain']} ms, a-trous {row['t40_atrous']} ms, "
-            f"learned {row['t40_learned']} ms", flush=True)
+            f"learned {row['t40_learned']} ms")
         out.append(row)
     return {"size": size, "ref_spp": ref_spp, "scenes": out}
 
@@ -197,7 +204,7 @@ def main() -> None:
         net = train(dev, feats, tgts, steps=args.steps)
         if args.out:
             save(args.out, net)
-            print(f"saved {args.out} ({net.n_params} parameters)", flush=True)
+            _say(f"saved {args.out} ({net.n_params} parameters)")
     finally:
         dev.close()
 
