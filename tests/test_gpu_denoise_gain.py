@@ -3,7 +3,7 @@ molecules (seeds the training never saw), the frame with the learned
 denoiser at 2 samples per pixel against a 1024-sample render is several dB
 closer than the plain path tracer and than the a-trous filter at the same
 samples (umbreon_amd/weights/gi_unet.npz, trained by
-``python -m umbreon_amd.denoise_train --scenes 96 --steps 3000``)."""
+``python -m umbreon_amd.denoise_train --scenes 192 --steps 6000``)."""
 from __future__ import annotations
 
 from dataclasses import replace

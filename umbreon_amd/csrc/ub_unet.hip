@@ -86,57 +86,69 @@ __device__ __forceinline__ uint16_t load1(const SrcDev& s, int n, int y, int x, 
 // 3x3 convolution, zero padding: out[p, co] = sum over taps and input
 // channels of in[p + tap, ci] * w[tap][co][ci] (flip: w[8 - tap]), plus bias
 // and ReLU when asked; output bf16 or fp32. Input channels [0, s0.C) come from
-// s0, the rest from s1. One wave = 16 pixels of one row x the NT * 16 output
-// channels from coBase.
-template <int NT>
+// s0, the rest from s1. One wave = NP x 16 consecutive pixels of one row x
+// the NT * 16 output channels from coBase: each weight fragment loaded feeds
+// NP matrix products.
+template <int NT, int NP>
 __global__ __launch_bounds__(256) void k_conv3x3(SrcDev s0, SrcDev s1, int N, int H, int W, int Cin, int Cout,
                                                  int coBase, const uint16_t* __restrict__ w, int flip,
                                                  const float* __restrict__ bias, int relu,
                                                  uint16_t* __restrict__ outBf, float* __restrict__ outF) {
   const int lane = threadIdx.x & 63;
   const int wave = static_cast<int>((static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x) >> 6);
-  const int tilesX = (W + 15) >> 4;
+  const int span = 16 * NP;
+  const int tilesX = (W + span - 1) / span;
   if (wave >= N * H * tilesX) return;  // the whole wave
   const int tx = wave % tilesX, rest = wave / tilesX;
   const int y = rest % H, n = rest / H;
-  const int x0 = tx * 16, row = lane & 15, grp = lane >> 4;
-  f32x4 acc[NT];
+  const int x0 = tx * span, row = lane & 15, grp = lane >> 4;
+  f32x4 acc[NP][NT];
 #pragma unroll
-  for (int j = 0; j < NT; ++j) acc[j] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+  for (int q = 0; q < NP; ++q)
+#pragma unroll
+    for (int j = 0; j < NT; ++j) acc[q][j] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
   for (int tap = 0; tap < 9; ++tap) {
-    const int yy = y + tap / 3 - 1, xx = x0 + row + tap % 3 - 1;
-    const bool in = yy >= 0 && yy < H && xx >= 0 && xx < W;
+    const int yy = y + tap / 3 - 1;
+    const bool rowIn = yy >= 0 && yy < H;
     const uint16_t* wt = w + static_cast<size_t>(flip ? 8 - tap : tap) * Cout * Cin;
     for (int ci0 = 0; ci0 < Cin; ci0 += 32) {
       const bool first = ci0 < s0.C;
       const SrcDev& s = first ? s0 : s1;
       const int c = (first ? ci0 : ci0 - s0.C) + 8 * grp;
-      uint4 av = make_uint4(0u, 0u, 0u, 0u);
-      if (in) av = load8(s, n, yy, xx, c);
-      const bf16x8 a = __builtin_bit_cast(bf16x8, av);
+      bf16x8 a[NP];
+#pragma unroll
+      for (int q = 0; q < NP; ++q) {
+        const int xx = x0 + 16 * q + row + tap % 3 - 1;
+        uint4 av = make_uint4(0u, 0u, 0u, 0u);
+        if (rowIn && xx >= 0 && xx < W) av = load8(s, n, yy, xx, c);
+        a[q] = __builtin_bit_cast(bf16x8, av);
+      }
       const uint16_t* wp = wt + static_cast<size_t>(coBase + row) * Cin + ci0 + 8 * grp;
 #pragma unroll
       for (int j = 0; j < NT; ++j) {
-        const uint4 bv = *reinterpret_cast<const uint4*>(wp + static_cast<size_t>(16 * j) * Cin);
-        acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, __builtin_bit_cast(bf16x8, bv), acc[j], 0, 0, 0);
+        const bf16x8 b = __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4*>(wp + static_cast<size_t>(16 * j) * Cin));
+#pragma unroll
+        for (int q = 0; q < NP; ++q) acc[q][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[q], b, acc[q][j], 0, 0, 0);
       }
     }
   }
 #pragma unroll
-  for (int j = 0; j < NT; ++j) {
-    const int co = coBase + 16 * j + row;
-    const float b = bias ? bias[co] : 0.0f;
+  for (int q = 0; q < NP; ++q)
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const int x = x0 + 4 * grp + r;
-      if (x >= W) continue;
-      float v = acc[j][r] + b;
-      if (relu) v = fmaxf(v, 0.0f);
-      const size_t idx = ((static_cast<size_t>(n) * H + y) * W + x) * Cout + co;
-      if (outBf) outBf[idx] = f2bf(v);
-      else outF[idx] = v;
+    for (int j = 0; j < NT; ++j) {
+      const int co = coBase + 16 * j + row;
+      const float bb = bias ? bias[co] : 0.0f;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int x = x0 + 16 * q + 4 * grp + r;
+        if (x >= W) continue;
+        float v = acc[q][j][r] + bb;
+        if (relu) v = fmaxf(v, 0.0f);
+        const size_t idx = ((static_cast<size_t>(n) * H + y) * W + x) * Cout + co;
+        if (outBf) outBf[idx] = f2bf(v);
+        else outF[idx] = v;
+      }
     }
-  }
 }
 
 // Weight gradient partials: part[g][tap][co][ci] = sum over the pixels p of
@@ -322,20 +334,35 @@ __global__ void k_weights_bf16(const float* __restrict__ w, int cout, int cin, u
 
 inline unsigned blocks(int64_t n, int bs) { return static_cast<unsigned>((n + bs - 1) / bs); }
 
-// Output channels in passes of at most 128 (Cout a multiple of 32).
+// Output channels in passes of at most 128 (Cout a multiple of 32). A wave
+// takes 4 pixel tiles for 32 output channels, 2 for more (rows narrower than
+// that: 1), so each weight fragment feeds 2 to 4 products.
 void launchConv(SrcDev s0, SrcDev s1, int N, int H, int W, int Cin, int Cout, const uint16_t* w, int flip,
                 const float* bias, int relu, uint16_t* outBf, float* outF, hipStream_t st) {
-  const int64_t waves = static_cast<int64_t>(N) * H * ((W + 15) / 16);
-  const dim3 g(blocks(waves * 64, 256)), b(256);
   for (int co = 0; co < Cout; co += 128) {
     const int nt = std::min(8, (Cout - co) / 16);
-#define UBG_CONV(K) \
-  hipLaunchKernelGGL(k_conv3x3<K>, g, b, 0, st, s0, s1, N, H, W, Cin, Cout, co, w, flip, bias, relu, outBf, outF)
-    switch (nt) {
-      case 2: UBG_CONV(2); break;
-      case 4: UBG_CONV(4); break;
-      case 6: UBG_CONV(6); break;
-      default: UBG_CONV(8); break;
+    int np = nt <= 2 ? 4 : 2;
+    while (np > 1 && W < 16 * np) np /= 2;
+    const int64_t waves = static_cast<int64_t>(N) * H * ((W + 16 * np - 1) / (16 * np));
+    const dim3 g(blocks(waves * 64, 256)), b(256);
+#define UBG_CONV(K, P) \
+  hipLaunchKernelGGL((k_conv3x3<K, P>), g, b, 0, st, s0, s1, N, H, W, Cin, Cout, co, w, flip, bias, relu, outBf, outF)
+    if (np == 4) {
+      UBG_CONV(2, 4);
+    } else if (np == 2) {
+      switch (nt) {
+        case 2: UBG_CONV(2, 2); break;
+        case 4: UBG_CONV(4, 2); break;
+        case 6: UBG_CONV(6, 2); break;
+        default: UBG_CONV(8, 2); break;
+      }
+    } else {
+      switch (nt) {
+        case 2: UBG_CONV(2, 1); break;
+        case 4: UBG_CONV(4, 1); break;
+        case 6: UBG_CONV(6, 1); break;
+        default: UBG_CONV(8, 1); break;
+      }
     }
 #undef UBG_CONV
   }
@@ -499,7 +526,7 @@ hipError_t Unet::ensure(int n, int h, int w) {
     maxDx = std::max(maxDx, P * c.cin);
     const int64_t nChunks = std::min<int64_t>(64, std::max<int64_t>(1, P / 1024));
     maxPart = std::max<int64_t>(maxPart, nChunks * 9LL * c.cout * c.cin);
-    maxPart = std::max<int64_t>(maxPart, 256LL * c.cout);
+    maxPart = std::max<int64_t>(maxPart, 4096LL * c.cout);
   }
   if (e == hipSuccess) e = alloc(static_cast<size_t>(P0) * kUnetOutPad * 4, reinterpret_cast<void**>(&outF_));
   if (e == hipSuccess) e = alloc(static_cast<size_t>(P0) * kUnetOutPad * 4, reinterpret_cast<void**>(&dOut_));
@@ -567,7 +594,7 @@ hipError_t Unet::gradients(int n, int h, int w, const float* feat, const float* 
     hipLaunchKernelGGL(k_mask_grad, dim3(blocks(P * c.cout, 256)), dim3(256), 0, st, P * c.cout, dy, y, relu, dz_);
     // bias gradient
     {
-      const int64_t g = std::min<int64_t>(256, std::max<int64_t>(1, P / 512));
+      const int64_t g = std::min<int64_t>(4096, std::max<int64_t>(1, P / 64));
       const int64_t chunk = (P + g - 1) / g;
       hipLaunchKernelGGL(k_bias_part, dim3(static_cast<unsigned>(g)), dim3(128), 0, st, dy, y, relu, c.cout, P,
                          chunk, part_);

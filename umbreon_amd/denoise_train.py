@@ -3,7 +3,7 @@ generates itself: seeded synthetic molecules (CPK spheres, ball-and-stick,
 molecular surfaces) rendered with pt2 GI at 1-8 samples per pixel (the
 noisy input features) and at a high sample count (the target irradiance).
 
-    python -m umbreon_amd.denoise_train --scenes 96 --steps 3000 --out umbreon_amd/weights/gi_unet.npz
+    python -m umbreon_amd.denoise_train --scenes 192 --steps 6000 --out umbreon_amd/weights/gi_unet.npz
     python -m umbreon_amd.denoise_train --evaluate umbreon_amd/weights/gi_unet.npz
 
 The evaluation reports, per held-out scene and for the full frame (linear
