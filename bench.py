@@ -123,16 +123,23 @@ def time_to_40db(dev) -> dict:
         r = evaluate(dev, net, [seed], size=1024, ref_spp=4096, log=log)["scenes"][0]
     finally:
         net.close()
-    plain = {str(n): [round(ms, 3), round(db, 2)] for n, (ms, db) in zip((1, 2, 4, 8, 16, 32, 64, 128, 256), r["plain"])}
-    learned = {str(n): [round(ms, 3), round(db, 2)] for n, (ms, db) in zip((1, 2, 4, 8), r["learned"])}
-    atrous = {str(n): [round(ms, 3), round(db, 2)] for n, (ms, db) in zip((1, 2, 4, 8), r["atrous"])}
+    def pts(spps, rows):
+        return {str(n): [round(p[0], 3), round(p[1], 2), round(p[2], 3)] for n, p in zip(spps, rows)}
+
+    def rnd(v):
+        return None if v is None else round(v, 3)
+
     tp, tl = r["t40_plain"], r["t40_learned"]
+    dp, dl = r["t40_plain_device"], r["t40_learned_device"]
     return {"workload": "held-out molecular surface (seed 100001), 1024x1024 ss1, pt2 GI 2 bounces, "
-                        "PSNR vs 4096 spp",
-            "plain_ms": None if tp is None else round(tp, 3), "learned_ms": None if tl is None else round(tl, 3),
-            "atrous_ms": None if r["t40_atrous"] is None else round(r["t40_atrous"], 3),
+                        "PSNR (sRGB, whole frame) vs 4096 spp",
+            "plain_ms": rnd(tp), "learned_ms": rnd(tl), "atrous_ms": rnd(r["t40_atrous"]),
             "speedup": None if (tp is None or tl is None) else round(tp / tl, 2),
-            "per_spp_ms_db": {"plain": plain, "learned": learned, "atrous": atrous}}
+            "plain_device_ms": rnd(dp), "learned_device_ms": rnd(dl), "atrous_device_ms": rnd(r["t40_atrous_device"]),
+            "speedup_device": None if (dp is None or dl is None) else round(dp / dl, 2),
+            "per_spp_wall_ms_db_device_ms": {"plain": pts((1, 2, 4, 8, 16, 32, 64, 128, 256), r["plain"]),
+                                              "learned": pts((1, 2, 4, 8), r["learned"]),
+                                              "atrous": pts((1, 2, 4, 8), r["atrous"])}}
 
 
 def main() -> int:
