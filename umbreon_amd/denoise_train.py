@@ -5,6 +5,7 @@ noisy input features) and at a high sample count (the target irradiance).
 
     python -m umbreon_amd.denoise_train --scenes 192 --steps 6000 --out umbreon_amd/weights/gi_unet.npz
     python -m umbreon_amd.denoise_train --evaluate umbreon_amd/weights/gi_unet.npz
+    python -m umbreon_amd.denoise_train --bench-infer 1024 --calls 11
 
 The evaluation reports, per held-out scene and for the full frame (linear
 colour, sRGB PSNR against a 4096-sample render as the reference's bench
@@ -188,6 +189,23 @@ def evaluate(dev: Device, net: Optional[Unet], seeds: List[int], size: int = 512
     return {"size": size, "ref_spp": ref_spp, "scenes": out}
 
 
+def bench_infer(dev: Device, weights: str, size: int, calls: int) -> float:
+    """Wall ms per U-Net inference on one size x size image of seeded random
+    features (host copies included); the kernels' own times come from a
+    kernel trace of this run."""
+    net = load(dev, weights)
+    rng = np.random.default_rng(0)
+    f = rng.uniform(0.0, 1.0, size=(1, size, size, 10)).astype(np.float32)
+    net.infer(f)
+    t = time.perf_counter()
+    for _ in range(calls):
+        net.infer(f)
+    ms = (time.perf_counter() - t) * 1e3 / max(1, calls)
+    _say(f"unet inference {size}x{size}: {ms:.3f} ms per call over {calls} calls")
+    net.close()
+    return ms
+
+
 def main() -> None:
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
     ap.add_argument("--scenes", type=int, default=96)
@@ -198,9 +216,17 @@ def main() -> None:
     ap.add_argument("--evaluate", default="", help="weights to evaluate on held-out scenes")
     ap.add_argument("--eval-scenes", type=int, default=3)
     ap.add_argument("--eval-size", type=int, default=512)
+    ap.add_argument("--bench-infer", type=int, default=0, metavar="SIZE",
+                    help="time inference of the shipped weights on one SIZE x SIZE image")
+    ap.add_argument("--calls", type=int, default=11)
     args = ap.parse_args()
     dev = Device(0)
     try:
+        if args.bench_infer:
+            import os
+            bench_infer(dev, os.path.join(os.path.dirname(__file__), "weights", "gi_unet.npz"),
+                        args.bench_infer, args.calls)
+            return
         if args.evaluate:
             net = load(dev, args.evaluate)
             evaluate(dev, net, [100000 + i for i in range(args.eval_scenes)], size=args.eval_size)
