@@ -86,69 +86,130 @@ __device__ __forceinline__ uint16_t load1(const SrcDev& s, int n, int y, int x, 
 // 3x3 convolution, zero padding: out[p, co] = sum over taps and input
 // channels of in[p + tap, ci] * w[tap][co][ci] (flip: w[8 - tap]), plus bias
 // and ReLU when asked; output bf16 or fp32. Input channels [0, s0.C) come from
-// s0, the rest from s1. One wave = NP x 16 consecutive pixels of one row x
-// the NT * 16 output channels from coBase: each weight fragment loaded feeds
-// NP matrix products.
+// s0, the rest from s1.
+// A workgroup (4 waves) covers kConvRows consecutive rows x 16 NP pixels of
+// one image and the NT * 16 output channels from coBase; wave w takes row
+// y0 + w. Per 32 input channels, the workgroup stages in LDS the patch those
+// rows read ((rows + 2) x (16 NP + 2) pixels, pooling / upsampling applied,
+// zero outside the image) and, one tap row (3 taps) at a time, the weight
+// fragments of those taps; every wave then runs the taps on them, each
+// weight fragment feeding NP matrix products. A bf16 result goes back
+// through LDS so each lane stores 16 consecutive bytes.
+constexpr int kConvRows = 4;
+// LDS of one instantiation: patch + one tap row of weights, or the bf16
+// output tiles, whichever is larger
+constexpr int convLdsBytes(int nt, int np) {
+  return (kConvRows + 2) * (16 * np + 2) * 64 + 3 * 16 * nt * 64 > kConvRows * 16 * np * nt * 32
+             ? (kConvRows + 2) * (16 * np + 2) * 64 + 3 * 16 * nt * 64
+             : kConvRows * 16 * np * nt * 32;
+}
+
 template <int NT, int NP>
 __global__ __launch_bounds__(256) void k_conv3x3(SrcDev s0, SrcDev s1, int N, int H, int W, int Cin, int Cout,
                                                  int coBase, const uint16_t* __restrict__ w, int flip,
                                                  const float* __restrict__ bias, int relu,
                                                  uint16_t* __restrict__ outBf, float* __restrict__ outF) {
-  const int lane = threadIdx.x & 63;
-  const int wave = static_cast<int>((static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x) >> 6);
-  const int span = 16 * NP;
-  const int tilesX = (W + span - 1) / span;
-  if (wave >= N * H * tilesX) return;  // the whole wave
-  const int tx = wave % tilesX, rest = wave / tilesX;
-  const int y = rest % H, n = rest / H;
-  const int x0 = tx * span, row = lane & 15, grp = lane >> 4;
+  constexpr int S = 16 * NP;        // pixels per row of the tile
+  constexpr int PW = S + 2;         // patch width
+  constexpr int PU = (kConvRows + 2) * PW * 4;  // patch size in 16-byte units (32 channels)
+  constexpr int WU = 3 * 16 * NT * 4;           // one tap row of weights, 16-byte units
+  constexpr int kLdsBytes = convLdsBytes(NT, NP);
+  static_assert((PU + WU) * 16 <= kLdsBytes && kConvRows * S * NT * 16 * 2 <= kLdsBytes, "LDS buffer too small");
+  __shared__ uint4 lds[kLdsBytes / 16];
+  uint4* wl = lds + PU;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int tilesX = (W + S - 1) / S, tilesY = (H + kConvRows - 1) / kConvRows;
+  const int tx = blockIdx.x % tilesX, rest = blockIdx.x / tilesX;
+  const int ty = rest % tilesY, n = rest / tilesY;
+  const int x0 = tx * S, y0 = ty * kConvRows, y = y0 + wv;
+  const int row = lane & 15, grp = lane >> 4;
   f32x4 acc[NP][NT];
 #pragma unroll
   for (int q = 0; q < NP; ++q)
 #pragma unroll
     for (int j = 0; j < NT; ++j) acc[q][j] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-  for (int tap = 0; tap < 9; ++tap) {
-    const int yy = y + tap / 3 - 1;
-    const bool rowIn = yy >= 0 && yy < H;
-    const uint16_t* wt = w + static_cast<size_t>(flip ? 8 - tap : tap) * Cout * Cin;
-    for (int ci0 = 0; ci0 < Cin; ci0 += 32) {
-      const bool first = ci0 < s0.C;
-      const SrcDev& s = first ? s0 : s1;
-      const int c = (first ? ci0 : ci0 - s0.C) + 8 * grp;
-      bf16x8 a[NP];
+  for (int ci0 = 0; ci0 < Cin; ci0 += 32) {
+    const bool first = ci0 < s0.C;
+    const SrcDev& s = first ? s0 : s1;
+    const int cBase = first ? ci0 : ci0 - s0.C;
+    if (ci0 > 0) __syncthreads();  // the previous chunk's patch and weights have been read
+    for (int u = threadIdx.x; u < PU; u += 256) {
+      const int c8 = u & 3, pix = u >> 2;
+      const int px = pix % PW, pr = pix / PW;
+      const int yy = y0 + pr - 1, xx = x0 + px - 1;
+      uint4 v = make_uint4(0u, 0u, 0u, 0u);
+      if (yy >= 0 && yy < H && xx >= 0 && xx < W) v = load8(s, n, yy, xx, cBase + 8 * c8);
+      lds[u] = v;
+    }
 #pragma unroll
-      for (int q = 0; q < NP; ++q) {
-        const int xx = x0 + 16 * q + row + tap % 3 - 1;
-        uint4 av = make_uint4(0u, 0u, 0u, 0u);
-        if (rowIn && xx >= 0 && xx < W) av = load8(s, n, yy, xx, c);
-        a[q] = __builtin_bit_cast(bf16x8, av);
+    for (int dy = 0; dy < 3; ++dy) {
+      if (dy > 0) __syncthreads();  // the previous tap row's weights have been read
+      // wl[(t * 16 NT + c) * 4 + k8] = w[tap 3 dy + t (flipped)][coBase + c][ci0 + 8 k8 ..]
+      for (int u = threadIdx.x; u < WU; u += 256) {
+        const int k8 = u & 3, c = (u >> 2) % (16 * NT), t = (u >> 2) / (16 * NT);
+        const int tap = 3 * dy + t;
+        wl[u] = *reinterpret_cast<const uint4*>(w + (static_cast<size_t>(flip ? 8 - tap : tap) * Cout + coBase + c) * Cin +
+                                                ci0 + 8 * k8);
       }
-      const uint16_t* wp = wt + static_cast<size_t>(coBase + row) * Cin + ci0 + 8 * grp;
+      __syncthreads();
 #pragma unroll
-      for (int j = 0; j < NT; ++j) {
-        const bf16x8 b = __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4*>(wp + static_cast<size_t>(16 * j) * Cin));
+      for (int dx = 0; dx < 3; ++dx) {
+        bf16x8 a[NP];
 #pragma unroll
-        for (int q = 0; q < NP; ++q) acc[q][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[q], b, acc[q][j], 0, 0, 0);
+        for (int q = 0; q < NP; ++q)
+          a[q] = __builtin_bit_cast(bf16x8, lds[((wv + dy) * PW + 16 * q + row + dx) * 4 + grp]);
+#pragma unroll
+        for (int j = 0; j < NT; ++j) {
+          const bf16x8 b = __builtin_bit_cast(bf16x8, wl[(dx * 16 * NT + 16 * j + row) * 4 + grp]);
+#pragma unroll
+          for (int q = 0; q < NP; ++q) acc[q][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[q], b, acc[q][j], 0, 0, 0);
+        }
       }
     }
   }
+  if (outF) {
+    if (y >= H) return;
+#pragma unroll
+    for (int q = 0; q < NP; ++q)
+#pragma unroll
+      for (int j = 0; j < NT; ++j) {
+        const int co = coBase + 16 * j + row;
+        const float bb = bias ? bias[co] : 0.0f;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int x = x0 + 16 * q + 4 * grp + r;
+          if (x >= W) continue;
+          float v = acc[q][j][r] + bb;
+          if (relu) v = fmaxf(v, 0.0f);
+          outF[((static_cast<size_t>(n) * H + y) * W + x) * Cout + co] = v;
+        }
+      }
+    return;
+  }
+  // bf16: the wave's tile [S pixels][16 NT channels] through LDS
+  __syncthreads();  // every wave is done with the patch
+  uint16_t* tile = reinterpret_cast<uint16_t*>(lds) + wv * (S * NT * 16);
 #pragma unroll
   for (int q = 0; q < NP; ++q)
 #pragma unroll
     for (int j = 0; j < NT; ++j) {
-      const int co = coBase + 16 * j + row;
-      const float bb = bias ? bias[co] : 0.0f;
+      const float bb = bias ? bias[coBase + 16 * j + row] : 0.0f;
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int x = x0 + 16 * q + 4 * grp + r;
-        if (x >= W) continue;
         float v = acc[q][j][r] + bb;
         if (relu) v = fmaxf(v, 0.0f);
-        const size_t idx = ((static_cast<size_t>(n) * H + y) * W + x) * Cout + co;
-        if (outBf) outBf[idx] = f2bf(v);
-        else outF[idx] = v;
+        tile[(16 * q + 4 * grp + r) * (NT * 16) + 16 * j + row] = f2bf(v);
       }
     }
+  __syncthreads();
+  if (y >= H) return;
+  const uint4* t4 = reinterpret_cast<const uint4*>(tile);
+  for (int u = lane; u < S * NT * 2; u += 64) {
+    const int px = u / (NT * 2), c8 = u % (NT * 2);
+    const int x = x0 + px;
+    if (x < W)
+      *reinterpret_cast<uint4*>(outBf + ((static_cast<size_t>(n) * H + y) * W + x) * Cout + coBase + 8 * c8) = t4[u];
+  }
 }
 
 // Weight gradient partials: part[g][tap][co][ci] = sum over the pixels p of
@@ -335,16 +396,19 @@ __global__ void k_weights_bf16(const float* __restrict__ w, int cout, int cin, u
 inline unsigned blocks(int64_t n, int bs) { return static_cast<unsigned>((n + bs - 1) / bs); }
 
 // Output channels in passes of at most 128 (Cout a multiple of 32). A wave
-// takes 4 pixel tiles for 32 output channels, 2 for more (rows narrower than
-// that: 1), so each weight fragment feeds 2 to 4 products.
+// takes 4 pixel tiles for 32 output channels, 2 for more, so each weight
+// fragment feeds 2 to 4 products; fewer where rows are narrower than that
+// or where the launch would have fewer than 1024 workgroups (the small deep
+// levels: there, parallelism matters more than weight reuse).
 void launchConv(SrcDev s0, SrcDev s1, int N, int H, int W, int Cin, int Cout, const uint16_t* w, int flip,
                 const float* bias, int relu, uint16_t* outBf, float* outF, hipStream_t st) {
+  const int64_t rowTiles = static_cast<int64_t>(N) * ((H + kConvRows - 1) / kConvRows);
   for (int co = 0; co < Cout; co += 128) {
     const int nt = std::min(8, (Cout - co) / 16);
     int np = nt <= 2 ? 4 : 2;
-    while (np > 1 && W < 16 * np) np /= 2;
-    const int64_t waves = static_cast<int64_t>(N) * H * ((W + 16 * np - 1) / (16 * np));
-    const dim3 g(blocks(waves * 64, 256)), b(256);
+    while (np > 1 && (W < 16 * np || rowTiles * ((W + 16 * np - 1) / (16 * np)) < 1024)) np /= 2;
+    const int64_t wgs = rowTiles * ((W + 16 * np - 1) / (16 * np));
+    const dim3 g(static_cast<unsigned>(wgs)), b(256);
 #define UBG_CONV(K, P) \
   hipLaunchKernelGGL((k_conv3x3<K, P>), g, b, 0, st, s0, s1, N, H, W, Cin, Cout, co, w, flip, bias, relu, outBf, outF)
     if (np == 4) {
