@@ -31,8 +31,10 @@ __device__ __forceinline__ uint16_t f2bf(float f) {
 }
 
 // A convolution operand: a bf16 NHWC tensor of C channels and H x W pixels,
-// read directly, 2x2-averaged (its pixels are twice the convolution's) or
-// nearest-upsampled (half the convolution's).
+// read directly (mode 0), 2x2-averaged (1: its pixels are twice the
+// convolution's) or nearest-upsampled (2: half the convolution's); mode 3:
+// the fp32 input features (kUnetFeatures per pixel at p) read as the bf16
+// input padded with zeros to C channels.
 struct SrcDev {
   const uint16_t* p;
   int C, mode, H, W;
@@ -64,6 +66,13 @@ __device__ __forceinline__ uint4 avg4(uint4 a, uint4 b, uint4 c, uint4 d) {
 // 8 consecutive channels c .. c + 7 of the operand at (n, y, x) in the
 // convolution's pixel grid
 __device__ __forceinline__ uint4 load8(const SrcDev& s, int n, int y, int x, int c) {
+  if (s.mode == 3) {
+    const float* f = reinterpret_cast<const float*>(s.p) + ((static_cast<size_t>(n) * s.H + y) * s.W + x) * kUnetFeatures;
+    uint32_t h[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) h[i] = c + i < kUnetFeatures ? f2bf(f[c + i]) : 0u;
+    return make_uint4(h[0] | h[1] << 16, h[2] | h[3] << 16, h[4] | h[5] << 16, h[6] | h[7] << 16);
+  }
   if (s.mode == 0)
     return *reinterpret_cast<const uint4*>(s.p + ((static_cast<size_t>(n) * s.H + y) * s.W + x) * s.C + c);
   if (s.mode == 2)
@@ -108,7 +117,8 @@ template <int NT, int NP>
 __global__ __launch_bounds__(256) void k_conv3x3(SrcDev s0, SrcDev s1, int N, int H, int W, int Cin, int Cout,
                                                  int coBase, const uint16_t* __restrict__ w, int flip,
                                                  const float* __restrict__ bias, int relu,
-                                                 uint16_t* __restrict__ outBf, float* __restrict__ outF) {
+                                                 uint16_t* __restrict__ outBf, float* __restrict__ outF,
+                                                 const float* __restrict__ resid, float* __restrict__ pred) {
   constexpr int S = 16 * NP;        // pixels per row of the tile
   constexpr int PW = S + 2;         // patch width
   constexpr int PU = (kConvRows + 2) * PW * 4;  // patch size in 16-byte units (32 channels)
@@ -167,7 +177,7 @@ __global__ __launch_bounds__(256) void k_conv3x3(SrcDev s0, SrcDev s1, int N, in
       }
     }
   }
-  if (outF) {
+  if (outF || pred) {
     if (y >= H) return;
 #pragma unroll
     for (int q = 0; q < NP; ++q)
@@ -181,7 +191,11 @@ __global__ __launch_bounds__(256) void k_conv3x3(SrcDev s0, SrcDev s1, int N, in
           if (x >= W) continue;
           float v = acc[q][j][r] + bb;
           if (relu) v = fmaxf(v, 0.0f);
-          outF[((static_cast<size_t>(n) * H + y) * W + x) * Cout + co] = v;
+          const size_t pix = (static_cast<size_t>(n) * H + y) * W + x;
+          if (!pred)
+            outF[pix * Cout + co] = v;
+          else if (co < 3)  // the prediction: log(1 + E) of the input + the first 3 channels
+            pred[pix * 3 + co] = v + resid[pix * kUnetFeatures + co];
         }
       }
     return;
@@ -328,16 +342,6 @@ __global__ void k_unet_input(int64_t P, const float* __restrict__ feat, uint16_t
   act[i] = c < kUnetFeatures ? f2bf(feat[p * kUnetFeatures + c]) : static_cast<uint16_t>(0);
 }
 
-// prediction = output conv's first 3 channels + the input's log(1 + E)
-__global__ void k_unet_output(int64_t P, const float* __restrict__ outF, const float* __restrict__ feat,
-                              float* __restrict__ pred) {
-  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i >= P * 3) return;
-  const int64_t p = i / 3;
-  const int c = static_cast<int>(i % 3);
-  pred[i] = outF[p * kUnetOutPad + c] + feat[p * kUnetFeatures + c];
-}
-
 // L1 loss: per-block partial sums of |pred - target| and the output gradient
 // sign(pred - target) / count on the 3 used channels
 __global__ void k_unet_loss(int64_t P, const float* __restrict__ outF, const float* __restrict__ feat,
@@ -400,8 +404,10 @@ inline unsigned blocks(int64_t n, int bs) { return static_cast<unsigned>((n + bs
 // fragment feeds 2 to 4 products; fewer where rows are narrower than that
 // or where the launch would have fewer than 1024 workgroups (the small deep
 // levels: there, parallelism matters more than weight reuse).
+// resid / pred: write only the prediction (fp32 output, see k_conv3x3).
 void launchConv(SrcDev s0, SrcDev s1, int N, int H, int W, int Cin, int Cout, const uint16_t* w, int flip,
-                const float* bias, int relu, uint16_t* outBf, float* outF, hipStream_t st) {
+                const float* bias, int relu, uint16_t* outBf, float* outF, hipStream_t st,
+                const float* resid = nullptr, float* pred = nullptr) {
   const int64_t rowTiles = static_cast<int64_t>(N) * ((H + kConvRows - 1) / kConvRows);
   for (int co = 0; co < Cout; co += 128) {
     const int nt = std::min(8, (Cout - co) / 16);
@@ -410,7 +416,8 @@ void launchConv(SrcDev s0, SrcDev s1, int N, int H, int W, int Cin, int Cout, co
     const int64_t wgs = rowTiles * ((W + 16 * np - 1) / (16 * np));
     const dim3 g(static_cast<unsigned>(wgs)), b(256);
 #define UBG_CONV(K, P) \
-  hipLaunchKernelGGL((k_conv3x3<K, P>), g, b, 0, st, s0, s1, N, H, W, Cin, Cout, co, w, flip, bias, relu, outBf, outF)
+  hipLaunchKernelGGL((k_conv3x3<K, P>), g, b, 0, st, s0, s1, N, H, W, Cin, Cout, co, w, flip, bias, relu, outBf, outF, \
+                     resid, pred)
     if (np == 4) {
       UBG_CONV(2, 4);
     } else if (np == 2) {
@@ -613,13 +620,17 @@ hipError_t Unet::forward(int n, int h, int w, const float* feat, float* out, hip
   if (e != hipSuccess) return e;
   syncBf16(st);
   const int64_t P0 = static_cast<int64_t>(n) * h * w;
-  hipLaunchKernelGGL(k_unet_input, dim3(blocks(P0 * kUnetInPad, 256)), dim3(256), 0, st, P0, feat, actBf_[0]);
+  // inference reads the features directly in the first convolution and
+  // writes the prediction from the last; training keeps the padded input
+  // (the weight gradient reads it) and the full output (the loss reads it)
+  if (!out) hipLaunchKernelGGL(k_unet_input, dim3(blocks(P0 * kUnetInPad, 256)), dim3(256), 0, st, P0, feat, actBf_[0]);
   for (size_t k = 0; k < convs_.size(); ++k) {
     const Conv& c = convs_[k];
     const int H = h >> c.level, W = w >> c.level;
     auto src = [&](const Src& s) {
       SrcDev d{nullptr, 0, 0, 0, 0};
       if (s.act < 0) return d;
+      if (s.act == 0 && out) return SrcDev{reinterpret_cast<const uint16_t*>(feat), kUnetInPad, 3, h, w};
       const int lv = acts_[s.act].level;
       d.p = actBf_[s.act];
       d.C = acts_[s.act].C;
@@ -628,10 +639,11 @@ hipError_t Unet::forward(int n, int h, int w, const float* feat, float* out, hip
       d.W = w >> lv;
       return d;
     };
-    launchConv(src(c.s0), src(c.s1), n, H, W, c.cin, c.cout, wBf_ + wBfOff_[k], 0,
-               params_ + c.bOff, c.relu, c.out >= 0 ? actBf_[c.out] : nullptr, c.out >= 0 ? nullptr : outF_, st);
+    const bool last = c.out < 0;
+    launchConv(src(c.s0), src(c.s1), n, H, W, c.cin, c.cout, wBf_ + wBfOff_[k], 0, params_ + c.bOff, c.relu,
+               last ? nullptr : actBf_[c.out], last && !out ? outF_ : nullptr, st, last ? feat : nullptr,
+               last ? out : nullptr);
   }
-  if (out) hipLaunchKernelGGL(k_unet_output, dim3(blocks(P0 * 3, 256)), dim3(256), 0, st, P0, outF_, feat, out);
   return hipGetLastError();
 }
 
