@@ -66,12 +66,17 @@ __device__ __forceinline__ uint4 avg4(uint4 a, uint4 b, uint4 c, uint4 d) {
 // 8 consecutive channels c .. c + 7 of the operand at (n, y, x) in the
 // convolution's pixel grid
 __device__ __forceinline__ uint4 load8(const SrcDev& s, int n, int y, int x, int c) {
-  if (s.mode == 3) {
-    const float* f = reinterpret_cast<const float*>(s.p) + ((static_cast<size_t>(n) * s.H + y) * s.W + x) * kUnetFeatures;
-    uint32_t h[8];
-#pragma unroll
-    for (int i = 0; i < 8; ++i) h[i] = c + i < kUnetFeatures ? f2bf(f[c + i]) : 0u;
-    return make_uint4(h[0] | h[1] << 16, h[2] | h[3] << 16, h[4] | h[5] << 16, h[6] | h[7] << 16);
+  if (s.mode == 3) {  // channels c .. c + 7 of 10 (c = 0, 8, 16, 24); 8-byte aligned pairs
+    const float2* f = reinterpret_cast<const float2*>(reinterpret_cast<const float*>(s.p) +
+                                                      ((static_cast<size_t>(n) * s.H + y) * s.W + x) * kUnetFeatures);
+    if (c >= 16) return make_uint4(0u, 0u, 0u, 0u);
+    if (c == 8) {
+      const float2 a = f[4];
+      return make_uint4(f2bf(a.x) | static_cast<uint32_t>(f2bf(a.y)) << 16, 0u, 0u, 0u);
+    }
+    const float2 a = f[0], b = f[1], d = f[2], e = f[3];
+    return make_uint4(f2bf(a.x) | static_cast<uint32_t>(f2bf(a.y)) << 16, f2bf(b.x) | static_cast<uint32_t>(f2bf(b.y)) << 16,
+                      f2bf(d.x) | static_cast<uint32_t>(f2bf(d.y)) << 16, f2bf(e.x) | static_cast<uint32_t>(f2bf(e.y)) << 16);
   }
   if (s.mode == 0)
     return *reinterpret_cast<const uint4*>(s.p + ((static_cast<size_t>(n) * s.H + y) * s.W + x) * s.C + c);

@@ -156,12 +156,13 @@ def evaluate(dev: Device, net: Optional[Unet], seeds: List[int], size: int = 512
             ref = bs.render(sc, gi_options(size, size, ref_spp, seed=9000 + s)).color
 
             def timed(opt, **kw):
-                # wall time of the whole frame call and the device's frame
-                # window (trace + shade + gather + denoise + fog + resolve)
+                # wall time of the whole frame call (colour copied to the
+                # host) and the device's frame window (trace + shade +
+                # gather + denoise + fog + resolve)
                 best, dev_ms, f = None, None, None
                 for _ in range(repeats):
                     t = time.perf_counter()
-                    f = bs.render(sc, opt, **kw)
+                    f = bs.render(sc, opt, color_only=True, **kw)
                     ms = (time.perf_counter() - t) * 1e3
                     best = ms if best is None else min(best, ms)
                     d = (f.render_seconds + f.post_seconds) * 1e3

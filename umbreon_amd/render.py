@@ -132,8 +132,10 @@ class BuiltScene:
                want_ids: bool = False, want_ao: bool = False, count_visits: bool = False,
                kernel_times: bool = False, outputs: bool = True,
                cancel_flag: Optional[C.c_int32] = None, want_gi_features: bool = False,
-               denoise_net=None) -> FrameResult:
-        """One frame. outputs=False leaves the frame in HBM (no host copy).
+               denoise_net=None, color_only: bool = False) -> FrameResult:
+        """One frame. outputs=False leaves the frame in HBM (no host copy);
+        color_only copies the colour alone (depth, view z and the GI AOVs
+        stay on the device).
         want_gi_features returns the learned denoiser's input features of the
         gathered irradiance (hi-res H x W x 10); denoise_net (an
         umbreon_amd.denoise.Unet) runs the learned denoiser in place of the
@@ -147,9 +149,10 @@ class BuiltScene:
         ss = max(1, int(opt.supersample))
         hw, hh = opt.width * ss, opt.height * ss
         ow, oh = (hw, hh) if hires_only else (opt.width, opt.height)
-        color = np.zeros((oh, ow, 4), np.float32) if outputs else None
-        depth = np.zeros((hh, hw), np.float32) if outputs else None
-        view_z = np.zeros((hh, hw), np.float32) if outputs else None
+        color = (np.empty if color_only else np.zeros)((oh, ow, 4), np.float32) if outputs else None
+        aovs = outputs and not color_only
+        depth = np.zeros((hh, hw), np.float32) if aovs else None
+        view_z = np.zeros((hh, hw), np.float32) if aovs else None
         hit_id = np.zeros((hh, hw), np.uint32) if want_ids else None
         ao = np.zeros((hh, hw), np.float32) if want_ao else None
         fr = capi.Frame()
@@ -161,8 +164,8 @@ class BuiltScene:
         adaptive = opt.aa_mode == 1 and (ss > 1 or opt.aa_depth > 1)
         aa_mask = np.zeros((opt.height, opt.width), np.float32) if opt.aa_debug and adaptive else None
         fr.aa_mask = aa_mask.ctypes.data if aa_mask is not None else None
-        indirect = np.zeros((hh, hw, 3), np.float32) if opt.gi and outputs else None
-        gi_occ = np.zeros((hh, hw), np.float32) if opt.gi and outputs else None
+        indirect = np.zeros((hh, hw, 3), np.float32) if opt.gi and aovs else None
+        gi_occ = np.zeros((hh, hw), np.float32) if opt.gi and aovs else None
         fr.gi_indirect = indirect.ctypes.data if indirect is not None else None
         fr.gi_occlusion = gi_occ.ctypes.data if gi_occ is not None else None
         feats = np.zeros((hh, hw, 10), np.float32) if want_gi_features else None
