@@ -97,6 +97,27 @@ __device__ __forceinline__ uint16_t load1(const SrcDev& s, int n, int y, int x, 
   return f2bf(sum * 0.25f);
 }
 
+// Direct global -> LDS loads (global_load_lds_dwordx4: lane l of the wave
+// writes 16 bytes at the wave-uniform LDS base + 16 l) of a unit array:
+// dst[u] for u = threadIdx.x + 256 k < count, with src(u) the global
+// address of unit u, or zero where src(u) is null. Nothing waits for a
+// register, so every load of the staging is in flight at once.
+template <int COUNT, typename Src>
+__device__ __forceinline__ void stageDma(uint4* dst, Src src) {
+  const int wv = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < (COUNT + 255) / 256; ++k) {
+    const int u = static_cast<int>(threadIdx.x) + 256 * k;
+    if (u < COUNT) {
+      const uint16_t* g = src(u);
+      if (g)
+        __builtin_amdgcn_global_load_lds(g, dst + 256 * k + 64 * wv, 16, 0, 0);
+      else
+        dst[u] = make_uint4(0u, 0u, 0u, 0u);
+    }
+  }
+}
+
 // 3x3 convolution, zero padding: out[p, co] = sum over taps and input
 // channels of in[p + tap, ci] * w[tap][co][ci] (flip: w[8 - tap]), plus bias
 // and ReLU when asked; output bf16 or fp32. Input channels [0, s0.C) come from
@@ -148,24 +169,34 @@ __global__ __launch_bounds__(256) void k_conv3x3(SrcDev s0, SrcDev s1, int N, in
     const SrcDev& s = first ? s0 : s1;
     const int cBase = first ? ci0 : ci0 - s0.C;
     if (ci0 > 0) __syncthreads();  // the previous chunk's patch and weights have been read
-    for (int u = threadIdx.x; u < PU; u += 256) {
-      const int c8 = u & 3, pix = u >> 2;
-      const int px = pix % PW, pr = pix / PW;
-      const int yy = y0 + pr - 1, xx = x0 + px - 1;
-      uint4 v = make_uint4(0u, 0u, 0u, 0u);
-      if (yy >= 0 && yy < H && xx >= 0 && xx < W) v = load8(s, n, yy, xx, cBase + 8 * c8);
-      lds[u] = v;
+    if (s.mode == 0 || s.mode == 2) {  // plain loads: straight into LDS
+      const int sh = s.mode == 2 ? 1 : 0;
+      stageDma<PU>(lds, [&](int u) -> const uint16_t* {
+        const int c8 = u & 3, pix = u >> 2;
+        const int px = pix % PW, pr = pix / PW;
+        const int yy = y0 + pr - 1, xx = x0 + px - 1;
+        if (yy < 0 || yy >= H || xx < 0 || xx >= W) return nullptr;
+        return s.p + ((static_cast<size_t>(n) * s.H + (yy >> sh)) * s.W + (xx >> sh)) * s.C + cBase + 8 * c8;
+      });
+    } else {  // pooled or converted: through registers
+      for (int u = threadIdx.x; u < PU; u += 256) {
+        const int c8 = u & 3, pix = u >> 2;
+        const int px = pix % PW, pr = pix / PW;
+        const int yy = y0 + pr - 1, xx = x0 + px - 1;
+        uint4 v = make_uint4(0u, 0u, 0u, 0u);
+        if (yy >= 0 && yy < H && xx >= 0 && xx < W) v = load8(s, n, yy, xx, cBase + 8 * c8);
+        lds[u] = v;
+      }
     }
 #pragma unroll
     for (int dy = 0; dy < 3; ++dy) {
       if (dy > 0) __syncthreads();  // the previous tap row's weights have been read
       // wl[(t * 16 NT + c) * 4 + k8] = w[tap 3 dy + t (flipped)][coBase + c][ci0 + 8 k8 ..]
-      for (int u = threadIdx.x; u < WU; u += 256) {
+      stageDma<WU>(wl, [&](int u) -> const uint16_t* {
         const int k8 = u & 3, c = (u >> 2) % (16 * NT), t = (u >> 2) / (16 * NT);
         const int tap = 3 * dy + t;
-        wl[u] = *reinterpret_cast<const uint4*>(w + (static_cast<size_t>(flip ? 8 - tap : tap) * Cout + coBase + c) * Cin +
-                                                ci0 + 8 * k8);
-      }
+        return w + (static_cast<size_t>(flip ? 8 - tap : tap) * Cout + coBase + c) * Cin + ci0 + 8 * k8;
+      });
       __syncthreads();
 #pragma unroll
       for (int dx = 0; dx < 3; ++dx) {
