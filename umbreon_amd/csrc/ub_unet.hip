@@ -131,11 +131,16 @@ __device__ __forceinline__ void stageDma(uint4* dst, Src src) {
 // weight fragment feeding NP matrix products. A bf16 result goes back
 // through LDS so each lane stores 16 consecutive bytes.
 constexpr int kConvRows = 4;
-// LDS of one instantiation: patch + one tap row of weights, or the bf16
-// output tiles, whichever is larger
+// Tap rows of weights staged at once: all three where patch + weights stay
+// within 53 KB (3 workgroups per CU), else one.
+constexpr int convTapRows(int nt, int np) {
+  return (kConvRows + 2) * (16 * np + 2) * 64 + 9 * 16 * nt * 64 <= 53 * 1024 ? 3 : 1;
+}
+// LDS of one instantiation: patch + staged weights, or the bf16 output
+// tiles, whichever is larger
 constexpr int convLdsBytes(int nt, int np) {
-  return (kConvRows + 2) * (16 * np + 2) * 64 + 3 * 16 * nt * 64 > kConvRows * 16 * np * nt * 32
-             ? (kConvRows + 2) * (16 * np + 2) * 64 + 3 * 16 * nt * 64
+  return (kConvRows + 2) * (16 * np + 2) * 64 + 3 * convTapRows(nt, np) * 16 * nt * 64 > kConvRows * 16 * np * nt * 32
+             ? (kConvRows + 2) * (16 * np + 2) * 64 + 3 * convTapRows(nt, np) * 16 * nt * 64
              : kConvRows * 16 * np * nt * 32;
 }
 
@@ -148,7 +153,8 @@ __global__ __launch_bounds__(256) void k_conv3x3(SrcDev s0, SrcDev s1, int N, in
   constexpr int S = 16 * NP;        // pixels per row of the tile
   constexpr int PW = S + 2;         // patch width
   constexpr int PU = (kConvRows + 2) * PW * 4;  // patch size in 16-byte units (32 channels)
-  constexpr int WU = 3 * 16 * NT * 4;           // one tap row of weights, 16-byte units
+  constexpr int TR = convTapRows(NT, NP);       // tap rows per weight stage
+  constexpr int WU = 3 * TR * 16 * NT * 4;      // one weight stage, 16-byte units
   constexpr int kLdsBytes = convLdsBytes(NT, NP);
   static_assert((PU + WU) * 16 <= kLdsBytes && kConvRows * S * NT * 16 * 2 <= kLdsBytes, "LDS buffer too small");
   __shared__ uint4 lds[kLdsBytes / 16];
@@ -189,31 +195,54 @@ __global__ __launch_bounds__(256) void k_conv3x3(SrcDev s0, SrcDev s1, int N, in
       }
     }
 #pragma unroll
-    for (int dy = 0; dy < 3; ++dy) {
-      if (dy > 0) __syncthreads();  // the previous tap row's weights have been read
-      // wl[(t * 16 NT + c) * 4 + k8] = w[tap 3 dy + t (flipped)][coBase + c][ci0 + 8 k8 ..]
+    for (int d0 = 0; d0 < 3; d0 += TR) {
+      if (d0 > 0) __syncthreads();  // the previous stage's weights have been read
+      // wl[(t * 16 NT + c) * 4 + k8] = w[tap 3 d0 + t (flipped)][coBase + c][ci0 + 8 k8 ..]
       stageDma<WU>(wl, [&](int u) -> const uint16_t* {
         const int k8 = u & 3, c = (u >> 2) % (16 * NT), t = (u >> 2) / (16 * NT);
-        const int tap = 3 * dy + t;
+        const int tap = 3 * d0 + t;
         return w + (static_cast<size_t>(flip ? 8 - tap : tap) * Cout + coBase + c) * Cin + ci0 + 8 * k8;
       });
       __syncthreads();
 #pragma unroll
-      for (int dx = 0; dx < 3; ++dx) {
+      for (int t = 0; t < 3 * TR; ++t) {
+        const int dy = d0 + t / 3, dx = t % 3;
         bf16x8 a[NP];
 #pragma unroll
         for (int q = 0; q < NP; ++q)
           a[q] = __builtin_bit_cast(bf16x8, lds[((wv + dy) * PW + 16 * q + row + dx) * 4 + grp]);
 #pragma unroll
         for (int j = 0; j < NT; ++j) {
-          const bf16x8 b = __builtin_bit_cast(bf16x8, wl[(dx * 16 * NT + 16 * j + row) * 4 + grp]);
+          const bf16x8 b = __builtin_bit_cast(bf16x8, wl[(t * 16 * NT + 16 * j + row) * 4 + grp]);
 #pragma unroll
           for (int q = 0; q < NP; ++q) acc[q][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[q], b, acc[q][j], 0, 0, 0);
         }
       }
     }
   }
-  if (outF || pred) {
+  if (pred) {  // the prediction: log(1 + E) of the input + output channels 0..2
+    if (y >= H || coBase != 0 || row >= 3) return;
+    const float bb = bias ? bias[row] : 0.0f;
+    float rv[NP][4];
+#pragma unroll
+    for (int q = 0; q < NP; ++q)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int x = min(x0 + 16 * q + 4 * grp + r, W - 1);
+        rv[q][r] = resid[((static_cast<size_t>(n) * H + y) * W + x) * kUnetFeatures + row];
+      }
+#pragma unroll
+    for (int q = 0; q < NP; ++q)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int x = x0 + 16 * q + 4 * grp + r;
+        float v = acc[q][0][r] + bb;
+        if (relu) v = fmaxf(v, 0.0f);
+        if (x < W) pred[((static_cast<size_t>(n) * H + y) * W + x) * 3 + row] = v + rv[q][r];
+      }
+    return;
+  }
+  if (outF) {
     if (y >= H) return;
 #pragma unroll
     for (int q = 0; q < NP; ++q)
@@ -227,11 +256,7 @@ __global__ __launch_bounds__(256) void k_conv3x3(SrcDev s0, SrcDev s1, int N, in
           if (x >= W) continue;
           float v = acc[q][j][r] + bb;
           if (relu) v = fmaxf(v, 0.0f);
-          const size_t pix = (static_cast<size_t>(n) * H + y) * W + x;
-          if (!pred)
-            outF[pix * Cout + co] = v;
-          else if (co < 3)  // the prediction: log(1 + E) of the input + the first 3 channels
-            pred[pix * 3 + co] = v + resid[pix * kUnetFeatures + co];
+          outF[((static_cast<size_t>(n) * H + y) * W + x) * Cout + co] = v;
         }
       }
     return;
