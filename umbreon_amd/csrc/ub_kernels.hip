@@ -1826,19 +1826,35 @@ __device__ __forceinline__ bool closestHitWave(const SceneView& s, bool live, V3
 // fraction. The lanes advance as a state machine with one path segment per
 // iteration, so every iteration's gather rays go through one wave-wide
 // closest-hit traversal (node and primitive steps voted across the wave).
+// Pixels are handed out as work: a lane that has finished its pixel goes
+// idle, and once kGiRefill lanes of the wave are idle (or all of them), the
+// wave takes that many consecutive list entries from a partition's cursor
+// (one atomic), moving on to the next partition when one is used up; a wave
+// leaves when every partition is used up and none of its lanes has work.
+constexpr int kGiRefill = 16;
+
 template <bool kCount, int kLayout>
 __global__ __launch_bounds__(kTraceBlock) void k_gi_gather(FrameConst fc, SceneView s, FrameBufs b, GiField gf) {
-  const int tid = blockIdx.x * blockDim.x + threadIdx.x;
-  const int part = tid / b.partCap;
-  bool live = part < kQueueParts && static_cast<uint32_t>(tid % b.partCap) < queueFill(b.counters, gf.fillBase, part);
+  bool live = false;
   uint32_t nGi = 0, nNee = 0, nv = 0, nt = 0;
   // per-pixel constants
   uint32_t pix = 0, seed = 0, sBase = 0, sSeed = 0;
   V3 N = mk(0, 0, 1), Ng = N, O = mk(0, 0, 0);
   float eps = 0.0f, cpx = 0.0f, cpy = 0.0f;
   const int spp = gf.spp;
-  if (live) {
-    pix = gf.list[tid];
+  // per-sample path state
+  int si = 0, bo = 0, nOcc = 0;
+  bool inSample = false;
+  uint32_t s0 = 0, s1 = 0;
+  V3 org = O, wi = mk(0, 0, 1), tp = mk(1, 1, 1), L = mk(0, 0, 0), Esum = mk(0, 0, 0);
+  float tnear = eps;
+  auto beginPixel = [&](uint32_t slot) {
+    live = true;
+    si = 0;
+    nOcc = 0;
+    inSample = false;
+    Esum = mk(0.0f, 0.0f, 0.0f);
+    pix = gf.list[slot];
     const V3 P = ld3(gf.pos[pix]);
     N = ld3(gf.nrm[pix]);
     // the render grid's G-buffer has no geometric normal: the horizon test
@@ -1865,13 +1881,11 @@ __global__ __launch_bounds__(kTraceBlock) void k_gi_gather(FrameConst fc, SceneV
       cpx = u32ToUnorm(c0);
       cpy = u32ToUnorm(c1);
     }
-  }
-  // per-sample path state
-  int si = 0, bo = 0, nOcc = 0;
-  bool inSample = false;
-  uint32_t s0 = 0, s1 = 0;
-  V3 org = O, wi = mk(0, 0, 1), tp = mk(1, 1, 1), L = mk(0, 0, 0), Esum = mk(0, 0, 0);
-  float tnear = eps;
+  };
+  const int curBase = gf.fillBase == kGiFill ? kGiCursor : kGiPatchCursor;
+  uint32_t part = ((blockIdx.x * blockDim.x + threadIdx.x) >> 6) % kQueueParts;
+  int tried = 0;
+  bool drained = false;  // every partition is used up
   auto endSample = [&]() {
     if (fc.giClamp > 0.0f) {
       const float lum = 0.2126f * L.x + 0.7152f * L.y + 0.0722f * L.z;
@@ -1889,6 +1903,32 @@ __global__ __launch_bounds__(kTraceBlock) void k_gi_gather(FrameConst fc, SceneV
     inSample = false;
   };
   for (;;) {
+    const unsigned long long idle = __ballot(!live);
+    const int nIdle = __popcll(idle);
+    if (!drained && (nIdle >= kGiRefill || nIdle == 64)) {
+      uint32_t base = 0, take = 0;
+      while (take == 0u && !drained) {
+        const uint32_t fill = queueFill(b.counters, gf.fillBase, static_cast<int>(part));
+        unsigned long long* cur = &b.counters[curBase + kLineWords * part];
+        uint32_t got = fill;
+        if (__lane_id() == 0) {
+          // a plain read first: a used-up partition costs no atomic
+          const uint64_t seen = __hip_atomic_load(cur, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          if (seen < fill) got = static_cast<uint32_t>(atomicAdd(cur, static_cast<unsigned long long>(nIdle)));
+        }
+        got = __shfl(got, 0, 64);
+        if (got < fill) {
+          base = got;
+          take = min(static_cast<uint32_t>(nIdle), fill - got);
+        } else if (++tried == kQueueParts) {
+          drained = true;
+        } else {
+          part = (part + 1u) % kQueueParts;
+        }
+      }
+      const uint32_t rank = static_cast<uint32_t>(__popcll(idle & ((1ull << __lane_id()) - 1ull)));
+      if (!live && rank < take) beginPixel(part * static_cast<uint32_t>(b.partCap) + base + rank);
+    }
     // lanes between samples start the next one (a below-horizon draw is a
     // sample of L = 0 that traces nothing); a lane whose samples are done
     // stores E
@@ -1925,7 +1965,10 @@ __global__ __launch_bounds__(kTraceBlock) void k_gi_gather(FrameConst fc, SceneV
       inSample = true;
     }
     const bool ray = live && inSample;
-    if (__ballot(ray) == 0ull) break;
+    if (__ballot(ray) == 0ull) {  // no lane has work: refill, or leave
+      if (drained) break;
+      continue;
+    }
     Hit h;
     const bool hit = closestHitWave<kCount, kLayout>(s, ray, org, wi, tnear, fc.giMaxDist, h, &nv, &nt);
     if (!ray) continue;

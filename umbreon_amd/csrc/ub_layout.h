@@ -237,6 +237,9 @@ constexpr int kAaRepFill = kAaFlagFill + kLineWords;
 constexpr int kGiFill = kAaRepFill + kLineWords;
 // GI rim-patch list (reduced gather grids), partitioned the same way
 constexpr int kGiPatchFill = kGiFill + kLineWords * kQueueParts;
-constexpr int kCounterWords = kGiPatchFill + kLineWords * kQueueParts;
+// gather work cursors, one per partition of the receiver / rim-patch list
+constexpr int kGiCursor = kGiPatchFill + kLineWords * kQueueParts;
+constexpr int kGiPatchCursor = kGiCursor + kLineWords * kQueueParts;
+constexpr int kCounterWords = kGiPatchCursor + kLineWords * kQueueParts;
 
 }  // namespace ubg
