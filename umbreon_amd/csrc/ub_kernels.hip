@@ -1831,7 +1831,7 @@ __device__ __forceinline__ bool closestHitWave(const SceneView& s, bool live, V3
 // wave takes that many consecutive list entries from a partition's cursor
 // (one atomic), moving on to the next partition when one is used up; a wave
 // leaves when every partition is used up and none of its lanes has work.
-constexpr int kGiRefill = 16;
+constexpr int kGiRefill = 2;
 
 template <bool kCount, int kLayout>
 __global__ __launch_bounds__(kTraceBlock) void k_gi_gather(FrameConst fc, SceneView s, FrameBufs b, GiField gf) {
