@@ -74,6 +74,36 @@ __device__ __forceinline__ bool traceClosestWave8(const SceneView& s, bool live,
   return found;
 }
 
+// Any hit over the BVH8 for every lane of a wave at once, voted like
+// traceClosestWave8; each lane's own sequence of node and primitive tests is
+// traverse8's any-hit one, so the answer and the counts are the same.
+template <bool kCount>
+__device__ __forceinline__ bool traceAnyWave8(const SceneView& s, bool live, V3 o, V3 d, float tn, float tf,
+                                              uint32_t* nv, uint32_t* nt) {
+  uint32_t* sb = ubgLds + threadIdx.x;
+  uint32_t* sh = sb + s.stackDepth * blockDim.x;
+  Trav8 t;
+  trav8Init(t, o, d, tn, tf);
+  bool occluded = false;
+  live = live && s.nNodes > 0;
+  while (__ballot(live) != 0ull) {
+    const bool wantPrim = live && t.tHits != 0u;
+    const int nPrim = __popcll(__ballot(wantPrim));
+    const int nNode = __popcll(__ballot(live && !wantPrim));
+    int r = 0;
+    if (2 * nPrim >= nNode) {
+      if (wantPrim) r = trav8Prim<kCount>(s, t, nt);
+    } else {
+      if (live && !wantPrim) r = trav8Node<kCount>(s, t, sb, sh, blockDim.x, nv);
+    }
+    if (r != 0) {
+      live = false;
+      occluded = r > 0;
+    }
+  }
+  return occluded;
+}
+
 template <int kLayout, bool kAny, bool kCount>
 __device__ __forceinline__ bool trace(const SceneView& s, V3 o, V3 d, float tn, float tf, Hit& h, uint32_t* nv,
                                       uint32_t* nt) {
@@ -1817,6 +1847,15 @@ __device__ __forceinline__ bool closestHitWave(const SceneView& s, bool live, V3
   return live && trace<kLayout, false, kCount>(s, o, d, tn, tf, h, nv, nt);
 }
 
+// Any hit for the lanes with `live`, called by every lane of the wave.
+template <bool kCount, int kLayout>
+__device__ __forceinline__ bool anyHitWave(const SceneView& s, bool live, V3 o, V3 d, float tn, float tf, uint32_t* nv,
+                                           uint32_t* nt) {
+  if (kLayout == kLayoutBvh8) return traceAnyWave8<kCount>(s, live, o, d, tn, tf, nv, nt);
+  Hit h;
+  return live && trace<kLayout, true, kCount>(s, o, d, tn, tf, h, nv, nt);
+}
+
 // pt1GatherPoint (pt1_gather.hpp:228-485) for one listed pixel per lane: spp
 // cosine paths from its first hit, each vertex adding its shadow-tested
 // direct light (pt1EvalVertex, :117-212) times the path throughput, Russian
@@ -1971,89 +2010,102 @@ __global__ __launch_bounds__(kTraceBlock) void k_gi_gather(FrameConst fc, SceneV
     }
     Hit h;
     const bool hit = closestHitWave<kCount, kLayout>(s, ray, org, wi, tnear, fc.giMaxDist, h, &nv, &nt);
-    if (!ray) continue;
-    ++nGi;
-    if (!hit) {
-      // irradiance_cache.hpp:188-194
-      const float w = 0.5f * (dot(wi, mk(fc.aoUpAxis[0], fc.aoUpAxis[1], fc.aoUpAxis[2])) + 1.0f);
-      L.x += tp.x * (fc.giEnvScale[0] * (fc.giGround[0] + (fc.giSky[0] - fc.giGround[0]) * w));
-      L.y += tp.y * (fc.giEnvScale[1] * (fc.giGround[1] + (fc.giSky[1] - fc.giGround[1]) * w));
-      L.z += tp.z * (fc.giEnvScale[2] * (fc.giGround[2] + (fc.giSky[2] - fc.giGround[2]) * w));
-      endSample();
-      continue;
-    }
-    if (bo == 1) ++nOcc;
-    // pt1EvalVertex: the hit as a path vertex
-    const uint32_t kind = h.bits >> kKindShift, id = h.bits & kIdMask;
-    V3 Ny, Cy, NgS;
-    float kd, em = 0.0f;
-    bool absorbed = false;
-    if (kind == 0u) {
-      const uint4 tri = reinterpret_cast<const uint4*>(s.triIdx)[id];
-      const float4* vn = reinterpret_cast<const float4*>(s.vtxNrm);
-      const float4* vc = reinterpret_cast<const float4*>(s.vtxCol);
-      const float4 n0 = vn[tri.x], n1 = vn[tri.y], n2 = vn[tri.z];
-      const float4 c0 = vc[tri.x], c1 = vc[tri.y], c2 = vc[tri.z];
-      const float w0 = 1.0f - h.u - h.v;
-      Ny = normalize(mk(w0 * n0.x + h.u * n1.x + h.v * n2.x, w0 * n0.y + h.u * n1.y + h.v * n2.y,
-                        w0 * n0.z + h.u * n1.z + h.v * n2.z));
-      Cy = mk(w0 * c0.x + h.u * c1.x + h.v * c2.x, w0 * c0.y + h.u * c1.y + h.v * c2.y,
-              w0 * c0.z + h.u * c1.z + h.v * c2.z);
-      const DevMaterial& mm = s.meshMats[tri.w & kInfoMatMask];
-      kd = diffuseWeight(mm);
-      if (fc.giPt2 && fc.giEmissive) em = mm.emission;
-      NgS = h.ng;
-    } else {
-      float4 col;
-      uint32_t info;
-      if (kind == 1u) {
-        col = reinterpret_cast<const float4*>(s.sphColor)[id];
-        info = s.sphInfo[id];
-      } else if (kind == 3u) {
-        col = reinterpret_cast<const float4*>(s.coneColor)[id];
-        info = s.coneInfo[id];
+    // the hit: an escape adds the environment, an outline decoration absorbs
+    // the path, anything else is a path vertex (vert)
+    bool vert = false;
+    V3 Ny = mk(0.0f, 0.0f, 1.0f), Cy = mk(0.0f, 0.0f, 0.0f), NgS = Ny, Py = mk(0.0f, 0.0f, 0.0f);
+    float kd = 0.0f, em = 0.0f, epsY = 0.0f;
+    if (ray) {
+      ++nGi;
+      if (!hit) {
+        // irradiance_cache.hpp:188-194
+        const float w = 0.5f * (dot(wi, mk(fc.aoUpAxis[0], fc.aoUpAxis[1], fc.aoUpAxis[2])) + 1.0f);
+        L.x += tp.x * (fc.giEnvScale[0] * (fc.giGround[0] + (fc.giSky[0] - fc.giGround[0]) * w));
+        L.y += tp.y * (fc.giEnvScale[1] * (fc.giGround[1] + (fc.giSky[1] - fc.giGround[1]) * w));
+        L.z += tp.z * (fc.giEnvScale[2] * (fc.giGround[2] + (fc.giSky[2] - fc.giGround[2]) * w));
+        endSample();
       } else {
-        col = reinterpret_cast<const float4*>(s.rndColor)[id];
-        info = s.rndInfo[id];
+        if (bo == 1) ++nOcc;
+        // pt1EvalVertex: the hit as a path vertex
+        const uint32_t kind = h.bits >> kKindShift, id = h.bits & kIdMask;
+        bool absorbed = false;
+        if (kind == 0u) {
+          const uint4 tri = reinterpret_cast<const uint4*>(s.triIdx)[id];
+          const float4* vn = reinterpret_cast<const float4*>(s.vtxNrm);
+          const float4* vc = reinterpret_cast<const float4*>(s.vtxCol);
+          const float4 n0 = vn[tri.x], n1 = vn[tri.y], n2 = vn[tri.z];
+          const float4 c0 = vc[tri.x], c1 = vc[tri.y], c2 = vc[tri.z];
+          const float w0 = 1.0f - h.u - h.v;
+          Ny = normalize(mk(w0 * n0.x + h.u * n1.x + h.v * n2.x, w0 * n0.y + h.u * n1.y + h.v * n2.y,
+                            w0 * n0.z + h.u * n1.z + h.v * n2.z));
+          Cy = mk(w0 * c0.x + h.u * c1.x + h.v * c2.x, w0 * c0.y + h.u * c1.y + h.v * c2.y,
+                  w0 * c0.z + h.u * c1.z + h.v * c2.z);
+          const DevMaterial& mm = s.meshMats[tri.w & kInfoMatMask];
+          kd = diffuseWeight(mm);
+          if (fc.giPt2 && fc.giEmissive) em = mm.emission;
+          NgS = h.ng;
+        } else {
+          float4 col;
+          uint32_t info;
+          if (kind == 1u) {
+            col = reinterpret_cast<const float4*>(s.sphColor)[id];
+            info = s.sphInfo[id];
+          } else if (kind == 3u) {
+            col = reinterpret_cast<const float4*>(s.coneColor)[id];
+            info = s.coneInfo[id];
+          } else {
+            col = reinterpret_cast<const float4*>(s.rndColor)[id];
+            info = s.rndInfo[id];
+          }
+          absorbed = (info & kInfoFromEdge) != 0u;  // outline decoration absorbs the path
+          Ny = safeNormalize(h.ng);
+          Cy = mk(col.x, col.y, col.z);
+          const DevMaterial& pm = s.primMats[info & kInfoMatMask];
+          kd = diffuseWeight(pm);
+          if (fc.giPt2 && fc.giEmissive) em = pm.emission;
+          NgS = Ny;
+        }
+        if (absorbed) {
+          endSample();
+        } else {
+          if (dot(Ny, wi) > 0.0f) Ny = neg(Ny);
+          Py = mk(org.x + wi.x * h.t, org.y + wi.y * h.t, org.z + wi.z * h.t);
+          epsY = selfIntersectEps(Py, wi, h.t);
+          vert = true;
+        }
       }
-      absorbed = (info & kInfoFromEdge) != 0u;  // outline decoration absorbs the path
-      Ny = safeNormalize(h.ng);
-      Cy = mk(col.x, col.y, col.z);
-      const DevMaterial& pm = s.primMats[info & kInfoMatMask];
-      kd = diffuseWeight(pm);
-      if (fc.giPt2 && fc.giEmissive) em = pm.emission;
-      NgS = Ny;
     }
-    if (absorbed) {
-      endSample();
-      continue;
-    }
-    if (dot(Ny, wi) > 0.0f) Ny = neg(Ny);
-    const V3 Py = mk(org.x + wi.x * h.t, org.y + wi.y * h.t, org.z + wi.z * h.t);
-    const float epsY = selfIntersectEps(Py, wi, h.t);
+    // next-event estimation at the vertices: every lane of the wave takes
+    // part in each light's voted shadow traversal (lanes without a vertex or
+    // facing away from the light as idle lanes)
     V3 E = mk(0.0f, 0.0f, 0.0f);
     for (int li = 0; li < fc.nLights; ++li) {
       const DevLight& l = fc.lights[li];
       const V3 Ld = mk(l.L[0], l.L[1], l.L[2]);
-      const float ndl = dot(Ny, Ld);
-      if (ndl <= 0.0f) continue;
-      // computeShadow's single hard ray (secondary_rays.hpp:118-124), or
-      // (pt2, an area light) one cone-jittered ray from the path's stream
-      V3 ng = dot(NgS, Ny) < 0.0f ? neg(NgS) : NgS;
-      ng = safeNormalize(ng, Ny);
-      const V3 So = Py + ng * epsY;
-      V3 dir = Ld;
-      if (fc.giPt2 && l.radius > 0.0f) {
-        tea2(s0, s1);
-        dir = sampleLightDir(Ld, l.tanRadius, u32ToUnorm(s0), u32ToUnorm(s1));
+      const float ndl = vert ? dot(Ny, Ld) : 0.0f;
+      const bool shadowRay = vert && ndl > 0.0f;
+      V3 So = mk(0.0f, 0.0f, 0.0f), dir = Ld;
+      if (shadowRay) {
+        // computeShadow's single hard ray (secondary_rays.hpp:118-124), or
+        // (pt2, an area light) one cone-jittered ray from the path's stream
+        V3 ng = dot(NgS, Ny) < 0.0f ? neg(NgS) : NgS;
+        ng = safeNormalize(ng, Ny);
+        So = Py + ng * epsY;
+        if (fc.giPt2 && l.radius > 0.0f) {
+          tea2(s0, s1);
+          dir = sampleLightDir(Ld, l.tanRadius, u32ToUnorm(s0), u32ToUnorm(s1));
+        }
+        ++nNee;
       }
-      Hit hs;
-      ++nNee;
-      const float sh = trace<kLayout, true, kCount>(s, So, dir, epsY, __builtin_inff(), hs, &nv, &nt) ? 0.0f : 1.0f;
-      E.x += ndl * l.color[0] * sh;
-      E.y += ndl * l.color[1] * sh;
-      E.z += ndl * l.color[2] * sh;
+      const bool occl = anyHitWave<kCount, kLayout>(s, shadowRay, So, dir, epsY, __builtin_inff(), &nv, &nt);
+      if (shadowRay) {
+        const float sh = occl ? 0.0f : 1.0f;
+        E.x += ndl * l.color[0] * sh;
+        E.y += ndl * l.color[1] * sh;
+        E.z += ndl * l.color[2] * sh;
+      }
     }
+    if (!vert) continue;
     // radiance + emitted (pt1EvalVertex's split; emitted is 0 unless pt2
     // transports the vertex's emission)
     const V3 emd = em > 0.0f ? mk(em * Cy.x, em * Cy.y, em * Cy.z) : mk(0.0f, 0.0f, 0.0f);
