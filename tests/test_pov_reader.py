@@ -441,3 +441,58 @@ def test_load_pov_scene(tmp_path):
     with pytest.raises(pov.PovError, match="_scene"):
         (tmp_path / "n.pov").write_text("camera { }")
         pov.load_pov_scene(str(tmp_path / "n.pov"))
+
+
+def _exp_to_rough(e: float) -> float:
+    return math.sqrt(min(math.sqrt(2.0 / (e + 2.0)), 1.0))
+
+
+def test_material_convert():
+    """The rows of test_material_convert.cpp:27-161."""
+    conv = pov.to_principled_material
+    out = conv(Material(ambient=0.15, emission=2.0))
+    assert out.model == 1 and (out.ambient, out.diffuse, out.emission) == pytest.approx((0.15, 0.8, 2.0))
+    assert out.pbr_specular == 0.0 and out.pbr_metallic == 0.0
+    out = conv(Material(specular=0.4, roughness=0.02))  # Blinn exponent 50
+    assert out.pbr_specular == pytest.approx(0.4) and out.pbr_metallic == 0.0
+    assert out.pbr_roughness == pytest.approx(_exp_to_rough(50.0), abs=1e-4)
+    out = conv(Material(phong=0.8, phong_size=50.0))
+    assert out.pbr_specular == pytest.approx(0.8)
+    assert out.pbr_roughness == pytest.approx(_exp_to_rough(50.0), abs=1e-4)
+    toon = Material(ambient=0.3, diffuse=0.5, brilliance=0.0, phong=10000.0, phong_size=50.0)
+    assert pov.is_non_physical_finish(toon) and conv(toon) == toon
+    assert conv(Material(brilliance=0.0)).model == 0
+    assert conv(Material(specular=4.0)).model == 0
+    unlit = Material(ambient=1.0, diffuse=0.0, specular=0.0)
+    assert pov.is_non_physical_finish(unlit) and conv(unlit).model == 0
+    assert conv(Material(metallic=True, brilliance=2.0, specular=0.8, roughness=0.05)).model == 1
+    assert conv(Material(metallic=True, specular=0.8, roughness=0.05)).pbr_metallic == 1.0
+    out = conv(Material(reflection=0.8))
+    assert (out.pbr_metallic, out.pbr_roughness, out.reflection) == pytest.approx((1.0, 0.0, 0.8))
+    out = conv(Material(reflection=0.3, specular=0.6, roughness=0.02))
+    assert out.pbr_metallic == pytest.approx(0.3) and out.pbr_specular == pytest.approx(0.6)
+    assert out.pbr_roughness == pytest.approx(_exp_to_rough(50.0), abs=1e-4)
+    p = Material(model=1, pbr_metallic=0.7, pbr_roughness=0.25)
+    assert conv(p) == p
+    # the Blinn exponent is clamped to [1, 1e6]: roughness 0 -> e = 1e6, 2 -> e = 1
+    assert conv(Material(specular=0.5, roughness=0.0)).pbr_roughness == pytest.approx(_exp_to_rough(1.0e6), rel=1e-5)
+    assert conv(Material(specular=0.5, roughness=2.0)).pbr_roughness == pytest.approx(_exp_to_rough(1.0), rel=1e-6)
+
+
+def test_load_pov_scene_principled(tmp_path):
+    """--material principled: every finish but the edge decoration's."""
+    src = open(os.path.join(HERE, "golden", "pov", "1ab0_scene1.pov")).read()
+    (tmp_path / "s.pov").write_text(src.replace("1ab0_scene1.inc", "s.inc"))
+    (tmp_path / "s.inc").write_text(r'''
+      #declare _t = texture { pigment { rgb 1 } finish { specular 0.4 roughness 0.02 } }
+      mesh2 { vertex_vectors { 3, <-1,-1,0>, <1,-1,0>, <0,1,0> } texture_list { 1, texture { _t } } face_indices { 1, <0,1,2>, 0 } }
+      sphere { <0,0,0>, 1 texture { _t } }
+      sphere { <0,0,0>, 1 texture { _x_sl_tex } }
+      edge_line(<0,0,0>, <0,0,1>, <1,0,0>, <0,0,1>, 0.0, 0.05, _t, <0,0,0>)
+    ''')
+    sc, _, _ = pov.load_pov_scene(str(tmp_path / "s.pov"), 64, 48, CUEMOL, material_model="principled")
+    assert sc.mesh_material.model == 1
+    mats = [sc.prim_materials[i] for i in sc.sphere_material] + [sc.prim_materials[i] for i in sc.cyl_material]
+    assert [m.model for m in mats] == [1, 0, 0]  # the real sphere converts; edge joint and edge line stay
+    with pytest.raises(ValueError):
+        pov.load_pov_scene(str(tmp_path / "s.pov"), material_model="toon")

@@ -16,6 +16,7 @@ description language CueMol's templates use:
                      edges), declared textures and finishes, `_show_<id>`
                      sections as transparency groups
   load_pov_scene     src/bench/scene_setup.cpp:26-230 (buildSceneFromPov)
+  to_principled_material  src/bench/material_convert.cpp (--material principled)
 
 Values are double-precision vectors of up to four components (a scalar has
 one and broadcasts); unknown identifiers and functions evaluate to an empty
@@ -1362,6 +1363,59 @@ def read_geometry(path: str, symbols: Optional[Dict[str, Sequence[float]]] = Non
 
 
 # --------------------------------------------------------------------------
+# POV finish -> principled material  (src/bench/material_convert.cpp)
+# --------------------------------------------------------------------------
+def is_non_physical_finish(m: Material) -> bool:
+    """Material::toonLike (include/umbreon/scene.hpp): toon model, or a POV
+    finish whose look needs the POV model — highlight amount > 1,
+    brilliance 0, or no lit term at all (ambient-only)."""
+    if m.model == 2:
+        return True
+    if m.model != 0:
+        return False
+    if m.specular > 1.0 or m.phong > 1.0 or m.brilliance == 0.0:
+        return True
+    return m.diffuse <= 0.0 and m.specular <= 0.0 and m.phong <= 0.0 and m.reflection <= 0.0
+
+
+def _exponent_to_roughness(e) -> float:
+    """alpha = sqrt(2 / (e + 2)) of the Blinn exponent e, roughness =
+    sqrt(alpha) (alpha clamped to 1); float arithmetic."""
+    e = np.float32(e)
+    alpha = np.sqrt(np.float32(2.0) / (e + np.float32(2.0)), dtype=np.float32)
+    return float(np.sqrt(min(alpha, np.float32(1.0)), dtype=np.float32))
+
+
+def to_principled_material(m: Material) -> Material:
+    """toPrincipledMaterial (material_convert.cpp): a POV finish as the
+    principled parameters — the Blinn highlight (exponent 1 / roughness,
+    clamped to [1, 1e6]) or else the Phong one (exponent phong_size >= 1)
+    gives the GGX roughness and the highlight amount the specular; a POV
+    metallic finish is metallic 1; reflection without a highlight is a
+    polished metal, with one the metallic amount is at least the reflection.
+    Principled and non-physical (toon) materials come back unchanged."""
+    if m.model == 1 or is_non_physical_finish(m):
+        return m
+    clamp01 = lambda x: float(min(np.float32(1.0), max(np.float32(0.0), np.float32(x))))
+    high = m.specular > 0.0 or m.phong > 0.0
+    if m.specular > 0.0:
+        e = np.float32(1.0) / max(np.float32(m.roughness), np.float32(1e-6))
+        e = min(max(e, np.float32(1.0)), np.float32(1.0e6))
+        rough, spec = _exponent_to_roughness(e), clamp01(m.specular)
+    elif m.phong > 0.0:
+        rough, spec = _exponent_to_roughness(max(np.float32(1.0), np.float32(m.phong_size))), clamp01(m.phong)
+    else:
+        rough, spec = 0.5, 0.0
+    metal = 1.0 if m.metallic else m.pbr_metallic
+    if m.reflection > 0.0:
+        if not high:
+            metal, rough = 1.0, 0.0
+        else:
+            metal = max(metal, clamp01(m.reflection))
+    return replace(m, model=1, pbr_roughness=rough, pbr_specular=spec, pbr_metallic=metal)
+
+
+# --------------------------------------------------------------------------
 # scene assembly  (src/bench/scene_setup.cpp:26-230)
 # --------------------------------------------------------------------------
 def geometry_to_scene(geo: SceneGeometry, scene: Optional[Scene] = None) -> Scene:
@@ -1383,14 +1437,19 @@ def geometry_to_scene(geo: SceneGeometry, scene: Optional[Scene] = None) -> Scen
 
 def load_pov_scene(pov_path: str, width: int = 300, height: int = 300,
                    declares: Optional[Dict[str, float]] = None, gi: bool = False,
-                   ) -> Tuple[Scene, List[str], float]:
+                   material_model: str = "pov") -> Tuple[Scene, List[str], float]:
     """A Scene from a CueMol .pov and the .inc it includes as `_scene`, the
     way the reference bench sets it up: with gi, the radiosity balance
     (_light_inten 1.6, _amb_frac 0.5, _flash_frac 0.5) replaces the defaults
     that were not declared otherwise and the ambient colour is
     _light_inten * _amb_frac (else white); assumed gamma 2.2. Returns the
     scene, the section names per group and the AO distance the bench uses
-    (0.7 x the geometry's bounding diagonal, at least 1e-3)."""
+    (0.7 x the geometry's bounding diagonal, at least 1e-3). material_model
+    "principled" converts every finish with to_principled_material except
+    the edge decoration's (edge_line cylinders, `_sl_tex` spheres), as the
+    bench's `--material principled` does."""
+    if material_model not in ("pov", "principled"):
+        raise ValueError(f"material_model must be 'pov' or 'principled', not {material_model!r}")
     decl = dict(declares or {})
     if gi:
         for k, plain, rad in (("_light_inten", 1.3, 1.6), ("_amb_frac", 0.0, 0.5), ("_flash_frac", 0.6, 0.5)):
@@ -1402,6 +1461,11 @@ def load_pov_scene(pov_path: str, width: int = 300, height: int = 300,
     inc = ps.include_path if os.path.isabs(ps.include_path) else os.path.join(
         os.path.dirname(pov_path) or ".", ps.include_path)
     geo = read_geometry(inc, ps.declares)
+    if material_model == "principled":
+        geo.materials = [to_principled_material(m) for m in geo.materials]
+        for p in geo.spheres + geo.cylinders:
+            if not p.from_edge:
+                p.material = to_principled_material(p.material)
     sc = geometry_to_scene(geo)
     lo = np.full(3, np.inf, np.float32)
     hi = np.full(3, -np.inf, np.float32)
