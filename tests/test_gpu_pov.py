@@ -68,11 +68,14 @@ def _include_text(seed: int = 3) -> str:
     return "\n".join(out)
 
 
-def test_pov_scene_parity(device, tmp_path):
+@pytest.mark.parametrize("model", ["pov", "principled"])
+def test_pov_scene_parity(device, tmp_path, model):
     src = open(os.path.join(HERE, "golden", "pov", "1ab0_scene1.pov")).read()
     (tmp_path / "s.pov").write_text(src.replace("1ab0_scene1.inc", "s.inc"))
     (tmp_path / "s.inc").write_text(_include_text())
-    sc, groups, ao_dist = pov.load_pov_scene(str(tmp_path / "s.pov"), 256, 256, CUEMOL)
+    sc, groups, ao_dist = pov.load_pov_scene(str(tmp_path / "s.pov"), 256, 256, CUEMOL, material_model=model)
+    if model == "principled":
+        assert sc.mesh_material.model == 1 and any(m.model == 1 and m.pbr_specular > 0 for m in sc.prim_materials)
     print(f"pov scene: {sc.n_triangles} triangles, {len(sc.sphere_radius)} spheres, "
           f"{len(sc.cyl_radius)} cylinders, groups {groups}", flush=True)
     assert groups == ["", "_34_35", "_1_2"]
@@ -80,6 +83,6 @@ def test_pov_scene_parity(device, tmp_path):
     opt = RenderOptions(width=256, height=256, supersample=1, shadows=True, ao_samples=8, ao_distance=ao_dist)
     g = _gpu(device, sc, opt, hires_only=True, want_ids=True, want_ao=True)
     c = ob.render(sc, opt, hires_only=True)
-    _compare("pov scene", g, c, ao=True)
+    _compare(f"pov scene ({model})", g, c, ao=True)
     hit = g.hit_id != capi.MISS_ID
     assert hit.mean() > 0.1  # the geometry is in view
