@@ -3,7 +3,7 @@
 // serve callers that fog a frame themselves.
 #pragma once
 
-#include "scene.hpp"
+#include "../scene.hpp"
 
 namespace umbreon {
 

@@ -6,7 +6,7 @@
 #include <cstdint>
 #include <vector>
 
-#include "render/render_types.hpp"
+#include "../render/render_types.hpp"
 
 namespace umbreon {
 

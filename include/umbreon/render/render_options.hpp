@@ -10,7 +10,7 @@
 // return a different image.
 #pragma once
 
-#include "render/edge_types.hpp"
+#include "edge_types.hpp"
 
 namespace umbreon {
 

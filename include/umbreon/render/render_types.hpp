@@ -1,8 +1,8 @@
 // Umbrella for the renderer-agnostic public types.
 #pragma once
 
-#include "render/edge_types.hpp"
-#include "render/frame_result.hpp"
-#include "render/render_options.hpp"
-#include "render/render_progress.hpp"
-#include "log.hpp"
+#include "edge_types.hpp"
+#include "frame_result.hpp"
+#include "render_options.hpp"
+#include "render_progress.hpp"
+#include "../log.hpp"

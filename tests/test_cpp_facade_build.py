@@ -6,6 +6,8 @@ import re
 import subprocess
 from pathlib import Path
 
+import pytest
+
 ROOT = Path(__file__).resolve().parent.parent
 LIB = ROOT / "umbreon_amd" / "lib"
 
@@ -48,5 +50,22 @@ def test_facade_links_only_the_c_abi():
 
 
 def test_examples_built():
-    for exe in ("api_selftest", "minimal_render"):
+    for exe in ("api_selftest", "molecule_async"):
         assert (LIB / exe).exists(), exe
+
+
+REF_EXAMPLE = Path("/root/reference/examples/minimal_render.cpp")
+
+
+@pytest.mark.skipif(not REF_EXAMPLE.exists(), reason="the reference tree is not present")
+def test_reference_example_compiles_against_the_facade(tmp_path):
+    """Source compatibility: the reference's own minimal consumer, compiled
+    where it lies, against these public headers and linked with
+    libumbreon.so (compiled and linked only; nothing from the reference is
+    copied or run)."""
+    exe = tmp_path / "ref_minimal_render"
+    r = subprocess.run(["g++", "-std=c++17", "-O0", "-I", str(ROOT / "include"), str(REF_EXAMPLE), "-L", str(LIB),
+                        "-lumbreon", "-lumbreon_gpu", f"-Wl,-rpath,{LIB}", "-pthread", "-o", str(exe)],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr[-3000:]
+    assert exe.exists()

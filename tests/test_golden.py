@@ -72,7 +72,7 @@ def test_fp32_oracle_meets_gates_against_fixture(name):
 
 
 def test_psnr_helper_closed_forms():
-    from imagecmp import linear_to_srgb, psnr_srgb
+    from umbreon_amd.image_io import linear_to_srgb, psnr_srgb
     assert abs(float(linear_to_srgb(np.array(0.5))) - 0.7353569830524495) < 1e-12  # 188/255 in the reference's test
     a = np.zeros((2, 2, 4))
     b = a.copy()

@@ -20,8 +20,10 @@ def test_cpp_api_selftest(tmp_path):
     assert "0 failure(s)" in r.stdout
 
 
-def test_minimal_example_writes_image(tmp_path):
-    r = subprocess.run([str(LIB / "minimal_render")], capture_output=True, text=True, timeout=120, cwd=tmp_path)
+def test_async_molecule_example_writes_image(tmp_path):
+    r = subprocess.run([str(LIB / "molecule_async")], capture_output=True, text=True, timeout=120, cwd=tmp_path)
+    print(r.stdout[-3000:], r.stderr[-2000:])
     assert r.returncode == 0, r.stdout + r.stderr
-    ppm = (tmp_path / "minimal.ppm").read_bytes()
-    assert ppm.startswith(b"P6\n256 256\n255\n") and len(ppm) == len(b"P6\n256 256\n255\n") + 256 * 256 * 3
+    ppm = (tmp_path / "benzene.ppm").read_bytes()
+    head = b"P6\n320 240\n255\n"
+    assert ppm.startswith(head) and len(ppm) == len(head) + 320 * 240 * 3

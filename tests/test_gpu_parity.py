@@ -18,7 +18,7 @@ import numpy as np
 import pytest
 
 from oracle import binding as ob
-from imagecmp import psnr_srgb
+from umbreon_amd.image_io import psnr_srgb
 from umbreon_amd import capi
 from umbreon_amd.scenes import ball_and_stick_scene, molecular_surface_scene, spheres_scene
 

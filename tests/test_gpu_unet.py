@@ -30,10 +30,21 @@ def _batch(rng, n, h, w):
     return feat, tgt
 
 
-@pytest.mark.parametrize("levels,ch,shape", [(2, [32, 64], (2, 16, 32)), (4, [32, 64, 96, 128], (1, 32, 48))])
-def test_forward_and_gradients_match_oracle(device, levels, ch, shape):
+# min_wgs 0 makes every convolution take the widest pixel tiles its rows
+# allow, so these small images run the instantiations that 512^2 and 1024^2
+# frames use: (1, 16, 256) with 32/64/96/128 channels covers NT 2 / NP 4 at
+# level 0, NT 4 / NP 2, NT 6 / NP 2 and NT 8 / NP 2 below it, forward and
+# (transposed) backward.
+@pytest.mark.parametrize("levels,ch,shape,min_wgs", [
+    (2, [32, 64], (2, 16, 32), 1024),
+    (4, [32, 64, 96, 128], (1, 32, 48), 1024),
+    (4, [32, 64, 96, 128], (1, 16, 256), 0),
+    (3, [32, 64, 96], (2, 16, 128), 0),
+])
+def test_forward_and_gradients_match_oracle(device, levels, ch, shape, min_wgs):
     from umbreon_amd.denoise import Unet
     net = Unet(device, levels, ch, seed=3)
+    net.set_min_workgroups(min_wgs)
     try:
         assert net.n_params == U.param_count(levels, ch)
         p = net.get_params()
@@ -47,7 +58,8 @@ def test_forward_and_gradients_match_oracle(device, levels, ch, shape):
         g_pred = net.infer(feat)
         c_pred, _ = U.forward(p.astype(np.float64), feat, levels, ch, bf16=True)
         d = np.abs(g_pred - c_pred)
-        print(f"unet L{levels} forward: max|d| {d.max():.3g} mean|d| {d.mean():.3g}", flush=True)
+        print(f"unet L{levels} {shape} min_wgs {min_wgs} forward: max|d| {d.max():.3g} mean|d| {d.mean():.3g}",
+              flush=True)
         assert d.mean() <= 2e-3 and d.max() <= 5e-2
         g_loss, g_grads = net.train_step(feat, tgt, lr=0.0, want_grads=True)
         c_loss, c_grads = U.loss_and_grads(p.astype(np.float64), feat, tgt, levels, ch, bf16=True)
@@ -63,6 +75,36 @@ def test_forward_and_gradients_match_oracle(device, levels, ch, shape):
             rel = float(np.linalg.norm(a - b) / nb)
             print(f"  conv {k} ({c['cin']}->{c['cout']}): cos {cos:.6f} rel {rel:.3g}", flush=True)
             assert cos >= 0.999 and rel <= 5e-2, (k, cos, rel)
+    finally:
+        net.close()
+
+
+def test_forward_at_a_training_crop(device):
+    """A 128^2 crop through the 4-level net at the default launch shapes.
+    Device and oracle round the same operands to bf16, so they differ only
+    by fp32 vs float64 accumulation and the bf16 roundings that difference
+    flips: bounded here by a few bf16 ulps (2^-8 relative) of the output's
+    scale, max and mean."""
+    from umbreon_amd.denoise import Unet
+    levels, ch = 4, [32, 64, 96, 128]
+    net = Unet(device, levels, ch, seed=9)
+    try:
+        p = net.get_params()
+        rng = np.random.default_rng(13)
+        _, convs = U.layout(levels, ch)
+        for c in convs:
+            p[c["b_off"]:c["b_off"] + c["cout"]] = rng.normal(0.0, 0.05, c["cout"])
+        net.set_params(p)
+        feat, _ = _batch(rng, 1, 128, 128)
+        g_pred = net.infer(feat)
+        c_pred, cache = U.forward(p.astype(np.float64), feat, levels, ch, bf16=True)
+        z = np.abs(cache["out"][..., :3])
+        d = np.abs(g_pred - c_pred)
+        ulp = 2.0 ** -8
+        print(f"unet 128^2 forward: max|d| {d.max():.3g} mean|d| {d.mean():.3g} (output scale max {z.max():.3g} "
+              f"mean {z.mean():.3g})", flush=True)
+        assert d.mean() <= 0.5 * ulp * z.mean()
+        assert d.max() <= 8 * ulp * z.max()
     finally:
         net.close()
 

@@ -17,6 +17,7 @@ import os
 import shutil
 import subprocess
 import sys
+from concurrent.futures import ThreadPoolExecutor
 from pathlib import Path
 
 ROOT = Path(__file__).resolve().parent.parent
@@ -69,12 +70,17 @@ def build_gpu() -> Path:
     """Compile the HIP backend into umbreon_amd/lib/libumbreon_gpu.so."""
     BUILD.mkdir(exist_ok=True)
     LIB.mkdir(exist_ok=True)
-    objs = []
-    for src in GPU_SOURCES:
-        obj = BUILD / (src + ".o")
+    objs = [BUILD / (src + ".o") for src in GPU_SOURCES]
+
+    def compile_one(src: str, obj: Path) -> None:
         lang = ["-x", "hip"] if src.endswith(".hip") else []
         _run([HIPCC, *HIP_FLAGS, *lang, "-I", INCLUDE, "-c", CSRC / src, "-o", obj])
-        objs.append(obj)
+
+    # the translation units are independent: compile them side by side
+    jobs = max(1, min(len(GPU_SOURCES), int(os.environ.get("UBG_BUILD_JOBS", "6"))))
+    with ThreadPoolExecutor(jobs) as pool:
+        for f in [pool.submit(compile_one, src, obj) for src, obj in zip(GPU_SOURCES, objs)]:
+            f.result()
     out = LIB / "libumbreon_gpu.so"
     tmp = LIB / "libumbreon_gpu.so.tmp"
     _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", *objs, "-o", tmp,

@@ -256,6 +256,8 @@ def load() -> C.CDLL:
     lib.ubgx_host_query.argtypes = [C.POINTER(SceneDesc), C.c_int32, C.c_int64, C.c_void_p, C.c_void_p,
                                     C.c_void_p, C.c_void_p]
     lib.ubgx_host_query.restype = C.c_int
+    lib.ubgx_unet_set_min_workgroups.argtypes = [C.c_void_p, C.c_int64]
+    lib.ubgx_unet_set_min_workgroups.restype = C.c_int
     lib.ubg_scene_destroy.argtypes = [C.c_void_p]
     lib.ubg_scene_destroy.restype = None
     lib.ubg_scene_get_info.argtypes = [C.c_void_p, C.POINTER(SceneInfo)]

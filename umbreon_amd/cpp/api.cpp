@@ -11,9 +11,9 @@
 #include <algorithm>
 #include <array>
 #include <atomic>
-#include <condition_variable>
 #include <cstring>
 #include <exception>
+#include <future>
 #include <map>
 #include <mutex>
 #include <stdexcept>
@@ -84,7 +84,18 @@ ubg_material toC(const Material& m) {
   return c;
 }
 
-// Scene -> structure-of-arrays C description (owns the arrays).
+// The section groups one render pass draws: a group is drawn unless its
+// entry in `hidden` is set. An empty filter draws everything.
+struct GroupFilter {
+  std::vector<uint8_t> hidden;
+  bool draws(uint16_t g) const { return g >= hidden.size() || hidden[g] == 0; }
+  bool drawsAll() const { return std::find(hidden.begin(), hidden.end(), 1) == hidden.end(); }
+};
+
+// Scene -> structure-of-arrays C description (owns the arrays), keeping only
+// the primitives of the groups the filter draws. Triangles of hidden groups
+// leave the index list (a filtered soup becomes an explicit index over the
+// same vertices), so the vertex streams are shared by every pass.
 struct FlatScene {
   std::vector<float> pos, nrm, col;
   std::vector<uint32_t> idx;
@@ -102,6 +113,8 @@ struct FlatScene {
   std::vector<ubg_material> primMats;
   std::vector<float> inst;
   std::map<std::array<uint32_t, 16>, int32_t> matIndex;
+  int64_t nTris = 0;
+  bool explicitIndex = false;
   ubg_scene_desc desc{};
 
   int32_t primMaterial(const Material& m) {
@@ -117,7 +130,7 @@ struct FlatScene {
     return i;
   }
 
-  explicit FlatScene(const Scene& s) {
+  FlatScene(const Scene& s, const GroupFilter& filter) {
     const Mesh& m = s.mesh;
     const std::size_t nv = m.positions.size();
     pos.reserve(3 * nv);
@@ -129,15 +142,30 @@ struct FlatScene {
       const Vec4 c = v < m.colors.size() ? m.colors[v] : Vec4{};
       col.insert(col.end(), {c.x, c.y, c.z, c.w});
     }
-    idx = m.index;
     if (m.triMaterialId.empty()) {
       meshMats.push_back(toC(m.material));
     } else {
       for (const Material& mm : m.materials) meshMats.push_back(toC(mm));
-      triMat = m.triMaterialId;
     }
-    triGroup = m.triGroupId;
+    const std::size_t nt = m.triangleCount();
+    if (filter.drawsAll()) {
+      idx = m.index;
+      triMat = m.triMaterialId;
+      triGroup = m.triGroupId;
+      nTris = static_cast<int64_t>(nt);
+      explicitIndex = !idx.empty();
+    } else {
+      for (std::size_t t = 0; t < nt; ++t) {
+        if (!filter.draws(m.groupForTri(t))) continue;
+        idx.insert(idx.end(), {m.cornerVertex(3 * t), m.cornerVertex(3 * t + 1), m.cornerVertex(3 * t + 2)});
+        if (!m.triMaterialId.empty()) triMat.push_back(m.triMaterialId[t]);
+        if (!m.triGroupId.empty()) triGroup.push_back(m.triGroupId[t]);
+      }
+      nTris = static_cast<int64_t>(idx.size() / 3);
+      explicitIndex = true;
+    }
     for (const Sphere& sp : s.spheres) {
+      if (!filter.draws(sp.group)) continue;
       sph.insert(sph.end(), {sp.center.x, sp.center.y, sp.center.z, sp.radius});
       sphCol.insert(sphCol.end(), {sp.color.x, sp.color.y, sp.color.z, sp.color.w});
       sphMat.push_back(primMaterial(sp.material));
@@ -145,35 +173,63 @@ struct FlatScene {
       sphEdge.push_back(sp.fromEdgeMacro ? 1 : 0);
     }
     for (const Cylinder& cy : s.cylinders) {
-      c0.insert(c0.end(), {cy.p0.x, cy.p0.y, cy.p0.z});
-      c1.insert(c1.end(), {cy.p1.x, cy.p1.y, cy.p1.z});
-      crad.push_back(cy.radius);
-      ccol.insert(ccol.end(), {cy.color.x, cy.color.y, cy.color.z, cy.color.w});
-      cop1.push_back(cy.opacity1);
-      cmat.push_back(primMaterial(cy.material));
-      cgroup.push_back(cy.group);
-      copen.push_back(cy.open ? 1 : 0);
-      cedge.push_back(cy.fromEdgeMacro ? 1 : 0);
+      if (!filter.draws(cy.group)) continue;
+      addCylinder(cy.p0, cy.p1, cy.radius, cy.color, cy.opacity1, primMaterial(cy.material), cy.group, cy.open,
+                  cy.fromEdgeMacro);
     }
     for (const Vec3& o : s.instanceOffsets) inst.insert(inst.end(), {o.x, o.y, o.z});
+    publish();
+  }
 
-    desc.n_vertices = static_cast<int64_t>(nv);
+  void addCylinder(const Vec3& p0, const Vec3& p1, float r, const Vec4& color, float op1, int32_t mat,
+                   uint16_t group, bool open, bool fromEdge) {
+    c0.insert(c0.end(), {p0.x, p0.y, p0.z});
+    c1.insert(c1.end(), {p1.x, p1.y, p1.z});
+    crad.push_back(r);
+    ccol.insert(ccol.end(), {color.x, color.y, color.z, color.w});
+    cop1.push_back(op1);
+    cmat.push_back(mat);
+    cgroup.push_back(group);
+    copen.push_back(open ? 1 : 0);
+    cedge.push_back(fromEdge ? 1 : 0);
+  }
+
+  // Generated object-space edges join the cylinders as open curves with the
+  // flat outline material, not fromEdgeMacro (object_space_edges.cpp:240-262).
+  void appendEdges(const ubg_edges& e) {
+    const int32_t mat = primMaterial(Material::flatOutline());
+    for (int64_t i = 0; i < e.n; ++i)
+      addCylinder(Vec3{e.p0[3 * i], e.p0[3 * i + 1], e.p0[3 * i + 2]},
+                  Vec3{e.p1[3 * i], e.p1[3 * i + 1], e.p1[3 * i + 2]}, e.radius,
+                  Vec4{e.color[4 * i], e.color[4 * i + 1], e.color[4 * i + 2], e.color[4 * i + 3]}, e.opacity1[i],
+                  mat, e.group[i], true, false);
+    publish();
+  }
+
+  std::size_t effectiveTriangles() const {
+    return static_cast<std::size_t>(nTris) * (inst.empty() ? 1 : inst.size() / 3);
+  }
+
+  // Point the C description at the arrays (again after they grew).
+  void publish() {
+    desc = ubg_scene_desc{};
+    desc.n_vertices = static_cast<int64_t>(pos.size() / 3);
     desc.positions = pos.data();
     desc.normals = nrm.data();
     desc.colors = col.data();
-    desc.n_triangles = static_cast<int64_t>(m.triangleCount());
-    desc.index = idx.empty() ? nullptr : idx.data();
+    desc.n_triangles = nTris;
+    desc.index = explicitIndex && !idx.empty() ? idx.data() : nullptr;
     desc.mesh_materials = meshMats.data();
     desc.n_mesh_materials = static_cast<int32_t>(meshMats.size());
     desc.tri_material = triMat.empty() ? nullptr : triMat.data();
     desc.tri_group = triGroup.empty() ? nullptr : triGroup.data();
-    desc.n_spheres = static_cast<int64_t>(s.spheres.size());
+    desc.n_spheres = static_cast<int64_t>(sphGroup.size());
     desc.sphere_center_radius = sph.data();
     desc.sphere_color = sphCol.data();
     desc.sphere_material = sphMat.data();
     desc.sphere_group = sphGroup.data();
     desc.sphere_from_edge = sphEdge.data();
-    desc.n_cylinders = static_cast<int64_t>(s.cylinders.size());
+    desc.n_cylinders = static_cast<int64_t>(crad.size());
     desc.cyl_p0 = c0.data();
     desc.cyl_p1 = c1.data();
     desc.cyl_radius = crad.data();
@@ -185,7 +241,7 @@ struct FlatScene {
     desc.cyl_from_edge = cedge.data();
     desc.prim_materials = primMats.data();
     desc.n_prim_materials = static_cast<int32_t>(primMats.size());
-    desc.n_instances = static_cast<int64_t>(s.instanceOffsets.size());
+    desc.n_instances = static_cast<int64_t>(inst.size() / 3);
     desc.instance_offsets = inst.empty() ? nullptr : inst.data();
   }
 };
@@ -221,12 +277,11 @@ ubg_camera toCamera(const Camera& c) {
   return k;
 }
 
-// generateObjectSpaceEdges (object_space_edges.cpp:172-264): the scene with
-// the generated edges appended to its cylinders (open, flat outline
-// material, not fromEdgeMacro), the caller's scene untouched.
-Scene withObjectSpaceEdges(const Scene& scene, const ObjectSpaceEdgeOptions& o) {
-  FlatScene flat(scene);
-  const ubg_camera cam = toCamera(scene.camera);
+// generateObjectSpaceEdges (object_space_edges.cpp:172-264) over the
+// primitives a pass draws, appended to that pass's flat scene; the caller's
+// Scene is never touched.
+void addObjectSpaceEdges(FlatScene& flat, const Camera& camera, const ObjectSpaceEdgeOptions& o) {
+  const ubg_camera cam = toCamera(camera);
   ubg_edge_options eo{};
   eo.width = o.width;
   eo.raise = o.raise;
@@ -246,44 +301,29 @@ Scene withObjectSpaceEdges(const Scene& scene, const ObjectSpaceEdgeOptions& o) 
   ubg_edges e{};
   const int st = ubg_object_space_edges(device(), &flat.desc, &cam, &eo, &e);
   if (st != UBG_OK) throwStatus(st, "object-space edges");
-  Scene out = scene;
-  out.cylinders.reserve(out.cylinders.size() + static_cast<std::size_t>(e.n));
-  for (int64_t i = 0; i < e.n; ++i) {
-    Cylinder c;
-    c.p0 = Vec3{e.p0[3 * i], e.p0[3 * i + 1], e.p0[3 * i + 2]};
-    c.p1 = Vec3{e.p1[3 * i], e.p1[3 * i + 1], e.p1[3 * i + 2]};
-    c.radius = e.radius;
-    c.color = Vec4{e.color[4 * i], e.color[4 * i + 1], e.color[4 * i + 2], e.color[4 * i + 3]};
-    c.material = Material::flatOutline();
-    c.group = e.group[i];
-    c.opacity1 = e.opacity1[i];
-    c.open = true;
-    c.fromEdgeMacro = false;
-    out.cylinders.push_back(c);
-  }
+  flat.appendEdges(e);
   ubg_edges_free(&e);
-  return out;
 }
 
 // renderFrame (pipeline.cpp:20-256) for one pass on the GPU.
-FrameResult renderPass(const Scene& sceneIn, const RenderOptions& opt, RenderProgress* progress) {
-  // pipeline.cpp:30-42: object-space edges on a private copy of the pass's scene
-  Scene edged;
-  if (opt.objectSpaceEdges.enable) edged = withObjectSpaceEdges(sceneIn, opt.objectSpaceEdges);
-  const Scene& scene = opt.objectSpaceEdges.enable ? edged : sceneIn;
+FrameResult renderPass(const Scene& scene, const GroupFilter& groups, const RenderOptions& opt,
+                       RenderProgress* progress) {
   ubg_context* ctx = device();
   const int ss = std::max(1, opt.supersample);
   if (progress) progress->beginPhase(RenderPhase::Setup);
-  logMessage(LogLevel::Info, "render %dx%d ss%d (grid %dx%d), %zu tris / %zu spheres / %zu cylinders%s", opt.width,
-             opt.height, ss, opt.width * ss, opt.height * ss, scene.mesh.triangleCount(), scene.spheres.size(),
-             scene.cylinders.size(), opt.aoSamples > 0 ? ", AO" : "");
+  FlatScene flat(scene, groups);
+  // pipeline.cpp:30-42: the object-space edges of what this pass draws
+  if (opt.objectSpaceEdges.enable) addObjectSpaceEdges(flat, scene.camera, opt.objectSpaceEdges);
+  logMessage(LogLevel::Info, "render %dx%d ss%d (grid %dx%d), %lld tris / %lld spheres / %lld cylinders%s", opt.width,
+             opt.height, ss, opt.width * ss, opt.height * ss, static_cast<long long>(flat.desc.n_triangles),
+             static_cast<long long>(flat.desc.n_spheres), static_cast<long long>(flat.desc.n_cylinders),
+             opt.aoSamples > 0 ? ", AO" : "");
   FrameResult f;
-  f.effectiveTriangles = scene.effectiveTriangles();
+  f.effectiveTriangles = flat.effectiveTriangles();
   if (progress && progress->cancelRequested()) {
     f.cancelled = true;
     return f;
   }
-  FlatScene flat(scene);
   ubg_scene* gs = nullptr;
   int st = ubg_scene_create(ctx, &flat.desc, &gs);
   if (st != UBG_OK) throwStatus(st, "scene build");
@@ -433,94 +473,83 @@ FrameResult renderPass(const Scene& sceneIn, const RenderOptions& opt, RenderPro
   return f;
 }
 
-// Scene minus the primitives of hidden groups (umbreon.cpp:53-94).
-void hideGroups(const Scene& s, const std::vector<uint8_t>& hide, Scene& out) {
-  auto hidden = [&](uint16_t g) { return g < hide.size() && hide[g] != 0; };
-  out.mesh = Mesh{};
-  out.mesh.positions = s.mesh.positions;
-  out.mesh.normals = s.mesh.normals;
-  out.mesh.colors = s.mesh.colors;
-  out.mesh.material = s.mesh.material;
-  out.mesh.materials = s.mesh.materials;
-  std::vector<uint32_t> idx;
-  for (std::size_t t = 0; t < s.mesh.triangleCount(); ++t) {
-    if (hidden(s.mesh.groupForTri(t))) continue;
-    for (int k = 0; k < 3; ++k) idx.push_back(s.mesh.cornerVertex(3 * t + k));
-    if (!s.mesh.triMaterialId.empty()) out.mesh.triMaterialId.push_back(s.mesh.triMaterialId[t]);
-    if (!s.mesh.triGroupId.empty()) out.mesh.triGroupId.push_back(s.mesh.triGroupId[t]);
+// Group-alpha compositing (umbreon.cpp:141-254 defines the result): the
+// frame is a weighted sum of passes taken on sRGB-encoded colour with linear
+// alpha, then decoded. One pass draws the backdrop (every blended group
+// hidden) with weight 1 - sum of the alphas; each blend entry adds a pass
+// that also draws its group, with its alpha as weight.
+struct BlendPass {
+  GroupFilter groups;
+  float weight = 0.0f;
+};
+
+std::vector<BlendPass> blendPlan(const std::vector<GroupBlend>& blend) {
+  std::size_t span = 0;
+  float backdrop = 1.0f;
+  for (const GroupBlend& g : blend) {
+    span = std::max<std::size_t>(span, static_cast<std::size_t>(g.group) + 1);
+    backdrop -= g.alpha;
   }
-  if (idx.empty()) out.mesh = Mesh{};  // an empty index would mean "soup": drop the mesh instead
-  else out.mesh.index = std::move(idx);
-  out.spheres.clear();
-  for (const Sphere& sp : s.spheres)
-    if (!hidden(sp.group)) out.spheres.push_back(sp);
-  out.cylinders.clear();
-  for (const Cylinder& cy : s.cylinders)
-    if (!hidden(cy.group)) out.cylinders.push_back(cy);
+  GroupFilter none;
+  none.hidden.assign(span, 0);
+  for (const GroupBlend& g : blend) none.hidden[g.group] = 1;
+  std::vector<BlendPass> plan;
+  plan.push_back({none, backdrop});
+  for (const GroupBlend& g : blend) {
+    BlendPass p{none, g.alpha};
+    p.groups.hidden[g.group] = 0;
+    plan.push_back(std::move(p));
+  }
+  return plan;
 }
+
+// Running weighted sum of RGBA frames: rgb in the sRGB-encoded domain,
+// alpha linear.
+class SrgbBlend {
+ public:
+  void add(const std::vector<float>& rgba, float w) {
+    if (sum_.size() < rgba.size()) sum_.resize(rgba.size(), 0.0f);
+    for (std::size_t i = 0; i < rgba.size(); ++i)
+      sum_[i] += (i % 4 == 3) ? w * rgba[i] : w * srgbEncodeF(rgba[i]);
+  }
+  // Decoded frame: rgb = decode(max(0, sum)), alpha clamped to [0, 1].
+  std::vector<float> take() {
+    for (std::size_t i = 0; i < sum_.size(); ++i)
+      sum_[i] = (i % 4 == 3) ? std::clamp(sum_[i], 0.0f, 1.0f) : srgbDecodeF(std::max(0.0f, sum_[i]));
+    return std::move(sum_);
+  }
+
+ private:
+  std::vector<float> sum_;
+};
 
 FrameResult renderImpl(const Scene& scene, const RenderOptions& opt, RenderProgress* progress) {
   checkSupported(scene, opt);
   if (scene.groupBlend.empty()) {
-    FrameResult f = renderPass(scene, opt, progress);
+    FrameResult f = renderPass(scene, GroupFilter{}, opt, progress);
     if (progress && !f.cancelled) progress->markDone();
     return f;
   }
-  // Group-alpha multipass (umbreon.cpp:149-254): out = (1 - sum a) * bg pass
-  // + sum a_i * pass_i, blended on sRGB-encoded RGB (linear alpha).
-  float sumA = 0.0f;
-  uint16_t maxGroup = 0;
-  for (const GroupBlend& gb : scene.groupBlend) {
-    sumA += gb.alpha;
-    maxGroup = std::max(maxGroup, gb.group);
+  const std::vector<BlendPass> plan = blendPlan(scene.groupBlend);
+  SrgbBlend blend;
+  FrameResult out;
+  double traceSeconds = 0.0, buildSeconds = 0.0;
+  for (std::size_t i = 0; i < plan.size() && !out.cancelled; ++i) {
+    if (progress) progress->beginPass(i, plan.size());
+    logMessage(LogLevel::Info, "group-alpha: pass %zu of %zu, weight %.3f", i + 1, plan.size(),
+               static_cast<double>(plan[i].weight));
+    FrameResult f = renderPass(scene, plan[i].groups, opt, progress);
+    traceSeconds += f.renderSeconds;
+    buildSeconds += f.pt1Timing.bvhBuild;
+    blend.add(f.color, plan[i].weight);
+    out = std::move(f);  // the last pass's buffers carry depth and the AOVs
   }
-  const float bgW = 1.0f - sumA;
-  std::vector<uint8_t> hideAll(static_cast<std::size_t>(maxGroup) + 1, 0);
-  for (const GroupBlend& gb : scene.groupBlend) hideAll[gb.group] = 1;
-  Scene work = scene;
-  work.groupBlend.clear();
-  FrameResult carrier;
-  std::vector<float> acc;
-  double seconds = 0.0, bvh = 0.0;
-  const std::uint64_t passCount = 1 + scene.groupBlend.size();
-  std::uint64_t passIndex = 0;
-  bool cancelled = false;
-  auto addPass = [&](const std::vector<uint8_t>& hide, float w) {
-    if (cancelled) return;
-    if (progress) progress->beginPass(passIndex, passCount);
-    ++passIndex;
-    logMessage(LogLevel::Info, "group-alpha pass %llu/%llu (weight %.3f)", static_cast<unsigned long long>(passIndex),
-               static_cast<unsigned long long>(passCount), w);
-    hideGroups(scene, hide, work);
-    FrameResult f = renderPass(work, opt, progress);
-    if (f.cancelled) cancelled = true;
-    seconds += f.renderSeconds;
-    bvh += f.pt1Timing.bvhBuild;
-    if (acc.empty()) acc.assign(f.color.size(), 0.0f);
-    const std::size_t n = std::min(acc.size(), f.color.size()) / 4;
-    for (std::size_t q = 0; q < n; ++q) {
-      for (int ch = 0; ch < 3; ++ch) acc[4 * q + ch] += w * srgbEncodeF(f.color[4 * q + ch]);
-      acc[4 * q + 3] += w * f.color[4 * q + 3];
-    }
-    carrier = std::move(f);
-  };
-  addPass(hideAll, bgW);
-  for (const GroupBlend& gb : scene.groupBlend) {
-    std::vector<uint8_t> hide = hideAll;
-    hide[gb.group] = 0;
-    addPass(hide, gb.alpha);
-  }
-  for (std::size_t q = 0; q < acc.size() / 4; ++q) {
-    for (int ch = 0; ch < 3; ++ch) acc[4 * q + ch] = srgbDecodeF(std::fmax(0.0f, acc[4 * q + ch]));
-    acc[4 * q + 3] = std::fmin(1.0f, std::fmax(0.0f, acc[4 * q + 3]));
-  }
-  carrier.color = std::move(acc);
-  carrier.renderSeconds = seconds;
-  carrier.pt1Timing.bvhBuild = bvh;
-  carrier.pt1Timing.primary = seconds;
-  carrier.cancelled = cancelled;
-  if (progress && !cancelled) progress->markDone();
-  return carrier;
+  out.color = blend.take();
+  out.renderSeconds = traceSeconds;
+  out.pt1Timing.bvhBuild = buildSeconds;
+  out.pt1Timing.primary = traceSeconds;
+  if (progress && !out.cancelled) progress->markDone();
+  return out;
 }
 
 }  // namespace
@@ -532,79 +561,65 @@ FrameResult render(const Scene& scene, const RenderOptions& opt, RenderProgress&
 }
 
 // ---------------------------------------------------------------- async
+// A background render is a packaged task on its own thread: the future
+// carries the frame or the exception, the task's progress object is the one
+// the render polls for cancellation.
 struct RenderTask::Impl {
   Scene scene;
   RenderOptions opt;
   RenderProgress progress;
-  FrameResult result;
-  std::exception_ptr error;
-  std::atomic<bool> finished{false};
-  mutable std::mutex mu;
-  mutable std::condition_variable cv;
+  std::future<FrameResult> frame;
   std::thread worker;
-  bool taken = false;
+  bool collected = false;
+
+  bool ready() const { return frame.valid() && frame.wait_for(std::chrono::seconds(0)) == std::future_status::ready; }
+  void stop() {
+    progress.requestCancel();
+    if (worker.joinable()) worker.join();
+  }
 };
 
 RenderTask::RenderTask(std::unique_ptr<Impl> impl) noexcept : p_(std::move(impl)) {}
 RenderTask::RenderTask(RenderTask&&) noexcept = default;
 RenderTask& RenderTask::operator=(RenderTask&& o) noexcept {
   if (this != &o) {
-    if (p_) {
-      p_->progress.requestCancel();
-      if (p_->worker.joinable()) p_->worker.join();
-    }
+    if (p_) p_->stop();
     p_ = std::move(o.p_);
   }
   return *this;
 }
 RenderTask::~RenderTask() {
-  if (!p_) return;
-  p_->progress.requestCancel();
-  if (p_->worker.joinable()) p_->worker.join();
+  if (p_) p_->stop();
 }
 
 float RenderTask::progress() const noexcept { return p_ ? p_->progress.fraction() : 0.0f; }
 RenderPhase RenderTask::phase() const noexcept { return p_ ? p_->progress.phase() : RenderPhase::Idle; }
-bool RenderTask::done() const noexcept { return p_ && p_->finished.load(); }
+bool RenderTask::done() const noexcept { return p_ && p_->ready(); }
 void RenderTask::cancel() noexcept {
   if (p_) p_->progress.requestCancel();
 }
 bool RenderTask::wait_for(std::chrono::milliseconds timeout) const {
-  if (!p_) return true;
-  std::unique_lock<std::mutex> lk(p_->mu);
-  return p_->cv.wait_for(lk, timeout, [&] { return p_->finished.load(); });
+  return !p_ || !p_->frame.valid() || p_->frame.wait_for(timeout) == std::future_status::ready;
 }
 void RenderTask::wait() const {
-  if (!p_) return;
-  std::unique_lock<std::mutex> lk(p_->mu);
-  p_->cv.wait(lk, [&] { return p_->finished.load(); });
+  if (p_ && p_->frame.valid()) p_->frame.wait();
 }
 FrameResult RenderTask::get() {
   if (!p_) throw std::runtime_error("umbreon::RenderTask::get on an empty task");
+  if (p_->collected) throw std::runtime_error("umbreon::RenderTask::get called twice");
+  p_->collected = true;
   if (p_->worker.joinable()) p_->worker.join();
-  if (p_->taken) throw std::runtime_error("umbreon::RenderTask::get called twice");
-  p_->taken = true;
-  if (p_->error) std::rethrow_exception(p_->error);
-  return std::move(p_->result);
+  return p_->frame.get();  // rethrows the render's exception
 }
 
 RenderTask renderAsync(Scene scene, RenderOptions opt) {
   auto impl = std::make_unique<RenderTask::Impl>();
   impl->scene = std::move(scene);
   impl->opt = std::move(opt);
-  RenderTask::Impl* raw = impl.get();
-  raw->worker = std::thread([raw] {
-    try {
-      raw->result = render(raw->scene, raw->opt, raw->progress);
-    } catch (...) {
-      raw->error = std::current_exception();
-    }
-    {
-      std::lock_guard<std::mutex> lk(raw->mu);
-      raw->finished.store(true);
-    }
-    raw->cv.notify_all();
-  });
+  RenderTask::Impl* task = impl.get();
+  std::packaged_task<FrameResult()> job([task] { return render(task->scene, task->opt, task->progress); });
+  task->frame = job.get_future();
+  task->worker = std::thread(std::move(job));
   return RenderTask(std::move(impl));
 }
 

@@ -135,18 +135,13 @@ void RenderProgress::markDone() noexcept {
   phase_.store(static_cast<int>(RenderPhase::Done), std::memory_order_relaxed);
 }
 
+// Display names of the phases, indexed by the enum value; the strings are
+// the reference's (src/umbreon/render/render_progress.cpp:109-121), which a
+// UI may show verbatim.
 const char* toString(RenderPhase p) noexcept {
-  switch (p) {
-    case RenderPhase::Idle: return "Idle";
-    case RenderPhase::Setup: return "Setup";
-    case RenderPhase::CoarseAo: return "CoarseAo";
-    case RenderPhase::Primary: return "Primary";
-    case RenderPhase::GlobalIllum: return "GlobalIllum";
-    case RenderPhase::Edges: return "Edges";
-    case RenderPhase::Postprocess: return "Postprocess";
-    case RenderPhase::Done: return "Done";
-  }
-  return "Unknown";
+  static const char* const kNames[] = {"Idle", "Setup", "CoarseAO", "Primary", "GI", "Edges", "Postprocess", "Done"};
+  const int i = static_cast<int>(p);
+  return i >= 0 && i < static_cast<int>(sizeof(kNames) / sizeof(kNames[0])) ? kNames[i] : "Unknown";
 }
 
 // ---------------------------------------------------------------- image ops

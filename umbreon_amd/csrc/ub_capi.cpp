@@ -28,6 +28,7 @@
 #include "ub_kernels.h"
 #include "ub_unet.h"
 #include "ub_layout.h"
+#include "ub_testing.h"
 
 namespace {
 
@@ -1313,6 +1314,13 @@ void ubg_unet_destroy(ubg_unet* u) {
 }
 
 int64_t ubg_unet_param_count(const ubg_unet* u) { return u ? u->net.paramCount() : 0; }
+
+int ubgx_unet_set_min_workgroups(ubg_unet* u, int64_t n) {
+  if (!u || n < 0) return fail(UBG_ERR_INVALID_ARGUMENT, "ubgx_unet_set_min_workgroups: bad argument");
+  std::lock_guard<std::mutex> lk(u->ctx->mu);
+  u->net.setMinWorkgroups(n);
+  return UBG_OK;
+}
 
 int ubg_unet_get_params(ubg_unet* u, float* params) {
   if (!u || !params) return fail(UBG_ERR_INVALID_ARGUMENT, "ubg_unet_get_params: bad argument");

@@ -22,6 +22,12 @@ extern "C" {
 int ubgx_host_query(const ubg_scene_desc* desc, int32_t bvh_layout, int64_t n_rays, const ubg_ray* rays,
                     ubg_hit* hits, uint8_t* occ, uint64_t* stats);
 
+/* The U-Net convolutions' launch-shape threshold (ubg::Unet::setMinWorkgroups):
+ * 0 makes every convolution take the widest pixel tiles its rows allow, so
+ * small test images run the kernel instantiations that large frames use. */
+typedef struct ubg_unet ubg_unet;
+int ubgx_unet_set_min_workgroups(ubg_unet* net, int64_t n);
+
 #ifdef __cplusplus
 }
 #endif

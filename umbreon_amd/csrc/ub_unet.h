@@ -49,6 +49,10 @@ class Unet {
   hipError_t copyGrads(float* host, hipStream_t st);
   // one Adam step (beta 0.9 / 0.999, eps 1e-8) with the last gradients
   hipError_t adam(float lr, hipStream_t st);
+  // Convolution launch shape: a launch takes the widest pixel tiles per wave
+  // (4 for 32 output channels, else 2) that still give it at least this many
+  // workgroups (default 1,024); 0 = always the widest.
+  void setMinWorkgroups(int64_t n) { minWgs_ = n; }
 
  private:
   struct Src {
@@ -78,6 +82,7 @@ class Unet {
   std::vector<int64_t> wBfOff_;                 // per conv offset into wBf_ / wtBf_
   int step_ = 0;
   bool bf16Dirty_ = true;
+  int64_t minWgs_ = 1024;
   // activation scratch for the current shape
   int n_ = 0, h_ = 0, w_ = 0;
   std::vector<uint16_t*> actBf_;   // bf16 activations

@@ -101,7 +101,9 @@ __device__ __forceinline__ uint16_t load1(const SrcDev& s, int n, int y, int x, 
 // writes 16 bytes at the wave-uniform LDS base + 16 l) of a unit array:
 // dst[u] for u = threadIdx.x + 256 k < count, with src(u) the global
 // address of unit u, or zero where src(u) is null. Nothing waits for a
-// register, so every load of the staging is in flight at once.
+// register, so every load of the staging is in flight at once. The LDS
+// writes of these loads complete asynchronously and are counted by vmcnt, so
+// a workgroup barrier that publishes them must be preceded by dmaDone().
 template <int COUNT, typename Src>
 __device__ __forceinline__ void stageDma(uint4* dst, Src src) {
   const int wv = threadIdx.x >> 6;
@@ -117,6 +119,12 @@ __device__ __forceinline__ void stageDma(uint4* dst, Src src) {
     }
   }
 }
+
+// Wait until every direct-to-LDS load this wave issued has written LDS
+// (vmcnt 0; expcnt and lgkmcnt left at their no-wait maxima, gfx9 encoding).
+// A plain workgroup barrier does not imply it: the memory model's release
+// fence at a barrier waits on lgkmcnt only.
+__device__ __forceinline__ void dmaDone() { __builtin_amdgcn_s_waitcnt(0x0F70); }
 
 // 3x3 convolution, zero padding: out[p, co] = sum over taps and input
 // channels of in[p + tap, ci] * w[tap][co][ci] (flip: w[8 - tap]), plus bias
@@ -203,6 +211,7 @@ __global__ __launch_bounds__(256) void k_conv3x3(SrcDev s0, SrcDev s1, int N, in
         const int tap = 3 * d0 + t;
         return w + (static_cast<size_t>(flip ? 8 - tap : tap) * Cout + coBase + c) * Cin + ci0 + 8 * k8;
       });
+      dmaDone();  // this wave's patch and weight loads have landed in LDS
       __syncthreads();
 #pragma unroll
       for (int t = 0; t < 3 * TR; ++t) {
@@ -463,17 +472,18 @@ inline unsigned blocks(int64_t n, int bs) { return static_cast<unsigned>((n + bs
 // Output channels in passes of at most 128 (Cout a multiple of 32). A wave
 // takes 4 pixel tiles for 32 output channels, 2 for more, so each weight
 // fragment feeds 2 to 4 products; fewer where rows are narrower than that
-// or where the launch would have fewer than 1024 workgroups (the small deep
-// levels: there, parallelism matters more than weight reuse).
+// or where the launch would have fewer than minWgs workgroups (1,024 by
+// default: on the small deep levels parallelism matters more than weight
+// reuse).
 // resid / pred: write only the prediction (fp32 output, see k_conv3x3).
 void launchConv(SrcDev s0, SrcDev s1, int N, int H, int W, int Cin, int Cout, const uint16_t* w, int flip,
-                const float* bias, int relu, uint16_t* outBf, float* outF, hipStream_t st,
+                const float* bias, int relu, uint16_t* outBf, float* outF, hipStream_t st, int64_t minWgs,
                 const float* resid = nullptr, float* pred = nullptr) {
   const int64_t rowTiles = static_cast<int64_t>(N) * ((H + kConvRows - 1) / kConvRows);
   for (int co = 0; co < Cout; co += 128) {
     const int nt = std::min(8, (Cout - co) / 16);
     int np = nt <= 2 ? 4 : 2;
-    while (np > 1 && (W < 16 * np || rowTiles * ((W + 16 * np - 1) / (16 * np)) < 1024)) np /= 2;
+    while (np > 1 && (W < 16 * np || rowTiles * ((W + 16 * np - 1) / (16 * np)) < minWgs)) np /= 2;
     const int64_t wgs = rowTiles * ((W + 16 * np - 1) / (16 * np));
     const dim3 g(static_cast<unsigned>(wgs)), b(256);
 #define UBG_CONV(K, P) \
@@ -702,8 +712,8 @@ hipError_t Unet::forward(int n, int h, int w, const float* feat, float* out, hip
     };
     const bool last = c.out < 0;
     launchConv(src(c.s0), src(c.s1), n, H, W, c.cin, c.cout, wBf_ + wBfOff_[k], 0, params_ + c.bOff, c.relu,
-               last ? nullptr : actBf_[c.out], last && !out ? outF_ : nullptr, st, last ? feat : nullptr,
-               last ? out : nullptr);
+               last ? nullptr : actBf_[c.out], last && !out ? outF_ : nullptr, st, minWgs_,
+               last ? feat : nullptr, last ? out : nullptr);
   }
   return hipGetLastError();
 }
@@ -768,7 +778,7 @@ hipError_t Unet::gradients(int n, int h, int w, const float* feat, const float* 
     SrcDev dzs{dz_, c.cout, 0, H, W};
     SrcDev none{nullptr, 0, 0, 0, 0};
     launchConv(dzs, none, n, H, W, c.cout, c.cin, wtBf_ + wBfOff_[static_cast<size_t>(k)], 1, nullptr, 0,
-               nullptr, dx_, st);
+               nullptr, dx_, st, minWgs_);
     auto scatter = [&](const Src& s, int off) {
       const int lv = acts_[s.act].level;
       const int Hs = h >> lv, Ws = w >> lv, Cs = acts_[s.act].C;

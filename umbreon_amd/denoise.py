@@ -42,6 +42,11 @@ class Unet:
         except Exception:
             pass
 
+    def set_min_workgroups(self, n: int) -> None:
+        """Convolution launch-shape threshold (test hook): 0 makes every
+        convolution use the widest pixel tiles its rows allow."""
+        capi.check(self._lib.ubgx_unet_set_min_workgroups(self.handle, int(n)))
+
     def get_params(self) -> np.ndarray:
         p = np.empty(self.n_params, np.float32)
         capi.check(self._lib.ubg_unet_get_params(self.handle, p.ctypes.data))

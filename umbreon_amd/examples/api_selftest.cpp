@@ -7,6 +7,8 @@
 #include <cmath>
 #include <cstdio>
 #include <stdexcept>
+#include <string>
+#include <utility>
 
 #include "postprocess/image_ops.hpp"
 #include "umbreon.hpp"
@@ -50,6 +52,17 @@ constexpr std::size_t kC = (2 * 5 + 2) * 4;
 }  // namespace
 
 int main() {
+  {
+    // render_progress.cpp:109-121: the phase names a UI shows
+    using P = umbreon::RenderPhase;
+    const std::pair<P, const char*> names[] = {{P::Idle, "Idle"},       {P::Setup, "Setup"},
+                                               {P::CoarseAo, "CoarseAO"}, {P::Primary, "Primary"},
+                                               {P::GlobalIllum, "GI"},    {P::Edges, "Edges"},
+                                               {P::Postprocess, "Postprocess"}, {P::Done, "Done"}};
+    bool all = true;
+    for (const auto& n : names) all = all && std::string(umbreon::toString(n.first)) == n.second;
+    check("toString(RenderPhase) names", all);
+  }
   umbreon::RenderOptions o;
   o.width = 5;
   o.height = 5;

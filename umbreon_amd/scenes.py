@@ -37,12 +37,20 @@ RADII = np.array([1.2, 1.55, 1.7, 1.8], np.float32)
 LIT = Material(ambient=0.2, diffuse=0.7, specular=0.3, roughness=0.01)
 
 
+def _cross3(a, b) -> np.ndarray:
+    """a x b of two 3-vectors in float64, component for component the
+    products and differences np.cross forms (without its per-call overhead)."""
+    a0, a1, a2 = float(a[0]), float(a[1]), float(a[2])
+    b0, b1, b2 = float(b[0]), float(b[1]), float(b[2])
+    return np.array([a1 * b2 - a2 * b1, a2 * b0 - a0 * b2, a0 * b1 - a1 * b0])
+
+
 def _perp(v: np.ndarray, rng: np.random.Generator) -> np.ndarray:
-    a = np.array([1.0, 0.0, 0.0]) if abs(v[0]) < 0.9 else np.array([0.0, 1.0, 0.0])
-    p = np.cross(v, a)
+    a = (1.0, 0.0, 0.0) if abs(v[0]) < 0.9 else (0.0, 1.0, 0.0)
+    p = _cross3(v, a)
     p /= np.linalg.norm(p)
     ang = rng.uniform(0.0, 2.0 * math.pi)
-    q = np.cross(v, p)
+    q = _cross3(v, p)
     return math.cos(ang) * p + math.sin(ang) * q
 
 
@@ -74,19 +82,28 @@ def _protein_atoms(n_atoms: int, seed: int = 0) -> Tuple[np.ndarray, np.ndarray]
     def key(p):
         return (int(math.floor(p[0] / cell)), int(math.floor(p[1] / cell)), int(math.floor(p[2] / cell)))
 
+    flat = []  # the positions again as Python float triples (fast scalar distance tests)
+
     def clear(p, skip):
         kx, ky, kz = key(p)
+        px, py, pz = float(p[0]), float(p[1]), float(p[2])
         for dx in (-1, 0, 1):
             for dy in (-1, 0, 1):
                 for dz in (-1, 0, 1):
                     for j in grid.get((kx + dx, ky + dy, kz + dz), ()):
-                        if j != skip and float(np.sum((pos[j] - p) ** 2)) < cell * cell:
+                        if j == skip:
+                            continue
+                        qx, qy, qz = flat[j]
+                        ex, ey, ez = qx - px, qy - py, qz - pz
+                        # the same float64 sum as np.sum of the squared differences
+                        if ex * ex + ey * ey + ez * ez < cell * cell:
                             return False
         return True
 
     def place(p, e):
         grid.setdefault(key(p), []).append(len(pos))
         pos.append(p)
+        flat.append((float(p[0]), float(p[1]), float(p[2])))
         elem.append(e)
 
     while len(pos) < n_atoms:
