@@ -1,13 +1,17 @@
 #!/usr/bin/env python3
 """Benchmark: Mrays/s (primary + AO + shadow) of the gfx950 per-ray path.
 
-Workload (default): the ball-and-stick scene -- 20,000 atom spheres (r 0.4 A)
-plus every pair closer than 1.9 A as two capped half-bond cylinders (r 0.2 A),
-16 cosine AO rays per hit (max distance 8 A), hard shadows from one distant
-light, linear depth fog over 60 A, 1920x1080 with 2x2 supersampling (a
-3840x2160 hi-res grid). One step = one full frame: primary rays, hit setup,
-AO, shadows, POV shading, fog and the supersample resolve, with the scene and
-its BVH already resident in HBM and the frame left in HBM (no host copy).
+Workload (default): the molecular surface -- the Gaussian-density isosurface
+(sigma 1 A, iso 0.5, 0.5 A grid) of a 40,000-atom synthetic protein as 4.03 M
+indexed triangles at alpha 0.6 over its 40,000 opaque CPK spheres, with the
+view's object-space silhouette edges (sphere rings clipped to the union
+boundary, mesh silhouettes, creases and borders) as chained open curves;
+hard shadows from one distant light, 16 cosine AO rays per shaded hit (8 A),
+1920x1080 at 1 spp. One step = one full frame: every transparency layer's
+primary rays, hit setup, AO, shadows, POV shading and compositing, then the
+resolve, with the scene and its BVH already resident in HBM and the frame
+left in HBM (no host copy). `--workload ballstick` runs the ball-and-stick
+frame (20,000 atoms + half-bond cylinders, 1920x1080 ss2, AO16, fog).
 
 Prints one JSON line on stdout; progress goes to stderr.
 """
@@ -33,12 +37,38 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def host_cpus() -> dict:
+    """The host's processor as lscpu reports it, and the CPUs this process
+    may run on."""
+    import subprocess
+    info = {"affinity_cpus": len(os.sched_getaffinity(0))}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+    except (OSError, subprocess.SubprocessError):
+        return info
+    keys = {"Model name": "model", "Socket(s)": "sockets", "Core(s) per socket": "cores_per_socket",
+            "Thread(s) per core": "threads_per_core", "CPU(s)": "cpus"}
+    for line in out.splitlines():
+        k, _, v = line.partition(":")
+        if k.strip() in keys and keys[k.strip()] not in info:
+            v = v.strip()
+            info[keys[k.strip()]] = int(v) if v.isdigit() else v
+    return info
+
+
 def cpu_baseline(scene, opt, target_s: float):
-    """Oracle (CPU restatement, FP32) on the host cores: the full frame, median
-    of 5 runs after 1 warm-up, when that fits the time target; otherwise a
-    row sample sized to about target_s seconds."""
+    """Oracle (CPU restatement, FP32) on the host: the full frame, median of
+    5 runs after 1 warm-up, when that fits the time target; otherwise a row
+    sample sized to about target_s seconds. Threads: every CPU this process
+    may use, capped by the host share the job is given (OMP_NUM_THREADS when
+    set: one GPU's share of a shared host)."""
     from oracle import binding as ob
-    threads = max(1, min(16, os.cpu_count() or 1))
+    cpus = host_cpus()
+    threads = cpus["affinity_cpus"]
+    share = os.environ.get("OMP_NUM_THREADS", "")
+    if share.isdigit() and int(share) > 0:
+        threads = min(threads, int(share))
+    ob.set_scene_cache(True)  # the oracle's BVH is built once, outside the timed window
     H = opt.height * opt.supersample
     W = opt.width * opt.supersample
     step = 64
@@ -61,12 +91,14 @@ def cpu_baseline(scene, opt, target_s: float):
         f = ob.render(scene, opt, hires_only=True, threads=threads, rows=(0, H, step2))
         rays, sec = f.total_rays, f.render_seconds
         sample = f"hi-res rows y = 0 mod {step2} ({len(range(0, H, step2))} of {H} rows x {W} px), {rays} rays"
-    log(f"cpu baseline: {rays} rays in {sec:.3f}s with {threads} threads")
+    ob.set_scene_cache(False)
+    log(f"cpu baseline: {rays} rays in {sec:.3f}s with {threads} threads ({cpus})")
     return {
         "value": rays / sec / 1e6,
         "unit": "Mrays/s",
         "cores": threads,
         "kind": "port",
+        "host": cpus,
         "sample": sample + "; CPU restatement (not Embree), FP32, own binned-SAH BVH, std::thread over rows; "
                            "time window = trace + shade + fog (BVH build excluded)",
     }
@@ -147,7 +179,9 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", choices=["ballstick", "spheres", "surface", "gi"], default="ballstick")
+    ap.add_argument("--workload", choices=["ballstick", "spheres", "surface", "gi"], default="surface")
+    ap.add_argument("--no-edges", action="store_true",
+                    help="surface workload without the object-space silhouette edges")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--layout", choices=["bvh8", "bvh8gpu", "bvh2"], default="bvh8",
@@ -192,6 +226,17 @@ def main() -> int:
     log(f"scene generated in {time.perf_counter() - t0:.1f}s: {workload}")
 
     dev = Device(0)
+    edges_info = None
+    if args.workload == "surface" and not args.no_edges:
+        from umbreon_amd.edges import ObjectSpaceEdgeOptions, append_edges, object_space_edges
+        t0 = time.perf_counter()
+        e = object_space_edges(scene, ObjectSpaceEdgeOptions(enable=True), dev)
+        edge_s = time.perf_counter() - t0
+        scene = append_edges(scene, e)
+        edges_info = {"cylinders": len(e["group"]), "generate_ms": round(edge_s * 1e3, 1)}
+        workload += (f", object-space edges ({len(e['group'])} open curve segments r{e['radius']:g}: "
+                     f"clipped sphere rings, mesh silhouettes, creases, borders)")
+        log(f"object-space edges: {len(e['group'])} in {edge_s * 1e3:.0f} ms")
     bs = BuiltScene(dev, scene, {"bvh8": capi.BVH8, "bvh8gpu": capi.BVH8_GPU_BUILD, "bvh2": capi.BVH2}[args.layout])
     info = bs.info
     log(f"BVH ({args.layout}): {info['n_nodes']} nodes of {info['node_bytes']} B, depth {info['bvh_depth']}, "
@@ -291,6 +336,7 @@ def main() -> int:
             "bvh_build_ms": round(info["build_seconds"] * 1e3, 2),
             "bvh_nodes": info["n_nodes"],
             "bvh_depth": info["bvh_depth"],
+            "object_space_edges": edges_info,
             "parallelism": "single GPU",
         },
         "roofline": roofline,

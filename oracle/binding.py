@@ -57,6 +57,8 @@ def load():
     lib.orc_edges_free.restype = None
     lib.orc_tea2.argtypes = [C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]
     lib.orc_tea2.restype = None
+    lib.orc_set_scene_cache.argtypes = [C.c_int32]
+    lib.orc_set_scene_cache.restype = None
     lib.orc_fog_factor.argtypes = [C.POINTER(capi.Fog), C.c_float]
     lib.orc_fog_factor.restype = C.c_float
     lib.orc_box_downsample.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p]
@@ -94,6 +96,12 @@ def _opts(precision=32, threads=0, rows=None, brute=False) -> OrcOptions:
         o.row_begin, o.row_end, o.row_step = rows
     o.brute_force = 1 if brute else 0
     return o
+
+
+def set_scene_cache(enable: bool) -> None:
+    """Reuse the last built scene (geometry + BVH) for renders of an
+    unchanged scene (timing only; results are the same)."""
+    load().orc_set_scene_cache(1 if enable else 0)
 
 
 def render(scene: Scene, opt: RenderOptions, hires_only: bool = False, precision: int = 32,

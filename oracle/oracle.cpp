@@ -30,6 +30,8 @@
 #include <functional>
 #include <limits>
 #include <map>
+#include <memory>
+#include <mutex>
 #include <thread>
 #include <utility>
 #include <vector>
@@ -377,11 +379,59 @@ struct Hit {
   V3<T> ng;
 };
 
+template <class T>
+inline T maxAbsT(V3<T> v) {
+  return std::fmax(std::fmax(std::fabs(v.x), std::fabs(v.y)), std::fabs(v.z));
+}
+
 // Closest-approach origin shift: returns ts with o + ts*d nearest to c.
 template <class T>
 inline T shiftTo(V3<T> o, V3<T> d, V3<T> c, T A) {
   return dot(c - o, d) / A;
 }
+
+// FP64 polish of grazing roots, the FP32 path only (the FP64 path solves in
+// double throughout). In the shifted frame a grazing ray's discriminant is a
+// small difference of squares whose FP32 terms carry the rounding of the
+// shifted origin (an ulp of the coordinates' magnitude m). When |disc| lies
+// within that budget the quadratic is set up again in double from the exact
+// inputs -- shifted origin (o - p) + d ts -- solved, and each root takes one
+// Newton step (when the step is at most half the roots' separation). This is
+// the library's own rule (its intersectors restate it bit for bit); Embree's
+// ROBUST mode (scene_build.cpp:138) is what it stands in for, and no
+// reference test pins either.
+namespace graze {
+struct D3 {
+  double x, y, z;
+};
+inline D3 toD(V3<float> v) { return D3{v.x, v.y, v.z}; }
+inline double dotD(D3 a, D3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+inline D3 subD(D3 a, D3 b) { return D3{a.x - b.x, a.y - b.y, a.z - b.z}; }
+inline D3 shifted(V3<float> o, V3<float> p, V3<float> d, float ts) {
+  const double t = ts;
+  return D3{(double(o.x) - double(p.x)) + double(d.x) * t, (double(o.y) - double(p.y)) + double(d.y) * t,
+            (double(o.z) - double(p.z)) + double(d.z) * t};
+}
+inline float maxAbs(V3<float> v) { return std::fmax(std::fmax(std::fabs(v.x), std::fabs(v.y)), std::fabs(v.z)); }
+inline float slack(float scale, float size, float m) { return scale * 0x1.0p-19f * size * m; }
+inline bool quad(double A, double B, double C, double& x0, double& x1) {
+  const double disc = B * B - A * C;
+  if (disc < 0.0) return false;
+  const double q = std::sqrt(disc);
+  x0 = (-B - q) / A;
+  x1 = (-B + q) / A;
+  const double half = 0.5 * std::fabs(x1 - x0);
+  for (double* x : {&x0, &x1}) {
+    const double g = (A * *x + 2.0 * B) * *x + C;
+    const double gp = 2.0 * (A * *x + B);
+    if (gp != 0.0) {
+      const double st = g / gp;
+      if (std::fabs(st) <= half) *x = *x - st;
+    }
+  }
+  return true;
+}
+}  // namespace graze
 
 // Sphere (RTC_GEOMETRY_TYPE_SPHERE_POINT): near root if in range, else far.
 template <class T>
@@ -392,10 +442,26 @@ inline bool isectSphere(V3<T> o, V3<T> d, T tnear, T tfar, V3<T> c, T r, Hit<T>&
   const T B = dot(f, d);
   const T C = dot(f, f) - r * r;
   const T disc = B * B - A * C;
-  if (disc < T(0)) return false;
-  const T q = std::sqrt(disc);
-  const T t0 = ts + (-B - q) / A;
-  const T t1 = ts + (-B + q) / A;
+  T t0, t1;
+  bool polished = false;
+  if constexpr (std::is_same_v<T, float>) {
+    if (std::fabs(disc) <= graze::slack(A, r, std::fmax(graze::maxAbs(o), graze::maxAbs(c)))) {
+      const graze::D3 fd = graze::shifted(o, c, d, ts), dd = graze::toD(d);
+      const double rd = r;
+      double x0, x1;
+      if (!graze::quad(graze::dotD(dd, dd), graze::dotD(fd, dd), graze::dotD(fd, fd) - rd * rd, x0, x1))
+        return false;
+      t0 = float(double(ts) + x0);
+      t1 = float(double(ts) + x1);
+      polished = true;
+    }
+  }
+  if (!polished) {
+    if (disc < T(0)) return false;
+    const T q = std::sqrt(disc);
+    t0 = ts + (-B - q) / A;
+    t1 = ts + (-B + q) / A;
+  }
   T t;
   if (tnear <= t0 && t0 <= tfar) t = t0;
   else if (tnear <= t1 && t1 <= tfar) t = t1;
@@ -409,8 +475,13 @@ inline bool isectSphere(V3<T> o, V3<T> d, T tnear, T tfar, V3<T> c, T r, Hit<T>&
 
 // Infinite-cone body roots in a shifted frame. Radius r(hax) = r0 + k*hax
 // along the unit axis w from a. Returns number of roots (0 or 2) in dt[].
+// (FP32 path: oa = (o + d ts) - a rounded; a grazing discriminant is solved
+// again from (o - a) + d ts in double, with the axis and slope from ax =
+// b - a and the end radii; m the magnitude of the rounded coordinates, ext
+// the segment length.)
 template <class T>
-inline int coneBody(V3<T> oa, V3<T> d, V3<T> w, T r0, T k, T dt[2]) {
+inline int coneBody(V3<T> oa, V3<T> d, V3<T> w, T r0, T k, T dt[2], V3<T> o, V3<T> a, V3<T> ax, T r1, T ts, T m,
+                    T ext) {
   const T ho = dot(oa, w);
   const T hd = dot(d, w);
   const T rr = r0 + k * ho;
@@ -419,6 +490,23 @@ inline int coneBody(V3<T> oa, V3<T> d, V3<T> w, T r0, T k, T dt[2]) {
   const T C = dot(oa, oa) - ho * ho - rr * rr;
   if (A == T(0)) return 0;
   const T disc = B * B - A * C;
+  if constexpr (std::is_same_v<T, float>) {
+    if (std::fabs(disc) <= graze::slack(std::fabs(A) + dot(d, d), std::fabs(rr) + std::fabs(r0) + ext, m)) {
+      const graze::D3 od = graze::shifted(o, a, d, ts), dd = graze::toD(d), axd = graze::toD(ax);
+      const double Ld = std::sqrt(graze::dotD(axd, axd));
+      const graze::D3 wd{axd.x / Ld, axd.y / Ld, axd.z / Ld};
+      const double kd = (double(r1) - double(r0)) / Ld;
+      const double hoD = graze::dotD(od, wd), hdD = graze::dotD(dd, wd), rrD = double(r0) + kd * hoD;
+      const double AD = graze::dotD(dd, dd) - hdD * hdD - kd * kd * hdD * hdD;
+      const double BD = graze::dotD(od, dd) - hoD * hdD - kd * hdD * rrD;
+      const double CD = graze::dotD(od, od) - hoD * hoD - rrD * rrD;
+      double x0, x1;
+      if (AD == 0.0 || !graze::quad(AD, BD, CD, x0, x1)) return 0;
+      dt[0] = float(x0);
+      dt[1] = float(x1);
+      return 2;
+    }
+  }
   if (disc < T(0)) return 0;
   const T q = std::sqrt(disc);
   dt[0] = (-B - q) / A;
@@ -447,7 +535,8 @@ inline bool isectCone(V3<T> o, V3<T> d, T tnear, T tfar, V3<T> a, T ra, V3<T> b,
   T bu = T(0);
   V3<T> bn{T(0), T(0), T(0)};
   T dt[2];
-  if (coneBody(oa, d, w, ra, k, dt) == 2) {
+  const T m = std::fmax(maxAbsT(o), maxAbsT(mid));
+  if (coneBody(oa, d, w, ra, k, dt, o, a, ax, rb, ts, m, L) == 2) {
     for (int i = 0; i < 2; ++i) {
       const T t = ts + dt[i];
       if (!(tnear <= t && t <= best)) continue;
@@ -549,7 +638,8 @@ inline bool isectRound(V3<T> o, V3<T> d, T tnear, T tfar, V3<T> a, T ra, V3<T> b
     found = true;
   };
   T dt[2];
-  if (coneBody(oa, d, w, ra, k, dt) == 2) {
+  const T m = std::fmax(maxAbsT(o), maxAbsT(mid));
+  if (coneBody(oa, d, w, ra, k, dt, o, a, ax, rb, ts, m, L) == 2) {
     for (int i = 0; i < 2; ++i) {
       const T hh = ho + dt[i] * hd;
       if (hh < T(0) || hh > L) continue;
@@ -565,9 +655,25 @@ inline bool isectRound(V3<T> o, V3<T> d, T tnear, T tfar, V3<T> a, T ra, V3<T> b
     const T B = dot(f, d);
     const T C = dot(f, f) - r * r;
     const T disc = B * B - A * C;
-    if (disc < T(0)) continue;
-    const T q = std::sqrt(disc);
-    const T r0 = (-B - q) / A, r1 = (-B + q) / A;
+    T r0, r1;
+    bool polished = false;
+    if constexpr (std::is_same_v<T, float>) {
+      if (std::fabs(disc) <= graze::slack(A, r + L, m)) {
+        const graze::D3 fd = graze::subD(graze::shifted(o, a, d, ts), graze::toD(c)), dd = graze::toD(d);
+        const double rd = r;
+        double x0, x1;
+        if (!graze::quad(graze::dotD(dd, dd), graze::dotD(fd, dd), graze::dotD(fd, fd) - rd * rd, x0, x1)) continue;
+        r0 = float(x0);
+        r1 = float(x1);
+        polished = true;
+      }
+    }
+    if (!polished) {
+      if (disc < T(0)) continue;
+      const T q = std::sqrt(disc);
+      r0 = (-B - q) / A;
+      r1 = (-B + q) / A;
+    }
     for (T dtv : {r0, r1}) {
       const T hh = ho + dtv * hd;
       if (e == 0 && hh >= T(0)) continue;
@@ -818,6 +924,84 @@ static void buildScene(const ubg_scene_desc* d, bool brute, OScene& s) {
   }
 }
 
+// Reuse of the last built scene across render calls (orc_set_scene_cache;
+// off by default). The key hashes every array the description points to, so
+// a changed scene always rebuilds; only the timing of repeated renders of
+// one scene changes.
+namespace cache {
+std::mutex mu;
+bool enabled = false;
+uint64_t key = 0;
+std::shared_ptr<OScene> scene;
+
+uint64_t mix(uint64_t h, const void* p, size_t n) {
+  const unsigned char* b = static_cast<const unsigned char*>(p);
+  size_t i = 0;
+  for (; i + 8 <= n; i += 8) {
+    uint64_t w;
+    std::memcpy(&w, b + i, 8);
+    h = (h ^ (w * 0x9E3779B97F4A7C15ull)) * 0xBF58476D1CE4E5B9ull;
+    h ^= h >> 31;
+  }
+  for (; i < n; ++i) h = (h ^ b[i]) * 0x100000001B3ull;
+  return (h ^ n) * 0x94D049BB133111EBull;
+}
+
+uint64_t descKey(const ubg_scene_desc* d, bool brute) {
+  uint64_t h = brute ? 1 : 2;
+  auto arr = [&](const void* p, int64_t count, size_t elem) {
+    const int64_t n = p ? count : -1;
+    h = mix(h, &n, sizeof(n));
+    if (p && count > 0) h = mix(h, p, static_cast<size_t>(count) * elem);
+  };
+  const int64_t nv = d->n_vertices, nt = d->n_triangles, ns = d->n_spheres, nc = d->n_cylinders;
+  arr(d->positions, 3 * nv, 4);
+  arr(d->normals, 3 * nv, 4);
+  arr(d->colors, 4 * nv, 4);
+  arr(d->index, 3 * nt, 4);
+  arr(d->mesh_materials, d->n_mesh_materials, sizeof(ubg_material));
+  arr(d->tri_material, nt, 1);
+  arr(d->tri_group, nt, 2);
+  arr(d->sphere_center_radius, 4 * ns, 4);
+  arr(d->sphere_color, 4 * ns, 4);
+  arr(d->sphere_material, ns, 4);
+  arr(d->sphere_group, ns, 2);
+  arr(d->sphere_from_edge, ns, 1);
+  arr(d->cyl_p0, 3 * nc, 4);
+  arr(d->cyl_p1, 3 * nc, 4);
+  arr(d->cyl_radius, nc, 4);
+  arr(d->cyl_color, 4 * nc, 4);
+  arr(d->cyl_opacity1, nc, 4);
+  arr(d->cyl_material, nc, 4);
+  arr(d->cyl_group, nc, 2);
+  arr(d->cyl_open, nc, 1);
+  arr(d->cyl_from_edge, nc, 1);
+  arr(d->prim_materials, d->n_prim_materials, sizeof(ubg_material));
+  arr(d->instance_offsets, 3 * d->n_instances, 4);
+  return h;
+}
+}  // namespace cache
+
+// The scene of a render call: built here, or the cached one when reuse is on
+// and the description is unchanged.
+static std::shared_ptr<const OScene> sceneFor(const ubg_scene_desc* d, bool brute) {
+  std::lock_guard<std::mutex> lk(cache::mu);
+  if (!cache::enabled) {
+    auto s = std::make_shared<OScene>();
+    buildScene(d, brute, *s);
+    return s;
+  }
+  const uint64_t k = cache::descKey(d, brute);
+  if (!cache::scene || cache::key != k) {
+    cache::scene.reset();
+    auto s = std::make_shared<OScene>();
+    buildScene(d, brute, *s);
+    cache::scene = s;
+    cache::key = k;
+  }
+  return cache::scene;
+}
+
 template <class T>
 static bool isectPrim(const Geometry& g, const PRef& p, V3<T> o, V3<T> d, T tnear, T tfar,
                       Hit<T>& h) {
@@ -857,16 +1041,20 @@ static inline bool boxHit(const Box& b, const double o[3], const double inv[3], 
   return t0 <= t1 * (1.0 + 1e-9) + 1e-12;
 }
 
-// rtcIntersect1 analogue: closest hit in [tnear, tfar]; ties keep the first
-// primitive found.
+// rtcIntersect1 analogue: closest hit in [tnear, tfar]; of primitives hit at
+// exactly the same t the one with the smaller (kind << 30 | id) wins, so the
+// answer does not depend on the BVH (Embree's own tie order is its traversal
+// order, which no test pins).
 template <class T>
 static Hit<T> intersect(const OScene& s, V3<T> o, V3<T> d, T tnear, T tfar) {
   Hit<T> best;
   best.kind = -1;
   best.t = tfar;
   Hit<T> h;
+  auto key = [](int kind, uint32_t id) { return static_cast<uint32_t>(kind) << 30 | id; };
   auto test = [&](const PRef& p) {
-    if (isectPrim(s.g, p, o, d, tnear, best.t, h) && (best.kind < 0 || h.t < best.t)) {
+    if (isectPrim(s.g, p, o, d, tnear, best.t, h) &&
+        (best.kind < 0 || h.t < best.t || (h.t == best.t && key(p.kind, p.id) < key(best.kind, best.id)))) {
       best = h;
       best.kind = p.kind;
       best.id = p.id;
@@ -2326,8 +2514,8 @@ static bool giSupported(const ubg_scene_desc* d, const ubg_render_params* p) {
 template <class T>
 static int renderT(const ubg_scene_desc* desc, const ubg_render_params* p, ubg_frame* fr,
                    const orc_options* opt) {
-  OScene s;
-  buildScene(desc, opt->brute_force != 0, s);
+  const std::shared_ptr<const OScene> scene = sceneFor(desc, opt->brute_force != 0);
+  const OScene& s = *scene;
   const int ss = std::max(1, p->supersample);
   const int W = p->width * ss, H = p->height * ss;
   Ctx<T> c;
@@ -3284,6 +3472,12 @@ int64_t orc_chain_curves(const ubg_scene_desc* desc, int64_t cap, int32_t* src, 
 }
 
 void orc_tea2(uint32_t* v0, uint32_t* v1) { orc::tea2(*v0, *v1); }
+
+void orc_set_scene_cache(int32_t enable) {
+  std::lock_guard<std::mutex> lk(orc::cache::mu);
+  orc::cache::enabled = enable != 0;
+  if (!orc::cache::enabled) orc::cache::scene.reset();
+}
 
 int orc_object_space_edges(const ubg_scene_desc* desc, const ubg_camera* cam, const ubg_edge_options* opt,
                            ubg_edges* out) {

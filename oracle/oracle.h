@@ -30,6 +30,11 @@ typedef struct orc_options {
   int32_t brute_force;  /* 1 = test every primitive (no BVH) */
 } orc_options;
 
+/* Keep the last built scene (geometry + BVH) for later orc_render calls on
+ * an unchanged description (keyed by a hash of its arrays); 0 turns the
+ * reuse off and frees it. Off by default. */
+void orc_set_scene_cache(int32_t enable);
+
 /* Full frame (renderFrame trace/fog/resolve). Outputs are host pointers. */
 int orc_render(const ubg_scene_desc* desc, const ubg_render_params* params,
                ubg_frame* frame, const orc_options* opt);

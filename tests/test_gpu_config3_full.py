@@ -7,8 +7,9 @@ the frame the benchmark's headline is measured on.
 
 * determinism: two renders of the frame are bit-identical (colour, ids, AO,
   ray counts);
-* the full-precision BVH2 and the quantised BVH8 give the same first hits
-  and colours (quantisation only adds node visits);
+* the full-precision BVH2 and the quantised BVH8 give the same first hits,
+  colours and ray counts (quantisation only adds node visits; equal-t hits
+  resolve by (kind, primID), whatever the traversal order);
 * the CPU oracle on the whole frame: the per-ray gates of the benchmark
   (first hit >= 99.99%, |d| <= 1e-3 linear HDR, AO mean |d| <= 1e-3) and
   exact ray accounting (primary + continuation, shaded hits, AO traced and
@@ -80,13 +81,22 @@ def test_deterministic(device, c3, gpu_frame):
 def test_bvh8_matches_full_precision_bvh2(device, c3, gpu_frame):
     sc, opt = c3
     b2, info = _render(device, sc, opt, capi.BVH2)
-    ids = float((b2.hit_id == gpu_frame.hit_id).mean())
+    same = b2.hit_id == gpu_frame.hit_id
+    ids = float(same.mean())
     d = np.abs(b2.color - gpu_frame.color).max(axis=2)
-    print(f"BVH2 vs BVH8: ids agree {ids:.7f}, pixels |d| > 1e-6: {int((d > 1e-6).sum())}", flush=True)
-    assert ids >= 0.99999
+    # primitives hit at exactly the same distance (edge curve segments meeting
+    # at a joint, a curve touching a sphere) resolve by (kind, primID), not by
+    # the traversal order, so a first hit may differ only by an ulp-level
+    # difference of t between the layouts' traversals, never by a tie
+    dt = np.abs(b2.depth - gpu_frame.depth)[~same]
+    tie = dt <= 1e-5 * np.maximum(1.0, np.abs(gpu_frame.depth[~same]))
+    print(f"BVH2 vs BVH8: ids agree {ids:.7f} ({int((~same).sum())} differ, {int(tie.sum())} of them at equal t), "
+          f"pixels |d| > 1e-6: {int((d > 1e-6).sum())}", flush=True)
+    assert ids >= 0.9999
+    assert tie.all()
     assert (d > 1e-6).mean() <= 1e-4
-    assert b2.stats["shaded_hits"] == gpu_frame.stats["shaded_hits"]
-    assert b2.stats["primary_rays"] == gpu_frame.stats["primary_rays"]
+    for k in ("primary_rays", "shaded_hits", "ao_rays_traced", "ao_rays_sampled", "shadow_rays"):
+        assert b2.stats[k] == gpu_frame.stats[k], (k, b2.stats[k], gpu_frame.stats[k])
 
 
 def test_matches_oracle_and_ray_accounting(c3, gpu_frame):

@@ -572,7 +572,10 @@ struct RenderTask::Impl {
   std::thread worker;
   bool collected = false;
 
-  bool ready() const { return frame.valid() && frame.wait_for(std::chrono::seconds(0)) == std::future_status::ready; }
+  // finished: the frame was collected, or the future holds it
+  bool ready() const {
+    return collected || (frame.valid() && frame.wait_for(std::chrono::seconds(0)) == std::future_status::ready);
+  }
   void stop() {
     progress.requestCancel();
     if (worker.joinable()) worker.join();

@@ -148,10 +148,66 @@ struct Hit {
   V3 ng;
 };
 
+// Closest-hit order: the nearer t first; at equal t the smaller
+// (kind << 30 | primID) wins, so which of two primitives hit at exactly the
+// same distance is reported does not depend on the traversal order (BVH
+// layout, builder, leaf size).
+UBG_HD bool closer(const Hit& h, const Hit& best) { return h.t < best.t || (h.t == best.t && h.bits < best.bits); }
+
 // ---------------------------------------------------------------------------
 // Intersectors. A hit is accepted when tnear <= t <= tfar. Spheres and curves
 // solve in a frame moved along the ray to the closest approach of the
 // primitive's center so far cameras do not cancel the discriminant.
+
+// ---------------------------------------------------------------------------
+// FP64 polish of grazing roots. In the shifted frame a sphere's or a cone
+// body's discriminant is a small difference of squares for a ray passing
+// near the silhouette, and its FP32 terms carry the rounding of the shifted
+// origin: an ulp of the coordinates' magnitude m. At large coordinates
+// (m ~ 1e4) a ray grazing within ~m 2^-24 of the silhouette is then dropped
+// or hit at a wrong t. When |disc| lies within that error budget
+// (grazeSlack) the quadratic is set up again in FP64 from the exact inputs
+// -- the shifted origin rebuilt as (o - p) + d ts, exact float inputs
+// subtracted in double -- solved there, and each root takes one Newton step
+// on the FP64 quadratic. Away from the silhouette the FP32 path is untouched.
+struct V3d {
+  double x, y, z;
+};
+UBG_HD V3d toD(V3 v) { return V3d{v.x, v.y, v.z}; }
+UBG_HD double dotD(V3d a, V3d b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+UBG_HD V3d subD(V3d a, V3d b) { return V3d{a.x - b.x, a.y - b.y, a.z - b.z}; }
+// (o - p) + d ts in FP64
+UBG_HD V3d shiftedD(V3 o, V3 p, V3 d, float ts) {
+  const double t = ts;
+  return V3d{(static_cast<double>(o.x) - static_cast<double>(p.x)) + static_cast<double>(d.x) * t,
+             (static_cast<double>(o.y) - static_cast<double>(p.y)) + static_cast<double>(d.y) * t,
+             (static_cast<double>(o.z) - static_cast<double>(p.z)) + static_cast<double>(d.z) * t};
+}
+UBG_HD float maxAbs(V3 v) { return fmaxf(fmaxf(fabsf(v.x), fabsf(v.y)), fabsf(v.z)); }
+// |disc| below which the FP32 discriminant is not trusted: scale x size x
+// magnitude of the rounded coordinates, with a 2^5 margin over 2^-24.
+UBG_HD float grazeSlack(float scale, float size, float m) { return scale * 0x1.0p-19f * size * m; }
+// Roots of A x^2 + 2 B x + C in FP64, each with one Newton step (taken when
+// it moves the root by at most half the roots' separation).
+UBG_HD bool quadPolish64(double A, double B, double C, double& x0, double& x1) {
+  const double disc = B * B - A * C;
+  if (disc < 0.0) return false;
+  const double q = sqrt(disc);
+  x0 = (-B - q) / A;
+  x1 = (-B + q) / A;
+  const double half = 0.5 * fabs(x1 - x0);
+  double* xs[2] = {&x0, &x1};
+  for (int i = 0; i < 2; ++i) {
+    const double x = *xs[i];
+    const double g = (A * x + 2.0 * B) * x + C;
+    const double gp = 2.0 * (A * x + B);
+    if (gp != 0.0) {
+      const double st = g / gp;
+      if (fabs(st) <= half) *xs[i] = x - st;
+    }
+  }
+  return true;
+}
 
 UBG_HD bool isectSphere(V3 o, V3 d, float tnear, float tfar, V3 c, float r, Hit& h) {
   const float A = dot(d, d);
@@ -160,10 +216,20 @@ UBG_HD bool isectSphere(V3 o, V3 d, float tnear, float tfar, V3 c, float r, Hit&
   const float B = dot(f, d);
   const float C = dot(f, f) - r * r;
   const float disc = B * B - A * C;
-  if (disc < 0.0f) return false;
-  const float q = sqrtf(disc);
-  const float t0 = ts + (-B - q) / A;
-  const float t1 = ts + (-B + q) / A;
+  float t0, t1;
+  if (fabsf(disc) <= grazeSlack(A, r, fmaxf(maxAbs(o), maxAbs(c)))) {
+    const V3d fd = shiftedD(o, c, d, ts), dd = toD(d);
+    const double rd = r;
+    double x0, x1;
+    if (!quadPolish64(dotD(dd, dd), dotD(fd, dd), dotD(fd, fd) - rd * rd, x0, x1)) return false;
+    t0 = static_cast<float>(static_cast<double>(ts) + x0);
+    t1 = static_cast<float>(static_cast<double>(ts) + x1);
+  } else {
+    if (disc < 0.0f) return false;
+    const float q = sqrtf(disc);
+    t0 = ts + (-B - q) / A;
+    t1 = ts + (-B + q) / A;
+  }
   float t;
   if (tnear <= t0 && t0 <= tfar) t = t0;
   else if (tnear <= t1 && t1 <= tfar) t = t1;
@@ -177,7 +243,12 @@ UBG_HD bool isectSphere(V3 o, V3 d, float tnear, float tfar, V3 c, float r, Hit&
 
 // Roots (in the shifted frame) of the infinite cone r(h) = r0 + k*h around
 // the unit axis w through the origin of `oa`. Returns false when none.
-UBG_HD bool coneBody(V3 oa, V3 d, V3 w, float r0, float k, float& dt0, float& dt1) {
+// oa = (o + d ts) - a rounded in FP32; a grazing discriminant is solved again
+// from (o - a) + d ts in FP64, with the axis and slope from the segment ax =
+// b - a and the end radii (m: magnitude of the rounded coordinates, ext: the
+// segment length, bounding |oa|).
+UBG_HD bool coneBody(V3 oa, V3 d, V3 w, float r0, float k, float& dt0, float& dt1, V3 o, V3 a, V3 ax, float r1,
+                     float ts, float m, float ext) {
   const float ho = dot(oa, w);
   const float hd = dot(d, w);
   const float rr = r0 + k * ho;
@@ -186,6 +257,22 @@ UBG_HD bool coneBody(V3 oa, V3 d, V3 w, float r0, float k, float& dt0, float& dt
   const float C = dot(oa, oa) - ho * ho - rr * rr;
   if (A == 0.0f) return false;
   const float disc = B * B - A * C;
+  if (fabsf(disc) <= grazeSlack(fabsf(A) + dot(d, d), fabsf(rr) + fabsf(r0) + ext, m)) {
+    // the axis, its length and the slope again in FP64 from the segment
+    const V3d od = shiftedD(o, a, d, ts), dd = toD(d), axd = toD(ax);
+    const double Ld = sqrt(dotD(axd, axd));
+    const V3d wd{axd.x / Ld, axd.y / Ld, axd.z / Ld};
+    const double kd = (static_cast<double>(r1) - static_cast<double>(r0)) / Ld;
+    const double hoD = dotD(od, wd), hdD = dotD(dd, wd), rrD = static_cast<double>(r0) + kd * hoD;
+    const double AD = dotD(dd, dd) - hdD * hdD - kd * kd * hdD * hdD;
+    const double BD = dotD(od, dd) - hoD * hdD - kd * hdD * rrD;
+    const double CD = dotD(od, od) - hoD * hoD - rrD * rrD;
+    double x0, x1;
+    if (AD == 0.0 || !quadPolish64(AD, BD, CD, x0, x1)) return false;
+    dt0 = static_cast<float>(x0);
+    dt1 = static_cast<float>(x1);
+    return true;
+  }
   if (disc < 0.0f) return false;
   const float q = sqrtf(disc);
   dt0 = (-B - q) / A;
@@ -211,7 +298,7 @@ UBG_HD bool isectCone(V3 o, V3 d, float tnear, float tfar, V3 a, float ra, V3 b,
   float best = tfar, bu = 0.0f;
   V3 bn = mk(0.0f, 0.0f, 0.0f);
   float r0, r1;
-  if (coneBody(oa, d, w, ra, k, r0, r1)) {
+  if (coneBody(oa, d, w, ra, k, r0, r1, o, a, ax, rb, ts, fmaxf(maxAbs(o), maxAbs(mid)), L)) {
 #pragma unroll
     for (int i = 0; i < 2; ++i) {
       const float dt = i == 0 ? r0 : r1;
@@ -285,6 +372,7 @@ UBG_HD bool isectRound(V3 o, V3 d, float tnear, float tfar, V3 a, float ra, V3 b
   const V3 oa = (o + d * ts) - a;
   const float ho = dot(oa, w);
   const float hd = dot(d, w);
+  const float m = fmaxf(maxAbs(o), maxAbs(mid));
   bool found = false;
   float best = tfar, bu = 0.0f;
   V3 bn = mk(0.0f, 0.0f, 0.0f);
@@ -293,7 +381,7 @@ UBG_HD bool isectRound(V3 o, V3 d, float tnear, float tfar, V3 a, float ra, V3 b
   int ckind[6];  // 0 body, 1 sphere at a, 2 sphere at b, -1 none
   {
     float r0 = 0.0f, r1 = 0.0f;
-    const bool ok = coneBody(oa, d, w, ra, k, r0, r1);
+    const bool ok = coneBody(oa, d, w, ra, k, r0, r1, o, a, ax, rb, ts, m, L);
     cdt[0] = r0;
     cdt[1] = r1;
     ckind[0] = ok ? 0 : -1;
@@ -307,10 +395,20 @@ UBG_HD bool isectRound(V3 o, V3 d, float tnear, float tfar, V3 a, float ra, V3 b
     const float B = dot(f, d);
     const float C = dot(f, f) - r * r;
     const float disc = B * B - A * C;
-    const bool ok = disc >= 0.0f;
-    const float q = ok ? sqrtf(disc) : 0.0f;
-    cdt[2 + 2 * e] = (-B - q) / A;
-    cdt[3 + 2 * e] = (-B + q) / A;
+    bool ok;
+    if (fabsf(disc) <= grazeSlack(A, r + L, m)) {
+      const V3d fd = subD(shiftedD(o, a, d, ts), toD(c)), dd = toD(d);
+      const double rd = r;
+      double x0 = 0.0, x1 = 0.0;
+      ok = quadPolish64(dotD(dd, dd), dotD(fd, dd), dotD(fd, fd) - rd * rd, x0, x1);
+      cdt[2 + 2 * e] = static_cast<float>(x0);
+      cdt[3 + 2 * e] = static_cast<float>(x1);
+    } else {
+      ok = disc >= 0.0f;
+      const float q = ok ? sqrtf(disc) : 0.0f;
+      cdt[2 + 2 * e] = (-B - q) / A;
+      cdt[3 + 2 * e] = (-B + q) / A;
+    }
     ckind[2 + 2 * e] = ok ? 1 + e : -1;
     ckind[3 + 2 * e] = ok ? 1 + e : -1;
   }
@@ -422,7 +520,7 @@ UBG_HD V3 safeInv(V3 d) {
 // BVH2 traversal (full-precision nodes). Stack slot i of this lane lives at
 // stk[i * stride] (LDS on the GPU: stride = block size, so the 64 lanes of a
 // wave use 64 distinct banks). kAny returns at the first hit in
-// [tnear, tfar]; otherwise the closest hit (strictly nearer hits replace it).
+// [tnear, tfar]; otherwise the closest hit in the order of closer().
 template <bool kAny, bool kCount, class F = AcceptAll>
 UBG_HD bool traverse2(const SceneView& s, V3 o, V3 d, float tnear, float tfar, Hit& best, uint32_t* stk,
                       int stride, uint32_t* nodeVisits, uint32_t* primTests, const F& filt = F()) {
@@ -471,7 +569,7 @@ UBG_HD bool traverse2(const SceneView& s, V3 o, V3 d, float tnear, float tfar, H
         if (kCount) ++*primTests;
         if (isectPrimF(s, i, o, d, tnear, tmax, h, filt)) {
           if (kAny) return true;
-          if (!found || h.t < best.t) {
+          if (!found || closer(h, best)) {
             best = h;
             tmax = h.t;
             found = true;
@@ -598,7 +696,7 @@ UBG_HD bool traverse8(const SceneView& s, V3 o, V3 d, float tnear, float tfar, H
       if (kCount) ++*primTests;
       if (isectPrimF(s, tBase + b, o, d, tnear, tmax, h, filt)) {
         if (kAny) return true;
-        if (!found || h.t < best.t) {
+        if (!found || closer(h, best)) {
           best = h;
           tmax = h.t;
           found = true;
@@ -746,7 +844,7 @@ UBG_HD int trav8PrimClosest(const SceneView& s, Trav8& t, Hit& best, bool& found
   if (kCount) ++*primTests;
   Hit h;
   if (isectPrim(s, t.tBase + b, t.o, t.d, t.tnear, t.tmax, h)) {
-    if (!found || h.t < best.t) {
+    if (!found || closer(h, best)) {
       best = h;
       t.tmax = h.t;
       found = true;
