@@ -398,6 +398,29 @@ int ubg_scene_get_info(const ubg_scene* scene, ubg_scene_info* out);
 int ubg_render(ubg_context* ctx, const ubg_scene* scene,
                const ubg_render_params* params, ubg_frame* frame);
 
+/* Size of the fixed frame scratch of ubg_render_noalloc: one device array
+ * of this many bytes per GPU the library runs on, held until the process
+ * ends. */
+#define UBG_NOALLOC_SCRATCH_BYTES (320u << 20)
+
+/* ubg_render without device allocation. It has no counterpart in the
+ * reference; it is for callers that cannot allocate device memory during a
+ * frame (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc). Same kernels and same images as ubg_render, but every
+ * frame-scratch buffer (ray queues, walk state, hi-res colour, depth and view
+ * z, counters, adaptive-AA and GI work arrays) is carved from the library's
+ * fixed per-GPU scratch array of UBG_NOALLOC_SCRATCH_BYTES (320 MiB: a
+ * 1920x1080 frame at 1 spp with AO, shadows and transparency fits; about
+ * 150 bytes per hi-res pixel are needed), so it calls no hipMalloc / hipFree.
+ * The outputs must be device buffers (frame->memory == 1); a frame that needs
+ * more scratch, UBG_FLAG_KERNEL_TIMES (its events are created on demand) and
+ * the learned denoiser (its activations are its own allocations) are
+ * rejected. Every call on one GPU shares that GPU's array: calls that overlap
+ * on one GPU (from two contexts, or concurrently with another process's
+ * context on the same device) race; calls on one context are serialized. */
+int ubg_render_noalloc(ubg_context* ctx, const ubg_scene* scene,
+                       const ubg_render_params* params, ubg_frame* frame);
+
 /* Batched closest-hit and any-hit queries (host or device buffers, per
  * `memory` as in ubg_frame). occluded[i] = 1 when any primitive lies in
  * [tnear, tfar] along ray i. */

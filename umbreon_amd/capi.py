@@ -215,7 +215,7 @@ def hit_dtype():
 EXPORTS = [
     "ubg_abi_version", "ubg_status_string", "ubg_context_create", "ubg_context_destroy",
     "ubg_last_error", "ubg_scene_create", "ubg_scene_create_ex", "ubg_scene_destroy", "ubg_scene_get_info",
-    "ubg_render", "ubg_intersect", "ubg_occluded", "ubg_occluded_segments", "ubg_object_space_edges",
+    "ubg_render", "ubg_render_noalloc", "ubg_intersect", "ubg_occluded", "ubg_occluded_segments", "ubg_object_space_edges",
     "ubg_edges_free", "ubg_unet_create", "ubg_unet_destroy", "ubg_unet_param_count", "ubg_unet_get_params",
     "ubg_unet_set_params", "ubg_unet_infer", "ubg_unet_train_step",
 ]
@@ -264,6 +264,8 @@ def load() -> C.CDLL:
     lib.ubg_scene_get_info.restype = C.c_int
     lib.ubg_render.argtypes = [C.c_void_p, C.c_void_p, C.POINTER(RenderParams), C.POINTER(Frame)]
     lib.ubg_render.restype = C.c_int
+    lib.ubg_render_noalloc.argtypes = [C.c_void_p, C.c_void_p, C.POINTER(RenderParams), C.POINTER(Frame)]
+    lib.ubg_render_noalloc.restype = C.c_int
     lib.ubg_intersect.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_int32]
     lib.ubg_intersect.restype = C.c_int
     lib.ubg_occluded.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_int32]

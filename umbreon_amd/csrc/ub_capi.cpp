@@ -79,6 +79,8 @@ struct ubg_context {
   // AOV, the padded network input, its prediction and a max-depth word
   DevBuf unetIn, unetTgt, unetOut, giFeat, giFeatP, giPred, giMax;
   unsigned long long* hostCounters = nullptr;  // pinned
+  void* staticScratch = nullptr;               // device address of the fixed scratch (ub_static.hip)
+  hipError_t staticStatus = hipSuccess;
   std::vector<DevBuf*> all() {
     return {&hitPix, &hitT,   &hitO,    &hitN,  &hitC,  &hitAo,   &activeA,  &activeB,  &acc,     &tnear,
             &hiColor, &depth, &viewZ,   &hitId, &aoOpen, &outColor, &counters, &query, &queryOut, &aoT,
@@ -106,7 +108,27 @@ struct ubg_scene {
 
 namespace {
 
+// Bump allocation of frame scratch from the fixed per-GPU array during a
+// ubg_render_noalloc call (set on the calling thread for that call only).
+struct StaticArena {
+  char* base = nullptr;
+  size_t cap = 0, used = 0;
+};
+thread_local StaticArena* t_arena = nullptr;
+
 int ensure(DevBuf& b, size_t bytes) {
+  if (t_arena) {
+    if (b.p && b.bytes >= bytes) return UBG_OK;
+    const size_t want = (std::max<size_t>(bytes, 256) + 255) & ~size_t(255);
+    if (t_arena->used + want > t_arena->cap)
+      return fail(UBG_ERR_OUT_OF_MEMORY, "ubg_render_noalloc: the frame needs more scratch than the fixed " +
+                                             std::to_string(t_arena->cap >> 20) + " MiB (" +
+                                             std::to_string((t_arena->used + want) >> 20) + "+ MiB requested)");
+    b.p = t_arena->base + t_arena->used;
+    b.bytes = want;
+    t_arena->used += want;
+    return UBG_OK;
+  }
   if (b.bytes >= bytes && b.p) return UBG_OK;
   if (b.p) {
     (void)hipFree(b.p);
@@ -207,6 +229,9 @@ int ubg_context_create(int device_ordinal, ubg_context** out) {
     ubg_context_destroy(c);
     return fail(UBG_ERR_OUT_OF_MEMORY, "hipHostMalloc(counters) failed");
   }
+  // this device's copy of the fixed scratch of ubg_render_noalloc (device
+  // current); a failed lookup only disables that entry point
+  c->staticStatus = ubg::staticScratchAddress(device_ordinal, &c->staticScratch);
   *out = c;
   return UBG_OK;
 }
@@ -382,8 +407,9 @@ int ubg_scene_get_info(const ubg_scene* scene, ubg_scene_info* out) {
   return UBG_OK;
 }
 
-int ubg_render(ubg_context* ctx, const ubg_scene* scene, const ubg_render_params* p,
-               ubg_frame* fr) {
+// One frame; haveLock: the caller already holds ctx->mu.
+static int renderFrame(ubg_context* ctx, const ubg_scene* scene, const ubg_render_params* p, ubg_frame* fr,
+                       bool haveLock) {
   if (!ctx || !scene || !p || !fr) return fail(UBG_ERR_INVALID_ARGUMENT, "ubg_render: null argument");
   if (scene->ctx != ctx) return fail(UBG_ERR_INVALID_ARGUMENT, "ubg_render: scene belongs to another context");
   if (p->width <= 0 || p->height <= 0) return fail(UBG_ERR_INVALID_ARGUMENT, "ubg_render: width/height must be > 0");
@@ -401,7 +427,8 @@ int ubg_render(ubg_context* ctx, const ubg_scene* scene, const ubg_render_params
   const int npx = W * H;
   const bool countVisits = (p->flags & UBG_FLAG_COUNT_VISITS) != 0u;
 
-  std::lock_guard<std::mutex> lk(ctx->mu);
+  std::unique_lock<std::mutex> lk(ctx->mu, std::defer_lock);
+  if (!haveLock) lk.lock();
   UBG_HIP(hipSetDevice(ctx->device), "hipSetDevice");
   hipStream_t st = ctx->stream;
 
@@ -1100,6 +1127,51 @@ static int runQuery(ubg_context* ctx, const ubg_scene* scene, int64_t n, const u
             "hipMemcpy(query result)");
   UBG_HIP(hipStreamSynchronize(st), "query sync");
   return UBG_OK;
+}
+
+int ubg_render(ubg_context* ctx, const ubg_scene* scene, const ubg_render_params* p, ubg_frame* fr) {
+  return renderFrame(ctx, scene, p, fr, false);
+}
+
+int ubg_render_noalloc(ubg_context* ctx, const ubg_scene* scene, const ubg_render_params* p, ubg_frame* fr) {
+  if (!ctx || !scene || !p || !fr) return fail(UBG_ERR_INVALID_ARGUMENT, "ubg_render_noalloc: null argument");
+  if (fr->memory != 1)
+    return fail(UBG_ERR_INVALID_ARGUMENT, "ubg_render_noalloc: the outputs must be device buffers (memory 1)");
+  if (p->flags & UBG_FLAG_KERNEL_TIMES)
+    return fail(UBG_ERR_INVALID_ARGUMENT, "ubg_render_noalloc: UBG_FLAG_KERNEL_TIMES creates events; not allowed");
+  if (p->gi && p->pt1_denoise == 2)
+    return fail(UBG_ERR_UNSUPPORTED, "ubg_render_noalloc: the learned denoiser allocates its activations");
+  if (ctx->staticStatus != hipSuccess || !ctx->staticScratch)
+    return hipFail(ctx->staticStatus != hipSuccess ? ctx->staticStatus : hipErrorInvalidSymbol,
+                   "ubg_render_noalloc: fixed scratch address");
+  // The basic path's scratch (ray queues and per-pixel state) must fit before
+  // anything is launched; a feature's extra arrays are checked as carved.
+  const int ss = std::max(1, p->supersample);
+  const int64_t W = static_cast<int64_t>(std::max(0, p->width)) * ss, H = static_cast<int64_t>(std::max(0, p->height)) * ss;
+  if (W > 0 && H > 0 && W * H < (int64_t(1) << 31)) {
+    const int64_t qn = static_cast<int64_t>(ubg::kQueueParts) * ubg::queuePartCap(static_cast<int>(W), static_cast<int>(H));
+    const int64_t need = qn * 68 + W * H * 52 + static_cast<int64_t>(p->width) * p->height * 16;
+    if (need > static_cast<int64_t>(UBG_NOALLOC_SCRATCH_BYTES))
+      return fail(UBG_ERR_INVALID_ARGUMENT, "ubg_render_noalloc: a " + std::to_string(W) + " x " + std::to_string(H) +
+                                                " hi-res frame needs " + std::to_string(need >> 20) +
+                                                " MiB of scratch, more than the fixed " +
+                                                std::to_string(UBG_NOALLOC_SCRATCH_BYTES >> 20) + " MiB");
+  }
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  // The context's own (allocated) buffers step aside for the call and come
+  // back afterwards: ensure() carves from the fixed array meanwhile.
+  std::vector<DevBuf> saved;
+  for (DevBuf* b : ctx->all()) {
+    saved.push_back(*b);
+    *b = DevBuf{};
+  }
+  StaticArena arena{static_cast<char*>(ctx->staticScratch), UBG_NOALLOC_SCRATCH_BYTES, 0};
+  t_arena = &arena;
+  const int st = renderFrame(ctx, scene, p, fr, true);
+  t_arena = nullptr;
+  const std::vector<DevBuf*> bufs = ctx->all();
+  for (size_t i = 0; i < bufs.size(); ++i) *bufs[i] = saved[i];
+  return st;
 }
 
 int ubg_intersect(ubg_context* ctx, const ubg_scene* scene, int64_t n_rays, const ubg_ray* rays,

@@ -127,6 +127,10 @@ void launchGiComposite(const FrameConst& fc, const FrameBufs& b, float* indirect
 // Queue slots per partition for a W x H hi-res grid.
 int queuePartCap(int W, int H);
 
+// Device address of this device's copy of the fixed frame scratch
+// (UBG_NOALLOC_SCRATCH_BYTES, ub_static.hip); looked up once per device.
+hipError_t staticScratchAddress(int device, void** out);
+
 // Per-partition fills of the continuation queue a layer consumes.
 struct PartFill {
   uint32_t n[kQueueParts];
