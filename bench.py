@@ -123,6 +123,33 @@ def measured_traffic(workload: str, layout: str, kernel: str):
     return None
 
 
+COMPUTE_TABLE = ROOT / "profiles" / "pmc_valu_lds_l2_surface_gi_unet.md"
+COMPUTE_SECTIONS = {"surface-edges": "## Molecular surface + object-space edges", "gi": "## pt1 GI on the molecular surface"}
+
+
+def measured_compute(workload: str, kernel: str):
+    """VALU issue share, lane utilization and L2 hit rate of `kernel` from the
+    committed rocprofv3 counter summary; None without a row for it."""
+    head = COMPUTE_SECTIONS.get(workload)
+    if head is None or not COMPUTE_TABLE.exists():
+        return None
+    inside = False
+    for line in COMPUTE_TABLE.read_text().splitlines():
+        if line.startswith("## "):
+            inside = line.startswith(head)
+            continue
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if inside and len(cells) >= 10 and cells[0].strip("`").startswith(kernel + "<false"):
+            try:
+                return {"valu_issue_share": round(float(cells[4]) / 100, 3),
+                        "valu_lane_utilization": round(float(cells[5]) / 100, 3),
+                        "l2_hit": round(float(cells[8]) / 100, 3),
+                        "source": COMPUTE_TABLE.name}
+            except ValueError:
+                return None
+    return None
+
+
 def calib_copy(mib: int) -> None:
     """One hipMemcpy device-to-device of `mib` MiB (the runtime's copy kernel:
     reads and writes exactly that many bytes), for counter calibration."""
@@ -284,13 +311,15 @@ def main() -> int:
              "resolve": "k_resolve"}[dom]
     if args.workload == "gi" and dom == "ao":
         kname = "k_gi_gather"
-    per_launch = measured_traffic(args.workload, args.layout, kname)
+    traffic_key = "surface-edges" if (args.workload == "surface" and not args.no_edges) else args.workload
+    per_launch = measured_traffic(traffic_key, args.layout, kname)
     launches = kn[dom] // args.steps
     roofline = {
         "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 4),
         "traffic": None if per_launch is None else int(per_launch * launches),
         "kernel": kname,
+        "compute": measured_compute(traffic_key, kname),
         "kernel_ms_per_frame": round(dur_s * 1e3, 4),
         "launches_per_frame": kn[dom] // args.steps,
         "algorithmic_bytes_per_frame": int(bytes_per_frame),
