@@ -328,6 +328,27 @@ def main() -> int:
                     "node_bytes": node_b, "prim_bytes": prim_b, "ray_bytes": 48},
     }
 
+    # the same frame through the BVH8 built on the GPU (the C++ facade's
+    # builder): build time and frame rate
+    gpu_build = None
+    if args.layout == "bvh8":
+        bg = BuiltScene(dev, scene, capi.BVH8_GPU_BUILD)
+        try:
+            for _ in range(max(1, args.warmup)):
+                bg.render(scene, opt, outputs=False)
+            tg = time.perf_counter()
+            for _ in range(args.steps):
+                fg = bg.render(scene, opt, outputs=False)
+            eg = time.perf_counter() - tg
+            assert fg.total_rays == rays_per_frame
+            gpu_build = {"build_ms": round(bg.info["build_seconds"] * 1e3, 1), "bvh_nodes": bg.info["n_nodes"],
+                         "bvh_depth": bg.info["bvh_depth"], "ms_per_frame": round(eg / args.steps * 1e3, 3),
+                         "mrays_per_s": round(rays_per_frame * args.steps / eg / 1e6, 1)}
+            log(f"GPU-built BVH8: build {gpu_build['build_ms']} ms, {gpu_build['ms_per_frame']} ms/frame, "
+                f"{gpu_build['mrays_per_s']} Mrays/s")
+        finally:
+            bg.close()
+
     if args.calib_copy_mib > 0:
         calib_copy(args.calib_copy_mib)
 
@@ -363,6 +384,7 @@ def main() -> int:
                               "gi_gbuffer": st["gi_gbuffer_rays"]} if args.workload == "gi" else {})},
             "bvh_layout": args.layout,
             "bvh_build_ms": round(info["build_seconds"] * 1e3, 2),
+            "gpu_built_bvh8": gpu_build,
             "bvh_nodes": info["n_nodes"],
             "bvh_depth": info["bvh_depth"],
             "object_space_edges": edges_info,

@@ -45,7 +45,7 @@ def test_facade_links_only_the_c_abi():
     # the facade reaches the GPU only through the C-ABI symbols
     und = subprocess.run(["nm", "-D", "--undefined-only", str(LIB / "libumbreon.so")], capture_output=True,
                          text=True, check=True).stdout
-    assert set(re.findall(r"\b(ubg_\w+)", und)) >= {"ubg_context_create", "ubg_scene_create", "ubg_render"}
+    assert set(re.findall(r"\b(ubg_\w+)", und)) >= {"ubg_context_create", "ubg_scene_create_ex", "ubg_render"}
     assert "hipLaunchKernel" not in und
 
 

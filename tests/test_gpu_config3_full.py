@@ -10,6 +10,7 @@ the frame the benchmark's headline is measured on.
 * the full-precision BVH2 and the quantised BVH8 give the same first hits,
   colours and ray counts (quantisation only adds node visits; equal-t hits
   resolve by (kind, primID), whatever the traversal order);
+* the BVH8 built on the GPU (the facade's builder) renders the same frame;
 * the CPU oracle on the whole frame: the per-ray gates of the benchmark
   (first hit >= 99.99%, |d| <= 1e-3 linear HDR, AO mean |d| <= 1e-3) and
   exact ray accounting (primary + continuation, shaded hits, AO traced and
@@ -97,6 +98,23 @@ def test_bvh8_matches_full_precision_bvh2(device, c3, gpu_frame):
     assert (d > 1e-6).mean() <= 1e-4
     for k in ("primary_rays", "shaded_hits", "ao_rays_traced", "ao_rays_sampled", "shadow_rays"):
         assert b2.stats[k] == gpu_frame.stats[k], (k, b2.stats[k], gpu_frame.stats[k])
+
+
+def test_gpu_built_bvh8_matches(device, c3, gpu_frame):
+    """The BVH8 built on the GPU (Morton order + PLOC + the SAH-optimal
+    8-wide collapse; what the C++ facade uses) renders the frame the
+    host-built tree does: same first hits, colours and ray counts."""
+    sc, opt = c3
+    g, info = _render(device, sc, opt, capi.BVH8_GPU_BUILD)
+    same = g.hit_id == gpu_frame.hit_id
+    d = np.abs(g.color - gpu_frame.color).max(axis=2)
+    print(f"GPU-built BVH8: build {info['build_seconds'] * 1e3:.0f} ms, {info['n_nodes']} nodes, depth "
+          f"{info['bvh_depth']}, SAH {info['sah_cost']:.1f}; ids agree {float(same.mean()):.7f}, "
+          f"pixels |d| > 1e-6: {int((d > 1e-6).sum())}", flush=True)
+    assert float(same.mean()) >= 0.99999
+    assert (d > 1e-6).mean() <= 1e-4
+    for k in ("primary_rays", "shaded_hits", "ao_rays_traced", "ao_rays_sampled", "shadow_rays"):
+        assert g.stats[k] == gpu_frame.stats[k], (k, g.stats[k], gpu_frame.stats[k])
 
 
 def test_matches_oracle_and_ray_accounting(c3, gpu_frame):

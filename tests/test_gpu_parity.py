@@ -172,6 +172,27 @@ def test_config3_surface_reduced(device):
     assert p >= 60.0
 
 
+@pytest.mark.parametrize("which", ["ballstick", "surface"])
+def test_gpu_built_bvh8_renders_match_oracle(device, which):
+    """Frames traced through the GPU-built BVH8 against the CPU oracle under
+    the benchmark gates, with exact ray accounting."""
+    from umbreon_amd.render import BuiltScene
+    if which == "ballstick":
+        sc, opt = ball_and_stick_scene(n_atoms=6000, width=320, height=180)
+    else:
+        sc, opt = molecular_surface_scene(n_atoms=3000, width=320, height=180)
+    bs = BuiltScene(device, sc, capi.BVH8_GPU_BUILD)
+    try:
+        g = bs.render(sc, opt, hires_only=True, want_ids=True, want_ao=True)
+        print(f"{which}: GPU build {bs.info['build_seconds'] * 1e3:.1f} ms, {bs.info['n_nodes']} nodes", flush=True)
+    finally:
+        bs.close()
+    c = ob.render(sc, opt, hires_only=True)
+    _compare(f"{which} (GPU-built BVH8)", g, c, ao=True)
+    for k in ("primary_rays", "shaded_hits", "ao_rays_traced", "ao_rays_sampled", "shadow_rays"):
+        assert g.stats[k] == c.stats[k], (k, g.stats[k], c.stats[k])
+
+
 def test_gpu_built_bvh8_matches_host_built():
     """The GPU builder (Morton + PLOC + SAH-optimal collapse) gives the same
     closest hits as the host builder on random rays (equal t; the primitive

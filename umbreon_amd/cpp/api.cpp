@@ -3,7 +3,7 @@
 // Host orchestration of the reference's public flow (umbreon.cpp:141-341 and
 // pipeline.cpp:20-256): option checks, the group-alpha multipass blend in the
 // sRGB domain, and per pass the scene flattening into the C-ABI's
-// structure-of-arrays description (umbreon_gpu.h), one ubg_scene_create
+// structure-of-arrays description (umbreon_gpu.h), one ubg_scene_create_ex
 // (upload + BVH) and one ubg_render (hi-res trace + shade + fog, downsample,
 // gamma) on a process-wide device context. Failures surface as
 // std::runtime_error with the C-ABI's message.
@@ -325,7 +325,10 @@ FrameResult renderPass(const Scene& scene, const GroupFilter& groups, const Rend
     return f;
   }
   ubg_scene* gs = nullptr;
-  int st = ubg_scene_create(ctx, &flat.desc, &gs);
+  // the reference rebuilds its BVH on every render (scene_build.cpp:133-170);
+  // here the BVH8 is built on the GPU (Morton order + PLOC + the 8-wide
+  // collapse), the host only flattening the scene into primitive records
+  int st = ubg_scene_create_ex(ctx, &flat.desc, UBG_BVH8_GPU_BUILD, &gs);
   if (st != UBG_OK) throwStatus(st, "scene build");
   ubg_scene_info info{};
   ubg_scene_get_info(gs, &info);

@@ -1,7 +1,7 @@
 // ub_devbuild.hip -- BVH8 construction on the GPU.
 //
 // 1. Morton order: centroid bounds (block reduction + ordered-int atomics),
-//    63-bit Morton codes (21 bits per axis), radix sort (hipCUB).
+//    63-bit Morton codes (21 bits per axis), rocPRIM's device radix sort.
 // 2. Binary tree by PLOC (Meister and Bittner, "Parallel Locally-Ordered
 //    Clustering for Bounding Volume Hierarchy Construction", TVCG 2018):
 //    clusters start as the sorted leaves; each round every cluster finds the
@@ -24,7 +24,8 @@
 //    vary from run to run; what a ray visits and in which order does not.
 #include "ub_devbuild.h"
 
-#include <hipcub/hipcub.hpp>
+#include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_scan.hpp>
 
 #include <algorithm>
 #include <chrono>
@@ -543,13 +544,15 @@ int deviceBuildBvh8(const float4* recs, const float4* boxes, int64_t n, hipStrea
                      bounds);
   hipLaunchKernelGGL(k_morton, dim3(gridFor(n, kBlock)), dim3(kBlock), 0, st, boxes, n, bounds, keys, vals);
   size_t tmpBytes = 0, scanBytes = 0;
-  ok(hipcub::DeviceRadixSort::SortPairs(nullptr, tmpBytes, keys, keys2, vals, perm, static_cast<int>(n), 0, 63, st),
+  ok(rocprim::radix_sort_pairs(nullptr, tmpBytes, keys, keys2, vals, perm, static_cast<size_t>(n), 0u, 63u, st),
      "sort size");
-  ok(hipcub::DeviceScan::ExclusiveSum(nullptr, scanBytes, packed, pre, static_cast<int>(n), st), "scan size");
+  ok(rocprim::exclusive_scan(nullptr, scanBytes, packed, pre, 0ull, static_cast<size_t>(n),
+                             rocprim::plus<unsigned long long>(), st),
+     "scan size");
   void* tmp = nullptr;
   ok(mem.alloc(reinterpret_cast<uint8_t**>(&tmp), std::max(tmpBytes, scanBytes)), "alloc");
   if (e != hipSuccess) return UBG_ERR_OUT_OF_MEMORY;
-  ok(hipcub::DeviceRadixSort::SortPairs(tmp, tmpBytes, keys, keys2, vals, perm, static_cast<int>(n), 0, 63, st),
+  ok(rocprim::radix_sort_pairs(tmp, tmpBytes, keys, keys2, vals, perm, static_cast<size_t>(n), 0u, 63u, st),
      "sort");
   hipLaunchKernelGGL(k_leaves, dim3(gridFor(n, kBlock)), dim3(kBlock), 0, st, boxes, perm, n, t, cl);
   ok(hipGetLastError(), "launch");
@@ -563,7 +566,9 @@ int deviceBuildBvh8(const float4* recs, const float4* boxes, int64_t n, hipStrea
     hipLaunchKernelGGL(k_nearest, dim3(gridFor(m, kBlock)), dim3(kBlock), 0, st, cl, m, t, nn);
     hipLaunchKernelGGL(k_flags, dim3(gridFor(m, kBlock)), dim3(kBlock), 0, st, nn, m, packed);
     size_t sb = scanBytes;
-    ok(hipcub::DeviceScan::ExclusiveSum(tmp, sb, packed, pre, m, st), "scan");
+    ok(rocprim::exclusive_scan(tmp, sb, packed, pre, 0ull, static_cast<size_t>(m),
+                               rocprim::plus<unsigned long long>(), st),
+       "scan");
     hipLaunchKernelGGL(k_merge, dim3(gridFor(m, kBlock)), dim3(kBlock), 0, st, cl, nn, m, packed, pre, nodeBase, t,
                        cl2, totals);
     ok(hipGetLastError(), "launch");
