@@ -201,6 +201,59 @@ def time_to_40db(dev) -> dict:
                                               "atrous": pts((1, 2, 4, 8), r["atrous"])}}
 
 
+def serving(dev, n_requests: int = 256, rate: float = 40.0) -> dict:
+    """Batched serving: the seeded stream of n_requests renders (Poisson
+    arrivals at `rate` per second; atom counts log-uniform 500..200,000 over
+    a pool of 8 molecule sizes, surfaces up to 36,106 atoms; CPK, ball-and-
+    stick or molecular-surface representation; 512x512, 1024x768 or
+    1920x1080; AO 16, hard shadows) served in real time by umbreon_amd.serve:
+    scenes built on the GPU on first use and cached, requests admitted by an
+    HBM budget into batches rendered by ubg_render_batch, tuning picked by
+    the autotuner on calibration requests first. Latency = completion (the
+    image on the host) - arrival."""
+    from umbreon_amd import serve
+    stream = serve.request_stream(n_requests, rate=rate, seed=0)
+    pool = serve.MoleculePool()
+    t0 = time.perf_counter()
+    for key in sorted({(r.atoms, r.rep) for r in stream}):
+        pool.get(*key)
+    gen_s = time.perf_counter() - t0
+    log(f"serving: {len(pool.scenes)} molecules generated on the host in {gen_s:.1f}s")
+    budget = 96 << 30
+    calib = serve.request_stream(32, rate=rate, seed=1)
+    t0 = time.perf_counter()
+    tun = serve.autotune(dev, pool, calib, budget, log=log)
+    tune_s = time.perf_counter() - t0
+    log(f"autotuned in {tune_s:.1f}s: {tun}")
+    # batching's gain on the calibration requests: one request per batch vs the tuned batches
+    single = serve.Server(dev, pool, budget, serve.Tuning(ao_waves_per_simd=tun.ao_waves_per_simd, max_batch=1,
+                                                          leaf_max=dict(tun.leaf_max)))
+    try:
+        serve.offline_throughput(single, calib)
+        one = serve.offline_throughput(single, calib)
+    finally:
+        single.close()
+    srv = serve.Server(dev, pool, budget, tun)
+    try:
+        res = srv.run(stream, log=None)
+    finally:
+        srv.close()
+    sizes = serve.pool_sizes()
+    res.update({"workload": f"{n_requests} seeded requests, Poisson {rate:g}/s, atoms log-uniform 500-200000 "
+                            f"(pool {sizes}, surfaces <= {max(v for v in sizes if v <= 40000)}), "
+                            f"CPK / ball-and-stick / surface, "
+                            f"512x512 / 1024x768 / 1920x1080, AO16, hard shadows",
+                "tuning": {"ao_waves_per_simd": tun.ao_waves_per_simd, "max_batch": tun.max_batch,
+                           "leaf_max": tun.leaf_max},
+                "calibration_images_per_s": {"one_request_per_launch": round(one, 1)},
+                "host_generation_s": round(gen_s, 1), "autotune_s": round(tune_s, 1)})
+    for k in ("images_per_s", "p50_latency_ms", "p99_latency_ms", "mean_latency_ms", "makespan_s", "mrays_per_s",
+              "scene_build_s"):
+        res[k] = round(res[k], 3)
+    log(f"serving: {res}")
+    return res
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -216,6 +269,8 @@ def main() -> int:
                          "or full-precision binary")
     ap.add_argument("--no-denoise-metric", action="store_true",
                     help="skip the time-to-40-dB measurement of the learned GI denoiser")
+    ap.add_argument("--no-serve", action="store_true",
+                    help="skip the batched-serving measurement (256-request Poisson stream)")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="also run one device-to-device copy of this many MiB (a copy kernel of known "
                          "traffic, for scaling FETCH_SIZE / WRITE_SIZE counter readings)")
@@ -360,6 +415,10 @@ def main() -> int:
     if not args.no_denoise_metric:
         t40 = time_to_40db(dev)
 
+    serve_res = None
+    if not args.no_serve:
+        serve_res = serving(dev)
+
     out = {
         "metric": METRIC,
         "value": round(value, 2),
@@ -393,6 +452,7 @@ def main() -> int:
         "roofline": roofline,
         "cpu_baseline": cpu,
         "time_to_40db": t40,
+        "serving": serve_res,
     }
     print(json.dumps(out), flush=True)
     bs.close()

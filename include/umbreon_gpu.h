@@ -391,12 +391,50 @@ int ubg_scene_create(ubg_context* ctx, const ubg_scene_desc* desc, ubg_scene** o
  * primitives hit at exactly the same distance wins), the layout and the
  * builder only change build and traversal speed. */
 int ubg_scene_create_ex(ubg_context* ctx, const ubg_scene_desc* desc, int32_t bvh_layout, ubg_scene** out);
+/* Scene build options. leaf_max: primitives per BVH8 leaf slot, 1..3
+ * (0 = 3); a tuning knob -- images do not depend on it (primitives hit at
+ * the same distance resolve by (kind, primID), whatever the tree). */
+typedef struct ubg_scene_options {
+  int32_t bvh_layout;   /* UBG_BVH2 / UBG_BVH8 / UBG_BVH8_GPU_BUILD */
+  int32_t leaf_max;
+  int32_t reserved[6];
+} ubg_scene_options;
+int ubg_scene_create_opts(ubg_context* ctx, const ubg_scene_desc* desc, const ubg_scene_options* opts,
+                          ubg_scene** out);
 void ubg_scene_destroy(ubg_scene* scene);
 int ubg_scene_get_info(const ubg_scene* scene, ubg_scene_info* out);
 
 /* Render one frame (hi-res trace + shade + fog, then resolve). */
 int ubg_render(ubg_context* ctx, const ubg_scene* scene,
                const ubg_render_params* params, ubg_frame* frame);
+
+/* Tuning of a batched render (may be NULL: the defaults). Any choice gives
+ * the same images; it only changes speed. */
+typedef struct ubg_batch_options {
+  int32_t ao_waves_per_simd;   /* persistent AO grid: waves per SIMD, 1..6 (0 = 6) */
+  int32_t no_graph;            /* 1: launch each layer's kernels directly instead of
+                                  replaying its instantiated HIP graph */
+  int32_t reserved[6];
+} ubg_batch_options;
+
+/* Batched render (a server's path; the reference serves one request per
+ * call, umbreon.cpp:322-341 renderAsync): n <= 256 requests, each with its
+ * own scene, frame parameters and outputs (frames[i].memory per request),
+ * traced by the same wavefront launches -- one launch of each kernel per
+ * transparency layer covers every request still walking layers, the AO pass
+ * is one persistent grid that drains the requests' hit queues in turn, and
+ * the queues and frame buffers are carved from one shared pool. A layer's
+ * launch sequence (queue reset, primary, AO, resolve, fill read-back) is
+ * captured once per launch shape into a HIP graph kept by the context and
+ * replayed by later layers and batches of that shape. Every
+ * request's image, AOVs and ray counts equal those of ubg_render on it alone
+ * (the determinism contract of docs/api/libumbreon.md:586-587 extended to
+ * batch composition). Supported: BVH8 scenes with POV materials, legacy AO,
+ * hard / soft shadows, transparency, fog, supersampling; GI, coarse or
+ * enhanced AO, adaptive AA, visit counting and kernel timing return
+ * UBG_ERR_UNSUPPORTED. render_seconds is the batch's trace window. */
+int ubg_render_batch(ubg_context* ctx, int32_t n, const ubg_scene* const* scenes,
+                     const ubg_render_params* params, ubg_frame* frames, const ubg_batch_options* opts);
 
 /* Size of the fixed frame scratch of ubg_render_noalloc: one device array
  * of this many bytes per GPU the library runs on, held until the process

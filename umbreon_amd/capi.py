@@ -210,12 +210,22 @@ def hit_dtype():
                      ("prim_id", "<u4"), ("ng", "<f4", 3)])
 
 
+class SceneOptions(C.Structure):
+    """ubg_scene_options: BVH layout and leaf size (images do not depend on the leaf size)."""
+    _fields_ = [("bvh_layout", C.c_int32), ("leaf_max", C.c_int32), ("reserved", C.c_int32 * 6)]
+
+
+class BatchOptions(C.Structure):
+    """ubg_batch_options: tuning of a batched render (results do not depend on it)."""
+    _fields_ = [("ao_waves_per_simd", C.c_int32), ("no_graph", C.c_int32), ("reserved", C.c_int32 * 6)]
+
+
 # Exported symbols of libumbreon_gpu.so (the C-ABI). Kept in sync with the
 # header; tests check both directions.
 EXPORTS = [
     "ubg_abi_version", "ubg_status_string", "ubg_context_create", "ubg_context_destroy",
-    "ubg_last_error", "ubg_scene_create", "ubg_scene_create_ex", "ubg_scene_destroy", "ubg_scene_get_info",
-    "ubg_render", "ubg_render_noalloc", "ubg_intersect", "ubg_occluded", "ubg_occluded_segments", "ubg_object_space_edges",
+    "ubg_last_error", "ubg_scene_create", "ubg_scene_create_ex", "ubg_scene_create_opts", "ubg_scene_destroy", "ubg_scene_get_info",
+    "ubg_render", "ubg_render_noalloc", "ubg_render_batch", "ubg_intersect", "ubg_occluded", "ubg_occluded_segments", "ubg_object_space_edges",
     "ubg_edges_free", "ubg_unet_create", "ubg_unet_destroy", "ubg_unet_param_count", "ubg_unet_get_params",
     "ubg_unet_set_params", "ubg_unet_infer", "ubg_unet_train_step",
 ]
@@ -253,6 +263,9 @@ def load() -> C.CDLL:
     lib.ubg_scene_create.restype = C.c_int
     lib.ubg_scene_create_ex.argtypes = [C.c_void_p, C.POINTER(SceneDesc), C.c_int32, C.POINTER(C.c_void_p)]
     lib.ubg_scene_create_ex.restype = C.c_int
+    lib.ubg_scene_create_opts.argtypes = [C.c_void_p, C.POINTER(SceneDesc), C.POINTER(SceneOptions),
+                                          C.POINTER(C.c_void_p)]
+    lib.ubg_scene_create_opts.restype = C.c_int
     lib.ubgx_host_query.argtypes = [C.POINTER(SceneDesc), C.c_int32, C.c_int64, C.c_void_p, C.c_void_p,
                                     C.c_void_p, C.c_void_p]
     lib.ubgx_host_query.restype = C.c_int
@@ -266,6 +279,9 @@ def load() -> C.CDLL:
     lib.ubg_render.restype = C.c_int
     lib.ubg_render_noalloc.argtypes = [C.c_void_p, C.c_void_p, C.POINTER(RenderParams), C.POINTER(Frame)]
     lib.ubg_render_noalloc.restype = C.c_int
+    lib.ubg_render_batch.argtypes = [C.c_void_p, C.c_int32, C.POINTER(C.c_void_p), C.POINTER(RenderParams),
+                                     C.POINTER(Frame), C.POINTER(BatchOptions)]
+    lib.ubg_render_batch.restype = C.c_int
     lib.ubg_intersect.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_int32]
     lib.ubg_intersect.restype = C.c_int
     lib.ubg_occluded.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_int32]

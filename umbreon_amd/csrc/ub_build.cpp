@@ -490,7 +490,7 @@ int flattenBvh2(const std::vector<BNode>& bn, int32_t root, const std::vector<ui
 // (the wide-BVH collapse of Ylitie, Karras and Laine, "Efficient Incoherent
 // Ray Traversal on GPUs Through Compressed Wide BVHs", HPG 2017). cost[n][i]
 // is the least SAH cost of subtree n spread over at most i slots; a subtree
-// of at most kMaxLeaf8 primitives may become one leaf slot. Node and
+// of at most leafMax (<= kMaxLeaf8) primitives may become one leaf slot. Node and
 // primitive costs are the relative VALU costs of one 8-slot box test and one
 // primitive test of the gfx950 kernels.
 class WideCollapse {
@@ -498,7 +498,7 @@ class WideCollapse {
   static constexpr float kNodeCost = 1.0f;
   static constexpr float kPrimCost = 1.0f;
 
-  WideCollapse(const std::vector<BNode>& bn, int32_t root) : bn_(bn) {
+  WideCollapse(const std::vector<BNode>& bn, int32_t root, int leafMax) : bn_(bn), leafMax_(leafMax) {
     const size_t n = bn.size();
     cost_.resize(n);
     distK_.resize(n);
@@ -545,6 +545,7 @@ class WideCollapse {
 
  private:
   const std::vector<BNode>& bn_;
+  int leafMax_;
   std::vector<std::array<float, 9>> cost_;
   std::vector<std::array<uint8_t, 9>> distK_;
   std::vector<std::array<uint8_t, 9>> useDist_;
@@ -581,7 +582,7 @@ class WideCollapse {
       distK_[ni][j] = static_cast<uint8_t>(bk);
     }
     const float asNode = A * kNodeCost + dist[8];
-    const float asLeaf = count_[ni] <= kMaxLeaf8 ? A * static_cast<float>(count_[ni]) * kPrimCost
+    const float asLeaf = count_[ni] <= leafMax_ ? A * static_cast<float>(count_[ni]) * kPrimCost
                                                  : std::numeric_limits<float>::infinity();
     leaf_[ni] = asLeaf <= asNode ? 1 : 0;
     cost_[ni][1] = std::min(asLeaf, asNode);
@@ -605,8 +606,8 @@ class WideCollapse {
 };
 
 int collapseBvh8(const std::vector<BNode>& bn, int32_t root, const std::vector<uint32_t>& order,
-                 const std::vector<F4>& recs, HostScene& out, std::string& err) {
-  const WideCollapse wc(bn, root);
+                 const std::vector<F4>& recs, HostScene& out, std::string& err, int leafMax) {
+  const WideCollapse wc(bn, root, leafMax);
   struct Item {
     int32_t node;
     uint32_t idx;
@@ -729,7 +730,8 @@ int collapseBvh8(const std::vector<BNode>& bn, int32_t root, const std::vector<u
 
 }  // namespace
 
-int buildHostScene(const ubg_scene_desc* d, int layout, HostScene& out, std::string& err, bool recordsOnly) {
+int buildHostScene(const ubg_scene_desc* d, int layout, HostScene& out, std::string& err, bool recordsOnly,
+                   int leafMax8) {
   out = HostScene();
   if (layout != kLayoutBvh2 && layout != kLayoutBvh8) {
     err = "unknown BVH layout";
@@ -1068,7 +1070,8 @@ int buildHostScene(const ubg_scene_desc* d, int layout, HostScene& out, std::str
   const std::vector<BNode>& bn = bld.nodes();
   const std::vector<uint32_t>& order = bld.order();
   out.layout = layout;
-  int st = layout == kLayoutBvh8 ? collapseBvh8(bn, root, order, recs, out, err)
+  int st = layout == kLayoutBvh8 ? collapseBvh8(bn, root, order, recs, out, err,
+                                                std::max(1, std::min(kMaxLeaf8, leafMax8)))
                                  : flattenBvh2(bn, root, order, recs, out, err);
   if (st != UBG_OK) return st;
   // SAH cost of the binary tree (traversal 1, intersection 1 per primitive)

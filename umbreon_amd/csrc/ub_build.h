@@ -62,10 +62,10 @@ struct HostScene {
 // Flatten `desc` (curve chaining, instance baking, side tables) and build the
 // BVH in `layout` (kLayoutBvh2 / kLayoutBvh8). With recordsOnly the BVH is
 // left to the caller: `prims` keeps the records in geometry order and
-// `boxes` holds their bounds. Returns UBG_OK or an error status with `err`
-// set.
+// `boxes` holds their bounds. leafMax8: primitives per BVH8 leaf slot
+// (1..kMaxLeaf8). Returns UBG_OK or an error status with `err` set.
 int buildHostScene(const ubg_scene_desc* desc, int layout, HostScene& out, std::string& err,
-                   bool recordsOnly = false);
+                   bool recordsOnly = false, int leafMax8 = kMaxLeaf8);
 
 // shading.hpp:34-41
 float blinnExponent(float roughness);

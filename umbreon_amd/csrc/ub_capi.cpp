@@ -17,6 +17,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <limits>
+#include <map>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -79,6 +80,19 @@ struct ubg_context {
   // AOV, the padded network input, its prediction and a max-depth word
   DevBuf unetIn, unetTgt, unetOut, giFeat, giFeatP, giPred, giMax;
   unsigned long long* hostCounters = nullptr;  // pinned
+  // batched frames: the shared queue / frame-buffer pool, the request table,
+  // launch tables and read-backs (device), their pinned host staging
+  DevBuf batchPool, batchItems, batchLaunch, batchOut;
+  void* batchHost = nullptr;  // pinned: items + launch tables + read-backs
+  size_t batchHostBytes = 0;
+  // instantiated graphs of a batched layer (reset, primary, AO, resolve,
+  // fill read-back), keyed by the launch shape; valid while the batch
+  // buffers stay where they are
+  std::map<std::vector<int64_t>, hipGraphExec_t> layerGraphs;
+  void clearLayerGraphs() {
+    for (auto& kv : layerGraphs) (void)hipGraphExecDestroy(kv.second);
+    layerGraphs.clear();
+  }
   void* staticScratch = nullptr;               // device address of the fixed scratch (ub_static.hip)
   hipError_t staticStatus = hipSuccess;
   std::vector<DevBuf*> all() {
@@ -87,7 +101,8 @@ struct ubg_context {
             &hitOpen, &hitBent, &cGuide, &cOpen, &cBent, &hitOut, &aaSetC, &aaSetR, &aaSetF, &aaFirstN, &hitPrim,
             &aaPix, &giPos, &giNrm, &giRefl, &giE, &giList, &giInd, &giOcc, &dnA, &dnB, &dnGrad,
             &dnVar, &giLow, &giPatch, &giSnap, &edgePts, &edgeSolids, &edgeOut, &unetIn,
-            &unetTgt, &unetOut, &giFeat, &giFeatP, &giPred, &giMax};
+            &unetTgt, &unetOut, &giFeat, &giFeatP, &giPred, &giMax, &batchPool, &batchItems, &batchLaunch,
+            &batchOut};
   }
 };
 
@@ -246,6 +261,8 @@ void ubg_context_destroy(ubg_context* ctx) {
     if (ev) (void)hipEventDestroy(ev);
   for (hipEvent_t ev : ctx->tev) (void)hipEventDestroy(ev);
   if (ctx->hostCounters) (void)hipHostFree(ctx->hostCounters);
+  ctx->clearLayerGraphs();
+  if (ctx->batchHost) (void)hipHostFree(ctx->batchHost);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
 }
@@ -257,7 +274,18 @@ static void freeSceneLocked(ubg_scene* s) {
 }
 
 int ubg_scene_create_ex(ubg_context* ctx, const ubg_scene_desc* desc, int32_t bvh_layout, ubg_scene** out) {
-  if (!ctx || !desc || !out) return fail(UBG_ERR_INVALID_ARGUMENT, "ubg_scene_create: null argument");
+  ubg_scene_options o{};
+  o.bvh_layout = bvh_layout;
+  return ubg_scene_create_opts(ctx, desc, &o, out);
+}
+
+int ubg_scene_create_opts(ubg_context* ctx, const ubg_scene_desc* desc, const ubg_scene_options* opts,
+                          ubg_scene** out) {
+  if (!ctx || !desc || !out || !opts) return fail(UBG_ERR_INVALID_ARGUMENT, "ubg_scene_create: null argument");
+  const int32_t bvh_layout = opts->bvh_layout;
+  if (opts->leaf_max < 0 || opts->leaf_max > ubg::kMaxLeaf8)
+    return fail(UBG_ERR_INVALID_ARGUMENT, "ubg_scene_create: leaf_max must be 0 (default) or 1..3");
+  const int leafMax = opts->leaf_max > 0 ? opts->leaf_max : ubg::kMaxLeaf8;
   *out = nullptr;
   if (bvh_layout != UBG_BVH2 && bvh_layout != UBG_BVH8 && bvh_layout != UBG_BVH8_GPU_BUILD)
     return fail(UBG_ERR_INVALID_ARGUMENT, "ubg_scene_create: unknown BVH layout");
@@ -268,7 +296,7 @@ int ubg_scene_create_ex(ubg_context* ctx, const ubg_scene_desc* desc, int32_t bv
   ubg::HostScene hs;
   std::string err;
   const int st = ubg::buildHostScene(desc, bvh_layout == UBG_BVH2 ? ubg::kLayoutBvh2 : ubg::kLayoutBvh8, hs, err,
-                                     gpuBuild);
+                                     gpuBuild, leafMax);
   if (st != UBG_OK) return fail(st, "scene build failed: " + err);
   ubg_scene* s = new ubg_scene();
   s->ctx = ctx;
@@ -287,7 +315,7 @@ int ubg_scene_create_ex(ubg_context* ctx, const ubg_scene_desc* desc, int32_t bv
     ubg::DeviceBvh8 db;
     if (r == UBG_OK) {
       r = ubg::deviceBuildBvh8(static_cast<const float4*>(recs), static_cast<const float4*>(boxes),
-                               static_cast<int64_t>(hs.prims.size() / 3), ctx->stream, db, err);
+                               static_cast<int64_t>(hs.prims.size() / 3), ctx->stream, db, err, leafMax);
       if (r != UBG_OK) (void)fail(r, "scene build failed: " + err);
     }
     if (r == UBG_OK) {
@@ -407,33 +435,12 @@ int ubg_scene_get_info(const ubg_scene* scene, ubg_scene_info* out) {
   return UBG_OK;
 }
 
-// One frame; haveLock: the caller already holds ctx->mu.
-static int renderFrame(ubg_context* ctx, const ubg_scene* scene, const ubg_render_params* p, ubg_frame* fr,
-                       bool haveLock) {
-  if (!ctx || !scene || !p || !fr) return fail(UBG_ERR_INVALID_ARGUMENT, "ubg_render: null argument");
-  if (scene->ctx != ctx) return fail(UBG_ERR_INVALID_ARGUMENT, "ubg_render: scene belongs to another context");
-  if (p->width <= 0 || p->height <= 0) return fail(UBG_ERR_INVALID_ARGUMENT, "ubg_render: width/height must be > 0");
-  if (p->n_lights < 0 || (p->n_lights > 0 && !p->lights))
-    return fail(UBG_ERR_INVALID_ARGUMENT, "ubg_render: bad light array");
-  if (p->n_lights > ubg::kMaxLights)
-    return fail(UBG_ERR_UNSUPPORTED, "ubg_render: more than 16 lights");
-  if (p->ao_samples < 0 || p->shadow_samples < 0 || p->max_transparent_layers < 0)
-    return fail(UBG_ERR_INVALID_ARGUMENT, "ubg_render: negative sample or layer count");
-  const int ss = std::max(1, p->supersample);
-  const int64_t W64 = static_cast<int64_t>(p->width) * ss, H64 = static_cast<int64_t>(p->height) * ss;
-  if (W64 * H64 >= (int64_t(1) << 31) || W64 * H64 * 4 >= (int64_t(1) << 40))
-    return fail(UBG_ERR_INVALID_ARGUMENT, "ubg_render: frame too large");
-  const int W = static_cast<int>(W64), H = static_cast<int>(H64);
-  const int npx = W * H;
-  const bool countVisits = (p->flags & UBG_FLAG_COUNT_VISITS) != 0u;
-
-  std::unique_lock<std::mutex> lk(ctx->mu, std::defer_lock);
-  if (!haveLock) lk.lock();
-  UBG_HIP(hipSetDevice(ctx->device), "hipSetDevice");
-  hipStream_t st = ctx->stream;
-
-  // ---- frame constants (embree_renderer.cpp:375-386, 436-458) ----
-  ubg::FrameConst fc;
+// Frame constants of one render (embree_renderer.cpp:375-386, 436-458): the
+// camera basis, the per-ray options, the GI and coarse-AO plans and the
+// lights; giDiv receives the GI gather-grid divisor. Fails on options the
+// path does not implement.
+static int frameConstants(ubg_context* ctx, const ubg_scene* scene, const ubg_render_params* p, const ubg_frame* fr,
+                          int W, int H, int ss, ubg::FrameConst& fc, int& giDiv) {
   std::memset(&fc, 0, sizeof(fc));
   fc.W = W;
   fc.H = H;
@@ -499,7 +506,7 @@ static int renderFrame(ubg_context* ctx, const ubg_scene* scene, const ubg_rende
   // embree_renderer.cpp:716-718 (pipeline.cpp:55 resolves -1 to the
   // supersample factor)
   const bool gi = p->gi != 0;
-  int giDiv = 1;
+  giDiv = 1;
   if (gi) {
     if (p->gi_integrator != 1 && p->gi_integrator != 2)
       return fail(UBG_ERR_UNSUPPORTED, "ubg_render: the irradiance-cache GI integrator (gi_integrator 0) is not implemented");
@@ -525,8 +532,6 @@ static int renderFrame(ubg_context* ctx, const ubg_scene* scene, const ubg_rende
         static_cast<int64_t>(std::max(1, p->pt1_spp)) * std::max(1, p->pt1_edge_patch_spp_mul) > (1 << 24))
       return fail(UBG_ERR_INVALID_ARGUMENT, "ubg_render: pt1_spp / gi_bounces / edge-patch samples out of range");
   }
-  const int giLw = (W + giDiv - 1) / giDiv, giLh = (H + giDiv - 1) / giDiv;
-  const size_t giLn = static_cast<size_t>(giLw) * giLh;
   fc.gi = gi ? 1 : 0;
   fc.giPt2 = (gi && p->gi_integrator == 2) ? 1 : 0;
   fc.giPattern = p->pt2_pattern;
@@ -582,6 +587,46 @@ static int renderFrame(ubg_context* ctx, const ubg_scene* scene, const ubg_rende
     l.tanRadius = std::tan(l.radius);
     l.soft = (p->shadow_samples > 1 && l.radius > 0.0f) ? 1 : 0;
   }
+
+  return UBG_OK;
+}
+
+// One frame; haveLock: the caller already holds ctx->mu.
+static int renderFrame(ubg_context* ctx, const ubg_scene* scene, const ubg_render_params* p, ubg_frame* fr,
+                       bool haveLock) {
+  if (!ctx || !scene || !p || !fr) return fail(UBG_ERR_INVALID_ARGUMENT, "ubg_render: null argument");
+  if (scene->ctx != ctx) return fail(UBG_ERR_INVALID_ARGUMENT, "ubg_render: scene belongs to another context");
+  if (p->width <= 0 || p->height <= 0) return fail(UBG_ERR_INVALID_ARGUMENT, "ubg_render: width/height must be > 0");
+  if (p->n_lights < 0 || (p->n_lights > 0 && !p->lights))
+    return fail(UBG_ERR_INVALID_ARGUMENT, "ubg_render: bad light array");
+  if (p->n_lights > ubg::kMaxLights)
+    return fail(UBG_ERR_UNSUPPORTED, "ubg_render: more than 16 lights");
+  if (p->ao_samples < 0 || p->shadow_samples < 0 || p->max_transparent_layers < 0)
+    return fail(UBG_ERR_INVALID_ARGUMENT, "ubg_render: negative sample or layer count");
+  const int ss = std::max(1, p->supersample);
+  const int64_t W64 = static_cast<int64_t>(p->width) * ss, H64 = static_cast<int64_t>(p->height) * ss;
+  if (W64 * H64 >= (int64_t(1) << 31) || W64 * H64 * 4 >= (int64_t(1) << 40))
+    return fail(UBG_ERR_INVALID_ARGUMENT, "ubg_render: frame too large");
+  const int W = static_cast<int>(W64), H = static_cast<int>(H64);
+  const int npx = W * H;
+  const bool countVisits = (p->flags & UBG_FLAG_COUNT_VISITS) != 0u;
+
+  std::unique_lock<std::mutex> lk(ctx->mu, std::defer_lock);
+  if (!haveLock) lk.lock();
+  UBG_HIP(hipSetDevice(ctx->device), "hipSetDevice");
+  hipStream_t st = ctx->stream;
+
+  // ---- frame constants (embree_renderer.cpp:375-386, 436-458) ----
+  ubg::FrameConst fc;
+  int giDiv = 1;
+  {
+    const int rs = frameConstants(ctx, scene, p, fr, W, H, ss, fc, giDiv);
+    if (rs != UBG_OK) return rs;
+  }
+  const bool gi = p->gi != 0;
+  const ubg_camera& cam = p->camera;
+  const int giLw = (W + giDiv - 1) / giDiv, giLh = (H + giDiv - 1) / giDiv;
+  const size_t giLn = static_cast<size_t>(giLw) * giLh;
 
   // ---- adaptive AA plan (embree_renderer.cpp:1379-1385, 1430-1434, 1549, 1597-1602) ----
   // (pipeline.cpp:60-65: adaptive AA falls back to the grid under GI)
@@ -1131,6 +1176,336 @@ static int runQuery(ubg_context* ctx, const ubg_scene* scene, int64_t n, const u
 
 int ubg_render(ubg_context* ctx, const ubg_scene* scene, const ubg_render_params* p, ubg_frame* fr) {
   return renderFrame(ctx, scene, p, fr, false);
+}
+
+// ---- batched frames ---------------------------------------------------------
+// n requests rendered by the same launches (ub_kernels.hip, k_*_batch): every
+// request keeps its own scene, frame constants, queues and outputs, carved
+// from one shared pool; per transparency layer one launch of each kernel
+// covers every request still walking layers, and one read-back brings all
+// their continuation fills. Each request's image and ray counts are those of
+// ubg_render on it alone.
+int ubg_render_batch(ubg_context* ctx, int32_t n, const ubg_scene* const* scenes, const ubg_render_params* params,
+                     ubg_frame* frames, const ubg_batch_options* opts) {
+  if (!ctx || n <= 0 || !scenes || !params || !frames)
+    return fail(UBG_ERR_INVALID_ARGUMENT, "ubg_render_batch: bad argument");
+  if (n > ubg::kMaxBatch)
+    return fail(UBG_ERR_INVALID_ARGUMENT, "ubg_render_batch: more than " + std::to_string(ubg::kMaxBatch) +
+                                              " requests in one batch");
+  const int wavesPerSimd = opts && opts->ao_waves_per_simd > 0 ? std::min(opts->ao_waves_per_simd, 6) : 6;
+  const bool useGraph = !opts || opts->no_graph == 0;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  UBG_HIP(hipSetDevice(ctx->device), "hipSetDevice");
+  hipStream_t st = ctx->stream;
+  struct Req {
+    int W, H, ss, partCap;
+    size_t npx, qn, outN, off;
+  };
+  std::vector<Req> rq(static_cast<size_t>(n));
+  std::vector<ubg::BatchItem> items(static_cast<size_t>(n));
+  auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
+  size_t pool = 0;
+  for (int i = 0; i < n; ++i) {
+    const ubg_scene* sc = scenes[i];
+    const ubg_render_params* p = &params[i];
+    ubg_frame* fr = &frames[i];
+    const std::string who = "ubg_render_batch: request " + std::to_string(i) + ": ";
+    if (!sc || sc->ctx != ctx) return fail(UBG_ERR_INVALID_ARGUMENT, who + "scene missing or on another context");
+    if (sc->view.layout != ubg::kLayoutBvh8 || sc->principled || sc->anisoPrims)
+      return fail(UBG_ERR_UNSUPPORTED, who + "batches take BVH8 scenes with POV materials");
+    if (p->width <= 0 || p->height <= 0 || p->n_lights < 0 || (p->n_lights > 0 && !p->lights) ||
+        p->n_lights > ubg::kMaxLights || p->ao_samples < 0 || p->shadow_samples < 0 ||
+        p->max_transparent_layers < 0)
+      return fail(UBG_ERR_INVALID_ARGUMENT, who + "bad frame parameters");
+    if (p->flags & (UBG_FLAG_COUNT_VISITS | UBG_FLAG_KERNEL_TIMES))
+      return fail(UBG_ERR_UNSUPPORTED, who + "visit counting and kernel timing are single-frame options");
+    Req& q = rq[i];
+    q.ss = std::max(1, p->supersample);
+    const int64_t W64 = static_cast<int64_t>(p->width) * q.ss, H64 = static_cast<int64_t>(p->height) * q.ss;
+    if (W64 * H64 >= (int64_t(1) << 31)) return fail(UBG_ERR_INVALID_ARGUMENT, who + "frame too large");
+    q.W = static_cast<int>(W64);
+    q.H = static_cast<int>(H64);
+    ubg::FrameConst& fc = items[i].fc;
+    int giDiv = 1;
+    const int rs = frameConstants(ctx, sc, p, fr, q.W, q.H, q.ss, fc, giDiv);
+    if (rs != UBG_OK) return rs;
+    const bool adaptive = p->aa_mode == 1 && (q.ss > 1 || p->aa_depth > 1);
+    if (fc.gi || fc.coarse || fc.aoEnh || adaptive)
+      return fail(UBG_ERR_UNSUPPORTED, who + "GI, coarse or enhanced AO and adaptive AA are single-frame options");
+    q.partCap = ubg::queuePartCap(q.W, q.H);
+    q.qn = static_cast<size_t>(ubg::kQueueParts) * q.partCap;
+    q.npx = static_cast<size_t>(q.W) * q.H;
+    q.outN = static_cast<size_t>(p->width) * p->height;
+    q.off = pool;
+    // queues (pix, t, O, N, C, AO count, two continuation lists), per-pixel
+    // walk state and outputs, the final colour, the counters
+    pool += al(q.qn * 4) * 4 + al(q.qn * 16) * 3 + al(q.qn * 4) + al(q.npx * 16) * 2 + al(q.npx * 4) * 5 +
+            al(q.outN * 16) + al(ubg::kCounterWords * sizeof(unsigned long long));
+  }
+  const size_t launchBytes = al(sizeof(ubg::BatchLaunch));
+  const size_t itemBytes = al(sizeof(ubg::BatchItem) * static_cast<size_t>(n));
+  const size_t fillBytes = al(sizeof(uint32_t) * ubg::kQueueParts * n);
+  const size_t statBytes = al(sizeof(unsigned long long) * ubg::kNumCounters * n);
+  const void* before[4] = {ctx->batchPool.p, ctx->batchItems.p, ctx->batchLaunch.p, ctx->batchOut.p};
+  int r = ensure(ctx->batchPool, pool);
+  if (r == UBG_OK) r = ensure(ctx->batchItems, itemBytes);
+  if (r == UBG_OK) r = ensure(ctx->batchLaunch, 4 * launchBytes);
+  if (r == UBG_OK) r = ensure(ctx->batchOut, fillBytes + statBytes);
+  if (r != UBG_OK) return r;
+  if (before[0] != ctx->batchPool.p || before[1] != ctx->batchItems.p || before[2] != ctx->batchLaunch.p ||
+      before[3] != ctx->batchOut.p)
+    ctx->clearLayerGraphs();
+  const size_t hostNeed = itemBytes + 4 * launchBytes + fillBytes + statBytes;
+  if (ctx->batchHostBytes < hostNeed) {
+    ctx->clearLayerGraphs();
+    if (ctx->batchHost) (void)hipHostFree(ctx->batchHost);
+    ctx->batchHost = nullptr;
+    ctx->batchHostBytes = 0;
+    UBG_HIP(hipHostMalloc(&ctx->batchHost, hostNeed, hipHostMallocDefault), "hipHostMalloc(batch staging)");
+    ctx->batchHostBytes = hostNeed;
+  }
+  char* hb = static_cast<char*>(ctx->batchHost);
+  auto* hItems = reinterpret_cast<ubg::BatchItem*>(hb);
+  ubg::BatchLaunch* hL[4];
+  for (int k = 0; k < 4; ++k) hL[k] = reinterpret_cast<ubg::BatchLaunch*>(hb + itemBytes + k * launchBytes);
+  auto* hFills = reinterpret_cast<uint32_t*>(hb + itemBytes + 4 * launchBytes);
+  auto* hStats = reinterpret_cast<unsigned long long*>(hb + itemBytes + 4 * launchBytes + fillBytes);
+  const auto* dItems = static_cast<const ubg::BatchItem*>(ctx->batchItems.p);
+  const ubg::BatchLaunch* dL[4];
+  for (int k = 0; k < 4; ++k)
+    dL[k] = reinterpret_cast<const ubg::BatchLaunch*>(static_cast<char*>(ctx->batchLaunch.p) + k * launchBytes);
+  auto* dFills = static_cast<uint32_t*>(ctx->batchOut.p);
+  auto* dStats = reinterpret_cast<unsigned long long*>(static_cast<char*>(ctx->batchOut.p) + fillBytes);
+
+  for (int i = 0; i < n; ++i) {
+    const Req& q = rq[i];
+    char* base = static_cast<char*>(ctx->batchPool.p) + q.off;
+    auto take = [&](size_t bytes) {
+      char* at = base;
+      base += al(bytes);
+      return at;
+    };
+    ubg::BatchItem& it = items[i];
+    it.s = scenes[i]->view;
+    ubg::FrameBufs& b = it.b;
+    std::memset(&b, 0, sizeof(b));
+    b.hitPix = reinterpret_cast<uint32_t*>(take(q.qn * 4));
+    b.hitT = reinterpret_cast<float*>(take(q.qn * 4));
+    b.activeIn = reinterpret_cast<uint32_t*>(take(q.qn * 4));
+    b.activeOut = reinterpret_cast<uint32_t*>(take(q.qn * 4));
+    b.hitO = reinterpret_cast<float4*>(take(q.qn * 16));
+    b.hitN = reinterpret_cast<float4*>(take(q.qn * 16));
+    b.hitC = reinterpret_cast<float4*>(take(q.qn * 16));
+    b.hitAo = reinterpret_cast<uint32_t*>(take(q.qn * 4));
+    b.acc = reinterpret_cast<float4*>(take(q.npx * 16));
+    b.hiColor = reinterpret_cast<float4*>(take(q.npx * 16));
+    b.tnear = reinterpret_cast<float*>(take(q.npx * 4));
+    b.depth = reinterpret_cast<float*>(take(q.npx * 4));
+    b.viewZ = reinterpret_cast<float*>(take(q.npx * 4));
+    uint32_t* hitId = reinterpret_cast<uint32_t*>(take(q.npx * 4));
+    float* aoOpen = reinterpret_cast<float*>(take(q.npx * 4));
+    b.hitId = frames[i].hit_id ? hitId : nullptr;
+    b.aoOpen = frames[i].ao_open ? aoOpen : nullptr;
+    it.out = reinterpret_cast<float4*>(take(q.outN * 16));
+    b.counters = reinterpret_cast<unsigned long long*>(take(ubg::kCounterWords * sizeof(unsigned long long)));
+    b.partCap = q.partCap;
+    it.ss = q.ss;
+    it.gamma = params[i].assumed_gamma;
+    frames[i].cancelled = 0;
+    frames[i].layers = 0;
+  }
+  auto pushItems = [&]() -> int {
+    std::memcpy(hItems, items.data(), sizeof(ubg::BatchItem) * static_cast<size_t>(n));
+    UBG_HIP(hipMemcpyAsync(ctx->batchItems.p, hItems, sizeof(ubg::BatchItem) * static_cast<size_t>(n),
+                           hipMemcpyHostToDevice, st),
+            "hipMemcpy(batch items)");
+    return UBG_OK;
+  };
+  auto pushLaunch = [&](int k) -> int {
+    UBG_HIP(hipMemcpyAsync(const_cast<ubg::BatchLaunch*>(dL[k]), hL[k], sizeof(ubg::BatchLaunch),
+                           hipMemcpyHostToDevice, st),
+            "hipMemcpy(batch launch)");
+    return UBG_OK;
+  };
+  if ((r = pushItems()) != UBG_OK) return r;
+  ubg::launchZeroBatch(dItems, n, st);
+  UBG_HIP(hipEventRecord(ctx->ev[0], st), "hipEventRecord");
+  // the requests still walking layers, and their continuation fills
+  std::vector<int> live(static_cast<size_t>(n));
+  for (int i = 0; i < n; ++i) live[i] = i;
+  std::vector<ubg::PartFill> fill(static_cast<size_t>(n));
+  const int64_t aoResident = ubg::aoResidentBlocks(wavesPerSimd);
+  for (int iter = 0; !live.empty(); ++iter) {
+    for (int i = 0; i < n; ++i)
+      if (params[i].cancel_flag && *params[i].cancel_flag) frames[i].cancelled = 1;
+    ubg::BatchLaunch& LP = *hL[0];
+    ubg::BatchLaunch& LA = *hL[1];
+    ubg::BatchLaunch& LR = *hL[2];
+    LP.n = LA.n = LR.n = 0;
+    uint32_t blocksP = 0, blocksR = 0;
+    int64_t aoRays = 0;
+    size_t lds = 0;
+    for (int i : live) {
+      const Req& q = rq[i];
+      const ubg::FrameConst& fc = items[i].fc;
+      uint32_t nb;
+      int partBlocks = 1;
+      if (iter == 0) {
+        nb = static_cast<uint32_t>((static_cast<int64_t>((q.W + 7) / 8) * ((q.H + 7) / 8) * 64 + 127) / 128);
+      } else {
+        uint32_t mx = 0;
+        for (int p = 0; p < ubg::kQueueParts; ++p) mx = std::max(mx, fill[i].n[p]);
+        partBlocks = static_cast<int>((mx + 127) / 128);
+        nb = static_cast<uint32_t>(partBlocks * ubg::kQueueParts);
+      }
+      const int e = LP.n++;
+      LP.item[e] = i;
+      LP.partBlocks[e] = partBlocks;
+      LP.fill[e] = fill[i];
+      blocksP += nb;
+      LP.blockEnd[e] = blocksP;
+      const int er = LR.n++;
+      LR.item[er] = i;
+      LR.partBlocks[er] = 1;
+      blocksR += static_cast<uint32_t>(q.qn / 128);
+      LR.blockEnd[er] = blocksR;
+      if (fc.aoSamples > 0) {
+        LA.item[LA.n] = i;
+        LA.partBlocks[LA.n] = 1;
+        LA.blockEnd[LA.n] = 0;
+        ++LA.n;
+        // rays this layer can spawn: every ray of the layer hits at most once
+        const int64_t rays = iter == 0 ? static_cast<int64_t>(q.npx) : static_cast<int64_t>(nb) * 128;
+        aoRays += rays * fc.aoSamples;
+      }
+      lds = std::max(lds, ubg::traceLdsBytes(items[i].s));
+      ++frames[i].layers;
+    }
+    for (int k = 0; k < 3; ++k)
+      if ((r = pushLaunch(k)) != UBG_OK) return r;
+    const int64_t aoBlocks = LA.n > 0 ? std::min<int64_t>(aoResident, (aoRays + 127) / 128) : 0;
+    // The layer: reset, primary, AO, resolve, then one read-back for the
+    // whole batch (every continuation fill). Its launches depend only on the
+    // shape below (the tables they read are uploaded before), so a layer of a
+    // known shape replays an instantiated graph.
+    auto layer = [&]() -> int {
+      ubg::launchResetBatch(dItems, dL[0], LP.n, st);
+      ubg::launchPrimaryBatch(dItems, dL[0], blocksP, iter, lds, st);
+      if (aoBlocks > 0) ubg::launchAoBatch(dItems, dL[1], aoBlocks, lds, st);
+      ubg::launchResolveBatch(dItems, dL[2], blocksR, iter, lds, st);
+      ubg::launchFillsBatch(dItems, n, dFills, st);
+      UBG_HIP(hipMemcpyAsync(hFills, dFills, sizeof(uint32_t) * ubg::kQueueParts * n, hipMemcpyDeviceToHost, st),
+              "hipMemcpy(batch fills)");
+      return UBG_OK;
+    };
+    if (useGraph) {
+      // (the layer index is a kernel argument: the first layer and the last
+      // permitted one take other branches)
+      const std::vector<int64_t> key{iter, n, LP.n, blocksP, aoBlocks, blocksR, static_cast<int64_t>(lds)};
+      auto it = ctx->layerGraphs.find(key);
+      if (it == ctx->layerGraphs.end()) {
+        hipGraph_t g = nullptr;
+        hipGraphExec_t ge = nullptr;
+        UBG_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+        const int lr = layer();
+        const hipError_t ce = hipStreamEndCapture(st, &g);
+        if (lr != UBG_OK) {
+          if (g) (void)hipGraphDestroy(g);
+          return lr;
+        }
+        UBG_HIP(ce, "hipStreamEndCapture");
+        const hipError_t ie = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        UBG_HIP(ie, "hipGraphInstantiate");
+        if (ctx->layerGraphs.size() >= 32) ctx->clearLayerGraphs();
+        it = ctx->layerGraphs.emplace(key, ge).first;
+      }
+      UBG_HIP(hipGraphLaunch(it->second, st), "hipGraphLaunch");
+    } else {
+      if ((r = layer()) != UBG_OK) return r;
+    }
+    UBG_HIP(hipGetLastError(), "kernel launch");
+    UBG_HIP(hipStreamSynchronize(st), "batch layer sync");
+    std::vector<int> next;
+    for (int i : live) {
+      int64_t act = 0;
+      for (int p = 0; p < ubg::kQueueParts; ++p) {
+        fill[i].n[p] = hFills[i * ubg::kQueueParts + p];
+        act += fill[i].n[p];
+      }
+      std::swap(items[i].b.activeIn, items[i].b.activeOut);
+      if (act > 0 && iter + 1 < items[i].fc.maxIters && !frames[i].cancelled) next.push_back(i);
+    }
+    live.swap(next);
+    if (!live.empty() && (r = pushItems()) != UBG_OK) return r;
+  }
+  UBG_HIP(hipEventRecord(ctx->ev[1], st), "hipEventRecord");
+  // resolve every request's frame in one launch
+  {
+    ubg::BatchLaunch& LQ = *hL[3];
+    LQ.n = 0;
+    uint32_t blocks = 0;
+    for (int i = 0; i < n; ++i) {
+      if (params[i].flags & UBG_FLAG_HIRES_ONLY) continue;
+      LQ.item[LQ.n] = i;
+      LQ.partBlocks[LQ.n] = 1;
+      blocks += static_cast<uint32_t>((rq[i].outN + 255) / 256);
+      LQ.blockEnd[LQ.n] = blocks;
+      ++LQ.n;
+    }
+    if (LQ.n > 0) {
+      if ((r = pushLaunch(3)) != UBG_OK) return r;
+      ubg::launchPostBatch(dItems, dL[3], blocks, st);
+    }
+  }
+  UBG_HIP(hipEventRecord(ctx->ev[2], st), "hipEventRecord");
+  ubg::launchStatsBatch(dItems, n, dStats, st);
+  UBG_HIP(hipGetLastError(), "kernel launch");
+  for (int i = 0; i < n && r == UBG_OK; ++i) {
+    ubg_frame* fr = &frames[i];
+    const Req& q = rq[i];
+    const bool hiresOnly = (params[i].flags & UBG_FLAG_HIRES_ONLY) != 0u;
+    const ubg::FrameBufs& b = items[i].b;
+    fr->hi_w = q.W;
+    fr->hi_h = q.H;
+    fr->out_w = hiresOnly ? q.W : params[i].width;
+    fr->out_h = hiresOnly ? q.H : params[i].height;
+    r = copyOut(fr->color, hiresOnly ? static_cast<const void*>(b.hiColor) : static_cast<const void*>(items[i].out),
+                (hiresOnly ? q.npx : q.outN) * 16, fr->memory, st);
+    if (r == UBG_OK) r = copyOut(fr->depth, b.depth, q.npx * 4, fr->memory, st);
+    if (r == UBG_OK) r = copyOut(fr->view_z, b.viewZ, q.npx * 4, fr->memory, st);
+    if (r == UBG_OK && fr->hit_id) r = copyOut(fr->hit_id, b.hitId, q.npx * 4, fr->memory, st);
+    if (r == UBG_OK && fr->ao_open) r = copyOut(fr->ao_open, b.aoOpen, q.npx * 4, fr->memory, st);
+  }
+  if (r != UBG_OK) return r;
+  UBG_HIP(hipMemcpyAsync(hStats, dStats, sizeof(unsigned long long) * ubg::kNumCounters * n, hipMemcpyDeviceToHost,
+                         st),
+          "hipMemcpy(batch stats)");
+  UBG_HIP(hipStreamSynchronize(st), "batch sync");
+  float msTrace = 0.0f, msPost = 0.0f;
+  UBG_HIP(hipEventElapsedTime(&msTrace, ctx->ev[0], ctx->ev[1]), "hipEventElapsedTime");
+  UBG_HIP(hipEventElapsedTime(&msPost, ctx->ev[1], ctx->ev[2]), "hipEventElapsedTime");
+  int status = UBG_OK;
+  for (int i = 0; i < n; ++i) {
+    ubg_frame* fr = &frames[i];
+    const unsigned long long* c = hStats + static_cast<size_t>(i) * ubg::kNumCounters;
+    fr->render_seconds = msTrace * 1e-3;  // the batch's trace window
+    fr->post_seconds = msPost * 1e-3;
+    std::memset(&fr->stats, 0, sizeof(fr->stats));
+    fr->stats.primary_rays = c[ubg::kCntPrimary];
+    fr->stats.shaded_hits = c[ubg::kCntShaded];
+    fr->stats.ao_rays_traced = c[ubg::kCntAoTraced];
+    fr->stats.ao_rays_sampled = c[ubg::kCntAoSampled];
+    fr->stats.shadow_rays = c[ubg::kCntShadow];
+    fr->stats.capped_rays = c[ubg::kCntCapped];
+    for (int k = 0; k < 4; ++k) {
+      fr->kernel_ms[k] = 0.0;
+      fr->kernel_launches[k] = 0;
+    }
+    if (fr->cancelled) status = UBG_ERR_CANCELLED;
+  }
+  if (status != UBG_OK) return fail(status, "ubg_render_batch: cancelled");
+  return UBG_OK;
 }
 
 int ubg_render_noalloc(ubg_context* ctx, const ubg_scene* scene, const ubg_render_params* p, ubg_frame* fr) {

@@ -26,6 +26,6 @@ struct DeviceBvh8 {
 // Returns UBG_OK or an error status with `err` set. `recs` = 3 float4 per
 // primitive, `boxes` = 2 float4 per primitive (lo.xyz, hi.xyz), n >= 1.
 int deviceBuildBvh8(const float4* recs, const float4* boxes, int64_t n, hipStream_t st, DeviceBvh8& out,
-                    std::string& err);
+                    std::string& err, int leafMax = 3);
 
 }  // namespace ubg

@@ -209,7 +209,8 @@ __global__ __launch_bounds__(kBlock) void k_flags(const int32_t* nn, int m, unsi
 
 __global__ __launch_bounds__(kBlock) void k_merge(const int32_t* cl, const int32_t* nn, int m,
                                                   const unsigned long long* packed, const unsigned long long* pre,
-                                                  int32_t nodeBase, Tree t, int32_t* cl2, int32_t* totals) {
+                                                  int32_t nodeBase, Tree t, int32_t* cl2, int32_t* totals,
+                                                  int leafMax) {
   const int i = blockIdx.x * kBlock + threadIdx.x;
   if (i >= m) return;
   const unsigned long long f = packed[i], p = pre[i];
@@ -256,7 +257,7 @@ __global__ __launch_bounds__(kBlock) void k_merge(const int32_t* cl, const int32
     dk[j] = static_cast<uint8_t>(bk);
   }
   const float asNode = A * kNodeCost + dist[8];
-  const float asLeaf = cnt <= kMaxLeaf8 ? A * static_cast<float>(cnt) * kPrimCost : INFINITY;
+  const float asLeaf = cnt <= leafMax ? A * static_cast<float>(cnt) * kPrimCost : INFINITY;
   t.leafSel[node] = asLeaf <= asNode ? 1 : 0;
   float prev = fminf(asLeaf, asNode);
   const size_t row = 8 * static_cast<size_t>(node);
@@ -492,7 +493,8 @@ inline unsigned gridFor(int64_t n, int b) { return static_cast<unsigned>((n + b 
 }  // namespace
 
 int deviceBuildBvh8(const float4* recs, const float4* boxes, int64_t n, hipStream_t st, DeviceBvh8& out,
-                    std::string& err) {
+                    std::string& err, int leafMax) {
+  leafMax = std::max(1, std::min(kMaxLeaf8, leafMax));
   out = DeviceBvh8();
   if (n < 1 || n >= (int64_t(1) << 28)) {
     err = "device BVH build: primitive count out of range";
@@ -570,7 +572,7 @@ int deviceBuildBvh8(const float4* recs, const float4* boxes, int64_t n, hipStrea
                                rocprim::plus<unsigned long long>(), st),
        "scan");
     hipLaunchKernelGGL(k_merge, dim3(gridFor(m, kBlock)), dim3(kBlock), 0, st, cl, nn, m, packed, pre, nodeBase, t,
-                       cl2, totals);
+                       cl2, totals, leafMax);
     ok(hipGetLastError(), "launch");
     ok(hipMemcpyAsync(hostTot, totals, 2 * sizeof(int32_t), hipMemcpyDeviceToHost, st), "copy");
     ok(hipStreamSynchronize(st), "round");

@@ -127,6 +127,7 @@ void launchGiComposite(const FrameConst& fc, const FrameBufs& b, float* indirect
 // Queue slots per partition for a W x H hi-res grid.
 int queuePartCap(int W, int H);
 
+
 // Device address of this device's copy of the fixed frame scratch
 // (UBG_NOALLOC_SCRATCH_BYTES, ub_static.hip); looked up once per device.
 hipError_t staticScratchAddress(int device, void** out);
@@ -135,6 +136,40 @@ hipError_t staticScratchAddress(int device, void** out);
 struct PartFill {
   uint32_t n[kQueueParts];
 };
+
+// Batched frames (ubg_render_batch). One request of a batch: its frame
+// constants, scene and buffers, the resolve parameters and output.
+constexpr int kMaxBatch = 256;
+struct BatchItem {
+  FrameConst fc;
+  SceneView s;
+  FrameBufs b;
+  float4* out;     // final (downsampled, gamma) colour
+  int32_t ss;
+  float gamma;
+};
+// The requests one batched launch covers: batch item, inclusive prefix of
+// the blocks, and for continuation layers the blocks per queue partition and
+// the partition fills.
+struct BatchLaunch {
+  int32_t n;
+  int32_t item[kMaxBatch];
+  uint32_t blockEnd[kMaxBatch];
+  int32_t partBlocks[kMaxBatch];
+  PartFill fill[kMaxBatch];
+};
+size_t traceLdsBytes(const SceneView& s);
+int64_t aoResidentBlocks(int wavesPerSimd);
+void launchZeroBatch(const BatchItem* items, int n, hipStream_t st);
+void launchResetBatch(const BatchItem* items, const BatchLaunch* L, int n, hipStream_t st);
+void launchPrimaryBatch(const BatchItem* items, const BatchLaunch* L, int64_t blocks, int iter, size_t lds,
+                        hipStream_t st);
+void launchAoBatch(const BatchItem* items, const BatchLaunch* L, int64_t blocks, size_t lds, hipStream_t st);
+void launchResolveBatch(const BatchItem* items, const BatchLaunch* L, int64_t blocks, int iter, size_t lds,
+                        hipStream_t st);
+void launchPostBatch(const BatchItem* items, const BatchLaunch* L, int64_t blocks, hipStream_t st);
+void launchFillsBatch(const BatchItem* items, int n, uint32_t* fills, hipStream_t st);
+void launchStatsBatch(const BatchItem* items, int n, unsigned long long* stats, hipStream_t st);
 
 void launchZeroCounters(unsigned long long* c, int n, hipStream_t st);
 void launchResetQueues(unsigned long long* c, hipStream_t st);

@@ -314,10 +314,13 @@ __device__ bool sampleCoarseAo(const FrameConst& fc, const FrameBufs& b, uint32_
   return true;
 }
 
+// One block of the primary pass; blk = the block's index within its frame's
+// launch (the block itself for a single frame, its offset into the frame's
+// block range for a batch).
 template <bool kCount, int kLayout>
-__global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(kTraceWavesPerSimd))) void k_primary(FrameConst fc, SceneView s, FrameBufs b,
-                                                         int iter, PartFill fill, int partBlocks) {
-  const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void primaryBlock(const FrameConst& fc, const SceneView& s, const FrameBufs& b, int iter,
+                                             const PartFill& fill, int partBlocks, uint32_t blk) {
+  const int tid = static_cast<int>(blk) * blockDim.x + threadIdx.x;
   bool valid;
   uint32_t pix = 0;  // output index of the pass
   if (iter == 0 && (fc.smMode == kSmGrid || (fc.smMode == kSmCenter && b.smList == nullptr))) {
@@ -336,8 +339,8 @@ __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(kTr
     pix = static_cast<uint32_t>(tid);
   } else {
     // later layers: partBlocks blocks per continuation-queue partition
-    const uint32_t p = blockIdx.x / static_cast<uint32_t>(partBlocks);
-    const uint32_t local = (blockIdx.x % static_cast<uint32_t>(partBlocks)) * blockDim.x + threadIdx.x;
+    const uint32_t p = blk / static_cast<uint32_t>(partBlocks);
+    const uint32_t local = (blk % static_cast<uint32_t>(partBlocks)) * blockDim.x + threadIdx.x;
     valid = local < fill.n[p];
     if (valid) pix = b.activeIn[p * static_cast<uint32_t>(b.partCap) + local];
   }
@@ -494,6 +497,12 @@ __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(kTr
 }
 
 template <bool kCount, int kLayout>
+__global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(kTraceWavesPerSimd))) void k_primary(FrameConst fc, SceneView s, FrameBufs b,
+                                                         int iter, PartFill fill, int partBlocks) {
+  primaryBlock<kCount, kLayout>(fc, s, b, iter, fill, partBlocks, blockIdx.x);
+}
+
+template <bool kCount, int kLayout>
 __global__ __launch_bounds__(kTraceBlock) void k_ao(FrameConst fc, SceneView s, FrameBufs b,
                                                     int unused) {
   (void)unused;
@@ -560,9 +569,10 @@ __global__ __launch_bounds__(kTraceBlock) void k_ao(FrameConst fc, SceneView s, 
 constexpr int kAoChunk = 256;
 constexpr int kAoRefill = 16;
 
+// The persistent AO loop of one frame: returns once every partition of its
+// hit queue is used up and every lane of the wave is idle.
 template <bool kCount>
-__global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(kAoWavesPerSimd))) void k_ao_persistent(
-    FrameConst fc, SceneView s, FrameBufs b) {
+__device__ __forceinline__ void aoPersistentFrame(const FrameConst& fc, const SceneView& s, const FrameBufs& b) {
   const uint32_t n = static_cast<uint32_t>(fc.aoSamples);
   uint32_t* sb = ubgLds + threadIdx.x;
   uint32_t* sh = sb + s.stackDepth * blockDim.x;
@@ -674,6 +684,12 @@ __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(kAo
     if (sa) atomicAdd(&statGroup(b.counters)[kCntAoSampled], (unsigned long long)sa);
   }
   if (kCount) addVisits(b.counters, 1, nv, nt);
+}
+
+template <bool kCount>
+__global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(kAoWavesPerSimd))) void k_ao_persistent(
+    FrameConst fc, SceneView s, FrameBufs b) {
+  aoPersistentFrame<kCount>(fc, s, b);
 }
 
 // Enhanced AO rays (ambient_occlusion.hpp:171-246): one lane per (hit-queue
@@ -1021,8 +1037,9 @@ __device__ V3 shadePrincipled(const FrameConst& fc, const SceneView& s, const De
 }
 
 template <bool kCount, int kLayout, bool kPbr>
-__device__ __forceinline__ void resolveHits(const FrameConst& fc, const SceneView& s, const FrameBufs& b, int iter) {
-  const int tid = blockIdx.x * blockDim.x + threadIdx.x;  // hit-queue slot
+__device__ __forceinline__ void resolveHits(const FrameConst& fc, const SceneView& s, const FrameBufs& b, int iter,
+                                            uint32_t blk) {
+  const int tid = static_cast<int>(blk) * blockDim.x + threadIdx.x;  // hit-queue slot
   const int part = tid / b.partCap;
   const bool valid = part < kQueueParts && static_cast<uint32_t>(tid % b.partCap) < queueFill(b.counters, kHitFill, part);
   uint32_t nShadow = 0, nv = 0, nt = 0;
@@ -1204,20 +1221,19 @@ __device__ __forceinline__ void resolveHits(const FrameConst& fc, const SceneVie
 
 template <bool kCount, int kLayout>
 __global__ __launch_bounds__(kTraceBlock) void k_resolve(FrameConst fc, SceneView s, FrameBufs b, int iter) {
-  resolveHits<kCount, kLayout, false>(fc, s, b, iter);
+  resolveHits<kCount, kLayout, false>(fc, s, b, iter, blockIdx.x);
 }
 template <bool kCount, int kLayout>
 __global__ __launch_bounds__(kTraceBlock) void k_resolve_pbr(FrameConst fc, SceneView s, FrameBufs b, int iter) {
-  resolveHits<kCount, kLayout, true>(fc, s, b, iter);
+  resolveHits<kCount, kLayout, true>(fc, s, b, iter, blockIdx.x);
 }
 
 // image_ops.cpp:9-46 -- box resolve (non-finite samples contribute 0) and
 // assumed gamma. ss == 1 keeps values as they are (the reference skips the
 // downsample entirely).
-__global__ void k_post(const float4* __restrict__ hi, int W, int H, int ss, float gamma,
-                       float4* __restrict__ out) {
+__device__ __forceinline__ void postPixel(const float4* __restrict__ hi, int W, int H, int ss, float gamma,
+                                          float4* __restrict__ out, int i) {
   const int ow = W / ss, oh = H / ss;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= ow * oh) return;
   const int x = i % ow, y = i / ow;
   float4 v;
@@ -1242,6 +1258,92 @@ __global__ void k_post(const float4* __restrict__ hi, int W, int H, int ss, floa
     v.z = v.z > 0.0f ? powf(v.z, gamma) : 0.0f;
   }
   out[i] = v;
+}
+__global__ void k_post(const float4* __restrict__ hi, int W, int H, int ss, float gamma,
+                       float4* __restrict__ out) {
+  postPixel(hi, W, H, ss, gamma, out, blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+
+// ---------------------------------------------------------------------------
+// Batched frames (ubg_render_batch): the frames of several requests share
+// each launch. A BatchItem (device memory) holds one request's frame
+// constants, scene and buffers; a BatchLaunch lists the requests a launch
+// covers with their block ranges, so a block finds its request by a binary
+// search over the block prefix (block-uniform, scalar loads) and runs the
+// single-frame block code on that request's data.
+
+// Entry of the launch list that block `blk` belongs to.
+__device__ __forceinline__ int batchEntry(const BatchLaunch& L, uint32_t blk) {
+  int lo = 0, hi = L.n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (blk < L.blockEnd[mid]) hi = mid;
+    else lo = mid + 1;
+  }
+  return lo;
+}
+
+__global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(kTraceWavesPerSimd))) void k_primary_batch(
+    const BatchItem* __restrict__ items, const BatchLaunch* __restrict__ L, int iter) {
+  const int e = batchEntry(*L, blockIdx.x);
+  const uint32_t first = e == 0 ? 0u : L->blockEnd[e - 1];
+  const BatchItem& it = items[L->item[e]];
+  primaryBlock<false, kLayoutBvh8>(it.fc, it.s, it.b, iter, L->fill[e], L->partBlocks[e], blockIdx.x - first);
+}
+
+__global__ __launch_bounds__(kTraceBlock) void k_resolve_batch(const BatchItem* __restrict__ items,
+                                                               const BatchLaunch* __restrict__ L, int iter) {
+  const int e = batchEntry(*L, blockIdx.x);
+  const uint32_t first = e == 0 ? 0u : L->blockEnd[e - 1];
+  const BatchItem& it = items[L->item[e]];
+  resolveHits<false, kLayoutBvh8, false>(it.fc, it.s, it.b, iter, blockIdx.x - first);
+}
+
+// One persistent grid walks the listed requests in order: every wave drains
+// a request's hit-queue partitions (its lanes all idle) before it moves to
+// the next, so a wave's lanes always trace one scene.
+__global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(kAoWavesPerSimd))) void k_ao_batch(
+    const BatchItem* __restrict__ items, const BatchLaunch* __restrict__ L) {
+  for (int e = 0; e < L->n; ++e) {
+    const BatchItem& it = items[L->item[e]];
+    if (it.fc.aoSamples > 0) aoPersistentFrame<false>(it.fc, it.s, it.b);
+  }
+}
+
+__global__ void k_post_batch(const BatchItem* __restrict__ items, const BatchLaunch* __restrict__ L) {
+  const int e = batchEntry(*L, blockIdx.x);
+  const uint32_t first = e == 0 ? 0u : L->blockEnd[e - 1];
+  const BatchItem& it = items[L->item[e]];
+  postPixel(it.b.hiColor, it.fc.W, it.fc.H, it.ss, it.gamma, it.out,
+            static_cast<int>((blockIdx.x - first) * blockDim.x + threadIdx.x));
+}
+
+// Counters of every request to zero (block y = request), or only the queue
+// fills and AO cursors of the listed ones.
+__global__ void k_zero_batch(const BatchItem* __restrict__ items, int words) {
+  unsigned long long* c = items[blockIdx.y].b.counters;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < words; i += gridDim.x * blockDim.x) c[i] = 0ull;
+}
+__global__ void k_reset_batch(const BatchItem* __restrict__ items, const BatchLaunch* __restrict__ L) {
+  if (static_cast<int>(blockIdx.x) >= L->n) return;
+  unsigned long long* c = items[L->item[blockIdx.x]].b.counters;
+  if (threadIdx.x < 3 * kQueueParts) c[kHitFill + kLineWords * threadIdx.x] = 0ull;
+}
+// Continuation-queue fills (request-major, kQueueParts each) and the summed
+// statistics (request-major, kNumCounters each) of every request.
+__global__ void k_fills_batch(const BatchItem* __restrict__ items, uint32_t* __restrict__ fills) {
+  const unsigned long long* c = items[blockIdx.x].b.counters;
+  if (threadIdx.x < kQueueParts)
+    fills[blockIdx.x * kQueueParts + threadIdx.x] = static_cast<uint32_t>(c[kActiveFill + kLineWords * threadIdx.x]);
+}
+__global__ void k_stats_batch(const BatchItem* __restrict__ items, unsigned long long* __restrict__ stats) {
+  const unsigned long long* c = items[blockIdx.x].b.counters;
+  if (threadIdx.x < kNumCounters) {
+    unsigned long long v = 0;
+    for (int g = 0; g < kStatSpread; ++g) v += c[g * kNumCounters + threadIdx.x];
+    stats[blockIdx.x * kNumCounters + threadIdx.x] = v;
+  }
 }
 
 struct RayIn {
@@ -2602,6 +2704,48 @@ void launchAo(const FrameConst& fc, const SceneView& s, const FrameBufs& b, bool
     return;
   }
   UBG_DISPATCH(k_ao, blocksFor(slots * fc.aoLanes, kTraceBlock), countVisits, fc, s, b, 0);
+}
+
+// ---- batched frames ----
+size_t traceLdsBytes(const SceneView& s) { return stackLdsBytes(s, kTraceBlock); }
+int64_t aoResidentBlocks(int wavesPerSimd) {
+  return static_cast<int64_t>(computeUnits()) * 4 * wavesPerSimd * 64 / kTraceBlock;
+}
+void launchZeroBatch(const BatchItem* items, int n, hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_zero_batch, dim3(8, static_cast<unsigned>(n)), dim3(256), 0, st, items, kCounterWords);
+}
+void launchResetBatch(const BatchItem* items, const BatchLaunch* L, int n, hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_reset_batch, dim3(static_cast<unsigned>(n)), dim3(64), 0, st, items, L);
+}
+void launchPrimaryBatch(const BatchItem* items, const BatchLaunch* L, int64_t blocks, int iter, size_t lds,
+                        hipStream_t st) {
+  if (blocks <= 0) return;
+  hipLaunchKernelGGL(k_primary_batch, dim3(static_cast<unsigned>(blocks)), dim3(kTraceBlock), lds, st, items, L,
+                     iter);
+}
+void launchAoBatch(const BatchItem* items, const BatchLaunch* L, int64_t blocks, size_t lds, hipStream_t st) {
+  if (blocks <= 0) return;
+  hipLaunchKernelGGL(k_ao_batch, dim3(static_cast<unsigned>(blocks)), dim3(kTraceBlock), lds, st, items, L);
+}
+void launchResolveBatch(const BatchItem* items, const BatchLaunch* L, int64_t blocks, int iter, size_t lds,
+                        hipStream_t st) {
+  if (blocks <= 0) return;
+  hipLaunchKernelGGL(k_resolve_batch, dim3(static_cast<unsigned>(blocks)), dim3(kTraceBlock), lds, st, items, L,
+                     iter);
+}
+void launchPostBatch(const BatchItem* items, const BatchLaunch* L, int64_t blocks, hipStream_t st) {
+  if (blocks <= 0) return;
+  hipLaunchKernelGGL(k_post_batch, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, st, items, L);
+}
+void launchFillsBatch(const BatchItem* items, int n, uint32_t* fills, hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_fills_batch, dim3(static_cast<unsigned>(n)), dim3(64), 0, st, items, fills);
+}
+void launchStatsBatch(const BatchItem* items, int n, unsigned long long* stats, hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_stats_batch, dim3(static_cast<unsigned>(n)), dim3(64), 0, st, items, stats);
 }
 void launchResolve(const FrameConst& fc, const SceneView& s, const FrameBufs& b, int iter, bool countVisits,
                    hipStream_t st) {

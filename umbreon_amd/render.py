@@ -9,7 +9,7 @@ from __future__ import annotations
 
 import ctypes as C
 from dataclasses import dataclass, field
-from typing import Optional
+from typing import Optional, Sequence
 
 import numpy as np
 
@@ -79,12 +79,14 @@ class Device:
 class BuiltScene:
     """Scene geometry resident in HBM with its BVH (buildEmbreeScene analogue)."""
 
-    def __init__(self, device: Device, scene: Scene, layout: int = capi.BVH8):
+    def __init__(self, device: Device, scene: Scene, layout: int = capi.BVH8, leaf_max: int = 0):
         self.device = device
         self._lib = device._lib
         desc, keep = scene.to_desc()
         h = C.c_void_p()
-        capi.check(self._lib.ubg_scene_create_ex(device.handle, C.byref(desc), int(layout), C.byref(h)))
+        so = capi.SceneOptions()
+        so.bvh_layout, so.leaf_max = int(layout), int(leaf_max)
+        capi.check(self._lib.ubg_scene_create_opts(device.handle, C.byref(desc), C.byref(so), C.byref(h)))
         del keep
         self.handle = h
         info = capi.SceneInfo()
@@ -181,6 +183,59 @@ class BuiltScene:
                            kernel_launches={k: int(fr.kernel_launches[i]) for i, k in enumerate(capi.KERNELS)},
                            aa_mask=aa_mask, aa_flagged=int(fr.aa_flagged), indirect=indirect,
                            gi_occlusion=gi_occ, gi_features=feats)
+
+
+def render_batch(device: Device, built: Sequence["BuiltScene"], scenes: Sequence[Scene],
+                 opts: Sequence[RenderOptions], want_ids: bool = False, want_ao: bool = False,
+                 hires_only: bool = False, ao_waves_per_simd: int = 0, color_only: bool = False,
+                 graph: bool = True) -> list:
+    """Several requests in one batched render (ubg_render_batch): request i
+    is scene scenes[i] (built as built[i]) with options opts[i]; returns one
+    FrameResult per request, each equal to BuiltScene.render of it alone
+    (color_only: only the colour comes back to the host)."""
+    n = len(built)
+    assert len(scenes) == n and len(opts) == n and n > 0
+    lib = device._lib
+    keeps, outs = [], []
+    P = (capi.RenderParams * n)()
+    F = (capi.Frame * n)()
+    H = (C.c_void_p * n)(*[b.handle.value for b in built])
+    for i in range(n):
+        flags = capi.FLAG_HIRES_ONLY if hires_only else 0
+        p, keep = params(scenes[i], opts[i], flags)
+        P[i] = p
+        keeps.append(keep)
+        opt = opts[i]
+        ss = max(1, int(opt.supersample))
+        hw, hh = opt.width * ss, opt.height * ss
+        ow, oh = (hw, hh) if hires_only else (opt.width, opt.height)
+        o = {"color": np.empty((oh, ow, 4), np.float32),
+             "depth": None if color_only else np.zeros((hh, hw), np.float32),
+             "view_z": None if color_only else np.zeros((hh, hw), np.float32),
+             "hit_id": np.zeros((hh, hw), np.uint32) if want_ids else None,
+             "ao_open": np.zeros((hh, hw), np.float32) if want_ao else None, "w": ow, "h": oh}
+        outs.append(o)
+        F[i].color = o["color"].ctypes.data
+        F[i].depth = None if color_only else o["depth"].ctypes.data
+        F[i].view_z = None if color_only else o["view_z"].ctypes.data
+        F[i].hit_id = o["hit_id"].ctypes.data if want_ids else None
+        F[i].ao_open = o["ao_open"].ctypes.data if want_ao else None
+        F[i].memory = 0
+    bo = capi.BatchOptions()
+    bo.ao_waves_per_simd = int(ao_waves_per_simd)
+    bo.no_graph = 0 if graph else 1
+    capi.check(lib.ubg_render_batch(device.handle, n, H, P, F, C.byref(bo)))
+    del keeps
+    res = []
+    for i, o in enumerate(outs):
+        fr = F[i]
+        res.append(FrameResult(width=o["w"], height=o["h"], color=o["color"], depth=o["depth"], view_z=o["view_z"],
+                               hit_id=o["hit_id"], ao_open=o["ao_open"], stats=fr.stats.as_dict(),
+                               render_seconds=fr.render_seconds, post_seconds=fr.post_seconds,
+                               cancelled=bool(fr.cancelled), layers=int(fr.layers),
+                               kernel_ms={k: 0.0 for k in capi.KERNELS}, kernel_launches={k: 0 for k in capi.KERNELS},
+                               aa_mask=None, aa_flagged=0, indirect=None, gi_occlusion=None, gi_features=None))
+    return res
 
 
 def host_query(scene: Scene, rays: np.ndarray, layout: int = capi.BVH8):
