@@ -436,6 +436,65 @@ typedef struct ubg_batch_options {
 int ubg_render_batch(ubg_context* ctx, int32_t n, const ubg_scene* const* scenes,
                      const ubg_render_params* params, ubg_frame* frames, const ubg_batch_options* opts);
 
+/* ---- Physically based path tracing (ubg_pathtrace) ----
+ * No reference counterpart: the reference's GI integrators (pt1 / pt2) are
+ * diffuse-only and refraction is outside its scope (docs/api/libumbreon.md:33).
+ * GGX conductors (Schlick Fresnel, F0 = the primitive's colour) and GGX
+ * dielectrics (exact Fresnel, reflection + refraction, Walter et al. 2007),
+ * Lambertian surfaces, spherical area lights reached by next-event
+ * estimation only (resampled importance sampling over M candidates at every
+ * vertex, ReSTIR spatial reuse over K neighbours at the first), a constant
+ * environment for rays that leave the scene. */
+typedef struct ubg_sphere_light {
+  float center[3];
+  float radius;
+  float radiance[3];
+  float reserved;
+} ubg_sphere_light;
+
+enum { UBG_PT_DIFFUSE = 0, UBG_PT_CONDUCTOR = 1, UBG_PT_DIELECTRIC = 2 };
+
+typedef struct ubg_pt_material {    /* indexed like the scene's material indices */
+  int32_t type;                      /* UBG_PT_* */
+  float roughness;                   /* GGX alpha = roughness^2; < 0.02: smooth */
+  float ior;                         /* dielectric: inside / outside */
+  int32_t reserved;
+} ubg_pt_material;
+
+typedef struct ubg_pt_params {
+  int32_t width, height;             /* image (1 camera ray per pixel per sample) */
+  ubg_camera camera;
+  int32_t spp;                       /* samples per pixel, 1..65536 */
+  int32_t bounces;                   /* shaded path vertices, 1..64 */
+  uint32_t seed;
+  const ubg_sphere_light* lights;    /* host array */
+  int32_t n_lights;                  /* 0..4096 */
+  float env_radiance[3];
+  const ubg_pt_material* prim_materials;  /* host arrays; NULL / 0: diffuse */
+  int32_t n_prim_materials;
+  const ubg_pt_material* mesh_materials;
+  int32_t n_mesh_materials;
+  int32_t restir_candidates;         /* M, 1..12 (0 = 8) */
+  int32_t restir_spatial;            /* K, 0..16 */
+  float restir_radius;               /* pixels (0 = 16) */
+  int32_t nee_all_lights;            /* 1: one cone sample of every light at every vertex */
+  struct ubg_unet* denoise_net;      /* optional: the colour is the network's denoising */
+  int32_t reserved[8];
+} ubg_pt_params;
+
+typedef struct ubg_pt_frame {
+  float* color;           /* width * height * 4 (linear HDR, alpha 1) */
+  float* features;        /* width * height * 10 or NULL: log(1 + noisy colour) rgb,
+                             first-hit albedo rgb, normal xyz, distance / frame max */
+  int32_t memory;         /* 0 host buffers, 1 device */
+  double trace_seconds;   /* the spp sample passes (device events) */
+  double denoise_seconds; /* features + network + output */
+  uint64_t rays;          /* camera, continuation and shadow rays traced */
+  int32_t reserved[4];
+} ubg_pt_frame;
+
+int ubg_pathtrace(ubg_context* ctx, const ubg_scene* scene, const ubg_pt_params* params, ubg_pt_frame* frame);
+
 /* Size of the fixed frame scratch of ubg_render_noalloc: one device array
  * of this many bytes per GPU the library runs on, held until the process
  * ends. */

@@ -82,6 +82,17 @@ def load():
     lib.orc_pt2_pixel_sampler.argtypes = [C.c_int32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
                                           C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]
     lib.orc_pt2_pixel_sampler.restype = None
+    lib.orc_pathtrace.argtypes = [C.POINTER(capi.SceneDesc), C.POINTER(capi.PtParams), C.c_void_p, C.c_void_p,
+                                  C.POINTER(C.c_uint64), C.POINTER(OrcOptions)]
+    lib.orc_pathtrace.restype = C.c_int
+    lib.orc_pt_bsdf_eval.argtypes = [C.c_int32, C.c_float, C.c_float, C.c_void_p, C.c_int64, C.c_void_p,
+                                     C.c_void_p, C.c_void_p]
+    lib.orc_pt_bsdf_eval.restype = None
+    lib.orc_pt_bsdf_sample.argtypes = [C.c_int32, C.c_float, C.c_float, C.c_void_p, C.c_int64, C.c_void_p,
+                                       C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+    lib.orc_pt_bsdf_sample.restype = None
+    lib.orc_pt_rnd.argtypes = [C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32]
+    lib.orc_pt_rnd.restype = C.c_float
     _lib = lib
     return lib
 
@@ -300,3 +311,52 @@ def object_space_edges(scene: Scene, opt) -> dict:
     finally:
         lib.orc_edges_free(C.byref(out))
         del keep
+
+
+def pathtrace(scene: Scene, opt, precision: int = 32, threads: int = 0, brute: bool = False):
+    """The path tracer's oracle (umbreon_amd.pathtrace.PtOptions): returns
+    (colour H x W x 4, first-hit record H x W x 8, rays traced)."""
+    from umbreon_amd.pathtrace import pt_params
+    lib = load()
+    desc, keep1 = scene.to_desc()
+    p, keep2 = pt_params(scene.camera, opt)
+    color = np.zeros((opt.height, opt.width, 4), np.float32)
+    first = np.zeros((opt.height, opt.width, 8), np.float32)
+    rays = C.c_uint64(0)
+    o = _opts(precision, threads, None, brute)
+    capi.check(lib.orc_pathtrace(C.byref(desc), C.byref(p), color.ctypes.data, first.ctypes.data, C.byref(rays),
+                                 C.byref(o)))
+    del keep1, keep2
+    return color, first, int(rays.value)
+
+
+def pt_bsdf_eval(mtype: int, roughness: float, ior: float, base, wo, wi) -> np.ndarray:
+    """f(wo, wi) |cos wi| (n x 3) of the path tracer's BSDF in the local frame
+    (n = +z), FP64, for direction arrays wo, wi (n x 3)."""
+    lib = load()
+    c = np.ascontiguousarray(np.asarray(base, np.float32).reshape(3))
+    a, b = np.broadcast_arrays(np.asarray(wo, np.float64).reshape(-1, 3), np.asarray(wi, np.float64).reshape(-1, 3))
+    a, b = np.ascontiguousarray(a), np.ascontiguousarray(b)
+    out = np.zeros_like(b)
+    lib.orc_pt_bsdf_eval(int(mtype), float(roughness), float(ior), c.ctypes.data, len(b), a.ctypes.data,
+                         b.ctypes.data, out.ctypes.data)
+    return out
+
+
+def pt_bsdf_sample(mtype: int, roughness: float, ior: float, base, wo, u):
+    """BSDF samples for wo (n x 3, or one vector) and uniforms u (n x 3):
+    (wi n x 3, weight n x 3 = f |cos| / pdf, ok n)."""
+    lib = load()
+    c = np.ascontiguousarray(np.asarray(base, np.float32).reshape(3))
+    uu = np.ascontiguousarray(np.asarray(u, np.float64).reshape(-1, 3))
+    a = np.ascontiguousarray(np.broadcast_to(np.asarray(wo, np.float64).reshape(-1, 3), uu.shape))
+    wi = np.zeros_like(uu)
+    w = np.zeros_like(uu)
+    ok = np.zeros(len(uu), np.uint8)
+    lib.orc_pt_bsdf_sample(int(mtype), float(roughness), float(ior), c.ctypes.data, len(uu), a.ctypes.data,
+                           uu.ctypes.data, wi.ctypes.data, w.ctypes.data, ok.ctypes.data)
+    return wi, w, ok.astype(bool)
+
+
+def pt_rnd(seed: int, pixel: int, sample: int, dim: int) -> float:
+    return float(load().orc_pt_rnd(seed, pixel, sample, dim))

@@ -3609,3 +3609,727 @@ float orc_srgb_decode(float s) {
 }
 
 }  // extern "C"
+
+// ===========================================================================
+// Physically based path tracer of ubg_pathtrace (no reference counterpart:
+// the reference's integrators are diffuse-only and docs/api/libumbreon.md:33
+// puts refraction out of its scope). This restates the MI355X kernels'
+// algorithm step for step -- the same counter-based random numbers and
+// dimension layout, the same float-op order for T = float -- so the FP32
+// oracle follows the device per sample, and T = double is the FP64 oracle of
+// the same estimator. Its pins are closed forms: the white furnace of a
+// smooth dielectric / diffuse / mirror sphere (radiance exactly the
+// environment's), the irradiance of one spherical light on a plane, the
+// BSDF sampling weights against quadrature of the evaluated BSDF, the
+// (generalised) reciprocity of the evaluated BSDF, and ReSTIR's estimate
+// against the all-lights next-event estimator (tests/test_pt_oracle.py).
+namespace orc {
+namespace pt {
+
+template <class T>
+struct Onb {
+  V3<T> t, b, n;
+};
+template <class T>
+inline T tpi() { return T(3.14159265358979323846); }
+
+inline uint32_t hash32(uint32_t x) {
+  x ^= x >> 16;
+  x *= 0x7FEB352Du;
+  x ^= x >> 15;
+  x *= 0x846CA68Bu;
+  x ^= x >> 16;
+  return x;
+}
+inline float rndf(uint32_t seed, uint32_t pixel, uint32_t sample, uint32_t dim) {
+  const uint32_t h = hash32(seed ^ hash32(pixel ^ hash32((sample * 0x9E3779B9u) ^ hash32(dim + 0x632BE5ABu))));
+  return static_cast<float>(h >> 8) * 0x1.0p-24f;
+}
+template <class T>
+inline T rnd(uint32_t seed, uint32_t pixel, uint32_t sample, uint32_t dim) {
+  return T(rndf(seed, pixel, sample, dim));
+}
+
+template <class T>
+inline V3<T> mkv(T x, T y, T z) { return V3<T>{x, y, z}; }
+template <class T>
+inline V3<T> mul(V3<T> a, V3<T> b) { return V3<T>{a.x * b.x, a.y * b.y, a.z * b.z}; }
+
+// Duff et al. 2017
+template <class T>
+inline Onb<T> onb(V3<T> n) {
+  const T sgn = n.z >= T(0) ? T(1) : T(-1);
+  const T a = T(-1) / (sgn + n.z);
+  const T b = n.x * n.y * a;
+  return Onb<T>{mkv<T>(T(1) + sgn * n.x * n.x * a, sgn * b, -sgn * n.x), mkv<T>(b, sgn + n.y * n.y * a, -n.y), n};
+}
+template <class T>
+inline V3<T> toLocal(const Onb<T>& f, V3<T> v) { return mkv<T>(dot(v, f.t), dot(v, f.b), dot(v, f.n)); }
+template <class T>
+inline V3<T> toWorld(const Onb<T>& f, V3<T> v) {
+  return mkv<T>(f.t.x * v.x + f.b.x * v.y + f.n.x * v.z, f.t.y * v.x + f.b.y * v.y + f.n.y * v.z,
+                f.t.z * v.x + f.b.z * v.y + f.n.z * v.z);
+}
+
+// GGX in the local frame (n = +z): D, Smith Lambda / G1 / height-correlated G2
+template <class T>
+inline T ggxD(V3<T> m, T a) {
+  const T c2 = m.z * m.z;
+  if (c2 <= T(0)) return T(0);
+  const T a2 = a * a;
+  const T k = c2 * (a2 - T(1)) + T(1);
+  return a2 / (tpi<T>() * k * k);
+}
+template <class T>
+inline T ggxLambda(V3<T> w, T a) {
+  const T c2 = w.z * w.z;
+  if (c2 <= T(0)) return T(0);
+  const T t2 = std::fmax(T(0), T(1) - c2) / c2;
+  return T(0.5f) * (std::sqrt(T(1) + a * a * t2) - T(1));
+}
+template <class T>
+inline T ggxG1(V3<T> w, T a) { return T(1) / (T(1) + ggxLambda(w, a)); }
+template <class T>
+inline T ggxG2(V3<T> wo, V3<T> wi, T a) { return T(1) / (T(1) + ggxLambda(wo, a) + ggxLambda(wi, a)); }
+// visible-normal sampling (Heitz 2018)
+template <class T>
+inline V3<T> ggxSampleVisible(V3<T> wo, T a, T u1, T u2) {
+  const V3<T> vh = normalize(mkv<T>(a * wo.x, a * wo.y, wo.z));
+  const T l2 = vh.x * vh.x + vh.y * vh.y;
+  const V3<T> T1 = l2 > T(0) ? mkv<T>(-vh.y, vh.x, T(0)) * (T(1) / std::sqrt(l2)) : mkv<T>(T(1), T(0), T(0));
+  const V3<T> T2 = cross(vh, T1);
+  const T r = std::sqrt(u1);
+  const T phi = T(2) * tpi<T>() * u2;
+  const T t1 = r * std::cos(phi);
+  T t2 = r * std::sin(phi);
+  const T s = T(0.5f) * (T(1) + vh.z);
+  t2 = (T(1) - s) * std::sqrt(std::fmax(T(0), T(1) - t1 * t1)) + s * t2;
+  const T tz = std::sqrt(std::fmax(T(0), T(1) - t1 * t1 - t2 * t2));
+  const V3<T> nh = mkv<T>(T1.x * t1 + T2.x * t2 + vh.x * tz, T1.y * t1 + T2.y * t2 + vh.y * tz,
+                          T1.z * t1 + T2.z * t2 + vh.z * tz);
+  return normalize(mkv<T>(a * nh.x, a * nh.y, std::fmax(T(1.0e-6f), nh.z)));
+}
+// unpolarised dielectric Fresnel; eta = n_t / n_i
+template <class T>
+inline T fresnelDielectric(T cosI, T eta) {
+  const T s2 = std::fmax(T(0), T(1) - cosI * cosI) / (eta * eta);
+  if (s2 >= T(1)) return T(1);
+  const T cosT = std::sqrt(T(1) - s2);
+  const T rs = (cosI - eta * cosT) / (cosI + eta * cosT);
+  const T rp = (eta * cosI - cosT) / (eta * cosI + cosT);
+  return T(0.5f) * (rs * rs + rp * rp);
+}
+template <class T>
+inline V3<T> schlick(V3<T> f0, T c) {
+  const T m = std::fmin(T(1), std::fmax(T(0), T(1) - c));
+  const T m5 = (m * m) * (m * m) * m;
+  return mkv<T>(f0.x + (T(1) - f0.x) * m5, f0.y + (T(1) - f0.y) * m5, f0.z + (T(1) - f0.z) * m5);
+}
+
+template <class T>
+struct Surf {
+  Onb<T> f;
+  V3<T> C;
+  int type;
+  T alpha, ior;
+};
+constexpr int kDiffuse = 0, kConductor = 1, kDielectric = 2;
+constexpr float kSmoothAlpha = 4.0e-4f;
+
+template <class T>
+inline V3<T> bsdfCos(const Surf<T>& s, V3<T> wo, V3<T> wi) {
+  const V3<T> zero{T(0), T(0), T(0)};
+  if (s.type == kDiffuse) {
+    if (wo.z <= T(0) || wi.z <= T(0)) return zero;
+    return s.C * (wi.z / tpi<T>());
+  }
+  if (s.alpha < T(kSmoothAlpha)) return zero;
+  if (s.type == kConductor) {
+    if (wo.z <= T(0) || wi.z <= T(0)) return zero;
+    const V3<T> m = normalize(wo + wi);
+    const V3<T> F = schlick(s.C, dot(wi, m));
+    const T k = ggxD(m, s.alpha) * ggxG2(wo, wi, s.alpha) / (T(4) * wo.z);
+    return F * k;
+  }
+  const T cosO = wo.z, cosI = wi.z;
+  if (cosO == T(0) || cosI == T(0)) return zero;
+  const bool refl = cosO * cosI > T(0);
+  const T eta = cosO > T(0) ? s.ior : T(1) / s.ior;
+  V3<T> m = refl ? (wo + wi) : (wi * eta + wo);
+  if (dot(m, m) == T(0)) return zero;
+  m = normalize(m);
+  if (m.z < T(0)) m = neg(m);
+  if (dot(m, wi) * cosI < T(0) || dot(m, wo) * cosO < T(0)) return zero;
+  const T F = fresnelDielectric(std::fabs(dot(wo, m)), cosO > T(0) ? s.ior : T(1) / s.ior);
+  const T D = ggxD(m, s.alpha);
+  const T G = ggxG2(mkv<T>(wo.x, wo.y, std::fabs(wo.z)), mkv<T>(wi.x, wi.y, std::fabs(wi.z)), s.alpha);
+  if (refl) {
+    const T k = D * G * F / (T(4) * std::fabs(cosO));
+    return mkv<T>(k, k, k);
+  }
+  const T dn = dot(wi, m) + dot(wo, m) / eta;
+  const T k = D * (T(1) - F) * G * std::fabs(dot(wi, m) * dot(wo, m) / (dn * dn * cosO)) / (eta * eta);
+  return s.C * k;
+}
+
+template <class T>
+inline bool bsdfSample(const Surf<T>& s, V3<T> wo, T u1, T u2, T u3, V3<T>& wi, V3<T>& weight) {
+  if (s.type == kDiffuse) {
+    if (wo.z <= T(0)) return false;
+    const T r = std::sqrt(u1), phi = T(2) * tpi<T>() * u2;
+    wi = mkv<T>(r * std::cos(phi), r * std::sin(phi), std::sqrt(std::fmax(T(0), T(1) - u1)));
+    weight = s.C;
+    return true;
+  }
+  if (s.type == kConductor) {
+    if (wo.z <= T(0)) return false;
+    const V3<T> m = s.alpha < T(kSmoothAlpha) ? mkv<T>(T(0), T(0), T(1)) : ggxSampleVisible(wo, s.alpha, u1, u2);
+    const T om = dot(wo, m);
+    wi = m * (T(2) * om) - wo;
+    if (wi.z <= T(0)) return false;
+    const V3<T> F = schlick(s.C, dot(wi, m));
+    weight = s.alpha < T(kSmoothAlpha) ? F : F * (ggxG2(wo, wi, s.alpha) / ggxG1(wo, s.alpha));
+    return true;
+  }
+  const bool outside = wo.z > T(0);
+  const V3<T> w = outside ? wo : mkv<T>(wo.x, wo.y, -wo.z);
+  if (w.z == T(0)) return false;
+  const T eta = outside ? s.ior : T(1) / s.ior;
+  const V3<T> m = s.alpha < T(kSmoothAlpha) ? mkv<T>(T(0), T(0), T(1)) : ggxSampleVisible(w, s.alpha, u1, u2);
+  const T c = dot(w, m);
+  const T F = fresnelDielectric(c, eta);
+  V3<T> wl;
+  bool refracted = false;
+  if (u3 < F) {
+    wl = m * (T(2) * c) - w;
+    if (wl.z <= T(0)) return false;
+  } else {
+    const T s2 = std::fmax(T(0), T(1) - c * c) / (eta * eta);
+    if (s2 >= T(1)) return false;
+    const T ct = std::sqrt(T(1) - s2);
+    wl = m * (c / eta - ct) - w * (T(1) / eta);
+    if (wl.z >= T(0)) return false;
+    refracted = true;
+  }
+  const T g = s.alpha < T(kSmoothAlpha) ? T(1) : ggxG2(w, mkv<T>(wl.x, wl.y, std::fabs(wl.z)), s.alpha) / ggxG1(w, s.alpha);
+  wi = outside ? wl : mkv<T>(wl.x, wl.y, -wl.z);
+  weight = refracted ? s.C * (g / (eta * eta)) : mkv<T>(g, g, g);
+  return true;
+}
+
+template <class T>
+struct SLight {
+  V3<T> c;
+  T r;
+  V3<T> Le;
+};
+template <class T>
+inline bool lightSample(const SLight<T>& L, V3<T> P, T u1, T u2, int nLights, V3<T>& Q, T& pdfA) {
+  const V3<T> dcv = L.c - P;
+  const T d2 = dot(dcv, dcv);
+  const T r2 = L.r * L.r;
+  if (d2 <= r2) return false;
+  const T d = std::sqrt(d2);
+  const V3<T> axis = dcv * (T(1) / d);
+  const T cosMax = std::sqrt(std::fmax(T(0), T(1) - r2 / d2));
+  const T cosT = T(1) - u1 * (T(1) - cosMax);
+  const T sinT = std::sqrt(std::fmax(T(0), T(1) - cosT * cosT));
+  const T phi = T(2) * tpi<T>() * u2;
+  const Onb<T> f = onb(axis);
+  const V3<T> dir = toWorld(f, mkv<T>(sinT * std::cos(phi), sinT * std::sin(phi), cosT));
+  const T b = dot(dcv, dir);
+  const T disc = b * b - (d2 - r2);
+  const T t = b - std::sqrt(std::fmax(T(0), disc));
+  Q = P + dir * t;
+  const V3<T> nl = (Q - L.c) * (T(1) / L.r);
+  const T cosL = std::fabs(dot(nl, dir));
+  const T pdfW = T(1) / (T(2) * tpi<T>() * (T(1) - cosMax));
+  pdfA = pdfW * cosL / (t * t) / T(static_cast<float>(nLights));
+  return pdfA > T(0);
+}
+template <class T>
+inline T lum(V3<T> c) { return T(0.2126f) * c.x + T(0.7152f) * c.y + T(0.0722f) * c.z; }
+
+struct PMat {
+  int type;
+  float alpha, ior;
+};
+
+template <class T>
+struct PCtx {
+  const OScene* s;
+  int W, H;
+  bool ortho;
+  V3<T> pos, dir, right, up;
+  T halfW, halfH, persHalfW, persHalfH;
+  uint32_t seed;
+  int bounces, nLights, M, K;
+  float radius;
+  bool neeAll;
+  V3<T> env;
+  std::vector<SLight<T>> lights;
+  std::vector<PMat> primMats, meshMats;
+};
+
+template <class T>
+inline V3<T> lightContrib(const PCtx<T>& c, const Surf<T>& s, V3<T> P, V3<T> wo, V3<T> Q, int li, T& dist) {
+  const SLight<T>& L = c.lights[li];
+  const V3<T> dv = Q - P;
+  const T d2 = dot(dv, dv);
+  dist = std::sqrt(d2);
+  const V3<T> zero{T(0), T(0), T(0)};
+  if (!(d2 > T(0))) return zero;
+  const V3<T> wi = dv * (T(1) / dist);
+  const V3<T> nl = (Q - L.c) * (T(1) / L.r);
+  const T cosL = -dot(nl, wi);
+  if (cosL <= T(0)) return zero;
+  const V3<T> fc = bsdfCos(s, toLocal(s.f, wo), toLocal(s.f, wi));
+  const T g = cosL / d2;
+  return mkv<T>(fc.x * L.Le.x * g, fc.y * L.Le.y * g, fc.z * L.Le.z * g);
+}
+
+template <class T>
+inline Surf<T> surfaceAt(const PCtx<T>& c, const Hit<T>& h, V3<T> rd, V3<T>& Ng) {
+  const Geometry& g = c.s->g;
+  V3<T> N, C;
+  uint32_t mat;
+  bool mesh = false;
+  if (h.kind == UBG_KIND_MESH) {
+    const uint32_t* ix = &g.idx[3 * static_cast<size_t>(h.id)];
+    const T w0 = T(1) - h.u - h.v;
+    T nb[3], cb[3];
+    for (int k = 0; k < 3; ++k) {
+      nb[k] = w0 * T(g.nrm[3 * ix[0] + k]) + h.u * T(g.nrm[3 * ix[1] + k]) + h.v * T(g.nrm[3 * ix[2] + k]);
+      cb[k] = w0 * T(g.col[4 * ix[0] + k]) + h.u * T(g.col[4 * ix[1] + k]) + h.v * T(g.col[4 * ix[2] + k]);
+    }
+    N = normalize(mkv<T>(nb[0], nb[1], nb[2]));
+    C = mkv<T>(cb[0], cb[1], cb[2]);
+    Ng = normalize(h.ng);
+    if (dot(Ng, N) < T(0)) Ng = neg(Ng);
+    const int64_t baseTri = static_cast<int64_t>(h.id % static_cast<uint32_t>(g.baseT));
+    mat = g.triMat.empty() ? 0u : g.triMat[baseTri];
+    mesh = true;
+  } else {
+    const PrimSide* side = h.kind == UBG_KIND_SPHERE        ? &g.sphSide[h.id]
+                           : h.kind == UBG_KIND_CONE_CURVE ? &g.cSide[h.id]
+                                                           : &g.rSide[h.id];
+    N = normalize(h.ng);
+    Ng = N;
+    C = mkv<T>(T(side->color[0]), T(side->color[1]), T(side->color[2]));
+    mat = static_cast<uint32_t>(side->mat);
+  }
+  const std::vector<PMat>& tab = mesh ? c.meshMats : c.primMats;
+  const PMat m = tab[mat < tab.size() ? mat : 0u];
+  if (m.type != kDielectric && dot(N, rd) > T(0)) {
+    N = neg(N);
+    Ng = neg(Ng);
+  }
+  Surf<T> sf;
+  sf.f = onb(N);
+  sf.C = C;
+  sf.type = m.type;
+  sf.alpha = T(m.alpha);
+  sf.ior = T(m.ior);
+  return sf;
+}
+
+template <class T>
+inline V3<T> offsetOrigin(V3<T> P, V3<T> Ng, V3<T> d, T t) {
+  const T e = std::fmax(std::fmax(std::fabs(P.x), std::fabs(P.y)), std::fmax(std::fabs(P.z), t)) * T(0x1.0p-19f) +
+              T(1.0e-6f);
+  return P + Ng * (dot(d, Ng) >= T(0) ? e : -e);
+}
+
+template <class T>
+inline void cameraRay(const PCtx<T>& c, T fx, T fy, V3<T>& o, V3<T>& d) {
+  const T sx = T(2) * fx / T(static_cast<float>(c.W)) - T(1);
+  const T sy = T(1) - T(2) * fy / T(static_cast<float>(c.H));
+  if (c.ortho) {
+    o = c.pos + c.right * (sx * c.halfW) + c.up * (sy * c.halfH);
+    d = c.dir;
+  } else {
+    o = c.pos;
+    d = normalize(c.dir + c.right * (sx * c.persHalfW) + c.up * (sy * c.persHalfH));
+  }
+}
+
+constexpr uint32_t kDimCam = 0;
+inline uint32_t dimVertex(int b) { return 16u + 64u * static_cast<uint32_t>(b); }
+constexpr uint32_t kDimSpatial = 0x10000u;
+inline uint32_t dimAllLights(int b) { return 0x100000u + 0x2000u * static_cast<uint32_t>(b); }
+
+template <class T>
+struct Pick {
+  V3<T> Q;
+  int li;
+  T wsum, target;
+};
+
+template <class T>
+inline bool traceClosest(const PCtx<T>& c, V3<T> o, V3<T> d, Hit<T>& h) {
+  h = intersect<T>(*c.s, o, d, T(0), std::numeric_limits<T>::infinity());
+  return h.kind >= 0;
+}
+template <class T>
+inline bool traceAny(const PCtx<T>& c, V3<T> o, V3<T> d, T tf) { return occluded<T>(*c.s, o, d, T(0), tf); }
+
+template <class T>
+inline Pick<T> risLights(const PCtx<T>& c, const Surf<T>& sf, V3<T> P, V3<T> wo, uint32_t pix, uint32_t smp,
+                         uint32_t dim) {
+  Pick<T> k{mkv<T>(T(0), T(0), T(0)), -1, T(0), T(0)};
+  for (int ci = 0; ci < c.M; ++ci) {
+    const uint32_t d = dim + 4u * static_cast<uint32_t>(ci);
+    const int li = std::min(c.nLights - 1, static_cast<int>(rndf(c.seed, pix, smp, d) * static_cast<float>(c.nLights)));
+    V3<T> Q;
+    T pdfA;
+    if (!lightSample(c.lights[li], P, rnd<T>(c.seed, pix, smp, d + 1), rnd<T>(c.seed, pix, smp, d + 2), c.nLights, Q,
+                     pdfA))
+      continue;
+    T dist;
+    const T t = lum(lightContrib(c, sf, P, wo, Q, li, dist));
+    const T w = t / pdfA;
+    if (!(w > T(0))) continue;
+    k.wsum += w;
+    if (rnd<T>(c.seed, pix, smp, d + 3) * k.wsum < w) {
+      k.Q = Q;
+      k.li = li;
+      k.target = t;
+    }
+  }
+  return k;
+}
+
+template <class T>
+inline V3<T> directLight(const PCtx<T>& c, const Surf<T>& sf, V3<T> P, V3<T> Ng, V3<T> wo, uint32_t pix, uint32_t smp,
+                         int b, uint64_t& nr) {
+  V3<T> sum{T(0), T(0), T(0)};
+  if (c.nLights <= 0) return sum;
+  if (c.neeAll) {
+    for (int li = 0; li < c.nLights; ++li) {
+      const uint32_t d = dimAllLights(b) + 2u * static_cast<uint32_t>(li);
+      V3<T> Q;
+      T pdfA;
+      if (!lightSample(c.lights[li], P, rnd<T>(c.seed, pix, smp, d), rnd<T>(c.seed, pix, smp, d + 1), 1, Q, pdfA))
+        continue;
+      T dist;
+      const V3<T> ct = lightContrib(c, sf, P, wo, Q, li, dist);
+      if (!(lum(ct) > T(0))) continue;
+      const V3<T> wi = (Q - P) * (T(1) / dist);
+      ++nr;
+      if (traceAny(c, offsetOrigin(P, Ng, wi, T(0)), wi, dist * T(0.9999f))) continue;
+      sum = sum + ct * (T(1) / pdfA);
+    }
+    return sum;
+  }
+  const Pick<T> k = risLights(c, sf, P, wo, pix, smp, dimVertex(b));
+  if (k.li < 0 || !(k.target > T(0))) return sum;
+  T dist;
+  const V3<T> ct = lightContrib(c, sf, P, wo, k.Q, k.li, dist);
+  const V3<T> wi = (k.Q - P) * (T(1) / dist);
+  ++nr;
+  if (traceAny(c, offsetOrigin(P, Ng, wi, T(0)), wi, dist * T(0.9999f))) return sum;
+  const T Wt = k.wsum / (T(static_cast<float>(c.M)) * k.target);
+  return ct * Wt;
+}
+
+template <class T>
+struct GP {  // first-hit record of one pixel and sample
+  bool hit = false;
+  V3<T> P, N, Ng, C, wo;
+  T t;
+  int type;
+  T alpha, ior;
+  // reservoir
+  V3<T> Q;
+  int li = -1;
+  T wsum = T(0), M = T(0), W = T(0), target = T(0);
+};
+template <class T>
+inline Surf<T> surfOf(const GP<T>& g) {
+  Surf<T> sf;
+  sf.f = onb(g.N);
+  sf.C = g.C;
+  sf.type = g.type;
+  sf.alpha = g.alpha;
+  sf.ior = g.ior;
+  return sf;
+}
+
+template <class T>
+static void primaryPixel(const PCtx<T>& c, int pix, uint32_t smp, GP<T>& g, V3<T>& acc, float* first) {
+  const uint32_t upix = static_cast<uint32_t>(pix);
+  const int px = pix % c.W, py = pix / c.W;
+  V3<T> o, d;
+  cameraRay(c, T(static_cast<float>(px)) + rnd<T>(c.seed, upix, smp, kDimCam),
+            T(static_cast<float>(py)) + rnd<T>(c.seed, upix, smp, kDimCam + 1), o, d);
+  Hit<T> h;
+  g = GP<T>();
+  if (!traceClosest(c, o, d, h)) {
+    acc = acc + c.env;
+    if (first && smp == 0)
+      for (int k = 0; k < 8; ++k) first[8 * static_cast<size_t>(pix) + k] = 0.0f;
+    return;
+  }
+  V3<T> Ng;
+  const Surf<T> sf = surfaceAt(c, h, d, Ng);
+  const V3<T> P = o + d * h.t;
+  const V3<T> wo = neg(d);
+  g.hit = true;
+  g.P = P;
+  g.t = h.t;
+  g.N = sf.f.n;
+  g.Ng = Ng;
+  g.C = sf.C;
+  g.wo = wo;
+  g.type = sf.type;
+  g.alpha = sf.alpha;
+  g.ior = sf.ior;
+  if (first && smp == 0) {
+    float* f = first + 8 * static_cast<size_t>(pix);
+    const V3<T> alb = sf.type == kDielectric ? mkv<T>(T(1), T(1), T(1)) : sf.C;
+    f[0] = float(alb.x), f[1] = float(alb.y), f[2] = float(alb.z), f[3] = float(h.t);
+    f[4] = float(sf.f.n.x), f[5] = float(sf.f.n.y), f[6] = float(sf.f.n.z), f[7] = 1.0f;
+  }
+  if (c.nLights > 0 && !c.neeAll) {
+    const Pick<T> k = risLights(c, sf, P, wo, upix, smp, dimVertex(0));
+    g.Q = k.Q;
+    g.li = k.li;
+    g.wsum = k.wsum;
+    g.M = T(static_cast<float>(c.M));
+    g.target = k.target;
+    g.W = (k.li >= 0 && k.target > T(0)) ? k.wsum / (g.M * k.target) : T(0);
+  }
+}
+
+template <class T>
+static V3<T> shadePixel(const PCtx<T>& c, int pix, uint32_t smp, const std::vector<GP<T>>& gb, uint64_t& nr) {
+  const GP<T>& g = gb[pix];
+  const uint32_t upix = static_cast<uint32_t>(pix);
+  const Surf<T> sf0 = surfOf(g);
+  const V3<T> P0 = g.P, Ng0 = g.Ng, wo0 = g.wo;
+  V3<T> L{T(0), T(0), T(0)};
+  if (c.nLights > 0) {
+    if (c.neeAll) {
+      L = directLight(c, sf0, P0, Ng0, wo0, upix, smp, 0, nr);
+    } else {
+      const int px = pix % c.W, py = pix / c.W;
+      int src[17];
+      int nsrc = 0;
+      src[nsrc++] = pix;
+      for (int j = 0; j < c.K; ++j) {
+        const uint32_t d = kDimSpatial + 3u * static_cast<uint32_t>(j);
+        const int qx = std::min(c.W - 1, std::max(0, px + static_cast<int>(std::floor(
+                                                             (2.0f * rndf(c.seed, upix, smp, d) - 1.0f) * c.radius + 0.5f))));
+        const int qy = std::min(c.H - 1, std::max(0, py + static_cast<int>(std::floor(
+                                                             (2.0f * rndf(c.seed, upix, smp, d + 1) - 1.0f) * c.radius + 0.5f))));
+        const int q = qy * c.W + qx;
+        if (q == pix || !gb[q].hit) continue;
+        src[nsrc++] = q;
+      }
+      // balance-heuristic resampling weights m_i = M_i p_i / sum_j M_j p_j
+      V3<T> Q{T(0), T(0), T(0)};
+      int li = -1;
+      T wsum = T(0), tsel = T(0);
+      for (int i = 0; i < nsrc; ++i) {
+        const GP<T>& r = gb[src[i]];
+        if (r.li < 0 || !(r.wsum > T(0))) continue;
+        T dist;
+        const T t = lum(lightContrib(c, sf0, P0, wo0, r.Q, r.li, dist));
+        if (!(t > T(0))) continue;
+        T den = T(static_cast<float>(c.M)) * t;
+        for (int j = 1; j < nsrc; ++j) {
+          const GP<T>& q = gb[src[j]];
+          const T tj = j == i ? r.target : lum(lightContrib(c, surfOf(q), q.P, q.wo, r.Q, r.li, dist));
+          den += q.M * tj;
+        }
+        const T w = t * r.wsum / den;
+        if (!(w > T(0))) continue;
+        wsum += w;
+        if (rnd<T>(c.seed, upix, smp, kDimSpatial + 3u * static_cast<uint32_t>(i) + 2u) * wsum < w) {
+          Q = r.Q;
+          li = r.li;
+          tsel = t;
+        }
+      }
+      if (li >= 0 && tsel > T(0)) {
+        T dist;
+        const V3<T> ct = lightContrib(c, sf0, P0, wo0, Q, li, dist);
+        const V3<T> wi = (Q - P0) * (T(1) / dist);
+        ++nr;
+        if (!traceAny(c, offsetOrigin(P0, Ng0, wi, T(0)), wi, dist * T(0.9999f))) L = ct * (wsum / tsel);
+      }
+    }
+  }
+  V3<T> thr{T(1), T(1), T(1)};
+  Surf<T> sf = sf0;
+  V3<T> P = P0, Ng = Ng0, wo = wo0;
+  T tPrev = g.t;
+  for (int b = 1; b <= c.bounces; ++b) {
+    const uint32_t dv = dimVertex(b - 1) + 56u;
+    V3<T> wiL, wgt;
+    if (!bsdfSample(sf, toLocal(sf.f, wo), rnd<T>(c.seed, upix, smp, dv), rnd<T>(c.seed, upix, smp, dv + 1),
+                    rnd<T>(c.seed, upix, smp, dv + 2), wiL, wgt))
+      break;
+    thr = mul(thr, wgt);
+    if (!(lum(thr) > T(0))) break;
+    const V3<T> wi = toWorld(sf.f, wiL);
+    const V3<T> o = offsetOrigin(P, Ng, wi, tPrev);
+    Hit<T> h;
+    ++nr;
+    if (!traceClosest(c, o, wi, h)) {
+      L = L + mul(thr, c.env);
+      break;
+    }
+    if (b == c.bounces) break;
+    V3<T> ng;
+    sf = surfaceAt(c, h, wi, ng);
+    P = o + wi * h.t;
+    Ng = ng;
+    wo = neg(wi);
+    tPrev = h.t;
+    const V3<T> Ld = directLight(c, sf, P, Ng, wo, upix, smp, b, nr);
+    L = L + mul(thr, Ld);
+  }
+  return L;
+}
+
+template <class T>
+static int pathtraceT(const ubg_scene_desc* desc, const ubg_pt_params* p, float* color, float* first,
+                      uint64_t* rays, const orc_options* opt) {
+  const std::shared_ptr<const OScene> scene = sceneFor(desc, opt->brute_force != 0);
+  PCtx<T> c;
+  c.s = scene.get();
+  c.W = p->width;
+  c.H = p->height;
+  const ubg_camera& cam = p->camera;
+  const T aspect = T(static_cast<float>(c.W)) / T(static_cast<float>(c.H));
+  c.dir = normalize(v3<T>(cam.direction));
+  c.right = normalize(cross(c.dir, v3<T>(cam.up)));
+  c.up = normalize(cross(c.right, c.dir));
+  c.pos = v3<T>(cam.position);
+  c.ortho = cam.orthographic != 0;
+  c.halfH = T(cam.height) * T(0.5f);
+  c.halfW = c.halfH * aspect;
+  c.persHalfH = std::tan(radiansT(T(cam.fovy)) * T(0.5f));
+  c.persHalfW = c.persHalfH * aspect;
+  c.seed = p->seed;
+  c.bounces = p->bounces;
+  c.nLights = p->n_lights;
+  c.M = p->restir_candidates > 0 ? p->restir_candidates : 8;
+  c.K = p->restir_spatial;
+  c.radius = p->restir_radius > 0.0f ? p->restir_radius : 16.0f;
+  c.neeAll = p->nee_all_lights != 0;
+  c.env = v3<T>(p->env_radiance);
+  for (int i = 0; i < p->n_lights; ++i) {
+    const ubg_sphere_light& l = p->lights[i];
+    c.lights.push_back(SLight<T>{v3<T>(l.center), T(l.radius), v3<T>(l.radiance)});
+  }
+  auto mats = [](const ubg_pt_material* m, int n) {
+    std::vector<PMat> v;
+    for (int i = 0; i < n; ++i) v.push_back(PMat{m[i].type, m[i].roughness * m[i].roughness, m[i].ior});
+    if (v.empty()) v.push_back(PMat{kDiffuse, 0.0f, 1.5f});
+    return v;
+  };
+  c.primMats = mats(p->prim_materials, p->n_prim_materials);
+  c.meshMats = mats(p->mesh_materials, p->n_mesh_materials);
+  const int n = c.W * c.H;
+  std::vector<GP<T>> gb(static_cast<size_t>(n));
+  std::vector<V3<T>> acc(static_cast<size_t>(n), V3<T>{T(0), T(0), T(0)});
+  int nth = opt->threads > 0 ? opt->threads : static_cast<int>(std::thread::hardware_concurrency());
+  nth = std::max(1, std::min(nth, c.H));
+  std::atomic<uint64_t> total{0};
+  auto rowsParallel = [&](const std::function<void(int)>& fn) {
+    std::atomic<int> next{0};
+    std::vector<std::thread> pool;
+    for (int t = 0; t < nth; ++t)
+      pool.emplace_back([&]() {
+        for (int y; (y = next.fetch_add(1)) < c.H;) fn(y);
+      });
+    for (auto& t : pool) t.join();
+  };
+  for (int smp = 0; smp < p->spp; ++smp) {
+    const uint32_t us = static_cast<uint32_t>(smp);
+    rowsParallel([&](int y) {
+      for (int x = 0; x < c.W; ++x) {
+        const int pix = y * c.W + x;
+        primaryPixel(c, pix, us, gb[pix], acc[pix], first);
+      }
+      total.fetch_add(static_cast<uint64_t>(c.W));
+    });
+    rowsParallel([&](int y) {
+      uint64_t nr = 0;
+      for (int x = 0; x < c.W; ++x) {
+        const int pix = y * c.W + x;
+        if (!gb[pix].hit) continue;
+        acc[pix] = acc[pix] + shadePixel(c, pix, us, gb, nr);
+      }
+      total.fetch_add(nr);
+    });
+  }
+  const T inv = T(1) / T(static_cast<float>(p->spp));
+  for (int i = 0; i < n; ++i) {
+    color[4 * i] = float(acc[i].x * inv);
+    color[4 * i + 1] = float(acc[i].y * inv);
+    color[4 * i + 2] = float(acc[i].z * inv);
+    color[4 * i + 3] = 1.0f;
+  }
+  if (rays) *rays = total.load();
+  return UBG_OK;
+}
+
+template <class T>
+static Surf<T> surfFromArgs(int type, float roughness, float ior, const float* C) {
+  Surf<T> s;
+  s.f = onb(mkv<T>(T(0), T(0), T(1)));
+  s.C = v3<T>(C);
+  s.type = type;
+  s.alpha = T(roughness * roughness);
+  s.ior = T(ior);
+  return s;
+}
+
+}  // namespace pt
+}  // namespace orc
+
+extern "C" {
+
+int orc_pathtrace(const ubg_scene_desc* desc, const ubg_pt_params* params, float* color, float* first,
+                  uint64_t* rays, const orc_options* opt) {
+  if (!desc || !params || !color) return UBG_ERR_INVALID_ARGUMENT;
+  if (params->width <= 0 || params->height <= 0 || params->spp < 1 || params->bounces < 1)
+    return UBG_ERR_INVALID_ARGUMENT;
+  const orc_options* o = opt ? opt : &kDefaultOpt;
+  return o->precision == 64 ? orc::pt::pathtraceT<double>(desc, params, color, first, rays, o)
+                            : orc::pt::pathtraceT<float>(desc, params, color, first, rays, o);
+}
+
+// bsdfCos / bsdfSample in the local frame (n = +z), double precision, over
+// n direction pairs / uniform triples
+void orc_pt_bsdf_eval(int32_t type, float roughness, float ior, const float* C, int64_t n, const double* wo,
+                      const double* wi, double* out) {
+  using namespace orc::pt;
+  const Surf<double> s = surfFromArgs<double>(type, roughness, ior, C);
+  for (int64_t i = 0; i < n; ++i) {
+    const double* a = wo + 3 * i;
+    const double* b = wi + 3 * i;
+    const orc::V3<double> f = bsdfCos(s, orc::V3<double>{a[0], a[1], a[2]}, orc::V3<double>{b[0], b[1], b[2]});
+    out[3 * i] = f.x, out[3 * i + 1] = f.y, out[3 * i + 2] = f.z;
+  }
+}
+void orc_pt_bsdf_sample(int32_t type, float roughness, float ior, const float* C, int64_t n, const double* wo,
+                        const double* u, double* wi, double* weight, uint8_t* ok) {
+  using namespace orc::pt;
+  const Surf<double> s = surfFromArgs<double>(type, roughness, ior, C);
+  for (int64_t i = 0; i < n; ++i) {
+    const double* a = wo + 3 * i;
+    orc::V3<double> w{0.0, 0.0, 0.0}, g{0.0, 0.0, 0.0};
+    ok[i] = bsdfSample(s, orc::V3<double>{a[0], a[1], a[2]}, u[3 * i], u[3 * i + 1], u[3 * i + 2], w, g) ? 1 : 0;
+    wi[3 * i] = w.x, wi[3 * i + 1] = w.y, wi[3 * i + 2] = w.z;
+    weight[3 * i] = g.x, weight[3 * i + 1] = g.y, weight[3 * i + 2] = g.z;
+  }
+}
+float orc_pt_rnd(uint32_t seed, uint32_t pixel, uint32_t sample, uint32_t dim) {
+  return orc::pt::rndf(seed, pixel, sample, dim);
+}
+
+}  // extern "C"

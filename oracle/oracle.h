@@ -98,6 +98,24 @@ int orc_object_space_edges(const ubg_scene_desc* desc, const ubg_camera* cam, co
                            ubg_edges* out);
 void orc_edges_free(ubg_edges* e);
 
+/* The physically based path tracer of ubg_pathtrace (no reference
+ * counterpart), restated: color = width * height * 4 (the mean over spp),
+ * first = width * height * 8 or NULL (first hit of sample 0: albedo rgb, t,
+ * normal xyz, 1; zeros where the camera ray misses), rays = rays traced. The
+ * denoise_net field is ignored. */
+int orc_pathtrace(const ubg_scene_desc* desc, const ubg_pt_params* params, float* color, float* first,
+                  uint64_t* rays, const orc_options* opt);
+/* Its BSDF in the local frame (n = +z), FP64, over n inputs: f(wo, wi)
+ * |cos wi| of a material (type, roughness, ior, base colour C); and one
+ * sample of wi for each wo from three uniforms (ok = 0 where the path
+ * ends), weight = f |cos| / pdf. Vectors are 3 doubles each. */
+void orc_pt_bsdf_eval(int32_t type, float roughness, float ior, const float* C, int64_t n, const double* wo,
+                      const double* wi, double* out);
+void orc_pt_bsdf_sample(int32_t type, float roughness, float ior, const float* C, int64_t n, const double* wo,
+                        const double* u, double* wi, double* weight, uint8_t* ok);
+/* Its counter-based uniform for (seed, pixel, sample, dimension). */
+float orc_pt_rnd(uint32_t seed, uint32_t pixel, uint32_t sample, uint32_t dim);
+
 /* Post-process helpers (fog.cpp:7-36, image_ops.cpp:9-79). */
 float orc_fog_factor(const ubg_fog* fog, float z);
 void orc_box_downsample(const float* src, int w, int h, int channels, int ss,

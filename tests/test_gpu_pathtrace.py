@@ -1,0 +1,162 @@
+"""ubg_pathtrace on the GPU against its oracle (oracle/oracle.cpp, the FP32
+restatement with the same counter-based random numbers) and the closed forms
+of tests/test_pt_oracle.py:
+
+* parity on the path-traced molecular surface (1,500 atoms, GGX conductor and
+  dielectric materials of roughness 0.05-0.6, 64 spherical lights, ReSTIR
+  with 8 candidates and 4 neighbours, 4 bounces) at 160x90, 2 spp: per-pixel
+  mean |d| <= 1e-3 and rays traced within 0.5 % (a path whose float rounding
+  flips a Fresnel or a resampling choice follows another branch);
+* the denoiser features equal their restatement from the oracle's first-hit
+  record, and the denoised colour is expm1 of the network on those features;
+* white furnace of a smooth dielectric sphere (exactly 1), the Fresnel plane
+  head-on, ReSTIR's mean against the all-lights estimator at 256 spp;
+* argument refusals.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from oracle import binding as ob
+from umbreon_amd import capi
+from umbreon_amd.pathtrace import CONDUCTOR, DIELECTRIC, DIFFUSE, PtMaterial, PtOptions, pathtrace
+from umbreon_amd.scene import Camera, Scene
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def device():
+    from umbreon_amd.render import Device
+    d = Device(0)
+    yield d
+    d.close()
+
+
+def _built(device, sc):
+    from umbreon_amd.render import BuiltScene
+    return BuiltScene(device, sc, capi.BVH8)
+
+
+@pytest.fixture(scope="module")
+def surface():
+    from umbreon_amd.scenes import pathtraced_surface_scene
+    return pathtraced_surface_scene(n_atoms=1500, width=160, height=90, spp=2)
+
+
+def test_parity_with_oracle(device, surface):
+    sc, opt = surface
+    bs = _built(device, sc)
+    try:
+        g = pathtrace(bs, sc, opt, want_features=True)
+    finally:
+        bs.close()
+    c, first, rays = ob.pathtrace(sc, opt, precision=32, threads=16)
+    d = np.abs(g.color[..., :3] - c[..., :3])
+    hit = first[..., 7] > 0
+    print(f"pathtrace parity: mean|d| {d.mean():.3g}, pixels |d| > 1e-3 {(d.max(-1) > 1e-3).mean():.4f}, "
+          f"max {d.max():.3g}, rays gpu {g.rays} oracle {rays}, trace {g.trace_seconds * 1e3:.2f} ms", flush=True)
+    assert hit.mean() > 0.1
+    assert d.mean() <= 1e-3
+    assert (d.max(-1) > 1e-3).mean() <= 0.02
+    assert abs(g.rays / rays - 1) <= 5e-3
+    assert np.all(g.color[..., 3] == 1.0)
+    # features: log(1 + colour), first-hit albedo, normal, t / max t
+    f = g.features
+    np.testing.assert_allclose(f[..., :3], np.log1p(np.maximum(g.color[..., :3], 0)), rtol=1e-6, atol=1e-7)
+    same = np.abs(first[..., 3] - 0) >= 0  # all pixels
+    np.testing.assert_allclose(f[..., 3:6], first[..., 0:3], atol=1e-6)
+    np.testing.assert_allclose(f[..., 6:9], first[..., 4:7], atol=1e-6)
+    depth = first[..., 3] / first[..., 3].max()
+    np.testing.assert_allclose(f[..., 9][same], depth[same], rtol=1e-5, atol=1e-6)
+
+
+def test_denoised_colour_is_the_network_on_the_features(device, surface):
+    from umbreon_amd.denoise import Unet
+    sc, opt = surface
+    bs = _built(device, sc)
+    net = Unet(device, levels=5, channels=(32, 32, 32, 32, 32), seed=3)
+    try:
+        plain = pathtrace(bs, sc, opt, want_features=True)
+        den = pathtrace(bs, sc, opt, denoise_net=net)
+        m = 16
+        H, W = opt.height, opt.width
+        Hp, Wp = -(-H // m) * m, -(-W // m) * m
+        feat = np.pad(plain.features, ((0, Hp - H), (0, Wp - W), (0, 0)), mode="edge")
+        pred = net.infer(feat[None])[0, :H, :W]
+        want = np.maximum(np.expm1(pred), 0.0)
+        np.testing.assert_allclose(den.color[..., :3], want, rtol=1e-5, atol=1e-6)
+        assert den.denoise_seconds > 0
+    finally:
+        net.close()
+        bs.close()
+
+
+def test_white_furnace_dielectric_sphere(device):
+    sc = Scene()
+    sc.add_spheres([[0, 0, 0]], [1.0], [[1, 1, 1, 1]])
+    sc.camera = Camera(position=(0, 0, 5), direction=(0, 0, -1), up=(0, 1, 0), fovy=30.0)
+    opt = PtOptions(width=64, height=64, spp=16, bounces=64, env=(1, 1, 1),
+                    prim_materials=[PtMaterial(DIELECTRIC, 0.0, 1.5)])
+    bs = _built(device, sc)
+    try:
+        g = pathtrace(bs, sc, opt)
+    finally:
+        bs.close()
+    assert np.abs(g.color[..., :3] - 1.0).max() <= 2e-5
+
+
+def test_fresnel_plane(device):
+    sc = Scene()
+    v = np.array([[-6, -6, 0], [6, -6, 0], [6, 6, 0], [-6, 6, 0]], np.float32)
+    sc.set_mesh(v, np.tile([0, 0, 1.0], (4, 1)), np.ones((4, 4)), np.array([0, 1, 2, 0, 2, 3], np.uint32))
+    sc.camera = Camera(position=(0, 0, 10), direction=(0, 0, -1), up=(0, 1, 0), orthographic=True, height=2.0)
+    opt = PtOptions(width=64, height=64, spp=256, bounces=1, env=(1, 1, 1),
+                    mesh_materials=[PtMaterial(DIELECTRIC, 0.0, 1.5)])
+    bs = _built(device, sc)
+    try:
+        g = pathtrace(bs, sc, opt)
+    finally:
+        bs.close()
+    F = 0.04
+    want = F + (1 - F) / 2.25
+    se = np.sqrt(F * (1 - F)) * (1 - 1 / 2.25) / np.sqrt(64 * 64 * 256)
+    assert abs(g.color[..., 0].mean() - want) <= 5 * se
+
+
+def test_restir_mean_matches_all_lights(device):
+    from umbreon_amd.scenes import pathtraced_surface_scene
+    sc, opt = pathtraced_surface_scene(n_atoms=1500, width=192, height=108, spp=256)
+    opt.prim_materials = [PtMaterial(m.type, 0.8, m.ior) for m in opt.prim_materials]
+    opt.mesh_materials = [PtMaterial(DIFFUSE, 0.8, 1.5)]
+    opt.bounces = 1
+    bs = _built(device, sc)
+    try:
+        means = {}
+        for name, K, all_lights in (("restir", 4, False), ("all", 0, True)):
+            opt.restir_spatial, opt.nee_all_lights = K, all_lights
+            means[name] = float(pathtrace(bs, sc, opt).color[..., :3].mean())
+    finally:
+        bs.close()
+    print("ReSTIR vs all lights:", means, flush=True)
+    assert abs(means["restir"] / means["all"] - 1) <= 0.01
+
+
+def test_refusals(device, surface):
+    sc, opt = surface
+    from umbreon_amd.render import BuiltScene
+    bs = _built(device, sc)
+    b2 = BuiltScene(device, sc, capi.BVH2)
+    try:
+        import dataclasses
+        for bad in (dict(restir_candidates=13), dict(restir_spatial=17), dict(bounces=0), dict(spp=0),
+                    dict(lights=np.zeros((4097, 7), np.float32)),
+                    dict(prim_materials=[PtMaterial(7, 0.3, 1.5)])):
+            with pytest.raises(capi.UmbreonError):
+                pathtrace(bs, sc, dataclasses.replace(opt, **bad))
+        with pytest.raises(capi.UmbreonError):
+            pathtrace(b2, sc, opt)
+    finally:
+        b2.close()
+        bs.close()

@@ -1,0 +1,212 @@
+"""Pins of the path tracer's oracle (oracle/oracle.cpp, the restatement of
+ubg_pathtrace), on the CPU. The path tracer has no reference counterpart
+(the reference's integrators are diffuse-only; docs/api/libumbreon.md:33 puts
+refraction out of scope), so every pin is a closed form or an identity:
+
+* white furnace: in a constant environment of radiance 1, a diffuse white,
+  a smooth white mirror and a smooth clear dielectric sphere have radiance
+  exactly 1 at every pixel (the dielectric's radiance scaling 1/eta^2 in and
+  eta^2 out cancels; a dropped scaling gives 0.44 or 2.25); a rough one never
+  exceeds 1;
+* Fresnel: a smooth dielectric plane (eta 1.5) seen head-on under a unit
+  environment returns F + (1 - F) / eta^2 = 0.04 + 0.96 / 2.25;
+* one spherical light over a diffuse plane: L = rho Le (r / d)^2 cos(beta)
+  (the sphere is above the horizon), averaged over each pixel's footprint,
+  by RIS with 1 and 8 candidates, by ReSTIR spatial reuse, and by the
+  all-lights estimator;
+* multiple bounces: inside a closed diffuse sphere (albedo rho) with a small
+  light at its centre, L = rho Le (r / R)^2 (1 - rho^B) / (1 - rho) for B
+  shaded vertices (lights are reached by next-event estimation only);
+* BSDF sampling: the mean sampling weight f |cos| / pdf equals the
+  quadrature of the evaluated f |cos| over the sphere (diffuse, GGX
+  conductor, GGX dielectric from both sides);
+* BSDF reciprocity: f(wo, wi) = f(wi, wo) for reflection and
+  f(wo, wi) / eta_o^2 = f(wi, wo) / eta_i^2 for transmission (Veach 1997, 5.14);
+* ReSTIR's estimate equals the all-lights estimator's on a 64-light
+  molecule scene (unbiasedness of the spatial reuse).
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from oracle import binding as ob
+from umbreon_amd.pathtrace import CONDUCTOR, DIELECTRIC, DIFFUSE, PtMaterial, PtOptions
+from umbreon_amd.scene import Camera, Scene
+
+
+def _sphere_scene(R=1.0, camera=None):
+    sc = Scene()
+    sc.add_spheres([[0, 0, 0]], [R], [[1, 1, 1, 1]])
+    sc.camera = camera or Camera(position=(0, 0, 5), direction=(0, 0, -1), up=(0, 1, 0), fovy=30.0)
+    return sc
+
+
+@pytest.mark.parametrize("mat", [PtMaterial(DIFFUSE), PtMaterial(CONDUCTOR, 0.0), PtMaterial(DIELECTRIC, 0.0, 1.5),
+                                 PtMaterial(DIELECTRIC, 0.0, 1.33)])
+@pytest.mark.parametrize("precision", [32, 64])
+def test_white_furnace(mat, precision):
+    sc = _sphere_scene()
+    opt = PtOptions(width=24, height=24, spp=32, bounces=64, env=(1, 1, 1), prim_materials=[mat])
+    col, first, rays = ob.pathtrace(sc, opt, precision=precision, threads=4)
+    hit = first[..., 7] > 0
+    assert hit.sum() > 200
+    v = col[..., :3][hit]
+    tol = 1e-12 if precision == 64 else 2e-5
+    # a path still inside the dielectric after 64 vertices carries nothing
+    # (a grazing ray trapped by total reflection); none does here
+    assert np.abs(v - 1.0).max() <= tol + 1e-7, (v.min(), v.max())
+    assert np.array_equal(col[..., :3][~hit], np.ones_like(col[..., :3][~hit]))
+
+
+@pytest.mark.parametrize("mat", [PtMaterial(DIELECTRIC, 0.4, 1.5), PtMaterial(CONDUCTOR, 0.5)])
+def test_rough_furnace_never_gains_energy(mat):
+    sc = _sphere_scene()
+    opt = PtOptions(width=16, height=16, spp=16, bounces=32, env=(1, 1, 1), prim_materials=[mat])
+    col, first, _ = ob.pathtrace(sc, opt, precision=64, threads=4)
+    v = col[..., :3][first[..., 7] > 0]
+    assert v.max() <= 1.0 + 1e-7 and v.mean() > 0.85
+
+
+def _plane(size=6.0, rho=0.8):
+    sc = Scene()
+    v = np.array([[-size, -size, 0], [size, -size, 0], [size, size, 0], [-size, size, 0]], np.float32)
+    n = np.tile([0, 0, 1.0], (4, 1)).astype(np.float32)
+    col = np.tile([rho, rho, rho, 1.0], (4, 1)).astype(np.float32)
+    sc.set_mesh(v, n, col, np.array([0, 1, 2, 0, 2, 3], np.uint32))
+    return sc
+
+
+def test_fresnel_plane_head_on():
+    sc = _plane(rho=1.0)
+    sc.camera = Camera(position=(0, 0, 10), direction=(0, 0, -1), up=(0, 1, 0), orthographic=True, height=2.0)
+    opt = PtOptions(width=16, height=16, spp=1024, bounces=1, env=(1, 1, 1),
+                    mesh_materials=[PtMaterial(DIELECTRIC, 0.0, 1.5)])
+    col, _, _ = ob.pathtrace(sc, opt, precision=64, threads=4)
+    F = (0.5 / 2.5) ** 2
+    want = F + (1 - F) / 1.5 ** 2
+    got = col[..., 0].mean()
+    # per sample the radiance is 1 (reflected) or 1/2.25 (transmitted)
+    se = np.sqrt(F * (1 - F)) * (1 - 1 / 2.25) / np.sqrt(16 * 16 * 1024)
+    assert abs(got - want) <= 5 * se, (got, want, se)
+
+
+def _light_truth(W, H, half, c, r, Le, rho, sub=8):
+    xs = (np.arange(W)[:, None] + (np.arange(sub) + 0.5) / sub) / W * 2 - 1
+    ys = 1 - ((np.arange(H)[:, None] + (np.arange(sub) + 0.5) / sub) / H * 2)
+    X = xs.reshape(1, W, 1, sub) * half * W / H
+    Y = ys.reshape(H, 1, sub, 1) * half
+    dx, dy, dz = c[0] - X, c[1] - Y, c[2]
+    d2 = dx * dx + dy * dy + dz * dz
+    return (rho * Le * r * r / d2 * (dz / np.sqrt(d2))).mean(axis=(2, 3))
+
+
+@pytest.mark.parametrize("M,K,all_lights", [(1, 0, False), (8, 0, False), (8, 4, False), (1, 0, True)])
+def test_one_spherical_light_irradiance(M, K, all_lights):
+    W = H = 24
+    sc = _plane(rho=0.8)
+    sc.camera = Camera(position=(0, 0, 10), direction=(0, 0, -1), up=(0, 1, 0), orthographic=True, height=4.0)
+    c, r, Le = np.array([1.0, 0.5, 3.0]), 0.5, 2.0
+    opt = PtOptions(width=W, height=H, spp=256, bounces=1, lights=np.array([[*c, r, Le, Le, Le]]),
+                    restir_candidates=M, restir_spatial=K, restir_radius=4.0, nee_all_lights=all_lights)
+    col, _, _ = ob.pathtrace(sc, opt, precision=64, threads=4)
+    truth = _light_truth(W, H, 2.0, c, r, Le, 0.8)
+    rel = col[..., 0] / truth - 1.0
+    assert abs(rel.mean()) <= 2e-3, rel.mean()
+    assert np.sqrt((rel ** 2).mean()) <= 2e-2
+    # a dropped cos(beta) or (r/d)^2 would be off by >= 10 % across the image
+    assert np.abs(rel).max() <= 0.08
+
+
+@pytest.mark.parametrize("bounces", [1, 2, 3, 5])
+def test_closed_diffuse_sphere_bounces(bounces):
+    R, r, Le, rho = 10.0, 1.0, 4.0, 0.6
+    sc = Scene()
+    sc.add_spheres([[0, 0, 0]], [R], [[rho, rho, rho, 1]])
+    sc.camera = Camera(position=(0, 0, 5), direction=(0, 0.3, -1), up=(0, 1, 0), fovy=40.0)
+    opt = PtOptions(width=16, height=16, spp=256, bounces=bounces, lights=np.array([[0, 0, 0, r, Le, Le, Le]]),
+                    restir_candidates=1, restir_spatial=0)
+    col, _, _ = ob.pathtrace(sc, opt, precision=64, threads=4)
+    want = rho * Le * (r / R) ** 2 * (1 - rho ** bounces) / (1 - rho)
+    got = col[..., 0].mean()
+    assert abs(got / want - 1) <= 5e-3, (got, want)
+
+
+def _sphere_dirs(n):
+    ct = -1 + (np.arange(2 * n) + 0.5) / n
+    ph = (np.arange(2 * n) + 0.5) / (2 * n) * 2 * np.pi
+    CT, PH = np.meshgrid(ct, ph, indexing="ij")
+    st = np.sqrt(1 - CT ** 2)
+    return np.stack([st * np.cos(PH), st * np.sin(PH), CT], -1).reshape(-1, 3), (1.0 / n) * (np.pi / n)
+
+
+@pytest.mark.parametrize("mtype,rough,ior", [(DIFFUSE, 0.5, 1.5), (CONDUCTOR, 0.3, 1.5), (CONDUCTOR, 0.6, 1.5),
+                                             (CONDUCTOR, 1.0, 1.5), (DIELECTRIC, 0.3, 1.5), (DIELECTRIC, 0.6, 1.5),
+                                             (DIELECTRIC, 1.0, 1.33)])
+def test_bsdf_sampling_matches_quadrature(mtype, rough, ior):
+    rng = np.random.default_rng(3)
+    base = np.array([0.9, 0.6, 0.3])
+    dirs, dw = _sphere_dirs(700)
+    sides = (1, -1) if mtype == DIELECTRIC else (1,)
+    for th in (0.2, 0.8, 1.3):
+        for side in sides:
+            wo = np.array([np.sin(th), 0.0, side * np.cos(th)])
+            quad = ob.pt_bsdf_eval(mtype, rough, ior, base, wo, dirs).sum(0) * dw
+            N = 200_000
+            _, w, ok = ob.pt_bsdf_sample(mtype, rough, ior, base, wo, rng.random((N, 3)))
+            mc = w.sum(0) / N
+            se = w.std(0) / np.sqrt(N)
+            assert np.all(np.abs(mc - quad) <= 5 * se + 4e-3 * quad), (th, side, mc, quad, se)
+
+
+@pytest.mark.parametrize("mtype,rough,ior", [(DIFFUSE, 0.5, 1.5), (CONDUCTOR, 0.4, 1.5), (DIELECTRIC, 0.4, 1.5),
+                                             (DIELECTRIC, 0.7, 1.33)])
+def test_bsdf_reciprocity(mtype, rough, ior):
+    rng = np.random.default_rng(0)
+    wo = rng.normal(size=(4000, 3))
+    wi = rng.normal(size=(4000, 3))
+    wo /= np.linalg.norm(wo, axis=1, keepdims=True)
+    wi /= np.linalg.norm(wi, axis=1, keepdims=True)
+    base = [0.9, 0.7, 0.5]
+    f1 = ob.pt_bsdf_eval(mtype, rough, ior, base, wo, wi)[:, 0] / np.abs(wi[:, 2])
+    f2 = ob.pt_bsdf_eval(mtype, rough, ior, base, wi, wo)[:, 0] / np.abs(wo[:, 2])
+    refl = wo[:, 2] * wi[:, 2] > 0
+    some = (f1 > 0) | (f2 > 0)
+    assert (refl & some).sum() > 400
+    r = refl & some
+    np.testing.assert_allclose(f1[r], f2[r], rtol=1e-9, atol=0)
+    t = ~refl & some
+    if mtype == DIELECTRIC:
+        assert t.sum() > 100
+        eta = float(np.float32(ior))  # the material table holds float
+        eta_o = np.where(wo[:, 2] > 0, 1.0, eta)
+        eta_i = np.where(wi[:, 2] > 0, 1.0, eta)
+        # (1 - F) near total reflection loses digits: a looser relative bound
+        np.testing.assert_allclose((f1 / eta_o ** 2)[t], (f2 / eta_i ** 2)[t], rtol=1e-6, atol=0)
+    else:
+        assert t.sum() == 0
+
+
+def test_restir_matches_all_lights_estimator():
+    from umbreon_amd.scenes import pathtraced_surface_scene
+    sc, opt = pathtraced_surface_scene(n_atoms=1500, width=64, height=36, spp=192)
+    # rough materials keep the per-sample variance of every estimator bounded
+    opt.prim_materials = [PtMaterial(m.type, 0.8, m.ior) for m in opt.prim_materials]
+    opt.mesh_materials = [PtMaterial(DIFFUSE, 0.8, 1.5)]
+    opt.bounces = 1
+    means = {}
+    for name, M, K, all_lights in (("ris", 8, 0, False), ("restir", 8, 4, False), ("all", 8, 0, True)):
+        opt.restir_candidates, opt.restir_spatial, opt.nee_all_lights = M, K, all_lights
+        col, first, _ = ob.pathtrace(sc, opt, precision=32, threads=8)
+        means[name] = col[..., :3][first[..., 7] > 0].mean()
+    assert abs(means["restir"] / means["all"] - 1) <= 0.02, means
+    assert abs(means["ris"] / means["all"] - 1) <= 0.02, means
+
+
+def test_counter_rng_uniform_and_keyed():
+    u = np.array([ob.pt_rnd(7, p, 3, 16) for p in range(20000)])
+    assert 0.0 <= u.min() and u.max() < 1.0
+    assert abs(u.mean() - 0.5) < 0.01 and abs(u.var() - 1 / 12) < 0.003
+    # every key component changes the stream
+    base = ob.pt_rnd(1, 2, 3, 4)
+    assert len({base, ob.pt_rnd(9, 2, 3, 4), ob.pt_rnd(1, 9, 3, 4), ob.pt_rnd(1, 2, 9, 4), ob.pt_rnd(1, 2, 3, 9)}) == 5

@@ -47,8 +47,8 @@ HIP_FLAGS = [
 ]
 HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall"]
 
-GPU_SOURCES = ["ub_kernels.hip", "ub_devbuild.hip", "ub_unet.hip", "ub_static.hip", "ub_capi.cpp", "ub_build.cpp",
-               "ub_edges.cpp"]
+GPU_SOURCES = ["ub_kernels.hip", "ub_devbuild.hip", "ub_unet.hip", "ub_static.hip", "ub_pathtrace.hip", "ub_capi.cpp",
+               "ub_build.cpp", "ub_edges.cpp"]
 
 
 def _run(cmd: list[str], cwd: Path | None = None) -> None:

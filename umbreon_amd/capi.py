@@ -220,6 +220,39 @@ class BatchOptions(C.Structure):
     _fields_ = [("ao_waves_per_simd", C.c_int32), ("no_graph", C.c_int32), ("reserved", C.c_int32 * 6)]
 
 
+class SphereLight(C.Structure):
+    """ubg_sphere_light: a spherical area light of uniform radiance."""
+    _fields_ = [("center", C.c_float * 3), ("radius", C.c_float), ("radiance", C.c_float * 3),
+                ("reserved", C.c_float)]
+
+
+PT_DIFFUSE, PT_CONDUCTOR, PT_DIELECTRIC = 0, 1, 2
+
+
+class PtMaterial(C.Structure):
+    """ubg_pt_material: the path tracer's surface model of one material index."""
+    _fields_ = [("type", C.c_int32), ("roughness", C.c_float), ("ior", C.c_float), ("reserved", C.c_int32)]
+
+
+class PtParams(C.Structure):
+    """ubg_pt_params: one path-traced image."""
+    _fields_ = [
+        ("width", C.c_int32), ("height", C.c_int32), ("camera", Camera), ("spp", C.c_int32),
+        ("bounces", C.c_int32), ("seed", C.c_uint32), ("lights", C.POINTER(SphereLight)), ("n_lights", C.c_int32),
+        ("env_radiance", C.c_float * 3), ("prim_materials", C.POINTER(PtMaterial)), ("n_prim_materials", C.c_int32),
+        ("mesh_materials", C.POINTER(PtMaterial)), ("n_mesh_materials", C.c_int32),
+        ("restir_candidates", C.c_int32), ("restir_spatial", C.c_int32), ("restir_radius", C.c_float),
+        ("nee_all_lights", C.c_int32), ("denoise_net", C.c_void_p), ("reserved", C.c_int32 * 8),
+    ]
+
+
+class PtFrame(C.Structure):
+    """ubg_pt_frame: outputs and timings of ubg_pathtrace."""
+    _fields_ = [("color", C.c_void_p), ("features", C.c_void_p), ("memory", C.c_int32),
+                ("trace_seconds", C.c_double), ("denoise_seconds", C.c_double), ("rays", C.c_uint64),
+                ("reserved", C.c_int32 * 4)]
+
+
 # Exported symbols of libumbreon_gpu.so (the C-ABI). Kept in sync with the
 # header; tests check both directions.
 EXPORTS = [
@@ -227,7 +260,7 @@ EXPORTS = [
     "ubg_last_error", "ubg_scene_create", "ubg_scene_create_ex", "ubg_scene_create_opts", "ubg_scene_destroy", "ubg_scene_get_info",
     "ubg_render", "ubg_render_noalloc", "ubg_render_batch", "ubg_intersect", "ubg_occluded", "ubg_occluded_segments", "ubg_object_space_edges",
     "ubg_edges_free", "ubg_unet_create", "ubg_unet_destroy", "ubg_unet_param_count", "ubg_unet_get_params",
-    "ubg_unet_set_params", "ubg_unet_infer", "ubg_unet_train_step",
+    "ubg_unet_set_params", "ubg_unet_infer", "ubg_unet_train_step", "ubg_pathtrace",
 ]
 
 _lib = None
@@ -309,6 +342,8 @@ def load() -> C.CDLL:
     lib.ubg_unet_train_step.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p,
                                         C.c_float, C.POINTER(C.c_float), C.c_void_p, C.c_int32]
     lib.ubg_unet_train_step.restype = C.c_int
+    lib.ubg_pathtrace.argtypes = [C.c_void_p, C.c_void_p, C.POINTER(PtParams), C.POINTER(PtFrame)]
+    lib.ubg_pathtrace.restype = C.c_int
     if lib.ubg_abi_version() != ABI_VERSION:
         raise RuntimeError("libumbreon_gpu.so ABI version mismatch")
     _lib = lib

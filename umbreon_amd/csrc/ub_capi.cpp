@@ -27,6 +27,7 @@
 #include "ub_devbuild.h"
 #include "ub_edges.h"
 #include "ub_kernels.h"
+#include "ub_pathtrace.h"
 #include "ub_unet.h"
 #include "ub_layout.h"
 #include "ub_testing.h"
@@ -83,6 +84,9 @@ struct ubg_context {
   // batched frames: the shared queue / frame-buffer pool, the request table,
   // launch tables and read-backs (device), their pinned host staging
   DevBuf batchPool, batchItems, batchLaunch, batchOut;
+  // path tracer: accumulator, first-hit records, G-buffer, reservoirs,
+  // lights + materials, padded features, prediction, colour staging, words
+  DevBuf ptAcc, ptFirst, ptG, ptRes, ptTab, ptFeat, ptPred, ptColor, ptWords;
   void* batchHost = nullptr;  // pinned: items + launch tables + read-backs
   size_t batchHostBytes = 0;
   // instantiated graphs of a batched layer (reset, primary, AO, resolve,
@@ -102,7 +106,7 @@ struct ubg_context {
             &aaPix, &giPos, &giNrm, &giRefl, &giE, &giList, &giInd, &giOcc, &dnA, &dnB, &dnGrad,
             &dnVar, &giLow, &giPatch, &giSnap, &edgePts, &edgeSolids, &edgeOut, &unetIn,
             &unetTgt, &unetOut, &giFeat, &giFeatP, &giPred, &giMax, &batchPool, &batchItems, &batchLaunch,
-            &batchOut};
+            &batchOut, &ptAcc, &ptFirst, &ptG, &ptRes, &ptTab, &ptFeat, &ptPred, &ptColor, &ptWords};
   }
 };
 
@@ -1505,6 +1509,157 @@ int ubg_render_batch(ubg_context* ctx, int32_t n, const ubg_scene* const* scenes
     if (fr->cancelled) status = UBG_ERR_CANCELLED;
   }
   if (status != UBG_OK) return fail(status, "ubg_render_batch: cancelled");
+  return UBG_OK;
+}
+
+// ---- physically based path tracing --------------------------------------------
+int ubg_pathtrace(ubg_context* ctx, const ubg_scene* scene, const ubg_pt_params* p, ubg_pt_frame* fr) {
+  if (!ctx || !scene || !p || !fr || !fr->color) return fail(UBG_ERR_INVALID_ARGUMENT, "ubg_pathtrace: null argument");
+  if (scene->ctx != ctx) return fail(UBG_ERR_INVALID_ARGUMENT, "ubg_pathtrace: scene belongs to another context");
+  if (scene->view.layout != ubg::kLayoutBvh8)
+    return fail(UBG_ERR_UNSUPPORTED, "ubg_pathtrace: needs a BVH8 scene");
+  if (p->width <= 0 || p->height <= 0 || static_cast<int64_t>(p->width) * p->height >= (int64_t(1) << 28))
+    return fail(UBG_ERR_INVALID_ARGUMENT, "ubg_pathtrace: bad image size");
+  if (p->spp < 1 || p->spp > 65536 || p->bounces < 1 || p->bounces > 64)
+    return fail(UBG_ERR_INVALID_ARGUMENT, "ubg_pathtrace: spp must be 1..65536 and bounces 1..64");
+  if (p->n_lights < 0 || p->n_lights > 4096 || (p->n_lights > 0 && !p->lights))
+    return fail(UBG_ERR_INVALID_ARGUMENT, "ubg_pathtrace: bad light array (0..4096 lights)");
+  const int M = p->restir_candidates > 0 ? p->restir_candidates : 8;
+  if (M > ubg::pt::kMaxCandidates || p->restir_spatial < 0 || p->restir_spatial > ubg::pt::kMaxSpatial)
+    return fail(UBG_ERR_INVALID_ARGUMENT, "ubg_pathtrace: restir_candidates 1..12, restir_spatial 0..16");
+  if ((p->n_prim_materials > 0 && !p->prim_materials) || (p->n_mesh_materials > 0 && !p->mesh_materials) ||
+      p->n_prim_materials < 0 || p->n_mesh_materials < 0)
+    return fail(UBG_ERR_INVALID_ARGUMENT, "ubg_pathtrace: bad material arrays");
+  for (int k = 0; k < p->n_prim_materials + p->n_mesh_materials; ++k) {
+    const ubg_pt_material& m = k < p->n_prim_materials ? p->prim_materials[k] : p->mesh_materials[k - p->n_prim_materials];
+    if (m.type < 0 || m.type > 2 || !(m.roughness >= 0.0f) || (m.type == UBG_PT_DIELECTRIC && !(m.ior > 0.0f)))
+      return fail(UBG_ERR_INVALID_ARGUMENT, "ubg_pathtrace: bad material");
+  }
+  ubg::Unet* net = nullptr;
+  if (p->denoise_net) {
+    if (p->denoise_net->ctx != ctx) return fail(UBG_ERR_INVALID_ARGUMENT, "ubg_pathtrace: network on another context");
+    net = &p->denoise_net->net;
+  }
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  UBG_HIP(hipSetDevice(ctx->device), "hipSetDevice");
+  hipStream_t st = ctx->stream;
+  const int W = p->width, H = p->height;
+  const size_t n = static_cast<size_t>(W) * H;
+  int Wp = W, Hp = H;
+  if (net) {
+    const int m = 1 << (net->levels() - 1);
+    Wp = (W + m - 1) / m * m;
+    Hp = (H + m - 1) / m * m;
+  }
+  const size_t np = static_cast<size_t>(Wp) * Hp;
+  const bool wantFeat = net || fr->features;
+  const size_t tabBytes = sizeof(ubg::pt::Light) * std::max(1, p->n_lights) +
+                          sizeof(ubg::pt::Mat) * (std::max(1, p->n_prim_materials) + std::max(1, p->n_mesh_materials));
+  int r = ensure(ctx->ptAcc, n * 16);
+  if (r == UBG_OK) r = ensure(ctx->ptFirst, n * 32);
+  if (r == UBG_OK) r = ensure(ctx->ptG, n * sizeof(ubg::pt::GPix));
+  if (r == UBG_OK) r = ensure(ctx->ptRes, n * sizeof(ubg::pt::Reservoir));
+  if (r == UBG_OK) r = ensure(ctx->ptTab, tabBytes);
+  if (r == UBG_OK) r = ensure(ctx->ptWords, 256);
+  if (r == UBG_OK && wantFeat) r = ensure(ctx->ptFeat, np * ubg::pt::kPtFeatures * 4);
+  if (r == UBG_OK && net) r = ensure(ctx->ptPred, np * 3 * 4);
+  if (r == UBG_OK && fr->memory == 0) r = ensure(ctx->ptColor, n * 16);
+  if (r != UBG_OK) return r;
+  // lights and materials
+  std::vector<ubg::pt::Light> lights(static_cast<size_t>(std::max(1, p->n_lights)));
+  for (int i = 0; i < p->n_lights; ++i) {
+    const ubg_sphere_light& l = p->lights[i];
+    if (!(l.radius > 0.0f)) return fail(UBG_ERR_INVALID_ARGUMENT, "ubg_pathtrace: light radius must be > 0");
+    lights[i] = ubg::pt::Light{{l.center[0], l.center[1], l.center[2]}, l.radius,
+                               {l.radiance[0], l.radiance[1], l.radiance[2]}, 0.0f};
+  }
+  auto mats = [](const ubg_pt_material* src, int cnt) {
+    std::vector<ubg::pt::Mat> out(static_cast<size_t>(std::max(1, cnt)), ubg::pt::Mat{ubg::pt::kMatDiffuse, 0.0f, 1.5f, 0});
+    for (int i = 0; i < cnt; ++i) out[i] = ubg::pt::Mat{src[i].type, src[i].roughness * src[i].roughness, src[i].ior, 0};
+    return out;
+  };
+  const std::vector<ubg::pt::Mat> pm = mats(p->prim_materials, p->n_prim_materials);
+  const std::vector<ubg::pt::Mat> mm = mats(p->mesh_materials, p->n_mesh_materials);
+  char* tab = static_cast<char*>(ctx->ptTab.p);
+  const size_t lb = sizeof(ubg::pt::Light) * lights.size(), pb = sizeof(ubg::pt::Mat) * pm.size();
+  UBG_HIP(hipMemcpyAsync(tab, lights.data(), lb, hipMemcpyHostToDevice, st), "hipMemcpy(lights)");
+  UBG_HIP(hipMemcpyAsync(tab + lb, pm.data(), pb, hipMemcpyHostToDevice, st), "hipMemcpy(materials)");
+  UBG_HIP(hipMemcpyAsync(tab + lb + pb, mm.data(), sizeof(ubg::pt::Mat) * mm.size(), hipMemcpyHostToDevice, st),
+          "hipMemcpy(materials)");
+  // camera basis (embree_renderer.cpp:375-386, on the image grid)
+  ubg::pt::Params q;
+  std::memset(&q, 0, sizeof(q));
+  {
+    const ubg_camera& cam = p->camera;
+    const float aspect = static_cast<float>(W) / static_cast<float>(H);
+    const V3h dir = normalize(V3h{cam.direction[0], cam.direction[1], cam.direction[2]});
+    const V3h right = normalize(cross(dir, V3h{cam.up[0], cam.up[1], cam.up[2]}));
+    const V3h up = normalize(cross(right, dir));
+    q.W = W;
+    q.H = H;
+    q.ortho = cam.orthographic ? 1 : 0;
+    std::memcpy(q.pos, cam.position, sizeof(q.pos));
+    q.dir[0] = dir.x, q.dir[1] = dir.y, q.dir[2] = dir.z;
+    q.right[0] = right.x, q.right[1] = right.y, q.right[2] = right.z;
+    q.up[0] = up.x, q.up[1] = up.y, q.up[2] = up.z;
+    q.halfH = cam.height * 0.5f;
+    q.halfW = q.halfH * aspect;
+    q.persHalfH = std::tan(cam.fovy * 0.01745329252f * 0.5f);
+    q.persHalfW = q.persHalfH * aspect;
+  }
+  q.bounces = p->bounces;
+  q.seed = p->seed;
+  q.nLights = p->n_lights;
+  std::memcpy(q.env, p->env_radiance, sizeof(q.env));
+  q.restirM = M;
+  q.restirK = p->restir_spatial;
+  q.restirRadius = p->restir_radius > 0.0f ? p->restir_radius : 16.0f;
+  q.neeAll = p->nee_all_lights ? 1 : 0;
+  q.lights = reinterpret_cast<const ubg::pt::Light*>(tab);
+  q.primMats = reinterpret_cast<const ubg::pt::Mat*>(tab + lb);
+  q.meshMats = reinterpret_cast<const ubg::pt::Mat*>(tab + lb + pb);
+  q.nPrimMats = static_cast<int32_t>(pm.size());
+  q.nMeshMats = static_cast<int32_t>(mm.size());
+  q.rays = reinterpret_cast<unsigned long long*>(ctx->ptWords.p) + 1;
+  auto* maxBits = static_cast<unsigned int*>(ctx->ptWords.p);
+  auto* acc = static_cast<float4*>(ctx->ptAcc.p);
+  auto* first = static_cast<float4*>(ctx->ptFirst.p);
+  ubg::ptZero(acc, static_cast<int>(n), maxBits, q.rays, st);
+  UBG_HIP(hipEventRecord(ctx->ev[0], st), "hipEventRecord");
+  for (int smp = 0; smp < p->spp; ++smp) {
+    q.sample = smp;
+    ubg::ptSample(q, scene->view, static_cast<ubg::pt::GPix*>(ctx->ptG.p), static_cast<ubg::pt::Reservoir*>(ctx->ptRes.p),
+                  acc, first, st);
+  }
+  UBG_HIP(hipGetLastError(), "kernel launch");
+  UBG_HIP(hipEventRecord(ctx->ev[1], st), "hipEventRecord");
+  float4* color = fr->memory ? reinterpret_cast<float4*>(fr->color) : static_cast<float4*>(ctx->ptColor.p);
+  float* feat = wantFeat ? static_cast<float*>(ctx->ptFeat.p) : nullptr;
+  ubg::ptResolve(W, H, Wp, Hp, p->spp, acc, first, maxBits, color, feat, st);
+  if (net) {
+    UBG_HIP(net->forward(1, Hp, Wp, feat, static_cast<float*>(ctx->ptPred.p), st), "unet forward");
+    ubg::ptApplyNet(W, H, Wp, static_cast<const float*>(ctx->ptPred.p), color, st);
+  }
+  UBG_HIP(hipGetLastError(), "kernel launch");
+  UBG_HIP(hipEventRecord(ctx->ev[2], st), "hipEventRecord");
+  if (fr->memory == 0) r = copyOut(fr->color, color, n * 16, 0, st);
+  if (r == UBG_OK && fr->features) {
+    // the unpadded rows of the (possibly padded) feature image
+    UBG_HIP(hipMemcpy2DAsync(fr->features, static_cast<size_t>(W) * 40, feat, static_cast<size_t>(Wp) * 40,
+                             static_cast<size_t>(W) * 40, static_cast<size_t>(H),
+                             fr->memory ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, st),
+            "hipMemcpy(features)");
+  }
+  if (r != UBG_OK) return r;
+  unsigned long long rays = 0;
+  UBG_HIP(hipMemcpyAsync(&rays, q.rays, sizeof(rays), hipMemcpyDeviceToHost, st), "hipMemcpy(rays)");
+  UBG_HIP(hipStreamSynchronize(st), "pathtrace sync");
+  float ms0 = 0.0f, ms1 = 0.0f;
+  UBG_HIP(hipEventElapsedTime(&ms0, ctx->ev[0], ctx->ev[1]), "hipEventElapsedTime");
+  UBG_HIP(hipEventElapsedTime(&ms1, ctx->ev[1], ctx->ev[2]), "hipEventElapsedTime");
+  fr->trace_seconds = ms0 * 1e-3;
+  fr->denoise_seconds = ms1 * 1e-3;
+  fr->rays = rays;
   return UBG_OK;
 }
 

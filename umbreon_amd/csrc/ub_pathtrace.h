@@ -1,0 +1,77 @@
+// ub_pathtrace.h -- host-side interface of the physically based path tracer
+// (ub_pathtrace.hip) behind ubg_pathtrace.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "ub_layout.h"
+
+namespace ubg {
+namespace pt {
+
+constexpr int kMatDiffuse = 0;
+constexpr int kMatConductor = 1;
+constexpr int kMatDielectric = 2;
+constexpr int kMaxSpatial = 16;   // ReSTIR neighbours
+constexpr int kMaxCandidates = 12;
+constexpr int kPtFeatures = 10;
+
+struct Light {      // a sphere of constant radiance
+  float c[3];
+  float r;
+  float Le[3];
+  float pad;
+};
+struct Mat {        // per material index of the scene
+  int32_t type;
+  float alpha;      // GGX alpha = roughness^2
+  float ior;
+  int32_t pad;
+};
+// One sample pass over the image.
+struct Params {
+  int32_t W, H;
+  int32_t ortho;
+  float pos[3], dir[3], right[3], up[3];
+  float halfW, halfH, persHalfW, persHalfH;
+  int32_t sample;        // index of this pass's sample
+  int32_t bounces;       // shaded path vertices (BSDF scatterings)
+  uint32_t seed;
+  int32_t nLights;
+  float env[3];
+  int32_t restirM;       // light candidates per vertex
+  int32_t restirK;       // spatial neighbours at the first vertex
+  float restirRadius;    // pixels
+  int32_t neeAll;        // 1: one cone sample of every light instead of RIS / ReSTIR
+  const Light* lights;
+  const Mat* primMats;
+  const Mat* meshMats;
+  int32_t nPrimMats, nMeshMats;
+  unsigned long long* rays;  // traced-ray counter
+};
+// First-hit record: position + ray distance (w = 0: no surface), shading
+// normal + material type, geometric normal + GGX alpha, base colour + IOR,
+// direction to the eye.
+struct GPix {
+  float4 P, N, Ng, C, wo;
+};
+// The first vertex's light reservoir: selected light point + light index,
+// weight sum, candidate count, unbiased contribution weight, target.
+struct Reservoir {
+  float4 Q;
+  float wsum, M, W, target;
+};
+
+}  // namespace pt
+
+void ptZero(float4* acc, int n, unsigned int* maxBits, unsigned long long* rays, hipStream_t st);
+void ptSample(const pt::Params& p, const SceneView& s, pt::GPix* gb, pt::Reservoir* res, float4* acc, float4* first,
+              hipStream_t st);
+// colour = acc / spp (W x H); with feat: the Wp x Hp edge-padded denoiser
+// features as well.
+void ptResolve(int W, int H, int Wp, int Hp, int spp, const float4* acc, const float4* first, unsigned int* maxBits,
+               float4* color, float* feat, hipStream_t st);
+void ptApplyNet(int W, int H, int Wp, const float* pred, float4* color, hipStream_t st);
+
+}  // namespace ubg

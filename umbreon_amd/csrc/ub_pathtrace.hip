@@ -1,0 +1,723 @@
+// ub_pathtrace.hip -- the physically based path tracer of ubg_pathtrace:
+// GGX conductors and dielectrics (Walter et al. 2007 microfacet BSDF with
+// visible-normal sampling, Heitz 2018), Lambertian surfaces, spherical area
+// lights sampled by resampled importance sampling with ReSTIR spatial reuse
+// at the first vertex (Bitterli et al. 2020; unbiased, with balance-heuristic
+// resampling weights over the combined reservoirs),
+// a constant environment, a fixed number of path vertices.
+//
+// No reference counterpart: the reference's integrators are diffuse-only
+// (pt1 / pt2); docs/api/libumbreon.md:33 puts refraction out of its scope.
+//
+// One sample of every pixel per pass: k_pt_primary traces the camera rays,
+// stores the first-hit G-buffer and draws M light candidates per pixel into
+// a reservoir; k_pt_shade combines each pixel's reservoir with K neighbours'
+// (each candidate's target re-evaluated at every contributing pixel for its
+// resampling weight), traces the shadow ray, then continues the path through the
+// remaining vertices with per-vertex RIS next-event estimation; the
+// radiance adds into a per-pixel accumulator in sample order. Lights are not
+// intersectable: they are reached by next-event estimation only, and rays
+// that leave the scene return the environment radiance.
+//
+// Random numbers are a pure function of (seed, pixel, sample, dimension)
+// (hash32 below), so an image does not depend on launch shapes.
+
+#include <hip/hip_runtime.h>
+
+#include "ub_device.h"
+#include "ub_pathtrace.h"
+
+namespace ubg {
+namespace pt {
+
+extern __shared__ uint32_t ptLds[];
+
+constexpr float kPi = 3.14159265358979f;
+constexpr int kBlock = 128;
+
+// ---------------------------------------------------------------- random
+__host__ __device__ inline uint32_t hash32(uint32_t x) {
+  x ^= x >> 16;
+  x *= 0x7FEB352Du;
+  x ^= x >> 15;
+  x *= 0x846CA68Bu;
+  x ^= x >> 16;
+  return x;
+}
+__device__ __forceinline__ float rnd(uint32_t seed, uint32_t pixel, uint32_t sample, uint32_t dim) {
+  const uint32_t h = hash32(seed ^ hash32(pixel ^ hash32((sample * 0x9E3779B9u) ^ hash32(dim + 0x632BE5ABu))));
+  return static_cast<float>(h >> 8) * 0x1.0p-24f;
+}
+
+// ---------------------------------------------------------------- frames
+struct Onb {
+  V3 t, b, n;
+};
+// Orthonormal basis around a unit n (Duff et al. 2017, "Building an
+// Orthonormal Basis, Revisited").
+__device__ __forceinline__ Onb onb(V3 n) {
+  const float sgn = n.z >= 0.0f ? 1.0f : -1.0f;
+  const float a = -1.0f / (sgn + n.z);
+  const float b = n.x * n.y * a;
+  return Onb{mk(1.0f + sgn * n.x * n.x * a, sgn * b, -sgn * n.x), mk(b, sgn + n.y * n.y * a, -n.y), n};
+}
+__device__ __forceinline__ V3 toLocal(const Onb& f, V3 v) { return mk(dot(v, f.t), dot(v, f.b), dot(v, f.n)); }
+__device__ __forceinline__ V3 toWorld(const Onb& f, V3 v) {
+  return mk(f.t.x * v.x + f.b.x * v.y + f.n.x * v.z, f.t.y * v.x + f.b.y * v.y + f.n.y * v.z,
+            f.t.z * v.x + f.b.z * v.y + f.n.z * v.z);
+}
+
+// ---------------------------------------------------------------- GGX (local frame, n = +z)
+__device__ __forceinline__ float ggxD(V3 m, float a) {
+  const float c2 = m.z * m.z;
+  if (c2 <= 0.0f) return 0.0f;
+  const float a2 = a * a;
+  const float k = c2 * (a2 - 1.0f) + 1.0f;
+  return a2 / (kPi * k * k);
+}
+__device__ __forceinline__ float ggxLambda(V3 w, float a) {
+  const float c2 = w.z * w.z;
+  if (c2 <= 0.0f) return 0.0f;
+  const float t2 = fmaxf(0.0f, 1.0f - c2) / c2;
+  return 0.5f * (sqrtf(1.0f + a * a * t2) - 1.0f);
+}
+__device__ __forceinline__ float ggxG1(V3 w, float a) { return 1.0f / (1.0f + ggxLambda(w, a)); }
+__device__ __forceinline__ float ggxG2(V3 wo, V3 wi, float a) { return 1.0f / (1.0f + ggxLambda(wo, a) + ggxLambda(wi, a)); }
+// Visible normal of wo (wo.z > 0) from two uniforms (Heitz 2018).
+__device__ __forceinline__ V3 ggxSampleVisible(V3 wo, float a, float u1, float u2) {
+  const V3 vh = normalize(mk(a * wo.x, a * wo.y, wo.z));
+  const float l2 = vh.x * vh.x + vh.y * vh.y;
+  const V3 T1 = l2 > 0.0f ? mk(-vh.y, vh.x, 0.0f) * (1.0f / sqrtf(l2)) : mk(1.0f, 0.0f, 0.0f);
+  const V3 T2 = cross(vh, T1);
+  const float r = sqrtf(u1);
+  const float phi = 2.0f * kPi * u2;
+  const float t1 = r * cosf(phi);
+  float t2 = r * sinf(phi);
+  const float s = 0.5f * (1.0f + vh.z);
+  t2 = (1.0f - s) * sqrtf(fmaxf(0.0f, 1.0f - t1 * t1)) + s * t2;
+  const float tz = sqrtf(fmaxf(0.0f, 1.0f - t1 * t1 - t2 * t2));
+  const V3 nh = mk(T1.x * t1 + T2.x * t2 + vh.x * tz, T1.y * t1 + T2.y * t2 + vh.y * tz,
+                   T1.z * t1 + T2.z * t2 + vh.z * tz);
+  return normalize(mk(a * nh.x, a * nh.y, fmaxf(1.0e-6f, nh.z)));
+}
+// Density of that visible normal: G1(wo) max(0, wo.m) D(m) / wo.z.
+__device__ __forceinline__ float ggxVisiblePdf(V3 wo, V3 m, float a) {
+  return ggxG1(wo, a) * fmaxf(0.0f, dot(wo, m)) * ggxD(m, a) / wo.z;
+}
+// Unpolarized Fresnel reflectance of a dielectric; cosI = cos of the
+// incident angle to the normal on the incident side, eta = n_t / n_i.
+__device__ __forceinline__ float fresnelDielectric(float cosI, float eta) {
+  const float s2 = fmaxf(0.0f, 1.0f - cosI * cosI) / (eta * eta);
+  if (s2 >= 1.0f) return 1.0f;
+  const float cosT = sqrtf(1.0f - s2);
+  const float rs = (cosI - eta * cosT) / (cosI + eta * cosT);
+  const float rp = (eta * cosI - cosT) / (eta * cosI + cosT);
+  return 0.5f * (rs * rs + rp * rp);
+}
+__device__ __forceinline__ V3 schlick(V3 f0, float c) {
+  const float m = fminf(1.0f, fmaxf(0.0f, 1.0f - c));
+  const float m5 = (m * m) * (m * m) * m;
+  return mk(f0.x + (1.0f - f0.x) * m5, f0.y + (1.0f - f0.y) * m5, f0.z + (1.0f - f0.z) * m5);
+}
+
+// The surface seen at a vertex: local frame around the outward normal n
+// (geometric for primitives, interpolated for meshes), base colour and
+// material.
+struct Surf {
+  Onb f;
+  V3 C;
+  int type;
+  float alpha;   // GGX alpha (roughness squared)
+  float ior;
+};
+constexpr float kSmoothAlpha = 4.0e-4f;  // below: a smooth (delta) interface
+
+// f(wo, wi) * |cos wi| in the local frame. Conductors and diffuse surfaces
+// are one-sided (both directions on the outward side); dielectrics reflect
+// and transmit by the microfacet model (no delta lobe is evaluated).
+__device__ __forceinline__ V3 bsdfCos(const Surf& s, V3 wo, V3 wi) {
+  if (s.type == kMatDiffuse) {
+    if (wo.z <= 0.0f || wi.z <= 0.0f) return mk(0.0f, 0.0f, 0.0f);
+    return s.C * (wi.z / kPi);
+  }
+  if (s.alpha < kSmoothAlpha) return mk(0.0f, 0.0f, 0.0f);
+  if (s.type == kMatConductor) {
+    if (wo.z <= 0.0f || wi.z <= 0.0f) return mk(0.0f, 0.0f, 0.0f);
+    const V3 m = normalize(wo + wi);
+    const V3 F = schlick(s.C, dot(wi, m));
+    const float k = ggxD(m, s.alpha) * ggxG2(wo, wi, s.alpha) / (4.0f * wo.z);
+    return F * k;
+  }
+  // dielectric: eta = n_inside / n_outside seen from the side of wo
+  const float cosO = wo.z, cosI = wi.z;
+  if (cosO == 0.0f || cosI == 0.0f) return mk(0.0f, 0.0f, 0.0f);
+  const bool refl = cosO * cosI > 0.0f;
+  const float eta = cosO > 0.0f ? s.ior : 1.0f / s.ior;  // n_t / n_i for light leaving along wo
+  V3 m = refl ? (wo + wi) : (wi * eta + wo);
+  if (dot(m, m) == 0.0f) return mk(0.0f, 0.0f, 0.0f);
+  m = normalize(m);
+  if (m.z < 0.0f) m = neg(m);
+  if (dot(m, wi) * cosI < 0.0f || dot(m, wo) * cosO < 0.0f) return mk(0.0f, 0.0f, 0.0f);
+  const float F = fresnelDielectric(fabsf(dot(wo, m)), cosO > 0.0f ? s.ior : 1.0f / s.ior);
+  const float D = ggxD(m, s.alpha);
+  const float G = ggxG2(mk(wo.x, wo.y, fabsf(wo.z)), mk(wi.x, wi.y, fabsf(wi.z)), s.alpha);
+  if (refl) {
+    const float k = D * G * F / (4.0f * fabsf(cosO));
+    return mk(k, k, k);
+  }
+  const float dn = dot(wi, m) + dot(wo, m) / eta;
+  const float k = D * (1.0f - F) * G * fabsf(dot(wi, m) * dot(wo, m) / (dn * dn * cosO)) / (eta * eta);
+  return s.C * k;
+}
+
+// Samples wi for wo: returns false when the path ends; weight = f |cos| / pdf.
+__device__ __forceinline__ bool bsdfSample(const Surf& s, V3 wo, float u1, float u2, float u3, V3& wi, V3& weight) {
+  if (s.type == kMatDiffuse) {
+    if (wo.z <= 0.0f) return false;
+    const float r = sqrtf(u1), phi = 2.0f * kPi * u2;
+    wi = mk(r * cosf(phi), r * sinf(phi), sqrtf(fmaxf(0.0f, 1.0f - u1)));
+    weight = s.C;
+    return true;
+  }
+  if (s.type == kMatConductor) {
+    if (wo.z <= 0.0f) return false;
+    V3 m;
+    if (s.alpha < kSmoothAlpha) {
+      m = mk(0.0f, 0.0f, 1.0f);
+    } else {
+      m = ggxSampleVisible(wo, s.alpha, u1, u2);
+    }
+    const float om = dot(wo, m);
+    wi = m * (2.0f * om) - wo;
+    if (wi.z <= 0.0f) return false;
+    const V3 F = schlick(s.C, dot(wi, m));
+    weight = s.alpha < kSmoothAlpha ? F : F * (ggxG2(wo, wi, s.alpha) / ggxG1(wo, s.alpha));
+    return true;
+  }
+  // dielectric, in the frame flipped to the side of wo
+  const bool outside = wo.z > 0.0f;
+  const V3 w = outside ? wo : mk(wo.x, wo.y, -wo.z);
+  if (w.z == 0.0f) return false;
+  const float eta = outside ? s.ior : 1.0f / s.ior;  // n_t / n_i
+  const V3 m = s.alpha < kSmoothAlpha ? mk(0.0f, 0.0f, 1.0f) : ggxSampleVisible(w, s.alpha, u1, u2);
+  const float c = dot(w, m);
+  const float F = fresnelDielectric(c, eta);
+  V3 wl;
+  bool refracted = false;
+  if (u3 < F) {
+    wl = m * (2.0f * c) - w;
+    if (wl.z <= 0.0f) return false;
+  } else {
+    const float s2 = fmaxf(0.0f, 1.0f - c * c) / (eta * eta);
+    if (s2 >= 1.0f) return false;  // (F is 1 under total internal reflection)
+    const float ct = sqrtf(1.0f - s2);
+    wl = m * (c / eta - ct) - w * (1.0f / eta);
+    if (wl.z >= 0.0f) return false;
+    refracted = true;
+  }
+  const float g = s.alpha < kSmoothAlpha ? 1.0f
+                                         : ggxG2(w, mk(wl.x, wl.y, fabsf(wl.z)), s.alpha) / ggxG1(w, s.alpha);
+  wi = outside ? wl : mk(wl.x, wl.y, -wl.z);
+  // radiance crossing the interface scales by (n_i / n_t)^2 = 1 / eta^2
+  weight = refracted ? s.C * (g / (eta * eta)) : mk(g, g, g);
+  return true;
+}
+
+// ---------------------------------------------------------------- lights
+// Uniform direction in the cone the light subtends from P; returns the hit
+// point on the sphere and the area-measure density (light choice included).
+__device__ __forceinline__ bool lightSample(const Light& L, V3 P, float u1, float u2, int nLights, V3& Q, float& pdfA) {
+  const V3 c = mk(L.c[0], L.c[1], L.c[2]);
+  const V3 dcv = c - P;
+  const float d2 = dot(dcv, dcv);
+  const float r2 = L.r * L.r;
+  if (d2 <= r2) return false;
+  const float d = sqrtf(d2);
+  const V3 axis = dcv * (1.0f / d);
+  const float cosMax = sqrtf(fmaxf(0.0f, 1.0f - r2 / d2));
+  const float cosT = 1.0f - u1 * (1.0f - cosMax);
+  const float sinT = sqrtf(fmaxf(0.0f, 1.0f - cosT * cosT));
+  const float phi = 2.0f * kPi * u2;
+  const Onb f = onb(axis);
+  const V3 dir = toWorld(f, mk(sinT * cosf(phi), sinT * sinf(phi), cosT));
+  // nearest intersection of P + t dir with the sphere
+  const float b = dot(dcv, dir);
+  const float disc = b * b - (d2 - r2);
+  const float t = b - sqrtf(fmaxf(0.0f, disc));
+  Q = P + dir * t;
+  const V3 nl = (Q - c) * (1.0f / L.r);
+  const float cosL = fabsf(dot(nl, dir));
+  const float pdfW = 1.0f / (2.0f * kPi * (1.0f - cosMax));
+  pdfA = pdfW * cosL / (t * t) / static_cast<float>(nLights);
+  return pdfA > 0.0f;
+}
+// Unshadowed contribution of light point Q (on light li) at the surface:
+// f |cos| Le |cos_l| / dist^2 (area measure); target = its luminance.
+__device__ __forceinline__ V3 lightContrib(const Params& p, const Surf& s, V3 P, V3 wo, V3 Q, int li, float& dist) {
+  const Light& L = p.lights[li];
+  const V3 dv = Q - P;
+  const float d2 = dot(dv, dv);
+  dist = sqrtf(d2);
+  if (!(d2 > 0.0f)) return mk(0.0f, 0.0f, 0.0f);
+  const V3 wi = dv * (1.0f / dist);
+  const V3 nl = (Q - mk(L.c[0], L.c[1], L.c[2])) * (1.0f / L.r);
+  const float cosL = -dot(nl, wi);
+  if (cosL <= 0.0f) return mk(0.0f, 0.0f, 0.0f);
+  const V3 fc = bsdfCos(s, toLocal(s.f, wo), toLocal(s.f, wi));
+  const float g = cosL / d2;
+  return mk(fc.x * L.Le[0] * g, fc.y * L.Le[1] * g, fc.z * L.Le[2] * g);
+}
+__device__ __forceinline__ float lum(V3 c) { return 0.2126f * c.x + 0.7152f * c.y + 0.0722f * c.z; }
+
+// Adds the wave's traced rays to the frame counter (one atomic per wave).
+__device__ __forceinline__ void countRays(unsigned long long* c, uint32_t n) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) n += __shfl_xor(n, off, 64);
+  if (__lane_id() == 0 && n) atomicAdd(c, static_cast<unsigned long long>(n));
+}
+
+// ---------------------------------------------------------------- scene access
+__device__ __forceinline__ bool traceClosest(const SceneView& s, V3 o, V3 d, float tf, Hit& h) {
+  uint32_t* base = ptLds + threadIdx.x;
+  return traverse8<false, false>(s, o, d, 0.0f, tf, h, base, base + s.stackDepth * blockDim.x, blockDim.x, nullptr,
+                                 nullptr);
+}
+__device__ __forceinline__ bool traceAny(const SceneView& s, V3 o, V3 d, float tf) {
+  uint32_t* base = ptLds + threadIdx.x;
+  Hit h;
+  return traverse8<true, false>(s, o, d, 0.0f, tf, h, base, base + s.stackDepth * blockDim.x, blockDim.x, nullptr,
+                                nullptr);
+}
+
+// Surface at a hit: outward normal, base colour, material.
+__device__ __forceinline__ Surf surfaceAt(const Params& p, const SceneView& s, const Hit& h, V3 rd, V3& Ng) {
+  const uint32_t kind = h.bits >> kKindShift, id = h.bits & kIdMask;
+  V3 N, C;
+  uint32_t mat;
+  bool mesh = false;
+  if (kind == 0u) {
+    const uint4 tri = reinterpret_cast<const uint4*>(s.triIdx)[id];
+    const float4* vn = reinterpret_cast<const float4*>(s.vtxNrm);
+    const float4* vc = reinterpret_cast<const float4*>(s.vtxCol);
+    const float4 n0 = vn[tri.x], n1 = vn[tri.y], n2 = vn[tri.z];
+    const float4 c0 = vc[tri.x], c1 = vc[tri.y], c2 = vc[tri.z];
+    const float w0 = 1.0f - h.u - h.v;
+    N = normalize(mk(w0 * n0.x + h.u * n1.x + h.v * n2.x, w0 * n0.y + h.u * n1.y + h.v * n2.y,
+                     w0 * n0.z + h.u * n1.z + h.v * n2.z));
+    C = mk(w0 * c0.x + h.u * c1.x + h.v * c2.x, w0 * c0.y + h.u * c1.y + h.v * c2.y,
+           w0 * c0.z + h.u * c1.z + h.v * c2.z);
+    Ng = normalize(h.ng);
+    if (dot(Ng, N) < 0.0f) Ng = neg(Ng);  // the mesh's outward side is its normals'
+    mat = tri.w & kInfoMatMask;
+    mesh = true;
+  } else {
+    float4 col;
+    uint32_t info;
+    if (kind == 1u) {
+      col = reinterpret_cast<const float4*>(s.sphColor)[id];
+      info = s.sphInfo[id];
+    } else if (kind == 3u) {
+      col = reinterpret_cast<const float4*>(s.coneColor)[id];
+      info = s.coneInfo[id];
+    } else {
+      col = reinterpret_cast<const float4*>(s.rndColor)[id];
+      info = s.rndInfo[id];
+    }
+    N = normalize(h.ng);  // primitive normals point outward
+    Ng = N;
+    C = mk(col.x, col.y, col.z);
+    mat = info & kInfoMatMask;
+  }
+  const Mat* tab = mesh ? p.meshMats : p.primMats;
+  const int n = mesh ? p.nMeshMats : p.nPrimMats;
+  const Mat m = n > 0 ? tab[mat < static_cast<uint32_t>(n) ? mat : 0u] : Mat{kMatDiffuse, 0.0f, 1.5f, 0};
+  Surf sf;
+  // opaque surfaces shade the side the ray arrives on
+  if (m.type != kMatDielectric && dot(N, rd) > 0.0f) {
+    N = neg(N);
+    Ng = neg(Ng);
+  }
+  sf.f = onb(N);
+  sf.C = C;
+  sf.type = m.type;
+  sf.alpha = m.alpha;
+  sf.ior = m.ior;
+  return sf;
+}
+
+// Offset origin for a ray leaving the surface at P along d.
+__device__ __forceinline__ V3 offsetOrigin(V3 P, V3 Ng, V3 d, float t) {
+  const float e = fmaxf(fmaxf(fabsf(P.x), fabsf(P.y)), fmaxf(fabsf(P.z), t)) * 0x1.0p-19f + 1.0e-6f;
+  return P + Ng * (dot(d, Ng) >= 0.0f ? e : -e);
+}
+
+__device__ __forceinline__ void cameraRay(const Params& p, float fx, float fy, V3& o, V3& d) {
+  const float sx = 2.0f * fx / static_cast<float>(p.W) - 1.0f;
+  const float sy = 1.0f - 2.0f * fy / static_cast<float>(p.H);
+  const V3 dir = mk(p.dir[0], p.dir[1], p.dir[2]);
+  const V3 right = mk(p.right[0], p.right[1], p.right[2]);
+  const V3 up = mk(p.up[0], p.up[1], p.up[2]);
+  const V3 pos = mk(p.pos[0], p.pos[1], p.pos[2]);
+  if (p.ortho) {
+    o = pos + right * (sx * p.halfW) + up * (sy * p.halfH);
+    d = dir;
+  } else {
+    o = pos;
+    d = normalize(dir + right * (sx * p.persHalfW) + up * (sy * p.persHalfH));
+  }
+}
+
+// Dimension layout of one sample's random numbers: camera jitter 0..1;
+// vertex b: RIS candidates at 4c .. 4c + 3 (c < 12), the BSDF sample at
+// 56 .. 58; the spatial reuse's neighbour offsets and selections; the
+// all-lights estimator's cone samples, 2 per light.
+constexpr uint32_t kDimCam = 0;
+__device__ __forceinline__ uint32_t dimVertex(int b) { return 16u + 64u * static_cast<uint32_t>(b); }
+constexpr uint32_t kDimSpatial = 0x10000u;
+__device__ __forceinline__ uint32_t dimAllLights(int b) { return 0x100000u + 0x2000u * static_cast<uint32_t>(b); }
+
+// RIS over M light candidates at a surface: the selected point, its light,
+// the reservoir weight sum and the selected target.
+struct Pick {
+  V3 Q;
+  int li;
+  float wsum, target;
+};
+__device__ __forceinline__ Pick risLights(const Params& p, const Surf& sf, V3 P, V3 wo, uint32_t pix, uint32_t smp,
+                                          uint32_t dim) {
+  Pick k{mk(0.0f, 0.0f, 0.0f), -1, 0.0f, 0.0f};
+  for (int c = 0; c < p.restirM; ++c) {
+    const uint32_t d = dim + 4u * static_cast<uint32_t>(c);
+    const int li = min(p.nLights - 1, static_cast<int>(rnd(p.seed, pix, smp, d) * static_cast<float>(p.nLights)));
+    V3 Q;
+    float pdfA;
+    if (!lightSample(p.lights[li], P, rnd(p.seed, pix, smp, d + 1), rnd(p.seed, pix, smp, d + 2), p.nLights, Q,
+                     pdfA))
+      continue;
+    float dist;
+    const float t = lum(lightContrib(p, sf, P, wo, Q, li, dist));
+    const float w = t / pdfA;
+    if (!(w > 0.0f)) continue;
+    k.wsum += w;
+    if (rnd(p.seed, pix, smp, d + 3) * k.wsum < w) {
+      k.Q = Q;
+      k.li = li;
+      k.target = t;
+    }
+  }
+  return k;
+}
+
+// Direct light at a vertex by RIS NEE (or, with neeAll, one cone sample of
+// every light), shadow rays included.
+__device__ __forceinline__ V3 directLight(const Params& p, const SceneView& s, const Surf& sf, V3 P, V3 Ng, V3 wo,
+                                          uint32_t pix, uint32_t smp, int b, uint32_t& nr) {
+  V3 sum = mk(0.0f, 0.0f, 0.0f);
+  if (p.nLights <= 0) return sum;
+  if (p.neeAll) {
+    for (int li = 0; li < p.nLights; ++li) {
+      const uint32_t d = dimAllLights(b) + 2u * static_cast<uint32_t>(li);
+      V3 Q;
+      float pdfA;
+      if (!lightSample(p.lights[li], P, rnd(p.seed, pix, smp, d), rnd(p.seed, pix, smp, d + 1), 1, Q, pdfA)) continue;
+      float dist;
+      const V3 c = lightContrib(p, sf, P, wo, Q, li, dist);
+      if (!(lum(c) > 0.0f)) continue;
+      const V3 wi = (Q - P) * (1.0f / dist);
+      ++nr;
+      if (traceAny(s, offsetOrigin(P, Ng, wi, 0.0f), wi, dist * 0.9999f)) continue;
+      sum = sum + c * (1.0f / pdfA);
+    }
+    return sum;
+  }
+  const Pick k = risLights(p, sf, P, wo, pix, smp, dimVertex(b));
+  if (k.li < 0 || !(k.target > 0.0f)) return sum;
+  float dist;
+  const V3 c = lightContrib(p, sf, P, wo, k.Q, k.li, dist);
+  const V3 wi = (k.Q - P) * (1.0f / dist);
+  ++nr;
+  if (traceAny(s, offsetOrigin(P, Ng, wi, 0.0f), wi, dist * 0.9999f)) return sum;
+  const float W = k.wsum / (static_cast<float>(p.restirM) * k.target);
+  return c * W;
+}
+
+// ---------------------------------------------------------------- kernels
+// One camera ray; returns the rays traced (1).
+__device__ __forceinline__ uint32_t primaryPixel(const Params& p, const SceneView& s, int pix, GPix* __restrict__ gb,
+                                                 Reservoir* __restrict__ res, float4* __restrict__ acc,
+                                                 float4* __restrict__ first) {
+  const uint32_t upix = static_cast<uint32_t>(pix), smp = static_cast<uint32_t>(p.sample);
+  const int px = pix % p.W, py = pix / p.W;
+  V3 o, d;
+  cameraRay(p, px + rnd(p.seed, upix, smp, kDimCam), py + rnd(p.seed, upix, smp, kDimCam + 1), o, d);
+  Hit h;
+  GPix g;
+  Reservoir r;
+  r.Q = make_float4(0.0f, 0.0f, 0.0f, __int_as_float(-1));
+  r.wsum = 0.0f;
+  r.M = 0.0f;
+  r.W = 0.0f;
+  r.target = 0.0f;
+  if (!traceClosest(s, o, d, INFINITY, h)) {
+    g.P = make_float4(0.0f, 0.0f, 0.0f, 0.0f);  // w = 0: no surface
+    gb[pix] = g;
+    res[pix] = r;
+    float4 a = acc[pix];
+    acc[pix] = make_float4(a.x + p.env[0], a.y + p.env[1], a.z + p.env[2], 0.0f);
+    if (first && p.sample == 0) first[2 * pix] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (first && p.sample == 0) first[2 * pix + 1] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    return 1u;
+  }
+  V3 Ng;
+  const Surf sf = surfaceAt(p, s, h, d, Ng);
+  const V3 P = o + d * h.t;
+  const V3 wo = neg(d);
+  g.P = make_float4(P.x, P.y, P.z, h.t);
+  g.N = make_float4(sf.f.n.x, sf.f.n.y, sf.f.n.z, __int_as_float(sf.type));
+  g.Ng = make_float4(Ng.x, Ng.y, Ng.z, sf.alpha);
+  g.C = make_float4(sf.C.x, sf.C.y, sf.C.z, sf.ior);
+  g.wo = make_float4(wo.x, wo.y, wo.z, 0.0f);
+  gb[pix] = g;
+  if (first && p.sample == 0) {
+    const V3 alb = sf.type == kMatDielectric ? mk(1.0f, 1.0f, 1.0f) : sf.C;
+    first[2 * pix] = make_float4(alb.x, alb.y, alb.z, h.t);
+    first[2 * pix + 1] = make_float4(sf.f.n.x, sf.f.n.y, sf.f.n.z, 1.0f);
+  }
+  if (p.nLights > 0 && !p.neeAll) {
+    const Pick k = risLights(p, sf, P, wo, upix, smp, dimVertex(0));
+    r.Q = make_float4(k.Q.x, k.Q.y, k.Q.z, __int_as_float(k.li));
+    r.wsum = k.wsum;
+    r.M = static_cast<float>(p.restirM);
+    r.target = k.target;
+    r.W = (k.li >= 0 && k.target > 0.0f) ? k.wsum / (r.M * k.target) : 0.0f;
+  }
+  res[pix] = r;
+  return 1u;
+}
+__global__ __launch_bounds__(kBlock) void k_pt_primary(Params p, SceneView s, GPix* __restrict__ gb,
+                                                      Reservoir* __restrict__ res, float4* __restrict__ acc,
+                                                      float4* __restrict__ first) {
+  const int pix = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t nr = pix < p.W * p.H ? primaryPixel(p, s, pix, gb, res, acc, first) : 0u;
+  countRays(p.rays, nr);
+}
+
+__device__ __forceinline__ Surf surfOfG(const GPix& g) {
+  Surf sf;
+  sf.f = onb(mk(g.N.x, g.N.y, g.N.z));
+  sf.C = mk(g.C.x, g.C.y, g.C.z);
+  sf.type = __float_as_int(g.N.w);
+  sf.alpha = g.Ng.w;
+  sf.ior = g.C.w;
+  return sf;
+}
+
+// The rest of one path; returns the rays traced.
+__device__ __forceinline__ uint32_t shadePixel(const Params& p, const SceneView& s, int pix,
+                                               const GPix* __restrict__ gb, const Reservoir* __restrict__ res,
+                                               float4* __restrict__ acc) {
+  const GPix g = gb[pix];
+  if (g.P.w == 0.0f) return 0u;  // the environment was added by the primary pass
+  uint32_t nr = 0;
+  const uint32_t upix = static_cast<uint32_t>(pix), smp = static_cast<uint32_t>(p.sample);
+  const Surf sf0 = surfOfG(g);
+  const V3 P0 = mk(g.P.x, g.P.y, g.P.z), Ng0 = mk(g.Ng.x, g.Ng.y, g.Ng.z), wo0 = mk(g.wo.x, g.wo.y, g.wo.z);
+  V3 L = mk(0.0f, 0.0f, 0.0f);
+  // ---- first vertex: ReSTIR over the own and K neighbours' reservoirs
+  if (p.nLights > 0) {
+    if (p.neeAll) {
+      L = directLight(p, s, sf0, P0, Ng0, wo0, upix, smp, 0, nr);
+    } else {
+      const int px = pix % p.W, py = pix / p.W;
+      int src[1 + kMaxSpatial];
+      int nsrc = 0;
+      src[nsrc++] = pix;
+      for (int j = 0; j < p.restirK; ++j) {
+        const uint32_t d = kDimSpatial + 3u * static_cast<uint32_t>(j);
+        const int qx = min(p.W - 1, max(0, px + static_cast<int>(floorf((2.0f * rnd(p.seed, upix, smp, d) - 1.0f) *
+                                                                           p.restirRadius + 0.5f))));
+        const int qy = min(p.H - 1, max(0, py + static_cast<int>(floorf((2.0f * rnd(p.seed, upix, smp, d + 1) - 1.0f) *
+                                                                           p.restirRadius + 0.5f))));
+        const int q = qy * p.W + qx;
+        if (q == pix || gb[q].P.w == 0.0f) continue;
+        src[nsrc++] = q;
+      }
+      // combine with balance-heuristic resampling weights: candidate y_i of
+      // reservoir i (weight sum wsum_i, M_i candidates, target p_i) enters
+      // with m_i(y_i) p_pix(y_i) W_i = p_pix(y_i) wsum_i / sum_j M_j p_j(y_i),
+      // m_i = M_i p_i / sum_j M_j p_j summing to one wherever p_pix > 0
+      V3 Q = mk(0.0f, 0.0f, 0.0f);
+      int li = -1;
+      float wsum = 0.0f, tsel = 0.0f;
+      for (int i = 0; i < nsrc; ++i) {
+        const Reservoir r = res[src[i]];
+        const int l = __float_as_int(r.Q.w);
+        if (l < 0 || !(r.wsum > 0.0f)) continue;
+        const V3 Qi = mk(r.Q.x, r.Q.y, r.Q.z);
+        float dist;
+        const float t = lum(lightContrib(p, sf0, P0, wo0, Qi, l, dist));
+        if (!(t > 0.0f)) continue;
+        float den = static_cast<float>(p.restirM) * t;
+        for (int j = 1; j < nsrc; ++j) {
+          const GPix gq = gb[src[j]];
+          const float tj =
+              j == i ? r.target
+                     : lum(lightContrib(p, surfOfG(gq), mk(gq.P.x, gq.P.y, gq.P.z), mk(gq.wo.x, gq.wo.y, gq.wo.z), Qi,
+                                        l, dist));
+          den += res[src[j]].M * tj;
+        }
+        const float w = t * r.wsum / den;
+        if (!(w > 0.0f)) continue;
+        wsum += w;
+        if (rnd(p.seed, upix, smp, kDimSpatial + 3u * static_cast<uint32_t>(i) + 2u) * wsum < w) {
+          Q = Qi;
+          li = l;
+          tsel = t;
+        }
+      }
+      if (li >= 0 && tsel > 0.0f) {
+        float dist;
+        const V3 c = lightContrib(p, sf0, P0, wo0, Q, li, dist);
+        const V3 wi = (Q - P0) * (1.0f / dist);
+        ++nr;
+        if (!traceAny(s, offsetOrigin(P0, Ng0, wi, 0.0f), wi, dist * 0.9999f)) L = c * (wsum / tsel);
+      }
+    }
+  }
+  // ---- the scatterings: a BSDF sample at vertex b - 1, then the
+  // environment on an escape, or vertex b with its own next-event estimate
+  // while b < bounces (vertices 0 .. bounces - 1 are shaded)
+  V3 thr = mk(1.0f, 1.0f, 1.0f);
+  Surf sf = sf0;
+  V3 P = P0, Ng = Ng0, wo = wo0;
+  float tPrev = g.P.w;
+  for (int b = 1; b <= p.bounces; ++b) {
+    const uint32_t dv = dimVertex(b - 1) + 56u;
+    V3 wiL, wgt;
+    if (!bsdfSample(sf, toLocal(sf.f, wo), rnd(p.seed, upix, smp, dv), rnd(p.seed, upix, smp, dv + 1),
+                    rnd(p.seed, upix, smp, dv + 2), wiL, wgt))
+      break;
+    thr = mk(thr.x * wgt.x, thr.y * wgt.y, thr.z * wgt.z);
+    if (!(lum(thr) > 0.0f)) break;
+    const V3 wi = toWorld(sf.f, wiL);
+    const V3 o = offsetOrigin(P, Ng, wi, tPrev);
+    Hit h;
+    ++nr;
+    if (!traceClosest(s, o, wi, INFINITY, h)) {
+      L = L + mk(thr.x * p.env[0], thr.y * p.env[1], thr.z * p.env[2]);
+      break;
+    }
+    if (b == p.bounces) break;  // the vertex budget is spent
+    V3 ng;
+    sf = surfaceAt(p, s, h, wi, ng);
+    P = o + wi * h.t;
+    Ng = ng;
+    wo = neg(wi);
+    tPrev = h.t;
+    const V3 Ld = directLight(p, s, sf, P, Ng, wo, upix, smp, b, nr);
+    L = L + mk(thr.x * Ld.x, thr.y * Ld.y, thr.z * Ld.z);
+  }
+  float4 a = acc[pix];
+  acc[pix] = make_float4(a.x + L.x, a.y + L.y, a.z + L.z, 0.0f);
+  return nr;
+}
+__global__ __launch_bounds__(kBlock) void k_pt_shade(Params p, SceneView s, const GPix* __restrict__ gb,
+                                                    const Reservoir* __restrict__ res, float4* __restrict__ acc) {
+  const int pix = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t nr = pix < p.W * p.H ? shadePixel(p, s, pix, gb, res, acc) : 0u;
+  countRays(p.rays, nr);
+}
+
+// Mean over the samples, and the denoiser features: log(1 + colour), albedo,
+// normal, first-hit distance / its frame maximum (maxBits: that maximum as
+// float bits, from k_pt_depth_max).
+__global__ void k_pt_depth_max(int n, const float4* __restrict__ first, unsigned int* __restrict__ maxBits) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) atomicMax(maxBits, __float_as_uint(fmaxf(0.0f, first[2 * i].w)));
+}
+__global__ void k_pt_resolve(int W, int H, int Wp, int Hp, int spp, const float4* __restrict__ acc,
+                             const float4* __restrict__ first, const unsigned int* __restrict__ maxBits,
+                             float4* __restrict__ color, float* __restrict__ feat) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (feat) {
+    if (i >= Wp * Hp) return;
+    const int x = min(i % Wp, W - 1), y = min(i / Wp, H - 1);  // edge-replicated padding
+    const int j = y * W + x;
+    const float inv = 1.0f / static_cast<float>(spp);
+    const float4 a = acc[j];
+    const float4 f0 = first[2 * j], f1 = first[2 * j + 1];
+    const float mx = __uint_as_float(*maxBits);
+    float* o = feat + static_cast<size_t>(i) * kPtFeatures;
+    o[0] = log1pf(fmaxf(0.0f, a.x * inv));
+    o[1] = log1pf(fmaxf(0.0f, a.y * inv));
+    o[2] = log1pf(fmaxf(0.0f, a.z * inv));
+    o[3] = f0.x;
+    o[4] = f0.y;
+    o[5] = f0.z;
+    o[6] = f1.x;
+    o[7] = f1.y;
+    o[8] = f1.z;
+    o[9] = mx > 0.0f ? f0.w / mx : 0.0f;
+    if (color && i < W * H) {
+      const float4 c = acc[i];
+      color[i] = make_float4(c.x * inv, c.y * inv, c.z * inv, 1.0f);
+    }
+    return;
+  }
+  if (i >= W * H) return;
+  const float inv = 1.0f / static_cast<float>(spp);
+  const float4 c = acc[i];
+  color[i] = make_float4(c.x * inv, c.y * inv, c.z * inv, 1.0f);
+}
+// The denoised colour: expm1 of the network's log(1 + colour) prediction.
+__global__ void k_pt_apply_net(int W, int H, int Wp, const float* __restrict__ pred, float4* __restrict__ color) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= W * H) return;
+  const int x = i % W, y = i / W;
+  const float* q = pred + (static_cast<size_t>(y) * Wp + x) * 3;
+  color[i] = make_float4(fmaxf(0.0f, expm1f(q[0])), fmaxf(0.0f, expm1f(q[1])), fmaxf(0.0f, expm1f(q[2])), 1.0f);
+}
+__global__ void k_pt_zero(float4* __restrict__ a, int n, unsigned int* __restrict__ maxBits,
+                          unsigned long long* __restrict__ rays) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  if (i == 0) {
+    *maxBits = 0u;
+    *rays = 0ull;
+  }
+}
+
+}  // namespace pt
+
+// ---------------------------------------------------------------- launchers
+namespace {
+inline unsigned gridOf(int64_t n, int b) { return static_cast<unsigned>((n + b - 1) / b); }
+}  // namespace
+
+void ptZero(float4* acc, int n, unsigned int* maxBits, unsigned long long* rays, hipStream_t st) {
+  hipLaunchKernelGGL(pt::k_pt_zero, dim3(gridOf(n, 256)), dim3(256), 0, st, acc, n, maxBits, rays);
+}
+void ptSample(const pt::Params& p, const SceneView& s, pt::GPix* gb, pt::Reservoir* res, float4* acc, float4* first,
+              hipStream_t st) {
+  const int n = p.W * p.H;
+  const size_t lds = static_cast<size_t>(2) * static_cast<size_t>(s.stackDepth > 0 ? s.stackDepth : 1) * pt::kBlock * 4u;
+  hipLaunchKernelGGL(pt::k_pt_primary, dim3(gridOf(n, pt::kBlock)), dim3(pt::kBlock), lds, st, p, s, gb, res, acc,
+                     first);
+  hipLaunchKernelGGL(pt::k_pt_shade, dim3(gridOf(n, pt::kBlock)), dim3(pt::kBlock), lds, st, p, s, gb, res, acc);
+}
+void ptResolve(int W, int H, int Wp, int Hp, int spp, const float4* acc, const float4* first, unsigned int* maxBits,
+               float4* color, float* feat, hipStream_t st) {
+  if (feat) {
+    hipLaunchKernelGGL(pt::k_pt_depth_max, dim3(gridOf(W * H, 256)), dim3(256), 0, st, W * H, first, maxBits);
+    hipLaunchKernelGGL(pt::k_pt_resolve, dim3(gridOf(static_cast<int64_t>(Wp) * Hp, 256)), dim3(256), 0, st, W, H, Wp,
+                       Hp, spp, acc, first, maxBits, color, feat);
+    return;
+  }
+  hipLaunchKernelGGL(pt::k_pt_resolve, dim3(gridOf(W * H, 256)), dim3(256), 0, st, W, H, Wp, Hp, spp, acc, first,
+                     maxBits, color, nullptr);
+}
+void ptApplyNet(int W, int H, int Wp, const float* pred, float4* color, hipStream_t st) {
+  hipLaunchKernelGGL(pt::k_pt_apply_net, dim3(gridOf(W * H, 256)), dim3(256), 0, st, W, H, Wp, pred, color);
+}
+
+}  // namespace ubg

@@ -405,3 +405,57 @@ def molecular_surface_scene(n_atoms: int = 40000, width: int = 1920, height: int
     opt = RenderOptions(width=width, height=height, supersample=supersample, shadows=True,
                         ao_samples=ao_samples, ao_distance=8.0)
     return sc, opt
+
+
+def pathtraced_surface_scene(n_atoms: int = 40000, width: int = 1920, height: int = 1080, spp: int = 1,
+                             bounces: int = 4, n_lights: int = 64, seed: int = 0, sigma: float = 1.0,
+                             iso: float = 0.5, spacing: float = 0.5):
+    """Path-traced GI over the molecular surface: the Gaussian isosurface of
+    :func:`molecular_surface_scene` (its vertex colours the transmission tint)
+    as a GGX dielectric over the CPK spheres of the same atoms, each element
+    a GGX conductor (the palette colour its F0) or a GGX dielectric;
+    roughness uniform in [0.05, 0.6] per material, index of refraction in
+    [1.33, 1.6]; ``n_lights`` spherical area lights (radius 3-8 % of the
+    molecule's bounding radius, 1.1-1.6 bounding radii from its centre,
+    random warm-to-cool tints), a dim constant environment; 4 bounces.
+
+    Returns (Scene, PtOptions); the scene's camera frames the surface."""
+    from .pathtrace import CONDUCTOR, DIELECTRIC, PtMaterial, PtOptions
+    pos, elem, v, nrm, idx, velem = _surface_mesh(int(n_atoms), int(seed), float(sigma), float(iso),
+                                                  float(spacing))
+    rng = np.random.default_rng(seed + 7)
+    radii = RADII[np.random.default_rng(seed + 1).integers(0, len(RADII), len(pos))]
+    sc = Scene()
+    sc.set_mesh(v, nrm, PALETTE[velem], idx, material=LIT)
+    sc.add_spheres(pos, radii, PALETTE[elem], LIT)
+    # one material index per element (the path tracer's table is indexed by it)
+    sc.prim_materials = [Material(ambient=0.2, diffuse=0.7 - 0.01 * k, specular=0.3, roughness=0.01)
+                         for k in range(len(PALETTE))]
+    sc.sphere_material = np.asarray(elem, np.int32)
+    sc.camera, _ = frame_camera(v, np.zeros(len(v), np.float32), 40.0, width / height)
+    sc.background = (0.0, 0.0, 0.0)
+    rough = lambda: float(rng.uniform(0.05, 0.6))  # noqa: E731
+    mesh_mats = [PtMaterial(DIELECTRIC, rough(), float(rng.uniform(1.33, 1.6)))]
+    prim_mats = []
+    for _ in range(len(PALETTE)):
+        if rng.random() < 0.6:
+            prim_mats.append(PtMaterial(CONDUCTOR, rough(), 1.5))
+        else:
+            prim_mats.append(PtMaterial(DIELECTRIC, rough(), float(rng.uniform(1.33, 1.6))))
+    lo, hi = v.min(axis=0), v.max(axis=0)
+    c = 0.5 * (lo + hi)
+    R = float(np.max(np.linalg.norm(v - c, axis=1)))
+    d = rng.normal(size=(n_lights, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    centers = c + d * (R * rng.uniform(1.1, 1.6, n_lights))[:, None]
+    lr = R * rng.uniform(0.03, 0.08, n_lights)
+    tint = np.stack([np.ones(n_lights), rng.uniform(0.7, 1.0, n_lights), rng.uniform(0.4, 1.0, n_lights)], 1)
+    tint = np.where(rng.random(n_lights)[:, None] < 0.5, tint, tint[:, ::-1])
+    # each light's radiance so that the summed irradiance at the centre is ~1
+    E1 = np.pi * (lr / np.linalg.norm(centers - c, axis=1)) ** 2
+    Le = tint * (2.0 / (E1.sum() + 1e-12))
+    lights = np.concatenate([centers, lr[:, None], Le], 1).astype(np.float32)
+    opt = PtOptions(width=width, height=height, spp=spp, bounces=bounces, seed=seed, lights=lights,
+                    env=(0.03, 0.03, 0.035), prim_materials=prim_mats, mesh_materials=mesh_mats,
+                    restir_candidates=8, restir_spatial=4, restir_radius=16.0)
+    return sc, opt
