@@ -485,7 +485,12 @@ typedef struct ubg_pt_params {
 typedef struct ubg_pt_frame {
   float* color;           /* width * height * 4 (linear HDR, alpha 1) */
   float* features;        /* width * height * 10 or NULL: log(1 + noisy colour) rgb,
-                             first-hit albedo rgb, normal xyz, distance / frame max */
+                             albedo guide rgb, normal guide xyz (camera space: right,
+                             up, towards the eye), first-hit distance / frame max.
+                             The guides are the first hit's, or for a dielectric
+                             first hit F * white + (1 - F) * the tinted colour of the
+                             first other surface along the ideal refraction (up to 4
+                             interfaces) and that surface's normal */
   int32_t memory;         /* 0 host buffers, 1 device */
   double trace_seconds;   /* the spp sample passes (device events) */
   double denoise_seconds; /* features + network + output */

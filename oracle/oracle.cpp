@@ -4032,6 +4032,57 @@ inline V3<T> directLight(const PCtx<T>& c, const Surf<T>& sf, V3<T> P, V3<T> Ng,
   return ct * Wt;
 }
 
+// Denoiser guides of a first hit (ub_pathtrace.hip `guides`): a dielectric
+// first hit follows the ideal refraction about the interface normal through
+// up to 4 dielectric interfaces to the first other surface; albedo = F * white
+// + (1 - F) * tint * its colour (F of the first interface, tint = the product
+// of the crossed interfaces' colours), normal = its normal; a chain that
+// misses, is totally reflected or runs out of hops keeps F * white and the
+// first hit's normal. The normal is returned in camera space.
+template <class T>
+static void guides(const PCtx<T>& c, const Surf<T>& sf0, V3<T> P0, V3<T> Ng0, V3<T> d0, T t0, V3<T>& alb,
+                   V3<T>& nrm) {
+  V3<T> n = sf0.f.n;
+  if (sf0.type != kDielectric) {
+    alb = sf0.C;
+  } else {
+    Surf<T> sf = sf0;
+    V3<T> P = P0, Ng = Ng0, d = d0;
+    T t = t0, F0 = T(0);
+    V3<T> tint = mkv<T>(T(1), T(1), T(1)), behind = mkv<T>(T(0), T(0), T(0));
+    for (int k = 0; k < 4; ++k) {
+      const T cosO = -dot(d, sf.f.n);
+      const bool outside = cosO > T(0);
+      const T cc = std::fabs(cosO);
+      const T eta = outside ? sf.ior : T(1) / sf.ior;
+      const T F = fresnelDielectric(cc, eta);
+      if (k == 0) F0 = F;
+      const T s2 = std::fmax(T(0), T(1) - cc * cc) / (eta * eta);
+      if (s2 >= T(1)) break;
+      const T ct = std::sqrt(T(1) - s2);
+      const V3<T> ns = outside ? sf.f.n : neg(sf.f.n);
+      d = normalize(d * (T(1) / eta) + ns * (cc / eta - ct));
+      tint = mkv<T>(tint.x * sf.C.x, tint.y * sf.C.y, tint.z * sf.C.z);
+      const V3<T> o = offsetOrigin(P, Ng, d, t);
+      Hit<T> h;
+      if (!traceClosest(c, o, d, h)) break;
+      V3<T> ng;
+      sf = surfaceAt(c, h, d, ng);
+      P = o + d * h.t;
+      Ng = ng;
+      t = h.t;
+      if (sf.type != kDielectric) {
+        behind = mkv<T>(tint.x * sf.C.x, tint.y * sf.C.y, tint.z * sf.C.z);
+        n = sf.f.n;
+        break;
+      }
+    }
+    const T Tr = T(1) - F0;
+    alb = mkv<T>(F0 + Tr * behind.x, F0 + Tr * behind.y, F0 + Tr * behind.z);
+  }
+  nrm = mkv<T>(dot(n, c.right), dot(n, c.up), -dot(n, c.dir));
+}
+
 template <class T>
 struct GP {  // first-hit record of one pixel and sample
   bool hit = false;
@@ -4086,9 +4137,10 @@ static void primaryPixel(const PCtx<T>& c, int pix, uint32_t smp, GP<T>& g, V3<T
   g.ior = sf.ior;
   if (first && smp == 0) {
     float* f = first + 8 * static_cast<size_t>(pix);
-    const V3<T> alb = sf.type == kDielectric ? mkv<T>(T(1), T(1), T(1)) : sf.C;
+    V3<T> alb, nrm;
+    guides(c, sf, P, Ng, d, h.t, alb, nrm);
     f[0] = float(alb.x), f[1] = float(alb.y), f[2] = float(alb.z), f[3] = float(h.t);
-    f[4] = float(sf.f.n.x), f[5] = float(sf.f.n.y), f[6] = float(sf.f.n.z), f[7] = 1.0f;
+    f[4] = float(nrm.x), f[5] = float(nrm.y), f[6] = float(nrm.z), f[7] = 1.0f;
   }
   if (c.nLights > 0 && !c.neeAll) {
     const Pick<T> k = risLights(c, sf, P, wo, upix, smp, dimVertex(0));

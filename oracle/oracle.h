@@ -100,9 +100,10 @@ void orc_edges_free(ubg_edges* e);
 
 /* The physically based path tracer of ubg_pathtrace (no reference
  * counterpart), restated: color = width * height * 4 (the mean over spp),
- * first = width * height * 8 or NULL (first hit of sample 0: albedo rgb, t,
- * normal xyz, 1; zeros where the camera ray misses), rays = rays traced. The
- * denoise_net field is ignored. */
+ * first = width * height * 8 or NULL (first hit of sample 0: albedo guide rgb,
+ * t, camera-space normal guide xyz, 1; zeros where the camera ray misses; a
+ * dielectric first hit's guides follow the ideal refraction to the first
+ * other surface), rays = rays traced. The denoise_net field is ignored. */
 int orc_pathtrace(const ubg_scene_desc* desc, const ubg_pt_params* params, float* color, float* first,
                   uint64_t* rays, const orc_options* opt);
 /* Its BSDF in the local frame (n = +z), FP64, over n inputs: f(wo, wi)

@@ -8,7 +8,9 @@ of tests/test_pt_oracle.py:
   mean |d| <= 1e-3 and rays traced within 0.5 % (a path whose float rounding
   flips a Fresnel or a resampling choice follows another branch);
 * the denoiser features equal their restatement from the oracle's first-hit
-  record, and the denoised colour is expm1 of the network on those features;
+  record (the guides followed through dielectrics: at 99 % of pixels within
+  1e-4), the guides through a glass sphere equal their closed form, and the
+  denoised colour is expm1 of the network on those features;
 * white furnace of a smooth dielectric sphere (exactly 1), the Fresnel plane
   head-on, ReSTIR's mean against the all-lights estimator at 256 spp;
 * argument refusals.
@@ -65,11 +67,13 @@ def test_parity_with_oracle(device, surface):
     # features: log(1 + colour), first-hit albedo, normal, t / max t
     f = g.features
     np.testing.assert_allclose(f[..., :3], np.log1p(np.maximum(g.color[..., :3], 0)), rtol=1e-6, atol=1e-7)
-    same = np.abs(first[..., 3] - 0) >= 0  # all pixels
-    np.testing.assert_allclose(f[..., 3:6], first[..., 0:3], atol=1e-6)
-    np.testing.assert_allclose(f[..., 6:9], first[..., 4:7], atol=1e-6)
     depth = first[..., 3] / first[..., 3].max()
-    np.testing.assert_allclose(f[..., 9][same], depth[same], rtol=1e-5, atol=1e-6)
+    np.testing.assert_allclose(f[..., 9], depth, rtol=1e-5, atol=1e-6)
+    # the guides: a dielectric first hit's chain re-traverses the scene, so a
+    # rounding difference can end it on another primitive at a few pixels
+    dg = np.abs(f[..., 3:9] - first[..., [0, 1, 2, 4, 5, 6]]).max(-1)
+    print(f"guides: mean|d| {dg.mean():.3g}, pixels |d| > 1e-4 {(dg > 1e-4).mean():.4f}", flush=True)
+    assert (dg > 1e-4).mean() <= 0.01 and dg.mean() <= 2e-3
 
 
 def test_denoised_colour_is_the_network_on_the_features(device, surface):
@@ -91,6 +95,29 @@ def test_denoised_colour_is_the_network_on_the_features(device, surface):
     finally:
         net.close()
         bs.close()
+
+
+def test_guides_through_a_glass_sphere(device):
+    """The closed form of tests/test_pt_oracle.py on the GPU: along the axis
+    of a smooth glass sphere (ior 1.5, colour Cs) over a diffuse plane
+    (colour Cp) the albedo guide is 0.04 + 0.96 Cs^2 Cp and the normal guide
+    the plane's, (0, 0, 1) in camera space."""
+    sc = Scene()
+    v = np.array([[-4, -4, 0], [4, -4, 0], [4, 4, 0], [-4, 4, 0]], np.float32)
+    sc.set_mesh(v, np.tile([0, 0, 1.0], (4, 1)).astype(np.float32),
+                np.tile([0.3, 0.6, 0.9, 1.0], (4, 1)).astype(np.float32), np.array([0, 1, 2, 0, 2, 3], np.uint32))
+    sc.add_spheres([[0, 0, 2]], [1.0], [[0.8, 0.5, 0.25, 1]])
+    sc.camera = Camera(position=(0, 0, 10), direction=(0, 0, -1), up=(0, 1, 0), orthographic=True, height=0.01)
+    opt = PtOptions(width=5, height=5, spp=1, bounces=1, prim_materials=[PtMaterial(DIELECTRIC, 0.0, 1.5)],
+                    mesh_materials=[PtMaterial(DIFFUSE)])
+    bs = _built(device, sc)
+    try:
+        f = pathtrace(bs, sc, opt, want_features=True).features
+    finally:
+        bs.close()
+    want = 0.04 + 0.96 * np.array([0.8, 0.5, 0.25]) ** 2 * np.array([0.3, 0.6, 0.9])
+    assert np.abs(f[..., 3:6] - want).max() <= 2e-5, f[2, 2, 3:6]
+    assert np.abs(f[..., 6:9] - [0, 0, 1]).max() <= 1e-6
 
 
 def test_white_furnace_dielectric_sphere(device):

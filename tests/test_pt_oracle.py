@@ -210,3 +210,58 @@ def test_counter_rng_uniform_and_keyed():
     # every key component changes the stream
     base = ob.pt_rnd(1, 2, 3, 4)
     assert len({base, ob.pt_rnd(9, 2, 3, 4), ob.pt_rnd(1, 9, 3, 4), ob.pt_rnd(1, 2, 9, 4), ob.pt_rnd(1, 2, 3, 9)}) == 5
+
+
+def _glass_over_plane(plane: bool, sphere_mat):
+    sc = _plane(size=4.0, rho=1.0) if plane else Scene()
+    if plane:
+        sc.colors[:, :3] = [0.3, 0.6, 0.9]
+    sc.add_spheres([[0, 0, 2]], [1.0], [[0.8, 0.5, 0.25, 1]])
+    sc.camera = Camera(position=(0, 0, 10), direction=(0, 0, -1), up=(0, 1, 0), orthographic=True, height=0.01)
+    opt = PtOptions(width=5, height=5, spp=1, bounces=1, env=(0, 0, 0), prim_materials=[sphere_mat],
+                    mesh_materials=[PtMaterial(DIFFUSE)])
+    return sc, opt
+
+
+@pytest.mark.parametrize("precision", [32, 64])
+def test_guides_through_a_glass_sphere(precision):
+    """Denoiser guides along the axis of a smooth glass sphere (ior 1.5,
+    colour Cs) over a diffuse plane (colour Cp), orthographic camera with a
+    0.01-wide image: every ray is within 0.005 of the axis, so the chain
+    crosses the sphere twice at normal incidence (no bending to first order)
+    and the albedo guide is F + (1 - F) Cs^2 Cp with F = (0.5 / 2.5)^2 = 0.04,
+    the normal guide the plane's, (0, 0, 1) in camera space. Without the
+    plane the chain leaves the scene: albedo F white, the sphere's own normal.
+    A diffuse sphere keeps its colour and normal."""
+    Cs, Cp = np.array([0.8, 0.5, 0.25]), np.array([0.3, 0.6, 0.9])
+    sc, opt = _glass_over_plane(True, PtMaterial(DIELECTRIC, 0.0, 1.5))
+    _, first, _ = ob.pathtrace(sc, opt, precision=precision, threads=2)
+    want = 0.04 + 0.96 * Cs * Cs * Cp
+    assert np.abs(first[..., 0:3] - want).max() <= 2e-5, first[2, 2, 0:3]
+    assert np.abs(first[..., 4:7] - [0, 0, 1]).max() <= 1e-6
+    assert np.allclose(first[..., 3], 7.0, atol=1e-3)  # the first hit's distance, not the plane's
+    sc, opt = _glass_over_plane(False, PtMaterial(DIELECTRIC, 0.0, 1.5))
+    _, first, _ = ob.pathtrace(sc, opt, precision=precision, threads=2)
+    assert np.abs(first[..., 0:3] - 0.04).max() <= 2e-5
+    # the sphere's normal at a ray offset (x, y) from the axis is (x, y, ...)
+    assert np.abs(first[..., 4:7] - [0, 0, 1]).max() <= 8e-3
+    sc, opt = _glass_over_plane(True, PtMaterial(DIFFUSE))
+    _, first, _ = ob.pathtrace(sc, opt, precision=precision, threads=2)
+    assert np.abs(first[..., 0:3] - Cs).max() <= 1e-6
+    assert np.abs(first[..., 4:7] - [0, 0, 1]).max() <= 8e-3
+
+
+def test_guides_camera_space_normal():
+    """A camera looking down +x with up +z: a diffuse sphere's normal guide
+    at the image centre is (0, 0, 1) (towards the eye), and the guide of a
+    point left of centre has a negative first (right-axis) component."""
+    sc = Scene()
+    sc.add_spheres([[0, 0, 0]], [1.0], [[0.5, 0.5, 0.5, 1]])
+    sc.camera = Camera(position=(-10, 0, 0), direction=(1, 0, 0), up=(0, 0, 1), orthographic=True, height=2.4)
+    opt = PtOptions(width=9, height=9, spp=1, bounces=1, prim_materials=[PtMaterial(DIFFUSE)])
+    _, first, _ = ob.pathtrace(sc, opt, precision=64, threads=2)
+    n = first[..., 4:7]
+    assert abs(n[4, 4, 2] - 1.0) <= 0.02 and abs(n[4, 4, 0]) <= 0.15 and abs(n[4, 4, 1]) <= 0.15
+    assert n[4, 1, 0] < -0.3 and n[1, 4, 1] > 0.3  # left of centre: -right; top row: +up
+    hit = first[..., 7] > 0
+    assert np.allclose(np.linalg.norm(n[hit], axis=-1), 1.0, atol=1e-6)

@@ -1,0 +1,56 @@
+"""Host logic of the path-traced denoiser's training (umbreon_amd/pt_train.py):
+the crop augmentation turns the camera-space normals with the image, and the
+training windows copy the full frame's pixel footprint."""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+import pytest
+
+from umbreon_amd.pt_train import FULL_FOVY, FULL_H, _dihedral, window_scene
+
+
+def _sphere_normals(n=33):
+    yy, xx = np.mgrid[0:n, 0:n]
+    c = (n - 1) / 2
+    x, y = (xx - c) / c, -(yy - c) / c  # columns run along +right, rows along -up
+    z = np.sqrt(np.maximum(0.0, 1.0 - x * x - y * y))
+    f = np.zeros((1, n, n, 10))
+    f[0, ..., 6], f[0, ..., 7], f[0, ..., 8] = x, y, z
+    f[0, ..., 0] = xx  # a channel that is not a normal moves without changing sign
+    return f
+
+
+@pytest.mark.parametrize("k", range(8))
+def test_dihedral_turns_normals_with_the_image(k):
+    # the image of a centred sphere is the same under every flip / quarter
+    # turn only if the normals turn with the pixels
+    f = _sphere_normals()
+    g = _dihedral(f, k, normals=True)
+    assert np.allclose(g[..., 6:9], f[..., 6:9])
+    assert g[..., 0].min() == 0 and sorted(np.unique(g[..., 0])) == sorted(np.unique(f[..., 0]))
+    # without the normal handling a reflection breaks the sphere (k != 0)
+    if k:
+        assert not np.allclose(_dihedral(f, k)[..., 6:8], f[..., 6:8])
+
+
+def test_window_scene_copies_the_full_frame_footprint():
+    sc, opt = window_scene(1234, size=512)
+    assert (opt.width, opt.height, opt.bounces, len(opt.lights)) == (512, 512, 4, 64)
+    want = 2 * math.degrees(math.atan(math.tan(math.radians(FULL_FOVY / 2)) * 512 / FULL_H))
+    assert abs(sc.camera.fovy - want) < 1e-9
+    v = np.asarray(sc.positions, np.float64)
+    c = 0.5 * (v.min(0) + v.max(0))
+    r = np.linalg.norm(v - c, axis=1).max()
+    dist = np.linalg.norm(np.asarray(sc.camera.position) - c)
+    assert abs(dist - r / math.sin(math.radians(FULL_FOVY / 2)) * 1.02) < 1e-6 * dist
+    d, up = np.asarray(sc.camera.direction), np.asarray(sc.camera.up)
+    assert abs(np.linalg.norm(d) - 1) < 1e-9 and abs(np.linalg.norm(up) - 1) < 1e-9
+    # aimed at a surface vertex: that vertex lies on the axis
+    p = np.asarray(sc.camera.position)
+    off = np.cross(v - p, d)
+    assert np.linalg.norm(off, axis=1).min() < 1e-6 * dist
+    # reproducible
+    sc2, _ = window_scene(1234, size=512)
+    assert sc2.camera == sc.camera and np.array_equal(sc2.positions, sc.positions)

@@ -1629,7 +1629,7 @@ int ubg_pathtrace(ubg_context* ctx, const ubg_scene* scene, const ubg_pt_params*
   for (int smp = 0; smp < p->spp; ++smp) {
     q.sample = smp;
     ubg::ptSample(q, scene->view, static_cast<ubg::pt::GPix*>(ctx->ptG.p), static_cast<ubg::pt::Reservoir*>(ctx->ptRes.p),
-                  acc, first, st);
+                  acc, wantFeat ? first : nullptr, st);
   }
   UBG_HIP(hipGetLastError(), "kernel launch");
   UBG_HIP(hipEventRecord(ctx->ev[1], st), "hipEventRecord");

@@ -441,6 +441,62 @@ __device__ __forceinline__ V3 directLight(const Params& p, const SceneView& s, c
   return c * W;
 }
 
+// ---------------------------------------------------------------- denoiser guides
+// Albedo and normal guides of a first hit. A dielectric is mostly seen
+// through, so from one the guides follow the ideal refraction (about the
+// interface's own normal, no microfacet sampling) through up to kGuideHops
+// dielectric interfaces to the first other surface: albedo = F * white +
+// (1 - F) * (the colours of the interfaces crossed) * that surface's colour,
+// F the first interface's Fresnel reflectance, and the normal is that
+// surface's. A chain that leaves the scene, meets total internal reflection
+// or runs out of hops keeps F * white and the first hit's normal. Normals
+// are returned in camera space (right, up, towards the eye). Guide rays are
+// not counted as traced rays: they feed the network, not the estimate.
+constexpr int kGuideHops = 4;
+__device__ __noinline__ void guides(const Params& p, const SceneView& s, const Surf& sf0, V3 P0, V3 Ng0, V3 d0,
+                                    float t0, V3& alb, V3& nrm) {
+  V3 n = sf0.f.n;
+  if (sf0.type != kMatDielectric) {
+    alb = sf0.C;
+  } else {
+    Surf sf = sf0;
+    V3 P = P0, Ng = Ng0, d = d0;
+    float t = t0, F0 = 0.0f;
+    V3 tint = mk(1.0f, 1.0f, 1.0f), behind = mk(0.0f, 0.0f, 0.0f);
+    for (int k = 0; k < kGuideHops; ++k) {
+      const float cosO = -dot(d, sf.f.n);
+      const bool outside = cosO > 0.0f;
+      const float c = fabsf(cosO);
+      const float eta = outside ? sf.ior : 1.0f / sf.ior;  // n_t / n_i
+      const float F = fresnelDielectric(c, eta);
+      if (k == 0) F0 = F;
+      const float s2 = fmaxf(0.0f, 1.0f - c * c) / (eta * eta);
+      if (s2 >= 1.0f) break;  // total internal reflection
+      const float ct = sqrtf(1.0f - s2);
+      const V3 ns = outside ? sf.f.n : neg(sf.f.n);  // normal on the incident side
+      d = normalize(d * (1.0f / eta) + ns * (c / eta - ct));
+      tint = mk(tint.x * sf.C.x, tint.y * sf.C.y, tint.z * sf.C.z);
+      const V3 o = offsetOrigin(P, Ng, d, t);
+      Hit h;
+      if (!traceClosest(s, o, d, INFINITY, h)) break;
+      V3 ng;
+      sf = surfaceAt(p, s, h, d, ng);
+      P = o + d * h.t;
+      Ng = ng;
+      t = h.t;
+      if (sf.type != kMatDielectric) {
+        behind = mk(tint.x * sf.C.x, tint.y * sf.C.y, tint.z * sf.C.z);
+        n = sf.f.n;
+        break;
+      }
+    }
+    const float T = 1.0f - F0;
+    alb = mk(F0 + T * behind.x, F0 + T * behind.y, F0 + T * behind.z);
+  }
+  nrm = mk(n.x * p.right[0] + n.y * p.right[1] + n.z * p.right[2], n.x * p.up[0] + n.y * p.up[1] + n.z * p.up[2],
+           -(n.x * p.dir[0] + n.y * p.dir[1] + n.z * p.dir[2]));
+}
+
 // ---------------------------------------------------------------- kernels
 // One camera ray; returns the rays traced (1).
 __device__ __forceinline__ uint32_t primaryPixel(const Params& p, const SceneView& s, int pix, GPix* __restrict__ gb,
@@ -479,9 +535,10 @@ __device__ __forceinline__ uint32_t primaryPixel(const Params& p, const SceneVie
   g.wo = make_float4(wo.x, wo.y, wo.z, 0.0f);
   gb[pix] = g;
   if (first && p.sample == 0) {
-    const V3 alb = sf.type == kMatDielectric ? mk(1.0f, 1.0f, 1.0f) : sf.C;
+    V3 alb, nrm;
+    guides(p, s, sf, P, Ng, d, h.t, alb, nrm);
     first[2 * pix] = make_float4(alb.x, alb.y, alb.z, h.t);
-    first[2 * pix + 1] = make_float4(sf.f.n.x, sf.f.n.y, sf.f.n.z, 1.0f);
+    first[2 * pix + 1] = make_float4(nrm.x, nrm.y, nrm.z, 1.0f);
   }
   if (p.nLights > 0 && !p.neeAll) {
     const Pick k = risLights(p, sf, P, wo, upix, smp, dimVertex(0));
