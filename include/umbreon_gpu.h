@@ -478,13 +478,15 @@ typedef struct ubg_pt_params {
   int32_t restir_spatial;            /* K, 0..16 */
   float restir_radius;               /* pixels (0 = 16) */
   int32_t nee_all_lights;            /* 1: one cone sample of every light at every vertex */
-  struct ubg_unet* denoise_net;      /* optional: the colour is the network's denoising */
+  struct ubg_unet* denoise_net;      /* optional: the colour is the network's denoising, its
+                                        prediction u decoded as c = expm1(4 u) / 32 */
   int32_t reserved[8];
 } ubg_pt_params;
 
 typedef struct ubg_pt_frame {
   float* color;           /* width * height * 4 (linear HDR, alpha 1) */
-  float* features;        /* width * height * 10 or NULL: log(1 + noisy colour) rgb,
+  float* features;        /* width * height * 10 or NULL: log(1 + 32 c) / 4 of the noisy
+                             colour c (rgb),
                              albedo guide rgb, normal guide xyz (camera space: right,
                              up, towards the eye), first-hit distance / frame max.
                              The guides are the first hit's, or for a dielectric

@@ -91,6 +91,9 @@ def load():
     lib.orc_pt_bsdf_sample.argtypes = [C.c_int32, C.c_float, C.c_float, C.c_void_p, C.c_int64, C.c_void_p,
                                        C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
     lib.orc_pt_bsdf_sample.restype = None
+    lib.orc_pt_bsdf_pdf.argtypes = [C.c_int32, C.c_float, C.c_float, C.c_void_p, C.c_int64, C.c_void_p,
+                                    C.c_void_p, C.c_void_p]
+    lib.orc_pt_bsdf_pdf.restype = None
     lib.orc_pt_rnd.argtypes = [C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32]
     lib.orc_pt_rnd.restype = C.c_float
     _lib = lib
@@ -356,6 +359,19 @@ def pt_bsdf_sample(mtype: int, roughness: float, ior: float, base, wo, u):
     lib.orc_pt_bsdf_sample(int(mtype), float(roughness), float(ior), c.ctypes.data, len(uu), a.ctypes.data,
                            uu.ctypes.data, wi.ctypes.data, w.ctypes.data, ok.ctypes.data)
     return wi, w, ok.astype(bool)
+
+
+def pt_bsdf_pdf(mtype: int, roughness: float, ior: float, base, wo, wi) -> np.ndarray:
+    """Density (solid angle, n) with which pt_bsdf_sample draws wi for wo,
+    FP64, for direction arrays wo, wi (n x 3)."""
+    lib = load()
+    c = np.ascontiguousarray(np.asarray(base, np.float32).reshape(3))
+    a, b = np.broadcast_arrays(np.asarray(wo, np.float64).reshape(-1, 3), np.asarray(wi, np.float64).reshape(-1, 3))
+    a, b = np.ascontiguousarray(a), np.ascontiguousarray(b)
+    out = np.zeros(len(b))
+    lib.orc_pt_bsdf_pdf(int(mtype), float(roughness), float(ior), c.ctypes.data, len(b), a.ctypes.data,
+                        b.ctypes.data, out.ctypes.data)
+    return out
 
 
 def pt_rnd(seed: int, pixel: int, sample: int, dim: int) -> float:

@@ -3817,6 +3817,36 @@ inline bool bsdfSample(const Surf<T>& s, V3<T> wo, T u1, T u2, T u3, V3<T>& wi, 
   return true;
 }
 
+// Density (solid angle) with which bsdfSample draws wi for wo
+// (ub_pathtrace.hip `bsdfPdf`); smooth lobes are handled by the callers.
+template <class T>
+inline T bsdfPdf(const Surf<T>& s, V3<T> wo, V3<T> wi) {
+  if (s.type == kDiffuse) return (wo.z > T(0) && wi.z > T(0)) ? wi.z / tpi<T>() : T(0);
+  if (s.type == kConductor) {
+    if (wo.z <= T(0) || wi.z <= T(0)) return T(0);
+    const V3<T> m = normalize(wo + wi);
+    return ggxG1(wo, s.alpha) * ggxD(m, s.alpha) / (T(4) * wo.z);
+  }
+  const bool outside = wo.z > T(0);
+  const V3<T> w = outside ? wo : mkv<T>(wo.x, wo.y, -wo.z);
+  const V3<T> wl = outside ? wi : mkv<T>(wi.x, wi.y, -wi.z);
+  if (w.z == T(0) || wl.z == T(0)) return T(0);
+  const T eta = outside ? s.ior : T(1) / s.ior;
+  const T g1 = ggxG1(w, s.alpha);
+  if (wl.z > T(0)) {
+    const V3<T> m = normalize(w + wl);
+    return fresnelDielectric(dot(w, m), eta) * g1 * ggxD(m, s.alpha) / (T(4) * w.z);
+  }
+  V3<T> m = w + wl * eta;
+  if (dot(m, m) == T(0)) return T(0);
+  m = normalize(m);
+  if (m.z < T(0)) m = neg(m);
+  const T cw = dot(w, m), cl = dot(wl, m);
+  if (cw <= T(0) || cl >= T(0)) return T(0);
+  const T dn = cl + cw / eta;
+  return (T(1) - fresnelDielectric(cw, eta)) * g1 * cw * ggxD(m, s.alpha) / w.z * std::fabs(cl) / (dn * dn);
+}
+
 template <class T>
 struct SLight {
   V3<T> c;
@@ -3886,6 +3916,50 @@ inline V3<T> lightContrib(const PCtx<T>& c, const Surf<T>& s, V3<T> P, V3<T> wo,
   const V3<T> fc = bsdfCos(s, toLocal(s.f, wo), toLocal(s.f, wi));
   const T g = cosL / d2;
   return mkv<T>(fc.x * L.Le.x * g, fc.y * L.Le.y * g, fc.z * L.Le.z * g);
+}
+
+// Multiple importance sampling of the lights between next-event estimation
+// and BSDF rays (ub_pathtrace.hip `misVertex` / `neeDensity` / `misNee` /
+// `misBsdf`): balance heuristic at conductor and dielectric vertices, the
+// next-event density of a direction towards light l = (1 / n, or 1 with
+// neeAll) / (2 pi (1 - cos theta_max)); a BSDF ray collects every light
+// sphere whose near side it crosses before the scene hit.
+template <class T>
+inline bool misVertex(const Surf<T>& s) { return s.type != kDiffuse; }
+template <class T>
+inline T neeDensity(const PCtx<T>& c, const SLight<T>& L, V3<T> P) {
+  const V3<T> dcv = L.c - P;
+  const T d2 = dot(dcv, dcv), r2 = L.r * L.r;
+  if (d2 <= r2) return T(0);
+  const T cosMax = std::sqrt(std::fmax(T(0), T(1) - r2 / d2));
+  const T sel = c.neeAll ? T(1) : T(1) / T(static_cast<float>(c.nLights));
+  return sel / (T(2) * tpi<T>() * (T(1) - cosMax));
+}
+template <class T>
+inline T misNee(const PCtx<T>& c, const Surf<T>& sf, V3<T> P, V3<T> wo, V3<T> wi, int li) {
+  if (!misVertex(sf)) return T(1);
+  const T pb = sf.alpha < T(kSmoothAlpha) ? T(0) : bsdfPdf(sf, toLocal(sf.f, wo), toLocal(sf.f, wi));
+  const T pn = neeDensity(c, c.lights[li], P);
+  return pn > T(0) ? pn / (pn + pb) : T(0);
+}
+template <class T>
+inline V3<T> misBsdf(const PCtx<T>& c, V3<T> P, V3<T> wi, T tHit, T pb) {
+  V3<T> sum{T(0), T(0), T(0)};
+  for (int li = 0; li < c.nLights; ++li) {
+    const SLight<T>& L = c.lights[li];
+    const V3<T> dcv = L.c - P;
+    const T b = dot(dcv, wi);
+    if (b <= T(0)) continue;
+    const T d2 = dot(dcv, dcv), r2 = L.r * L.r;
+    const T disc = b * b - (d2 - r2);
+    if (disc < T(0) || d2 <= r2) continue;
+    const T t = b - std::sqrt(disc);
+    if (!(t < tHit)) continue;
+    const T pn = neeDensity(c, L, P);
+    const T w = std::isinf(pb) ? T(1) : pb / (pb + pn);
+    sum = sum + L.Le * w;
+  }
+  return sum;
 }
 
 template <class T>
@@ -4000,8 +4074,8 @@ inline Pick<T> risLights(const PCtx<T>& c, const Surf<T>& sf, V3<T> P, V3<T> wo,
 }
 
 template <class T>
-inline V3<T> directLight(const PCtx<T>& c, const Surf<T>& sf, V3<T> P, V3<T> Ng, V3<T> wo, uint32_t pix, uint32_t smp,
-                         int b, uint64_t& nr) {
+inline V3<T> directLight(const PCtx<T>& c, const Surf<T>& sf, V3<T> P, V3<T> Ng, V3<T> wo, T tIn, uint32_t pix,
+                         uint32_t smp, int b, uint64_t& nr) {
   V3<T> sum{T(0), T(0), T(0)};
   if (c.nLights <= 0) return sum;
   if (c.neeAll) {
@@ -4016,8 +4090,8 @@ inline V3<T> directLight(const PCtx<T>& c, const Surf<T>& sf, V3<T> P, V3<T> Ng,
       if (!(lum(ct) > T(0))) continue;
       const V3<T> wi = (Q - P) * (T(1) / dist);
       ++nr;
-      if (traceAny(c, offsetOrigin(P, Ng, wi, T(0)), wi, dist * T(0.9999f))) continue;
-      sum = sum + ct * (T(1) / pdfA);
+      if (traceAny(c, offsetOrigin(P, Ng, wi, tIn), wi, dist * T(0.9999f))) continue;
+      sum = sum + ct * (misNee(c, sf, P, wo, wi, li) / pdfA);
     }
     return sum;
   }
@@ -4027,9 +4101,9 @@ inline V3<T> directLight(const PCtx<T>& c, const Surf<T>& sf, V3<T> P, V3<T> Ng,
   const V3<T> ct = lightContrib(c, sf, P, wo, k.Q, k.li, dist);
   const V3<T> wi = (k.Q - P) * (T(1) / dist);
   ++nr;
-  if (traceAny(c, offsetOrigin(P, Ng, wi, T(0)), wi, dist * T(0.9999f))) return sum;
+  if (traceAny(c, offsetOrigin(P, Ng, wi, tIn), wi, dist * T(0.9999f))) return sum;
   const T Wt = k.wsum / (T(static_cast<float>(c.M)) * k.target);
-  return ct * Wt;
+  return ct * (Wt * misNee(c, sf, P, wo, wi, k.li));
 }
 
 // Denoiser guides of a first hit (ub_pathtrace.hip `guides`): a dielectric
@@ -4162,7 +4236,7 @@ static V3<T> shadePixel(const PCtx<T>& c, int pix, uint32_t smp, const std::vect
   V3<T> L{T(0), T(0), T(0)};
   if (c.nLights > 0) {
     if (c.neeAll) {
-      L = directLight(c, sf0, P0, Ng0, wo0, upix, smp, 0, nr);
+      L = directLight(c, sf0, P0, Ng0, wo0, g.t, upix, smp, 0, nr);
     } else {
       const int px = pix % c.W, py = pix / c.W;
       int src[17];
@@ -4208,7 +4282,8 @@ static V3<T> shadePixel(const PCtx<T>& c, int pix, uint32_t smp, const std::vect
         const V3<T> ct = lightContrib(c, sf0, P0, wo0, Q, li, dist);
         const V3<T> wi = (Q - P0) * (T(1) / dist);
         ++nr;
-        if (!traceAny(c, offsetOrigin(P0, Ng0, wi, T(0)), wi, dist * T(0.9999f))) L = ct * (wsum / tsel);
+        if (!traceAny(c, offsetOrigin(P0, Ng0, wi, g.t), wi, dist * T(0.9999f)))
+          L = ct * ((wsum / tsel) * misNee(c, sf0, P0, wo0, wi, li));
       }
     }
   }
@@ -4228,7 +4303,13 @@ static V3<T> shadePixel(const PCtx<T>& c, int pix, uint32_t smp, const std::vect
     const V3<T> o = offsetOrigin(P, Ng, wi, tPrev);
     Hit<T> h;
     ++nr;
-    if (!traceClosest(c, o, wi, h)) {
+    const bool hit = traceClosest(c, o, wi, h);
+    if (c.nLights > 0 && misVertex(sf)) {
+      const T pb = sf.alpha < T(kSmoothAlpha) ? std::numeric_limits<T>::infinity()
+                                              : bsdfPdf(sf, toLocal(sf.f, wo), wiL);
+      L = L + mul(thr, misBsdf(c, P, wi, hit ? h.t : std::numeric_limits<T>::infinity(), pb));
+    }
+    if (!hit) {
       L = L + mul(thr, c.env);
       break;
     }
@@ -4239,7 +4320,7 @@ static V3<T> shadePixel(const PCtx<T>& c, int pix, uint32_t smp, const std::vect
     Ng = ng;
     wo = neg(wi);
     tPrev = h.t;
-    const V3<T> Ld = directLight(c, sf, P, Ng, wo, upix, smp, b, nr);
+    const V3<T> Ld = directLight(c, sf, P, Ng, wo, tPrev, upix, smp, b, nr);
     L = L + mul(thr, Ld);
   }
   return L;
@@ -4378,6 +4459,16 @@ void orc_pt_bsdf_sample(int32_t type, float roughness, float ior, const float* C
     ok[i] = bsdfSample(s, orc::V3<double>{a[0], a[1], a[2]}, u[3 * i], u[3 * i + 1], u[3 * i + 2], w, g) ? 1 : 0;
     wi[3 * i] = w.x, wi[3 * i + 1] = w.y, wi[3 * i + 2] = w.z;
     weight[3 * i] = g.x, weight[3 * i + 1] = g.y, weight[3 * i + 2] = g.z;
+  }
+}
+void orc_pt_bsdf_pdf(int32_t type, float roughness, float ior, const float* C, int64_t n, const double* wo,
+                     const double* wi, double* out) {
+  using namespace orc::pt;
+  const Surf<double> s = surfFromArgs<double>(type, roughness, ior, C);
+  for (int64_t i = 0; i < n; ++i) {
+    const double* a = wo + 3 * i;
+    const double* b = wi + 3 * i;
+    out[i] = bsdfPdf(s, orc::V3<double>{a[0], a[1], a[2]}, orc::V3<double>{b[0], b[1], b[2]});
   }
 }
 float orc_pt_rnd(uint32_t seed, uint32_t pixel, uint32_t sample, uint32_t dim) {

@@ -115,6 +115,10 @@ void orc_pt_bsdf_eval(int32_t type, float roughness, float ior, const float* C, 
 void orc_pt_bsdf_sample(int32_t type, float roughness, float ior, const float* C, int64_t n, const double* wo,
                         const double* u, double* wi, double* weight, uint8_t* ok);
 /* Its counter-based uniform for (seed, pixel, sample, dimension). */
+/* Its BSDF sampling density (solid angle) of wi for wo, FP64, over n pairs
+ * (0 where bsdf_sample never draws wi; smooth lobes are not evaluated). */
+void orc_pt_bsdf_pdf(int32_t type, float roughness, float ior, const float* C, int64_t n, const double* wo,
+                     const double* wi, double* out);
 float orc_pt_rnd(uint32_t seed, uint32_t pixel, uint32_t sample, uint32_t dim);
 
 /* Post-process helpers (fog.cpp:7-36, image_ops.cpp:9-79). */

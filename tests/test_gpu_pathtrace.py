@@ -10,9 +10,12 @@ of tests/test_pt_oracle.py:
 * the denoiser features equal their restatement from the oracle's first-hit
   record (the guides followed through dielectrics: at 99 % of pixels within
   1e-4), the guides through a glass sphere equal their closed form, and the
-  denoised colour is expm1 of the network on those features;
+  denoised colour is the network's prediction on those features decoded
+  (expm1(4 u) / 32);
 * white furnace of a smooth dielectric sphere (exactly 1), the Fresnel plane
   head-on, ReSTIR's mean against the all-lights estimator at 256 spp;
+* multiple importance sampling of the lights: a glossy plane's direct light
+  equals the quadrature over the light's cone;
 * argument refusals.
 """
 from __future__ import annotations
@@ -64,9 +67,11 @@ def test_parity_with_oracle(device, surface):
     assert (d.max(-1) > 1e-3).mean() <= 0.02
     assert abs(g.rays / rays - 1) <= 5e-3
     assert np.all(g.color[..., 3] == 1.0)
-    # features: log(1 + colour), first-hit albedo, normal, t / max t
+    # features: encoded colour, albedo and normal guides, t / max t
     f = g.features
-    np.testing.assert_allclose(f[..., :3], np.log1p(np.maximum(g.color[..., :3], 0)), rtol=1e-6, atol=1e-7)
+    # the colour encoded as log(1 + 32 c) / 4
+    np.testing.assert_allclose(f[..., :3], 0.25 * np.log1p(32 * np.maximum(g.color[..., :3], 0)), rtol=1e-6,
+                               atol=1e-7)
     depth = first[..., 3] / first[..., 3].max()
     np.testing.assert_allclose(f[..., 9], depth, rtol=1e-5, atol=1e-6)
     # the guides: a dielectric first hit's chain re-traverses the scene, so a
@@ -89,7 +94,7 @@ def test_denoised_colour_is_the_network_on_the_features(device, surface):
         Hp, Wp = -(-H // m) * m, -(-W // m) * m
         feat = np.pad(plain.features, ((0, Hp - H), (0, Wp - W), (0, 0)), mode="edge")
         pred = net.infer(feat[None])[0, :H, :W]
-        want = np.maximum(np.expm1(pred), 0.0)
+        want = np.maximum(np.expm1(4 * pred) / 32, 0.0)
         np.testing.assert_allclose(den.color[..., :3], want, rtol=1e-5, atol=1e-6)
         assert den.denoise_seconds > 0
     finally:
@@ -118,6 +123,26 @@ def test_guides_through_a_glass_sphere(device):
     want = 0.04 + 0.96 * np.array([0.8, 0.5, 0.25]) ** 2 * np.array([0.3, 0.6, 0.9])
     assert np.abs(f[..., 3:6] - want).max() <= 2e-5, f[2, 2, 3:6]
     assert np.abs(f[..., 6:9] - [0, 0, 1]).max() <= 1e-6
+
+
+@pytest.mark.parametrize("mat", [PtMaterial(CONDUCTOR, 0.12), PtMaterial(CONDUCTOR, 0.5)])
+def test_mis_direct_light_matches_quadrature(device, mat):
+    """The lights reached by next-event estimation and by BSDF rays (balance
+    heuristic) on the GPU: direct light of one spherical light on a glossy
+    conductor plane equals the quadrature of f |cos| Le over the light's cone
+    (tests/test_pt_oracle.py's case), and the image equals the oracle's."""
+    from test_pt_oracle import _cone_quadrature, _glossy_under_light
+    sc, opt, wo, c = _glossy_under_light(mat, spp=1024)
+    bs = _built(device, sc)
+    try:
+        g = pathtrace(bs, sc, opt)
+    finally:
+        bs.close()
+    want = _cone_quadrature(mat, wo, c)
+    got = g.color[..., :3].reshape(-1, 3).mean(0)
+    assert np.all(np.abs(got / want - 1) <= 0.02), (got, want)
+    col, _, _ = ob.pathtrace(sc, opt, precision=32, threads=4)
+    assert np.abs(g.color[..., :3] - col[..., :3]).max() <= 1e-4 * max(1.0, float(col.max()))
 
 
 def test_white_furnace_dielectric_sphere(device):

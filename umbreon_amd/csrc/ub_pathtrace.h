@@ -16,6 +16,12 @@ constexpr int kMatDielectric = 2;
 constexpr int kMaxSpatial = 16;   // ReSTIR neighbours
 constexpr int kMaxCandidates = 12;
 constexpr int kPtFeatures = 10;
+// Colour encoding of the denoiser's input and output: u = log(1 + 32 c) / 4.
+// Its slope over c follows the sRGB transfer's within a factor of ~1.5 from
+// c = 0.01 to 1 (the images are compared in sRGB), where log(1 + c) would
+// weigh the dark colours 5-10 times less than the comparison does.
+constexpr float kColourGain = 32.0f;
+constexpr float kColourScale = 0.25f;
 
 struct Light {      // a sphere of constant radiance
   float c[3];

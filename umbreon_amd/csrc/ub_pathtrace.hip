@@ -15,9 +15,11 @@
 // (each candidate's target re-evaluated at every contributing pixel for its
 // resampling weight), traces the shadow ray, then continues the path through the
 // remaining vertices with per-vertex RIS next-event estimation; the
-// radiance adds into a per-pixel accumulator in sample order. Lights are not
-// intersectable: they are reached by next-event estimation only, and rays
-// that leave the scene return the environment radiance.
+// radiance adds into a per-pixel accumulator in sample order. At glossy
+// vertices the lights are also reached by the BSDF rays, combined with the
+// next-event estimates by multiple importance sampling (balance heuristic);
+// camera rays do not see the lights, and rays that leave the scene return
+// the environment radiance.
 //
 // Random numbers are a pure function of (seed, pixel, sample, dimension)
 // (hash32 below), so an image does not depend on launch shapes.
@@ -223,6 +225,38 @@ __device__ __forceinline__ bool bsdfSample(const Surf& s, V3 wo, float u1, float
   return true;
 }
 
+// Density (solid angle, local frame) with which bsdfSample draws wi for wo;
+// 0 where it never does. Smooth (delta) lobes are not evaluated here: a
+// light reached through one takes the whole weight (see misBsdf).
+__device__ __forceinline__ float bsdfPdf(const Surf& s, V3 wo, V3 wi) {
+  if (s.type == kMatDiffuse) return (wo.z > 0.0f && wi.z > 0.0f) ? wi.z / kPi : 0.0f;
+  if (s.type == kMatConductor) {
+    if (wo.z <= 0.0f || wi.z <= 0.0f) return 0.0f;
+    const V3 m = normalize(wo + wi);
+    return ggxG1(wo, s.alpha) * ggxD(m, s.alpha) / (4.0f * wo.z);
+  }
+  // dielectric, in the frame flipped to the side of wo (as bsdfSample)
+  const bool outside = wo.z > 0.0f;
+  const V3 w = outside ? wo : mk(wo.x, wo.y, -wo.z);
+  const V3 wl = outside ? wi : mk(wi.x, wi.y, -wi.z);
+  if (w.z == 0.0f || wl.z == 0.0f) return 0.0f;
+  const float eta = outside ? s.ior : 1.0f / s.ior;  // n_t / n_i
+  const float g1 = ggxG1(w, s.alpha);
+  if (wl.z > 0.0f) {  // reflection: the visible-normal density / (4 w.m), times F
+    const V3 m = normalize(w + wl);
+    return fresnelDielectric(dot(w, m), eta) * g1 * ggxD(m, s.alpha) / (4.0f * w.z);
+  }
+  // refraction: m along -(w + eta wl); Jacobian |wl.m| / (wl.m + w.m / eta)^2
+  V3 m = w + wl * eta;
+  if (dot(m, m) == 0.0f) return 0.0f;
+  m = normalize(m);
+  if (m.z < 0.0f) m = neg(m);
+  const float c = dot(w, m), cl = dot(wl, m);
+  if (c <= 0.0f || cl >= 0.0f) return 0.0f;
+  const float dn = cl + c / eta;
+  return (1.0f - fresnelDielectric(c, eta)) * g1 * c * ggxD(m, s.alpha) / w.z * fabsf(cl) / (dn * dn);
+}
+
 // ---------------------------------------------------------------- lights
 // Uniform direction in the cone the light subtends from P; returns the hit
 // point on the sphere and the area-measure density (light choice included).
@@ -252,6 +286,55 @@ __device__ __forceinline__ bool lightSample(const Light& L, V3 P, float u1, floa
   return pdfA > 0.0f;
 }
 // Unshadowed contribution of light point Q (on light li) at the surface:
+// Multiple importance sampling of the lights (balance heuristic, Veach
+// 1997) between next-event estimation and BSDF rays, at glossy vertices
+// (conductors and dielectrics; a diffuse vertex keeps next-event estimation
+// alone, a smooth one BSDF rays alone, since next-event estimation cannot
+// reach a delta lobe). Lights are mutually transparent emitters that camera
+// rays do not see; a BSDF ray collects every light sphere whose near side it
+// crosses before the scene hit. The next-event density of a direction
+// towards light l is that of its candidates -- a uniform light choice
+// (1 / n, or 1 with neeAll) times the uniform cone density 1 / (2 pi (1 -
+// cos theta_max)) -- whatever RIS / ReSTIR then selects: the weights sum to
+// one for every path, so the estimate stays unbiased.
+__device__ __forceinline__ bool misVertex(const Surf& s) { return s.type != kMatDiffuse; }
+__device__ __forceinline__ float neeDensity(const Params& p, const Light& L, V3 P) {
+  const V3 dcv = mk(L.c[0], L.c[1], L.c[2]) - P;
+  const float d2 = dot(dcv, dcv), r2 = L.r * L.r;
+  if (d2 <= r2) return 0.0f;
+  const float cosMax = sqrtf(fmaxf(0.0f, 1.0f - r2 / d2));
+  const float sel = p.neeAll ? 1.0f : 1.0f / static_cast<float>(p.nLights);
+  return sel / (2.0f * kPi * (1.0f - cosMax));
+}
+// Weight of a next-event sample towards light li in direction wi (world).
+__device__ __forceinline__ float misNee(const Params& p, const Surf& sf, V3 P, V3 wo, V3 wi, int li) {
+  if (!misVertex(sf)) return 1.0f;
+  const float pb = sf.alpha < kSmoothAlpha ? 0.0f : bsdfPdf(sf, toLocal(sf.f, wo), toLocal(sf.f, wi));
+  const float pn = neeDensity(p, p.lights[li], P);
+  return pn > 0.0f ? pn / (pn + pb) : 0.0f;
+}
+// Light a BSDF ray from vertex P in direction wi (world) collects before the
+// scene hit at tHit, with its weights; pb = the ray's BSDF density (+inf for
+// a delta lobe).
+__device__ __forceinline__ V3 misBsdf(const Params& p, V3 P, V3 wi, float tHit, float pb) {
+  V3 sum = mk(0.0f, 0.0f, 0.0f);
+  for (int li = 0; li < p.nLights; ++li) {
+    const Light& L = p.lights[li];
+    const V3 dcv = mk(L.c[0], L.c[1], L.c[2]) - P;
+    const float b = dot(dcv, wi);
+    if (b <= 0.0f) continue;
+    const float d2 = dot(dcv, dcv), r2 = L.r * L.r;
+    const float disc = b * b - (d2 - r2);
+    if (disc < 0.0f || d2 <= r2) continue;
+    const float t = b - sqrtf(disc);
+    if (!(t < tHit)) continue;
+    const float pn = neeDensity(p, L, P);
+    const float w = pb == INFINITY ? 1.0f : pb / (pb + pn);
+    sum = sum + mk(L.Le[0] * w, L.Le[1] * w, L.Le[2] * w);
+  }
+  return sum;
+}
+
 // f |cos| Le |cos_l| / dist^2 (area measure); target = its luminance.
 __device__ __forceinline__ V3 lightContrib(const Params& p, const Surf& s, V3 P, V3 wo, V3 Q, int li, float& dist) {
   const Light& L = p.lights[li];
@@ -409,9 +492,11 @@ __device__ __forceinline__ Pick risLights(const Params& p, const Surf& sf, V3 P,
 }
 
 // Direct light at a vertex by RIS NEE (or, with neeAll, one cone sample of
-// every light), shadow rays included.
+// every light), shadow rays included; tIn = the distance along the ray that
+// found the vertex (it scales the shadow rays' origin offset, like every
+// other ray's).
 __device__ __forceinline__ V3 directLight(const Params& p, const SceneView& s, const Surf& sf, V3 P, V3 Ng, V3 wo,
-                                          uint32_t pix, uint32_t smp, int b, uint32_t& nr) {
+                                          float tIn, uint32_t pix, uint32_t smp, int b, uint32_t& nr) {
   V3 sum = mk(0.0f, 0.0f, 0.0f);
   if (p.nLights <= 0) return sum;
   if (p.neeAll) {
@@ -425,8 +510,8 @@ __device__ __forceinline__ V3 directLight(const Params& p, const SceneView& s, c
       if (!(lum(c) > 0.0f)) continue;
       const V3 wi = (Q - P) * (1.0f / dist);
       ++nr;
-      if (traceAny(s, offsetOrigin(P, Ng, wi, 0.0f), wi, dist * 0.9999f)) continue;
-      sum = sum + c * (1.0f / pdfA);
+      if (traceAny(s, offsetOrigin(P, Ng, wi, tIn), wi, dist * 0.9999f)) continue;
+      sum = sum + c * (misNee(p, sf, P, wo, wi, li) / pdfA);
     }
     return sum;
   }
@@ -436,9 +521,9 @@ __device__ __forceinline__ V3 directLight(const Params& p, const SceneView& s, c
   const V3 c = lightContrib(p, sf, P, wo, k.Q, k.li, dist);
   const V3 wi = (k.Q - P) * (1.0f / dist);
   ++nr;
-  if (traceAny(s, offsetOrigin(P, Ng, wi, 0.0f), wi, dist * 0.9999f)) return sum;
+  if (traceAny(s, offsetOrigin(P, Ng, wi, tIn), wi, dist * 0.9999f)) return sum;
   const float W = k.wsum / (static_cast<float>(p.restirM) * k.target);
-  return c * W;
+  return c * (W * misNee(p, sf, P, wo, wi, k.li));
 }
 
 // ---------------------------------------------------------------- denoiser guides
@@ -583,7 +668,7 @@ __device__ __forceinline__ uint32_t shadePixel(const Params& p, const SceneView&
   // ---- first vertex: ReSTIR over the own and K neighbours' reservoirs
   if (p.nLights > 0) {
     if (p.neeAll) {
-      L = directLight(p, s, sf0, P0, Ng0, wo0, upix, smp, 0, nr);
+      L = directLight(p, s, sf0, P0, Ng0, wo0, g.P.w, upix, smp, 0, nr);
     } else {
       const int px = pix % p.W, py = pix / p.W;
       int src[1 + kMaxSpatial];
@@ -637,7 +722,8 @@ __device__ __forceinline__ uint32_t shadePixel(const Params& p, const SceneView&
         const V3 c = lightContrib(p, sf0, P0, wo0, Q, li, dist);
         const V3 wi = (Q - P0) * (1.0f / dist);
         ++nr;
-        if (!traceAny(s, offsetOrigin(P0, Ng0, wi, 0.0f), wi, dist * 0.9999f)) L = c * (wsum / tsel);
+        if (!traceAny(s, offsetOrigin(P0, Ng0, wi, g.P.w), wi, dist * 0.9999f))
+          L = c * ((wsum / tsel) * misNee(p, sf0, P0, wo0, wi, li));
       }
     }
   }
@@ -660,7 +746,13 @@ __device__ __forceinline__ uint32_t shadePixel(const Params& p, const SceneView&
     const V3 o = offsetOrigin(P, Ng, wi, tPrev);
     Hit h;
     ++nr;
-    if (!traceClosest(s, o, wi, INFINITY, h)) {
+    const bool hit = traceClosest(s, o, wi, INFINITY, h);
+    if (p.nLights > 0 && misVertex(sf)) {
+      const float pb = sf.alpha < kSmoothAlpha ? INFINITY : bsdfPdf(sf, toLocal(sf.f, wo), wiL);
+      const V3 Le = misBsdf(p, P, wi, hit ? h.t : INFINITY, pb);
+      L = L + mk(thr.x * Le.x, thr.y * Le.y, thr.z * Le.z);
+    }
+    if (!hit) {
       L = L + mk(thr.x * p.env[0], thr.y * p.env[1], thr.z * p.env[2]);
       break;
     }
@@ -671,7 +763,7 @@ __device__ __forceinline__ uint32_t shadePixel(const Params& p, const SceneView&
     Ng = ng;
     wo = neg(wi);
     tPrev = h.t;
-    const V3 Ld = directLight(p, s, sf, P, Ng, wo, upix, smp, b, nr);
+    const V3 Ld = directLight(p, s, sf, P, Ng, wo, tPrev, upix, smp, b, nr);
     L = L + mk(thr.x * Ld.x, thr.y * Ld.y, thr.z * Ld.z);
   }
   float4 a = acc[pix];
@@ -685,8 +777,8 @@ __global__ __launch_bounds__(kBlock) void k_pt_shade(Params p, SceneView s, cons
   countRays(p.rays, nr);
 }
 
-// Mean over the samples, and the denoiser features: log(1 + colour), albedo,
-// normal, first-hit distance / its frame maximum (maxBits: that maximum as
+// Mean over the samples, and the denoiser features: the encoded colour
+// log(1 + 32 c) / 4, albedo and normal guides, first-hit distance / its frame maximum (maxBits: that maximum as
 // float bits, from k_pt_depth_max).
 __global__ void k_pt_depth_max(int n, const float4* __restrict__ first, unsigned int* __restrict__ maxBits) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -705,9 +797,9 @@ __global__ void k_pt_resolve(int W, int H, int Wp, int Hp, int spp, const float4
     const float4 f0 = first[2 * j], f1 = first[2 * j + 1];
     const float mx = __uint_as_float(*maxBits);
     float* o = feat + static_cast<size_t>(i) * kPtFeatures;
-    o[0] = log1pf(fmaxf(0.0f, a.x * inv));
-    o[1] = log1pf(fmaxf(0.0f, a.y * inv));
-    o[2] = log1pf(fmaxf(0.0f, a.z * inv));
+    o[0] = kColourScale * log1pf(kColourGain * fmaxf(0.0f, a.x * inv));
+    o[1] = kColourScale * log1pf(kColourGain * fmaxf(0.0f, a.y * inv));
+    o[2] = kColourScale * log1pf(kColourGain * fmaxf(0.0f, a.z * inv));
     o[3] = f0.x;
     o[4] = f0.y;
     o[5] = f0.z;
@@ -726,13 +818,15 @@ __global__ void k_pt_resolve(int W, int H, int Wp, int Hp, int spp, const float4
   const float4 c = acc[i];
   color[i] = make_float4(c.x * inv, c.y * inv, c.z * inv, 1.0f);
 }
-// The denoised colour: expm1 of the network's log(1 + colour) prediction.
+// The denoised colour: the network's prediction decoded, c = expm1(4 u) / 32.
 __global__ void k_pt_apply_net(int W, int H, int Wp, const float* __restrict__ pred, float4* __restrict__ color) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= W * H) return;
   const int x = i % W, y = i / W;
   const float* q = pred + (static_cast<size_t>(y) * Wp + x) * 3;
-  color[i] = make_float4(fmaxf(0.0f, expm1f(q[0])), fmaxf(0.0f, expm1f(q[1])), fmaxf(0.0f, expm1f(q[2])), 1.0f);
+  const float k = 1.0f / kColourScale, g = 1.0f / kColourGain;
+  color[i] = make_float4(fmaxf(0.0f, expm1f(q[0] * k) * g), fmaxf(0.0f, expm1f(q[1] * k) * g),
+                         fmaxf(0.0f, expm1f(q[2] * k) * g), 1.0f);
 }
 __global__ void k_pt_zero(float4* __restrict__ a, int n, unsigned int* __restrict__ maxBits,
                           unsigned long long* __restrict__ rays) {

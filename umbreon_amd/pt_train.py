@@ -14,9 +14,11 @@ full-frame camera rule -- the 1920x1080, 40-degree framing of the molecule,
 from a random side, narrowed to the window's share of that frame and aimed
 at a random point of the surface -- so a pixel covers about what it covers
 in the full frame. Noisy inputs are the path tracer's 10 denoiser features
-at 1, 2, 4 and 8 samples per pixel; the target is log(1 + colour) at 4096
-samples. The network (5 levels, 32-128 channels, bf16 MFMA convolutions with
-fp32 accumulation) predicts log(1 + colour) from the features.
+at 1, 2, 4 and 8 samples per pixel; the target is the encoded colour
+log(1 + 32 c) / 4 at 4096 samples. The network (5 levels, 32-128 channels,
+bf16 MFMA convolutions with fp32 accumulation) predicts the encoded colour
+from the features; the L1 loss on that encoding weighs dark colours about as
+the sRGB comparison does.
 
     python -m umbreon_amd.pt_train --gen-seconds 600 --train-seconds 420 --out w.npz [--init w0.npz]
     python -m umbreon_amd.pt_train --evaluate umbreon_amd/weights/pt_unet.npz
@@ -50,6 +52,12 @@ WINDOW = 512
 NOISY_SPP = (1, 2, 4, 8)
 TARGET_SPP = 4096
 WEIGHTS = Path(__file__).resolve().parent / "weights" / "pt_unet.npz"
+
+
+def encode_colour(c: np.ndarray) -> np.ndarray:
+    """The path tracer's colour encoding of the denoiser's features and
+    prediction, u = log(1 + 32 c) / 4 (sRGB-like weighting of dark colours)."""
+    return (0.25 * np.log1p(32.0 * np.maximum(np.asarray(c, np.float64), 0.0))).astype(np.float32)
 
 
 def _say(msg: str) -> None:
@@ -102,7 +110,7 @@ def make_pairs(dev: Device, seeds: Sequence[int], size: int = WINDOW, target_spp
 
     Returns (features (n * len(noisy_spp), size, size, 10) float16, owner
     (n * len(noisy_spp),) index of each input's target, targets (n, size,
-    size, 3) float32 log(1 + colour)). With a scene_pool the next molecules
+    size, 3) float32 encoded colour). With a scene_pool the next molecules
     are generated on the host while the device renders."""
     log = log or _say
     feats, owner, tgts = [], [], []
@@ -120,7 +128,7 @@ def make_pairs(dev: Device, seeds: Sequence[int], size: int = WINDOW, target_spp
         bs = BuiltScene(dev, sc, capi.BVH8_GPU_BUILD)
         try:
             t = pathtrace(bs, sc, replace(opt, spp=target_spp, seed=(7919 * s + 1) & 0xFFFFFFFF))
-            tgts.append(np.log1p(np.maximum(t.color[..., :3], 0.0)).astype(np.float32))
+            tgts.append(encode_colour(t.color[..., :3]))
             for k, n in enumerate(noisy_spp):
                 f = pathtrace(bs, sc, replace(opt, spp=n, seed=(7919 * s + 11 + k) & 0xFFFFFFFF),
                               want_features=True)
