@@ -169,36 +169,21 @@ def calib_copy(mib: int) -> None:
 
 
 def time_to_40db(dev) -> dict:
-    """The metric's second half: wall time per frame for the GI frame to reach
-    40 dB PSNR (sRGB, the whole frame) against a 4096-sample render, for the
-    plain path tracer (1-256 samples per pixel) and for the learned denoiser
-    (1-8 samples, umbreon_amd/weights/gi_unet.npz), on a held-out molecular
-    surface at 1024 x 1024 with pt2 GI (2 bounces, hard shadows)."""
-    from umbreon_amd.denoise_train import evaluate, load
-    weights = Path(__file__).resolve().parent / "umbreon_amd" / "weights" / "gi_unet.npz"
-    net = load(dev, str(weights))
+    """The metric's second half on the path-traced GI workload: the 40,000-
+    atom molecular surface as a GGX dielectric over GGX conductor / dielectric
+    CPK spheres, 64 spherical area lights (RIS + ReSTIR, MIS with BSDF rays),
+    4 bounces, 1920x1080. Time for a frame to reach 40 dB PSNR (sRGB, the
+    whole frame) against a 4096-sample render: the plain path tracer at
+    1, 2, 4, ... samples per pixel and the learned denoiser
+    (umbreon_amd/weights/pt_unet.npz) on the same frames, each until it
+    reaches 40 dB."""
+    from umbreon_amd.denoise_train import load
+    from umbreon_amd.pt_train import WEIGHTS, evaluate_full
+    net = load(dev, str(WEIGHTS))
     try:
-        seed = 100001  # a molecular surface the training never saw
-        r = evaluate(dev, net, [seed], size=1024, ref_spp=4096, log=log)["scenes"][0]
+        return evaluate_full(dev, net, log=log)
     finally:
         net.close()
-    def pts(spps, rows):
-        return {str(n): [round(p[0], 3), round(p[1], 2), round(p[2], 3)] for n, p in zip(spps, rows)}
-
-    def rnd(v):
-        return None if v is None else round(v, 3)
-
-    tp, tl = r["t40_plain"], r["t40_learned"]
-    dp, dl = r["t40_plain_device"], r["t40_learned_device"]
-    return {"workload": "held-out molecular surface (seed 100001), 1024x1024 ss1, pt2 GI 2 bounces, "
-                        "PSNR (sRGB, whole frame) vs 4096 spp",
-            "plain_ms": rnd(tp), "learned_ms": rnd(tl), "atrous_ms": rnd(r["t40_atrous"]),
-            "speedup": None if (tp is None or tl is None) else round(tp / tl, 2),
-            "plain_device_ms": rnd(dp), "learned_device_ms": rnd(dl), "atrous_device_ms": rnd(r["t40_atrous_device"]),
-            "speedup_device": None if (dp is None or dl is None) else round(dp / dl, 2),
-            "per_spp_wall_ms_db_device_ms": {"plain": pts((1, 2, 4, 8, 16, 32, 64, 128, 256), r["plain"]),
-                                              "learned": pts((1, 2, 4, 8), r["learned"]),
-                                              "atrous": pts((1, 2, 4, 8), r["atrous"])}}
 
 
 def serving(dev, n_requests: int = 256, rate: float = 40.0) -> dict:

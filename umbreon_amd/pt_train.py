@@ -234,12 +234,18 @@ def holdout_psnr(dev: Device, net: Unet, seeds: Sequence[int], size: int = WINDO
     return out
 
 
+# samples per pixel of the evaluation's frames: steps of 2^(1/2) (rounded),
+# so a crossing is timed within a factor 1.5 of its sample count
+SPP_STEPS = (1, 2, 3, 4, 6, 8, 12, 16, 24, 32, 48, 64, 96, 128, 192, 256, 384, 512, 768, 1024, 1536, 2048)
+
+
 def evaluate_full(dev: Device, net: Optional[Unet], ref_spp: int = TARGET_SPP, max_plain_spp: int = 2048,
-                  den_spp: Sequence[int] = NOISY_SPP, repeats: int = 3, log=None) -> dict:
+                  den_spp: Sequence[int] = SPP_STEPS, repeats: int = 3, log=None) -> dict:
     """Time to 40 dB on the full workload (1920x1080, seed 0, a molecule the
     training never saw): PSNR (sRGB, whole frame) against a ref_spp render;
-    the plain path tracer at 1, 2, 4, ... samples until it reaches 40 dB (at
-    most max_plain_spp), the learned denoiser at den_spp. Each point is
+    the plain path tracer at 1, 2, 3, 4, 6, 8, ... samples until it reaches
+    40 dB (at most max_plain_spp), the learned denoiser at den_spp until it
+    does; the time each first reaches 30, 35 and 40 dB. Each point is
     (wall ms of the frame call with the colour copied to the host, dB, device
     ms = sample passes + resolve + network)."""
     log = log or _say
@@ -256,7 +262,7 @@ def evaluate_full(dev: Device, net: Optional[Unet], ref_spp: int = TARGET_SPP, m
         log(f"reference: {ref_spp} spp in {time.perf_counter() - t:.1f} s")
 
         def timed(o, **kw):
-            best, dev_ms, f = None, None, None
+            best, dev_ms, den_ms, f = None, None, None, None
             for _ in range(repeats if o.spp <= 64 else 1):
                 t = time.perf_counter()
                 f = pathtrace(bs, sc, o, **kw)
@@ -264,23 +270,29 @@ def evaluate_full(dev: Device, net: Optional[Unet], ref_spp: int = TARGET_SPP, m
                 d = (f.trace_seconds + f.denoise_seconds) * 1e3
                 best = ms if best is None else min(best, ms)
                 dev_ms = d if dev_ms is None else min(dev_ms, d)
-            return (best, psnr_srgb(f.color, ref), dev_ms)
+                den_ms = f.denoise_seconds * 1e3 if den_ms is None else min(den_ms, f.denoise_seconds * 1e3)
+            return (best, psnr_srgb(f.color, ref), dev_ms, den_ms)
 
         plain, plain_spp = [], []
-        n = 1
-        while n <= max_plain_spp:
+        for n in SPP_STEPS:
+            if n > max_plain_spp:
+                break
             plain.append(timed(replace(opt, spp=n, seed=11)))
             plain_spp.append(n)
             log(f"plain {n} spp: {plain[-1][1]:.2f} dB, {plain[-1][0]:.1f} ms wall, {plain[-1][2]:.1f} ms device")
             if plain[-1][1] >= 40.0:
                 break
-            n *= 2
-        learned = []
+        learned, den_done = [], []
         if net is not None:
             for n in den_spp:
+                if n > plain_spp[-1]:
+                    break  # no faster than the plain path tracer any more
                 learned.append(timed(replace(opt, spp=n, seed=11), denoise_net=net))
+                den_done.append(n)
                 log(f"learned {n} spp: {learned[-1][1]:.2f} dB, {learned[-1][0]:.1f} ms wall, "
-                    f"{learned[-1][2]:.1f} ms device")
+                    f"{learned[-1][2]:.1f} ms device ({learned[-1][3]:.2f} ms resolve + network)")
+                if learned[-1][1] >= 40.0:
+                    break
     finally:
         bs.close()
     res = {"workload": (f"path-traced GI: molecular surface ({len(sc.index)} triangles) as a GGX dielectric over "
@@ -288,17 +300,18 @@ def evaluate_full(dev: Device, net: Optional[Unet], ref_spp: int = TARGET_SPP, m
                         f"U[0.05, 0.6]), {len(opt.lights)} spherical area lights (RIS + ReSTIR spatial), "
                         f"{opt.bounces} bounces, 1920x1080; PSNR (sRGB, whole frame) vs {ref_spp} spp"),
            "ref_spp": ref_spp,
-           "plain": {str(n): [round(p[0], 2), round(p[1], 2), round(p[2], 2)] for n, p in zip(plain_spp, plain)},
-           "learned": {str(n): [round(p[0], 2), round(p[1], 2), round(p[2], 2)] for n, p in zip(den_spp, learned)},
+           "points": "spp: [wall ms, dB, device ms, of which resolve + network ms]",
+           "plain": {str(n): [round(float(v), 2) for v in p] for n, p in zip(plain_spp, plain)},
+           "learned": {str(n): [round(float(v), 2) for v in p] for n, p in zip(den_done, learned)},
            "t40_plain_ms": time_to_psnr(plain, 40.0), "t40_learned_ms": time_to_psnr(learned, 40.0),
            "t40_plain_device_ms": time_to_psnr(plain, 40.0, 2),
            "t40_learned_device_ms": time_to_psnr(learned, 40.0, 2)}
-    if learned:
-        # iso-quality comparison at the best PSNR the learned path reaches
-        best = max(learned, key=lambda p: p[1])
-        res["learned_best_db"] = round(best[1], 2)
-        res["plain_ms_to_learned_best"] = time_to_psnr(plain, best[1])
-        res["plain_device_ms_to_learned_best"] = time_to_psnr(plain, best[1], 2)
+    # time (wall ms) each first reaches a PSNR, and the ratio
+    res["time_to_db"] = {}
+    for db in (30.0, 35.0, 40.0):
+        a, b = time_to_psnr(plain, db), time_to_psnr(learned, db)
+        res["time_to_db"][f"{db:g}"] = {"plain_ms": a, "learned_ms": b,
+                                       "speedup": None if (a is None or b is None) else round(a / b, 2)}
     for a, b, k in (("t40_plain_ms", "t40_learned_ms", "speedup"),
                     ("t40_plain_device_ms", "t40_learned_device_ms", "speedup_device")):
         res[k] = None if (res[a] is None or res[b] is None) else round(res[a] / res[b], 2)
@@ -313,6 +326,8 @@ def main() -> None:
     ap.add_argument("--gen-seconds", type=float, default=600.0)
     ap.add_argument("--train-seconds", type=float, default=300.0)
     ap.add_argument("--target-spp", type=int, default=TARGET_SPP)
+    ap.add_argument("--noisy-spp", default=",".join(str(n) for n in NOISY_SPP),
+                    help="comma-separated samples per pixel of the noisy inputs of every molecule")
     ap.add_argument("--batch", type=int, default=8)
     ap.add_argument("--crop", type=int, default=128)
     ap.add_argument("--lr", type=float, default=1e-3)
@@ -331,8 +346,9 @@ def main() -> None:
             evaluate_full(dev, net)
             return
         seeds = range(args.first_seed, args.first_seed + args.scenes)
-        feats, owner, tgts = make_pairs(dev, seeds, target_spp=args.target_spp, seconds=args.gen_seconds,
-                                        pool=pool)
+        noisy = tuple(int(v) for v in args.noisy_spp.split(","))
+        feats, owner, tgts = make_pairs(dev, seeds, target_spp=args.target_spp, noisy_spp=noisy,
+                                        seconds=args.gen_seconds, pool=pool)
         pool.terminate()
         pool = None
         _say(f"{len(tgts)} molecules, {len(feats)} noisy inputs")
