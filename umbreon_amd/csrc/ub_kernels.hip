@@ -566,7 +566,14 @@ __global__ __launch_bounds__(kTraceBlock) void k_ao(FrameConst fc, SceneView s, 
 // stall), and each iteration runs one kind of step for the whole wave (node
 // tests or primitive tests, by lane-count vote). Occluded rays add 1 to their
 // hit's count: integer atomics, so the result does not depend on the order.
+// The chunk is 256 rays while a partition holds that many for every wave the
+// grid gives it, and shrinks (to 128, then 64) when it holds fewer, so the
+// small late transparency layers spread over more waves instead of running
+// four rays after one another on each lane of a few waves. Chunks of
+// different sizes still split a partition into disjoint ranges (each wave
+// takes [base, base + its chunk) from the atomic cursor).
 constexpr int kAoChunk = 256;
+constexpr int kAoChunkMin = 64;
 constexpr int kAoRefill = 16;
 
 // The persistent AO loop of one frame: returns once every partition of its
@@ -581,6 +588,7 @@ __device__ __forceinline__ void aoPersistentFrame(const FrameConst& fc, const Sc
   // wave-uniform: current partition and cursor into the current chunk
   uint32_t part = ((blockIdx.x * blockDim.x + threadIdx.x) >> 6) % kQueueParts;
   uint32_t chunkPart = part;
+  const uint64_t wavesPerPart = max(1u, (gridDim.x * (blockDim.x >> 6)) / static_cast<uint32_t>(kQueueParts));
   int tried = 0;
   uint64_t next = 0, end = 0;
   bool drained = false;  // every partition is used up
@@ -597,16 +605,18 @@ __device__ __forceinline__ void aoPersistentFrame(const FrameConst& fc, const Sc
       while (next == end) {
         const uint64_t len = static_cast<uint64_t>(queueFill(b.counters, kHitFill, part)) * n;
         unsigned long long* cur = &b.counters[kAoCursor + kLineWords * part];
+        uint64_t chunk = kAoChunk;
+        while (chunk > kAoChunkMin && len < chunk * wavesPerPart) chunk >>= 1;
         uint64_t base = len;
         if (__lane_id() == 0) {
           // a plain read first: a used-up partition costs no atomic
           const uint64_t seen = __hip_atomic_load(cur, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-          if (seen < len) base = atomicAdd(cur, (unsigned long long)kAoChunk);
+          if (seen < len) base = atomicAdd(cur, (unsigned long long)chunk);
         }
         base = __shfl(base, 0, 64);
         if (base < len) {
           next = base;
-          end = min(base + static_cast<uint64_t>(kAoChunk), len);
+          end = min(base + chunk, len);
           chunkPart = part;
         } else {
           if (++tried == kQueueParts) {

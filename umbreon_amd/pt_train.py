@@ -22,6 +22,7 @@ the sRGB comparison does.
 
     python -m umbreon_amd.pt_train --gen-seconds 600 --train-seconds 420 --out w.npz [--init w0.npz]
     python -m umbreon_amd.pt_train --evaluate umbreon_amd/weights/pt_unet.npz
+    python -m umbreon_amd.pt_train --bench-frame 1 --weights umbreon_amd/weights/pt_unet.npz --calls 5
 """
 from __future__ import annotations
 
@@ -319,6 +320,36 @@ def evaluate_full(dev: Device, net: Optional[Unet], ref_spp: int = TARGET_SPP, m
     return res
 
 
+def bench_frame(dev: Device, weights: Optional[str], spp: int, calls: int, log=None) -> dict:
+    """Wall and device ms per full-workload frame (1920x1080) at spp samples
+    per pixel, denoised by the network at `weights` (None: plain), after one
+    warm-up frame; for kernel traces of the path tracer and the network."""
+    log = log or _say
+    sc, opt = pathtraced_surface_scene()
+    bs = BuiltScene(dev, sc, capi.BVH8_GPU_BUILD)
+    net = load(dev, weights) if weights else None
+    try:
+        o = replace(opt, spp=spp, seed=11)
+        pathtrace(bs, sc, o, denoise_net=net)
+        walls, traces, dens, rays = [], [], [], 0
+        for _ in range(calls):
+            t = time.perf_counter()
+            f = pathtrace(bs, sc, o, denoise_net=net)
+            walls.append((time.perf_counter() - t) * 1e3)
+            traces.append(f.trace_seconds * 1e3)
+            dens.append(f.denoise_seconds * 1e3)
+            rays = f.rays
+    finally:
+        if net is not None:
+            net.close()
+        bs.close()
+    res = {"spp": spp, "denoised": net is not None, "wall_ms": float(np.median(walls)),
+           "trace_ms": float(np.median(traces)), "resolve_network_ms": float(np.median(dens)), "rays": int(rays),
+           "mrays_per_s": rays / (float(np.median(traces)) * 1e3)}
+    log(f"frame: {res}")
+    return res
+
+
 def main() -> None:
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
     ap.add_argument("--first-seed", type=int, default=1000)
@@ -337,7 +368,18 @@ def main() -> None:
     ap.add_argument("--evaluate", default="", help="weights to evaluate on the full workload")
     ap.add_argument("--evaluate-after", action="store_true", help="evaluate the trained net on the full workload")
     ap.add_argument("--workers", type=int, default=4, help="host processes generating molecules")
+    ap.add_argument("--bench-frame", type=int, default=0, metavar="SPP",
+                    help="time full-workload frames at SPP samples per pixel (denoised with --weights)")
+    ap.add_argument("--weights", default="", help="network for --bench-frame")
+    ap.add_argument("--calls", type=int, default=5)
     args = ap.parse_args()
+    if args.bench_frame:
+        dev = Device(0)
+        try:
+            bench_frame(dev, args.weights or None, args.bench_frame, args.calls)
+        finally:
+            dev.close()
+        return
     pool = None if args.evaluate else scene_pool(args.workers)
     dev = Device(0)
     try:
