@@ -1621,6 +1621,7 @@ int ubg_pathtrace(ubg_context* ctx, const ubg_scene* scene, const ubg_pt_params*
   q.nPrimMats = static_cast<int32_t>(pm.size());
   q.nMeshMats = static_cast<int32_t>(mm.size());
   q.rays = reinterpret_cast<unsigned long long*>(ctx->ptWords.p) + 1;
+  q.depthMax = static_cast<unsigned int*>(ctx->ptWords.p);
   auto* maxBits = static_cast<unsigned int*>(ctx->ptWords.p);
   auto* acc = static_cast<float4*>(ctx->ptAcc.p);
   auto* first = static_cast<float4*>(ctx->ptFirst.p);

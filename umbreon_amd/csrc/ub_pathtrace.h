@@ -55,6 +55,7 @@ struct Params {
   const Mat* meshMats;
   int32_t nPrimMats, nMeshMats;
   unsigned long long* rays;  // traced-ray counter
+  unsigned int* depthMax;    // sample 0 with features: maximum first-hit distance (float bits)
 };
 // First-hit record: position + ray distance (w = 0: no surface), shading
 // normal + material type, geometric normal + GGX alpha, base colour + IOR,
@@ -75,9 +76,10 @@ void ptZero(float4* acc, int n, unsigned int* maxBits, unsigned long long* rays,
 void ptSample(const pt::Params& p, const SceneView& s, pt::GPix* gb, pt::Reservoir* res, float4* acc, float4* first,
               hipStream_t st);
 // colour = acc / spp (W x H); with feat: the Wp x Hp edge-padded denoiser
-// features as well.
-void ptResolve(int W, int H, int Wp, int Hp, int spp, const float4* acc, const float4* first, unsigned int* maxBits,
-               float4* color, float* feat, hipStream_t st);
+// features as well (maxBits: the maximum first-hit distance k_pt_primary
+// recorded in Params::depthMax).
+void ptResolve(int W, int H, int Wp, int Hp, int spp, const float4* acc, const float4* first,
+               const unsigned int* maxBits, float4* color, float* feat, hipStream_t st);
 void ptApplyNet(int W, int H, int Wp, const float* pred, float4* color, hipStream_t st);
 
 }  // namespace ubg

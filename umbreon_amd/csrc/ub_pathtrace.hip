@@ -586,7 +586,7 @@ __device__ __noinline__ void guides(const Params& p, const SceneView& s, const S
 // One camera ray; returns the rays traced (1).
 __device__ __forceinline__ uint32_t primaryPixel(const Params& p, const SceneView& s, int pix, GPix* __restrict__ gb,
                                                  Reservoir* __restrict__ res, float4* __restrict__ acc,
-                                                 float4* __restrict__ first) {
+                                                 float4* __restrict__ first, float& tFirst) {
   const uint32_t upix = static_cast<uint32_t>(pix), smp = static_cast<uint32_t>(p.sample);
   const int px = pix % p.W, py = pix / p.W;
   V3 o, d;
@@ -624,6 +624,7 @@ __device__ __forceinline__ uint32_t primaryPixel(const Params& p, const SceneVie
     guides(p, s, sf, P, Ng, d, h.t, alb, nrm);
     first[2 * pix] = make_float4(alb.x, alb.y, alb.z, h.t);
     first[2 * pix + 1] = make_float4(nrm.x, nrm.y, nrm.z, 1.0f);
+    tFirst = h.t;
   }
   if (p.nLights > 0 && !p.neeAll) {
     const Pick k = risLights(p, sf, P, wo, upix, smp, dimVertex(0));
@@ -640,8 +641,17 @@ __global__ __launch_bounds__(kBlock) void k_pt_primary(Params p, SceneView s, GP
                                                       Reservoir* __restrict__ res, float4* __restrict__ acc,
                                                       float4* __restrict__ first) {
   const int pix = blockIdx.x * blockDim.x + threadIdx.x;
-  const uint32_t nr = pix < p.W * p.H ? primaryPixel(p, s, pix, gb, res, acc, first) : 0u;
+  float tFirst = 0.0f;
+  const uint32_t nr = pix < p.W * p.H ? primaryPixel(p, s, pix, gb, res, acc, first, tFirst) : 0u;
   countRays(p.rays, nr);
+  if (first && p.sample == 0) {
+    // the frame's maximum first-hit distance (>= 0: float bits order like
+    // the floats), one atomic per wave over the kernel's whole run
+    int v = static_cast<int>(__float_as_uint(fmaxf(0.0f, tFirst)));
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = max(v, __shfl_xor(v, off, 64));
+    if (__lane_id() == 0 && v > 0) atomicMax(p.depthMax, static_cast<unsigned int>(v));
+  }
 }
 
 __device__ __forceinline__ Surf surfOfG(const GPix& g) {
@@ -778,12 +788,8 @@ __global__ __launch_bounds__(kBlock) void k_pt_shade(Params p, SceneView s, cons
 }
 
 // Mean over the samples, and the denoiser features: the encoded colour
-// log(1 + 32 c) / 4, albedo and normal guides, first-hit distance / its frame maximum (maxBits: that maximum as
-// float bits, from k_pt_depth_max).
-__global__ void k_pt_depth_max(int n, const float4* __restrict__ first, unsigned int* __restrict__ maxBits) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) atomicMax(maxBits, __float_as_uint(fmaxf(0.0f, first[2 * i].w)));
-}
+// log(1 + 32 c) / 4, albedo and normal guides, first-hit distance / its frame
+// maximum (maxBits: that maximum as float bits, from k_pt_primary).
 __global__ void k_pt_resolve(int W, int H, int Wp, int Hp, int spp, const float4* __restrict__ acc,
                              const float4* __restrict__ first, const unsigned int* __restrict__ maxBits,
                              float4* __restrict__ color, float* __restrict__ feat) {
@@ -856,10 +862,9 @@ void ptSample(const pt::Params& p, const SceneView& s, pt::GPix* gb, pt::Reservo
                      first);
   hipLaunchKernelGGL(pt::k_pt_shade, dim3(gridOf(n, pt::kBlock)), dim3(pt::kBlock), lds, st, p, s, gb, res, acc);
 }
-void ptResolve(int W, int H, int Wp, int Hp, int spp, const float4* acc, const float4* first, unsigned int* maxBits,
-               float4* color, float* feat, hipStream_t st) {
+void ptResolve(int W, int H, int Wp, int Hp, int spp, const float4* acc, const float4* first,
+               const unsigned int* maxBits, float4* color, float* feat, hipStream_t st) {
   if (feat) {
-    hipLaunchKernelGGL(pt::k_pt_depth_max, dim3(gridOf(W * H, 256)), dim3(256), 0, st, W * H, first, maxBits);
     hipLaunchKernelGGL(pt::k_pt_resolve, dim3(gridOf(static_cast<int64_t>(Wp) * Hp, 256)), dim3(256), 0, st, W, H, Wp,
                        Hp, spp, acc, first, maxBits, color, feat);
     return;
