@@ -411,7 +411,8 @@ int ubg_render(ubg_context* ctx, const ubg_scene* scene,
 /* Tuning of a batched render (may be NULL: the defaults). Any choice gives
  * the same images; it only changes speed. */
 typedef struct ubg_batch_options {
-  int32_t ao_waves_per_simd;   /* persistent AO grid: waves per SIMD, 1..6 (0 = 6) */
+  int32_t ao_waves_per_simd;   /* persistent AO grid: waves per SIMD, 1..5 (0 = 5; larger
+                                  values are clamped to 5, the kernel's register budget) */
   int32_t no_graph;            /* 1: launch each layer's kernels directly instead of
                                   replaying its instantiated HIP graph */
   int32_t reserved[6];

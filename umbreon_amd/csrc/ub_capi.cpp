@@ -1196,7 +1196,8 @@ int ubg_render_batch(ubg_context* ctx, int32_t n, const ubg_scene* const* scenes
   if (n > ubg::kMaxBatch)
     return fail(UBG_ERR_INVALID_ARGUMENT, "ubg_render_batch: more than " + std::to_string(ubg::kMaxBatch) +
                                               " requests in one batch");
-  const int wavesPerSimd = opts && opts->ao_waves_per_simd > 0 ? std::min(opts->ao_waves_per_simd, 6) : 6;
+  const int wavesPerSimd = opts && opts->ao_waves_per_simd > 0 ? std::min(opts->ao_waves_per_simd, ubg::kAoWavesPerSimd)
+                                                                : ubg::kAoWavesPerSimd;
   const bool useGraph = !opts || opts->no_graph == 0;
   std::lock_guard<std::mutex> lk(ctx->mu);
   UBG_HIP(hipSetDevice(ctx->device), "hipSetDevice");

@@ -187,6 +187,17 @@ UBG_HD float maxAbs(V3 v) { return fmaxf(fmaxf(fabsf(v.x), fabsf(v.y)), fabsf(v.
 // |disc| below which the FP32 discriminant is not trusted: scale x size x
 // magnitude of the rounded coordinates, with a 2^5 margin over 2^-24.
 UBG_HD float grazeSlack(float scale, float size, float m) { return scale * 0x1.0p-19f * size * m; }
+// One Newton step on A x^2 + 2 B x + C at x, taken when it moves x by at most
+// `half`.
+UBG_HD double newtonStep64(double A, double B, double C, double x, double half) {
+  const double g = (A * x + 2.0 * B) * x + C;
+  const double gp = 2.0 * (A * x + B);
+  if (gp != 0.0) {
+    const double st = g / gp;
+    if (fabs(st) <= half) return x - st;
+  }
+  return x;
+}
 // Roots of A x^2 + 2 B x + C in FP64, each with one Newton step (taken when
 // it moves the root by at most half the roots' separation).
 UBG_HD bool quadPolish64(double A, double B, double C, double& x0, double& x1) {
@@ -196,16 +207,8 @@ UBG_HD bool quadPolish64(double A, double B, double C, double& x0, double& x1) {
   x0 = (-B - q) / A;
   x1 = (-B + q) / A;
   const double half = 0.5 * fabs(x1 - x0);
-  double* xs[2] = {&x0, &x1};
-  for (int i = 0; i < 2; ++i) {
-    const double x = *xs[i];
-    const double g = (A * x + 2.0 * B) * x + C;
-    const double gp = 2.0 * (A * x + B);
-    if (gp != 0.0) {
-      const double st = g / gp;
-      if (fabs(st) <= half) *xs[i] = x - st;
-    }
-  }
+  x0 = newtonStep64(A, B, C, x0, half);
+  x1 = newtonStep64(A, B, C, x1, half);
   return true;
 }
 

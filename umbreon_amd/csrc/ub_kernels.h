@@ -140,6 +140,9 @@ struct PartFill {
 // Batched frames (ubg_render_batch). One request of a batch: its frame
 // constants, scene and buffers, the resolve parameters and output.
 constexpr int kMaxBatch = 256;
+// Waves per SIMD the persistent AO kernels are compiled for (their register
+// budget), and so the most a batch's AO grid can keep resident.
+constexpr int kAoWavesPerSimd = 5;
 struct BatchItem {
   FrameConst fc;
   SceneView s;

@@ -30,9 +30,14 @@ namespace ubg {
 
 constexpr int kTraceBlock = 128;
 // Waves per SIMD the traversal kernels are register-limited to: k_primary to
-// 7 (<= 72 VGPRs), the persistent AO kernel to 6 (<= 84 VGPRs, no spills).
-constexpr int kTraceWavesPerSimd = 7;
-constexpr int kAoWavesPerSimd = 6;
+// 6 (<= 80 VGPRs), the persistent AO kernel to kAoWavesPerSimd = 5 (<= 96
+// VGPRs). The FP64
+// polish of grazing roots (ub_device.h) needs ~30 more VGPRs than the FP32
+// tests; under the tighter budgets of 7 / 6 waves the whole traversal loop
+// spilled to scratch around it (molecular-surface frame 7.38 ms; 6.75 ms
+// here; 6.19 ms with the polish compiled out).
+constexpr int kTraceWavesPerSimd = 6;
+static_assert(kAoWavesPerSimd == 5, "the AO register budget (ub_kernels.h)");
 
 // Dynamic LDS: the per-lane traversal stacks, sized at launch from the scene's
 // tree depth (BVH2: one u32 node index per entry; BVH8: a u32 base array and

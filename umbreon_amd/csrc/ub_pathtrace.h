@@ -52,7 +52,7 @@ struct Params {
   int32_t neeAll;        // 1: one cone sample of every light instead of RIS / ReSTIR
   const Light* lights;
   const Mat* primMats;
-  const Mat* meshMats;
+  const Mat* meshMats;   // == primMats + nPrimMats (one table, read by offset)
   int32_t nPrimMats, nMeshMats;
   unsigned long long* rays;  // traced-ray counter
   unsigned int* depthMax;    // sample 0 with features: maximum first-hit distance (float bits)

@@ -36,6 +36,17 @@ extern __shared__ uint32_t ptLds[];
 
 constexpr float kPi = 3.14159265358979f;
 constexpr int kBlock = 128;
+// Waves per SIMD the sample-pass kernels are compiled for (their register
+// budget: 512 / waves VGPRs). Traversal is latency-bound, so occupancy pays
+// for itself up to the point where the budget forces spills.
+#ifndef UBG_PT_PRIMARY_WAVES
+#define UBG_PT_PRIMARY_WAVES 4
+#endif
+#ifndef UBG_PT_SHADE_WAVES
+#define UBG_PT_SHADE_WAVES 3
+#endif
+constexpr int kPrimaryWaves = UBG_PT_PRIMARY_WAVES;
+constexpr int kShadeWaves = UBG_PT_SHADE_WAVES;
 
 // ---------------------------------------------------------------- random
 __host__ __device__ inline uint32_t hash32(uint32_t x) {
@@ -411,7 +422,10 @@ __device__ __forceinline__ Surf surfaceAt(const Params& p, const SceneView& s, c
     C = mk(col.x, col.y, col.z);
     mat = info & kInfoMatMask;
   }
-  const Mat* tab = mesh ? p.meshMats : p.primMats;
+  // the mesh table follows the primitive table (Params::meshMats ==
+  // primMats + nPrimMats): an offset, not a choice between two pointers,
+  // which the compiler would keep in private memory
+  const Mat* tab = p.primMats + (mesh ? p.nPrimMats : 0);
   const int n = mesh ? p.nMeshMats : p.nPrimMats;
   const Mat m = n > 0 ? tab[mat < static_cast<uint32_t>(n) ? mat : 0u] : Mat{kMatDiffuse, 0.0f, 1.5f, 0};
   Surf sf;
@@ -538,7 +552,7 @@ __device__ __forceinline__ V3 directLight(const Params& p, const SceneView& s, c
 // are returned in camera space (right, up, towards the eye). Guide rays are
 // not counted as traced rays: they feed the network, not the estimate.
 constexpr int kGuideHops = 4;
-__device__ __noinline__ void guides(const Params& p, const SceneView& s, const Surf& sf0, V3 P0, V3 Ng0, V3 d0,
+__device__ __forceinline__ void guides(const Params& p, const SceneView& s, const Surf& sf0, V3 P0, V3 Ng0, V3 d0,
                                     float t0, V3& alb, V3& nrm) {
   V3 n = sf0.f.n;
   if (sf0.type != kMatDielectric) {
@@ -585,8 +599,7 @@ __device__ __noinline__ void guides(const Params& p, const SceneView& s, const S
 // ---------------------------------------------------------------- kernels
 // One camera ray; returns the rays traced (1).
 __device__ __forceinline__ uint32_t primaryPixel(const Params& p, const SceneView& s, int pix, GPix* __restrict__ gb,
-                                                 Reservoir* __restrict__ res, float4* __restrict__ acc,
-                                                 float4* __restrict__ first, float& tFirst) {
+                                                 Reservoir* __restrict__ res, float4* __restrict__ acc) {
   const uint32_t upix = static_cast<uint32_t>(pix), smp = static_cast<uint32_t>(p.sample);
   const int px = pix % p.W, py = pix / p.W;
   V3 o, d;
@@ -605,8 +618,6 @@ __device__ __forceinline__ uint32_t primaryPixel(const Params& p, const SceneVie
     res[pix] = r;
     float4 a = acc[pix];
     acc[pix] = make_float4(a.x + p.env[0], a.y + p.env[1], a.z + p.env[2], 0.0f);
-    if (first && p.sample == 0) first[2 * pix] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    if (first && p.sample == 0) first[2 * pix + 1] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     return 1u;
   }
   V3 Ng;
@@ -619,13 +630,6 @@ __device__ __forceinline__ uint32_t primaryPixel(const Params& p, const SceneVie
   g.C = make_float4(sf.C.x, sf.C.y, sf.C.z, sf.ior);
   g.wo = make_float4(wo.x, wo.y, wo.z, 0.0f);
   gb[pix] = g;
-  if (first && p.sample == 0) {
-    V3 alb, nrm;
-    guides(p, s, sf, P, Ng, d, h.t, alb, nrm);
-    first[2 * pix] = make_float4(alb.x, alb.y, alb.z, h.t);
-    first[2 * pix + 1] = make_float4(nrm.x, nrm.y, nrm.z, 1.0f);
-    tFirst = h.t;
-  }
   if (p.nLights > 0 && !p.neeAll) {
     const Pick k = risLights(p, sf, P, wo, upix, smp, dimVertex(0));
     r.Q = make_float4(k.Q.x, k.Q.y, k.Q.z, __int_as_float(k.li));
@@ -637,21 +641,11 @@ __device__ __forceinline__ uint32_t primaryPixel(const Params& p, const SceneVie
   res[pix] = r;
   return 1u;
 }
-__global__ __launch_bounds__(kBlock) void k_pt_primary(Params p, SceneView s, GPix* __restrict__ gb,
-                                                      Reservoir* __restrict__ res, float4* __restrict__ acc,
-                                                      float4* __restrict__ first) {
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kPrimaryWaves))) void k_pt_primary(
+    Params p, SceneView s, GPix* __restrict__ gb, Reservoir* __restrict__ res, float4* __restrict__ acc) {
   const int pix = blockIdx.x * blockDim.x + threadIdx.x;
-  float tFirst = 0.0f;
-  const uint32_t nr = pix < p.W * p.H ? primaryPixel(p, s, pix, gb, res, acc, first, tFirst) : 0u;
+  const uint32_t nr = pix < p.W * p.H ? primaryPixel(p, s, pix, gb, res, acc) : 0u;
   countRays(p.rays, nr);
-  if (first && p.sample == 0) {
-    // the frame's maximum first-hit distance (>= 0: float bits order like
-    // the floats), one atomic per wave over the kernel's whole run
-    int v = static_cast<int>(__float_as_uint(fmaxf(0.0f, tFirst)));
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v = max(v, __shfl_xor(v, off, 64));
-    if (__lane_id() == 0 && v > 0) atomicMax(p.depthMax, static_cast<unsigned int>(v));
-  }
 }
 
 __device__ __forceinline__ Surf surfOfG(const GPix& g) {
@@ -662,6 +656,35 @@ __device__ __forceinline__ Surf surfOfG(const GPix& g) {
   sf.alpha = g.Ng.w;
   sf.ior = g.C.w;
   return sf;
+}
+
+// The denoiser's first-hit record of sample 0 (guides + distance), from the
+// G-buffer k_pt_primary wrote for that sample, and the frame's maximum
+// first-hit distance (>= 0: float bits order like the floats; one atomic
+// per wave). A kernel of its own, launched once per frame with features, so
+// the guide chain's registers do not lower the sample passes' occupancy.
+__global__ __launch_bounds__(kBlock) void k_pt_guides(Params p, SceneView s, const GPix* __restrict__ gb,
+                                                     float4* __restrict__ first) {
+  const int pix = blockIdx.x * blockDim.x + threadIdx.x;
+  float tFirst = 0.0f;
+  if (pix < p.W * p.H) {
+    const GPix g = gb[pix];
+    if (g.P.w == 0.0f) {
+      first[2 * pix] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      first[2 * pix + 1] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    } else {
+      V3 alb, nrm;
+      guides(p, s, surfOfG(g), mk(g.P.x, g.P.y, g.P.z), mk(g.Ng.x, g.Ng.y, g.Ng.z), mk(-g.wo.x, -g.wo.y, -g.wo.z),
+             g.P.w, alb, nrm);
+      first[2 * pix] = make_float4(alb.x, alb.y, alb.z, g.P.w);
+      first[2 * pix + 1] = make_float4(nrm.x, nrm.y, nrm.z, 1.0f);
+      tFirst = g.P.w;
+    }
+  }
+  int v = static_cast<int>(__float_as_uint(fmaxf(0.0f, tFirst)));
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = max(v, __shfl_xor(v, off, 64));
+  if (__lane_id() == 0 && v > 0) atomicMax(p.depthMax, static_cast<unsigned int>(v));
 }
 
 // The rest of one path; returns the rays traced.
@@ -780,8 +803,8 @@ __device__ __forceinline__ uint32_t shadePixel(const Params& p, const SceneView&
   acc[pix] = make_float4(a.x + L.x, a.y + L.y, a.z + L.z, 0.0f);
   return nr;
 }
-__global__ __launch_bounds__(kBlock) void k_pt_shade(Params p, SceneView s, const GPix* __restrict__ gb,
-                                                    const Reservoir* __restrict__ res, float4* __restrict__ acc) {
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kShadeWaves))) void k_pt_shade(
+    Params p, SceneView s, const GPix* __restrict__ gb, const Reservoir* __restrict__ res, float4* __restrict__ acc) {
   const int pix = blockIdx.x * blockDim.x + threadIdx.x;
   const uint32_t nr = pix < p.W * p.H ? shadePixel(p, s, pix, gb, res, acc) : 0u;
   countRays(p.rays, nr);
@@ -858,8 +881,9 @@ void ptSample(const pt::Params& p, const SceneView& s, pt::GPix* gb, pt::Reservo
               hipStream_t st) {
   const int n = p.W * p.H;
   const size_t lds = static_cast<size_t>(2) * static_cast<size_t>(s.stackDepth > 0 ? s.stackDepth : 1) * pt::kBlock * 4u;
-  hipLaunchKernelGGL(pt::k_pt_primary, dim3(gridOf(n, pt::kBlock)), dim3(pt::kBlock), lds, st, p, s, gb, res, acc,
-                     first);
+  hipLaunchKernelGGL(pt::k_pt_primary, dim3(gridOf(n, pt::kBlock)), dim3(pt::kBlock), lds, st, p, s, gb, res, acc);
+  if (first && p.sample == 0)
+    hipLaunchKernelGGL(pt::k_pt_guides, dim3(gridOf(n, pt::kBlock)), dim3(pt::kBlock), lds, st, p, s, gb, first);
   hipLaunchKernelGGL(pt::k_pt_shade, dim3(gridOf(n, pt::kBlock)), dim3(pt::kBlock), lds, st, p, s, gb, res, acc);
 }
 void ptResolve(int W, int H, int Wp, int Hp, int spp, const float4* acc, const float4* first,

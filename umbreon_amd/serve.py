@@ -151,7 +151,7 @@ def frame_bytes(width: int, height: int) -> int:
 
 @dataclass
 class Tuning:
-    ao_waves_per_simd: int = 6
+    ao_waves_per_simd: int = 5
     max_batch: int = 64
     leaf_max: Dict[str, int] = field(default_factory=lambda: {r: 3 for r in REPS})
 
@@ -302,7 +302,7 @@ def autotune(device: Device, pool: MoleculePool, calib: List[Request], budget: i
                 best = (dt, leaf)
         tun.leaf_max[rep] = best[1]
     best = None
-    for waves in (4, 6):
+    for waves in (3, 5):
         for mb in (16, 64):
             t = Tuning(ao_waves_per_simd=waves, max_batch=mb, leaf_max=dict(tun.leaf_max))
             srv = Server(device, pool, budget, t)
