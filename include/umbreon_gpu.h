@@ -622,11 +622,16 @@ int ubg_unet_set_params(ubg_unet* net, const float* params); /* host buffer */
 /* Denoised log(1 + E) of n images (features and out per `memory`). */
 int ubg_unet_infer(ubg_unet* net, int32_t n, int32_t h, int32_t w, const float* features, float* out,
                    int32_t memory);
-/* One training step on a batch: the mean L1 loss on log(1 + E) into *loss,
- * the gradients into `grads` (host, may be NULL), then an Adam step (beta
- * 0.9 / 0.999, eps 1e-8) when lr > 0. Features and targets per `memory`. */
+/* One training step on a batch: the mean loss on log(1 + E) into *loss (L1
+ * unless ubg_unet_set_loss chose the squared error), the gradients into
+ * `grads` (host, may be NULL), then an Adam step (beta 0.9 / 0.999, eps 1e-8)
+ * when lr > 0. Features and targets per `memory`. */
 int ubg_unet_train_step(ubg_unet* net, int32_t n, int32_t h, int32_t w, const float* features,
                         const float* targets, float lr, float* loss, float* grads, int32_t memory);
+/* Training loss of the network: 0 = mean absolute error (the default), 1 =
+ * mean squared error (the PSNR's measure). No counterpart in the reference
+ * (OIDN is pretrained there). */
+int ubg_unet_set_loss(ubg_unet* net, int32_t mode);
 
 #ifdef __cplusplus
 }

@@ -142,18 +142,20 @@ def forward(params: np.ndarray, feat: np.ndarray, levels: int, channels, bf16: b
 
 
 def loss_and_grads(params: np.ndarray, feat: np.ndarray, target: np.ndarray, levels: int, channels,
-                   bf16: bool = True):
-    """Mean L1 loss on log(1 + E) and the parameter gradients."""
+                   bf16: bool = True, loss: str = "l1"):
+    """Mean L1 (loss 'l1') or mean squared (loss 'l2') loss on the
+    prediction and the parameter gradients (ub_unet.hip k_unet_loss)."""
     acts, convs = layout(levels, channels)
     rnd = to_bf16 if bf16 else (lambda v: np.asarray(v, np.float64))
     pred, cache = forward(params, feat, levels, channels, bf16)
     d = pred - np.asarray(target, np.float64)
     count = d.size
-    loss = float(np.abs(d).mean())
+    l2 = loss == "l2"
+    loss = float((d * d).mean()) if l2 else float(np.abs(d).mean())
     grads = np.zeros(params.size, np.float64)
     n, h, w, _ = feat.shape
     dout = np.zeros(cache["out"].shape)
-    dout[..., :3] = np.sign(d) / count
+    dout[..., :3] = (2.0 * d if l2 else np.sign(d)) / count
     dacts = {a: np.zeros(cache["vals"][a].shape) for a in cache["vals"] if a > 0}
     for k in range(len(convs) - 1, -1, -1):
         c = convs[k]

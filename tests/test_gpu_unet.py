@@ -79,6 +79,40 @@ def test_forward_and_gradients_match_oracle(device, levels, ch, shape, min_wgs):
         net.close()
 
 
+def test_squared_loss_gradients_match_oracle(device):
+    """The mean-squared training loss (ubg_unet_set_loss 1) on the device
+    against oracle/unet.py's (its gradients pinned by finite differences)."""
+    from umbreon_amd.denoise import Unet
+    levels, ch = 3, [32, 64, 96]
+    net = Unet(device, levels, ch, seed=5)
+    try:
+        net.set_loss("l2")
+        p = net.get_params()
+        rng = np.random.default_rng(17)
+        _, convs = U.layout(levels, ch)
+        for c in convs:
+            p[c["b_off"]:c["b_off"] + c["cout"]] = rng.normal(0.0, 0.05, c["cout"])
+        net.set_params(p)
+        feat, tgt = _batch(rng, 2, 16, 32)
+        g_loss, g_grads = net.train_step(feat, tgt, lr=0.0, want_grads=True)
+        c_loss, c_grads = U.loss_and_grads(p.astype(np.float64), feat, tgt, levels, ch, bf16=True, loss="l2")
+        l1, _ = U.loss_and_grads(p.astype(np.float64), feat, tgt, levels, ch, bf16=True, loss="l1")
+        print(f"unet squared loss: device {g_loss:.6g} oracle {c_loss:.6g} (L1 {l1:.6g})", flush=True)
+        assert abs(g_loss - c_loss) <= 2e-3 * c_loss and abs(g_loss - l1) > 1e-2 * l1
+        for k, c in enumerate(convs):
+            a = g_grads[c["w_off"]:c["b_off"] + c["cout"]].astype(np.float64)
+            b = c_grads[c["w_off"]:c["b_off"] + c["cout"]]
+            nb = np.linalg.norm(b)
+            cos = float(a @ b / (np.linalg.norm(a) * nb))
+            rel = float(np.linalg.norm(a - b) / nb)
+            assert cos >= 0.999 and rel <= 5e-2, (k, cos, rel)
+        net.set_loss("l1")
+        g1, _ = net.train_step(feat, tgt, lr=0.0)
+        assert abs(g1 - l1) <= 2e-3 * l1
+    finally:
+        net.close()
+
+
 def test_forward_at_a_training_crop(device):
     """A 128^2 crop through the 4-level net at the default launch shapes.
     Device and oracle round the same operands to bf16, so they differ only

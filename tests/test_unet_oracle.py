@@ -30,22 +30,26 @@ def test_layout_counts():
     assert cv[2]["s0"][1] == 1 and cv[4]["s0"][1] == 1  # encoder inputs pooled
 
 
+@pytest.mark.parametrize("loss", ["l1", "l2"])
 @pytest.mark.parametrize("levels,ch,shape", [(2, [32, 32], (1, 4, 4)), (3, [32, 64, 32], (2, 8, 4))])
-def test_gradients_match_finite_differences(levels, ch, shape):
+def test_gradients_match_finite_differences(levels, ch, shape, loss):
     rng = np.random.default_rng(levels)
     p = _params(levels, ch, rng)
     n, h, w = shape
     feat = rng.uniform(0.0, 1.0, (n, h, w, U.N_FEATURES))
     tgt = rng.uniform(0.0, 1.0, (n, h, w, 3)) + 5.0  # far from the prediction: a smooth L1 branch
-    _, g = U.loss_and_grads(p, feat, tgt, levels, ch, bf16=False)
+    l0, g = U.loss_and_grads(p, feat, tgt, levels, ch, bf16=False, loss=loss)
+    pred, _ = U.forward(p, feat, levels, ch, bf16=False)
+    d = pred - tgt
+    assert l0 == pytest.approx(float((d * d).mean() if loss == "l2" else np.abs(d).mean()), rel=1e-12)
     idx = rng.choice(np.flatnonzero(g != 0.0), 40, replace=False)
     eps = 1e-6
     for i in idx:
         q = p.copy()
         q[i] += eps
-        lp, _ = U.loss_and_grads(q, feat, tgt, levels, ch, bf16=False)
+        lp, _ = U.loss_and_grads(q, feat, tgt, levels, ch, bf16=False, loss=loss)
         q[i] -= 2 * eps
-        lm, _ = U.loss_and_grads(q, feat, tgt, levels, ch, bf16=False)
+        lm, _ = U.loss_and_grads(q, feat, tgt, levels, ch, bf16=False, loss=loss)
         fd = (lp - lm) / (2 * eps)
         assert abs(fd - g[i]) <= 1e-5 * max(abs(fd), abs(g[i])) + 1e-12, (i, fd, g[i])
 

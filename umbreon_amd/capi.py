@@ -260,7 +260,7 @@ EXPORTS = [
     "ubg_last_error", "ubg_scene_create", "ubg_scene_create_ex", "ubg_scene_create_opts", "ubg_scene_destroy", "ubg_scene_get_info",
     "ubg_render", "ubg_render_noalloc", "ubg_render_batch", "ubg_intersect", "ubg_occluded", "ubg_occluded_segments", "ubg_object_space_edges",
     "ubg_edges_free", "ubg_unet_create", "ubg_unet_destroy", "ubg_unet_param_count", "ubg_unet_get_params",
-    "ubg_unet_set_params", "ubg_unet_infer", "ubg_unet_train_step", "ubg_pathtrace",
+    "ubg_unet_set_params", "ubg_unet_infer", "ubg_unet_train_step", "ubg_unet_set_loss", "ubg_pathtrace",
 ]
 
 _lib = None
@@ -302,6 +302,8 @@ def load() -> C.CDLL:
     lib.ubgx_host_query.argtypes = [C.POINTER(SceneDesc), C.c_int32, C.c_int64, C.c_void_p, C.c_void_p,
                                     C.c_void_p, C.c_void_p]
     lib.ubgx_host_query.restype = C.c_int
+    lib.ubg_unet_set_loss.argtypes = [C.c_void_p, C.c_int32]
+    lib.ubg_unet_set_loss.restype = C.c_int
     lib.ubgx_unet_set_min_workgroups.argtypes = [C.c_void_p, C.c_int64]
     lib.ubgx_unet_set_min_workgroups.restype = C.c_int
     lib.ubg_scene_destroy.argtypes = [C.c_void_p]

@@ -1919,6 +1919,13 @@ void ubg_unet_destroy(ubg_unet* u) {
 
 int64_t ubg_unet_param_count(const ubg_unet* u) { return u ? u->net.paramCount() : 0; }
 
+int ubg_unet_set_loss(ubg_unet* u, int32_t mode) {
+  if (!u || mode < 0 || mode > 1) return fail(UBG_ERR_INVALID_ARGUMENT, "ubg_unet_set_loss: mode must be 0 or 1");
+  std::lock_guard<std::mutex> lk(u->ctx->mu);
+  u->net.setLossL2(mode == 1);
+  return UBG_OK;
+}
+
 int ubgx_unet_set_min_workgroups(ubg_unet* u, int64_t n) {
   if (!u || n < 0) return fail(UBG_ERR_INVALID_ARGUMENT, "ubgx_unet_set_min_workgroups: bad argument");
   std::lock_guard<std::mutex> lk(u->ctx->mu);

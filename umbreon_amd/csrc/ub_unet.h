@@ -42,8 +42,9 @@ class Unet {
   // features: n x h x w x 10 fp32 (device); out: n x h x w x 3 fp32 log(1 + E)
   // (device). h and w must be multiples of 2^(levels - 1).
   hipError_t forward(int n, int h, int w, const float* feat, float* out, hipStream_t st);
-  // L1 loss of the prediction against target (n x h x w x 3 log(1 + E),
-  // device) and the parameter gradients (device, paramCount floats).
+  // Mean L1 (default) or mean squared loss of the prediction against target
+  // (n x h x w x 3 log(1 + E), device) and the parameter gradients (device,
+  // paramCount floats).
   hipError_t gradients(int n, int h, int w, const float* feat, const float* target, float* loss,
                        hipStream_t st);
   hipError_t copyGrads(float* host, hipStream_t st);
@@ -53,6 +54,8 @@ class Unet {
   // (4 for 32 output channels, else 2) that still give it at least this many
   // workgroups (default 1,024); 0 = always the widest.
   void setMinWorkgroups(int64_t n) { minWgs_ = n; }
+  // Training loss: false = mean L1, true = mean squared error.
+  void setLossL2(bool l2) { lossL2_ = l2; }
 
  private:
   struct Src {
@@ -83,6 +86,7 @@ class Unet {
   int step_ = 0;
   bool bf16Dirty_ = true;
   int64_t minWgs_ = 1024;
+  bool lossL2_ = false;
   // activation scratch for the current shape
   int n_ = 0, h_ = 0, w_ = 0;
   std::vector<uint16_t*> actBf_;   // bf16 activations

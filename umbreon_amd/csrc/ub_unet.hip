@@ -412,10 +412,12 @@ __global__ void k_unet_input(int64_t P, const float* __restrict__ feat, uint16_t
   act[i] = c < kUnetFeatures ? f2bf(feat[p * kUnetFeatures + c]) : static_cast<uint16_t>(0);
 }
 
-// L1 loss: per-block partial sums of |pred - target| and the output gradient
-// sign(pred - target) / count on the 3 used channels
+// Loss: per-block partial sums of |pred - target| (l2 == 0: mean L1) or
+// (pred - target)^2 (l2 == 1: mean squared error) and the output gradient
+// sign(pred - target) / count or 2 (pred - target) / count on the 3 used
+// channels
 __global__ void k_unet_loss(int64_t P, const float* __restrict__ outF, const float* __restrict__ feat,
-                            const float* __restrict__ target, float invCount, float* __restrict__ dOut,
+                            const float* __restrict__ target, float invCount, int l2, float* __restrict__ dOut,
                             float* __restrict__ lossPart) {
   __shared__ float red[256];
   const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -426,8 +428,13 @@ __global__ void k_unet_loss(int64_t P, const float* __restrict__ outF, const flo
     float g = 0.0f;
     if (c < 3) {
       const float d = (outF[i] + feat[p * kUnetFeatures + c]) - target[p * 3 + c];
-      l = fabsf(d);
-      g = d > 0.0f ? invCount : (d < 0.0f ? -invCount : 0.0f);
+      if (l2) {
+        l = d * d;
+        g = 2.0f * d * invCount;
+      } else {
+        l = fabsf(d);
+        g = d > 0.0f ? invCount : (d < 0.0f ? -invCount : 0.0f);
+      }
     }
     dOut[i] = g;
   }
@@ -725,7 +732,7 @@ hipError_t Unet::gradients(int n, int h, int w, const float* feat, const float* 
   const int64_t P0 = static_cast<int64_t>(n) * h * w;
   const unsigned lb = blocks(P0 * kUnetOutPad, 256);
   hipLaunchKernelGGL(k_unet_loss, dim3(lb), dim3(256), 0, st, P0, outF_, feat, target,
-                     1.0f / static_cast<float>(P0 * 3), dOut_, lossPart_);
+                     1.0f / static_cast<float>(P0 * 3), lossL2_ ? 1 : 0, dOut_, lossPart_);
   for (size_t a = 1; a < acts_.size(); ++a) {
     const int64_t P = P0 >> (2 * acts_[a].level);
     e = hipMemsetAsync(actGrad_[a], 0, static_cast<size_t>(P * acts_[a].C) * 4, st);

@@ -47,6 +47,11 @@ class Unet:
         convolution use the widest pixel tiles its rows allow."""
         capi.check(self._lib.ubgx_unet_set_min_workgroups(self.handle, int(n)))
 
+    def set_loss(self, mode: str) -> None:
+        """Training loss: 'l1' (mean absolute error, the default) or 'l2'
+        (mean squared error)."""
+        capi.check(self._lib.ubg_unet_set_loss(self.handle, {"l1": 0, "l2": 1}[mode]))
+
     def get_params(self) -> np.ndarray:
         p = np.empty(self.n_params, np.float32)
         capi.check(self._lib.ubg_unet_get_params(self.handle, p.ctypes.data))

@@ -363,6 +363,8 @@ def main() -> None:
     ap.add_argument("--crop", type=int, default=128)
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--init", default="", help="weights to continue training from")
+    ap.add_argument("--loss", choices=["l1", "l2"], default="l1",
+                    help="training loss on the encoded colour: mean absolute (l1) or squared (l2) error")
     ap.add_argument("--out", default="")
     ap.add_argument("--holdout", type=int, default=2, help="held-out windows to report after training")
     ap.add_argument("--evaluate", default="", help="weights to evaluate on the full workload")
@@ -394,7 +396,8 @@ def main() -> None:
         pool.terminate()
         pool = None
         _say(f"{len(tgts)} molecules, {len(feats)} noisy inputs")
-        net = load(dev, args.init) if args.init else None
+        net = load(dev, args.init) if args.init else Unet(dev, LEVELS, CHANNELS, seed=args.first_seed)
+        net.set_loss(args.loss)
         net = train(dev, feats, owner, tgts, seconds=args.train_seconds, batch=args.batch, crop=args.crop,
                     lr=args.lr, net=net, seed=args.first_seed)
         if args.out:
