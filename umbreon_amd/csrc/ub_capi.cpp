@@ -1554,8 +1554,10 @@ int ubg_pathtrace(ubg_context* ctx, const ubg_scene* scene, const ubg_pt_params*
   }
   const size_t np = static_cast<size_t>(Wp) * Hp;
   const bool wantFeat = net || fr->features;
+  const int nGroups = std::max(1, (p->n_lights + ubg::pt::kLightGroup - 1) / ubg::pt::kLightGroup);
   const size_t tabBytes = sizeof(ubg::pt::Light) * std::max(1, p->n_lights) +
-                          sizeof(ubg::pt::Mat) * (std::max(1, p->n_prim_materials) + std::max(1, p->n_mesh_materials));
+                          sizeof(ubg::pt::Mat) * (std::max(1, p->n_prim_materials) + std::max(1, p->n_mesh_materials)) +
+                          sizeof(float4) * nGroups;
   int r = ensure(ctx->ptAcc, n * 16);
   if (r == UBG_OK) r = ensure(ctx->ptFirst, n * 32);
   if (r == UBG_OK) r = ensure(ctx->ptG, n * sizeof(ubg::pt::GPix));
@@ -1581,12 +1583,34 @@ int ubg_pathtrace(ubg_context* ctx, const ubg_scene* scene, const ubg_pt_params*
   };
   const std::vector<ubg::pt::Mat> pm = mats(p->prim_materials, p->n_prim_materials);
   const std::vector<ubg::pt::Mat> mm = mats(p->mesh_materials, p->n_mesh_materials);
+  // bounding spheres of the lights in groups of kLightGroup (index order),
+  // in double, widened by 1e-4 relative so that rounding in the device's
+  // group test never culls a ray that the per-light test would accept
+  std::vector<float4> groups(static_cast<size_t>(nGroups), make_float4(0.0f, 0.0f, 0.0f, 0.0f));
+  for (int g = 0; g * ubg::pt::kLightGroup < p->n_lights; ++g) {
+    const int l0 = g * ubg::pt::kLightGroup, l1 = std::min(p->n_lights, l0 + ubg::pt::kLightGroup);
+    double c[3] = {0.0, 0.0, 0.0};
+    for (int i = l0; i < l1; ++i)
+      for (int k = 0; k < 3; ++k) c[k] += lights[i].c[k];
+    for (int k = 0; k < 3; ++k) c[k] /= (l1 - l0);
+    double R = 0.0;
+    for (int i = l0; i < l1; ++i) {
+      const double dx = lights[i].c[0] - c[0], dy = lights[i].c[1] - c[1], dz = lights[i].c[2] - c[2];
+      R = std::max(R, std::sqrt(dx * dx + dy * dy + dz * dz) + lights[i].r);
+    }
+    groups[g] = make_float4(static_cast<float>(c[0]), static_cast<float>(c[1]), static_cast<float>(c[2]),
+                            static_cast<float>(R * (1.0 + 1e-4) + 1e-6 * std::max({std::fabs(c[0]), std::fabs(c[1]),
+                                                                                  std::fabs(c[2]), 1.0})));
+  }
   char* tab = static_cast<char*>(ctx->ptTab.p);
   const size_t lb = sizeof(ubg::pt::Light) * lights.size(), pb = sizeof(ubg::pt::Mat) * pm.size();
   UBG_HIP(hipMemcpyAsync(tab, lights.data(), lb, hipMemcpyHostToDevice, st), "hipMemcpy(lights)");
   UBG_HIP(hipMemcpyAsync(tab + lb, pm.data(), pb, hipMemcpyHostToDevice, st), "hipMemcpy(materials)");
   UBG_HIP(hipMemcpyAsync(tab + lb + pb, mm.data(), sizeof(ubg::pt::Mat) * mm.size(), hipMemcpyHostToDevice, st),
           "hipMemcpy(materials)");
+  const size_t gOff = lb + pb + sizeof(ubg::pt::Mat) * mm.size();  // 16-byte aligned (Light 32 B, Mat 16 B)
+  UBG_HIP(hipMemcpyAsync(tab + gOff, groups.data(), sizeof(float4) * groups.size(), hipMemcpyHostToDevice, st),
+          "hipMemcpy(light groups)");
   // camera basis (embree_renderer.cpp:375-386, on the image grid)
   ubg::pt::Params q;
   std::memset(&q, 0, sizeof(q));
@@ -1617,6 +1641,7 @@ int ubg_pathtrace(ubg_context* ctx, const ubg_scene* scene, const ubg_pt_params*
   q.restirRadius = p->restir_radius > 0.0f ? p->restir_radius : 16.0f;
   q.neeAll = p->nee_all_lights ? 1 : 0;
   q.lights = reinterpret_cast<const ubg::pt::Light*>(tab);
+  q.lightGroups = reinterpret_cast<const float4*>(tab + gOff);
   q.primMats = reinterpret_cast<const ubg::pt::Mat*>(tab + lb);
   q.meshMats = reinterpret_cast<const ubg::pt::Mat*>(tab + lb + pb);
   q.nPrimMats = static_cast<int32_t>(pm.size());

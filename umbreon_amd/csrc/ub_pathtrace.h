@@ -16,6 +16,7 @@ constexpr int kMatDielectric = 2;
 constexpr int kMaxSpatial = 16;   // ReSTIR neighbours
 constexpr int kMaxCandidates = 12;
 constexpr int kPtFeatures = 10;
+constexpr int kLightGroup = 8;  // lights per bounding sphere of the BSDF rays' light test
 // Colour encoding of the denoiser's input and output: u = log(1 + 32 c) / 4.
 // Its slope over c follows the sRGB transfer's within a factor of ~1.5 from
 // c = 0.01 to 1 (the images are compared in sRGB), where log(1 + c) would
@@ -51,6 +52,9 @@ struct Params {
   float restirRadius;    // pixels
   int32_t neeAll;        // 1: one cone sample of every light instead of RIS / ReSTIR
   const Light* lights;
+  // bounding spheres of the lights in groups of kLightGroup (index order):
+  // c[3], r; ceil(nLights / kLightGroup) of them
+  const float4* lightGroups;
   const Mat* primMats;
   const Mat* meshMats;   // == primMats + nPrimMats (one table, read by offset)
   int32_t nPrimMats, nMeshMats;

@@ -327,21 +327,34 @@ __device__ __forceinline__ float misNee(const Params& p, const Surf& sf, V3 P, V
 // Light a BSDF ray from vertex P in direction wi (world) collects before the
 // scene hit at tHit, with its weights; pb = the ray's BSDF density (+inf for
 // a delta lobe).
+// The lights are tested in groups of kLightGroup behind their bounding sphere:
+// a ray that misses the group's sphere misses each of its lights, which
+// would add nothing, so the sum (over the lights hit, in index order) is the
+// one of testing every light.
 __device__ __forceinline__ V3 misBsdf(const Params& p, V3 P, V3 wi, float tHit, float pb) {
   V3 sum = mk(0.0f, 0.0f, 0.0f);
-  for (int li = 0; li < p.nLights; ++li) {
-    const Light& L = p.lights[li];
-    const V3 dcv = mk(L.c[0], L.c[1], L.c[2]) - P;
-    const float b = dot(dcv, wi);
-    if (b <= 0.0f) continue;
-    const float d2 = dot(dcv, dcv), r2 = L.r * L.r;
-    const float disc = b * b - (d2 - r2);
-    if (disc < 0.0f || d2 <= r2) continue;
-    const float t = b - sqrtf(disc);
-    if (!(t < tHit)) continue;
-    const float pn = neeDensity(p, L, P);
-    const float w = pb == INFINITY ? 1.0f : pb / (pb + pn);
-    sum = sum + mk(L.Le[0] * w, L.Le[1] * w, L.Le[2] * w);
+  for (int g0 = 0; g0 < p.nLights; g0 += kLightGroup) {
+    const float4 G = p.lightGroups[g0 / kLightGroup];
+    const V3 dg = mk(G.x, G.y, G.z) - P;
+    const float bg = dot(dg, wi), dg2 = dot(dg, dg), rg2 = G.w * G.w;
+    // the forward ray meets the group's sphere: origin inside, or ahead and
+    // within the radius
+    if (dg2 > rg2 && (bg <= 0.0f || bg * bg - (dg2 - rg2) < 0.0f)) continue;
+    const int g1 = min(p.nLights, g0 + kLightGroup);
+    for (int li = g0; li < g1; ++li) {
+      const Light& L = p.lights[li];
+      const V3 dcv = mk(L.c[0], L.c[1], L.c[2]) - P;
+      const float b = dot(dcv, wi);
+      if (b <= 0.0f) continue;
+      const float d2 = dot(dcv, dcv), r2 = L.r * L.r;
+      const float disc = b * b - (d2 - r2);
+      if (disc < 0.0f || d2 <= r2) continue;
+      const float t = b - sqrtf(disc);
+      if (!(t < tHit)) continue;
+      const float pn = neeDensity(p, L, P);
+      const float w = pb == INFINITY ? 1.0f : pb / (pb + pn);
+      sum = sum + mk(L.Le[0] * w, L.Le[1] * w, L.Le[2] * w);
+    }
   }
   return sum;
 }
