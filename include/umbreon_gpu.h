@@ -327,7 +327,8 @@ typedef struct ubg_frame {
   /* The learned denoiser's input features of the gathered (not yet
    * denoised) irradiance, hi_w * hi_h * 10: log(1 + E) rgb, albedo rgb,
    * normal xyz, first-hit distance / its frame maximum; may be NULL; needs
-   * the full-resolution gather. */
+   * the full-resolution gather. The network (ubg_unet_*) reads 12 features
+   * per pixel: these 10, then two zeros for the noise descriptors. */
   float* gi_features;
 } ubg_frame;
 
@@ -486,10 +487,14 @@ typedef struct ubg_pt_params {
 
 typedef struct ubg_pt_frame {
   float* color;           /* width * height * 4 (linear HDR, alpha 1) */
-  float* features;        /* width * height * 10 or NULL: log(1 + 32 c) / 4 of the noisy
+  float* features;        /* width * height * 12 or NULL: log(1 + 32 c) / 4 of the noisy
                              colour c (rgb),
                              albedo guide rgb, normal guide xyz (camera space: right,
-                             up, towards the eye), first-hit distance / frame max.
+                             up, towards the eye), first-hit distance / frame max,
+                             1 / sqrt(spp), and the standard error of the pixel's mean
+                             (r + g + b) / 3 over its spp samples (unbiased variance,
+                             0 at 1 spp) times the encoding's slope at that mean,
+                             8 / (1 + 32 m), at most 4.
                              The guides are the first hit's, or for a dielectric
                              first hit F * white + (1 - F) * the tinted colour of the
                              first other surface along the ideal refraction (up to 4
@@ -609,9 +614,11 @@ void ubg_edges_free(ubg_edges* edges);
 /* The learned denoiser: a U-Net over the GI irradiance field and its guides
  * (it replaces denoisePt1E / OIDN, pt1_denoise.cpp:20-81), with bf16 MFMA
  * convolutions and fp32 accumulation, trained on the device. levels 2..6,
- * channels[l] in {32, 64, 96, 128}. Features per pixel (NHWC fp32, 10):
- * log(1 + E) rgb, albedo rgb, normal xyz, depth / max depth; outputs and
- * targets (3): log(1 + E). h and w must be multiples of 2^(levels - 1).
+ * channels[l] in {32, 64, 96, 128}. Features per pixel (NHWC fp32, 12):
+ * log(1 + E) rgb, albedo rgb, normal xyz, depth / max depth, and two noise
+ * descriptors (the path tracer's 1 / sqrt(spp) and colour standard error,
+ * ubg_pt_frame::features; zero for the GI gather); outputs and targets (3):
+ * log(1 + E). h and w must be multiples of 2^(levels - 1).
  * Parameters are fp32, per convolution W[9][cout][cin] then b[cout]. */
 typedef struct ubg_unet ubg_unet;
 int ubg_unet_create(ubg_context* ctx, int32_t levels, const int32_t* channels, uint64_t seed, ubg_unet** out);

@@ -18,7 +18,7 @@ import numpy as np
 
 IN_PAD = 32
 OUT_PAD = 32
-N_FEATURES = 10
+N_FEATURES = 12  # log(1 + E) rgb, albedo rgb, normal xyz, depth, two noise descriptors
 
 
 def to_bf16(x: np.ndarray) -> np.ndarray:
@@ -116,7 +116,8 @@ def forward(params: np.ndarray, feat: np.ndarray, levels: int, channels, bf16: b
     """Predicted log(1 + E) (n, h, w, 3) and the cache for backward."""
     acts, convs = layout(levels, channels)
     rnd = to_bf16 if bf16 else (lambda v: np.asarray(v, np.float64))
-    n, h, w, _ = feat.shape
+    n, h, w, nf = feat.shape
+    assert nf == N_FEATURES, nf
     x0 = np.zeros((n, h, w, IN_PAD))
     x0[..., :N_FEATURES] = rnd(feat)
     vals = {0: x0}

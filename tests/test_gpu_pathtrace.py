@@ -11,7 +11,8 @@ of tests/test_pt_oracle.py:
   record (the guides followed through dielectrics: at 99 % of pixels within
   1e-4), the guides through a glass sphere equal their closed form, and the
   denoised colour is the network's prediction on those features decoded
-  (expm1(4 u) / 32);
+  (expm1(4 u) / 32); the noise descriptors equal 1 / sqrt(spp) and numpy's
+  standard error over the per-sample values;
 * white furnace of a smooth dielectric sphere (exactly 1), the Fresnel plane
   head-on, ReSTIR's mean against the all-lights estimator at 256 spp;
 * multiple importance sampling of the lights: a glossy plane's direct light
@@ -19,6 +20,8 @@ of tests/test_pt_oracle.py:
 * argument refusals.
 """
 from __future__ import annotations
+
+from dataclasses import replace
 
 import numpy as np
 import pytest
@@ -79,6 +82,35 @@ def test_parity_with_oracle(device, surface):
     dg = np.abs(f[..., 3:9] - first[..., [0, 1, 2, 4, 5, 6]]).max(-1)
     print(f"guides: mean|d| {dg.mean():.3g}, pixels |d| > 1e-4 {(dg > 1e-4).mean():.4f}", flush=True)
     assert (dg > 1e-4).mean() <= 0.01 and dg.mean() <= 2e-3
+
+
+def test_noise_descriptors_are_the_sample_spread(device, surface):
+    """Features 10-11: 1 / sqrt(n) and the standard error of the mean of the
+    samples' (r + g + b) / 3 (unbiased variance), times the encoding's slope
+    8 / (1 + 32 m) at that mean, at most 4. The samples of pass k do not
+    depend on how many follow, so sample k's value is k c_k - (k - 1) c_(k-1)
+    from the colours at k and k - 1 spp; numpy's variance of those values is
+    the check."""
+    sc, opt = surface
+    n = 4
+    bs = _built(device, sc)
+    try:
+        cols = [pathtrace(bs, sc, replace(opt, spp=k)).color[..., :3].astype(np.float64) for k in range(1, n)]
+        g = pathtrace(bs, sc, replace(opt, spp=n), want_features=True)
+    finally:
+        bs.close()
+    cols.append(g.color[..., :3].astype(np.float64))
+    samples = [cols[0]] + [(k + 1) * cols[k] - k * cols[k - 1] for k in range(1, n)]
+    lum = np.stack([s_.mean(-1) for s_ in samples])  # (n, H, W)
+    m = lum.mean(0)
+    se = np.sqrt(lum.var(0, ddof=1) / n)
+    want = np.minimum(4.0, 8.0 * se / (1.0 + 32.0 * np.maximum(m, 0.0)))
+    f = g.features
+    assert np.all(f[..., 10] == np.float32(1.0 / np.sqrt(n)))
+    d = np.abs(f[..., 11] - want)
+    print(f"noise descriptor: mean {want.mean():.3g}, mean|d| {d.mean():.3g}, max {d.max():.3g}", flush=True)
+    assert want.mean() > 1e-3  # a noisy frame (the background pixels have no spread)
+    assert d.mean() <= 1e-4 and (d > 2e-3).mean() <= 1e-3
 
 
 def test_denoised_colour_is_the_network_on_the_features(device, surface):

@@ -20,12 +20,14 @@ def device():
 
 
 def _batch(rng, n, h, w):
-    feat = np.zeros((n, h, w, 10), np.float32)
+    feat = np.zeros((n, h, w, 12), np.float32)
     feat[..., 0:3] = rng.uniform(0.0, 2.0, (n, h, w, 3))
     feat[..., 3:6] = rng.uniform(0.0, 1.0, (n, h, w, 3))
     nrm = rng.normal(size=(n, h, w, 3))
     feat[..., 6:9] = nrm / np.linalg.norm(nrm, axis=-1, keepdims=True)
     feat[..., 9] = rng.uniform(0.0, 1.0, (n, h, w))
+    feat[..., 10] = rng.uniform(1.0 / 64, 1.0, (n, h, w))  # 1 / sqrt(spp)
+    feat[..., 11] = rng.uniform(0.0, 0.5, (n, h, w))  # colour standard error
     tgt = (feat[..., 0:3] * 0.5 + 0.2).astype(np.float32)
     return feat, tgt
 

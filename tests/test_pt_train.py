@@ -16,7 +16,7 @@ def _sphere_normals(n=33):
     c = (n - 1) / 2
     x, y = (xx - c) / c, -(yy - c) / c  # columns run along +right, rows along -up
     z = np.sqrt(np.maximum(0.0, 1.0 - x * x - y * y))
-    f = np.zeros((1, n, n, 10))
+    f = np.zeros((1, n, n, 12))
     f[0, ..., 6], f[0, ..., 7], f[0, ..., 8] = x, y, z
     f[0, ..., 0] = xx  # a channel that is not a normal moves without changing sign
     return f

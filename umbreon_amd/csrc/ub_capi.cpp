@@ -716,7 +716,7 @@ static int renderFrame(ubg_context* ctx, const ubg_scene* scene, const ubg_rende
     if (p->pt1_denoise == 2) {
       const int m = 1 << (p->pt1_denoise_net->net.levels() - 1);
       const size_t np = static_cast<size_t>((W + m - 1) / m * m) * ((H + m - 1) / m * m);
-      need(ctx->giFeatP, np * 40);
+      need(ctx->giFeatP, np * ubg::kUnetFeatures * 4);
       need(ctx->giPred, np * 12);
     }
     if (fr->gi_occlusion) need(ctx->giOcc, n * 4);
@@ -1020,7 +1020,7 @@ static int renderFrame(ubg_context* ctx, const ubg_scene* scene, const ubg_rende
         ubg::launchGiList(full, nullptr, true, b, st);
         timed(UBG_KERNEL_AO, [&] { ubg::launchGiGather(fc, scene->view, b, full, countVisits, st); });
         if (fr->gi_features)
-          ubg::launchGiFeatures(W, H, W, H, b, static_cast<unsigned int*>(ctx->giMax.p),
+          ubg::launchGiFeatures(W, H, W, H, b, static_cast<unsigned int*>(ctx->giMax.p), 10,
                                 static_cast<float*>(ctx->giFeat.p), st);
         if (p->pt1_denoise == 2) {
           // the learned denoiser in place of the a-trous filter: features on
@@ -1029,7 +1029,7 @@ static int renderFrame(ubg_context* ctx, const ubg_scene* scene, const ubg_rende
           ubg::Unet& net = p->pt1_denoise_net->net;
           const int m = 1 << (net.levels() - 1);
           const int Wp = (W + m - 1) / m * m, Hp = (H + m - 1) / m * m;
-          ubg::launchGiFeatures(W, H, Wp, Hp, b, static_cast<unsigned int*>(ctx->giMax.p),
+          ubg::launchGiFeatures(W, H, Wp, Hp, b, static_cast<unsigned int*>(ctx->giMax.p), ubg::kUnetFeatures,
                                 static_cast<float*>(ctx->giFeatP.p), st);
           UBG_HIP(net.forward(1, Hp, Wp, static_cast<const float*>(ctx->giFeatP.p), static_cast<float*>(ctx->giPred.p),
                               st),
@@ -1553,6 +1553,7 @@ int ubg_pathtrace(ubg_context* ctx, const ubg_scene* scene, const ubg_pt_params*
     Hp = (H + m - 1) / m * m;
   }
   const size_t np = static_cast<size_t>(Wp) * Hp;
+  static_assert(ubg::pt::kPtFeatures == ubg::kUnetFeatures, "the network reads the path tracer's features");
   const bool wantFeat = net || fr->features;
   const int nGroups = std::max(1, (p->n_lights + ubg::pt::kLightGroup - 1) / ubg::pt::kLightGroup);
   const size_t tabBytes = sizeof(ubg::pt::Light) * std::max(1, p->n_lights) +
@@ -1672,8 +1673,9 @@ int ubg_pathtrace(ubg_context* ctx, const ubg_scene* scene, const ubg_pt_params*
   if (fr->memory == 0) r = copyOut(fr->color, color, n * 16, 0, st);
   if (r == UBG_OK && fr->features) {
     // the unpadded rows of the (possibly padded) feature image
-    UBG_HIP(hipMemcpy2DAsync(fr->features, static_cast<size_t>(W) * 40, feat, static_cast<size_t>(Wp) * 40,
-                             static_cast<size_t>(W) * 40, static_cast<size_t>(H),
+    const size_t px = sizeof(float) * ubg::pt::kPtFeatures;
+    UBG_HIP(hipMemcpy2DAsync(fr->features, static_cast<size_t>(W) * px, feat, static_cast<size_t>(Wp) * px,
+                             static_cast<size_t>(W) * px, static_cast<size_t>(H),
                              fr->memory ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, st),
             "hipMemcpy(features)");
   }

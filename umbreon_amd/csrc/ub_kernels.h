@@ -184,11 +184,12 @@ void launchPrimary(const FrameConst& fc, const SceneView& s, const FrameBufs& b,
                    const PartFill& fill, bool countVisits, hipStream_t st);
 // AO and resolve read the hit-queue fills on the device (no host round trip).
 void launchAo(const FrameConst& fc, const SceneView& s, const FrameBufs& b, bool countVisits, hipStream_t st);
-// The learned denoiser's features of the render-grid gather (Wp x Hp x 10,
-// edge-replicated beyond W x H; maxBits: one scratch word) and the network's
-// log(1 + E) back onto the receivers' E.
-void launchGiFeatures(int W, int H, int Wp, int Hp, const FrameBufs& b, unsigned int* maxBits, float* feat,
-                      hipStream_t st);
+// The learned denoiser's features of the render-grid gather (Wp x Hp x
+// stride: 10 for the caller's copy, 12 = kUnetFeatures for the network with
+// the two noise descriptors zero; edge-replicated beyond W x H; maxBits: one
+// scratch word) and the network's log(1 + E) back onto the receivers' E.
+void launchGiFeatures(int W, int H, int Wp, int Hp, const FrameBufs& b, unsigned int* maxBits, int stride,
+                      float* feat, hipStream_t st);
 void launchGiApplyNet(int W, int H, int Wp, const float* pred, const FrameBufs& b, hipStream_t st);
 // Point-in-solid tests of the object-space edge clip over a uniform grid
 // (spheres x, y, z, r; cylinders p0.xyz r, p1.xyz 0; points x, y, z, ring

@@ -2504,8 +2504,10 @@ __global__ void k_aa_compose(FrameConst fc, AaSet cs, AaSet rs, AaSet fs, const 
 // The learned denoiser's input features on the render grid, from the gather
 // before any denoise: log(1 + E) rgb, the receiver's albedo (its diffuse
 // reflectance), the normal AOV, and the first-hit distance over the frame's
-// largest (k_gi_depth_max). Pixels of the Wp x Hp feature grid beyond W x H
-// (the network's padding) repeat the nearest edge pixel.
+// largest (k_gi_depth_max); with stride 12 (the network's input) two noise
+// descriptors follow as zeros (the gather records no per-sample spread).
+// Pixels of the Wp x Hp feature grid beyond W x H (the network's padding)
+// repeat the nearest edge pixel.
 __global__ void k_gi_depth_max(int64_t n, const float* __restrict__ depth, unsigned int* __restrict__ maxBits) {
   const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   unsigned int v = 0u;
@@ -2516,7 +2518,7 @@ __global__ void k_gi_depth_max(int64_t n, const float* __restrict__ depth, unsig
 __global__ void k_gi_features(int W, int H, int Wp, int Hp, const float4* __restrict__ E,
                               const float4* __restrict__ refl, const float4* __restrict__ nrm,
                               const float* __restrict__ depth, const unsigned int* __restrict__ maxBits,
-                              float* __restrict__ feat) {
+                              int stride, float* __restrict__ feat) {
   const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= static_cast<int64_t>(Wp) * Hp) return;
   const int x = min(static_cast<int>(i % Wp), W - 1), y = min(static_cast<int>(i / Wp), H - 1);
@@ -2524,7 +2526,7 @@ __global__ void k_gi_features(int W, int H, int Wp, int Hp, const float4* __rest
   const float4 e = E[q], a = refl[q], nn = nrm[q];
   const bool recv = nn.w != 0.0f;
   const float dmax = __uint_as_float(*maxBits);
-  float* f = feat + i * 10;
+  float* f = feat + i * stride;
   f[0] = log1pf(fmaxf(e.x, 0.0f));
   f[1] = log1pf(fmaxf(e.y, 0.0f));
   f[2] = log1pf(fmaxf(e.z, 0.0f));
@@ -2535,6 +2537,10 @@ __global__ void k_gi_features(int W, int H, int Wp, int Hp, const float4* __rest
   f[7] = nn.y;
   f[8] = nn.z;
   f[9] = dmax > 0.0f ? depth[q] / dmax : 0.0f;
+  if (stride == 12) {
+    f[10] = 0.0f;
+    f[11] = 0.0f;
+  }
 }
 // the network's log(1 + E) back onto the receivers' E (the gather's hit
 // fraction in E.w is kept)
@@ -2616,14 +2622,14 @@ __global__ void k_reset_queues(unsigned long long* c) {
 static inline unsigned blocksFor(int64_t n, int bs) {
   return static_cast<unsigned>((n + bs - 1) / bs);
 }
-void launchGiFeatures(int W, int H, int Wp, int Hp, const FrameBufs& b, unsigned int* maxBits, float* feat,
-                      hipStream_t st) {
+void launchGiFeatures(int W, int H, int Wp, int Hp, const FrameBufs& b, unsigned int* maxBits, int stride,
+                      float* feat, hipStream_t st) {
   const int64_t n = static_cast<int64_t>(W) * H;
   hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, st, maxBits, 1);
   hipLaunchKernelGGL(k_gi_depth_max, dim3(blocksFor(n, 256)), dim3(256), 0, st, n, b.depth, maxBits);
   const int64_t np = static_cast<int64_t>(Wp) * Hp;
   hipLaunchKernelGGL(k_gi_features, dim3(blocksFor(np, 256)), dim3(256), 0, st, W, H, Wp, Hp, b.giE, b.giRefl,
-                     b.giNrm, b.depth, maxBits, feat);
+                     b.giNrm, b.depth, maxBits, stride, feat);
 }
 void launchGiApplyNet(int W, int H, int Wp, const float* pred, const FrameBufs& b, hipStream_t st) {
   const int64_t n = static_cast<int64_t>(W) * H;

@@ -15,7 +15,7 @@ constexpr int kMatConductor = 1;
 constexpr int kMatDielectric = 2;
 constexpr int kMaxSpatial = 16;   // ReSTIR neighbours
 constexpr int kMaxCandidates = 12;
-constexpr int kPtFeatures = 10;
+constexpr int kPtFeatures = 12;  // the denoiser's input features per pixel
 constexpr int kLightGroup = 8;  // lights per bounding sphere of the BSDF rays' light test
 // Colour encoding of the denoiser's input and output: u = log(1 + 32 c) / 4.
 // Its slope over c follows the sRGB transfer's within a factor of ~1.5 from
@@ -79,9 +79,10 @@ struct Reservoir {
 void ptZero(float4* acc, int n, unsigned int* maxBits, unsigned long long* rays, hipStream_t st);
 void ptSample(const pt::Params& p, const SceneView& s, pt::GPix* gb, pt::Reservoir* res, float4* acc, float4* first,
               hipStream_t st);
-// colour = acc / spp (W x H); with feat: the Wp x Hp edge-padded denoiser
-// features as well (maxBits: the maximum first-hit distance k_pt_primary
-// recorded in Params::depthMax).
+// colour = acc.xyz / spp (W x H; acc.w holds the sum of the samples' squared
+// mean-of-rgb); with feat: the Wp x Hp edge-padded denoiser features as well
+// (maxBits: the maximum first-hit distance k_pt_guides recorded in
+// Params::depthMax).
 void ptResolve(int W, int H, int Wp, int Hp, int spp, const float4* acc, const float4* first,
                const unsigned int* maxBits, float4* color, float* feat, hipStream_t st);
 void ptApplyNet(int W, int H, int Wp, const float* pred, float4* color, hipStream_t st);

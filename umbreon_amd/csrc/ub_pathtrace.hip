@@ -629,8 +629,9 @@ __device__ __forceinline__ uint32_t primaryPixel(const Params& p, const SceneVie
     g.P = make_float4(0.0f, 0.0f, 0.0f, 0.0f);  // w = 0: no surface
     gb[pix] = g;
     res[pix] = r;
+    const float l = (p.env[0] + p.env[1] + p.env[2]) * (1.0f / 3.0f);
     float4 a = acc[pix];
-    acc[pix] = make_float4(a.x + p.env[0], a.y + p.env[1], a.z + p.env[2], 0.0f);
+    acc[pix] = make_float4(a.x + p.env[0], a.y + p.env[1], a.z + p.env[2], a.w + l * l);
     return 1u;
   }
   V3 Ng;
@@ -812,8 +813,9 @@ __device__ __forceinline__ uint32_t shadePixel(const Params& p, const SceneView&
     const V3 Ld = directLight(p, s, sf, P, Ng, wo, tPrev, upix, smp, b, nr);
     L = L + mk(thr.x * Ld.x, thr.y * Ld.y, thr.z * Ld.z);
   }
+  const float l = (L.x + L.y + L.z) * (1.0f / 3.0f);
   float4 a = acc[pix];
-  acc[pix] = make_float4(a.x + L.x, a.y + L.y, a.z + L.z, 0.0f);
+  acc[pix] = make_float4(a.x + L.x, a.y + L.y, a.z + L.z, a.w + l * l);
   return nr;
 }
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kShadeWaves))) void k_pt_shade(
@@ -824,8 +826,12 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kShadeWa
 }
 
 // Mean over the samples, and the denoiser features: the encoded colour
-// log(1 + 32 c) / 4, albedo and normal guides, first-hit distance / its frame
-// maximum (maxBits: that maximum as float bits, from k_pt_primary).
+// u = log(1 + 32 c) / 4, albedo and normal guides, first-hit distance / its
+// frame maximum (maxBits: that maximum as float bits, from k_pt_guides), and
+// two noise descriptors: 1 / sqrt(n) for the n samples, and the standard error
+// of the mean of the samples' (r + g + b) / 3, s = sqrt(var / n) with the
+// unbiased var over the n samples (0 for n = 1), carried into u through its
+// slope at the mean m: 8 s / (1 + 32 m), at most 4.
 __global__ void k_pt_resolve(int W, int H, int Wp, int Hp, int spp, const float4* __restrict__ acc,
                              const float4* __restrict__ first, const unsigned int* __restrict__ maxBits,
                              float4* __restrict__ color, float* __restrict__ feat) {
@@ -849,6 +855,11 @@ __global__ void k_pt_resolve(int W, int H, int Wp, int Hp, int spp, const float4
     o[7] = f1.y;
     o[8] = f1.z;
     o[9] = mx > 0.0f ? f0.w / mx : 0.0f;
+    const float m = (a.x + a.y + a.z) * (1.0f / 3.0f) * inv;
+    const float var = spp > 1 ? fmaxf(0.0f, a.w * inv - m * m) * static_cast<float>(spp) / static_cast<float>(spp - 1)
+                              : 0.0f;
+    o[10] = sqrtf(inv);
+    o[11] = fminf(4.0f, kColourScale * kColourGain * sqrtf(var * inv) / (1.0f + kColourGain * fmaxf(0.0f, m)));
     if (color && i < W * H) {
       const float4 c = acc[i];
       color[i] = make_float4(c.x * inv, c.y * inv, c.z * inv, 1.0f);

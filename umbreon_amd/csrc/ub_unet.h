@@ -3,8 +3,9 @@
 // and training on the device.
 //
 // Network (levels L, feature channels c[0..L-1], every count a multiple of
-// 32): input 10 features (log(1 + E) rgb, albedo rgb, normal xyz, depth)
-// padded to 32 channels;
+// 32): input 12 features (log(1 + E) rgb, albedo rgb, normal xyz, depth, and
+// two noise descriptors: 1 / sqrt(samples) and the colour's standard error,
+// zero where the producer has none) padded to 32 channels;
 //   encoder level l: conv3x3(l == 0 ? input : avgpool2(e[l-1])) + ReLU,
 //                    conv3x3 + ReLU -> e[l]
 //   decoder level l = L-2 .. 0: conv3x3(concat(e[l], up2(d[l+1]))) + ReLU,
@@ -24,7 +25,7 @@
 namespace ubg {
 
 constexpr int kUnetMaxLevels = 6;
-constexpr int kUnetFeatures = 10;  // input features per pixel
+constexpr int kUnetFeatures = 12;  // input features per pixel
 constexpr int kUnetInPad = 32;     // padded input channels
 constexpr int kUnetOutPad = 32;    // padded output channels (3 used)
 

@@ -66,13 +66,15 @@ __device__ __forceinline__ uint4 avg4(uint4 a, uint4 b, uint4 c, uint4 d) {
 // 8 consecutive channels c .. c + 7 of the operand at (n, y, x) in the
 // convolution's pixel grid
 __device__ __forceinline__ uint4 load8(const SrcDev& s, int n, int y, int x, int c) {
-  if (s.mode == 3) {  // channels c .. c + 7 of 10 (c = 0, 8, 16, 24); 8-byte aligned pairs
+  if (s.mode == 3) {  // channels c .. c + 7 of 12 (c = 0, 8, 16, 24); 8-byte aligned pairs
+    static_assert(kUnetFeatures == 12, "the feature loads below read 12 channels");
     const float2* f = reinterpret_cast<const float2*>(reinterpret_cast<const float*>(s.p) +
                                                       ((static_cast<size_t>(n) * s.H + y) * s.W + x) * kUnetFeatures);
     if (c >= 16) return make_uint4(0u, 0u, 0u, 0u);
     if (c == 8) {
-      const float2 a = f[4];
-      return make_uint4(f2bf(a.x) | static_cast<uint32_t>(f2bf(a.y)) << 16, 0u, 0u, 0u);
+      const float2 a = f[4], b = f[5];
+      return make_uint4(f2bf(a.x) | static_cast<uint32_t>(f2bf(a.y)) << 16,
+                        f2bf(b.x) | static_cast<uint32_t>(f2bf(b.y)) << 16, 0u, 0u);
     }
     const float2 a = f[0], b = f[1], d = f[2], e = f[3];
     return make_uint4(f2bf(a.x) | static_cast<uint32_t>(f2bf(a.y)) << 16, f2bf(b.x) | static_cast<uint32_t>(f2bf(b.y)) << 16,
@@ -403,7 +405,7 @@ __global__ void k_scatter(const float* __restrict__ dx, int Cin, int off, int N,
   dS[i] += v;
 }
 
-// input features (fp32, 10 per pixel) -> bf16, padded to kUnetInPad
+// input features (fp32, kUnetFeatures per pixel) -> bf16, padded to kUnetInPad
 __global__ void k_unet_input(int64_t P, const float* __restrict__ feat, uint16_t* __restrict__ act) {
   const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= P * kUnetInPad) return;

@@ -2,8 +2,10 @@
 irradiance field E and its guides that stands in for denoisePt1E / OIDN
 (src/umbreon/integrator/pt1/pt1_denoise.cpp:20-81), trained on the device.
 
-Features per pixel (NHWC float32, 10 channels): log(1 + E) rgb, albedo rgb,
-normal xyz, depth / max depth; the network predicts log(1 + E).
+Features per pixel (NHWC float32, 12 channels): log(1 + E) rgb, albedo rgb,
+normal xyz, depth / max depth, and two noise descriptors (the path tracer's
+1 / sqrt(spp) and colour standard error); the GI gather's 10 features are
+passed with the last two zero. The network predicts log(1 + E).
 """
 from __future__ import annotations
 
@@ -14,7 +16,19 @@ import numpy as np
 
 from . import capi
 
-N_FEATURES = 10
+N_FEATURES = 12
+GI_FEATURES = 10  # the GI gather's features (no noise descriptors)
+
+
+def network_features(features: np.ndarray) -> np.ndarray:
+    """(..., 12) float32 network input: 12-channel features as they are,
+    the GI gather's 10 with two zero noise descriptors appended."""
+    f = np.asarray(features, np.float32)
+    if f.shape[-1] == GI_FEATURES:
+        f = np.concatenate([f, np.zeros(f.shape[:-1] + (N_FEATURES - GI_FEATURES,), np.float32)], axis=-1)
+    if f.shape[-1] != N_FEATURES:
+        raise ValueError(f"expected {GI_FEATURES} or {N_FEATURES} features per pixel, got {f.shape[-1]}")
+    return np.ascontiguousarray(f)
 
 
 class Unet:
@@ -64,17 +78,16 @@ class Unet:
         capi.check(self._lib.ubg_unet_set_params(self.handle, p.ctypes.data))
 
     def infer(self, features: np.ndarray) -> np.ndarray:
-        """features (n, h, w, 10) -> predicted log(1 + E) (n, h, w, 3)."""
-        f = np.ascontiguousarray(features, np.float32)
-        n, h, w, c = f.shape
-        assert c == N_FEATURES
+        """features (n, h, w, 12 or 10) -> predicted log(1 + E) (n, h, w, 3)."""
+        f = network_features(features)
+        n, h, w, _ = f.shape
         out = np.empty((n, h, w, 3), np.float32)
         capi.check(self._lib.ubg_unet_infer(self.handle, n, h, w, f.ctypes.data, out.ctypes.data, 0))
         return out
 
     def train_step(self, features: np.ndarray, targets: np.ndarray, lr: float,
                    want_grads: bool = False) -> Tuple[float, Optional[np.ndarray]]:
-        f = np.ascontiguousarray(features, np.float32)
+        f = network_features(features)
         t = np.ascontiguousarray(targets, np.float32)
         n, h, w, _ = f.shape
         loss = C.c_float(0.0)

@@ -51,7 +51,7 @@ class PtOptions:
 @dataclass
 class PtResult:
     color: np.ndarray                 # H x W x 4 linear HDR
-    features: Optional[np.ndarray]    # H x W x 10 or None
+    features: Optional[np.ndarray]    # H x W x 12 or None
     trace_seconds: float
     denoise_seconds: float
     rays: int
@@ -100,11 +100,11 @@ def pt_params(camera: Camera, opt: PtOptions, denoise_net=None):
 def pathtrace(built, scene: Scene, opt: PtOptions, want_features: bool = False, denoise_net=None) -> PtResult:
     """One path-traced image of a built scene (umbreon_amd.render.BuiltScene)
     with the scene's camera. denoise_net (an umbreon_amd.denoise.Unet whose
-    input is the 10 features) replaces the colour by its denoising."""
+    input is the 12 features) replaces the colour by its denoising."""
     lib = capi.load()
     p, keep = pt_params(scene.camera, opt, denoise_net)
     color = np.empty((opt.height, opt.width, 4), np.float32)
-    feats = np.empty((opt.height, opt.width, 10), np.float32) if want_features else None
+    feats = np.empty((opt.height, opt.width, 12), np.float32) if want_features else None
     fr = capi.PtFrame()
     fr.color = color.ctypes.data
     fr.features = feats.ctypes.data if feats is not None else None

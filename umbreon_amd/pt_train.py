@@ -13,8 +13,10 @@ with their own materials and lights) seen through 512x512 windows of the
 full-frame camera rule -- the 1920x1080, 40-degree framing of the molecule,
 from a random side, narrowed to the window's share of that frame and aimed
 at a random point of the surface -- so a pixel covers about what it covers
-in the full frame. Noisy inputs are the path tracer's 10 denoiser features
-at 1, 2, 4 and 8 samples per pixel; the target is the encoded colour
+in the full frame. Noisy inputs are the path tracer's 12 denoiser features
+(encoded colour, guides, and the noise descriptors 1 / sqrt(spp) and the
+colour's standard error) at 1, 2, 4 and 8 samples per pixel; the target is
+the encoded colour
 log(1 + 32 c) / 4 at 4096 samples. The network (5 levels, 32-128 channels,
 bf16 MFMA convolutions with fp32 accumulation) predicts the encoded colour
 from the features; the L1 loss on that encoding weighs dark colours about as
@@ -109,7 +111,7 @@ def make_pairs(dev: Device, seeds: Sequence[int], size: int = WINDOW, target_spp
                noisy_spp: Sequence[int] = NOISY_SPP, seconds: Optional[float] = None, pool=None, log=None):
     """Render training pairs until the seeds or the time budget run out.
 
-    Returns (features (n * len(noisy_spp), size, size, 10) float16, owner
+    Returns (features (n * len(noisy_spp), size, size, 12) float16, owner
     (n * len(noisy_spp),) index of each input's target, targets (n, size,
     size, 3) float32 encoded colour). With a scene_pool the next molecules
     are generated on the host while the device renders."""
