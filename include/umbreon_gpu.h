@@ -40,7 +40,7 @@
 extern "C" {
 #endif
 
-#define UBG_ABI_VERSION 7
+#define UBG_ABI_VERSION 8
 
 /* Status codes. */
 enum {

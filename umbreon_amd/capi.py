@@ -23,7 +23,7 @@ UBG_ERR_UNSUPPORTED = 5
 UBG_ERR_INTERNAL = 6
 
 # UBG_ABI_VERSION of include/umbreon_gpu.h this mirror is written against
-ABI_VERSION = 7
+ABI_VERSION = 8
 
 BVH2 = 0
 BVH8 = 1
