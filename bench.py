@@ -357,7 +357,11 @@ def main() -> int:
     roofline = {
         "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 4),
-        "traffic": None if per_launch is None else int(per_launch * launches),
+        # HBM bytes per launch (mean over the frame's launches, one per
+        # transparency layer) beside the algorithmic bytes per launch
+        "traffic": None if per_launch is None else int(per_launch),
+        "algorithmic_bytes_per_launch": int(bytes_per_frame / max(1, launches)),
+        "traffic_per_frame": None if per_launch is None else int(per_launch * launches),
         "kernel": kname,
         "compute": measured_compute(traffic_key, kname),
         "kernel_ms_per_frame": round(dur_s * 1e3, 4),
