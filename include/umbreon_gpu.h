@@ -482,7 +482,13 @@ typedef struct ubg_pt_params {
   int32_t nee_all_lights;            /* 1: one cone sample of every light at every vertex */
   struct ubg_unet* denoise_net;      /* optional: the colour is the network's denoising, its
                                         prediction u decoded as c = expm1(4 u) / 32 */
-  int32_t reserved[8];
+  int32_t denoise_ensemble;          /* 1, 2, 4 or 8: u is the mean of the network's
+                                        predictions on that many flips / quarter turns of the
+                                        features (identity, mirrored columns, mirrored rows,
+                                        both, then the same after a transpose), each turned
+                                        back; 0 = 8 from 32 spp on (the extra network passes
+                                        then cost < 1 % of the frame), else 1 */
+  int32_t reserved[7];
 } ubg_pt_params;
 
 typedef struct ubg_pt_frame {

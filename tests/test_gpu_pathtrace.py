@@ -129,6 +129,30 @@ def test_denoised_colour_is_the_network_on_the_features(device, surface):
         want = np.maximum(np.expm1(4 * pred) / 32, 0.0)
         np.testing.assert_allclose(den.color[..., :3], want, rtol=1e-5, atol=1e-6)
         assert den.denoise_seconds > 0
+        # below 32 spp the default ensemble is the single pass
+        one = pathtrace(bs, sc, replace(opt, denoise_ensemble=1), denoise_net=net)
+        assert np.array_equal(one.color, den.color)
+        # the self-ensemble: the mean of the network's predictions on the
+        # turned features (pt_train._dihedral, normals turned with the pixels),
+        # each turned back
+        from umbreon_amd.pt_train import _dihedral
+
+        def unturn(a, k):
+            if k & 2:
+                a = a[:, ::-1]
+            if k & 1:
+                a = a[:, :, ::-1]
+            if k & 4:
+                a = a.transpose(0, 2, 1, 3)
+            return a
+
+        for passes in (4, 8):
+            ens = pathtrace(bs, sc, replace(opt, denoise_ensemble=passes), denoise_net=net)
+            u = np.mean([unturn(net.infer(np.ascontiguousarray(_dihedral(feat[None], k, normals=True))), k)
+                         for k in range(passes)], axis=0)[0, :H, :W]
+            want = np.maximum(np.expm1(4 * u) / 32, 0.0)
+            np.testing.assert_allclose(ens.color[..., :3], want, rtol=2e-5, atol=2e-6)
+            assert not np.allclose(ens.color, den.color)
     finally:
         net.close()
         bs.close()
@@ -236,7 +260,7 @@ def test_refusals(device, surface):
         import dataclasses
         for bad in (dict(restir_candidates=13), dict(restir_spatial=17), dict(bounces=0), dict(spp=0),
                     dict(lights=np.zeros((4097, 7), np.float32)),
-                    dict(prim_materials=[PtMaterial(7, 0.3, 1.5)])):
+                    dict(prim_materials=[PtMaterial(7, 0.3, 1.5)]), dict(denoise_ensemble=3)):
             with pytest.raises(capi.UmbreonError):
                 pathtrace(bs, sc, dataclasses.replace(opt, **bad))
         with pytest.raises(capi.UmbreonError):

@@ -8,7 +8,7 @@ import math
 import numpy as np
 import pytest
 
-from umbreon_amd.pt_train import FULL_FOVY, FULL_H, _dihedral, window_scene
+from umbreon_amd.pt_train import FULL_FOVY, FULL_H, SPP_STEPS, _dihedral, refine_crossings, window_scene
 
 
 def _sphere_normals(n=33):
@@ -54,3 +54,28 @@ def test_window_scene_copies_the_full_frame_footprint():
     # reproducible
     sc2, _ = window_scene(1234, size=512)
     assert sc2.camera == sc.camera and np.array_equal(sc2.positions, sc.positions)
+
+
+def test_refine_crossings_finds_the_first_sample_count_at_each_threshold():
+    # a PSNR that grows 3 dB per doubling from 24 dB at 1 spp: 30 dB at 4 spp
+    # exactly (a step), 35 dB at 12.7 spp, 40 dB at 40.3 spp
+    def measure(n):
+        return (10.0 * n, 24.0 + 3.0 * math.log2(n))
+    pts = {}
+    for n in SPP_STEPS:
+        pts[n] = measure(n)
+        if pts[n][1] >= 40.0:
+            break
+    seen = []
+    refine_crossings(pts, lambda n: (seen.append(n), measure(n))[1], log=lambda *_: None)
+    # 30 dB: 3 -> 4, candidates 3.22, 3.46, 3.72 round to the steps: none
+    # 35 dB: 12 -> 16, candidates 13, 14, 15: 13 reaches it (35.10 dB)
+    # 40 dB: 32 -> 48, candidates 35, 39, 43 (32 * 1.5^(i/4)): 43 is the
+    # first that reaches it (40.28 dB)
+    assert seen == [13, 35, 39, 43]
+    first = {db: min(n for n, p in pts.items() if p[1] >= db) for db in (30.0, 35.0, 40.0)}
+    assert first == {30.0: 4, 35.0: 13, 40.0: 43}
+    # a sweep whose first step already reaches a threshold is left alone
+    pts = {1: (1.0, 41.0)}
+    refine_crossings(pts, lambda n: (_ for _ in ()).throw(AssertionError(n)), log=lambda *_: None)
+    assert list(pts) == [1]

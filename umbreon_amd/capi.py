@@ -242,7 +242,8 @@ class PtParams(C.Structure):
         ("env_radiance", C.c_float * 3), ("prim_materials", C.POINTER(PtMaterial)), ("n_prim_materials", C.c_int32),
         ("mesh_materials", C.POINTER(PtMaterial)), ("n_mesh_materials", C.c_int32),
         ("restir_candidates", C.c_int32), ("restir_spatial", C.c_int32), ("restir_radius", C.c_float),
-        ("nee_all_lights", C.c_int32), ("denoise_net", C.c_void_p), ("reserved", C.c_int32 * 8),
+        ("nee_all_lights", C.c_int32), ("denoise_net", C.c_void_p), ("denoise_ensemble", C.c_int32),
+        ("reserved", C.c_int32 * 7),
     ]
 
 

@@ -86,7 +86,7 @@ struct ubg_context {
   DevBuf batchPool, batchItems, batchLaunch, batchOut;
   // path tracer: accumulator, first-hit records, G-buffer, reservoirs,
   // lights + materials, padded features, prediction, colour staging, words
-  DevBuf ptAcc, ptFirst, ptG, ptRes, ptTab, ptFeat, ptPred, ptColor, ptWords;
+  DevBuf ptAcc, ptFirst, ptG, ptRes, ptTab, ptFeat, ptPred, ptColor, ptWords, ptFeatT, ptPredT;
   void* batchHost = nullptr;  // pinned: items + launch tables + read-backs
   size_t batchHostBytes = 0;
   // instantiated graphs of a batched layer (reset, primary, AO, resolve,
@@ -106,7 +106,7 @@ struct ubg_context {
             &aaPix, &giPos, &giNrm, &giRefl, &giE, &giList, &giInd, &giOcc, &dnA, &dnB, &dnGrad,
             &dnVar, &giLow, &giPatch, &giSnap, &edgePts, &edgeSolids, &edgeOut, &unetIn,
             &unetTgt, &unetOut, &giFeat, &giFeatP, &giPred, &giMax, &batchPool, &batchItems, &batchLaunch,
-            &batchOut, &ptAcc, &ptFirst, &ptG, &ptRes, &ptTab, &ptFeat, &ptPred, &ptColor, &ptWords};
+            &batchOut, &ptAcc, &ptFirst, &ptG, &ptRes, &ptTab, &ptFeat, &ptPred, &ptColor, &ptWords, &ptFeatT, &ptPredT};
   }
 };
 
@@ -1541,6 +1541,12 @@ int ubg_pathtrace(ubg_context* ctx, const ubg_scene* scene, const ubg_pt_params*
     if (p->denoise_net->ctx != ctx) return fail(UBG_ERR_INVALID_ARGUMENT, "ubg_pathtrace: network on another context");
     net = &p->denoise_net->net;
   }
+  const int ens = p->denoise_ensemble;
+  if (ens != 0 && ens != 1 && ens != 2 && ens != 4 && ens != 8)
+    return fail(UBG_ERR_INVALID_ARGUMENT, "ubg_pathtrace: denoise_ensemble must be 0, 1, 2, 4 or 8");
+  // the network's passes: 0 = 8 from 32 spp on, where the 7 extra passes cost
+  // under 1 % of the sample passes, else 1
+  const int passes = ens != 0 ? ens : (p->spp >= 32 ? 8 : 1);
   std::lock_guard<std::mutex> lk(ctx->mu);
   UBG_HIP(hipSetDevice(ctx->device), "hipSetDevice");
   hipStream_t st = ctx->stream;
@@ -1567,6 +1573,8 @@ int ubg_pathtrace(ubg_context* ctx, const ubg_scene* scene, const ubg_pt_params*
   if (r == UBG_OK) r = ensure(ctx->ptWords, 256);
   if (r == UBG_OK && wantFeat) r = ensure(ctx->ptFeat, np * ubg::pt::kPtFeatures * 4);
   if (r == UBG_OK && net) r = ensure(ctx->ptPred, np * 3 * 4);
+  if (r == UBG_OK && net && passes > 1) r = ensure(ctx->ptFeatT, np * ubg::pt::kPtFeatures * 4);
+  if (r == UBG_OK && net && passes > 1) r = ensure(ctx->ptPredT, np * 3 * 4);
   if (r == UBG_OK && fr->memory == 0) r = ensure(ctx->ptColor, n * 16);
   if (r != UBG_OK) return r;
   // lights and materials
@@ -1665,8 +1673,22 @@ int ubg_pathtrace(ubg_context* ctx, const ubg_scene* scene, const ubg_pt_params*
   float* feat = wantFeat ? static_cast<float*>(ctx->ptFeat.p) : nullptr;
   ubg::ptResolve(W, H, Wp, Hp, p->spp, acc, first, maxBits, color, feat, st);
   if (net) {
-    UBG_HIP(net->forward(1, Hp, Wp, feat, static_cast<float*>(ctx->ptPred.p), st), "unet forward");
-    ubg::ptApplyNet(W, H, Wp, static_cast<const float*>(ctx->ptPred.p), color, st);
+    float* pred = static_cast<float*>(ctx->ptPred.p);
+    if (passes == 1) {
+      UBG_HIP(net->forward(1, Hp, Wp, feat, pred, st), "unet forward");
+    } else {
+      // self-ensemble: the network on each turn of the features, the
+      // predictions turned back and averaged
+      float* featT = static_cast<float*>(ctx->ptFeatT.p);
+      float* predT = static_cast<float*>(ctx->ptPredT.p);
+      for (int k = 0; k < passes; ++k) {
+        const bool tr = (k & 4) != 0;
+        if (k > 0) ubg::ptFeatTurn(k, Wp, Hp, feat, featT, st);
+        UBG_HIP(net->forward(1, tr ? Wp : Hp, tr ? Hp : Wp, k > 0 ? featT : feat, predT, st), "unet forward");
+        ubg::ptPredUnturn(k, W, H, Wp, Hp, 1.0f / static_cast<float>(passes), predT, pred, st);
+      }
+    }
+    ubg::ptApplyNet(W, H, Wp, pred, color, st);
   }
   UBG_HIP(hipGetLastError(), "kernel launch");
   UBG_HIP(hipEventRecord(ctx->ev[2], st), "hipEventRecord");

@@ -85,6 +85,12 @@ void ptSample(const pt::Params& p, const SceneView& s, pt::GPix* gb, pt::Reservo
 // Params::depthMax).
 void ptResolve(int W, int H, int Wp, int Hp, int spp, const float4* acc, const float4* first,
                const unsigned int* maxBits, float4* color, float* feat, hipStream_t st);
+// The denoiser's self-ensemble: turn k (0..7; bit 2 transpose, then bit 0
+// mirrored columns, bit 1 mirrored rows) of the Wp x Hp feature image with
+// its normals, and the prediction on turn k turned back onto W x H (stride
+// Wp) with weight wgt (k = 0 writes, the others add).
+void ptFeatTurn(int k, int Wp, int Hp, const float* feat, float* turned, hipStream_t st);
+void ptPredUnturn(int k, int W, int H, int Wp, int Hp, float wgt, const float* predK, float* sum, hipStream_t st);
 void ptApplyNet(int W, int H, int Wp, const float* pred, float4* color, hipStream_t st);
 
 }  // namespace ubg

@@ -881,6 +881,55 @@ __global__ void k_pt_apply_net(int W, int H, int Wp, const float* __restrict__ p
   color[i] = make_float4(fmaxf(0.0f, expm1f(q[0] * k) * g), fmaxf(0.0f, expm1f(q[1] * k) * g),
                          fmaxf(0.0f, expm1f(q[2] * k) * g), 1.0f);
 }
+// The self-ensemble of the denoiser. Turn k (0..7) of the Wp x Hp feature
+// image: a transpose if bit 2 is set, then the columns mirrored if bit 0 is,
+// then the rows if bit 1 is. The camera-space normal (right, up) turns with
+// the pixels: a transpose maps (x, y) to (-y, -x), a column mirror negates x,
+// a row mirror y. The turned image is W1 x H1 (Hp x Wp after a transpose).
+__global__ void k_pt_feat_turn(int k, int Wp, int Hp, const float* __restrict__ src, float* __restrict__ dst) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= Wp * Hp) return;
+  const bool tr = (k & 4) != 0;
+  const int W1 = tr ? Hp : Wp, H1 = tr ? Wp : Hp;
+  const int x1 = i % W1, y1 = i / W1;
+  const int xa = (k & 1) ? W1 - 1 - x1 : x1, ya = (k & 2) ? H1 - 1 - y1 : y1;  // before the mirrors
+  const int x = tr ? ya : xa, y = tr ? xa : ya;                                // before the transpose
+  const float* a = src + (static_cast<size_t>(y) * Wp + x) * kPtFeatures;
+  float* d = dst + static_cast<size_t>(i) * kPtFeatures;
+#pragma unroll
+  for (int c = 0; c < kPtFeatures; ++c) d[c] = a[c];
+  float nx = a[6], ny = a[7];
+  if (tr) {
+    const float t = nx;
+    nx = -ny;
+    ny = -t;
+  }
+  d[6] = (k & 1) ? -nx : nx;
+  d[7] = (k & 2) ? -ny : ny;
+}
+// The network's prediction on turn k (W1 x H1) turned back onto the frame's
+// W x H pixels (stride Wp) and weighted by wgt: turn 0 writes, the others add.
+__global__ void k_pt_pred_unturn(int k, int W, int H, int Wp, int Hp, float wgt, const float* __restrict__ pk,
+                                 float* __restrict__ sum) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= W * H) return;
+  const int x = i % W, y = i / W;
+  const bool tr = (k & 4) != 0;
+  const int W1 = tr ? Hp : Wp, H1 = tr ? Wp : Hp;
+  const int xa = tr ? y : x, ya = tr ? x : y;
+  const int x1 = (k & 1) ? W1 - 1 - xa : xa, y1 = (k & 2) ? H1 - 1 - ya : ya;
+  const float* q = pk + (static_cast<size_t>(y1) * W1 + x1) * 3;
+  float* o = sum + (static_cast<size_t>(y) * Wp + x) * 3;
+  if (k == 0) {
+    o[0] = wgt * q[0];
+    o[1] = wgt * q[1];
+    o[2] = wgt * q[2];
+  } else {
+    o[0] += wgt * q[0];
+    o[1] += wgt * q[1];
+    o[2] += wgt * q[2];
+  }
+}
 __global__ void k_pt_zero(float4* __restrict__ a, int n, unsigned int* __restrict__ maxBits,
                           unsigned long long* __restrict__ rays) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -919,6 +968,14 @@ void ptResolve(int W, int H, int Wp, int Hp, int spp, const float4* acc, const f
   }
   hipLaunchKernelGGL(pt::k_pt_resolve, dim3(gridOf(W * H, 256)), dim3(256), 0, st, W, H, Wp, Hp, spp, acc, first,
                      maxBits, color, nullptr);
+}
+void ptFeatTurn(int k, int Wp, int Hp, const float* feat, float* turned, hipStream_t st) {
+  hipLaunchKernelGGL(pt::k_pt_feat_turn, dim3(gridOf(static_cast<int64_t>(Wp) * Hp, 256)), dim3(256), 0, st, k, Wp, Hp,
+                     feat, turned);
+}
+void ptPredUnturn(int k, int W, int H, int Wp, int Hp, float wgt, const float* predK, float* sum, hipStream_t st) {
+  hipLaunchKernelGGL(pt::k_pt_pred_unturn, dim3(gridOf(static_cast<int64_t>(W) * H, 256)), dim3(256), 0, st, k, W, H,
+                     Wp, Hp, wgt, predK, sum);
 }
 void ptApplyNet(int W, int H, int Wp, const float* pred, float4* color, hipStream_t st) {
   hipLaunchKernelGGL(pt::k_pt_apply_net, dim3(gridOf(W * H, 256)), dim3(256), 0, st, W, H, Wp, pred, color);

@@ -89,7 +89,7 @@ class Unet {
   int64_t minWgs_ = 1024;
   bool lossL2_ = false;
   // activation scratch for the current shape
-  int n_ = 0, h_ = 0, w_ = 0;
+  int64_t capP0_ = 0;  // pixels (n h w) the activation and gradient buffers hold
   std::vector<uint16_t*> actBf_;   // bf16 activations
   std::vector<float*> actGrad_;    // fp32 gradients of the activations
   float* outF_ = nullptr;          // output conv, fp32, 32 channels
@@ -98,7 +98,6 @@ class Unet {
   float* dx_ = nullptr;            // fp32 gradient w.r.t. a conv's (transformed) input
   float* part_ = nullptr;          // split-K partials of the weight / bias gradients
   float* lossPart_ = nullptr;
-  size_t partBytes_ = 0;
   std::vector<void*> scratch_;
 };
 

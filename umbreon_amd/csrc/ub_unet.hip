@@ -653,7 +653,10 @@ void Unet::syncBf16(hipStream_t st) {
 }
 
 hipError_t Unet::ensure(int n, int h, int w) {
-  if (n == n_ && h == h_ && w == w_) return hipSuccess;
+  // every buffer's size is a function of the pixel count n h w alone, so a
+  // shape of no more pixels (a transposed image, a smaller batch) reuses them
+  const int64_t P0 = static_cast<int64_t>(n) * h * w;
+  if (P0 <= capP0_) return hipSuccess;
   for (void* p : scratch_) (void)hipFree(p);
   scratch_.clear();
   actBf_.assign(acts_.size(), nullptr);
@@ -663,7 +666,7 @@ hipError_t Unet::ensure(int n, int h, int w) {
     if (e == hipSuccess) scratch_.push_back(*p);
     return e;
   };
-  const int64_t P0 = static_cast<int64_t>(n) * h * w;
+  capP0_ = 0;
   hipError_t e = hipSuccess;
   int64_t maxDz = 0, maxDx = 0, maxPart = 0;
   for (size_t a = 0; a < acts_.size() && e == hipSuccess; ++a) {
@@ -685,13 +688,8 @@ hipError_t Unet::ensure(int n, int h, int w) {
   if (e == hipSuccess) e = alloc(static_cast<size_t>(maxDx) * 4, reinterpret_cast<void**>(&dx_));
   if (e == hipSuccess) e = alloc(static_cast<size_t>(maxPart) * 4, reinterpret_cast<void**>(&part_));
   if (e == hipSuccess) e = alloc(static_cast<size_t>(blocks(P0 * kUnetOutPad, 256)) * 4, reinterpret_cast<void**>(&lossPart_));
-  if (e != hipSuccess) {
-    n_ = h_ = w_ = 0;
-    return e;
-  }
-  n_ = n;
-  h_ = h;
-  w_ = w;
+  if (e != hipSuccess) return e;
+  capP0_ = P0;
   return hipSuccess;
 }
 

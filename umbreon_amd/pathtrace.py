@@ -46,6 +46,9 @@ class PtOptions:
     restir_spatial: int = 4
     restir_radius: float = 16.0
     nee_all_lights: bool = False
+    # network passes averaged by the learned denoiser (ubg_pt_params::
+    # denoise_ensemble): 1, 2, 4, 8, or 0 = 8 from 32 spp on, else 1
+    denoise_ensemble: int = 0
 
 
 @dataclass
@@ -94,6 +97,7 @@ def pt_params(camera: Camera, opt: PtOptions, denoise_net=None):
     p.restir_radius = float(opt.restir_radius)
     p.nee_all_lights = int(bool(opt.nee_all_lights))
     p.denoise_net = denoise_net.handle if denoise_net is not None else None
+    p.denoise_ensemble = int(opt.denoise_ensemble)
     return p, keep
 
 

@@ -237,9 +237,36 @@ def holdout_psnr(dev: Device, net: Unet, seeds: Sequence[int], size: int = WINDO
     return out
 
 
-# samples per pixel of the evaluation's frames: steps of 2^(1/2) (rounded),
-# so a crossing is timed within a factor 1.5 of its sample count
+# samples per pixel of the evaluation's frames: steps of 2^(1/2) (rounded);
+# each crossing is then refined between the last step below it and the first
+# at or above it (refine_crossings), to within 2^(1/8)
 SPP_STEPS = (1, 2, 3, 4, 6, 8, 12, 16, 24, 32, 48, 64, 96, 128, 192, 256, 384, 512, 768, 1024, 1536, 2048)
+THRESHOLDS_DB = (30.0, 35.0, 40.0)
+
+
+def refine_crossings(points: dict, measure, thresholds: Sequence[float] = THRESHOLDS_DB, log=None) -> None:
+    """points: {spp: (wall ms, dB, ...)} of a sweep over SPP_STEPS. For each
+    threshold that some step reaches, except at the first step, measure
+    (spp -> point) the three sample counts 2^(1/8) apart between the last
+    step below it and the first step at or above it, in increasing order,
+    until one reaches it; the points join `points`. The same rule serves
+    both methods, so neither's crossing is rounded up further than the
+    other's."""
+    log = log or _say
+    for db in thresholds:
+        steps = sorted(points)
+        ok = [n for n in steps if points[n][1] >= db]
+        if not ok or ok[0] == steps[0]:
+            continue
+        hi = ok[0]
+        lo = max(n for n in steps if n < hi)
+        cand = sorted({int(round(lo * (hi / lo) ** (i / 4))) for i in (1, 2, 3)} - {lo, hi})
+        for n in cand:
+            if n not in points:
+                points[n] = measure(n)
+                log(f"  refine {db:g} dB: {n} spp: {points[n][1]:.2f} dB, {points[n][0]:.1f} ms wall")
+            if points[n][1] >= db:
+                break
 
 
 def evaluate_full(dev: Device, net: Optional[Unet], ref_spp: int = TARGET_SPP, max_plain_spp: int = 2048,
@@ -248,7 +275,10 @@ def evaluate_full(dev: Device, net: Optional[Unet], ref_spp: int = TARGET_SPP, m
     training never saw): PSNR (sRGB, whole frame) against a ref_spp render;
     the plain path tracer at 1, 2, 3, 4, 6, 8, ... samples until it reaches
     40 dB (at most max_plain_spp), the learned denoiser at den_spp until it
-    does; the time each first reaches 30, 35 and 40 dB. Each point is
+    does, each crossing of 30, 35 and 40 dB then refined to within 2^(1/8)
+    samples (refine_crossings); the time each first reaches 30, 35 and
+    40 dB. The denoiser runs with its default ensemble (8 network passes
+    from 32 spp on). Each point is
     (wall ms of the frame call with the colour copied to the host, dB, device
     ms = sample passes + resolve + network)."""
     log = log or _say
@@ -276,26 +306,30 @@ def evaluate_full(dev: Device, net: Optional[Unet], ref_spp: int = TARGET_SPP, m
                 den_ms = f.denoise_seconds * 1e3 if den_ms is None else min(den_ms, f.denoise_seconds * 1e3)
             return (best, psnr_srgb(f.color, ref), dev_ms, den_ms)
 
-        plain, plain_spp = [], []
+        plain_pts = {}
         for n in SPP_STEPS:
             if n > max_plain_spp:
                 break
-            plain.append(timed(replace(opt, spp=n, seed=11)))
-            plain_spp.append(n)
-            log(f"plain {n} spp: {plain[-1][1]:.2f} dB, {plain[-1][0]:.1f} ms wall, {plain[-1][2]:.1f} ms device")
-            if plain[-1][1] >= 40.0:
+            plain_pts[n] = timed(replace(opt, spp=n, seed=11))
+            log(f"plain {n} spp: {plain_pts[n][1]:.2f} dB, {plain_pts[n][0]:.1f} ms wall, "
+                f"{plain_pts[n][2]:.1f} ms device")
+            if plain_pts[n][1] >= 40.0:
                 break
-        learned, den_done = [], []
+        refine_crossings(plain_pts, lambda n: timed(replace(opt, spp=n, seed=11)), log=log)
+        learned_pts = {}
         if net is not None:
             for n in den_spp:
-                if n > plain_spp[-1]:
+                if n > max(plain_pts):
                     break  # no faster than the plain path tracer any more
-                learned.append(timed(replace(opt, spp=n, seed=11), denoise_net=net))
-                den_done.append(n)
-                log(f"learned {n} spp: {learned[-1][1]:.2f} dB, {learned[-1][0]:.1f} ms wall, "
-                    f"{learned[-1][2]:.1f} ms device ({learned[-1][3]:.2f} ms resolve + network)")
-                if learned[-1][1] >= 40.0:
+                learned_pts[n] = timed(replace(opt, spp=n, seed=11), denoise_net=net)
+                p = learned_pts[n]
+                log(f"learned {n} spp: {p[1]:.2f} dB, {p[0]:.1f} ms wall, {p[2]:.1f} ms device "
+                    f"({p[3]:.2f} ms resolve + network)")
+                if p[1] >= 40.0:
                     break
+            refine_crossings(learned_pts, lambda n: timed(replace(opt, spp=n, seed=11), denoise_net=net), log=log)
+        plain_spp, plain = sorted(plain_pts), [plain_pts[n] for n in sorted(plain_pts)]
+        den_done, learned = sorted(learned_pts), [learned_pts[n] for n in sorted(learned_pts)]
     finally:
         bs.close()
     res = {"workload": (f"path-traced GI: molecular surface ({len(sc.index)} triangles) as a GGX dielectric over "
@@ -311,7 +345,7 @@ def evaluate_full(dev: Device, net: Optional[Unet], ref_spp: int = TARGET_SPP, m
            "t40_learned_device_ms": time_to_psnr(learned, 40.0, 2)}
     # time (wall ms) each first reaches a PSNR, and the ratio
     res["time_to_db"] = {}
-    for db in (30.0, 35.0, 40.0):
+    for db in THRESHOLDS_DB:
         a, b = time_to_psnr(plain, db), time_to_psnr(learned, db)
         res["time_to_db"][f"{db:g}"] = {"plain_ms": a, "learned_ms": b,
                                        "speedup": None if (a is None or b is None) else round(a / b, 2)}
