@@ -150,24 +150,6 @@ def measured_compute(workload: str, kernel: str):
     return None
 
 
-def calib_copy(mib: int) -> None:
-    """One hipMemcpy device-to-device of `mib` MiB (the runtime's copy kernel:
-    reads and writes exactly that many bytes), for counter calibration."""
-    import ctypes as C
-    hip = C.CDLL("libamdhip64.so")
-    n = mib << 20
-    a, b = C.c_void_p(), C.c_void_p()
-    assert hip.hipMalloc(C.byref(a), C.c_size_t(n)) == 0
-    assert hip.hipMalloc(C.byref(b), C.c_size_t(n)) == 0
-    assert hip.hipMemset(a, 1, C.c_size_t(n)) == 0
-    assert hip.hipDeviceSynchronize() == 0
-    assert hip.hipMemcpy(b, a, C.c_size_t(n), 3) == 0  # hipMemcpyDeviceToDevice
-    assert hip.hipDeviceSynchronize() == 0
-    hip.hipFree(a)
-    hip.hipFree(b)
-    log(f"calibration copy: {n} bytes device-to-device")
-
-
 def time_to_40db(dev) -> dict:
     """The metric's second half on the path-traced GI workload: the 40,000-
     atom molecular surface as a GGX dielectric over GGX conductor / dielectric
@@ -394,7 +376,8 @@ def main() -> int:
             bg.close()
 
     if args.calib_copy_mib > 0:
-        calib_copy(args.calib_copy_mib)
+        from umbreon_amd.counters import calib_copy
+        calib_copy(args.calib_copy_mib, log=log)
 
     cpu = None
     if not args.no_cpu_baseline:

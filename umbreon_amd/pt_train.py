@@ -410,10 +410,15 @@ def main() -> None:
                     help="time full-workload frames at SPP samples per pixel (denoised with --weights)")
     ap.add_argument("--weights", default="", help="network for --bench-frame")
     ap.add_argument("--calls", type=int, default=5)
+    ap.add_argument("--calib-copy-mib", type=int, default=0,
+                    help="with --bench-frame: one device copy of this many MiB first (counter calibration)")
     args = ap.parse_args()
     if args.bench_frame:
         dev = Device(0)
         try:
+            if args.calib_copy_mib > 0:
+                from .counters import calib_copy
+                calib_copy(args.calib_copy_mib, log=_say)
             bench_frame(dev, args.weights or None, args.bench_frame, args.calls)
         finally:
             dev.close()
