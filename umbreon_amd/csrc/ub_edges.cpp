@@ -69,28 +69,28 @@ struct Loop {
 // eye (perspective), raised outward along the surface normal.
 bool sphereRing(P3 O, float r, const Cam& cam, int N, float raise, Loop& out) {
   if (r <= 0.0f) return false;
-  P3 axis, Q;
-  float rho;
+  P3 view, ringC;  // the ring lies in the plane through ringC normal to view
+  float ringR;
   if (cam.ortho) {
-    axis = normalize(cam.dir);
-    Q = O;
-    rho = r;
+    view = normalize(cam.dir);
+    ringC = O;
+    ringR = r;
   } else {
     const P3 d = O - cam.pos;
     const float L = length(d);
     if (L <= r * 1.0001f) return false;  // the eye on or inside the sphere
-    axis = d * (1.0f / L);
-    Q = O - axis * (r * r / L);
+    view = d * (1.0f / L);
+    ringC = O - view * (r * r / L);
     const float disc = L * L - r * r;
-    rho = r * std::sqrt(disc > 0.0f ? disc : 0.0f) / L;
+    ringR = r * std::sqrt(disc > 0.0f ? disc : 0.0f) / L;
   }
-  if (rho <= 0.0f) return false;
-  const Fr fr = frameFromNormal(axis);
+  if (ringR <= 0.0f) return false;
+  const Fr fr = frameFromNormal(view);
   out.pts.resize(static_cast<size_t>(N));
   for (int i = 0; i < N; ++i) {
     const float theta = 2.0f * kPiF * static_cast<float>(i) / static_cast<float>(N);
     const P3 dir = fr.t * std::cos(theta) + fr.b * std::sin(theta);
-    P3 P = Q + dir * rho;
+    P3 P = ringC + dir * ringR;
     const P3 nrm = normalize(P - O);
     P = P + nrm * raise;
     out.pts[static_cast<size_t>(i)] = P;
@@ -105,28 +105,30 @@ bool sphereRing(P3 O, float r, const Cam& cam, int N, float raise, Loop& out) {
 bool contactDirs(P3 P, P3 u, float r, const Cam& cam, P3& dPlus, P3& dMinus) {
   constexpr float kEps = 1.0e-4f;
   if (cam.ortho) {
-    const P3 w = normalize(cam.dir) * -1.0f;
-    P3 eT = cross(u, w);
-    const float l = length(eT);
+    const P3 toEye = normalize(cam.dir) * -1.0f;
+    P3 side = cross(u, toEye);  // across the axis, normal to the view
+    const float l = length(side);
     if (l < kEps) return false;
-    eT = eT * (1.0f / l);
-    dPlus = eT;
-    dMinus = eT * -1.0f;
+    side = side * (1.0f / l);
+    dPlus = side;
+    dMinus = side * -1.0f;
     return true;
   }
-  const P3 cp = cam.pos - P;
-  const P3 cpPerp = cp - u * dot(u, cp);
-  const float Lp = length(cpPerp);
-  if (Lp <= r * 1.0001f) return false;
-  const P3 eR = cpPerp * (1.0f / Lp);
-  P3 eT = cross(u, eR);
-  const float lt = length(eT);
-  if (lt < kEps) return false;
-  eT = eT * (1.0f / lt);
-  const float cosA = r / Lp;
-  const float sinA = std::sqrt(std::fmax(0.0f, 1.0f - cosA * cosA));
-  dPlus = eR * cosA + eT * sinA;
-  dMinus = eR * cosA - eT * sinA;
+  const P3 toEye = cam.pos - P;
+  const P3 radial = toEye - u * dot(u, toEye);  // the eye's offset from the axis
+  const float dist = length(radial);
+  if (dist <= r * 1.0001f) return false;
+  const P3 inward = radial * (1.0f / dist);
+  P3 side = cross(u, inward);
+  const float ls = length(side);
+  if (ls < kEps) return false;
+  side = side * (1.0f / ls);
+  // the tangent from the eye touches the circle at angle acos(r / dist) off
+  // the direction to the eye
+  const float c = r / dist;
+  const float s = std::sqrt(std::fmax(0.0f, 1.0f - c * c));
+  dPlus = inward * c + side * s;
+  dMinus = inward * c - side * s;
   return true;
 }
 
