@@ -159,3 +159,18 @@ def test_pool_and_upsample_closed_forms():
     up = U._operand(acts, (0, 2), bf16=False)
     assert up.shape == (2, 8, 12, 1)
     assert up[0, 3, 5, 0] == x[0, 1, 2, 0] and up[1, 7, 11, 0] == x[1, 3, 5, 0]
+
+
+def test_network_features_pads_the_gather_features():
+    # the GI gather writes 10 features; the network reads 12, the two noise
+    # descriptors zero for the gather, and 12-channel features pass unchanged
+    from umbreon_amd.denoise import N_FEATURES, network_features
+    rng = np.random.default_rng(3)
+    f10 = rng.uniform(-1.0, 1.0, (2, 4, 8, 10)).astype(np.float32)
+    f = network_features(f10)
+    assert f.shape == (2, 4, 8, N_FEATURES) and f.dtype == np.float32 and f.flags.c_contiguous
+    assert np.array_equal(f[..., :10], f10) and not f[..., 10:].any()
+    f12 = rng.uniform(-1.0, 1.0, (1, 4, 8, 12)).astype(np.float32)
+    assert np.array_equal(network_features(f12), f12)
+    with pytest.raises(ValueError):
+        network_features(np.zeros((1, 4, 8, 11), np.float32))
