@@ -1,6 +1,6 @@
 """gfx950 path vs the CPU oracle on the benchmark workloads.
 
-Gates (the per-ray north-star bar):
+Gates (the per-ray output bar):
 * primary first-hit (kind, primID) agree on >= 99.99% of hi-res pixels;
 * deterministic shading + shadows + fog: max abs <= 1e-3 (linear HDR) on
   every pixel whose first hit and binary visibility decisions agree; the
@@ -165,7 +165,7 @@ def test_config3_surface_reduced(device):
     c = ob.render(sc, opt, hires_only=True)
     _compare("config3-reduced", g, c, ao=True)
     assert g.layers >= 3  # primary_rays (continuations included) already match the oracle
-    # the north-star output bar for this scene: PSNR >= 60 dB (sRGB) against the FP64 oracle
+    # the output bar for this scene: PSNR >= 60 dB (sRGB) against the FP64 oracle
     c64 = ob.render(sc, opt, hires_only=True, precision=64)
     p = psnr_srgb(g.color, c64.color)
     print(f"config3-reduced PSNR vs FP64 oracle: {p:.1f} dB", flush=True)
